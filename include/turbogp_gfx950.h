@@ -1,0 +1,383 @@
+/*
+ * turbogp_gfx950.h -- C ABI of the MI355X (gfx950) operator library.
+ *
+ * This is the drop-in boundary for turboGP's data-parallel operators:
+ * predicate scan -> selection bitmap -> compaction, grouped aggregation with
+ * exact integer sums, and hash-join build/probe.  Every entry point is plain
+ * C: integers, plain pointers and sizes, opaque handles.  No torch or HIP
+ * types appear in a signature (a stream is passed as `void*`, NULL = the
+ * context's own stream).
+ *
+ * Conventions
+ *   - Every function returns int32_t: TGP_OK (0) or a negative TGP_ERR_*.
+ *     Nothing aborts; tgp_last_error() gives a message for the calling thread.
+ *   - Buffers named d_* are DEVICE pointers (allocate them with tgp_hbm_alloc,
+ *     which accounts them in the context's HBM tier).  Buffers named h_* are
+ *     host pointers.
+ *   - Operator calls are asynchronous on `stream` unless the name ends in
+ *     _sync.  Calls that overlap on one context and one stream are ordered;
+ *     different streams may run concurrently (results of two calls never share
+ *     writable memory unless the caller passes the same output buffer).
+ *   - Integer semantics follow the reference engine: a row value is computed
+ *     in wrapping 64-bit two's complement (query_interpreter/expr.rs binop,
+ *     :2037-2104); sums are accumulated EXACTLY in 128 bits (the reference
+ *     sums expressions in f64, query_interpreter/aggregate.rs:1485-1792; the
+ *     exact sum is the value the f64 approximates).
+ *
+ * Reference interfaces replaced (paths relative to the turboGP checkout):
+ *   src/kernel/mod.rs:47-74     Operator                 -> TGP_OP_*
+ *   src/kernel/mod.rs:84-92     PredicateOp              -> tgp_kernel_params.pred*_op
+ *   src/kernel/mod.rs:96-148    KernelParams/KernelResult-> tgp_kernel_params/_result
+ *   src/kernel/mod.rs:158-177   trait Kernel::execute    -> tgp_kernel_execute
+ *   src/kernel/mod.rs:179-242   KernelTable select/list  -> tgp_kernel_table_*
+ *   src/memory/tier.rs:11-29    MemoryTier (Hbm)         -> TGP_TIER_*, tgp_hbm_*
+ *   src/datasource/table.rs:25  Table column cells       -> tgp_column
+ *   src/exec/bitmap.rs:42-314   Bitmap + SetBitIter      -> tgp_scan_filter, tgp_bitmap_*
+ *   src/exec/bitmap.rs:321-477  filter_{eq..ge}_{u64,i64}-> tgp_pred
+ *   query_interpreter/expr.rs:419-1577  build_mask (AND/OR/BETWEEN) -> tgp_scan_filter + tgp_bitmap_combine
+ *   query_interpreter/aggregate.rs:507-697, accum_grouped.rs:55-409 -> tgp_group_agg
+ *   src/exec/join_hash_table.rs:38-241, kernel/hash.rs:84-225      -> tgp_hash_build / tgp_hash_probe_*
+ *   query_interpreter/join.rs:544-939 hash_join_with_keys (+ A9 agg)-> tgp_hash_probe_agg
+ *   src/exec/join.rs:56-199 hash_join / semi_join / anti_join       -> tgp_hash_probe_pairs / _mark
+ */
+#ifndef TURBOGP_GFX950_H
+#define TURBOGP_GFX950_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TGP_ABI_VERSION 1
+
+/* ---- status codes -------------------------------------------------------- */
+#define TGP_OK 0
+#define TGP_ERR_INVALID (-1)     /* bad argument (null, size, range, alignment) */
+#define TGP_ERR_HIP (-2)         /* a HIP runtime call failed */
+#define TGP_ERR_OOM (-3)         /* device allocation failed */
+#define TGP_ERR_BUDGET (-4)      /* HBM-tier budget would be exceeded */
+#define TGP_ERR_UNSUPPORTED (-5) /* shape outside the compiled limits */
+#define TGP_ERR_NO_DEVICE (-6)   /* no usable gfx950 device */
+#define TGP_ERR_CAPACITY (-7)    /* hash table or output buffer too small */
+#define TGP_ERR_CANCELLED (-8)   /* request cancelled before it ran */
+#define TGP_ERR_INJECTED (-9)    /* failure injected by tgp_debug_inject_failure */
+
+/* ---- element types and comparisons --------------------------------------- */
+enum tgp_dtype {
+  TGP_I8 = 0, TGP_U8 = 1, TGP_I16 = 2, TGP_U16 = 3,
+  TGP_I32 = 4, TGP_U32 = 5, TGP_I64 = 6, TGP_U64 = 7
+};
+/* bitmap.rs filter_{eq,ne,lt,le,gt,ge}_*: EQ/NE bitwise, the rest ordered. */
+enum tgp_cmp { TGP_EQ = 0, TGP_NE = 1, TGP_LT = 2, TGP_LE = 3, TGP_GT = 4, TGP_GE = 5 };
+enum tgp_bitop { TGP_BIT_AND = 0, TGP_BIT_OR = 1, TGP_BIT_ANDNOT = 2 };
+
+/* Compile-time limits of the fused kernels. */
+#define TGP_MAX_COLS 8   /* distinct columns referenced by one query */
+#define TGP_MAX_PREDS 8  /* AND-ed predicate leaves */
+#define TGP_MAX_AGGS 6   /* aggregates besides the implicit COUNT(*) */
+#define TGP_MAX_FACTORS 3
+#define TGP_MAX_GROUP_PARTS 2
+
+/* A device-resident column at its native width (row 0 at `data`). */
+typedef struct tgp_column {
+  const void* data;  /* device pointer */
+  uint64_t len;      /* rows */
+  int32_t dtype;     /* enum tgp_dtype */
+  int32_t _pad;
+} tgp_column;
+
+/* One predicate leaf: cols[col] <op> lit.  Values are widened to 64 bits
+ * (sign- or zero-extended by dtype) and compared as int64 when is_signed
+ * (bitmap.rs filter_*_i64, INT columns) or as uint64 otherwise
+ * (bitmap.rs filter_*_u64, DATE columns; expr.rs:1482-1508). */
+typedef struct tgp_pred {
+  int32_t col;
+  int32_t op;        /* enum tgp_cmp */
+  int32_t is_signed;
+  int32_t _pad;
+  int64_t lit;
+} tgp_pred;
+
+/* One part of a composite dense group key:
+ *   part = (v / divisor) - base,  0 <= part < card   (divisor >= 1; v >= 0
+ *   when divisor > 1).  group id = part0 + card0 * part1.
+ * A row whose part falls outside [0, card) is counted in no group and is
+ * reported through tgp_group_result.n_out_of_range. */
+typedef struct tgp_group_part {
+  int32_t col;
+  int32_t divisor;
+  int64_t base;
+  int64_t card;
+} tgp_group_part;
+
+enum tgp_agg_kind {
+  TGP_AGG_SUM = 0,          /* SUM(coef * prod_k (fa[k] + fb[k]*cols[fcol[k]])) */
+  TGP_AGG_COUNT_NONZERO = 1 /* COUNT(col): rows with cols[fcol[0]] != 0
+                               (accum_grouped.rs:243-247 counts non-zero cells) */
+};
+
+typedef struct tgp_agg {
+  int32_t kind;      /* enum tgp_agg_kind */
+  int32_t nfactors;  /* 0..TGP_MAX_FACTORS (0 => SUM(coef)) */
+  int32_t fcol[TGP_MAX_FACTORS];
+  int32_t _pad;
+  int64_t coef;
+  int64_t fa[TGP_MAX_FACTORS];
+  int64_t fb[TGP_MAX_FACTORS];
+} tgp_agg;
+
+/* A filter + GROUP BY + aggregate query over rows [row_begin, row_end). */
+typedef struct tgp_agg_query {
+  uint64_t row_begin, row_end;
+  tgp_column cols[TGP_MAX_COLS];
+  int32_t ncols, npreds, nparts, naggs;
+  tgp_pred preds[TGP_MAX_PREDS];
+  tgp_group_part parts[TGP_MAX_GROUP_PARTS];
+  tgp_agg aggs[TGP_MAX_AGGS];
+  const uint64_t* d_sel_bitmap; /* optional extra AND mask (bit i <-> row_begin+i), or NULL */
+} tgp_agg_query;
+
+/* Dense result of tgp_group_agg: for g < ngroups,
+ *   count[g]                 COUNT(*) of rows in group g (0 => group absent)
+ *   sums[(g*naggs + a)*2+0]  low 64 bits of the exact int128 sum of agg a
+ *   sums[(g*naggs + a)*2+1]  high 64 bits (two's complement)
+ * Device buffers sized ngroups and ngroups*naggs*2 words. */
+typedef struct tgp_group_result {
+  uint64_t* d_count;
+  uint64_t* d_sums;
+  uint64_t* d_n_out_of_range; /* one word, may be NULL */
+} tgp_group_result;
+
+/* ---- context, device, streams --------------------------------------------- */
+typedef struct tgp_ctx tgp_ctx;
+
+typedef struct tgp_device_info {
+  char name[64];
+  char arch[32];          /* e.g. "gfx950" */
+  int32_t cu_count;
+  int32_t wave_size;
+  int64_t hbm_bytes;
+  int64_t lds_bytes_per_cu;
+  int64_t l2_bytes;
+  int32_t clock_khz;
+  int32_t mem_clock_khz;
+} tgp_device_info;
+
+/* Mirrors KernelTable::new + detect_cpu (kernel/mod.rs:191, cpu.rs:67):
+ * binds `device`, checks it is gfx950, creates the context stream and an HBM
+ * tier with a byte budget (0 => 90% of the device's HBM). */
+int32_t tgp_ctx_create(int32_t device, uint64_t hbm_budget_bytes, tgp_ctx** out);
+int32_t tgp_ctx_destroy(tgp_ctx* ctx);
+int32_t tgp_device_info_get(tgp_ctx* ctx, tgp_device_info* out);
+const char* tgp_last_error(void);
+int32_t tgp_abi_version(void);
+int32_t tgp_stream_create(tgp_ctx* ctx, void** out_stream);
+int32_t tgp_stream_destroy(tgp_ctx* ctx, void* stream);
+int32_t tgp_stream_sync(tgp_ctx* ctx, void* stream);
+
+/* ---- HBM tier (memory/tier.rs MemoryTier::Hbm: resident columns) ------------ */
+int32_t tgp_hbm_alloc(tgp_ctx* ctx, uint64_t bytes, void** d_out);
+int32_t tgp_hbm_free(tgp_ctx* ctx, void* d_ptr);
+int32_t tgp_hbm_usage(tgp_ctx* ctx, uint64_t* used, uint64_t* budget, uint64_t* peak);
+int32_t tgp_hbm_set_budget(tgp_ctx* ctx, uint64_t budget_bytes);
+int32_t tgp_memcpy_h2d(tgp_ctx* ctx, void* d_dst, const void* h_src, uint64_t bytes, void* stream);
+int32_t tgp_memcpy_d2h(tgp_ctx* ctx, void* h_dst, const void* d_src, uint64_t bytes, void* stream);
+int32_t tgp_memcpy_d2d(tgp_ctx* ctx, void* d_dst, const void* d_src, uint64_t bytes, void* stream);
+/* Fills with a kernel (graph-capture safe; no memset node). */
+int32_t tgp_fill_u64(tgp_ctx* ctx, uint64_t* d_dst, uint64_t nwords, uint64_t value, void* stream);
+/* Column min/max (as int64 if signed dtype, else as uint64 bits), one pass. */
+int32_t tgp_column_minmax_sync(tgp_ctx* ctx, const tgp_column* col, int64_t* h_min, int64_t* h_max);
+
+/* ---- predicate scan, selection bitmap, compaction ------------------------- */
+/* Bitmap layout (bitmap.rs:36-45): bit i of the selection is bit (i & 63) of
+ * 64-bit word i >> 6, i.e. LSB-first bytes on a little-endian host; the words
+ * past the last row are written with zero tail bits (bitmap.rs:57-67).
+ * Capacity of a bitmap for n rows: tgp_bitmap_words(n) words. */
+uint64_t tgp_bitmap_words(uint64_t nbits);
+
+/* AND of preds over rows [row_begin, row_end): writes the bitmap (bit i <->
+ * row row_begin+i) when d_bitmap != NULL and adds the number of selected rows
+ * to *d_count (device u64, caller zeroes it) when d_count != NULL.
+ * Replaces filter_* + Bitmap::and_inplace chains (expr.rs:633-656) and the
+ * kernel-table scans ScanEq/ScanRange/ScanMultiPredicate (kernel/scan.rs). */
+int32_t tgp_scan_filter(tgp_ctx* ctx, const tgp_column* cols, int32_t ncols,
+                        const tgp_pred* preds, int32_t npreds,
+                        uint64_t row_begin, uint64_t row_end,
+                        uint64_t* d_bitmap, uint64_t* d_count, void* stream);
+
+/* Bitmap algebra over nbits bits (bitmap.rs and/or/not/and_inplace):
+ * out = a op b (out may alias a); NOT clears tail bits (bitmap.rs:121-136). */
+int32_t tgp_bitmap_combine(tgp_ctx* ctx, int32_t op, const uint64_t* d_a, const uint64_t* d_b,
+                           uint64_t* d_out, uint64_t nbits, void* stream);
+int32_t tgp_bitmap_not(tgp_ctx* ctx, const uint64_t* d_a, uint64_t* d_out, uint64_t nbits,
+                       void* stream);
+/* count_ones_range(begin, end) (bitmap.rs:248-273), added to *d_count. */
+int32_t tgp_bitmap_count_range(tgp_ctx* ctx, const uint64_t* d_a, uint64_t nbits,
+                               uint64_t begin, uint64_t end, uint64_t* d_count, void* stream);
+
+/* Compaction (SetBitIter, bitmap.rs:278-314): ascending indices of the set
+ * bits, each plus `base`, as u64 (index_bytes 8, like Vec<usize>) or u32
+ * (index_bytes 4).  d_out needs room for popcount entries; *h_count receives
+ * the number written.  Synchronous. */
+int32_t tgp_bitmap_to_indices_sync(tgp_ctx* ctx, const uint64_t* d_bitmap, uint64_t nbits,
+                                   uint64_t base, int32_t index_bytes, void* d_out,
+                                   uint64_t out_capacity, uint64_t* h_count, void* stream);
+
+/* ---- fused filter + GROUP BY + exact aggregation ---------------------------- */
+/* One pass over the referenced columns: preds AND-ed (plus the optional
+ * bitmap), dense composite group key, COUNT(*) and up to TGP_MAX_AGGS sums,
+ * all exact.  ngroups = prod(card) (1 when nparts == 0).  The result buffers
+ * are overwritten.  Replaces execute_grouped -> try_accumulator_grouped /
+ * HashMap grouping + sum_vec (aggregate.rs:507-697, :1485-1792;
+ * accum_grouped.rs:55-409) and the scalar aggregates (aggregate.rs:1055-1289). */
+int32_t tgp_group_agg(tgp_ctx* ctx, const tgp_agg_query* q, const tgp_group_result* out,
+                      void* stream);
+/* Number of groups the query's key space spans (prod of card). */
+int32_t tgp_group_agg_ngroups(const tgp_agg_query* q, uint64_t* out_ngroups);
+
+/* ---- hash join --------------------------------------------------------------- */
+typedef struct tgp_hash_table tgp_hash_table;
+
+/* Build an open-addressing table (64-bit CAS on the key slot) over the
+ * build rows: key column (int64/uint64/int32 widened) and an optional 32-bit
+ * payload column stored inline (NULL => the build row index is the payload).
+ * Duplicate keys keep one slot each (multiset, like JoinHashTable chains,
+ * join_hash_table.rs:127-198).  Capacity = next_pow2(2*n) slots of 16 bytes,
+ * accounted in the HBM tier.  Synchronous (returns once the table is built). */
+int32_t tgp_hash_build(tgp_ctx* ctx, const tgp_column* keys, const tgp_column* payload,
+                       tgp_hash_table** out, void* stream);
+int32_t tgp_hash_table_destroy(tgp_ctx* ctx, tgp_hash_table* t);
+/* Bytes of HBM the table holds (admission control). */
+int32_t tgp_hash_table_footprint(const tgp_hash_table* t, uint64_t* out_bytes);
+/* Bytes a table over n build rows would take, before building it. */
+uint64_t tgp_hash_table_bytes_for(uint64_t n_build_rows);
+int32_t tgp_hash_table_info(const tgp_hash_table* t, uint64_t* n_rows, uint64_t* capacity,
+                            int32_t* has_duplicates);
+
+/* Total number of (probe row, build row) matches over rows [row_begin,row_end)
+ * of the probe keys, added to *d_count (HashProbe, kernel/hash.rs:160-187). */
+int32_t tgp_hash_probe_count(tgp_ctx* ctx, const tgp_hash_table* t, const tgp_column* probe_keys,
+                             uint64_t row_begin, uint64_t row_end, uint64_t* d_count, void* stream);
+
+/* Semi/anti join marks: bit i set iff probe row row_begin+i has >= 1 match
+ * (exec/join.rs semi_join); anti = tgp_bitmap_not of it. */
+int32_t tgp_hash_probe_mark(tgp_ctx* ctx, const tgp_hash_table* t, const tgp_column* probe_keys,
+                            uint64_t row_begin, uint64_t row_end, uint64_t* d_bitmap, void* stream);
+
+/* Inner-join pairs (probe row, build payload) in ascending probe-row order
+ * (exec/join.rs hash_join INNER; query_interpreter/join.rs:643-794).  Within
+ * one probe row, duplicates appear in ascending build-row order.
+ * Synchronous; *h_npairs receives the number of pairs (capacity exceeded =>
+ * TGP_ERR_CAPACITY with *h_npairs set to the number needed). */
+int32_t tgp_hash_probe_pairs_sync(tgp_ctx* ctx, const tgp_hash_table* t,
+                                  const tgp_column* probe_keys, uint64_t row_begin,
+                                  uint64_t row_end, uint64_t* d_probe_idx, uint32_t* d_build_payload,
+                                  uint64_t capacity, uint64_t* h_npairs, void* stream);
+
+enum tgp_join_group_by {
+  TGP_JOIN_GROUP_BUILD_PAYLOAD = 0, /* GROUP BY the inline build payload */
+  TGP_JOIN_GROUP_PROBE_COL = 1,     /* GROUP BY a probe-side column */
+  TGP_JOIN_GROUP_NONE = 2           /* one group */
+};
+
+/* Fused probe -> GROUP BY -> SUM(value), COUNT(*) over the inner-join rows,
+ * without materialising them.  Group keys must be dense: key - key_base in
+ * [0, ngroups) (others are counted in *d_n_out_of_range).  Output layout as
+ * tgp_group_result with naggs = 1.  Replaces hash_join_with_keys +
+ * try_accumulator_grouped (join.rs:544-939, accum_grouped.rs:198-335). */
+typedef struct tgp_join_agg_query {
+  const tgp_hash_table* table;
+  tgp_column probe_keys;
+  tgp_column value;       /* summed column (int64 / int32 ...) */
+  tgp_column group_col;   /* used when group_by == TGP_JOIN_GROUP_PROBE_COL */
+  int32_t group_by;
+  int32_t _pad;
+  int64_t key_base;
+  uint64_t ngroups;
+  uint64_t row_begin, row_end;
+} tgp_join_agg_query;
+int32_t tgp_hash_probe_agg(tgp_ctx* ctx, const tgp_join_agg_query* q, const tgp_group_result* out,
+                           void* stream);
+
+/* ---- synthetic data (counter-based, seeded) ------------------------------- */
+/* Row i of column `column_id`: u = mix64(seed ^ (column_id << 40) ^ i)
+ * (SplitMix64 finaliser), value = lo + mulhi64(u, hi - lo + 1).  Written at
+ * the column's native width.  Deterministic for any row range. */
+int32_t tgp_synth_uniform(tgp_ctx* ctx, tgp_column* col, uint64_t seed, uint64_t column_id,
+                          int64_t lo, int64_t hi, uint64_t row_begin, uint64_t row_end,
+                          void* stream);
+/* out[i] = table[idx(i)], idx(i) = number of entries of the ascending table
+ * cdf[0..n_cdf) that are <= unit(u_i), unit(u) = (u >> 11) * 2^-53, u_i the
+ * counter hash above (cdf[n_cdf-1] must be 1.0).  Used for the Zipf
+ * foreign-key column: cdf is the cumulative rank distribution and table maps
+ * a rank to a build key. */
+int32_t tgp_synth_lookup_cdf(tgp_ctx* ctx, tgp_column* out, uint64_t seed, uint64_t column_id,
+                             const double* d_cdf, uint64_t n_cdf, const int64_t* d_table,
+                             uint64_t row_begin, uint64_t row_end, void* stream);
+
+/* ---- Kernel trait / KernelTable mirror (kernel/mod.rs:47-242) ------------- */
+enum tgp_operator {           /* kernel/mod.rs:47-74, same order */
+  TGP_OP_SCAN_EQ_U64 = 0, TGP_OP_SCAN_RANGE_U64 = 1, TGP_OP_SCAN_MULTI_PREDICATE = 2,
+  TGP_OP_HASH_BUILD = 3, TGP_OP_HASH_PROBE = 4, TGP_OP_AGGREGATE_SUM_F64 = 5,
+  TGP_OP_AGGREGATE_COUNT_DISTINCT = 6, TGP_OP_SIMILARITY_HAMMING = 7, TGP_OP_LEAPFROG_JOIN = 8
+};
+enum tgp_pred_op { TGP_PRED_EQ = 0, TGP_PRED_GT = 1, TGP_PRED_LT = 2 }; /* mod.rs:84-92 */
+enum tgp_tier {               /* memory/tier.rs:11-29, same order */
+  TGP_TIER_L1L2 = 0, TGP_TIER_L3 = 1, TGP_TIER_DDR5 = 2, TGP_TIER_HBM = 3, TGP_TIER_CXL = 4,
+  TGP_TIER_NVME = 5, TGP_TIER_NVMEOF = 6, TGP_TIER_NETWORK = 7
+};
+#define TGP_TARGET_GFX950 100 /* device target beside CpuTarget (cpu.rs:30-41) */
+
+/* repr(C) image of KernelParams (mod.rs:96-119). */
+typedef struct tgp_kernel_params {
+  uint64_t target_u64, low_u64, high_u64;
+  uint32_t max_distance, _pad0;
+  uint64_t cell_count;
+  uint64_t target2_u64, target3_u64;
+  int32_t pred1_op, pred2_op, pred3_op;
+  uint8_t predicate_count, _pad1[3];
+} tgp_kernel_params;
+/* repr(C) image of KernelResult (mod.rs:141-148). */
+typedef struct tgp_kernel_result {
+  uint64_t count;
+  double sum;
+  uint64_t mask;
+} tgp_kernel_result;
+
+typedef struct tgp_kernel_table tgp_kernel_table;
+typedef struct tgp_kernel tgp_kernel;
+
+int32_t tgp_kernel_table_new(tgp_ctx* ctx, tgp_kernel_table** out);
+int32_t tgp_kernel_table_destroy(tgp_kernel_table* t);
+/* select(op, tier): exact (op, gfx950, tier) -> (op, gfx950, Hbm) -> any with
+ * op (mod.rs:209-221).  *out = NULL when no kernel implements op. */
+int32_t tgp_kernel_table_select(tgp_kernel_table* t, int32_t op, int32_t tier, tgp_kernel** out);
+int32_t tgp_kernel_table_len(tgp_kernel_table* t, int32_t* out_n);
+/* list(): i-th entry's operator, target, tier and name. */
+int32_t tgp_kernel_table_entry(tgp_kernel_table* t, int32_t i, int32_t* op, int32_t* target,
+                               int32_t* tier, const char** name);
+int32_t tgp_kernel_info(const tgp_kernel* k, int32_t* op, int32_t* target, int32_t* tier,
+                        const char** name);
+/* Kernel::execute(input, output, &params) -> KernelResult (mod.rs:171-176).
+ * `input`/`output` are host or device pointers with the reference's layouts:
+ *   scans:      input = cell_count u64 cells; result.count (+ mask of first 64)
+ *   HashBuild:  input = cell_count u64 keys; writes a tgp_hash_table* to output
+ *   HashProbe:  input = [tgp_hash_table* (8 B)][cell_count u64 keys]; result.count
+ *   AggregateSumF64: input = cell_count f64 bits; result.sum (f64, fixed
+ *               pairwise order) and result.count
+ * Host inputs are staged through HBM (PCIe); tier HBM kernels expect device
+ * pointers.  Errors return a status; `result` is then zeroed. */
+int32_t tgp_kernel_execute(const tgp_kernel* k, const void* input, void* output,
+                           const tgp_kernel_params* params, tgp_kernel_result* result);
+
+/* ---- test hook -------------------------------------------------------------- */
+/* Make the next `count` operator calls on ctx fail with TGP_ERR_INJECTED
+ * before touching the device (failure-path tests on the host). */
+int32_t tgp_debug_inject_failure(tgp_ctx* ctx, int32_t count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TURBOGP_GFX950_H */

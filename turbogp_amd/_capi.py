@@ -1,0 +1,173 @@
+"""ctypes image of include/turbogp_gfx950.h and loader of the in-tree library.
+
+The library is required: there is no Python or CPU fallback for any operator.
+Loading fails loudly when libturbogp_gfx950.so has not been built.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libturbogp_gfx950.so")
+
+# status codes
+OK = 0
+ERR_INVALID = -1
+ERR_HIP = -2
+ERR_OOM = -3
+ERR_BUDGET = -4
+ERR_UNSUPPORTED = -5
+ERR_NO_DEVICE = -6
+ERR_CAPACITY = -7
+ERR_CANCELLED = -8
+ERR_INJECTED = -9
+
+# dtypes
+I8, U8, I16, U16, I32, U32, I64, U64 = range(8)
+DTYPE_WIDTH = {I8: 1, U8: 1, I16: 2, U16: 2, I32: 4, U32: 4, I64: 8, U64: 8}
+# comparisons (bitmap.rs filter_{eq,ne,lt,le,gt,ge})
+EQ, NE, LT, LE, GT, GE = range(6)
+BIT_AND, BIT_OR, BIT_ANDNOT = range(3)
+AGG_SUM, AGG_COUNT_NONZERO = 0, 1
+JOIN_GROUP_BUILD_PAYLOAD, JOIN_GROUP_PROBE_COL, JOIN_GROUP_NONE = 0, 1, 2
+
+MAX_COLS, MAX_PREDS, MAX_AGGS, MAX_FACTORS, MAX_GROUP_PARTS = 8, 8, 6, 3, 2
+
+
+class Column(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("len", C.c_uint64), ("dtype", C.c_int32),
+                ("_pad", C.c_int32)]
+
+
+class Pred(C.Structure):
+    _fields_ = [("col", C.c_int32), ("op", C.c_int32), ("is_signed", C.c_int32),
+                ("_pad", C.c_int32), ("lit", C.c_int64)]
+
+
+class GroupPart(C.Structure):
+    _fields_ = [("col", C.c_int32), ("divisor", C.c_int32), ("base", C.c_int64),
+                ("card", C.c_int64)]
+
+
+class Agg(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("nfactors", C.c_int32), ("fcol", C.c_int32 * 3),
+                ("_pad", C.c_int32), ("coef", C.c_int64), ("fa", C.c_int64 * 3),
+                ("fb", C.c_int64 * 3)]
+
+
+class AggQuery(C.Structure):
+    _fields_ = [("row_begin", C.c_uint64), ("row_end", C.c_uint64),
+                ("cols", Column * MAX_COLS),
+                ("ncols", C.c_int32), ("npreds", C.c_int32), ("nparts", C.c_int32),
+                ("naggs", C.c_int32),
+                ("preds", Pred * MAX_PREDS), ("parts", GroupPart * MAX_GROUP_PARTS),
+                ("aggs", Agg * MAX_AGGS), ("d_sel_bitmap", C.c_void_p)]
+
+
+class GroupResult(C.Structure):
+    _fields_ = [("d_count", C.c_void_p), ("d_sums", C.c_void_p),
+                ("d_n_out_of_range", C.c_void_p)]
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("arch", C.c_char * 32), ("cu_count", C.c_int32),
+                ("wave_size", C.c_int32), ("hbm_bytes", C.c_int64),
+                ("lds_bytes_per_cu", C.c_int64), ("l2_bytes", C.c_int64),
+                ("clock_khz", C.c_int32), ("mem_clock_khz", C.c_int32)]
+
+
+class JoinAggQuery(C.Structure):
+    _fields_ = [("table", C.c_void_p), ("probe_keys", Column), ("value", Column),
+                ("group_col", Column), ("group_by", C.c_int32), ("_pad", C.c_int32),
+                ("key_base", C.c_int64), ("ngroups", C.c_uint64),
+                ("row_begin", C.c_uint64), ("row_end", C.c_uint64)]
+
+
+class KernelParams(C.Structure):
+    """repr(C) image of KernelParams (kernel/mod.rs:96-119)."""
+    _fields_ = [("target_u64", C.c_uint64), ("low_u64", C.c_uint64), ("high_u64", C.c_uint64),
+                ("max_distance", C.c_uint32), ("_pad0", C.c_uint32),
+                ("cell_count", C.c_uint64), ("target2_u64", C.c_uint64),
+                ("target3_u64", C.c_uint64), ("pred1_op", C.c_int32), ("pred2_op", C.c_int32),
+                ("pred3_op", C.c_int32), ("predicate_count", C.c_uint8),
+                ("_pad1", C.c_uint8 * 3)]
+
+
+class KernelResult(C.Structure):
+    """repr(C) image of KernelResult (kernel/mod.rs:141-148)."""
+    _fields_ = [("count", C.c_uint64), ("sum", C.c_double), ("mask", C.c_uint64)]
+
+
+P = C.c_void_p
+I32_ = C.c_int32
+U64_ = C.c_uint64
+I64_ = C.c_int64
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "tgp_last_error": (C.c_char_p, []),
+    "tgp_abi_version": (I32_, []),
+    "tgp_ctx_create": (I32_, [I32_, U64_, C.POINTER(P)]),
+    "tgp_ctx_destroy": (I32_, [P]),
+    "tgp_device_info_get": (I32_, [P, C.POINTER(DeviceInfo)]),
+    "tgp_stream_create": (I32_, [P, C.POINTER(P)]),
+    "tgp_stream_destroy": (I32_, [P, P]),
+    "tgp_stream_sync": (I32_, [P, P]),
+    "tgp_hbm_alloc": (I32_, [P, U64_, C.POINTER(P)]),
+    "tgp_hbm_free": (I32_, [P, P]),
+    "tgp_hbm_usage": (I32_, [P, C.POINTER(U64_), C.POINTER(U64_), C.POINTER(U64_)]),
+    "tgp_hbm_set_budget": (I32_, [P, U64_]),
+    "tgp_memcpy_h2d": (I32_, [P, P, P, U64_, P]),
+    "tgp_memcpy_d2h": (I32_, [P, P, P, U64_, P]),
+    "tgp_memcpy_d2d": (I32_, [P, P, P, U64_, P]),
+    "tgp_fill_u64": (I32_, [P, P, U64_, U64_, P]),
+    "tgp_column_minmax_sync": (I32_, [P, C.POINTER(Column), C.POINTER(I64_), C.POINTER(I64_)]),
+    "tgp_bitmap_words": (U64_, [U64_]),
+    "tgp_scan_filter": (I32_, [P, C.POINTER(Column), I32_, C.POINTER(Pred), I32_, U64_, U64_,
+                               P, P, P]),
+    "tgp_bitmap_combine": (I32_, [P, I32_, P, P, P, U64_, P]),
+    "tgp_bitmap_not": (I32_, [P, P, P, U64_, P]),
+    "tgp_bitmap_count_range": (I32_, [P, P, U64_, U64_, U64_, P, P]),
+    "tgp_bitmap_to_indices_sync": (I32_, [P, P, U64_, U64_, I32_, P, U64_, C.POINTER(U64_), P]),
+    "tgp_group_agg": (I32_, [P, C.POINTER(AggQuery), C.POINTER(GroupResult), P]),
+    "tgp_group_agg_ngroups": (I32_, [C.POINTER(AggQuery), C.POINTER(U64_)]),
+    "tgp_synth_uniform": (I32_, [P, C.POINTER(Column), U64_, U64_, I64_, I64_, U64_, U64_, P]),
+    "tgp_synth_lookup_cdf": (I32_, [P, C.POINTER(Column), U64_, U64_, P, U64_, P, U64_, U64_,
+                                    P]),
+    "tgp_debug_inject_failure": (I32_, [P, I32_]),
+}
+
+
+class TgpError(RuntimeError):
+    def __init__(self, code, where, msg):
+        super().__init__(f"{where} failed with status {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Loads the operator library (once).  Raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
+            "g.build()'` (make -C turbogp_amd/csrc).  There is no CPU fallback.")
+    L = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(L, name, None)
+        if f is None:
+            continue  # declared in a later ABI; checked by the export test
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(code, where):
+    if code != OK:
+        msg = lib().tgp_last_error()
+        raise TgpError(code, where, msg.decode() if msg else "")
+    return code

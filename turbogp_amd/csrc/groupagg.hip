@@ -1,0 +1,274 @@
+// Fused filter + GROUP BY + exact aggregation.
+//
+// Reference path replaced (query_interpreter):
+//   aggregate.rs:507-697   execute_grouped: mask -> iter_set_bits -> per-chunk
+//                          FxHashMap<keyhash, Vec<usize>> -> per group sum_vec
+//   aggregate.rs:1485-1792 sum_vec: exact i64 for SUM(col), f64 for expressions
+//   accum_grouped.rs:55-409 try_accumulator_grouped: wrapping u64 accumulators
+//   aggregate.rs:1055-1289 scalar (no GROUP BY) aggregates
+// Here one pass reads each referenced column once at its native width, the
+// group key is a dense composite id (the reference keys groups by a combined
+// 64-bit hash, accum_grouped.rs:213-218; for dense small keys both identify
+// the same groups), and every sum is an exact 128-bit integer.
+//
+// Accumulation strategies (chosen by the planner from the group count):
+//   Reg1 / Reg8  per-thread registers, a wave64 shuffle reduction and one set
+//                of 64-bit device atomics per workgroup (<= 8 groups);
+//   Lds          a per-workgroup table in LDS updated with 64-bit LDS atomics,
+//                flushed to the result with device atomics;
+//   Global       64-bit device atomics straight into the result table.
+// int128 sums are kept as (low word, high word); see atomic_add128.
+#include "plan.h"
+
+namespace tgp {
+
+constexpr int kRowsPerThread = kTileRows / kBlock;
+
+// ---- register strategy ----------------------------------------------------------------
+template <int NST, int GMAX, int AMAX>
+__global__ void __launch_bounds__(kBlock) k_agg_reg(DevPlan P) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint32_t wave = threadIdx.x / kWave;
+  uint32_t cnt[GMAX];
+  uint64_t lo[GMAX][AMAX];
+  int32_t hi[GMAX][AMAX];
+#pragma unroll
+  for (int g = 0; g < GMAX; ++g) {
+    cnt[g] = 0;
+#pragma unroll
+    for (int a = 0; a < AMAX; ++a) { lo[g][a] = 0; hi[g][a] = 0; }
+  }
+  uint32_t oor = 0;
+  u32x4 reg[NST];
+  uint32_t addr[NST];
+  uint64_t t = blockIdx.x;
+  if (t < P.ntiles) {
+    const uint64_t r0 = P.row_begin + t * kTileRows;
+    stage_load<NST>(P, r0, min((uint64_t)kTileRows, P.row_end - r0), reg, addr);
+  }
+  for (; t < P.ntiles; t += gridDim.x) {
+    const uint64_t r0 = P.row_begin + t * kTileRows;
+    __syncthreads();
+    stage_store<NST>(lds, reg, addr);
+    __syncthreads();
+    const uint64_t tn = t + gridDim.x;
+    if (tn < P.ntiles) {
+      const uint64_t rn = P.row_begin + tn * kTileRows;
+      stage_load<NST>(P, rn, min((uint64_t)kTileRows, P.row_end - rn), reg, addr);
+    }
+#pragma unroll
+    for (int k = 0; k < kRowsPerThread; ++k) {
+      const uint32_t j = threadIdx.x + k * kBlock;
+      if (r0 + j >= P.row_end || !row_selected(P, lds, r0, j)) continue;
+      const int64_t gid = P.nparts ? group_id(P, lds, r0, j) : 0;
+      if (gid < 0 || gid >= GMAX) { ++oor; continue; }
+      int64_t v[AMAX];
+#pragma unroll
+      for (int a = 0; a < AMAX; ++a) v[a] = a < P.naggs ? agg_value(P, P.aggs[a], lds, r0, j) : 0;
+#pragma unroll
+      for (int g = 0; g < GMAX; ++g) {
+        if (gid == g) {
+          ++cnt[g];
+#pragma unroll
+          for (int a = 0; a < AMAX; ++a)
+            if (a < P.naggs) add128(lo[g][a], hi[g][a], v[a]);
+        }
+      }
+    }
+  }
+  // ---- flush: wave reduction, then one workgroup reduction through LDS ----
+  __syncthreads();
+  constexpr int W = 1 + 2 * AMAX;  // words per group: count, (lo, hi) per agg
+  unsigned long long* red = reinterpret_cast<unsigned long long*>(lds);
+#pragma unroll
+  for (int g = 0; g < GMAX; ++g) {
+    unsigned long long c = cnt[g];
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    if (lane == 0) red[(wave * GMAX + g) * W] = c;
+#pragma unroll
+    for (int a = 0; a < AMAX; ++a) {
+      unsigned long long l = lo[g][a];
+      long long h = hi[g][a];
+      for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long l2 = __shfl_xor(l, off);
+        const long long h2 = __shfl_xor(h, off);
+        const unsigned long long s = l + l2;
+        h += h2 + (long long)(s < l);
+        l = s;
+      }
+      if (lane == 0) {
+        red[(wave * GMAX + g) * W + 1 + 2 * a] = l;
+        red[(wave * GMAX + g) * W + 2 + 2 * a] = (unsigned long long)h;
+      }
+    }
+  }
+  __syncthreads();
+  constexpr int kWaves = kBlock / kWave;
+  for (uint32_t i = threadIdx.x; i < (uint32_t)(GMAX * (1 + AMAX)); i += kBlock) {
+    const int g = i / (1 + AMAX), a = (int)(i % (1 + AMAX)) - 1;
+    if ((uint64_t)g >= P.ngroups || a >= P.naggs) continue;
+    if (a < 0) {
+      unsigned long long c = 0;
+      for (int w = 0; w < kWaves; ++w) c += red[(w * GMAX + g) * W];
+      if (c) atomicAdd(&P.out_count[g], c);
+    } else {
+      unsigned long long l = 0;
+      long long h = 0;
+      for (int w = 0; w < kWaves; ++w) {
+        const unsigned long long l2 = red[(w * GMAX + g) * W + 1 + 2 * a];
+        const unsigned long long s = l + l2;
+        h += (long long)red[(w * GMAX + g) * W + 2 + 2 * a] + (long long)(s < l);
+        l = s;
+      }
+      atomic_add128(&P.out_sums[((uint64_t)g * P.naggs + a) * 2], l, h);
+    }
+  }
+  if (oor && P.out_oor) atomicAdd(P.out_oor, (unsigned long long)oor);
+}
+
+// ---- table strategies (LDS table or global table) ----------------------------------------
+template <int NST, bool USE_LDS>
+__global__ void __launch_bounds__(kBlock) k_agg_table(DevPlan P) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  unsigned long long* tcnt;
+  unsigned long long* tsum;
+  if (USE_LDS) {
+    tcnt = reinterpret_cast<unsigned long long*>(lds + P.stage_bytes);
+    tsum = tcnt + P.ngroups;
+    const uint64_t words = P.ngroups * (1 + 2 * (uint64_t)P.naggs);
+    for (uint64_t i = threadIdx.x; i < words; i += kBlock) tcnt[i] = 0;
+  } else {
+    tcnt = P.out_count;
+    tsum = P.out_sums;
+  }
+  uint32_t oor = 0;
+  u32x4 reg[NST];
+  uint32_t addr[NST];
+  uint64_t t = blockIdx.x;
+  if (t < P.ntiles) {
+    const uint64_t r0 = P.row_begin + t * kTileRows;
+    stage_load<NST>(P, r0, min((uint64_t)kTileRows, P.row_end - r0), reg, addr);
+  }
+  for (; t < P.ntiles; t += gridDim.x) {
+    const uint64_t r0 = P.row_begin + t * kTileRows;
+    __syncthreads();
+    stage_store<NST>(lds, reg, addr);
+    __syncthreads();
+    const uint64_t tn = t + gridDim.x;
+    if (tn < P.ntiles) {
+      const uint64_t rn = P.row_begin + tn * kTileRows;
+      stage_load<NST>(P, rn, min((uint64_t)kTileRows, P.row_end - rn), reg, addr);
+    }
+#pragma unroll
+    for (int k = 0; k < kRowsPerThread; ++k) {
+      const uint32_t j = threadIdx.x + k * kBlock;
+      if (r0 + j >= P.row_end || !row_selected(P, lds, r0, j)) continue;
+      const int64_t gid = group_id(P, lds, r0, j);
+      if (gid < 0) { ++oor; continue; }
+      atomicAdd(&tcnt[gid], 1ull);
+      for (int a = 0; a < P.naggs; ++a) {
+        const int64_t v = agg_value(P, P.aggs[a], lds, r0, j);
+        unsigned long long* p = &tsum[((uint64_t)gid * P.naggs + a) * 2];
+        const unsigned long long old = atomicAdd(p, (unsigned long long)v);
+        const long long h = (long long)((old + (uint64_t)v) < old) - (long long)(v < 0);
+        if (h) atomicAdd(p + 1, (unsigned long long)h);
+      }
+    }
+  }
+  if (USE_LDS) {
+    __syncthreads();
+    for (uint64_t g = threadIdx.x; g < P.ngroups; g += kBlock) {
+      const unsigned long long c = tcnt[g];
+      if (!c) continue;
+      atomicAdd(&P.out_count[g], c);
+      for (int a = 0; a < P.naggs; ++a) {
+        const uint64_t i = (g * P.naggs + a) * 2;
+        atomic_add128(&P.out_sums[i], tsum[i], (int64_t)tsum[i + 1]);
+      }
+    }
+  }
+  if (oor && P.out_oor) atomicAdd(P.out_oor, (unsigned long long)oor);
+}
+
+template <typename K>
+static int32_t launch_persistent(tgp_ctx* ctx, K kern, const DevPlan& P, uint32_t lds_bytes,
+                                 hipStream_t s) {
+  if (lds_bytes > 65536)
+    TGP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds_bytes));
+  int per_cu = 0;
+  TGP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, lds_bytes));
+  if (per_cu < 1) per_cu = 1;
+  uint64_t grid = (uint64_t)persistent_grid(ctx, per_cu);
+  if (grid > P.ntiles) grid = P.ntiles;
+  kern<<<dim3((unsigned)grid), dim3(kBlock), lds_bytes, s>>>(P);
+  TGP_HIP(hipGetLastError());
+  return TGP_OK;
+}
+
+template <int NST>
+static int32_t launch_agg_nst(tgp_ctx* ctx, const DevPlan& P, const PlanInfo& info,
+                              hipStream_t s) {
+  uint32_t lds = info.lds_bytes;
+  switch (info.strategy) {
+    case kAggReg1: {
+      const uint32_t red = 4 * 1 * (1 + 2 * TGP_MAX_AGGS) * 8;
+      return launch_persistent(ctx, k_agg_reg<NST, 1, TGP_MAX_AGGS>, P, lds > red ? lds : red, s);
+    }
+    case kAggReg8: {
+      const uint32_t red = 4 * 8 * (1 + 2 * 4) * 8;
+      return launch_persistent(ctx, k_agg_reg<NST, 8, 4>, P, lds > red ? lds : red, s);
+    }
+    case kAggLds: return launch_persistent(ctx, k_agg_table<NST, true>, P, lds, s);
+    default: return launch_persistent(ctx, k_agg_table<NST, false>, P, lds, s);
+  }
+}
+
+int32_t run_agg(tgp_ctx* ctx, const DevPlan& P, const PlanInfo& info, hipStream_t s) {
+  // Results are accumulated with atomics: clear them first (a fill kernel, so the
+  // sequence can be captured in a graph).
+  TGP_TRY(fill_u64(s, reinterpret_cast<uint64_t*>(P.out_count), P.ngroups, 0));
+  TGP_TRY(fill_u64(s, reinterpret_cast<uint64_t*>(P.out_sums), P.ngroups * P.naggs * 2, 0));
+  if (P.out_oor) TGP_TRY(fill_u64(s, reinterpret_cast<uint64_t*>(P.out_oor), 1, 0));
+  if (P.ntiles == 0) return TGP_OK;
+  switch (info.nst) {
+    case 2: return launch_agg_nst<2>(ctx, P, info, s);
+    case 4: return launch_agg_nst<4>(ctx, P, info, s);
+    case 8: return launch_agg_nst<8>(ctx, P, info, s);
+    case 12: return launch_agg_nst<12>(ctx, P, info, s);
+    case kMaxNst: return launch_agg_nst<kMaxNst>(ctx, P, info, s);
+  }
+  set_error("no aggregation kernel for %d staging rounds", info.nst);
+  return TGP_ERR_UNSUPPORTED;
+}
+
+}  // namespace tgp
+
+using namespace tgp;
+
+extern "C" {
+
+int32_t tgp_group_agg_ngroups(const tgp_agg_query* q, uint64_t* out_ngroups) {
+  TGP_REQUIRE(q && out_ngroups, "tgp_group_agg_ngroups: null argument");
+  uint64_t g = 1;
+  for (int k = 0; k < q->nparts && k < TGP_MAX_GROUP_PARTS; ++k) g *= (uint64_t)q->parts[k].card;
+  *out_ngroups = g;
+  return TGP_OK;
+}
+
+int32_t tgp_group_agg(tgp_ctx* ctx, const tgp_agg_query* q, const tgp_group_result* out,
+                      void* stream) {
+  TGP_REQUIRE(ctx && q && out, "tgp_group_agg: null argument");
+  TGP_TRY(check_injected(ctx));
+  TGP_REQUIRE(out->d_count && (q->naggs == 0 || out->d_sums), "tgp_group_agg: null result buffer");
+  DevPlan P;
+  PlanInfo info;
+  TGP_TRY(plan_agg(q, &P, &info));
+  P.out_count = reinterpret_cast<unsigned long long*>(out->d_count);
+  P.out_sums = reinterpret_cast<unsigned long long*>(out->d_sums);
+  P.out_oor = reinterpret_cast<unsigned long long*>(out->d_n_out_of_range);
+  return run_agg(ctx, P, info, pick_stream(ctx, stream));
+}
+
+}  // extern "C"

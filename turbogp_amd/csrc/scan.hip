@@ -1,0 +1,395 @@
+// Predicate scan -> selection bitmap + count, bitmap algebra, compaction,
+// column min/max.
+//
+// Reference: exec/bitmap.rs (Bitmap :42-273, SetBitIter :278-314, filter_*
+// :321-477, filter_u64_avx512_inner :642-694) and query_interpreter/expr.rs
+// build_mask (:419-431, AND chain :633-656).  The reference evaluates one
+// column pass per predicate leaf plus one AND pass per leaf; here all leaves
+// are fused into one pass over HBM, and the bitmap word of 64 rows is a
+// single wave64 ballot.
+#include "plan.h"
+
+namespace tgp {
+
+template <int NST>
+__global__ void __launch_bounds__(kBlock) k_scan(DevPlan P) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint32_t wave = threadIdx.x / kWave;
+  u32x4 reg[NST];
+  uint32_t addr[NST];
+  uint64_t t = blockIdx.x;
+  if (t < P.ntiles) {
+    const uint64_t r0 = P.row_begin + t * kTileRows;
+    stage_load<NST>(P, r0, min((uint64_t)kTileRows, P.row_end - r0), reg, addr);
+  }
+  unsigned long long cnt = 0;
+  for (; t < P.ntiles; t += gridDim.x) {
+    const uint64_t r0 = P.row_begin + t * kTileRows;
+    __syncthreads();
+    stage_store<NST>(lds, reg, addr);
+    __syncthreads();
+    const uint64_t tn = t + gridDim.x;
+    if (tn < P.ntiles) {
+      const uint64_t rn = P.row_begin + tn * kTileRows;
+      stage_load<NST>(P, rn, min((uint64_t)kTileRows, P.row_end - rn), reg, addr);
+    }
+#pragma unroll
+    for (int k = 0; k < kTileRows / kBlock; ++k) {
+      const uint32_t j = threadIdx.x + k * kBlock;
+      const uint64_t row = r0 + j;
+      const bool sel = row < P.row_end && row_selected(P, lds, r0, j);
+      const unsigned long long word = __ballot(sel);
+      const uint64_t wrow = r0 + k * kBlock + wave * kWave;  // first row of this word
+      if (lane == 0 && wrow < P.row_end) {
+        if (P.out_bitmap) P.out_bitmap[(wrow - P.row_begin) >> 6] = word;
+        cnt += __popcll(word);
+      }
+    }
+  }
+  if (P.out_count && lane == 0 && cnt) atomicAdd(P.out_count, cnt);
+}
+
+template <int NST>
+static int32_t launch_scan(tgp_ctx* ctx, const DevPlan& P, const PlanInfo& info, hipStream_t s) {
+  auto kern = k_scan<NST>;
+  if (info.lds_bytes > 65536)
+    TGP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)info.lds_bytes));
+  int per_cu = 0;
+  TGP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, info.lds_bytes));
+  if (per_cu < 1) per_cu = 1;
+  uint64_t grid = (uint64_t)persistent_grid(ctx, per_cu);
+  if (grid > P.ntiles) grid = P.ntiles;
+  kern<<<dim3((unsigned)grid), dim3(kBlock), info.lds_bytes, s>>>(P);
+  TGP_HIP(hipGetLastError());
+  return TGP_OK;
+}
+
+int32_t run_scan(tgp_ctx* ctx, const DevPlan& P, const PlanInfo& info, hipStream_t s) {
+  if (P.ntiles == 0) return TGP_OK;
+  switch (info.nst) {
+    case 2: return launch_scan<2>(ctx, P, info, s);
+    case 4: return launch_scan<4>(ctx, P, info, s);
+    case 8: return launch_scan<8>(ctx, P, info, s);
+    case 12: return launch_scan<12>(ctx, P, info, s);
+    case kMaxNst: return launch_scan<kMaxNst>(ctx, P, info, s);
+  }
+  set_error("no scan kernel for %d staging rounds", info.nst);
+  return TGP_ERR_UNSUPPORTED;
+}
+
+// ---- bitmap algebra -------------------------------------------------------------
+
+__device__ __forceinline__ uint64_t tail_mask(uint64_t nbits, uint64_t word) {
+  const uint64_t first = word * 64;
+  if (first + 64 <= nbits) return ~0ull;
+  const uint64_t valid = nbits > first ? nbits - first : 0;
+  return valid ? (~0ull >> (64 - valid)) : 0ull;
+}
+
+__global__ void __launch_bounds__(256) k_bitmap_combine(int op, const uint64_t* __restrict__ a,
+                                                        const uint64_t* __restrict__ b,
+                                                        uint64_t* out, uint64_t nbits) {
+  const uint64_t nw = (nbits + 63) / 64;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nw;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t x = a[i], y = b[i], r;
+    if (op == TGP_BIT_AND) r = x & y;
+    else if (op == TGP_BIT_OR) r = x | y;
+    else r = x & ~y;
+    out[i] = r & tail_mask(nbits, i);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_bitmap_not(const uint64_t* __restrict__ a, uint64_t* out,
+                                                    uint64_t nbits) {
+  const uint64_t nw = (nbits + 63) / 64;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nw;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = ~a[i] & tail_mask(nbits, i);
+}
+
+__global__ void __launch_bounds__(256) k_bitmap_count(const uint64_t* __restrict__ a,
+                                                      uint64_t begin, uint64_t end,
+                                                      unsigned long long* out) {
+  unsigned long long c = 0;
+  const uint64_t w0 = begin / 64, w1 = (end + 63) / 64;
+  for (uint64_t i = w0 + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < w1;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t x = a[i];
+    const uint64_t lo = i * 64, hi = lo + 64;
+    if (begin > lo) x &= ~0ull << (begin - lo);
+    if (end < hi) x &= (end > lo) ? (~0ull >> (hi - end)) : 0ull;
+    c += __popcll(x);
+  }
+  // wave reduction then one atomic per wave
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
+}
+
+static unsigned grid_for(uint64_t items) {
+  uint64_t g = (items + 255) / 256;
+  if (g > 8192) g = 8192;
+  return (unsigned)(g ? g : 1);
+}
+
+// ---- compaction -------------------------------------------------------------------
+// Three passes: per-block popcounts (256 words = 16384 bits per block), an
+// exclusive scan of the block counts in one workgroup, then the scatter:
+// a wave handles one word at a time, lane l owns bit l, and its output slot
+// is the block offset + the word's prefix + popcount(word & lanes below l),
+// so 64 consecutive rows are written by 64 consecutive lanes.
+constexpr int kCompactWords = 256;
+
+__global__ void __launch_bounds__(256) k_compact_count(const uint64_t* __restrict__ bm,
+                                                       uint64_t nbits, uint64_t* block_cnt) {
+  __shared__ unsigned long long red[4];
+  const uint64_t nw = (nbits + 63) / 64;
+  const uint64_t w = blockIdx.x * (uint64_t)kCompactWords + threadIdx.x;
+  unsigned long long c = w < nw ? __popcll(bm[w] & tail_mask(nbits, w)) : 0;
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) block_cnt[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// Exclusive scan of n block counts in place; total written to block_cnt[n].
+__global__ void __launch_bounds__(1024) k_compact_scan(uint64_t* block_cnt, uint64_t n) {
+  __shared__ unsigned long long part[1024];
+  const uint64_t per = (n + 1023) / 1024;
+  const uint64_t b = threadIdx.x * per, e = min(b + per, n);
+  unsigned long long s = 0;
+  for (uint64_t i = b; i < e; ++i) s += block_cnt[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    unsigned long long v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  unsigned long long run = threadIdx.x ? part[threadIdx.x - 1] : 0;
+  for (uint64_t i = b; i < e; ++i) {
+    unsigned long long c = block_cnt[i];
+    block_cnt[i] = run;
+    run += c;
+  }
+  if (threadIdx.x == 1023) block_cnt[n] = part[1023];
+}
+
+template <typename IDX>
+__global__ void __launch_bounds__(256) k_compact_scatter(const uint64_t* __restrict__ bm,
+                                                         uint64_t nbits, uint64_t base,
+                                                         const uint64_t* __restrict__ block_off,
+                                                         IDX* __restrict__ out) {
+  __shared__ uint64_t words[kCompactWords];
+  __shared__ unsigned long long pre[kCompactWords];
+  const uint64_t nw = (nbits + 63) / 64;
+  const uint64_t w0 = blockIdx.x * (uint64_t)kCompactWords;
+  const uint64_t w = w0 + threadIdx.x;
+  const uint64_t word = w < nw ? (bm[w] & tail_mask(nbits, w)) : 0;
+  words[threadIdx.x] = word;
+  pre[threadIdx.x] = __popcll(word);
+  __syncthreads();
+  for (int off = 1; off < kCompactWords; off <<= 1) {
+    unsigned long long v = threadIdx.x >= (unsigned)off ? pre[threadIdx.x - off] : 0;
+    __syncthreads();
+    pre[threadIdx.x] += v;
+    __syncthreads();
+  }
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x / 64;
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const uint64_t boff = block_off[blockIdx.x];
+  for (int i = wave; i < kCompactWords; i += 4) {
+    const uint64_t x = words[i];
+    if (!x) continue;
+    const uint64_t start = boff + (i ? pre[i - 1] : 0);
+    if ((x >> lane) & 1) {
+      const uint64_t pos = start + __popcll(x & below);
+      out[pos] = (IDX)(base + (w0 + i) * 64 + lane);
+    }
+  }
+}
+
+// ---- min / max ------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_minmax(const uint8_t* __restrict__ data, int32_t dtype,
+                                                uint64_t n, long long* out) {
+  const bool sgn = dtype == TGP_I8 || dtype == TGP_I16 || dtype == TGP_I32 || dtype == TGP_I64;
+  long long mn = sgn ? LLONG_MAX : (long long)~0ull, mx = sgn ? LLONG_MIN : 0;
+  unsigned long long umn = ~0ull, umx = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    long long v;
+    switch (dtype) {
+      case TGP_I8: v = ((const int8_t*)data)[i]; break;
+      case TGP_U8: v = data[i]; break;
+      case TGP_I16: v = ((const int16_t*)data)[i]; break;
+      case TGP_U16: v = ((const uint16_t*)data)[i]; break;
+      case TGP_I32: v = ((const int32_t*)data)[i]; break;
+      case TGP_U32: v = ((const uint32_t*)data)[i]; break;
+      default: v = ((const long long*)data)[i]; break;
+    }
+    if (sgn) { mn = v < mn ? v : mn; mx = v > mx ? v : mx; }
+    else {
+      const unsigned long long u = (unsigned long long)v;
+      umn = u < umn ? u : umn; umx = u > umx ? u : umx;
+    }
+  }
+  if (sgn) { atomicMin(&out[0], mn); atomicMax(&out[1], mx); }
+  else {
+    atomicMin((unsigned long long*)&out[0], umn);
+    atomicMax((unsigned long long*)&out[1], umx);
+  }
+}
+
+}  // namespace tgp
+
+using namespace tgp;
+
+extern "C" {
+
+uint64_t tgp_bitmap_words(uint64_t nbits) { return (nbits + 63) / 64; }
+
+int32_t tgp_scan_filter(tgp_ctx* ctx, const tgp_column* cols, int32_t ncols, const tgp_pred* preds,
+                        int32_t npreds, uint64_t row_begin, uint64_t row_end, uint64_t* d_bitmap,
+                        uint64_t* d_count, void* stream) {
+  TGP_REQUIRE(ctx, "tgp_scan_filter: null ctx");
+  TGP_TRY(check_injected(ctx));
+  DevPlan P;
+  std::memset(&P, 0, sizeof(P));
+  PlanInfo info;
+  TGP_TRY(plan_slots(cols, ncols, row_begin, row_end, &P, &info));
+  TGP_TRY(plan_preds(preds, npreds, ncols, &P));
+  P.out_bitmap = reinterpret_cast<unsigned long long*>(d_bitmap);
+  P.out_count = reinterpret_cast<unsigned long long*>(d_count);
+  if (ncols == 0) {
+    // No column to stage: a one-byte dummy slot would still need memory, so
+    // an empty predicate list over no columns selects every row directly.
+    hipStream_t s = pick_stream(ctx, stream);
+    const uint64_t n = row_end - row_begin;
+    if (d_bitmap) {
+      TGP_TRY(fill_u64(s, d_bitmap, tgp_bitmap_words(n), ~0ull));
+      if (n % 64) {
+        const uint64_t last = ~0ull >> (64 - n % 64);
+        TGP_HIP(hipMemcpyAsync(d_bitmap + n / 64, &last, 8, hipMemcpyHostToDevice, s));
+        TGP_HIP(hipStreamSynchronize(s));
+      }
+    }
+    if (d_count) {
+      uint64_t cur = 0;
+      TGP_HIP(hipMemcpyAsync(&cur, d_count, 8, hipMemcpyDeviceToHost, s));
+      TGP_HIP(hipStreamSynchronize(s));
+      cur += n;
+      TGP_HIP(hipMemcpyAsync(d_count, &cur, 8, hipMemcpyHostToDevice, s));
+      TGP_HIP(hipStreamSynchronize(s));
+    }
+    return TGP_OK;
+  }
+  return run_scan(ctx, P, info, pick_stream(ctx, stream));
+}
+
+int32_t tgp_bitmap_combine(tgp_ctx* ctx, int32_t op, const uint64_t* d_a, const uint64_t* d_b,
+                           uint64_t* d_out, uint64_t nbits, void* stream) {
+  TGP_REQUIRE(ctx && (nbits == 0 || (d_a && d_b && d_out)), "tgp_bitmap_combine: null argument");
+  TGP_REQUIRE(op >= TGP_BIT_AND && op <= TGP_BIT_ANDNOT, "tgp_bitmap_combine: bad op %d", op);
+  TGP_TRY(check_injected(ctx));
+  if (nbits == 0) return TGP_OK;
+  k_bitmap_combine<<<grid_for((nbits + 63) / 64), 256, 0, pick_stream(ctx, stream)>>>(
+      op, d_a, d_b, d_out, nbits);
+  TGP_HIP(hipGetLastError());
+  return TGP_OK;
+}
+
+int32_t tgp_bitmap_not(tgp_ctx* ctx, const uint64_t* d_a, uint64_t* d_out, uint64_t nbits,
+                       void* stream) {
+  TGP_REQUIRE(ctx && (nbits == 0 || (d_a && d_out)), "tgp_bitmap_not: null argument");
+  TGP_TRY(check_injected(ctx));
+  if (nbits == 0) return TGP_OK;
+  k_bitmap_not<<<grid_for((nbits + 63) / 64), 256, 0, pick_stream(ctx, stream)>>>(d_a, d_out,
+                                                                                 nbits);
+  TGP_HIP(hipGetLastError());
+  return TGP_OK;
+}
+
+int32_t tgp_bitmap_count_range(tgp_ctx* ctx, const uint64_t* d_a, uint64_t nbits, uint64_t begin,
+                               uint64_t end, uint64_t* d_count, void* stream) {
+  TGP_REQUIRE(ctx && d_count, "tgp_bitmap_count_range: null argument");
+  TGP_TRY(check_injected(ctx));
+  if (end > nbits) end = nbits;
+  if (begin >= end) return TGP_OK;  // count_ones_range returns 0 (bitmap.rs:249-251)
+  TGP_REQUIRE(d_a, "tgp_bitmap_count_range: null bitmap");
+  const uint64_t words = (end + 63) / 64 - begin / 64;
+  k_bitmap_count<<<grid_for(words), 256, 0, pick_stream(ctx, stream)>>>(
+      d_a, begin, end, reinterpret_cast<unsigned long long*>(d_count));
+  TGP_HIP(hipGetLastError());
+  return TGP_OK;
+}
+
+int32_t tgp_bitmap_to_indices_sync(tgp_ctx* ctx, const uint64_t* d_bitmap, uint64_t nbits,
+                                   uint64_t base, int32_t index_bytes, void* d_out,
+                                   uint64_t out_capacity, uint64_t* h_count, void* stream) {
+  TGP_REQUIRE(ctx && h_count, "tgp_bitmap_to_indices_sync: null argument");
+  TGP_REQUIRE(index_bytes == 4 || index_bytes == 8, "index_bytes must be 4 or 8");
+  TGP_TRY(check_injected(ctx));
+  *h_count = 0;
+  if (nbits == 0) return TGP_OK;
+  TGP_REQUIRE(d_bitmap, "tgp_bitmap_to_indices_sync: null bitmap");
+  if (index_bytes == 4)
+    TGP_REQUIRE(base + nbits <= (1ull << 32), "u32 indices cannot hold rows up to %llu",
+                (unsigned long long)(base + nbits));
+  hipStream_t s = pick_stream(ctx, stream);
+  const uint64_t nw = (nbits + 63) / 64;
+  const uint64_t nblocks = (nw + kCompactWords - 1) / kCompactWords;
+  void* scratch = nullptr;
+  TGP_TRY(get_scratch(ctx, s, (nblocks + 1) * 8, &scratch));
+  uint64_t* bc = static_cast<uint64_t*>(scratch);
+  k_compact_count<<<(unsigned)nblocks, 256, 0, s>>>(d_bitmap, nbits, bc);
+  TGP_HIP(hipGetLastError());
+  k_compact_scan<<<1, 1024, 0, s>>>(bc, nblocks);
+  TGP_HIP(hipGetLastError());
+  uint64_t total = 0;
+  TGP_HIP(hipMemcpyAsync(&total, bc + nblocks, 8, hipMemcpyDeviceToHost, s));
+  TGP_HIP(hipStreamSynchronize(s));
+  *h_count = total;
+  if (total > out_capacity) {
+    set_error("compaction needs %llu entries, capacity %llu", (unsigned long long)total,
+              (unsigned long long)out_capacity);
+    return TGP_ERR_CAPACITY;
+  }
+  if (total == 0) return TGP_OK;
+  TGP_REQUIRE(d_out, "tgp_bitmap_to_indices_sync: null output");
+  if (index_bytes == 8)
+    k_compact_scatter<unsigned long long><<<(unsigned)nblocks, 256, 0, s>>>(
+        d_bitmap, nbits, base, bc, static_cast<unsigned long long*>(d_out));
+  else
+    k_compact_scatter<uint32_t><<<(unsigned)nblocks, 256, 0, s>>>(d_bitmap, nbits, base, bc,
+                                                                  static_cast<uint32_t*>(d_out));
+  TGP_HIP(hipGetLastError());
+  TGP_HIP(hipStreamSynchronize(s));
+  return TGP_OK;
+}
+
+int32_t tgp_column_minmax_sync(tgp_ctx* ctx, const tgp_column* col, int64_t* h_min,
+                               int64_t* h_max) {
+  TGP_REQUIRE(ctx && col && h_min && h_max, "tgp_column_minmax_sync: null argument");
+  TGP_TRY(check_column(col, "tgp_column_minmax_sync"));
+  TGP_REQUIRE(col->len > 0, "tgp_column_minmax_sync: empty column");
+  hipStream_t s = ctx->stream;
+  void* scratch = nullptr;
+  TGP_TRY(get_scratch(ctx, s, 16, &scratch));
+  const bool sgn = dtype_signed(col->dtype);
+  long long init[2] = {sgn ? LLONG_MAX : (long long)~0ull, sgn ? LLONG_MIN : 0};
+  TGP_HIP(hipMemcpyAsync(scratch, init, 16, hipMemcpyHostToDevice, s));
+  k_minmax<<<grid_for(col->len), 256, 0, s>>>(static_cast<const uint8_t*>(col->data), col->dtype,
+                                              col->len, static_cast<long long*>(scratch));
+  TGP_HIP(hipGetLastError());
+  long long r[2];
+  TGP_HIP(hipMemcpyAsync(r, scratch, 16, hipMemcpyDeviceToHost, s));
+  TGP_HIP(hipStreamSynchronize(s));
+  *h_min = r[0];
+  *h_max = r[1];
+  return TGP_OK;
+}
+
+}  // extern "C"

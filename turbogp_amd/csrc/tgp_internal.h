@@ -1,0 +1,103 @@
+// Internal definitions shared by the operator library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+
+#include "../../include/turbogp_gfx950.h"
+
+namespace tgp {
+
+// ---- errors ------------------------------------------------------------------
+void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+
+#define TGP_HIP(call)                                                            \
+  do {                                                                           \
+    hipError_t e_ = (call);                                                      \
+    if (e_ != hipSuccess) {                                                      \
+      ::tgp::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),     \
+                       __FILE__, __LINE__);                                      \
+      return e_ == hipErrorOutOfMemory ? TGP_ERR_OOM : TGP_ERR_HIP;              \
+    }                                                                            \
+  } while (0)
+
+#define TGP_REQUIRE(cond, ...)                                                   \
+  do {                                                                           \
+    if (!(cond)) {                                                               \
+      ::tgp::set_error(__VA_ARGS__);                                             \
+      return TGP_ERR_INVALID;                                                    \
+    }                                                                            \
+  } while (0)
+
+#define TGP_TRY(expr)                 \
+  do {                                \
+    int32_t rc_ = (expr);             \
+    if (rc_ != TGP_OK) return rc_;    \
+  } while (0)
+
+// ---- launch geometry -----------------------------------------------------------
+constexpr int kWave = 64;
+constexpr int kBlock = 256;            // threads per workgroup for streaming kernels
+constexpr int kTileRows = 1024;        // rows staged per tile in the fused kernels
+
+inline int dtype_width(int32_t dt) {
+  switch (dt) {
+    case TGP_I8: case TGP_U8: return 1;
+    case TGP_I16: case TGP_U16: return 2;
+    case TGP_I32: case TGP_U32: return 4;
+    case TGP_I64: case TGP_U64: return 8;
+    default: return 0;
+  }
+}
+inline bool dtype_signed(int32_t dt) {
+  return dt == TGP_I8 || dt == TGP_I16 || dt == TGP_I32 || dt == TGP_I64;
+}
+
+// ---- scratch arena: device memory reused by calls on one stream ----------------
+struct Scratch {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace tgp
+
+struct tgp_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  tgp_device_info info{};
+  // HBM tier accounting (memory/tier.rs MemoryTier::Hbm).
+  std::mutex mu;
+  std::map<void*, uint64_t> allocs;
+  uint64_t used = 0, peak = 0, budget = 0;
+  // Per-stream scratch (stream-ordered reuse).
+  std::map<hipStream_t, tgp::Scratch> scratch;
+  std::atomic<int32_t> inject_failures{0};
+};
+
+namespace tgp {
+// Returns the stream to use for a call.
+inline hipStream_t pick_stream(tgp_ctx* ctx, void* s) {
+  return s ? static_cast<hipStream_t>(s) : ctx->stream;
+}
+// Consumes one injected failure, if any (host-side failure tests).
+int32_t check_injected(tgp_ctx* ctx);
+// Device scratch of at least `bytes` for `stream` (grown on demand; not accounted
+// against the HBM budget, bounded by the largest call).
+int32_t get_scratch(tgp_ctx* ctx, hipStream_t stream, size_t bytes, void** out);
+// Tier-accounted allocation used by library objects (hash tables).
+int32_t hbm_alloc(tgp_ctx* ctx, uint64_t bytes, void** out);
+int32_t hbm_free(tgp_ctx* ctx, void* p);
+// Device fill helper (kernel based, graph safe).
+int32_t fill_u64(hipStream_t s, uint64_t* dst, uint64_t n, uint64_t value);
+// Number of workgroups that fill the device for a persistent streaming kernel.
+int persistent_grid(tgp_ctx* ctx, int wg_per_cu);
+// Validates a column descriptor (non-null data, known dtype, 16-byte alignment).
+int32_t check_column(const tgp_column* c, const char* what);
+}  // namespace tgp

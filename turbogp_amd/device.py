@@ -1,0 +1,310 @@
+"""Device context, HBM-tier columns and the operator calls of the scan /
+group-aggregate path, over the C ABI (include/turbogp_gfx950.h).
+
+This is the host-side mirror of the reference's operator surface:
+  - `DeviceColumn` / `DeviceTable`  <- datasource/table.rs Table (columns at
+    native width, resident in the HBM tier, memory/tier.rs MemoryTier::Hbm)
+  - `Context.scan_filter`           <- exec/bitmap.rs filter_* + and_inplace
+  - `DeviceBitmap`                  <- exec/bitmap.rs Bitmap / SetBitIter
+  - `Context.group_agg`             <- query_interpreter/aggregate.rs execute_grouped
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _capi as A
+
+_NP_DTYPE = {A.I8: np.int8, A.U8: np.uint8, A.I16: np.int16, A.U16: np.uint16,
+             A.I32: np.int32, A.U32: np.uint32, A.I64: np.int64, A.U64: np.uint64}
+_FROM_NP = {np.dtype(v): k for k, v in _NP_DTYPE.items()}
+
+
+def dtype_of(arr):
+    return _FROM_NP[np.asarray(arr).dtype]
+
+
+def int128(lo, hi):
+    """Python int from the (low, high) 64-bit words of a two's complement int128."""
+    v = (int(hi) << 64) | int(lo)
+    return v - (1 << 128) if v >> 127 else v
+
+
+class DeviceBuffer:
+    """A tier-accounted HBM allocation."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        A.check(A.lib().tgp_hbm_alloc(ctx.handle, self.nbytes, C.byref(p)), "tgp_hbm_alloc")
+        self.ptr = p.value
+
+    def free(self):
+        if self.ptr is not None and self.ctx.handle:
+            A.check(A.lib().tgp_hbm_free(self.ctx.handle, self.ptr), "tgp_hbm_free")
+        self.ptr = None
+
+    def upload(self, arr, stream=None):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        A.check(A.lib().tgp_memcpy_h2d(self.ctx.handle, self.ptr, arr.ctypes.data, arr.nbytes,
+                                       stream), "tgp_memcpy_h2d")
+
+    def download(self, dtype, count, stream=None, offset_bytes=0):
+        out = np.empty(int(count), dtype=dtype)
+        if out.nbytes:
+            A.check(A.lib().tgp_memcpy_d2h(self.ctx.handle, out.ctypes.data,
+                                           self.ptr + offset_bytes, out.nbytes, stream),
+                    "tgp_memcpy_d2h")
+        return out
+
+    def fill_u64(self, value=0, nwords=None, stream=None):
+        n = self.nbytes // 8 if nwords is None else nwords
+        A.check(A.lib().tgp_fill_u64(self.ctx.handle, self.ptr, n, value & (2**64 - 1), stream),
+                "tgp_fill_u64")
+
+
+class DeviceColumn:
+    """A column resident in the HBM tier at its native width."""
+
+    def __init__(self, ctx, n, dtype, name=None):
+        self.ctx = ctx
+        self.n = int(n)
+        self.dtype = dtype
+        self.width = A.DTYPE_WIDTH[dtype]
+        self.name = name
+        self.buf = DeviceBuffer(ctx, max(16, self.n * self.width))
+
+    @classmethod
+    def from_numpy(cls, ctx, arr, name=None):
+        arr = np.ascontiguousarray(arr)
+        c = cls(ctx, arr.shape[0], dtype_of(arr), name)
+        if arr.nbytes:
+            c.buf.upload(arr)
+        return c
+
+    def desc(self):
+        return A.Column(self.buf.ptr, self.n, self.dtype, 0)
+
+    def to_numpy(self):
+        return self.buf.download(_NP_DTYPE[self.dtype], self.n)
+
+    def minmax(self):
+        lo, hi = C.c_int64(), C.c_int64()
+        d = self.desc()
+        A.check(A.lib().tgp_column_minmax_sync(self.ctx.handle, C.byref(d), C.byref(lo),
+                                               C.byref(hi)), "tgp_column_minmax_sync")
+        return lo.value, hi.value
+
+    def free(self):
+        self.buf.free()
+
+
+class DeviceBitmap:
+    """Selection bitmap on the device (bit i <-> row i, LSB-first words)."""
+
+    def __init__(self, ctx, nbits):
+        self.ctx = ctx
+        self.nbits = int(nbits)
+        self.nwords = int(A.lib().tgp_bitmap_words(self.nbits))
+        self.buf = DeviceBuffer(ctx, max(8, self.nwords * 8))
+
+    def words(self):
+        return self.buf.download(np.uint64, self.nwords)
+
+    def to_bytes(self):
+        """The reference's Vec<u8> layout: ceil(nbits/8) LSB-first bytes."""
+        return self.words().view(np.uint8)[: (self.nbits + 7) // 8].copy()
+
+    def count_ones_range(self, begin, end):
+        cnt = DeviceBuffer(self.ctx, 8)
+        try:
+            cnt.fill_u64(0)
+            A.check(A.lib().tgp_bitmap_count_range(self.ctx.handle, self.buf.ptr, self.nbits,
+                                                   begin, end, cnt.ptr, None),
+                    "tgp_bitmap_count_range")
+            return int(cnt.download(np.uint64, 1)[0])
+        finally:
+            cnt.free()
+
+    def count_ones(self):
+        return self.count_ones_range(0, self.nbits)
+
+    def combine(self, other, op):
+        out = DeviceBitmap(self.ctx, min(self.nbits, other.nbits))
+        A.check(A.lib().tgp_bitmap_combine(self.ctx.handle, op, self.buf.ptr, other.buf.ptr,
+                                           out.buf.ptr, out.nbits, None), "tgp_bitmap_combine")
+        return out
+
+    def not_(self):
+        out = DeviceBitmap(self.ctx, self.nbits)
+        A.check(A.lib().tgp_bitmap_not(self.ctx.handle, self.buf.ptr, out.buf.ptr, self.nbits,
+                                       None), "tgp_bitmap_not")
+        return out
+
+    def iter_set_bits(self, base=0, index_bytes=8):
+        """Ascending indices of the set bits (SetBitIter) computed on the device."""
+        cap = max(1, self.count_ones())
+        out = DeviceBuffer(self.ctx, cap * index_bytes)
+        try:
+            n = C.c_uint64()
+            A.check(A.lib().tgp_bitmap_to_indices_sync(self.ctx.handle, self.buf.ptr, self.nbits,
+                                                       base, index_bytes, out.ptr, cap,
+                                                       C.byref(n), None),
+                    "tgp_bitmap_to_indices_sync")
+            return out.download(np.uint64 if index_bytes == 8 else np.uint32, n.value)
+        finally:
+            out.free()
+
+    def free(self):
+        self.buf.free()
+
+
+class Context:
+    """tgp_ctx: one gfx950 device, its stream and its HBM tier."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0):
+        h = C.c_void_p()
+        A.check(A.lib().tgp_ctx_create(device, hbm_budget_bytes, C.byref(h)), "tgp_ctx_create")
+        self.handle = h.value
+        info = A.DeviceInfo()
+        A.check(A.lib().tgp_device_info_get(self.handle, C.byref(info)), "tgp_device_info_get")
+        self.info = info
+
+    def close(self):
+        if self.handle:
+            A.lib().tgp_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sync(self, stream=None):
+        A.check(A.lib().tgp_stream_sync(self.handle, stream), "tgp_stream_sync")
+
+    def hbm_usage(self):
+        u, b, p = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        A.check(A.lib().tgp_hbm_usage(self.handle, C.byref(u), C.byref(b), C.byref(p)),
+                "tgp_hbm_usage")
+        return u.value, b.value, p.value
+
+    # ---- scan -------------------------------------------------------------------------
+    def scan_filter(self, cols, preds, row_begin=0, row_end=None, bitmap=True, count=True,
+                    stream=None):
+        """AND of predicate leaves (col_index, op, lit, is_signed) over cols.
+        Returns (DeviceBitmap or None, count or None)."""
+        n = cols[0].n if cols else 0
+        row_end = n if row_end is None else row_end
+        carr = (A.Column * max(1, len(cols)))(*[c.desc() for c in cols])
+        parr = (A.Pred * max(1, len(preds)))(
+            *[A.Pred(ci, op, int(bool(sg)), 0, lit) for (ci, op, lit, sg) in preds])
+        bm = DeviceBitmap(self, row_end - row_begin) if bitmap else None
+        cnt = DeviceBuffer(self, 8) if count else None
+        try:
+            if cnt:
+                cnt.fill_u64(0, stream=stream)
+            A.check(A.lib().tgp_scan_filter(self.handle, carr, len(cols), parr, len(preds),
+                                            row_begin, row_end, bm.buf.ptr if bm else None,
+                                            cnt.ptr if cnt else None, stream),
+                    "tgp_scan_filter")
+            c = int(cnt.download(np.uint64, 1, stream)[0]) if cnt else None
+        finally:
+            if cnt:
+                cnt.free()
+        return bm, c
+
+    # ---- aggregation --------------------------------------------------------------------
+    def group_agg(self, q, stream=None, result=None):
+        """Runs an AggQuery; returns GroupAggResult (downloaded, exact Python ints)."""
+        ng = C.c_uint64()
+        A.check(A.lib().tgp_group_agg_ngroups(C.byref(q), C.byref(ng)), "tgp_group_agg_ngroups")
+        res = result or GroupAggBuffers(self, ng.value, q.naggs)
+        A.check(A.lib().tgp_group_agg(self.handle, C.byref(q), C.byref(res.desc), stream),
+                "tgp_group_agg")
+        out = res.fetch(stream)
+        if result is None:
+            res.free()
+        return out
+
+
+class GroupAggBuffers:
+    """Device result buffers of tgp_group_agg."""
+
+    def __init__(self, ctx, ngroups, naggs):
+        self.ctx = ctx
+        self.ngroups, self.naggs = int(ngroups), int(naggs)
+        self.count = DeviceBuffer(ctx, max(8, self.ngroups * 8))
+        self.sums = DeviceBuffer(ctx, max(16, self.ngroups * self.naggs * 16))
+        self.oor = DeviceBuffer(ctx, 8)
+        self.desc = A.GroupResult(self.count.ptr, self.sums.ptr, self.oor.ptr)
+
+    def fetch(self, stream=None):
+        cnt = self.count.download(np.uint64, self.ngroups, stream)
+        sums = self.sums.download(np.uint64, self.ngroups * self.naggs * 2, stream)
+        oor = int(self.oor.download(np.uint64, 1, stream)[0])
+        return GroupAggResult(cnt, sums.reshape(self.ngroups, max(1, self.naggs), 2)
+                              if self.naggs else np.zeros((self.ngroups, 0, 2), np.uint64), oor)
+
+    def free(self):
+        for b in (self.count, self.sums, self.oor):
+            b.free()
+
+
+class GroupAggResult:
+    """Dense per-group COUNT(*) and exact int128 sums (as Python ints)."""
+
+    def __init__(self, count, sums_words, oor):
+        self.count = count
+        self.sums_words = sums_words
+        self.n_out_of_range = oor
+
+    def sums(self):
+        """list over groups of [int per aggregate]."""
+        return [[int128(self.sums_words[g, a, 0], self.sums_words[g, a, 1])
+                 for a in range(self.sums_words.shape[1])] for g in range(len(self.count))]
+
+    def rows(self):
+        """Present groups as (gid, count, [sums]) sorted by gid (compare after sorting)."""
+        s = self.sums()
+        return [(g, int(self.count[g]), s[g]) for g in range(len(self.count)) if self.count[g]]
+
+
+def make_agg_query(cols, preds=(), parts=(), aggs=(), row_begin=0, row_end=None, sel_bitmap=None):
+    """Builds an AggQuery.
+    cols:  list of DeviceColumn (or objects with .desc())
+    preds: (col, op, lit, is_signed)
+    parts: (col, divisor, base, card)
+    aggs:  ('sum', coef, [(col, fa, fb), ...]) or ('count_nonzero', col)
+    """
+    q = A.AggQuery()
+    n = cols[0].n if cols else 0
+    q.row_begin = row_begin
+    q.row_end = n if row_end is None else row_end
+    assert len(cols) <= A.MAX_COLS and len(preds) <= A.MAX_PREDS
+    for i, c in enumerate(cols):
+        q.cols[i] = c.desc()
+    q.ncols = len(cols)
+    for i, (ci, op, lit, sg) in enumerate(preds):
+        q.preds[i] = A.Pred(ci, op, int(bool(sg)), 0, lit)
+    q.npreds = len(preds)
+    for i, (ci, div, base, card) in enumerate(parts):
+        q.parts[i] = A.GroupPart(ci, div, base, card)
+    q.nparts = len(parts)
+    for i, ag in enumerate(aggs):
+        a = A.Agg()
+        if ag[0] == "count_nonzero":
+            a.kind, a.nfactors = A.AGG_COUNT_NONZERO, 1
+            a.fcol[0] = ag[1]
+            a.coef = 1
+        else:
+            _, coef, factors = ag
+            a.kind, a.nfactors, a.coef = A.AGG_SUM, len(factors), coef
+            for k, (ci, fa, fb) in enumerate(factors):
+                a.fcol[k], a.fa[k], a.fb[k] = ci, fa, fb
+        q.aggs[i] = a
+    q.naggs = len(aggs)
+    q.d_sel_bitmap = sel_bitmap.buf.ptr if sel_bitmap is not None else None
+    return q
