@@ -1,0 +1,226 @@
+"""ctypes binding of liboracle.so (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module.  The structs have the product ABI's layout but hold host pointers.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+_NP = {0: np.int8, 1: np.uint8, 2: np.int16, 3: np.uint16, 4: np.int32, 5: np.uint32,
+       6: np.int64, 7: np.uint64}
+_FROM_NP = {np.dtype(v): k for k, v in _NP.items()}
+
+
+class OColumn(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("len", C.c_uint64), ("dtype", C.c_int32),
+                ("_pad", C.c_int32)]
+
+
+class OPred(C.Structure):
+    _fields_ = [("col", C.c_int32), ("op", C.c_int32), ("is_signed", C.c_int32),
+                ("_pad", C.c_int32), ("lit", C.c_int64)]
+
+
+class OPart(C.Structure):
+    _fields_ = [("col", C.c_int32), ("divisor", C.c_int32), ("base", C.c_int64),
+                ("card", C.c_int64)]
+
+
+class OAgg(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("nfactors", C.c_int32), ("fcol", C.c_int32 * 3),
+                ("_pad", C.c_int32), ("coef", C.c_int64), ("fa", C.c_int64 * 3),
+                ("fb", C.c_int64 * 3)]
+
+
+class OQuery(C.Structure):
+    _fields_ = [("row_begin", C.c_uint64), ("row_end", C.c_uint64), ("cols", OColumn * 8),
+                ("ncols", C.c_int32), ("npreds", C.c_int32), ("nparts", C.c_int32),
+                ("naggs", C.c_int32), ("preds", OPred * 8), ("parts", OPart * 2),
+                ("aggs", OAgg * 6), ("sel_bitmap", C.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run make -C oracle")
+        L = C.CDLL(LIB_PATH)
+        P, U64, I64, I32 = C.c_void_p, C.c_uint64, C.c_int64, C.c_int32
+        sig = {
+            "orc_mix64": (U64, [U64]),
+            "orc_synth_uniform": (None, [P, I32, U64, U64, I64, I64, U64, U64]),
+            "orc_synth_cdf": (None, [P, I32, U64, U64, P, U64, P, U64, U64]),
+            "orc_widen_cells": (None, [P, I32, U64, P]),
+            "orc_filter_cells": (None, [P, U64, I32, I32, I64, P]),
+            "orc_bitmap_and_inplace": (None, [P, P, U64]),
+            "orc_bitmap_or_inplace": (None, [P, P, U64]),
+            "orc_bitmap_not": (None, [P, P, U64]),
+            "orc_bitmap_count_range": (U64, [P, U64, U64, U64]),
+            "orc_set_bits": (U64, [P, U64, P]),
+            "orc_scan_multi": (U64, [P, U64, I32, P, P, P]),
+            "orc_query_filter": (I32, [C.POINTER(OQuery), P]),
+            "orc_group_agg": (I32, [C.POINTER(OQuery), I32, I32, P, P, P, P]),
+            "orc_join_agg": (I32, [P, P, U64, P, P, P, U64, U64, I32, I64, U64, I32, P, P, P]),
+            "orc_join_pairs": (U64, [P, U64, P, U64, P, P, U64]),
+            "orc_join_hash": (U64, [U64]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    return a.ctypes.data if a is not None and a.size else None
+
+
+def synth_uniform(dtype, seed, column_id, lo, hi, row_begin, row_end):
+    out = np.empty(row_end - row_begin, dtype=_NP[dtype])
+    lib().orc_synth_uniform(ptr(out), dtype, seed, column_id, lo, hi, row_begin, row_end)
+    return out
+
+
+def synth_cdf(dtype, seed, column_id, cdf, table, row_begin, row_end):
+    out = np.empty(row_end - row_begin, dtype=_NP[dtype])
+    cdf = np.ascontiguousarray(cdf, np.float64)
+    table = np.ascontiguousarray(table, np.int64)
+    lib().orc_synth_cdf(ptr(out), dtype, seed, column_id, ptr(cdf), len(cdf), ptr(table),
+                        row_begin, row_end)
+    return out
+
+
+def filter_cells(cells, op, is_signed, lit):
+    cells = np.ascontiguousarray(cells, np.uint64)
+    bits = np.zeros((len(cells) + 7) // 8, np.uint8)
+    lib().orc_filter_cells(ptr(cells), len(cells), op, int(is_signed), lit, ptr(bits))
+    return bits
+
+
+def set_bits(bits, nbits):
+    out = np.empty(max(1, nbits), np.uint64)
+    n = lib().orc_set_bits(ptr(bits), nbits, ptr(out))
+    return out[:n]
+
+
+def bitmap_not(bits, nbits):
+    out = np.zeros_like(bits)
+    lib().orc_bitmap_not(ptr(bits), ptr(out), nbits)
+    return out
+
+
+def count_range(bits, nbits, start, end):
+    return lib().orc_bitmap_count_range(ptr(bits), nbits, start, end)
+
+
+def scan_multi(cells, preds):
+    """preds: [(target, op {0 Eq, 1 Gt, 2 Lt})] -> (count, mask)"""
+    cells = np.ascontiguousarray(cells, np.uint64)
+    ops = np.array([p[1] for p in preds] or [0], np.int32)
+    tg = np.array([p[0] for p in preds] or [0], np.uint64)
+    mask = np.zeros(1, np.uint64)
+    c = lib().orc_scan_multi(ptr(cells), len(cells), len(preds), ptr(ops), ptr(tg), ptr(mask))
+    return c, int(mask[0])
+
+
+def make_query(arrays, preds=(), parts=(), aggs=(), row_begin=0, row_end=None, sel_words=None):
+    """Same description as turbogp_amd.device.make_agg_query over host arrays.
+    Keeps references to the arrays on the returned object."""
+    q = OQuery()
+    q._keep = [np.ascontiguousarray(a) for a in arrays]
+    n = len(q._keep[0]) if q._keep else 0
+    q.row_begin, q.row_end = row_begin, n if row_end is None else row_end
+    for i, a in enumerate(q._keep):
+        q.cols[i] = OColumn(ptr(a), len(a), _FROM_NP[a.dtype], 0)
+    q.ncols = len(arrays)
+    for i, (ci, op, lit, sg) in enumerate(preds):
+        q.preds[i] = OPred(ci, op, int(bool(sg)), 0, lit)
+    q.npreds = len(preds)
+    for i, (ci, div, base, card) in enumerate(parts):
+        q.parts[i] = OPart(ci, div, base, card)
+    q.nparts = len(parts)
+    for i, ag in enumerate(aggs):
+        a = OAgg()
+        if ag[0] == "count_nonzero":
+            a.kind, a.nfactors, a.coef = 1, 1, 1
+            a.fcol[0] = ag[1]
+        else:
+            _, coef, factors = ag
+            a.kind, a.nfactors, a.coef = 0, len(factors), coef
+            for k, (ci, fa, fb) in enumerate(factors):
+                a.fcol[k], a.fa[k], a.fb[k] = ci, fa, fb
+        q.aggs[i] = a
+    q.naggs = len(aggs)
+    if sel_words is not None:
+        q._keep.append(np.ascontiguousarray(sel_words, np.uint64))
+        q.sel_bitmap = ptr(q._keep[-1])
+    return q
+
+
+def _i128(lo, hi):
+    v = (int(hi) << 64) | int(lo)
+    return v - (1 << 128) if v >> 127 else v
+
+
+def group_agg(q, mode=0, nthreads=0):
+    """-> (count[ngroups], sums[ngroups][naggs] as Python ints or floats, n_out_of_range)"""
+    ng = 1
+    for k in range(q.nparts):
+        ng *= q.parts[k].card
+    cnt = np.zeros(ng, np.uint64)
+    sums = np.zeros(ng * max(1, q.naggs) * 2, np.uint64)
+    f64 = np.zeros(ng * max(1, q.naggs), np.float64)
+    oor = np.zeros(1, np.uint64)
+    rc = lib().orc_group_agg(C.byref(q), mode, nthreads, ptr(cnt), ptr(sums), ptr(f64), ptr(oor))
+    assert rc == 0
+    if mode == 0:
+        s = [[_i128(sums[(g * q.naggs + a) * 2], sums[(g * q.naggs + a) * 2 + 1])
+              for a in range(q.naggs)] for g in range(ng)]
+    else:
+        s = [[float(f64[g * q.naggs + a]) for a in range(q.naggs)] for g in range(ng)]
+    return cnt, s, int(oor[0])
+
+
+def query_filter(q):
+    n = q.row_end - q.row_begin
+    bits = np.zeros((n + 7) // 8 + 8, np.uint8)
+    assert lib().orc_query_filter(C.byref(q), ptr(bits)) == 0
+    return bits[: (n + 7) // 8]
+
+
+def join_agg(build_keys, build_payload, probe_keys, probe_vals, probe_group, group_by, key_base,
+             ngroups, row_begin=0, row_end=None, nthreads=0):
+    bk = np.ascontiguousarray(build_keys, np.int64)
+    bp = np.ascontiguousarray(build_payload, np.int32) if build_payload is not None else None
+    pk = np.ascontiguousarray(probe_keys, np.int64)
+    pv = np.ascontiguousarray(probe_vals, np.int64)
+    pg = np.ascontiguousarray(probe_group, np.int32) if probe_group is not None else None
+    row_end = len(pk) if row_end is None else row_end
+    cnt = np.zeros(max(1, ngroups), np.uint64)
+    sums = np.zeros(max(1, ngroups) * 2, np.uint64)
+    oor = np.zeros(1, np.uint64)
+    rc = lib().orc_join_agg(ptr(bk), ptr(bp), len(bk), ptr(pk), ptr(pv), ptr(pg), row_begin,
+                            row_end, group_by, key_base, ngroups, nthreads, ptr(cnt), ptr(sums),
+                            ptr(oor))
+    assert rc == 0
+    s = [_i128(sums[2 * g], sums[2 * g + 1]) for g in range(ngroups)]
+    return cnt[:ngroups], s, int(oor[0])
+
+
+def join_pairs(build_keys, probe_keys, capacity=None):
+    bk = np.ascontiguousarray(build_keys, np.int64)
+    pk = np.ascontiguousarray(probe_keys, np.int64)
+    cap = capacity if capacity is not None else max(16, 4 * len(pk))
+    op = np.zeros(cap, np.uint64)
+    ob = np.zeros(cap, np.uint32)
+    n = lib().orc_join_pairs(ptr(bk), len(bk), ptr(pk), len(pk), ptr(op), ptr(ob), cap)
+    return op[:min(n, cap)], ob[:min(n, cap)], n

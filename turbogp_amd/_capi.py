@@ -134,6 +134,10 @@ SIGNATURES = {
     "tgp_synth_lookup_cdf": (I32_, [P, C.POINTER(Column), U64_, U64_, P, U64_, P, U64_, U64_,
                                     P]),
     "tgp_debug_inject_failure": (I32_, [P, I32_]),
+    "tgp_event_create": (I32_, [P, C.POINTER(P)]),
+    "tgp_event_destroy": (I32_, [P, P]),
+    "tgp_event_record": (I32_, [P, P, P]),
+    "tgp_event_elapsed_ms": (I32_, [P, P, P, C.POINTER(C.c_float)]),
 }
 
 

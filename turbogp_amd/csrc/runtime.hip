@@ -286,6 +286,34 @@ int32_t tgp_fill_u64(tgp_ctx* ctx, uint64_t* d_dst, uint64_t nwords, uint64_t va
   return fill_u64(pick_stream(ctx, stream), d_dst, nwords, value);
 }
 
+int32_t tgp_event_create(tgp_ctx* ctx, void** out_event) {
+  TGP_REQUIRE(ctx && out_event, "tgp_event_create: null argument");
+  hipEvent_t e;
+  TGP_HIP(hipEventCreate(&e));
+  *out_event = e;
+  return TGP_OK;
+}
+
+int32_t tgp_event_destroy(tgp_ctx* ctx, void* event) {
+  TGP_REQUIRE(ctx && event, "tgp_event_destroy: null argument");
+  TGP_HIP(hipEventDestroy(static_cast<hipEvent_t>(event)));
+  return TGP_OK;
+}
+
+int32_t tgp_event_record(tgp_ctx* ctx, void* event, void* stream) {
+  TGP_REQUIRE(ctx && event, "tgp_event_record: null argument");
+  TGP_HIP(hipEventRecord(static_cast<hipEvent_t>(event), pick_stream(ctx, stream)));
+  return TGP_OK;
+}
+
+int32_t tgp_event_elapsed_ms(tgp_ctx* ctx, void* start, void* end, float* out_ms) {
+  TGP_REQUIRE(ctx && start && end && out_ms, "tgp_event_elapsed_ms: null argument");
+  TGP_HIP(hipEventSynchronize(static_cast<hipEvent_t>(end)));
+  TGP_HIP(hipEventElapsedTime(out_ms, static_cast<hipEvent_t>(start),
+                              static_cast<hipEvent_t>(end)));
+  return TGP_OK;
+}
+
 int32_t tgp_debug_inject_failure(tgp_ctx* ctx, int32_t count) {
   TGP_REQUIRE(ctx && count >= 0, "tgp_debug_inject_failure: bad argument");
   ctx->inject_failures.store(count);

@@ -229,6 +229,35 @@ class Context:
             res.free()
         return out
 
+    def group_agg_async(self, q, res, stream=None):
+        """Queues tgp_group_agg into preallocated GroupAggBuffers (no sync, no copy)."""
+        A.check(A.lib().tgp_group_agg(self.handle, C.byref(q), C.byref(res.desc), stream),
+                "tgp_group_agg")
+
+
+class Event:
+    """hipEvent on the context (timing of work queued on a stream)."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = C.c_void_p()
+        A.check(A.lib().tgp_event_create(ctx.handle, C.byref(h)), "tgp_event_create")
+        self.h = h.value
+
+    def record(self, stream=None):
+        A.check(A.lib().tgp_event_record(self.ctx.handle, self.h, stream), "tgp_event_record")
+
+    def elapsed_ms(self, end):
+        ms = C.c_float()
+        A.check(A.lib().tgp_event_elapsed_ms(self.ctx.handle, self.h, end.h, C.byref(ms)),
+                "tgp_event_elapsed_ms")
+        return ms.value
+
+    def free(self):
+        if self.h:
+            A.lib().tgp_event_destroy(self.ctx.handle, self.h)
+            self.h = None
+
 
 class GroupAggBuffers:
     """Device result buffers of tgp_group_agg."""

@@ -1,0 +1,100 @@
+"""Synthetic fact table and the queries of the benchmark configurations.
+
+Fact table (one row per line item; all values seeded and counter-based):
+  dt     int32  uniform [0, 2526)            days (named dt: DATE is a keyword
+                                            in the reference parser, parser.rs:539-549)
+  qty    int64  uniform 1..50
+  price  int64  uniform 90 000..10 500 000   cents
+  disc   int64  uniform 0..10                hundredths
+  tax    int64  uniform 0..8                 hundredths
+  flaga  uint8  uniform {0, 1, 2}
+  flagb  uint8  uniform {0, 1}
+Row i of column c is lo + mulhi64(mix64(SEED ^ (c << 40) ^ i), hi - lo + 1).
+"""
+import numpy as np
+
+from . import _capi as A
+from .device import DeviceColumn, make_agg_query
+
+SEED = 0x74626770_5F4D4933
+
+# name -> (column id, dtype, lo, hi)
+FACT_SCHEMA = {
+    "dt": (0, A.I32, 0, 2525),
+    "qty": (1, A.I64, 1, 50),
+    "price": (2, A.I64, 90_000, 10_500_000),
+    "disc": (3, A.I64, 0, 10),
+    "tax": (4, A.I64, 0, 8),
+    "flaga": (5, A.U8, 0, 2),
+    "flagb": (6, A.U8, 0, 1),
+}
+FACT_COLUMNS = list(FACT_SCHEMA)
+ROW_BYTES = {k: A.DTYPE_WIDTH[v[1]] for k, v in FACT_SCHEMA.items()}
+
+Q1_DATE_CUTOFF = 2435          # dt <= 2435 (~96.4 % selected)
+Q6_WINDOW = (731, 1096)        # 731 <= dt < 1096 (365 days)
+Q6_DISC = (5, 7)               # disc BETWEEN 5 AND 7
+Q6_QTY = 24                    # qty < 24
+
+Q1_COLS = ["dt", "qty", "price", "disc", "tax", "flaga", "flagb"]
+Q6_COLS = ["dt", "disc", "qty", "price"]
+Q1_BYTES_PER_ROW = sum(ROW_BYTES[c] for c in Q1_COLS)   # 38
+Q6_BYTES_PER_ROW = sum(ROW_BYTES[c] for c in Q6_COLS)   # 28
+
+
+def make_fact_table(ctx, n, columns=None, stream=None):
+    """Generates the fact table on the device (tgp_synth_uniform)."""
+    import ctypes as C
+    out = {}
+    for name in columns or FACT_COLUMNS:
+        cid, dt, lo, hi = FACT_SCHEMA[name]
+        col = DeviceColumn(ctx, n, dt, name)
+        d = col.desc()
+        A.check(A.lib().tgp_synth_uniform(ctx.handle, C.byref(d), SEED, cid, lo, hi, 0, n, stream),
+                "tgp_synth_uniform")
+        out[name] = col
+    ctx.sync(stream)
+    return out
+
+
+def q1_spec(date_cutoff=Q1_DATE_CUTOFF, group="ab"):
+    """Q1-shape: WHERE dt <= cutoff GROUP BY <group>; COUNT(*), SUM(qty), SUM(price),
+    SUM(price*(100-disc)), SUM(price*(100-disc)*(100+tax)).
+    group: 'ab' (flaga, flagb), 'a', 'b', 'month' (dt/30), '' (none)."""
+    c = {n: i for i, n in enumerate(Q1_COLS)}
+    preds = [(c["dt"], A.LE, date_cutoff, False)]
+    parts = {"ab": [(c["flaga"], 1, 0, 3), (c["flagb"], 1, 0, 2)],
+             "a": [(c["flaga"], 1, 0, 3)], "b": [(c["flagb"], 1, 0, 2)],
+             "month": [(c["dt"], 30, 0, 85)], "": []}[group]
+    aggs = [("sum", 1, [(c["qty"], 0, 1)]),
+            ("sum", 1, [(c["price"], 0, 1)]),
+            ("sum", 1, [(c["price"], 0, 1), (c["disc"], 100, -1)]),
+            ("sum", 1, [(c["price"], 0, 1), (c["disc"], 100, -1), (c["tax"], 100, 1)])]
+    return Q1_COLS, preds, parts, aggs
+
+
+def q6_spec():
+    """Q6-shape: WHERE 731 <= dt < 1096 AND disc BETWEEN 5 AND 7 AND qty < 24;
+    SUM(price*disc)."""
+    c = {n: i for i, n in enumerate(Q6_COLS)}
+    preds = [(c["dt"], A.GE, Q6_WINDOW[0], False), (c["dt"], A.LT, Q6_WINDOW[1], False),
+             (c["disc"], A.GE, Q6_DISC[0], True), (c["disc"], A.LE, Q6_DISC[1], True),
+             (c["qty"], A.LT, Q6_QTY, True)]
+    aggs = [("sum", 1, [(c["price"], 0, 1), (c["disc"], 0, 1)])]
+    return Q6_COLS, preds, [], aggs
+
+
+def build_query(table, spec, row_begin=0, row_end=None, sel_bitmap=None):
+    names, preds, parts, aggs = spec
+    cols = [table[n] for n in names]
+    return make_agg_query(cols, preds, parts, aggs, row_begin, row_end, sel_bitmap)
+
+
+def host_fact_columns(n, gen):
+    """Host copy of the fact table from a generator function
+    gen(dtype, seed, column_id, lo, hi, row_begin, row_end) -> np.ndarray."""
+    return {name: gen(dt, SEED, cid, lo, hi, 0, n) for name, (cid, dt, lo, hi) in FACT_SCHEMA.items()}
+
+
+NP_DTYPE = {A.I8: np.int8, A.U8: np.uint8, A.I16: np.int16, A.U16: np.uint16,
+            A.I32: np.int32, A.U32: np.uint32, A.I64: np.int64, A.U64: np.uint64}
