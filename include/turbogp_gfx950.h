@@ -244,6 +244,71 @@ int32_t tgp_group_agg(tgp_ctx* ctx, const tgp_agg_query* q, const tgp_group_resu
 /* Number of groups the query's key space spans (prod of card). */
 int32_t tgp_group_agg_ngroups(const tgp_agg_query* q, uint64_t* out_ngroups);
 
+/* ---- hash join --------------------------------------------------------------- */
+typedef struct tgp_hash_table tgp_hash_table;
+
+/* Build an open-addressing table (64-bit CAS on the key slot) over the
+ * build rows: key column (int64/uint64/int32 widened) and an optional 32-bit
+ * payload column stored inline (NULL => the build row index is the payload).
+ * Duplicate keys keep one slot each (multiset, like JoinHashTable chains,
+ * join_hash_table.rs:127-198).  Capacity = next_pow2(2*n) slots of 16 bytes,
+ * accounted in the HBM tier.  Synchronous (returns once the table is built). */
+int32_t tgp_hash_build(tgp_ctx* ctx, const tgp_column* keys, const tgp_column* payload,
+                       tgp_hash_table** out, void* stream);
+int32_t tgp_hash_table_destroy(tgp_ctx* ctx, tgp_hash_table* t);
+/* Bytes of HBM the table holds (admission control). */
+int32_t tgp_hash_table_footprint(const tgp_hash_table* t, uint64_t* out_bytes);
+/* Bytes a table over n build rows would take, before building it. */
+uint64_t tgp_hash_table_bytes_for(uint64_t n_build_rows);
+int32_t tgp_hash_table_info(const tgp_hash_table* t, uint64_t* n_rows, uint64_t* capacity,
+                            int32_t* has_duplicates);
+
+/* Total number of (probe row, build row) matches over rows [row_begin,row_end)
+ * of the probe keys, added to *d_count (HashProbe, kernel/hash.rs:160-187). */
+int32_t tgp_hash_probe_count(tgp_ctx* ctx, const tgp_hash_table* t, const tgp_column* probe_keys,
+                             uint64_t row_begin, uint64_t row_end, uint64_t* d_count, void* stream);
+
+/* Semi/anti join marks: bit i set iff probe row row_begin+i has >= 1 match
+ * (exec/join.rs semi_join); anti = tgp_bitmap_not of it. */
+int32_t tgp_hash_probe_mark(tgp_ctx* ctx, const tgp_hash_table* t, const tgp_column* probe_keys,
+                            uint64_t row_begin, uint64_t row_end, uint64_t* d_bitmap, void* stream);
+
+/* Inner-join pairs (probe row, build payload) in ascending probe-row order
+ * (exec/join.rs hash_join INNER; query_interpreter/join.rs:643-794).  Within
+ * one probe row, duplicate matches appear in ascending payload order (build-row
+ * order when the table was built without a payload column).
+ * Synchronous; *h_npairs receives the number of pairs (capacity exceeded =>
+ * TGP_ERR_CAPACITY with *h_npairs set to the number needed). */
+int32_t tgp_hash_probe_pairs_sync(tgp_ctx* ctx, const tgp_hash_table* t,
+                                  const tgp_column* probe_keys, uint64_t row_begin,
+                                  uint64_t row_end, uint64_t* d_probe_idx, uint32_t* d_build_payload,
+                                  uint64_t capacity, uint64_t* h_npairs, void* stream);
+
+enum tgp_join_group_by {
+  TGP_JOIN_GROUP_BUILD_PAYLOAD = 0, /* GROUP BY the inline build payload */
+  TGP_JOIN_GROUP_PROBE_COL = 1,     /* GROUP BY a probe-side column */
+  TGP_JOIN_GROUP_NONE = 2           /* one group */
+};
+
+/* Fused probe -> GROUP BY -> SUM(value), COUNT(*) over the inner-join rows,
+ * without materialising them.  Group keys must be dense: key - key_base in
+ * [0, ngroups) (others are counted in *d_n_out_of_range).  Output layout as
+ * tgp_group_result with naggs = 1.  Replaces hash_join_with_keys +
+ * try_accumulator_grouped (join.rs:544-939, accum_grouped.rs:198-335). */
+typedef struct tgp_join_agg_query {
+  const tgp_hash_table* table;
+  tgp_column probe_keys;
+  tgp_column value;       /* summed column (int64 / int32 ...) */
+  tgp_column group_col;   /* used when group_by == TGP_JOIN_GROUP_PROBE_COL */
+  int32_t group_by;
+  int32_t _pad;
+  int64_t key_base;
+  uint64_t ngroups;
+  uint64_t row_begin, row_end;
+} tgp_join_agg_query;
+int32_t tgp_hash_probe_agg(tgp_ctx* ctx, const tgp_join_agg_query* q, const tgp_group_result* out,
+                           void* stream);
+
 /* ---- synthetic data (counter-based, seeded) ------------------------------- */
 /* Row i of column `column_id`: u = mix64(seed ^ (column_id << 40) ^ i)
  * (SplitMix64 finaliser), value = lo + mulhi64(u, hi - lo + 1).  Written at
@@ -259,6 +324,61 @@ int32_t tgp_synth_uniform(tgp_ctx* ctx, tgp_column* col, uint64_t seed, uint64_t
 int32_t tgp_synth_lookup_cdf(tgp_ctx* ctx, tgp_column* out, uint64_t seed, uint64_t column_id,
                              const double* d_cdf, uint64_t n_cdf, const int64_t* d_table,
                              uint64_t row_begin, uint64_t row_end, void* stream);
+
+/* ---- Kernel trait / KernelTable mirror (kernel/mod.rs:47-242) ------------- */
+enum tgp_operator {           /* kernel/mod.rs:47-74, same order */
+  TGP_OP_SCAN_EQ_U64 = 0, TGP_OP_SCAN_RANGE_U64 = 1, TGP_OP_SCAN_MULTI_PREDICATE = 2,
+  TGP_OP_HASH_BUILD = 3, TGP_OP_HASH_PROBE = 4, TGP_OP_AGGREGATE_SUM_F64 = 5,
+  TGP_OP_AGGREGATE_COUNT_DISTINCT = 6, TGP_OP_SIMILARITY_HAMMING = 7, TGP_OP_LEAPFROG_JOIN = 8
+};
+enum tgp_pred_op { TGP_PRED_EQ = 0, TGP_PRED_GT = 1, TGP_PRED_LT = 2 }; /* mod.rs:84-92 */
+enum tgp_tier {               /* memory/tier.rs:11-29, same order */
+  TGP_TIER_L1L2 = 0, TGP_TIER_L3 = 1, TGP_TIER_DDR5 = 2, TGP_TIER_HBM = 3, TGP_TIER_CXL = 4,
+  TGP_TIER_NVME = 5, TGP_TIER_NVMEOF = 6, TGP_TIER_NETWORK = 7
+};
+#define TGP_TARGET_GFX950 100 /* device target beside CpuTarget (cpu.rs:30-41) */
+
+/* repr(C) image of KernelParams (mod.rs:96-119). */
+typedef struct tgp_kernel_params {
+  uint64_t target_u64, low_u64, high_u64;
+  uint32_t max_distance, _pad0;
+  uint64_t cell_count;
+  uint64_t target2_u64, target3_u64;
+  int32_t pred1_op, pred2_op, pred3_op;
+  uint8_t predicate_count, _pad1[3];
+} tgp_kernel_params;
+/* repr(C) image of KernelResult (mod.rs:141-148). */
+typedef struct tgp_kernel_result {
+  uint64_t count;
+  double sum;
+  uint64_t mask;
+} tgp_kernel_result;
+
+typedef struct tgp_kernel_table tgp_kernel_table;
+typedef struct tgp_kernel tgp_kernel;
+
+int32_t tgp_kernel_table_new(tgp_ctx* ctx, tgp_kernel_table** out);
+int32_t tgp_kernel_table_destroy(tgp_kernel_table* t);
+/* select(op, tier): exact (op, gfx950, tier) -> (op, gfx950, Hbm) -> any with
+ * op (mod.rs:209-221).  *out = NULL when no kernel implements op. */
+int32_t tgp_kernel_table_select(tgp_kernel_table* t, int32_t op, int32_t tier, tgp_kernel** out);
+int32_t tgp_kernel_table_len(tgp_kernel_table* t, int32_t* out_n);
+/* list(): i-th entry's operator, target, tier and name. */
+int32_t tgp_kernel_table_entry(tgp_kernel_table* t, int32_t i, int32_t* op, int32_t* target,
+                               int32_t* tier, const char** name);
+int32_t tgp_kernel_info(const tgp_kernel* k, int32_t* op, int32_t* target, int32_t* tier,
+                        const char** name);
+/* Kernel::execute(input, output, &params) -> KernelResult (mod.rs:171-176).
+ * `input`/`output` are host or device pointers with the reference's layouts:
+ *   scans:      input = cell_count u64 cells; result.count (+ mask of first 64)
+ *   HashBuild:  input = cell_count u64 keys; writes a tgp_hash_table* to output
+ *   HashProbe:  input = [tgp_hash_table* (8 B)][cell_count u64 keys]; result.count
+ *   AggregateSumF64: input = cell_count f64 bits; result.sum (f64, fixed
+ *               pairwise order) and result.count
+ * Host inputs are staged through HBM (PCIe); tier HBM kernels expect device
+ * pointers.  Errors return a status; `result` is then zeroed. */
+int32_t tgp_kernel_execute(const tgp_kernel* k, const void* input, void* output,
+                           const tgp_kernel_params* params, tgp_kernel_result* result);
 
 /* ---- test hook -------------------------------------------------------------- */
 /* Make the next `count` operator calls on ctx fail with TGP_ERR_INJECTED

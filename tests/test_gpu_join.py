@@ -1,0 +1,160 @@
+"""GPU parity of the hash join (build / probe count / marks / pairs / fused
+probe-aggregate) against the oracle and the reference's known answers."""
+import numpy as np
+import pytest
+
+from oracle import binding as B
+from turbogp_amd import DeviceColumn, HashTable, _capi as A
+from turbogp_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+I64MIN = np.iinfo(np.int64).min
+
+
+def dev(ctx, arr):
+    return DeviceColumn.from_numpy(ctx, np.ascontiguousarray(arr))
+
+
+def build(ctx, keys, payload=None):
+    kc = dev(ctx, np.array(keys, np.int64))
+    pc = dev(ctx, np.array(payload, np.int32)) if payload is not None else None
+    t = HashTable(ctx, kc, pc)
+    return t, [kc] + ([pc] if pc else [])
+
+
+def pairs(ctx, bkeys, pkeys):
+    t, keep = build(ctx, bkeys)
+    pc = dev(ctx, np.array(pkeys, np.int64))
+    try:
+        pi, pp = t.probe_pairs(pc)
+        return list(zip(pi.tolist(), pp.tolist())), t.probe_count(pc)
+    finally:
+        t.free()
+        for c in keep + [pc]:
+            c.free()
+
+
+def test_basic_insert_probe(ctx):  # join_hash_table.rs test_basic_insert_probe
+    build_keys = [i * 7 for i in range(100)]
+    p, n = pairs(ctx, build_keys, build_keys + [1, 999])
+    assert n == 100 and p == [(i, i) for i in range(100)]
+
+
+def test_duplicate_keys(ctx):  # join_hash_table.rs test_duplicate_keys
+    p, n = pairs(ctx, [5, 42, 42, 42], [42])
+    assert n == 3 and p == [(0, 1), (0, 2), (0, 3)]
+
+
+def test_hash_kernel_kats(ctx):  # kernel/hash.rs:231-285
+    p, _ = pairs(ctx, [1, 2, 3, 2, 4, 2, 5], [2, 99])
+    assert [b for _, b in p] == [1, 3, 5]
+    _, n = pairs(ctx, [1, 2, 3, 2, 4], [2, 4, 99])
+    assert n == 3
+
+
+def test_semi_anti_marks(ctx):  # exec/join.rs test_semi_join / test_anti_join
+    t, keep = build(ctx, [2, 4])
+    left = dev(ctx, np.array([1, 2, 3, 4], np.int64))
+    bm = t.probe_mark(left)
+    anti = bm.not_()
+    assert list(bm.iter_set_bits()) == [1, 3] and list(anti.iter_set_bits()) == [0, 2]
+    for x in [bm, anti, left] + keep:
+        x.free()
+    t.free()
+
+
+def test_large_load_and_info(ctx):  # join_hash_table.rs test_large_load
+    n = 100_000
+    t, keep = build(ctx, np.arange(n))
+    nr, cap, dup = t.info()
+    assert nr == n and cap == 262_144 and not dup
+    assert t.footprint() == A.lib().tgp_hash_table_bytes_for(n)
+    pc = dev(ctx, np.arange(n + 1000, dtype=np.int64))
+    pi, pp = t.probe_pairs(pc)
+    assert np.array_equal(pi, np.arange(n)) and np.array_equal(pp, np.arange(n))
+    for x in [pc] + keep:
+        x.free()
+    t.free()
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_random_pairs_vs_oracle(ctx, seed):
+    rng = np.random.default_rng(seed)
+    nb, npb = int(rng.integers(1, 5000)), int(rng.integers(1, 50000))
+    bk = rng.integers(-300, 3000, nb).astype(np.int64)
+    bk[rng.integers(0, nb, 3)] = I64MIN
+    pk = rng.integers(-400, 3100, npb).astype(np.int64)
+    pk[::53] = I64MIN
+    got, _ = pairs(ctx, bk, pk)
+    oi, ob, n = B.join_pairs(bk, pk)
+    assert got == list(zip(oi.tolist(), ob.tolist()))
+
+
+@pytest.mark.parametrize("group_by", [A.JOIN_GROUP_BUILD_PAYLOAD, A.JOIN_GROUP_PROBE_COL,
+                                      A.JOIN_GROUP_NONE])
+@pytest.mark.parametrize("dups", [False, True])
+def test_probe_agg_vs_oracle(ctx, group_by, dups):
+    rng = np.random.default_rng(10 * group_by + dups)
+    nb, npb = 20_000, 300_000
+    bk = (rng.integers(0, nb // 2, nb) if dups else rng.permutation(nb) * 7919).astype(np.int64)
+    bk[5] = I64MIN
+    bp = rng.integers(-1, 26, nb).astype(np.int32)   # -1 and 25 fall outside [0, 25)
+    pk = bk[rng.integers(0, nb, npb)]
+    pk[::11] += 3                                     # some misses
+    pv = rng.integers(-2**62, 2**62, npb).astype(np.int64)
+    pg = rng.integers(0, 70_000, npb).astype(np.int32)
+    ng = {A.JOIN_GROUP_BUILD_PAYLOAD: 25, A.JOIN_GROUP_PROBE_COL: 65_000,
+          A.JOIN_GROUP_NONE: 1}[group_by]
+    kb = 0
+    b, e = 123, npb - 77
+    t, keep = build(ctx, bk, bp)
+    cols = [dev(ctx, pk), dev(ctx, pv), dev(ctx, pg)]
+    try:
+        got = t.probe_agg(cols[0], cols[1], group_by, ng, kb, cols[2], b, e)
+    finally:
+        t.free()
+        for c in keep + cols:
+            c.free()
+    cnt, sums, oor = B.join_agg(bk, bp, pk, pv, pg, group_by, kb, ng, b, e)
+    assert got.n_out_of_range == oor
+    assert np.array_equal(got.count, cnt)
+    assert [s[0] for s in got.sums()] == sums
+
+
+def test_c3_slice_vs_oracle(ctx):
+    """The join workload's shape at 10K build / 100K probe rows (Zipf 1.1 keys)."""
+    nb, npb = 10_000, 100_000
+    bkey, _ = W.join_build_side(nb)
+    pay = B.synth_uniform(A.I32, W.SEED, W.COL_PAY, 0, 24, 0, nb)
+    cdf = W.zipf_cdf(nb)
+    ktab = W.zipf_key_table(bkey)
+    fk = B.synth_cdf(A.I64, W.SEED, W.COL_FK, cdf, ktab, 0, npb)
+    val = B.synth_uniform(A.I64, W.SEED, W.COL_VAL, 90_000, 10_500_000, 0, npb)
+    gkey = B.synth_uniform(A.I32, W.SEED, W.COL_GKEY, 0, 14_999, 0, npb)
+    t, keep = build(ctx, bkey, pay)
+    cols = [dev(ctx, fk), dev(ctx, val), dev(ctx, gkey)]
+    try:
+        r1 = t.probe_agg(cols[0], cols[1], A.JOIN_GROUP_BUILD_PAYLOAD, 25)
+        r2 = t.probe_agg(cols[0], cols[1], A.JOIN_GROUP_PROBE_COL, 15_000, 0, cols[2])
+    finally:
+        t.free()
+        for c in keep + cols:
+            c.free()
+    c1, s1, _ = B.join_agg(bkey, pay, fk, val, gkey, 0, 0, 25)
+    c2, s2, _ = B.join_agg(bkey, pay, fk, val, gkey, 1, 0, 15_000)
+    assert np.array_equal(r1.count, c1) and [s[0] for s in r1.sums()] == s1
+    assert np.array_equal(r2.count, c2) and [s[0] for s in r2.sums()] == s2
+    assert int(r1.count.sum()) == npb  # every foreign key matches exactly one build row
+
+
+def test_device_zipf_generator_matches_oracle(ctx):
+    nb, npb = 5000, 200_000
+    bkey, _ = W.join_build_side(nb)
+    cdf = W.zipf_cdf(nb)
+    ktab = W.zipf_key_table(bkey)
+    want = B.synth_cdf(A.I64, W.SEED, W.COL_FK, cdf, ktab, 0, npb)
+    got = W.device_lookup_cdf(ctx, npb, A.I64, W.COL_FK, cdf, ktab)
+    try:
+        assert np.array_equal(got.to_numpy(), want)
+    finally:
+        got.free()

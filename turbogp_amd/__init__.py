@@ -2,7 +2,7 @@
 hash-join path, behind the C ABI declared in include/turbogp_gfx950.h."""
 from . import _capi  # noqa: F401
 from .device import (Context, DeviceBitmap, DeviceBuffer, DeviceColumn, Event,  # noqa: F401
-                     GroupAggBuffers, GroupAggResult, int128, make_agg_query)
+                     GroupAggBuffers, GroupAggResult, HashTable, int128, make_agg_query)
 
 __all__ = ["Context", "DeviceBitmap", "DeviceBuffer", "DeviceColumn", "GroupAggBuffers",
-           "GroupAggResult", "int128", "make_agg_query"]
+           "GroupAggResult", "HashTable", "int128", "make_agg_query"]

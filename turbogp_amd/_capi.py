@@ -134,6 +134,25 @@ SIGNATURES = {
     "tgp_synth_lookup_cdf": (I32_, [P, C.POINTER(Column), U64_, U64_, P, U64_, P, U64_, U64_,
                                     P]),
     "tgp_debug_inject_failure": (I32_, [P, I32_]),
+    "tgp_kernel_table_new": (I32_, [P, C.POINTER(P)]),
+    "tgp_kernel_table_destroy": (I32_, [P]),
+    "tgp_kernel_table_select": (I32_, [P, I32_, I32_, C.POINTER(P)]),
+    "tgp_kernel_table_len": (I32_, [P, C.POINTER(I32_)]),
+    "tgp_kernel_table_entry": (I32_, [P, I32_, C.POINTER(I32_), C.POINTER(I32_),
+                                      C.POINTER(I32_), C.POINTER(C.c_char_p)]),
+    "tgp_kernel_info": (I32_, [P, C.POINTER(I32_), C.POINTER(I32_), C.POINTER(I32_),
+                               C.POINTER(C.c_char_p)]),
+    "tgp_kernel_execute": (I32_, [P, P, P, C.POINTER(KernelParams), C.POINTER(KernelResult)]),
+    "tgp_hash_build": (I32_, [P, C.POINTER(Column), C.POINTER(Column), C.POINTER(P), P]),
+    "tgp_hash_table_destroy": (I32_, [P, P]),
+    "tgp_hash_table_footprint": (I32_, [P, C.POINTER(U64_)]),
+    "tgp_hash_table_bytes_for": (U64_, [U64_]),
+    "tgp_hash_table_info": (I32_, [P, C.POINTER(U64_), C.POINTER(U64_), C.POINTER(I32_)]),
+    "tgp_hash_probe_count": (I32_, [P, P, C.POINTER(Column), U64_, U64_, P, P]),
+    "tgp_hash_probe_mark": (I32_, [P, P, C.POINTER(Column), U64_, U64_, P, P]),
+    "tgp_hash_probe_pairs_sync": (I32_, [P, P, C.POINTER(Column), U64_, U64_, P, P, U64_,
+                                         C.POINTER(U64_), P]),
+    "tgp_hash_probe_agg": (I32_, [P, C.POINTER(JoinAggQuery), C.POINTER(GroupResult), P]),
     "tgp_event_create": (I32_, [P, C.POINTER(P)]),
     "tgp_event_destroy": (I32_, [P, P]),
     "tgp_event_record": (I32_, [P, P, P]),

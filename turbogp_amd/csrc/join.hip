@@ -1,0 +1,629 @@
+// Hash join: open-addressing build with 64-bit CAS, probe -> count / marks /
+// pairs, and the fused probe -> GROUP BY -> exact SUM, COUNT.
+//
+// Reference path replaced:
+//   exec/join_hash_table.rs:38-241  JoinHashTable (CRC32 hash, 16-bit bloom tag
+//                                   directory, chained entries, serial insert)
+//   kernel/hash.rs:84-225           HashBuild / HashProbe kernels (match count)
+//   query_interpreter/join.rs:544-939 hash_join_with_keys: serial build, rayon
+//                                   probe_all in 65 536-row chunks, then full
+//                                   materialisation of the joined rows
+//   exec/join.rs:56-199             hash_join pairs, semi_join, anti_join
+//   accum_grouped.rs:198-335        grouping of the joined rows
+//
+// Table layout (HBM): capacity = next_pow2(max(16, 2n)) slots of 16 bytes
+// {int64 key, uint32 build row, uint32 payload}; a slot is free while its key
+// holds the sentinel INT64_MIN.  Build threads claim a slot with one 64-bit CAS
+// on the key; duplicate build keys each take their own slot (multiset
+// semantics, like the reference's chains), and the table remembers whether any
+// duplicate exists so that probes of a unique-key table stop at the first
+// match.  Build rows whose key equals the sentinel go to a side list.  A probe
+// is one 16-byte load per visited slot (linear probing, load factor <= 0.5).
+#include "tgp_internal.h"
+
+namespace tgp {
+
+constexpr unsigned long long kEmpty = 0x8000000000000000ull;
+
+struct Slot {
+  unsigned long long key;
+  uint32_t row;
+  uint32_t payload;
+};
+static_assert(sizeof(Slot) == 16, "slot must be 16 bytes");
+
+struct JoinTableDev {
+  Slot* slots;
+  uint64_t mask;
+  const uint2* side;  // (row, payload) of build rows whose key is the sentinel
+  uint64_t n_side;
+  int32_t has_dup;
+  int32_t _pad;
+};
+
+__device__ __forceinline__ uint64_t hmix(uint64_t z) {
+  // SplitMix64 finaliser: a bijection with full avalanche, so the low bits of
+  // the mixed key index the table.
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ int64_t load_i64(const void* base, int32_t dtype, uint64_t i) {
+  switch (dtype) {
+    case TGP_I8: return ((const int8_t*)base)[i];
+    case TGP_U8: return ((const uint8_t*)base)[i];
+    case TGP_I16: return ((const int16_t*)base)[i];
+    case TGP_U16: return ((const uint16_t*)base)[i];
+    case TGP_I32: return ((const int32_t*)base)[i];
+    case TGP_U32: return ((const uint32_t*)base)[i];
+    default: return ((const int64_t*)base)[i];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_table_init(Slot* slots, uint64_t cap) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < cap;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    Slot s;
+    s.key = kEmpty;
+    s.row = 0;
+    s.payload = 0;
+    slots[i] = s;
+  }
+}
+
+// flags[0]: duplicate seen (u32 as u64 word), flags[1]: sentinel-key rows
+__global__ void __launch_bounds__(256) k_build(const void* keys, int32_t kdt, const void* pay,
+                                               int32_t pdt, uint64_t n, Slot* slots,
+                                               uint64_t mask, unsigned long long* flags) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const unsigned long long k = (unsigned long long)load_i64(keys, kdt, i);
+    if (k == kEmpty) {
+      atomicAdd(&flags[1], 1ull);
+      continue;
+    }
+    const uint32_t p = pay ? (uint32_t)load_i64(pay, pdt, i) : (uint32_t)i;
+    uint64_t s = hmix(k) & mask;
+    bool dup = false;
+    for (;;) {
+      const unsigned long long old = atomicCAS(&slots[s].key, kEmpty, k);
+      if (old == kEmpty) {
+        slots[s].row = (uint32_t)i;
+        slots[s].payload = p;
+        break;
+      }
+      dup |= (old == k);
+      s = (s + 1) & mask;
+    }
+    if (dup) atomicOr(&flags[0], 1ull);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_build_side(const void* keys, int32_t kdt, const void* pay,
+                                                    int32_t pdt, uint64_t n, uint2* side,
+                                                    unsigned long long* counter) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    if ((unsigned long long)load_i64(keys, kdt, i) != kEmpty) continue;
+    const unsigned long long at = atomicAdd(counter, 1ull);
+    side[at] = make_uint2((uint32_t)i, pay ? (uint32_t)load_i64(pay, pdt, i) : (uint32_t)i);
+  }
+}
+
+// Calls f(row, payload) for every build row matching key k (visit order: the
+// probe sequence, then the side list).
+template <typename F>
+__device__ __forceinline__ uint32_t probe_key(const JoinTableDev& T, unsigned long long k, F&& f) {
+  uint32_t m = 0;
+  if (k == kEmpty) {
+    for (uint64_t j = 0; j < T.n_side; ++j) {
+      f(T.side[j].x, T.side[j].y);
+      ++m;
+    }
+    return m;
+  }
+  uint64_t s = hmix(k) & T.mask;
+  for (;;) {
+    const Slot e = T.slots[s];
+    if (e.key == kEmpty) break;
+    if (e.key == k) {
+      f(e.row, e.payload);
+      ++m;
+      if (!T.has_dup) break;
+    }
+    s = (s + 1) & T.mask;
+  }
+  return m;
+}
+
+constexpr int kProbeUnroll = 4;
+
+// ---- match count (HashProbe) ----------------------------------------------------------
+__global__ void __launch_bounds__(256) k_probe_count(JoinTableDev T, const void* keys, int32_t kdt,
+                                                     uint64_t b, uint64_t e,
+                                                     unsigned long long* out) {
+  unsigned long long c = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i0 = b + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i0 < e;
+       i0 += stride * kProbeUnroll) {
+    unsigned long long k[kProbeUnroll];
+#pragma unroll
+    for (int u = 0; u < kProbeUnroll; ++u) {
+      const uint64_t i = i0 + u * stride;
+      k[u] = i < e ? (unsigned long long)load_i64(keys, kdt, i) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kProbeUnroll; ++u)
+      if (i0 + u * stride < e) c += probe_key(T, k[u], [](uint32_t, uint32_t) {});
+  }
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
+}
+
+// ---- semi-join marks: one bitmap word per wave-row-group of 64 -------------------------
+__global__ void __launch_bounds__(256) k_probe_mark(JoinTableDev T, const void* keys, int32_t kdt,
+                                                    uint64_t b, uint64_t e, uint64_t* bitmap) {
+  const uint64_t n = e - b;
+  for (uint64_t base = blockIdx.x * 256ull; base < n; base += gridDim.x * 256ull) {
+    const uint64_t r = base + threadIdx.x;
+    bool hit = false;
+    if (r < n) {
+      const unsigned long long k = (unsigned long long)load_i64(keys, kdt, b + r);
+      T.has_dup = 0;  // existence only: stop at the first match
+      hit = probe_key(T, k, [](uint32_t, uint32_t) {}) > 0;
+    }
+    const unsigned long long w = __ballot(hit);
+    const uint64_t wbase = base + (threadIdx.x & ~63u);
+    if ((threadIdx.x & 63) == 0 && wbase < n) bitmap[wbase >> 6] = w;
+  }
+}
+
+// ---- fused probe -> group -> exact sum ---------------------------------------------------
+struct ProbeAggDev {
+  const void* keys;
+  const void* val;
+  const void* grp;
+  int32_t kdt, vdt, gdt, group_by;
+  int64_t key_base;
+  uint64_t ngroups;
+  uint64_t b, e;
+  unsigned long long* out_count;
+  unsigned long long* out_sums;
+  unsigned long long* out_oor;
+};
+
+__device__ __forceinline__ void acc_add(unsigned long long* cnt, unsigned long long* sum,
+                                        uint64_t g, int64_t v) {
+  atomicAdd(&cnt[g], 1ull);
+  const unsigned long long old = atomicAdd(&sum[2 * g], (unsigned long long)v);
+  const long long h = (long long)((old + (uint64_t)v) < old) - (long long)(v < 0);
+  if (h) atomicAdd(&sum[2 * g + 1], (unsigned long long)h);
+}
+
+template <bool USE_LDS>
+__global__ void __launch_bounds__(256) k_probe_agg(JoinTableDev T, ProbeAggDev Q) {
+  extern __shared__ __align__(16) unsigned long long ltab[];
+  unsigned long long* cnt;
+  unsigned long long* sum;
+  if (USE_LDS) {
+    cnt = ltab;
+    sum = ltab + Q.ngroups;
+    for (uint64_t i = threadIdx.x; i < 3 * Q.ngroups; i += blockDim.x) ltab[i] = 0;
+    __syncthreads();
+  } else {
+    cnt = Q.out_count;
+    sum = Q.out_sums;
+  }
+  uint32_t oor = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i0 = Q.b + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i0 < Q.e;
+       i0 += stride * kProbeUnroll) {
+    unsigned long long k[kProbeUnroll];
+    int64_t v[kProbeUnroll], g[kProbeUnroll];
+#pragma unroll
+    for (int u = 0; u < kProbeUnroll; ++u) {
+      const uint64_t i = i0 + u * stride;
+      const bool ok = i < Q.e;
+      k[u] = ok ? (unsigned long long)load_i64(Q.keys, Q.kdt, i) : 0;
+      v[u] = ok ? load_i64(Q.val, Q.vdt, i) : 0;
+      g[u] = (ok && Q.group_by == TGP_JOIN_GROUP_PROBE_COL) ? load_i64(Q.grp, Q.gdt, i) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kProbeUnroll; ++u) {
+      if (i0 + u * stride >= Q.e) continue;
+      const int64_t val = v[u];
+      if (Q.group_by == TGP_JOIN_GROUP_BUILD_PAYLOAD) {
+        probe_key(T, k[u], [&](uint32_t, uint32_t pay) {
+          const int64_t gg = (int64_t)(int32_t)pay - Q.key_base;
+          if (gg < 0 || (uint64_t)gg >= Q.ngroups) { ++oor; return; }
+          acc_add(cnt, sum, (uint64_t)gg, val);
+        });
+      } else {
+        const int64_t gg = Q.group_by == TGP_JOIN_GROUP_PROBE_COL ? g[u] - Q.key_base : 0;
+        const bool in = gg >= 0 && (uint64_t)gg < Q.ngroups;
+        probe_key(T, k[u], [&](uint32_t, uint32_t) {
+          if (!in) { ++oor; return; }
+          acc_add(cnt, sum, (uint64_t)gg, val);
+        });
+      }
+    }
+  }
+  if (USE_LDS) {
+    __syncthreads();
+    for (uint64_t gi = threadIdx.x; gi < Q.ngroups; gi += blockDim.x) {
+      if (!cnt[gi]) continue;
+      atomicAdd(&Q.out_count[gi], cnt[gi]);
+      const uint64_t lo = sum[2 * gi];
+      const int64_t hi = (int64_t)sum[2 * gi + 1];
+      if (lo || hi) {
+        const unsigned long long old = atomicAdd(&Q.out_sums[2 * gi], (unsigned long long)lo);
+        const int64_t h = hi + (int64_t)((old + lo) < old);
+        if (h) atomicAdd(&Q.out_sums[2 * gi + 1], (unsigned long long)h);
+      }
+    }
+  }
+  if (oor && Q.out_oor) atomicAdd(Q.out_oor, (unsigned long long)oor);
+}
+
+// ---- inner-join pairs in probe order ---------------------------------------------------------
+constexpr int kPairBlock = 1024;  // probe rows per block (256 threads x 4)
+
+__global__ void __launch_bounds__(256) k_pairs_count(JoinTableDev T, const void* keys, int32_t kdt,
+                                                     uint64_t b, uint64_t e, uint32_t* per_row,
+                                                     uint64_t* block_tot) {
+  __shared__ unsigned long long red[4];
+  const uint64_t r0 = b + blockIdx.x * (uint64_t)kPairBlock;
+  unsigned long long c = 0;
+  for (int u = 0; u < kPairBlock / 256; ++u) {
+    const uint64_t i = r0 + u * 256 + threadIdx.x;
+    if (i >= e) continue;
+    const uint32_t m = probe_key(T, (unsigned long long)load_i64(keys, kdt, i),
+                                 [](uint32_t, uint32_t) {});
+    per_row[i - b] = m;
+    c += m;
+  }
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) block_tot[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void __launch_bounds__(1024) k_scan_blocks(uint64_t* v, uint64_t n) {
+  __shared__ unsigned long long part[1024];
+  const uint64_t per = (n + 1023) / 1024;
+  const uint64_t lo = threadIdx.x * per, hi = min(lo + per, n);
+  unsigned long long s = 0;
+  for (uint64_t i = lo; i < hi; ++i) s += v[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const unsigned long long x = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += x;
+    __syncthreads();
+  }
+  unsigned long long run = threadIdx.x ? part[threadIdx.x - 1] : 0;
+  for (uint64_t i = lo; i < hi; ++i) {
+    const unsigned long long c = v[i];
+    v[i] = run;
+    run += c;
+  }
+  if (threadIdx.x == 1023) v[n] = part[1023];
+}
+
+__global__ void __launch_bounds__(256) k_pairs_write(JoinTableDev T, const void* keys, int32_t kdt,
+                                                     uint64_t b, uint64_t e,
+                                                     const uint32_t* per_row,
+                                                     const uint64_t* block_off,
+                                                     unsigned long long* out_probe,
+                                                     uint32_t* out_pay) {
+  __shared__ unsigned long long pre[kPairBlock];
+  const uint64_t r0 = b + blockIdx.x * (uint64_t)kPairBlock;
+  for (int u = 0; u < kPairBlock / 256; ++u) {
+    const uint64_t i = r0 + u * 256 + threadIdx.x;
+    pre[u * 256 + threadIdx.x] = i < e ? per_row[i - b] : 0;
+  }
+  __syncthreads();
+  // inclusive scan of the block's 1024 counts (Hillis-Steele, 4 per thread)
+  for (int off = 1; off < kPairBlock; off <<= 1) {
+    unsigned long long x[4];
+    for (int u = 0; u < 4; ++u) {
+      const int j = u * 256 + threadIdx.x;
+      x[u] = j >= off ? pre[j - off] : 0;
+    }
+    __syncthreads();
+    for (int u = 0; u < 4; ++u) pre[u * 256 + threadIdx.x] += x[u];
+    __syncthreads();
+  }
+  const uint64_t boff = block_off[blockIdx.x];
+  for (int u = 0; u < kPairBlock / 256; ++u) {
+    const int j = u * 256 + threadIdx.x;
+    const uint64_t i = r0 + j;
+    if (i >= e) continue;
+    const uint32_t m = per_row[i - b];
+    if (!m) continue;
+    const uint64_t start = boff + pre[j] - m;
+    uint32_t w = 0;
+    probe_key(T, (unsigned long long)load_i64(keys, kdt, i), [&](uint32_t row, uint32_t pay) {
+      out_probe[start + w] = i;
+      out_pay[start + w] = pay;
+      ++w;
+    });
+    // duplicates: ascending build payload within the probe row (insertion sort)
+    for (uint32_t x = 1; x < m; ++x) {
+      const uint32_t key = out_pay[start + x];
+      uint32_t y = x;
+      while (y > 0 && out_pay[start + y - 1] > key) {
+        out_pay[start + y] = out_pay[start + y - 1];
+        --y;
+      }
+      out_pay[start + y] = key;
+    }
+  }
+}
+
+static unsigned grid_for(uint64_t items, uint64_t cap = 8192) {
+  uint64_t g = (items + 255) / 256;
+  if (g > cap) g = cap;
+  return (unsigned)(g ? g : 1);
+}
+
+}  // namespace tgp
+
+using namespace tgp;
+
+struct tgp_hash_table {
+  tgp_ctx* ctx = nullptr;
+  void* mem = nullptr;       // slots, then 2 flag words
+  void* side_mem = nullptr;  // side list (only when sentinel keys exist)
+  uint64_t cap = 0, n_rows = 0, n_side = 0, bytes = 0;
+  int32_t has_dup = 0;
+  JoinTableDev dev() const {
+    JoinTableDev d;
+    d.slots = static_cast<Slot*>(mem);
+    d.mask = cap - 1;
+    d.side = static_cast<const uint2*>(side_mem);
+    d.n_side = n_side;
+    d.has_dup = has_dup;
+    d._pad = 0;
+    return d;
+  }
+};
+
+static uint64_t table_capacity(uint64_t n) {
+  uint64_t c = 16;
+  while (c < 2 * n) c <<= 1;  // join_hash_table.rs:76-85 sizes its directory the same way
+  return c;
+}
+
+static int32_t check_key_column(const tgp_column* c, const char* what) {
+  TGP_REQUIRE(c != nullptr, "%s: null column", what);
+  TGP_REQUIRE(dtype_width(c->dtype) > 0, "%s: unknown dtype", what);
+  TGP_REQUIRE(c->len == 0 || c->data != nullptr, "%s: null data", what);
+  return TGP_OK;
+}
+
+extern "C" {
+
+uint64_t tgp_hash_table_bytes_for(uint64_t n_build_rows) {
+  return table_capacity(n_build_rows) * sizeof(Slot) + 16;
+}
+
+int32_t tgp_hash_build(tgp_ctx* ctx, const tgp_column* keys, const tgp_column* payload,
+                       tgp_hash_table** out, void* stream) {
+  TGP_REQUIRE(ctx && out, "tgp_hash_build: null argument");
+  *out = nullptr;
+  TGP_TRY(check_injected(ctx));
+  TGP_TRY(check_key_column(keys, "tgp_hash_build keys"));
+  if (payload) {
+    TGP_TRY(check_key_column(payload, "tgp_hash_build payload"));
+    TGP_REQUIRE(payload->len >= keys->len, "tgp_hash_build: payload shorter than keys");
+  }
+  TGP_REQUIRE(keys->len < (1ull << 32), "tgp_hash_build: build rows must fit in 32 bits");
+  hipStream_t s = pick_stream(ctx, stream);
+  tgp_hash_table* t = new tgp_hash_table();
+  t->ctx = ctx;
+  t->n_rows = keys->len;
+  t->cap = table_capacity(keys->len);
+  t->bytes = t->cap * sizeof(Slot) + 16;
+  int32_t rc = hbm_alloc(ctx, t->bytes, &t->mem);
+  if (rc != TGP_OK) {
+    delete t;
+    return rc;
+  }
+  Slot* slots = static_cast<Slot*>(t->mem);
+  unsigned long long* flags = reinterpret_cast<unsigned long long*>(slots + t->cap);
+  auto fail = [&](int32_t code) {
+    hbm_free(ctx, t->mem);
+    delete t;
+    return code;
+  };
+  if (fill_u64(s, reinterpret_cast<uint64_t*>(flags), 2, 0) != TGP_OK) return fail(TGP_ERR_HIP);
+  k_table_init<<<grid_for(t->cap), 256, 0, s>>>(slots, t->cap);
+  const void* pd = payload ? payload->data : nullptr;
+  const int32_t pdt = payload ? payload->dtype : TGP_U32;
+  if (keys->len)
+    k_build<<<grid_for(keys->len), 256, 0, s>>>(keys->data, keys->dtype, pd, pdt, keys->len, slots,
+                                                t->cap - 1, flags);
+  if (hipGetLastError() != hipSuccess) {
+    set_error("tgp_hash_build: kernel launch failed");
+    return fail(TGP_ERR_HIP);
+  }
+  unsigned long long hflags[2] = {0, 0};
+  if (hipMemcpyAsync(hflags, flags, 16, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    set_error("tgp_hash_build: reading build flags failed");
+    return fail(TGP_ERR_HIP);
+  }
+  t->has_dup = hflags[0] ? 1 : 0;
+  t->n_side = hflags[1];
+  if (t->n_side) {
+    rc = hbm_alloc(ctx, t->n_side * sizeof(uint2) + 8, &t->side_mem);
+    if (rc != TGP_OK) return fail(rc);
+    t->bytes += t->n_side * sizeof(uint2) + 8;
+    unsigned long long* counter =
+        reinterpret_cast<unsigned long long*>(static_cast<uint2*>(t->side_mem) + t->n_side);
+    fill_u64(s, reinterpret_cast<uint64_t*>(counter), 1, 0);
+    k_build_side<<<grid_for(keys->len), 256, 0, s>>>(keys->data, keys->dtype, pd, pdt, keys->len,
+                                                     static_cast<uint2*>(t->side_mem), counter);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+      hbm_free(ctx, t->side_mem);
+      set_error("tgp_hash_build: side list failed");
+      return fail(TGP_ERR_HIP);
+    }
+    if (t->n_side > 1) t->has_dup = 1;
+  }
+  *out = t;
+  return TGP_OK;
+}
+
+int32_t tgp_hash_table_destroy(tgp_ctx* ctx, tgp_hash_table* t) {
+  if (!t) return TGP_OK;
+  TGP_REQUIRE(ctx && t->ctx == ctx, "tgp_hash_table_destroy: table belongs to another context");
+  TGP_HIP(hipStreamSynchronize(ctx->stream));
+  int32_t rc = hbm_free(ctx, t->mem);
+  if (t->side_mem) hbm_free(ctx, t->side_mem);
+  delete t;
+  return rc;
+}
+
+int32_t tgp_hash_table_footprint(const tgp_hash_table* t, uint64_t* out_bytes) {
+  TGP_REQUIRE(t && out_bytes, "tgp_hash_table_footprint: null argument");
+  *out_bytes = t->bytes;
+  return TGP_OK;
+}
+
+int32_t tgp_hash_table_info(const tgp_hash_table* t, uint64_t* n_rows, uint64_t* capacity,
+                            int32_t* has_duplicates) {
+  TGP_REQUIRE(t, "tgp_hash_table_info: null table");
+  if (n_rows) *n_rows = t->n_rows;
+  if (capacity) *capacity = t->cap;
+  if (has_duplicates) *has_duplicates = t->has_dup;
+  return TGP_OK;
+}
+
+int32_t tgp_hash_probe_count(tgp_ctx* ctx, const tgp_hash_table* t, const tgp_column* probe_keys,
+                             uint64_t row_begin, uint64_t row_end, uint64_t* d_count,
+                             void* stream) {
+  TGP_REQUIRE(ctx && t && d_count, "tgp_hash_probe_count: null argument");
+  TGP_TRY(check_injected(ctx));
+  TGP_TRY(check_key_column(probe_keys, "tgp_hash_probe_count keys"));
+  TGP_REQUIRE(row_begin <= row_end && row_end <= probe_keys->len, "bad probe row range");
+  if (row_begin == row_end) return TGP_OK;
+  k_probe_count<<<grid_for((row_end - row_begin + kProbeUnroll - 1) / kProbeUnroll, 4096), 256, 0,
+                  pick_stream(ctx, stream)>>>(t->dev(), probe_keys->data, probe_keys->dtype,
+                                              row_begin, row_end,
+                                              reinterpret_cast<unsigned long long*>(d_count));
+  TGP_HIP(hipGetLastError());
+  return TGP_OK;
+}
+
+int32_t tgp_hash_probe_mark(tgp_ctx* ctx, const tgp_hash_table* t, const tgp_column* probe_keys,
+                            uint64_t row_begin, uint64_t row_end, uint64_t* d_bitmap,
+                            void* stream) {
+  TGP_REQUIRE(ctx && t && d_bitmap, "tgp_hash_probe_mark: null argument");
+  TGP_TRY(check_injected(ctx));
+  TGP_TRY(check_key_column(probe_keys, "tgp_hash_probe_mark keys"));
+  TGP_REQUIRE(row_begin <= row_end && row_end <= probe_keys->len, "bad probe row range");
+  if (row_begin == row_end) return TGP_OK;
+  k_probe_mark<<<grid_for(row_end - row_begin, 4096), 256, 0, pick_stream(ctx, stream)>>>(
+      t->dev(), probe_keys->data, probe_keys->dtype, row_begin, row_end, d_bitmap);
+  TGP_HIP(hipGetLastError());
+  return TGP_OK;
+}
+
+int32_t tgp_hash_probe_pairs_sync(tgp_ctx* ctx, const tgp_hash_table* t,
+                                  const tgp_column* probe_keys, uint64_t row_begin,
+                                  uint64_t row_end, uint64_t* d_probe_idx,
+                                  uint32_t* d_build_payload, uint64_t capacity, uint64_t* h_npairs,
+                                  void* stream) {
+  TGP_REQUIRE(ctx && t && h_npairs, "tgp_hash_probe_pairs_sync: null argument");
+  TGP_TRY(check_injected(ctx));
+  TGP_TRY(check_key_column(probe_keys, "tgp_hash_probe_pairs_sync keys"));
+  TGP_REQUIRE(row_begin <= row_end && row_end <= probe_keys->len, "bad probe row range");
+  *h_npairs = 0;
+  const uint64_t n = row_end - row_begin;
+  if (n == 0) return TGP_OK;
+  hipStream_t s = pick_stream(ctx, stream);
+  const uint64_t nblocks = (n + kPairBlock - 1) / kPairBlock;
+  void* scratch = nullptr;
+  TGP_TRY(get_scratch(ctx, s, n * 4 + (nblocks + 1) * 8 + 16, &scratch));
+  uint64_t* block_tot = static_cast<uint64_t*>(scratch);
+  uint32_t* per_row = reinterpret_cast<uint32_t*>(block_tot + nblocks + 1);
+  k_pairs_count<<<(unsigned)nblocks, 256, 0, s>>>(t->dev(), probe_keys->data, probe_keys->dtype,
+                                                  row_begin, row_end, per_row, block_tot);
+  TGP_HIP(hipGetLastError());
+  k_scan_blocks<<<1, 1024, 0, s>>>(block_tot, nblocks);
+  TGP_HIP(hipGetLastError());
+  uint64_t total = 0;
+  TGP_HIP(hipMemcpyAsync(&total, block_tot + nblocks, 8, hipMemcpyDeviceToHost, s));
+  TGP_HIP(hipStreamSynchronize(s));
+  *h_npairs = total;
+  if (total > capacity) {
+    set_error("join produces %llu pairs, capacity %llu", (unsigned long long)total,
+              (unsigned long long)capacity);
+    return TGP_ERR_CAPACITY;
+  }
+  if (total == 0) return TGP_OK;
+  TGP_REQUIRE(d_probe_idx && d_build_payload, "tgp_hash_probe_pairs_sync: null output");
+  k_pairs_write<<<(unsigned)nblocks, 256, 0, s>>>(
+      t->dev(), probe_keys->data, probe_keys->dtype, row_begin, row_end, per_row, block_tot,
+      reinterpret_cast<unsigned long long*>(d_probe_idx), d_build_payload);
+  TGP_HIP(hipGetLastError());
+  TGP_HIP(hipStreamSynchronize(s));
+  return TGP_OK;
+}
+
+int32_t tgp_hash_probe_agg(tgp_ctx* ctx, const tgp_join_agg_query* q, const tgp_group_result* out,
+                           void* stream) {
+  TGP_REQUIRE(ctx && q && out && q->table, "tgp_hash_probe_agg: null argument");
+  TGP_TRY(check_injected(ctx));
+  TGP_REQUIRE(out->d_count && out->d_sums, "tgp_hash_probe_agg: null result buffer");
+  TGP_TRY(check_key_column(&q->probe_keys, "probe keys"));
+  TGP_TRY(check_key_column(&q->value, "value column"));
+  TGP_REQUIRE(q->group_by >= TGP_JOIN_GROUP_BUILD_PAYLOAD && q->group_by <= TGP_JOIN_GROUP_NONE,
+              "bad group_by %d", q->group_by);
+  if (q->group_by == TGP_JOIN_GROUP_PROBE_COL) {
+    TGP_TRY(check_key_column(&q->group_col, "group column"));
+    TGP_REQUIRE(q->group_col.len >= q->row_end, "group column shorter than row_end");
+  }
+  TGP_REQUIRE(q->ngroups >= 1 && q->ngroups <= (1ull << 32), "ngroups out of range");
+  TGP_REQUIRE(q->group_by != TGP_JOIN_GROUP_NONE || q->ngroups == 1,
+              "GROUP_NONE needs ngroups == 1");
+  TGP_REQUIRE(q->row_begin <= q->row_end && q->row_end <= q->probe_keys.len &&
+                  q->row_end <= q->value.len,
+              "bad probe row range");
+  hipStream_t s = pick_stream(ctx, stream);
+  TGP_TRY(fill_u64(s, out->d_count, q->ngroups, 0));
+  TGP_TRY(fill_u64(s, out->d_sums, 2 * q->ngroups, 0));
+  if (out->d_n_out_of_range) TGP_TRY(fill_u64(s, out->d_n_out_of_range, 1, 0));
+  const uint64_t n = q->row_end - q->row_begin;
+  if (n == 0) return TGP_OK;
+  ProbeAggDev Q;
+  Q.keys = q->probe_keys.data;
+  Q.kdt = q->probe_keys.dtype;
+  Q.val = q->value.data;
+  Q.vdt = q->value.dtype;
+  Q.grp = q->group_col.data;
+  Q.gdt = q->group_col.dtype;
+  Q.group_by = q->group_by;
+  Q.key_base = q->group_by == TGP_JOIN_GROUP_NONE ? 0 : q->key_base;
+  Q.ngroups = q->ngroups;
+  Q.b = q->row_begin;
+  Q.e = q->row_end;
+  Q.out_count = reinterpret_cast<unsigned long long*>(out->d_count);
+  Q.out_sums = reinterpret_cast<unsigned long long*>(out->d_sums);
+  Q.out_oor = reinterpret_cast<unsigned long long*>(out->d_n_out_of_range);
+  const uint64_t lds = 3 * 8 * q->ngroups;
+  const unsigned grid = grid_for((n + kProbeUnroll - 1) / kProbeUnroll,
+                                 (uint64_t)persistent_grid(ctx, 8));
+  if (lds <= 48 * 1024)
+    k_probe_agg<true><<<grid, 256, lds, s>>>(q->table->dev(), Q);
+  else
+    k_probe_agg<false><<<grid, 256, 0, s>>>(q->table->dev(), Q);
+  TGP_HIP(hipGetLastError());
+  return TGP_OK;
+}
+
+}  // extern "C"

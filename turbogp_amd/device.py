@@ -337,3 +337,103 @@ def make_agg_query(cols, preds=(), parts=(), aggs=(), row_begin=0, row_end=None,
     q.naggs = len(aggs)
     q.d_sel_bitmap = sel_bitmap.buf.ptr if sel_bitmap is not None else None
     return q
+
+
+class HashTable:
+    """Device hash table of a join's build side (tgp_hash_build).
+    Mirrors JoinHashTable (exec/join_hash_table.rs) / HashBuildScalar (kernel/hash.rs)."""
+
+    def __init__(self, ctx, keys, payload=None, stream=None):
+        self.ctx = ctx
+        h = C.c_void_p()
+        kd = keys.desc()
+        pd = payload.desc() if payload is not None else None
+        A.check(A.lib().tgp_hash_build(ctx.handle, C.byref(kd), C.byref(pd) if pd else None,
+                                       C.byref(h), stream), "tgp_hash_build")
+        self.handle = h.value
+
+    def footprint(self):
+        b = C.c_uint64()
+        A.check(A.lib().tgp_hash_table_footprint(self.handle, C.byref(b)),
+                "tgp_hash_table_footprint")
+        return b.value
+
+    def info(self):
+        n, cap, dup = C.c_uint64(), C.c_uint64(), C.c_int32()
+        A.check(A.lib().tgp_hash_table_info(self.handle, C.byref(n), C.byref(cap), C.byref(dup)),
+                "tgp_hash_table_info")
+        return n.value, cap.value, bool(dup.value)
+
+    def probe_count(self, keys, row_begin=0, row_end=None, stream=None):
+        kd = keys.desc()
+        row_end = keys.n if row_end is None else row_end
+        cnt = DeviceBuffer(self.ctx, 8)
+        try:
+            cnt.fill_u64(0, stream=stream)
+            A.check(A.lib().tgp_hash_probe_count(self.ctx.handle, self.handle, C.byref(kd),
+                                                 row_begin, row_end, cnt.ptr, stream),
+                    "tgp_hash_probe_count")
+            return int(cnt.download(np.uint64, 1, stream)[0])
+        finally:
+            cnt.free()
+
+    def probe_mark(self, keys, row_begin=0, row_end=None, stream=None):
+        kd = keys.desc()
+        row_end = keys.n if row_end is None else row_end
+        bm = DeviceBitmap(self.ctx, row_end - row_begin)
+        A.check(A.lib().tgp_hash_probe_mark(self.ctx.handle, self.handle, C.byref(kd), row_begin,
+                                            row_end, bm.buf.ptr, stream), "tgp_hash_probe_mark")
+        return bm
+
+    def probe_pairs(self, keys, row_begin=0, row_end=None, capacity=None, stream=None):
+        kd = keys.desc()
+        row_end = keys.n if row_end is None else row_end
+        npairs = C.c_uint64()
+        if capacity is None:
+            rc = A.lib().tgp_hash_probe_pairs_sync(self.ctx.handle, self.handle, C.byref(kd),
+                                                   row_begin, row_end, None, None, 0,
+                                                   C.byref(npairs), stream)
+            if rc not in (A.OK, A.ERR_CAPACITY):
+                A.check(rc, "tgp_hash_probe_pairs_sync")
+            capacity = npairs.value
+        pi = DeviceBuffer(self.ctx, max(8, capacity * 8))
+        pp = DeviceBuffer(self.ctx, max(8, capacity * 4))
+        try:
+            A.check(A.lib().tgp_hash_probe_pairs_sync(self.ctx.handle, self.handle, C.byref(kd),
+                                                      row_begin, row_end, pi.ptr, pp.ptr,
+                                                      capacity, C.byref(npairs), stream),
+                    "tgp_hash_probe_pairs_sync")
+            n = npairs.value
+            return pi.download(np.uint64, n), pp.download(np.uint32, n)
+        finally:
+            pi.free()
+            pp.free()
+
+    def probe_agg(self, keys, value, group_by, ngroups=1, key_base=0, group_col=None,
+                  row_begin=0, row_end=None, result=None, stream=None, fetch=True):
+        q = A.JoinAggQuery()
+        q.table = self.handle
+        q.probe_keys = keys.desc()
+        q.value = value.desc()
+        if group_col is not None:
+            q.group_col = group_col.desc()
+        q.group_by = group_by
+        q.key_base = key_base
+        q.ngroups = ngroups
+        q.row_begin = row_begin
+        q.row_end = keys.n if row_end is None else row_end
+        res = result or GroupAggBuffers(self.ctx, ngroups, 1)
+        A.check(A.lib().tgp_hash_probe_agg(self.ctx.handle, C.byref(q), C.byref(res.desc),
+                                           stream), "tgp_hash_probe_agg")
+        if not fetch:
+            return res
+        out = res.fetch(stream)
+        if result is None:
+            res.free()
+        return out
+
+    def free(self):
+        if self.handle and self.ctx.handle:
+            A.check(A.lib().tgp_hash_table_destroy(self.ctx.handle, self.handle),
+                    "tgp_hash_table_destroy")
+        self.handle = None

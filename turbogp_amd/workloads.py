@@ -98,3 +98,77 @@ def host_fact_columns(n, gen):
 
 NP_DTYPE = {A.I8: np.int8, A.U8: np.uint8, A.I16: np.int16, A.U16: np.uint16,
             A.I32: np.int32, A.U32: np.uint32, A.I64: np.int64, A.U64: np.uint64}
+
+
+# ---- hash-join workload (build 2M unique keys, probe 60M Zipf(1.1) foreign keys) ----------
+JOIN_BUILD_ROWS = 2_000_000
+JOIN_PROBE_ROWS = 60_000_000
+JOIN_GKEY_CARD = 1_500_000
+JOIN_PAY_CARD = 25
+ZIPF_S = 1.1
+# column ids of the join tables in the counter-based generator
+COL_PAY, COL_FK, COL_VAL, COL_GKEY = 10, 11, 12, 13
+_KEY_SALT = 0x6A6F696E5F6B6579  # spreads the permuted build keys over int64
+
+
+def mix64_np(x):
+    """SplitMix64 finaliser over a uint64 numpy array."""
+    z = x.astype(np.uint64)
+    with np.errstate(over="ignore"):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def join_build_side(n_build, seed=SEED):
+    """Build keys: a seeded permutation of 0..n-1 passed through mix64 (unique,
+    spread over int64); payload int32 uniform 0..24 (counter-based)."""
+    rng = np.random.Generator(np.random.PCG64(seed & 0xFFFFFFFF))
+    perm = rng.permutation(n_build).astype(np.uint64)
+    bkey = mix64_np(perm ^ np.uint64(_KEY_SALT)).view(np.int64)
+    return bkey, perm
+
+
+def zipf_cdf(n, s=ZIPF_S):
+    w = np.arange(1, n + 1, dtype=np.float64) ** (-s)
+    cdf = np.cumsum(w)
+    cdf /= cdf[-1]
+    cdf[-1] = 1.0
+    return cdf
+
+
+def zipf_key_table(bkey, seed=SEED):
+    """rank r -> build key of a seeded permutation of the build rows."""
+    rng = np.random.Generator(np.random.PCG64((seed >> 32) & 0xFFFFFFFF))
+    perm2 = rng.permutation(len(bkey))
+    return np.ascontiguousarray(bkey[perm2])
+
+
+def device_lookup_cdf(ctx, n, dtype, column_id, cdf, table, seed=SEED, stream=None):
+    """Device column out[i] = table[#(cdf <= unit(u_i))] (tgp_synth_lookup_cdf)."""
+    import ctypes as C
+    from .device import DeviceBuffer
+    dcdf = DeviceBuffer(ctx, len(cdf) * 8)
+    dtab = DeviceBuffer(ctx, len(table) * 8)
+    try:
+        dcdf.upload(np.ascontiguousarray(cdf, np.float64))
+        dtab.upload(np.ascontiguousarray(table, np.int64))
+        col = DeviceColumn(ctx, n, dtype)
+        d = col.desc()
+        A.check(A.lib().tgp_synth_lookup_cdf(ctx.handle, C.byref(d), seed, column_id, dcdf.ptr,
+                                             len(cdf), dtab.ptr, 0, n, stream),
+                "tgp_synth_lookup_cdf")
+        ctx.sync(stream)
+        return col
+    finally:
+        dcdf.free()
+        dtab.free()
+
+
+def device_uniform(ctx, n, dtype, column_id, lo, hi, seed=SEED, stream=None):
+    import ctypes as C
+    col = DeviceColumn(ctx, n, dtype)
+    d = col.desc()
+    A.check(A.lib().tgp_synth_uniform(ctx.handle, C.byref(d), seed, column_id, lo, hi, 0, n,
+                                      stream), "tgp_synth_uniform")
+    return col
