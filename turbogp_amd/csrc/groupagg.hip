@@ -22,11 +22,9 @@
 
 namespace tgp {
 
-constexpr int kRowsPerThread = kTileRows / kBlock;
-
 // ---- register strategy ----------------------------------------------------------------
-template <int NST, int GMAX, int AMAX>
-__global__ void __launch_bounds__(kBlock) k_agg_reg(DevPlan P) {
+template <int GMAX, int AMAX>
+__global__ void __launch_bounds__(kBlock, 2) k_agg_reg(DevPlan P) {
   extern __shared__ __align__(16) uint8_t lds[];
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint32_t wave = threadIdx.x / kWave;
@@ -40,45 +38,42 @@ __global__ void __launch_bounds__(kBlock) k_agg_reg(DevPlan P) {
     for (int a = 0; a < AMAX; ++a) { lo[g][a] = 0; hi[g][a] = 0; }
   }
   uint32_t oor = 0;
-  u32x4 reg[NST];
-  uint32_t addr[NST];
-  uint64_t t = blockIdx.x;
-  if (t < P.ntiles) {
-    const uint64_t r0 = P.row_begin + t * kTileRows;
-    stage_load<NST>(P, r0, min((uint64_t)kTileRows, P.row_end - r0), reg, addr);
-  }
-  for (; t < P.ntiles; t += gridDim.x) {
-    const uint64_t r0 = P.row_begin + t * kTileRows;
-    __syncthreads();
-    stage_store<NST>(lds, reg, addr);
-    __syncthreads();
-    const uint64_t tn = t + gridDim.x;
-    if (tn < P.ntiles) {
-      const uint64_t rn = P.row_begin + tn * kTileRows;
-      stage_load<NST>(P, rn, min((uint64_t)kTileRows, P.row_end - rn), reg, addr);
+  for_each_tile(P, lds, [&](const uint8_t* buf, uint64_t row0) {
+    uint32_t sel = eval_select(P, buf, row0);
+    if (__ballot(sel != 0) == 0) return;  // nothing selected in this wave's rows
+    int64_t gid[kRows];
+    if (P.nparts) {
+      eval_groups(P, buf, gid);
+    } else {
+#pragma unroll
+      for (int r = 0; r < kRows; ++r) gid[r] = 0;
     }
 #pragma unroll
-    for (int k = 0; k < kRowsPerThread; ++k) {
-      const uint32_t j = threadIdx.x + k * kBlock;
-      if (r0 + j >= P.row_end || !row_selected(P, lds, r0, j)) continue;
-      const int64_t gid = P.nparts ? group_id(P, lds, r0, j) : 0;
-      if (gid < 0 || gid >= GMAX) { ++oor; continue; }
-      int64_t v[AMAX];
+    for (int r = 0; r < kRows; ++r) {
+      if (!((sel >> r) & 1)) continue;
+      if (gid[r] < 0 || gid[r] >= GMAX) {
+        ++oor;
+        sel &= ~(1u << r);
+        continue;
+      }
 #pragma unroll
-      for (int a = 0; a < AMAX; ++a) v[a] = a < P.naggs ? agg_value(P, P.aggs[a], lds, r0, j) : 0;
+      for (int g = 0; g < GMAX; ++g) cnt[g] += gid[r] == g;
+    }
 #pragma unroll
-      for (int g = 0; g < GMAX; ++g) {
-        if (gid == g) {
-          ++cnt[g];
+    for (int a = 0; a < AMAX; ++a) {
+      if (a >= P.naggs) break;
+      uint64_t v[kRows];
+      eval_agg(P.aggs[a], buf, v);
 #pragma unroll
-          for (int a = 0; a < AMAX; ++a)
-            if (a < P.naggs) add128(lo[g][a], hi[g][a], v[a]);
-        }
+      for (int r = 0; r < kRows; ++r) {
+        if (!((sel >> r) & 1)) continue;
+#pragma unroll
+        for (int g = 0; g < GMAX; ++g)
+          if (gid[r] == g) add128(lo[g][a], hi[g][a], (int64_t)v[r]);
       }
     }
-  }
+  });
   // ---- flush: wave reduction, then one workgroup reduction through LDS ----
-  __syncthreads();
   constexpr int W = 1 + 2 * AMAX;  // words per group: count, (lo, hi) per agg
   unsigned long long* red = reinterpret_cast<unsigned long long*>(lds);
 #pragma unroll
@@ -128,56 +123,47 @@ __global__ void __launch_bounds__(kBlock) k_agg_reg(DevPlan P) {
 }
 
 // ---- table strategies (LDS table or global table) ----------------------------------------
-template <int NST, bool USE_LDS>
-__global__ void __launch_bounds__(kBlock) k_agg_table(DevPlan P) {
+template <bool USE_LDS>
+__global__ void __launch_bounds__(kBlock, 2) k_agg_table(DevPlan P) {
   extern __shared__ __align__(16) uint8_t lds[];
   unsigned long long* tcnt;
   unsigned long long* tsum;
   if (USE_LDS) {
-    tcnt = reinterpret_cast<unsigned long long*>(lds + P.stage_bytes);
+    tcnt = reinterpret_cast<unsigned long long*>(lds + 2 * P.stage_bytes);
     tsum = tcnt + P.ngroups;
     const uint64_t words = P.ngroups * (1 + 2 * (uint64_t)P.naggs);
     for (uint64_t i = threadIdx.x; i < words; i += kBlock) tcnt[i] = 0;
+    // made visible by the barrier at the top of the first tile
   } else {
     tcnt = P.out_count;
     tsum = P.out_sums;
   }
   uint32_t oor = 0;
-  u32x4 reg[NST];
-  uint32_t addr[NST];
-  uint64_t t = blockIdx.x;
-  if (t < P.ntiles) {
-    const uint64_t r0 = P.row_begin + t * kTileRows;
-    stage_load<NST>(P, r0, min((uint64_t)kTileRows, P.row_end - r0), reg, addr);
-  }
-  for (; t < P.ntiles; t += gridDim.x) {
-    const uint64_t r0 = P.row_begin + t * kTileRows;
-    __syncthreads();
-    stage_store<NST>(lds, reg, addr);
-    __syncthreads();
-    const uint64_t tn = t + gridDim.x;
-    if (tn < P.ntiles) {
-      const uint64_t rn = P.row_begin + tn * kTileRows;
-      stage_load<NST>(P, rn, min((uint64_t)kTileRows, P.row_end - rn), reg, addr);
-    }
+  for_each_tile(P, lds, [&](const uint8_t* buf, uint64_t row0) {
+    uint32_t sel = eval_select(P, buf, row0);
+    if (__ballot(sel != 0) == 0) return;
+    int64_t gid[kRows];
+    eval_groups(P, buf, gid);
 #pragma unroll
-    for (int k = 0; k < kRowsPerThread; ++k) {
-      const uint32_t j = threadIdx.x + k * kBlock;
-      if (r0 + j >= P.row_end || !row_selected(P, lds, r0, j)) continue;
-      const int64_t gid = group_id(P, lds, r0, j);
-      if (gid < 0) { ++oor; continue; }
-      atomicAdd(&tcnt[gid], 1ull);
-      for (int a = 0; a < P.naggs; ++a) {
-        const int64_t v = agg_value(P, P.aggs[a], lds, r0, j);
-        unsigned long long* p = &tsum[((uint64_t)gid * P.naggs + a) * 2];
-        const unsigned long long old = atomicAdd(p, (unsigned long long)v);
-        const long long h = (long long)((old + (uint64_t)v) < old) - (long long)(v < 0);
-        if (h) atomicAdd(p + 1, (unsigned long long)h);
+    for (int r = 0; r < kRows; ++r) {
+      if (!((sel >> r) & 1)) continue;
+      if (gid[r] < 0) {
+        ++oor;
+        sel &= ~(1u << r);
+        continue;
       }
+      atomicAdd(&tcnt[gid[r]], 1ull);
     }
-  }
+    for (int a = 0; a < P.naggs; ++a) {
+      uint64_t v[kRows];
+      eval_agg(P.aggs[a], buf, v);
+#pragma unroll
+      for (int r = 0; r < kRows; ++r)
+        if ((sel >> r) & 1)
+          atomic_add128_i64(&tsum[((uint64_t)gid[r] * P.naggs + a) * 2], (int64_t)v[r]);
+    }
+  });
   if (USE_LDS) {
-    __syncthreads();
     for (uint64_t g = threadIdx.x; g < P.ngroups; g += kBlock) {
       const unsigned long long c = tcnt[g];
       if (!c) continue;
@@ -191,56 +177,18 @@ __global__ void __launch_bounds__(kBlock) k_agg_table(DevPlan P) {
   if (oor && P.out_oor) atomicAdd(P.out_oor, (unsigned long long)oor);
 }
 
-template <typename K>
-static int32_t launch_persistent(tgp_ctx* ctx, K kern, const DevPlan& P, uint32_t lds_bytes,
-                                 hipStream_t s) {
-  if (lds_bytes > 65536)
-    TGP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds_bytes));
-  int per_cu = 0;
-  TGP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, lds_bytes));
-  if (per_cu < 1) per_cu = 1;
-  uint64_t grid = (uint64_t)persistent_grid(ctx, per_cu);
-  if (grid > P.ntiles) grid = P.ntiles;
-  kern<<<dim3((unsigned)grid), dim3(kBlock), lds_bytes, s>>>(P);
-  TGP_HIP(hipGetLastError());
-  return TGP_OK;
-}
-
-template <int NST>
-static int32_t launch_agg_nst(tgp_ctx* ctx, const DevPlan& P, const PlanInfo& info,
-                              hipStream_t s) {
-  uint32_t lds = info.lds_bytes;
-  switch (info.strategy) {
-    case kAggReg1: {
-      const uint32_t red = 4 * 1 * (1 + 2 * TGP_MAX_AGGS) * 8;
-      return launch_persistent(ctx, k_agg_reg<NST, 1, TGP_MAX_AGGS>, P, lds > red ? lds : red, s);
-    }
-    case kAggReg8: {
-      const uint32_t red = 4 * 8 * (1 + 2 * 4) * 8;
-      return launch_persistent(ctx, k_agg_reg<NST, 8, 4>, P, lds > red ? lds : red, s);
-    }
-    case kAggLds: return launch_persistent(ctx, k_agg_table<NST, true>, P, lds, s);
-    default: return launch_persistent(ctx, k_agg_table<NST, false>, P, lds, s);
-  }
-}
-
 int32_t run_agg(tgp_ctx* ctx, const DevPlan& P, const PlanInfo& info, hipStream_t s) {
   // Results are accumulated with atomics: clear them first (a fill kernel, so the
   // sequence can be captured in a graph).
   TGP_TRY(fill_u64(s, reinterpret_cast<uint64_t*>(P.out_count), P.ngroups, 0));
   TGP_TRY(fill_u64(s, reinterpret_cast<uint64_t*>(P.out_sums), P.ngroups * P.naggs * 2, 0));
   if (P.out_oor) TGP_TRY(fill_u64(s, reinterpret_cast<uint64_t*>(P.out_oor), 1, 0));
-  if (P.ntiles == 0) return TGP_OK;
-  switch (info.nst) {
-    case 2: return launch_agg_nst<2>(ctx, P, info, s);
-    case 4: return launch_agg_nst<4>(ctx, P, info, s);
-    case 8: return launch_agg_nst<8>(ctx, P, info, s);
-    case 12: return launch_agg_nst<12>(ctx, P, info, s);
-    case kMaxNst: return launch_agg_nst<kMaxNst>(ctx, P, info, s);
+  switch (info.strategy) {
+    case kAggReg1: return launch_tiles(ctx, k_agg_reg<1, TGP_MAX_AGGS>, P, info.lds_bytes, s);
+    case kAggReg8: return launch_tiles(ctx, k_agg_reg<8, 4>, P, info.lds_bytes, s);
+    case kAggLds: return launch_tiles(ctx, k_agg_table<true>, P, info.lds_bytes, s);
+    default: return launch_tiles(ctx, k_agg_table<false>, P, info.lds_bytes, s);
   }
-  set_error("no aggregation kernel for %d staging rounds", info.nst);
-  return TGP_ERR_UNSUPPORTED;
 }
 
 }  // namespace tgp

@@ -1,5 +1,5 @@
 // Host-side query planner: validates a tgp_agg_query (or a scan's predicate
-// list) and lays it out as a DevPlan for the tile engine.
+// list) and compiles it into the tile engine's per-slot program (DevPlan).
 #pragma once
 
 #include "tile_engine.h"
@@ -14,25 +14,44 @@ enum AggStrategy {
 };
 
 struct PlanInfo {
-  int nst = 0;              // staging rounds (16-byte chunks per thread per tile)
-  uint32_t lds_bytes = 0;   // dynamic LDS of the kernel (stage + table)
+  uint32_t lds_bytes = 0;  // dynamic LDS of the kernel (2 stage buffers + table)
   int strategy = kAggReg1;
 };
 
-// Largest staging-round count the kernels are instantiated for.
-constexpr int kMaxNst = 17;
 // LDS table limit of the kAggLds strategy.
 constexpr uint32_t kLdsTableBytes = 48 * 1024;
+// LDS of the register strategies' final workgroup reduction.
+constexpr uint32_t kRegReduceBytes = 4 * 8 * (1 + 2 * TGP_MAX_AGGS) * 8;
 
-int nst_bucket(int nst);  // smallest instantiated round count >= nst (0 => too big)
-
-// Fills the slot, chunk and tile fields of P for columns `cols`.
+// Fills slots and tile geometry of P for columns `cols` and the row range.
 int32_t plan_slots(const tgp_column* cols, int32_t ncols, uint64_t row_begin, uint64_t row_end,
                    DevPlan* P, PlanInfo* info);
-int32_t plan_preds(const tgp_pred* preds, int32_t npreds, int32_t ncols, DevPlan* P);
-// Full aggregation plan (slots, predicates, group parts, aggregates, strategy).
+// Predicate-only program (scans).
+int32_t plan_scan(const tgp_column* cols, int32_t ncols, const tgp_pred* preds, int32_t npreds,
+                  uint64_t row_begin, uint64_t row_end, DevPlan* P, PlanInfo* info);
+// Full aggregation program (predicates, group parts, aggregates, strategy).
 int32_t plan_agg(const tgp_agg_query* q, DevPlan* P, PlanInfo* info);
 // Round-up multiplier for unsigned 32-bit division by d >= 2.
 void divisor_magic(uint32_t d, uint32_t* magic, int32_t* shift);
+
+// Workgroups per CU for the persistent kernels: as many as LDS and registers
+// allow, at most 4.
+template <typename K>
+inline int32_t launch_tiles(tgp_ctx* ctx, K kern, const DevPlan& P, uint32_t lds_bytes,
+                            hipStream_t s) {
+  if (P.ntiles == 0) return TGP_OK;
+  if (lds_bytes > 65536)
+    TGP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds_bytes));
+  int per_cu = 0;
+  TGP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, lds_bytes));
+  if (per_cu < 1) per_cu = 1;
+  if (per_cu > 4) per_cu = 4;
+  uint64_t grid = (uint64_t)persistent_grid(ctx, per_cu);
+  if (grid > P.ntiles) grid = P.ntiles;
+  kern<<<dim3((unsigned)grid), dim3(kBlock), lds_bytes, s>>>(P);
+  TGP_HIP(hipGetLastError());
+  return TGP_OK;
+}
 
 }  // namespace tgp

@@ -1,14 +1,9 @@
-// Host-side planner for the tile engine.
+// Host-side planner for the tile engine: compiles a query into per-slot ops.
+#include <vector>
+
 #include "plan.h"
 
 namespace tgp {
-
-int nst_bucket(int nst) {
-  static const int buckets[] = {2, 4, 8, 12, kMaxNst};
-  for (int b : buckets)
-    if (nst <= b) return b;
-  return 0;
-}
 
 // Unsigned division by d (2 <= d < 2^31) as q = (t + ((n - t) >> 1)) >> (l - 1),
 // t = mulhi(m, n), l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1
@@ -24,16 +19,17 @@ void divisor_magic(uint32_t d, uint32_t* magic, int32_t* shift) {
 
 int32_t plan_slots(const tgp_column* cols, int32_t ncols, uint64_t row_begin, uint64_t row_end,
                    DevPlan* P, PlanInfo* info) {
-  TGP_REQUIRE(ncols >= 0 && ncols <= TGP_MAX_COLS, "ncols %d outside [0,%d]", ncols,
+  TGP_REQUIRE(ncols >= 1 && ncols <= TGP_MAX_COLS, "ncols %d outside [1,%d]", ncols,
               TGP_MAX_COLS);
-  TGP_REQUIRE(ncols == 0 || cols != nullptr, "null column array");
+  TGP_REQUIRE(cols != nullptr, "null column array");
   TGP_REQUIRE(row_begin <= row_end, "row_begin %llu > row_end %llu",
               (unsigned long long)row_begin, (unsigned long long)row_end);
   P->row_begin = row_begin;
   P->row_end = row_end;
-  P->ntiles = (row_end - row_begin + kTileRows - 1) / kTileRows;
+  P->base0 = row_begin & ~63ull;
+  P->ntiles = row_end > row_begin ? (row_end - P->base0 + kTileRows - 1) / kTileRows : 0;
   P->nslots = ncols;
-  uint32_t chunks = 0;
+  uint32_t off = 0;
   for (int s = 0; s < ncols; ++s) {
     TGP_TRY(check_column(&cols[s], "column"));
     TGP_REQUIRE(cols[s].len >= row_end, "column %d has %llu rows < row_end %llu", s,
@@ -41,23 +37,26 @@ int32_t plan_slots(const tgp_column* cols, int32_t ncols, uint64_t row_begin, ui
     DevSlot& S = P->slots[s];
     S.data = static_cast<const uint8_t*>(cols[s].data);
     S.width = dtype_width(cols[s].dtype);
-    S.dtype = cols[s].dtype;
     S.limit_bytes = cols[s].len * (uint64_t)S.width;
-    S.chunk_pre = chunks;
-    S.lds_off = chunks * 16;
-    // one tile spans kTileRows*width bytes plus up to 15 bytes of phase
-    chunks += (uint32_t)(kTileRows * S.width / 16 + 1);
+    S.lds_off = off;
+    off += (uint32_t)(kTileRows * S.width);
   }
-  P->nchunks = chunks;
-  P->stage_bytes = chunks * 16;
-  int nst = (int)((chunks + kBlock - 1) / kBlock);
-  info->nst = nst_bucket(nst < 1 ? 1 : nst);
-  TGP_REQUIRE(info->nst > 0, "query stages %u chunks per tile, above the compiled limit", chunks);
-  info->lds_bytes = P->stage_bytes;
+  P->stage_bytes = off;
+  info->lds_bytes = 2 * off;
   return TGP_OK;
 }
 
-int32_t plan_preds(const tgp_pred* preds, int32_t npreds, int32_t ncols, DevPlan* P) {
+namespace {
+DevCol col_of(const tgp_column* cols, const DevPlan* P, int s) {
+  DevCol c{};
+  c.lds_off = P->slots[s].lds_off;
+  c.width = P->slots[s].width;
+  c.is_signed = dtype_signed(cols[s].dtype) ? 1 : 0;
+  return c;
+}
+
+int32_t plan_preds(const tgp_column* cols, const tgp_pred* preds, int32_t npreds, int32_t ncols,
+                   DevPlan* P) {
   TGP_REQUIRE(npreds >= 0 && npreds <= TGP_MAX_PREDS, "npreds %d outside [0,%d]", npreds,
               TGP_MAX_PREDS);
   TGP_REQUIRE(npreds == 0 || preds != nullptr, "null predicate array");
@@ -67,11 +66,22 @@ int32_t plan_preds(const tgp_pred* preds, int32_t npreds, int32_t ncols, DevPlan
                 p, preds[p].col);
     TGP_REQUIRE(preds[p].op >= TGP_EQ && preds[p].op <= TGP_GE, "predicate %d: bad op %d", p,
                 preds[p].op);
-    P->preds[p].slot = preds[p].col;
-    P->preds[p].op = preds[p].op;
-    P->preds[p].is_signed = preds[p].is_signed ? 1 : 0;
-    P->preds[p].lit = preds[p].lit;
+    DevPred& d = P->preds[p];
+    d.c = col_of(cols, P, preds[p].col);
+    d.cmp = preds[p].op;
+    d.is_signed = preds[p].is_signed ? 1 : 0;
+    d.lit = preds[p].lit;
   }
+  return TGP_OK;
+}
+}  // namespace
+
+int32_t plan_scan(const tgp_column* cols, int32_t ncols, const tgp_pred* preds, int32_t npreds,
+                  uint64_t row_begin, uint64_t row_end, DevPlan* P, PlanInfo* info) {
+  std::memset(P, 0, sizeof(*P));
+  TGP_TRY(plan_slots(cols, ncols, row_begin, row_end, P, info));
+  TGP_TRY(plan_preds(cols, preds, npreds, ncols, P));
+  P->ngroups = 1;
   return TGP_OK;
 }
 
@@ -79,7 +89,7 @@ int32_t plan_agg(const tgp_agg_query* q, DevPlan* P, PlanInfo* info) {
   TGP_REQUIRE(q != nullptr, "null query");
   std::memset(P, 0, sizeof(*P));
   TGP_TRY(plan_slots(q->cols, q->ncols, q->row_begin, q->row_end, P, info));
-  TGP_TRY(plan_preds(q->preds, q->npreds, q->ncols, P));
+  TGP_TRY(plan_preds(q->cols, q->preds, q->npreds, q->ncols, P));
   TGP_REQUIRE(q->nparts >= 0 && q->nparts <= TGP_MAX_GROUP_PARTS, "nparts %d out of range",
               q->nparts);
   TGP_REQUIRE(q->naggs >= 0 && q->naggs <= TGP_MAX_AGGS, "naggs %d out of range", q->naggs);
@@ -93,7 +103,7 @@ int32_t plan_agg(const tgp_agg_query* q, DevPlan* P, PlanInfo* info) {
     TGP_REQUIRE(g.card >= 1 && g.card <= (1ll << 31), "group part %d: card %lld out of range", k,
                 (long long)g.card);
     DevPart& D = P->parts[k];
-    D.slot = g.col;
+    D.c = col_of(q->cols, P, g.col);
     D.divisor = g.divisor;
     D.magic = 0;
     D.shift = 0;
@@ -116,18 +126,17 @@ int32_t plan_agg(const tgp_agg_query* q, DevPlan* P, PlanInfo* info) {
     TGP_REQUIRE(A.kind != TGP_AGG_COUNT_NONZERO || A.nfactors >= 1,
                 "agg %d: COUNT(col) needs its column in fcol[0]", a);
     D.kind = A.kind;
-    D.nf = A.nfactors;
+    D.nf = A.kind == TGP_AGG_SUM ? A.nfactors : 1;
     D.coef = A.coef;
-    for (int k = 0; k < A.nfactors; ++k) {
+    for (int k = 0; k < D.nf; ++k) {
       TGP_REQUIRE(A.fcol[k] >= 0 && A.fcol[k] < q->ncols, "agg %d factor %d: column %d", a, k,
                   A.fcol[k]);
-      D.fslot[k] = A.fcol[k];
-      D.fa[k] = A.fa[k];
-      D.fb[k] = A.fb[k];
+      D.f[k].c = col_of(q->cols, P, A.fcol[k]);
+      D.f[k].fa = A.fa[k];
+      D.f[k].fb = A.fb[k];
     }
   }
   P->sel_bitmap = q->d_sel_bitmap;
-  // Accumulation strategy by group count.
   const uint64_t table_bytes = ngroups * (1 + 2 * (uint64_t)q->naggs) * 8;
   if (ngroups == 1)
     info->strategy = kAggReg1;
@@ -137,8 +146,10 @@ int32_t plan_agg(const tgp_agg_query* q, DevPlan* P, PlanInfo* info) {
     info->strategy = kAggLds;
   else
     info->strategy = kAggGlobal;
-  info->lds_bytes = P->stage_bytes;
   if (info->strategy == kAggLds) info->lds_bytes += (uint32_t)table_bytes;
+  if ((info->strategy == kAggReg1 || info->strategy == kAggReg8) &&
+      info->lds_bytes < kRegReduceBytes)
+    info->lds_bytes = kRegReduceBytes;
   return TGP_OK;
 }
 

@@ -11,72 +11,45 @@
 
 namespace tgp {
 
-template <int NST>
+// One bit per row: the AND of the predicates.  Lane l of wave w holds row
+// row0 + 256 r + 64 w + l, so the wave's ballot for step r is the bitmap word
+// of those 64 rows.  When row_begin is not a multiple of 64 the word straddles
+// two output words and is OR-ed into both (the output is zeroed first).
 __global__ void __launch_bounds__(kBlock) k_scan(DevPlan P) {
   extern __shared__ __align__(16) uint8_t lds[];
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint32_t wave = threadIdx.x / kWave;
-  u32x4 reg[NST];
-  uint32_t addr[NST];
-  uint64_t t = blockIdx.x;
-  if (t < P.ntiles) {
-    const uint64_t r0 = P.row_begin + t * kTileRows;
-    stage_load<NST>(P, r0, min((uint64_t)kTileRows, P.row_end - r0), reg, addr);
-  }
+  const uint32_t d = (uint32_t)(P.row_begin & 63);
+  const uint64_t nwords = (P.row_end - P.row_begin + 63) / 64;
   unsigned long long cnt = 0;
-  for (; t < P.ntiles; t += gridDim.x) {
-    const uint64_t r0 = P.row_begin + t * kTileRows;
-    __syncthreads();
-    stage_store<NST>(lds, reg, addr);
-    __syncthreads();
-    const uint64_t tn = t + gridDim.x;
-    if (tn < P.ntiles) {
-      const uint64_t rn = P.row_begin + tn * kTileRows;
-      stage_load<NST>(P, rn, min((uint64_t)kTileRows, P.row_end - rn), reg, addr);
-    }
+  for_each_tile(P, lds, [&](const uint8_t* buf, uint64_t row0) {
+    const uint32_t sel = eval_select(P, buf, row0);
 #pragma unroll
-    for (int k = 0; k < kTileRows / kBlock; ++k) {
-      const uint32_t j = threadIdx.x + k * kBlock;
-      const uint64_t row = r0 + j;
-      const bool sel = row < P.row_end && row_selected(P, lds, r0, j);
-      const unsigned long long word = __ballot(sel);
-      const uint64_t wrow = r0 + k * kBlock + wave * kWave;  // first row of this word
-      if (lane == 0 && wrow < P.row_end) {
-        if (P.out_bitmap) P.out_bitmap[(wrow - P.row_begin) >> 6] = word;
+    for (int r = 0; r < kRows; ++r) {
+      const unsigned long long word = __ballot((sel >> r) & 1);
+      const uint64_t x = row0 + r * kBlock + wave * kWave;  // first row of the word
+      if (lane == 0 && x < P.row_end) {
         cnt += __popcll(word);
+        if (P.out_bitmap) {
+          const uint64_t k = (x - P.base0) >> 6;
+          if (d == 0) {
+            P.out_bitmap[k] = word;
+          } else {
+            if (k >= 1 && (word << (64 - d))) atomicOr(&P.out_bitmap[k - 1], word << (64 - d));
+            if (k < nwords && (word >> d)) atomicOr(&P.out_bitmap[k], word >> d);
+          }
+        }
       }
     }
-  }
+  });
   if (P.out_count && lane == 0 && cnt) atomicAdd(P.out_count, cnt);
 }
 
-template <int NST>
-static int32_t launch_scan(tgp_ctx* ctx, const DevPlan& P, const PlanInfo& info, hipStream_t s) {
-  auto kern = k_scan<NST>;
-  if (info.lds_bytes > 65536)
-    TGP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)info.lds_bytes));
-  int per_cu = 0;
-  TGP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, info.lds_bytes));
-  if (per_cu < 1) per_cu = 1;
-  uint64_t grid = (uint64_t)persistent_grid(ctx, per_cu);
-  if (grid > P.ntiles) grid = P.ntiles;
-  kern<<<dim3((unsigned)grid), dim3(kBlock), info.lds_bytes, s>>>(P);
-  TGP_HIP(hipGetLastError());
-  return TGP_OK;
-}
-
 int32_t run_scan(tgp_ctx* ctx, const DevPlan& P, const PlanInfo& info, hipStream_t s) {
-  if (P.ntiles == 0) return TGP_OK;
-  switch (info.nst) {
-    case 2: return launch_scan<2>(ctx, P, info, s);
-    case 4: return launch_scan<4>(ctx, P, info, s);
-    case 8: return launch_scan<8>(ctx, P, info, s);
-    case 12: return launch_scan<12>(ctx, P, info, s);
-    case kMaxNst: return launch_scan<kMaxNst>(ctx, P, info, s);
-  }
-  set_error("no scan kernel for %d staging rounds", info.nst);
-  return TGP_ERR_UNSUPPORTED;
+  if (P.out_bitmap && (P.row_begin & 63))
+    TGP_TRY(fill_u64(s, reinterpret_cast<uint64_t*>(P.out_bitmap),
+                     (P.row_end - P.row_begin + 63) / 64, 0));
+  return launch_tiles(ctx, k_scan, P, info.lds_bytes, s);
 }
 
 // ---- bitmap algebra -------------------------------------------------------------
@@ -257,10 +230,9 @@ int32_t tgp_scan_filter(tgp_ctx* ctx, const tgp_column* cols, int32_t ncols, con
   TGP_REQUIRE(ctx, "tgp_scan_filter: null ctx");
   TGP_TRY(check_injected(ctx));
   DevPlan P;
-  std::memset(&P, 0, sizeof(P));
   PlanInfo info;
-  TGP_TRY(plan_slots(cols, ncols, row_begin, row_end, &P, &info));
-  TGP_TRY(plan_preds(preds, npreds, ncols, &P));
+  if (ncols > 0) TGP_TRY(plan_scan(cols, ncols, preds, npreds, row_begin, row_end, &P, &info));
+  else TGP_REQUIRE(npreds == 0, "predicates need columns");
   P.out_bitmap = reinterpret_cast<unsigned long long*>(d_bitmap);
   P.out_count = reinterpret_cast<unsigned long long*>(d_count);
   if (ncols == 0) {

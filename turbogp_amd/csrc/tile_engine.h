@@ -1,72 +1,88 @@
 // LDS-staged tile engine shared by the scan and the fused aggregation kernels.
 //
-// A workgroup of 256 threads walks tiles of kTileRows (1024) rows of the
-// query's row range.  For each tile every referenced column ("slot") is
-// copied HBM -> VGPR -> LDS at its native width with 16-byte loads; the wave
-// reads 64 consecutive 16-byte chunks, so each load instruction touches one
-// contiguous KiB.  Rows are then evaluated from LDS, row j of the tile by
-// thread j % 256, so that the 64 lanes of a wave hold 64 consecutive rows and
-// a wave-wide ballot of a predicate is exactly one 64-bit word of the
-// selection bitmap (bit i <-> row i, LSB first; exec/bitmap.rs:36-45).
+// Staging.  A workgroup of 256 threads (4 waves) walks tiles of 1024 rows.  Each
+// referenced column ("slot") of a tile is width KiB at its native width, i.e.
+// `width` groups of 64 16-byte chunks.  A wave moves one such group with a
+// single gfx950 LDS-DMA instruction (global_load_lds_dwordx4): lane l reads
+// 16 bytes from HBM and the hardware writes them to LDS at base + 16 l, so the
+// data never occupies VGPRs.  Tiles are double-buffered: the DMA of tile t+1 is
+// in flight while tile t is evaluated from LDS.
 //
-// The loads of tile t+1 are issued into registers before tile t is evaluated
-// from LDS, so HBM traffic overlaps the evaluation.  Row ranges may start at any
-// row: each slot's tile segment starts at the 16-byte boundary at or below its
-// first byte ("phase" bytes early), which keeps every load aligned.
+// Tiles start at a multiple of 64 rows (the row range's first row rounded
+// down), so every chunk is 16-byte aligned, every 64-row wave slice of a tile is
+// one 64-bit word of the selection bitmap (bit i <-> row i, LSB first;
+// exec/bitmap.rs:36-45), and rows outside [row_begin, row_end) are masked.
+//
+// Evaluation.  Row j of a tile belongs to thread j % 256 (R = 4 rows per
+// thread).  The query is a small per-slot program prepared on the host: for
+// each slot, the predicates, group-key parts and aggregate factors that read
+// it.  The kernel walks slots and their ops in wave-uniform loops, so all plan
+// metadata is loaded once per tile into scalar registers, and the per-row work
+// is straight-line vector code on the 4 rows.
 #pragma once
 
 #include "tgp_internal.h"
 
 namespace tgp {
 
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kRows = kTileRows / kBlock;  // rows per thread per tile
+
+// Where a slot's tile segment sits in a stage buffer and how to widen it.
+struct DevCol {
+  uint32_t lds_off;
+  int32_t width;      // 1, 2, 4 or 8
+  int32_t is_signed;  // sign-extend when widening
+  int32_t _pad;
+};
 
 struct DevSlot {
   const uint8_t* data;
   uint64_t limit_bytes;  // bytes readable from data (column length * width)
-  int32_t width;         // 1, 2, 4 or 8
-  int32_t dtype;
-  uint32_t lds_off;      // byte offset of the slot's segment in the stage area
-  uint32_t chunk_pre;    // prefix count of 16-byte chunks before this slot
+  int32_t width;
+  uint32_t lds_off;
+  uint64_t _pad;
 };
 
 struct DevPred {
-  int32_t slot, op, is_signed, _pad;
+  DevCol c;
+  int32_t cmp;        // tgp_cmp
+  int32_t is_signed;  // compare as int64 (else uint64)
   int64_t lit;
 };
 
 struct DevPart {
-  int32_t slot;
-  int32_t divisor;   // 1 => no division
-  uint32_t magic;    // 32-bit round-up multiplier for divisor (see host planner)
-  int32_t shift;     // l - 1
-  int64_t base;
-  int64_t card;
-  int64_t stride;
+  DevCol c;
+  int32_t divisor;    // 1 => none
+  uint32_t magic;     // round-up multiplier for the divisor
+  int32_t shift;      // l - 1
+  int32_t _pad;
+  int64_t base, card, stride;
+};
+
+struct DevFactor {
+  DevCol c;
+  int64_t fa, fb;     // factor = fa + fb * value
 };
 
 struct DevAgg {
-  int32_t kind;      // tgp_agg_kind
+  int32_t kind;       // tgp_agg_kind
   int32_t nf;
-  int32_t fslot[TGP_MAX_FACTORS];
-  int32_t _pad;
   int64_t coef;
-  int64_t fa[TGP_MAX_FACTORS];
-  int64_t fb[TGP_MAX_FACTORS];
+  DevFactor f[TGP_MAX_FACTORS];
 };
 
-// Everything a kernel needs about one query; passed by value (kernarg) or,
-// for the batched executor, read from a device array.
+// Everything a kernel needs about one query (kernel argument, < 4 KiB).
 struct DevPlan {
   uint64_t row_begin, row_end;
+  uint64_t base0;        // row_begin rounded down to a multiple of 64
   uint64_t ntiles;
   DevSlot slots[TGP_MAX_COLS];
   DevPred preds[TGP_MAX_PREDS];
   DevPart parts[TGP_MAX_GROUP_PARTS];
   DevAgg aggs[TGP_MAX_AGGS];
   int32_t nslots, npreds, nparts, naggs;
-  uint32_t nchunks;          // total 16-byte chunks per tile (all slots)
-  uint32_t stage_bytes;      // LDS bytes of the stage area
+  uint32_t stage_bytes;  // LDS bytes of one stage buffer (sum of widths x 1 KiB)
+  uint32_t _pad;
   uint64_t ngroups;
   const uint64_t* sel_bitmap;
   // outputs
@@ -76,157 +92,186 @@ struct DevPlan {
   unsigned long long* out_bitmap;  // scan only
 };
 
-// ---- staging -------------------------------------------------------------------
+// ---- staging ----------------------------------------------------------------------
 
-// Chunk c (0 <= c < P.nchunks) belongs to the slot whose prefix is the largest
-// one <= c.  Slots are few (<= 8), so a linear search is cheapest.
-__device__ __forceinline__ int slot_of_chunk(const DevPlan& P, uint32_t c) {
-  int s = 0;
-#pragma unroll
-  for (int i = 1; i < TGP_MAX_COLS; ++i)
-    if (i < P.nslots && c >= P.slots[i].chunk_pre) s = i;
-  return s;
-}
-
-// Loads this thread's share of tile `tile_row0` into registers.  `ldsaddr[i]`
-// receives the LDS byte address the chunk goes to (0xffffffff => nothing).
-template <int NST>
-__device__ __forceinline__ void stage_load(const DevPlan& P, uint64_t tile_row0, uint64_t tile_rows,
-                                           u32x4 (&reg)[NST], uint32_t (&ldsaddr)[NST]) {
-#pragma unroll
-  for (int i = 0; i < NST; ++i) {
-    const uint32_t c = threadIdx.x + i * kBlock;
-    ldsaddr[i] = 0xffffffffu;
-    reg[i] = u32x4{0u, 0u, 0u, 0u};
-    if (c < P.nchunks) {
-      const int s = slot_of_chunk(P, c);
-      const DevSlot& S = P.slots[s];
-      const uint32_t cc = c - S.chunk_pre;
-      const uint64_t first = tile_row0 * (uint64_t)S.width;
-      const uint64_t g0 = first & ~(uint64_t)15;
-      const uint32_t phase = (uint32_t)(first - g0);
-      const uint64_t need = (uint64_t)phase + tile_rows * (uint64_t)S.width;  // bytes of this tile
-      const uint64_t off = (uint64_t)cc * 16;
-      if (off < need) {
-        const uint64_t gb = g0 + off;
-        ldsaddr[i] = S.lds_off + (uint32_t)off;
-        if (gb + 16 <= S.limit_bytes) {
-          reg[i] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(S.data + gb));
-        } else if (gb < S.limit_bytes) {
-          // Last chunk of the column: copy the valid bytes only.
-          uint32_t w[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-          for (int b = 0; b < 16; ++b)
-            if (gb + b < S.limit_bytes) w[b >> 2] |= (uint32_t)S.data[gb + b] << ((b & 3) * 8);
-          reg[i] = u32x4{w[0], w[1], w[2], w[3]};
-        }
+// Issues the LDS-DMA of tile `t` into `buf`.  Wave w moves chunk groups w, w+4, ...
+// A group that straddles the end of a column is copied byte by byte (only the
+// final tile of a column can have one).
+__device__ __forceinline__ void stage_issue(const DevPlan& P, uint64_t t, uint8_t* buf) {
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t row0 = P.base0 + t * kTileRows;
+  uint32_t g = 0;  // group index across slots, uniform
+  for (int s = 0; s < P.nslots; ++s) {
+    const DevSlot& S = P.slots[s];
+    const uint32_t ngroups = (uint32_t)S.width;  // width KiB per tile
+    const uint8_t* src = S.data + row0 * (uint64_t)S.width;
+    const uint64_t left = S.limit_bytes > row0 * (uint64_t)S.width
+                              ? S.limit_bytes - row0 * (uint64_t)S.width : 0;
+    for (uint32_t k = 0; k < ngroups; ++k, ++g) {
+      if ((g & 3) != wave) continue;
+      const uint64_t off = (uint64_t)k * 1024;
+      if (off >= left) continue;  // past the column: rows are masked, LDS left as is
+      uint8_t* dst = buf + S.lds_off + (uint32_t)off;
+      if (off + 1024 <= left) {
+        __builtin_amdgcn_global_load_lds(
+            reinterpret_cast<const void*>(src + off + lane * 16), dst, 16, 0, 0);
+      } else {
+        for (uint32_t b = lane; b < 1024; b += 64)
+          dst[b] = off + b < left ? src[off + b] : 0;
       }
     }
   }
 }
 
-template <int NST>
-__device__ __forceinline__ void stage_store(uint8_t* lds, const u32x4 (&reg)[NST],
-                                            const uint32_t (&ldsaddr)[NST]) {
-#pragma unroll
-  for (int i = 0; i < NST; ++i)
-    if (ldsaddr[i] != 0xffffffffu) *reinterpret_cast<u32x4*>(lds + ldsaddr[i]) = reg[i];
-}
+// Waits for this wave's outstanding DMA; a barrier must follow before other
+// waves read the data.
+__device__ __forceinline__ void stage_wait() { __builtin_amdgcn_s_waitcnt(0); }
 
-// Value of slot s at tile row j, widened to 64 bits by the slot's dtype.
-__device__ __forceinline__ int64_t lds_value(const DevPlan& P, const uint8_t* lds, int s,
-                                             uint64_t tile_row0, uint32_t j) {
-  const DevSlot& S = P.slots[s];
-  const uint32_t phase = (uint32_t)((tile_row0 * (uint64_t)S.width) & 15);
-  const uint8_t* p = lds + S.lds_off + phase + j * (uint32_t)S.width;
-  switch (S.dtype) {
-    case TGP_I8: return (int64_t)*reinterpret_cast<const int8_t*>(p);
-    case TGP_U8: return (int64_t)*p;
-    case TGP_I16: return (int64_t)*reinterpret_cast<const int16_t*>(p);
-    case TGP_U16: return (int64_t)*reinterpret_cast<const uint16_t*>(p);
-    case TGP_I32: return (int64_t)*reinterpret_cast<const int32_t*>(p);
-    case TGP_U32: return (int64_t)*reinterpret_cast<const uint32_t*>(p);
-    default: return *reinterpret_cast<const int64_t*>(p);
+// ---- row evaluation -----------------------------------------------------------------
+
+// Loads a slot's value for this thread's R rows (rows j = tid + 256 r).
+__device__ __forceinline__ void load_col(const DevCol& c, const uint8_t* buf, int64_t (&x)[kRows]) {
+  const uint8_t* base = buf + c.lds_off;
+  const uint32_t tid = threadIdx.x;
+  switch (c.width) {
+    case 1:
+#pragma unroll
+      for (int r = 0; r < kRows; ++r) {
+        const uint32_t v = base[tid + r * kBlock];
+        x[r] = c.is_signed ? (int64_t)(int8_t)v : (int64_t)v;
+      }
+      break;
+    case 2:
+#pragma unroll
+      for (int r = 0; r < kRows; ++r) {
+        const uint32_t v = reinterpret_cast<const uint16_t*>(base)[tid + r * kBlock];
+        x[r] = c.is_signed ? (int64_t)(int16_t)v : (int64_t)v;
+      }
+      break;
+    case 4:
+#pragma unroll
+      for (int r = 0; r < kRows; ++r) {
+        const uint32_t v = reinterpret_cast<const uint32_t*>(base)[tid + r * kBlock];
+        x[r] = c.is_signed ? (int64_t)(int32_t)v : (int64_t)v;
+      }
+      break;
+    default:
+#pragma unroll
+      for (int r = 0; r < kRows; ++r)
+        x[r] = reinterpret_cast<const int64_t*>(base)[tid + r * kBlock];
+      break;
   }
 }
 
 // bitmap.rs filter_* semantics on the 64-bit widened value.
-__device__ __forceinline__ bool eval_pred(const DevPred& p, int64_t v) {
-  if (p.is_signed) {
-    const int64_t l = p.lit;
-    switch (p.op) {
-      case TGP_EQ: return v == l;
-      case TGP_NE: return v != l;
-      case TGP_LT: return v < l;
-      case TGP_LE: return v <= l;
-      case TGP_GT: return v > l;
-      default: return v >= l;
+__device__ __forceinline__ bool cmp_value(int32_t op, bool sgn, int64_t v, int64_t lit) {
+  if (sgn) {
+    switch (op) {
+      case TGP_EQ: return v == lit;
+      case TGP_NE: return v != lit;
+      case TGP_LT: return v < lit;
+      case TGP_LE: return v <= lit;
+      case TGP_GT: return v > lit;
+      default: return v >= lit;
     }
-  } else {
-    const uint64_t u = (uint64_t)v, l = (uint64_t)p.lit;
-    switch (p.op) {
-      case TGP_EQ: return u == l;
-      case TGP_NE: return u != l;
-      case TGP_LT: return u < l;
-      case TGP_LE: return u <= l;
-      case TGP_GT: return u > l;
-      default: return u >= l;
-    }
+  }
+  const uint64_t u = (uint64_t)v, l = (uint64_t)lit;
+  switch (op) {
+    case TGP_EQ: return u == l;
+    case TGP_NE: return u != l;
+    case TGP_LT: return u < l;
+    case TGP_LE: return u <= l;
+    case TGP_GT: return u > l;
+    default: return u >= l;
   }
 }
 
-__device__ __forceinline__ bool row_selected(const DevPlan& P, const uint8_t* lds,
-                                             uint64_t tile_row0, uint32_t j) {
-  bool sel = true;
-  for (int p = 0; p < P.npreds; ++p) {
-    const DevPred& pr = P.preds[p];
-    sel = sel && eval_pred(pr, lds_value(P, lds, pr.slot, tile_row0, j));
+// Selection bits (bit r = row r of this thread) of a staged tile: row range,
+// the optional input bitmap, then the AND of the predicates.
+__device__ __forceinline__ uint32_t eval_select(const DevPlan& P, const uint8_t* buf,
+                                                uint64_t row0) {
+  uint32_t sel = 0;
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    const uint64_t row = row0 + threadIdx.x + r * kBlock;
+    if (row >= P.row_begin && row < P.row_end) sel |= 1u << r;
   }
   if (P.sel_bitmap) {
-    const uint64_t i = tile_row0 + j - P.row_begin;
-    sel = sel && ((P.sel_bitmap[i >> 6] >> (i & 63)) & 1);
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      const uint64_t row = row0 + threadIdx.x + r * kBlock;
+      if ((sel >> r) & 1) {
+        const uint64_t i = row - P.row_begin;
+        if (!((P.sel_bitmap[i >> 6] >> (i & 63)) & 1)) sel &= ~(1u << r);
+      }
+    }
+  }
+  for (int p = 0; p < P.npreds; ++p) {
+    const DevPred& pr = P.preds[p];
+    int64_t x[kRows];
+    load_col(pr.c, buf, x);
+    const bool sgn = pr.is_signed != 0;
+#pragma unroll
+    for (int r = 0; r < kRows; ++r)
+      if (!cmp_value(pr.cmp, sgn, x[r], pr.lit)) sel &= ~(1u << r);
   }
   return sel;
 }
 
-// Dense group id; returns -1 when a key part is outside [0, card).
-__device__ __forceinline__ int64_t group_id(const DevPlan& P, const uint8_t* lds,
-                                            uint64_t tile_row0, uint32_t j) {
-  int64_t gid = 0;
+// Dense group ids of the thread's rows (-1: a key part is out of range).
+__device__ __forceinline__ void eval_groups(const DevPlan& P, const uint8_t* buf,
+                                            int64_t (&gid)[kRows]) {
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) gid[r] = 0;
   for (int k = 0; k < P.nparts; ++k) {
     const DevPart& D = P.parts[k];
-    int64_t v = lds_value(P, lds, D.slot, tile_row0, j);
-    if (D.divisor > 1) {
-      if (v >= 0 && v <= 0xffffffffll) {
-        const uint32_t n = (uint32_t)v;
-        const uint32_t t = __umulhi(D.magic, n);
-        v = (int64_t)((t + ((n - t) >> 1)) >> D.shift);
-      } else {
-        v = v / D.divisor;  // reference semantics need v >= 0 here; kept total
+    int64_t x[kRows];
+    load_col(D.c, buf, x);
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      int64_t v = x[r];
+      if (D.divisor > 1) {
+        if (v >= 0 && v <= 0xffffffffll) {
+          const uint32_t n = (uint32_t)v;
+          const uint32_t q = __umulhi(D.magic, n);
+          v = (int64_t)((q + ((n - q) >> 1)) >> D.shift);
+        } else {
+          v = v / D.divisor;
+        }
       }
+      const int64_t part = v - D.base;
+      gid[r] = (part < 0 || part >= D.card || gid[r] < 0) ? -1 : gid[r] + part * D.stride;
     }
-    const int64_t part = v - D.base;
-    if (part < 0 || part >= D.card) return -1;
-    gid += part * D.stride;
   }
-  return gid;
 }
 
-// Per-row value of aggregate a, wrapping 64-bit arithmetic (expr.rs binop).
-__device__ __forceinline__ int64_t agg_value(const DevPlan& P, const DevAgg& A, const uint8_t* lds,
-                                             uint64_t tile_row0, uint32_t j) {
-  if (A.kind == TGP_AGG_COUNT_NONZERO)
-    return lds_value(P, lds, A.fslot[0], tile_row0, j) != 0 ? 1 : 0;
-  uint64_t v = (uint64_t)A.coef;
-  for (int k = 0; k < A.nf; ++k) {
-    const uint64_t x = (uint64_t)lds_value(P, lds, A.fslot[k], tile_row0, j);
-    uint64_t f;
-    if (A.fb[k] == 1) f = (uint64_t)A.fa[k] + x;
-    else if (A.fb[k] == -1) f = (uint64_t)A.fa[k] - x;
-    else f = (uint64_t)A.fa[k] + (uint64_t)A.fb[k] * x;
-    v = (k == 0 && A.coef == 1) ? f : v * f;
+// Per-row value of aggregate A for the thread's rows, in wrapping 64-bit
+// arithmetic (expr.rs binop wrapping_add/sub/mul).
+__device__ __forceinline__ void eval_agg(const DevAgg& A, const uint8_t* buf,
+                                         uint64_t (&v)[kRows]) {
+  if (A.kind == TGP_AGG_COUNT_NONZERO) {
+    int64_t x[kRows];
+    load_col(A.f[0].c, buf, x);
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) v[r] = x[r] != 0 ? 1 : 0;
+    return;
   }
-  return (int64_t)v;
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) v[r] = (uint64_t)A.coef;
+  for (int k = 0; k < A.nf; ++k) {
+    const DevFactor& F = A.f[k];
+    int64_t x[kRows];
+    load_col(F.c, buf, x);
+    const bool assign = k == 0 && A.coef == 1;
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      uint64_t f;
+      if (F.fb == 1) f = (uint64_t)F.fa + (uint64_t)x[r];
+      else if (F.fb == -1) f = (uint64_t)F.fa - (uint64_t)x[r];
+      else f = (uint64_t)F.fa + (uint64_t)F.fb * (uint64_t)x[r];
+      v[r] = assign ? f : v[r] * f;
+    }
+  }
 }
 
 // Exact 128-bit accumulation: value = hi * 2^64 + lo.
@@ -245,6 +290,33 @@ __device__ __forceinline__ void atomic_add128(unsigned long long* p, uint64_t lo
   const uint64_t old = atomicAdd(p, (unsigned long long)lo);
   const int64_t h = hi + (int64_t)((old + lo) < old);
   if (h != 0) atomicAdd(p + 1, (unsigned long long)h);
+}
+
+// Adds one signed 64-bit value to an int128 in memory (LDS or global).
+__device__ __forceinline__ void atomic_add128_i64(unsigned long long* p, int64_t v) {
+  const unsigned long long old = atomicAdd(p, (unsigned long long)v);
+  const long long h = (long long)((old + (uint64_t)v) < old) - (long long)(v < 0);
+  if (h) atomicAdd(p + 1, (unsigned long long)h);
+}
+
+// The persistent tile loop with double-buffered LDS-DMA staging.  `body` is
+// called with (stage buffer, first row of the tile) for each tile this
+// workgroup owns.
+template <typename Body>
+__device__ __forceinline__ void for_each_tile(const DevPlan& P, uint8_t* lds, Body&& body) {
+  uint64_t t = blockIdx.x;
+  if (t >= P.ntiles) return;
+  stage_issue(P, t, lds);
+  for (uint32_t i = 0; t < P.ntiles; t += gridDim.x, ++i) {
+    stage_wait();
+    __syncthreads();  // tile t is in LDS; every wave is done with the other buffer
+    const uint64_t tn = t + gridDim.x;
+    uint8_t* cur = lds + (i & 1) * P.stage_bytes;
+    uint8_t* nxt = lds + ((i + 1) & 1) * P.stage_bytes;
+    if (tn < P.ntiles) stage_issue(P, tn, nxt);
+    body(cur, P.base0 + t * kTileRows);
+  }
+  __syncthreads();
 }
 
 }  // namespace tgp
