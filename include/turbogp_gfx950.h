@@ -184,6 +184,16 @@ int32_t tgp_event_record(tgp_ctx* ctx, void* event, void* stream);
 /* Milliseconds between two recorded events (waits for `end`). */
 int32_t tgp_event_elapsed_ms(tgp_ctx* ctx, void* start, void* end, float* out_ms);
 
+/* Launch tunables (used by the autotuner; 0 = automatic). */
+enum tgp_tune_key {
+  TGP_TUNE_WG_PER_CU = 0, /* workgroups per CU of the persistent scan/aggregate kernels */
+  TGP_TUNE_GRID_MAX = 1,  /* upper bound on their grid size */
+  TGP_TUNE_SPECIALIZE = 2 /* 1 (default): use shape-specialised kernels when a query
+                             matches one; 0: always the generic engine */
+};
+int32_t tgp_tune_set(tgp_ctx* ctx, int32_t key, int64_t value);
+int32_t tgp_tune_get(tgp_ctx* ctx, int32_t key, int64_t* value);
+
 /* ---- HBM tier (memory/tier.rs MemoryTier::Hbm: resident columns) ------------ */
 int32_t tgp_hbm_alloc(tgp_ctx* ctx, uint64_t bytes, void** d_out);
 int32_t tgp_hbm_free(tgp_ctx* ctx, void* d_ptr);

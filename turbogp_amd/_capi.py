@@ -29,6 +29,7 @@ EQ, NE, LT, LE, GT, GE = range(6)
 BIT_AND, BIT_OR, BIT_ANDNOT = range(3)
 AGG_SUM, AGG_COUNT_NONZERO = 0, 1
 JOIN_GROUP_BUILD_PAYLOAD, JOIN_GROUP_PROBE_COL, JOIN_GROUP_NONE = 0, 1, 2
+TUNE_WG_PER_CU, TUNE_GRID_MAX, TUNE_SPECIALIZE = 0, 1, 2
 
 MAX_COLS, MAX_PREDS, MAX_AGGS, MAX_FACTORS, MAX_GROUP_PARTS = 8, 8, 6, 3, 2
 
@@ -154,6 +155,8 @@ SIGNATURES = {
                                          C.POINTER(U64_), P]),
     "tgp_hash_probe_agg": (I32_, [P, C.POINTER(JoinAggQuery), C.POINTER(GroupResult), P]),
     "tgp_event_create": (I32_, [P, C.POINTER(P)]),
+    "tgp_tune_set": (I32_, [P, I32_, I64_]),
+    "tgp_tune_get": (I32_, [P, I32_, C.POINTER(I64_)]),
     "tgp_event_destroy": (I32_, [P, P]),
     "tgp_event_record": (I32_, [P, P, P]),
     "tgp_event_elapsed_ms": (I32_, [P, P, P, C.POINTER(C.c_float)]),

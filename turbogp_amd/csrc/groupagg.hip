@@ -177,12 +177,21 @@ __global__ void __launch_bounds__(kBlock, 2) k_agg_table(DevPlan P) {
   if (oor && P.out_oor) atomicAdd(P.out_oor, (unsigned long long)oor);
 }
 
-int32_t run_agg(tgp_ctx* ctx, const DevPlan& P, const PlanInfo& info, hipStream_t s) {
+int32_t try_spec(tgp_ctx* ctx, const tgp_agg_query* q, const tgp_group_result* out,
+                 hipStream_t st, bool* launched);
+
+int32_t run_agg(tgp_ctx* ctx, const DevPlan& P, const PlanInfo& info, hipStream_t s,
+                const tgp_agg_query* q = nullptr, const tgp_group_result* out = nullptr) {
   // Results are accumulated with atomics: clear them first (a fill kernel, so the
   // sequence can be captured in a graph).
   TGP_TRY(fill_u64(s, reinterpret_cast<uint64_t*>(P.out_count), P.ngroups, 0));
   TGP_TRY(fill_u64(s, reinterpret_cast<uint64_t*>(P.out_sums), P.ngroups * P.naggs * 2, 0));
   if (P.out_oor) TGP_TRY(fill_u64(s, reinterpret_cast<uint64_t*>(P.out_oor), 1, 0));
+  if (q) {
+    bool launched = false;
+    TGP_TRY(try_spec(ctx, q, out, s, &launched));
+    if (launched) return TGP_OK;
+  }
   switch (info.strategy) {
     case kAggReg1: return launch_tiles(ctx, k_agg_reg<1, TGP_MAX_AGGS>, P, info.lds_bytes, s);
     case kAggReg8: return launch_tiles(ctx, k_agg_reg<8, 4>, P, info.lds_bytes, s);
@@ -216,7 +225,7 @@ int32_t tgp_group_agg(tgp_ctx* ctx, const tgp_agg_query* q, const tgp_group_resu
   P.out_count = reinterpret_cast<unsigned long long*>(out->d_count);
   P.out_sums = reinterpret_cast<unsigned long long*>(out->d_sums);
   P.out_oor = reinterpret_cast<unsigned long long*>(out->d_n_out_of_range);
-  return run_agg(ctx, P, info, pick_stream(ctx, stream));
+  return run_agg(ctx, P, info, pick_stream(ctx, stream), q, out);
 }
 
 }  // extern "C"

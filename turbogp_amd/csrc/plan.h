@@ -47,7 +47,11 @@ inline int32_t launch_tiles(tgp_ctx* ctx, K kern, const DevPlan& P, uint32_t lds
   TGP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, lds_bytes));
   if (per_cu < 1) per_cu = 1;
   if (per_cu > 4) per_cu = 4;
+  const int64_t want = ctx->tune_wg_per_cu.load();
+  if (want > 0 && want < per_cu) per_cu = (int)want;
   uint64_t grid = (uint64_t)persistent_grid(ctx, per_cu);
+  const int64_t gmax = ctx->tune_grid_max.load();
+  if (gmax > 0 && grid > (uint64_t)gmax) grid = (uint64_t)gmax;
   if (grid > P.ntiles) grid = P.ntiles;
   kern<<<dim3((unsigned)grid), dim3(kBlock), lds_bytes, s>>>(P);
   TGP_HIP(hipGetLastError());

@@ -286,6 +286,28 @@ int32_t tgp_fill_u64(tgp_ctx* ctx, uint64_t* d_dst, uint64_t nwords, uint64_t va
   return fill_u64(pick_stream(ctx, stream), d_dst, nwords, value);
 }
 
+int32_t tgp_tune_set(tgp_ctx* ctx, int32_t key, int64_t value) {
+  TGP_REQUIRE(ctx && value >= 0, "tgp_tune_set: bad argument");
+  switch (key) {
+    case TGP_TUNE_WG_PER_CU: ctx->tune_wg_per_cu.store(value); return TGP_OK;
+    case TGP_TUNE_GRID_MAX: ctx->tune_grid_max.store(value); return TGP_OK;
+    case TGP_TUNE_SPECIALIZE: ctx->tune_no_spec.store(value ? 0 : 1); return TGP_OK;
+  }
+  set_error("tgp_tune_set: unknown key %d", key);
+  return TGP_ERR_INVALID;
+}
+
+int32_t tgp_tune_get(tgp_ctx* ctx, int32_t key, int64_t* value) {
+  TGP_REQUIRE(ctx && value, "tgp_tune_get: null argument");
+  switch (key) {
+    case TGP_TUNE_WG_PER_CU: *value = ctx->tune_wg_per_cu.load(); return TGP_OK;
+    case TGP_TUNE_GRID_MAX: *value = ctx->tune_grid_max.load(); return TGP_OK;
+    case TGP_TUNE_SPECIALIZE: *value = ctx->tune_no_spec.load() ? 0 : 1; return TGP_OK;
+  }
+  set_error("tgp_tune_get: unknown key %d", key);
+  return TGP_ERR_INVALID;
+}
+
 int32_t tgp_event_create(tgp_ctx* ctx, void** out_event) {
   TGP_REQUIRE(ctx && out_event, "tgp_event_create: null argument");
   hipEvent_t e;

@@ -79,6 +79,9 @@ struct tgp_ctx {
   // Per-stream scratch (stream-ordered reuse).
   std::map<hipStream_t, tgp::Scratch> scratch;
   std::atomic<int32_t> inject_failures{0};
+  std::atomic<int64_t> tune_wg_per_cu{0};
+  std::atomic<int64_t> tune_grid_max{0};
+  std::atomic<int64_t> tune_no_spec{0};
 };
 
 namespace tgp {

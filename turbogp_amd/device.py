@@ -185,6 +185,9 @@ class Context:
     def sync(self, stream=None):
         A.check(A.lib().tgp_stream_sync(self.handle, stream), "tgp_stream_sync")
 
+    def tune(self, key, value):
+        A.check(A.lib().tgp_tune_set(self.handle, key, value), "tgp_tune_set")
+
     def hbm_usage(self):
         u, b, p = C.c_uint64(), C.c_uint64(), C.c_uint64()
         A.check(A.lib().tgp_hbm_usage(self.handle, C.byref(u), C.byref(b), C.byref(p)),
