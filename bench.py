@@ -56,6 +56,19 @@ def cpu_baseline(rows, reps=3):
                       f"oracle built -O3 -march=x86-64-v3 -fopenmp"}
 
 
+def measured_traffic(rows):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs (calibrated), for the 60M-row workload."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            t = json.load(f)["k_spec_q1_60M"]
+    except (OSError, KeyError, ValueError):
+        return None
+    if rows != 60_000_000:
+        return None
+    return t["fetch_bytes"] + t["write_bytes"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,8 +160,9 @@ def main():
                     "selected": int(res_b.count[0])},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None,
-                     "kernel": "k_agg_reg (query a), algorithmic 38 B/row x rows per launch"},
+                     "traffic": measured_traffic(n),
+                     "kernel": "k_spec<Q1 shape> (query a), algorithmic 38 B/row x rows per "
+                               "launch"},
     }
     if not args.no_cpu:
         log("cpu baseline ...")

@@ -251,6 +251,11 @@ int32_t tgp_bitmap_to_indices_sync(tgp_ctx* ctx, const uint64_t* d_bitmap, uint6
  * accum_grouped.rs:55-409) and the scalar aggregates (aggregate.rs:1055-1289). */
 int32_t tgp_group_agg(tgp_ctx* ctx, const tgp_agg_query* q, const tgp_group_result* out,
                       void* stream);
+/* Plan of a query without running it (host only): the accumulation strategy
+ * of the generic engine (0 registers/1 group, 1 registers/<=8 groups, 2 LDS
+ * table, 3 global table) and the index of the shape-specialised kernel that
+ * would run it (-1: none; the generic engine runs it). */
+int32_t tgp_group_agg_explain(const tgp_agg_query* q, int32_t* strategy, int32_t* spec_shape);
 /* Number of groups the query's key space spans (prod of card). */
 int32_t tgp_group_agg_ngroups(const tgp_agg_query* q, uint64_t* out_ngroups);
 

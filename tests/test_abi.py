@@ -63,3 +63,34 @@ def test_plan_validation_needs_no_device():
     n = ctypes.c_uint64()
     assert _capi.lib().tgp_group_agg_ngroups(ctypes.byref(q), ctypes.byref(n)) == 0
     assert n.value == 255
+
+
+def _explain(spec):
+    from turbogp_amd import make_agg_query
+    from turbogp_amd import workloads as W
+
+    class FakeCol:  # descriptors only: explain never touches device memory
+        def __init__(self, n, dt, addr):
+            self.n, self.dt, self.addr = n, dt, addr
+
+        def desc(self):
+            return _capi.Column(self.addr, self.n, self.dt, 0)
+    names, preds, parts, aggs = spec
+    cols = [FakeCol(1000, W.FACT_SCHEMA[nm][1], 0x100000 * (i + 1)) for i, nm in enumerate(names)]
+    q = make_agg_query(cols, preds, parts, aggs)
+    st, sh = ctypes.c_int32(), ctypes.c_int32()
+    _capi.check(_capi.lib().tgp_group_agg_explain(ctypes.byref(q), ctypes.byref(st),
+                                                  ctypes.byref(sh)), "explain")
+    return st.value, sh.value
+
+
+def test_benchmark_queries_take_the_specialised_kernels():
+    from turbogp_amd import workloads as W
+    assert [_explain(W.q1_spec(group=g))[1] for g in ("ab", "a", "b", "month", "")] == [0, 1, 2,
+                                                                                        3, 4]
+    assert _explain(W.q6_spec())[1] == 5
+    # a predicate whose value set is two intervals keeps the generic engine
+    names, preds, parts, aggs = W.q6_spec()
+    assert _explain((names, [(0, _capi.GE, 731, False)], parts, aggs))[1] == -1
+    # strategies of the generic engine
+    assert _explain(W.q1_spec(group="ab"))[0] == 1 and _explain(W.q1_spec(group="month"))[0] == 2

@@ -131,6 +131,7 @@ SIGNATURES = {
     "tgp_bitmap_to_indices_sync": (I32_, [P, P, U64_, U64_, I32_, P, U64_, C.POINTER(U64_), P]),
     "tgp_group_agg": (I32_, [P, C.POINTER(AggQuery), C.POINTER(GroupResult), P]),
     "tgp_group_agg_ngroups": (I32_, [C.POINTER(AggQuery), C.POINTER(U64_)]),
+    "tgp_group_agg_explain": (I32_, [C.POINTER(AggQuery), C.POINTER(I32_), C.POINTER(I32_)]),
     "tgp_synth_uniform": (I32_, [P, C.POINTER(Column), U64_, U64_, I64_, I64_, U64_, U64_, P]),
     "tgp_synth_lookup_cdf": (I32_, [P, C.POINTER(Column), U64_, U64_, P, U64_, P, U64_, U64_,
                                     P]),

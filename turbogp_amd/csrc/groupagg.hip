@@ -179,6 +179,7 @@ __global__ void __launch_bounds__(kBlock, 2) k_agg_table(DevPlan P) {
 
 int32_t try_spec(tgp_ctx* ctx, const tgp_agg_query* q, const tgp_group_result* out,
                  hipStream_t st, bool* launched);
+int32_t spec_shape_index(const tgp_agg_query* q);
 
 int32_t run_agg(tgp_ctx* ctx, const DevPlan& P, const PlanInfo& info, hipStream_t s,
                 const tgp_agg_query* q = nullptr, const tgp_group_result* out = nullptr) {
@@ -205,6 +206,16 @@ int32_t run_agg(tgp_ctx* ctx, const DevPlan& P, const PlanInfo& info, hipStream_
 using namespace tgp;
 
 extern "C" {
+
+int32_t tgp_group_agg_explain(const tgp_agg_query* q, int32_t* strategy, int32_t* spec_shape) {
+  TGP_REQUIRE(q && strategy && spec_shape, "tgp_group_agg_explain: null argument");
+  DevPlan P;
+  PlanInfo info;
+  TGP_TRY(plan_agg(q, &P, &info));
+  *strategy = info.strategy;
+  *spec_shape = spec_shape_index(q);
+  return TGP_OK;
+}
 
 int32_t tgp_group_agg_ngroups(const tgp_agg_query* q, uint64_t* out_ngroups) {
   TGP_REQUIRE(q && out_ngroups, "tgp_group_agg_ngroups: null argument");

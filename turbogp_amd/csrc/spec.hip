@@ -481,9 +481,11 @@ int32_t launch_shape(tgp_ctx* ctx, const Shape& s, const SpecArgs& A, hipStream_
 
 typedef __int128 i128;
 
-struct Interval {
-  int kind;  // 0 empty, 1 full, 2 range [lo, hi], 3 all but the point lo
-  i128 lo, hi;
+// A set of values as at most two closed intervals, or "all but one point".
+struct ValueSet {
+  int n;           // number of intervals (0 = empty)
+  i128 lo[2], hi[2];
+  bool not_point;  // the set is everything except lo[0] (n unused)
 };
 
 static void dtype_range(int dt, i128* mn, i128* mx) {
@@ -501,15 +503,13 @@ static void dtype_range(int dt, i128* mn, i128* mx) {
 
 // {x in the dtype's value range : x op lit}, where the value is compared as
 // int64 (is_signed) or as its uint64 bit pattern.  Values of signed columns are
-// sign-extended; u64 values are their own bit pattern.  Returns false when the
-// set is not one interval or the complement of one point.
-static bool pred_interval(int dt, int op, int is_signed, int64_t lit, Interval* out) {
+// sign-extended; u64 values are their own bit pattern.
+static ValueSet pred_set(int dt, int op, int is_signed, int64_t lit) {
   i128 mn, mx;
   dtype_range(dt, &mn, &mx);
-  const i128 two64 = (i128)1 << 64;
-  // compare-domain interval of the literal condition (EQ for NE)
-  i128 L = is_signed ? (i128)lit : (i128)(uint64_t)lit;
-  i128 clo = is_signed ? -((i128)1 << 63) : 0, chi = is_signed ? ((i128)1 << 63) - 1 : two64 - 1;
+  const i128 two64 = (i128)1 << 64, two63 = (i128)1 << 63;
+  const i128 L = is_signed ? (i128)lit : (i128)(uint64_t)lit;
+  const i128 clo = is_signed ? -two63 : 0, chi = is_signed ? two63 - 1 : two64 - 1;
   i128 a = clo, b = chi;
   switch (op) {
     case TGP_EQ: case TGP_NE: a = L; b = L; break;
@@ -518,46 +518,49 @@ static bool pred_interval(int dt, int op, int is_signed, int64_t lit, Interval* 
     case TGP_GT: a = L + 1; break;
     default: a = L; break;
   }
-  // preimage in the value domain: value v maps to v (signed compare, or unsigned
-  // compare of a non-negative value) or v + 2^64 (unsigned compare of v < 0);
-  // u64 columns compared signed map v >= 2^63 to v - 2^64.
-  i128 p1lo = 0, p1hi = -1, p2lo = 0, p2hi = -1;  // two candidate pieces
+  // preimage of the compare-domain interval [a, b] in the value domain
+  i128 lo[2] = {0, 0}, hi[2] = {-1, -1};
   if (dt == TGP_U64) {
-    if (!is_signed) { p1lo = a; p1hi = b; }
-    else {
-      // v in [0, 2^63): compare v; v in [2^63, 2^64): compare v - 2^64
-      p1lo = a > 0 ? a : 0; p1hi = b < ((i128)1 << 63) - 1 ? b : ((i128)1 << 63) - 1;
-      p2lo = (a + two64) > ((i128)1 << 63) ? (a + two64) : ((i128)1 << 63);
-      p2hi = (b + two64) < two64 - 1 ? (b + two64) : two64 - 1;
+    if (!is_signed) { lo[0] = a; hi[0] = b; }
+    else {  // v < 2^63 compares as v, v >= 2^63 as v - 2^64
+      lo[0] = a > 0 ? a : 0; hi[0] = b < two63 - 1 ? b : two63 - 1;
+      lo[1] = (a + two64) > two63 ? (a + two64) : two63;
+      hi[1] = (b + two64) < two64 - 1 ? (b + two64) : two64 - 1;
     }
   } else if (is_signed) {
-    p1lo = a; p1hi = b;
-  } else {
-    p1lo = a > 0 ? a : 0; p1hi = b;            // non-negative values
-    p2lo = a - two64; p2hi = b - two64;        // negative values
-    if (p2hi > -1) p2hi = -1;
+    lo[0] = a; hi[0] = b;
+  } else {  // v >= 0 compares as v, v < 0 as v + 2^64
+    lo[0] = a > 0 ? a : 0; hi[0] = b;
+    lo[1] = a - two64; hi[1] = b - two64 < -1 ? b - two64 : -1;
   }
-  auto clip = [&](i128& lo, i128& hi) {
-    if (lo < mn) lo = mn;
-    if (hi > mx) hi = mx;
-  };
-  clip(p1lo, p1hi);
-  clip(p2lo, p2hi);
-  const bool e1 = p1lo > p1hi, e2 = p2lo > p2hi;
-  if (!e1 && !e2) return false;  // two pieces
-  Interval r{};
-  if (e1 && e2) r.kind = 0;
-  else {
-    r.kind = 2;
-    r.lo = e1 ? p2lo : p1lo;
-    r.hi = e1 ? p2hi : p1hi;
+  ValueSet r{};
+  for (int k = 0; k < 2; ++k) {
+    i128 l = lo[k] < mn ? mn : lo[k], h = hi[k] > mx ? mx : hi[k];
+    if (l <= h) { r.lo[r.n] = l; r.hi[r.n] = h; ++r.n; }
   }
   if (op == TGP_NE) {
-    if (r.kind == 0) r.kind = 1;           // nothing equals: NE always true
-    else r.kind = 3;                       // single point complement
-  } else if (r.kind == 2 && r.lo == mn && r.hi == mx) {
-    r.kind = 1;
+    ValueSet ne{};
+    if (r.n == 0) { ne.n = 1; ne.lo[0] = mn; ne.hi[0] = mx; }   // nothing equal: all pass
+    else { ne.not_point = true; ne.lo[0] = r.lo[0]; }          // EQ set is one point
+    return ne;
   }
+  return r;
+}
+
+// Intersection of two interval sets of at most two intervals each; false when
+// the result needs more than two intervals.
+static bool intersect(const ValueSet& x, const ValueSet& y, ValueSet* out) {
+  ValueSet r{};
+  for (int i = 0; i < x.n; ++i)
+    for (int j = 0; j < y.n; ++j) {
+      const i128 l = x.lo[i] > y.lo[j] ? x.lo[i] : y.lo[j];
+      const i128 h = x.hi[i] < y.hi[j] ? x.hi[i] : y.hi[j];
+      if (l > h) continue;
+      if (r.n == 2) return false;
+      r.lo[r.n] = l;
+      r.hi[r.n] = h;
+      ++r.n;
+    }
   *out = r;
   return true;
 }
@@ -585,29 +588,30 @@ static bool extract_shape(const tgp_agg_query* q, Shape* s, SpecArgs* A) {
   for (int c = 0; c < q->ncols; ++c) {
     i128 mn, mx;
     dtype_range(q->cols[c].dtype, &mn, &mx);
-    i128 lo = mn, hi = mx;
-    bool any = false;
+    ValueSet acc{};
+    acc.n = 1;
+    acc.lo[0] = mn;
+    acc.hi[0] = mx;
     for (int p = 0; p < q->npreds; ++p) {
       if (q->preds[p].col != c) continue;
-      Interval iv;
-      if (!pred_interval(q->cols[c].dtype, q->preds[p].op, q->preds[p].is_signed, q->preds[p].lit,
-                         &iv))
-        return false;
-      if (iv.kind == 0) empty = true;
-      if (iv.kind == 2) {
-        lo = iv.lo > lo ? iv.lo : lo;
-        hi = iv.hi < hi ? iv.hi : hi;
-        any = true;
+      const ValueSet v =
+          pred_set(q->cols[c].dtype, q->preds[p].op, q->preds[p].is_signed, q->preds[p].lit);
+      if (v.not_point) {
+        excl.push_back({c, v.lo[0]});
+        continue;
       }
-      if (iv.kind == 3) excl.push_back({c, iv.lo});
+      if (!intersect(acc, v, &acc)) return false;
     }
-    if (any && !(lo == mn && hi == mx)) {
+    if (acc.n == 0) {
+      empty = true;
+    } else if (acc.n == 2) {
+      return false;  // not one range
+    } else if (!(acc.lo[0] == mn && acc.hi[0] == mx)) {
       if (s->npreds >= kMaxSpecPreds) return false;
       const int p = s->npreds++;
       s->pslot[p] = c;
-      if (lo > hi) { lo = 1; hi = 0; }  // never true
-      A->plo[p] = (int64_t)(uint64_t)lo;  // u64 columns: bit pattern
-      A->phi[p] = (int64_t)(uint64_t)hi;
+      A->plo[p] = (int64_t)(uint64_t)acc.lo[0];  // u64 columns: bit pattern
+      A->phi[p] = (int64_t)(uint64_t)acc.hi[0];
     }
   }
   for (auto& e : excl) {
@@ -618,9 +622,9 @@ static bool extract_shape(const tgp_agg_query* q, Shape* s, SpecArgs* A) {
     A->plo[p] = A->phi[p] = (int64_t)(uint64_t)e.second;
   }
   if (empty) {  // a predicate selects nothing: one never-true range on column 0
+    std::memset(s->pslot, 0, sizeof(s->pslot));
+    std::memset(s->pneg, 0, sizeof(s->pneg));
     s->npreds = 1;
-    s->pslot[0] = 0;
-    s->pneg[0] = 0;
     A->plo[0] = 1;
     A->phi[0] = 0;
   }
@@ -675,6 +679,16 @@ static bool extract_shape(const tgp_agg_query* q, Shape* s, SpecArgs* A) {
   A->nslices = q->row_end > q->row_begin
                    ? (q->row_end - A->base0 + kSliceRows - 1) / kSliceRows : 0;
   return true;
+}
+
+// Index of the compiled shape that runs q, or -1.
+int32_t spec_shape_index(const tgp_agg_query* q) {
+  Shape s;
+  SpecArgs A;
+  if (!extract_shape(q, &s, &A)) return -1;
+  for (int i = 0; i < kNumShapes; ++i)
+    if (shape_eq(s, kShapes[i])) return i;
+  return -1;
 }
 
 // Launches the specialised kernel for q when one is compiled; *launched tells.
