@@ -259,6 +259,39 @@ int32_t tgp_group_agg_explain(const tgp_agg_query* q, int32_t* strategy, int32_t
 /* Number of groups the query's key space spans (prod of card). */
 int32_t tgp_group_agg_ngroups(const tgp_agg_query* q, uint64_t* out_ngroups);
 
+/* ---- batched executor (many queries, one persistent launch) --------------- */
+/* A batch holds up to max_queries tgp_agg_query descriptors.  PERSISTENT mode
+ * runs every query whose shape has a specialised kernel in ONE persistent
+ * kernel launch (captured once in a hipGraph and replayed); the others run one
+ * launch each after it.  PER_QUERY mode runs tgp_group_agg for each query (the
+ * one-launch-per-query baseline).  Results of query i live in the batch's
+ * device arena at the offsets tgp_batch_result_layout gives, in the layout of
+ * tgp_group_result (count words, then (lo, hi) sum words, then the
+ * out-of-range count).  Not thread safe per batch. */
+typedef struct tgp_batch tgp_batch;
+enum tgp_batch_mode { TGP_BATCH_PERSISTENT = 0, TGP_BATCH_PER_QUERY = 1 };
+int32_t tgp_batch_create(tgp_ctx* ctx, uint32_t max_queries, tgp_batch** out);
+int32_t tgp_batch_destroy(tgp_batch* b);
+/* Plans and uploads the queries (synchronous); invalidates the captured graph. */
+int32_t tgp_batch_set_queries(tgp_batch* b, const tgp_agg_query* qs, uint32_t n, void* stream);
+/* wg_per_cu 0 = as many as fit; use_graph 0 = plain launches. */
+int32_t tgp_batch_set_tuning(tgp_batch* b, uint32_t wg_per_cu, int32_t use_graph);
+/* Times the persistent launch for several workgroups-per-CU settings on the
+ * current queries and keeps the fastest (the batch's autotuner). */
+int32_t tgp_batch_autotune(tgp_batch* b, void* stream, uint32_t* best_wg_per_cu, float* best_ms);
+int32_t tgp_batch_run(tgp_batch* b, int32_t mode, void* stream);
+/* tgp_batch_run, synchronous, with device timing: *h_total_ms for the whole
+ * batch and, when h_query_ms != NULL, each query's completion time measured
+ * from the start of the batch (PERSISTENT: all equal the total). */
+int32_t tgp_batch_run_timed(tgp_batch* b, int32_t mode, void* stream, float* h_query_ms,
+                            float* h_total_ms);
+int32_t tgp_batch_info(tgp_batch* b, uint64_t* result_words, uint64_t* total_slices,
+                       uint32_t* n_batched, uint32_t* n_fallback);
+int32_t tgp_batch_result_layout(tgp_batch* b, uint32_t i, uint64_t* count_off, uint64_t* sums_off,
+                                uint64_t* oor_off, uint64_t* ngroups);
+/* Copies the first nwords words of the result arena to the host (synchronous). */
+int32_t tgp_batch_results_sync(tgp_batch* b, uint64_t* h_words, uint64_t nwords, void* stream);
+
 /* ---- hash join --------------------------------------------------------------- */
 typedef struct tgp_hash_table tgp_hash_table;
 

@@ -1,0 +1,119 @@
+"""Batched executor: every query's result from the persistent batch launch
+(graph and plain), from one launch per query, from a single tgp_group_agg call
+and from the oracle must agree bit for bit."""
+import numpy as np
+import pytest
+
+from oracle import binding as B
+from turbogp_amd import DeviceColumn, make_agg_query, _capi as A
+from turbogp_amd import workloads as W
+from turbogp_amd.batch import Batch
+
+pytestmark = pytest.mark.gpu
+N = 3_000_017
+
+
+@pytest.fixture(scope="module")
+def table(ctx):
+    host = W.host_fact_columns(N, B.synth_uniform)
+    dev = {k: DeviceColumn.from_numpy(ctx, v) for k, v in host.items()}
+    yield host, dev
+    for c in dev.values():
+        c.free()
+
+
+def params(n):
+    p = W.batch_query_params(n, N, lmin=1, lmax=N)
+    # edges: empty, single rows, whole table, the last rows, an unaligned short range
+    p += [(5, 5, 100, "ab"), (0, 1, 2525, "a"), (N - 1, N, 2525, "b"), (0, N, 1200, "month"),
+          (N - 300, N, 2000, "ab"), (77, 333, 1500, "month")]
+    return p
+
+
+def same(r1, r2):
+    return np.array_equal(r1.count, r2.count) and np.array_equal(r1.sums_words, r2.sums_words) \
+        and r1.n_out_of_range == r2.n_out_of_range
+
+
+def test_batch_matches_single_queries_and_oracle(ctx, table):
+    host, dev = table
+    p = params(300)
+    qs = W.batch_queries(dev, p)
+    # one query with a shape that has no batch kernel: it runs alone (fallback)
+    names6, preds6, parts6, aggs6 = W.q6_spec()
+    qs.append(make_agg_query([dev[c] for c in names6], preds6 + [(2, A.NE, 7, True)], parts6,
+                             aggs6 + [("count_nonzero", 1)], 10, 2_000_000))
+    b = Batch(ctx, len(qs))
+    try:
+        b.set_queries(qs)
+        info = b.info()
+        assert info["batched"] == len(qs) - 1 and info["fallback"] == 1
+        b.run(A.BATCH_PERSISTENT)
+        persistent = b.results()
+        b.run(A.BATCH_PERSISTENT)          # graph replay
+        replay = b.results()
+        single = [ctx.group_agg(q) for q in qs[:40]]   # eager work between replays
+        b.run(A.BATCH_PERSISTENT)
+        replay2 = b.results()
+        b.run(A.BATCH_PER_QUERY)
+        per_query = b.results()
+        b.set_tuning(0, use_graph=False)
+        b.run(A.BATCH_PERSISTENT)
+        plain = b.results()
+    finally:
+        b.close()
+    for i in range(len(qs)):
+        assert same(persistent[i], per_query[i]), i
+        assert same(persistent[i], replay[i]) and same(persistent[i], replay2[i]), i
+        assert same(persistent[i], plain[i]), i
+    for i in range(40):
+        assert same(persistent[i], single[i]), i
+    # oracle on a subset (all edge queries included)
+    for i in list(range(0, 300, 23)) + list(range(300, len(p))):
+        bb, e, c, g = p[i]
+        names, preds, parts, aggs = W.q1_spec(date_cutoff=c, group=g)
+        cnt, sums, oor = B.group_agg(B.make_query([host[x] for x in names], preds, parts, aggs,
+                                                  bb, e))
+        assert [int(x) for x in persistent[i].count] == [int(x) for x in cnt], i
+        assert persistent[i].sums() == sums, i
+        assert persistent[i].n_out_of_range == oor
+
+
+def test_batch_flush_every_2_20_rows_with_a_tiny_grid(ctx, table):
+    host, dev = table
+    p = [(0, N, 2400, "ab"), (1, N - 1, 1000, "month"), (100, 2_500_000, 2525, "a")]
+    qs = W.batch_queries(dev, p)
+    b = Batch(ctx, len(qs))
+    try:
+        b.set_queries(qs)
+        ctx.tune(A.TUNE_GRID_MAX, 2)   # 8 waves: each wave adds millions of rows
+        b.set_tuning(0, use_graph=False)
+        b.run(A.BATCH_PERSISTENT)
+        got = b.results()
+    finally:
+        ctx.tune(A.TUNE_GRID_MAX, 0)
+        b.close()
+    for q, r in zip(qs, got):
+        assert same(r, ctx.group_agg(q))
+
+
+def test_batch_autotune_keeps_results(ctx, table):
+    _, dev = table
+    p = params(100)
+    qs = W.batch_queries(dev, p)
+    b = Batch(ctx, len(qs))
+    try:
+        b.set_queries(qs)
+        b.run()
+        before = b.results()
+        wg, ms = b.autotune()
+        assert wg in (1, 2, 3, 4, 6, 8) and ms > 0
+        b.run()
+        after = b.results()
+        tot, per = b.run_timed(A.BATCH_PERSISTENT)
+        assert tot > 0 and np.all(per == np.float32(tot))
+        tot2, per2 = b.run_timed(A.BATCH_PER_QUERY)
+        assert np.all(np.diff(per2) >= 0) and abs(per2[-1] - tot2) < 1e-3 * max(1, tot2)
+    finally:
+        b.close()
+    assert all(same(x, y) for x, y in zip(before, after))

@@ -30,6 +30,7 @@ BIT_AND, BIT_OR, BIT_ANDNOT = range(3)
 AGG_SUM, AGG_COUNT_NONZERO = 0, 1
 JOIN_GROUP_BUILD_PAYLOAD, JOIN_GROUP_PROBE_COL, JOIN_GROUP_NONE = 0, 1, 2
 TUNE_WG_PER_CU, TUNE_GRID_MAX, TUNE_SPECIALIZE = 0, 1, 2
+BATCH_PERSISTENT, BATCH_PER_QUERY = 0, 1
 
 MAX_COLS, MAX_PREDS, MAX_AGGS, MAX_FACTORS, MAX_GROUP_PARTS = 8, 8, 6, 3, 2
 
@@ -145,6 +146,18 @@ SIGNATURES = {
     "tgp_kernel_info": (I32_, [P, C.POINTER(I32_), C.POINTER(I32_), C.POINTER(I32_),
                                C.POINTER(C.c_char_p)]),
     "tgp_kernel_execute": (I32_, [P, P, P, C.POINTER(KernelParams), C.POINTER(KernelResult)]),
+    "tgp_batch_create": (I32_, [P, C.c_uint32, C.POINTER(P)]),
+    "tgp_batch_destroy": (I32_, [P]),
+    "tgp_batch_set_queries": (I32_, [P, C.POINTER(AggQuery), C.c_uint32, P]),
+    "tgp_batch_set_tuning": (I32_, [P, C.c_uint32, I32_]),
+    "tgp_batch_autotune": (I32_, [P, P, C.POINTER(C.c_uint32), C.POINTER(C.c_float)]),
+    "tgp_batch_run": (I32_, [P, I32_, P]),
+    "tgp_batch_run_timed": (I32_, [P, I32_, P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "tgp_batch_info": (I32_, [P, C.POINTER(U64_), C.POINTER(U64_), C.POINTER(C.c_uint32),
+                              C.POINTER(C.c_uint32)]),
+    "tgp_batch_result_layout": (I32_, [P, C.c_uint32, C.POINTER(U64_), C.POINTER(U64_),
+                                       C.POINTER(U64_), C.POINTER(U64_)]),
+    "tgp_batch_results_sync": (I32_, [P, C.POINTER(U64_), U64_, P]),
     "tgp_hash_build": (I32_, [P, C.POINTER(Column), C.POINTER(Column), C.POINTER(P), P]),
     "tgp_hash_table_destroy": (I32_, [P, P]),
     "tgp_hash_table_footprint": (I32_, [P, C.POINTER(U64_)]),

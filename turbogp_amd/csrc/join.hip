@@ -193,12 +193,26 @@ struct ProbeAggDev {
   unsigned long long* out_oor;
 };
 
+// Exact sums without returning atomics: a value v is added as its unsigned low
+// 32 bits into one 64-bit word and its signed high 32 bits into another.  Each
+// word stays exact for fewer than 2^32 additions, and
+//   sum(v) = sum(lo32) + 2^32 * sum(hi32)
+// is rebuilt as int128 when the partial table is flushed.
 __device__ __forceinline__ void acc_add(unsigned long long* cnt, unsigned long long* sum,
                                         uint64_t g, int64_t v) {
   atomicAdd(&cnt[g], 1ull);
-  const unsigned long long old = atomicAdd(&sum[2 * g], (unsigned long long)v);
-  const long long h = (long long)((old + (uint64_t)v) < old) - (long long)(v < 0);
+  atomicAdd(&sum[2 * g], (unsigned long long)(uint32_t)v);
+  const int64_t h = v >> 32;
   if (h) atomicAdd(&sum[2 * g + 1], (unsigned long long)h);
+}
+
+// (sum of lo32, sum of hi32) -> int128 words (lo, hi).
+__device__ __forceinline__ void split_to_i128(uint64_t slo, int64_t shi, uint64_t* lo,
+                                              int64_t* hi) {
+  const uint64_t add = (uint64_t)shi << 32;
+  const uint64_t l = slo + add;
+  *hi = (shi >> 32) + (int64_t)(l < slo);
+  *lo = l;
 }
 
 template <bool USE_LDS>
@@ -212,7 +226,7 @@ __global__ void __launch_bounds__(256) k_probe_agg(JoinTableDev T, ProbeAggDev Q
     for (uint64_t i = threadIdx.x; i < 3 * Q.ngroups; i += blockDim.x) ltab[i] = 0;
     __syncthreads();
   } else {
-    cnt = Q.out_count;
+    cnt = Q.out_count;  // the result buffers hold the split sums until k_split_finalize
     sum = Q.out_sums;
   }
   uint32_t oor = 0;
@@ -254,8 +268,9 @@ __global__ void __launch_bounds__(256) k_probe_agg(JoinTableDev T, ProbeAggDev Q
     for (uint64_t gi = threadIdx.x; gi < Q.ngroups; gi += blockDim.x) {
       if (!cnt[gi]) continue;
       atomicAdd(&Q.out_count[gi], cnt[gi]);
-      const uint64_t lo = sum[2 * gi];
-      const int64_t hi = (int64_t)sum[2 * gi + 1];
+      uint64_t lo;
+      int64_t hi;
+      split_to_i128(sum[2 * gi], (int64_t)sum[2 * gi + 1], &lo, &hi);
       if (lo || hi) {
         const unsigned long long old = atomicAdd(&Q.out_sums[2 * gi], (unsigned long long)lo);
         const int64_t h = hi + (int64_t)((old + lo) < old);
@@ -264,6 +279,18 @@ __global__ void __launch_bounds__(256) k_probe_agg(JoinTableDev T, ProbeAggDev Q
     }
   }
   if (oor && Q.out_oor) atomicAdd(Q.out_oor, (unsigned long long)oor);
+}
+
+// In place: (sum of lo32, sum of hi32) pairs -> int128 (lo, hi) words.
+__global__ void __launch_bounds__(256) k_split_finalize(unsigned long long* sums, uint64_t n) {
+  for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < n;
+       g += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t lo;
+    int64_t hi;
+    split_to_i128(sums[2 * g], (int64_t)sums[2 * g + 1], &lo, &hi);
+    sums[2 * g] = lo;
+    sums[2 * g + 1] = (unsigned long long)hi;
+  }
 }
 
 // ---- inner-join pairs in probe order ---------------------------------------------------------
@@ -620,8 +647,13 @@ int32_t tgp_hash_probe_agg(tgp_ctx* ctx, const tgp_join_agg_query* q, const tgp_
                                  (uint64_t)persistent_grid(ctx, 8));
   if (lds <= 48 * 1024)
     k_probe_agg<true><<<grid, 256, lds, s>>>(q->table->dev(), Q);
-  else
+  else {
+    TGP_REQUIRE(q->row_end - q->row_begin < (1ull << 32),
+                "tgp_hash_probe_agg: a global-table call takes fewer than 2^32 probe rows");
     k_probe_agg<false><<<grid, 256, 0, s>>>(q->table->dev(), Q);
+    TGP_HIP(hipGetLastError());
+    k_split_finalize<<<grid_for(q->ngroups), 256, 0, s>>>(Q.out_sums, q->ngroups);
+  }
   TGP_HIP(hipGetLastError());
   return TGP_OK;
 }

@@ -172,3 +172,34 @@ def device_uniform(ctx, n, dtype, column_id, lo, hi, seed=SEED, stream=None):
     A.check(A.lib().tgp_synth_uniform(ctx.handle, C.byref(d), seed, column_id, lo, hi, 0, n,
                                       stream), "tgp_synth_uniform")
     return col
+
+
+# ---- batched workload: many Q1-shape queries over one resident table ------------------------
+BATCH_TABLE_ROWS = 256_000_000
+BATCH_QUERIES = 4096
+BATCH_GROUPS = ("a", "b", "ab", "month")
+
+
+def batch_query_params(n_queries, table_rows, seed=SEED, lmin=1_000, lmax=4_000_000):
+    """Seeded query parameters: row range length log-uniform in [lmin, lmax],
+    start uniform, date cutoff uniform in [0, 2526), group key uniform over
+    {flaga, flagb, flaga x flagb, dt/30}."""
+    rng = np.random.Generator(np.random.PCG64((seed ^ 0xBA7C4) & 0xFFFFFFFF))
+    lmax = min(lmax, table_rows)
+    lmin = min(lmin, lmax)
+    L = np.floor(np.exp(rng.uniform(np.log(lmin), np.log(lmax), n_queries))).astype(np.int64)
+    start = np.floor(rng.uniform(0, 1, n_queries) * (table_rows - L + 1)).astype(np.int64)
+    cutoff = rng.integers(0, 2526, n_queries)
+    group = rng.integers(0, len(BATCH_GROUPS), n_queries)
+    return [(int(s), int(s + l), int(c), BATCH_GROUPS[g])
+            for s, l, c, g in zip(start, L, cutoff, group)]
+
+
+def batch_queries(table, params):
+    return [build_query(table, q1_spec(date_cutoff=c, group=g), b, e) for (b, e, c, g) in params]
+
+
+def batch_bytes(params):
+    """Algorithmic bytes: rows x (dt + group-key columns + 4 summed columns)."""
+    key_bytes = {"a": 1, "b": 1, "ab": 2, "month": 0}
+    return sum((e - b) * (4 + key_bytes[g] + 32) for (b, e, _, g) in params)
