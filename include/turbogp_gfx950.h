@@ -274,11 +274,22 @@ int32_t tgp_batch_create(tgp_ctx* ctx, uint32_t max_queries, tgp_batch** out);
 int32_t tgp_batch_destroy(tgp_batch* b);
 /* Plans and uploads the queries (synchronous); invalidates the captured graph. */
 int32_t tgp_batch_set_queries(tgp_batch* b, const tgp_agg_query* qs, uint32_t n, void* stream);
-/* wg_per_cu 0 = as many as fit; use_graph 0 = plain launches. */
-int32_t tgp_batch_set_tuning(tgp_batch* b, uint32_t wg_per_cu, int32_t use_graph);
-/* Times the persistent launch for several workgroups-per-CU settings on the
- * current queries and keeps the fastest (the batch's autotuner). */
-int32_t tgp_batch_autotune(tgp_batch* b, void* stream, uint32_t* best_wg_per_cu, float* best_ms);
+/* Launch settings of the persistent kernel.  wg_per_cu: workgroups per CU (0 =
+ * as many as fit); pipeline: 256-row slices in flight per wave (1 or 2; 0 =
+ * default 1); chunk_slices: slices per dynamically scheduled chunk (0 = default
+ * 32); use_graph: replay a captured hipGraph (0 = plain launches). */
+typedef struct tgp_batch_tuning {
+  uint32_t wg_per_cu;
+  int32_t pipeline;
+  uint32_t chunk_slices;
+  int32_t use_graph;
+} tgp_batch_tuning;
+int32_t tgp_batch_set_tuning(tgp_batch* b, const tgp_batch_tuning* t);
+int32_t tgp_batch_get_tuning(tgp_batch* b, tgp_batch_tuning* t);
+/* The batch's autotuner: times the persistent launch on the current queries
+ * over chunk sizes, workgroups per CU (up to the occupancy limit) and pipeline
+ * depths, keeps the fastest settings and returns them. */
+int32_t tgp_batch_autotune(tgp_batch* b, void* stream, tgp_batch_tuning* best, float* best_ms);
 int32_t tgp_batch_run(tgp_batch* b, int32_t mode, void* stream);
 /* tgp_batch_run, synchronous, with device timing: *h_total_ms for the whole
  * batch and, when h_query_ms != NULL, each query's completion time measured

@@ -60,12 +60,15 @@ def test_batch_matches_single_queries_and_oracle(ctx, table):
         b.set_tuning(0, use_graph=False)
         b.run(A.BATCH_PERSISTENT)
         plain = b.results()
+        b.set_tuning(3, use_graph=True, pipeline=2, chunk_slices=5)
+        b.run(A.BATCH_PERSISTENT)
+        pipe1 = b.results()
     finally:
         b.close()
     for i in range(len(qs)):
         assert same(persistent[i], per_query[i]), i
         assert same(persistent[i], replay[i]) and same(persistent[i], replay2[i]), i
-        assert same(persistent[i], plain[i]), i
+        assert same(persistent[i], plain[i]) and same(persistent[i], pipe1[i]), i
     for i in range(40):
         assert same(persistent[i], single[i]), i
     # oracle on a subset (all edge queries included)
@@ -87,14 +90,17 @@ def test_batch_flush_every_2_20_rows_with_a_tiny_grid(ctx, table):
     try:
         b.set_queries(qs)
         ctx.tune(A.TUNE_GRID_MAX, 2)   # 8 waves: each wave adds millions of rows
-        b.set_tuning(0, use_graph=False)
-        b.run(A.BATCH_PERSISTENT)
-        got = b.results()
+        got = []
+        for pipe, chunk in ((1, 65536), (2, 7000)):
+            b.set_tuning(0, use_graph=False, pipeline=pipe, chunk_slices=chunk)
+            b.run(A.BATCH_PERSISTENT)
+            got.append(b.results())
     finally:
         ctx.tune(A.TUNE_GRID_MAX, 0)
         b.close()
-    for q, r in zip(qs, got):
-        assert same(r, ctx.group_agg(q))
+    for i, q in enumerate(qs):
+        want = ctx.group_agg(q)
+        assert same(got[0][i], want) and same(got[1][i], want), i
 
 
 def test_batch_autotune_keeps_results(ctx, table):
@@ -106,8 +112,9 @@ def test_batch_autotune_keeps_results(ctx, table):
         b.set_queries(qs)
         b.run()
         before = b.results()
-        wg, ms = b.autotune()
-        assert wg in (1, 2, 3, 4, 6, 8) and ms > 0
+        best, ms = b.autotune()
+        assert 2 <= best["wg_per_cu"] <= 8 and best["pipeline"] in (1, 2) and ms > 0
+        assert best["chunk_slices"] in (8, 16, 32, 64, 128) and b.tuning() == best
         b.run()
         after = b.results()
         tot, per = b.run_timed(A.BATCH_PERSISTENT)

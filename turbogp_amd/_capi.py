@@ -104,6 +104,11 @@ I32_ = C.c_int32
 U64_ = C.c_uint64
 I64_ = C.c_int64
 
+class BatchTuning(C.Structure):
+    _fields_ = [("wg_per_cu", C.c_uint32), ("pipeline", C.c_int32), ("chunk_slices", C.c_uint32),
+                ("use_graph", C.c_int32)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "tgp_last_error": (C.c_char_p, []),
@@ -149,8 +154,9 @@ SIGNATURES = {
     "tgp_batch_create": (I32_, [P, C.c_uint32, C.POINTER(P)]),
     "tgp_batch_destroy": (I32_, [P]),
     "tgp_batch_set_queries": (I32_, [P, C.POINTER(AggQuery), C.c_uint32, P]),
-    "tgp_batch_set_tuning": (I32_, [P, C.c_uint32, I32_]),
-    "tgp_batch_autotune": (I32_, [P, P, C.POINTER(C.c_uint32), C.POINTER(C.c_float)]),
+    "tgp_batch_set_tuning": (I32_, [P, C.POINTER(BatchTuning)]),
+    "tgp_batch_get_tuning": (I32_, [P, C.POINTER(BatchTuning)]),
+    "tgp_batch_autotune": (I32_, [P, P, C.POINTER(BatchTuning), C.POINTER(C.c_float)]),
     "tgp_batch_run": (I32_, [P, I32_, P]),
     "tgp_batch_run_timed": (I32_, [P, I32_, P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "tgp_batch_info": (I32_, [P, C.POINTER(U64_), C.POINTER(U64_), C.POINTER(C.c_uint32),

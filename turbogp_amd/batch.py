@@ -22,15 +22,24 @@ class Batch:
         self.n = len(queries)
         self.naggs = [q.naggs for q in queries]
 
-    def set_tuning(self, wg_per_cu=0, use_graph=True):
-        A.check(A.lib().tgp_batch_set_tuning(self.handle, wg_per_cu, int(use_graph)),
-                "tgp_batch_set_tuning")
+    def set_tuning(self, wg_per_cu=0, use_graph=True, pipeline=0, chunk_slices=0):
+        t = A.BatchTuning(wg_per_cu, pipeline, chunk_slices, int(use_graph))
+        A.check(A.lib().tgp_batch_set_tuning(self.handle, C.byref(t)), "tgp_batch_set_tuning")
+
+    def tuning(self):
+        t = A.BatchTuning()
+        A.check(A.lib().tgp_batch_get_tuning(self.handle, C.byref(t)), "tgp_batch_get_tuning")
+        return {"wg_per_cu": t.wg_per_cu, "pipeline": t.pipeline,
+                "chunk_slices": t.chunk_slices, "use_graph": bool(t.use_graph)}
 
     def autotune(self, stream=None):
-        wg, ms = C.c_uint32(), C.c_float()
-        A.check(A.lib().tgp_batch_autotune(self.handle, stream, C.byref(wg), C.byref(ms)),
+        """Times the launch settings on the current queries and keeps the fastest.
+        -> (settings dict, best ms)"""
+        t, ms = A.BatchTuning(), C.c_float()
+        A.check(A.lib().tgp_batch_autotune(self.handle, stream, C.byref(t), C.byref(ms)),
                 "tgp_batch_autotune")
-        return wg.value, ms.value
+        return {"wg_per_cu": t.wg_per_cu, "pipeline": t.pipeline,
+                "chunk_slices": t.chunk_slices, "use_graph": bool(t.use_graph)}, ms.value
 
     def run(self, mode=A.BATCH_PERSISTENT, stream=None):
         A.check(A.lib().tgp_batch_run(self.handle, mode, stream), "tgp_batch_run")

@@ -2,15 +2,18 @@
 // kernel launch, captured in a hipGraph.
 //
 // Every query of a batch is planned into its specialised shape (spec.h) and cut
-// into 256-row slices.  The slices of all queries form one flattened list; wave
-// w of the persistent grid processes the w-th equal share of that list, i.e. a
-// contiguous run of slices that crosses query boundaries.  A wave accumulates
-// into a wave-private LDS region (counts and 64-bit partial sums per group) and
-// flushes it into the query's exact int128 result with device atomics when it
-// moves to the next query, when 2^20 rows have been added since the last flush
-// (the partials stay exact for |value| < 2^43; larger values take an exact slow
-// path per slice), and at the end.  Load balance is by construction: every
-// wave gets the same number of slices, whatever the sizes of the queries.
+// into 256-row slices.  The slices of all queries form one flattened list, cut
+// into chunks of a few dozen slices that the waves of the persistent grid take
+// from a device counter (a chunk may cross query boundaries), so that waves
+// that run slower take fewer chunks.  A wave accumulates into a wave-private LDS
+// region (counts and 64-bit partial sums per group) and flushes it into the
+// query's exact int128 result with device atomics at the end of each segment
+// (the part of one query inside one chunk) and after every 2^20 rows (the
+// partials stay exact for |value| < 2^43; larger values take an exact slow path
+// per slice).  The region holds R = 2^k copies of the
+// query's histogram (as many as fit, at most 64) and lane l updates copy
+// l mod R, so that few lanes of one LDS atomic instruction hit the same word
+// even when the query has only two or three groups.
 //
 // The one-launch-per-query form (tgp_group_agg per query) is kept as the
 // baseline mode.  The reference has no batched executor; its per-query entry is
@@ -19,9 +22,10 @@
 
 namespace tgp {
 
-constexpr int kBatchWords = 512;                 // per-wave region (groups * (1 + naggs))
+constexpr int kBatchWords = 1024;                // per-wave region: R copies of groups * (1 + naggs)
 constexpr uint64_t kFlushRows = 1ull << 20;      // rows between flushes
 constexpr int kBatchBigBits = 43;                // |value| < 2^43 on the fast path
+constexpr uint64_t kGranSlices = 16;             // granularity of the slice -> query table
 
 struct BatchQuery {
   SpecArgs a;           // arguments; a.out_* point into the batch's result arena
@@ -29,7 +33,7 @@ struct BatchQuery {
   int32_t shape;        // index into kShapes
   int32_t words;        // groups * (1 + naggs)
   int32_t naggs;
-  int32_t _pad;
+  int32_t rshift;       // log2 of the number of histogram copies R
 };
 
 __device__ __forceinline__ bool big_batch(uint64_t v) {
@@ -37,26 +41,36 @@ __device__ __forceinline__ bool big_batch(uint64_t v) {
   return (((uint64_t)(s ^ (s >> 63))) >> kBatchBigBits) != 0;
 }
 
-// One slice of query Q (shape S) into the wave-private region `reg`.
+// Query plans are read through the constant address space: the plan is not
+// written while the kernel runs, so uniform reads become scalar loads into SGPRs
+// (as kernel arguments are) instead of per-lane vector loads.
+typedef const __attribute__((address_space(4))) BatchQuery ConstQuery;
+
+// One slice of a query of shape S into the wave-private region `reg`: the
+// values `v` of the slice starting at query-relative slice `slice` are loaded.
+template <Shape S>
+__device__ __forceinline__ void batch_load(const SpecArgs& A, uint64_t slice, uint64_t min_len,
+                                           SliceVals<S>& v) {
+  const uint64_t slice0 = A.base0 + slice * kSliceRows;
+  const uint64_t row = slice0 + (threadIdx.x & 63) * kSR;
+  load_all<S, 0>(A, row, slice0 + kSliceRows <= min_len, v);
+}
+
 template <Shape S>
 __device__ __forceinline__ void batch_slice(const SpecArgs& A, uint64_t slice,
-                                            unsigned long long* reg, uint32_t& oor) {
+                                            const SliceVals<S>& v, unsigned long long* reg,
+                                            uint32_t rshift, uint32_t& oor) {
   const uint32_t lane = threadIdx.x & 63;
-  const uint64_t slice0 = A.base0 + slice * kSliceRows;
-  const uint64_t row = slice0 + lane * kSR;
-  uint64_t min_len = A.col_len[0];
-#pragma unroll
-  for (int s = 1; s < S.nslots; ++s) min_len = A.col_len[s] < min_len ? A.col_len[s] : min_len;
-  const bool fast = slice0 + kSliceRows <= min_len;
-  SliceVals<S> v;
-  load_all<S, 0>(A, row, fast, v);
+  // this lane's copy of the histogram: word w at reg[(w << rshift) + copy]
+  unsigned long long* mine = reg + (lane & ((1u << rshift) - 1));
+  const uint64_t row = A.base0 + slice * kSliceRows + lane * kSR;
   int32_t gid[kSR];
   const uint32_t sel = eval_slice<S>(A, row, v, gid, oor);
   if (!__ballot(sel != 0)) return;
   constexpr int W1 = 1 + S.naggs;
 #pragma unroll
   for (int r = 0; r < kSR; ++r)
-    if ((sel >> r) & 1) atomicAdd(&reg[gid[r] * W1], 1ull);
+    if ((sel >> r) & 1) atomicAdd(&mine[(uint32_t)(gid[r] * W1) << rshift], 1ull);
   static_for<S.naggs>([&](auto AI) {
     constexpr int a = decltype(AI)::value;
     uint64_t val[kSR];
@@ -72,83 +86,133 @@ __device__ __forceinline__ void batch_slice(const SpecArgs& A, uint64_t slice,
     } else {
 #pragma unroll
       for (int r = 0; r < kSR; ++r)
-        if ((sel >> r) & 1) atomicAdd(&reg[gid[r] * W1 + 1 + a], (unsigned long long)val[r]);
+        if ((sel >> r) & 1)
+          atomicAdd(&mine[(uint32_t)(gid[r] * W1 + 1 + a) << rshift], (unsigned long long)val[r]);
     }
   });
 }
 
-// Adds the wave region into query Q's result and clears it.
-__device__ __forceinline__ void batch_flush(const BatchQuery& Q, unsigned long long* reg,
-                                            int naggs) {
+// Adds the wave region (`words` words, groups x (count, naggs sums), in 2^rshift
+// copies) into the query's exact result and clears it.  Every copy's partials
+// come from at most 2^20 values of magnitude < 2^43, so the 64-bit sum of the
+// copies is exact.
+__device__ __forceinline__ void batch_flush(unsigned long long* out_count,
+                                            unsigned long long* out_sums, uint32_t words,
+                                            int naggs, unsigned long long* reg, uint32_t rshift) {
   const uint32_t lane = threadIdx.x & 63;
   __builtin_amdgcn_s_waitcnt(0);
   __builtin_amdgcn_wave_barrier();
-  const uint32_t W1 = 1 + naggs;
-  for (uint32_t w = lane; w < (uint32_t)Q.words; w += 64) {
-    const unsigned long long x = reg[w];
+  const uint32_t W1 = 1 + naggs, R = 1u << rshift;
+  for (uint32_t w = lane; w < words; w += 64) {
+    unsigned long long x = 0;
+    for (uint32_t c = 0; c < R; ++c) {
+      x += reg[(w << rshift) + c];
+      reg[(w << rshift) + c] = 0;
+    }
     if (!x) continue;
-    reg[w] = 0;
     const uint32_t g = w / W1, k = w % W1;
-    if (k == 0) atomicAdd(&Q.a.out_count[g], x);
-    else atomic_add128(&Q.a.out_sums[((uint64_t)g * naggs + (k - 1)) * 2], x,
+    if (k == 0) atomicAdd(&out_count[g], x);
+    else atomic_add128(&out_sums[((uint64_t)g * naggs + (k - 1)) * 2], x,
                        (int64_t)x < 0 ? -1 : 0);
   }
   __builtin_amdgcn_wave_barrier();
 }
 
-template <int I>
-__device__ __forceinline__ void dispatch_slice(int shape, const SpecArgs& A, uint64_t slice,
-                                               unsigned long long* reg, uint32_t& oor) {
-  if constexpr (I < kNumShapes) {
-    if constexpr (kShapes[I].nslots > 0 && kShapes[I].has_sel == 0) {
-      if (shape == I) {
-        batch_slice<kShapes[I]>(A, slice, reg, oor);
-        return;
+// Slices [s_begin, s_end) (query-relative) of one query of shape S, with a
+// flush every kFlushRows rows and at the end.  PIPE = 2 issues the loads of the
+// next slice before it processes the current one (two slices in flight per
+// wave, more registers); PIPE = 1 loads and processes one slice at a time.
+template <Shape S, int PIPE>
+__device__ __forceinline__ void batch_segment(const SpecArgs& A, uint64_t s_begin,
+                                              uint64_t s_end, unsigned long long* reg,
+                                              uint32_t rshift, uint32_t& oor) {
+  uint64_t min_len = A.col_len[0];
+#pragma unroll
+  for (int s = 1; s < S.nslots; ++s) min_len = A.col_len[s] < min_len ? A.col_len[s] : min_len;
+  constexpr uint64_t kFlushSlices = kFlushRows / kSliceRows;
+  for (uint64_t c = s_begin; c < s_end; c += kFlushSlices) {
+    const uint64_t e = c + kFlushSlices < s_end ? c + kFlushSlices : s_end;
+    if constexpr (PIPE == 1) {
+      for (uint64_t s = c; s < e; ++s) {
+        SliceVals<S> v;
+        batch_load<S>(A, s, min_len, v);
+        batch_slice<S>(A, s, v, reg, rshift, oor);
+      }
+    } else {
+      SliceVals<S> cur;
+      batch_load<S>(A, c, min_len, cur);
+      for (uint64_t s = c; s < e; ++s) {
+        SliceVals<S> nxt;
+        if (s + 1 < e) batch_load<S>(A, s + 1, min_len, nxt);
+        batch_slice<S>(A, s, cur, reg, rshift, oor);
+        cur = nxt;
       }
     }
-    dispatch_slice<I + 1>(shape, A, slice, reg, oor);
+    batch_flush(A.out_count, A.out_sums, A.words, S.naggs, reg, rshift);
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_batch(const BatchQuery* __restrict__ qs, uint32_t nq,
-                                                  uint64_t total_slices) {
+template <int I, int PIPE>
+__device__ __forceinline__ void dispatch_segment(int shape, const SpecArgs& A, uint64_t s_begin,
+                                                 uint64_t s_end, unsigned long long* reg,
+                                                 uint32_t rshift, uint32_t& oor) {
+  if constexpr (I < kNumShapes) {
+    if constexpr (kShapes[I].nslots > 0 && kShapes[I].has_sel == 0) {
+      if (shape == I) {
+        batch_segment<kShapes[I], PIPE>(A, s_begin, s_end, reg, rshift, oor);
+        return;
+      }
+    }
+    dispatch_segment<I + 1, PIPE>(shape, A, s_begin, s_end, reg, rshift, oor);
+  }
+}
+
+// Takes the next chunk index from the batch's counter (one atomic per wave).
+__device__ __forceinline__ uint64_t grab_chunk(unsigned long long* counter) {
+  unsigned long long k = 0;
+  if ((threadIdx.x & 63) == 0) k = atomicAdd(counter, 1ull);
+  return __builtin_amdgcn_readfirstlane((uint32_t)k) |
+         ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(k >> 32)) << 32);
+}
+
+// Persistent kernel: every wave takes chunks of `chunk` consecutive slices of
+// the flattened slice list from a device counter until none are left, so a
+// slow wave takes fewer chunks and all waves end together.  granule_q[g] is the
+// query that contains slice g * kGranSlices.
+template <int PIPE>
+__global__ void __launch_bounds__(kBlock) k_batch(const BatchQuery* __restrict__ plan, uint32_t nq,
+                                                  uint64_t total_slices,
+                                                  const uint32_t* __restrict__ granule_q,
+                                                  uint32_t chunk, unsigned long long* counter) {
   __shared__ unsigned long long lreg[(kBlock / kWave) * kBatchWords];
+  ConstQuery* qs = (ConstQuery*)plan;
+  const __attribute__((address_space(4))) uint32_t* gq =
+      (const __attribute__((address_space(4))) uint32_t*)granule_q;
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   unsigned long long* reg = lreg + wave * kBatchWords;
   for (int w = lane; w < kBatchWords; w += 64) reg[w] = 0;
   __builtin_amdgcn_wave_barrier();
-  const uint64_t nwaves = (uint64_t)gridDim.x * (kBlock / kWave);
-  const uint64_t gw = (uint64_t)blockIdx.x * (kBlock / kWave) + wave;
-  const uint64_t s0 = total_slices * gw / nwaves, s1 = total_slices * (gw + 1) / nwaves;
-  if (s0 >= s1) return;
-  // first query whose slices include s0: the last q with qs[q].slice0 <= s0
-  uint32_t lo = 0, hi = nq;
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (qs[mid].slice0 <= s0) lo = mid; else hi = mid;
-  }
-  uint32_t q = __builtin_amdgcn_readfirstlane(lo);
-  uint32_t oor = 0;
-  uint64_t rows_since_flush = 0;
-  uint64_t q_end = q + 1 < nq ? qs[q + 1].slice0 : total_slices;
-  for (uint64_t s = s0; s < s1; ++s) {
-    while (s >= q_end) {  // next query (skips queries without slices)
-      if (rows_since_flush) batch_flush(qs[q], reg, qs[q].naggs);
-      if (oor) { atomicAdd(qs[q].a.out_oor, (unsigned long long)oor); oor = 0; }
-      rows_since_flush = 0;
-      ++q;
-      q_end = q + 1 < nq ? qs[q + 1].slice0 : total_slices;
+  uint64_t k = grab_chunk(counter);
+  while (k * chunk < total_slices) {
+    const uint64_t c0 = k * chunk;
+    const uint64_t c1 = c0 + chunk < total_slices ? c0 + chunk : total_slices;
+    const uint64_t next = grab_chunk(counter);  // in flight while this chunk runs
+    uint32_t q = gq[c0 / kGranSlices];
+    while (q + 1 < nq && qs[q + 1].slice0 <= c0) ++q;
+    // the chunk as segments of consecutive queries
+    for (; q < nq; ++q) {
+      const uint64_t q0 = qs[q].slice0;
+      if (q0 >= c1) break;
+      const uint64_t q1 = q + 1 < nq ? qs[q + 1].slice0 : total_slices;
+      const uint64_t b = c0 > q0 ? c0 : q0, e = c1 < q1 ? c1 : q1;
+      if (b >= e) continue;
+      const SpecArgs& A = *(const SpecArgs*)&qs[q].a;
+      uint32_t oor = 0;
+      dispatch_segment<0, PIPE>(qs[q].shape, A, b - q0, e - q0, reg, (uint32_t)qs[q].rshift, oor);
+      if (oor) atomicAdd(A.out_oor, (unsigned long long)oor);
     }
-    const BatchQuery& Q = qs[q];
-    dispatch_slice<0>(Q.shape, Q.a, s - Q.slice0, reg, oor);
-    rows_since_flush += kSliceRows;
-    if (rows_since_flush >= kFlushRows) {
-      batch_flush(Q, reg, Q.naggs);
-      rows_since_flush = 0;
-    }
+    k = next;
   }
-  if (rows_since_flush) batch_flush(qs[q], reg, qs[q].naggs);
-  if (oor) atomicAdd(qs[q].a.out_oor, (unsigned long long)oor);
 }
 
 }  // namespace tgp
@@ -170,6 +234,10 @@ struct tgp_batch {
   uint64_t plan_cap = 0, results_cap = 0;
   // tuning / graph
   uint32_t wg_per_cu = 0;  // 0 = occupancy
+  int32_t pipeline = 1;    // slices in flight per wave (1 or 2)
+  uint32_t chunk = 32;     // slices per dynamically scheduled chunk
+  uint32_t* d_granule = nullptr;
+  uint64_t granule_cap = 0;
   bool use_graph = true;
   hipGraphExec_t graph = nullptr;
   hipGraph_t graph_def = nullptr;
@@ -183,24 +251,41 @@ static void drop_graph(tgp_batch* b) {
   b->graph_def = nullptr;
 }
 
-static int32_t batch_launch(tgp_batch* b, hipStream_t s) {
-  TGP_TRY(fill_u64(s, b->d_results, b->result_words, 0));
-  if (b->total_slices) {
-    int per_cu = 0;
-    TGP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch, kBlock, 0));
-    if (per_cu < 1) per_cu = 1;
-    if (b->wg_per_cu && (int)b->wg_per_cu < per_cu) per_cu = (int)b->wg_per_cu;
-    uint64_t grid = (uint64_t)persistent_grid(b->ctx, per_cu);
-    const uint64_t need = (b->total_slices + 3) / 4;
-    if (grid > need) grid = need;
-    const int64_t gmax = b->ctx->tune_grid_max.load();
-    if (gmax > 0 && grid > (uint64_t)gmax) grid = (uint64_t)gmax;
-    if (grid == 0) grid = 1;
-    k_batch<<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>(b->d_plan, (uint32_t)b->plan.size(),
-                                                        b->total_slices);
-    TGP_HIP(hipGetLastError());
-  }
+template <int PIPE>
+static int32_t launch_variant(tgp_batch* b, hipStream_t s) {
+  auto kern = k_batch<PIPE>;
+  int per_cu = 0;
+  TGP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0));
+  if (per_cu < 1) per_cu = 1;
+  if (b->wg_per_cu && (int)b->wg_per_cu < per_cu) per_cu = (int)b->wg_per_cu;
+  uint64_t grid = (uint64_t)persistent_grid(b->ctx, per_cu);
+  const uint64_t need = (b->total_slices + (uint64_t)b->chunk * 4 - 1) / ((uint64_t)b->chunk * 4);
+  if (grid > need) grid = need;
+  const int64_t gmax = b->ctx->tune_grid_max.load();
+  if (gmax > 0 && grid > (uint64_t)gmax) grid = (uint64_t)gmax;
+  if (grid == 0) grid = 1;
+  kern<<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>(
+      b->d_plan, (uint32_t)b->plan.size(), b->total_slices, b->d_granule, b->chunk,
+      reinterpret_cast<unsigned long long*>(b->d_results + b->result_words));
+  TGP_HIP(hipGetLastError());
   return TGP_OK;
+}
+
+static int32_t batch_launch(tgp_batch* b, hipStream_t s) {
+  // results and the chunk counter behind them start at zero
+  TGP_TRY(fill_u64(s, b->d_results, b->result_words + 1, 0));
+  if (!b->total_slices) return TGP_OK;
+  return b->pipeline == 1 ? launch_variant<1>(b, s) : launch_variant<2>(b, s);
+}
+
+// Resident workgroups per CU of a kernel variant (its occupancy limit).
+static int variant_occupancy(int pipeline) {
+  int per_cu = 0;
+  if (pipeline == 1)
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<1>, kBlock, 0);
+  else
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<2>, kBlock, 0);
+  return per_cu < 1 ? 1 : per_cu;
 }
 
 extern "C" {
@@ -220,6 +305,7 @@ int32_t tgp_batch_destroy(tgp_batch* b) {
   drop_graph(b);
   if (b->d_plan) hbm_free(b->ctx, b->d_plan);
   if (b->d_results) hbm_free(b->ctx, b->d_results);
+  if (b->d_granule) hbm_free(b->ctx, b->d_granule);
   delete b;
   return TGP_OK;
 }
@@ -254,13 +340,14 @@ int32_t tgp_batch_set_queries(tgp_batch* b, const tgp_agg_query* qs, uint32_t n,
     words += 1;
   }
   b->result_words = words;
-  if (b->results_cap < words) {
+  // the arena ends with the persistent kernel's chunk counter
+  if (b->results_cap < words + 1) {
     if (b->d_results) TGP_TRY(hbm_free(b->ctx, b->d_results));
     b->d_results = nullptr;
     void* p = nullptr;
-    TGP_TRY(hbm_alloc(b->ctx, words * 8, &p));
+    TGP_TRY(hbm_alloc(b->ctx, (words + 1) * 8, &p));
     b->d_results = static_cast<uint64_t*>(p);
-    b->results_cap = words;
+    b->results_cap = words + 1;
   }
   uint64_t slices = 0;
   for (uint32_t i = 0; i < n; ++i) {
@@ -278,6 +365,9 @@ int32_t tgp_batch_set_queries(tgp_batch* b, const tgp_agg_query* qs, uint32_t n,
     bq.shape = idx;
     bq.words = (int32_t)bq.a.words;
     bq.naggs = qs[i].naggs;
+    bq.rshift = 6;
+    while (bq.rshift > 0 && ((uint64_t)bq.a.words << bq.rshift) > (uint64_t)kBatchWords)
+      --bq.rshift;
     bq.slice0 = slices;
     bq.a.out_count = reinterpret_cast<unsigned long long*>(b->d_results + b->count_off[i]);
     bq.a.out_sums = reinterpret_cast<unsigned long long*>(b->d_results + b->sums_off[i]);
@@ -295,18 +385,49 @@ int32_t tgp_batch_set_queries(tgp_batch* b, const tgp_agg_query* qs, uint32_t n,
     b->d_plan = static_cast<BatchQuery*>(p);
     b->plan_cap = b->plan.size();
   }
+  // slice -> query table: granule g holds the plan index of the query that
+  // contains slice g * kGranSlices
+  const uint64_t ngran = (slices + kGranSlices - 1) / kGranSlices;
+  std::vector<uint32_t> gran(ngran ? ngran : 1, 0);
+  for (uint64_t g = 0, q = 0; g < ngran; ++g) {
+    while (q + 1 < b->plan.size() && b->plan[q + 1].slice0 <= g * kGranSlices) ++q;
+    gran[g] = (uint32_t)q;
+  }
+  if (b->granule_cap < gran.size()) {
+    if (b->d_granule) TGP_TRY(hbm_free(b->ctx, b->d_granule));
+    b->d_granule = nullptr;
+    void* p = nullptr;
+    TGP_TRY(hbm_alloc(b->ctx, gran.size() * 4, &p));
+    b->d_granule = static_cast<uint32_t*>(p);
+    b->granule_cap = gran.size();
+  }
   if (bytes) {
     TGP_HIP(hipMemcpyAsync(b->d_plan, b->plan.data(), bytes, hipMemcpyHostToDevice, s));
+    TGP_HIP(hipMemcpyAsync(b->d_granule, gran.data(), gran.size() * 4, hipMemcpyHostToDevice, s));
     TGP_HIP(hipStreamSynchronize(s));
   }
   return TGP_OK;
 }
 
-int32_t tgp_batch_set_tuning(tgp_batch* b, uint32_t wg_per_cu, int32_t use_graph) {
-  TGP_REQUIRE(b, "tgp_batch_set_tuning: null batch");
+int32_t tgp_batch_set_tuning(tgp_batch* b, const tgp_batch_tuning* t) {
+  TGP_REQUIRE(b && t, "tgp_batch_set_tuning: null argument");
+  TGP_REQUIRE(t->pipeline >= 0 && t->pipeline <= 2, "pipeline depth %d not in {0, 1, 2}",
+              t->pipeline);
+  TGP_REQUIRE(t->chunk_slices <= (1u << 16), "chunk of %u slices above 65536", t->chunk_slices);
   drop_graph(b);
-  b->wg_per_cu = wg_per_cu;
-  b->use_graph = use_graph != 0;
+  b->wg_per_cu = t->wg_per_cu;
+  b->pipeline = t->pipeline == 0 ? 1 : t->pipeline;
+  b->chunk = t->chunk_slices == 0 ? 32 : t->chunk_slices;
+  b->use_graph = t->use_graph != 0;
+  return TGP_OK;
+}
+
+int32_t tgp_batch_get_tuning(tgp_batch* b, tgp_batch_tuning* t) {
+  TGP_REQUIRE(b && t, "tgp_batch_get_tuning: null argument");
+  t->wg_per_cu = b->wg_per_cu;
+  t->pipeline = b->pipeline;
+  t->chunk_slices = b->chunk;
+  t->use_graph = b->use_graph ? 1 : 0;
   return TGP_OK;
 }
 
@@ -421,41 +542,70 @@ int32_t tgp_batch_results_sync(tgp_batch* b, uint64_t* h_words, uint64_t nwords,
   return TGP_OK;
 }
 
-int32_t tgp_batch_autotune(tgp_batch* b, void* stream, uint32_t* best_wg_per_cu, float* best_ms) {
+int32_t tgp_batch_autotune(tgp_batch* b, void* stream, tgp_batch_tuning* best_out,
+                           float* best_ms) {
   TGP_REQUIRE(b, "tgp_batch_autotune: null batch");
   hipStream_t s = pick_stream(b->ctx, stream);
-  hipEvent_t e0, e1;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   TGP_HIP(hipEventCreate(&e0));
-  TGP_HIP(hipEventCreate(&e1));
-  const uint32_t cand[] = {1, 2, 3, 4, 6, 8};
-  float best = 1e30f;
-  uint32_t pick = 0;
-  const bool graph = b->use_graph;
-  for (uint32_t c : cand) {
-    b->wg_per_cu = c;
-    drop_graph(b);
-    TGP_TRY(batch_launch(b, s));  // warm
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    set_error("tgp_batch_autotune: hipEventCreate failed");
+    return TGP_ERR_HIP;
+  }
+  int32_t rc = TGP_OK;
+  // best of 3 timed launches (after one warm-up) of the current settings
+  auto time_it = [&](float* out) -> int32_t {
+    TGP_TRY(batch_launch(b, s));
     float ms_min = 1e30f;
     for (int rep = 0; rep < 3; ++rep) {
+      float ms = 0;
       TGP_HIP(hipEventRecord(e0, s));
       TGP_TRY(batch_launch(b, s));
       TGP_HIP(hipEventRecord(e1, s));
       TGP_HIP(hipEventSynchronize(e1));
-      float ms = 0;
       TGP_HIP(hipEventElapsedTime(&ms, e0, e1));
       ms_min = ms < ms_min ? ms : ms_min;
     }
-    if (ms_min < best) {
-      best = ms_min;
-      pick = c;
+    *out = ms_min;
+    return TGP_OK;
+  };
+  float best = 1e30f;
+  uint32_t best_wg = 0, best_chunk = 32;
+  int32_t best_pipe = 1;
+  auto trial = [&](int32_t pipe, uint32_t wg, uint32_t chunk) {
+    if (rc != TGP_OK) return;
+    b->pipeline = pipe;
+    b->wg_per_cu = wg;
+    b->chunk = chunk;
+    float ms = 0;
+    rc = time_it(&ms);
+    if (rc == TGP_OK && ms < best) {
+      best = ms;
+      best_pipe = pipe;
+      best_wg = wg;
+      best_chunk = chunk;
     }
-  }
+  };
+  // coordinate search: chunk size at full occupancy, then workgroups per CU,
+  // then the two-slice pipeline at its own occupancy
+  const uint32_t occ1 = (uint32_t)variant_occupancy(1), occ2 = (uint32_t)variant_occupancy(2);
+  for (uint32_t chunk : {8u, 16u, 32u, 64u, 128u}) trial(1, occ1, chunk);
+  for (uint32_t wg = 2; wg < occ1; ++wg) trial(1, wg, best_chunk);
+  for (uint32_t wg = 2; wg <= occ2; ++wg) trial(2, wg, best_chunk);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  b->wg_per_cu = pick;
-  b->use_graph = graph;
+  b->pipeline = best_pipe;
+  b->wg_per_cu = best_wg;
+  b->chunk = best_chunk;
   drop_graph(b);
-  if (best_wg_per_cu) *best_wg_per_cu = pick;
+  if (rc != TGP_OK) return rc;
+  if (best_out) {
+    best_out->wg_per_cu = best_wg;
+    best_out->pipeline = best_pipe;
+    best_out->chunk_slices = best_chunk;
+    best_out->use_graph = b->use_graph ? 1 : 0;
+  }
   if (best_ms) *best_ms = best;
   return TGP_OK;
 }
