@@ -188,8 +188,14 @@ int32_t tgp_event_elapsed_ms(tgp_ctx* ctx, void* start, void* end, float* out_ms
 enum tgp_tune_key {
   TGP_TUNE_WG_PER_CU = 0, /* workgroups per CU of the persistent scan/aggregate kernels */
   TGP_TUNE_GRID_MAX = 1,  /* upper bound on their grid size */
-  TGP_TUNE_SPECIALIZE = 2 /* 1 (default): use shape-specialised kernels when a query
-                             matches one; 0: always the generic engine */
+  TGP_TUNE_SPECIALIZE = 2, /* 1 (default): use shape-specialised kernels when a query
+                              matches one; 0: always the generic engine */
+  TGP_TUNE_SPEC_KERNEL = 3, /* specialised kernel form: 1: a shared per-wave histogram
+                               folded into registers after every slice; 2: waves with
+                               replicated LDS histograms (where the groups fit); 0
+                               (default): 2 for queries of 2..4 groups, else 1 */
+  TGP_TUNE_CHUNK = 4        /* form 0's schedule: 0 (default) static interleave of slices;
+                               n > 0: chunks of n slices (<= 4096) taken from a counter */
 };
 int32_t tgp_tune_set(tgp_ctx* ctx, int32_t key, int64_t value);
 int32_t tgp_tune_get(tgp_ctx* ctx, int32_t key, int64_t* value);

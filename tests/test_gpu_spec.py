@@ -21,16 +21,37 @@ def fact(ctx):
         c.free()
 
 
+def kernel_forms(ctx, q):
+    """The query through every kernel form: the default choice, the
+    replicated-histogram specialised kernel (static interleave, then one-slice
+    chunks from a counter), the per-slice-fold specialised kernel and the
+    generic engine."""
+    out = []
+    try:
+        ctx.tune(A.TUNE_SPECIALIZE, 1)
+        out.append(ctx.group_agg(q))
+        ctx.tune(A.TUNE_SPEC_KERNEL, 2)
+        out.append(ctx.group_agg(q))
+        ctx.tune(A.TUNE_CHUNK, 1)
+        out.append(ctx.group_agg(q))
+        ctx.tune(A.TUNE_CHUNK, 0)
+        ctx.tune(A.TUNE_SPEC_KERNEL, 1)
+        out.append(ctx.group_agg(q))
+        ctx.tune(A.TUNE_SPECIALIZE, 0)
+        out.append(ctx.group_agg(q))
+    finally:
+        ctx.tune(A.TUNE_SPECIALIZE, 1)
+        ctx.tune(A.TUNE_SPEC_KERNEL, 0)
+        ctx.tune(A.TUNE_CHUNK, 0)
+    return out
+
+
 def both_paths(ctx, host, dev, spec, b=0, e=None):
     names, preds, parts, aggs = spec
     q = make_agg_query([dev[c] for c in names], preds, parts, aggs, b, e)
-    ctx.tune(A.TUNE_SPECIALIZE, 1)
-    got_s = ctx.group_agg(q)
-    ctx.tune(A.TUNE_SPECIALIZE, 0)
-    got_g = ctx.group_agg(q)
-    ctx.tune(A.TUNE_SPECIALIZE, 1)
+    forms = kernel_forms(ctx, q)
     cnt, sums, oor = B.group_agg(B.make_query([host[c] for c in names], preds, parts, aggs, b, e))
-    for got in (got_s, got_g):
+    for got in forms:
         assert [int(x) for x in got.count] == [int(x) for x in cnt]
         assert got.sums() == sums
         assert got.n_out_of_range == oor
@@ -87,3 +108,35 @@ def test_short_columns_tail_slices(ctx):
         finally:
             for c in dev.values():
                 c.free()
+
+
+def test_wave_kernel_flushes_every_2_20_rows(ctx):
+    """4 waves over 5.3M rows: in 4096-slice chunks a wave takes a second chunk
+    and flushes its LDS partials in between; statically interleaved, every wave
+    adds 1.3M rows and flushes once on the way."""
+    n = 5_300_000
+    table = W.make_fact_table(ctx, n)
+    try:
+        for spec in (W.q1_spec(date_cutoff=2000, group=g) for g in ("ab", "b", "month")):
+            q = W.build_query(table, spec, 3, n - 5)
+            want = ctx.group_agg(q)  # automatic settings
+            try:
+                ctx.tune(A.TUNE_SPEC_KERNEL, 2)
+                ctx.tune(A.TUNE_GRID_MAX, 1)
+                static = ctx.group_agg(q)
+                ctx.tune(A.TUNE_CHUNK, 4096)
+                got = ctx.group_agg(q)
+                ctx.tune(A.TUNE_SPECIALIZE, 0)
+                generic = ctx.group_agg(q)
+            finally:
+                ctx.tune(A.TUNE_GRID_MAX, 0)
+                ctx.tune(A.TUNE_CHUNK, 0)
+                ctx.tune(A.TUNE_SPECIALIZE, 1)
+                ctx.tune(A.TUNE_SPEC_KERNEL, 0)
+            for r in (static, got, generic):
+                assert np.array_equal(r.count, want.count)
+                assert np.array_equal(r.sums_words, want.sums_words)
+                assert r.n_out_of_range == want.n_out_of_range
+    finally:
+        for c in table.values():
+            c.free()

@@ -18,14 +18,11 @@
 // The one-launch-per-query form (tgp_group_agg per query) is kept as the
 // baseline mode.  The reference has no batched executor; its per-query entry is
 // QueryEngine::execute (engine/mod.rs:1487).
-#include "spec.h"
+#include "wave_agg.h"
 
 namespace tgp {
 
-constexpr int kBatchWords = 1024;                // per-wave region: R copies of groups * (1 + naggs)
-constexpr uint64_t kFlushRows = 1ull << 20;      // rows between flushes
-constexpr int kBatchBigBits = 43;                // |value| < 2^43 on the fast path
-constexpr uint64_t kGranSlices = 16;             // granularity of the slice -> query table
+constexpr uint64_t kGranSlices = 16;  // granularity of the slice -> query table
 
 struct BatchQuery {
   SpecArgs a;           // arguments; a.out_* point into the batch's result arena
@@ -36,87 +33,10 @@ struct BatchQuery {
   int32_t rshift;       // log2 of the number of histogram copies R
 };
 
-__device__ __forceinline__ bool big_batch(uint64_t v) {
-  const int64_t s = (int64_t)v;
-  return (((uint64_t)(s ^ (s >> 63))) >> kBatchBigBits) != 0;
-}
-
 // Query plans are read through the constant address space: the plan is not
 // written while the kernel runs, so uniform reads become scalar loads into SGPRs
 // (as kernel arguments are) instead of per-lane vector loads.
 typedef const __attribute__((address_space(4))) BatchQuery ConstQuery;
-
-// One slice of a query of shape S into the wave-private region `reg`: the
-// values `v` of the slice starting at query-relative slice `slice` are loaded.
-template <Shape S>
-__device__ __forceinline__ void batch_load(const SpecArgs& A, uint64_t slice, uint64_t min_len,
-                                           SliceVals<S>& v) {
-  const uint64_t slice0 = A.base0 + slice * kSliceRows;
-  const uint64_t row = slice0 + (threadIdx.x & 63) * kSR;
-  load_all<S, 0>(A, row, slice0 + kSliceRows <= min_len, v);
-}
-
-template <Shape S>
-__device__ __forceinline__ void batch_slice(const SpecArgs& A, uint64_t slice,
-                                            const SliceVals<S>& v, unsigned long long* reg,
-                                            uint32_t rshift, uint32_t& oor) {
-  const uint32_t lane = threadIdx.x & 63;
-  // this lane's copy of the histogram: word w at reg[(w << rshift) + copy]
-  unsigned long long* mine = reg + (lane & ((1u << rshift) - 1));
-  const uint64_t row = A.base0 + slice * kSliceRows + lane * kSR;
-  int32_t gid[kSR];
-  const uint32_t sel = eval_slice<S>(A, row, v, gid, oor);
-  if (!__ballot(sel != 0)) return;
-  constexpr int W1 = 1 + S.naggs;
-#pragma unroll
-  for (int r = 0; r < kSR; ++r)
-    if ((sel >> r) & 1) atomicAdd(&mine[(uint32_t)(gid[r] * W1) << rshift], 1ull);
-  static_for<S.naggs>([&](auto AI) {
-    constexpr int a = decltype(AI)::value;
-    uint64_t val[kSR];
-    agg_values<S, a>(A, v, val);
-    bool b = false;
-#pragma unroll
-    for (int r = 0; r < kSR; ++r) b |= ((sel >> r) & 1) && big_batch(val[r]);
-    if (__ballot(b)) {
-#pragma unroll
-      for (int r = 0; r < kSR; ++r)
-        if ((sel >> r) & 1)
-          atomic_add128_i64(&A.out_sums[((uint64_t)gid[r] * S.naggs + a) * 2], (int64_t)val[r]);
-    } else {
-#pragma unroll
-      for (int r = 0; r < kSR; ++r)
-        if ((sel >> r) & 1)
-          atomicAdd(&mine[(uint32_t)(gid[r] * W1 + 1 + a) << rshift], (unsigned long long)val[r]);
-    }
-  });
-}
-
-// Adds the wave region (`words` words, groups x (count, naggs sums), in 2^rshift
-// copies) into the query's exact result and clears it.  Every copy's partials
-// come from at most 2^20 values of magnitude < 2^43, so the 64-bit sum of the
-// copies is exact.
-__device__ __forceinline__ void batch_flush(unsigned long long* out_count,
-                                            unsigned long long* out_sums, uint32_t words,
-                                            int naggs, unsigned long long* reg, uint32_t rshift) {
-  const uint32_t lane = threadIdx.x & 63;
-  __builtin_amdgcn_s_waitcnt(0);
-  __builtin_amdgcn_wave_barrier();
-  const uint32_t W1 = 1 + naggs, R = 1u << rshift;
-  for (uint32_t w = lane; w < words; w += 64) {
-    unsigned long long x = 0;
-    for (uint32_t c = 0; c < R; ++c) {
-      x += reg[(w << rshift) + c];
-      reg[(w << rshift) + c] = 0;
-    }
-    if (!x) continue;
-    const uint32_t g = w / W1, k = w % W1;
-    if (k == 0) atomicAdd(&out_count[g], x);
-    else atomic_add128(&out_sums[((uint64_t)g * naggs + (k - 1)) * 2], x,
-                       (int64_t)x < 0 ? -1 : 0);
-  }
-  __builtin_amdgcn_wave_barrier();
-}
 
 // Slices [s_begin, s_end) (query-relative) of one query of shape S, with a
 // flush every kFlushRows rows and at the end.  PIPE = 2 issues the loads of the
@@ -135,20 +55,20 @@ __device__ __forceinline__ void batch_segment(const SpecArgs& A, uint64_t s_begi
     if constexpr (PIPE == 1) {
       for (uint64_t s = c; s < e; ++s) {
         SliceVals<S> v;
-        batch_load<S>(A, s, min_len, v);
-        batch_slice<S>(A, s, v, reg, rshift, oor);
+        wave_load<S>(A, s, min_len, v);
+        wave_slice<S>(A, s, v, reg, rshift, oor);
       }
     } else {
       SliceVals<S> cur;
-      batch_load<S>(A, c, min_len, cur);
+      wave_load<S>(A, c, min_len, cur);
       for (uint64_t s = c; s < e; ++s) {
         SliceVals<S> nxt;
-        if (s + 1 < e) batch_load<S>(A, s + 1, min_len, nxt);
-        batch_slice<S>(A, s, cur, reg, rshift, oor);
+        if (s + 1 < e) wave_load<S>(A, s + 1, min_len, nxt);
+        wave_slice<S>(A, s, cur, reg, rshift, oor);
         cur = nxt;
       }
     }
-    batch_flush(A.out_count, A.out_sums, A.words, S.naggs, reg, rshift);
+    wave_flush(A.out_count, A.out_sums, A.words, S.naggs, reg, rshift);
   }
 }
 
@@ -167,14 +87,6 @@ __device__ __forceinline__ void dispatch_segment(int shape, const SpecArgs& A, u
   }
 }
 
-// Takes the next chunk index from the batch's counter (one atomic per wave).
-__device__ __forceinline__ uint64_t grab_chunk(unsigned long long* counter) {
-  unsigned long long k = 0;
-  if ((threadIdx.x & 63) == 0) k = atomicAdd(counter, 1ull);
-  return __builtin_amdgcn_readfirstlane((uint32_t)k) |
-         ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(k >> 32)) << 32);
-}
-
 // Persistent kernel: every wave takes chunks of `chunk` consecutive slices of
 // the flattened slice list from a device counter until none are left, so a
 // slow wave takes fewer chunks and all waves end together.  granule_q[g] is the
@@ -184,13 +96,13 @@ __global__ void __launch_bounds__(kBlock) k_batch(const BatchQuery* __restrict__
                                                   uint64_t total_slices,
                                                   const uint32_t* __restrict__ granule_q,
                                                   uint32_t chunk, unsigned long long* counter) {
-  __shared__ unsigned long long lreg[(kBlock / kWave) * kBatchWords];
+  __shared__ unsigned long long lreg[(kBlock / kWave) * kWaveWords];
   ConstQuery* qs = (ConstQuery*)plan;
   const __attribute__((address_space(4))) uint32_t* gq =
       (const __attribute__((address_space(4))) uint32_t*)granule_q;
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  unsigned long long* reg = lreg + wave * kBatchWords;
-  for (int w = lane; w < kBatchWords; w += 64) reg[w] = 0;
+  unsigned long long* reg = lreg + wave * kWaveWords;
+  for (int w = lane; w < kWaveWords; w += 64) reg[w] = 0;
   __builtin_amdgcn_wave_barrier();
   uint64_t k = grab_chunk(counter);
   while (k * chunk < total_slices) {
@@ -358,16 +270,14 @@ int32_t tgp_batch_set_queries(tgp_batch* b, const tgp_agg_query* qs, uint32_t n,
       for (int k = 0; k < kNumShapes; ++k)
         if (shape_eq(sh, kShapes[k])) idx = k;
     }
-    if (idx < 0 || sh.has_sel || bq.a.words > (uint32_t)kBatchWords) {
+    if (idx < 0 || sh.has_sel || wave_rshift(bq.a.words) < 0) {
       b->fallback.push_back((int32_t)i);
       continue;
     }
     bq.shape = idx;
     bq.words = (int32_t)bq.a.words;
     bq.naggs = qs[i].naggs;
-    bq.rshift = 6;
-    while (bq.rshift > 0 && ((uint64_t)bq.a.words << bq.rshift) > (uint64_t)kBatchWords)
-      --bq.rshift;
+    bq.rshift = wave_rshift(bq.a.words);
     bq.slice0 = slices;
     bq.a.out_count = reinterpret_cast<unsigned long long*>(b->d_results + b->count_off[i]);
     bq.a.out_sums = reinterpret_cast<unsigned long long*>(b->d_results + b->sums_off[i]);

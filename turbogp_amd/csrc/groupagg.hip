@@ -183,16 +183,19 @@ int32_t spec_shape_index(const tgp_agg_query* q);
 
 int32_t run_agg(tgp_ctx* ctx, const DevPlan& P, const PlanInfo& info, hipStream_t s,
                 const tgp_agg_query* q = nullptr, const tgp_group_result* out = nullptr) {
-  // Results are accumulated with atomics: clear them first (a fill kernel, so the
-  // sequence can be captured in a graph).
-  TGP_TRY(fill_u64(s, reinterpret_cast<uint64_t*>(P.out_count), P.ngroups, 0));
-  TGP_TRY(fill_u64(s, reinterpret_cast<uint64_t*>(P.out_sums), P.ngroups * P.naggs * 2, 0));
-  if (P.out_oor) TGP_TRY(fill_u64(s, reinterpret_cast<uint64_t*>(P.out_oor), 1, 0));
   if (q) {
+    // the specialised kernels zero the result themselves
     bool launched = false;
     TGP_TRY(try_spec(ctx, q, out, s, &launched));
     if (launched) return TGP_OK;
   }
+  // Results are accumulated with atomics: clear them first (one fill kernel, so
+  // the sequence can be captured in a graph).
+  uint64_t* p[3] = {reinterpret_cast<uint64_t*>(P.out_count),
+                    reinterpret_cast<uint64_t*>(P.out_sums),
+                    reinterpret_cast<uint64_t*>(P.out_oor)};
+  const uint64_t n[3] = {P.ngroups, P.ngroups * P.naggs * 2, 1};
+  TGP_TRY(zero_ranges(s, 3, p, n));
   switch (info.strategy) {
     case kAggReg1: return launch_tiles(ctx, k_agg_reg<1, TGP_MAX_AGGS>, P, info.lds_bytes, s);
     case kAggReg8: return launch_tiles(ctx, k_agg_reg<8, 4>, P, info.lds_bytes, s);

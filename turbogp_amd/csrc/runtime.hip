@@ -119,6 +119,37 @@ int32_t fill_u64(hipStream_t s, uint64_t* dst, uint64_t n, uint64_t value) {
   return TGP_OK;
 }
 
+struct ZeroRanges {
+  uint64_t* p[kMaxZeroRanges];
+  uint64_t n[kMaxZeroRanges];
+};
+
+__global__ void __launch_bounds__(256) k_zero_ranges(ZeroRanges r) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+#pragma unroll
+  for (int k = 0; k < kMaxZeroRanges; ++k)
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < r.n[k]; i += stride)
+      r.p[k][i] = 0;
+}
+
+int32_t zero_ranges(hipStream_t s, int count, uint64_t* const* ptrs, const uint64_t* n) {
+  TGP_REQUIRE(count >= 0 && count <= kMaxZeroRanges, "zero_ranges: %d ranges", count);
+  ZeroRanges r{};
+  uint64_t most = 0;
+  for (int k = 0; k < count; ++k) {
+    if (!ptrs[k] || !n[k]) continue;
+    r.p[k] = ptrs[k];
+    r.n[k] = n[k];
+    most = n[k] > most ? n[k] : most;
+  }
+  if (most == 0) return TGP_OK;
+  uint64_t blocks = (most + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  k_zero_ranges<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(r);
+  TGP_HIP(hipGetLastError());
+  return TGP_OK;
+}
+
 int persistent_grid(tgp_ctx* ctx, int wg_per_cu) {
   int cus = ctx->info.cu_count > 0 ? ctx->info.cu_count : 256;
   return cus * wg_per_cu;
@@ -292,6 +323,14 @@ int32_t tgp_tune_set(tgp_ctx* ctx, int32_t key, int64_t value) {
     case TGP_TUNE_WG_PER_CU: ctx->tune_wg_per_cu.store(value); return TGP_OK;
     case TGP_TUNE_GRID_MAX: ctx->tune_grid_max.store(value); return TGP_OK;
     case TGP_TUNE_SPECIALIZE: ctx->tune_no_spec.store(value ? 0 : 1); return TGP_OK;
+    case TGP_TUNE_SPEC_KERNEL:
+      TGP_REQUIRE(value <= 2, "TGP_TUNE_SPEC_KERNEL is 0, 1 or 2");
+      ctx->tune_spec_kernel.store(value);
+      return TGP_OK;
+    case TGP_TUNE_CHUNK:
+      TGP_REQUIRE(value <= 4096, "TGP_TUNE_CHUNK above 4096 slices");
+      ctx->tune_chunk.store(value);
+      return TGP_OK;
   }
   set_error("tgp_tune_set: unknown key %d", key);
   return TGP_ERR_INVALID;
@@ -303,6 +342,8 @@ int32_t tgp_tune_get(tgp_ctx* ctx, int32_t key, int64_t* value) {
     case TGP_TUNE_WG_PER_CU: *value = ctx->tune_wg_per_cu.load(); return TGP_OK;
     case TGP_TUNE_GRID_MAX: *value = ctx->tune_grid_max.load(); return TGP_OK;
     case TGP_TUNE_SPECIALIZE: *value = ctx->tune_no_spec.load() ? 0 : 1; return TGP_OK;
+    case TGP_TUNE_SPEC_KERNEL: *value = ctx->tune_spec_kernel.load(); return TGP_OK;
+    case TGP_TUNE_CHUNK: *value = ctx->tune_chunk.load(); return TGP_OK;
   }
   set_error("tgp_tune_get: unknown key %d", key);
   return TGP_ERR_INVALID;

@@ -1,6 +1,6 @@
 // Shape-specialised scan-aggregate kernel (one query per launch) and the
 // planner's shape extraction.  Design notes are in spec.h.
-#include "spec.h"
+#include "wave_agg.h"
 
 namespace tgp {
 
@@ -98,27 +98,118 @@ __global__ void __launch_bounds__(kBlock) k_spec(SpecArgs A) {
 }
 
 
+// Single-query kernel on the wave-level blocks of wave_agg.h.  With chunk == 0
+// wave w of the grid takes slices w, w + nwaves, ... (static interleave: all
+// waves sweep the table together, no atomics); with chunk > 0 the waves take
+// chunks of `chunk` slices from a device counter (a slow wave takes fewer).  A
+// wave accumulates in its replicated LDS histogram and flushes into the result
+// every 2^20 rows and at the end.  The LDS region of a wave is words << rshift
+// words (dynamic shared memory of 4 such regions per workgroup).
+template <Shape S>
+__device__ __forceinline__ void wave_rows(const SpecArgs& A, uint64_t s, uint64_t min_len,
+                                          unsigned long long* reg, uint32_t rshift,
+                                          uint32_t& oor, uint64_t& since) {
+  if (since + kSliceRows > kFlushRows) {
+    wave_flush(A.out_count, A.out_sums, A.words, S.naggs, reg, rshift);
+    since = 0;
+  }
+  SliceVals<S> v;
+  wave_load<S>(A, s, min_len, v);
+  wave_slice<S>(A, s, v, reg, rshift, oor);
+  since += kSliceRows;
+}
+
+template <Shape S>
+__global__ void __launch_bounds__(kBlock) k_wave(SpecArgs A, uint32_t rshift, uint32_t chunk,
+                                                 unsigned long long* counter) {
+  extern __shared__ __align__(16) unsigned long long lreg[];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t region = A.words << rshift;
+  unsigned long long* reg = lreg + wave * region;
+  for (uint32_t w = lane; w < region; w += 64) reg[w] = 0;
+  __builtin_amdgcn_wave_barrier();
+  uint64_t min_len = A.col_len[0];
+#pragma unroll
+  for (int s = 1; s < S.nslots; ++s) min_len = A.col_len[s] < min_len ? A.col_len[s] : min_len;
+  uint32_t oor = 0;
+  uint64_t since = 0;  // rows added to the region since its last flush
+  if (chunk == 0) {
+    const uint64_t nwaves = (uint64_t)gridDim.x * (kBlock / kWave);
+    for (uint64_t s = (uint64_t)blockIdx.x * (kBlock / kWave) + wave; s < A.nslices; s += nwaves)
+      wave_rows<S>(A, s, min_len, reg, rshift, oor, since);
+  } else {
+    uint64_t k = grab_chunk(counter);
+    while (k * chunk < A.nslices) {
+      const uint64_t c0 = k * chunk;
+      const uint64_t c1 = c0 + chunk < A.nslices ? c0 + chunk : A.nslices;
+      const uint64_t next = grab_chunk(counter);  // in flight while this chunk runs
+      for (uint64_t s = c0; s < c1; ++s) wave_rows<S>(A, s, min_len, reg, rshift, oor, since);
+      k = next;
+    }
+  }
+  if (since) wave_flush(A.out_count, A.out_sums, A.words, S.naggs, reg, rshift);
+  if (oor && A.out_oor) atomicAdd(A.out_oor, (unsigned long long)oor);
+}
+
+// Zeroes the result (and the chunk counter when given) in one launch.
+static int32_t zero_result(hipStream_t st, const SpecArgs& A, int naggs,
+                           unsigned long long* counter) {
+  uint64_t* p[4] = {reinterpret_cast<uint64_t*>(A.out_count),
+                    reinterpret_cast<uint64_t*>(A.out_sums),
+                    reinterpret_cast<uint64_t*>(A.out_oor), reinterpret_cast<uint64_t*>(counter)};
+  const uint64_t n[4] = {A.ngroups, A.ngroups * (uint64_t)naggs * 2, 1, 1};
+  return zero_ranges(st, 4, p, n);
+}
+
 template <int I>
 int32_t launch_shape(tgp_ctx* ctx, const Shape& s, const SpecArgs& A, hipStream_t st,
                      bool* launched) {
   if constexpr (I < kNumShapes) {
     if (shape_eq(s, kShapes[I])) {
       constexpr Shape S = kShapes[I];
-      auto kern = k_spec<S>;
-      uint32_t lds = S.table == 0 ? (4 * kHistWords + 4 * 64 * 2) * 8
-                                  : (uint32_t)(A.ngroups * (1 + 2 * S.naggs) * 8);
+      // kernel form: the replicated-histogram waves where 2..4 groups would make
+      // the lanes of one LDS atomic collide; the register fold otherwise
+      const int rshift = wave_rshift(A.words);
+      const int64_t form = ctx->tune_spec_kernel.load();
+      const bool wave = rshift >= 0 && (form == 2 || (form == 0 && A.ngroups >= 2 &&
+                                                       A.ngroups <= 4));
       int per_cu = 0;
-      TGP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, lds));
+      uint32_t lds = 0;
+      if (wave) {
+        lds = (uint32_t)((A.words << rshift) * (kBlock / kWave) * 8);
+        TGP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_wave<S>, kBlock, lds));
+      } else {
+        lds = S.table == 0 ? (4 * kHistWords + 4 * 64 * 2) * 8
+                           : (uint32_t)(A.ngroups * (1 + 2 * S.naggs) * 8);
+        TGP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_spec<S>, kBlock, lds));
+      }
       if (per_cu < 1) per_cu = 1;
       const int64_t want = ctx->tune_wg_per_cu.load();
       if (want > 0 && want < per_cu) per_cu = (int)want;
       uint64_t grid = (uint64_t)persistent_grid(ctx, per_cu);
-      const uint64_t need = (A.nslices + 3) / 4;
-      if (grid > need) grid = need;
       const int64_t gmax = ctx->tune_grid_max.load();
-      if (gmax > 0 && grid > (uint64_t)gmax) grid = (uint64_t)gmax;
-      if (grid == 0) grid = 1;
-      kern<<<dim3((unsigned)grid), dim3(kBlock), lds, st>>>(A);
+      if (wave) {
+        // 0: static interleave (default); > 0: slices per dynamically taken chunk
+        const uint64_t chunk = (uint64_t)ctx->tune_chunk.load();
+        const uint64_t per_wg = 4 * (chunk ? chunk : 1);
+        const uint64_t need = (A.nslices + per_wg - 1) / per_wg;
+        if (grid > need) grid = need;
+        if (gmax > 0 && grid > (uint64_t)gmax) grid = (uint64_t)gmax;
+        if (grid == 0) grid = 1;
+        void* counter = nullptr;
+        TGP_TRY(get_scratch(ctx, st, 256, &counter));
+        auto* cnt = static_cast<unsigned long long*>(counter);
+        TGP_TRY(zero_result(st, A, S.naggs, cnt));
+        k_wave<S><<<dim3((unsigned)grid), dim3(kBlock), lds, st>>>(A, (uint32_t)rshift,
+                                                                  (uint32_t)chunk, cnt);
+      } else {
+        const uint64_t need = (A.nslices + 3) / 4;
+        if (grid > need) grid = need;
+        if (gmax > 0 && grid > (uint64_t)gmax) grid = (uint64_t)gmax;
+        if (grid == 0) grid = 1;
+        TGP_TRY(zero_result(st, A, S.naggs, nullptr));
+        k_spec<S><<<dim3((unsigned)grid), dim3(kBlock), lds, st>>>(A);
+      }
       TGP_HIP(hipGetLastError());
       *launched = true;
       return TGP_OK;
@@ -343,8 +434,8 @@ int32_t spec_shape_index(const tgp_agg_query* q) {
   return -1;
 }
 
-// Launches the specialised kernel for q when one is compiled; *launched tells.
-// The result buffers must already be zeroed.
+// Launches the specialised kernel for q when one is compiled (after zeroing the
+// result buffers in the same stream); *launched tells.
 int32_t try_spec(tgp_ctx* ctx, const tgp_agg_query* q, const tgp_group_result* out,
                  hipStream_t st, bool* launched) {
   *launched = false;
@@ -356,6 +447,7 @@ int32_t try_spec(tgp_ctx* ctx, const tgp_agg_query* q, const tgp_group_result* o
   A.out_sums = reinterpret_cast<unsigned long long*>(out->d_sums);
   A.out_oor = reinterpret_cast<unsigned long long*>(out->d_n_out_of_range);
   if (A.nslices == 0) {
+    TGP_TRY(zero_result(st, A, q->naggs, nullptr));
     *launched = true;
     return TGP_OK;
   }

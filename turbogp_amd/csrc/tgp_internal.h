@@ -82,6 +82,8 @@ struct tgp_ctx {
   std::atomic<int64_t> tune_wg_per_cu{0};
   std::atomic<int64_t> tune_grid_max{0};
   std::atomic<int64_t> tune_no_spec{0};
+  std::atomic<int64_t> tune_spec_kernel{0};  // 0: by group count; 1: register fold; 2: replicated
+  std::atomic<int64_t> tune_chunk{0};        // 0: static interleave; n: chunks of n slices
 };
 
 namespace tgp {
@@ -99,6 +101,10 @@ int32_t hbm_alloc(tgp_ctx* ctx, uint64_t bytes, void** out);
 int32_t hbm_free(tgp_ctx* ctx, void* p);
 // Device fill helper (kernel based, graph safe).
 int32_t fill_u64(hipStream_t s, uint64_t* dst, uint64_t n, uint64_t value);
+// Zeroes up to kMaxZeroRanges word ranges with one kernel launch (graph safe);
+// null or empty ranges are skipped.
+constexpr int kMaxZeroRanges = 4;
+int32_t zero_ranges(hipStream_t s, int count, uint64_t* const* ptrs, const uint64_t* n);
 // Number of workgroups that fill the device for a persistent streaming kernel.
 int persistent_grid(tgp_ctx* ctx, int wg_per_cu);
 // Validates a column descriptor (non-null data, known dtype, 16-byte alignment).

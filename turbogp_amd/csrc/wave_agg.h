@@ -1,0 +1,118 @@
+// Wave-level scan-aggregate building blocks shared by the batched executor
+// (batch.hip) and the single-query kernels (spec.hip).
+//
+// A wave works on 256-row slices (spec.h) of one query and accumulates COUNT(*)
+// and the 64-bit partial sums per group in a wave-private LDS region, updated
+// with 64-bit LDS atomics.  The region holds R = 2^rshift copies of the
+// query's histogram (as many as fit in kWaveWords words, at most 64) and lane l
+// updates copy l mod R, so that few lanes of one atomic instruction hit the
+// same word even when the query has two or three groups.  The region is flushed
+// into the query's exact int128 result with device atomics at least every
+// kFlushRows rows: a copy then holds partial sums of at most 2^20 values of
+// magnitude < 2^43, so the 64-bit sum of all copies is exact.  A slice with a
+// larger value adds its values straight into the int128 result instead.
+#pragma once
+
+#include "spec.h"
+
+namespace tgp {
+
+constexpr int kWaveWords = 1024;             // per-wave region: R copies of groups * (1 + naggs)
+constexpr uint64_t kFlushRows = 1ull << 20;  // rows between flushes
+constexpr int kWaveBigBits = 43;             // |value| < 2^43 on the LDS path
+
+// log2 of the number of histogram copies for a query of `words` words
+// (groups * (1 + naggs)), or -1 when one copy does not fit.
+inline int wave_rshift(uint64_t words) {
+  if (words == 0 || words > (uint64_t)kWaveWords) return -1;
+  int r = 6;
+  while (r > 0 && (words << r) > (uint64_t)kWaveWords) --r;
+  return r;
+}
+
+__device__ __forceinline__ bool big43(uint64_t v) {
+  const int64_t s = (int64_t)v;
+  return (((uint64_t)(s ^ (s >> 63))) >> kWaveBigBits) != 0;
+}
+
+// One slice of a query of shape S into the wave-private region `reg`: the
+// values `v` of the slice starting at query-relative slice `slice` are loaded.
+template <Shape S>
+__device__ __forceinline__ void wave_load(const SpecArgs& A, uint64_t slice, uint64_t min_len,
+                                           SliceVals<S>& v) {
+  const uint64_t slice0 = A.base0 + slice * kSliceRows;
+  const uint64_t row = slice0 + (threadIdx.x & 63) * kSR;
+  load_all<S, 0>(A, row, slice0 + kSliceRows <= min_len, v);
+}
+
+template <Shape S>
+__device__ __forceinline__ void wave_slice(const SpecArgs& A, uint64_t slice,
+                                            const SliceVals<S>& v, unsigned long long* reg,
+                                            uint32_t rshift, uint32_t& oor) {
+  const uint32_t lane = threadIdx.x & 63;
+  // this lane's copy of the histogram: word w at reg[(w << rshift) + copy]
+  unsigned long long* mine = reg + (lane & ((1u << rshift) - 1));
+  const uint64_t row = A.base0 + slice * kSliceRows + lane * kSR;
+  int32_t gid[kSR];
+  const uint32_t sel = eval_slice<S>(A, row, v, gid, oor);
+  if (!__ballot(sel != 0)) return;
+  constexpr int W1 = 1 + S.naggs;
+#pragma unroll
+  for (int r = 0; r < kSR; ++r)
+    if ((sel >> r) & 1) atomicAdd(&mine[(uint32_t)(gid[r] * W1) << rshift], 1ull);
+  static_for<S.naggs>([&](auto AI) {
+    constexpr int a = decltype(AI)::value;
+    uint64_t val[kSR];
+    agg_values<S, a>(A, v, val);
+    bool b = false;
+#pragma unroll
+    for (int r = 0; r < kSR; ++r) b |= ((sel >> r) & 1) && big43(val[r]);
+    if (__ballot(b)) {
+#pragma unroll
+      for (int r = 0; r < kSR; ++r)
+        if ((sel >> r) & 1)
+          atomic_add128_i64(&A.out_sums[((uint64_t)gid[r] * S.naggs + a) * 2], (int64_t)val[r]);
+    } else {
+#pragma unroll
+      for (int r = 0; r < kSR; ++r)
+        if ((sel >> r) & 1)
+          atomicAdd(&mine[(uint32_t)(gid[r] * W1 + 1 + a) << rshift], (unsigned long long)val[r]);
+    }
+  });
+}
+
+// Adds the wave region (`words` words, groups x (count, naggs sums), in 2^rshift
+// copies) into the query's exact result and clears it.  Every copy's partials
+// come from at most 2^20 values of magnitude < 2^43, so the 64-bit sum of the
+// copies is exact.
+__device__ __forceinline__ void wave_flush(unsigned long long* out_count,
+                                            unsigned long long* out_sums, uint32_t words,
+                                            int naggs, unsigned long long* reg, uint32_t rshift) {
+  const uint32_t lane = threadIdx.x & 63;
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+  const uint32_t W1 = 1 + naggs, R = 1u << rshift;
+  for (uint32_t w = lane; w < words; w += 64) {
+    unsigned long long x = 0;
+    for (uint32_t c = 0; c < R; ++c) {
+      x += reg[(w << rshift) + c];
+      reg[(w << rshift) + c] = 0;
+    }
+    if (!x) continue;
+    const uint32_t g = w / W1, k = w % W1;
+    if (k == 0) atomicAdd(&out_count[g], x);
+    else atomic_add128(&out_sums[((uint64_t)g * naggs + (k - 1)) * 2], x,
+                       (int64_t)x < 0 ? -1 : 0);
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Takes the next chunk index from a device counter (one atomic per wave).
+__device__ __forceinline__ uint64_t grab_chunk(unsigned long long* counter) {
+  unsigned long long k = 0;
+  if ((threadIdx.x & 63) == 0) k = atomicAdd(counter, 1ull);
+  return __builtin_amdgcn_readfirstlane((uint32_t)k) |
+         ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(k >> 32)) << 32);
+}
+
+}  // namespace tgp
