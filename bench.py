@@ -1,16 +1,32 @@
 #!/usr/bin/env python3
 """Benchmark of the scan / group-aggregate path on one MI355X.
 
-Workload: the 60M-row fact table resident in HBM, run as two queries per step
-  (a) Q1-shape: WHERE dt <= 2435 GROUP BY flaga, flagb -> COUNT(*), SUM(qty),
-      SUM(price), SUM(price*(100-disc)), SUM(price*(100-disc)*(100+tax))
-  (b) Q6-shape: WHERE 731 <= dt < 1096 AND disc BETWEEN 5 AND 7 AND qty < 24
-      -> SUM(price*disc)
-value = algorithmic bytes of both queries (rows x native widths of the referenced
-columns: 38 B/row and 28 B/row) / timed seconds.  The dominant kernel (query a)
-is timed with hipEvents on the stream it runs on for the roofline.
+Headline workload (one step = one batch): 4096 filter + GROUP BY queries over a
+256M-row fact table resident in HBM (row ranges log-uniform in 1K..4M rows,
+date cutoffs uniform, group keys drawn from {flagA, flagB, flagA x flagB,
+date/30}; each query computes COUNT(*), SUM(qty), SUM(price),
+SUM(price*(100-disc)), SUM(price*(100-disc)*(100+tax)) exactly).  The batch runs
+as one persistent-kernel launch replayed from a hipGraph, with launch settings
+from the batch autotuner.  value = queries per second of the batched form.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--rows N]
+Also measured in the same run:
+  - the one-launch-per-query form of the same 4096 queries (tgp_group_agg per
+    query, launches queued back to back on one stream), its queries/s and
+    per-query completion latencies (p50/p99), and the speedup of the batch;
+  - the 60M-row scan-aggregate queries over the table's first 60M rows:
+    (a) Q1-shape WHERE dt <= 2435 GROUP BY flagA, flagB (6 groups), and
+    (b) Q6-shape WHERE 731 <= dt < 1096 AND disc BETWEEN 5 AND 7 AND qty < 24
+        -> SUM(price*disc), in GB/s of algorithmic bytes and as a fraction of
+        HBM peak;
+  - the oracle's restatement of the reference CPU path on a bounded sample of
+    the batch (cpu_baseline).
+
+Algorithmic bytes of a query: rows in its range x native widths of the columns
+it reads (date 4 B + group-key columns 1 B each + four 8-byte summed columns).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+       python bench.py --calibrate   (known-size kernels for scaling PMC byte counters:
+                                      k_fill_u64 writes and k_minmax reads exactly 512 MiB)
 """
 import argparse
 import json
@@ -19,54 +35,90 @@ import statistics
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 METRIC = "batched queries/s + p99 vs per-query launches; scan GB/s as % of HBM peak"
+SCAN_ROWS = 60_000_000
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(rows, reps=3):
+def pct(xs, p):
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), p))
+
+
+def cpu_baseline(params, rows_cap, reps=3):
     """The oracle's restatement of the reference CPU path (filter passes ->
-    bitmap -> compaction -> chunked hash grouping -> f64 sum_vec), all cores,
-    on a bounded sample of the same workload."""
+    bitmap -> compaction -> chunked hash grouping -> f64 sum_vec), all host
+    cores, on the first batch queries up to `rows_cap` rows; reported as the
+    queries/s the same per-row rate gives for the whole batch."""
     from oracle import binding as B
     from turbogp_amd import workloads as W
-    cols = W.host_fact_columns(rows, B.synth_uniform)
-    n1, p1, g1, a1 = W.q1_spec()
-    n6, p6, g6, a6 = W.q6_spec()
-    q1 = B.make_query([cols[c] for c in n1], p1, g1, a1)
-    q6 = B.make_query([cols[c] for c in n6], p6, g6, a6)
+    sample, rows = [], 0
+    for p in params:
+        if rows + (p[1] - p[0]) > rows_cap and sample:
+            break
+        sample.append(p)
+        rows += p[1] - p[0]
     threads = os.cpu_count() or 1
+    # host columns of each sampled range, generated by the same counter-based generator
+    queries = []
+    for (b, e, cutoff, g) in sample:
+        names, preds, parts, aggs = W.q1_spec(date_cutoff=cutoff, group=g)
+        cols = [B.synth_uniform(W.FACT_SCHEMA[c][1], W.SEED, W.FACT_SCHEMA[c][0],
+                                W.FACT_SCHEMA[c][2], W.FACT_SCHEMA[c][3], b, e) for c in names]
+        queries.append(B.make_query(cols, preds, parts, aggs))
     times = []
     for _ in range(reps):
         t0 = time.perf_counter()
-        B.group_agg(q1, mode=1, nthreads=threads)
-        B.group_agg(q6, mode=1, nthreads=threads)
+        for q in queries:
+            B.group_agg(q, mode=1, nthreads=threads)
         times.append(time.perf_counter() - t0)
     t = statistics.median(times)
-    gbytes = rows * (W.Q1_BYTES_PER_ROW + W.Q6_BYTES_PER_ROW) / 1e9
-    return {"value": round(gbytes / t, 3), "unit": "GB/s", "cores": threads, "kind": "port",
-            "sample": f"Q1+Q6 over {rows} rows of the same table (u64-cell reference "
-                      f"algorithm, f64 sums), median of {reps}, {t*1e3:.1f} ms per pass; "
-                      f"oracle built -O3 -march=x86-64-v3 -fopenmp"}
+    total_rows = sum(p[1] - p[0] for p in params)
+    batch_s = t * total_rows / rows
+    return {"value": round(len(params) / batch_s, 2), "unit": "queries/s", "cores": threads,
+            "kind": "port",
+            "sample": f"first {len(sample)} of the {len(params)} batch queries ({rows} rows), "
+                      f"median of {reps}: {t:.3f} s, scaled by rows to the whole batch "
+                      f"({total_rows} rows, {batch_s:.1f} s); reference algorithm (u64 "
+                      f"cells, f64 sums) in the oracle, built -O3 -march=x86-64-v3 -fopenmp"}
 
 
-def measured_traffic(rows):
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE runs (calibrated), for the 60M-row workload."""
+def measured_traffic(name, rows):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs (scaled by known-size read and write kernels), for the workload they
+    were taken on."""
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
-            t = json.load(f)["k_spec_q1_60M"]
+            t = json.load(f)[name]
     except (OSError, KeyError, ValueError):
         return None
-    if rows != 60_000_000:
+    if t.get("rows") != rows:
         return None
     return t["fetch_bytes"] + t["write_bytes"]
+
+
+def calibrate():
+    """One 512 MiB write (k_fill_u64) and one 512 MiB read (k_minmax) for the
+    FETCH_SIZE / WRITE_SIZE scale factors."""
+    from turbogp_amd import Context, DeviceColumn, _capi as A
+    ctx = Context(0)
+    n = 1 << 26  # 64M int64 words = 512 MiB
+    col = DeviceColumn(ctx, n, A.I64)
+    col.buf.fill_u64(7)
+    lo, hi = col.minmax()
+    assert lo == hi == 7
+    col.free()
+    ctx.close()
+    print(json.dumps({"calibrate": {"write_bytes_k_fill_u64": n * 8,
+                                    "read_bytes_k_minmax": n * 8}}), flush=True)
 
 
 def main():
@@ -74,101 +126,153 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--rows", type=int, default=60_000_000)
-    ap.add_argument("--cpu-rows", type=int, default=20_000_000)
+    ap.add_argument("--per-query-reps", type=int, default=3)
+    ap.add_argument("--cpu-rows", type=int, default=40_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate", action="store_true")
     args = ap.parse_args()
+    if args.calibrate:
+        calibrate()
+        return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU design "
-              "is described in DESIGN.md and not run here.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for {args.gpus}): this build runs on "
+              "a single MI355X; the multi-GPU sharding is a design, not a measured path.",
+              file=sys.stderr)
         sys.exit(2)
 
-    from turbogp_amd import Context, Event, GroupAggBuffers
+    from turbogp_amd import Context, Event, GroupAggBuffers, _capi as A
     from turbogp_amd import workloads as W
+    from turbogp_amd.batch import Batch
 
     ctx = Context(0)
     log(f"device {ctx.info.name.decode()} {ctx.info.arch.decode()} CUs={ctx.info.cu_count}")
-    n = args.rows
+    n = W.BATCH_TABLE_ROWS
     table = W.make_fact_table(ctx, n)
     log(f"generated {n} rows on the device")
-    qa = W.build_query(table, W.q1_spec())
-    qb = W.build_query(table, W.q6_spec())
-    ra = GroupAggBuffers(ctx, 6, 4)
-    rb = GroupAggBuffers(ctx, 1, 1)
-    ev = [Event(ctx) for _ in range(3)]
 
-    def step(timed):
-        if timed:
-            ev[0].record()
-        ctx.group_agg_async(qa, ra)
-        if timed:
-            ev[1].record()
-        ctx.group_agg_async(qb, rb)
-        if timed:
-            ev[2].record()
-
+    # ---- batch of 4096 queries -------------------------------------------------
+    params = W.batch_query_params(W.BATCH_QUERIES, n)
+    queries = W.batch_queries(table, params)
+    batch_bytes = W.batch_bytes(params)
+    b = Batch(ctx, len(queries))
+    b.set_queries(queries)
+    info = b.info()
+    assert info["fallback"] == 0, info
+    tuned, tune_ms = b.autotune()
+    log(f"autotuned {tuned} ({tune_ms:.3f} ms)")
+    ev = [Event(ctx) for _ in range(2)]
     for _ in range(args.warmup):
-        step(False)
+        b.run()
     ctx.sync()
-    ta, tb = [], []
+    step_ms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step(True)
-        ta.append(ev[0].elapsed_ms(ev[1]))
-        tb.append(ev[1].elapsed_ms(ev[2]))
+        ev[0].record()
+        b.run()
+        ev[1].record()
+        step_ms.append(ev[0].elapsed_ms(ev[1]))
     ctx.sync()
     wall = time.perf_counter() - t0
-    log(f"timed {args.steps} steps in {wall:.3f} s")
+    batch_results = b.results()
+    ms_batch = statistics.mean(step_ms)
+    log(f"batch: {args.steps} steps, mean {ms_batch:.3f} ms (wall {wall:.3f} s)")
 
-    # results sanity: every selected row lands in a group; Q1 selects ~96.4 %
+    # the one-launch-per-query form of the same queries
+    pq_tot, pq_lat = [], None
+    for _ in range(args.per_query_reps):
+        tot, per = b.run_timed(A.BATCH_PER_QUERY)
+        pq_tot.append(tot)
+        pq_lat = per if pq_lat is None or tot <= min(pq_tot) else pq_lat
+    per_query_results = b.results()
+    same = all(np.array_equal(x.count, y.count) and np.array_equal(x.sums_words, y.sums_words)
+               and x.n_out_of_range == y.n_out_of_range
+               for x, y in zip(batch_results, per_query_results))
+    assert same, "batched and per-query results differ"
+    ms_pq = min(pq_tot)
+    b.close()
+    log(f"per-query: {ms_pq:.3f} ms; speedup {ms_pq / ms_batch:.2f}x")
+
+    # ---- 60M-row scan-aggregate queries (Q1 and Q6 shapes) ------------------------
+    qa = W.build_query(table, W.q1_spec(), 0, SCAN_ROWS)
+    qb = W.build_query(table, W.q6_spec(), 0, SCAN_ROWS)
+    ra = GroupAggBuffers(ctx, 6, 4)
+    rb = GroupAggBuffers(ctx, 1, 1)
+    ev3 = [Event(ctx) for _ in range(3)]
+    for _ in range(args.warmup):
+        ctx.group_agg_async(qa, ra)
+        ctx.group_agg_async(qb, rb)
+    ctx.sync()
+    ta, tb = [], []
+    for _ in range(args.steps):
+        ev3[0].record()
+        ctx.group_agg_async(qa, ra)
+        ev3[1].record()
+        ctx.group_agg_async(qb, rb)
+        ev3[2].record()
+        ta.append(ev3[0].elapsed_ms(ev3[1]))
+        tb.append(ev3[1].elapsed_ms(ev3[2]))
     res_a = ra.fetch()
-    res_b = rb.fetch()
     sel_a = int(res_a.count.sum())
-    assert 0.95 * n < sel_a < 0.975 * n, sel_a
-    assert res_a.n_out_of_range == 0
+    assert 0.95 * SCAN_ROWS < sel_a < 0.975 * SCAN_ROWS and res_a.n_out_of_range == 0, sel_a
+    res_b = rb.fetch()
+    ms_a, ms_b = statistics.mean(ta), statistics.mean(tb)
+    gbs_a = SCAN_ROWS * W.Q1_BYTES_PER_ROW / 1e9 / (ms_a / 1e3)
+    gbs_b = SCAN_ROWS * W.Q6_BYTES_PER_ROW / 1e9 / (ms_b / 1e3)
+    for e in ev + ev3:
+        e.free()
 
-    bytes_a = n * W.Q1_BYTES_PER_ROW
-    bytes_b = n * W.Q6_BYTES_PER_ROW
-    ms_a = statistics.mean(ta)
-    ms_b = statistics.mean(tb)
-    ms_step = ms_a + ms_b
-    value = (bytes_a + bytes_b) / 1e9 / (ms_step / 1e3)
-    achieved = bytes_a / 1e9 / (ms_a / 1e3)
+    achieved = batch_bytes / 1e9 / (ms_batch / 1e3)
+    nq = len(queries)
     out = {
         "metric": METRIC,
-        "value": round(value, 2),
-        "unit": "GB/s",
+        "value": round(nq / (ms_batch / 1e3), 1),
+        "unit": "queries/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": round(ms_batch, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int64/int128",
         "data": "synthetic",
-        "config": {"workload": "fact table 60M rows resident in HBM: (a) Q1-shape filter + "
-                               "6-group exact aggregate, (b) Q6-shape selective scan-sum",
-                   "rows": n, "bytes_per_row_a": W.Q1_BYTES_PER_ROW,
-                   "bytes_per_row_b": W.Q6_BYTES_PER_ROW},
-        "queries_per_s": round(2 / (ms_step / 1e3), 1),
-        "query_a": {"ms": round(ms_a, 4), "GB_s": round(achieved, 1),
-                    "hbm_frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "p99_ms": round(sorted(ta)[int(0.99 * (len(ta) - 1))], 4)},
-        "query_b": {"ms": round(ms_b, 4), "GB_s": round(bytes_b / 1e9 / (ms_b / 1e3), 1),
-                    "hbm_frac": round(bytes_b / 1e9 / (ms_b / 1e3) / HBM_PEAK_GBS, 4),
-                    "selected": int(res_b.count[0])},
+        "config": {"workload": "4096 filter+group-by queries over a 256M-row resident fact "
+                               "table (ranges log-uniform 1K..4M rows, uniform date cutoffs, "
+                               "group keys from {flagA, flagB, flagA x flagB, date/30}), one "
+                               "persistent-kernel hipGraph launch per batch",
+                   "table_rows": n, "queries": nq, "batch_rows": int(sum(e - s for s, e, _, _ in params)),
+                   "algorithmic_bytes_per_batch": int(batch_bytes), "tuning": tuned},
+        "batch": {"ms": round(ms_batch, 4), "GB_s": round(achieved, 1),
+                  "hbm_frac": round(achieved / HBM_PEAK_GBS, 4),
+                  "p50_latency_ms": round(pct(step_ms, 50), 4),
+                  "p99_latency_ms": round(pct(step_ms, 99), 4)},
+        "per_query_launches": {
+            "ms": round(ms_pq, 4), "queries_per_s": round(nq / (ms_pq / 1e3), 1),
+            "GB_s": round(batch_bytes / 1e9 / (ms_pq / 1e3), 1),
+            "p50_latency_ms": round(pct(pq_lat, 50), 4),
+            "p99_latency_ms": round(pct(pq_lat, 99), 4),
+            "latency_note": "completion time of each query after the 4096 launches were "
+                            "queued together; batched queries all complete at the batch end"},
+        "speedup_vs_per_query": round(ms_pq / ms_batch, 2),
+        "results_identical_batched_vs_per_query": bool(same),
+        "scan_60M": {
+            "q1_groupby": {"ms": round(ms_a, 4), "GB_s": round(gbs_a, 1),
+                           "hbm_frac": round(gbs_a / HBM_PEAK_GBS, 4),
+                           "bytes_per_row": W.Q1_BYTES_PER_ROW},
+            "q6_scan_sum": {"ms": round(ms_b, 4), "GB_s": round(gbs_b, 1),
+                            "hbm_frac": round(gbs_b / HBM_PEAK_GBS, 4),
+                            "bytes_per_row": W.Q6_BYTES_PER_ROW,
+                            "selected": int(res_b.count[0])}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(n),
-                     "kernel": "k_spec<Q1 shape> (query a), algorithmic 38 B/row x rows per "
-                               "launch"},
+                     "traffic": measured_traffic("k_batch_4096q_256M", n),
+                     "kernel": "k_batch (persistent batch kernel): algorithmic bytes of the "
+                               "4096 queries per launch / mean launch time (hipEvents on the "
+                               "batch stream, graph replay incl. the result-zeroing fill)"},
     }
     if not args.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(args.cpu_rows)
-    for e in ev:
-        e.free()
+        out["cpu_baseline"] = cpu_baseline(params, args.cpu_rows)
     ctx.close()
     print(json.dumps(out), flush=True)
 

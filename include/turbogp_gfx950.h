@@ -193,9 +193,12 @@ enum tgp_tune_key {
   TGP_TUNE_SPEC_KERNEL = 3, /* specialised kernel form: 1: a shared per-wave histogram
                                folded into registers after every slice; 2: waves with
                                replicated LDS histograms (where the groups fit); 0
-                               (default): 2 for queries of 2..4 groups, else 1 */
-  TGP_TUNE_CHUNK = 4        /* form 0's schedule: 0 (default) static interleave of slices;
+                               (default): 2 for queries of 2..4 groups and >= 4M rows,
+                               else 1 */
+  TGP_TUNE_CHUNK = 4,       /* form 2's schedule: 0 (default) static interleave of slices;
                                n > 0: chunks of n slices (<= 4096) taken from a counter */
+  TGP_TUNE_SLICES_PER_WAVE = 5 /* fewest 256-row slices per wave of the specialised
+                                  kernels before their grid shrinks (0 = default 2) */
 };
 int32_t tgp_tune_set(tgp_ctx* ctx, int32_t key, int64_t value);
 int32_t tgp_tune_get(tgp_ctx* ctx, int32_t key, int64_t* value);
@@ -281,20 +284,24 @@ int32_t tgp_batch_destroy(tgp_batch* b);
 /* Plans and uploads the queries (synchronous); invalidates the captured graph. */
 int32_t tgp_batch_set_queries(tgp_batch* b, const tgp_agg_query* qs, uint32_t n, void* stream);
 /* Launch settings of the persistent kernel.  wg_per_cu: workgroups per CU (0 =
- * as many as fit); pipeline: 256-row slices in flight per wave (1 or 2; 0 =
- * default 1); chunk_slices: slices per dynamically scheduled chunk (0 = default
- * 32); use_graph: replay a captured hipGraph (0 = plain launches). */
+ * as many as fit); chunk_slices: 256-row slices per dynamically scheduled unit
+ * (0 = default 32); order: 0 = units in query order (the slices of query 0,
+ * then of query 1, ...), 1 = units sorted by table position, so the queries
+ * that cover a table region read it at about the same time; cached_loads: 1 =
+ * keep column lines in the caches (pays off with order 1), 0 = streaming loads;
+ * use_graph: replay a captured hipGraph (0 = plain launches). */
 typedef struct tgp_batch_tuning {
   uint32_t wg_per_cu;
-  int32_t pipeline;
   uint32_t chunk_slices;
+  int32_t order;
+  int32_t cached_loads;
   int32_t use_graph;
 } tgp_batch_tuning;
 int32_t tgp_batch_set_tuning(tgp_batch* b, const tgp_batch_tuning* t);
 int32_t tgp_batch_get_tuning(tgp_batch* b, tgp_batch_tuning* t);
 /* The batch's autotuner: times the persistent launch on the current queries
- * over chunk sizes, workgroups per CU (up to the occupancy limit) and pipeline
- * depths, keeps the fastest settings and returns them. */
+ * over schedules (order, cached loads), chunk sizes and workgroups per CU, keeps
+ * the fastest settings and returns them. */
 int32_t tgp_batch_autotune(tgp_batch* b, void* stream, tgp_batch_tuning* best, float* best_ms);
 int32_t tgp_batch_run(tgp_batch* b, int32_t mode, void* stream);
 /* tgp_batch_run, synchronous, with device timing: *h_total_ms for the whole

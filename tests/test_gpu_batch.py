@@ -60,15 +60,19 @@ def test_batch_matches_single_queries_and_oracle(ctx, table):
         b.set_tuning(0, use_graph=False)
         b.run(A.BATCH_PERSISTENT)
         plain = b.results()
-        b.set_tuning(3, use_graph=True, pipeline=2, chunk_slices=5)
+        b.set_tuning(3, use_graph=True, chunk_slices=5, order=1, cached_loads=True)
         b.run(A.BATCH_PERSISTENT)
         pipe1 = b.results()
+        b.set_tuning(0, use_graph=True, chunk_slices=1, order=1, cached_loads=False)
+        b.run(A.BATCH_PERSISTENT)
+        table1 = b.results()
     finally:
         b.close()
     for i in range(len(qs)):
         assert same(persistent[i], per_query[i]), i
         assert same(persistent[i], replay[i]) and same(persistent[i], replay2[i]), i
         assert same(persistent[i], plain[i]) and same(persistent[i], pipe1[i]), i
+        assert same(persistent[i], table1[i]), i
     for i in range(40):
         assert same(persistent[i], single[i]), i
     # oracle on a subset (all edge queries included)
@@ -91,8 +95,8 @@ def test_batch_flush_every_2_20_rows_with_a_tiny_grid(ctx, table):
         b.set_queries(qs)
         ctx.tune(A.TUNE_GRID_MAX, 2)   # 8 waves: each wave adds millions of rows
         got = []
-        for pipe, chunk in ((1, 65536), (2, 7000)):
-            b.set_tuning(0, use_graph=False, pipeline=pipe, chunk_slices=chunk)
+        for order, chunk in ((0, 65536), (1, 7000)):
+            b.set_tuning(0, use_graph=False, chunk_slices=chunk, order=order)
             b.run(A.BATCH_PERSISTENT)
             got.append(b.results())
     finally:
@@ -113,7 +117,7 @@ def test_batch_autotune_keeps_results(ctx, table):
         b.run()
         before = b.results()
         best, ms = b.autotune()
-        assert 2 <= best["wg_per_cu"] <= 8 and best["pipeline"] in (1, 2) and ms > 0
+        assert 0 <= best["wg_per_cu"] <= 4 and best["order"] in (0, 1) and ms > 0
         assert best["chunk_slices"] in (8, 16, 32, 64, 128) and b.tuning() == best
         b.run()
         after = b.results()

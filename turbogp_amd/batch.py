@@ -22,15 +22,20 @@ class Batch:
         self.n = len(queries)
         self.naggs = [q.naggs for q in queries]
 
-    def set_tuning(self, wg_per_cu=0, use_graph=True, pipeline=0, chunk_slices=0):
-        t = A.BatchTuning(wg_per_cu, pipeline, chunk_slices, int(use_graph))
+    def set_tuning(self, wg_per_cu=0, use_graph=True, chunk_slices=0, order=0,
+                   cached_loads=False):
+        t = A.BatchTuning(wg_per_cu, chunk_slices, order, int(cached_loads), int(use_graph))
         A.check(A.lib().tgp_batch_set_tuning(self.handle, C.byref(t)), "tgp_batch_set_tuning")
+
+    @staticmethod
+    def _tuning_dict(t):
+        return {"wg_per_cu": t.wg_per_cu, "chunk_slices": t.chunk_slices, "order": t.order,
+                "cached_loads": bool(t.cached_loads), "use_graph": bool(t.use_graph)}
 
     def tuning(self):
         t = A.BatchTuning()
         A.check(A.lib().tgp_batch_get_tuning(self.handle, C.byref(t)), "tgp_batch_get_tuning")
-        return {"wg_per_cu": t.wg_per_cu, "pipeline": t.pipeline,
-                "chunk_slices": t.chunk_slices, "use_graph": bool(t.use_graph)}
+        return self._tuning_dict(t)
 
     def autotune(self, stream=None):
         """Times the launch settings on the current queries and keeps the fastest.
@@ -38,8 +43,7 @@ class Batch:
         t, ms = A.BatchTuning(), C.c_float()
         A.check(A.lib().tgp_batch_autotune(self.handle, stream, C.byref(t), C.byref(ms)),
                 "tgp_batch_autotune")
-        return {"wg_per_cu": t.wg_per_cu, "pipeline": t.pipeline,
-                "chunk_slices": t.chunk_slices, "use_graph": bool(t.use_graph)}, ms.value
+        return self._tuning_dict(t), ms.value
 
     def run(self, mode=A.BATCH_PERSISTENT, stream=None):
         A.check(A.lib().tgp_batch_run(self.handle, mode, stream), "tgp_batch_run")

@@ -18,6 +18,10 @@
 // The one-launch-per-query form (tgp_group_agg per query) is kept as the
 // baseline mode.  The reference has no batched executor; its per-query entry is
 // QueryEngine::execute (engine/mod.rs:1487).
+#include <algorithm>
+#include <utility>
+#include <vector>
+
 #include "wave_agg.h"
 
 namespace tgp {
@@ -39,10 +43,10 @@ struct BatchQuery {
 typedef const __attribute__((address_space(4))) BatchQuery ConstQuery;
 
 // Slices [s_begin, s_end) (query-relative) of one query of shape S, with a
-// flush every kFlushRows rows and at the end.  PIPE = 2 issues the loads of the
-// next slice before it processes the current one (two slices in flight per
-// wave, more registers); PIPE = 1 loads and processes one slice at a time.
-template <Shape S, int PIPE>
+// flush every kFlushRows rows and at the end.  NT: streaming loads (each row is
+// read once) or cached loads (the table-ordered schedule, where the queries
+// that cover a region read it at about the same time).
+template <Shape S, bool NT>
 __device__ __forceinline__ void batch_segment(const SpecArgs& A, uint64_t s_begin,
                                               uint64_t s_end, unsigned long long* reg,
                                               uint32_t rshift, uint32_t& oor) {
@@ -52,46 +56,46 @@ __device__ __forceinline__ void batch_segment(const SpecArgs& A, uint64_t s_begi
   constexpr uint64_t kFlushSlices = kFlushRows / kSliceRows;
   for (uint64_t c = s_begin; c < s_end; c += kFlushSlices) {
     const uint64_t e = c + kFlushSlices < s_end ? c + kFlushSlices : s_end;
-    if constexpr (PIPE == 1) {
-      for (uint64_t s = c; s < e; ++s) {
-        SliceVals<S> v;
-        wave_load<S>(A, s, min_len, v);
-        wave_slice<S>(A, s, v, reg, rshift, oor);
-      }
-    } else {
-      SliceVals<S> cur;
-      wave_load<S>(A, c, min_len, cur);
-      for (uint64_t s = c; s < e; ++s) {
-        SliceVals<S> nxt;
-        if (s + 1 < e) wave_load<S>(A, s + 1, min_len, nxt);
-        wave_slice<S>(A, s, cur, reg, rshift, oor);
-        cur = nxt;
-      }
+    for (uint64_t s = c; s < e; ++s) {
+      SliceVals<S> v;
+      wave_load<S, NT>(A, s, min_len, v);
+      wave_slice<S>(A, s, v, reg, rshift, oor);
     }
     wave_flush(A.out_count, A.out_sums, A.words, S.naggs, reg, rshift);
   }
 }
 
-template <int I, int PIPE>
+template <int I, bool NT>
 __device__ __forceinline__ void dispatch_segment(int shape, const SpecArgs& A, uint64_t s_begin,
                                                  uint64_t s_end, unsigned long long* reg,
                                                  uint32_t rshift, uint32_t& oor) {
   if constexpr (I < kNumShapes) {
     if constexpr (kShapes[I].nslots > 0 && kShapes[I].has_sel == 0) {
       if (shape == I) {
-        batch_segment<kShapes[I], PIPE>(A, s_begin, s_end, reg, rshift, oor);
+        batch_segment<kShapes[I], NT>(A, s_begin, s_end, reg, rshift, oor);
         return;
       }
     }
-    dispatch_segment<I + 1, PIPE>(shape, A, s_begin, s_end, reg, rshift, oor);
+    dispatch_segment<I + 1, NT>(shape, A, s_begin, s_end, reg, rshift, oor);
   }
 }
 
-// Persistent kernel: every wave takes chunks of `chunk` consecutive slices of
-// the flattened slice list from a device counter until none are left, so a
-// slow wave takes fewer chunks and all waves end together.  granule_q[g] is the
-// query that contains slice g * kGranSlices.
-template <int PIPE>
+// One query-relative slice range of query q.
+template <bool NT>
+__device__ __forceinline__ void batch_run_segment(ConstQuery* qs, uint32_t q, uint64_t b,
+                                                  uint64_t e, unsigned long long* reg) {
+  const SpecArgs& A = *(const SpecArgs*)&qs[q].a;
+  uint32_t oor = 0;
+  dispatch_segment<0, NT>(qs[q].shape, A, b, e, reg, (uint32_t)qs[q].rshift, oor);
+  if (oor) atomicAdd(A.out_oor, (unsigned long long)oor);
+}
+
+// Persistent kernel, query order: every wave takes chunks of `chunk`
+// consecutive slices of the flattened slice list (all slices of query 0, then
+// of query 1, ...) from a device counter until none are left, so a slow wave
+// takes fewer chunks and all waves end together.  granule_q[g] is the query
+// that contains flattened slice g * kGranSlices.
+template <bool NT>
 __global__ void __launch_bounds__(kBlock) k_batch(const BatchQuery* __restrict__ plan, uint32_t nq,
                                                   uint64_t total_slices,
                                                   const uint32_t* __restrict__ granule_q,
@@ -117,12 +121,37 @@ __global__ void __launch_bounds__(kBlock) k_batch(const BatchQuery* __restrict__
       if (q0 >= c1) break;
       const uint64_t q1 = q + 1 < nq ? qs[q + 1].slice0 : total_slices;
       const uint64_t b = c0 > q0 ? c0 : q0, e = c1 < q1 ? c1 : q1;
-      if (b >= e) continue;
-      const SpecArgs& A = *(const SpecArgs*)&qs[q].a;
-      uint32_t oor = 0;
-      dispatch_segment<0, PIPE>(qs[q].shape, A, b - q0, e - q0, reg, (uint32_t)qs[q].rshift, oor);
-      if (oor) atomicAdd(A.out_oor, (unsigned long long)oor);
+      if (b < e) batch_run_segment<NT>(qs, q, b - q0, e - q0, reg);
     }
+    k = next;
+  }
+}
+
+// Persistent kernel, table order: work item i = (query items[2i], first slice
+// items[2i+1]) covers `chunk` slices of that query; the items are sorted by the
+// table row they start at, so the waves sweep the table once and the queries
+// that cover a region read it within a short time of each other (from the
+// caches after the first).
+template <bool NT>
+__global__ void __launch_bounds__(kBlock) k_batch_items(const BatchQuery* __restrict__ plan,
+                                                        const uint32_t* __restrict__ items,
+                                                        uint64_t nitems, uint32_t chunk,
+                                                        unsigned long long* counter) {
+  __shared__ unsigned long long lreg[(kBlock / kWave) * kWaveWords];
+  ConstQuery* qs = (ConstQuery*)plan;
+  const __attribute__((address_space(4))) uint32_t* it =
+      (const __attribute__((address_space(4))) uint32_t*)items;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  unsigned long long* reg = lreg + wave * kWaveWords;
+  for (int w = lane; w < kWaveWords; w += 64) reg[w] = 0;
+  __builtin_amdgcn_wave_barrier();
+  uint64_t k = grab_chunk(counter);
+  while (k < nitems) {
+    const uint64_t next = grab_chunk(counter);
+    const uint32_t q = it[2 * k];
+    const uint64_t b = it[2 * k + 1];
+    const uint64_t n = qs[q].a.nslices;
+    batch_run_segment<NT>(qs, q, b, b + chunk < n ? b + chunk : n, reg);
     k = next;
   }
 }
@@ -146,8 +175,13 @@ struct tgp_batch {
   uint64_t plan_cap = 0, results_cap = 0;
   // tuning / graph
   uint32_t wg_per_cu = 0;  // 0 = occupancy
-  int32_t pipeline = 1;    // slices in flight per wave (1 or 2)
-  uint32_t chunk = 32;     // slices per dynamically scheduled chunk
+  uint32_t chunk = 32;     // slices per dynamically scheduled chunk / work item
+  int32_t order = 0;       // 0: query order; 1: table order (work items)
+  bool cached = false;     // cached instead of streaming column loads
+  uint32_t* d_items = nullptr;
+  uint64_t items_cap = 0, nitems = 0;
+  uint32_t items_chunk = 0;
+  bool items_valid = false;
   uint32_t* d_granule = nullptr;
   uint64_t granule_cap = 0;
   bool use_graph = true;
@@ -163,23 +197,72 @@ static void drop_graph(tgp_batch* b) {
   b->graph_def = nullptr;
 }
 
-template <int PIPE>
-static int32_t launch_variant(tgp_batch* b, hipStream_t s) {
-  auto kern = k_batch<PIPE>;
+template <typename K>
+static int32_t persistent_launch_grid(tgp_batch* b, K kern, uint64_t units, uint64_t* grid_out) {
   int per_cu = 0;
   TGP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0));
   if (per_cu < 1) per_cu = 1;
   if (b->wg_per_cu && (int)b->wg_per_cu < per_cu) per_cu = (int)b->wg_per_cu;
   uint64_t grid = (uint64_t)persistent_grid(b->ctx, per_cu);
-  const uint64_t need = (b->total_slices + (uint64_t)b->chunk * 4 - 1) / ((uint64_t)b->chunk * 4);
+  const uint64_t need = (units + (kBlock / kWave) - 1) / (kBlock / kWave);
   if (grid > need) grid = need;
   const int64_t gmax = b->ctx->tune_grid_max.load();
   if (gmax > 0 && grid > (uint64_t)gmax) grid = (uint64_t)gmax;
-  if (grid == 0) grid = 1;
-  kern<<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>(
-      b->d_plan, (uint32_t)b->plan.size(), b->total_slices, b->d_granule, b->chunk,
-      reinterpret_cast<unsigned long long*>(b->d_results + b->result_words));
+  *grid_out = grid == 0 ? 1 : grid;
+  return TGP_OK;
+}
+
+template <bool NT>
+static int32_t launch_variant(tgp_batch* b, hipStream_t s) {
+  auto* counter = reinterpret_cast<unsigned long long*>(b->d_results + b->result_words);
+  uint64_t grid = 0;
+  if (b->order == 0) {
+    auto kern = k_batch<NT>;
+    const uint64_t chunks = (b->total_slices + b->chunk - 1) / b->chunk;
+    TGP_TRY(persistent_launch_grid(b, kern, chunks, &grid));
+    kern<<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>(b->d_plan, (uint32_t)b->plan.size(),
+                                                       b->total_slices, b->d_granule, b->chunk,
+                                                       counter);
+  } else {
+    auto kern = k_batch_items<NT>;
+    TGP_TRY(persistent_launch_grid(b, kern, b->nitems, &grid));
+    kern<<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>(b->d_plan, b->d_items, b->nitems,
+                                                       b->chunk, counter);
+  }
   TGP_HIP(hipGetLastError());
+  return TGP_OK;
+}
+
+// Work items of the table-ordered schedule for the current chunk size: every
+// query's slices in chunks, sorted by the table row each chunk starts at
+// (synchronous upload; called outside graph capture).
+static int32_t build_items(tgp_batch* b, hipStream_t s) {
+  if (b->order == 0 || (b->items_chunk == b->chunk && b->items_valid)) return TGP_OK;
+  std::vector<std::pair<uint64_t, uint64_t>> keyed;  // (start row, query << 32 | first slice)
+  for (uint32_t q = 0; q < b->plan.size(); ++q) {
+    const SpecArgs& A = b->plan[q].a;
+    for (uint64_t c = 0; c < A.nslices; c += b->chunk)
+      keyed.emplace_back(A.base0 + c * kSliceRows, ((uint64_t)q << 32) | c);
+  }
+  std::sort(keyed.begin(), keyed.end());
+  std::vector<uint32_t> items(2 * (keyed.empty() ? 1 : keyed.size()), 0);
+  for (size_t i = 0; i < keyed.size(); ++i) {
+    items[2 * i] = (uint32_t)(keyed[i].second >> 32);
+    items[2 * i + 1] = (uint32_t)keyed[i].second;
+  }
+  if (b->items_cap < items.size()) {
+    if (b->d_items) TGP_TRY(hbm_free(b->ctx, b->d_items));
+    b->d_items = nullptr;
+    void* p = nullptr;
+    TGP_TRY(hbm_alloc(b->ctx, items.size() * 4, &p));
+    b->d_items = static_cast<uint32_t*>(p);
+    b->items_cap = items.size();
+  }
+  TGP_HIP(hipMemcpyAsync(b->d_items, items.data(), items.size() * 4, hipMemcpyHostToDevice, s));
+  TGP_HIP(hipStreamSynchronize(s));
+  b->nitems = keyed.size();
+  b->items_chunk = b->chunk;
+  b->items_valid = true;
   return TGP_OK;
 }
 
@@ -187,17 +270,7 @@ static int32_t batch_launch(tgp_batch* b, hipStream_t s) {
   // results and the chunk counter behind them start at zero
   TGP_TRY(fill_u64(s, b->d_results, b->result_words + 1, 0));
   if (!b->total_slices) return TGP_OK;
-  return b->pipeline == 1 ? launch_variant<1>(b, s) : launch_variant<2>(b, s);
-}
-
-// Resident workgroups per CU of a kernel variant (its occupancy limit).
-static int variant_occupancy(int pipeline) {
-  int per_cu = 0;
-  if (pipeline == 1)
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<1>, kBlock, 0);
-  else
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<2>, kBlock, 0);
-  return per_cu < 1 ? 1 : per_cu;
+  return b->cached ? launch_variant<false>(b, s) : launch_variant<true>(b, s);
 }
 
 extern "C" {
@@ -218,6 +291,7 @@ int32_t tgp_batch_destroy(tgp_batch* b) {
   if (b->d_plan) hbm_free(b->ctx, b->d_plan);
   if (b->d_results) hbm_free(b->ctx, b->d_results);
   if (b->d_granule) hbm_free(b->ctx, b->d_granule);
+  if (b->d_items) hbm_free(b->ctx, b->d_items);
   delete b;
   return TGP_OK;
 }
@@ -316,27 +390,29 @@ int32_t tgp_batch_set_queries(tgp_batch* b, const tgp_agg_query* qs, uint32_t n,
     TGP_HIP(hipMemcpyAsync(b->d_granule, gran.data(), gran.size() * 4, hipMemcpyHostToDevice, s));
     TGP_HIP(hipStreamSynchronize(s));
   }
-  return TGP_OK;
+  b->items_valid = false;
+  return build_items(b, s);
 }
 
 int32_t tgp_batch_set_tuning(tgp_batch* b, const tgp_batch_tuning* t) {
   TGP_REQUIRE(b && t, "tgp_batch_set_tuning: null argument");
-  TGP_REQUIRE(t->pipeline >= 0 && t->pipeline <= 2, "pipeline depth %d not in {0, 1, 2}",
-              t->pipeline);
   TGP_REQUIRE(t->chunk_slices <= (1u << 16), "chunk of %u slices above 65536", t->chunk_slices);
+  TGP_REQUIRE(t->order == 0 || t->order == 1, "order %d not 0 (queries) or 1 (table)", t->order);
   drop_graph(b);
   b->wg_per_cu = t->wg_per_cu;
-  b->pipeline = t->pipeline == 0 ? 1 : t->pipeline;
   b->chunk = t->chunk_slices == 0 ? 32 : t->chunk_slices;
+  b->order = t->order;
+  b->cached = t->cached_loads != 0;
   b->use_graph = t->use_graph != 0;
-  return TGP_OK;
+  return build_items(b, b->ctx->stream);
 }
 
 int32_t tgp_batch_get_tuning(tgp_batch* b, tgp_batch_tuning* t) {
   TGP_REQUIRE(b && t, "tgp_batch_get_tuning: null argument");
   t->wg_per_cu = b->wg_per_cu;
-  t->pipeline = b->pipeline;
   t->chunk_slices = b->chunk;
+  t->order = b->order;
+  t->cached_loads = b->cached ? 1 : 0;
   t->use_graph = b->use_graph ? 1 : 0;
   return TGP_OK;
 }
@@ -480,42 +556,39 @@ int32_t tgp_batch_autotune(tgp_batch* b, void* stream, tgp_batch_tuning* best_ou
     *out = ms_min;
     return TGP_OK;
   };
+  tgp_batch_tuning best_t{};
   float best = 1e30f;
-  uint32_t best_wg = 0, best_chunk = 32;
-  int32_t best_pipe = 1;
-  auto trial = [&](int32_t pipe, uint32_t wg, uint32_t chunk) {
+  auto trial = [&](uint32_t wg, uint32_t chunk, int32_t order, bool cached) {
     if (rc != TGP_OK) return;
-    b->pipeline = pipe;
     b->wg_per_cu = wg;
     b->chunk = chunk;
+    b->order = order;
+    b->cached = cached;
+    rc = build_items(b, s);
     float ms = 0;
-    rc = time_it(&ms);
+    if (rc == TGP_OK) rc = time_it(&ms);
     if (rc == TGP_OK && ms < best) {
       best = ms;
-      best_pipe = pipe;
-      best_wg = wg;
-      best_chunk = chunk;
+      best_t = tgp_batch_tuning{wg, chunk, order, cached ? 1 : 0, b->use_graph ? 1 : 0};
     }
   };
-  // coordinate search: chunk size at full occupancy, then workgroups per CU,
-  // then the two-slice pipeline at its own occupancy
-  const uint32_t occ1 = (uint32_t)variant_occupancy(1), occ2 = (uint32_t)variant_occupancy(2);
-  for (uint32_t chunk : {8u, 16u, 32u, 64u, 128u}) trial(1, occ1, chunk);
-  for (uint32_t wg = 2; wg < occ1; ++wg) trial(1, wg, best_chunk);
-  for (uint32_t wg = 2; wg <= occ2; ++wg) trial(2, wg, best_chunk);
+  // chunk sizes of both schedules at full occupancy, then workgroups per CU
+  for (uint32_t chunk : {16u, 32u, 64u, 128u}) trial(0, chunk, 0, false);
+  for (int cached = 0; cached < 2; ++cached)
+    for (uint32_t chunk : {8u, 16u, 32u, 64u}) trial(0, chunk, 1, cached != 0);
+  const tgp_batch_tuning at_occ = best_t;
+  for (uint32_t wg = 2; wg <= 4; ++wg)
+    trial(wg, at_occ.chunk_slices, at_occ.order, at_occ.cached_loads != 0);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  b->pipeline = best_pipe;
-  b->wg_per_cu = best_wg;
-  b->chunk = best_chunk;
+  b->wg_per_cu = best_t.wg_per_cu;
+  b->chunk = best_t.chunk_slices;
+  b->order = best_t.order;
+  b->cached = best_t.cached_loads != 0;
   drop_graph(b);
+  if (rc == TGP_OK) rc = build_items(b, s);
   if (rc != TGP_OK) return rc;
-  if (best_out) {
-    best_out->wg_per_cu = best_wg;
-    best_out->pipeline = best_pipe;
-    best_out->chunk_slices = best_chunk;
-    best_out->use_graph = b->use_graph ? 1 : 0;
-  }
+  if (best_out) *best_out = best_t;
   if (best_ms) *best_ms = best;
   return TGP_OK;
 }

@@ -331,6 +331,10 @@ int32_t tgp_tune_set(tgp_ctx* ctx, int32_t key, int64_t value) {
       TGP_REQUIRE(value <= 4096, "TGP_TUNE_CHUNK above 4096 slices");
       ctx->tune_chunk.store(value);
       return TGP_OK;
+    case TGP_TUNE_SLICES_PER_WAVE:
+      TGP_REQUIRE(value <= 65536, "TGP_TUNE_SLICES_PER_WAVE above 65536");
+      ctx->tune_slices_per_wave.store(value);
+      return TGP_OK;
   }
   set_error("tgp_tune_set: unknown key %d", key);
   return TGP_ERR_INVALID;
@@ -344,6 +348,7 @@ int32_t tgp_tune_get(tgp_ctx* ctx, int32_t key, int64_t* value) {
     case TGP_TUNE_SPECIALIZE: *value = ctx->tune_no_spec.load() ? 0 : 1; return TGP_OK;
     case TGP_TUNE_SPEC_KERNEL: *value = ctx->tune_spec_kernel.load(); return TGP_OK;
     case TGP_TUNE_CHUNK: *value = ctx->tune_chunk.load(); return TGP_OK;
+    case TGP_TUNE_SLICES_PER_WAVE: *value = ctx->tune_slices_per_wave.load(); return TGP_OK;
   }
   set_error("tgp_tune_get: unknown key %d", key);
   return TGP_ERR_INVALID;

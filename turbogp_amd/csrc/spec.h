@@ -101,36 +101,44 @@ constexpr int width_of(int dt) {
          : (dt == TGP_I32 || dt == TGP_U32) ? 4 : 8;
 }
 
+// A vector load, streaming (non-temporal: the line is not kept in the caches)
+// or cached (for data other waves read again soon).
+template <bool NT, typename T>
+__device__ __forceinline__ T vload(const T* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
 // The lane's 4 consecutive values of a column, widened to int64 (u64 values
 // keep their bit pattern).  `fast`: the whole slice lies inside the column.
-template <int DT>
+template <int DT, bool NT = true>
 __device__ __forceinline__ void load4(const uint8_t* col, uint64_t len, uint64_t row, bool fast,
                                       int64_t (&x)[kSR]) {
   constexpr int W = width_of(DT);
   if (fast) {
     const uint8_t* p = col + row * W;
     if constexpr (W == 8) {
-      const u32x4 a = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
-      const u32x4 b = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p) + 1);
+      const u32x4 a = vload<NT>(reinterpret_cast<const u32x4*>(p));
+      const u32x4 b = vload<NT>(reinterpret_cast<const u32x4*>(p) + 1);
       x[0] = (int64_t)(((uint64_t)a.y << 32) | a.x);
       x[1] = (int64_t)(((uint64_t)a.w << 32) | a.z);
       x[2] = (int64_t)(((uint64_t)b.y << 32) | b.x);
       x[3] = (int64_t)(((uint64_t)b.w << 32) | b.z);
     } else if constexpr (W == 4) {
-      const u32x4 a = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+      const u32x4 a = vload<NT>(reinterpret_cast<const u32x4*>(p));
       const uint32_t v[4] = {a.x, a.y, a.z, a.w};
 #pragma unroll
       for (int r = 0; r < kSR; ++r)
         x[r] = DT == TGP_I32 ? (int64_t)(int32_t)v[r] : (int64_t)v[r];
     } else if constexpr (W == 2) {
       typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-      const u32x2 a = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(p));
+      const u32x2 a = vload<NT>(reinterpret_cast<const u32x2*>(p));
       const uint32_t v[4] = {a.x & 0xffffu, a.x >> 16, a.y & 0xffffu, a.y >> 16};
 #pragma unroll
       for (int r = 0; r < kSR; ++r)
         x[r] = DT == TGP_I16 ? (int64_t)(int16_t)v[r] : (int64_t)v[r];
     } else {
-      const uint32_t a = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(p));
+      const uint32_t a = vload<NT>(reinterpret_cast<const uint32_t*>(p));
 #pragma unroll
       for (int r = 0; r < kSR; ++r) {
         const uint32_t b = (a >> (8 * r)) & 0xffu;
@@ -178,12 +186,12 @@ struct SliceVals {
   int64_t x[TGP_MAX_COLS][kSR];
 };
 
-template <Shape S, int I>
+template <Shape S, int I, bool NT = true>
 __device__ __forceinline__ void load_all(const SpecArgs& A, uint64_t row, bool fast,
                                          SliceVals<S>& v) {
   if constexpr (I < S.nslots) {
-    load4<S.dt[I]>(A.col[I], A.col_len[I], row, fast, v.x[I]);
-    load_all<S, I + 1>(A, row, fast, v);
+    load4<S.dt[I], NT>(A.col[I], A.col_len[I], row, fast, v.x[I]);
+    load_all<S, I + 1, NT>(A, row, fast, v);
   }
 }
 
@@ -192,10 +200,16 @@ template <Shape S>
 __device__ __forceinline__ uint32_t eval_slice(const SpecArgs& A, uint64_t row,
                                                const SliceVals<S>& v, int32_t (&gid)[kSR],
                                                uint32_t& oor) {
-  uint32_t sel = 0;
+  // rows of the query's range; a slice wholly inside it (all but the first and
+  // last slice of a query) skips the per-row compares
+  uint32_t sel = (1u << kSR) - 1;
+  const uint64_t slice0 = row - (threadIdx.x & 63) * kSR;
+  if (slice0 < A.row_begin || slice0 + kSliceRows > A.row_end) {
+    sel = 0;
 #pragma unroll
-  for (int r = 0; r < kSR; ++r)
-    if (row + r >= A.row_begin && row + r < A.row_end) sel |= 1u << r;
+    for (int r = 0; r < kSR; ++r)
+      if (row + r >= A.row_begin && row + r < A.row_end) sel |= 1u << r;
+  }
   if constexpr (S.has_sel) {
 #pragma unroll
     for (int r = 0; r < kSR; ++r) {
@@ -212,6 +226,10 @@ __device__ __forceinline__ uint32_t eval_slice(const SpecArgs& A, uint64_t row,
       if (in == (S.pneg[p] != 0)) sel &= ~(1u << r);
     }
   });
+  // group id: sum of part * stride over the key parts, branch free; a part
+  // outside [0, card) makes the row out of range.  ngroups <= 2^20, so parts and
+  // strides fit the 24-bit multiplier.
+  uint32_t ok = (1u << kSR) - 1;
 #pragma unroll
   for (int r = 0; r < kSR; ++r) gid[r] = 0;
   static_for<S.nparts>([&](auto K) {
@@ -228,18 +246,19 @@ __device__ __forceinline__ uint32_t eval_slice(const SpecArgs& A, uint64_t row,
           x = x / A.gdivisor[k];
         }
       }
-      const int64_t part = x - A.gbase[k];
-      if (part < 0 || part >= A.gcard[k]) gid[r] = -1;
-      else if (gid[r] >= 0) gid[r] += (int32_t)(part * A.gstride[k]);
+      const uint64_t part = (uint64_t)(x - A.gbase[k]);
+      if (part >= (uint64_t)A.gcard[k]) ok &= ~(1u << r);
+      gid[r] += (int32_t)__umul24((uint32_t)part, (uint32_t)A.gstride[k]);
     }
   });
   if constexpr (S.nparts > 0) {
+    const uint32_t bad = sel & ~ok;
+    if (bad) {
+      oor += __popc(bad);
+      sel &= ok;
+    }
 #pragma unroll
-    for (int r = 0; r < kSR; ++r)
-      if (((sel >> r) & 1) && gid[r] < 0) {
-        ++oor;
-        sel &= ~(1u << r);
-      }
+    for (int r = 0; r < kSR; ++r) gid[r] = ((ok >> r) & 1) ? gid[r] : 0;
   }
   return sel;
 }

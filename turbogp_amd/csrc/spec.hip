@@ -147,8 +147,35 @@ __global__ void __launch_bounds__(kBlock) k_wave(SpecArgs A, uint32_t rshift, ui
       k = next;
     }
   }
-  if (since) wave_flush(A.out_count, A.out_sums, A.words, S.naggs, reg, rshift);
+  // workgroup flush: every wave folds its copies, then one thread per word adds
+  // the 4 waves' totals (as int128) into the result
+  wave_fold_copies(reg, A.words, rshift);
+  __syncthreads();
+  constexpr uint32_t W1 = 1 + S.naggs;
+  for (uint32_t w = threadIdx.x; w < A.words; w += kBlock) {
+    const bool is_count = w % W1 == 0;
+    uint64_t lo = 0;
+    int64_t hi = 0;
+    for (int v = 0; v < kBlock / kWave; ++v) {
+      const uint64_t x = lreg[v * region + w];
+      const uint64_t t = lo + x;
+      hi += (int64_t)(t < lo) - ((!is_count && (int64_t)x < 0) ? 1 : 0);
+      lo = t;
+    }
+    if (lo || hi) add_result_word(A.out_count, A.out_sums, w, S.naggs, lo, hi);
+  }
   if (oor && A.out_oor) atomicAdd(A.out_oor, (unsigned long long)oor);
+}
+
+// Queries of at least this many slices (4M rows) take the replicated form.
+constexpr uint64_t kWaveMinSlices = 16384;
+
+// Fewest slices a wave of a single-query kernel gets before the grid shrinks
+// (each workgroup ends with a flush of device atomics, so small queries run on
+// fewer, longer waves).
+static uint64_t min_slices_per_wave(tgp_ctx* ctx) {
+  const int64_t v = ctx->tune_slices_per_wave.load();
+  return v > 0 ? (uint64_t)v : 2;
 }
 
 // Zeroes the result (and the chunk counter when given) in one launch.
@@ -167,12 +194,14 @@ int32_t launch_shape(tgp_ctx* ctx, const Shape& s, const SpecArgs& A, hipStream_
   if constexpr (I < kNumShapes) {
     if (shape_eq(s, kShapes[I])) {
       constexpr Shape S = kShapes[I];
-      // kernel form: the replicated-histogram waves where 2..4 groups would make
-      // the lanes of one LDS atomic collide; the register fold otherwise
+      // kernel form: the replicated-histogram waves for large queries of 2..4
+      // groups (the lanes of one LDS atomic would collide); the register fold
+      // otherwise (less LDS to set up and flush, which small queries feel)
       const int rshift = wave_rshift(A.words);
       const int64_t form = ctx->tune_spec_kernel.load();
-      const bool wave = rshift >= 0 && (form == 2 || (form == 0 && A.ngroups >= 2 &&
-                                                       A.ngroups <= 4));
+      const bool wave = rshift >= 0 &&
+                        (form == 2 || (form == 0 && A.ngroups >= 2 && A.ngroups <= 4 &&
+                                       A.nslices >= kWaveMinSlices));
       int per_cu = 0;
       uint32_t lds = 0;
       if (wave) {
@@ -191,7 +220,7 @@ int32_t launch_shape(tgp_ctx* ctx, const Shape& s, const SpecArgs& A, hipStream_
       if (wave) {
         // 0: static interleave (default); > 0: slices per dynamically taken chunk
         const uint64_t chunk = (uint64_t)ctx->tune_chunk.load();
-        const uint64_t per_wg = 4 * (chunk ? chunk : 1);
+        const uint64_t per_wg = (kBlock / kWave) * (chunk ? chunk : min_slices_per_wave(ctx));
         const uint64_t need = (A.nslices + per_wg - 1) / per_wg;
         if (grid > need) grid = need;
         if (gmax > 0 && grid > (uint64_t)gmax) grid = (uint64_t)gmax;
@@ -203,7 +232,8 @@ int32_t launch_shape(tgp_ctx* ctx, const Shape& s, const SpecArgs& A, hipStream_
         k_wave<S><<<dim3((unsigned)grid), dim3(kBlock), lds, st>>>(A, (uint32_t)rshift,
                                                                   (uint32_t)chunk, cnt);
       } else {
-        const uint64_t need = (A.nslices + 3) / 4;
+        const uint64_t per_wg = (kBlock / kWave) * min_slices_per_wave(ctx);
+        const uint64_t need = (A.nslices + per_wg - 1) / per_wg;
         if (grid > need) grid = need;
         if (gmax > 0 && grid > (uint64_t)gmax) grid = (uint64_t)gmax;
         if (grid == 0) grid = 1;

@@ -84,6 +84,7 @@ struct tgp_ctx {
   std::atomic<int64_t> tune_no_spec{0};
   std::atomic<int64_t> tune_spec_kernel{0};  // 0: by group count; 1: register fold; 2: replicated
   std::atomic<int64_t> tune_chunk{0};        // 0: static interleave; n: chunks of n slices
+  std::atomic<int64_t> tune_slices_per_wave{0};  // fewest slices per wave (0 = 2)
 };
 
 namespace tgp {

@@ -37,12 +37,12 @@ __device__ __forceinline__ bool big43(uint64_t v) {
 
 // One slice of a query of shape S into the wave-private region `reg`: the
 // values `v` of the slice starting at query-relative slice `slice` are loaded.
-template <Shape S>
+template <Shape S, bool NT = true>
 __device__ __forceinline__ void wave_load(const SpecArgs& A, uint64_t slice, uint64_t min_len,
-                                           SliceVals<S>& v) {
+                                          SliceVals<S>& v) {
   const uint64_t slice0 = A.base0 + slice * kSliceRows;
   const uint64_t row = slice0 + (threadIdx.x & 63) * kSR;
-  load_all<S, 0>(A, row, slice0 + kSliceRows <= min_len, v);
+  load_all<S, 0, NT>(A, row, slice0 + kSliceRows <= min_len, v);
 }
 
 template <Shape S>
@@ -81,28 +81,57 @@ __device__ __forceinline__ void wave_slice(const SpecArgs& A, uint64_t slice,
   });
 }
 
-// Adds the wave region (`words` words, groups x (count, naggs sums), in 2^rshift
-// copies) into the query's exact result and clears it.  Every copy's partials
-// come from at most 2^20 values of magnitude < 2^43, so the 64-bit sum of the
-// copies is exact.
-__device__ __forceinline__ void wave_flush(unsigned long long* out_count,
-                                            unsigned long long* out_sums, uint32_t words,
-                                            int naggs, unsigned long long* reg, uint32_t rshift) {
+// Sums the R = 2^rshift copies of every word of the wave region in place:
+// afterwards reg[w] (w < words) holds word w's 64-bit total and the rest of the
+// region is zero.  Lanes read consecutive words (no bank conflicts); the copies
+// of a word then sit in R consecutive lanes and are added with a butterfly.
+// Every copy holds partials of at most 2^20 values of magnitude < 2^43, so the
+// 64-bit total is exact.
+__device__ __forceinline__ void wave_fold_copies(unsigned long long* reg, uint32_t words,
+                                                 uint32_t rshift) {
   const uint32_t lane = threadIdx.x & 63;
   __builtin_amdgcn_s_waitcnt(0);
   __builtin_amdgcn_wave_barrier();
-  const uint32_t W1 = 1 + naggs, R = 1u << rshift;
+  const uint32_t T = words << rshift, R = 1u << rshift;
+  // sums of iteration `base` go to words below (base + 64) / R <= base + 64, i.e.
+  // never to a word a later iteration still has to read
+  for (uint32_t base = 0; base < T; base += 64) {
+    const uint32_t idx = base + lane;
+    unsigned long long x = idx < T ? reg[idx] : 0ull;
+    for (uint32_t m = 1; m < R; m <<= 1) x += __shfl_xor(x, (int)m);
+    if (idx < T) reg[idx] = 0;
+    if ((lane & (R - 1)) == 0 && idx < T) reg[idx >> rshift] = x;
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Adds word w's total (a count when w % (1 + naggs) == 0, else a signed 64-bit
+// partial sum of aggregate w % (1 + naggs) - 1) given as int128 (lo, hi) into
+// the query's result.
+__device__ __forceinline__ void add_result_word(unsigned long long* out_count,
+                                                unsigned long long* out_sums, uint32_t w,
+                                                int naggs, uint64_t lo, int64_t hi) {
+  const uint32_t W1 = 1 + naggs;
+  const uint32_t g = w / W1, k = w % W1;
+  if (k == 0) atomicAdd(&out_count[g], (unsigned long long)lo);
+  else atomic_add128(&out_sums[((uint64_t)g * naggs + (k - 1)) * 2], lo, hi);
+}
+
+// Adds the wave region (`words` words, groups x (count, naggs sums), in 2^rshift
+// copies) into the query's exact result and clears it.
+__device__ __forceinline__ void wave_flush(unsigned long long* out_count,
+                                           unsigned long long* out_sums, uint32_t words,
+                                           int naggs, unsigned long long* reg, uint32_t rshift) {
+  const uint32_t lane = threadIdx.x & 63;
+  wave_fold_copies(reg, words, rshift);
+  const uint32_t W1 = 1 + naggs;
   for (uint32_t w = lane; w < words; w += 64) {
-    unsigned long long x = 0;
-    for (uint32_t c = 0; c < R; ++c) {
-      x += reg[(w << rshift) + c];
-      reg[(w << rshift) + c] = 0;
-    }
+    const unsigned long long x = reg[w];
+    reg[w] = 0;
     if (!x) continue;
-    const uint32_t g = w / W1, k = w % W1;
-    if (k == 0) atomicAdd(&out_count[g], x);
-    else atomic_add128(&out_sums[((uint64_t)g * naggs + (k - 1)) * 2], x,
-                       (int64_t)x < 0 ? -1 : 0);
+    const bool is_count = w % W1 == 0;
+    add_result_word(out_count, out_sums, w, naggs, x,
+                    (!is_count && (int64_t)x < 0) ? -1 : 0);
   }
   __builtin_amdgcn_wave_barrier();
 }
