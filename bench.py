@@ -24,7 +24,7 @@ Also measured in the same run:
 Algorithmic bytes of a query: rows in its range x native widths of the columns
 it reads (date 4 B + group-key columns 1 B each + four 8-byte summed columns).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--tuning wg,chunk,order,cached]
        python bench.py --calibrate   (known-size kernels for scaling PMC byte counters:
                                       k_fill_u64 writes and k_minmax reads exactly 512 MiB)
 """
@@ -130,6 +130,9 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=40_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate", action="store_true")
+    ap.add_argument("--tuning", default="",
+                    help="wg_per_cu,chunk_slices,order,cached_loads for the batch instead of "
+                         "the autotuner (profiling runs that must time one configuration)")
     args = ap.parse_args()
     if args.calibrate:
         calibrate()
@@ -154,12 +157,19 @@ def main():
     params = W.batch_query_params(W.BATCH_QUERIES, n)
     queries = W.batch_queries(table, params)
     batch_bytes = W.batch_bytes(params)
+    unique_bytes = W.batch_unique_bytes(params)
     b = Batch(ctx, len(queries))
     b.set_queries(queries)
     info = b.info()
     assert info["fallback"] == 0, info
-    tuned, tune_ms = b.autotune()
-    log(f"autotuned {tuned} ({tune_ms:.3f} ms)")
+    if args.tuning:
+        wg, chunk, order, cached = (int(x) for x in args.tuning.split(","))
+        b.set_tuning(wg, True, chunk, order, bool(cached))
+        tuned = b.tuning()
+        log(f"tuning given: {tuned}")
+    else:
+        tuned, tune_ms = b.autotune()
+        log(f"autotuned {tuned} ({tune_ms:.3f} ms)")
     ev = [Event(ctx) for _ in range(2)]
     for _ in range(args.warmup):
         b.run()
@@ -178,19 +188,21 @@ def main():
     log(f"batch: {args.steps} steps, mean {ms_batch:.3f} ms (wall {wall:.3f} s)")
 
     # the one-launch-per-query form of the same queries
-    pq_tot, pq_lat = [], None
+    pq_tot, pq_lat, same, ms_pq = [], None, None, None
     for _ in range(args.per_query_reps):
         tot, per = b.run_timed(A.BATCH_PER_QUERY)
         pq_tot.append(tot)
         pq_lat = per if pq_lat is None or tot <= min(pq_tot) else pq_lat
-    per_query_results = b.results()
-    same = all(np.array_equal(x.count, y.count) and np.array_equal(x.sums_words, y.sums_words)
-               and x.n_out_of_range == y.n_out_of_range
-               for x, y in zip(batch_results, per_query_results))
-    assert same, "batched and per-query results differ"
-    ms_pq = min(pq_tot)
+    if pq_tot:
+        per_query_results = b.results()
+        same = all(np.array_equal(x.count, y.count) and
+                   np.array_equal(x.sums_words, y.sums_words) and
+                   x.n_out_of_range == y.n_out_of_range
+                   for x, y in zip(batch_results, per_query_results))
+        assert same, "batched and per-query results differ"
+        ms_pq = min(pq_tot)
+        log(f"per-query: {ms_pq:.3f} ms; speedup {ms_pq / ms_batch:.2f}x")
     b.close()
-    log(f"per-query: {ms_pq:.3f} ms; speedup {ms_pq / ms_batch:.2f}x")
 
     # ---- 60M-row scan-aggregate queries (Q1 and Q6 shapes) ------------------------
     qa = W.build_query(table, W.q1_spec(), 0, SCAN_ROWS)
@@ -244,17 +256,19 @@ def main():
                    "algorithmic_bytes_per_batch": int(batch_bytes), "tuning": tuned},
         "batch": {"ms": round(ms_batch, 4), "GB_s": round(achieved, 1),
                   "hbm_frac": round(achieved / HBM_PEAK_GBS, 4),
+                  "unique_bytes": int(unique_bytes),
+                  "unique_GB_s": round(unique_bytes / 1e9 / (ms_batch / 1e3), 1),
                   "p50_latency_ms": round(pct(step_ms, 50), 4),
                   "p99_latency_ms": round(pct(step_ms, 99), 4)},
-        "per_query_launches": {
+        "per_query_launches": None if ms_pq is None else {
             "ms": round(ms_pq, 4), "queries_per_s": round(nq / (ms_pq / 1e3), 1),
             "GB_s": round(batch_bytes / 1e9 / (ms_pq / 1e3), 1),
             "p50_latency_ms": round(pct(pq_lat, 50), 4),
             "p99_latency_ms": round(pct(pq_lat, 99), 4),
             "latency_note": "completion time of each query after the 4096 launches were "
                             "queued together; batched queries all complete at the batch end"},
-        "speedup_vs_per_query": round(ms_pq / ms_batch, 2),
-        "results_identical_batched_vs_per_query": bool(same),
+        "speedup_vs_per_query": None if ms_pq is None else round(ms_pq / ms_batch, 2),
+        "results_identical_batched_vs_per_query": same,
         "scan_60M": {
             "q1_groupby": {"ms": round(ms_a, 4), "GB_s": round(gbs_a, 1),
                            "hbm_frac": round(gbs_a / HBM_PEAK_GBS, 4),
@@ -266,9 +280,13 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic("k_batch_4096q_256M", n),
-                     "kernel": "k_batch (persistent batch kernel): algorithmic bytes of the "
-                               "4096 queries per launch / mean launch time (hipEvents on the "
-                               "batch stream, graph replay incl. the result-zeroing fill)"},
+                     "kernel": "persistent batch kernel (k_batch_items in table order, or "
+                               "k_batch in query order, as autotuned): algorithmic bytes of the "
+                               "4096 queries per launch (a row read by several queries counts "
+                               "each time) / mean launch time (hipEvents on the batch stream, "
+                               "graph replay incl. the result-zeroing fill); unique_bytes in "
+                               "'batch' is the union the queries read, traffic the HBM bytes "
+                               "FETCH_SIZE / WRITE_SIZE measured"},
     }
     if not args.no_cpu:
         log("cpu baseline ...")

@@ -199,7 +199,33 @@ def batch_queries(table, params):
     return [build_query(table, q1_spec(date_cutoff=c, group=g), b, e) for (b, e, c, g) in params]
 
 
+BATCH_KEY_COLUMNS = {"a": ("flaga",), "b": ("flagb",), "ab": ("flaga", "flagb"), "month": ()}
+BATCH_COMMON_COLUMNS = ("dt", "qty", "price", "disc", "tax")
+
+
 def batch_bytes(params):
-    """Algorithmic bytes: rows x (dt + group-key columns + 4 summed columns)."""
+    """Algorithmic bytes: rows x (dt + group-key columns + 4 summed columns),
+    summed over the queries (a row read by several queries counts each time)."""
     key_bytes = {"a": 1, "b": 1, "ab": 2, "month": 0}
     return sum((e - b) * (4 + key_bytes[g] + 32) for (b, e, _, g) in params)
+
+
+def _union_rows(ranges):
+    total, end = 0, -1
+    for b, e in sorted(ranges):
+        if e <= end:
+            continue
+        total += e - max(b, end)
+        end = e
+    return total
+
+
+def batch_unique_bytes(params):
+    """Bytes of the distinct column cells the batch reads: per column, the union
+    of the ranges of the queries that read it x the column width."""
+    out = 0
+    for c in BATCH_COMMON_COLUMNS + ("flaga", "flagb"):
+        ranges = [(b, e) for (b, e, _, g) in params
+                  if c in BATCH_COMMON_COLUMNS or c in BATCH_KEY_COLUMNS[g]]
+        out += _union_rows(ranges) * ROW_BYTES[c]
+    return out
