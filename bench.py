@@ -105,6 +105,139 @@ def measured_traffic(name, rows):
     return t["fetch_bytes"] + t["write_bytes"]
 
 
+def bench_join(ctx, args):
+    """Hash join: 2M-row build side (unique int64 keys, int32 payload 0..24),
+    60M-row probe side (Zipf(1.1) foreign keys, int64 value, int32 key with 1.5M
+    values); build, probe count, and the fused probe -> GROUP BY -> SUM, COUNT
+    by the build payload (25 groups) and by the probe key (1.5M groups)."""
+    from turbogp_amd import Event, GroupAggBuffers, HashTable, _capi as A
+    from turbogp_amd import workloads as W
+    from turbogp_amd.joinload import join_bytes, make_join_tables
+    build, probe = make_join_tables(ctx)
+    nb, npr = W.JOIN_BUILD_ROWS, W.JOIN_PROBE_ROWS
+    ev = [Event(ctx) for _ in range(2)]
+
+    def timed(fn, reps):
+        ts = []
+        for _ in range(reps):
+            ev[0].record()
+            fn()
+            ev[1].record()
+            ts.append(ev[0].elapsed_ms(ev[1]))
+        return statistics.median(ts)
+
+    tables = []
+    build_ms = timed(lambda: tables.append(HashTable(ctx, build["bkey"], build["pay"])), 5)
+    for t in tables[:-1]:
+        t.free()
+    t = tables[-1]
+    cnt = DeviceCounter(ctx)
+    count_ms = timed(lambda: t.probe_count(probe["fk"], result=cnt.buf, fetch=False), 10)
+    matches = cnt.value()
+    rp = GroupAggBuffers(ctx, W.JOIN_PAY_CARD, 1)
+    rg = GroupAggBuffers(ctx, W.JOIN_GKEY_CARD, 1)
+    pay_ms = timed(lambda: t.probe_agg(probe["fk"], probe["val"], A.JOIN_GROUP_BUILD_PAYLOAD,
+                                       ngroups=W.JOIN_PAY_CARD, result=rp, fetch=False), 10)
+    gkey_ms = timed(lambda: t.probe_agg(probe["fk"], probe["val"], A.JOIN_GROUP_PROBE_COL,
+                                        ngroups=W.JOIN_GKEY_CARD, group_col=probe["gkey"],
+                                        result=rg, fetch=False), 10)
+    res_pay = rp.fetch()
+    assert matches == npr and int(res_pay.count.sum()) == npr, (matches, res_pay.count.sum())
+    jb = join_bytes(nb, npr)
+    for e in ev:
+        e.free()
+    t.free()
+    rp.free()
+    rg.free()
+    cnt.free()
+    return {"build_ms": round(build_ms, 4), "probe_count_ms": round(count_ms, 4),
+            "probe_count_Gprobes_s": round(npr / (count_ms / 1e3) / 1e9, 2),
+            "group_by_payload": {"ms": round(pay_ms, 4), "groups": W.JOIN_PAY_CARD,
+                                 "GB_s": round(jb["by_payload"] / 1e9 / (pay_ms / 1e3), 1)},
+            "group_by_probe_key": {"ms": round(gkey_ms, 4), "groups": W.JOIN_GKEY_CARD,
+                                   "GB_s": round(jb["by_gkey"] / 1e9 / (gkey_ms / 1e3), 1)},
+            "matches": int(matches), "_tables": (build, probe)}
+
+
+class DeviceCounter:
+    """One device word for tgp_hash_probe_count."""
+
+    def __init__(self, ctx):
+        from turbogp_amd import DeviceBuffer
+        self.buf = DeviceBuffer(ctx, 8)
+
+    def value(self):
+        return int(self.buf.download(np.uint64, 1)[0])
+
+    def free(self):
+        self.buf.free()
+
+
+def bench_serve(ctx, table, join_tables, args):
+    """Serving: an open-loop Poisson stream of the mixed requests (70 % Q1/Q6
+    scan-aggregates on log-uniform ranges, 25 % join-aggregates over build
+    prefixes of 1K..2M rows, 5 % full-table Q1), batched server against the
+    one-launch-per-query server, at 50/70/90 % of the batched capacity."""
+    from turbogp_amd import _capi as A
+    from turbogp_amd import workloads as W
+    from turbogp_amd.serve import Server
+    build, probe = join_tables
+    n = args.serve_requests
+    mix = W.serve_mix(n, W.BATCH_TABLE_ROWS, W.JOIN_BUILD_ROWS, W.JOIN_PROBE_ROWS)
+    reqs = W.serve_requests(table, build, probe, mix)
+    configs = {"batched": dict(max_batch_queries=1024, batch_slots=3, join_slots=2,
+                               heavy_rows=50_000_000),
+               "per_query_launch": dict(batch_slots=3, join_slots=2, heavy_rows=50_000_000,
+                                        per_query_launch=True)}
+
+    def run(srv, offsets, rq=reqs):
+        ids, start = srv.replay(rq, offsets)
+        res = [srv.result(i) for i in ids]
+        assert all(t["status"] == A.OK for _, t in res)
+        lat = [(t["done_ns"] - t["arrival_ns"]) / 1e6 for _, t in res]
+        end = max(t["done_ns"] for _, t in res)
+        return lat, (end - start) / 1e9, [r for r, _ in res]
+
+    out = {"requests": n, "mix": {k: sum(1 for m in mix if m[0] == k) for k in
+                                  ("q1", "q6", "join", "full")}}
+    results = {}
+    for name, kw in configs.items():
+        srv = Server(ctx, **kw)
+        lat, span, res = run(srv, np.zeros(n, np.int64))
+        results[name] = res
+        out[name] = {"capacity_req_s": round(n / span, 1)}
+        srv.close()
+    same = all(np.array_equal(a.count, b.count) and np.array_equal(a.sums_words, b.sums_words)
+               for a, b in zip(results["batched"], results["per_query_launch"]))
+    assert same, "batched and per-query serving results differ"
+    cap = out["batched"]["capacity_req_s"]
+    for name, kw in configs.items():
+        srv = Server(ctx, **kw)
+        levels = []
+        for frac in (0.5, 0.7, 0.9):
+            qps = frac * cap
+            lat, span, _ = run(srv, W.poisson_offsets(n, qps))
+            levels.append({"offered_req_s": round(qps, 1), "fraction_of_batched_capacity": frac,
+                           "achieved_req_s": round(n / span, 1),
+                           "p50_ms": round(pct(lat, 50), 3), "p99_ms": round(pct(lat, 99), 3)})
+        out[name]["open_loop"] = levels
+        out[name]["stats"] = srv.stats()
+        srv.close()
+    # the batchable part of the mix alone (scan-aggregates), all submitted at once
+    scan = [m for m in mix if m[0] in ("q1", "q6")]
+    rq = W.serve_requests(table, build, probe, scan)
+    for name, kw in configs.items():
+        srv = Server(ctx, **kw)
+        _, span, _ = run(srv, np.zeros(len(rq), np.int64), rq)
+        out[name]["scan_only_capacity_req_s"] = round(len(rq) / span, 1)
+        srv.close()
+    out["capacity_ratio"] = round(cap / out["per_query_launch"]["capacity_req_s"], 2)
+    out["scan_only_capacity_ratio"] = round(out["batched"]["scan_only_capacity_req_s"] /
+                                            out["per_query_launch"]["scan_only_capacity_req_s"], 2)
+    out["results_identical_batched_vs_per_query"] = bool(same)
+    return out
+
+
 def calibrate():
     """One 512 MiB write (k_fill_u64) and one 512 MiB read (k_minmax) for the
     FETCH_SIZE / WRITE_SIZE scale factors."""
@@ -130,6 +263,7 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=40_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--tuning", default="",
                     help="wg_per_cu,chunk_slices,order,cached_loads for the batch instead of "
                          "the autotuner (profiling runs that must time one configuration)")
@@ -233,6 +367,9 @@ def main():
     for e in ev + ev3:
         e.free()
 
+    join = bench_join(ctx, args)
+    serve = bench_serve(ctx, table, join.pop("_tables"), args)
+
     achieved = batch_bytes / 1e9 / (ms_batch / 1e3)
     nq = len(queries)
     out = {
@@ -277,6 +414,8 @@ def main():
                             "hbm_frac": round(gbs_b / HBM_PEAK_GBS, 4),
                             "bytes_per_row": W.Q6_BYTES_PER_ROW,
                             "selected": int(res_b.count[0])}},
+        "join_2M_x_60M": join,
+        "serving": serve,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic("k_batch_4096q_256M", n),

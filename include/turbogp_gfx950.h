@@ -381,6 +381,72 @@ typedef struct tgp_join_agg_query {
 int32_t tgp_hash_probe_agg(tgp_ctx* ctx, const tgp_join_agg_query* q, const tgp_group_result* out,
                            void* stream);
 
+/* ---- serving: continuous batching over resident columns -------------------- */
+/* The reference serves one query at a time per pooled connection
+ * (route_and_execute, engine/mod.rs:522-545; server/pool.rs).  A tgp_server
+ * instead admits requests continuously: waiting aggregate requests form the next
+ * persistent-kernel batch whenever one of `batch_slots` batches finishes (batches
+ * overlap on the device, one stream each), and join requests are admitted when
+ * their hash table fits the HBM budget (deferred while it does not, rejected
+ * with TGP_ERR_BUDGET when it never can).  Results are exact and copied to host
+ * memory per request; latency runs from the request's arrival time. */
+typedef struct tgp_server tgp_server;
+enum tgp_serve_kind { TGP_SERVE_AGG = 0, TGP_SERVE_JOIN_AGG = 1 };
+typedef struct tgp_serve_join {
+  tgp_column build_keys;       /* build side: all `len` rows are inserted */
+  tgp_column build_payload;    /* data NULL => the build row index is the payload */
+  tgp_join_agg_query probe;    /* `table` is filled in by the server */
+} tgp_serve_join;
+typedef struct tgp_serve_request {
+  int32_t kind; /* tgp_serve_kind */
+  int32_t _pad;
+  tgp_agg_query agg;           /* kind TGP_SERVE_AGG */
+  tgp_serve_join join;         /* kind TGP_SERVE_JOIN_AGG */
+} tgp_serve_request;
+typedef struct tgp_serve_config {
+  uint32_t max_batch_queries;  /* requests per batch */
+  uint32_t batch_slots;        /* batches in flight (1..8) */
+  uint32_t join_slots;         /* joins in flight (1..8) */
+  uint32_t _pad;
+  uint64_t heavy_rows;         /* ranges of >= this many rows run alone, one at a time (0 = off) */
+  uint64_t hash_budget_bytes;  /* cap on hash-table HBM (0 = the tier's budget) */
+  uint32_t wg_per_cu, chunk_slices; /* batch tuning (tgp_batch_tuning) */
+  int32_t order, cached_loads;
+  int32_t per_query_launch;    /* 1: no batching, one tgp_group_agg launch per request */
+  int32_t _pad2;
+} tgp_serve_config;
+typedef struct tgp_serve_timing {
+  int64_t arrival_ns, start_ns, done_ns; /* tgp_serve_now_ns() clock */
+  int32_t status, deferred;              /* deferred: waited for HBM budget */
+  uint64_t result_words;
+} tgp_serve_timing;
+typedef struct tgp_serve_stats {
+  uint64_t submitted, completed, failed, rejected, deferred, admitted_joins;
+  uint64_t batches, batched_requests, max_batch, heavy_in_flight;
+} tgp_serve_stats;
+int64_t tgp_serve_now_ns(void);
+int32_t tgp_serve_create(tgp_ctx* ctx, const tgp_serve_config* cfg, tgp_server** out);
+/* Finishes the requests already submitted, then stops the workers. */
+int32_t tgp_serve_destroy(tgp_server* s);
+/* Queues one request; arrival_ns 0 = now.  Thread safe. */
+int32_t tgp_serve_submit(tgp_server* s, const tgp_serve_request* req, int64_t arrival_ns,
+                         uint64_t* out_id);
+/* Waits until request `id` is finished (timeout_ns < 0: no limit). */
+int32_t tgp_serve_wait(tgp_server* s, uint64_t id, int64_t timeout_ns, int32_t* out_done);
+/* Result words of a finished request: count[ngroups] | sums[ngroups*naggs*2] | oor
+ * (tgp_group_result layout, naggs = 1 for joins); returns the request's status. */
+int32_t tgp_serve_result(tgp_server* s, uint64_t id, uint64_t* h_words, uint64_t nwords,
+                         tgp_serve_timing* timing);
+/* Forgets a finished request. */
+int32_t tgp_serve_release(tgp_server* s, uint64_t id);
+/* Open-loop replay: request i is submitted at start + arrival_offset_ns[i]
+ * (non-decreasing) from a submitter thread, with that time as its arrival;
+ * returns when all are finished. */
+int32_t tgp_serve_replay(tgp_server* s, const tgp_serve_request* reqs,
+                         const int64_t* arrival_offset_ns, uint32_t n, uint64_t* out_ids,
+                         int64_t* out_start_ns);
+int32_t tgp_serve_get_stats(tgp_server* s, tgp_serve_stats* out);
+
 /* ---- synthetic data (counter-based, seeded) ------------------------------- */
 /* Row i of column `column_id`: u = mix64(seed ^ (column_id << 40) ^ i)
  * (SplitMix64 finaliser), value = lo + mulhi64(u, hi - lo + 1).  Written at

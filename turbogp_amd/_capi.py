@@ -110,8 +110,49 @@ class BatchTuning(C.Structure):
                 ("cached_loads", C.c_int32), ("use_graph", C.c_int32)]
 
 
+SERVE_AGG, SERVE_JOIN_AGG = 0, 1
+
+
+class ServeJoin(C.Structure):
+    _fields_ = [("build_keys", Column), ("build_payload", Column), ("probe", JoinAggQuery)]
+
+
+class ServeRequest(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("_pad", C.c_int32), ("agg", AggQuery), ("join", ServeJoin)]
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("max_batch_queries", C.c_uint32), ("batch_slots", C.c_uint32),
+                ("join_slots", C.c_uint32), ("_pad", C.c_uint32), ("heavy_rows", C.c_uint64),
+                ("hash_budget_bytes", C.c_uint64), ("wg_per_cu", C.c_uint32),
+                ("chunk_slices", C.c_uint32), ("order", C.c_int32), ("cached_loads", C.c_int32),
+                ("per_query_launch", C.c_int32), ("_pad2", C.c_int32)]
+
+
+class ServeTiming(C.Structure):
+    _fields_ = [("arrival_ns", C.c_int64), ("start_ns", C.c_int64), ("done_ns", C.c_int64),
+                ("status", C.c_int32), ("deferred", C.c_int32), ("result_words", C.c_uint64)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "submitted", "completed", "failed", "rejected", "deferred", "admitted_joins",
+        "batches", "batched_requests", "max_batch", "heavy_in_flight")]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
+    "tgp_serve_now_ns": (I64_, []),
+    "tgp_serve_create": (I32_, [P, C.POINTER(ServeConfig), C.POINTER(C.c_void_p)]),
+    "tgp_serve_destroy": (I32_, [P]),
+    "tgp_serve_submit": (I32_, [P, C.POINTER(ServeRequest), I64_, C.POINTER(C.c_uint64)]),
+    "tgp_serve_wait": (I32_, [P, C.c_uint64, I64_, C.POINTER(C.c_int32)]),
+    "tgp_serve_result": (I32_, [P, C.c_uint64, C.POINTER(C.c_uint64), C.c_uint64,
+                                C.POINTER(ServeTiming)]),
+    "tgp_serve_release": (I32_, [P, C.c_uint64]),
+    "tgp_serve_replay": (I32_, [P, C.POINTER(ServeRequest), C.POINTER(C.c_int64), C.c_uint32,
+                                C.POINTER(C.c_uint64), C.POINTER(C.c_int64)]),
+    "tgp_serve_get_stats": (I32_, [P, C.POINTER(ServeStats)]),
     "tgp_last_error": (C.c_char_p, []),
     "tgp_abi_version": (I32_, []),
     "tgp_ctx_create": (I32_, [I32_, U64_, C.POINTER(P)]),

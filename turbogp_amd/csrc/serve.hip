@@ -1,0 +1,658 @@
+// Continuous-batching query server over device-resident columns.
+//
+// Requests (filter + GROUP BY aggregates, and hash-join aggregates) are
+// submitted from any thread at any time.  Two worker threads run them:
+//
+//   - the aggregate worker keeps up to `batch_slots` batches in flight, each on
+//     its own stream.  Whenever a slot is free it takes every waiting aggregate
+//     request (up to `max_batch_queries`) into one persistent-kernel batch
+//     (tgp_batch_*), so requests that arrive while others run join the next
+//     batch and batches overlap on the device.  Requests whose row range is at
+//     least `heavy_rows` run one at a time as single tgp_group_agg launches on a
+//     slot of their own, so a full-table scan does not hold back the small
+//     requests, and many full-table scans do not contend inside one batch.
+//   - the join worker admits a join when the hash table it will build fits the
+//     HBM budget (the context's tier budget, optionally capped by
+//     `hash_budget_bytes`): tables of admitted joins are built one at a time and
+//     their probe->aggregate kernels run on `join_slots` streams; a join waits
+//     while the budget is held by others, and is rejected if it can never fit.
+//
+// With `per_query_launch` the aggregate worker instead runs each request as its
+// own tgp_group_agg launch (one request per slot): the one-launch-per-query
+// serving form that batching is measured against.
+//
+// Completion copies each request's exact result words to host memory and
+// stamps it.  Latency is measured from the request's arrival time (given by the
+// submitter, so an open-loop load generator that falls behind does not hide
+// queueing time).  The reference serves one query at a time behind a
+// connection-pool semaphore (server/pool.rs, engine/mod.rs:522-545
+// route_and_execute); it has no batching.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "tgp_internal.h"
+
+using namespace tgp;
+
+namespace {
+
+int64_t now_ns() {
+  return std::chrono::duration_cast<std::chrono::nanoseconds>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+struct Request {
+  uint64_t id = 0;
+  tgp_serve_request req{};
+  uint64_t ngroups = 0;
+  int32_t naggs = 0;
+  uint64_t rows = 0;   // rows in the aggregate's range / probe range
+  uint64_t words = 0;  // result words: count[ngroups] + sums[ngroups*naggs*2] + oor
+  int64_t t_arrive = 0, t_start = 0, t_done = 0;
+  int32_t status = TGP_OK;
+  bool done = false;
+  bool deferred = false;  // waited at least once for HBM budget
+  std::vector<uint64_t> result;
+  std::string error;      // tgp_last_error() text of a failed request
+};
+
+struct AggSlot {
+  tgp_batch* batch = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t done_ev = nullptr;
+  std::vector<Request*> reqs;
+  bool busy = false;
+  bool heavy = false;
+  bool single = false;          // running one request with tgp_group_agg
+  uint64_t* h_arena = nullptr;  // pinned copy of the batch's result arena
+  uint64_t h_cap = 0;
+  uint64_t* d_res = nullptr;    // one-launch-per-query form: the request's result words
+  uint64_t d_cap = 0;
+};
+
+struct JoinSlot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done_ev = nullptr;
+  Request* req = nullptr;
+  tgp_hash_table* table = nullptr;
+  uint64_t table_bytes = 0;
+  uint64_t* d_res = nullptr;  // count[ng] | sums[ng*2] | oor
+  uint64_t d_cap = 0;
+  bool busy = false;
+};
+
+}  // namespace
+
+struct tgp_server {
+  tgp_ctx* ctx = nullptr;
+  tgp_serve_config cfg{};
+  std::mutex mu;
+  std::condition_variable work_cv;  // new requests / stop
+  std::condition_variable done_cv;  // completions
+  std::deque<Request*> agg_q, heavy_q, join_q;
+  std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
+  uint64_t next_id = 1;
+  bool stop = false;
+  std::thread agg_thread, join_thread;
+  std::vector<AggSlot> agg_slots;
+  std::vector<JoinSlot> join_slots;
+  uint64_t hash_in_use = 0;
+  tgp_serve_stats stats{};
+};
+
+namespace {
+
+int32_t result_shape(const tgp_serve_request& r, uint64_t* ngroups, int32_t* naggs,
+                     uint64_t* rows) {
+  if (r.kind == TGP_SERVE_AGG) {
+    TGP_TRY(tgp_group_agg_ngroups(&r.agg, ngroups));
+    *naggs = r.agg.naggs;
+    *rows = r.agg.row_end > r.agg.row_begin ? r.agg.row_end - r.agg.row_begin : 0;
+    return TGP_OK;
+  }
+  if (r.kind == TGP_SERVE_JOIN_AGG) {
+    const tgp_join_agg_query& j = r.join.probe;
+    TGP_REQUIRE(j.group_by == TGP_JOIN_GROUP_NONE || j.ngroups > 0,
+                "join request: ngroups must be > 0");
+    TGP_REQUIRE(j.ngroups <= (1ull << 26), "join request: %llu groups above 2^26",
+                (unsigned long long)j.ngroups);
+    TGP_REQUIRE(r.join.build_keys.data && r.join.build_keys.len > 0,
+                "join request: empty build side");
+    *ngroups = j.group_by == TGP_JOIN_GROUP_NONE ? 1 : j.ngroups;
+    *naggs = 1;
+    *rows = j.row_end > j.row_begin ? j.row_end - j.row_begin : 0;
+    return TGP_OK;
+  }
+  set_error("serve request: unknown kind %d", r.kind);
+  return TGP_ERR_INVALID;
+}
+
+void finish(tgp_server* s, Request* r, int32_t status) {
+  r->status = status;
+  if (status != TGP_OK) r->error = tgp_last_error();
+  r->t_done = now_ns();
+  r->done = true;
+  s->stats.completed += 1;
+  if (status != TGP_OK) s->stats.failed += 1;
+}
+
+// ---- aggregate worker -------------------------------------------------------------
+
+// One-launch-per-query form: the slot's single request ran with tgp_group_agg.
+int32_t harvest_single(tgp_server* s, AggSlot& sl) {
+  Request* r = sl.reqs[0];
+  r->result.resize(r->words);
+  int32_t rc = TGP_OK;
+  if (hipMemcpyAsync(r->result.data(), sl.d_res, r->words * 8, hipMemcpyDeviceToHost,
+                     sl.stream) != hipSuccess ||
+      hipStreamSynchronize(sl.stream) != hipSuccess) {
+    set_error("serve: result copy failed");
+    rc = TGP_ERR_HIP;
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  finish(s, r, rc);
+  sl.reqs.clear();
+  sl.busy = false;
+  if (sl.heavy) s->stats.heavy_in_flight -= 1;
+  s->done_cv.notify_all();
+  return rc;
+}
+
+int32_t launch_single(tgp_server* s, AggSlot& sl, Request* r) {
+  if (sl.d_cap < r->words) {
+    if (sl.d_res) TGP_TRY(hbm_free(s->ctx, sl.d_res));
+    sl.d_res = nullptr;
+    sl.d_cap = 0;
+    void* p = nullptr;
+    TGP_TRY(hbm_alloc(s->ctx, r->words * 8, &p));
+    sl.d_res = static_cast<uint64_t*>(p);
+    sl.d_cap = r->words;
+  }
+  const tgp_group_result out{sl.d_res, sl.d_res + r->ngroups,
+                             sl.d_res + r->ngroups + r->ngroups * (uint64_t)r->naggs * 2};
+  TGP_TRY(tgp_group_agg(s->ctx, &r->req.agg, &out, sl.stream));
+  TGP_HIP(hipEventRecord(sl.done_ev, sl.stream));
+  return TGP_OK;
+}
+
+// Copies a finished batch's results out and completes its requests.
+int32_t harvest_agg(tgp_server* s, AggSlot& sl) {
+  if (sl.single) return harvest_single(s, sl);
+  uint64_t words = 0;
+  int32_t rc = tgp_batch_info(sl.batch, &words, nullptr, nullptr, nullptr);
+  if (rc == TGP_OK && words > sl.h_cap) {
+    if (sl.h_arena) (void)hipHostFree(sl.h_arena);
+    sl.h_arena = nullptr;
+    sl.h_cap = 0;
+    if (hipHostMalloc((void**)&sl.h_arena, words * 8) != hipSuccess) {
+      set_error("serve: hipHostMalloc of %llu bytes failed", (unsigned long long)words * 8);
+      rc = TGP_ERR_OOM;
+    } else {
+      sl.h_cap = words;
+    }
+  }
+  if (rc == TGP_OK) rc = tgp_batch_results_sync(sl.batch, sl.h_arena, words, sl.stream);
+  std::lock_guard<std::mutex> lk(s->mu);
+  for (uint32_t i = 0; i < sl.reqs.size(); ++i) {
+    Request* r = sl.reqs[i];
+    uint64_t off = 0;
+    if (rc == TGP_OK) rc = tgp_batch_result_layout(sl.batch, i, &off, nullptr, nullptr, nullptr);
+    if (rc == TGP_OK) r->result.assign(sl.h_arena + off, sl.h_arena + off + r->words);
+    finish(s, r, rc);
+  }
+  sl.reqs.clear();
+  sl.busy = false;
+  if (sl.heavy) s->stats.heavy_in_flight -= 1;
+  s->done_cv.notify_all();
+  return rc;
+}
+
+// Launches the given requests as one batch on the slot.
+void launch_agg(tgp_server* s, AggSlot& sl, std::vector<Request*>&& take, bool heavy) {
+  std::vector<tgp_agg_query> qs(take.size());
+  for (size_t i = 0; i < take.size(); ++i) qs[i] = take[i]->req.agg;
+  const int64_t t0 = now_ns();
+  int32_t rc = TGP_OK;
+  // heavy requests run alone: a batch of many scans over the same large range
+  // makes them contend for the same lines instead of sharing them
+  sl.single = s->cfg.per_query_launch || heavy;
+  if (sl.single) {
+    rc = launch_single(s, sl, take[0]);
+  } else {
+    rc = tgp_batch_set_queries(sl.batch, qs.data(), (uint32_t)qs.size(), sl.stream);
+    if (rc == TGP_OK) rc = tgp_batch_run(sl.batch, TGP_BATCH_PERSISTENT, sl.stream);
+    if (rc == TGP_OK && hipEventRecord(sl.done_ev, sl.stream) != hipSuccess) {
+      set_error("serve: hipEventRecord failed");
+      rc = TGP_ERR_HIP;
+    }
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (rc != TGP_OK) {
+    for (Request* r : take) finish(s, r, rc);
+    if (heavy) s->stats.heavy_in_flight -= 1;
+    s->done_cv.notify_all();
+    return;
+  }
+  for (Request* r : take) r->t_start = t0;
+  sl.reqs = std::move(take);
+  sl.busy = true;
+  sl.heavy = heavy;
+  s->stats.batches += 1;
+  s->stats.batched_requests += sl.reqs.size();
+  if (sl.reqs.size() > s->stats.max_batch) s->stats.max_batch = sl.reqs.size();
+}
+
+void agg_worker(tgp_server* s) {
+  (void)hipSetDevice(s->ctx->device);
+  const uint32_t maxq = s->cfg.per_query_launch ? 1 : s->cfg.max_batch_queries;
+  for (;;) {
+    bool progressed = false;
+    for (AggSlot& sl : s->agg_slots) {
+      if (sl.busy && hipEventQuery(sl.done_ev) == hipSuccess) {
+        (void)harvest_agg(s, sl);
+        progressed = true;
+      }
+    }
+    // launch: a free slot takes the waiting requests (heavy ones in their own batch)
+    for (AggSlot& sl : s->agg_slots) {
+      if (sl.busy) continue;
+      std::vector<Request*> take;
+      bool heavy = false;
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (!s->heavy_q.empty() && s->stats.heavy_in_flight == 0) {
+          take.push_back(s->heavy_q.front());
+          s->heavy_q.pop_front();
+          heavy = true;
+          s->stats.heavy_in_flight += 1;
+        } else {
+          while (!s->agg_q.empty() && take.size() < maxq) {
+            take.push_back(s->agg_q.front());
+            s->agg_q.pop_front();
+          }
+        }
+      }
+      if (take.empty()) break;
+      launch_agg(s, sl, std::move(take), heavy);
+      progressed = true;
+    }
+    std::unique_lock<std::mutex> lk(s->mu);
+    bool busy = false;
+    for (AggSlot& sl : s->agg_slots) busy |= sl.busy;
+    if (s->stop && !busy && s->agg_q.empty() && s->heavy_q.empty()) return;
+    if (!progressed) {
+      // poll in-flight batches every few microseconds; sleep when idle
+      if (busy) s->work_cv.wait_for(lk, std::chrono::microseconds(20));
+      else s->work_cv.wait_for(lk, std::chrono::milliseconds(5));
+    }
+  }
+}
+
+// ---- join worker --------------------------------------------------------------------
+
+uint64_t hash_budget(tgp_server* s) {
+  uint64_t used = 0, budget = 0, peak = 0;
+  (void)tgp_hbm_usage(s->ctx, &used, &budget, &peak);
+  // room the tier has now, plus what admitted tables already hold
+  uint64_t room = budget > used ? budget - used : 0;
+  room += s->hash_in_use;
+  if (s->cfg.hash_budget_bytes && s->cfg.hash_budget_bytes < room) room = s->cfg.hash_budget_bytes;
+  return room;
+}
+
+int32_t harvest_join(tgp_server* s, JoinSlot& sl) {
+  Request* r = sl.req;
+  r->result.resize(r->words);
+  int32_t rc = TGP_OK;
+  if (hipMemcpyAsync(r->result.data(), sl.d_res, r->words * 8, hipMemcpyDeviceToHost,
+                     sl.stream) != hipSuccess ||
+      hipStreamSynchronize(sl.stream) != hipSuccess) {
+    set_error("serve: join result copy failed");
+    rc = TGP_ERR_HIP;
+  }
+  const int32_t rd = tgp_hash_table_destroy(s->ctx, sl.table);
+  if (rc == TGP_OK) rc = rd;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->hash_in_use -= sl.table_bytes;
+  sl.table = nullptr;
+  sl.table_bytes = 0;
+  sl.req = nullptr;
+  sl.busy = false;
+  finish(s, r, rc);
+  s->done_cv.notify_all();
+  return rc;
+}
+
+int32_t start_join(tgp_server* s, JoinSlot& sl, Request* r, uint64_t need) {
+  const tgp_serve_join& J = r->req.join;
+  r->t_start = now_ns();
+  // result buffer: count | sums | oor
+  if (sl.d_cap < r->words) {
+    if (sl.d_res) TGP_TRY(hbm_free(s->ctx, sl.d_res));
+    sl.d_res = nullptr;
+    sl.d_cap = 0;
+    void* p = nullptr;
+    TGP_TRY(hbm_alloc(s->ctx, r->words * 8, &p));
+    sl.d_res = static_cast<uint64_t*>(p);
+    sl.d_cap = r->words;
+  }
+  tgp_hash_table* t = nullptr;
+  const tgp_column* pay = J.build_payload.data ? &J.build_payload : nullptr;
+  TGP_TRY(tgp_hash_build(s->ctx, &J.build_keys, pay, &t, sl.stream));
+  sl.table = t;
+  sl.table_bytes = need;
+  tgp_join_agg_query q = J.probe;
+  q.table = t;
+  q.ngroups = r->ngroups;
+  tgp_group_result out{sl.d_res, sl.d_res + r->ngroups, sl.d_res + r->ngroups * 3};
+  TGP_TRY(tgp_hash_probe_agg(s->ctx, &q, &out, sl.stream));
+  TGP_HIP(hipEventRecord(sl.done_ev, sl.stream));
+  sl.req = r;
+  sl.busy = true;
+  return TGP_OK;
+}
+
+void join_worker(tgp_server* s) {
+  (void)hipSetDevice(s->ctx->device);
+  for (;;) {
+    bool progressed = false;
+    for (JoinSlot& sl : s->join_slots) {
+      if (sl.busy && hipEventQuery(sl.done_ev) == hipSuccess) {
+        (void)harvest_join(s, sl);
+        progressed = true;
+      }
+    }
+    for (JoinSlot& sl : s->join_slots) {
+      if (sl.busy) continue;
+      Request* r = nullptr;
+      uint64_t need = 0;
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->join_q.empty()) break;
+        Request* front = s->join_q.front();
+        need = tgp_hash_table_bytes_for(front->req.join.build_keys.len);
+        const uint64_t room = hash_budget(s);
+        uint64_t used = 0, budget = 0, peak = 0;
+        (void)tgp_hbm_usage(s->ctx, &used, &budget, &peak);
+        const uint64_t cap = s->cfg.hash_budget_bytes ? s->cfg.hash_budget_bytes : budget;
+        if (need > cap) {
+          // can never fit: reject
+          s->join_q.pop_front();
+          s->stats.rejected += 1;
+          set_error("serve: join needs %llu bytes of hash table, budget %llu",
+                    (unsigned long long)need, (unsigned long long)cap);
+          finish(s, front, TGP_ERR_BUDGET);
+          s->done_cv.notify_all();
+          progressed = true;
+          continue;
+        }
+        if (s->hash_in_use + need > room) {
+          // wait for admitted tables to be released
+          if (!front->deferred) {
+            front->deferred = true;
+            s->stats.deferred += 1;
+          }
+          break;
+        }
+        s->join_q.pop_front();
+        s->hash_in_use += need;
+        s->stats.admitted_joins += 1;
+        r = front;
+      }
+      const int32_t rc = start_join(s, sl, r, need);
+      if (rc != TGP_OK) {
+        if (sl.table) (void)tgp_hash_table_destroy(s->ctx, sl.table);
+        bool others = false;
+        for (JoinSlot& o : s->join_slots) others |= o.busy && &o != &sl;
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->hash_in_use -= need;
+        sl.table = nullptr;
+        sl.table_bytes = 0;
+        sl.busy = false;
+        if (rc == TGP_ERR_BUDGET && others) {
+          // the tier filled up meanwhile: wait for the joins in flight to release it
+          s->join_q.push_front(r);
+          s->stats.admitted_joins -= 1;
+          if (!r->deferred) {
+            r->deferred = true;
+            s->stats.deferred += 1;
+          }
+        } else {
+          finish(s, r, rc);
+          s->done_cv.notify_all();
+        }
+      }
+      progressed = true;
+    }
+    std::unique_lock<std::mutex> lk(s->mu);
+    bool busy = false;
+    for (JoinSlot& sl : s->join_slots) busy |= sl.busy;
+    if (s->stop && !busy && s->join_q.empty()) return;
+    if (!progressed) {
+      if (busy) s->work_cv.wait_for(lk, std::chrono::microseconds(20));
+      else s->work_cv.wait_for(lk, std::chrono::milliseconds(5));
+    }
+  }
+}
+
+void release(tgp_server* s) {
+  for (AggSlot& sl : s->agg_slots) {
+    if (sl.batch) (void)tgp_batch_destroy(sl.batch);
+    if (sl.done_ev) (void)hipEventDestroy(sl.done_ev);
+    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    if (sl.h_arena) (void)hipHostFree(sl.h_arena);
+    if (sl.d_res) (void)hbm_free(s->ctx, sl.d_res);
+  }
+  for (JoinSlot& sl : s->join_slots) {
+    if (sl.table) (void)tgp_hash_table_destroy(s->ctx, sl.table);
+    if (sl.d_res) (void)hbm_free(s->ctx, sl.d_res);
+    if (sl.done_ev) (void)hipEventDestroy(sl.done_ev);
+    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t tgp_serve_now_ns(void) { return now_ns(); }
+
+int32_t tgp_serve_create(tgp_ctx* ctx, const tgp_serve_config* cfg, tgp_server** out) {
+  TGP_REQUIRE(ctx && cfg && out, "tgp_serve_create: null argument");
+  TGP_REQUIRE(cfg->max_batch_queries > 0 && cfg->max_batch_queries <= (1u << 16),
+              "max_batch_queries %u not in 1..65536", cfg->max_batch_queries);
+  TGP_REQUIRE(cfg->batch_slots >= 1 && cfg->batch_slots <= 8, "batch_slots %u not in 1..8",
+              cfg->batch_slots);
+  TGP_REQUIRE(cfg->join_slots >= 1 && cfg->join_slots <= 8, "join_slots %u not in 1..8",
+              cfg->join_slots);
+  TGP_TRY(check_injected(ctx));
+  *out = nullptr;
+  auto s = std::make_unique<tgp_server>();
+  s->ctx = ctx;
+  s->cfg = *cfg;
+  s->agg_slots.resize(cfg->batch_slots);
+  s->join_slots.resize(cfg->join_slots);
+  int32_t rc = TGP_OK;
+  for (AggSlot& sl : s->agg_slots) {
+    if (rc == TGP_OK) rc = tgp_batch_create(ctx, cfg->max_batch_queries, &sl.batch);
+    if (rc == TGP_OK) {
+      const tgp_batch_tuning t{cfg->wg_per_cu, cfg->chunk_slices, cfg->order, cfg->cached_loads,
+                               0};
+      rc = tgp_batch_set_tuning(sl.batch, &t);
+    }
+    if (rc == TGP_OK && (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+                         hipEventCreateWithFlags(&sl.done_ev, hipEventDisableTiming) !=
+                             hipSuccess)) {
+      set_error("tgp_serve_create: stream/event creation failed");
+      rc = TGP_ERR_HIP;
+    }
+  }
+  for (JoinSlot& sl : s->join_slots) {
+    if (rc == TGP_OK && (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+                         hipEventCreateWithFlags(&sl.done_ev, hipEventDisableTiming) !=
+                             hipSuccess)) {
+      set_error("tgp_serve_create: stream/event creation failed");
+      rc = TGP_ERR_HIP;
+    }
+  }
+  if (rc != TGP_OK) {
+    release(s.get());
+    return rc;
+  }
+  tgp_server* raw = s.release();
+  raw->agg_thread = std::thread(agg_worker, raw);
+  raw->join_thread = std::thread(join_worker, raw);
+  *out = raw;
+  return TGP_OK;
+}
+
+int32_t tgp_serve_destroy(tgp_server* s) {
+  if (!s) return TGP_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->work_cv.notify_all();
+  if (s->agg_thread.joinable()) s->agg_thread.join();
+  if (s->join_thread.joinable()) s->join_thread.join();
+  release(s);
+  delete s;
+  return TGP_OK;
+}
+
+int32_t tgp_serve_submit(tgp_server* s, const tgp_serve_request* req, int64_t arrival_ns,
+                         uint64_t* out_id) {
+  TGP_REQUIRE(s && req && out_id, "tgp_serve_submit: null argument");
+  auto r = std::make_unique<Request>();
+  r->req = *req;
+  TGP_TRY(result_shape(*req, &r->ngroups, &r->naggs, &r->rows));
+  r->words = r->ngroups + r->ngroups * (uint64_t)r->naggs * 2 + 1;
+  if (req->kind == TGP_SERVE_AGG) {
+    // validation now, so a bad request fails its submit, not a whole batch
+    uint64_t ng = 0;
+    TGP_TRY(tgp_group_agg_ngroups(&req->agg, &ng));
+    int32_t strategy = 0, shape = 0;
+    TGP_TRY(tgp_group_agg_explain(&req->agg, &strategy, &shape));
+  }
+  r->t_arrive = arrival_ns ? arrival_ns : now_ns();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    TGP_REQUIRE(!s->stop, "tgp_serve_submit: server is stopping");
+    r->id = s->next_id++;
+    *out_id = r->id;
+    Request* raw = r.get();
+    s->reqs.emplace(raw->id, std::move(r));
+    s->stats.submitted += 1;
+    if (raw->req.kind == TGP_SERVE_JOIN_AGG) s->join_q.push_back(raw);
+    else if (s->cfg.heavy_rows && raw->rows >= s->cfg.heavy_rows) s->heavy_q.push_back(raw);
+    else s->agg_q.push_back(raw);
+  }
+  s->work_cv.notify_all();
+  return TGP_OK;
+}
+
+int32_t tgp_serve_wait(tgp_server* s, uint64_t id, int64_t timeout_ns, int32_t* out_done) {
+  TGP_REQUIRE(s && out_done, "tgp_serve_wait: null argument");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  TGP_REQUIRE(it != s->reqs.end(), "tgp_serve_wait: unknown request %llu",
+              (unsigned long long)id);
+  Request* r = it->second.get();
+  if (timeout_ns < 0) s->done_cv.wait(lk, [&] { return r->done; });
+  else s->done_cv.wait_for(lk, std::chrono::nanoseconds(timeout_ns), [&] { return r->done; });
+  *out_done = r->done ? 1 : 0;
+  return TGP_OK;
+}
+
+int32_t tgp_serve_result(tgp_server* s, uint64_t id, uint64_t* h_words, uint64_t nwords,
+                         tgp_serve_timing* timing) {
+  TGP_REQUIRE(s, "tgp_serve_result: null server");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  TGP_REQUIRE(it != s->reqs.end(), "tgp_serve_result: unknown request %llu",
+              (unsigned long long)id);
+  Request* r = it->second.get();
+  TGP_REQUIRE(r->done, "tgp_serve_result: request %llu not finished", (unsigned long long)id);
+  if (timing) {
+    timing->arrival_ns = r->t_arrive;
+    timing->start_ns = r->t_start;
+    timing->done_ns = r->t_done;
+    timing->status = r->status;
+    timing->deferred = r->deferred ? 1 : 0;
+    timing->result_words = r->words;
+  }
+  if (r->status != TGP_OK) {
+    set_error("request %llu failed: %s", (unsigned long long)id, r->error.c_str());
+    return r->status;
+  }
+  TGP_REQUIRE(nwords == 0 || h_words, "tgp_serve_result: null buffer");
+  TGP_REQUIRE(nwords == 0 || nwords >= r->words, "tgp_serve_result: %llu words needed",
+              (unsigned long long)r->words);
+  if (nwords) std::memcpy(h_words, r->result.data(), r->words * 8);
+  return TGP_OK;
+}
+
+int32_t tgp_serve_release(tgp_server* s, uint64_t id) {
+  TGP_REQUIRE(s, "tgp_serve_release: null server");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  TGP_REQUIRE(it != s->reqs.end() && it->second->done, "tgp_serve_release: request %llu "
+              "unknown or not finished", (unsigned long long)id);
+  s->reqs.erase(it);
+  return TGP_OK;
+}
+
+int32_t tgp_serve_replay(tgp_server* s, const tgp_serve_request* reqs,
+                         const int64_t* arrival_offset_ns, uint32_t n, uint64_t* out_ids,
+                         int64_t* out_start_ns) {
+  TGP_REQUIRE(s && (n == 0 || (reqs && arrival_offset_ns && out_ids)),
+              "tgp_serve_replay: null argument");
+  for (uint32_t i = 1; i < n; ++i)
+    TGP_REQUIRE(arrival_offset_ns[i] >= arrival_offset_ns[i - 1],
+                "tgp_serve_replay: arrival offsets must be non-decreasing");
+  const int64_t start = now_ns() + 1000000;  // 1 ms to set up
+  if (out_start_ns) *out_start_ns = start;
+  int32_t rc = TGP_OK;
+  std::string err;
+  // the open-loop submitter: request i enters at start + offset[i] whatever the
+  // state of the earlier ones, and carries that time as its arrival
+  std::thread submitter([&] {
+    for (uint32_t i = 0; i < n && rc == TGP_OK; ++i) {
+      const int64_t at = start + arrival_offset_ns[i];
+      const int64_t t = now_ns();
+      if (at - t > 200000) std::this_thread::sleep_for(std::chrono::nanoseconds(at - t - 100000));
+      while (now_ns() < at) {
+      }
+      rc = tgp_serve_submit(s, &reqs[i], at, &out_ids[i]);
+      if (rc != TGP_OK) err = tgp_last_error();
+    }
+  });
+  submitter.join();
+  if (rc != TGP_OK) {
+    set_error("tgp_serve_replay: %s", err.c_str());
+    return rc;
+  }
+  for (uint32_t i = 0; i < n; ++i) {
+    int32_t done = 0;
+    TGP_TRY(tgp_serve_wait(s, out_ids[i], -1, &done));
+  }
+  return TGP_OK;
+}
+
+int32_t tgp_serve_get_stats(tgp_server* s, tgp_serve_stats* out) {
+  TGP_REQUIRE(s && out, "tgp_serve_get_stats: null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  *out = s->stats;
+  return TGP_OK;
+}
+
+}  // extern "C"

@@ -367,18 +367,24 @@ class HashTable:
                 "tgp_hash_table_info")
         return n.value, cap.value, bool(dup.value)
 
-    def probe_count(self, keys, row_begin=0, row_end=None, stream=None):
+    def probe_count(self, keys, row_begin=0, row_end=None, stream=None, result=None,
+                    fetch=True):
+        """Number of matches.  `result`: a DeviceBuffer of one word to count into
+        (zeroed first); with fetch=False nothing is copied back."""
         kd = keys.desc()
         row_end = keys.n if row_end is None else row_end
-        cnt = DeviceBuffer(self.ctx, 8)
+        cnt = result or DeviceBuffer(self.ctx, 8)
         try:
-            cnt.fill_u64(0, stream=stream)
+            cnt.fill_u64(0, nwords=1, stream=stream)
             A.check(A.lib().tgp_hash_probe_count(self.ctx.handle, self.handle, C.byref(kd),
                                                  row_begin, row_end, cnt.ptr, stream),
                     "tgp_hash_probe_count")
-            return int(cnt.download(np.uint64, 1, stream)[0])
+            if fetch:
+                return int(cnt.download(np.uint64, 1, stream)[0])
+            return None
         finally:
-            cnt.free()
+            if result is None:
+                cnt.free()
 
     def probe_mark(self, keys, row_begin=0, row_end=None, stream=None):
         kd = keys.desc()

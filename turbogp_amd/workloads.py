@@ -229,3 +229,77 @@ def batch_unique_bytes(params):
                   if c in BATCH_COMMON_COLUMNS or c in BATCH_KEY_COLUMNS[g]]
         out += _union_rows(ranges) * ROW_BYTES[c]
     return out
+
+
+# ---- serving mix: open-loop Poisson arrivals over the resident tables ----------------
+SERVE_SCAN, SERVE_JOIN, SERVE_FULL = 0.70, 0.25, 0.05
+
+
+def _log_uniform(rng, lo, hi, n=None):
+    return np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), n))).astype(np.int64)
+
+
+def serve_mix(n, table_rows, build_rows, probe_rows, seed=SEED, lmax=4_000_000,
+              build_min=1_000, build_max=2_000_000):
+    """Seeded request descriptions (kind, begin, end, build_rows):
+    70 % scan-aggregates, half Q1-shape (WHERE dt <= 2435 GROUP BY flaga, flagb)
+    and half Q6-shape, over a range of log-uniform length in [1K, lmax];
+    25 % join-aggregates GROUP BY the build payload, over a build-side prefix of
+    log-uniform length in [build_min, build_max] and a probe range of log-uniform
+    length in [1K, lmax]; 5 % full-table Q1-shape aggregates."""
+    rng = np.random.Generator(np.random.PCG64((seed ^ 0x5E7E) & 0xFFFFFFFF))
+    out = []
+    for _ in range(n):
+        u = rng.random()
+        if u < SERVE_SCAN:
+            kind = "q1" if rng.random() < 0.5 else "q6"
+            L = int(_log_uniform(rng, 1_000, min(lmax, table_rows)))
+            b = int(rng.integers(0, table_rows - L + 1))
+            out.append((kind, b, b + L, 0))
+        elif u < SERVE_SCAN + SERVE_JOIN:
+            nb = int(_log_uniform(rng, min(build_min, build_rows), min(build_max, build_rows)))
+            L = int(_log_uniform(rng, 1_000, min(lmax, probe_rows)))
+            b = int(rng.integers(0, probe_rows - L + 1))
+            out.append(("join", b, b + L, nb))
+        else:
+            out.append(("full", 0, table_rows, 0))
+    return out
+
+
+def serve_requests(table, build, probe, mix):
+    """ServeRequest objects for serve_mix() descriptions over device tables
+    (fact table columns; build {bkey, pay}; probe {fk, val, gkey})."""
+    from .serve import agg_request, join_request
+    out = []
+    for kind, b, e, nb in mix:
+        if kind == "q1" or kind == "full":
+            out.append(agg_request(build_query(table, q1_spec(), b, e)))
+        elif kind == "q6":
+            out.append(agg_request(build_query(table, q6_spec(), b, e)))
+        else:
+            out.append(join_request(build["bkey"], probe["fk"], probe["val"],
+                                    A.JOIN_GROUP_BUILD_PAYLOAD, ngroups=JOIN_PAY_CARD,
+                                    build_payload=build["pay"], build_rows=nb, row_begin=b,
+                                    row_end=e))
+    return out
+
+
+def serve_bytes(mix):
+    """Algorithmic bytes per request: scan rows x referenced widths; joins:
+    build rows x 12 (key, payload) + probe rows x 16 (key, value)."""
+    out = []
+    for kind, b, e, nb in mix:
+        if kind in ("q1", "full"):
+            out.append((e - b) * Q1_BYTES_PER_ROW)
+        elif kind == "q6":
+            out.append((e - b) * Q6_BYTES_PER_ROW)
+        else:
+            out.append(nb * 12 + (e - b) * 16)
+    return out
+
+
+def poisson_offsets(n, qps, seed=SEED):
+    """Arrival offsets (ns, non-decreasing) of an open-loop Poisson stream."""
+    rng = np.random.Generator(np.random.PCG64((seed ^ 0xA5517) & 0xFFFFFFFF))
+    gaps = rng.exponential(1e9 / qps, n)
+    return np.cumsum(gaps).astype(np.int64)
