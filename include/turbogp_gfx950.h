@@ -197,8 +197,10 @@ enum tgp_tune_key {
                                else 1 */
   TGP_TUNE_CHUNK = 4,       /* form 2's schedule: 0 (default) static interleave of slices;
                                n > 0: chunks of n slices (<= 4096) taken from a counter */
-  TGP_TUNE_SLICES_PER_WAVE = 5 /* fewest 256-row slices per wave of the specialised
-                                  kernels before their grid shrinks (0 = default 2) */
+  TGP_TUNE_SLICES_PER_WAVE = 5, /* fewest 256-row slices per wave of the specialised
+                                   kernels before their grid shrinks (0 = default 2) */
+  TGP_TUNE_JOIN_ATOMICS = 6     /* join GROUP BY a probe column with more groups than LDS
+                                   holds: 0 (default) radix-partitioned, 1 global atomics */
 };
 int32_t tgp_tune_set(tgp_ctx* ctx, int32_t key, int64_t value);
 int32_t tgp_tune_get(tgp_ctx* ctx, int32_t key, int64_t* value);

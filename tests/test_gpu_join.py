@@ -158,3 +158,40 @@ def test_device_zipf_generator_matches_oracle(ctx):
         assert np.array_equal(got.to_numpy(), want)
     finally:
         got.free()
+
+
+@pytest.mark.parametrize("group_by,ng", [(A.JOIN_GROUP_PROBE_COL, 20), (A.JOIN_GROUP_PROBE_COL, 1_000),
+                                         (A.JOIN_GROUP_PROBE_COL, 2_047), (A.JOIN_GROUP_PROBE_COL, 2_049),
+                                         (A.JOIN_GROUP_PROBE_COL, 300_001),
+                                         (A.JOIN_GROUP_BUILD_PAYLOAD, 700),
+                                         (A.JOIN_GROUP_BUILD_PAYLOAD, 5_000)])
+def test_probe_agg_every_table_form(ctx, group_by, ng):
+    """Per-wave replicated LDS tables (few groups), one workgroup table, the
+    radix-partitioned form and global atomics must all match the oracle."""
+    rng = np.random.default_rng(ng)
+    nb, npb = 30_000, 400_000
+    bk = rng.integers(0, nb // 2, nb).astype(np.int64)      # duplicate keys
+    bp = rng.integers(-2, ng + 2, nb).astype(np.int32)
+    pk = bk[rng.integers(0, nb, npb)]
+    pk[::13] -= 1_000_000                                  # misses
+    pv = rng.integers(-2**62, 2**62, npb).astype(np.int64)
+    pg = rng.integers(-5, ng + 5, npb).astype(np.int32)    # some groups out of range
+    kb = 3 if group_by == A.JOIN_GROUP_PROBE_COL else 0
+    b, e = 17, npb - 1
+    t, keep = build(ctx, bk, bp)
+    cols = [dev(ctx, pk), dev(ctx, pv), dev(ctx, pg)]
+    got = []
+    try:
+        for atomics in (0, 1):
+            ctx.tune(A.TUNE_JOIN_ATOMICS, atomics)
+            got.append(t.probe_agg(cols[0], cols[1], group_by, ng, kb, cols[2], b, e))
+    finally:
+        ctx.tune(A.TUNE_JOIN_ATOMICS, 0)
+        t.free()
+        for c in keep + cols:
+            c.free()
+    cnt, sums, oor = B.join_agg(bk, bp, pk, pv, pg, group_by, kb, ng, b, e)
+    for r in got:
+        assert r.n_out_of_range == oor
+        assert np.array_equal(r.count, cnt)
+        assert [s[0] for s in r.sums()] == sums

@@ -30,7 +30,7 @@ BIT_AND, BIT_OR, BIT_ANDNOT = range(3)
 AGG_SUM, AGG_COUNT_NONZERO = 0, 1
 JOIN_GROUP_BUILD_PAYLOAD, JOIN_GROUP_PROBE_COL, JOIN_GROUP_NONE = 0, 1, 2
 TUNE_WG_PER_CU, TUNE_GRID_MAX, TUNE_SPECIALIZE, TUNE_SPEC_KERNEL, TUNE_CHUNK = 0, 1, 2, 3, 4
-TUNE_SLICES_PER_WAVE = 5
+TUNE_SLICES_PER_WAVE, TUNE_JOIN_ATOMICS = 5, 6
 BATCH_PERSISTENT, BATCH_PER_QUERY = 0, 1
 
 MAX_COLS, MAX_PREDS, MAX_AGGS, MAX_FACTORS, MAX_GROUP_PARTS = 8, 8, 6, 3, 2

@@ -137,6 +137,46 @@ __device__ __forceinline__ uint32_t probe_key(const JoinTableDev& T, unsigned lo
   return m;
 }
 
+// Probes U keys with their first slot loads issued together (U random loads in
+// flight per lane instead of one), then finishes each key's linear probe.
+// ok[u] false skips key u.  f(std::integral_constant<int, u>, build row,
+// payload) is called per match; the key index is a compile-time constant, so
+// per-key arrays of the caller stay in registers.
+template <typename F>
+__device__ __forceinline__ void probe_from(const JoinTableDev& T, unsigned long long k, uint64_t s,
+                                           Slot e, F&& f) {
+  if (k == kEmpty) {
+    for (uint64_t j = 0; j < T.n_side; ++j) f(T.side[j].x, T.side[j].y);
+    return;
+  }
+  for (;;) {
+    if (e.key == kEmpty) break;
+    if (e.key == k) {
+      f(e.row, e.payload);
+      if (!T.has_dup) break;
+    }
+    s = (s + 1) & T.mask;
+    e = T.slots[s];
+  }
+}
+
+template <int U, typename F>
+__device__ __forceinline__ void probe_keys(const JoinTableDev& T, const unsigned long long (&k)[U],
+                                           const bool (&ok)[U], F&& f) {
+  uint64_t s[U];
+  Slot e[U];
+  static_for<U>([&](auto I) {
+    constexpr int u = decltype(I)::value;
+    s[u] = hmix(k[u]) & T.mask;
+    e[u] = Slot{kEmpty, 0, 0};
+    if (ok[u] && k[u] != kEmpty) e[u] = T.slots[s[u]];
+  });
+  static_for<U>([&](auto I) {
+    constexpr int u = decltype(I)::value;
+    if (ok[u]) probe_from(T, k[u], s[u], e[u], [&](uint32_t row, uint32_t pay) { f(I, row, pay); });
+  });
+}
+
 constexpr int kProbeUnroll = 4;
 
 // ---- match count (HashProbe) ----------------------------------------------------------
@@ -153,9 +193,10 @@ __global__ void __launch_bounds__(256) k_probe_count(JoinTableDev T, const void*
       const uint64_t i = i0 + u * stride;
       k[u] = i < e ? (unsigned long long)load_i64(keys, kdt, i) : 0;
     }
+    bool ok[kProbeUnroll];
 #pragma unroll
-    for (int u = 0; u < kProbeUnroll; ++u)
-      if (i0 + u * stride < e) c += probe_key(T, k[u], [](uint32_t, uint32_t) {});
+    for (int u = 0; u < kProbeUnroll; ++u) ok[u] = i0 + u * stride < e;
+    probe_keys(T, k, ok, [&](auto, uint32_t, uint32_t) { ++c; });
   }
   for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
   if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
@@ -215,20 +256,44 @@ __device__ __forceinline__ void split_to_i128(uint64_t slo, int64_t shi, uint64_
   *lo = l;
 }
 
-template <bool USE_LDS>
-__global__ void __launch_bounds__(256) k_probe_agg(JoinTableDev T, ProbeAggDev Q) {
-  extern __shared__ __align__(16) unsigned long long ltab[];
-  unsigned long long* cnt;
-  unsigned long long* sum;
-  if (USE_LDS) {
-    cnt = ltab;
-    sum = ltab + Q.ngroups;
-    for (uint64_t i = threadIdx.x; i < 3 * Q.ngroups; i += blockDim.x) ltab[i] = 0;
-    __syncthreads();
-  } else {
-    cnt = Q.out_count;  // the result buffers hold the split sums until k_split_finalize
-    sum = Q.out_sums;
+// LDS form: every wave owns a table of ngroups x (count, sum lo32, sum hi32)
+// words in R = 2^rshift copies, word w of copy c at (w << rshift) + c, and lane
+// l updates copy l mod R, so that the lanes of one atomic rarely collide when
+// there are few groups.  The copies and the 4 waves are added up at the end.
+struct LdsTab {
+  unsigned long long* base;
+  uint32_t rshift, copy;
+  __device__ __forceinline__ unsigned long long* at(uint64_t w) const {
+    return base + ((uint32_t)w << rshift) + copy;
   }
+};
+
+__device__ __forceinline__ void acc_add_lds(const LdsTab& t, uint64_t ng, uint64_t g, int64_t v) {
+  atomicAdd(t.at(g), 1ull);
+  atomicAdd(t.at(ng + 2 * g), (unsigned long long)(uint32_t)v);
+  const int64_t h = v >> 32;
+  if (h) atomicAdd(t.at(ng + 2 * g + 1), (unsigned long long)h);
+}
+
+template <bool USE_LDS>
+__global__ void __launch_bounds__(256) k_probe_agg(JoinTableDev T, ProbeAggDev Q,
+                                                   uint32_t rshift, uint32_t shared_table) {
+  extern __shared__ __align__(16) unsigned long long ltab[];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t words = 3 * Q.ngroups;                  // per copy
+  const uint64_t region = words << rshift;               // per wave (or per workgroup)
+  const uint32_t nwt = shared_table ? 1 : 4;             // tables in the workgroup
+  LdsTab lt{ltab + (shared_table ? 0 : wave) * region, rshift, lane & ((1u << rshift) - 1)};
+  unsigned long long* cnt = Q.out_count;  // global form: the result buffers hold the
+  unsigned long long* sum = Q.out_sums;   // split sums until k_split_finalize
+  if (USE_LDS) {
+    for (uint64_t i = threadIdx.x; i < nwt * region; i += blockDim.x) ltab[i] = 0;
+    __syncthreads();
+  }
+  auto add = [&](uint64_t g, int64_t v) {
+    if constexpr (USE_LDS) acc_add_lds(lt, Q.ngroups, g, v);
+    else acc_add(cnt, sum, g, v);
+  };
   uint32_t oor = 0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i0 = Q.b + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i0 < Q.e;
@@ -243,34 +308,42 @@ __global__ void __launch_bounds__(256) k_probe_agg(JoinTableDev T, ProbeAggDev Q
       v[u] = ok ? load_i64(Q.val, Q.vdt, i) : 0;
       g[u] = (ok && Q.group_by == TGP_JOIN_GROUP_PROBE_COL) ? load_i64(Q.grp, Q.gdt, i) : 0;
     }
+    bool ok[kProbeUnroll];
 #pragma unroll
-    for (int u = 0; u < kProbeUnroll; ++u) {
-      if (i0 + u * stride >= Q.e) continue;
-      const int64_t val = v[u];
-      if (Q.group_by == TGP_JOIN_GROUP_BUILD_PAYLOAD) {
-        probe_key(T, k[u], [&](uint32_t, uint32_t pay) {
-          const int64_t gg = (int64_t)(int32_t)pay - Q.key_base;
-          if (gg < 0 || (uint64_t)gg >= Q.ngroups) { ++oor; return; }
-          acc_add(cnt, sum, (uint64_t)gg, val);
-        });
-      } else {
+    for (int u = 0; u < kProbeUnroll; ++u) ok[u] = i0 + u * stride < Q.e;
+    if (Q.group_by == TGP_JOIN_GROUP_BUILD_PAYLOAD) {
+      probe_keys(T, k, ok, [&](auto I, uint32_t, uint32_t pay) {
+        const int64_t gg = (int64_t)(int32_t)pay - Q.key_base;
+        if (gg < 0 || (uint64_t)gg >= Q.ngroups) { ++oor; return; }
+        add((uint64_t)gg, v[decltype(I)::value]);
+      });
+    } else {
+      probe_keys(T, k, ok, [&](auto I, uint32_t, uint32_t) {
+        constexpr int u = decltype(I)::value;
         const int64_t gg = Q.group_by == TGP_JOIN_GROUP_PROBE_COL ? g[u] - Q.key_base : 0;
-        const bool in = gg >= 0 && (uint64_t)gg < Q.ngroups;
-        probe_key(T, k[u], [&](uint32_t, uint32_t) {
-          if (!in) { ++oor; return; }
-          acc_add(cnt, sum, (uint64_t)gg, val);
-        });
-      }
+        if (gg < 0 || (uint64_t)gg >= Q.ngroups) { ++oor; return; }
+        add((uint64_t)gg, v[u]);
+      });
     }
   }
   if (USE_LDS) {
     __syncthreads();
+    // word w summed over the 4 waves and R copies: lo32 sums and counts stay
+    // below 2^64 (fewer than 2^32 additions per word and call), hi32 sums are
+    // signed 64-bit
+    auto total = [&](uint64_t w) {
+      unsigned long long t = 0;
+      for (uint32_t wv = 0; wv < nwt; ++wv)
+        for (uint32_t c = 0; c < (1u << rshift); ++c) t += ltab[wv * region + (w << rshift) + c];
+      return t;
+    };
     for (uint64_t gi = threadIdx.x; gi < Q.ngroups; gi += blockDim.x) {
-      if (!cnt[gi]) continue;
-      atomicAdd(&Q.out_count[gi], cnt[gi]);
+      const unsigned long long c = total(gi);
+      if (!c) continue;
+      atomicAdd(&Q.out_count[gi], c);
       uint64_t lo;
       int64_t hi;
-      split_to_i128(sum[2 * gi], (int64_t)sum[2 * gi + 1], &lo, &hi);
+      split_to_i128(total(Q.ngroups + 2 * gi), (int64_t)total(Q.ngroups + 2 * gi + 1), &lo, &hi);
       if (lo || hi) {
         const unsigned long long old = atomicAdd(&Q.out_sums[2 * gi], (unsigned long long)lo);
         const int64_t h = hi + (int64_t)((old + lo) < old);
@@ -290,6 +363,275 @@ __global__ void __launch_bounds__(256) k_split_finalize(unsigned long long* sums
     split_to_i128(sums[2 * g], (int64_t)sums[2 * g + 1], &lo, &hi);
     sums[2 * g] = lo;
     sums[2 * g + 1] = (unsigned long long)hi;
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_scan_blocks(uint64_t* v, uint64_t n) {
+  __shared__ unsigned long long part[1024];
+  const uint64_t per = (n + 1023) / 1024;
+  const uint64_t lo = threadIdx.x * per, hi = min(lo + per, n);
+  unsigned long long s = 0;
+  for (uint64_t i = lo; i < hi; ++i) s += v[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const unsigned long long x = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += x;
+    __syncthreads();
+  }
+  unsigned long long run = threadIdx.x ? part[threadIdx.x - 1] : 0;
+  for (uint64_t i = lo; i < hi; ++i) {
+    const unsigned long long c = v[i];
+    v[i] = run;
+    run += c;
+  }
+  if (threadIdx.x == 1023) v[n] = part[1023];
+}
+
+// Exclusive prefix sum of x over the 256 threads of a workgroup (wave
+// shuffles, then the 4 wave totals); *total receives the workgroup's sum.
+__device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t x, uint64_t* total) {
+  __shared__ unsigned long long wave_tot[4];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t incl = x;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint64_t y = __shfl_up(incl, off);
+    if (lane >= (uint32_t)off) incl += y;
+  }
+  if (lane == 63) wave_tot[wave] = incl;
+  __syncthreads();
+  uint64_t before = 0;
+  for (uint32_t w = 0; w < wave; ++w) before += wave_tot[w];
+  *total = wave_tot[0] + wave_tot[1] + wave_tot[2] + wave_tot[3];
+  __syncthreads();
+  return before + incl - x;
+}
+
+// Exclusive scan of n u64 words in place, the total stored at v[n]:
+// per-block sums, a scan of the block sums, then each block scans its chunk.
+constexpr int kScanPer = 16;                      // words per thread
+constexpr int kScanChunk = 256 * kScanPer;        // words per block
+
+__global__ void __launch_bounds__(256) k_scan_sums(const uint64_t* v, uint64_t n, uint64_t* bsum) {
+  const uint64_t c0 = (uint64_t)blockIdx.x * kScanChunk + threadIdx.x * kScanPer;
+  uint64_t x = 0;
+  for (int j = 0; j < kScanPer; ++j)
+    if (c0 + j < n) x += v[c0 + j];
+  uint64_t tot = 0;
+  (void)block_exclusive_scan(x, &tot);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(256) k_scan_chunks(uint64_t* v, uint64_t n, const uint64_t* bsum,
+                                                     uint64_t nblocks) {
+  const uint64_t c0 = (uint64_t)blockIdx.x * kScanChunk + threadIdx.x * kScanPer;
+  uint64_t w[kScanPer];
+  uint64_t x = 0;
+  for (int j = 0; j < kScanPer; ++j) {
+    w[j] = c0 + j < n ? v[c0 + j] : 0;
+    x += w[j];
+  }
+  uint64_t tot = 0;
+  uint64_t run = bsum[blockIdx.x] + block_exclusive_scan(x, &tot);
+  for (int j = 0; j < kScanPer; ++j) {
+    if (c0 + j < n) v[c0 + j] = run;
+    run += w[j];
+  }
+  if (blockIdx.x == nblocks - 1 && threadIdx.x == 0) v[n] = bsum[nblocks];
+}
+
+// bsum holds nblocks + 1 words.
+static int32_t scan_u64(hipStream_t s, uint64_t* v, uint64_t n, uint64_t* bsum) {
+  const uint64_t nb = (n + kScanChunk - 1) / kScanChunk;
+  if (nb == 0) return fill_u64(s, v, 1, 0);
+  k_scan_sums<<<(unsigned)nb, 256, 0, s>>>(v, n, bsum);
+  k_scan_blocks<<<1, 1024, 0, s>>>(bsum, nb);
+  k_scan_chunks<<<(unsigned)nb, 256, 0, s>>>(v, n, bsum, nb);
+  TGP_HIP(hipGetLastError());
+  return TGP_OK;
+}
+
+// ---- GROUP BY a probe column with many groups: radix partitioning -------------------
+// Random device atomics on a table of millions of groups are the cost of the
+// one-pass form.  Instead the probe rows are partitioned by
+// p = (group - key_base) >> kPartShift (groups outside [0, ngroups) go to an
+// extra partition P) and each partition is aggregated in LDS:
+//   (1) k_part_hist: workgroup w of G counts its contiguous share of the rows
+//       per partition into mat[p * G + w];
+//   (2) an exclusive scan of mat gives every (partition, workgroup) pair its
+//       output range, so the scatter needs no atomics and is deterministic;
+//   (3) k_part_scatter: each workgroup sorts a tile of its rows by partition in
+//       LDS and writes (key, value, group) records partition run by partition
+//       run, so neighbouring lanes write neighbouring records;
+//   (4) k_part_agg: one workgroup per partition probes its records (first
+//       slot loads of 4 records in flight per lane), aggregates in an LDS table
+//       of 2^kPartShift groups and stores its groups' results (each group
+//       belongs to exactly one partition: plain stores, no atomics).
+constexpr int kPartShift = 11;          // 2048 groups per partition: 48 KB of LDS
+constexpr int kPartTile = 2048;         // rows per scatter tile (staged in LDS)
+constexpr uint32_t kMaxParts = 1024;     // keeps the scatter's LDS within 64 KB
+
+__device__ __forceinline__ uint32_t part_of(const ProbeAggDev& Q, uint64_t i, uint32_t nparts) {
+  const int64_t g = load_i64(Q.grp, Q.gdt, i) - Q.key_base;
+  return (g < 0 || (uint64_t)g >= Q.ngroups) ? nparts : (uint32_t)((uint64_t)g >> kPartShift);
+}
+
+// Rows [Q.b, Q.e) split into G contiguous shares; share w = [row_lo(w), row_lo(w+1)).
+__device__ __forceinline__ uint64_t row_lo(const ProbeAggDev& Q, uint64_t w, uint64_t G) {
+  return Q.b + (Q.e - Q.b) * w / G;
+}
+
+__global__ void __launch_bounds__(256) k_part_hist(ProbeAggDev Q, uint32_t nparts,
+                                                   unsigned long long* mat) {
+  extern __shared__ __align__(16) uint32_t h[];  // nparts + 1 counters
+  const uint64_t G = gridDim.x;
+  for (uint32_t p = threadIdx.x; p <= nparts; p += blockDim.x) h[p] = 0;
+  __syncthreads();
+  const uint64_t lo = row_lo(Q, blockIdx.x, G), hi = row_lo(Q, blockIdx.x + 1, G);
+  for (uint64_t i0 = lo + threadIdx.x; i0 < hi; i0 += 4 * (uint64_t)blockDim.x) {
+    uint32_t p[4];
+    static_for<4>([&](auto I) {
+      constexpr int u = decltype(I)::value;
+      const uint64_t i = i0 + (uint64_t)u * blockDim.x;
+      p[u] = i < hi ? part_of(Q, i, nparts) : 0xffffffffu;
+    });
+    static_for<4>([&](auto I) {
+      if (p[decltype(I)::value] != 0xffffffffu) atomicAdd(&h[p[decltype(I)::value]], 1u);
+    });
+  }
+  __syncthreads();
+  for (uint32_t p = threadIdx.x; p <= nparts; p += blockDim.x)
+    mat[(uint64_t)p * G + blockIdx.x] = h[p];
+}
+
+// mat: exclusive offsets after the scan ((nparts + 1) * G entries, then the total).
+// Per tile: the rows' keys, values and local groups are staged in LDS with
+// coalesced loads, ranked per partition, and written partition run by
+// partition run.
+__global__ void __launch_bounds__(256) k_part_scatter(ProbeAggDev Q, uint32_t nparts,
+                                                      const unsigned long long* mat,
+                                                      unsigned long long* rec_key,
+                                                      long long* rec_val, uint32_t* rec_grp) {
+  extern __shared__ __align__(16) unsigned long long cursor[];        // nparts + 1
+  unsigned long long* tk = cursor + nparts + 1;                       // kPartTile keys
+  long long* tv = reinterpret_cast<long long*>(tk + kPartTile);       // kPartTile values
+  uint32_t* cnt = reinterpret_cast<uint32_t*>(tv + kPartTile);        // nparts + 1
+  uint32_t* tstart = cnt + nparts + 1;                                // nparts + 1
+  uint16_t* tp = reinterpret_cast<uint16_t*>(tstart + nparts + 1);    // kPartTile partitions
+  uint16_t* tr = tp + kPartTile;                                      // kPartTile ranks
+  uint16_t* tg = tr + kPartTile;                                      // kPartTile local groups
+  uint16_t* perm = tg + kPartTile;                                    // kPartTile
+  const uint64_t G = gridDim.x;
+  for (uint32_t p = threadIdx.x; p <= nparts; p += blockDim.x)
+    cursor[p] = mat[(uint64_t)p * G + blockIdx.x];
+  const uint64_t lo = row_lo(Q, blockIdx.x, G), hi = row_lo(Q, blockIdx.x + 1, G);
+  const uint32_t per = (nparts + 1 + blockDim.x - 1) / blockDim.x;  // scan entries per thread
+  for (uint64_t t0 = lo; t0 < hi; t0 += kPartTile) {
+    const uint32_t tlen = (uint32_t)(hi - t0 < (uint64_t)kPartTile ? hi - t0 : kPartTile);
+    for (uint32_t p = threadIdx.x; p <= nparts; p += blockDim.x) cnt[p] = 0;
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < tlen; j += blockDim.x) {
+      const uint64_t i = t0 + j;
+      const int64_t g = load_i64(Q.grp, Q.gdt, i) - Q.key_base;
+      const bool in = g >= 0 && (uint64_t)g < Q.ngroups;
+      const uint32_t p = in ? (uint32_t)((uint64_t)g >> kPartShift) : nparts;
+      tk[j] = (unsigned long long)load_i64(Q.keys, Q.kdt, i);
+      tv[j] = load_i64(Q.val, Q.vdt, i);
+      tg[j] = in ? (uint16_t)((uint64_t)g & ((1u << kPartShift) - 1)) : 0;
+      tp[j] = (uint16_t)p;
+      tr[j] = (uint16_t)atomicAdd(&cnt[p], 1u);
+    }
+    __syncthreads();
+    // exclusive scan of cnt[0..nparts] into tstart: per-thread runs, then a
+    // workgroup scan of the run totals
+    const uint32_t q0 = threadIdx.x * per;
+    uint32_t run = 0;
+    for (uint32_t q = q0; q < q0 + per && q <= nparts; ++q) run += cnt[q];
+    uint64_t tile_tot = 0;
+    run = (uint32_t)block_exclusive_scan(run, &tile_tot);
+    for (uint32_t q = q0; q < q0 + per && q <= nparts; ++q) {
+      tstart[q] = run;
+      run += cnt[q];
+    }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < tlen; j += blockDim.x) perm[tstart[tp[j]] + tr[j]] = (uint16_t)j;
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < tlen; k += blockDim.x) {
+      const uint32_t j = perm[k];
+      const uint32_t p = tp[j];
+      const uint64_t dst = cursor[p] + (k - tstart[p]);
+      rec_key[dst] = tk[j];
+      rec_val[dst] = tv[j];
+      rec_grp[dst] = tg[j];
+    }
+    __syncthreads();
+    for (uint32_t p = threadIdx.x; p <= nparts; p += blockDim.x) cursor[p] += cnt[p];
+    __syncthreads();
+  }
+}
+
+// One workgroup per partition (partition nparts: the out-of-range rows, whose
+// matches are only counted).  Partition p's records are
+// [mat[p * G], mat[(p + 1) * G]).
+__global__ void __launch_bounds__(256) k_part_agg(JoinTableDev T, ProbeAggDev Q, uint32_t nparts,
+                                                  uint64_t G, const unsigned long long* mat,
+                                                  const unsigned long long* rec_key,
+                                                  const long long* rec_val,
+                                                  const uint32_t* rec_grp) {
+  __shared__ unsigned long long cnt[1 << kPartShift], slo[1 << kPartShift], shi[1 << kPartShift];
+  constexpr int U = 4;
+  const uint32_t p = blockIdx.x;
+  const uint64_t r0 = mat[(uint64_t)p * G], r1 = mat[(uint64_t)(p + 1) * G];
+  const bool oor_part = p == nparts;
+  if (!oor_part) {
+    for (uint32_t g = threadIdx.x; g < (1u << kPartShift); g += blockDim.x) {
+      cnt[g] = 0;
+      slo[g] = 0;
+      shi[g] = 0;
+    }
+    __syncthreads();
+  }
+  unsigned long long m = 0;
+  for (uint64_t rb = r0 + threadIdx.x; rb < r1; rb += (uint64_t)blockDim.x * U) {
+    unsigned long long k[U];
+    int64_t v[U];
+    uint32_t g[U];
+    bool ok[U];
+    static_for<U>([&](auto I) {
+      constexpr int u = decltype(I)::value;
+      const uint64_t r = rb + (uint64_t)u * blockDim.x;
+      ok[u] = r < r1;
+      k[u] = ok[u] ? rec_key[r] : 0;
+      v[u] = ok[u] ? rec_val[r] : 0;
+      g[u] = ok[u] ? rec_grp[r] : 0;
+    });
+    if (oor_part) {
+      probe_keys(T, k, ok, [&](auto, uint32_t, uint32_t) { ++m; });
+    } else {
+      probe_keys(T, k, ok, [&](auto I, uint32_t, uint32_t) {
+        constexpr int u = decltype(I)::value;
+        atomicAdd(&cnt[g[u]], 1ull);
+        atomicAdd(&slo[g[u]], (unsigned long long)(uint32_t)v[u]);
+        const int64_t h = v[u] >> 32;
+        if (h) atomicAdd(&shi[g[u]], (unsigned long long)h);
+      });
+    }
+  }
+  if (oor_part) {
+    if (m && Q.out_oor) atomicAdd(Q.out_oor, m);
+    return;
+  }
+  __syncthreads();
+  const uint64_t g0 = (uint64_t)p << kPartShift;
+  for (uint32_t gl = threadIdx.x; gl < (1u << kPartShift); gl += blockDim.x) {
+    if (g0 + gl >= Q.ngroups) break;
+    uint64_t lo;
+    int64_t hi;
+    split_to_i128(slo[gl], (int64_t)shi[gl], &lo, &hi);
+    Q.out_count[g0 + gl] = cnt[gl];
+    Q.out_sums[2 * (g0 + gl)] = lo;
+    Q.out_sums[2 * (g0 + gl) + 1] = (unsigned long long)hi;
   }
 }
 
@@ -316,28 +658,6 @@ __global__ void __launch_bounds__(256) k_pairs_count(JoinTableDev T, const void*
   if (threadIdx.x == 0) block_tot[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
-__global__ void __launch_bounds__(1024) k_scan_blocks(uint64_t* v, uint64_t n) {
-  __shared__ unsigned long long part[1024];
-  const uint64_t per = (n + 1023) / 1024;
-  const uint64_t lo = threadIdx.x * per, hi = min(lo + per, n);
-  unsigned long long s = 0;
-  for (uint64_t i = lo; i < hi; ++i) s += v[i];
-  part[threadIdx.x] = s;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const unsigned long long x = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
-    __syncthreads();
-    part[threadIdx.x] += x;
-    __syncthreads();
-  }
-  unsigned long long run = threadIdx.x ? part[threadIdx.x - 1] : 0;
-  for (uint64_t i = lo; i < hi; ++i) {
-    const unsigned long long c = v[i];
-    v[i] = run;
-    run += c;
-  }
-  if (threadIdx.x == 1023) v[n] = part[1023];
-}
 
 __global__ void __launch_bounds__(256) k_pairs_write(JoinTableDev T, const void* keys, int32_t kdt,
                                                      uint64_t b, uint64_t e,
@@ -642,15 +962,52 @@ int32_t tgp_hash_probe_agg(tgp_ctx* ctx, const tgp_join_agg_query* q, const tgp_
   Q.out_count = reinterpret_cast<unsigned long long*>(out->d_count);
   Q.out_sums = reinterpret_cast<unsigned long long*>(out->d_sums);
   Q.out_oor = reinterpret_cast<unsigned long long*>(out->d_n_out_of_range);
-  const uint64_t lds = 3 * 8 * q->ngroups;
   const unsigned grid = grid_for((n + kProbeUnroll - 1) / kProbeUnroll,
                                  (uint64_t)persistent_grid(ctx, 8));
-  if (lds <= 48 * 1024)
-    k_probe_agg<true><<<grid, 256, lds, s>>>(q->table->dev(), Q);
-  else {
+  // LDS tables: per wave with as many copies as fit in 48 KB, else one per
+  // workgroup, else the radix-partitioned form (probe-column groups) or global
+  // atomics
+  constexpr uint64_t kLdsMax = 48 * 1024;
+  const uint64_t tab = 3 * 8 * q->ngroups;
+  int rshift = -1;
+  uint32_t shared_table = 0;
+  for (int r = 6; r >= 0; --r)
+    if ((tab << r) * 4 <= kLdsMax) { rshift = r; break; }
+  if (rshift < 0 && tab <= kLdsMax) { rshift = 0; shared_table = 1; }
+  const uint64_t nparts = (q->ngroups + (1u << kPartShift) - 1) >> kPartShift;
+  if (rshift >= 0) {
+    const uint64_t lds = (tab << rshift) * (shared_table ? 1 : 4);
+    k_probe_agg<true><<<grid, 256, lds, s>>>(q->table->dev(), Q, (uint32_t)rshift, shared_table);
+  } else if (q->group_by == TGP_JOIN_GROUP_PROBE_COL && nparts < kMaxParts &&
+             ctx->tune_join_partition.load() == 0) {
+    TGP_REQUIRE(n < (1ull << 32),
+                "tgp_hash_probe_agg: a partitioned call takes fewer than 2^32 probe rows");
+    // scratch: mat[(nparts + 1) * G + 1] | keys[n] | vals[n] | groups[n]
+    const uint64_t G = (uint64_t)persistent_grid(ctx, 2) < (n + kPartTile - 1) / kPartTile
+                           ? (uint64_t)persistent_grid(ctx, 2)
+                           : (n + kPartTile - 1) / kPartTile;
+    const uint64_t mat_n = (nparts + 1) * G + 1;
+    const uint64_t nsb = (mat_n - 1 + kScanChunk - 1) / kScanChunk + 1;  // scan block sums
+    const uint64_t head = (8 * (mat_n + nsb) + 255) & ~255ull;
+    const uint64_t bytes = head + 16 * n + 4 * n + 256;
+    void* scr = nullptr;
+    TGP_TRY(get_scratch(ctx, s, bytes, &scr));
+    auto* mat = static_cast<unsigned long long*>(scr);
+    auto* rec_key = reinterpret_cast<unsigned long long*>(static_cast<uint8_t*>(scr) + head);
+    auto* rec_val = reinterpret_cast<long long*>(rec_key + n);
+    auto* rec_grp = reinterpret_cast<uint32_t*>(rec_val + n);
+    const uint32_t np = (uint32_t)nparts;
+    k_part_hist<<<(unsigned)G, 256, (np + 1) * 4, s>>>(Q, np, mat);
+    TGP_TRY(scan_u64(s, reinterpret_cast<uint64_t*>(mat), mat_n - 1,
+                     reinterpret_cast<uint64_t*>(mat + mat_n)));
+    const uint32_t lds = (np + 1) * 16 + kPartTile * (16 + 4 * 2) + 16;
+    k_part_scatter<<<(unsigned)G, 256, lds, s>>>(Q, np, mat, rec_key, rec_val, rec_grp);
+    k_part_agg<<<np + 1, 256, 0, s>>>(q->table->dev(), Q, np, G, mat, rec_key, rec_val,
+                                      rec_grp);
+  } else {
     TGP_REQUIRE(q->row_end - q->row_begin < (1ull << 32),
                 "tgp_hash_probe_agg: a global-table call takes fewer than 2^32 probe rows");
-    k_probe_agg<false><<<grid, 256, 0, s>>>(q->table->dev(), Q);
+    k_probe_agg<false><<<grid, 256, 0, s>>>(q->table->dev(), Q, 0, 0);
     TGP_HIP(hipGetLastError());
     k_split_finalize<<<grid_for(q->ngroups), 256, 0, s>>>(Q.out_sums, q->ngroups);
   }

@@ -34,16 +34,6 @@ namespace tgp {
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-// Calls f(std::integral_constant<int, I>{}) for I = 0 .. N-1 (compile-time loop).
-template <typename F, int... I>
-__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
-  (f(std::integral_constant<int, I>{}), ...);
-}
-template <int N, typename F>
-__device__ __forceinline__ void static_for(F&& f) {
-  static_for_impl(f, std::make_integer_sequence<int, N>{});
-}
-
 constexpr int kSR = 4;                 // rows per lane
 constexpr int kSliceRows = kWave * kSR;  // 256 rows per wave slice
 constexpr int kMaxSpecPreds = 4;

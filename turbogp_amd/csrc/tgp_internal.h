@@ -10,6 +10,8 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <type_traits>
+#include <utility>
 
 #include "../../include/turbogp_gfx950.h"
 
@@ -41,6 +43,17 @@ void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
     int32_t rc_ = (expr);             \
     if (rc_ != TGP_OK) return rc_;    \
   } while (0)
+
+// Calls f(std::integral_constant<int, I>{}) for I = 0 .. N-1 (compile-time loop:
+// arrays indexed by I stay in registers).
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
 
 // ---- launch geometry -----------------------------------------------------------
 constexpr int kWave = 64;
@@ -85,6 +98,7 @@ struct tgp_ctx {
   std::atomic<int64_t> tune_spec_kernel{0};  // 0: by group count; 1: register fold; 2: replicated
   std::atomic<int64_t> tune_chunk{0};        // 0: static interleave; n: chunks of n slices
   std::atomic<int64_t> tune_slices_per_wave{0};  // fewest slices per wave (0 = 2)
+  std::atomic<int64_t> tune_join_partition{0};   // 0: radix-partitioned join GROUP BY; 1: off
 };
 
 namespace tgp {
