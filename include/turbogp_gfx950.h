@@ -289,8 +289,10 @@ int32_t tgp_batch_set_queries(tgp_batch* b, const tgp_agg_query* qs, uint32_t n,
  * as many as fit); chunk_slices: 256-row slices per dynamically scheduled unit
  * (0 = default 32); order: 0 = units in query order (the slices of query 0,
  * then of query 1, ...), 1 = units sorted by table position, so the queries
- * that cover a table region read it at about the same time; cached_loads: 1 =
- * keep column lines in the caches (pays off with order 1), 0 = streaming loads;
+ * that cover a table region read it at about the same time, 2 = as 1 with
+ * units cut at absolute table chunks and chunk k served by XCD k % 8, so those
+ * queries also share one L2; cached_loads: 1 = keep column lines in the caches
+ * (pays off with orders 1 and 2), 0 = streaming loads;
  * use_graph: replay a captured hipGraph (0 = plain launches). */
 typedef struct tgp_batch_tuning {
   uint32_t wg_per_cu;

@@ -127,16 +127,29 @@ __global__ void __launch_bounds__(kBlock) k_batch(const BatchQuery* __restrict__
   }
 }
 
-// Persistent kernel, table order: work item i = (query items[2i], first slice
-// items[2i+1]) covers `chunk` slices of that query; the items are sorted by the
-// table row they start at, so the waves sweep the table once and the queries
-// that cover a region read it within a short time of each other (from the
-// caches after the first).
+// Persistent kernel, table order: work item i = (query items[3i], slices
+// [items[3i+1], items[3i+2]) of it).  The items form `nq` queues, each sorted by
+// the table row its items start at, so the waves sweep the table and the
+// queries that cover a region read it within a short time of each other (from
+// the caches after the first).  With one queue every wave takes from it; with
+// kQueues queues, table chunk k's items are in queue k % kQueues and
+// workgroup w takes from queue w % kQueues first: workgroups are dispatched to
+// the 8 XCDs of an MI355X round robin, so the queries of one region then share
+// one XCD's L2.  A wave whose queue is empty moves on to the next queue, so
+// all waves end together.  The mapping only affects locality, not results.
+constexpr uint32_t kQueues = 8;
+
+struct ItemQueues {
+  uint64_t off[kQueues + 1];  // items of queue i: [off[i], off[i + 1])
+  uint32_t nq;
+  uint32_t _pad;
+};
+
 template <bool NT>
 __global__ void __launch_bounds__(kBlock) k_batch_items(const BatchQuery* __restrict__ plan,
                                                         const uint32_t* __restrict__ items,
-                                                        uint64_t nitems, uint32_t chunk,
-                                                        unsigned long long* counter) {
+                                                        ItemQueues Q,
+                                                        unsigned long long* counters) {
   __shared__ unsigned long long lreg[(kBlock / kWave) * kWaveWords];
   ConstQuery* qs = (ConstQuery*)plan;
   const __attribute__((address_space(4))) uint32_t* it =
@@ -145,14 +158,17 @@ __global__ void __launch_bounds__(kBlock) k_batch_items(const BatchQuery* __rest
   unsigned long long* reg = lreg + wave * kWaveWords;
   for (int w = lane; w < kWaveWords; w += 64) reg[w] = 0;
   __builtin_amdgcn_wave_barrier();
-  uint64_t k = grab_chunk(counter);
-  while (k < nitems) {
-    const uint64_t next = grab_chunk(counter);
-    const uint32_t q = it[2 * k];
-    const uint64_t b = it[2 * k + 1];
-    const uint64_t n = qs[q].a.nslices;
-    batch_run_segment<NT>(qs, q, b, b + chunk < n ? b + chunk : n, reg);
-    k = next;
+  const uint32_t home = blockIdx.x % Q.nq;
+  for (uint32_t t = 0; t < Q.nq; ++t) {
+    const uint32_t qi = home + t < Q.nq ? home + t : home + t - Q.nq;
+    const uint64_t base = Q.off[qi], n = Q.off[qi + 1] - base;
+    uint64_t k = grab_chunk(&counters[qi]);
+    while (k < n) {
+      const uint64_t next = grab_chunk(&counters[qi]);  // in flight while this item runs
+      const uint64_t i = base + k;
+      batch_run_segment<NT>(qs, it[3 * i], it[3 * i + 1], it[3 * i + 2], reg);
+      k = next;
+    }
   }
 }
 
@@ -176,11 +192,14 @@ struct tgp_batch {
   // tuning / graph
   uint32_t wg_per_cu = 0;  // 0 = occupancy
   uint32_t chunk = 32;     // slices per dynamically scheduled chunk / work item
-  int32_t order = 0;       // 0: query order; 1: table order (work items)
+  int32_t order = 0;       // 0: query order; 1: table order; 2: table order, XCD-affine
   bool cached = false;     // cached instead of streaming column loads
   uint32_t* d_items = nullptr;
   uint64_t items_cap = 0, nitems = 0;
+  uint64_t queue_off[kQueues + 1] = {};
+  uint32_t nqueues = 1;
   uint32_t items_chunk = 0;
+  int32_t items_order = 0;
   bool items_valid = false;
   uint32_t* d_granule = nullptr;
   uint64_t granule_cap = 0;
@@ -215,6 +234,7 @@ static int32_t persistent_launch_grid(tgp_batch* b, K kern, uint64_t units, uint
 template <bool NT>
 static int32_t launch_variant(tgp_batch* b, hipStream_t s) {
   auto* counter = reinterpret_cast<unsigned long long*>(b->d_results + b->result_words);
+  // counter[0]: chunk counter of the query-ordered kernel; counter[0..nq): queue counters
   uint64_t grid = 0;
   if (b->order == 0) {
     auto kern = k_batch<NT>;
@@ -226,30 +246,68 @@ static int32_t launch_variant(tgp_batch* b, hipStream_t s) {
   } else {
     auto kern = k_batch_items<NT>;
     TGP_TRY(persistent_launch_grid(b, kern, b->nitems, &grid));
-    kern<<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>(b->d_plan, b->d_items, b->nitems,
-                                                       b->chunk, counter);
+    ItemQueues Q{};
+    Q.nq = b->nqueues;
+    for (uint32_t i = 0; i <= kQueues; ++i) Q.off[i] = b->queue_off[i];
+    kern<<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>(b->d_plan, b->d_items, Q, counter);
   }
   TGP_HIP(hipGetLastError());
   return TGP_OK;
 }
 
-// Work items of the table-ordered schedule for the current chunk size: every
-// query's slices in chunks, sorted by the table row each chunk starts at
+// Work items of the table-ordered schedules for the current chunk size
 // (synchronous upload; called outside graph capture).
+//   order 1: every query's slices in chunks of `chunk` slices, one queue sorted by
+//            the table row each item starts at;
+//   order 2: the table cut into absolute chunks of chunk * 256 rows; every query
+//            has one item per chunk it overlaps, so the items of all queries on
+//            chunk k cover the same rows; chunk k's items go to queue k % kQueues.
 static int32_t build_items(tgp_batch* b, hipStream_t s) {
-  if (b->order == 0 || (b->items_chunk == b->chunk && b->items_valid)) return TGP_OK;
-  std::vector<std::pair<uint64_t, uint64_t>> keyed;  // (start row, query << 32 | first slice)
+  if (b->order == 0 || (b->items_chunk == b->chunk && b->items_order == b->order &&
+                        b->items_valid))
+    return TGP_OK;
+  struct Item {
+    uint64_t key;  // sort key: start row (order 1) or absolute chunk (order 2)
+    uint32_t q, sb, se;
+  };
+  const uint32_t nq = b->order == 2 ? kQueues : 1;
+  std::vector<std::vector<Item>> queues(nq);
+  const uint64_t C = (uint64_t)b->chunk * kSliceRows;  // rows per absolute chunk
   for (uint32_t q = 0; q < b->plan.size(); ++q) {
     const SpecArgs& A = b->plan[q].a;
-    for (uint64_t c = 0; c < A.nslices; c += b->chunk)
-      keyed.emplace_back(A.base0 + c * kSliceRows, ((uint64_t)q << 32) | c);
+    if (A.nslices == 0) continue;
+    if (b->order == 1) {
+      for (uint64_t c = 0; c < A.nslices; c += b->chunk)
+        queues[0].push_back(Item{A.base0 + c * kSliceRows, q, (uint32_t)c,
+                                 (uint32_t)(c + b->chunk < A.nslices ? c + b->chunk : A.nslices)});
+    } else {
+      const uint64_t end = A.base0 + A.nslices * kSliceRows;
+      for (uint64_t k = A.base0 / C; k * C < end; ++k) {
+        const uint64_t r0 = k * C > A.base0 ? k * C : A.base0;
+        const uint64_t r1 = (k + 1) * C < end ? (k + 1) * C : end;
+        queues[k % nq].push_back(Item{k, q, (uint32_t)((r0 - A.base0) / kSliceRows),
+                                      (uint32_t)((r1 - A.base0) / kSliceRows)});
+      }
+    }
   }
-  std::sort(keyed.begin(), keyed.end());
-  std::vector<uint32_t> items(2 * (keyed.empty() ? 1 : keyed.size()), 0);
-  for (size_t i = 0; i < keyed.size(); ++i) {
-    items[2 * i] = (uint32_t)(keyed[i].second >> 32);
-    items[2 * i + 1] = (uint32_t)keyed[i].second;
+  std::vector<uint32_t> items;
+  uint64_t total = 0;
+  for (auto& qv : queues) total += qv.size();
+  items.reserve(3 * (total ? total : 1));
+  for (uint32_t i = 0; i < nq; ++i) {
+    auto& qv = queues[i];
+    std::sort(qv.begin(), qv.end(), [](const Item& x, const Item& y) {
+      return x.key != y.key ? x.key < y.key : x.q < y.q;
+    });
+    b->queue_off[i] = items.size() / 3;
+    for (const Item& it : qv) {
+      items.push_back(it.q);
+      items.push_back(it.sb);
+      items.push_back(it.se);
+    }
   }
+  for (uint32_t i = nq; i <= kQueues; ++i) b->queue_off[i] = items.size() / 3;
+  if (items.empty()) items.assign(3, 0);
   if (b->items_cap < items.size()) {
     if (b->d_items) TGP_TRY(hbm_free(b->ctx, b->d_items));
     b->d_items = nullptr;
@@ -260,15 +318,17 @@ static int32_t build_items(tgp_batch* b, hipStream_t s) {
   }
   TGP_HIP(hipMemcpyAsync(b->d_items, items.data(), items.size() * 4, hipMemcpyHostToDevice, s));
   TGP_HIP(hipStreamSynchronize(s));
-  b->nitems = keyed.size();
+  b->nitems = total;
+  b->nqueues = nq;
   b->items_chunk = b->chunk;
+  b->items_order = b->order;
   b->items_valid = true;
   return TGP_OK;
 }
 
 static int32_t batch_launch(tgp_batch* b, hipStream_t s) {
-  // results and the chunk counter behind them start at zero
-  TGP_TRY(fill_u64(s, b->d_results, b->result_words + 1, 0));
+  // results and the counters behind them start at zero
+  TGP_TRY(fill_u64(s, b->d_results, b->result_words + kQueues, 0));
   if (!b->total_slices) return TGP_OK;
   return b->cached ? launch_variant<false>(b, s) : launch_variant<true>(b, s);
 }
@@ -326,14 +386,14 @@ int32_t tgp_batch_set_queries(tgp_batch* b, const tgp_agg_query* qs, uint32_t n,
     words += 1;
   }
   b->result_words = words;
-  // the arena ends with the persistent kernel's chunk counter
-  if (b->results_cap < words + 1) {
+  // the arena ends with the persistent kernels' work counters
+  if (b->results_cap < words + kQueues) {
     if (b->d_results) TGP_TRY(hbm_free(b->ctx, b->d_results));
     b->d_results = nullptr;
     void* p = nullptr;
-    TGP_TRY(hbm_alloc(b->ctx, (words + 1) * 8, &p));
+    TGP_TRY(hbm_alloc(b->ctx, (words + kQueues) * 8, &p));
     b->d_results = static_cast<uint64_t*>(p);
-    b->results_cap = words + 1;
+    b->results_cap = words + kQueues;
   }
   uint64_t slices = 0;
   for (uint32_t i = 0; i < n; ++i) {
@@ -397,7 +457,8 @@ int32_t tgp_batch_set_queries(tgp_batch* b, const tgp_agg_query* qs, uint32_t n,
 int32_t tgp_batch_set_tuning(tgp_batch* b, const tgp_batch_tuning* t) {
   TGP_REQUIRE(b && t, "tgp_batch_set_tuning: null argument");
   TGP_REQUIRE(t->chunk_slices <= (1u << 16), "chunk of %u slices above 65536", t->chunk_slices);
-  TGP_REQUIRE(t->order == 0 || t->order == 1, "order %d not 0 (queries) or 1 (table)", t->order);
+  TGP_REQUIRE(t->order >= 0 && t->order <= 2, "order %d not 0 (queries), 1 (table) or 2 "
+              "(table, XCD-affine)", t->order);
   drop_graph(b);
   b->wg_per_cu = t->wg_per_cu;
   b->chunk = t->chunk_slices == 0 ? 32 : t->chunk_slices;
@@ -576,6 +637,7 @@ int32_t tgp_batch_autotune(tgp_batch* b, void* stream, tgp_batch_tuning* best_ou
   for (uint32_t chunk : {16u, 32u, 64u, 128u}) trial(0, chunk, 0, false);
   for (int cached = 0; cached < 2; ++cached)
     for (uint32_t chunk : {8u, 16u, 32u, 64u}) trial(0, chunk, 1, cached != 0);
+  for (uint32_t chunk : {8u, 16u, 32u, 64u}) trial(0, chunk, 2, true);
   const tgp_batch_tuning at_occ = best_t;
   for (uint32_t wg = 2; wg <= 4; ++wg)
     trial(wg, at_occ.chunk_slices, at_occ.order, at_occ.cached_loads != 0);
