@@ -24,7 +24,7 @@ Also measured in the same run:
 Algorithmic bytes of a query: rows in its range x native widths of the columns
 it reads (date 4 B + group-key columns 1 B each + four 8-byte summed columns).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--tuning wg,chunk,order,cached]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--tuning wg,chunk,order,cached,grab]
        python bench.py --calibrate   (known-size kernels for scaling PMC byte counters:
                                       k_fill_u64 writes and k_minmax reads exactly 512 MiB)
 """
@@ -264,8 +264,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--sections", default="batch,scan,join,serve",
+                    help="which parts to run (profiling runs restrict them so a kernel's "
+                         "launches all come from one workload)")
     ap.add_argument("--tuning", default="",
-                    help="wg_per_cu,chunk_slices,order,cached_loads for the batch instead of "
+                    help="wg_per_cu,chunk_slices,order,cached_loads,items_per_grab for the batch "
+                         "instead of "
                          "the autotuner (profiling runs that must time one configuration)")
     args = ap.parse_args()
     if args.calibrate:
@@ -297,8 +301,8 @@ def main():
     info = b.info()
     assert info["fallback"] == 0, info
     if args.tuning:
-        wg, chunk, order, cached = (int(x) for x in args.tuning.split(","))
-        b.set_tuning(wg, True, chunk, order, bool(cached))
+        wg, chunk, order, cached, grab = (int(x) for x in args.tuning.split(","))
+        b.set_tuning(wg, True, chunk, order, bool(cached), grab)
         tuned = b.tuning()
         log(f"tuning given: {tuned}")
     else:
@@ -367,8 +371,12 @@ def main():
     for e in ev + ev3:
         e.free()
 
-    join = bench_join(ctx, args)
-    serve = bench_serve(ctx, table, join.pop("_tables"), args)
+    sections = set(args.sections.split(","))
+    join = bench_join(ctx, args) if "join" in sections or "serve" in sections else None
+    join_tables = join.pop("_tables") if join else None
+    serve = bench_serve(ctx, table, join_tables, args) if "serve" in sections else None
+    if "join" not in sections:
+        join = None
 
     achieved = batch_bytes / 1e9 / (ms_batch / 1e3)
     nq = len(queries)

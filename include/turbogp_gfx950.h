@@ -293,13 +293,16 @@ int32_t tgp_batch_set_queries(tgp_batch* b, const tgp_agg_query* qs, uint32_t n,
  * units cut at absolute table chunks and chunk k served by XCD k % 8, so those
  * queries also share one L2; cached_loads: 1 = keep column lines in the caches
  * (pays off with orders 1 and 2), 0 = streaming loads;
- * use_graph: replay a captured hipGraph (0 = plain launches). */
+ * use_graph: replay a captured hipGraph (0 = plain launches); items_per_grab:
+ * work units a wave takes per counter atomic in orders 1 and 2 (0 = 1; at
+ * most 64). */
 typedef struct tgp_batch_tuning {
   uint32_t wg_per_cu;
   uint32_t chunk_slices;
   int32_t order;
   int32_t cached_loads;
   int32_t use_graph;
+  uint32_t items_per_grab;
 } tgp_batch_tuning;
 int32_t tgp_batch_set_tuning(tgp_batch* b, const tgp_batch_tuning* t);
 int32_t tgp_batch_get_tuning(tgp_batch* b, tgp_batch_tuning* t);

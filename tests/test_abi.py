@@ -4,6 +4,7 @@ import glob
 import os
 import re
 
+import numpy as np
 import pytest
 
 from turbogp_amd import _capi
@@ -94,3 +95,40 @@ def test_benchmark_queries_take_the_specialised_kernels():
     assert _explain((names, [(0, _capi.GE, 731, False)], parts, aggs))[1] == -1
     # strategies of the generic engine
     assert _explain(W.q1_spec(group="ab"))[0] == 1 and _explain(W.q1_spec(group="month"))[0] == 2
+
+
+def test_batch_and_serve_entry_points_validate_without_a_device():
+    L = _capi.lib()
+    h = ctypes.c_void_p()
+    assert L.tgp_batch_create(None, 16, ctypes.byref(h)) == _capi.ERR_INVALID
+    assert L.tgp_batch_set_tuning(None, None) == _capi.ERR_INVALID
+    t = _capi.BatchTuning()
+    assert L.tgp_batch_get_tuning(None, ctypes.byref(t)) == _capi.ERR_INVALID
+    cfg = _capi.ServeConfig(1024, 2, 2, 0, 0, 0, 0, 32, 2, 1, 0, 0)
+    assert L.tgp_serve_create(None, ctypes.byref(cfg), ctypes.byref(h)) == _capi.ERR_INVALID
+    assert L.tgp_serve_submit(None, None, 0, None) == _capi.ERR_INVALID
+    assert L.tgp_serve_destroy(None) == _capi.OK
+    assert L.tgp_serve_now_ns() > 0
+
+
+def test_serve_struct_sizes_match_c_layout():
+    assert ctypes.sizeof(_capi.BatchTuning) == 24
+    assert ctypes.sizeof(_capi.ServeConfig) == 56
+    assert ctypes.sizeof(_capi.ServeTiming) == 40
+    assert ctypes.sizeof(_capi.ServeJoin) == 2 * 24 + ctypes.sizeof(_capi.JoinAggQuery)
+    assert ctypes.sizeof(_capi.ServeRequest) == 8 + ctypes.sizeof(_capi.AggQuery) + \
+        ctypes.sizeof(_capi.ServeJoin)
+
+
+def test_serve_mix_is_seeded_and_in_range():
+    from turbogp_amd import workloads as W
+    a = W.serve_mix(500, 1_000_000, 20_000, 3_000_000)
+    assert a == W.serve_mix(500, 1_000_000, 20_000, 3_000_000)
+    kinds = {k: sum(1 for m in a if m[0] == k) for k in ("q1", "q6", "join", "full")}
+    assert 0.6 < (kinds["q1"] + kinds["q6"]) / 500 < 0.8 and 0.18 < kinds["join"] / 500 < 0.32
+    for kind, b, e, nb in a:
+        limit = 3_000_000 if kind == "join" else 1_000_000
+        assert 0 <= b < e <= limit
+        assert (kind == "join") == (nb > 0) and nb <= 20_000
+    off = W.poisson_offsets(1000, 2000.0)
+    assert (np.diff(off) >= 0).all() and 0.3e9 < off[-1] < 0.7e9  # ~1000 / 2000 per s

@@ -66,7 +66,8 @@ def test_batch_matches_single_queries_and_oracle(ctx, table):
         b.set_tuning(0, use_graph=True, chunk_slices=1, order=1, cached_loads=False)
         b.run(A.BATCH_PERSISTENT)
         table1 = b.results()
-        b.set_tuning(2, use_graph=True, chunk_slices=3, order=2, cached_loads=True)
+        b.set_tuning(2, use_graph=True, chunk_slices=3, order=2, cached_loads=True,
+                     items_per_grab=5)
         b.run(A.BATCH_PERSISTENT)
         table2 = b.results()
     finally:

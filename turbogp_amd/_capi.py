@@ -107,7 +107,8 @@ I64_ = C.c_int64
 
 class BatchTuning(C.Structure):
     _fields_ = [("wg_per_cu", C.c_uint32), ("chunk_slices", C.c_uint32), ("order", C.c_int32),
-                ("cached_loads", C.c_int32), ("use_graph", C.c_int32)]
+                ("cached_loads", C.c_int32), ("use_graph", C.c_int32),
+                ("items_per_grab", C.c_uint32)]
 
 
 SERVE_AGG, SERVE_JOIN_AGG = 0, 1

@@ -23,14 +23,16 @@ class Batch:
         self.naggs = [q.naggs for q in queries]
 
     def set_tuning(self, wg_per_cu=0, use_graph=True, chunk_slices=0, order=0,
-                   cached_loads=False):
-        t = A.BatchTuning(wg_per_cu, chunk_slices, order, int(cached_loads), int(use_graph))
+                   cached_loads=False, items_per_grab=1):
+        t = A.BatchTuning(wg_per_cu, chunk_slices, order, int(cached_loads), int(use_graph),
+                          items_per_grab)
         A.check(A.lib().tgp_batch_set_tuning(self.handle, C.byref(t)), "tgp_batch_set_tuning")
 
     @staticmethod
     def _tuning_dict(t):
         return {"wg_per_cu": t.wg_per_cu, "chunk_slices": t.chunk_slices, "order": t.order,
-                "cached_loads": bool(t.cached_loads), "use_graph": bool(t.use_graph)}
+                "cached_loads": bool(t.cached_loads), "use_graph": bool(t.use_graph),
+                "items_per_grab": t.items_per_grab}
 
     def tuning(self):
         t = A.BatchTuning()

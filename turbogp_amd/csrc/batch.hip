@@ -149,7 +149,8 @@ template <bool NT>
 __global__ void __launch_bounds__(kBlock) k_batch_items(const BatchQuery* __restrict__ plan,
                                                         const uint32_t* __restrict__ items,
                                                         ItemQueues Q,
-                                                        unsigned long long* counters) {
+                                                        unsigned long long* counters,
+                                                        uint32_t per_grab) {
   __shared__ unsigned long long lreg[(kBlock / kWave) * kWaveWords];
   ConstQuery* qs = (ConstQuery*)plan;
   const __attribute__((address_space(4))) uint32_t* it =
@@ -162,11 +163,16 @@ __global__ void __launch_bounds__(kBlock) k_batch_items(const BatchQuery* __rest
   for (uint32_t t = 0; t < Q.nq; ++t) {
     const uint32_t qi = home + t < Q.nq ? home + t : home + t - Q.nq;
     const uint64_t base = Q.off[qi], n = Q.off[qi + 1] - base;
-    uint64_t k = grab_chunk(&counters[qi]);
+    // `per_grab` consecutive items per counter atomic: the same table chunk for
+    // several queries, so a wave re-reads lines it just loaded
+    uint64_t k = grab_chunks(&counters[qi], per_grab);
     while (k < n) {
-      const uint64_t next = grab_chunk(&counters[qi]);  // in flight while this item runs
-      const uint64_t i = base + k;
-      batch_run_segment<NT>(qs, it[3 * i], it[3 * i + 1], it[3 * i + 2], reg);
+      const uint64_t next = grab_chunks(&counters[qi], per_grab);  // in flight meanwhile
+      const uint64_t kend = k + per_grab < n ? k + per_grab : n;
+      for (uint64_t j = k; j < kend; ++j) {
+        const uint64_t i = base + j;
+        batch_run_segment<NT>(qs, it[3 * i], it[3 * i + 1], it[3 * i + 2], reg);
+      }
       k = next;
     }
   }
@@ -192,6 +198,7 @@ struct tgp_batch {
   // tuning / graph
   uint32_t wg_per_cu = 0;  // 0 = occupancy
   uint32_t chunk = 32;     // slices per dynamically scheduled chunk / work item
+  uint32_t per_grab = 1;   // work items per counter atomic (table orders)
   int32_t order = 0;       // 0: query order; 1: table order; 2: table order, XCD-affine
   bool cached = false;     // cached instead of streaming column loads
   uint32_t* d_items = nullptr;
@@ -245,11 +252,12 @@ static int32_t launch_variant(tgp_batch* b, hipStream_t s) {
                                                        counter);
   } else {
     auto kern = k_batch_items<NT>;
-    TGP_TRY(persistent_launch_grid(b, kern, b->nitems, &grid));
+    TGP_TRY(persistent_launch_grid(b, kern, (b->nitems + b->per_grab - 1) / b->per_grab, &grid));
     ItemQueues Q{};
     Q.nq = b->nqueues;
     for (uint32_t i = 0; i <= kQueues; ++i) Q.off[i] = b->queue_off[i];
-    kern<<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>(b->d_plan, b->d_items, Q, counter);
+    kern<<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>(b->d_plan, b->d_items, Q, counter,
+                                                       b->per_grab);
   }
   TGP_HIP(hipGetLastError());
   return TGP_OK;
@@ -459,12 +467,14 @@ int32_t tgp_batch_set_tuning(tgp_batch* b, const tgp_batch_tuning* t) {
   TGP_REQUIRE(t->chunk_slices <= (1u << 16), "chunk of %u slices above 65536", t->chunk_slices);
   TGP_REQUIRE(t->order >= 0 && t->order <= 2, "order %d not 0 (queries), 1 (table) or 2 "
               "(table, XCD-affine)", t->order);
+  TGP_REQUIRE(t->items_per_grab <= 64, "items_per_grab %u above 64", t->items_per_grab);
   drop_graph(b);
   b->wg_per_cu = t->wg_per_cu;
   b->chunk = t->chunk_slices == 0 ? 32 : t->chunk_slices;
   b->order = t->order;
   b->cached = t->cached_loads != 0;
   b->use_graph = t->use_graph != 0;
+  b->per_grab = t->items_per_grab == 0 ? 1 : t->items_per_grab;
   return build_items(b, b->ctx->stream);
 }
 
@@ -475,6 +485,7 @@ int32_t tgp_batch_get_tuning(tgp_batch* b, tgp_batch_tuning* t) {
   t->order = b->order;
   t->cached_loads = b->cached ? 1 : 0;
   t->use_graph = b->use_graph ? 1 : 0;
+  t->items_per_grab = b->per_grab;
   return TGP_OK;
 }
 
@@ -619,34 +630,43 @@ int32_t tgp_batch_autotune(tgp_batch* b, void* stream, tgp_batch_tuning* best_ou
   };
   tgp_batch_tuning best_t{};
   float best = 1e30f;
-  auto trial = [&](uint32_t wg, uint32_t chunk, int32_t order, bool cached) {
+  auto trial = [&](uint32_t wg, uint32_t chunk, int32_t order, bool cached, uint32_t grab) {
     if (rc != TGP_OK) return;
     b->wg_per_cu = wg;
     b->chunk = chunk;
     b->order = order;
     b->cached = cached;
+    b->per_grab = grab;
     rc = build_items(b, s);
     float ms = 0;
     if (rc == TGP_OK) rc = time_it(&ms);
     if (rc == TGP_OK && ms < best) {
       best = ms;
-      best_t = tgp_batch_tuning{wg, chunk, order, cached ? 1 : 0, b->use_graph ? 1 : 0};
+      best_t = tgp_batch_tuning{wg, chunk, order, cached ? 1 : 0, b->use_graph ? 1 : 0, grab};
     }
   };
-  // chunk sizes of both schedules at full occupancy, then workgroups per CU
-  for (uint32_t chunk : {16u, 32u, 64u, 128u}) trial(0, chunk, 0, false);
+  // coordinate search: the schedules' chunk sizes at full occupancy, items per
+  // grab for the best table-ordered setting, then workgroups per CU
+  for (uint32_t chunk : {16u, 32u, 64u, 128u}) trial(0, chunk, 0, false, 1);
   for (int cached = 0; cached < 2; ++cached)
-    for (uint32_t chunk : {8u, 16u, 32u, 64u}) trial(0, chunk, 1, cached != 0);
-  for (uint32_t chunk : {8u, 16u, 32u, 64u}) trial(0, chunk, 2, true);
+    for (uint32_t chunk : {8u, 16u, 32u, 64u}) trial(0, chunk, 1, cached != 0, 1);
+  for (uint32_t chunk : {8u, 16u, 32u, 64u}) trial(0, chunk, 2, true, 1);
+  if (best_t.order != 0) {
+    const tgp_batch_tuning t1 = best_t;
+    for (uint32_t chunk : {4u, 8u, 16u, 32u})
+      for (uint32_t grab : {2u, 4u, 8u}) trial(0, chunk, t1.order, t1.cached_loads != 0, grab);
+  }
   const tgp_batch_tuning at_occ = best_t;
   for (uint32_t wg = 2; wg <= 4; ++wg)
-    trial(wg, at_occ.chunk_slices, at_occ.order, at_occ.cached_loads != 0);
+    trial(wg, at_occ.chunk_slices, at_occ.order, at_occ.cached_loads != 0,
+          at_occ.items_per_grab);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   b->wg_per_cu = best_t.wg_per_cu;
   b->chunk = best_t.chunk_slices;
   b->order = best_t.order;
   b->cached = best_t.cached_loads != 0;
+  b->per_grab = best_t.items_per_grab == 0 ? 1 : best_t.items_per_grab;
   drop_graph(b);
   if (rc == TGP_OK) rc = build_items(b, s);
   if (rc != TGP_OK) return rc;

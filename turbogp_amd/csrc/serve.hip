@@ -486,7 +486,7 @@ int32_t tgp_serve_create(tgp_ctx* ctx, const tgp_serve_config* cfg, tgp_server**
     if (rc == TGP_OK) rc = tgp_batch_create(ctx, cfg->max_batch_queries, &sl.batch);
     if (rc == TGP_OK) {
       const tgp_batch_tuning t{cfg->wg_per_cu, cfg->chunk_slices, cfg->order, cfg->cached_loads,
-                               0};
+                               0, 1};
       rc = tgp_batch_set_tuning(sl.batch, &t);
     }
     if (rc == TGP_OK && (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||

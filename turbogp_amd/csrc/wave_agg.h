@@ -136,6 +136,15 @@ __device__ __forceinline__ void wave_flush(unsigned long long* out_count,
   __builtin_amdgcn_wave_barrier();
 }
 
+// Takes `n` consecutive work indices from a device counter (one atomic per
+// wave); returns the first.
+__device__ __forceinline__ uint64_t grab_chunks(unsigned long long* counter, uint32_t n) {
+  unsigned long long k = 0;
+  if ((threadIdx.x & 63) == 0) k = atomicAdd(counter, (unsigned long long)n);
+  return __builtin_amdgcn_readfirstlane((uint32_t)k) |
+         ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(k >> 32)) << 32);
+}
+
 // Takes the next chunk index from a device counter (one atomic per wave).
 __device__ __forceinline__ uint64_t grab_chunk(unsigned long long* counter) {
   unsigned long long k = 0;

@@ -64,7 +64,7 @@ def parse_words(words, ngroups, naggs):
 
 class Server:
     def __init__(self, ctx, max_batch_queries=1024, batch_slots=2, join_slots=2, heavy_rows=0,
-                 hash_budget_bytes=0, wg_per_cu=0, chunk_slices=32, order=1, cached_loads=True,
+                 hash_budget_bytes=0, wg_per_cu=0, chunk_slices=32, order=2, cached_loads=True,
                  per_query_launch=False):
         self.ctx = ctx
         cfg = A.ServeConfig(max_batch_queries, batch_slots, join_slots, 0, heavy_rows,
