@@ -334,6 +334,14 @@ typedef struct tgp_hash_table tgp_hash_table;
  * accounted in the HBM tier.  Synchronous (returns once the table is built). */
 int32_t tgp_hash_build(tgp_ctx* ctx, const tgp_column* keys, const tgp_column* payload,
                        tgp_hash_table** out, void* stream);
+/* As tgp_hash_build, with the slots in caller memory (16-byte aligned, at least
+ * tgp_hash_table_bytes_for(keys->len) bytes; e.g. a buffer a server reuses for
+ * one join after another): no device allocation and no device-wide
+ * synchronisation of a free when the table is destroyed, which leaves the
+ * caller's memory alone.  Build keys equal to INT64_MIN (rare) still take a
+ * small allocation of their own. */
+int32_t tgp_hash_build_in(tgp_ctx* ctx, const tgp_column* keys, const tgp_column* payload,
+                          void* d_mem, uint64_t mem_bytes, tgp_hash_table** out, void* stream);
 int32_t tgp_hash_table_destroy(tgp_ctx* ctx, tgp_hash_table* t);
 /* Bytes of HBM the table holds (admission control). */
 int32_t tgp_hash_table_footprint(const tgp_hash_table* t, uint64_t* out_bytes);

@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 from oracle import binding as B
-from turbogp_amd import DeviceColumn, HashTable, _capi as A
+from turbogp_amd import DeviceBuffer, DeviceColumn, HashTable, _capi as A
 from turbogp_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -88,6 +88,40 @@ def test_random_pairs_vs_oracle(ctx, seed):
     got, _ = pairs(ctx, bk, pk)
     oi, ob, n = B.join_pairs(bk, pk)
     assert got == list(zip(oi.tolist(), ob.tolist()))
+
+
+def test_build_into_caller_memory_reused(ctx):
+    """Tables built one after another in the same caller buffer (largest
+    first, then smaller ones over its stale slots) answer as freshly
+    allocated ones do; a buffer too small is an error."""
+    rng = np.random.default_rng(11)
+    mem = DeviceBuffer(ctx, A.lib().tgp_hash_table_bytes_for(20_000))
+    try:
+        for nb in (20_000, 3_000, 17):
+            bk = rng.integers(-500, 9_000, nb).astype(np.int64)
+            bk[rng.integers(0, nb, 2)] = I64MIN  # sentinel keys take the side list
+            pk = rng.integers(-600, 9_100, 40_000).astype(np.int64)
+            pk[::97] = I64MIN
+            kc, pc = dev(ctx, bk), dev(ctx, pk)
+            t = HashTable(ctx, kc, memory=mem)
+            try:
+                assert t.info()[0] == nb
+                pi, pp = t.probe_pairs(pc)
+                oi, ob, n = B.join_pairs(bk, pk)
+                assert list(zip(pi.tolist(), pp.tolist())) == list(zip(oi.tolist(), ob.tolist()))
+                assert t.probe_count(pc) == n
+            finally:
+                t.free()
+                kc.free()
+                pc.free()
+        kc = dev(ctx, np.arange(40_000, dtype=np.int64))  # 131072 slots, not 65536
+        try:
+            with pytest.raises(RuntimeError, match="bytes given"):
+                HashTable(ctx, kc, memory=mem)
+        finally:
+            kc.free()
+    finally:
+        mem.free()
 
 
 @pytest.mark.parametrize("group_by", [A.JOIN_GROUP_BUILD_PAYLOAD, A.JOIN_GROUP_PROBE_COL,

@@ -210,6 +210,8 @@ SIGNATURES = {
     "tgp_hash_build": (I32_, [P, C.POINTER(Column), C.POINTER(Column), C.POINTER(P), P]),
     "tgp_hash_table_destroy": (I32_, [P, P]),
     "tgp_hash_table_footprint": (I32_, [P, C.POINTER(U64_)]),
+    "tgp_hash_build_in": (I32_, [P, C.POINTER(Column), C.POINTER(Column), P, U64_,
+                                 C.POINTER(C.c_void_p), P]),
     "tgp_hash_table_bytes_for": (U64_, [U64_]),
     "tgp_hash_table_info": (I32_, [P, C.POINTER(U64_), C.POINTER(U64_), C.POINTER(I32_)]),
     "tgp_hash_probe_count": (I32_, [P, P, C.POINTER(Column), U64_, U64_, P, P]),

@@ -723,6 +723,7 @@ using namespace tgp;
 struct tgp_hash_table {
   tgp_ctx* ctx = nullptr;
   void* mem = nullptr;       // slots, then 2 flag words
+  bool owns_mem = true;      // false: caller memory (tgp_hash_build_in)
   void* side_mem = nullptr;  // side list (only when sentinel keys exist)
   uint64_t cap = 0, n_rows = 0, n_side = 0, bytes = 0;
   int32_t has_dup = 0;
@@ -757,8 +758,23 @@ uint64_t tgp_hash_table_bytes_for(uint64_t n_build_rows) {
   return table_capacity(n_build_rows) * sizeof(Slot) + 16;
 }
 
+static int32_t build_table(tgp_ctx* ctx, const tgp_column* keys, const tgp_column* payload,
+                           void* d_mem, uint64_t mem_bytes, tgp_hash_table** out, void* stream);
+
 int32_t tgp_hash_build(tgp_ctx* ctx, const tgp_column* keys, const tgp_column* payload,
                        tgp_hash_table** out, void* stream) {
+  return build_table(ctx, keys, payload, nullptr, 0, out, stream);
+}
+
+int32_t tgp_hash_build_in(tgp_ctx* ctx, const tgp_column* keys, const tgp_column* payload,
+                          void* d_mem, uint64_t mem_bytes, tgp_hash_table** out, void* stream) {
+  TGP_REQUIRE(d_mem && (reinterpret_cast<uintptr_t>(d_mem) & 15) == 0,
+              "tgp_hash_build_in: table memory must be non-null and 16-byte aligned");
+  return build_table(ctx, keys, payload, d_mem, mem_bytes, out, stream);
+}
+
+static int32_t build_table(tgp_ctx* ctx, const tgp_column* keys, const tgp_column* payload,
+                           void* d_mem, uint64_t mem_bytes, tgp_hash_table** out, void* stream) {
   TGP_REQUIRE(ctx && out, "tgp_hash_build: null argument");
   *out = nullptr;
   TGP_TRY(check_injected(ctx));
@@ -774,15 +790,27 @@ int32_t tgp_hash_build(tgp_ctx* ctx, const tgp_column* keys, const tgp_column* p
   t->n_rows = keys->len;
   t->cap = table_capacity(keys->len);
   t->bytes = t->cap * sizeof(Slot) + 16;
-  int32_t rc = hbm_alloc(ctx, t->bytes, &t->mem);
-  if (rc != TGP_OK) {
-    delete t;
-    return rc;
+  int32_t rc = TGP_OK;
+  if (d_mem) {
+    if (mem_bytes < t->bytes) {
+      set_error("tgp_hash_build_in: %llu bytes given, the table needs %llu",
+                (unsigned long long)mem_bytes, (unsigned long long)t->bytes);
+      delete t;
+      return TGP_ERR_INVALID;
+    }
+    t->mem = d_mem;
+    t->owns_mem = false;
+  } else {
+    rc = hbm_alloc(ctx, t->bytes, &t->mem);
+    if (rc != TGP_OK) {
+      delete t;
+      return rc;
+    }
   }
   Slot* slots = static_cast<Slot*>(t->mem);
   unsigned long long* flags = reinterpret_cast<unsigned long long*>(slots + t->cap);
   auto fail = [&](int32_t code) {
-    hbm_free(ctx, t->mem);
+    if (t->owns_mem) hbm_free(ctx, t->mem);
     delete t;
     return code;
   };
@@ -828,14 +856,17 @@ int32_t tgp_hash_build(tgp_ctx* ctx, const tgp_column* keys, const tgp_column* p
 int32_t tgp_hash_table_destroy(tgp_ctx* ctx, tgp_hash_table* t) {
   if (!t) return TGP_OK;
   TGP_REQUIRE(ctx && t->ctx == ctx, "tgp_hash_table_destroy: table belongs to another context");
-  TGP_HIP(hipStreamSynchronize(ctx->stream));
-  int32_t rc = hbm_free(ctx, t->mem);
-  if (t->side_mem) hbm_free(ctx, t->side_mem);
+  int32_t rc = TGP_OK;
+  if (t->owns_mem || t->side_mem) {
+    TGP_HIP(hipStreamSynchronize(ctx->stream));
+    if (t->owns_mem) rc = hbm_free(ctx, t->mem);
+    if (t->side_mem) hbm_free(ctx, t->side_mem);
+  }
   delete t;
   return rc;
 }
 
-int32_t tgp_hash_table_footprint(const tgp_hash_table* t, uint64_t* out_bytes) {
+int32_t tgp_hash_table_footprint(const tgp_hash_table* t, uint64_t* out_bytes) {  // incl. caller memory
   TGP_REQUIRE(t && out_bytes, "tgp_hash_table_footprint: null argument");
   *out_bytes = t->bytes;
   return TGP_OK;

@@ -8,14 +8,15 @@
 //     request (up to `max_batch_queries`) into one persistent-kernel batch
 //     (tgp_batch_*), so requests that arrive while others run join the next
 //     batch and batches overlap on the device.  Requests whose row range is at
-//     least `heavy_rows` run one at a time as single tgp_group_agg launches on a
-//     slot of their own, so a full-table scan does not hold back the small
-//     requests, and many full-table scans do not contend inside one batch.
-//   - the join worker admits a join when the hash table it will build fits the
-//     HBM budget (the context's tier budget, optionally capped by
-//     `hash_budget_bytes`): tables of admitted joins are built one at a time and
-//     their probe->aggregate kernels run on `join_slots` streams; a join waits
-//     while the budget is held by others, and is rejected if it can never fit.
+//     least `heavy_rows` (full-table scans) are batched on a slot of their own
+//     (slot 0, with 128-slice chunks in the XCD-affine table order: requests
+//     over the same rows then read them through one L2), so they do not hold
+//     back the small requests.
+//   - `join_slots` join workers (a thread and a stream each) admit a join when
+//     the hash table it will build fits the HBM budget (the context's tier
+//     budget, optionally capped by `hash_budget_bytes`), build it and run the
+//     fused probe->aggregate; a join waits while the budget is held by others,
+//     and is rejected if it can never fit.
 //
 // With `per_query_launch` the aggregate worker instead runs each request as its
 // own tgp_group_agg launch (one request per slot): the one-launch-per-query
@@ -85,6 +86,8 @@ struct JoinSlot {
   Request* req = nullptr;
   tgp_hash_table* table = nullptr;
   uint64_t table_bytes = 0;
+  void* pool = nullptr;       // the slot's table memory, reused join after join
+  uint64_t pool_bytes = 0;    // grows to the largest table the slot has built
   uint64_t* d_res = nullptr;  // count[ng] | sums[ng*2] | oor
   uint64_t d_cap = 0;
   bool busy = false;
@@ -102,10 +105,13 @@ struct tgp_server {
   std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
   uint64_t next_id = 1;
   bool stop = false;
-  std::thread agg_thread, join_thread;
+  std::thread agg_thread;
+  std::vector<std::thread> join_threads;
   std::vector<AggSlot> agg_slots;
   std::vector<JoinSlot> join_slots;
-  uint64_t hash_in_use = 0;
+  uint64_t hash_in_use = 0;  // bytes of the tables of the joins in flight
+  uint64_t pool_held = 0;    // bytes of all join slots' table pools
+  bool heavy_slot = false;  // slot 0 serves the heavy queue
   tgp_serve_stats stats{};
 };
 
@@ -222,9 +228,7 @@ void launch_agg(tgp_server* s, AggSlot& sl, std::vector<Request*>&& take, bool h
   for (size_t i = 0; i < take.size(); ++i) qs[i] = take[i]->req.agg;
   const int64_t t0 = now_ns();
   int32_t rc = TGP_OK;
-  // heavy requests run alone: a batch of many scans over the same large range
-  // makes them contend for the same lines instead of sharing them
-  sl.single = s->cfg.per_query_launch || heavy;
+  sl.single = s->cfg.per_query_launch != 0;
   if (sl.single) {
     rc = launch_single(s, sl, take[0]);
   } else {
@@ -262,18 +266,25 @@ void agg_worker(tgp_server* s) {
         progressed = true;
       }
     }
-    // launch: a free slot takes the waiting requests (heavy ones in their own batch)
-    for (AggSlot& sl : s->agg_slots) {
+    // launch: a free slot takes the waiting requests (heavy ones on slot 0)
+    for (size_t si = 0; si < s->agg_slots.size(); ++si) {
+      AggSlot& sl = s->agg_slots[si];
       if (sl.busy) continue;
       std::vector<Request*> take;
       bool heavy = false;
       {
         std::lock_guard<std::mutex> lk(s->mu);
-        if (!s->heavy_q.empty() && s->stats.heavy_in_flight == 0) {
-          take.push_back(s->heavy_q.front());
-          s->heavy_q.pop_front();
-          heavy = true;
-          s->stats.heavy_in_flight += 1;
+        const bool heavy_slot = s->heavy_slot && si == 0;
+        if (heavy_slot) {
+          const uint32_t hmax = s->cfg.per_query_launch ? 1 : maxq;
+          while (!s->heavy_q.empty() && take.size() < hmax) {
+            take.push_back(s->heavy_q.front());
+            s->heavy_q.pop_front();
+          }
+          if (!take.empty()) {
+            heavy = true;
+            s->stats.heavy_in_flight += 1;
+          }
         } else {
           while (!s->agg_q.empty() && take.size() < maxq) {
             take.push_back(s->agg_q.front());
@@ -281,7 +292,7 @@ void agg_worker(tgp_server* s) {
           }
         }
       }
-      if (take.empty()) break;
+      if (take.empty()) continue;
       launch_agg(s, sl, std::move(take), heavy);
       progressed = true;
     }
@@ -302,9 +313,9 @@ void agg_worker(tgp_server* s) {
 uint64_t hash_budget(tgp_server* s) {
   uint64_t used = 0, budget = 0, peak = 0;
   (void)tgp_hbm_usage(s->ctx, &used, &budget, &peak);
-  // room the tier has now, plus what admitted tables already hold
+  // room the tier has now, plus the table pools (all of it is table memory)
   uint64_t room = budget > used ? budget - used : 0;
-  room += s->hash_in_use;
+  room += s->pool_held;
   if (s->cfg.hash_budget_bytes && s->cfg.hash_budget_bytes < room) room = s->cfg.hash_budget_bytes;
   return room;
 }
@@ -319,6 +330,7 @@ int32_t harvest_join(tgp_server* s, JoinSlot& sl) {
     set_error("serve: join result copy failed");
     rc = TGP_ERR_HIP;
   }
+  // the table lives in the slot's pool: destroying it frees no device memory
   const int32_t rd = tgp_hash_table_destroy(s->ctx, sl.table);
   if (rc == TGP_OK) rc = rd;
   std::lock_guard<std::mutex> lk(s->mu);
@@ -330,6 +342,35 @@ int32_t harvest_join(tgp_server* s, JoinSlot& sl) {
   finish(s, r, rc);
   s->done_cv.notify_all();
   return rc;
+}
+
+void drop_pool(tgp_server* s, JoinSlot& sl) {  // caller holds s->mu
+  if (!sl.pool) return;
+  (void)hbm_free(s->ctx, sl.pool);
+  s->pool_held -= sl.pool_bytes;
+  sl.pool = nullptr;
+  sl.pool_bytes = 0;
+}
+
+// A slot's pool grows to the largest table it has to hold.  Freeing and
+// allocating device memory synchronises the device, so it happens only when a
+// join is larger than any before it on this slot.  When the tier is short, the
+// pools of idle slots are given back first.
+int32_t grow_pool(tgp_server* s, JoinSlot& sl, uint64_t need) {
+  std::lock_guard<std::mutex> lk(s->mu);
+  drop_pool(s, sl);
+  void* p = nullptr;
+  int32_t rc = hbm_alloc(s->ctx, need, &p);
+  if (rc == TGP_ERR_BUDGET) {
+    for (JoinSlot& o : s->join_slots)
+      if (&o != &sl && !o.busy) drop_pool(s, o);
+    rc = hbm_alloc(s->ctx, need, &p);
+  }
+  if (rc != TGP_OK) return rc;
+  sl.pool = p;
+  sl.pool_bytes = need;
+  s->pool_held += need;
+  return TGP_OK;
 }
 
 int32_t start_join(tgp_server* s, JoinSlot& sl, Request* r, uint64_t need) {
@@ -345,9 +386,10 @@ int32_t start_join(tgp_server* s, JoinSlot& sl, Request* r, uint64_t need) {
     sl.d_res = static_cast<uint64_t*>(p);
     sl.d_cap = r->words;
   }
+  if (sl.pool_bytes < need) TGP_TRY(grow_pool(s, sl, need));
   tgp_hash_table* t = nullptr;
   const tgp_column* pay = J.build_payload.data ? &J.build_payload : nullptr;
-  TGP_TRY(tgp_hash_build(s->ctx, &J.build_keys, pay, &t, sl.stream));
+  TGP_TRY(tgp_hash_build_in(s->ctx, &J.build_keys, pay, sl.pool, sl.pool_bytes, &t, sl.stream));
   sl.table = t;
   sl.table_bytes = need;
   tgp_join_agg_query q = J.probe;
@@ -361,23 +403,22 @@ int32_t start_join(tgp_server* s, JoinSlot& sl, Request* r, uint64_t need) {
   return TGP_OK;
 }
 
-void join_worker(tgp_server* s) {
+// One join worker per slot: hash builds are synchronous, so each worker has
+// its own thread and the builds of different joins overlap.
+void join_worker(tgp_server* s, size_t slot) {
   (void)hipSetDevice(s->ctx->device);
+  JoinSlot& sl = s->join_slots[slot];
   for (;;) {
     bool progressed = false;
-    for (JoinSlot& sl : s->join_slots) {
-      if (sl.busy && hipEventQuery(sl.done_ev) == hipSuccess) {
-        (void)harvest_join(s, sl);
-        progressed = true;
-      }
+    if (sl.busy && hipEventQuery(sl.done_ev) == hipSuccess) {
+      (void)harvest_join(s, sl);
+      progressed = true;
     }
-    for (JoinSlot& sl : s->join_slots) {
-      if (sl.busy) continue;
-      Request* r = nullptr;
-      uint64_t need = 0;
-      {
-        std::lock_guard<std::mutex> lk(s->mu);
-        if (s->join_q.empty()) break;
+    Request* r = nullptr;
+    uint64_t need = 0;
+    if (!sl.busy) {
+      std::lock_guard<std::mutex> lk(s->mu);
+      while (!s->join_q.empty()) {
         Request* front = s->join_q.front();
         need = tgp_hash_table_bytes_for(front->req.join.build_keys.len);
         const uint64_t room = hash_budget(s);
@@ -406,17 +447,22 @@ void join_worker(tgp_server* s) {
         s->join_q.pop_front();
         s->hash_in_use += need;
         s->stats.admitted_joins += 1;
+        sl.busy = true;  // claimed; start_join fills the slot
         r = front;
+        break;
       }
+    }
+    if (r) {
       const int32_t rc = start_join(s, sl, r, need);
       if (rc != TGP_OK) {
         if (sl.table) (void)tgp_hash_table_destroy(s->ctx, sl.table);
+        std::lock_guard<std::mutex> lk(s->mu);
         bool others = false;
         for (JoinSlot& o : s->join_slots) others |= o.busy && &o != &sl;
-        std::lock_guard<std::mutex> lk(s->mu);
         s->hash_in_use -= need;
         sl.table = nullptr;
         sl.table_bytes = 0;
+        sl.req = nullptr;
         sl.busy = false;
         if (rc == TGP_ERR_BUDGET && others) {
           // the tier filled up meanwhile: wait for the joins in flight to release it
@@ -434,11 +480,9 @@ void join_worker(tgp_server* s) {
       progressed = true;
     }
     std::unique_lock<std::mutex> lk(s->mu);
-    bool busy = false;
-    for (JoinSlot& sl : s->join_slots) busy |= sl.busy;
-    if (s->stop && !busy && s->join_q.empty()) return;
+    if (s->stop && !sl.busy && s->join_q.empty()) return;
     if (!progressed) {
-      if (busy) s->work_cv.wait_for(lk, std::chrono::microseconds(20));
+      if (sl.busy) s->work_cv.wait_for(lk, std::chrono::microseconds(20));
       else s->work_cv.wait_for(lk, std::chrono::milliseconds(5));
     }
   }
@@ -454,6 +498,7 @@ void release(tgp_server* s) {
   }
   for (JoinSlot& sl : s->join_slots) {
     if (sl.table) (void)tgp_hash_table_destroy(s->ctx, sl.table);
+    drop_pool(s, sl);
     if (sl.d_res) (void)hbm_free(s->ctx, sl.d_res);
     if (sl.done_ev) (void)hipEventDestroy(sl.done_ev);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
@@ -482,11 +527,15 @@ int32_t tgp_serve_create(tgp_ctx* ctx, const tgp_serve_config* cfg, tgp_server**
   s->agg_slots.resize(cfg->batch_slots);
   s->join_slots.resize(cfg->join_slots);
   int32_t rc = TGP_OK;
-  for (AggSlot& sl : s->agg_slots) {
+  s->heavy_slot = cfg->heavy_rows > 0 && cfg->batch_slots >= 2;
+  for (size_t si = 0; si < s->agg_slots.size(); ++si) {
+    AggSlot& sl = s->agg_slots[si];
     if (rc == TGP_OK) rc = tgp_batch_create(ctx, cfg->max_batch_queries, &sl.batch);
     if (rc == TGP_OK) {
-      const tgp_batch_tuning t{cfg->wg_per_cu, cfg->chunk_slices, cfg->order, cfg->cached_loads,
-                               0, 1};
+      // the heavy slot's batches are many scans over the same large ranges
+      const bool heavy = s->heavy_slot && si == 0;
+      const tgp_batch_tuning t{cfg->wg_per_cu, heavy ? 128u : cfg->chunk_slices,
+                               heavy ? 2 : cfg->order, heavy ? 1 : cfg->cached_loads, 0, 1};
       rc = tgp_batch_set_tuning(sl.batch, &t);
     }
     if (rc == TGP_OK && (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
@@ -510,7 +559,8 @@ int32_t tgp_serve_create(tgp_ctx* ctx, const tgp_serve_config* cfg, tgp_server**
   }
   tgp_server* raw = s.release();
   raw->agg_thread = std::thread(agg_worker, raw);
-  raw->join_thread = std::thread(join_worker, raw);
+  for (size_t j = 0; j < raw->join_slots.size(); ++j)
+    raw->join_threads.emplace_back(join_worker, raw, j);
   *out = raw;
   return TGP_OK;
 }
@@ -523,7 +573,8 @@ int32_t tgp_serve_destroy(tgp_server* s) {
   }
   s->work_cv.notify_all();
   if (s->agg_thread.joinable()) s->agg_thread.join();
-  if (s->join_thread.joinable()) s->join_thread.join();
+  for (std::thread& t : s->join_threads)
+    if (t.joinable()) t.join();
   release(s);
   delete s;
   return TGP_OK;
@@ -553,7 +604,7 @@ int32_t tgp_serve_submit(tgp_server* s, const tgp_serve_request* req, int64_t ar
     s->reqs.emplace(raw->id, std::move(r));
     s->stats.submitted += 1;
     if (raw->req.kind == TGP_SERVE_JOIN_AGG) s->join_q.push_back(raw);
-    else if (s->cfg.heavy_rows && raw->rows >= s->cfg.heavy_rows) s->heavy_q.push_back(raw);
+    else if (s->heavy_slot && raw->rows >= s->cfg.heavy_rows) s->heavy_q.push_back(raw);
     else s->agg_q.push_back(raw);
   }
   s->work_cv.notify_all();

@@ -346,13 +346,21 @@ class HashTable:
     """Device hash table of a join's build side (tgp_hash_build).
     Mirrors JoinHashTable (exec/join_hash_table.rs) / HashBuildScalar (kernel/hash.rs)."""
 
-    def __init__(self, ctx, keys, payload=None, stream=None):
+    def __init__(self, ctx, keys, payload=None, stream=None, memory=None):
+        """memory: a DeviceBuffer to hold the slots (tgp_hash_build_in), reusable
+        for another table once this one is freed."""
         self.ctx = ctx
         h = C.c_void_p()
         kd = keys.desc()
         pd = payload.desc() if payload is not None else None
-        A.check(A.lib().tgp_hash_build(ctx.handle, C.byref(kd), C.byref(pd) if pd else None,
-                                       C.byref(h), stream), "tgp_hash_build")
+        pdp = C.byref(pd) if pd else None
+        if memory is None:
+            A.check(A.lib().tgp_hash_build(ctx.handle, C.byref(kd), pdp, C.byref(h), stream),
+                    "tgp_hash_build")
+        else:
+            A.check(A.lib().tgp_hash_build_in(ctx.handle, C.byref(kd), pdp, memory.ptr,
+                                              memory.nbytes, C.byref(h), stream),
+                    "tgp_hash_build_in")
         self.handle = h.value
 
     def footprint(self):
