@@ -113,9 +113,18 @@ typedef struct tgp_group_part {
 } tgp_group_part;
 
 enum tgp_agg_kind {
-  TGP_AGG_SUM = 0,          /* SUM(coef * prod_k (fa[k] + fb[k]*cols[fcol[k]])) */
-  TGP_AGG_COUNT_NONZERO = 1 /* COUNT(col): rows with cols[fcol[0]] != 0
-                               (accum_grouped.rs:243-247 counts non-zero cells) */
+  TGP_AGG_SUM = 0,           /* SUM(coef * prod_k (fa[k] + fb[k]*cols[fcol[k]])) */
+  TGP_AGG_COUNT_NONZERO = 1, /* COUNT(col): rows with cols[fcol[0]] != 0
+                                (accum_grouped.rs:243-247 counts non-zero cells) */
+  /* MIN / MAX of the same per-row value as SUM (wrapping 64-bit; for MIN(col):
+   * nfactors 1, coef 1, fa 0, fb 1).  The result's sum words hold the value:
+   * low word = the 64 bits, high word = their sign extension (signed kinds) or 0
+   * (unsigned kinds), so the int128 reading is the value; 0 for a group with no
+   * rows. */
+  TGP_AGG_MIN = 2,     /* signed int64 order: minmax_vec on INT (aggregate.rs:1822-1836) */
+  TGP_AGG_MAX = 3,
+  TGP_AGG_MIN_U64 = 4, /* raw u64 order: MinCol/MaxCol (accum_grouped.rs:269-280, :320-329) */
+  TGP_AGG_MAX_U64 = 5
 };
 
 typedef struct tgp_agg {

@@ -132,6 +132,9 @@ def scan_multi(cells, preds):
     return c, int(mask[0])
 
 
+_MINMAX = ("min", "max", "min_u64", "max_u64")
+
+
 def make_query(arrays, preds=(), parts=(), aggs=(), row_begin=0, row_end=None, sel_words=None):
     """Same description as turbogp_amd.device.make_agg_query over host arrays.
     Keeps references to the arrays on the returned object."""
@@ -153,6 +156,16 @@ def make_query(arrays, preds=(), parts=(), aggs=(), row_begin=0, row_end=None, s
         if ag[0] == "count_nonzero":
             a.kind, a.nfactors, a.coef = 1, 1, 1
             a.fcol[0] = ag[1]
+        elif ag[0] in _MINMAX:
+            # ('min' | 'max' | 'min_u64' | 'max_u64', col) or (..., coef, [(col, fa, fb), ...])
+            a.kind = 2 + _MINMAX.index(ag[0])
+            if len(ag) == 2:
+                a.nfactors, a.coef = 1, 1
+                a.fcol[0], a.fa[0], a.fb[0] = ag[1], 0, 1
+            else:
+                a.nfactors, a.coef = len(ag[2]), ag[1]
+                for k, (ci, fa, fb) in enumerate(ag[2]):
+                    a.fcol[k], a.fa[k], a.fb[k] = ci, fa, fb
         else:
             _, coef, factors = ag
             a.kind, a.nfactors, a.coef = 0, len(factors), coef

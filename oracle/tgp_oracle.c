@@ -392,7 +392,26 @@ int32_t orc_group_agg(const orc_query* q, int32_t mode, int32_t nthreads, uint64
     out_count[gid] = g->n; /* COUNT(*) = indices.len(), aggregate.rs:1372-1373 */
     for (int a = 0; a < q->naggs; ++a) {
       const orc_agg* A = &q->aggs[a];
-      if (mode == 0) {
+      if (A->kind >= 2 && A->kind <= 5) {
+        /* MIN/MAX: signed (minmax_vec on INT, aggregate.rs:1822-1836) or raw u64
+         * (accum_grouped.rs:269-280: MinCol starts at u64::MAX, MaxCol at 0) */
+        const int sgn = A->kind == 2 || A->kind == 3, is_min = A->kind == 2 || A->kind == 4;
+        uint64_t best = 0;
+        for (uint64_t r = 0; r < g->n; ++r) {
+          const uint64_t v = (uint64_t)agg_row_value(q, A, g->rows[r]);
+          int better;
+          if (r == 0) better = 1;
+          else if (sgn) better = is_min ? (int64_t)v < (int64_t)best : (int64_t)v > (int64_t)best;
+          else better = is_min ? v < best : v > best;
+          if (better) best = v;
+        }
+        if (mode == 0) {
+          out_sums[(gid * q->naggs + a) * 2] = best;
+          out_sums[(gid * q->naggs + a) * 2 + 1] = sgn && (int64_t)best < 0 ? ~0ull : 0ull;
+        } else {
+          out_f64[gid * q->naggs + a] = sgn ? (double)(int64_t)best : (double)best;
+        }
+      } else if (mode == 0) {
         i128 s = 0;
         for (uint64_t r = 0; r < g->n; ++r) s += (i128)agg_row_value(q, A, g->rows[r]);
         out_sums[(gid * q->naggs + a) * 2] = (uint64_t)(u128)s;

@@ -68,7 +68,10 @@ uint64_t orc_scan_multi(const uint64_t* cells, uint64_t n, int32_t npred, const 
 /* filter -> bitmap over a query's predicates (one pass per leaf + AND) */
 int32_t orc_query_filter(const orc_query* q, uint8_t* bits /* (n+7)/8 bytes */);
 
-/* execute_grouped.  mode 0: exact int128 sums (out_sums lo,hi pairs);
+/* execute_grouped.  Aggregate kinds: 0 SUM, 1 COUNT(col) (non-zero cells),
+ * 2/3 MIN/MAX in signed order, 4/5 MIN/MAX in raw u64 order (the value in the
+ * low word, its sign extension / 0 in the high word).
+ * mode 0: exact int128 sums (out_sums lo,hi pairs);
  * mode 1: the reference's f64 sums (out_f64).  Both fill out_count.
  * ngroups = prod(card).  nthreads <= 0 => all cores. */
 int32_t orc_group_agg(const orc_query* q, int32_t mode, int32_t nthreads, uint64_t* out_count,

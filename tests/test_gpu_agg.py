@@ -144,3 +144,42 @@ def test_determinism_repeated_runs(ctx):
                for x in r)
     for c in dcols:
         c.free()
+
+
+@pytest.mark.parametrize("ncard", [1, 5, 7, 300, 5000])
+def test_min_max_vs_oracle(ctx, ncard):
+    """MIN/MAX in signed and raw-u64 order, over columns of every width and a
+    wrapping expression, through each accumulation strategy (registers for
+    <= 8 groups, LDS table, global table)."""
+    rng = np.random.default_rng(ncard)
+    n = 180_000
+    cols = [rng.integers(-2**62, 2**62, n).astype(np.int64),
+            rng.integers(0, 7, n).astype(np.uint8),
+            rng.integers(-1000, 1000, n).astype(np.int16),
+            rng.integers(0, 5000, n).astype(np.int32),
+            rng.integers(-5, 5, n).astype(np.int8),
+            rng.integers(0, 2**64, n, dtype=np.uint64)]
+    parts = [] if ncard == 1 else [(1, 1, 0, 7)] if ncard == 7 else \
+        [(3, max(1, 5000 // ncard), 0, ncard)]
+    preds = [(3, LT, 4_000, False)]
+    small = [("min", 2), ("max", 4), ("min_u64", 4), ("max_u64", 0)]
+    run_both(ctx, cols, preds, parts, small, 3, n - 5)
+    wide = small + [("max", 5), ("min", 1, [(0, 3, -1), (2, 1, 1)])]
+    run_both(ctx, cols, preds, parts, wide, 0, n)
+    run_both(ctx, cols, preds, parts, [("sum", 1, [(2, 0, 1)]), ("min_u64", 5)], 17, 100_017)
+    # nothing selected: every group is empty, values read 0
+    got = run_both(ctx, cols, [(3, LT, 0, False)], parts, small, 0, n)
+    assert not got.count.any() and not got.sums_words.any()
+
+
+def test_min_max_kats(ctx):  # tests/interpreter_fallback.rs min_max_together
+    region = np.array([1, 1, 2, 2, 3], np.int64)
+    amount = np.array([100, 200, 150, 300, 250], np.int64)
+    got = run_both(ctx, [region, amount], [], [], [("min", 1), ("max", 1)])
+    assert got.sums()[0] == [100, 300]
+    got = run_both(ctx, [region, amount], [], [(0, 1, 1, 3)], [("min", 1), ("max", 1)])
+    assert got.rows() == [(0, 2, [100, 200]), (1, 2, [150, 300]), (2, 1, [250, 250])]
+    # raw u64 order (accum_grouped.rs MinCol/MaxCol): -1 is the largest cell
+    v = np.array([-1, 5, 3], np.int64)
+    got = run_both(ctx, [v], [], [], [("min", 0), ("max", 0), ("min_u64", 0), ("max_u64", 0)])
+    assert got.sums()[0] == [-1, 5, 3, 2**64 - 1]

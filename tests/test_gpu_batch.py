@@ -43,11 +43,16 @@ def test_batch_matches_single_queries_and_oracle(ctx, table):
     names6, preds6, parts6, aggs6 = W.q6_spec()
     qs.append(make_agg_query([dev[c] for c in names6], preds6 + [(2, A.NE, 7, True)], parts6,
                              aggs6 + [("count_nonzero", 1)], 10, 2_000_000))
+    # MIN/MAX aggregates run in the generic engine, also alone
+    names1, preds1, parts1, aggs1 = W.q1_spec()
+    qs.append(make_agg_query([dev[c] for c in names1], preds1, parts1,
+                             [("min", 0), ("max_u64", 1), ("max", 1, [(1, 0, 1), (2, 100, -1)])],
+                             999, 2_500_000))
     b = Batch(ctx, len(qs))
     try:
         b.set_queries(qs)
         info = b.info()
-        assert info["batched"] == len(qs) - 1 and info["fallback"] == 1
+        assert info["batched"] == len(qs) - 2 and info["fallback"] == 2
         b.run(A.BATCH_PERSISTENT)
         persistent = b.results()
         b.run(A.BATCH_PERSISTENT)          # graph replay

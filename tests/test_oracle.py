@@ -133,6 +133,47 @@ def test_group_by_kat():  # tests/joins_groupby_orderby.rs:82-106 (grp 1 -> 60,3
     assert list(cnt) == [3, 2] and sums == [[60], [90]]
 
 
+def test_min_max_kats():
+    # tests/interpreter_fallback.rs min_max_together: min(amount) = 100, max = 300
+    region = np.array([1, 1, 2, 2, 3], np.int64)
+    amount = np.array([100, 200, 150, 300, 250], np.int64)
+    cnt, sums, _ = B.group_agg(B.make_query([region, amount], aggs=[("min", 1), ("max", 1)]))
+    assert int(cnt[0]) == 5 and sums[0] == [100, 300]
+    # tests/clickbench_executor.rs min_max_multi_aggregate: ids 1..5
+    ids = np.arange(1, 6, dtype=np.int64)
+    _, sums, _ = B.group_agg(B.make_query([ids], aggs=[("min", 0), ("max", 0)]))
+    assert sums[0] == [1, 5]
+    # the same in raw u64 order (accum_grouped.rs:269-280) differs only for negatives
+    v = np.array([-1, 5, 3], np.int64)
+    _, sums, _ = B.group_agg(B.make_query([v], aggs=[("min", 0), ("max", 0), ("min_u64", 0),
+                                                     ("max_u64", 0)]))
+    assert sums[0] == [-1, 5, 3, 2**64 - 1]
+    _, f64, _ = B.group_agg(B.make_query([v], aggs=[("min", 0), ("max_u64", 0)]), mode=1)
+    assert f64[0] == [-1.0, float(2**64 - 1)]
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_min_max_vs_numpy(seed):
+    """Per-group MIN/MAX against numpy's reductions, signed and unsigned, over a
+    plain column and a wrapping product expression."""
+    rng = np.random.default_rng(seed)
+    n = 20_000
+    key = rng.integers(0, 9, n).astype(np.int64)
+    a = rng.integers(-2**62, 2**62, n).astype(np.int64)
+    b = rng.integers(-7, 7, n).astype(np.int8)
+    aggs = [("min", 1), ("max", 1), ("min_u64", 1), ("max_u64", 1),
+            ("min", 3, [(1, 0, 1), (2, 5, -1)]), ("max_u64", 3, [(1, 0, 1), (2, 5, -1)])]
+    cnt, sums, _ = B.group_agg(B.make_query([key, a, b], parts=[(0, 1, 0, 9)], aggs=aggs))
+    with np.errstate(over="ignore"):
+        expr = (np.int64(3) * a * (np.int64(5) - b.astype(np.int64))).astype(np.int64)
+    for g in range(9):
+        m = key == g
+        assert int(cnt[g]) == int(m.sum())
+        au, eu = a[m].view(np.uint64), expr[m].view(np.uint64)
+        assert sums[g] == [int(a[m].min()), int(a[m].max()), int(au.min()), int(au.max()),
+                           int(expr[m].min()), int(eu.max())]
+
+
 def brute_group_agg(cols, preds, parts, aggs, b, e, sel=None):
     """Pure-Python brute force for small inputs."""
     ng = int(np.prod([p[3] for p in parts])) if parts else 1

@@ -28,6 +28,8 @@ DTYPE_WIDTH = {I8: 1, U8: 1, I16: 2, U16: 2, I32: 4, U32: 4, I64: 8, U64: 8}
 EQ, NE, LT, LE, GT, GE = range(6)
 BIT_AND, BIT_OR, BIT_ANDNOT = range(3)
 AGG_SUM, AGG_COUNT_NONZERO = 0, 1
+AGG_MIN, AGG_MAX, AGG_MIN_U64, AGG_MAX_U64 = 2, 3, 4, 5
+AGG_MINMAX = {"min": AGG_MIN, "max": AGG_MAX, "min_u64": AGG_MIN_U64, "max_u64": AGG_MAX_U64}
 JOIN_GROUP_BUILD_PAYLOAD, JOIN_GROUP_PROBE_COL, JOIN_GROUP_NONE = 0, 1, 2
 TUNE_WG_PER_CU, TUNE_GRID_MAX, TUNE_SPECIALIZE, TUNE_SPEC_KERNEL, TUNE_CHUNK = 0, 1, 2, 3, 4
 TUNE_SLICES_PER_WAVE, TUNE_JOIN_ATOMICS = 5, 6

@@ -64,6 +64,18 @@ __global__ void __launch_bounds__(kBlock, 2) k_agg_reg(DevPlan P) {
       if (a >= P.naggs) break;
       uint64_t v[kRows];
       eval_agg(P.aggs[a], buf, v);
+      const int32_t mk = (P.minmax_kinds >> (3 * a)) & 7;
+      if (mk) {
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+          if (!((sel >> r) & 1)) continue;
+          const uint64_t c = mm_encode(mk, v[r]);
+#pragma unroll
+          for (int g = 0; g < GMAX; ++g)
+            if (gid[r] == g && c > lo[g][a]) lo[g][a] = c;
+        }
+        continue;
+      }
 #pragma unroll
       for (int r = 0; r < kRows; ++r) {
         if (!((sel >> r) & 1)) continue;
@@ -85,12 +97,19 @@ __global__ void __launch_bounds__(kBlock, 2) k_agg_reg(DevPlan P) {
     for (int a = 0; a < AMAX; ++a) {
       unsigned long long l = lo[g][a];
       long long h = hi[g][a];
-      for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long l2 = __shfl_xor(l, off);
-        const long long h2 = __shfl_xor(h, off);
-        const unsigned long long s = l + l2;
-        h += h2 + (long long)(s < l);
-        l = s;
+      if ((P.minmax_kinds >> (3 * a)) & 7) {
+        for (int off = 32; off > 0; off >>= 1) {
+          const unsigned long long l2 = __shfl_xor(l, off);
+          l = l2 > l ? l2 : l;
+        }
+      } else {
+        for (int off = 32; off > 0; off >>= 1) {
+          const unsigned long long l2 = __shfl_xor(l, off);
+          const long long h2 = __shfl_xor(h, off);
+          const unsigned long long s = l + l2;
+          h += h2 + (long long)(s < l);
+          l = s;
+        }
       }
       if (lane == 0) {
         red[(wave * GMAX + g) * W + 1 + 2 * a] = l;
@@ -107,6 +126,13 @@ __global__ void __launch_bounds__(kBlock, 2) k_agg_reg(DevPlan P) {
       unsigned long long c = 0;
       for (int w = 0; w < kWaves; ++w) c += red[(w * GMAX + g) * W];
       if (c) atomicAdd(&P.out_count[g], c);
+    } else if ((P.minmax_kinds >> (3 * a)) & 7) {
+      unsigned long long l = 0;
+      for (int w = 0; w < kWaves; ++w) {
+        const unsigned long long l2 = red[(w * GMAX + g) * W + 1 + 2 * a];
+        l = l2 > l ? l2 : l;
+      }
+      if (l) atomicMax(&P.out_sums[((uint64_t)g * P.naggs + a) * 2], l);
     } else {
       unsigned long long l = 0;
       long long h = 0;
@@ -157,6 +183,15 @@ __global__ void __launch_bounds__(kBlock, 2) k_agg_table(DevPlan P) {
     for (int a = 0; a < P.naggs; ++a) {
       uint64_t v[kRows];
       eval_agg(P.aggs[a], buf, v);
+      const int32_t mk = (P.minmax_kinds >> (3 * a)) & 7;
+      if (mk) {
+#pragma unroll
+        for (int r = 0; r < kRows; ++r)
+          if ((sel >> r) & 1)
+            atomicMax(&tsum[((uint64_t)gid[r] * P.naggs + a) * 2],
+                      (unsigned long long)mm_encode(mk, v[r]));
+        continue;
+      }
 #pragma unroll
       for (int r = 0; r < kRows; ++r)
         if ((sel >> r) & 1)
@@ -170,11 +205,30 @@ __global__ void __launch_bounds__(kBlock, 2) k_agg_table(DevPlan P) {
       atomicAdd(&P.out_count[g], c);
       for (int a = 0; a < P.naggs; ++a) {
         const uint64_t i = (g * P.naggs + a) * 2;
-        atomic_add128(&P.out_sums[i], tsum[i], (int64_t)tsum[i + 1]);
+        if ((P.minmax_kinds >> (3 * a)) & 7) atomicMax(&P.out_sums[i], tsum[i]);
+        else atomic_add128(&P.out_sums[i], tsum[i], (int64_t)tsum[i + 1]);
       }
     }
   }
   if (oor && P.out_oor) atomicAdd(P.out_oor, (unsigned long long)oor);
+}
+
+// Turns the MIN/MAX codes into values: low word = the value, high word = its
+// sign extension for the signed kinds; 0 for groups without rows.
+__global__ void __launch_bounds__(256) k_minmax_finalize(const unsigned long long* __restrict__ cnt,
+                                                         unsigned long long* __restrict__ sums,
+                                                         uint64_t ngroups, int32_t naggs,
+                                                         uint32_t kinds) {
+  const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (i >= ngroups * (uint64_t)naggs) return;
+  const uint64_t g = i / (uint32_t)naggs;
+  const int32_t a = (int32_t)(i - g * (uint32_t)naggs);
+  const int32_t mk = (kinds >> (3 * a)) & 7;
+  if (!mk) return;
+  const uint64_t v = cnt[g] ? mm_decode(mk, sums[2 * i]) : 0;
+  const bool sgn = mk == TGP_AGG_MIN || mk == TGP_AGG_MAX;
+  sums[2 * i] = v;
+  sums[2 * i + 1] = sgn && (int64_t)v < 0 ? ~0ull : 0ull;
 }
 
 int32_t try_spec(tgp_ctx* ctx, const tgp_agg_query* q, const tgp_group_result* out,
@@ -197,11 +251,18 @@ int32_t run_agg(tgp_ctx* ctx, const DevPlan& P, const PlanInfo& info, hipStream_
   const uint64_t n[3] = {P.ngroups, P.ngroups * P.naggs * 2, 1};
   TGP_TRY(zero_ranges(s, 3, p, n));
   switch (info.strategy) {
-    case kAggReg1: return launch_tiles(ctx, k_agg_reg<1, TGP_MAX_AGGS>, P, info.lds_bytes, s);
-    case kAggReg8: return launch_tiles(ctx, k_agg_reg<8, 4>, P, info.lds_bytes, s);
-    case kAggLds: return launch_tiles(ctx, k_agg_table<true>, P, info.lds_bytes, s);
-    default: return launch_tiles(ctx, k_agg_table<false>, P, info.lds_bytes, s);
+    case kAggReg1: TGP_TRY(launch_tiles(ctx, k_agg_reg<1, TGP_MAX_AGGS>, P, info.lds_bytes, s)); break;
+    case kAggReg8: TGP_TRY(launch_tiles(ctx, k_agg_reg<8, 4>, P, info.lds_bytes, s)); break;
+    case kAggLds: TGP_TRY(launch_tiles(ctx, k_agg_table<true>, P, info.lds_bytes, s)); break;
+    default: TGP_TRY(launch_tiles(ctx, k_agg_table<false>, P, info.lds_bytes, s)); break;
   }
+  if (P.minmax_kinds && P.ngroups && P.naggs) {
+    const uint64_t items = P.ngroups * (uint64_t)P.naggs;
+    hipLaunchKernelGGL(k_minmax_finalize, dim3((uint32_t)((items + 255) / 256)), dim3(256), 0, s,
+                       P.out_count, P.out_sums, P.ngroups, P.naggs, P.minmax_kinds);
+    TGP_HIP(hipGetLastError());
+  }
+  return TGP_OK;
 }
 
 }  // namespace tgp

@@ -119,14 +119,15 @@ int32_t plan_agg(const tgp_agg_query* q, DevPlan* P, PlanInfo* info) {
   for (int a = 0; a < q->naggs; ++a) {
     const tgp_agg& A = q->aggs[a];
     DevAgg& D = P->aggs[a];
-    TGP_REQUIRE(A.kind == TGP_AGG_SUM || A.kind == TGP_AGG_COUNT_NONZERO, "agg %d: bad kind %d", a,
+    TGP_REQUIRE(A.kind >= TGP_AGG_SUM && A.kind <= TGP_AGG_MAX_U64, "agg %d: bad kind %d", a,
                 A.kind);
     TGP_REQUIRE(A.nfactors >= 0 && A.nfactors <= TGP_MAX_FACTORS, "agg %d: nfactors %d", a,
                 A.nfactors);
     TGP_REQUIRE(A.kind != TGP_AGG_COUNT_NONZERO || A.nfactors >= 1,
                 "agg %d: COUNT(col) needs its column in fcol[0]", a);
     D.kind = A.kind;
-    D.nf = A.kind == TGP_AGG_SUM ? A.nfactors : 1;
+    D.nf = A.kind == TGP_AGG_COUNT_NONZERO ? 1 : A.nfactors;
+    if (is_minmax(A.kind)) P->minmax_kinds |= (uint32_t)A.kind << (3 * a);
     D.coef = A.coef;
     for (int k = 0; k < D.nf; ++k) {
       TGP_REQUIRE(A.fcol[k] >= 0 && A.fcol[k] < q->ncols, "agg %d factor %d: column %d", a, k,

@@ -425,6 +425,7 @@ bool extract_shape(const tgp_agg_query* q, Shape* s, SpecArgs* A) {
   s->naggs = q->naggs;
   for (int a = 0; a < q->naggs; ++a) {
     const tgp_agg& G = q->aggs[a];
+    if (G.kind != TGP_AGG_SUM && G.kind != TGP_AGG_COUNT_NONZERO) return false;  // MIN/MAX
     s->kind[a] = G.kind;
     if (G.kind == TGP_AGG_COUNT_NONZERO) {
       s->nf[a] = 1;

@@ -82,7 +82,7 @@ struct DevPlan {
   DevAgg aggs[TGP_MAX_AGGS];
   int32_t nslots, npreds, nparts, naggs;
   uint32_t stage_bytes;  // LDS bytes of one stage buffer (sum of widths x 1 KiB)
-  uint32_t _pad;
+  uint32_t minmax_kinds; // 3 bits per aggregate: its kind if MIN/MAX, else 0
   uint64_t ngroups;
   const uint64_t* sel_bitmap;
   // outputs
@@ -272,6 +272,21 @@ __device__ __forceinline__ void eval_agg(const DevAgg& A, const uint8_t* buf,
       v[r] = assign ? f : v[r] * f;
     }
   }
+}
+
+// MIN / MAX run as an unsigned maximum of an order-preserving code with
+// identity 0 (the value a cleared result holds): signed values flip the sign
+// bit, and MIN complements the code.
+__host__ __device__ __forceinline__ bool is_minmax(int32_t kind) {
+  return kind >= TGP_AGG_MIN && kind <= TGP_AGG_MAX_U64;
+}
+__host__ __device__ __forceinline__ uint64_t mm_encode(int32_t kind, uint64_t v) {
+  const uint64_t k = (kind == TGP_AGG_MIN || kind == TGP_AGG_MAX) ? v ^ (1ull << 63) : v;
+  return (kind == TGP_AGG_MIN || kind == TGP_AGG_MIN_U64) ? ~k : k;
+}
+__host__ __device__ __forceinline__ uint64_t mm_decode(int32_t kind, uint64_t c) {
+  const uint64_t k = (kind == TGP_AGG_MIN || kind == TGP_AGG_MIN_U64) ? ~c : c;
+  return (kind == TGP_AGG_MIN || kind == TGP_AGG_MAX) ? k ^ (1ull << 63) : k;
 }
 
 // Exact 128-bit accumulation: value = hi * 2^64 + lo.

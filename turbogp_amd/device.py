@@ -304,12 +304,17 @@ class GroupAggResult:
         return [(g, int(self.count[g]), s[g]) for g in range(len(self.count)) if self.count[g]]
 
 
+_MINMAX = ("min", "max", "min_u64", "max_u64")
+
+
 def make_agg_query(cols, preds=(), parts=(), aggs=(), row_begin=0, row_end=None, sel_bitmap=None):
     """Builds an AggQuery.
     cols:  list of DeviceColumn (or objects with .desc())
     preds: (col, op, lit, is_signed)
     parts: (col, divisor, base, card)
-    aggs:  ('sum', coef, [(col, fa, fb), ...]) or ('count_nonzero', col)
+    aggs:  ('sum', coef, [(col, fa, fb), ...]), ('count_nonzero', col), or
+           ('min' | 'max' | 'min_u64' | 'max_u64', col) / (..., coef, factors):
+           MIN/MAX in signed int64 or raw u64 order
     """
     q = A.AggQuery()
     n = cols[0].n if cols else 0
@@ -331,6 +336,16 @@ def make_agg_query(cols, preds=(), parts=(), aggs=(), row_begin=0, row_end=None,
             a.kind, a.nfactors = A.AGG_COUNT_NONZERO, 1
             a.fcol[0] = ag[1]
             a.coef = 1
+        elif ag[0] in _MINMAX:
+            # ('min' | 'max' | 'min_u64' | 'max_u64', col) or (..., coef, [(col, fa, fb), ...])
+            a.kind = A.AGG_MINMAX[ag[0]]
+            if len(ag) == 2:
+                a.nfactors, a.coef = 1, 1
+                a.fcol[0], a.fa[0], a.fb[0] = ag[1], 0, 1
+            else:
+                a.nfactors, a.coef = len(ag[2]), ag[1]
+                for k, (ci, fa, fb) in enumerate(ag[2]):
+                    a.fcol[k], a.fa[k], a.fb[k] = ci, fa, fb
         else:
             _, coef, factors = ag
             a.kind, a.nfactors, a.coef = A.AGG_SUM, len(factors), coef
