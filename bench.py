@@ -185,9 +185,9 @@ def bench_serve(ctx, table, join_tables, args):
     n = args.serve_requests
     mix = W.serve_mix(n, W.BATCH_TABLE_ROWS, W.JOIN_BUILD_ROWS, W.JOIN_PROBE_ROWS)
     reqs = W.serve_requests(table, build, probe, mix)
-    configs = {"batched": dict(max_batch_queries=1024, batch_slots=3, join_slots=2,
+    configs = {"batched": dict(max_batch_queries=1024, batch_slots=3, join_slots=4,
                                heavy_rows=50_000_000),
-               "per_query_launch": dict(batch_slots=3, join_slots=2, heavy_rows=50_000_000,
+               "per_query_launch": dict(batch_slots=3, join_slots=4, heavy_rows=50_000_000,
                                         per_query_launch=True)}
 
     def run(srv, offsets, rq=reqs):

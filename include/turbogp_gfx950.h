@@ -182,6 +182,10 @@ int32_t tgp_ctx_destroy(tgp_ctx* ctx);
 int32_t tgp_device_info_get(tgp_ctx* ctx, tgp_device_info* out);
 const char* tgp_last_error(void);
 int32_t tgp_abi_version(void);
+/* Every stream the library runs work on gets device scratch of its own (kept
+ * for reuse).  tgp_stream_destroy waits for the stream, releases that scratch
+ * and destroys the stream; it also accepts streams made outside the library.
+ * Scratch of streams destroyed elsewhere is released by tgp_ctx_destroy. */
 int32_t tgp_stream_create(tgp_ctx* ctx, void** out_stream);
 int32_t tgp_stream_destroy(tgp_ctx* ctx, void* stream);
 int32_t tgp_stream_sync(tgp_ctx* ctx, void* stream);

@@ -207,9 +207,10 @@ int32_t tgp_ctx_create(int32_t device, uint64_t hbm_budget_bytes, tgp_ctx** out)
 int32_t tgp_ctx_destroy(tgp_ctx* ctx) {
   if (!ctx) return TGP_OK;
   (void)hipSetDevice(ctx->device);
-  (void)hipStreamSynchronize(ctx->stream);
+  // Streams the caller created may already be destroyed: wait for the device,
+  // not for the streams the scratch map remembers.
+  (void)hipDeviceSynchronize();
   for (auto& kv : ctx->scratch) {
-    (void)hipStreamSynchronize(kv.first);
     if (kv.second.ptr) (void)hipFree(kv.second.ptr);
   }
   for (auto& kv : ctx->allocs) (void)hipFree(kv.first);

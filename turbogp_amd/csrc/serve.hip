@@ -492,7 +492,7 @@ void release(tgp_server* s) {
   for (AggSlot& sl : s->agg_slots) {
     if (sl.batch) (void)tgp_batch_destroy(sl.batch);
     if (sl.done_ev) (void)hipEventDestroy(sl.done_ev);
-    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    if (sl.stream) (void)tgp_stream_destroy(s->ctx, sl.stream);  // frees its scratch too
     if (sl.h_arena) (void)hipHostFree(sl.h_arena);
     if (sl.d_res) (void)hbm_free(s->ctx, sl.d_res);
   }
@@ -501,7 +501,7 @@ void release(tgp_server* s) {
     drop_pool(s, sl);
     if (sl.d_res) (void)hbm_free(s->ctx, sl.d_res);
     if (sl.done_ev) (void)hipEventDestroy(sl.done_ev);
-    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    if (sl.stream) (void)tgp_stream_destroy(s->ctx, sl.stream);  // frees its scratch too
   }
 }
 

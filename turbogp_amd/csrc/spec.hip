@@ -167,8 +167,10 @@ __global__ void __launch_bounds__(kBlock) k_wave(SpecArgs A, uint32_t rshift, ui
   if (oor && A.out_oor) atomicAdd(A.out_oor, (unsigned long long)oor);
 }
 
-// Queries of at least this many slices (4M rows) take the replicated form.
+// Queries of at least this many slices (4M rows) take the replicated form, and
+// queries of more histogram words than kHistWords from kWaveManySlices (256K rows).
 constexpr uint64_t kWaveMinSlices = 16384;
+constexpr uint64_t kWaveManySlices = 1024;
 
 // Fewest slices a wave of a single-query kernel gets before the grid shrinks
 // (each workgroup ends with a flush of device atomics, so small queries run on
@@ -194,14 +196,18 @@ int32_t launch_shape(tgp_ctx* ctx, const Shape& s, const SpecArgs& A, hipStream_
   if constexpr (I < kNumShapes) {
     if (shape_eq(s, kShapes[I])) {
       constexpr Shape S = kShapes[I];
-      // kernel form: the replicated-histogram waves for large queries of 2..4
-      // groups (the lanes of one LDS atomic would collide); the register fold
-      // otherwise (less LDS to set up and flush, which small queries feel)
+      // Kernel form (measured over 20K..60M rows): the replicated-histogram waves
+      // for large queries of several groups, and from 256K rows up for queries
+      // whose histogram outgrows a wave's registers (the register fold then
+      // goes through a workgroup table with carry-exact LDS atomics); the
+      // register fold otherwise (less LDS to set up and flush, which small
+      // queries feel).
       const int rshift = wave_rshift(A.words);
       const int64_t form = ctx->tune_spec_kernel.load();
+      const bool large = A.nslices >= kWaveMinSlices;
+      const bool many = A.words > (uint32_t)kHistWords && A.nslices >= kWaveManySlices;
       const bool wave = rshift >= 0 &&
-                        (form == 2 || (form == 0 && A.ngroups >= 2 && A.ngroups <= 4 &&
-                                       A.nslices >= kWaveMinSlices));
+                        (form == 2 || (form == 0 && A.ngroups >= 2 && (large || many)));
       int per_cu = 0;
       uint32_t lds = 0;
       if (wave) {
@@ -215,12 +221,23 @@ int32_t launch_shape(tgp_ctx* ctx, const Shape& s, const SpecArgs& A, hipStream_
       if (per_cu < 1) per_cu = 1;
       const int64_t want = ctx->tune_wg_per_cu.load();
       if (want > 0 && want < per_cu) per_cu = (int)want;
+      // Large scans stream best from few concurrent waves as long as those hide
+      // the per-row work: measured on 60M rows, one group and 2..4 groups run
+      // fastest at 2 workgroups per CU, more groups at 3.
+      if (want == 0 && large && (wave || A.ngroups == 1)) {
+        const int cap = A.ngroups <= 4 ? 2 : 3;
+        if (per_cu > cap) per_cu = cap;
+      }
       uint64_t grid = (uint64_t)persistent_grid(ctx, per_cu);
       const int64_t gmax = ctx->tune_grid_max.load();
       if (wave) {
         // 0: static interleave (default); > 0: slices per dynamically taken chunk
         const uint64_t chunk = (uint64_t)ctx->tune_chunk.load();
-        const uint64_t per_wg = (kBlock / kWave) * (chunk ? chunk : min_slices_per_wave(ctx));
+        // every workgroup ends with one device atomic per histogram word, all on
+        // the same addresses: queries of many words run on fewer, longer waves
+        uint64_t spw = min_slices_per_wave(ctx);
+        if (ctx->tune_slices_per_wave.load() == 0 && A.words / 64 > spw) spw = A.words / 64;
+        const uint64_t per_wg = (kBlock / kWave) * (chunk ? chunk : spw);
         const uint64_t need = (A.nslices + per_wg - 1) / per_wg;
         if (grid > need) grid = need;
         if (gmax > 0 && grid > (uint64_t)gmax) grid = (uint64_t)gmax;
