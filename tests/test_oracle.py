@@ -174,6 +174,37 @@ def test_min_max_vs_numpy(seed):
                            int(expr[m].min()), int(eu.max())]
 
 
+def _result(cnt, sums):
+    from turbogp_amd.device import GroupAggResult
+    w = np.array([[[v % 2**64, (v >> 64) % 2**64] for v in g] for g in sums], np.uint64)
+    return GroupAggResult(np.asarray(cnt, np.uint64), w, 0)
+
+
+def test_avg_forms():
+    from fractions import Fraction
+    # tests/interpreter_fallback.rs multi_aggregate_sum_and_avg: avg(amount) = 200
+    amount = np.array([100, 200, 150, 300, 250], np.int64)
+    cnt, sums, _ = B.group_agg(B.make_query([amount], aggs=[("sum", 1, [(0, 0, 1)])]))
+    r = _result(cnt, sums)
+    assert r.avg(0) == [200] and r.avg(0, "f64") == [200.0] and r.avg(0, "u64") == [200]
+    # tests/joins_groupby_orderby.rs:71 data: avg(b) GROUP BY grp -> 150, 300
+    grp = np.array([1, 1, 2], np.int64)
+    b = np.array([100, 200, 300], np.int64)
+    cnt, sums, _ = B.group_agg(B.make_query([grp, b], parts=[(0, 1, 1, 2)],
+                                            aggs=[("sum", 1, [(1, 0, 1)])]))
+    assert _result(cnt, sums).avg(0, "f64") == [150.0, 300.0]
+    # exact vs the two reference forms on a negative, non-integral mean
+    v = np.array([-3, 1, 1], np.int64)
+    cnt, sums, _ = B.group_agg(B.make_query([v], aggs=[("sum", 1, [(0, 0, 1)])]))
+    r = _result(cnt, sums)
+    assert r.avg(0) == [Fraction(-1, 3)] and r.avg(0, "f64") == [-1 / 3]
+    assert r.avg(0, "u64") == [(2**64 - 1) // 3]  # wrapping u64 sum, unsigned division
+    # an empty group has no average
+    cnt, sums, _ = B.group_agg(B.make_query([grp, b], parts=[(0, 1, 1, 3)],
+                                            aggs=[("sum", 1, [(1, 0, 1)])]))
+    assert _result(cnt, sums).avg(0) == [150, 300, None]
+
+
 def brute_group_agg(cols, preds, parts, aggs, b, e, sel=None):
     """Pure-Python brute force for small inputs."""
     ng = int(np.prod([p[3] for p in parts])) if parts else 1

@@ -298,6 +298,32 @@ class GroupAggResult:
         return [[int128(self.sums_words[g, a, 0], self.sums_words[g, a, 1])
                  for a in range(self.sums_words.shape[1])] for g in range(len(self.count))]
 
+    def avg(self, a, form="exact"):
+        """AVG of aggregate `a` per group from its exact sum and COUNT(*) (None
+        for a group without rows):
+          'exact': fractions.Fraction(sum, count);
+          'f64':   the sum as f64 over the count as f64, as eval_agg_expr's Avg
+                   (query_interpreter/aggregate.rs:1416-1424) for exact integer sums;
+          'u64':   (sum mod 2^64) // count, the accumulator path's unsigned
+                   truncating AVG (query_interpreter/accum_grouped.rs:380-390)."""
+        from fractions import Fraction
+        out = []
+        for g in range(len(self.count)):
+            n = int(self.count[g])
+            if n == 0:
+                out.append(None)
+                continue
+            s = int128(self.sums_words[g, a, 0], self.sums_words[g, a, 1])
+            if form == "exact":
+                out.append(Fraction(s, n))
+            elif form == "f64":
+                out.append(float(s) / float(n))
+            elif form == "u64":
+                out.append((s % (1 << 64)) // n)
+            else:
+                raise ValueError(f"avg form {form!r}: 'exact', 'f64' or 'u64'")
+        return out
+
     def rows(self):
         """Present groups as (gid, count, [sums]) sorted by gid (compare after sorting)."""
         s = self.sums()
