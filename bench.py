@@ -223,14 +223,38 @@ def bench_serve(ctx, table, join_tables, args):
         out[name]["open_loop"] = levels
         out[name]["stats"] = srv.stats()
         srv.close()
-    # the batchable part of the mix alone (scan-aggregates), all submitted at once
+    # the batchable part of the mix alone (scan-aggregates): capacity of both forms, then
+    # both at the same offered loads (50 / 90 % of the one-launch form's capacity) and
+    # the batched form at 50 / 90 % of its own
     scan = [m for m in mix if m[0] in ("q1", "q6")]
     rq = W.serve_requests(table, build, probe, scan)
+    scap = {}
     for name, kw in configs.items():
         srv = Server(ctx, **kw)
         _, span, _ = run(srv, np.zeros(len(rq), np.int64), rq)
-        out[name]["scan_only_capacity_req_s"] = round(len(rq) / span, 1)
+        scap[name] = len(rq) / span
+        out[name]["scan_only_capacity_req_s"] = round(scap[name], 1)
         srv.close()
+    equal = []
+    for frac in (0.5, 0.9):
+        qps = frac * scap["per_query_launch"]
+        row = {"offered_req_s": round(qps, 1), "fraction_of_per_query_capacity": frac}
+        for name, kw in configs.items():
+            srv = Server(ctx, **kw)
+            lat, span, _ = run(srv, W.poisson_offsets(len(rq), qps), rq)
+            row[name] = {"p50_ms": round(pct(lat, 50), 3), "p99_ms": round(pct(lat, 99), 3)}
+            srv.close()
+        equal.append(row)
+    own = []
+    srv = Server(ctx, **configs["batched"])
+    for frac in (0.5, 0.9):
+        qps = frac * scap["batched"]
+        lat, span, _ = run(srv, W.poisson_offsets(len(rq), qps), rq)
+        own.append({"offered_req_s": round(qps, 1), "fraction_of_batched_capacity": frac,
+                    "achieved_req_s": round(len(rq) / span, 1),
+                    "p50_ms": round(pct(lat, 50), 3), "p99_ms": round(pct(lat, 99), 3)})
+    srv.close()
+    out["scan_only"] = {"requests": len(rq), "equal_load": equal, "batched_open_loop": own}
     out["capacity_ratio"] = round(cap / out["per_query_launch"]["capacity_req_s"], 2)
     out["scan_only_capacity_ratio"] = round(out["batched"]["scan_only_capacity_req_s"] /
                                             out["per_query_launch"]["scan_only_capacity_req_s"], 2)

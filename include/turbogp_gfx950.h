@@ -435,7 +435,9 @@ typedef struct tgp_serve_config {
   uint32_t max_batch_queries;  /* requests per batch */
   uint32_t batch_slots;        /* batches in flight (1..8) */
   uint32_t join_slots;         /* joins in flight (1..8) */
-  uint32_t _pad;
+  uint32_t batch_wait_us;      /* while another batch runs, a free slot waits for
+                                  max_batch_queries / 4 requests or until the oldest has
+                                  waited this long (0: launch whatever is queued) */
   uint64_t heavy_rows;         /* ranges of >= this many rows run alone, one at a time (0 = off) */
   uint64_t hash_budget_bytes;  /* cap on hash-table HBM (0 = the tier's budget) */
   uint32_t wg_per_cu, chunk_slices; /* batch tuning (tgp_batch_tuning) */
@@ -451,6 +453,10 @@ typedef struct tgp_serve_timing {
 typedef struct tgp_serve_stats {
   uint64_t submitted, completed, failed, rejected, deferred, admitted_joins;
   uint64_t batches, batched_requests, max_batch, heavy_in_flight;
+  /* host time (ns, summed over calls): inside tgp_serve_submit, launching aggregate
+   * work (batch planning and upload, or the single launches), and completing it
+   * (result copies) */
+  uint64_t submit_ns, launch_ns, harvest_ns;
 } tgp_serve_stats;
 int64_t tgp_serve_now_ns(void);
 int32_t tgp_serve_create(tgp_ctx* ctx, const tgp_serve_config* cfg, tgp_server** out);

@@ -126,7 +126,8 @@ class ServeRequest(C.Structure):
 
 class ServeConfig(C.Structure):
     _fields_ = [("max_batch_queries", C.c_uint32), ("batch_slots", C.c_uint32),
-                ("join_slots", C.c_uint32), ("_pad", C.c_uint32), ("heavy_rows", C.c_uint64),
+                ("join_slots", C.c_uint32), ("batch_wait_us", C.c_uint32),
+                ("heavy_rows", C.c_uint64),
                 ("hash_budget_bytes", C.c_uint64), ("wg_per_cu", C.c_uint32),
                 ("chunk_slices", C.c_uint32), ("order", C.c_int32), ("cached_loads", C.c_int32),
                 ("per_query_launch", C.c_int32), ("_pad2", C.c_int32)]
@@ -140,7 +141,8 @@ class ServeTiming(C.Structure):
 class ServeStats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "submitted", "completed", "failed", "rejected", "deferred", "admitted_joins",
-        "batches", "batched_requests", "max_batch", "heavy_in_flight")]
+        "batches", "batched_requests", "max_batch", "heavy_in_flight",
+        "submit_ns", "launch_ns", "harvest_ns")]
 
 
 # name -> (restype, argtypes)

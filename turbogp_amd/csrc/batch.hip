@@ -214,7 +214,24 @@ struct tgp_batch {
   hipGraphExec_t graph = nullptr;
   hipGraph_t graph_def = nullptr;
   hipStream_t graph_stream = nullptr;
+  std::vector<void*> retired;  // outgrown device buffers, freed with the batch
 };
+
+// Grows a device buffer to at least `need` elements.  The old buffer is not
+// freed now: hipFree synchronises the whole device, which would stall every
+// other stream's work (a server plans one batch while others run).  Buffers
+// grow at least twofold, so the retired ones add up to less than the last.
+template <typename T>
+static int32_t ensure_cap(tgp_batch* b, T** p, uint64_t* cap, uint64_t need) {
+  if (*cap >= need) return TGP_OK;
+  const uint64_t n = need > 2 * *cap ? need : 2 * *cap;
+  void* q = nullptr;
+  TGP_TRY(hbm_alloc(b->ctx, n * sizeof(T), &q));
+  if (*p) b->retired.push_back(*p);
+  *p = static_cast<T*>(q);
+  *cap = n;
+  return TGP_OK;
+}
 
 static void drop_graph(tgp_batch* b) {
   if (b->graph) (void)hipGraphExecDestroy(b->graph);
@@ -274,56 +291,77 @@ static int32_t build_items(tgp_batch* b, hipStream_t s) {
   if (b->order == 0 || (b->items_chunk == b->chunk && b->items_order == b->order &&
                         b->items_valid))
     return TGP_OK;
-  struct Item {
-    uint64_t key;  // sort key: start row (order 1) or absolute chunk (order 2)
-    uint32_t q, sb, se;
-  };
   const uint32_t nq = b->order == 2 ? kQueues : 1;
-  std::vector<std::vector<Item>> queues(nq);
-  const uint64_t C = (uint64_t)b->chunk * kSliceRows;  // rows per absolute chunk
-  for (uint32_t q = 0; q < b->plan.size(); ++q) {
-    const SpecArgs& A = b->plan[q].a;
-    if (A.nslices == 0) continue;
-    if (b->order == 1) {
-      for (uint64_t c = 0; c < A.nslices; c += b->chunk)
-        queues[0].push_back(Item{A.base0 + c * kSliceRows, q, (uint32_t)c,
-                                 (uint32_t)(c + b->chunk < A.nslices ? c + b->chunk : A.nslices)});
-    } else {
-      const uint64_t end = A.base0 + A.nslices * kSliceRows;
-      for (uint64_t k = A.base0 / C; k * C < end; ++k) {
-        const uint64_t r0 = k * C > A.base0 ? k * C : A.base0;
-        const uint64_t r1 = (k + 1) * C < end ? (k + 1) * C : end;
-        queues[k % nq].push_back(Item{k, q, (uint32_t)((r0 - A.base0) / kSliceRows),
-                                      (uint32_t)((r1 - A.base0) / kSliceRows)});
-      }
-    }
-  }
   std::vector<uint32_t> items;
   uint64_t total = 0;
-  for (auto& qv : queues) total += qv.size();
-  items.reserve(3 * (total ? total : 1));
-  for (uint32_t i = 0; i < nq; ++i) {
-    auto& qv = queues[i];
+  if (b->order == 1) {
+    // one queue sorted by the table row each item starts at
+    struct Item {
+      uint64_t key;
+      uint32_t q, sb, se;
+    };
+    std::vector<Item> qv;
+    for (uint32_t q = 0; q < b->plan.size(); ++q) {
+      const SpecArgs& A = b->plan[q].a;
+      for (uint64_t c = 0; c < A.nslices; c += b->chunk)
+        qv.push_back(Item{A.base0 + c * kSliceRows, q, (uint32_t)c,
+                          (uint32_t)(c + b->chunk < A.nslices ? c + b->chunk : A.nslices)});
+    }
     std::sort(qv.begin(), qv.end(), [](const Item& x, const Item& y) {
       return x.key != y.key ? x.key < y.key : x.q < y.q;
     });
-    b->queue_off[i] = items.size() / 3;
+    total = qv.size();
+    items.reserve(3 * (total ? total : 1));
+    b->queue_off[0] = 0;
     for (const Item& it : qv) {
       items.push_back(it.q);
       items.push_back(it.sb);
       items.push_back(it.se);
     }
+  } else {
+    // XCD-affine: absolute table chunk k goes to queue k % 8, items in (k, query)
+    // order.  Placed by counting (no sort): a chunk's items are the queries that
+    // overlap it, visited in query order.
+    const uint64_t C = (uint64_t)b->chunk * kSliceRows;  // rows per absolute chunk
+    uint64_t kmax = 0;
+    for (const BatchQuery& bq : b->plan)
+      if (bq.a.nslices) {
+        const uint64_t end = bq.a.base0 + bq.a.nslices * kSliceRows;
+        kmax = (end + C - 1) / C > kmax ? (end + C - 1) / C : kmax;
+      }
+    // position of chunk k's first item: queues in order, chunks k = i, i + 8, ... in queue i
+    std::vector<uint64_t> at(kmax + 1, 0);
+    for (const BatchQuery& bq : b->plan)
+      if (bq.a.nslices) {
+        const uint64_t end = bq.a.base0 + bq.a.nslices * kSliceRows;
+        for (uint64_t k = bq.a.base0 / C; k * C < end; ++k) at[k] += 1;
+      }
+    for (uint32_t i = 0; i < nq; ++i) {
+      b->queue_off[i] = total;
+      for (uint64_t k = i; k < kmax; k += nq) {
+        const uint64_t c = at[k];
+        at[k] = total;
+        total += c;
+      }
+    }
+    items.resize(3 * (total ? total : 1));
+    for (uint32_t q = 0; q < b->plan.size(); ++q) {
+      const SpecArgs& A = b->plan[q].a;
+      if (A.nslices == 0) continue;
+      const uint64_t end = A.base0 + A.nslices * kSliceRows;
+      for (uint64_t k = A.base0 / C; k * C < end; ++k) {
+        const uint64_t r0 = k * C > A.base0 ? k * C : A.base0;
+        const uint64_t r1 = (k + 1) * C < end ? (k + 1) * C : end;
+        const uint64_t pos = at[k]++;
+        items[3 * pos] = q;
+        items[3 * pos + 1] = (uint32_t)((r0 - A.base0) / kSliceRows);
+        items[3 * pos + 2] = (uint32_t)((r1 - A.base0) / kSliceRows);
+      }
+    }
   }
-  for (uint32_t i = nq; i <= kQueues; ++i) b->queue_off[i] = items.size() / 3;
+  for (uint32_t i = nq; i <= kQueues; ++i) b->queue_off[i] = total;
   if (items.empty()) items.assign(3, 0);
-  if (b->items_cap < items.size()) {
-    if (b->d_items) TGP_TRY(hbm_free(b->ctx, b->d_items));
-    b->d_items = nullptr;
-    void* p = nullptr;
-    TGP_TRY(hbm_alloc(b->ctx, items.size() * 4, &p));
-    b->d_items = static_cast<uint32_t*>(p);
-    b->items_cap = items.size();
-  }
+  TGP_TRY(ensure_cap(b, &b->d_items, &b->items_cap, (uint64_t)items.size()));
   TGP_HIP(hipMemcpyAsync(b->d_items, items.data(), items.size() * 4, hipMemcpyHostToDevice, s));
   TGP_HIP(hipStreamSynchronize(s));
   b->nitems = total;
@@ -360,6 +398,7 @@ int32_t tgp_batch_destroy(tgp_batch* b) {
   if (b->d_results) hbm_free(b->ctx, b->d_results);
   if (b->d_granule) hbm_free(b->ctx, b->d_granule);
   if (b->d_items) hbm_free(b->ctx, b->d_items);
+  for (void* p : b->retired) hbm_free(b->ctx, p);
   delete b;
   return TGP_OK;
 }
@@ -395,14 +434,7 @@ int32_t tgp_batch_set_queries(tgp_batch* b, const tgp_agg_query* qs, uint32_t n,
   }
   b->result_words = words;
   // the arena ends with the persistent kernels' work counters
-  if (b->results_cap < words + kQueues) {
-    if (b->d_results) TGP_TRY(hbm_free(b->ctx, b->d_results));
-    b->d_results = nullptr;
-    void* p = nullptr;
-    TGP_TRY(hbm_alloc(b->ctx, (words + kQueues) * 8, &p));
-    b->d_results = static_cast<uint64_t*>(p);
-    b->results_cap = words + kQueues;
-  }
+  TGP_TRY(ensure_cap(b, &b->d_results, &b->results_cap, words + kQueues));
   uint64_t slices = 0;
   for (uint32_t i = 0; i < n; ++i) {
     Shape sh;
@@ -429,14 +461,7 @@ int32_t tgp_batch_set_queries(tgp_batch* b, const tgp_agg_query* qs, uint32_t n,
   }
   b->total_slices = slices;
   const uint64_t bytes = b->plan.size() * sizeof(BatchQuery);
-  if (b->plan_cap < b->plan.size()) {
-    if (b->d_plan) TGP_TRY(hbm_free(b->ctx, b->d_plan));
-    b->d_plan = nullptr;
-    void* p = nullptr;
-    TGP_TRY(hbm_alloc(b->ctx, bytes ? bytes : 64, &p));
-    b->d_plan = static_cast<BatchQuery*>(p);
-    b->plan_cap = b->plan.size();
-  }
+  TGP_TRY(ensure_cap(b, &b->d_plan, &b->plan_cap, b->plan.size() ? b->plan.size() : 1));
   // slice -> query table: granule g holds the plan index of the query that
   // contains slice g * kGranSlices
   const uint64_t ngran = (slices + kGranSlices - 1) / kGranSlices;
@@ -445,14 +470,7 @@ int32_t tgp_batch_set_queries(tgp_batch* b, const tgp_agg_query* qs, uint32_t n,
     while (q + 1 < b->plan.size() && b->plan[q + 1].slice0 <= g * kGranSlices) ++q;
     gran[g] = (uint32_t)q;
   }
-  if (b->granule_cap < gran.size()) {
-    if (b->d_granule) TGP_TRY(hbm_free(b->ctx, b->d_granule));
-    b->d_granule = nullptr;
-    void* p = nullptr;
-    TGP_TRY(hbm_alloc(b->ctx, gran.size() * 4, &p));
-    b->d_granule = static_cast<uint32_t*>(p);
-    b->granule_cap = gran.size();
-  }
+  TGP_TRY(ensure_cap(b, &b->d_granule, &b->granule_cap, gran.size()));
   if (bytes) {
     TGP_HIP(hipMemcpyAsync(b->d_plan, b->plan.data(), bytes, hipMemcpyHostToDevice, s));
     TGP_HIP(hipMemcpyAsync(b->d_granule, gran.data(), gran.size() * 4, hipMemcpyHostToDevice, s));

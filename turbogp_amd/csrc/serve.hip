@@ -99,7 +99,10 @@ struct tgp_server {
   tgp_ctx* ctx = nullptr;
   tgp_serve_config cfg{};
   std::mutex mu;
-  std::condition_variable work_cv;  // new requests / stop
+  // New work wakes only the workers that can take it: one submit used to wake
+  // every worker thread, which then contended with the submitter for `mu`.
+  std::condition_variable agg_cv;   // aggregate requests / stop
+  std::condition_variable join_cv;  // join requests / stop
   std::condition_variable done_cv;  // completions
   std::deque<Request*> agg_q, heavy_q, join_q;
   std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
@@ -190,9 +193,19 @@ int32_t launch_single(tgp_server* s, AggSlot& sl, Request* r) {
   return TGP_OK;
 }
 
-// Copies a finished batch's results out and completes its requests.
+int32_t harvest_batch(tgp_server* s, AggSlot& sl);
+
+// Copies a finished batch's (or single launch's) results out and completes its
+// requests.
 int32_t harvest_agg(tgp_server* s, AggSlot& sl) {
-  if (sl.single) return harvest_single(s, sl);
+  const int64_t t0 = now_ns();
+  const int32_t rc = sl.single ? harvest_single(s, sl) : harvest_batch(s, sl);
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->stats.harvest_ns += (uint64_t)(now_ns() - t0);
+  return rc;
+}
+
+int32_t harvest_batch(tgp_server* s, AggSlot& sl) {
   uint64_t words = 0;
   int32_t rc = tgp_batch_info(sl.batch, &words, nullptr, nullptr, nullptr);
   if (rc == TGP_OK && words > sl.h_cap) {
@@ -240,6 +253,7 @@ void launch_agg(tgp_server* s, AggSlot& sl, std::vector<Request*>&& take, bool h
     }
   }
   std::lock_guard<std::mutex> lk(s->mu);
+  s->stats.launch_ns += (uint64_t)(now_ns() - t0);
   if (rc != TGP_OK) {
     for (Request* r : take) finish(s, r, rc);
     if (heavy) s->stats.heavy_in_flight -= 1;
@@ -285,8 +299,18 @@ void agg_worker(tgp_server* s) {
             heavy = true;
             s->stats.heavy_in_flight += 1;
           }
-        } else {
-          while (!s->agg_q.empty() && take.size() < maxq) {
+        } else if (!s->agg_q.empty()) {
+          // continuous batching with a bounded wait: while another batch runs,
+          // gather a larger batch (a quarter of the maximum, or the oldest
+          // request's wait reaching batch_wait_us) instead of launching a
+          // handful of requests; an idle device launches at once
+          bool others_busy = false;
+          for (size_t j = 0; j < s->agg_slots.size(); ++j)
+            others_busy |= j != si && !(s->heavy_slot && j == 0) && s->agg_slots[j].busy;
+          const int64_t waited = now_ns() - s->agg_q.front()->t_arrive;
+          const bool go = !others_busy || s->agg_q.size() >= (maxq + 3) / 4 ||
+                          waited >= (int64_t)s->cfg.batch_wait_us * 1000;
+          while (go && !s->agg_q.empty() && take.size() < maxq) {
             take.push_back(s->agg_q.front());
             s->agg_q.pop_front();
           }
@@ -302,8 +326,8 @@ void agg_worker(tgp_server* s) {
     if (s->stop && !busy && s->agg_q.empty() && s->heavy_q.empty()) return;
     if (!progressed) {
       // poll in-flight batches every few microseconds; sleep when idle
-      if (busy) s->work_cv.wait_for(lk, std::chrono::microseconds(20));
-      else s->work_cv.wait_for(lk, std::chrono::milliseconds(5));
+      if (busy) s->agg_cv.wait_for(lk, std::chrono::microseconds(20));
+      else s->agg_cv.wait_for(lk, std::chrono::milliseconds(5));
     }
   }
 }
@@ -482,8 +506,8 @@ void join_worker(tgp_server* s, size_t slot) {
     std::unique_lock<std::mutex> lk(s->mu);
     if (s->stop && !sl.busy && s->join_q.empty()) return;
     if (!progressed) {
-      if (sl.busy) s->work_cv.wait_for(lk, std::chrono::microseconds(20));
-      else s->work_cv.wait_for(lk, std::chrono::milliseconds(5));
+      if (sl.busy) s->join_cv.wait_for(lk, std::chrono::microseconds(20));
+      else s->join_cv.wait_for(lk, std::chrono::milliseconds(5));
     }
   }
 }
@@ -571,7 +595,8 @@ int32_t tgp_serve_destroy(tgp_server* s) {
     std::lock_guard<std::mutex> lk(s->mu);
     s->stop = true;
   }
-  s->work_cv.notify_all();
+  s->agg_cv.notify_all();
+  s->join_cv.notify_all();
   if (s->agg_thread.joinable()) s->agg_thread.join();
   for (std::thread& t : s->join_threads)
     if (t.joinable()) t.join();
@@ -583,6 +608,7 @@ int32_t tgp_serve_destroy(tgp_server* s) {
 int32_t tgp_serve_submit(tgp_server* s, const tgp_serve_request* req, int64_t arrival_ns,
                          uint64_t* out_id) {
   TGP_REQUIRE(s && req && out_id, "tgp_serve_submit: null argument");
+  const int64_t t_in = now_ns();
   auto r = std::make_unique<Request>();
   r->req = *req;
   TGP_TRY(result_shape(*req, &r->ngroups, &r->naggs, &r->rows));
@@ -595,6 +621,7 @@ int32_t tgp_serve_submit(tgp_server* s, const tgp_serve_request* req, int64_t ar
     TGP_TRY(tgp_group_agg_explain(&req->agg, &strategy, &shape));
   }
   r->t_arrive = arrival_ns ? arrival_ns : now_ns();
+  bool is_join = false;
   {
     std::lock_guard<std::mutex> lk(s->mu);
     TGP_REQUIRE(!s->stop, "tgp_serve_submit: server is stopping");
@@ -603,11 +630,15 @@ int32_t tgp_serve_submit(tgp_server* s, const tgp_serve_request* req, int64_t ar
     Request* raw = r.get();
     s->reqs.emplace(raw->id, std::move(r));
     s->stats.submitted += 1;
-    if (raw->req.kind == TGP_SERVE_JOIN_AGG) s->join_q.push_back(raw);
+    s->stats.submit_ns += (uint64_t)(now_ns() - t_in);
+    is_join = raw->req.kind == TGP_SERVE_JOIN_AGG;
+    if (is_join) s->join_q.push_back(raw);
     else if (s->heavy_slot && raw->rows >= s->cfg.heavy_rows) s->heavy_q.push_back(raw);
     else s->agg_q.push_back(raw);
   }
-  s->work_cv.notify_all();
+  // any idle join worker can take a join; the one aggregate worker takes the rest
+  if (is_join) s->join_cv.notify_all();
+  else s->agg_cv.notify_one();
   return TGP_OK;
 }
 

@@ -65,9 +65,9 @@ def parse_words(words, ngroups, naggs):
 class Server:
     def __init__(self, ctx, max_batch_queries=1024, batch_slots=2, join_slots=2, heavy_rows=0,
                  hash_budget_bytes=0, wg_per_cu=0, chunk_slices=32, order=2, cached_loads=True,
-                 per_query_launch=False):
+                 per_query_launch=False, batch_wait_us=250):
         self.ctx = ctx
-        cfg = A.ServeConfig(max_batch_queries, batch_slots, join_slots, 0, heavy_rows,
+        cfg = A.ServeConfig(max_batch_queries, batch_slots, join_slots, batch_wait_us, heavy_rows,
                             hash_budget_bytes, wg_per_cu, chunk_slices, order, int(cached_loads),
                             int(per_query_launch), 0)
         h = C.c_void_p()
