@@ -336,6 +336,27 @@ int32_t tgp_batch_result_layout(tgp_batch* b, uint32_t i, uint64_t* count_off, u
 /* Copies the first nwords words of the result arena to the host (synchronous). */
 int32_t tgp_batch_results_sync(tgp_batch* b, uint64_t* h_words, uint64_t nwords, void* stream);
 
+/* ---- COUNT(DISTINCT col) ------------------------------------------------------ */
+/* Distinct u64 cells (values widened as the reference stores them, signed types
+ * sign-extended) among rows [row_begin, row_end) of `col` selected by the
+ * optional bitmap (bit i <-> row_begin + i, NULL = all).
+ *   EXACT: the count of count_distinct_masked (exec/vectorized.rs:601-610);
+ *   HLL:   HyperLogLog, 16384 registers over xxh3_64 of each cell's 8 LE bytes,
+ *          the estimate of exec/hll.rs:53-75;
+ *   AUTO:  as COUNT(DISTINCT col) in query_interpreter/aggregate.rs:1379-1400:
+ *          exact below 100 000 selected rows, else HyperLogLog.
+ * Synchronous: *h_count is set on return. */
+enum tgp_distinct_mode { TGP_DISTINCT_AUTO = 0, TGP_DISTINCT_EXACT = 1, TGP_DISTINCT_HLL = 2 };
+int32_t tgp_count_distinct(tgp_ctx* ctx, const tgp_column* col, uint64_t row_begin,
+                           uint64_t row_end, const uint64_t* d_sel_bitmap, int32_t mode,
+                           uint64_t* h_count, void* stream);
+/* The 16384 HyperLogLog registers (HyperLogLog::add, exec/hll.rs:40-50) of the
+ * same selection, into host memory; mergeable by a bytewise max (HyperLogLog::merge). */
+int32_t tgp_hll_sketch(tgp_ctx* ctx, const tgp_column* col, uint64_t row_begin, uint64_t row_end,
+                       const uint64_t* d_sel_bitmap, uint8_t* h_registers, void* stream);
+/* HyperLogLog::estimate (exec/hll.rs:53-75) of 16384 host registers. */
+uint64_t tgp_hll_estimate(const uint8_t* h_registers);
+
 /* ---- hash join --------------------------------------------------------------- */
 typedef struct tgp_hash_table tgp_hash_table;
 

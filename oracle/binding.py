@@ -71,6 +71,10 @@ def lib():
             "orc_join_agg": (I32, [P, P, U64, P, P, P, U64, U64, I32, I64, U64, I32, P, P, P]),
             "orc_join_pairs": (U64, [P, U64, P, U64, P, P, U64]),
             "orc_join_hash": (U64, [U64]),
+            "orc_xxh3_u64": (U64, [U64]),
+            "orc_hll_registers": (None, [P, I32, U64, P, P]),
+            "orc_hll_estimate": (U64, [P]),
+            "orc_count_distinct": (U64, [P, I32, U64, P, I32]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -237,3 +241,29 @@ def join_pairs(build_keys, probe_keys, capacity=None):
     ob = np.zeros(cap, np.uint32)
     n = lib().orc_join_pairs(ptr(bk), len(bk), ptr(pk), len(pk), ptr(op), ptr(ob), cap)
     return op[:min(n, cap)], ob[:min(n, cap)], n
+
+
+def xxh3_u64(v):
+    return int(lib().orc_xxh3_u64(int(v) % (1 << 64)))
+
+
+def hll_registers(col, sel_words=None):
+    """16384 HyperLogLog registers of the selected cells (exec/hll.rs)."""
+    col = np.ascontiguousarray(col)
+    regs = np.zeros(1 << 14, np.uint8)
+    sel = None if sel_words is None else np.ascontiguousarray(sel_words, np.uint64)
+    lib().orc_hll_registers(ptr(col), _FROM_NP[col.dtype], len(col),
+                            ptr(sel) if sel is not None else None, ptr(regs))
+    return regs
+
+
+def hll_estimate(regs):
+    regs = np.ascontiguousarray(regs, np.uint8)
+    return int(lib().orc_hll_estimate(ptr(regs)))
+
+
+def count_distinct(col, sel_words=None, mode=0):
+    col = np.ascontiguousarray(col)
+    sel = None if sel_words is None else np.ascontiguousarray(sel_words, np.uint64)
+    return int(lib().orc_count_distinct(ptr(col), _FROM_NP[col.dtype], len(col),
+                                        ptr(sel) if sel is not None else None, mode))

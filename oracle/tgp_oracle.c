@@ -10,6 +10,7 @@
  */
 #include "tgp_oracle.h"
 
+#include <math.h>
 #include <omp.h>
 #include <stdlib.h>
 #include <string.h>
@@ -649,4 +650,92 @@ int32_t orc_join_agg(const int64_t* build_keys, const int32_t* build_payload, ui
   free(maps);
   if (out_oor) *out_oor = oor;
   return 0;
+}
+
+/* ---- COUNT(DISTINCT col) ----------------------------------------------------------- */
+/* XXH3_64bits of the 8 little-endian bytes of v, seed 0, default secret: the
+ * published XXH3 algorithm's 4..8-byte path (xxHash 0.8, which the reference's
+ * xxhash-rust 0.8 implements; exec/hll.rs:34 hashes value.to_le_bytes()).  The
+ * bit-flip constant is readLE64(secret + 8) ^ readLE64(secret + 16) of the
+ * default 192-byte secret; the input64 of that path is the value with its
+ * 32-bit halves swapped. */
+uint64_t orc_xxh3_u64(uint64_t v) {
+  const uint64_t kMx2 = 0x9FB21C651E98DF25ull;
+  const uint64_t bitflip = 0x1cad21f72c81017cull ^ 0xdb979083e96dd4deull;
+  uint64_t h = ((v << 32) | (v >> 32)) ^ bitflip;
+  h ^= ((h << 49) | (h >> 15)) ^ ((h << 24) | (h >> 40));
+  h *= kMx2;
+  h ^= (h >> 35) + 8;
+  h *= kMx2;
+  return h ^ (h >> 28);
+}
+
+/* HyperLogLog::add_hashed (exec/hll.rs:40-50): register = top 14 bits, rank =
+ * leading zeros of the remaining 50 bits (shifted to the top) + 1. */
+static void hll_add(uint8_t* regs, uint64_t h) {
+  const uint64_t idx = h >> 50;
+  const uint64_t w = h << 14;
+  const uint8_t lz = (uint8_t)((w ? __builtin_clzll(w) : 64) + 1);
+  if (lz > regs[idx]) regs[idx] = lz;
+}
+
+/* Registers (16384 bytes) of the selected cells of col[0, n): `sel` is a bitmap
+ * (bit i <-> element i) or NULL for all. */
+void orc_hll_registers(const void* col, int32_t dtype, uint64_t n, const uint64_t* sel,
+                       uint8_t* regs) {
+  memset(regs, 0, 1u << 14);
+  for (uint64_t i = 0; i < n; ++i) {
+    if (sel && !((sel[i >> 6] >> (i & 63)) & 1)) continue;
+    hll_add(regs, orc_xxh3_u64((uint64_t)get_native(col, dtype, i)));
+  }
+}
+
+/* HyperLogLog::estimate (exec/hll.rs:53-75): raw = alpha m^2 / sum 2^-r, the sum
+ * taken in register order; linear counting when raw <= 2.5 m and a register is 0;
+ * truncated to an integer. */
+uint64_t orc_hll_estimate(const uint8_t* regs) {
+  const double m = 16384.0;
+  const double alpha = 0.7213 / (1.0 + 1.079 / m);
+  double sum = 0.0;
+  uint64_t zeros = 0;
+  for (int i = 0; i < 16384; ++i) {
+    sum += pow(2.0, -(double)regs[i]);
+    if (regs[i] == 0) zeros++;
+  }
+  const double raw = alpha * m * m / sum;
+  const double est = (raw <= 2.5 * m && zeros > 0) ? m * log(m / (double)zeros) : raw;
+  return (uint64_t)est;
+}
+
+static int cmp_u64(const void* a, const void* b) {
+  const uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+  return x < y ? -1 : x > y;
+}
+
+/* COUNT(DISTINCT col) of the selected cells.  mode 1: exact (the HashSet of
+ * count_distinct_masked, vectorized.rs:601-610); 2: HyperLogLog; 0: as the
+ * interpreter chooses (aggregate.rs:1379-1400, hll.rs:97-128): exact below
+ * 100 000 selected cells, else HyperLogLog. */
+uint64_t orc_count_distinct(const void* col, int32_t dtype, uint64_t n, const uint64_t* sel,
+                            int32_t mode) {
+  uint64_t nsel = 0;
+  for (uint64_t i = 0; i < n; ++i)
+    if (!sel || ((sel[i >> 6] >> (i & 63)) & 1)) nsel++;
+  if (mode == 2 || (mode == 0 && nsel >= 100000)) {
+    uint8_t* regs = (uint8_t*)malloc(1u << 14);
+    orc_hll_registers(col, dtype, n, sel, regs);
+    const uint64_t e = orc_hll_estimate(regs);
+    free(regs);
+    return e;
+  }
+  uint64_t* v = (uint64_t*)malloc((nsel ? nsel : 1) * 8);
+  uint64_t k = 0;
+  for (uint64_t i = 0; i < n; ++i)
+    if (!sel || ((sel[i >> 6] >> (i & 63)) & 1)) v[k++] = (uint64_t)get_native(col, dtype, i);
+  qsort(v, k, 8, cmp_u64);
+  uint64_t d = 0;
+  for (uint64_t i = 0; i < k; ++i)
+    if (i == 0 || v[i] != v[i - 1]) d++;
+  free(v);
+  return d;
 }

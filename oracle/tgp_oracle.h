@@ -96,6 +96,17 @@ uint64_t orc_join_pairs(const int64_t* build_keys, uint64_t n_build, const int64
 /* JoinHashTable::hash (CRC32 + multiply) for the distribution test. */
 uint64_t orc_join_hash(uint64_t key);
 
+/* COUNT(DISTINCT col): xxh3_64 of a u64 cell's 8 LE bytes (xxhash-rust 0.8,
+ * exec/hll.rs:34), HyperLogLog p = 14 registers and estimate (exec/hll.rs),
+ * and the count (mode 0: exact below 100 000 selected cells, else HLL; 1:
+ * exact; 2: HLL).  sel: bitmap over the n elements or NULL. */
+uint64_t orc_xxh3_u64(uint64_t v);
+void orc_hll_registers(const void* col, int32_t dtype, uint64_t n, const uint64_t* sel,
+                       uint8_t* regs);
+uint64_t orc_hll_estimate(const uint8_t* regs);
+uint64_t orc_count_distinct(const void* col, int32_t dtype, uint64_t n, const uint64_t* sel,
+                            int32_t mode);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,6 +29,7 @@ EQ, NE, LT, LE, GT, GE = range(6)
 BIT_AND, BIT_OR, BIT_ANDNOT = range(3)
 AGG_SUM, AGG_COUNT_NONZERO = 0, 1
 AGG_MIN, AGG_MAX, AGG_MIN_U64, AGG_MAX_U64 = 2, 3, 4, 5
+DISTINCT_AUTO, DISTINCT_EXACT, DISTINCT_HLL = 0, 1, 2
 AGG_MINMAX = {"min": AGG_MIN, "max": AGG_MAX, "min_u64": AGG_MIN_U64, "max_u64": AGG_MAX_U64}
 JOIN_GROUP_BUILD_PAYLOAD, JOIN_GROUP_PROBE_COL, JOIN_GROUP_NONE = 0, 1, 2
 TUNE_WG_PER_CU, TUNE_GRID_MAX, TUNE_SPECIALIZE, TUNE_SPEC_KERNEL, TUNE_CHUNK = 0, 1, 2, 3, 4
@@ -214,6 +215,10 @@ SIGNATURES = {
     "tgp_hash_build": (I32_, [P, C.POINTER(Column), C.POINTER(Column), C.POINTER(P), P]),
     "tgp_hash_table_destroy": (I32_, [P, P]),
     "tgp_hash_table_footprint": (I32_, [P, C.POINTER(U64_)]),
+    "tgp_count_distinct": (I32_, [P, C.POINTER(Column), U64_, U64_, P, I32_,
+                                  C.POINTER(U64_), P]),
+    "tgp_hll_sketch": (I32_, [P, C.POINTER(Column), U64_, U64_, P, P, P]),
+    "tgp_hll_estimate": (U64_, [P]),
     "tgp_hash_build_in": (I32_, [P, C.POINTER(Column), C.POINTER(Column), P, U64_,
                                  C.POINTER(C.c_void_p), P]),
     "tgp_hash_table_bytes_for": (U64_, [U64_]),

@@ -1,0 +1,296 @@
+// COUNT(DISTINCT col) over a row range and an optional selection bitmap.
+//
+// Reference path replaced:
+//   exec/vectorized.rs:601-610   count_distinct_masked: exact HashSet of u64 cells
+//   query_interpreter/aggregate.rs:1379-1400 and exec/hll.rs:97-128: exact below
+//                                100 000 selected cells, else HyperLogLog (p = 14)
+//   exec/hll.rs:32-75            HyperLogLog add / estimate over xxh3_64 of the
+//                                cell's 8 little-endian bytes
+// Cells are the values widened to u64 as the reference stores them (signed
+// types sign-extended), so the distinct set and the hashes are the reference's.
+//
+// HyperLogLog: each workgroup keeps the 16384 registers as 32-bit LDS words
+// (LDS atomic max), writes them as bytes to its slice of scratch, and a second
+// kernel takes the per-register maximum over the workgroups.  The estimate is
+// computed on the host from the 16 KiB of registers, in the reference's order
+// of summation, so it is the reference's number bit for bit.
+//
+// Exact: an open-addressing set of u64 cells in scratch (capacity the power of
+// two >= 2x the rows in range, so at most half full and every probe sequence
+// ends at an empty slot), inserted with 64-bit CAS; each successful insert is
+// one distinct value.  The empty marker value itself is counted by a flag.
+#include "tgp_internal.h"
+
+namespace tgp {
+
+constexpr uint32_t kHllBits = 14;
+constexpr uint32_t kHllRegs = 1u << kHllBits;
+constexpr unsigned long long kSetEmpty = ~0ull;
+// the interpreter's switch from an exact set to HyperLogLog (aggregate.rs:1387, hll.rs:100)
+constexpr uint64_t kDistinctExactBelow = 100000;
+
+// XXH3_64bits(8 little-endian bytes of v), seed 0, default secret: the 4..8-byte
+// path of the published XXH3 algorithm (xxHash 0.8).  The constant is
+// readLE64(secret + 8) ^ readLE64(secret + 16) of the default secret; that
+// path's input word is v with its 32-bit halves swapped.
+__host__ __device__ __forceinline__ uint64_t xxh3_u64(uint64_t v) {
+  const uint64_t kMx2 = 0x9FB21C651E98DF25ull;
+  const uint64_t kFlip = 0x1cad21f72c81017cull ^ 0xdb979083e96dd4deull;
+  uint64_t h = ((v << 32) | (v >> 32)) ^ kFlip;
+  h ^= ((h << 49) | (h >> 15)) ^ ((h << 24) | (h >> 40));
+  h *= kMx2;
+  h ^= (h >> 35) + 8;
+  h *= kMx2;
+  return h ^ (h >> 28);
+}
+
+template <int DT>
+__device__ __forceinline__ uint64_t cell_at(const uint8_t* col, uint64_t i) {
+  if constexpr (DT == TGP_I8) return (uint64_t)(int64_t)reinterpret_cast<const int8_t*>(col)[i];
+  else if constexpr (DT == TGP_U8) return col[i];
+  else if constexpr (DT == TGP_I16) return (uint64_t)(int64_t)reinterpret_cast<const int16_t*>(col)[i];
+  else if constexpr (DT == TGP_U16) return reinterpret_cast<const uint16_t*>(col)[i];
+  else if constexpr (DT == TGP_I32) return (uint64_t)(int64_t)reinterpret_cast<const int32_t*>(col)[i];
+  else if constexpr (DT == TGP_U32) return reinterpret_cast<const uint32_t*>(col)[i];
+  else return reinterpret_cast<const uint64_t*>(col)[i];
+}
+
+__device__ __forceinline__ bool selected(const uint64_t* sel, uint64_t i) {
+  return !sel || ((sel[i >> 6] >> (i & 63)) & 1);
+}
+
+template <int DT>
+__global__ void __launch_bounds__(256) k_hll(const uint8_t* __restrict__ col, uint64_t row_begin,
+                                             uint64_t n, const uint64_t* __restrict__ sel,
+                                             uint8_t* __restrict__ partial) {
+  __shared__ uint32_t reg[kHllRegs];  // 64 KiB
+  for (uint32_t i = threadIdx.x; i < kHllRegs; i += 256) reg[i] = 0;
+  __syncthreads();
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += stride) {
+    if (!selected(sel, i)) continue;
+    const uint64_t h = xxh3_u64(cell_at<DT>(col, row_begin + i));
+    const uint64_t w = h << kHllBits;
+    const uint32_t rank = (w ? (uint32_t)__clzll((long long)w) : 64u) + 1u;
+    atomicMax(&reg[h >> (64 - kHllBits)], rank);
+  }
+  __syncthreads();
+  uint8_t* out = partial + (uint64_t)blockIdx.x * kHllRegs;
+  for (uint32_t i = threadIdx.x; i < kHllRegs; i += 256) out[i] = (uint8_t)reg[i];
+}
+
+// regs[i] = max over the `nparts` partial register sets; one thread per 4 registers.
+__global__ void __launch_bounds__(256) k_hll_merge(const uint32_t* __restrict__ partial,
+                                                   uint32_t nparts, uint32_t* __restrict__ regs) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= kHllRegs / 4) return;
+  uint32_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+  for (uint32_t p = 0; p < nparts; ++p) {
+    const uint32_t x = partial[(uint64_t)p * (kHllRegs / 4) + i];
+    m0 = max(m0, x & 0xffu);
+    m1 = max(m1, (x >> 8) & 0xffu);
+    m2 = max(m2, (x >> 16) & 0xffu);
+    m3 = max(m3, x >> 24);
+  }
+  regs[i] = m0 | (m1 << 8) | (m2 << 16) | (m3 << 24);
+}
+
+__device__ __forceinline__ uint64_t set_slot(uint64_t v) {  // SplitMix64 finaliser
+  v ^= v >> 30;
+  v *= 0xbf58476d1ce4e5b9ull;
+  v ^= v >> 27;
+  v *= 0x94d049bb133111ebull;
+  return v ^ (v >> 31);
+}
+
+// counters[0]: values inserted (distinct values other than the marker);
+// counters[1]: the marker value occurred.
+template <int DT>
+__global__ void __launch_bounds__(256) k_distinct_insert(const uint8_t* __restrict__ col,
+                                                         uint64_t row_begin, uint64_t n,
+                                                         const uint64_t* __restrict__ sel,
+                                                         unsigned long long* __restrict__ table,
+                                                         uint64_t mask,
+                                                         unsigned long long* counters) {
+  uint32_t inserted = 0, marker = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += stride) {
+    if (!selected(sel, i)) continue;
+    const unsigned long long v = cell_at<DT>(col, row_begin + i);
+    if (v == kSetEmpty) {
+      marker = 1;
+      continue;
+    }
+    // at most half the slots are taken, so an empty slot ends every probe run
+    for (uint64_t s = set_slot(v) & mask;; s = (s + 1) & mask) {
+      const unsigned long long old = atomicCAS(&table[s], kSetEmpty, v);
+      if (old == kSetEmpty) {
+        ++inserted;
+        break;
+      }
+      if (old == v) break;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    inserted += __shfl_xor(inserted, off);
+    marker |= __shfl_xor(marker, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (inserted) atomicAdd(&counters[0], (unsigned long long)inserted);
+    if (marker) atomicOr(&counters[1], 1ull);
+  }
+}
+
+typedef void (*HllKern)(const uint8_t*, uint64_t, uint64_t, const uint64_t*, uint8_t*);
+typedef void (*InsKern)(const uint8_t*, uint64_t, uint64_t, const uint64_t*, unsigned long long*,
+                        uint64_t, unsigned long long*);
+
+static HllKern hll_kernel(int32_t dt) {
+  switch (dt) {
+    case TGP_I8: return k_hll<TGP_I8>;
+    case TGP_U8: return k_hll<TGP_U8>;
+    case TGP_I16: return k_hll<TGP_I16>;
+    case TGP_U16: return k_hll<TGP_U16>;
+    case TGP_I32: return k_hll<TGP_I32>;
+    case TGP_U32: return k_hll<TGP_U32>;
+    case TGP_I64: return k_hll<TGP_I64>;
+    default: return k_hll<TGP_U64>;
+  }
+}
+
+static InsKern insert_kernel(int32_t dt) {
+  switch (dt) {
+    case TGP_I8: return k_distinct_insert<TGP_I8>;
+    case TGP_U8: return k_distinct_insert<TGP_U8>;
+    case TGP_I16: return k_distinct_insert<TGP_I16>;
+    case TGP_U16: return k_distinct_insert<TGP_U16>;
+    case TGP_I32: return k_distinct_insert<TGP_I32>;
+    case TGP_U32: return k_distinct_insert<TGP_U32>;
+    case TGP_I64: return k_distinct_insert<TGP_I64>;
+    default: return k_distinct_insert<TGP_U64>;
+  }
+}
+
+static int32_t check_range(const tgp_column* col, uint64_t row_begin, uint64_t row_end,
+                           const char* what) {
+  TGP_TRY(check_column(col, what));
+  TGP_REQUIRE(row_begin <= row_end && row_end <= col->len,
+              "%s: rows [%llu, %llu) outside the column's %llu rows", what,
+              (unsigned long long)row_begin, (unsigned long long)row_end,
+              (unsigned long long)col->len);
+  return TGP_OK;
+}
+
+// Registers into h_regs (16384 bytes).
+static int32_t hll_sketch(tgp_ctx* ctx, const tgp_column* col, uint64_t row_begin,
+                          uint64_t row_end, const uint64_t* d_sel, uint8_t* h_regs,
+                          hipStream_t st) {
+  const uint64_t n = row_end - row_begin;
+  uint64_t grid = (n + 4095) / 4096;
+  const uint64_t gmax = (uint64_t)persistent_grid(ctx, 2);  // 64 KiB of LDS per workgroup
+  if (grid > gmax) grid = gmax;
+  if (grid == 0) grid = 1;
+  void* p = nullptr;
+  TGP_TRY(get_scratch(ctx, st, (grid + 1) * kHllRegs, &p));
+  uint8_t* partial = static_cast<uint8_t*>(p);
+  uint8_t* regs = partial + grid * kHllRegs;
+  hipLaunchKernelGGL(hll_kernel(col->dtype), dim3((uint32_t)grid), dim3(256), 0, st,
+                     static_cast<const uint8_t*>(col->data), row_begin, n, d_sel, partial);
+  TGP_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_hll_merge, dim3(kHllRegs / 4 / 256), dim3(256), 0, st,
+                     reinterpret_cast<const uint32_t*>(partial), (uint32_t)grid,
+                     reinterpret_cast<uint32_t*>(regs));
+  TGP_HIP(hipGetLastError());
+  TGP_HIP(hipMemcpyAsync(h_regs, regs, kHllRegs, hipMemcpyDeviceToHost, st));
+  TGP_HIP(hipStreamSynchronize(st));
+  return TGP_OK;
+}
+
+static int32_t exact_distinct(tgp_ctx* ctx, const tgp_column* col, uint64_t row_begin,
+                              uint64_t row_end, const uint64_t* d_sel, uint64_t* out,
+                              hipStream_t st) {
+  const uint64_t n = row_end - row_begin;
+  uint64_t cap = 1024;
+  while (cap < 2 * n) cap <<= 1;
+  void* p = nullptr;
+  TGP_TRY(get_scratch(ctx, st, (cap + 2) * 8, &p));
+  auto* counters = static_cast<unsigned long long*>(p);
+  auto* table = counters + 2;
+  TGP_TRY(fill_u64(st, reinterpret_cast<uint64_t*>(counters), 2, 0));
+  TGP_TRY(fill_u64(st, reinterpret_cast<uint64_t*>(table), cap, kSetEmpty));
+  uint64_t grid = (n + 1023) / 1024;
+  const uint64_t gmax = (uint64_t)persistent_grid(ctx, 8);
+  if (grid > gmax) grid = gmax;
+  if (grid == 0) grid = 1;
+  hipLaunchKernelGGL(insert_kernel(col->dtype), dim3((uint32_t)grid), dim3(256), 0, st,
+                     static_cast<const uint8_t*>(col->data), row_begin, n, d_sel, table, cap - 1,
+                     counters);
+  TGP_HIP(hipGetLastError());
+  uint64_t h[2] = {0, 0};
+  TGP_HIP(hipMemcpyAsync(h, counters, 16, hipMemcpyDeviceToHost, st));
+  TGP_HIP(hipStreamSynchronize(st));
+  *out = h[0] + (h[1] ? 1 : 0);
+  return TGP_OK;
+}
+
+}  // namespace tgp
+
+using namespace tgp;
+
+extern "C" {
+
+uint64_t tgp_hll_estimate(const uint8_t* regs) {
+  // HyperLogLog::estimate (exec/hll.rs:53-75), summed in register order
+  const double m = (double)kHllRegs;
+  const double alpha = 0.7213 / (1.0 + 1.079 / m);
+  double sum = 0.0;
+  uint64_t zeros = 0;
+  for (uint32_t i = 0; i < kHllRegs; ++i) {
+    sum += std::pow(2.0, -(double)regs[i]);
+    zeros += regs[i] == 0;
+  }
+  const double raw = alpha * m * m / sum;
+  const double est = (raw <= 2.5 * m && zeros > 0) ? m * std::log(m / (double)zeros) : raw;
+  return (uint64_t)est;
+}
+
+int32_t tgp_hll_sketch(tgp_ctx* ctx, const tgp_column* col, uint64_t row_begin, uint64_t row_end,
+                       const uint64_t* d_sel_bitmap, uint8_t* h_registers, void* stream) {
+  TGP_REQUIRE(ctx && h_registers, "tgp_hll_sketch: null argument");
+  TGP_TRY(check_injected(ctx));
+  TGP_TRY(check_range(col, row_begin, row_end, "tgp_hll_sketch"));
+  return hll_sketch(ctx, col, row_begin, row_end, d_sel_bitmap, h_registers,
+                    pick_stream(ctx, stream));
+}
+
+int32_t tgp_count_distinct(tgp_ctx* ctx, const tgp_column* col, uint64_t row_begin,
+                           uint64_t row_end, const uint64_t* d_sel_bitmap, int32_t mode,
+                           uint64_t* h_count, void* stream) {
+  TGP_REQUIRE(ctx && h_count, "tgp_count_distinct: null argument");
+  TGP_REQUIRE(mode >= TGP_DISTINCT_AUTO && mode <= TGP_DISTINCT_HLL,
+              "tgp_count_distinct: mode %d is not AUTO, EXACT or HLL", mode);
+  TGP_TRY(check_injected(ctx));
+  TGP_TRY(check_range(col, row_begin, row_end, "tgp_count_distinct"));
+  hipStream_t st = pick_stream(ctx, stream);
+  bool hll = mode == TGP_DISTINCT_HLL;
+  if (mode == TGP_DISTINCT_AUTO) {
+    uint64_t nsel = row_end - row_begin;
+    if (d_sel_bitmap && nsel >= kDistinctExactBelow) {
+      void* p = nullptr;
+      TGP_TRY(get_scratch(ctx, st, 8, &p));
+      TGP_TRY(fill_u64(st, static_cast<uint64_t*>(p), 1, 0));
+      TGP_TRY(tgp_bitmap_count_range(ctx, d_sel_bitmap, nsel, 0, nsel,
+                                     static_cast<uint64_t*>(p), st));
+      TGP_HIP(hipMemcpyAsync(&nsel, p, 8, hipMemcpyDeviceToHost, st));
+      TGP_HIP(hipStreamSynchronize(st));
+    }
+    hll = nsel >= kDistinctExactBelow;
+  }
+  if (!hll) return exact_distinct(ctx, col, row_begin, row_end, d_sel_bitmap, h_count, st);
+  uint8_t regs[kHllRegs];
+  TGP_TRY(hll_sketch(ctx, col, row_begin, row_end, d_sel_bitmap, regs, st));
+  *h_count = tgp_hll_estimate(regs);
+  return TGP_OK;
+}
+
+}  // extern "C"

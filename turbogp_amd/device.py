@@ -219,6 +219,29 @@ class Context:
                 cnt.free()
         return bm, c
 
+    # ---- COUNT(DISTINCT col) ------------------------------------------------------------
+    def count_distinct(self, col, row_begin=0, row_end=None, sel_bitmap=None,
+                       mode=A.DISTINCT_AUTO, stream=None):
+        """COUNT(DISTINCT col) of rows [row_begin, row_end) selected by the optional
+        DeviceBitmap (bit i <-> row_begin + i); mode AUTO / EXACT / HLL."""
+        row_end = col.n if row_end is None else row_end
+        d = col.desc()
+        out = C.c_uint64()
+        A.check(A.lib().tgp_count_distinct(self.handle, C.byref(d), row_begin, row_end,
+                                           sel_bitmap.buf.ptr if sel_bitmap is not None else None,
+                                           mode, C.byref(out), stream), "tgp_count_distinct")
+        return out.value
+
+    def hll_registers(self, col, row_begin=0, row_end=None, sel_bitmap=None, stream=None):
+        """The 16384 HyperLogLog registers (uint8) of the same selection."""
+        row_end = col.n if row_end is None else row_end
+        d = col.desc()
+        regs = np.zeros(1 << 14, np.uint8)
+        A.check(A.lib().tgp_hll_sketch(self.handle, C.byref(d), row_begin, row_end,
+                                       sel_bitmap.buf.ptr if sel_bitmap is not None else None,
+                                       regs.ctypes.data, stream), "tgp_hll_sketch")
+        return regs
+
     # ---- aggregation --------------------------------------------------------------------
     def group_agg(self, q, stream=None, result=None):
         """Runs an AggQuery; returns GroupAggResult (downloaded, exact Python ints)."""
