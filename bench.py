@@ -173,6 +173,40 @@ class DeviceCounter:
         self.buf.free()
 
 
+def bench_distinct(ctx, table):
+    """COUNT(DISTINCT price) and COUNT(DISTINCT qty) over the table's first 60M
+    rows: HyperLogLog (the reference's path for >= 100K selected rows) and the
+    exact device set; and the 99 999-row exact case of the AUTO switch."""
+    from turbogp_amd import Event, _capi as A
+    ev = [Event(ctx) for _ in range(2)]
+
+    def timed(fn, reps):
+        ts, out = [], None
+        for _ in range(reps):
+            ev[0].record()
+            out = fn()
+            ev[1].record()
+            ts.append(ev[0].elapsed_ms(ev[1]))
+        return statistics.median(ts), out
+
+    res = {}
+    n = SCAN_ROWS
+    for name in ("price", "qty"):
+        col = table[name]
+        ms_h, est = timed(lambda: ctx.count_distinct(col, 0, n, mode=A.DISTINCT_HLL), 10)
+        ms_e, exact = timed(lambda: ctx.count_distinct(col, 0, n, mode=A.DISTINCT_EXACT), 3)
+        gbs = n * 8 / 1e9 / (ms_h / 1e3)
+        res[name] = {"rows": n, "hll": {"ms": round(ms_h, 4), "GB_s": round(gbs, 1),
+                                        "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "estimate": est},
+                     "exact": {"ms": round(ms_e, 4), "count": exact},
+                     "hll_relative_error": round(abs(est - exact) / max(1, exact), 5)}
+    ms_s, small = timed(lambda: ctx.count_distinct(table["price"], 0, 99_999), 10)
+    res["price_99999_rows_auto_exact"] = {"ms": round(ms_s, 4), "count": small}
+    for e in ev:
+        e.free()
+    return res
+
+
 def bench_serve(ctx, table, join_tables, args):
     """Serving: an open-loop Poisson stream of the mixed requests (70 % Q1/Q6
     scan-aggregates on log-uniform ranges, 25 % join-aggregates over build
@@ -288,7 +322,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=2000)
-    ap.add_argument("--sections", default="batch,scan,join,serve",
+    ap.add_argument("--sections", default="batch,scan,join,serve,distinct",
                     help="which parts to run (profiling runs restrict them so a kernel's "
                          "launches all come from one workload)")
     ap.add_argument("--tuning", default="",
@@ -399,6 +433,7 @@ def main():
     join = bench_join(ctx, args) if "join" in sections or "serve" in sections else None
     join_tables = join.pop("_tables") if join else None
     serve = bench_serve(ctx, table, join_tables, args) if "serve" in sections else None
+    distinct = bench_distinct(ctx, table) if "distinct" in sections else None
     if "join" not in sections:
         join = None
 
@@ -447,6 +482,7 @@ def main():
                             "bytes_per_row": W.Q6_BYTES_PER_ROW,
                             "selected": int(res_b.count[0])}},
         "join_2M_x_60M": join,
+        "count_distinct_60M": distinct,
         "serving": serve,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

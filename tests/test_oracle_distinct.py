@@ -75,3 +75,14 @@ def test_auto_switches_at_100k_selected():
     regs = B.hll_registers(col[:100_000])
     assert B.count_distinct(col[:100_000]) == B.hll_estimate(regs)
     assert B.count_distinct(col[:100_000], mode=1) == 100_000
+
+
+def test_library_estimate_matches_the_oracle():
+    # tgp_hll_estimate is host code: its table of 2^-r must give the oracle's pow() sums
+    from turbogp_amd import _capi as A
+    rng = np.random.default_rng(9)
+    for k in range(100):
+        r = rng.integers(0, int(rng.integers(1, 66)), 1 << 14).astype(np.uint8)
+        if k % 3 == 0:
+            r[rng.random(1 << 14) < 0.3] = 0
+        assert A.lib().tgp_hll_estimate(r.ctypes.data) == B.hll_estimate(r)
