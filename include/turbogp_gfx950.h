@@ -357,6 +357,23 @@ int32_t tgp_hll_sketch(tgp_ctx* ctx, const tgp_column* col, uint64_t row_begin, 
 /* HyperLogLog::estimate (exec/hll.rs:53-75) of 16384 host registers. */
 uint64_t tgp_hll_estimate(const uint8_t* h_registers);
 
+/* ---- SELECT rows: compaction, ORDER BY, LIMIT, gather -------------------------- */
+/* The rows a SELECT returns (engine/dispatch.rs:486-733): the rows of
+ * [row_begin, row_end) selected by the bitmap (bit i <-> row_begin + i; NULL =
+ * all), in scan order, or stably sorted by the u64 cells of order_col
+ * (ascending, or descending when `descending`; equal cells keep scan order),
+ * the first `limit` of them, as absolute row indices into d_rows (capacity
+ * entries).  *h_count = the number written.  Synchronous. */
+int32_t tgp_select_rows(tgp_ctx* ctx, uint64_t row_begin, uint64_t row_end,
+                        const uint64_t* d_sel_bitmap, const tgp_column* order_col,
+                        int32_t descending, uint64_t limit, uint64_t* d_rows, uint64_t capacity,
+                        uint64_t* h_count, void* stream);
+/* d_out[i] = col[d_rows[i]] for i < n: as u64 cells (ResultColumn values,
+ * signed types sign-extended) when as_cells, else at the column's width.  A row
+ * outside the column is an error (TGP_ERR_INVALID).  Synchronous. */
+int32_t tgp_gather(tgp_ctx* ctx, const tgp_column* col, const uint64_t* d_rows, uint64_t n,
+                   int32_t as_cells, void* d_out, void* stream);
+
 /* ---- hash join --------------------------------------------------------------- */
 typedef struct tgp_hash_table tgp_hash_table;
 

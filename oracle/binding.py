@@ -267,3 +267,28 @@ def count_distinct(col, sel_words=None, mode=0):
     sel = None if sel_words is None else np.ascontiguousarray(sel_words, np.uint64)
     return int(lib().orc_count_distinct(ptr(col), _FROM_NP[col.dtype], len(col),
                                         ptr(sel) if sel is not None else None, mode))
+
+
+def cells(col):
+    """u64 cells of a column as the reference stores them (signed types sign-extended)."""
+    col = np.asarray(col)
+    return col.astype(np.int64).view(np.uint64) if col.dtype.kind == "i" else col.astype(np.uint64)
+
+
+def select_rows(row_begin, row_end, sel_words=None, order_col=None, descending=False,
+                limit=None):
+    """dispatch.rs SelectStar/SelectColumn/SelectMulti (:486-733): selected rows of
+    [row_begin, row_end) in scan order, stably sorted by the order column's u64 cells
+    (`va.cmp(&vb)`, DESC `vb.cmp(&va)`), first `limit`."""
+    n = row_end - row_begin
+    if sel_words is None:
+        rows = np.arange(row_begin, row_end, dtype=np.uint64)
+    else:
+        bits = np.unpackbits(np.ascontiguousarray(sel_words, np.uint64).view(np.uint8),
+                             bitorder="little")[:n].astype(bool)
+        rows = (np.flatnonzero(bits) + row_begin).astype(np.uint64)
+    if order_col is not None and len(rows):
+        keys = cells(order_col)[rows.astype(np.int64)]
+        perm = np.argsort(~keys if descending else keys, kind="stable")
+        rows = rows[perm]
+    return rows if limit is None else rows[:limit]

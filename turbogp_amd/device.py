@@ -242,6 +242,41 @@ class Context:
                                        regs.ctypes.data, stream), "tgp_hll_sketch")
         return regs
 
+    # ---- SELECT rows ---------------------------------------------------------------------
+    def select_rows(self, row_begin, row_end, sel_bitmap=None, order_col=None, descending=False,
+                    limit=None, stream=None):
+        """Row indices of SELECT ... [ORDER BY order_col [DESC]] [LIMIT limit] over the rows
+        [row_begin, row_end) selected by the optional DeviceBitmap -> (DeviceBuffer of u64
+        rows, count)."""
+        n = row_end - row_begin
+        limit = n if limit is None else min(limit, n)
+        out = DeviceBuffer(self, max(8, limit * 8))
+        cnt = C.c_uint64()
+        od = order_col.desc() if order_col is not None else None
+        try:
+            A.check(A.lib().tgp_select_rows(self.handle, row_begin, row_end,
+                                            sel_bitmap.buf.ptr if sel_bitmap is not None else None,
+                                            C.byref(od) if od is not None else None,
+                                            int(bool(descending)), limit, out.ptr, limit,
+                                            C.byref(cnt), stream), "tgp_select_rows")
+        except Exception:
+            out.free()
+            raise
+        return out, cnt.value
+
+    def gather(self, col, rows, n, as_cells=True, stream=None):
+        """col at the n row indices in the DeviceBuffer `rows` -> numpy (u64 cells, or the
+        column's dtype)."""
+        dtype = np.uint64 if as_cells else _NP_DTYPE[col.dtype]
+        out = DeviceBuffer(self, max(8, n * np.dtype(dtype).itemsize))
+        try:
+            d = col.desc()
+            A.check(A.lib().tgp_gather(self.handle, C.byref(d), rows.ptr, n, int(as_cells),
+                                       out.ptr, stream), "tgp_gather")
+            return out.download(dtype, n)
+        finally:
+            out.free()
+
     # ---- aggregation --------------------------------------------------------------------
     def group_agg(self, q, stream=None, result=None):
         """Runs an AggQuery; returns GroupAggResult (downloaded, exact Python ints)."""
