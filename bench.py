@@ -207,6 +207,52 @@ def bench_distinct(ctx, table):
     return res
 
 
+def bench_select(ctx, table):
+    """SELECT price, qty FROM t WHERE <the Q6 predicates> ORDER BY price DESC LIMIT 100 on
+    the first 60M rows: scan -> bitmap, compaction + stable radix sort of (cell, row),
+    gather of two columns; the same without ORDER BY; ORDER BY over all 60M rows."""
+    import time
+    from turbogp_amd import workloads as W
+    names, preds, _, _ = W.q6_spec()
+    cols = [table[c] for c in names]
+    n = SCAN_ROWS
+
+    def wall(fn, reps):
+        ts, out = [], None
+        for _ in range(reps):
+            ctx.sync()
+            t0 = time.perf_counter()
+            out = fn()
+            ctx.sync()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        return statistics.median(ts), out
+
+    res = {}
+    bm, sel = ctx.scan_filter(cols, preds, 0, n)
+    try:
+        for label, order, limit in (("order_by_price_desc_limit_100", table["price"], 100),
+                                    ("scan_order_limit_1000", None, 1000)):
+            def q():
+                rows, cnt = ctx.select_rows(0, n, bm, order, True, limit)
+                try:
+                    return [ctx.gather(table[c], rows, cnt) for c in ("price", "qty")]
+                finally:
+                    rows.free()
+            ms, vals = wall(q, 5)
+            res[label] = {"ms": round(ms, 3), "selected_rows": sel, "returned": len(vals[0])}
+    finally:
+        bm.free()
+
+    def full():
+        rows, cnt = ctx.select_rows(0, n, None, table["price"], False, 10)
+        rows.free()
+        return cnt
+    ms, _ = wall(full, 3)
+    res["order_by_price_all_60M_limit_10"] = {"ms": round(ms, 3), "rows_sorted": n}
+    res["note"] = "host wall time per synchronous call sequence, including the scan only once"
+    return res
+
+
 def bench_serve(ctx, table, join_tables, args):
     """Serving: an open-loop Poisson stream of the mixed requests (70 % Q1/Q6
     scan-aggregates on log-uniform ranges, 25 % join-aggregates over build
@@ -322,7 +368,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=2000)
-    ap.add_argument("--sections", default="batch,scan,join,serve,distinct",
+    ap.add_argument("--sections", default="batch,scan,join,serve,distinct,select",
                     help="which parts to run (profiling runs restrict them so a kernel's "
                          "launches all come from one workload)")
     ap.add_argument("--tuning", default="",
@@ -434,6 +480,7 @@ def main():
     join_tables = join.pop("_tables") if join else None
     serve = bench_serve(ctx, table, join_tables, args) if "serve" in sections else None
     distinct = bench_distinct(ctx, table) if "distinct" in sections else None
+    select = bench_select(ctx, table) if "select" in sections else None
     if "join" not in sections:
         join = None
 
@@ -483,6 +530,7 @@ def main():
                             "selected": int(res_b.count[0])}},
         "join_2M_x_60M": join,
         "count_distinct_60M": distinct,
+        "select_60M": select,
         "serving": serve,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
