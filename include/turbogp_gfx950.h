@@ -374,6 +374,21 @@ int32_t tgp_select_rows(tgp_ctx* ctx, uint64_t row_begin, uint64_t row_end,
 int32_t tgp_gather(tgp_ctx* ctx, const tgp_column* col, const uint64_t* d_rows, uint64_t n,
                    int32_t as_cells, void* d_out, void* stream);
 
+/* ---- result encoding of exact int128 values (host) ------------------------------ */
+/* ResultColumn cells are u64 (engine/result.rs:37-53) and pgwire prints a cell as
+ * the decimal text of the u64 (server/async_pgwire.rs:881-910): an int128 sum
+ * travels instead as NUMERIC text (type OID 1700) in ResultColumn.string_values.
+ * tgp_int128_to_numeric writes "[-]digits[.digits]" of (lo + 2^64 hi) * 10^-scale;
+ * tgp_int128_avg_to_numeric writes the same value divided by `count`, with
+ * frac_digits (>= scale) fraction digits, rounded half away from zero.  buf
+ * receives the text and a terminating NUL; *out_len its length (also when buf
+ * is too small, which returns TGP_ERR_INVALID). */
+int32_t tgp_int128_to_numeric(uint64_t lo, uint64_t hi, int32_t scale, char* buf,
+                              uint64_t buf_size, uint64_t* out_len);
+int32_t tgp_int128_avg_to_numeric(uint64_t lo, uint64_t hi, uint64_t count, int32_t scale,
+                                  int32_t frac_digits, char* buf, uint64_t buf_size,
+                                  uint64_t* out_len);
+
 /* ---- hash join --------------------------------------------------------------- */
 typedef struct tgp_hash_table tgp_hash_table;
 

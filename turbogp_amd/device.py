@@ -23,6 +23,15 @@ def dtype_of(arr):
     return _FROM_NP[np.asarray(arr).dtype]
 
 
+def numeric_text(lo, hi, scale=0):
+    """NUMERIC text of the int128 (lo, hi) scaled by 10^-scale (tgp_int128_to_numeric)."""
+    buf = C.create_string_buffer(64)
+    ln = C.c_uint64()
+    A.check(A.lib().tgp_int128_to_numeric(int(lo), int(hi), scale, buf, 64, C.byref(ln)),
+            "tgp_int128_to_numeric")
+    return buf.value.decode()
+
+
 def int128(lo, hi):
     """Python int from the (low, high) 64-bit words of a two's complement int128."""
     v = (int(hi) << 64) | int(lo)
@@ -380,6 +389,31 @@ class GroupAggResult:
                 out.append((s % (1 << 64)) // n)
             else:
                 raise ValueError(f"avg form {form!r}: 'exact', 'f64' or 'u64'")
+        return out
+
+    def numeric(self, a, scale=0):
+        """Aggregate `a` per group as NUMERIC text (pgwire type 1700) of the exact int128
+        sum scaled by 10^-scale (tgp_int128_to_numeric)."""
+        return [numeric_text(self.sums_words[g, a, 0], self.sums_words[g, a, 1], scale)
+                for g in range(len(self.count))]
+
+    def avg_numeric(self, a, scale=0, frac_digits=None):
+        """AVG of aggregate `a` per group as NUMERIC text, rounded half away from zero to
+        frac_digits fraction digits (default scale + 4); None for a group without rows."""
+        frac_digits = scale + 4 if frac_digits is None else frac_digits
+        out = []
+        for g in range(len(self.count)):
+            n = int(self.count[g])
+            if n == 0:
+                out.append(None)
+                continue
+            buf = C.create_string_buffer(96)
+            ln = C.c_uint64()
+            A.check(A.lib().tgp_int128_avg_to_numeric(int(self.sums_words[g, a, 0]),
+                                                      int(self.sums_words[g, a, 1]), n, scale,
+                                                      frac_digits, buf, 96, C.byref(ln)),
+                    "tgp_int128_avg_to_numeric")
+            out.append(buf.value.decode())
         return out
 
     def rows(self):
