@@ -1,0 +1,34 @@
+"""QueryShape classification (engine/dispatch.rs:225-299), host only: the
+reference's classify_* tests (dispatch.rs:1740-1772) and the other shapes."""
+from turbogp_amd.dispatch import Aggregate, Column, QueryShape, SelectQuery, Star, classify_query
+
+
+def test_reference_classify_tests():
+    assert classify_query(SelectQuery([Aggregate("count", "*")])) == QueryShape.CountAll
+    assert classify_query(SelectQuery([Aggregate("count", "*")], where=[("id", "=", 5)])) == \
+        QueryShape.CountFilter
+    assert classify_query(SelectQuery([Aggregate("sum", "val")])) == QueryShape.SumCol
+    assert classify_query(SelectQuery([Column("grp"), Aggregate("count", "*")],
+                                      group_by=["grp"])) == QueryShape.GroupByCount
+
+
+def test_other_shapes():
+    agg = lambda f, a="val": SelectQuery([Aggregate(f, a)])
+    assert classify_query(agg("count")) == QueryShape.CountFilter  # COUNT(col), no WHERE
+    assert classify_query(agg("avg")) == QueryShape.AvgCol
+    assert classify_query(agg("min")) == QueryShape.MinMax
+    assert classify_query(agg("max")) == QueryShape.MinMax
+    assert classify_query(agg("count_distinct")) == QueryShape.CountDistinct
+    assert classify_query(agg("stddev")) == QueryShape.Complex
+    g = [Column("grp"), Aggregate("count", "*")]
+    assert classify_query(SelectQuery(g, group_by=["grp"], order_by=[("grp", True)], limit=3)) \
+        == QueryShape.GroupByOrderByLimit
+    assert classify_query(SelectQuery(g, group_by=["grp"], order_by=[("grp", True)])) == \
+        QueryShape.GroupByCount
+    assert classify_query(SelectQuery([Column("grp"), Aggregate("sum", "val")],
+                                      group_by=["grp"])) == QueryShape.GroupBySum
+    assert classify_query(SelectQuery([Column("grp")], group_by=["grp"])) == QueryShape.Complex
+    assert classify_query(SelectQuery([Star()])) == QueryShape.SelectStar
+    assert classify_query(SelectQuery([Column("a")])) == QueryShape.SelectColumn
+    assert classify_query(SelectQuery([Column("a"), Column("b")])) == QueryShape.SelectMulti
+    assert classify_query(SelectQuery([Column("a"), Aggregate("sum", "b")])) == QueryShape.Complex

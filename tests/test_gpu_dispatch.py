@@ -1,0 +1,114 @@
+"""Dispatch of SELECT shapes onto the device operators: the reference's
+dispatch tests (engine/dispatch.rs:1776-1860) and its engine tests for these
+shapes, then parity with the oracle on a 1M-row fact table."""
+import numpy as np
+import pytest
+
+from oracle import binding as B
+from turbogp_amd import DeviceColumn, _capi as A
+from turbogp_amd import workloads as W
+from turbogp_amd.dispatch import (Aggregate, Column, QueryShape, SelectQuery, Star, Table,
+                                  classify_query, execute_dispatched)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture()
+def make(ctx):
+    made = []
+
+    def build(**cols):
+        dev = {k: DeviceColumn.from_numpy(ctx, np.ascontiguousarray(v)) for k, v in cols.items()}
+        made.extend(dev.values())
+        return Table(ctx, dev)
+    yield build
+    for c in made:
+        c.free()
+
+
+def ref_table(make, n=100):  # dispatch.rs make_table: id = i, val = i % 20, grp = i % 5
+    i = np.arange(n, dtype=np.uint64)
+    return make(id=i, val=i % 20, grp=i % 5)
+
+
+def test_reference_dispatch_tests(make):
+    t = ref_table(make)
+    run = lambda q: execute_dispatched(q, t)
+    assert run(SelectQuery([Aggregate("count", "*")])).scalar() == 100
+    assert run(SelectQuery([Aggregate("count", "*")], where=[("val", "=", 5)])).scalar() == 5
+    assert run(SelectQuery([Aggregate("sum", "val")], where=[("val", ">", 15)])).scalar() == 350
+    assert run(SelectQuery([Column("grp"), Aggregate("count", "*")], group_by=["grp"])) \
+        .row_count == 5
+    r = run(SelectQuery([Column("grp"), Aggregate("count", "*")], group_by=["grp"],
+                        order_by=[("grp", True)], limit=3))
+    assert r.row_count == 3 and r.columns[0].values == [0, 1, 2] and r.columns[1].values == [20] * 3
+    assert run(SelectQuery([Aggregate("max", "val")])).scalar() == 19
+    assert run(SelectQuery([Aggregate("count_distinct", "val")])).scalar() == 20
+    assert run(SelectQuery([Column("grp")], group_by=["grp"])) is None  # Complex: not dispatched
+
+
+def test_reference_engine_tests(make):
+    # tests/dispatch_path_features.rs:14-35: SUM(price * 2) = 600, SUM(a * b) = 1400
+    t = make(price=np.array([100, 200], np.int64), discount=np.array([10, 20], np.int64))
+    assert execute_dispatched(SelectQuery([Aggregate("sum", (2, [("price", 0, 1)]))]), t) \
+        .scalar() == 600
+    t = make(a=np.array([10, 30], np.int64), b=np.array([20, 40], np.int64))
+    assert execute_dispatched(SelectQuery([Aggregate("sum", (1, [("a", 0, 1), ("b", 0, 1)]))]),
+                              t).scalar() == 1400
+    # tests/interpreter_fallback.rs sales table: avg 200, min 100, max 300, 3 regions
+    s = make(id=np.arange(1, 6, dtype=np.int64), region=np.array([1, 1, 2, 2, 3], np.int64),
+             amount=np.array([100, 200, 150, 300, 250], np.int64),
+             qty=np.array([2, 3, 1, 5, 4], np.int64))
+    assert execute_dispatched(SelectQuery([Aggregate("avg", "amount")]), s).scalar() == "200.0000"
+    assert execute_dispatched(SelectQuery([Aggregate("min", "amount")]), s).scalar() == 100
+    assert execute_dispatched(SelectQuery([Aggregate("max", "amount")]), s).scalar() == 300
+    assert execute_dispatched(SelectQuery([Aggregate("count_distinct", "region")]), s).scalar() == 3
+    # tests/joins_groupby_orderby.rs:82-106: grp 1 -> sum 60, count 3; grp 2 -> 90, 2
+    g = make(grp=np.array([1, 1, 1, 2, 2], np.int64), a=np.array([10, 20, 30, 40, 50], np.int64))
+    r = execute_dispatched(SelectQuery([Column("grp"), Aggregate("sum", "a"),
+                                        Aggregate("count", "*")], group_by=["grp"],
+                                       order_by=[("grp", True)]), g)
+    assert [c.values for c in r.columns] == [[1, 2], [60, 90], [3, 2]]
+    # tests/joins_groupby_orderby.rs:112-155: ORDER BY, DESC, LIMIT
+    m = make(a=np.array([5, 1, 3, 2, 4], np.int64), b=np.array([50, 10, 30, 20, 40], np.int64))
+    r = execute_dispatched(SelectQuery([Column("a"), Column("b")], order_by=[("a", True)],
+                                       limit=2), m)
+    assert r.columns[0].values == [1, 2] and r.columns[1].values == [10, 20]
+    r = execute_dispatched(SelectQuery([Column("a"), Column("b")], order_by=[("a", False)]), m)
+    assert r.columns[0].values == [5, 4, 3, 2, 1]
+    r = execute_dispatched(SelectQuery([Star()], where=[("a", ">", 2)], limit=2), m)
+    assert [c.values for c in r.columns] == [[5, 3], [50, 30]]
+    # an aggregate over a column the table lacks is an error, not column 0
+    with pytest.raises(KeyError):
+        execute_dispatched(SelectQuery([Aggregate("sum", "nope")]), m)
+
+
+def test_fact_table_parity_with_the_oracle(ctx, make):
+    n = 1_000_000
+    host = W.host_fact_columns(n, B.synth_uniform)
+    t = make(**host)
+    where = [("dt", ">=", 731), ("dt", "<", 1096), ("disc", ">=", 5), ("disc", "<=", 7),
+             ("qty", "<", 24)]
+    sel = ((host["dt"] >= 731) & (host["dt"] < 1096) & (host["disc"] >= 5) & (host["disc"] <= 7)
+           & (host["qty"] < 24))
+    run = lambda q: execute_dispatched(q, t)
+    assert run(SelectQuery([Aggregate("count", "*")], where=where)).scalar() == int(sel.sum())
+    rev = (1, [("price", 0, 1), ("disc", 0, 1)])
+    want = int((host["price"][sel].astype(object) * host["disc"][sel].astype(object)).sum())
+    assert run(SelectQuery([Aggregate("sum", rev)], where=where)).scalar() == want
+    assert run(SelectQuery([Aggregate("min", "price")], where=where)).scalar() == \
+        int(host["price"][sel].min())
+    assert run(SelectQuery([Aggregate("count_distinct", "price")], where=where)).scalar() == \
+        len(np.unique(host["price"][sel]))
+    r = run(SelectQuery([Column("flaga"), Aggregate("sum", "qty"), Aggregate("count", "*")],
+                        where=[("dt", "<=", 2435)], group_by=["flaga"]))
+    m = host["dt"] <= 2435
+    assert r.columns[0].values == [0, 1, 2]
+    assert r.columns[1].values == [int(host["qty"][m & (host["flaga"] == k)].sum()) for k in range(3)]
+    assert r.columns[2].values == [int((m & (host["flaga"] == k)).sum()) for k in range(3)]
+    r = run(SelectQuery([Column("price"), Column("qty")], where=where,
+                        order_by=[("price", False)], limit=50))
+    idx = np.flatnonzero(sel)
+    order = idx[np.argsort(~B.cells(host["price"])[idx], kind="stable")][:50]
+    assert r.columns[0].values == [int(v) for v in host["price"][order]]
+    assert r.columns[1].values == [int(v) for v in host["qty"][order]]

@@ -1,0 +1,330 @@
+"""Kernel dispatch of SELECT shapes onto the device operators.
+
+Mirrors engine/dispatch.rs: `QueryShape` (:196-222), `classify_query`
+(:225-299) and `execute_dispatched` / `execute_shape` (:304-735), for queries
+whose WHERE clause is an AND of column-vs-literal comparisons over resident
+device columns.  The SQL parser is not part of this path: a query arrives as
+the `SelectQuery` structure below, as the reference's parser would produce it.
+
+Results follow `QueryResult` / `ResultColumn` (engine/result.rs:37-131) with
+exact values instead of the dispatch path's encodings, as a caller of the
+device path needs them:
+  - SUM / AVG are exact: an integer when it fits INT8 (type_oid 20), else NUMERIC
+    text (type_oid 1700) in `string_values`; AVG is NUMERIC text rounded half
+    away from zero (the dispatch path returns `(u64 sum as f64).to_bits()` and a
+    u64 truncating average, vectorized.rs:514-522, :567-582);
+  - MIN / MAX compare u64 cells, as `min_masked` / `max_masked` do
+    (vectorized.rs:541-565);
+  - COUNT(DISTINCT) is exact, as `count_distinct_masked` (vectorized.rs:601-610);
+  - GROUP BY emits the groups ordered by key (the reference emits them in
+    first-seen order; rows compare equal after sorting by key);
+  - an aggregate argument that names no column is an error, not column 0
+    (dispatch.rs:1233 resolves it with `unwrap_or(0)`).
+"""
+from dataclasses import dataclass, field
+from enum import Enum
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _capi as A
+from .device import make_agg_query, numeric_text
+
+OPS = {"=": A.EQ, "!=": A.NE, "<>": A.NE, "<": A.LT, "<=": A.LE, ">": A.GT, ">=": A.GE}
+INT8_OID, NUMERIC_OID = 20, 1700
+
+
+class QueryShape(Enum):
+    CountAll = 0
+    CountFilter = 1
+    SumCol = 2
+    MinMax = 3
+    CountDistinct = 4
+    AvgCol = 5
+    GroupByCount = 6
+    GroupBySum = 7
+    GroupByOrderByLimit = 8
+    SelectStar = 9
+    SelectColumn = 10
+    SelectMulti = 11
+    Complex = 12
+
+
+@dataclass
+class Aggregate:
+    """SelectItem::Aggregate: func in COUNT, COUNT_DISTINCT, SUM, AVG, MIN, MAX; arg a
+    column name, "*" (COUNT), or a product (coef, [(column, fa, fb), ...]) meaning
+    coef * prod(fa + fb * column) for SUM / AVG."""
+    func: str
+    arg: object
+    alias: Optional[str] = None
+
+
+@dataclass
+class Column:
+    name: str
+
+
+@dataclass
+class Star:
+    pass
+
+
+@dataclass
+class SelectQuery:
+    select: List[object]
+    where: List[Tuple[str, str, int]] = field(default_factory=list)  # (column, op, literal)
+    group_by: List[str] = field(default_factory=list)
+    order_by: List[Tuple[str, bool]] = field(default_factory=list)  # (column, ascending)
+    limit: Optional[int] = None
+
+
+@dataclass
+class ResultColumn:
+    name: str
+    values: list
+    string_values: Optional[List[str]] = None
+    type_oid: int = 0
+
+
+@dataclass
+class QueryResult:
+    columns: List[ResultColumn]
+    row_count: int
+
+    def scalar(self):
+        c = self.columns[0]
+        return c.string_values[0] if c.string_values else c.values[0]
+
+
+class Table:
+    """Resident device columns by name (all of one length)."""
+
+    def __init__(self, ctx, columns: dict):
+        self.ctx = ctx
+        self.columns = dict(columns)
+        lens = {c.n for c in self.columns.values()}
+        assert len(lens) <= 1, "columns of different lengths"
+        self.row_count = lens.pop() if lens else 0
+        self.names = list(self.columns)
+
+    def col(self, name):
+        if name not in self.columns:
+            raise KeyError(f"unknown column {name!r}")
+        return self.columns[name]
+
+
+def classify_query(q: SelectQuery) -> QueryShape:
+    """dispatch.rs:225-299."""
+    has_where = bool(q.where)
+    if q.group_by:
+        aggs = [s for s in q.select if isinstance(s, Aggregate)]
+        if not aggs:
+            return QueryShape.Complex
+        func = aggs[0].func.upper()
+        if func == "COUNT":
+            return (QueryShape.GroupByOrderByLimit if q.order_by and q.limit is not None
+                    else QueryShape.GroupByCount)
+        if func == "SUM":
+            return QueryShape.GroupBySum
+        return QueryShape.Complex
+    if len(q.select) == 1:
+        s = q.select[0]
+        if isinstance(s, Aggregate):
+            f = s.func.upper()
+            if f == "COUNT":
+                return QueryShape.CountAll if s.arg == "*" and not has_where else \
+                    QueryShape.CountFilter
+            return {"COUNT_DISTINCT": QueryShape.CountDistinct, "SUM": QueryShape.SumCol,
+                    "AVG": QueryShape.AvgCol, "MIN": QueryShape.MinMax,
+                    "MAX": QueryShape.MinMax}.get(f, QueryShape.Complex)
+        if isinstance(s, Star):
+            return QueryShape.SelectStar
+        if isinstance(s, Column):
+            return QueryShape.SelectColumn
+        return QueryShape.Complex
+    if len(q.select) > 1:
+        if any(isinstance(s, Aggregate) for s in q.select):
+            return QueryShape.Complex
+        return QueryShape.SelectMulti
+    return QueryShape.Complex
+
+
+def execute_dispatched(q: SelectQuery, t: Table) -> Optional[QueryResult]:
+    """dispatch.rs:304-333: None for shapes the dispatcher does not run."""
+    shape = classify_query(q)
+    if shape == QueryShape.Complex:
+        return None
+    return _Executor(q, t).run(shape)
+
+
+def _exact_column(name, vals):
+    """INT8 when every value fits, else NUMERIC text."""
+    if all(-2**63 <= v < 2**63 for v in vals):
+        return ResultColumn(name, [v % 2**64 for v in vals], None, INT8_OID)
+    return ResultColumn(name, [0] * len(vals), [str(v) for v in vals], NUMERIC_OID)
+
+
+class _Executor:
+    def __init__(self, q, t):
+        self.q, self.t, self.ctx = q, t, t.ctx
+
+    # ---- WHERE: AND of comparisons -> selection bitmap over all rows --------------
+    def mask(self):
+        if not self.q.where:
+            return None, self.t.row_count
+        names = sorted({c for c, _, _ in self.q.where})
+        cols = [self.t.col(c) for c in names]
+        preds = []
+        for c, op, lit in self.q.where:
+            if op not in OPS:
+                raise ValueError(f"unsupported comparison {op!r}")
+            dt = self.t.col(c).dtype
+            preds.append((names.index(c), OPS[op], int(lit), dt in (A.I8, A.I16, A.I32, A.I64)))
+        return self.ctx.scan_filter(cols, preds, 0, self.t.row_count)
+
+    # ---- aggregates without GROUP BY -------------------------------------------------
+    def agg_query(self, items, parts=(), bitmap=None, names_extra=()):
+        names = []
+
+        def slot(c):
+            if c not in names:
+                names.append(c)
+            return names.index(c)
+
+        for c in names_extra:
+            slot(c)
+        aggs = []
+        for func, arg in items:
+            if func in ("SUM", "AVG"):
+                if isinstance(arg, str):
+                    aggs.append(("sum", 1, [(slot(arg), 0, 1)]))
+                else:
+                    coef, factors = arg
+                    aggs.append(("sum", coef, [(slot(c), fa, fb) for c, fa, fb in factors]))
+            elif func in ("MIN", "MAX"):
+                aggs.append((func.lower() + "_u64", slot(arg)))
+            else:
+                raise ValueError(func)
+        parts = [(slot(c), div, base, card) for c, div, base, card in parts]
+        cols = [self.t.col(c) for c in names] if names else [next(iter(self.t.columns.values()))]
+        return self.ctx.group_agg(make_agg_query(cols, (), parts, aggs, 0, self.t.row_count,
+                                                 bitmap))
+
+    def run(self, shape):
+        q, t = self.q, self.t
+        if shape == QueryShape.CountAll:
+            return QueryResult([ResultColumn("count", [t.row_count], None, INT8_OID)], 1)
+        bm, nsel = self.mask()
+        try:
+            return self._run(shape, bm, nsel)
+        finally:
+            if bm is not None:
+                bm.free()
+
+    def _run(self, shape, bm, nsel):
+        q, t, ctx = self.q, self.t, self.ctx
+        s0 = q.select[0]
+        if shape == QueryShape.CountFilter:
+            if s0.arg != "*":
+                t.col(s0.arg)  # COUNT(col) without NULLs counts every selected row
+            return QueryResult([ResultColumn("count", [nsel], None, INT8_OID)], 1)
+        if shape in (QueryShape.SumCol, QueryShape.AvgCol, QueryShape.MinMax):
+            func = s0.func.upper()
+            r = self.agg_query([(func, s0.arg)], bitmap=bm)
+            name = s0.alias or func.lower()
+            if func == "AVG":
+                v = r.avg_numeric(0) if nsel else [None]
+                return QueryResult([ResultColumn(name, [0], [v[0] or "0"], NUMERIC_OID)], 1)
+            if func == "SUM":
+                return QueryResult([_exact_column(name, [r.sums()[0][0]])], 1)
+            val = r.sums()[0][0] if nsel else (0)  # MIN/MAX of no rows: 0, as min_masked
+            return QueryResult([ResultColumn(name, [val % 2**64], None, INT8_OID)], 1)
+        if shape == QueryShape.CountDistinct:
+            n = ctx.count_distinct(t.col(s0.arg), 0, t.row_count, bm, A.DISTINCT_EXACT)
+            return QueryResult([ResultColumn(s0.alias or "count", [n], None, INT8_OID)], 1)
+        if shape in (QueryShape.GroupByCount, QueryShape.GroupBySum,
+                     QueryShape.GroupByOrderByLimit):
+            return self.group_by(bm)
+        return self.select(shape, bm)
+
+    # ---- GROUP BY one column: dense key range, exact aggregates -------------------
+    def group_by(self, bm):
+        q, t = self.q, self.t
+        if len(q.group_by) != 1:
+            raise ValueError("GROUP BY of one column only")
+        key = q.group_by[0]
+        lo, hi = t.col(key).minmax()
+        card = hi - lo + 1 if t.row_count else 1
+        if card > (1 << 24):
+            raise ValueError(f"GROUP BY {key}: key range {card} too wide for a dense table")
+        items = []
+        for s in q.select:
+            if isinstance(s, Aggregate) and s.func.upper() in ("SUM", "AVG", "MIN", "MAX"):
+                items.append((s.func.upper(), s.arg))
+        r = self.agg_query(items, parts=[(key, 1, lo, card)], bitmap=bm, names_extra=(key,))
+        gids = [g for g in range(len(r.count)) if r.count[g]]
+        sums = r.sums()
+        cols, k = [], 0
+        for s in q.select:
+            if isinstance(s, Column):
+                if s.name != key:
+                    raise ValueError(f"column {s.name!r} is neither grouped nor aggregated")
+                cols.append(_exact_column(s.name, [lo + g for g in gids]))
+            elif isinstance(s, Aggregate):
+                f = s.func.upper()
+                name = s.alias or f.lower()
+                if f == "COUNT":
+                    if s.arg != "*":
+                        t.col(s.arg)
+                    cols.append(ResultColumn(name, [int(r.count[g]) for g in gids], None,
+                                             INT8_OID))
+                    continue
+                if f == "COUNT_DISTINCT":
+                    raise ValueError("COUNT(DISTINCT) per group is not dispatched here")
+                if f == "SUM":
+                    cols.append(_exact_column(name, [sums[g][k] for g in gids]))
+                elif f == "AVG":
+                    v = r.avg_numeric(k)
+                    cols.append(ResultColumn(name, [0] * len(gids), [v[g] for g in gids],
+                                             NUMERIC_OID))
+                else:
+                    cols.append(ResultColumn(name, [sums[g][k] % 2**64 for g in gids], None,
+                                             INT8_OID))
+                k += 1
+        rows = list(range(len(gids)))
+        if q.order_by:
+            oname, asc = q.order_by[0]
+            names = [c.name for c in cols]
+            if oname not in names:
+                raise ValueError(f"ORDER BY {oname!r} is not a result column")
+            c = cols[names.index(oname)]
+            keyf = (lambda i: int(c.string_values[i]) if c.string_values else c.values[i])
+            rows = sorted(rows, key=keyf, reverse=not asc)
+        if q.limit is not None:
+            rows = rows[: q.limit]
+        for c in cols:
+            c.values = [c.values[i] for i in rows]
+            if c.string_values:
+                c.string_values = [c.string_values[i] for i in rows]
+        return QueryResult(cols, len(rows))
+
+    # ---- SELECT rows ---------------------------------------------------------------
+    def select(self, shape, bm):
+        q, t, ctx = self.q, self.t, self.ctx
+        if shape == QueryShape.SelectStar:
+            names, order = list(t.names), None  # SelectStar does not read ORDER BY
+        else:
+            names = [s.name for s in q.select]
+            order = q.order_by[0] if q.order_by else None
+        for n in names:
+            t.col(n)
+        rows, cnt = ctx.select_rows(0, t.row_count, bm,
+                                    t.col(order[0]) if order else None,
+                                    descending=bool(order and not order[1]), limit=q.limit)
+        try:
+            cols = [ResultColumn(n, [int(v) for v in ctx.gather(t.col(n), rows, cnt)], None,
+                                 INT8_OID) for n in names]
+        finally:
+            rows.free()
+        return QueryResult(cols, cnt)
