@@ -231,6 +231,19 @@ int32_t tgp_fill_u64(tgp_ctx* ctx, uint64_t* d_dst, uint64_t nwords, uint64_t va
 /* Column min/max (as int64 if signed dtype, else as uint64 bits), one pass. */
 int32_t tgp_column_minmax_sync(tgp_ctx* ctx, const tgp_column* col, int64_t* h_min, int64_t* h_max);
 
+/* ---- loading: reference cells -> native-width device columns ------------------ */
+/* The reference's u64 cells of one column (datasource/table.rs:25-30; i64 as
+ * `v as u64`, csv.rs:144-161) become a resident column of type `dtype` at d_out
+ * (n * width bytes, e.g. from tgp_hbm_alloc): uploaded in chunks and narrowed on
+ * the device.  A cell outside the type's range (signed types: the cell read as
+ * i64) is an error; *h_first_bad receives its row (UINT64_MAX when all fit). */
+int32_t tgp_column_from_cells(tgp_ctx* ctx, const uint64_t* h_cells, uint64_t n, int32_t dtype,
+                              void* d_out, uint64_t* h_first_bad, void* stream);
+/* NullBitmap (types/null_bitmap.rs: one bool per row, true = NULL) as a device
+ * bitmap of tgp_bitmap_words(n) words at d_bitmap: bit i set = row i is NULL. */
+int32_t tgp_null_bitmap_from_flags(tgp_ctx* ctx, const uint8_t* h_flags, uint64_t n,
+                                   uint64_t* d_bitmap, void* stream);
+
 /* ---- predicate scan, selection bitmap, compaction ------------------------- */
 /* Bitmap layout (bitmap.rs:36-45): bit i of the selection is bit (i & 63) of
  * 64-bit word i >> 6, i.e. LSB-first bytes on a little-endian host; the words

@@ -225,6 +225,8 @@ SIGNATURES = {
     "tgp_int128_to_numeric": (I32_, [U64_, U64_, I32_, C.c_char_p, U64_, C.POINTER(U64_)]),
     "tgp_int128_avg_to_numeric": (I32_, [U64_, U64_, U64_, I32_, I32_, C.c_char_p, U64_,
                                          C.POINTER(U64_)]),
+    "tgp_column_from_cells": (I32_, [P, P, U64_, I32_, P, C.POINTER(U64_), P]),
+    "tgp_null_bitmap_from_flags": (I32_, [P, P, U64_, P, P]),
     "tgp_hash_build_in": (I32_, [P, C.POINTER(Column), C.POINTER(Column), P, U64_,
                                  C.POINTER(C.c_void_p), P]),
     "tgp_hash_table_bytes_for": (U64_, [U64_]),

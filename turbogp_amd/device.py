@@ -92,6 +92,22 @@ class DeviceColumn:
             c.buf.upload(arr)
         return c
 
+    @classmethod
+    def from_cells(cls, ctx, cells, dtype, name=None):
+        """A column of `dtype` from the reference's u64 cells (tgp_column_from_cells);
+        a cell that does not fit the type raises."""
+        cells = np.ascontiguousarray(cells, np.uint64)
+        c = cls(ctx, cells.shape[0], dtype, name)
+        bad = C.c_uint64()
+        try:
+            A.check(A.lib().tgp_column_from_cells(ctx.handle, cells.ctypes.data, c.n, dtype,
+                                                  c.buf.ptr, C.byref(bad), None),
+                    "tgp_column_from_cells")
+        except Exception:
+            c.free()
+            raise
+        return c
+
     def desc(self):
         return A.Column(self.buf.ptr, self.n, self.dtype, 0)
 
@@ -138,6 +154,15 @@ class DeviceBitmap:
 
     def count_ones(self):
         return self.count_ones_range(0, self.nbits)
+
+    @classmethod
+    def from_null_flags(cls, ctx, flags):
+        """NullBitmap flags (one bool per row, True = NULL) as a device bitmap."""
+        flags = np.ascontiguousarray(flags, np.uint8)
+        bm = cls(ctx, len(flags))
+        A.check(A.lib().tgp_null_bitmap_from_flags(ctx.handle, flags.ctypes.data, len(flags),
+                                                   bm.buf.ptr, None), "tgp_null_bitmap_from_flags")
+        return bm
 
     def combine(self, other, op):
         out = DeviceBitmap(self.ctx, min(self.nbits, other.nbits))

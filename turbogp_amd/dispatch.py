@@ -20,15 +20,22 @@ device path needs them:
     first-seen order; rows compare equal after sorting by key);
   - an aggregate argument that names no column is an error, not column 0
     (dispatch.rs:1233 resolves it with `unwrap_or(0)`).
+NULLs (a device bitmap per nullable column, bit set = NULL, as loaded by
+tgp_null_bitmap_from_flags): COUNT(*) counts every selected row; COUNT(col),
+SUM, AVG, MIN, MAX and COUNT(DISTINCT) skip NULL cells of their columns
+(adjust_mask_for_nulls, dispatch.rs:1679-1687; tests/null_semantics.rs), also
+per group; a comparison with a NULL cell is false in WHERE (SQL; the dispatch
+path compares the stored cell); SELECT results carry a null_mask.
 """
 from dataclasses import dataclass, field
+from decimal import Decimal
 from enum import Enum
 from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
 from . import _capi as A
-from .device import make_agg_query, numeric_text
+from .device import DeviceBitmap, make_agg_query, numeric_text
 
 OPS = {"=": A.EQ, "!=": A.NE, "<>": A.NE, "<": A.LT, "<=": A.LE, ">": A.GT, ">=": A.GE}
 INT8_OID, NUMERIC_OID = 20, 1700
@@ -85,6 +92,7 @@ class ResultColumn:
     values: list
     string_values: Optional[List[str]] = None
     type_oid: int = 0
+    null_mask: Optional[List[bool]] = None
 
 
 @dataclass
@@ -98,15 +106,19 @@ class QueryResult:
 
 
 class Table:
-    """Resident device columns by name (all of one length)."""
+    """Resident device columns by name (all of one length), and the NULL bitmaps
+    (DeviceBitmap, bit set = NULL) of the nullable ones."""
 
-    def __init__(self, ctx, columns: dict):
+    def __init__(self, ctx, columns: dict, nulls: Optional[dict] = None):
         self.ctx = ctx
         self.columns = dict(columns)
+        self.nulls = dict(nulls or {})
         lens = {c.n for c in self.columns.values()}
         assert len(lens) <= 1, "columns of different lengths"
         self.row_count = lens.pop() if lens else 0
         self.names = list(self.columns)
+        for k in self.nulls:
+            self.col(k)
 
     def col(self, name):
         if name not in self.columns:
@@ -165,9 +177,21 @@ def _exact_column(name, vals):
     return ResultColumn(name, [0] * len(vals), [str(v) for v in vals], NUMERIC_OID)
 
 
+def _arg_columns(arg):
+    if isinstance(arg, str):
+        return [] if arg == "*" else [arg]
+    return [c for c, _, _ in arg[1]]
+
+
 class _Executor:
     def __init__(self, q, t):
         self.q, self.t, self.ctx = q, t, t.ctx
+        self.owned = []
+
+    def free(self):
+        for b in self.owned:
+            b.free()
+        self.owned = []
 
     # ---- WHERE: AND of comparisons -> selection bitmap over all rows --------------
     def mask(self):
@@ -181,7 +205,27 @@ class _Executor:
                 raise ValueError(f"unsupported comparison {op!r}")
             dt = self.t.col(c).dtype
             preds.append((names.index(c), OPS[op], int(lit), dt in (A.I8, A.I16, A.I32, A.I64)))
-        return self.ctx.scan_filter(cols, preds, 0, self.t.row_count)
+        bm, n = self.ctx.scan_filter(cols, preds, 0, self.t.row_count)
+        self.owned.append(bm)
+        # a comparison with a NULL cell is not true
+        bm, n = self.without_nulls(bm, n, names)
+        return bm, n
+
+    def without_nulls(self, bm, n, columns):
+        """The selection `bm` (None = all rows) minus the rows where any of `columns`
+        is NULL -> (bitmap or None, selected count)."""
+        for c in dict.fromkeys(columns):
+            nb = self.t.nulls.get(c)
+            if nb is None:
+                continue
+            if bm is None:
+                bm = nb.not_()
+            else:
+                bm = bm.combine(nb, A.BIT_ANDNOT)
+            self.owned.append(bm)
+        if bm is not None and self.t.nulls and any(c in self.t.nulls for c in columns):
+            n = bm.count_ones()
+        return bm, n
 
     # ---- aggregates without GROUP BY -------------------------------------------------
     def agg_query(self, items, parts=(), bitmap=None, names_extra=()):
@@ -215,22 +259,26 @@ class _Executor:
         q, t = self.q, self.t
         if shape == QueryShape.CountAll:
             return QueryResult([ResultColumn("count", [t.row_count], None, INT8_OID)], 1)
-        bm, nsel = self.mask()
         try:
+            bm, nsel = self.mask()
             return self._run(shape, bm, nsel)
         finally:
-            if bm is not None:
-                bm.free()
+            self.free()
 
     def _run(self, shape, bm, nsel):
         q, t, ctx = self.q, self.t, self.ctx
         s0 = q.select[0]
         if shape == QueryShape.CountFilter:
-            if s0.arg != "*":
-                t.col(s0.arg)  # COUNT(col) without NULLs counts every selected row
-            return QueryResult([ResultColumn("count", [nsel], None, INT8_OID)], 1)
+            cols = _arg_columns(s0.arg)
+            for c in cols:
+                t.col(c)
+            _, n = self.without_nulls(bm, nsel, cols)  # COUNT(col) skips NULLs
+            return QueryResult([ResultColumn("count", [n], None, INT8_OID)], 1)
         if shape in (QueryShape.SumCol, QueryShape.AvgCol, QueryShape.MinMax):
             func = s0.func.upper()
+            for c in _arg_columns(s0.arg):
+                t.col(c)
+            bm, nsel = self.without_nulls(bm, nsel, _arg_columns(s0.arg))
             r = self.agg_query([(func, s0.arg)], bitmap=bm)
             name = s0.alias or func.lower()
             if func == "AVG":
@@ -241,6 +289,7 @@ class _Executor:
             val = r.sums()[0][0] if nsel else (0)  # MIN/MAX of no rows: 0, as min_masked
             return QueryResult([ResultColumn(name, [val % 2**64], None, INT8_OID)], 1)
         if shape == QueryShape.CountDistinct:
+            bm, _ = self.without_nulls(bm, nsel, [s0.arg])
             n = ctx.count_distinct(t.col(s0.arg), 0, t.row_count, bm, A.DISTINCT_EXACT)
             return QueryResult([ResultColumn(s0.alias or "count", [n], None, INT8_OID)], 1)
         if shape in (QueryShape.GroupByCount, QueryShape.GroupBySum,
@@ -254,18 +303,32 @@ class _Executor:
         if len(q.group_by) != 1:
             raise ValueError("GROUP BY of one column only")
         key = q.group_by[0]
+        if key in t.nulls:
+            raise ValueError(f"GROUP BY the nullable column {key!r} is not dispatched")
         lo, hi = t.col(key).minmax()
         card = hi - lo + 1 if t.row_count else 1
         if card > (1 << 24):
             raise ValueError(f"GROUP BY {key}: key range {card} too wide for a dense table")
-        items = []
+        parts = [(key, 1, lo, card)]
+        base = self.agg_query([], parts=parts, bitmap=bm, names_extra=(key,))
+        gids = [g for g in range(len(base.count)) if base.count[g]]
+        # one pass per set of nullable columns the aggregates read: each aggregate
+        # sees the selected rows where its own columns are not NULL
+        passes = {}
         for s in q.select:
-            if isinstance(s, Aggregate) and s.func.upper() in ("SUM", "AVG", "MIN", "MAX"):
-                items.append((s.func.upper(), s.arg))
-        r = self.agg_query(items, parts=[(key, 1, lo, card)], bitmap=bm, names_extra=(key,))
-        gids = [g for g in range(len(r.count)) if r.count[g]]
-        sums = r.sums()
-        cols, k = [], 0
+            if isinstance(s, Aggregate) and not (s.func.upper() == "COUNT" and s.arg == "*"):
+                for c in _arg_columns(s.arg):
+                    t.col(c)
+                ns = tuple(sorted(c for c in _arg_columns(s.arg) if c in t.nulls))
+                passes.setdefault(ns, []).append(s)
+        results = {}
+        for ns, aggs in passes.items():
+            items = [(s.func.upper(), s.arg) for s in aggs
+                     if s.func.upper() in ("SUM", "AVG", "MIN", "MAX")]
+            pbm, _ = self.without_nulls(bm, 0, list(ns)) if ns else (bm, 0)
+            results[ns] = (self.agg_query(items, parts=parts, bitmap=pbm, names_extra=(key,))
+                           if ns else None, items)
+        cols = []
         for s in q.select:
             if isinstance(s, Column):
                 if s.name != key:
@@ -274,24 +337,34 @@ class _Executor:
             elif isinstance(s, Aggregate):
                 f = s.func.upper()
                 name = s.alias or f.lower()
+                if f == "COUNT_DISTINCT":
+                    raise ValueError("COUNT(DISTINCT) per group is not dispatched here")
+                if f == "COUNT" and s.arg == "*":
+                    cols.append(ResultColumn(name, [int(base.count[g]) for g in gids], None,
+                                             INT8_OID))
+                    continue
+                ns = tuple(sorted(c for c in _arg_columns(s.arg) if c in t.nulls))
+                r, items = results[ns]
+                if r is None:  # no nullable column: the base pass has the same rows
+                    r = results[ns] = (self.agg_query(items, parts=parts, bitmap=bm,
+                                                      names_extra=(key,)), items)
+                    r = r[0]
                 if f == "COUNT":
-                    if s.arg != "*":
-                        t.col(s.arg)
                     cols.append(ResultColumn(name, [int(r.count[g]) for g in gids], None,
                                              INT8_OID))
                     continue
-                if f == "COUNT_DISTINCT":
-                    raise ValueError("COUNT(DISTINCT) per group is not dispatched here")
+                k = items.index((f, s.arg))
+                sums = r.sums()
                 if f == "SUM":
                     cols.append(_exact_column(name, [sums[g][k] for g in gids]))
                 elif f == "AVG":
                     v = r.avg_numeric(k)
-                    cols.append(ResultColumn(name, [0] * len(gids), [v[g] for g in gids],
+                    cols.append(ResultColumn(name, [0] * len(gids),
+                                             [v[g] if v[g] is not None else "0" for g in gids],
                                              NUMERIC_OID))
                 else:
                     cols.append(ResultColumn(name, [sums[g][k] % 2**64 for g in gids], None,
                                              INT8_OID))
-                k += 1
         rows = list(range(len(gids)))
         if q.order_by:
             oname, asc = q.order_by[0]
@@ -299,7 +372,7 @@ class _Executor:
             if oname not in names:
                 raise ValueError(f"ORDER BY {oname!r} is not a result column")
             c = cols[names.index(oname)]
-            keyf = (lambda i: int(c.string_values[i]) if c.string_values else c.values[i])
+            keyf = (lambda i: Decimal(c.string_values[i]) if c.string_values else c.values[i])
             rows = sorted(rows, key=keyf, reverse=not asc)
         if q.limit is not None:
             rows = rows[: q.limit]
@@ -325,6 +398,15 @@ class _Executor:
         try:
             cols = [ResultColumn(n, [int(v) for v in ctx.gather(t.col(n), rows, cnt)], None,
                                  INT8_OID) for n in names]
+            if any(n in t.nulls for n in names) and cnt:
+                idx = rows.download(np.uint64, cnt).astype(np.int64)
+                for c in cols:
+                    nb = t.nulls.get(c.name)
+                    if nb is not None:
+                        w = nb.words()
+                        flags = (w[idx >> 6] >> (idx & 63).astype(np.uint64)) & 1
+                        if flags.any():
+                            c.null_mask = [bool(x) for x in flags]
         finally:
             rows.free()
         return QueryResult(cols, cnt)
