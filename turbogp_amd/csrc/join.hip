@@ -20,6 +20,7 @@
 // match.  Build rows whose key equals the sentinel go to a side list.  A probe
 // is one 16-byte load per visited slot (linear probing, load factor <= 0.5).
 #include "tgp_internal.h"
+#include "tile_engine.h"
 
 namespace tgp {
 
@@ -461,16 +462,23 @@ static int32_t scan_u64(hipStream_t s, uint64_t* v, uint64_t n, uint64_t* bsum) 
 //       per partition into mat[p * G + w];
 //   (2) an exclusive scan of mat gives every (partition, workgroup) pair its
 //       output range, so the scatter needs no atomics and is deterministic;
-//   (3) k_part_scatter: each workgroup sorts a tile of its rows by partition in
-//       LDS and writes (key, value, group) records partition run by partition
-//       run, so neighbouring lanes write neighbouring records;
-//   (4) k_part_agg: one workgroup per partition probes its records (first
-//       slot loads of 4 records in flight per lane), aggregates in an LDS table
-//       of 2^kPartShift groups and stores its groups' results (each group
-//       belongs to exactly one partition: plain stores, no atomics).
+//   (3) k_part_scatter: each workgroup probes a tile of its rows (first slot
+//       loads of 4 rows in flight per lane), sorts the tile by partition in LDS
+//       and writes one 10-byte record per row, (value, local group | matched),
+//       partition run by partition run, so neighbouring lanes write
+//       neighbouring records.  A row with several matches (duplicate build
+//       keys) adds matches x value to the zeroed result with device atomics
+//       and writes an unmatched record;
+//   (4) k_part_agg: one workgroup per partition aggregates its matched records
+//       in an LDS table of 2^kPartShift groups and adds its groups' totals to
+//       the result with plain stores (each group belongs to exactly one
+//       partition, and the scatter's atomics finished in the kernel before).
+// Probing in the scatter halves the records (the key is not written) and lets
+// the aggregation read them without touching the hash table.
 constexpr int kPartShift = 11;          // 2048 groups per partition: 48 KB of LDS
 constexpr int kPartTile = 2048;         // rows per scatter tile (staged in LDS)
-constexpr uint32_t kMaxParts = 1024;     // keeps the scatter's LDS within 64 KB
+constexpr uint32_t kMaxParts = 1024;     // scatter LDS <= 48 KB: 3 workgroups per CU
+constexpr uint16_t kRecMatched = 0x8000; // record flag: the row has exactly one match
 
 __device__ __forceinline__ uint32_t part_of(const ProbeAggDev& Q, uint64_t i, uint32_t nparts) {
   const int64_t g = load_i64(Q.grp, Q.gdt, i) - Q.key_base;
@@ -506,41 +514,69 @@ __global__ void __launch_bounds__(256) k_part_hist(ProbeAggDev Q, uint32_t npart
 }
 
 // mat: exclusive offsets after the scan ((nparts + 1) * G entries, then the total).
-// Per tile: the rows' keys, values and local groups are staged in LDS with
-// coalesced loads, ranked per partition, and written partition run by
-// partition run.
-__global__ void __launch_bounds__(256) k_part_scatter(ProbeAggDev Q, uint32_t nparts,
+// Per tile: the rows are loaded with coalesced loads and probed; their values
+// and flagged local groups are staged in LDS, ranked per partition, and
+// written partition run by partition run.
+__global__ void __launch_bounds__(256) k_part_scatter(JoinTableDev T, ProbeAggDev Q,
+                                                      uint32_t nparts,
                                                       const unsigned long long* mat,
-                                                      unsigned long long* rec_key,
-                                                      long long* rec_val, uint32_t* rec_grp) {
+                                                      long long* rec_val, uint16_t* rec_grp) {
   extern __shared__ __align__(16) unsigned long long cursor[];        // nparts + 1
-  unsigned long long* tk = cursor + nparts + 1;                       // kPartTile keys
-  long long* tv = reinterpret_cast<long long*>(tk + kPartTile);       // kPartTile values
+  long long* tv = reinterpret_cast<long long*>(cursor + nparts + 1);  // kPartTile values
   uint32_t* cnt = reinterpret_cast<uint32_t*>(tv + kPartTile);        // nparts + 1
   uint32_t* tstart = cnt + nparts + 1;                                // nparts + 1
   uint16_t* tp = reinterpret_cast<uint16_t*>(tstart + nparts + 1);    // kPartTile partitions
   uint16_t* tr = tp + kPartTile;                                      // kPartTile ranks
   uint16_t* tg = tr + kPartTile;                                      // kPartTile local groups
   uint16_t* perm = tg + kPartTile;                                    // kPartTile
+  constexpr int U = 4;
   const uint64_t G = gridDim.x;
   for (uint32_t p = threadIdx.x; p <= nparts; p += blockDim.x)
     cursor[p] = mat[(uint64_t)p * G + blockIdx.x];
   const uint64_t lo = row_lo(Q, blockIdx.x, G), hi = row_lo(Q, blockIdx.x + 1, G);
   const uint32_t per = (nparts + 1 + blockDim.x - 1) / blockDim.x;  // scan entries per thread
+  unsigned long long oor_multi = 0;  // matches of out-of-range rows with several matches
   for (uint64_t t0 = lo; t0 < hi; t0 += kPartTile) {
     const uint32_t tlen = (uint32_t)(hi - t0 < (uint64_t)kPartTile ? hi - t0 : kPartTile);
     for (uint32_t p = threadIdx.x; p <= nparts; p += blockDim.x) cnt[p] = 0;
     __syncthreads();
-    for (uint32_t j = threadIdx.x; j < tlen; j += blockDim.x) {
-      const uint64_t i = t0 + j;
-      const int64_t g = load_i64(Q.grp, Q.gdt, i) - Q.key_base;
-      const bool in = g >= 0 && (uint64_t)g < Q.ngroups;
-      const uint32_t p = in ? (uint32_t)((uint64_t)g >> kPartShift) : nparts;
-      tk[j] = (unsigned long long)load_i64(Q.keys, Q.kdt, i);
-      tv[j] = load_i64(Q.val, Q.vdt, i);
-      tg[j] = in ? (uint16_t)((uint64_t)g & ((1u << kPartShift) - 1)) : 0;
-      tp[j] = (uint16_t)p;
-      tr[j] = (uint16_t)atomicAdd(&cnt[p], 1u);
+    for (uint32_t j0 = threadIdx.x; j0 < tlen; j0 += U * blockDim.x) {
+      unsigned long long k[U];
+      int64_t v[U], g[U];
+      bool ok[U];
+      uint32_t m[U];
+      static_for<U>([&](auto I) {
+        constexpr int u = decltype(I)::value;
+        const uint32_t j = j0 + u * blockDim.x;
+        ok[u] = j < tlen;
+        const uint64_t i = t0 + j;
+        k[u] = ok[u] ? (unsigned long long)load_i64(Q.keys, Q.kdt, i) : 0;
+        v[u] = ok[u] ? load_i64(Q.val, Q.vdt, i) : 0;
+        g[u] = ok[u] ? load_i64(Q.grp, Q.gdt, i) - Q.key_base : 0;
+        m[u] = 0;
+      });
+      probe_keys(T, k, ok, [&](auto I, uint32_t, uint32_t) { ++m[decltype(I)::value]; });
+      static_for<U>([&](auto I) {
+        constexpr int u = decltype(I)::value;
+        if (!ok[u]) return;
+        const uint32_t j = j0 + u * blockDim.x;
+        const bool in = g[u] >= 0 && (uint64_t)g[u] < Q.ngroups;
+        const uint32_t p = in ? (uint32_t)((uint64_t)g[u] >> kPartShift) : nparts;
+        if (m[u] > 1) {  // duplicate build keys: matches x value straight into the result
+          if (in) {
+            atomicAdd(&Q.out_count[g[u]], (unsigned long long)m[u]);
+            const __int128 s = (__int128)v[u] * (__int128)m[u];
+            atomic_add128(&Q.out_sums[2 * g[u]], (uint64_t)s, (int64_t)(s >> 64));
+          } else {
+            oor_multi += m[u];
+          }
+        }
+        tv[j] = v[u];
+        tg[j] = (uint16_t)((in ? ((uint64_t)g[u] & ((1u << kPartShift) - 1)) : 0) |
+                           (m[u] == 1 ? kRecMatched : 0));
+        tp[j] = (uint16_t)p;
+        tr[j] = (uint16_t)atomicAdd(&cnt[p], 1u);
+      });
     }
     __syncthreads();
     // exclusive scan of cnt[0..nparts] into tstart: per-thread runs, then a
@@ -561,7 +597,6 @@ __global__ void __launch_bounds__(256) k_part_scatter(ProbeAggDev Q, uint32_t np
       const uint32_t j = perm[k];
       const uint32_t p = tp[j];
       const uint64_t dst = cursor[p] + (k - tstart[p]);
-      rec_key[dst] = tk[j];
       rec_val[dst] = tv[j];
       rec_grp[dst] = tg[j];
     }
@@ -569,16 +604,16 @@ __global__ void __launch_bounds__(256) k_part_scatter(ProbeAggDev Q, uint32_t np
     for (uint32_t p = threadIdx.x; p <= nparts; p += blockDim.x) cursor[p] += cnt[p];
     __syncthreads();
   }
+  if (oor_multi && Q.out_oor) atomicAdd(Q.out_oor, oor_multi);
 }
 
 // One workgroup per partition (partition nparts: the out-of-range rows, whose
 // matches are only counted).  Partition p's records are
 // [mat[p * G], mat[(p + 1) * G]).
-__global__ void __launch_bounds__(256) k_part_agg(JoinTableDev T, ProbeAggDev Q, uint32_t nparts,
-                                                  uint64_t G, const unsigned long long* mat,
-                                                  const unsigned long long* rec_key,
+__global__ void __launch_bounds__(256) k_part_agg(ProbeAggDev Q, uint32_t nparts, uint64_t G,
+                                                  const unsigned long long* mat,
                                                   const long long* rec_val,
-                                                  const uint32_t* rec_grp) {
+                                                  const uint16_t* rec_grp) {
   __shared__ unsigned long long cnt[1 << kPartShift], slo[1 << kPartShift], shi[1 << kPartShift];
   constexpr int U = 4;
   const uint32_t p = blockIdx.x;
@@ -594,44 +629,48 @@ __global__ void __launch_bounds__(256) k_part_agg(JoinTableDev T, ProbeAggDev Q,
   }
   unsigned long long m = 0;
   for (uint64_t rb = r0 + threadIdx.x; rb < r1; rb += (uint64_t)blockDim.x * U) {
-    unsigned long long k[U];
     int64_t v[U];
     uint32_t g[U];
-    bool ok[U];
     static_for<U>([&](auto I) {
       constexpr int u = decltype(I)::value;
       const uint64_t r = rb + (uint64_t)u * blockDim.x;
-      ok[u] = r < r1;
-      k[u] = ok[u] ? rec_key[r] : 0;
-      v[u] = ok[u] ? rec_val[r] : 0;
-      g[u] = ok[u] ? rec_grp[r] : 0;
+      const bool ok = r < r1;
+      v[u] = ok ? rec_val[r] : 0;
+      g[u] = ok ? rec_grp[r] : 0;
     });
-    if (oor_part) {
-      probe_keys(T, k, ok, [&](auto, uint32_t, uint32_t) { ++m; });
-    } else {
-      probe_keys(T, k, ok, [&](auto I, uint32_t, uint32_t) {
-        constexpr int u = decltype(I)::value;
-        atomicAdd(&cnt[g[u]], 1ull);
-        atomicAdd(&slo[g[u]], (unsigned long long)(uint32_t)v[u]);
-        const int64_t h = v[u] >> 32;
-        if (h) atomicAdd(&shi[g[u]], (unsigned long long)h);
-      });
-    }
+    static_for<U>([&](auto I) {
+      constexpr int u = decltype(I)::value;
+      if (!(g[u] & kRecMatched)) return;
+      if (oor_part) {
+        ++m;
+        return;
+      }
+      const uint32_t gl = g[u] & (kRecMatched - 1);
+      atomicAdd(&cnt[gl], 1ull);
+      atomicAdd(&slo[gl], (unsigned long long)(uint32_t)v[u]);
+      const int64_t h = v[u] >> 32;
+      if (h) atomicAdd(&shi[gl], (unsigned long long)h);
+    });
   }
   if (oor_part) {
     if (m && Q.out_oor) atomicAdd(Q.out_oor, m);
     return;
   }
   __syncthreads();
+  // the result was zeroed and holds the scatter's several-match rows: add to it
   const uint64_t g0 = (uint64_t)p << kPartShift;
   for (uint32_t gl = threadIdx.x; gl < (1u << kPartShift); gl += blockDim.x) {
     if (g0 + gl >= Q.ngroups) break;
+    if (!cnt[gl]) continue;
     uint64_t lo;
     int64_t hi;
     split_to_i128(slo[gl], (int64_t)shi[gl], &lo, &hi);
-    Q.out_count[g0 + gl] = cnt[gl];
-    Q.out_sums[2 * (g0 + gl)] = lo;
-    Q.out_sums[2 * (g0 + gl) + 1] = (unsigned long long)hi;
+    const uint64_t g = g0 + gl;
+    Q.out_count[g] += cnt[gl];
+    const uint64_t l0 = Q.out_sums[2 * g];
+    const uint64_t s = l0 + lo;
+    Q.out_sums[2 * g] = s;
+    Q.out_sums[2 * g + 1] += (unsigned long long)(hi + (int64_t)(s < l0));
   }
 }
 
@@ -1013,28 +1052,37 @@ int32_t tgp_hash_probe_agg(tgp_ctx* ctx, const tgp_join_agg_query* q, const tgp_
              ctx->tune_join_partition.load() == 0) {
     TGP_REQUIRE(n < (1ull << 32),
                 "tgp_hash_probe_agg: a partitioned call takes fewer than 2^32 probe rows");
-    // scratch: mat[(nparts + 1) * G + 1] | keys[n] | vals[n] | groups[n]
-    const uint64_t G = (uint64_t)persistent_grid(ctx, 2) < (n + kPartTile - 1) / kPartTile
-                           ? (uint64_t)persistent_grid(ctx, 2)
+    // scratch: mat[(nparts + 1) * G + 1] | vals[n] | flagged local groups[n]
+    const uint64_t G = (uint64_t)persistent_grid(ctx, 3) < (n + kPartTile - 1) / kPartTile
+                           ? (uint64_t)persistent_grid(ctx, 3)
                            : (n + kPartTile - 1) / kPartTile;
     const uint64_t mat_n = (nparts + 1) * G + 1;
     const uint64_t nsb = (mat_n - 1 + kScanChunk - 1) / kScanChunk + 1;  // scan block sums
     const uint64_t head = (8 * (mat_n + nsb) + 255) & ~255ull;
-    const uint64_t bytes = head + 16 * n + 4 * n + 256;
+    const uint64_t bytes = head + 8 * n + 2 * n + 256;
     void* scr = nullptr;
     TGP_TRY(get_scratch(ctx, s, bytes, &scr));
     auto* mat = static_cast<unsigned long long*>(scr);
-    auto* rec_key = reinterpret_cast<unsigned long long*>(static_cast<uint8_t*>(scr) + head);
-    auto* rec_val = reinterpret_cast<long long*>(rec_key + n);
-    auto* rec_grp = reinterpret_cast<uint32_t*>(rec_val + n);
+    auto* rec_val = reinterpret_cast<long long*>(static_cast<uint8_t*>(scr) + head);
+    auto* rec_grp = reinterpret_cast<uint16_t*>(rec_val + n);
     const uint32_t np = (uint32_t)nparts;
+    // the aggregation adds to the result (several-match rows arrive first)
+    {
+      uint64_t* zp[3] = {reinterpret_cast<uint64_t*>(Q.out_count),
+                         reinterpret_cast<uint64_t*>(Q.out_sums),
+                         reinterpret_cast<uint64_t*>(Q.out_oor)};
+      const uint64_t zn[3] = {q->ngroups, 2 * q->ngroups, Q.out_oor ? 1ull : 0ull};
+      TGP_TRY(zero_ranges(s, 3, zp, zn));
+    }
     k_part_hist<<<(unsigned)G, 256, (np + 1) * 4, s>>>(Q, np, mat);
     TGP_TRY(scan_u64(s, reinterpret_cast<uint64_t*>(mat), mat_n - 1,
                      reinterpret_cast<uint64_t*>(mat + mat_n)));
-    const uint32_t lds = (np + 1) * 16 + kPartTile * (16 + 4 * 2) + 16;
-    k_part_scatter<<<(unsigned)G, 256, lds, s>>>(Q, np, mat, rec_key, rec_val, rec_grp);
-    k_part_agg<<<np + 1, 256, 0, s>>>(q->table->dev(), Q, np, G, mat, rec_key, rec_val,
-                                      rec_grp);
+    const uint32_t lds = (np + 1) * 16 + kPartTile * (8 + 4 * 2) + 16;
+    if (lds > 65536)
+      TGP_HIP(hipFuncSetAttribute((const void*)k_part_scatter,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    k_part_scatter<<<(unsigned)G, 256, lds, s>>>(q->table->dev(), Q, np, mat, rec_val, rec_grp);
+    k_part_agg<<<np + 1, 256, 0, s>>>(Q, np, G, mat, rec_val, rec_grp);
   } else {
     TGP_REQUIRE(q->row_end - q->row_begin < (1ull << 32),
                 "tgp_hash_probe_agg: a global-table call takes fewer than 2^32 probe rows");
