@@ -196,7 +196,9 @@ def test_device_zipf_generator_matches_oracle(ctx):
 
 @pytest.mark.parametrize("group_by,ng", [(A.JOIN_GROUP_PROBE_COL, 20), (A.JOIN_GROUP_PROBE_COL, 1_000),
                                          (A.JOIN_GROUP_PROBE_COL, 2_047), (A.JOIN_GROUP_PROBE_COL, 2_049),
+                                         (A.JOIN_GROUP_PROBE_COL, 4_095), (A.JOIN_GROUP_PROBE_COL, 4_097),
                                          (A.JOIN_GROUP_PROBE_COL, 300_001),
+                                         (A.JOIN_GROUP_PROBE_COL, 2_000_001),
                                          (A.JOIN_GROUP_BUILD_PAYLOAD, 700),
                                          (A.JOIN_GROUP_BUILD_PAYLOAD, 5_000)])
 def test_probe_agg_every_table_form(ctx, group_by, ng):

@@ -461,23 +461,24 @@ static int32_t scan_u64(hipStream_t s, uint64_t* v, uint64_t n, uint64_t* bsum) 
 //   (1) k_part_hist: workgroup w of G counts its contiguous share of the rows
 //       per partition into mat[p * G + w];
 //   (2) an exclusive scan of mat gives every (partition, workgroup) pair its
-//       output range, so the scatter needs no atomics and is deterministic;
+//       output range, so the scatter needs no device atomics (the order of
+//       records inside a range varies, the integer totals do not);
 //   (3) k_part_scatter: each workgroup probes a tile of its rows (first slot
-//       loads of 4 rows in flight per lane), sorts the tile by partition in LDS
-//       and writes one 10-byte record per row, (value, local group | matched),
-//       partition run by partition run, so neighbouring lanes write
-//       neighbouring records.  A row with several matches (duplicate build
-//       keys) adds matches x value to the zeroed result with device atomics
-//       and writes an unmatched record;
+//       loads of 4 rows in flight per lane), ranks the rows per partition with
+//       LDS counters and writes one 10-byte record per row, (value, local
+//       group | matched), at its partition's cursor plus its rank.  A row with
+//       several matches (duplicate build keys) adds matches x value to the
+//       zeroed result with device atomics and writes an unmatched record;
 //   (4) k_part_agg: one workgroup per partition aggregates its matched records
 //       in an LDS table of 2^kPartShift groups and adds its groups' totals to
 //       the result with plain stores (each group belongs to exactly one
 //       partition, and the scatter's atomics finished in the kernel before).
 // Probing in the scatter halves the records (the key is not written) and lets
 // the aggregation read them without touching the hash table.
-constexpr int kPartShift = 11;          // 2048 groups per partition: 48 KB of LDS
-constexpr int kPartTile = 2048;         // rows per scatter tile (staged in LDS)
-constexpr uint32_t kMaxParts = 1024;     // scatter LDS <= 48 KB: 3 workgroups per CU
+constexpr int kPartShift = 12;          // 4096 groups per partition: 80 KB of LDS
+constexpr int kScatterRows = 16;        // rows per thread per scatter tile
+constexpr int kPartTile = 256 * kScatterRows;  // rows per scatter tile
+constexpr uint32_t kMaxParts = 1024;     // scatter LDS <= 12 KB; 4M groups
 constexpr uint16_t kRecMatched = 0x8000; // record flag: the row has exactly one match
 
 __device__ __forceinline__ uint32_t part_of(const ProbeAggDev& Q, uint64_t i, uint32_t nparts) {
@@ -514,95 +515,79 @@ __global__ void __launch_bounds__(256) k_part_hist(ProbeAggDev Q, uint32_t npart
 }
 
 // mat: exclusive offsets after the scan ((nparts + 1) * G entries, then the total).
-// Per tile: the rows are loaded with coalesced loads and probed; their values
-// and flagged local groups are staged in LDS, ranked per partition, and
-// written partition run by partition run.
+// Per tile of kPartTile rows (kScatterRows per thread, kept in registers): the
+// rows are loaded with coalesced loads and probed, ranked per partition with
+// LDS counters, and each record is written at its partition's cursor plus its
+// rank.  With hundreds of partitions a tile holds only a few rows of each, so
+// sorting the tile first would not make the writes contiguous; the small LDS
+// footprint keeps 8 workgroups per CU to hide the probes' latency instead.
 __global__ void __launch_bounds__(256) k_part_scatter(JoinTableDev T, ProbeAggDev Q,
                                                       uint32_t nparts,
                                                       const unsigned long long* mat,
                                                       long long* rec_val, uint16_t* rec_grp) {
   extern __shared__ __align__(16) unsigned long long cursor[];        // nparts + 1
-  long long* tv = reinterpret_cast<long long*>(cursor + nparts + 1);  // kPartTile values
-  uint32_t* cnt = reinterpret_cast<uint32_t*>(tv + kPartTile);        // nparts + 1
-  uint32_t* tstart = cnt + nparts + 1;                                // nparts + 1
-  uint16_t* tp = reinterpret_cast<uint16_t*>(tstart + nparts + 1);    // kPartTile partitions
-  uint16_t* tr = tp + kPartTile;                                      // kPartTile ranks
-  uint16_t* tg = tr + kPartTile;                                      // kPartTile local groups
-  uint16_t* perm = tg + kPartTile;                                    // kPartTile
-  constexpr int U = 4;
+  uint32_t* cnt = reinterpret_cast<uint32_t*>(cursor + nparts + 1);  // nparts + 1
+  constexpr int U = 4, R = kScatterRows;
+  static_assert(R % U == 0, "rows per thread: whole probe groups");
   const uint64_t G = gridDim.x;
   for (uint32_t p = threadIdx.x; p <= nparts; p += blockDim.x)
     cursor[p] = mat[(uint64_t)p * G + blockIdx.x];
   const uint64_t lo = row_lo(Q, blockIdx.x, G), hi = row_lo(Q, blockIdx.x + 1, G);
-  const uint32_t per = (nparts + 1 + blockDim.x - 1) / blockDim.x;  // scan entries per thread
   unsigned long long oor_multi = 0;  // matches of out-of-range rows with several matches
-  for (uint64_t t0 = lo; t0 < hi; t0 += kPartTile) {
-    const uint32_t tlen = (uint32_t)(hi - t0 < (uint64_t)kPartTile ? hi - t0 : kPartTile);
+  for (uint64_t t0 = lo; t0 < hi; t0 += (uint64_t)R * blockDim.x) {
     for (uint32_t p = threadIdx.x; p <= nparts; p += blockDim.x) cnt[p] = 0;
     __syncthreads();
-    for (uint32_t j0 = threadIdx.x; j0 < tlen; j0 += U * blockDim.x) {
+    int64_t v[R];
+    uint32_t pr[R], rk[R];
+    uint16_t rg[R];
+    static_for<R / U>([&](auto H) {
+      constexpr int h = decltype(H)::value;
       unsigned long long k[U];
-      int64_t v[U], g[U];
+      int64_t g[U];
       bool ok[U];
       uint32_t m[U];
       static_for<U>([&](auto I) {
         constexpr int u = decltype(I)::value;
-        const uint32_t j = j0 + u * blockDim.x;
-        ok[u] = j < tlen;
-        const uint64_t i = t0 + j;
+        const uint64_t i = t0 + (uint64_t)(h * U + u) * blockDim.x + threadIdx.x;
+        ok[u] = i < hi;
         k[u] = ok[u] ? (unsigned long long)load_i64(Q.keys, Q.kdt, i) : 0;
-        v[u] = ok[u] ? load_i64(Q.val, Q.vdt, i) : 0;
+        v[h * U + u] = ok[u] ? load_i64(Q.val, Q.vdt, i) : 0;
         g[u] = ok[u] ? load_i64(Q.grp, Q.gdt, i) - Q.key_base : 0;
         m[u] = 0;
       });
       probe_keys(T, k, ok, [&](auto I, uint32_t, uint32_t) { ++m[decltype(I)::value]; });
       static_for<U>([&](auto I) {
         constexpr int u = decltype(I)::value;
+        constexpr int r = h * U + u;
+        pr[r] = 0xffffffffu;
         if (!ok[u]) return;
-        const uint32_t j = j0 + u * blockDim.x;
         const bool in = g[u] >= 0 && (uint64_t)g[u] < Q.ngroups;
         const uint32_t p = in ? (uint32_t)((uint64_t)g[u] >> kPartShift) : nparts;
         if (m[u] > 1) {  // duplicate build keys: matches x value straight into the result
           if (in) {
             atomicAdd(&Q.out_count[g[u]], (unsigned long long)m[u]);
-            const __int128 s = (__int128)v[u] * (__int128)m[u];
+            const __int128 s = (__int128)v[r] * (__int128)m[u];
             atomic_add128(&Q.out_sums[2 * g[u]], (uint64_t)s, (int64_t)(s >> 64));
           } else {
             oor_multi += m[u];
           }
         }
-        tv[j] = v[u];
-        tg[j] = (uint16_t)((in ? ((uint64_t)g[u] & ((1u << kPartShift) - 1)) : 0) |
+        rg[r] = (uint16_t)((in ? ((uint64_t)g[u] & ((1u << kPartShift) - 1)) : 0) |
                            (m[u] == 1 ? kRecMatched : 0));
-        tp[j] = (uint16_t)p;
-        tr[j] = (uint16_t)atomicAdd(&cnt[p], 1u);
+        pr[r] = p;
+        rk[r] = atomicAdd(&cnt[p], 1u);
       });
-    }
+    });
     __syncthreads();
-    // exclusive scan of cnt[0..nparts] into tstart: per-thread runs, then a
-    // workgroup scan of the run totals
-    const uint32_t q0 = threadIdx.x * per;
-    uint32_t run = 0;
-    for (uint32_t q = q0; q < q0 + per && q <= nparts; ++q) run += cnt[q];
-    uint64_t tile_tot = 0;
-    run = (uint32_t)block_exclusive_scan(run, &tile_tot);
-    for (uint32_t q = q0; q < q0 + per && q <= nparts; ++q) {
-      tstart[q] = run;
-      run += cnt[q];
-    }
-    __syncthreads();
-    for (uint32_t j = threadIdx.x; j < tlen; j += blockDim.x) perm[tstart[tp[j]] + tr[j]] = (uint16_t)j;
-    __syncthreads();
-    for (uint32_t k = threadIdx.x; k < tlen; k += blockDim.x) {
-      const uint32_t j = perm[k];
-      const uint32_t p = tp[j];
-      const uint64_t dst = cursor[p] + (k - tstart[p]);
-      rec_val[dst] = tv[j];
-      rec_grp[dst] = tg[j];
-    }
+    static_for<R>([&](auto I) {
+      constexpr int r = decltype(I)::value;
+      if (pr[r] == 0xffffffffu) return;
+      const uint64_t dst = cursor[pr[r]] + rk[r];
+      rec_val[dst] = v[r];
+      rec_grp[dst] = rg[r];
+    });
     __syncthreads();
     for (uint32_t p = threadIdx.x; p <= nparts; p += blockDim.x) cursor[p] += cnt[p];
-    __syncthreads();
   }
   if (oor_multi && Q.out_oor) atomicAdd(Q.out_oor, oor_multi);
 }
@@ -614,7 +599,9 @@ __global__ void __launch_bounds__(256) k_part_agg(ProbeAggDev Q, uint32_t nparts
                                                   const unsigned long long* mat,
                                                   const long long* rec_val,
                                                   const uint16_t* rec_grp) {
-  __shared__ unsigned long long cnt[1 << kPartShift], slo[1 << kPartShift], shi[1 << kPartShift];
+  // counts fit 32 bits: a partitioned call takes fewer than 2^32 rows
+  __shared__ uint32_t cnt[1 << kPartShift];
+  __shared__ unsigned long long slo[1 << kPartShift], shi[1 << kPartShift];
   constexpr int U = 4;
   const uint32_t p = blockIdx.x;
   const uint64_t r0 = mat[(uint64_t)p * G], r1 = mat[(uint64_t)(p + 1) * G];
@@ -646,7 +633,7 @@ __global__ void __launch_bounds__(256) k_part_agg(ProbeAggDev Q, uint32_t nparts
         return;
       }
       const uint32_t gl = g[u] & (kRecMatched - 1);
-      atomicAdd(&cnt[gl], 1ull);
+      atomicAdd(&cnt[gl], 1u);
       atomicAdd(&slo[gl], (unsigned long long)(uint32_t)v[u]);
       const int64_t h = v[u] >> 32;
       if (h) atomicAdd(&shi[gl], (unsigned long long)h);
@@ -666,7 +653,7 @@ __global__ void __launch_bounds__(256) k_part_agg(ProbeAggDev Q, uint32_t nparts
     int64_t hi;
     split_to_i128(slo[gl], (int64_t)shi[gl], &lo, &hi);
     const uint64_t g = g0 + gl;
-    Q.out_count[g] += cnt[gl];
+    Q.out_count[g] += (unsigned long long)cnt[gl];
     const uint64_t l0 = Q.out_sums[2 * g];
     const uint64_t s = l0 + lo;
     Q.out_sums[2 * g] = s;
@@ -1053,8 +1040,8 @@ int32_t tgp_hash_probe_agg(tgp_ctx* ctx, const tgp_join_agg_query* q, const tgp_
     TGP_REQUIRE(n < (1ull << 32),
                 "tgp_hash_probe_agg: a partitioned call takes fewer than 2^32 probe rows");
     // scratch: mat[(nparts + 1) * G + 1] | vals[n] | flagged local groups[n]
-    const uint64_t G = (uint64_t)persistent_grid(ctx, 3) < (n + kPartTile - 1) / kPartTile
-                           ? (uint64_t)persistent_grid(ctx, 3)
+    const uint64_t G = (uint64_t)persistent_grid(ctx, 8) < (n + kPartTile - 1) / kPartTile
+                           ? (uint64_t)persistent_grid(ctx, 8)
                            : (n + kPartTile - 1) / kPartTile;
     const uint64_t mat_n = (nparts + 1) * G + 1;
     const uint64_t nsb = (mat_n - 1 + kScanChunk - 1) / kScanChunk + 1;  // scan block sums
@@ -1077,10 +1064,7 @@ int32_t tgp_hash_probe_agg(tgp_ctx* ctx, const tgp_join_agg_query* q, const tgp_
     k_part_hist<<<(unsigned)G, 256, (np + 1) * 4, s>>>(Q, np, mat);
     TGP_TRY(scan_u64(s, reinterpret_cast<uint64_t*>(mat), mat_n - 1,
                      reinterpret_cast<uint64_t*>(mat + mat_n)));
-    const uint32_t lds = (np + 1) * 16 + kPartTile * (8 + 4 * 2) + 16;
-    if (lds > 65536)
-      TGP_HIP(hipFuncSetAttribute((const void*)k_part_scatter,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const uint32_t lds = (np + 1) * 12 + 16;
     k_part_scatter<<<(unsigned)G, 256, lds, s>>>(q->table->dev(), Q, np, mat, rec_val, rec_grp);
     k_part_agg<<<np + 1, 256, 0, s>>>(Q, np, G, mat, rec_val, rec_grp);
   } else {
