@@ -62,6 +62,38 @@ __device__ __forceinline__ int64_t load_i64(const void* base, int32_t dtype, uin
   }
 }
 
+// Loads rows idx[u] of a column as int64 (0 where !ok[u]), for U rows at once.
+// The dtype switch sits around all U loads so that they are in flight
+// together: with a switch per load the compiler waits for each load inside its
+// branch.  idx[u] must be a valid row even where !ok[u] (callers pass a row of
+// the range), so that every load is unconditional.
+template <typename T, int U>
+__device__ __forceinline__ void load_rows_as(const void* base, const uint64_t (&idx)[U],
+                                             const bool (&ok)[U], int64_t (&out)[U]) {
+  const T* p = static_cast<const T*>(base);
+  T raw[U];
+  static_for<U>([&](auto I) { raw[decltype(I)::value] = p[idx[decltype(I)::value]]; });
+  static_for<U>([&](auto I) {
+    constexpr int u = decltype(I)::value;
+    out[u] = ok[u] ? (int64_t)raw[u] : 0;
+  });
+}
+
+template <int U>
+__device__ __forceinline__ void load_rows(const void* base, int32_t dtype,
+                                          const uint64_t (&idx)[U], const bool (&ok)[U],
+                                          int64_t (&out)[U]) {
+  switch (dtype) {
+    case TGP_I8: load_rows_as<int8_t>(base, idx, ok, out); break;
+    case TGP_U8: load_rows_as<uint8_t>(base, idx, ok, out); break;
+    case TGP_I16: load_rows_as<int16_t>(base, idx, ok, out); break;
+    case TGP_U16: load_rows_as<uint16_t>(base, idx, ok, out); break;
+    case TGP_I32: load_rows_as<int32_t>(base, idx, ok, out); break;
+    case TGP_U32: load_rows_as<uint32_t>(base, idx, ok, out); break;
+    default: load_rows_as<int64_t>(base, idx, ok, out); break;
+  }
+}
+
 __global__ void __launch_bounds__(256) k_table_init(Slot* slots, uint64_t cap) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < cap;
        i += (uint64_t)gridDim.x * blockDim.x) {
@@ -188,15 +220,18 @@ __global__ void __launch_bounds__(256) k_probe_count(JoinTableDev T, const void*
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i0 = b + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i0 < e;
        i0 += stride * kProbeUnroll) {
+    uint64_t idx[kProbeUnroll];
+    bool ok[kProbeUnroll];
+    int64_t kv[kProbeUnroll];
     unsigned long long k[kProbeUnroll];
 #pragma unroll
     for (int u = 0; u < kProbeUnroll; ++u) {
-      const uint64_t i = i0 + u * stride;
-      k[u] = i < e ? (unsigned long long)load_i64(keys, kdt, i) : 0;
+      ok[u] = i0 + u * stride < e;
+      idx[u] = ok[u] ? i0 + u * stride : i0;
     }
-    bool ok[kProbeUnroll];
+    load_rows(keys, kdt, idx, ok, kv);
 #pragma unroll
-    for (int u = 0; u < kProbeUnroll; ++u) ok[u] = i0 + u * stride < e;
+    for (int u = 0; u < kProbeUnroll; ++u) k[u] = (unsigned long long)kv[u];
     probe_keys(T, k, ok, [&](auto, uint32_t, uint32_t) { ++c; });
   }
   for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
@@ -299,19 +334,23 @@ __global__ void __launch_bounds__(256) k_probe_agg(JoinTableDev T, ProbeAggDev Q
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i0 = Q.b + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i0 < Q.e;
        i0 += stride * kProbeUnroll) {
+    uint64_t idx[kProbeUnroll];
+    bool ok[kProbeUnroll];
     unsigned long long k[kProbeUnroll];
-    int64_t v[kProbeUnroll], g[kProbeUnroll];
+    int64_t kv[kProbeUnroll], v[kProbeUnroll], g[kProbeUnroll];
 #pragma unroll
     for (int u = 0; u < kProbeUnroll; ++u) {
-      const uint64_t i = i0 + u * stride;
-      const bool ok = i < Q.e;
-      k[u] = ok ? (unsigned long long)load_i64(Q.keys, Q.kdt, i) : 0;
-      v[u] = ok ? load_i64(Q.val, Q.vdt, i) : 0;
-      g[u] = (ok && Q.group_by == TGP_JOIN_GROUP_PROBE_COL) ? load_i64(Q.grp, Q.gdt, i) : 0;
+      ok[u] = i0 + u * stride < Q.e;
+      idx[u] = ok[u] ? i0 + u * stride : i0;
     }
-    bool ok[kProbeUnroll];
+    load_rows(Q.keys, Q.kdt, idx, ok, kv);
+    load_rows(Q.val, Q.vdt, idx, ok, v);
+    if (Q.group_by == TGP_JOIN_GROUP_PROBE_COL) load_rows(Q.grp, Q.gdt, idx, ok, g);
 #pragma unroll
-    for (int u = 0; u < kProbeUnroll; ++u) ok[u] = i0 + u * stride < Q.e;
+    for (int u = 0; u < kProbeUnroll; ++u) {
+      k[u] = (unsigned long long)kv[u];
+      if (Q.group_by != TGP_JOIN_GROUP_PROBE_COL) g[u] = 0;
+    }
     if (Q.group_by == TGP_JOIN_GROUP_BUILD_PAYLOAD) {
       probe_keys(T, k, ok, [&](auto I, uint32_t, uint32_t pay) {
         const int64_t gg = (int64_t)(int32_t)pay - Q.key_base;
@@ -481,8 +520,9 @@ constexpr int kPartTile = 256 * kScatterRows;  // rows per scatter tile
 constexpr uint32_t kMaxParts = 1024;     // scatter LDS <= 12 KB; 4M groups
 constexpr uint16_t kRecMatched = 0x8000; // record flag: the row has exactly one match
 
-__device__ __forceinline__ uint32_t part_of(const ProbeAggDev& Q, uint64_t i, uint32_t nparts) {
-  const int64_t g = load_i64(Q.grp, Q.gdt, i) - Q.key_base;
+// Partition of a group-column value (nparts: outside [key_base, key_base + ngroups)).
+__device__ __forceinline__ uint32_t part_of(const ProbeAggDev& Q, int64_t cell, uint32_t nparts) {
+  const int64_t g = cell - Q.key_base;
   return (g < 0 || (uint64_t)g >= Q.ngroups) ? nparts : (uint32_t)((uint64_t)g >> kPartShift);
 }
 
@@ -498,15 +538,20 @@ __global__ void __launch_bounds__(256) k_part_hist(ProbeAggDev Q, uint32_t npart
   for (uint32_t p = threadIdx.x; p <= nparts; p += blockDim.x) h[p] = 0;
   __syncthreads();
   const uint64_t lo = row_lo(Q, blockIdx.x, G), hi = row_lo(Q, blockIdx.x + 1, G);
-  for (uint64_t i0 = lo + threadIdx.x; i0 < hi; i0 += 4 * (uint64_t)blockDim.x) {
-    uint32_t p[4];
-    static_for<4>([&](auto I) {
+  constexpr int U = 8;
+  for (uint64_t i0 = lo + threadIdx.x; i0 < hi; i0 += U * (uint64_t)blockDim.x) {
+    uint64_t idx[U];
+    bool ok[U];
+    int64_t g[U];
+    static_for<U>([&](auto I) {
       constexpr int u = decltype(I)::value;
-      const uint64_t i = i0 + (uint64_t)u * blockDim.x;
-      p[u] = i < hi ? part_of(Q, i, nparts) : 0xffffffffu;
+      ok[u] = i0 + (uint64_t)u * blockDim.x < hi;
+      idx[u] = ok[u] ? i0 + (uint64_t)u * blockDim.x : i0;
     });
-    static_for<4>([&](auto I) {
-      if (p[decltype(I)::value] != 0xffffffffu) atomicAdd(&h[p[decltype(I)::value]], 1u);
+    load_rows(Q.grp, Q.gdt, idx, ok, g);
+    static_for<U>([&](auto I) {
+      constexpr int u = decltype(I)::value;
+      if (ok[u]) atomicAdd(&h[part_of(Q, g[u], nparts)], 1u);
     });
   }
   __syncthreads();
@@ -543,17 +588,25 @@ __global__ void __launch_bounds__(256) k_part_scatter(JoinTableDev T, ProbeAggDe
     static_for<R / U>([&](auto H) {
       constexpr int h = decltype(H)::value;
       unsigned long long k[U];
-      int64_t g[U];
+      int64_t kv[U], vv[U], g[U];
+      uint64_t idx[U];
       bool ok[U];
       uint32_t m[U];
       static_for<U>([&](auto I) {
         constexpr int u = decltype(I)::value;
         const uint64_t i = t0 + (uint64_t)(h * U + u) * blockDim.x + threadIdx.x;
         ok[u] = i < hi;
-        k[u] = ok[u] ? (unsigned long long)load_i64(Q.keys, Q.kdt, i) : 0;
-        v[h * U + u] = ok[u] ? load_i64(Q.val, Q.vdt, i) : 0;
-        g[u] = ok[u] ? load_i64(Q.grp, Q.gdt, i) - Q.key_base : 0;
+        idx[u] = ok[u] ? i : t0;  // t0 < hi: a row of the share
         m[u] = 0;
+      });
+      load_rows(Q.keys, Q.kdt, idx, ok, kv);
+      load_rows(Q.val, Q.vdt, idx, ok, vv);
+      load_rows(Q.grp, Q.gdt, idx, ok, g);
+      static_for<U>([&](auto I) {
+        constexpr int u = decltype(I)::value;
+        k[u] = (unsigned long long)kv[u];
+        v[h * U + u] = vv[u];
+        g[u] -= Q.key_base;
       });
       probe_keys(T, k, ok, [&](auto I, uint32_t, uint32_t) { ++m[decltype(I)::value]; });
       static_for<U>([&](auto I) {
@@ -669,15 +722,29 @@ __global__ void __launch_bounds__(256) k_pairs_count(JoinTableDev T, const void*
                                                      uint64_t* block_tot) {
   __shared__ unsigned long long red[4];
   const uint64_t r0 = b + blockIdx.x * (uint64_t)kPairBlock;
-  unsigned long long c = 0;
-  for (int u = 0; u < kPairBlock / 256; ++u) {
+  constexpr int U = kPairBlock / 256;
+  uint64_t idx[U];
+  bool ok[U];
+  int64_t kv[U];
+  unsigned long long k[U];
+  uint32_t m[U];
+  static_for<U>([&](auto I) {
+    constexpr int u = decltype(I)::value;
     const uint64_t i = r0 + u * 256 + threadIdx.x;
-    if (i >= e) continue;
-    const uint32_t m = probe_key(T, (unsigned long long)load_i64(keys, kdt, i),
-                                 [](uint32_t, uint32_t) {});
-    per_row[i - b] = m;
-    c += m;
-  }
+    ok[u] = i < e;
+    idx[u] = ok[u] ? i : r0;  // r0 < e: the grid covers the range
+    m[u] = 0;
+  });
+  load_rows(keys, kdt, idx, ok, kv);
+  static_for<U>([&](auto I) { k[decltype(I)::value] = (unsigned long long)kv[decltype(I)::value]; });
+  probe_keys(T, k, ok, [&](auto I, uint32_t, uint32_t) { ++m[decltype(I)::value]; });
+  unsigned long long c = 0;
+  static_for<U>([&](auto I) {
+    constexpr int u = decltype(I)::value;
+    if (!ok[u]) return;
+    per_row[idx[u] - b] = m[u];
+    c += m[u];
+  });
   for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = c;
   __syncthreads();
