@@ -504,7 +504,7 @@ static int32_t scan_u64(hipStream_t s, uint64_t* v, uint64_t n, uint64_t* bsum) 
 //       records inside a range varies, the integer totals do not);
 //   (3) k_part_scatter: each workgroup probes a tile of its rows (first slot
 //       loads of 4 rows in flight per lane), ranks the rows per partition with
-//       LDS counters and writes one 10-byte record per row, (value, local
+//       LDS counters and writes one 12-byte record per row, (value, local
 //       group | matched), at its partition's cursor plus its rank.  A row with
 //       several matches (duplicate build keys) adds matches x value to the
 //       zeroed result with device atomics and writes an unmatched record;
@@ -512,13 +512,20 @@ static int32_t scan_u64(hipStream_t s, uint64_t* v, uint64_t n, uint64_t* bsum) 
 //       in an LDS table of 2^kPartShift groups and adds its groups' totals to
 //       the result with plain stores (each group belongs to exactly one
 //       partition, and the scatter's atomics finished in the kernel before).
-// Probing in the scatter halves the records (the key is not written) and lets
-// the aggregation read them without touching the hash table.
+// Probing in the scatter keeps the key out of the records and lets the
+// aggregation read them without touching the hash table.
 constexpr int kPartShift = 12;          // 4096 groups per partition: 80 KB of LDS
 constexpr int kScatterRows = 16;        // rows per thread per scatter tile
 constexpr int kPartTile = 256 * kScatterRows;  // rows per scatter tile
 constexpr uint32_t kMaxParts = 1024;     // scatter LDS <= 12 KB; 4M groups
-constexpr uint16_t kRecMatched = 0x8000; // record flag: the row has exactly one match
+constexpr uint32_t kRecMatched = 0x8000; // record flag: the row has exactly one match
+
+// One scatter record: a probe row's value and its flagged local group, written
+// and read as one 12-byte access.
+struct Rec12 {
+  uint32_t lo, hi, grp;
+};
+static_assert(sizeof(Rec12) == 12, "12-byte record");
 
 // Partition of a group-column value (nparts: outside [key_base, key_base + ngroups)).
 __device__ __forceinline__ uint32_t part_of(const ProbeAggDev& Q, int64_t cell, uint32_t nparts) {
@@ -569,7 +576,7 @@ __global__ void __launch_bounds__(256) k_part_hist(ProbeAggDev Q, uint32_t npart
 __global__ void __launch_bounds__(256) k_part_scatter(JoinTableDev T, ProbeAggDev Q,
                                                       uint32_t nparts,
                                                       const unsigned long long* mat,
-                                                      long long* rec_val, uint16_t* rec_grp) {
+                                                      Rec12* recs) {
   extern __shared__ __align__(16) unsigned long long cursor[];        // nparts + 1
   uint32_t* cnt = reinterpret_cast<uint32_t*>(cursor + nparts + 1);  // nparts + 1
   constexpr int U = 4, R = kScatterRows;
@@ -584,7 +591,7 @@ __global__ void __launch_bounds__(256) k_part_scatter(JoinTableDev T, ProbeAggDe
     __syncthreads();
     int64_t v[R];
     uint32_t pr[R], rk[R];
-    uint16_t rg[R];
+    uint32_t rg[R];
     static_for<R / U>([&](auto H) {
       constexpr int h = decltype(H)::value;
       unsigned long long k[U];
@@ -625,7 +632,7 @@ __global__ void __launch_bounds__(256) k_part_scatter(JoinTableDev T, ProbeAggDe
             oor_multi += m[u];
           }
         }
-        rg[r] = (uint16_t)((in ? ((uint64_t)g[u] & ((1u << kPartShift) - 1)) : 0) |
+        rg[r] = (uint32_t)((in ? ((uint64_t)g[u] & ((1u << kPartShift) - 1)) : 0) |
                            (m[u] == 1 ? kRecMatched : 0));
         pr[r] = p;
         rk[r] = atomicAdd(&cnt[p], 1u);
@@ -636,8 +643,7 @@ __global__ void __launch_bounds__(256) k_part_scatter(JoinTableDev T, ProbeAggDe
       constexpr int r = decltype(I)::value;
       if (pr[r] == 0xffffffffu) return;
       const uint64_t dst = cursor[pr[r]] + rk[r];
-      rec_val[dst] = v[r];
-      rec_grp[dst] = rg[r];
+      recs[dst] = Rec12{(uint32_t)(uint64_t)v[r], (uint32_t)((uint64_t)v[r] >> 32), rg[r]};
     });
     __syncthreads();
     for (uint32_t p = threadIdx.x; p <= nparts; p += blockDim.x) cursor[p] += cnt[p];
@@ -648,10 +654,9 @@ __global__ void __launch_bounds__(256) k_part_scatter(JoinTableDev T, ProbeAggDe
 // One workgroup per partition (partition nparts: the out-of-range rows, whose
 // matches are only counted).  Partition p's records are
 // [mat[p * G], mat[(p + 1) * G]).
-__global__ void __launch_bounds__(256) k_part_agg(ProbeAggDev Q, uint32_t nparts, uint64_t G,
+__global__ void __launch_bounds__(1024) k_part_agg(ProbeAggDev Q, uint32_t nparts, uint64_t G,
                                                   const unsigned long long* mat,
-                                                  const long long* rec_val,
-                                                  const uint16_t* rec_grp) {
+                                                  const Rec12* recs) {
   // counts fit 32 bits: a partitioned call takes fewer than 2^32 rows
   __shared__ uint32_t cnt[1 << kPartShift];
   __shared__ unsigned long long slo[1 << kPartShift], shi[1 << kPartShift];
@@ -675,8 +680,9 @@ __global__ void __launch_bounds__(256) k_part_agg(ProbeAggDev Q, uint32_t nparts
       constexpr int u = decltype(I)::value;
       const uint64_t r = rb + (uint64_t)u * blockDim.x;
       const bool ok = r < r1;
-      v[u] = ok ? rec_val[r] : 0;
-      g[u] = ok ? rec_grp[r] : 0;
+      const Rec12 x = recs[ok ? r : r0];  // r0 < r1 inside the loop
+      v[u] = (int64_t)(((uint64_t)x.hi << 32) | x.lo);
+      g[u] = ok ? x.grp : 0;
     });
     static_for<U>([&](auto I) {
       constexpr int u = decltype(I)::value;
@@ -1106,19 +1112,18 @@ int32_t tgp_hash_probe_agg(tgp_ctx* ctx, const tgp_join_agg_query* q, const tgp_
              ctx->tune_join_partition.load() == 0) {
     TGP_REQUIRE(n < (1ull << 32),
                 "tgp_hash_probe_agg: a partitioned call takes fewer than 2^32 probe rows");
-    // scratch: mat[(nparts + 1) * G + 1] | vals[n] | flagged local groups[n]
+    // scratch: mat[(nparts + 1) * G + 1] | records[n]
     const uint64_t G = (uint64_t)persistent_grid(ctx, 8) < (n + kPartTile - 1) / kPartTile
                            ? (uint64_t)persistent_grid(ctx, 8)
                            : (n + kPartTile - 1) / kPartTile;
     const uint64_t mat_n = (nparts + 1) * G + 1;
     const uint64_t nsb = (mat_n - 1 + kScanChunk - 1) / kScanChunk + 1;  // scan block sums
     const uint64_t head = (8 * (mat_n + nsb) + 255) & ~255ull;
-    const uint64_t bytes = head + 8 * n + 2 * n + 256;
+    const uint64_t bytes = head + sizeof(Rec12) * n + 256;
     void* scr = nullptr;
     TGP_TRY(get_scratch(ctx, s, bytes, &scr));
     auto* mat = static_cast<unsigned long long*>(scr);
-    auto* rec_val = reinterpret_cast<long long*>(static_cast<uint8_t*>(scr) + head);
-    auto* rec_grp = reinterpret_cast<uint16_t*>(rec_val + n);
+    auto* recs = reinterpret_cast<Rec12*>(static_cast<uint8_t*>(scr) + head);
     const uint32_t np = (uint32_t)nparts;
     // the aggregation adds to the result (several-match rows arrive first)
     {
@@ -1132,8 +1137,8 @@ int32_t tgp_hash_probe_agg(tgp_ctx* ctx, const tgp_join_agg_query* q, const tgp_
     TGP_TRY(scan_u64(s, reinterpret_cast<uint64_t*>(mat), mat_n - 1,
                      reinterpret_cast<uint64_t*>(mat + mat_n)));
     const uint32_t lds = (np + 1) * 12 + 16;
-    k_part_scatter<<<(unsigned)G, 256, lds, s>>>(q->table->dev(), Q, np, mat, rec_val, rec_grp);
-    k_part_agg<<<np + 1, 256, 0, s>>>(Q, np, G, mat, rec_val, rec_grp);
+    k_part_scatter<<<(unsigned)G, 256, lds, s>>>(q->table->dev(), Q, np, mat, recs);
+    k_part_agg<<<np + 1, 1024, 0, s>>>(Q, np, G, mat, recs);
   } else {
     TGP_REQUIRE(q->row_end - q->row_begin < (1ull << 32),
                 "tgp_hash_probe_agg: a global-table call takes fewer than 2^32 probe rows");
