@@ -93,3 +93,53 @@ def test_gather_rejects_rows_outside_the_column(ctx):
     finally:
         rows.free()
         col.free()
+
+
+@pytest.mark.parametrize("kind", ["full_range", "all_equal", "few_values"])
+def test_order_by_limit_top_k_vs_oracle(ctx, kind):
+    """LIMIT up to 4096 takes the radix-select path (LIMIT 4097 the full sort):
+    the first k rows of the stable (key, scan order) sort, ASC and DESC, with
+    every key digit in play, with all keys equal (scan order decides) and with
+    heavy ties."""
+    rng = np.random.default_rng({"full_range": 1, "all_equal": 2, "few_values": 3}[kind])
+    n = 1_500_007
+    if kind == "full_range":
+        host = rng.integers(-2**63, 2**63 - 1, n, dtype=np.int64, endpoint=True)
+        host[::99_991] = np.iinfo(np.int64).min
+        host[1::99_991] = np.iinfo(np.int64).max
+    elif kind == "all_equal":
+        host = np.full(n, -7, np.int64)
+    else:
+        host = rng.integers(0, 5, n).astype(np.int64)
+    col = dev(ctx, host)
+    try:
+        for b, e, use_bm in ((0, n, False), (11, n - 5, True)):
+            words = None
+            if use_bm:
+                words = rng.integers(0, 2**63, (e - b + 63) // 64).astype(np.uint64)
+            for limit in (1, 37, 4095, 4096, 4097):
+                for desc in (False, True):
+                    got = run_select(ctx, b, e, words, col, desc, limit)
+                    want = B.select_rows(b, e, words, host, desc, limit)
+                    assert np.array_equal(got, want), (kind, b, e, use_bm, limit, desc)
+    finally:
+        col.free()
+
+
+def test_top_k_unsigned_and_narrow_columns(ctx):
+    """u64 cells above 2^63 and narrow signed columns (sign-extended cells) in
+    the top-k path."""
+    rng = np.random.default_rng(4)
+    n = 300_001
+    for host in (rng.integers(0, 2**64 - 1, n, dtype=np.uint64, endpoint=True),
+                 rng.integers(-128, 127, n, dtype=np.int8, endpoint=True),
+                 rng.integers(0, 65535, n, dtype=np.uint16, endpoint=True)):
+        col = dev(ctx, host)
+        try:
+            for desc in (False, True):
+                for limit in (3, 2000):
+                    got = run_select(ctx, 0, n, None, col, desc, limit)
+                    want = B.select_rows(0, n, None, host, desc, limit)
+                    assert np.array_equal(got, want), (host.dtype, desc, limit)
+        finally:
+            col.free()

@@ -42,22 +42,23 @@ int32_t check_column(const tgp_column* c, const char* what) {
   return TGP_OK;
 }
 
-int32_t get_scratch(tgp_ctx* ctx, hipStream_t stream, size_t bytes, void** out) {
+int32_t get_scratch(tgp_ctx* ctx, hipStream_t stream, size_t bytes, void** out, int slot) {
+  TGP_REQUIRE(slot >= 0 && slot < kScratchSlots, "get_scratch: slot %d", slot);
   std::lock_guard<std::mutex> lk(ctx->mu);
   Scratch& s = ctx->scratch[stream];
-  if (s.bytes < bytes) {
-    if (s.ptr) {
+  if (s.bytes[slot] < bytes) {
+    if (s.ptr[slot]) {
       // The previous buffer may still be read by work queued on this stream.
       TGP_HIP(hipStreamSynchronize(stream));
-      TGP_HIP(hipFree(s.ptr));
-      s.ptr = nullptr;
-      s.bytes = 0;
+      TGP_HIP(hipFree(s.ptr[slot]));
+      s.ptr[slot] = nullptr;
+      s.bytes[slot] = 0;
     }
     size_t want = bytes < (1u << 20) ? (1u << 20) : bytes + bytes / 4;
-    TGP_HIP(hipMalloc(&s.ptr, want));
-    s.bytes = want;
+    TGP_HIP(hipMalloc(&s.ptr[slot], want));
+    s.bytes[slot] = want;
   }
-  *out = s.ptr;
+  *out = s.ptr[slot];
   return TGP_OK;
 }
 
@@ -210,9 +211,9 @@ int32_t tgp_ctx_destroy(tgp_ctx* ctx) {
   // Streams the caller created may already be destroyed: wait for the device,
   // not for the streams the scratch map remembers.
   (void)hipDeviceSynchronize();
-  for (auto& kv : ctx->scratch) {
-    if (kv.second.ptr) (void)hipFree(kv.second.ptr);
-  }
+  for (auto& kv : ctx->scratch)
+    for (void* p : kv.second.ptr)
+      if (p) (void)hipFree(p);
   for (auto& kv : ctx->allocs) (void)hipFree(kv.first);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -243,7 +244,8 @@ int32_t tgp_stream_destroy(tgp_ctx* ctx, void* stream) {
     std::lock_guard<std::mutex> lk(ctx->mu);
     auto it = ctx->scratch.find(s);
     if (it != ctx->scratch.end()) {
-      if (it->second.ptr) (void)hipFree(it->second.ptr);
+      for (void* p : it->second.ptr)
+        if (p) (void)hipFree(p);
       ctx->scratch.erase(it);
     }
   }

@@ -220,6 +220,46 @@ __global__ void __launch_bounds__(256) k_minmax(const uint8_t* __restrict__ data
 
 using namespace tgp;
 
+namespace tgp {
+
+int32_t compact_begin(tgp_ctx* ctx, hipStream_t s, const uint64_t* d_bitmap, uint64_t nbits,
+                      uint64_t* total) {
+  *total = 0;
+  if (nbits == 0) return TGP_OK;
+  const uint64_t nw = (nbits + 63) / 64;
+  const uint64_t nblocks = (nw + kCompactWords - 1) / kCompactWords;
+  void* scratch = nullptr;
+  TGP_TRY(get_scratch(ctx, s, (nblocks + 1) * 8, &scratch));
+  uint64_t* bc = static_cast<uint64_t*>(scratch);
+  k_compact_count<<<(unsigned)nblocks, 256, 0, s>>>(d_bitmap, nbits, bc);
+  TGP_HIP(hipGetLastError());
+  k_compact_scan<<<1, 1024, 0, s>>>(bc, nblocks);
+  TGP_HIP(hipGetLastError());
+  TGP_HIP(hipMemcpyAsync(total, bc + nblocks, 8, hipMemcpyDeviceToHost, s));
+  TGP_HIP(hipStreamSynchronize(s));
+  return TGP_OK;
+}
+
+int32_t compact_finish(tgp_ctx* ctx, hipStream_t s, const uint64_t* d_bitmap, uint64_t nbits,
+                       uint64_t base, int32_t index_bytes, void* d_out) {
+  if (nbits == 0) return TGP_OK;
+  const uint64_t nw = (nbits + 63) / 64;
+  const uint64_t nblocks = (nw + kCompactWords - 1) / kCompactWords;
+  void* scratch = nullptr;  // the block offsets compact_begin left (same size: same buffer)
+  TGP_TRY(get_scratch(ctx, s, (nblocks + 1) * 8, &scratch));
+  uint64_t* bc = static_cast<uint64_t*>(scratch);
+  if (index_bytes == 8)
+    k_compact_scatter<unsigned long long><<<(unsigned)nblocks, 256, 0, s>>>(
+        d_bitmap, nbits, base, bc, static_cast<unsigned long long*>(d_out));
+  else
+    k_compact_scatter<uint32_t><<<(unsigned)nblocks, 256, 0, s>>>(d_bitmap, nbits, base, bc,
+                                                                  static_cast<uint32_t*>(d_out));
+  TGP_HIP(hipGetLastError());
+  return TGP_OK;
+}
+
+}  // namespace tgp
+
 extern "C" {
 
 uint64_t tgp_bitmap_words(uint64_t nbits) { return (nbits + 63) / 64; }
@@ -311,18 +351,8 @@ int32_t tgp_bitmap_to_indices_sync(tgp_ctx* ctx, const uint64_t* d_bitmap, uint6
     TGP_REQUIRE(base + nbits <= (1ull << 32), "u32 indices cannot hold rows up to %llu",
                 (unsigned long long)(base + nbits));
   hipStream_t s = pick_stream(ctx, stream);
-  const uint64_t nw = (nbits + 63) / 64;
-  const uint64_t nblocks = (nw + kCompactWords - 1) / kCompactWords;
-  void* scratch = nullptr;
-  TGP_TRY(get_scratch(ctx, s, (nblocks + 1) * 8, &scratch));
-  uint64_t* bc = static_cast<uint64_t*>(scratch);
-  k_compact_count<<<(unsigned)nblocks, 256, 0, s>>>(d_bitmap, nbits, bc);
-  TGP_HIP(hipGetLastError());
-  k_compact_scan<<<1, 1024, 0, s>>>(bc, nblocks);
-  TGP_HIP(hipGetLastError());
   uint64_t total = 0;
-  TGP_HIP(hipMemcpyAsync(&total, bc + nblocks, 8, hipMemcpyDeviceToHost, s));
-  TGP_HIP(hipStreamSynchronize(s));
+  TGP_TRY(compact_begin(ctx, s, d_bitmap, nbits, &total));
   *h_count = total;
   if (total > out_capacity) {
     set_error("compaction needs %llu entries, capacity %llu", (unsigned long long)total,
@@ -331,13 +361,7 @@ int32_t tgp_bitmap_to_indices_sync(tgp_ctx* ctx, const uint64_t* d_bitmap, uint6
   }
   if (total == 0) return TGP_OK;
   TGP_REQUIRE(d_out, "tgp_bitmap_to_indices_sync: null output");
-  if (index_bytes == 8)
-    k_compact_scatter<unsigned long long><<<(unsigned)nblocks, 256, 0, s>>>(
-        d_bitmap, nbits, base, bc, static_cast<unsigned long long*>(d_out));
-  else
-    k_compact_scatter<uint32_t><<<(unsigned)nblocks, 256, 0, s>>>(d_bitmap, nbits, base, bc,
-                                                                  static_cast<uint32_t*>(d_out));
-  TGP_HIP(hipGetLastError());
+  TGP_TRY(compact_finish(ctx, s, d_bitmap, nbits, base, index_bytes, d_out));
   TGP_HIP(hipStreamSynchronize(s));
   return TGP_OK;
 }

@@ -6,10 +6,17 @@
 //   :524-618  SelectColumn, :620-733 SelectMulti: selected rows in scan order,
 //             stably sorted by the ORDER BY column's u64 cells (`va.cmp(&vb)`,
 //             or `vb.cmp(&va)` for DESC), first LIMIT, values gathered as cells
-// Selected rows come from the compaction of the selection bitmap
-// (tgp_bitmap_to_indices_sync).  ORDER BY sorts (key, row) pairs with a stable
-// LSD radix sort (hipCUB): key = the cell for ASC and its complement for DESC,
-// so equal keys keep scan order in both directions, as a stable sort_by does.
+// Selected rows come from the compaction of the selection bitmap.  The order
+// is that of (key, position): key = the cell for ASC and its complement for
+// DESC, position = the row's place in scan order, so equal keys keep scan order
+// in both directions, as a stable sort_by does.  ORDER BY without a small LIMIT
+// sorts (key, row) pairs with a stable LSD radix sort (hipCUB).  With LIMIT k
+// <= kTopKMax the k first rows are found by a radix select over the 96-bit
+// (key, position) value (9 passes of 11-bit digits: a histogram of the rows
+// that match the prefix chosen so far, then one workgroup picks the digit that
+// holds the k-th row), collected, and sorted in LDS by one workgroup.  The
+// positions make every value distinct, so exactly k rows are <= the k-th.
+// Buffers live in the stream's outer scratch region: no allocation per call.
 #include "tgp_internal.h"
 
 #include <hipcub/hipcub.hpp>
@@ -33,15 +40,17 @@ __global__ void __launch_bounds__(256) k_iota(uint64_t* __restrict__ out, uint64
   for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += stride) out[i] = base + i;
 }
 
-// keys[i] = cell of col at rows[i], complemented for DESC (rows are in range:
-// they come from the compaction of a bitmap over the column's rows).
+// keys[i] = cell of col at rows[i] (at base + i without a row list),
+// complemented for DESC (rows are in range: they come from the compaction of a
+// bitmap over the column's rows).
 template <int DT>
 __global__ void __launch_bounds__(256) k_sort_keys(const uint8_t* __restrict__ col,
-                                                   const uint64_t* __restrict__ rows, uint64_t n,
-                                                   uint64_t flip, uint64_t* __restrict__ keys) {
+                                                   const uint64_t* __restrict__ rows,
+                                                   uint64_t base, uint64_t n, uint64_t flip,
+                                                   uint64_t* __restrict__ keys) {
   const uint64_t stride = (uint64_t)gridDim.x * 256;
   for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += stride)
-    keys[i] = cell_of<DT>(col, rows[i]) ^ flip;
+    keys[i] = cell_of<DT>(col, rows ? rows[i] : base + i) ^ flip;
 }
 
 // out[i] = col[rows[i]] as a u64 cell (CELLS) or at the column's width; a row
@@ -78,10 +87,10 @@ static uint32_t grid_of(uint64_t n) {
 }
 
 template <int DT>
-static void launch_keys(hipStream_t st, const tgp_column* c, const uint64_t* rows, uint64_t n,
-                        uint64_t flip, uint64_t* keys) {
+static void launch_keys(hipStream_t st, const tgp_column* c, const uint64_t* rows, uint64_t base,
+                        uint64_t n, uint64_t flip, uint64_t* keys) {
   hipLaunchKernelGGL(k_sort_keys<DT>, dim3(grid_of(n)), dim3(256), 0, st,
-                     static_cast<const uint8_t*>(c->data), rows, n, flip, keys);
+                     static_cast<const uint8_t*>(c->data), rows, base, n, flip, keys);
 }
 
 template <int DT>
@@ -108,20 +117,170 @@ static void launch_gather(hipStream_t st, const tgp_column* c, const uint64_t* r
     default: F<TGP_U64>(__VA_ARGS__); break;      \
   }
 
-// Device buffers of one call, released on every return path.
-struct CallBuffers {
-  tgp_ctx* ctx;
-  std::vector<void*> p;
-  explicit CallBuffers(tgp_ctx* c) : ctx(c) {}
-  int32_t get(uint64_t bytes, void** out) {
-    TGP_TRY(hbm_alloc(ctx, bytes ? bytes : 8, out));
-    p.push_back(*out);
-    return TGP_OK;
-  }
-  ~CallBuffers() {
-    for (void* q : p) (void)hbm_free(ctx, q);
-  }
+// ---- top-k: radix select over (key, position) ---------------------------------------
+constexpr uint64_t kTopKMax = 4096;  // LIMIT up to this: select + one-workgroup sort
+constexpr int kRsBits = 11;
+constexpr int kRsBins = 1 << kRsBits;
+constexpr int kRsPasses = 9;  // 6 over the 64 key bits, 3 over the 32 position bits
+
+struct RsState {
+  unsigned long long prefk, maskk;  // key bits chosen so far
+  uint32_t prefp, maskp;            // position bits chosen so far
+  unsigned long long remaining;     // rank of the k-th row among those matching the prefix
+  unsigned long long taken;         // collect counter
+  uint32_t hist[kRsBins];
 };
+
+// Digit (shift, width) of pass p: key bits 63..53, 52..42, 41..31, 30..20,
+// 19..9, 8..0, then position bits 31..21, 20..10, 9..0.
+__device__ __forceinline__ void rs_digit(int p, int* shift, int* width, bool* on_key) {
+  *on_key = p < 6;
+  if (p < 6) {
+    *shift = p < 5 ? 53 - kRsBits * p : 0;
+    *width = p < 5 ? kRsBits : 9;
+  } else {
+    const int q = p - 6;
+    *shift = q < 2 ? 21 - kRsBits * q : 0;
+    *width = q < 2 ? kRsBits : 10;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_rs_init(RsState* st, uint64_t k) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    st->prefk = st->maskk = 0;
+    st->prefp = st->maskp = 0;
+    st->remaining = k;
+    st->taken = 0;
+  }
+  for (int b = threadIdx.x; b < kRsBins; b += blockDim.x) st->hist[b] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_rs_hist(const uint64_t* __restrict__ keys, uint64_t n,
+                                                 RsState* st, int pass) {
+  __shared__ uint32_t h[kRsBins];
+  for (int b = threadIdx.x; b < kRsBins; b += blockDim.x) h[b] = 0;
+  __syncthreads();
+  int shift, width;
+  bool on_key;
+  rs_digit(pass, &shift, &width, &on_key);
+  const unsigned long long prefk = st->prefk, maskk = st->maskk;
+  const uint32_t prefp = st->prefp, maskp = st->maskp;
+  const uint64_t dmask = (1ull << width) - 1;
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += stride) {
+    const uint64_t k = keys[i];
+    if ((k & maskk) != prefk) continue;
+    if (on_key) {
+      atomicAdd(&h[(k >> shift) & dmask], 1u);
+    } else if (((uint32_t)i & maskp) == prefp) {
+      atomicAdd(&h[((uint32_t)i >> shift) & dmask], 1u);
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < kRsBins; b += blockDim.x)
+    if (h[b]) atomicAdd(&st->hist[b], h[b]);
+}
+
+// One workgroup: the digit whose bin holds the remaining-th row; clears the bins.
+__global__ void __launch_bounds__(256) k_rs_pick(RsState* st, int pass) {
+  constexpr int per = kRsBins / 256;
+  __shared__ unsigned long long wsum[4];
+  uint64_t local[per];
+  uint64_t mine = 0;
+  for (int j = 0; j < per; ++j) {
+    local[j] = st->hist[threadIdx.x * per + j];
+    mine += local[j];
+  }
+  // exclusive scan of the per-thread sums
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t incl = mine;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint64_t y = __shfl_up(incl, off);
+    if (lane >= (uint32_t)off) incl += y;
+  }
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  uint64_t cum = incl - mine;
+  for (uint32_t w = 0; w < wave; ++w) cum += wsum[w];
+  const uint64_t rem = st->remaining;
+  int shift, width;
+  bool on_key;
+  rs_digit(pass, &shift, &width, &on_key);
+  for (int j = 0; j < per; ++j) {
+    if (cum < rem && rem <= cum + local[j]) {
+      const uint64_t d = (uint64_t)(threadIdx.x * per + j);
+      if (on_key) {
+        st->prefk |= d << shift;
+        st->maskk |= ((1ull << width) - 1) << shift;
+      } else {
+        st->prefp |= (uint32_t)d << shift;
+        st->maskp |= (uint32_t)(((1ull << width) - 1) << shift);
+      }
+      st->remaining = rem - cum;
+    }
+    cum += local[j];
+  }
+  __syncthreads();
+  for (int j = 0; j < per; ++j) st->hist[threadIdx.x * per + j] = 0;
+}
+
+// Rows whose (key, position) is <= the k-th: exactly k of them, in any order
+// (row = rows[i], or base + i without a row list).
+__global__ void __launch_bounds__(256) k_rs_collect(const uint64_t* __restrict__ keys,
+                                                    const uint64_t* __restrict__ rows,
+                                                    uint64_t base, uint64_t n, RsState* st,
+                                                    uint64_t* out_keys, uint64_t* out_rows,
+                                                    uint64_t k) {
+  const unsigned long long tk = st->prefk;
+  const uint32_t tp = st->prefp;
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += stride) {
+    const uint64_t key = keys[i];
+    if (key < tk || (key == tk && (uint32_t)i <= tp)) {
+      const unsigned long long at = atomicAdd(&st->taken, 1ull);
+      if (at < k) {  // exactly k qualify; the bound keeps a wrong count in range
+        out_keys[at] = key;
+        out_rows[at] = rows ? rows[i] : base + i;
+      }
+    }
+  }
+}
+
+// One workgroup of 1024 sorts m <= kTopKMax (key, row) pairs (rows distinct)
+// with a bitonic network in LDS and writes the rows in order.
+__global__ void __launch_bounds__(1024) k_topk_sort(const uint64_t* keys, const uint64_t* rows,
+                                                    uint64_t m, uint64_t* out_rows) {
+  __shared__ unsigned long long sk[kTopKMax], sr[kTopKMax];
+  uint32_t M = 1;
+  while (M < m) M <<= 1;
+  for (uint32_t i = threadIdx.x; i < M; i += blockDim.x) {
+    sk[i] = i < m ? keys[i] : ~0ull;
+    sr[i] = i < m ? rows[i] : ~0ull;
+  }
+  __syncthreads();
+  for (uint32_t size = 2; size <= M; size <<= 1) {
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      for (uint32_t i = threadIdx.x; i < M; i += blockDim.x) {
+        const uint32_t j = i ^ stride;
+        if (j > i) {
+          const bool up = (i & size) == 0;
+          const bool gt = sk[i] > sk[j] || (sk[i] == sk[j] && sr[i] > sr[j]);
+          if (gt == up) {
+            const unsigned long long tk = sk[i], tr = sr[i];
+            sk[i] = sk[j];
+            sr[i] = sr[j];
+            sk[j] = tk;
+            sr[j] = tr;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) out_rows[i] = sr[i];
+}
+
+static_assert(sizeof(RsState) % 8 == 0, "state keeps the buffers after it aligned");
 
 }  // namespace tgp
 
@@ -146,42 +305,66 @@ int32_t tgp_select_rows(tgp_ctx* ctx, uint64_t row_begin, uint64_t row_end,
   const uint64_t n = row_end - row_begin;
   TGP_REQUIRE(n < (1ull << 31), "tgp_select_rows: %llu rows above the sort's 2^31",
               (unsigned long long)n);
-  CallBuffers buf(ctx);
-  void* p = nullptr;
-  TGP_TRY(buf.get(n * 8, &p));
-  uint64_t* rows = static_cast<uint64_t*>(p);
   uint64_t cnt = n;
-  if (d_sel_bitmap) {
-    if (n) TGP_TRY(tgp_bitmap_to_indices_sync(ctx, d_sel_bitmap, n, row_begin, 8, rows, n, &cnt,
-                                              st));
-    else cnt = 0;
-  } else if (n) {
-    hipLaunchKernelGGL(k_iota, dim3(grid_of(n)), dim3(256), 0, st, rows, n, row_begin);
-    TGP_HIP(hipGetLastError());
-  }
+  if (d_sel_bitmap) TGP_TRY(compact_begin(ctx, st, d_sel_bitmap, n, &cnt));
   const uint64_t take = limit < cnt ? limit : cnt;
   TGP_REQUIRE(take <= capacity, "tgp_select_rows: %llu rows, capacity %llu",
               (unsigned long long)take, (unsigned long long)capacity);
-  if (take == 0) {
-    TGP_HIP(hipStreamSynchronize(st));
-    return TGP_OK;
-  }
+  if (take == 0) return TGP_OK;
   TGP_REQUIRE(d_rows, "tgp_select_rows: null output");
+  const bool sorted = order_col && cnt > 1;
+  const bool topk = sorted && take <= kTopKMax && take < cnt;
+  // outer scratch: rows[cnt] | keys[cnt] | (full sort) keys_out, rows_out, sort temp
+  //                                       | (top-k) state, keys[take], rows[take]
+  size_t temp_bytes = 0;
+  if (sorted && !topk)
+    TGP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, (uint64_t*)nullptr,
+                                               (uint64_t*)nullptr, (uint64_t*)nullptr,
+                                               (uint64_t*)nullptr, (int)cnt, 0, 64, st));
+  const uint64_t extra = !sorted ? 0
+                         : topk  ? sizeof(RsState) + 16 * take
+                                 : 16 * cnt + ((temp_bytes + 255) & ~(size_t)255);
+  void* p = nullptr;
+  TGP_TRY(get_scratch(ctx, st, 16 * cnt + extra + 256, &p, 1));
+  uint64_t* rows = static_cast<uint64_t*>(p);
+  uint64_t* keys = rows + cnt;
+  // the row list: compacted from the bitmap, or implicit (row_begin + i) for
+  // the top-k path without a bitmap
+  const bool implicit_rows = !d_sel_bitmap && topk;
+  if (d_sel_bitmap) {
+    TGP_TRY(compact_finish(ctx, st, d_sel_bitmap, n, row_begin, 8, rows));
+  } else if (!implicit_rows) {
+    hipLaunchKernelGGL(k_iota, dim3(grid_of(n)), dim3(256), 0, st, rows, n, row_begin);
+    TGP_HIP(hipGetLastError());
+  }
   const uint64_t* first = rows;
-  if (order_col && cnt > 1) {
-    void* q = nullptr;
-    TGP_TRY(buf.get(cnt * 24, &q));
-    uint64_t* keys = static_cast<uint64_t*>(q);
-    uint64_t* keys_out = keys + cnt;
-    uint64_t* rows_out = keys + 2 * cnt;
-    TGP_BY_DTYPE(order_col->dtype, launch_keys, st, order_col, rows, cnt,
+  const uint64_t* row_list = implicit_rows ? nullptr : rows;
+  if (sorted) {
+    TGP_BY_DTYPE(order_col->dtype, launch_keys, st, order_col, row_list, row_begin, cnt,
                  descending ? ~0ull : 0ull, keys);
     TGP_HIP(hipGetLastError());
-    size_t temp_bytes = 0;
-    TGP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, keys, keys_out, rows, rows_out,
-                                               (int)cnt, 0, 64, st));
-    void* temp = nullptr;
-    TGP_TRY(buf.get(temp_bytes, &temp));
+    if (topk) {
+      auto* state = reinterpret_cast<RsState*>(keys + cnt);
+      auto* tkeys = reinterpret_cast<uint64_t*>(state + 1);
+      uint64_t* trows = tkeys + take;
+      hipLaunchKernelGGL(k_rs_init, dim3(1), dim3(256), 0, st, state, take);
+      const uint32_t g = grid_of(cnt);
+      for (int pass = 0; pass < kRsPasses; ++pass) {
+        hipLaunchKernelGGL(k_rs_hist, dim3(g), dim3(256), 0, st, keys, cnt, state, pass);
+        hipLaunchKernelGGL(k_rs_pick, dim3(1), dim3(256), 0, st, state, pass);
+      }
+      hipLaunchKernelGGL(k_rs_collect, dim3(g), dim3(256), 0, st, keys, row_list, row_begin, cnt,
+                         state, tkeys, trows, take);
+      hipLaunchKernelGGL(k_topk_sort, dim3(1), dim3(1024), 0, st, tkeys, trows, take, d_rows);
+      TGP_HIP(hipGetLastError());
+      TGP_HIP(hipStreamSynchronize(st));
+      *h_count = take;
+      return TGP_OK;
+    }
+    uint64_t* keys_out = keys + cnt;
+    uint64_t* rows_out = keys_out + cnt;
+    void* temp = reinterpret_cast<void*>(
+        (reinterpret_cast<uintptr_t>(rows_out + cnt) + 255) & ~(uintptr_t)255);
     TGP_HIP(hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys, keys_out, rows, rows_out,
                                                (int)cnt, 0, 64, st));
     first = rows_out;

@@ -74,9 +74,12 @@ inline bool dtype_signed(int32_t dt) {
 }
 
 // ---- scratch arena: device memory reused by calls on one stream ----------------
+// Two regions per stream: slot 0 for a call's own work, slot 1 for calls that
+// run other library calls (which use slot 0) while holding buffers of their own.
+constexpr int kScratchSlots = 2;
 struct Scratch {
-  void* ptr = nullptr;
-  size_t bytes = 0;
+  void* ptr[kScratchSlots] = {nullptr, nullptr};
+  size_t bytes[kScratchSlots] = {0, 0};
 };
 
 }  // namespace tgp
@@ -109,8 +112,17 @@ inline hipStream_t pick_stream(tgp_ctx* ctx, void* s) {
 // Consumes one injected failure, if any (host-side failure tests).
 int32_t check_injected(tgp_ctx* ctx);
 // Device scratch of at least `bytes` for `stream` (grown on demand; not accounted
-// against the HBM budget, bounded by the largest call).
-int32_t get_scratch(tgp_ctx* ctx, hipStream_t stream, size_t bytes, void** out);
+// against the HBM budget, bounded by the largest call).  `slot` 1 is the outer
+// region of composite calls.
+int32_t get_scratch(tgp_ctx* ctx, hipStream_t stream, size_t bytes, void** out, int slot = 0);
+// Compaction of a bitmap to ascending row indices in two steps on one stream:
+// compact_begin counts the set bits (synchronous; the block offsets stay in the
+// stream's slot-0 scratch) and compact_finish writes base + i for every set bit
+// i (asynchronous).  Nothing may use the stream's slot-0 scratch in between.
+int32_t compact_begin(tgp_ctx* ctx, hipStream_t s, const uint64_t* d_bitmap, uint64_t nbits,
+                      uint64_t* total);
+int32_t compact_finish(tgp_ctx* ctx, hipStream_t s, const uint64_t* d_bitmap, uint64_t nbits,
+                       uint64_t base, int32_t index_bytes, void* d_out);
 // Tier-accounted allocation used by library objects (hash tables).
 int32_t hbm_alloc(tgp_ctx* ctx, uint64_t bytes, void** out);
 int32_t hbm_free(tgp_ctx* ctx, void* p);
