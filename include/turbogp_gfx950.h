@@ -212,8 +212,12 @@ enum tgp_tune_key {
                                n > 0: chunks of n slices (<= 4096) taken from a counter */
   TGP_TUNE_SLICES_PER_WAVE = 5, /* fewest 256-row slices per wave of the specialised
                                    kernels before their grid shrinks (0 = default 2) */
-  TGP_TUNE_JOIN_ATOMICS = 6     /* join GROUP BY a probe column with more groups than LDS
+  TGP_TUNE_JOIN_ATOMICS = 6,    /* join GROUP BY a probe column with more groups than LDS
                                    holds: 0 (default) radix-partitioned, 1 global atomics */
+  TGP_TUNE_JOIN_PACK_BITS = 7   /* multi-column join tables built from now on keep only the
+                                   low n bits (1..63) of the packed key, so that distinct
+                                   keys collide and the full-key verification is exercised
+                                   (tests); 0 (default): all 64 bits */
 };
 int32_t tgp_tune_set(tgp_ctx* ctx, int32_t key, int64_t value);
 int32_t tgp_tune_get(tgp_ctx* ctx, int32_t key, int64_t* value);
@@ -449,6 +453,40 @@ int32_t tgp_hash_probe_pairs_sync(tgp_ctx* ctx, const tgp_hash_table* t,
                                   const tgp_column* probe_keys, uint64_t row_begin,
                                   uint64_t row_end, uint64_t* d_probe_idx, uint32_t* d_build_payload,
                                   uint64_t capacity, uint64_t* h_npairs, void* stream);
+
+/* ---- multi-column keys and LEFT joins (exec/join.rs:51-160) ----
+ * A table over 2..TGP_MAX_JOIN_KEYS key columns is keyed by a 64-bit packing of
+ * each row's cells, which is not collision-free; every candidate match is then
+ * checked against all key columns, as hash_join does after probe_all
+ * (exec/join.rs:51-66 pack_key, :102-111).  The table keeps the build key
+ * column descriptors: those columns must stay allocated while it is used.
+ * Cells are compared as 64-bit values (narrow columns widened by their
+ * signedness), the reference's u64 cells.  Count and aggregate probes take
+ * single-key tables only. */
+#define TGP_MAX_JOIN_KEYS 4
+#define TGP_JOIN_NO_MATCH 0xFFFFFFFFu /* LEFT join pair of a probe row without a match */
+enum tgp_join_type {
+  TGP_JOIN_INNER = 0, /* JoinType::Inner */
+  TGP_JOIN_LEFT = 1   /* JoinType::Left: a probe row without a match gives one pair
+                         (row, TGP_JOIN_NO_MATCH); the reference fills the build
+                         columns of that row with 0 */
+};
+int32_t tgp_hash_build_multi(tgp_ctx* ctx, const tgp_column* keys, int32_t nkeys,
+                             const tgp_column* payload, tgp_hash_table** out, void* stream);
+/* Pairs of a join over nkeys probe key columns (nkeys = the table's key count)
+ * in ascending probe-row order, matches of one probe row in ascending payload
+ * order; join_type TGP_JOIN_INNER or TGP_JOIN_LEFT.  Otherwise as
+ * tgp_hash_probe_pairs_sync (exec/join.rs:68-160 hash_join). */
+int32_t tgp_hash_join_pairs_sync(tgp_ctx* ctx, const tgp_hash_table* t,
+                                 const tgp_column* probe_keys, int32_t nkeys, int32_t join_type,
+                                 uint64_t row_begin, uint64_t row_end, uint64_t* d_probe_idx,
+                                 uint32_t* d_build_payload, uint64_t capacity, uint64_t* h_npairs,
+                                 void* stream);
+/* Semi-join marks over nkeys probe key columns (exec/join.rs:162-199
+ * semi_join / anti_join: anti = tgp_bitmap_not of the marks). */
+int32_t tgp_hash_join_mark(tgp_ctx* ctx, const tgp_hash_table* t, const tgp_column* probe_keys,
+                           int32_t nkeys, uint64_t row_begin, uint64_t row_end, uint64_t* d_bitmap,
+                           void* stream);
 
 enum tgp_join_group_by {
   TGP_JOIN_GROUP_BUILD_PAYLOAD = 0, /* GROUP BY the inline build payload */

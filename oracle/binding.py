@@ -243,6 +243,39 @@ def join_pairs(build_keys, probe_keys, capacity=None):
     return op[:min(n, cap)], ob[:min(n, cap)], n
 
 
+def hash_join(build_cols, probe_cols, join_type=0, row_begin=0, row_end=None,
+              build_payload=None):
+    """exec/join.rs:68-160 hash_join over one or more key columns (INNER = 0,
+    LEFT = 1): for every probe row in order, the build rows whose cells equal the
+    probe row's cells in every key column (the reference verifies the full key
+    after probing, :102-111), in ascending build row; under LEFT a row without a
+    match gives one pair with build 0xFFFFFFFF (the reference fills that row's
+    build columns with 0).  Returns (probe rows u64, build payloads u32), the
+    payload being the build row unless `build_payload` is given."""
+    bcs = [cells(c) for c in (build_cols if isinstance(build_cols, (list, tuple)) else [build_cols])]
+    pcs = [cells(c) for c in (probe_cols if isinstance(probe_cols, (list, tuple)) else [probe_cols])]
+    assert len(bcs) == len(pcs)
+    nb = len(bcs[0])
+    row_end = len(pcs[0]) if row_end is None else row_end
+    table = {}
+    for r, key in enumerate(zip(*[c.tolist() for c in bcs])):
+        table.setdefault(key, []).append(r)
+    pay = None if build_payload is None else np.asarray(build_payload).astype(np.int64).tolist()
+    out_p, out_b = [], []
+    probe_lists = [c[row_begin:row_end].tolist() for c in pcs]
+    for off, key in enumerate(zip(*probe_lists)):
+        rows = table.get(key)
+        if rows:
+            vals = sorted(pay[r] & 0xFFFFFFFF for r in rows) if pay is not None else rows
+            out_p.extend([row_begin + off] * len(vals))
+            out_b.extend(vals)
+        elif join_type == 1:
+            out_p.append(row_begin + off)
+            out_b.append(0xFFFFFFFF)
+    assert nb >= 0
+    return np.array(out_p, np.uint64), np.array(out_b, np.uint32)
+
+
 def xxh3_u64(v):
     return int(lib().orc_xxh3_u64(int(v) % (1 << 64)))
 

@@ -231,3 +231,109 @@ def test_probe_agg_every_table_form(ctx, group_by, ng):
         assert r.n_out_of_range == oor
         assert np.array_equal(r.count, cnt)
         assert [s[0] for s in r.sums()] == sums
+
+
+def test_exec_join_kats_device(ctx):
+    """exec/join.rs:294-353 through the device: inner, left, multi-key, semi,
+    anti and no-match joins (build on the right table, probe with the left)."""
+    def cols(*arrs):
+        return [dev(ctx, np.array(a, np.int64)) for a in arrs]
+
+    keep = []
+    try:
+        (r,), (l,) = cols([2, 3, 4]), cols([1, 2, 3])
+        keep += [r, l]
+        t = HashTable(ctx, r)
+        p, b = t.join_pairs(l)
+        assert list(p) == [1, 2] and list(b) == [0, 1]                       # test_inner_join
+        t.free()
+        (r2,) = cols([2])
+        keep.append(r2)
+        t = HashTable(ctx, r2)
+        p, b = t.join_pairs(l, A.JOIN_LEFT)                                  # test_left_join
+        assert list(p) == [0, 1, 2] and list(b) == [A.JOIN_NO_MATCH, 0, A.JOIN_NO_MATCH]
+        t.free()
+        ra, rb, la, lb = cols([1, 2], [2, 1], [1, 1, 2], [1, 2, 1])          # test_multi_key_join
+        keep += [ra, rb, la, lb]
+        t = HashTable(ctx, [ra, rb])
+        p, b = t.join_pairs([la, lb])
+        assert list(p) == [1, 2] and list(b) == [0, 1]
+        bm = t.join_mark([la, lb])
+        assert list(bm.iter_set_bits()) == [1, 2]
+        bm.free()
+        t.free()
+        rs, ls = cols([2, 4], [1, 2, 3, 4])                                  # test_semi / anti
+        keep += [rs, ls]
+        t = HashTable(ctx, rs)
+        bm = t.join_mark(ls)
+        nb = bm.not_()
+        assert [int(x) + 1 for x in bm.iter_set_bits()] == [2, 4]
+        assert [int(x) + 1 for x in nb.iter_set_bits()] == [1, 3]
+        bm.free()
+        nb.free()
+        t.free()
+        rn, ln = cols([3, 4], [1, 2])                                        # test_no_match
+        keep += [rn, ln]
+        t = HashTable(ctx, rn)
+        assert len(t.join_pairs(ln)[0]) == 0
+        t.free()
+    finally:
+        for c in keep:
+            c.free()
+
+
+@pytest.mark.parametrize("nkeys,pack_bits", [(1, 0), (2, 0), (2, 3), (3, 0), (3, 5), (4, 1)])
+@pytest.mark.parametrize("join_type", [A.JOIN_INNER, A.JOIN_LEFT])
+def test_multi_key_join_vs_oracle(ctx, nkeys, pack_bits, join_type):
+    """Multi-column INNER / LEFT pairs and semi-join marks against the oracle,
+    with mixed key dtypes, duplicate build keys, misses, and (pack_bits > 0) the
+    packed key cut to a few bits so that distinct keys collide and only the
+    full-key check tells them apart."""
+    rng = np.random.default_rng(100 * nkeys + 10 * pack_bits + join_type)
+    nb, npb = 6000, 40_000
+    dtypes = [np.int64, np.int32, np.uint8, np.uint64][:nkeys]
+    bcols = [rng.integers(0, 6, nb).astype(dt) for dt in dtypes]
+    bcols[0][::7] = rng.integers(-50, 50, len(bcols[0][::7]))               # negative cells
+    pick = rng.integers(0, nb, npb)
+    pcols = [c[pick].copy() for c in bcols]
+    pcols[-1][::5] = pcols[-1][::5] + 1                                     # some misses
+    if nkeys >= 2:
+        pcols[1] = pcols[1].astype(np.int64)    # int32 build key against an int64 probe key
+    bdev = [dev(ctx, c) for c in bcols]
+    pdev = [dev(ctx, c) for c in pcols]
+    ctx.tune(A.TUNE_JOIN_PACK_BITS, pack_bits)
+    try:
+        t = HashTable(ctx, bdev if nkeys > 1 else bdev[0])
+    finally:
+        ctx.tune(A.TUNE_JOIN_PACK_BITS, 0)
+    try:
+        b, e = 13, npb - 7
+        p, pay = t.join_pairs(pdev if nkeys > 1 else pdev[0], join_type, b, e)
+        wp, wb = B.hash_join(bcols, pcols, join_type, b, e)
+        assert np.array_equal(p, wp) and np.array_equal(pay, wb), (len(p), len(wp))
+        bm = t.join_mark(pdev if nkeys > 1 else pdev[0], b, e)
+        try:
+            want = np.unique(wp[wb != A.JOIN_NO_MATCH]).astype(np.int64) - b
+            assert np.array_equal(bm.iter_set_bits().astype(np.int64), want)
+        finally:
+            bm.free()
+    finally:
+        t.free()
+        for c in bdev + pdev:
+            c.free()
+
+
+def test_multi_key_table_rejects_single_key_calls(ctx):
+    a, b2 = dev(ctx, np.arange(10, dtype=np.int64)), dev(ctx, np.arange(10, dtype=np.int64))
+    t = HashTable(ctx, [a, b2])
+    try:
+        with pytest.raises(RuntimeError, match="key columns"):
+            t.probe_count(a)
+        with pytest.raises(RuntimeError, match="key columns"):
+            t.join_pairs(a)
+        with pytest.raises(RuntimeError, match="key columns"):
+            t.probe_agg(a, a, A.JOIN_GROUP_NONE)
+    finally:
+        t.free()
+        a.free()
+        b2.free()

@@ -108,3 +108,29 @@ def test_join_agg_vs_brute_force(group_by):
     cnt, sm, oor = B.join_agg(bk, bp, pk, pv, pg, group_by, 0, ng)
     bc, bs, boor = brute_join_agg(bk, bp, pk, pv, pg, group_by, 0, ng)
     assert list(map(int, cnt)) == bc and sm == bs and oor == boor
+
+
+def test_hash_join_oracle_reference_kats():
+    """exec/join.rs:294-353: inner, left, multi-key and no-match joins
+    (hash_join builds on the right table and probes with the left)."""
+    p, b = B.hash_join(np.array([2, 3, 4]), np.array([1, 2, 3]))            # test_inner_join
+    assert len(p) == 2 and [[1, 2, 3][i] for i in p] == [2, 3]
+    p, b = B.hash_join(np.array([2]), np.array([1, 2, 3]), join_type=1)     # test_left_join
+    assert len(p) == 3 and list(p) == [0, 1, 2] and list(b) == [0xFFFFFFFF, 0, 0xFFFFFFFF]
+    la, lb = np.array([1, 1, 2]), np.array([1, 2, 1])                       # test_multi_key_join
+    ra, rb = np.array([1, 2]), np.array([2, 1])
+    p, b = B.hash_join([ra, rb], [la, lb])
+    assert len(p) == 2 and list(p) == [1, 2] and list(b) == [0, 1]
+    p, _ = B.hash_join(np.array([3, 4]), np.array([1, 2]))                   # test_no_match
+    assert len(p) == 0
+
+
+def test_hash_join_oracle_matches_the_c_restatement():
+    """Single-key pairs: the key-tuple dictionary against the C restatement of
+    JoinHashTable (CRC32-tagged chains) on random keys with duplicates."""
+    rng = np.random.default_rng(5)
+    bk = rng.integers(-300, 300, 2000).astype(np.int64)
+    pk = rng.integers(-400, 400, 5000).astype(np.int64)
+    p1, b1 = B.hash_join(bk, pk)
+    p2, b2, n = B.join_pairs(bk, pk)
+    assert n == len(p1) and np.array_equal(p1, p2) and np.array_equal(b1, b2)

@@ -33,7 +33,10 @@ DISTINCT_AUTO, DISTINCT_EXACT, DISTINCT_HLL = 0, 1, 2
 AGG_MINMAX = {"min": AGG_MIN, "max": AGG_MAX, "min_u64": AGG_MIN_U64, "max_u64": AGG_MAX_U64}
 JOIN_GROUP_BUILD_PAYLOAD, JOIN_GROUP_PROBE_COL, JOIN_GROUP_NONE = 0, 1, 2
 TUNE_WG_PER_CU, TUNE_GRID_MAX, TUNE_SPECIALIZE, TUNE_SPEC_KERNEL, TUNE_CHUNK = 0, 1, 2, 3, 4
-TUNE_SLICES_PER_WAVE, TUNE_JOIN_ATOMICS = 5, 6
+TUNE_SLICES_PER_WAVE, TUNE_JOIN_ATOMICS, TUNE_JOIN_PACK_BITS = 5, 6, 7
+JOIN_INNER, JOIN_LEFT = 0, 1
+JOIN_NO_MATCH = 0xFFFFFFFF
+MAX_JOIN_KEYS = 4
 BATCH_PERSISTENT, BATCH_PER_QUERY = 0, 1
 
 MAX_COLS, MAX_PREDS, MAX_AGGS, MAX_FACTORS, MAX_GROUP_PARTS = 8, 8, 6, 3, 2
@@ -236,6 +239,11 @@ SIGNATURES = {
     "tgp_hash_probe_pairs_sync": (I32_, [P, P, C.POINTER(Column), U64_, U64_, P, P, U64_,
                                          C.POINTER(U64_), P]),
     "tgp_hash_probe_agg": (I32_, [P, C.POINTER(JoinAggQuery), C.POINTER(GroupResult), P]),
+    "tgp_hash_build_multi": (I32_, [P, C.POINTER(Column), I32_, C.POINTER(Column),
+                                    C.POINTER(C.c_void_p), P]),
+    "tgp_hash_join_pairs_sync": (I32_, [P, P, C.POINTER(Column), I32_, I32_, U64_, U64_, P, P,
+                                        U64_, C.POINTER(U64_), P]),
+    "tgp_hash_join_mark": (I32_, [P, P, C.POINTER(Column), I32_, U64_, U64_, P, P]),
     "tgp_event_create": (I32_, [P, C.POINTER(P)]),
     "tgp_tune_set": (I32_, [P, I32_, I64_]),
     "tgp_tune_get": (I32_, [P, I32_, C.POINTER(I64_)]),

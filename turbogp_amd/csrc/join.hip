@@ -212,6 +212,43 @@ __device__ __forceinline__ void probe_keys(const JoinTableDev& T, const unsigned
 
 constexpr int kProbeUnroll = 4;
 
+// ---- multi-column keys -------------------------------------------------------------------
+// A table over several key columns is keyed by a 64-bit packing of the row's
+// cells (not collision-free), and every candidate match is verified against
+// all key columns of its build row, as exec/join.rs:51-66 and :102-111 do.
+constexpr int kMaxJoinKeys = 4;
+constexpr uint32_t kNoBuildRow = 0xffffffffu;  // LEFT join: a probe row without a match
+
+struct KeyCols {
+  const void* p[kMaxJoinKeys];
+  int32_t dt[kMaxJoinKeys];
+  int32_t n;
+  uint32_t pack_mask_bits;  // 0: all 64 bits of the packing; b: its low b bits (tests)
+};
+
+__device__ __forceinline__ unsigned long long pack_row(const KeyCols& K, uint64_t i) {
+  uint64_t h = 0x9E3779B97F4A7C15ull * (uint64_t)K.n;
+  for (int j = 0; j < K.n; ++j) h = hmix(h ^ (uint64_t)load_i64(K.p[j], K.dt[j], i));
+  if (K.pack_mask_bits) h &= (1ull << K.pack_mask_bits) - 1;
+  return h;
+}
+
+// Every key column of probe row i equals that of build row r (cells compared
+// as 64-bit values, narrow columns widened by their signedness).
+__device__ __forceinline__ bool keys_equal(const KeyCols& P, uint64_t i, const KeyCols& B,
+                                           uint64_t r) {
+  for (int j = 0; j < P.n; ++j)
+    if (load_i64(P.p[j], P.dt[j], i) != load_i64(B.p[j], B.dt[j], r)) return false;
+  return true;
+}
+
+__global__ void __launch_bounds__(256) k_pack_keys(KeyCols K, uint64_t n,
+                                                   unsigned long long* __restrict__ out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = pack_row(K, i);
+}
+
 // ---- match count (HashProbe) ----------------------------------------------------------
 __global__ void __launch_bounds__(256) k_probe_count(JoinTableDev T, const void* keys, int32_t kdt,
                                                      uint64_t b, uint64_t e,
@@ -239,16 +276,26 @@ __global__ void __launch_bounds__(256) k_probe_count(JoinTableDev T, const void*
 }
 
 // ---- semi-join marks: one bitmap word per wave-row-group of 64 -------------------------
-__global__ void __launch_bounds__(256) k_probe_mark(JoinTableDev T, const void* keys, int32_t kdt,
+// MULTI: P holds the probe's key columns, B the build's, and a candidate counts
+// only when every column matches.
+template <bool MULTI>
+__global__ void __launch_bounds__(256) k_probe_mark(JoinTableDev T, KeyCols P, KeyCols B,
                                                     uint64_t b, uint64_t e, uint64_t* bitmap) {
   const uint64_t n = e - b;
   for (uint64_t base = blockIdx.x * 256ull; base < n; base += gridDim.x * 256ull) {
     const uint64_t r = base + threadIdx.x;
     bool hit = false;
     if (r < n) {
-      const unsigned long long k = (unsigned long long)load_i64(keys, kdt, b + r);
-      T.has_dup = 0;  // existence only: stop at the first match
-      hit = probe_key(T, k, [](uint32_t, uint32_t) {}) > 0;
+      const uint64_t i = b + r;
+      if constexpr (MULTI) {
+        probe_key(T, pack_row(P, i), [&](uint32_t row, uint32_t) {
+          hit = hit || keys_equal(P, i, B, row);
+        });
+      } else {
+        const unsigned long long k = (unsigned long long)load_i64(P.p[0], P.dt[0], i);
+        T.has_dup = 0;  // existence only: stop at the first match
+        hit = probe_key(T, k, [](uint32_t, uint32_t) {}) > 0;
+      }
     }
     const unsigned long long w = __ballot(hit);
     const uint64_t wbase = base + (threadIdx.x & ~63u);
@@ -723,9 +770,12 @@ __global__ void __launch_bounds__(1024) k_part_agg(ProbeAggDev Q, uint32_t npart
 // ---- inner-join pairs in probe order ---------------------------------------------------------
 constexpr int kPairBlock = 1024;  // probe rows per block (256 threads x 4)
 
-__global__ void __launch_bounds__(256) k_pairs_count(JoinTableDev T, const void* keys, int32_t kdt,
-                                                     uint64_t b, uint64_t e, uint32_t* per_row,
-                                                     uint64_t* block_tot) {
+// Output rows per probe row: its (verified, for MULTI) matches, and 1 for a
+// row without any under LEFT.
+template <bool MULTI>
+__global__ void __launch_bounds__(256) k_pairs_count(JoinTableDev T, KeyCols P, KeyCols B,
+                                                     int32_t left, uint64_t b, uint64_t e,
+                                                     uint32_t* per_row, uint64_t* block_tot) {
   __shared__ unsigned long long red[4];
   const uint64_t r0 = b + blockIdx.x * (uint64_t)kPairBlock;
   constexpr int U = kPairBlock / 256;
@@ -741,15 +791,27 @@ __global__ void __launch_bounds__(256) k_pairs_count(JoinTableDev T, const void*
     idx[u] = ok[u] ? i : r0;  // r0 < e: the grid covers the range
     m[u] = 0;
   });
-  load_rows(keys, kdt, idx, ok, kv);
-  static_for<U>([&](auto I) { k[decltype(I)::value] = (unsigned long long)kv[decltype(I)::value]; });
-  probe_keys(T, k, ok, [&](auto I, uint32_t, uint32_t) { ++m[decltype(I)::value]; });
+  if constexpr (MULTI) {
+    static_for<U>([&](auto I) {
+      constexpr int u = decltype(I)::value;
+      k[u] = pack_row(P, idx[u]);
+    });
+    probe_keys(T, k, ok, [&](auto I, uint32_t row, uint32_t) {
+      constexpr int u = decltype(I)::value;
+      if (keys_equal(P, idx[u], B, row)) ++m[u];
+    });
+  } else {
+    load_rows(P.p[0], P.dt[0], idx, ok, kv);
+    static_for<U>([&](auto I) { k[decltype(I)::value] = (unsigned long long)kv[decltype(I)::value]; });
+    probe_keys(T, k, ok, [&](auto I, uint32_t, uint32_t) { ++m[decltype(I)::value]; });
+  }
   unsigned long long c = 0;
   static_for<U>([&](auto I) {
     constexpr int u = decltype(I)::value;
     if (!ok[u]) return;
-    per_row[idx[u] - b] = m[u];
-    c += m[u];
+    const uint32_t out_rows = (left && m[u] == 0) ? 1u : m[u];
+    per_row[idx[u] - b] = out_rows;
+    c += out_rows;
   });
   for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = c;
@@ -758,7 +820,8 @@ __global__ void __launch_bounds__(256) k_pairs_count(JoinTableDev T, const void*
 }
 
 
-__global__ void __launch_bounds__(256) k_pairs_write(JoinTableDev T, const void* keys, int32_t kdt,
+template <bool MULTI>
+__global__ void __launch_bounds__(256) k_pairs_write(JoinTableDev T, KeyCols P, KeyCols B,
                                                      uint64_t b, uint64_t e,
                                                      const uint32_t* per_row,
                                                      const uint64_t* block_off,
@@ -791,11 +854,25 @@ __global__ void __launch_bounds__(256) k_pairs_write(JoinTableDev T, const void*
     if (!m) continue;
     const uint64_t start = boff + pre[j] - m;
     uint32_t w = 0;
-    probe_key(T, (unsigned long long)load_i64(keys, kdt, i), [&](uint32_t row, uint32_t pay) {
-      out_probe[start + w] = i;
-      out_pay[start + w] = pay;
+    auto emit = [&](uint32_t pay) {
+      if (w < m) {  // the count pass found m rows: the bound keeps writes in the range
+        out_probe[start + w] = i;
+        out_pay[start + w] = pay;
+      }
       ++w;
-    });
+    };
+    if constexpr (MULTI) {
+      probe_key(T, pack_row(P, i), [&](uint32_t row, uint32_t pay) {
+        if (keys_equal(P, i, B, row)) emit(pay);
+      });
+    } else {
+      probe_key(T, (unsigned long long)load_i64(P.p[0], P.dt[0], i),
+                [&](uint32_t, uint32_t pay) { emit(pay); });
+    }
+    if (w == 0) {  // LEFT: the row without a match (m == 1 was counted for it)
+      emit(kNoBuildRow);
+      continue;
+    }
     // duplicates: ascending build payload within the probe row (insertion sort)
     for (uint32_t x = 1; x < m; ++x) {
       const uint32_t key = out_pay[start + x];
@@ -826,6 +903,7 @@ struct tgp_hash_table {
   void* side_mem = nullptr;  // side list (only when sentinel keys exist)
   uint64_t cap = 0, n_rows = 0, n_side = 0, bytes = 0;
   int32_t has_dup = 0;
+  KeyCols bkeys{};  // build key columns (n > 1: a packed multi-column table)
   JoinTableDev dev() const {
     JoinTableDev d;
     d.slots = static_cast<Slot*>(mem);
@@ -870,6 +948,41 @@ int32_t tgp_hash_build_in(tgp_ctx* ctx, const tgp_column* keys, const tgp_column
   TGP_REQUIRE(d_mem && (reinterpret_cast<uintptr_t>(d_mem) & 15) == 0,
               "tgp_hash_build_in: table memory must be non-null and 16-byte aligned");
   return build_table(ctx, keys, payload, d_mem, mem_bytes, out, stream);
+}
+
+int32_t tgp_hash_build_multi(tgp_ctx* ctx, const tgp_column* keys, int32_t nkeys,
+                             const tgp_column* payload, tgp_hash_table** out, void* stream) {
+  TGP_REQUIRE(ctx && keys && out, "tgp_hash_build_multi: null argument");
+  TGP_REQUIRE(nkeys >= 1 && nkeys <= kMaxJoinKeys, "tgp_hash_build_multi: %d key columns (1..%d)",
+              nkeys, kMaxJoinKeys);
+  *out = nullptr;
+  if (nkeys == 1) return build_table(ctx, keys, payload, nullptr, 0, out, stream);
+  KeyCols K{};
+  K.n = nkeys;
+  K.pack_mask_bits = (uint32_t)ctx->tune_join_pack_bits.load();
+  for (int j = 0; j < nkeys; ++j) {
+    TGP_TRY(check_key_column(&keys[j], "tgp_hash_build_multi key"));
+    TGP_REQUIRE(keys[j].len == keys[0].len, "tgp_hash_build_multi: key columns of %llu and %llu rows",
+                (unsigned long long)keys[0].len, (unsigned long long)keys[j].len);
+    K.p[j] = keys[j].data;
+    K.dt[j] = keys[j].dtype;
+  }
+  const uint64_t n = keys[0].len;
+  hipStream_t s = pick_stream(ctx, stream);
+  // the packed keys, in the stream's outer scratch region while the table is built
+  void* packed = nullptr;
+  TGP_TRY(get_scratch(ctx, s, n * 8 + 8, &packed, 1));
+  if (n) {
+    k_pack_keys<<<grid_for(n), 256, 0, s>>>(K, n, static_cast<unsigned long long*>(packed));
+    TGP_HIP(hipGetLastError());
+  }
+  tgp_column pk{};
+  pk.data = packed;
+  pk.len = n;
+  pk.dtype = TGP_U64;
+  TGP_TRY(build_table(ctx, &pk, payload, nullptr, 0, out, stream));
+  (*out)->bkeys = K;
+  return TGP_OK;
 }
 
 static int32_t build_table(tgp_ctx* ctx, const tgp_column* keys, const tgp_column* payload,
@@ -932,6 +1045,9 @@ static int32_t build_table(tgp_ctx* ctx, const tgp_column* keys, const tgp_colum
   }
   t->has_dup = hflags[0] ? 1 : 0;
   t->n_side = hflags[1];
+  t->bkeys.n = 1;
+  t->bkeys.p[0] = keys->data;
+  t->bkeys.dt[0] = keys->dtype;
   if (t->n_side) {
     rc = hbm_alloc(ctx, t->n_side * sizeof(uint2) + 8, &t->side_mem);
     if (rc != TGP_OK) return fail(rc);
@@ -980,10 +1096,55 @@ int32_t tgp_hash_table_info(const tgp_hash_table* t, uint64_t* n_rows, uint64_t*
   return TGP_OK;
 }
 
+static int32_t require_single_key(const tgp_hash_table* t, const char* what) {
+  TGP_REQUIRE(t->bkeys.n <= 1,
+              "%s: the table has %d key columns (use tgp_hash_join_pairs_sync / tgp_hash_join_mark)",
+              what, t->bkeys.n);
+  return TGP_OK;
+}
+
+// The probe's key columns as kernel arguments, checked against the table.
+static int32_t probe_key_cols(const tgp_hash_table* t, const tgp_column* keys, int32_t nkeys,
+                              uint64_t row_end, KeyCols* P) {
+  const int32_t want = t->bkeys.n > 1 ? t->bkeys.n : 1;
+  TGP_REQUIRE(keys && nkeys == want, "join probe: %d key columns for a table of %d", nkeys, want);
+  *P = KeyCols{};
+  P->n = nkeys;
+  P->pack_mask_bits = t->bkeys.pack_mask_bits;
+  for (int j = 0; j < nkeys; ++j) {
+    TGP_TRY(check_key_column(&keys[j], "join probe key"));
+    TGP_REQUIRE(row_end <= keys[j].len, "join probe: rows up to %llu, key column %d has %llu",
+                (unsigned long long)row_end, j, (unsigned long long)keys[j].len);
+    P->p[j] = keys[j].data;
+    P->dt[j] = keys[j].dtype;
+  }
+  return TGP_OK;
+}
+
+static int32_t join_mark(tgp_ctx* ctx, const tgp_hash_table* t, const KeyCols& P,
+                         uint64_t row_begin, uint64_t row_end, uint64_t* d_bitmap,
+                         void* stream) {
+  if (row_begin == row_end) return TGP_OK;
+  const unsigned g = grid_for(row_end - row_begin, 4096);
+  hipStream_t s = pick_stream(ctx, stream);
+  if (P.n > 1)
+    k_probe_mark<true><<<g, 256, 0, s>>>(t->dev(), P, t->bkeys, row_begin, row_end, d_bitmap);
+  else
+    k_probe_mark<false><<<g, 256, 0, s>>>(t->dev(), P, t->bkeys, row_begin, row_end, d_bitmap);
+  TGP_HIP(hipGetLastError());
+  return TGP_OK;
+}
+
+static int32_t join_pairs(tgp_ctx* ctx, const tgp_hash_table* t, const KeyCols& P, int32_t left,
+                          uint64_t row_begin, uint64_t row_end, uint64_t* d_probe_idx,
+                          uint32_t* d_build_payload, uint64_t capacity, uint64_t* h_npairs,
+                          void* stream);
+
 int32_t tgp_hash_probe_count(tgp_ctx* ctx, const tgp_hash_table* t, const tgp_column* probe_keys,
                              uint64_t row_begin, uint64_t row_end, uint64_t* d_count,
                              void* stream) {
   TGP_REQUIRE(ctx && t && d_count, "tgp_hash_probe_count: null argument");
+  TGP_TRY(require_single_key(t, "tgp_hash_probe_count"));
   TGP_TRY(check_injected(ctx));
   TGP_TRY(check_key_column(probe_keys, "tgp_hash_probe_count keys"));
   TGP_REQUIRE(row_begin <= row_end && row_end <= probe_keys->len, "bad probe row range");
@@ -1001,13 +1162,23 @@ int32_t tgp_hash_probe_mark(tgp_ctx* ctx, const tgp_hash_table* t, const tgp_col
                             void* stream) {
   TGP_REQUIRE(ctx && t && d_bitmap, "tgp_hash_probe_mark: null argument");
   TGP_TRY(check_injected(ctx));
+  TGP_TRY(require_single_key(t, "tgp_hash_probe_mark"));
   TGP_TRY(check_key_column(probe_keys, "tgp_hash_probe_mark keys"));
   TGP_REQUIRE(row_begin <= row_end && row_end <= probe_keys->len, "bad probe row range");
-  if (row_begin == row_end) return TGP_OK;
-  k_probe_mark<<<grid_for(row_end - row_begin, 4096), 256, 0, pick_stream(ctx, stream)>>>(
-      t->dev(), probe_keys->data, probe_keys->dtype, row_begin, row_end, d_bitmap);
-  TGP_HIP(hipGetLastError());
-  return TGP_OK;
+  KeyCols P;
+  TGP_TRY(probe_key_cols(t, probe_keys, 1, row_end, &P));
+  return join_mark(ctx, t, P, row_begin, row_end, d_bitmap, stream);
+}
+
+int32_t tgp_hash_join_mark(tgp_ctx* ctx, const tgp_hash_table* t, const tgp_column* probe_keys,
+                           int32_t nkeys, uint64_t row_begin, uint64_t row_end, uint64_t* d_bitmap,
+                           void* stream) {
+  TGP_REQUIRE(ctx && t && d_bitmap, "tgp_hash_join_mark: null argument");
+  TGP_TRY(check_injected(ctx));
+  TGP_REQUIRE(row_begin <= row_end, "bad probe row range");
+  KeyCols P;
+  TGP_TRY(probe_key_cols(t, probe_keys, nkeys, row_end, &P));
+  return join_mark(ctx, t, P, row_begin, row_end, d_bitmap, stream);
 }
 
 int32_t tgp_hash_probe_pairs_sync(tgp_ctx* ctx, const tgp_hash_table* t,
@@ -1017,9 +1188,39 @@ int32_t tgp_hash_probe_pairs_sync(tgp_ctx* ctx, const tgp_hash_table* t,
                                   void* stream) {
   TGP_REQUIRE(ctx && t && h_npairs, "tgp_hash_probe_pairs_sync: null argument");
   TGP_TRY(check_injected(ctx));
+  TGP_TRY(require_single_key(t, "tgp_hash_probe_pairs_sync"));
   TGP_TRY(check_key_column(probe_keys, "tgp_hash_probe_pairs_sync keys"));
   TGP_REQUIRE(row_begin <= row_end && row_end <= probe_keys->len, "bad probe row range");
   *h_npairs = 0;
+  KeyCols P;
+  TGP_TRY(probe_key_cols(t, probe_keys, 1, row_end, &P));
+  return join_pairs(ctx, t, P, 0, row_begin, row_end, d_probe_idx, d_build_payload, capacity,
+                    h_npairs, stream);
+}
+
+int32_t tgp_hash_join_pairs_sync(tgp_ctx* ctx, const tgp_hash_table* t,
+                                 const tgp_column* probe_keys, int32_t nkeys, int32_t join_type,
+                                 uint64_t row_begin, uint64_t row_end, uint64_t* d_probe_idx,
+                                 uint32_t* d_build_payload, uint64_t capacity, uint64_t* h_npairs,
+                                 void* stream) {
+  TGP_REQUIRE(ctx && t && h_npairs, "tgp_hash_join_pairs_sync: null argument");
+  TGP_TRY(check_injected(ctx));
+  TGP_REQUIRE(join_type == TGP_JOIN_INNER || join_type == TGP_JOIN_LEFT,
+              "tgp_hash_join_pairs_sync: join type %d (INNER or LEFT)", join_type);
+  TGP_REQUIRE(row_begin <= row_end, "bad probe row range");
+  *h_npairs = 0;
+  KeyCols P;
+  TGP_TRY(probe_key_cols(t, probe_keys, nkeys, row_end, &P));
+  return join_pairs(ctx, t, P, join_type == TGP_JOIN_LEFT, row_begin, row_end, d_probe_idx,
+                    d_build_payload, capacity, h_npairs, stream);
+}
+
+static int32_t join_pairs(tgp_ctx* ctx, const tgp_hash_table* t, const KeyCols& P, int32_t left,
+                          uint64_t row_begin, uint64_t row_end, uint64_t* d_probe_idx,
+                          uint32_t* d_build_payload, uint64_t capacity, uint64_t* h_npairs,
+                          void* stream) {
+  *h_npairs = 0;
+  const bool multi = P.n > 1;
   const uint64_t n = row_end - row_begin;
   if (n == 0) return TGP_OK;
   hipStream_t s = pick_stream(ctx, stream);
@@ -1028,8 +1229,12 @@ int32_t tgp_hash_probe_pairs_sync(tgp_ctx* ctx, const tgp_hash_table* t,
   TGP_TRY(get_scratch(ctx, s, n * 4 + (nblocks + 1) * 8 + 16, &scratch));
   uint64_t* block_tot = static_cast<uint64_t*>(scratch);
   uint32_t* per_row = reinterpret_cast<uint32_t*>(block_tot + nblocks + 1);
-  k_pairs_count<<<(unsigned)nblocks, 256, 0, s>>>(t->dev(), probe_keys->data, probe_keys->dtype,
-                                                  row_begin, row_end, per_row, block_tot);
+  if (multi)
+    k_pairs_count<true><<<(unsigned)nblocks, 256, 0, s>>>(t->dev(), P, t->bkeys, left, row_begin,
+                                                          row_end, per_row, block_tot);
+  else
+    k_pairs_count<false><<<(unsigned)nblocks, 256, 0, s>>>(t->dev(), P, t->bkeys, left, row_begin,
+                                                           row_end, per_row, block_tot);
   TGP_HIP(hipGetLastError());
   k_scan_blocks<<<1, 1024, 0, s>>>(block_tot, nblocks);
   TGP_HIP(hipGetLastError());
@@ -1044,9 +1249,14 @@ int32_t tgp_hash_probe_pairs_sync(tgp_ctx* ctx, const tgp_hash_table* t,
   }
   if (total == 0) return TGP_OK;
   TGP_REQUIRE(d_probe_idx && d_build_payload, "tgp_hash_probe_pairs_sync: null output");
-  k_pairs_write<<<(unsigned)nblocks, 256, 0, s>>>(
-      t->dev(), probe_keys->data, probe_keys->dtype, row_begin, row_end, per_row, block_tot,
-      reinterpret_cast<unsigned long long*>(d_probe_idx), d_build_payload);
+  if (multi)
+    k_pairs_write<true><<<(unsigned)nblocks, 256, 0, s>>>(
+        t->dev(), P, t->bkeys, row_begin, row_end, per_row, block_tot,
+        reinterpret_cast<unsigned long long*>(d_probe_idx), d_build_payload);
+  else
+    k_pairs_write<false><<<(unsigned)nblocks, 256, 0, s>>>(
+        t->dev(), P, t->bkeys, row_begin, row_end, per_row, block_tot,
+        reinterpret_cast<unsigned long long*>(d_probe_idx), d_build_payload);
   TGP_HIP(hipGetLastError());
   TGP_HIP(hipStreamSynchronize(s));
   return TGP_OK;
@@ -1056,6 +1266,7 @@ int32_t tgp_hash_probe_agg(tgp_ctx* ctx, const tgp_join_agg_query* q, const tgp_
                            void* stream) {
   TGP_REQUIRE(ctx && q && out && q->table, "tgp_hash_probe_agg: null argument");
   TGP_TRY(check_injected(ctx));
+  TGP_TRY(require_single_key(q->table, "tgp_hash_probe_agg"));
   TGP_REQUIRE(out->d_count && out->d_sums, "tgp_hash_probe_agg: null result buffer");
   TGP_TRY(check_key_column(&q->probe_keys, "probe keys"));
   TGP_TRY(check_key_column(&q->value, "value column"));

@@ -342,6 +342,10 @@ int32_t tgp_tune_set(tgp_ctx* ctx, int32_t key, int64_t value) {
       TGP_REQUIRE(value <= 1, "TGP_TUNE_JOIN_ATOMICS is 0 or 1");
       ctx->tune_join_partition.store(value);
       return TGP_OK;
+    case TGP_TUNE_JOIN_PACK_BITS:
+      TGP_REQUIRE(value <= 63, "TGP_TUNE_JOIN_PACK_BITS is 0..63");
+      ctx->tune_join_pack_bits.store(value);
+      return TGP_OK;
   }
   set_error("tgp_tune_set: unknown key %d", key);
   return TGP_ERR_INVALID;
@@ -357,6 +361,7 @@ int32_t tgp_tune_get(tgp_ctx* ctx, int32_t key, int64_t* value) {
     case TGP_TUNE_CHUNK: *value = ctx->tune_chunk.load(); return TGP_OK;
     case TGP_TUNE_SLICES_PER_WAVE: *value = ctx->tune_slices_per_wave.load(); return TGP_OK;
     case TGP_TUNE_JOIN_ATOMICS: *value = ctx->tune_join_partition.load(); return TGP_OK;
+    case TGP_TUNE_JOIN_PACK_BITS: *value = ctx->tune_join_pack_bits.load(); return TGP_OK;
   }
   set_error("tgp_tune_get: unknown key %d", key);
   return TGP_ERR_INVALID;

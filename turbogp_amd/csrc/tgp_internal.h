@@ -102,6 +102,7 @@ struct tgp_ctx {
   std::atomic<int64_t> tune_chunk{0};        // 0: static interleave; n: chunks of n slices
   std::atomic<int64_t> tune_slices_per_wave{0};  // fewest slices per wave (0 = 2)
   std::atomic<int64_t> tune_join_partition{0};   // 0: radix-partitioned join GROUP BY; 1: off
+  std::atomic<int64_t> tune_join_pack_bits{0};   // multi-column join keys: 0 = 64-bit packing
 };
 
 namespace tgp {

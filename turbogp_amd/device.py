@@ -500,18 +500,38 @@ def make_agg_query(cols, preds=(), parts=(), aggs=(), row_begin=0, row_end=None,
     return q
 
 
+def _key_array(keys):
+    """One DeviceColumn or a list of them -> (ctypes array of tgp_column, count)."""
+    cols = list(keys) if isinstance(keys, (list, tuple)) else [keys]
+    arr = (A.Column * len(cols))(*[c.desc() for c in cols])
+    return arr, len(cols)
+
+
 class HashTable:
     """Device hash table of a join's build side (tgp_hash_build).
-    Mirrors JoinHashTable (exec/join_hash_table.rs) / HashBuildScalar (kernel/hash.rs)."""
+    Mirrors JoinHashTable (exec/join_hash_table.rs) / HashBuildScalar (kernel/hash.rs).
+    `keys` may be a list of 2..4 columns: a multi-column table (tgp_hash_build_multi,
+    exec/join.rs pack_key), probed with join_pairs / join_mark; those key columns must
+    stay allocated while the table is used."""
 
     def __init__(self, ctx, keys, payload=None, stream=None, memory=None):
         """memory: a DeviceBuffer to hold the slots (tgp_hash_build_in), reusable
         for another table once this one is freed."""
         self.ctx = ctx
         h = C.c_void_p()
-        kd = keys.desc()
         pd = payload.desc() if payload is not None else None
         pdp = C.byref(pd) if pd else None
+        if isinstance(keys, (list, tuple)):
+            if memory is not None:
+                raise ValueError("multi-column tables are built in their own memory")
+            arr, nk = _key_array(keys)
+            self.nkeys = nk
+            A.check(A.lib().tgp_hash_build_multi(ctx.handle, arr, nk, pdp, C.byref(h), stream),
+                    "tgp_hash_build_multi")
+            self.handle = h.value
+            return
+        self.nkeys = 1
+        kd = keys.desc()
         if memory is None:
             A.check(A.lib().tgp_hash_build(ctx.handle, C.byref(kd), pdp, C.byref(h), stream),
                     "tgp_hash_build")
@@ -583,6 +603,46 @@ class HashTable:
         finally:
             pi.free()
             pp.free()
+
+    def join_pairs(self, keys, join_type=None, row_begin=0, row_end=None, stream=None):
+        """(probe rows, build payloads) of an INNER or LEFT join over the probe key
+        column(s) (tgp_hash_join_pairs_sync); a LEFT row without a match has payload
+        JOIN_NO_MATCH."""
+        arr, nk = _key_array(keys)
+        join_type = A.JOIN_INNER if join_type is None else join_type
+        row_end = arr[0].len if row_end is None else row_end
+        npairs = C.c_uint64()
+        rc = A.lib().tgp_hash_join_pairs_sync(self.ctx.handle, self.handle, arr, nk, join_type,
+                                              row_begin, row_end, None, None, 0,
+                                              C.byref(npairs), stream)
+        if rc not in (A.OK, A.ERR_CAPACITY):
+            A.check(rc, "tgp_hash_join_pairs_sync")
+        capacity = npairs.value
+        pi = DeviceBuffer(self.ctx, max(8, capacity * 8))
+        pp = DeviceBuffer(self.ctx, max(8, capacity * 4))
+        try:
+            A.check(A.lib().tgp_hash_join_pairs_sync(self.ctx.handle, self.handle, arr, nk,
+                                                     join_type, row_begin, row_end, pi.ptr,
+                                                     pp.ptr, capacity, C.byref(npairs), stream),
+                    "tgp_hash_join_pairs_sync")
+            n = npairs.value
+            return pi.download(np.uint64, n), pp.download(np.uint32, n)
+        finally:
+            pi.free()
+            pp.free()
+
+    def join_mark(self, keys, row_begin=0, row_end=None, stream=None):
+        """Semi-join marks over the probe key column(s) (tgp_hash_join_mark)."""
+        arr, nk = _key_array(keys)
+        row_end = arr[0].len if row_end is None else row_end
+        bm = DeviceBitmap(self.ctx, row_end - row_begin)
+        try:
+            A.check(A.lib().tgp_hash_join_mark(self.ctx.handle, self.handle, arr, nk, row_begin,
+                                               row_end, bm.buf.ptr, stream), "tgp_hash_join_mark")
+        except Exception:
+            bm.free()
+            raise
+        return bm
 
     def probe_agg(self, keys, value, group_by, ngroups=1, key_base=0, group_col=None,
                   row_begin=0, row_end=None, result=None, stream=None, fetch=True):
