@@ -387,7 +387,10 @@ int32_t tgp_select_rows(tgp_ctx* ctx, uint64_t row_begin, uint64_t row_end,
                         uint64_t* h_count, void* stream);
 /* d_out[i] = col[d_rows[i]] for i < n: as u64 cells (ResultColumn values,
  * signed types sign-extended) when as_cells, else at the column's width.  A row
- * outside the column is an error (TGP_ERR_INVALID).  Synchronous. */
+ * index of TGP_GATHER_NO_ROW gives 0: the missing side of an outer join's row,
+ * which hash_join fills with 0 cells (exec/join.rs:118-124, :141-152).  Any
+ * other row outside the column is an error (TGP_ERR_INVALID).  Synchronous. */
+#define TGP_GATHER_NO_ROW 0xFFFFFFFFFFFFFFFFull
 int32_t tgp_gather(tgp_ctx* ctx, const tgp_column* col, const uint64_t* d_rows, uint64_t n,
                    int32_t as_cells, void* d_out, void* stream);
 

@@ -7,8 +7,8 @@ import pytest
 from oracle import binding as B
 from turbogp_amd import DeviceColumn, _capi as A
 from turbogp_amd import workloads as W
-from turbogp_amd.dispatch import (Aggregate, Column, QueryShape, SelectQuery, Star, Table,
-                                  classify_query, execute_dispatched)
+from turbogp_amd.dispatch import (Aggregate, Column, Join, QueryShape, SelectQuery, Star, Table,
+                                  classify_query, execute_dispatched, execute_with_join)
 
 pytestmark = pytest.mark.gpu
 
@@ -112,3 +112,76 @@ def test_fact_table_parity_with_the_oracle(ctx, make):
     order = idx[np.argsort(~B.cells(host["price"])[idx], kind="stable")][:50]
     assert r.columns[0].values == [int(v) for v in host["price"][order]]
     assert r.columns[1].values == [int(v) for v in host["qty"][order]]
+
+
+def test_reference_join_engine_tests(make):
+    """tests/joins_groupby_orderby.rs:19-59 through execute_with_join (executor.rs:1747):
+    LEFT keeps unmatched left rows, INNER drops them, COUNT(*) over a LEFT join with
+    no matches counts every left row."""
+    orders = make(id=np.array([1, 2, 3], np.int64), cust=np.array([10, 20, 30], np.int64))
+    returns = make(oid=np.array([1], np.int64), reason=np.array([99], np.int64))
+    cat = {"orders": orders, "returns": returns}
+    star = SelectQuery([Star()])
+    r = execute_with_join(star, "orders", [Join("returns", [("id", "oid")], "LEFT")], cat)
+    assert r.row_count == 3
+    by_name = {c.name: c.values for c in r.columns}
+    assert by_name["orders.id"] == [1, 2, 3] and by_name["returns.reason"] == [99, 0, 0]
+    r = execute_with_join(star, "orders", [Join("returns", [("orders.id", "returns.oid")],
+                                                 "INNER")], cat)
+    assert r.row_count == 1
+    o = make(id=np.array([1, 2, 3, 4], np.int64))
+    rr = make(oid=np.zeros(0, np.int64))
+    r = execute_with_join(SelectQuery([Aggregate("count", "*")]), "o",
+                          [Join("r", [("id", "oid")], "LEFT")], {"o": o, "r": rr})
+    assert r.scalar() == 4
+
+
+def _joined_rows(left, right, on, jt):
+    """Python reference of hash_join (exec/join.rs:68-160) and cross_join_into
+    (executor.rs:1886-1948): (left row or None, right row or None) in output order."""
+    if jt == "CROSS":
+        return [(i, j) for i in range(len(next(iter(left.values()))))
+                for j in range(len(next(iter(right.values()))))]
+    bcols = [right[b] for _, b in on]
+    pcols = [left[a] for a, _ in on]
+    p, b = B.hash_join(bcols, pcols, 1 if jt in ("LEFT", "FULL") else 0)
+    rows = [(int(i), None if int(x) == 0xFFFFFFFF else int(x)) for i, x in zip(p, b)]
+    if jt in ("RIGHT", "FULL"):
+        seen = {x for _, x in rows if x is not None}
+        rows += [(None, j) for j in range(len(bcols[0])) if j not in seen]
+    return rows
+
+
+@pytest.mark.parametrize("jt", ["INNER", "LEFT", "RIGHT", "FULL", "CROSS"])
+def test_join_then_dispatch_vs_reference(make, jt):
+    """Every join type over a one- and a two-column key, then SELECT * and a GROUP BY
+    SUM over the joined table, against the Python restatement of hash_join."""
+    rng = np.random.default_rng(len(jt))
+    nl, nr = (60, 25) if jt == "CROSS" else (3000, 800)
+    left = {"k1": rng.integers(0, 40, nl).astype(np.int64),
+            "k2": rng.integers(0, 3, nl).astype(np.int32),
+            "v": rng.integers(-1000, 1000, nl).astype(np.int64)}
+    right = {"rk1": rng.integers(0, 50, nr).astype(np.int64),
+             "rk2": rng.integers(0, 3, nr).astype(np.int64),
+             "g": rng.integers(0, 7, nr).astype(np.uint8)}
+    cat = {"l": make(**left), "r": make(**right)}
+    for on in ([("k1", "rk1")], [("k1", "rk1"), ("k2", "rk2")]):
+        want = _joined_rows(left, right, on, jt)
+        r = execute_with_join(SelectQuery([Star()]), "l", [Join("r", on, jt)], cat)
+        assert r.row_count == len(want)
+        got = {c.name: c.values for c in r.columns}
+        cell = lambda a, i: 0 if i is None else int(B.cells(a)[i])
+        for name, arr in left.items():
+            assert got[f"l.{name}"] == [cell(arr, i) for i, _ in want], (jt, on, name)
+        for name, arr in right.items():
+            assert got[f"r.{name}"] == [cell(arr, j) for _, j in want], (jt, on, name)
+        # GROUP BY a right column, SUM of a left column, over the joined rows
+        q = SelectQuery([Column("r.g"), Aggregate("sum", "l.v")], group_by=["r.g"])
+        r = execute_with_join(q, "l", [Join("r", on, jt)], cat)
+        sums = {}
+        for i, j in want:
+            g = 0 if j is None else int(right["g"][j])
+            sums[g] = sums.get(g, 0) + (0 if i is None else int(left["v"][i]))
+        keys = [int(x) for x in r.columns[0].values]
+        vals = [int(np.int64(np.uint64(x))) for x in r.columns[1].values]
+        assert dict(zip(keys, vals)) == sums, (jt, on)

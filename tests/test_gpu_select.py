@@ -90,6 +90,13 @@ def test_gather_rejects_rows_outside_the_column(ctx):
             ctx.gather(col, rows, 3)
         rows.upload(np.array([1, 99, 0], np.uint64))
         assert list(ctx.gather(col, rows, 3)) == [1, 99, 0]
+        rows.upload(np.array([7, A.GATHER_NO_ROW, 5], np.uint64))  # no row: a 0 cell
+        assert list(ctx.gather(col, rows, 3)) == [7, 0, 5]
+        g = ctx.gather_column(col, rows, 3)
+        try:
+            assert list(g.to_numpy()) == [7, 0, 5]
+        finally:
+            g.free()
     finally:
         rows.free()
         col.free()

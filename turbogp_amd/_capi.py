@@ -37,6 +37,7 @@ TUNE_SLICES_PER_WAVE, TUNE_JOIN_ATOMICS, TUNE_JOIN_PACK_BITS = 5, 6, 7
 JOIN_INNER, JOIN_LEFT = 0, 1
 JOIN_NO_MATCH = 0xFFFFFFFF
 MAX_JOIN_KEYS = 4
+GATHER_NO_ROW = 0xFFFFFFFFFFFFFFFF
 BATCH_PERSISTENT, BATCH_PER_QUERY = 0, 1
 
 MAX_COLS, MAX_PREDS, MAX_AGGS, MAX_FACTORS, MAX_GROUP_PARTS = 8, 8, 6, 3, 2

@@ -53,7 +53,8 @@ __global__ void __launch_bounds__(256) k_sort_keys(const uint8_t* __restrict__ c
     keys[i] = cell_of<DT>(col, rows ? rows[i] : base + i) ^ flip;
 }
 
-// out[i] = col[rows[i]] as a u64 cell (CELLS) or at the column's width; a row
+// out[i] = col[rows[i]] as a u64 cell (CELLS) or at the column's width; the
+// row index ~0 (no row: an outer join's missing side) writes 0, and any other
 // index outside the column writes 0 and raises *bad.
 template <int DT, bool CELLS>
 __global__ void __launch_bounds__(256) k_gather(const uint8_t* __restrict__ col, uint64_t len,
@@ -64,7 +65,7 @@ __global__ void __launch_bounds__(256) k_gather(const uint8_t* __restrict__ col,
   for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += stride) {
     const uint64_t r = rows[i];
     const bool ok = r < len;
-    oob |= !ok;
+    oob |= !ok && r != ~0ull;
     const uint64_t c = ok ? cell_of<DT>(col, r) : 0;
     if constexpr (CELLS) {
       static_cast<uint64_t*>(out)[i] = c;

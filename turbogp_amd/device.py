@@ -298,6 +298,19 @@ class Context:
             raise
         return out, cnt.value
 
+    def gather_column(self, col, rows, n, stream=None):
+        """A new DeviceColumn of col's dtype holding col at the n row indices in the
+        DeviceBuffer `rows` (GATHER_NO_ROW gives 0), kept on the device."""
+        out = DeviceColumn(self, n, col.dtype)
+        try:
+            d = col.desc()
+            A.check(A.lib().tgp_gather(self.handle, C.byref(d), rows.ptr, n, 0, out.buf.ptr,
+                                       stream), "tgp_gather")
+        except Exception:
+            out.free()
+            raise
+        return out
+
     def gather(self, col, rows, n, as_cells=True, stream=None):
         """col at the n row indices in the DeviceBuffer `rows` -> numpy (u64 cells, or the
         column's dtype)."""

@@ -35,7 +35,7 @@ from typing import List, Optional, Sequence, Tuple
 import numpy as np
 
 from . import _capi as A
-from .device import DeviceBitmap, make_agg_query, numeric_text
+from .device import DeviceBitmap, DeviceBuffer, HashTable, make_agg_query, numeric_text
 
 OPS = {"=": A.EQ, "!=": A.NE, "<>": A.NE, "<": A.LT, "<=": A.LE, ">": A.GT, ">=": A.GE}
 INT8_OID, NUMERIC_OID = 20, 1700
@@ -105,6 +105,27 @@ class QueryResult:
         return c.string_values[0] if c.string_values else c.values[0]
 
 
+class _NullsByColumn(dict):
+    """NULL bitmaps keyed by column name, looked up through the table's name
+    resolution (so that `t.id` and `id` find the same column)."""
+
+    def __init__(self, table, d):
+        super().__init__(d)
+        self._t = table
+
+    def __contains__(self, name):
+        try:
+            return dict.__contains__(self, self._t.resolve(name))
+        except KeyError:
+            return False
+
+    def get(self, name, default=None):
+        try:
+            return dict.get(self, self._t.resolve(name), default)
+        except KeyError:
+            return default
+
+
 class Table:
     """Resident device columns by name (all of one length), and the NULL bitmaps
     (DeviceBitmap, bit set = NULL) of the nullable ones."""
@@ -112,18 +133,35 @@ class Table:
     def __init__(self, ctx, columns: dict, nulls: Optional[dict] = None):
         self.ctx = ctx
         self.columns = dict(columns)
-        self.nulls = dict(nulls or {})
         lens = {c.n for c in self.columns.values()}
         assert len(lens) <= 1, "columns of different lengths"
         self.row_count = lens.pop() if lens else 0
         self.names = list(self.columns)
-        for k in self.nulls:
-            self.col(k)
+        self.nulls = _NullsByColumn(self, {self.resolve(k): v for k, v in (nulls or {}).items()})
+
+    def resolve(self, name):
+        """resolve_col_name (dispatch.rs:1034-1061): the name itself, the query name
+        without its table prefix, or a qualified column whose bare name it is."""
+        if name in self.columns:
+            return name
+        parts = name.split(".")
+        if len(parts) > 1 and parts[1] in self.columns:
+            return parts[1]
+        for cn in self.names:
+            cparts = cn.split(".")
+            if cn == name or (len(cparts) > 1 and cparts[1] == name) or cn.endswith("." + name):
+                return cn
+        raise KeyError(f"unknown column {name!r}")
 
     def col(self, name):
-        if name not in self.columns:
-            raise KeyError(f"unknown column {name!r}")
-        return self.columns[name]
+        return self.columns[self.resolve(name)]
+
+    def free(self):
+        for c in self.columns.values():
+            c.free()
+        for b in dict.values(self.nulls):
+            b.free()
+        self.columns = {}
 
 
 def classify_query(q: SelectQuery) -> QueryShape:
@@ -168,6 +206,96 @@ def execute_dispatched(q: SelectQuery, t: Table) -> Optional[QueryResult]:
     if shape == QueryShape.Complex:
         return None
     return _Executor(q, t).run(shape)
+
+
+@dataclass
+class Join:
+    """One JOIN clause: the joined table, the ON equalities as (left column, right
+    column) pairs (what extract_join_keys yields, exec/join.rs), and the type:
+    INNER, LEFT, RIGHT, FULL or CROSS (executor.rs:1766-1788)."""
+    table: str
+    on: List[Tuple[str, str]] = field(default_factory=list)
+    join_type: str = "INNER"
+
+
+def execute_with_join(q: SelectQuery, from_table: str, joins: Sequence[Join],
+                      catalog: dict) -> Optional[QueryResult]:
+    """engine/executor.rs:1747-1811 execute_with_join: every JOIN is computed on the
+    device (hash_join: build on the right table, probe with the running left one; CROSS:
+    every left row with every right row, cross_join_into :1886-1948), the joined rows
+    are materialised as a device table whose columns are named `table.column`, and the
+    query without its JOINs is dispatched on it.  None when the dispatcher does not
+    take the query (the reference then runs its row executor).  As JoinResult::into_table,
+    the joined table has no NULL bitmaps: the missing side of an outer join's row reads
+    0 (exec/join.rs:118-124, :141-152).  A self-join's equal qualified names collide:
+    the right table's columns win."""
+    base = catalog[from_table]
+    running = base
+    try:
+        for j in joins:
+            joined = _join_tables(running, catalog[j.table], from_table, j)
+            if running is not base:
+                running.free()
+            running = joined
+        return execute_dispatched(q, running)
+    finally:
+        if running is not base:
+            running.free()
+
+
+def _join_tables(left: Table, right: Table, left_name: str, j: Join) -> Table:
+    """The rows of left JOIN right as a new device table (hash_join, exec/join.rs:68-160)."""
+    ctx = left.ctx
+    jt = j.join_type.upper()
+    u64 = np.uint64
+    if jt == "CROSS":
+        li = np.repeat(np.arange(left.row_count, dtype=u64), right.row_count)
+        ri = np.tile(np.arange(right.row_count, dtype=u64), left.row_count)
+    else:
+        if not j.on:
+            raise ValueError("JOIN needs at least one ON equality (hash_join requires a key)")
+        lk = [left.col(a) for a, _ in j.on]
+        rk = [right.col(b) for _, b in j.on]
+        t = HashTable(ctx, rk if len(rk) > 1 else rk[0])
+        try:
+            outer_left = jt in ("LEFT", "FULL")
+            p, b = t.join_pairs(lk if len(lk) > 1 else lk[0],
+                                A.JOIN_LEFT if outer_left else A.JOIN_INNER)
+        finally:
+            t.free()
+        hit = b != A.JOIN_NO_MATCH
+        li = p.astype(u64)
+        ri = np.where(hit, b.astype(u64), u64(A.GATHER_NO_ROW))
+        if jt in ("RIGHT", "FULL"):  # the right rows no left row matched, after the pairs
+            matched = np.zeros(right.row_count, bool)
+            matched[b[hit].astype(np.int64)] = True
+            extra = np.flatnonzero(~matched).astype(u64)
+            li = np.concatenate([li, np.full(len(extra), A.GATHER_NO_ROW, u64)])
+            ri = np.concatenate([ri, extra])
+    n = len(li)
+    cols = {}
+    bufs = [DeviceBuffer(ctx, max(8, 8 * n)) for _ in range(2)]
+    try:
+        if n:
+            bufs[0].upload(li)
+            bufs[1].upload(ri)
+        try:
+            for name in left.names:
+                qn = name if "." in name else f"{left_name}.{name}"
+                cols[qn] = ctx.gather_column(left.columns[name], bufs[0], n)
+            for name in right.names:
+                qn = f"{j.table}.{name}"
+                if qn in cols:
+                    cols[qn].free()
+                cols[qn] = ctx.gather_column(right.columns[name], bufs[1], n)
+        except Exception:
+            for c in cols.values():
+                c.free()
+            raise
+    finally:
+        for bf in bufs:
+            bf.free()
+    return Table(ctx, cols)
 
 
 def _exact_column(name, vals):
