@@ -68,7 +68,8 @@ __device__ __forceinline__ bool selected(const uint64_t* sel, uint64_t i) {
 constexpr uint32_t kHllBlock = 1024;
 constexpr uint32_t kHllUnroll = 4;
 
-template <int DT>
+// SEL: a selection bitmap is given (its words are read beside the cells).
+template <int DT, bool SEL>
 __global__ void __launch_bounds__(kHllBlock) k_hll(const uint8_t* __restrict__ col,
                                                    uint64_t row_begin, uint64_t n,
                                                    const uint64_t* __restrict__ sel,
@@ -79,11 +80,21 @@ __global__ void __launch_bounds__(kHllBlock) k_hll(const uint8_t* __restrict__ c
   const uint64_t stride = (uint64_t)gridDim.x * kHllBlock;
   for (uint64_t i0 = blockIdx.x * (uint64_t)kHllBlock + threadIdx.x; i0 < n;
        i0 += kHllUnroll * stride) {
+    // all loads first (rows past the end read row i0, which is in range), then
+    // the hashes: a load inside each row's branch would wait before the next
+    uint64_t c[kHllUnroll];
+    bool ok[kHllUnroll];
 #pragma unroll
     for (uint32_t k = 0; k < kHllUnroll; ++k) {
       const uint64_t i = i0 + k * stride;
-      if (i >= n || !selected(sel, i)) continue;
-      const uint64_t h = xxh3_u64(cell_at<DT>(col, row_begin + i));
+      const uint64_t r = i < n ? i : i0;
+      c[k] = cell_at<DT>(col, row_begin + r);
+      ok[k] = i < n && (!SEL || selected(sel, r));
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kHllUnroll; ++k) {
+      if (!ok[k]) continue;
+      const uint64_t h = xxh3_u64(c[k]);
       const uint64_t w = h << kHllBits;
       const uint32_t rank = (w ? (uint32_t)__clzll((long long)w) : 64u) + 1u;
       atomicMax(&reg[h >> (64 - kHllBits)], rank);
@@ -185,17 +196,22 @@ typedef void (*HllKern)(const uint8_t*, uint64_t, uint64_t, const uint64_t*, uin
 typedef void (*InsKern)(const uint8_t*, uint64_t, uint64_t, const uint64_t*, unsigned long long*,
                         uint64_t, unsigned long long*);
 
-static HllKern hll_kernel(int32_t dt) {
+template <bool SEL>
+static HllKern hll_kernel_sel(int32_t dt) {
   switch (dt) {
-    case TGP_I8: return k_hll<TGP_I8>;
-    case TGP_U8: return k_hll<TGP_U8>;
-    case TGP_I16: return k_hll<TGP_I16>;
-    case TGP_U16: return k_hll<TGP_U16>;
-    case TGP_I32: return k_hll<TGP_I32>;
-    case TGP_U32: return k_hll<TGP_U32>;
-    case TGP_I64: return k_hll<TGP_I64>;
-    default: return k_hll<TGP_U64>;
+    case TGP_I8: return k_hll<TGP_I8, SEL>;
+    case TGP_U8: return k_hll<TGP_U8, SEL>;
+    case TGP_I16: return k_hll<TGP_I16, SEL>;
+    case TGP_U16: return k_hll<TGP_U16, SEL>;
+    case TGP_I32: return k_hll<TGP_I32, SEL>;
+    case TGP_U32: return k_hll<TGP_U32, SEL>;
+    case TGP_I64: return k_hll<TGP_I64, SEL>;
+    default: return k_hll<TGP_U64, SEL>;
   }
+}
+
+static HllKern hll_kernel(int32_t dt, bool sel) {
+  return sel ? hll_kernel_sel<true>(dt) : hll_kernel_sel<false>(dt);
 }
 
 static InsKern insert_kernel(int32_t dt) {
@@ -238,7 +254,7 @@ static int32_t hll_sketch(tgp_ctx* ctx, const tgp_column* col, uint64_t row_begi
   uint8_t* partial = static_cast<uint8_t*>(p);
   uint8_t* merged = partial + grid * kHllRegs;
   uint8_t* regs = merged + (uint64_t)mid * kHllRegs;
-  hipLaunchKernelGGL(hll_kernel(col->dtype), dim3((uint32_t)grid), dim3(kHllBlock), 0, st,
+  hipLaunchKernelGGL(hll_kernel(col->dtype, d_sel != nullptr), dim3((uint32_t)grid), dim3(kHllBlock), 0, st,
                      static_cast<const uint8_t*>(col->data), row_begin, n, d_sel, partial);
   TGP_HIP(hipGetLastError());
   const dim3 cols(kHllRegs / 16 / 256);
