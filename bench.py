@@ -53,6 +53,32 @@ def pct(xs, p):
     return float(np.percentile(np.asarray(xs, dtype=np.float64), p))
 
 
+def host_cpu():
+    """The host CPU as lscpu reports it: model, sockets, physical cores, logical CPUs,
+    and the CPUs this process may run on."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            info = f.read()
+    except OSError:
+        return None
+    model, sockets, cores = None, set(), set()
+    phys = None
+    for line in info.splitlines():
+        key, _, val = line.partition(":")
+        key, val = key.strip(), val.strip()
+        if key == "model name" and model is None:
+            model = val
+        elif key == "physical id":
+            phys = val
+            sockets.add(val)
+        elif key == "core id":
+            cores.add((phys, val))
+    logical = info.count("processor\t:")
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else logical
+    return (f"{model}; {len(sockets) or 1} socket(s), {len(cores) or logical} cores, "
+            f"{logical} logical CPUs, {usable} usable by this process")
+
+
 def cpu_baseline(params, rows_cap, reps=3):
     """The oracle's restatement of the reference CPU path (filter passes ->
     bitmap -> compaction -> chunked hash grouping -> f64 sum_vec), all host
@@ -66,7 +92,8 @@ def cpu_baseline(params, rows_cap, reps=3):
             break
         sample.append(p)
         rows += p[1] - p[0]
-    threads = os.cpu_count() or 1
+    threads = (len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+               else os.cpu_count()) or 1
     # host columns of each sampled range, generated by the same counter-based generator
     queries = []
     for (b, e, cutoff, g) in sample:
@@ -84,7 +111,7 @@ def cpu_baseline(params, rows_cap, reps=3):
     total_rows = sum(p[1] - p[0] for p in params)
     batch_s = t * total_rows / rows
     return {"value": round(len(params) / batch_s, 2), "unit": "queries/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "host": host_cpu(),
             "sample": f"first {len(sample)} of the {len(params)} batch queries ({rows} rows), "
                       f"median of {reps}: {t:.3f} s, scaled by rows to the whole batch "
                       f"({total_rows} rows, {batch_s:.1f} s); reference algorithm (u64 "
