@@ -539,12 +539,13 @@ typedef struct tgp_serve_request {
   tgp_serve_join join;         /* kind TGP_SERVE_JOIN_AGG */
 } tgp_serve_request;
 typedef struct tgp_serve_config {
-  uint32_t max_batch_queries;  /* requests per batch */
+  uint32_t max_batch_queries;  /* requests per batch (a take of at most 8 runs as that
+                                  many single-query launches, with the same results) */
   uint32_t batch_slots;        /* batches in flight (1..8) */
   uint32_t join_slots;         /* joins in flight (1..8) */
-  uint32_t batch_wait_us;      /* while another batch runs, a free slot waits for
-                                  max_batch_queries / 4 requests or until the oldest has
-                                  waited this long (0: launch whatever is queued) */
+  uint32_t batch_wait_us;      /* while the other slots hold more than 16 requests, a free
+                                  slot waits for max_batch_queries / 4 requests or until the
+                                  oldest has waited this long (0: launch whatever is queued) */
   uint64_t heavy_rows;         /* ranges of >= this many rows run alone, one at a time (0 = off) */
   uint64_t hash_budget_bytes;  /* cap on hash-table HBM (0 = the tier's budget) */
   uint32_t wg_per_cu, chunk_slices; /* batch tuning (tgp_batch_tuning) */
