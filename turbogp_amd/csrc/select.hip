@@ -192,6 +192,8 @@ __global__ void __launch_bounds__(256) k_rs_pick(RsState* st, int pass) {
     local[j] = st->hist[threadIdx.x * per + j];
     mine += local[j];
   }
+  // read before the barrier: the winning thread rewrites it after the barrier
+  const uint64_t rem = st->remaining;
   // exclusive scan of the per-thread sums
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   uint64_t incl = mine;
@@ -203,7 +205,6 @@ __global__ void __launch_bounds__(256) k_rs_pick(RsState* st, int pass) {
   __syncthreads();
   uint64_t cum = incl - mine;
   for (uint32_t w = 0; w < wave; ++w) cum += wsum[w];
-  const uint64_t rem = st->remaining;
   int shift, width;
   bool on_key;
   rs_digit(pass, &shift, &width, &on_key);
