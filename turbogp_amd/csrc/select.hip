@@ -129,6 +129,8 @@ struct RsState {
   uint32_t prefp, maskp;            // position bits chosen so far
   unsigned long long remaining;     // rank of the k-th row among those matching the prefix
   unsigned long long taken;         // collect counter
+  uint32_t done;                    // every row matching the prefix is among the first k
+  uint32_t _pad;
   uint32_t hist[kRsBins];
 };
 
@@ -152,12 +154,14 @@ __global__ void __launch_bounds__(256) k_rs_init(RsState* st, uint64_t k) {
     st->prefp = st->maskp = 0;
     st->remaining = k;
     st->taken = 0;
+    st->done = 0;
   }
   for (int b = threadIdx.x; b < kRsBins; b += blockDim.x) st->hist[b] = 0;
 }
 
 __global__ void __launch_bounds__(256) k_rs_hist(const uint64_t* __restrict__ keys, uint64_t n,
                                                  RsState* st, int pass) {
+  if (st->done) return;  // decided by an earlier pass
   __shared__ uint32_t h[kRsBins];
   for (int b = threadIdx.x; b < kRsBins; b += blockDim.x) h[b] = 0;
   __syncthreads();
@@ -168,14 +172,20 @@ __global__ void __launch_bounds__(256) k_rs_hist(const uint64_t* __restrict__ ke
   const uint32_t prefp = st->prefp, maskp = st->maskp;
   const uint64_t dmask = (1ull << width) - 1;
   const uint64_t stride = (uint64_t)gridDim.x * 256;
+  const uint32_t lane = threadIdx.x & 63;
   for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += stride) {
     const uint64_t k = keys[i];
-    if ((k & maskk) != prefk) continue;
-    if (on_key) {
-      atomicAdd(&h[(k >> shift) & dmask], 1u);
-    } else if (((uint32_t)i & maskp) == prefp) {
-      atomicAdd(&h[((uint32_t)i >> shift) & dmask], 1u);
-    }
+    const bool act = (k & maskk) == prefk && (on_key || ((uint32_t)i & maskp) == prefp);
+    const uint32_t d = (uint32_t)((on_key ? (k >> shift) : ((uint32_t)i >> shift)) & dmask);
+    // the lanes that share the first active lane's digit add once (high key
+    // digits are often the same across a wave)
+    const uint64_t active = __ballot(act);
+    if (!active) continue;
+    const int first = __builtin_ctzll(active);
+    const uint32_t d0 = __shfl(d, first);
+    const uint64_t same = __ballot(act && d == d0);
+    if (lane == (uint32_t)first) atomicAdd(&h[d0], (uint32_t)__popcll(same));
+    if (act && d != d0) atomicAdd(&h[d], 1u);
   }
   __syncthreads();
   for (int b = threadIdx.x; b < kRsBins; b += blockDim.x)
@@ -184,6 +194,12 @@ __global__ void __launch_bounds__(256) k_rs_hist(const uint64_t* __restrict__ ke
 
 // One workgroup: the digit whose bin holds the remaining-th row; clears the bins.
 __global__ void __launch_bounds__(256) k_rs_pick(RsState* st, int pass) {
+  // `done` is read once for the whole workgroup: this kernel's winning thread
+  // may set it, and a wave reading it later must not leave the barriers below
+  __shared__ uint32_t skip;
+  if (threadIdx.x == 0) skip = st->done;
+  __syncthreads();
+  if (skip) return;
   constexpr int per = kRsBins / 256;
   __shared__ unsigned long long wsum[4];
   uint64_t local[per];
@@ -219,6 +235,8 @@ __global__ void __launch_bounds__(256) k_rs_pick(RsState* st, int pass) {
         st->maskp |= (uint32_t)(((1ull << width) - 1) << shift);
       }
       st->remaining = rem - cum;
+      // the bin holds exactly the rows still needed: all of them are taken
+      if (local[j] == rem - cum) st->done = 1;
     }
     cum += local[j];
   }
@@ -233,8 +251,10 @@ __global__ void __launch_bounds__(256) k_rs_collect(const uint64_t* __restrict__
                                                     uint64_t base, uint64_t n, RsState* st,
                                                     uint64_t* out_keys, uint64_t* out_rows,
                                                     uint64_t k) {
-  const unsigned long long tk = st->prefk;
-  const uint32_t tp = st->prefp;
+  // the largest (key, position) taken: the prefix with its undecided bits set
+  // (all bits decided after the last pass)
+  const unsigned long long tk = st->prefk | ~st->maskk;
+  const uint32_t tp = st->maskk == ~0ull ? (st->prefp | ~st->maskp) : 0xffffffffu;
   const uint64_t stride = (uint64_t)gridDim.x * 256;
   for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += stride) {
     const uint64_t key = keys[i];
