@@ -42,15 +42,35 @@ __global__ void __launch_bounds__(256) k_iota(uint64_t* __restrict__ out, uint64
 
 // keys[i] = cell of col at rows[i] (at base + i without a row list),
 // complemented for DESC (rows are in range: they come from the compaction of a
-// bitmap over the column's rows).
+// bitmap over the column's rows).  With `minmax`, the smallest and largest key
+// are folded into minmax[0] and minmax[1].
 template <int DT>
 __global__ void __launch_bounds__(256) k_sort_keys(const uint8_t* __restrict__ col,
                                                    const uint64_t* __restrict__ rows,
                                                    uint64_t base, uint64_t n, uint64_t flip,
-                                                   uint64_t* __restrict__ keys) {
+                                                   uint64_t* __restrict__ keys,
+                                                   unsigned long long* minmax) {
   const uint64_t stride = (uint64_t)gridDim.x * 256;
-  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += stride)
-    keys[i] = cell_of<DT>(col, rows ? rows[i] : base + i) ^ flip;
+  unsigned long long lo = ~0ull, hi = 0;
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += stride) {
+    const uint64_t k = cell_of<DT>(col, rows ? rows[i] : base + i) ^ flip;
+    keys[i] = k;
+    lo = k < lo ? k : lo;
+    hi = k > hi ? k : hi;
+  }
+  if (!minmax) return;
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long a = __shfl_xor(lo, off), b = __shfl_xor(hi, off);
+    lo = a < lo ? a : lo;
+    hi = b > hi ? b : hi;
+  }
+  // every wave on the same two words: an atomic only where this wave improves
+  // on the value read (the minimum only falls and the maximum only rises, so a
+  // stale read never skips a needed update)
+  if ((threadIdx.x & 63) == 0) {
+    if (lo < __atomic_load_n(&minmax[0], __ATOMIC_RELAXED)) atomicMin(&minmax[0], lo);
+    if (hi > __atomic_load_n(&minmax[1], __ATOMIC_RELAXED)) atomicMax(&minmax[1], hi);
+  }
 }
 
 // out[i] = col[rows[i]] as a u64 cell (CELLS) or at the column's width; the
@@ -89,9 +109,9 @@ static uint32_t grid_of(uint64_t n) {
 
 template <int DT>
 static void launch_keys(hipStream_t st, const tgp_column* c, const uint64_t* rows, uint64_t base,
-                        uint64_t n, uint64_t flip, uint64_t* keys) {
+                        uint64_t n, uint64_t flip, uint64_t* keys, unsigned long long* minmax) {
   hipLaunchKernelGGL(k_sort_keys<DT>, dim3(grid_of(n)), dim3(256), 0, st,
-                     static_cast<const uint8_t*>(c->data), rows, base, n, flip, keys);
+                     static_cast<const uint8_t*>(c->data), rows, base, n, flip, keys, minmax);
 }
 
 template <int DT>
@@ -131,8 +151,17 @@ struct RsState {
   unsigned long long taken;         // collect counter
   uint32_t done;                    // every row matching the prefix is among the first k
   uint32_t _pad;
+  unsigned long long kmin, kmax;    // smallest and largest key (k_sort_keys)
   uint32_t hist[kRsBins];
 };
+
+// A key pass whose digit lies in the leading bits every key shares: its digit
+// is that of the smallest key, with no histogram needed.
+__device__ __forceinline__ bool rs_shared_digit(const RsState* st, bool on_key, int shift) {
+  const unsigned long long diff = st->kmin ^ st->kmax;
+  const int common = diff ? __clzll((long long)diff) : 64;
+  return on_key && shift >= 64 - common;
+}
 
 // Digit (shift, width) of pass p: key bits 63..53, 52..42, 41..31, 30..20,
 // 19..9, 8..0, then position bits 31..21, 20..10, 9..0.
@@ -155,6 +184,8 @@ __global__ void __launch_bounds__(256) k_rs_init(RsState* st, uint64_t k) {
     st->remaining = k;
     st->taken = 0;
     st->done = 0;
+    st->kmin = ~0ull;
+    st->kmax = 0;
   }
   for (int b = threadIdx.x; b < kRsBins; b += blockDim.x) st->hist[b] = 0;
 }
@@ -162,30 +193,42 @@ __global__ void __launch_bounds__(256) k_rs_init(RsState* st, uint64_t k) {
 __global__ void __launch_bounds__(256) k_rs_hist(const uint64_t* __restrict__ keys, uint64_t n,
                                                  RsState* st, int pass) {
   if (st->done) return;  // decided by an earlier pass
-  __shared__ uint32_t h[kRsBins];
-  for (int b = threadIdx.x; b < kRsBins; b += blockDim.x) h[b] = 0;
-  __syncthreads();
   int shift, width;
   bool on_key;
   rs_digit(pass, &shift, &width, &on_key);
+  if (rs_shared_digit(st, on_key, shift)) return;
+  __shared__ uint32_t h[kRsBins];
+  for (int b = threadIdx.x; b < kRsBins; b += blockDim.x) h[b] = 0;
+  __syncthreads();
   const unsigned long long prefk = st->prefk, maskk = st->maskk;
   const uint32_t prefp = st->prefp, maskp = st->maskp;
   const uint64_t dmask = (1ull << width) - 1;
   const uint64_t stride = (uint64_t)gridDim.x * 256;
   const uint32_t lane = threadIdx.x & 63;
-  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += stride) {
-    const uint64_t k = keys[i];
-    const bool act = (k & maskk) == prefk && (on_key || ((uint32_t)i & maskp) == prefp);
-    const uint32_t d = (uint32_t)((on_key ? (k >> shift) : ((uint32_t)i >> shift)) & dmask);
-    // the lanes that share the first active lane's digit add once (high key
-    // digits are often the same across a wave)
-    const uint64_t active = __ballot(act);
-    if (!active) continue;
-    const int first = __builtin_ctzll(active);
-    const uint32_t d0 = __shfl(d, first);
-    const uint64_t same = __ballot(act && d == d0);
-    if (lane == (uint32_t)first) atomicAdd(&h[d0], (uint32_t)__popcll(same));
-    if (act && d != d0) atomicAdd(&h[d], 1u);
+  constexpr int U = 4;
+  for (uint64_t i0 = blockIdx.x * 256ull + threadIdx.x; i0 < n; i0 += U * stride) {
+    uint64_t kk[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {  // the loads first (rows past the end read row i0)
+      const uint64_t i = i0 + u * stride;
+      kk[u] = keys[i < n ? i : i0];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t i = i0 + u * stride;
+      const uint64_t k = kk[u];
+      const bool act = i < n && (k & maskk) == prefk && (on_key || ((uint32_t)i & maskp) == prefp);
+      const uint32_t d = (uint32_t)((on_key ? (k >> shift) : ((uint32_t)i >> shift)) & dmask);
+      // the lanes that share the first active lane's digit add once (high key
+      // digits are often the same across a wave)
+      const uint64_t active = __ballot(act);
+      if (!active) continue;
+      const int first = __builtin_ctzll(active);
+      const uint32_t d0 = __shfl(d, first);
+      const uint64_t same = __ballot(act && d == d0);
+      if (lane == (uint32_t)first) atomicAdd(&h[d0], (uint32_t)__popcll(same));
+      if (act && d != d0) atomicAdd(&h[d], 1u);
+    }
   }
   __syncthreads();
   for (int b = threadIdx.x; b < kRsBins; b += blockDim.x)
@@ -200,6 +243,19 @@ __global__ void __launch_bounds__(256) k_rs_pick(RsState* st, int pass) {
   if (threadIdx.x == 0) skip = st->done;
   __syncthreads();
   if (skip) return;
+  {
+    int shift, width;
+    bool on_key;
+    rs_digit(pass, &shift, &width, &on_key);
+    if (rs_shared_digit(st, on_key, shift)) {  // every candidate has the smallest key's digit
+      if (threadIdx.x == 0) {
+        const uint64_t dm = (1ull << width) - 1;
+        st->prefk |= ((st->kmin >> shift) & dm) << shift;
+        st->maskk |= dm << shift;
+      }
+      return;
+    }
+  }
   constexpr int per = kRsBins / 256;
   __shared__ unsigned long long wsum[4];
   uint64_t local[per];
@@ -362,14 +418,14 @@ int32_t tgp_select_rows(tgp_ctx* ctx, uint64_t row_begin, uint64_t row_end,
   const uint64_t* first = rows;
   const uint64_t* row_list = implicit_rows ? nullptr : rows;
   if (sorted) {
+    auto* state = reinterpret_cast<RsState*>(keys + cnt);  // top-k only
+    if (topk) hipLaunchKernelGGL(k_rs_init, dim3(1), dim3(256), 0, st, state, take);
     TGP_BY_DTYPE(order_col->dtype, launch_keys, st, order_col, row_list, row_begin, cnt,
-                 descending ? ~0ull : 0ull, keys);
+                 descending ? ~0ull : 0ull, keys, topk ? &state->kmin : nullptr);
     TGP_HIP(hipGetLastError());
     if (topk) {
-      auto* state = reinterpret_cast<RsState*>(keys + cnt);
       auto* tkeys = reinterpret_cast<uint64_t*>(state + 1);
       uint64_t* trows = tkeys + take;
-      hipLaunchKernelGGL(k_rs_init, dim3(1), dim3(256), 0, st, state, take);
       const uint32_t g = grid_of(cnt);
       for (int pass = 0; pass < kRsPasses; ++pass) {
         hipLaunchKernelGGL(k_rs_hist, dim3(g), dim3(256), 0, st, keys, cnt, state, pass);
