@@ -132,3 +132,22 @@ def test_serve_mix_is_seeded_and_in_range():
         assert (kind == "join") == (nb > 0) and nb <= 20_000
     off = W.poisson_offsets(1000, 2000.0)
     assert (np.diff(off) >= 0).all() and 0.3e9 < off[-1] < 0.7e9  # ~1000 / 2000 per s
+
+
+def test_join_select_distinct_entry_points_validate_without_a_device():
+    """The newer entry points reject missing arguments with a status, no device needed."""
+    L = _capi.lib()
+    h = ctypes.c_void_p()
+    keys = (_capi.Column * 2)()
+    n = ctypes.c_uint64()
+    assert L.tgp_hash_build_multi(None, keys, 2, None, ctypes.byref(h), None) == _capi.ERR_INVALID
+    assert L.tgp_hash_join_pairs_sync(None, None, keys, 2, _capi.JOIN_LEFT, 0, 0, None, None, 0,
+                                      ctypes.byref(n), None) == _capi.ERR_INVALID
+    assert L.tgp_hash_join_mark(None, None, keys, 2, 0, 0, None, None) == _capi.ERR_INVALID
+    assert L.tgp_select_rows(None, 0, 10, None, None, 0, 5, None, 5, ctypes.byref(n),
+                             None) == _capi.ERR_INVALID
+    assert L.tgp_gather(None, None, None, 0, 1, None, None) == _capi.ERR_INVALID
+    assert L.tgp_count_distinct(None, None, 0, 0, None, 0, ctypes.byref(n), None) == \
+        _capi.ERR_INVALID
+    assert L.tgp_tune_set(None, _capi.TUNE_JOIN_PACK_BITS, 3) == _capi.ERR_INVALID
+    assert b"tgp_tune_set" in L.tgp_last_error()
