@@ -214,10 +214,14 @@ enum tgp_tune_key {
                                    kernels before their grid shrinks (0 = default 2) */
   TGP_TUNE_JOIN_ATOMICS = 6,    /* join GROUP BY a probe column with more groups than LDS
                                    holds: 0 (default) radix-partitioned, 1 global atomics */
-  TGP_TUNE_JOIN_PACK_BITS = 7   /* multi-column join tables built from now on keep only the
+  TGP_TUNE_JOIN_PACK_BITS = 7,  /* multi-column join tables built from now on keep only the
                                    low n bits (1..63) of the packed key, so that distinct
                                    keys collide and the full-key verification is exercised
                                    (tests); 0 (default): all 64 bits */
+  TGP_TUNE_DISTINCT_SHRINK = 8  /* exact COUNT(DISTINCT) over >= 2^20 rows sizes its set from
+                                   the HyperLogLog estimate; n > 0 divides that size by 2^n
+                                   (0..30), so that the set overflows and the count is redone
+                                   in a set sized by the row count (tests); 0 (default) */
 };
 int32_t tgp_tune_set(tgp_ctx* ctx, int32_t key, int64_t value);
 int32_t tgp_tune_get(tgp_ctx* ctx, int32_t key, int64_t* value);

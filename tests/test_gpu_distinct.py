@@ -85,3 +85,32 @@ def test_hll_kat_large(ctx):  # exec/hll.rs test_hll_large_approximate / with_du
             assert got == B.count_distinct(host) and abs(got - want) / want < 0.02
         finally:
             col.free()
+
+
+@pytest.mark.parametrize("shrink", [0, 12])
+def test_exact_count_with_sketch_sized_set(ctx, shrink):
+    """From 2^20 rows the exact set is sized from the HyperLogLog estimate; with the
+    size cut by 2^12 it overflows (bounded probes) and the count is redone in a set
+    sized by the row count.  Exact either way, with and without a bitmap, for many and
+    for few distinct values, including the set's marker cell."""
+    rng = np.random.default_rng(40 + shrink)
+    n = 3_000_001
+    many = rng.integers(-2**40, 2**40, n).astype(np.int64) // 1024     # ~2.6M distinct
+    many[::1001] = -1                                                  # the marker cell
+    few = rng.integers(1, 51, n).astype(np.int64)                      # 50 distinct
+    words = rng.integers(0, 2**63, (n + 63) // 64).astype(np.uint64)
+    words[-1] &= np.uint64((1 << (n % 64)) - 1)
+    ctx.tune(A.TUNE_DISTINCT_SHRINK, shrink)
+    try:
+        for host in (many, few):
+            col = DeviceColumn.from_numpy(ctx, host)
+            bm = bitmap(ctx, words, n)
+            try:
+                assert ctx.count_distinct(col, mode=A.DISTINCT_EXACT) == len(np.unique(host))
+                got = ctx.count_distinct(col, 0, n, bm, A.DISTINCT_EXACT)
+                assert got == B.count_distinct(host, words, A.DISTINCT_EXACT)
+            finally:
+                bm.free()
+                col.free()
+    finally:
+        ctx.tune(A.TUNE_DISTINCT_SHRINK, 0)

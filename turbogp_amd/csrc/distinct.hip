@@ -149,7 +149,11 @@ __device__ __forceinline__ uint64_t set_slot(uint64_t v) {  // SplitMix64 finali
 }
 
 // counters[0]: values inserted (distinct values other than the marker);
-// counters[1]: the marker value occurred.
+// counters[1]: the marker value occurred; counters[2]: a probe run reached
+// kMaxProbe slots without an empty one (the set is too small: the caller
+// counts again in a set sized by the row count).
+constexpr uint32_t kMaxProbe = 4096;
+
 template <int DT>
 __global__ void __launch_bounds__(256) k_distinct_insert(const uint8_t* __restrict__ col,
                                                          uint64_t row_begin, uint64_t n,
@@ -157,19 +161,26 @@ __global__ void __launch_bounds__(256) k_distinct_insert(const uint8_t* __restri
                                                          unsigned long long* __restrict__ table,
                                                          uint64_t mask,
                                                          unsigned long long* counters) {
-  uint32_t inserted = 0, marker = 0;
+  uint32_t inserted = 0, marker = 0, overflow = 0;
   const uint64_t stride = (uint64_t)gridDim.x * 256;
+  // a run never needs more steps than the set has slots
+  const uint32_t max_probe = mask + 1 < kMaxProbe ? (uint32_t)(mask + 1) : kMaxProbe;
   for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += stride) {
+    if (overflow) break;  // the count is redone in a larger set: stop early
     if (!selected(sel, i)) continue;
     const unsigned long long v = cell_at<DT>(col, row_begin + i);
     if (v == kSetEmpty) {
       marker = 1;
       continue;
     }
-    // at most half the slots are taken, so an empty slot ends every probe run;
-    // a plain read first spares the atomic for values already present (a stale
-    // read only sends the value on to the CAS, which sees the slot as it is)
-    for (uint64_t s = set_slot(v) & mask;; s = (s + 1) & mask) {
+    // a set at most half full ends probe runs at an empty slot within a few
+    // steps; a run of kMaxProbe slots means the set is too small and is
+    // reported, not continued.  A plain read first spares the atomic for values
+    // already present (a stale read only sends the value on to the CAS, which
+    // sees the slot as it is).
+    uint64_t s = set_slot(v) & mask;
+    uint32_t step = 0;
+    for (; step < max_probe; ++step, s = (s + 1) & mask) {
       const unsigned long long seen = __hip_atomic_load(&table[s], __ATOMIC_RELAXED,
                                                         __HIP_MEMORY_SCOPE_AGENT);
       if (seen == v) break;
@@ -181,14 +192,17 @@ __global__ void __launch_bounds__(256) k_distinct_insert(const uint8_t* __restri
       }
       if (old == v) break;
     }
+    overflow |= step == max_probe;
   }
   for (int off = 32; off > 0; off >>= 1) {
     inserted += __shfl_xor(inserted, off);
     marker |= __shfl_xor(marker, off);
+    overflow |= __shfl_xor(overflow, off);
   }
   if ((threadIdx.x & 63) == 0) {
     if (inserted) atomicAdd(&counters[0], (unsigned long long)inserted);
     if (marker) atomicOr(&counters[1], 1ull);
+    if (overflow) atomicOr(&counters[2], 1ull);
   }
 }
 
@@ -271,17 +285,16 @@ static int32_t hll_sketch(tgp_ctx* ctx, const tgp_column* col, uint64_t row_begi
   return TGP_OK;
 }
 
-static int32_t exact_distinct(tgp_ctx* ctx, const tgp_column* col, uint64_t row_begin,
-                              uint64_t row_end, const uint64_t* d_sel, uint64_t* out,
-                              hipStream_t st) {
-  const uint64_t n = row_end - row_begin;
-  uint64_t cap = 1024;
-  while (cap < 2 * n) cap <<= 1;
+// Inserts the selected cells into a set of `cap` slots; *ok is false when a
+// probe run found no empty slot or the set ended more than half full.
+static int32_t insert_distinct(tgp_ctx* ctx, const tgp_column* col, uint64_t row_begin,
+                               uint64_t n, const uint64_t* d_sel, uint64_t cap, uint64_t* out,
+                               bool* ok, hipStream_t st) {
   void* p = nullptr;
-  TGP_TRY(get_scratch(ctx, st, (cap + 2) * 8, &p));
+  TGP_TRY(get_scratch(ctx, st, (cap + 4) * 8, &p));
   auto* counters = static_cast<unsigned long long*>(p);
-  auto* table = counters + 2;
-  TGP_TRY(fill_u64(st, reinterpret_cast<uint64_t*>(counters), 2, 0));
+  auto* table = counters + 4;
+  TGP_TRY(fill_u64(st, reinterpret_cast<uint64_t*>(counters), 4, 0));
   TGP_TRY(fill_u64(st, reinterpret_cast<uint64_t*>(table), cap, kSetEmpty));
   uint64_t grid = (n + 1023) / 1024;
   const uint64_t gmax = (uint64_t)persistent_grid(ctx, 8);
@@ -291,10 +304,44 @@ static int32_t exact_distinct(tgp_ctx* ctx, const tgp_column* col, uint64_t row_
                      static_cast<const uint8_t*>(col->data), row_begin, n, d_sel, table, cap - 1,
                      counters);
   TGP_HIP(hipGetLastError());
-  uint64_t h[2] = {0, 0};
-  TGP_HIP(hipMemcpyAsync(h, counters, 16, hipMemcpyDeviceToHost, st));
+  uint64_t h[3] = {0, 0, 0};
+  TGP_HIP(hipMemcpyAsync(h, counters, 24, hipMemcpyDeviceToHost, st));
   TGP_HIP(hipStreamSynchronize(st));
   *out = h[0] + (h[1] ? 1 : 0);
+  *ok = h[2] == 0 && 2 * h[0] <= cap;
+  return TGP_OK;
+}
+
+// Exact COUNT(DISTINCT): from kSizeBySketch rows on, the set is sized at 2.5x
+// the HyperLogLog estimate (0.8 % standard error) instead of twice the row
+// count, which is smaller to clear and better cached when values repeat (60M
+// prices, 10.38M distinct: 2^25 slots instead of 2^27); a set that turns out
+// too small is counted again at twice the row count, which always suffices.
+constexpr uint64_t kSizeBySketch = 1ull << 20;
+
+static int32_t exact_distinct(tgp_ctx* ctx, const tgp_column* col, uint64_t row_begin,
+                              uint64_t row_end, const uint64_t* d_sel, uint64_t* out,
+                              hipStream_t st) {
+  const uint64_t n = row_end - row_begin;
+  uint64_t full = 1024;
+  while (full < 2 * n) full <<= 1;
+  if (n >= kSizeBySketch) {
+    uint8_t regs[kHllRegs];
+    TGP_TRY(hll_sketch(ctx, col, row_begin, row_end, d_sel, regs, st));
+    const uint64_t est = tgp_hll_estimate(regs);
+    const uint64_t want = (2 * (est + est / 4 + 1024)) >> ctx->tune_distinct_shrink.load();
+    uint64_t cap = 1024;
+    while (cap < want && cap < full) cap <<= 1;
+    if (cap < full) {
+      bool ok = false;
+      TGP_TRY(insert_distinct(ctx, col, row_begin, n, d_sel, cap, out, &ok, st));
+      if (ok) return TGP_OK;
+    }
+  }
+  bool ok = false;
+  TGP_TRY(insert_distinct(ctx, col, row_begin, n, d_sel, full, out, &ok, st));
+  TGP_REQUIRE(ok, "tgp_count_distinct: the set of %llu slots overflowed",
+              (unsigned long long)full);
   return TGP_OK;
 }
 

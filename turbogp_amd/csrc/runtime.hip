@@ -346,6 +346,10 @@ int32_t tgp_tune_set(tgp_ctx* ctx, int32_t key, int64_t value) {
       TGP_REQUIRE(value <= 63, "TGP_TUNE_JOIN_PACK_BITS is 0..63");
       ctx->tune_join_pack_bits.store(value);
       return TGP_OK;
+    case TGP_TUNE_DISTINCT_SHRINK:
+      TGP_REQUIRE(value <= 30, "TGP_TUNE_DISTINCT_SHRINK is 0..30");
+      ctx->tune_distinct_shrink.store(value);
+      return TGP_OK;
   }
   set_error("tgp_tune_set: unknown key %d", key);
   return TGP_ERR_INVALID;
@@ -362,6 +366,7 @@ int32_t tgp_tune_get(tgp_ctx* ctx, int32_t key, int64_t* value) {
     case TGP_TUNE_SLICES_PER_WAVE: *value = ctx->tune_slices_per_wave.load(); return TGP_OK;
     case TGP_TUNE_JOIN_ATOMICS: *value = ctx->tune_join_partition.load(); return TGP_OK;
     case TGP_TUNE_JOIN_PACK_BITS: *value = ctx->tune_join_pack_bits.load(); return TGP_OK;
+    case TGP_TUNE_DISTINCT_SHRINK: *value = ctx->tune_distinct_shrink.load(); return TGP_OK;
   }
   set_error("tgp_tune_get: unknown key %d", key);
   return TGP_ERR_INVALID;

@@ -103,6 +103,7 @@ struct tgp_ctx {
   std::atomic<int64_t> tune_slices_per_wave{0};  // fewest slices per wave (0 = 2)
   std::atomic<int64_t> tune_join_partition{0};   // 0: radix-partitioned join GROUP BY; 1: off
   std::atomic<int64_t> tune_join_pack_bits{0};   // multi-column join keys: 0 = 64-bit packing
+  std::atomic<int64_t> tune_distinct_shrink{0};  // exact distinct: sketch-sized set / 2^n
 };
 
 namespace tgp {
