@@ -153,6 +153,8 @@ __device__ __forceinline__ uint64_t set_slot(uint64_t v) {  // SplitMix64 finali
 // kMaxProbe slots without an empty one (the set is too small: the caller
 // counts again in a set sized by the row count).
 constexpr uint32_t kMaxProbe = 4096;
+constexpr uint32_t kLdsSetSlots = 4096;  // the workgroup's front set: 32 KiB of LDS
+constexpr uint32_t kLdsProbe = 8;        // probe steps in it before going to the device set
 
 template <int DT>
 __global__ void __launch_bounds__(256) k_distinct_insert(const uint8_t* __restrict__ col,
@@ -161,6 +163,16 @@ __global__ void __launch_bounds__(256) k_distinct_insert(const uint8_t* __restri
                                                          unsigned long long* __restrict__ table,
                                                          uint64_t mask,
                                                          unsigned long long* counters) {
+  // A small set in LDS in front of the device set: a value the workgroup has
+  // seen before does not touch the device set (few distinct values otherwise
+  // send every row to the same few slots); the lane that puts a value into it
+  // inserts the value into the device set.  Once it is half full (many
+  // distinct values) the workgroup stops using it.
+  __shared__ unsigned long long lset[kLdsSetSlots];
+  __shared__ uint32_t lfill;
+  for (uint32_t j = threadIdx.x; j < kLdsSetSlots; j += blockDim.x) lset[j] = kSetEmpty;
+  if (threadIdx.x == 0) lfill = 0;
+  __syncthreads();
   uint32_t inserted = 0, marker = 0, overflow = 0;
   const uint64_t stride = (uint64_t)gridDim.x * 256;
   // a run never needs more steps than the set has slots
@@ -173,12 +185,35 @@ __global__ void __launch_bounds__(256) k_distinct_insert(const uint8_t* __restri
       marker = 1;
       continue;
     }
+    const uint64_t hv = set_slot(v);
+    if (lfill < kLdsSetSlots / 2) {
+      bool seen_here = false;
+      uint32_t ls = (uint32_t)(hv >> 32) & (kLdsSetSlots - 1);
+      for (uint32_t t = 0; t < kLdsProbe; ++t, ls = (ls + 1) & (kLdsSetSlots - 1)) {
+        const unsigned long long cur = lset[ls];
+        if (cur == v) {
+          seen_here = true;
+          break;
+        }
+        if (cur != kSetEmpty) continue;
+        const unsigned long long old = atomicCAS(&lset[ls], kSetEmpty, v);
+        if (old == kSetEmpty) {
+          atomicAdd(&lfill, 1u);
+          break;  // this lane inserts v into the device set
+        }
+        if (old == v) {
+          seen_here = true;
+          break;
+        }
+      }
+      if (seen_here) continue;
+    }
     // a set at most half full ends probe runs at an empty slot within a few
     // steps; a run of kMaxProbe slots means the set is too small and is
     // reported, not continued.  A plain read first spares the atomic for values
     // already present (a stale read only sends the value on to the CAS, which
     // sees the slot as it is).
-    uint64_t s = set_slot(v) & mask;
+    uint64_t s = hv & mask;
     uint32_t step = 0;
     for (; step < max_probe; ++step, s = (s + 1) & mask) {
       const unsigned long long seen = __hip_atomic_load(&table[s], __ATOMIC_RELAXED,
