@@ -177,39 +177,29 @@ __global__ void __launch_bounds__(256) k_distinct_insert(const uint8_t* __restri
   const uint64_t stride = (uint64_t)gridDim.x * 256;
   // a run never needs more steps than the set has slots
   const uint32_t max_probe = mask + 1 < kMaxProbe ? (uint32_t)(mask + 1) : kMaxProbe;
-  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += stride) {
-    if (overflow) break;  // the count is redone in a larger set: stop early
-    if (!selected(sel, i)) continue;
-    const unsigned long long v = cell_at<DT>(col, row_begin + i);
+  // one row's value: the LDS set, then the device set
+  auto add = [&](unsigned long long v) {
     if (v == kSetEmpty) {
       marker = 1;
-      continue;
+      return;
     }
     const uint64_t hv = set_slot(v);
     if (lfill < kLdsSetSlots / 2) {
-      bool seen_here = false;
       uint32_t ls = (uint32_t)(hv >> 32) & (kLdsSetSlots - 1);
       for (uint32_t t = 0; t < kLdsProbe; ++t, ls = (ls + 1) & (kLdsSetSlots - 1)) {
         const unsigned long long cur = lset[ls];
-        if (cur == v) {
-          seen_here = true;
-          break;
-        }
+        if (cur == v) return;
         if (cur != kSetEmpty) continue;
         const unsigned long long old = atomicCAS(&lset[ls], kSetEmpty, v);
         if (old == kSetEmpty) {
           atomicAdd(&lfill, 1u);
           break;  // this lane inserts v into the device set
         }
-        if (old == v) {
-          seen_here = true;
-          break;
-        }
+        if (old == v) return;
       }
-      if (seen_here) continue;
     }
     // a set at most half full ends probe runs at an empty slot within a few
-    // steps; a run of kMaxProbe slots means the set is too small and is
+    // steps; a run of max_probe slots means the set is too small and is
     // reported, not continued.  A plain read first spares the atomic for values
     // already present (a stale read only sends the value on to the CAS, which
     // sees the slot as it is).
@@ -228,6 +218,24 @@ __global__ void __launch_bounds__(256) k_distinct_insert(const uint8_t* __restri
       if (old == v) break;
     }
     overflow |= step == max_probe;
+  };
+  constexpr int U = 4;
+  for (uint64_t i0 = blockIdx.x * 256ull + threadIdx.x; i0 < n; i0 += U * stride) {
+    if (overflow) break;  // the count is redone in a larger set: stop early
+    // the cells of U rows are loaded before any is added (rows past the end
+    // read row i0, which is in range)
+    unsigned long long c[U];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t i = i0 + u * stride;
+      const uint64_t r = i < n ? i : i0;
+      c[u] = cell_at<DT>(col, row_begin + r);
+      ok[u] = i < n && selected(sel, r);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (ok[u]) add(c[u]);
   }
   for (int off = 32; off > 0; off >>= 1) {
     inserted += __shfl_xor(inserted, off);
