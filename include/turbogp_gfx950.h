@@ -269,6 +269,16 @@ int32_t tgp_scan_filter(tgp_ctx* ctx, const tgp_column* cols, int32_t ncols,
                         uint64_t row_begin, uint64_t row_end,
                         uint64_t* d_bitmap, uint64_t* d_count, void* stream);
 
+/* Column-vs-column comparison narrowing a selection: bit i of d_bitmap (row
+ * row_begin+i) stays set only where a[row] op b[row], cells compared as signed
+ * 64-bit values when is_signed (INT / DATE columns) and as u64 otherwise (EQ
+ * and NE agree either way); bits past the range are cleared.  d_bitmap holds
+ * the selection to narrow (all ones: no selection yet).  Replaces the Col-op-Col
+ * fast path of eval_comparison_vec (query_interpreter/expr.rs:1382-1451). */
+int32_t tgp_scan_compare_columns(tgp_ctx* ctx, const tgp_column* a, int32_t op,
+                                 const tgp_column* b, int32_t is_signed, uint64_t row_begin,
+                                 uint64_t row_end, uint64_t* d_bitmap, void* stream);
+
 /* Bitmap algebra over nbits bits (bitmap.rs and/or/not/and_inplace):
  * out = a op b (out may alias a); NOT clears tail bits (bitmap.rs:121-136). */
 int32_t tgp_bitmap_combine(tgp_ctx* ctx, int32_t op, const uint64_t* d_a, const uint64_t* d_b,

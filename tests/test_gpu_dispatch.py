@@ -185,3 +185,23 @@ def test_join_then_dispatch_vs_reference(make, jt):
         keys = [int(x) for x in r.columns[0].values]
         vals = [int(np.int64(np.uint64(x))) for x in r.columns[1].values]
         assert dict(zip(keys, vals)) == sums, (jt, on)
+
+
+def test_where_column_vs_column(make):
+    """A Q21-style column comparison (l_receiptdate > l_commitdate) in WHERE, with a
+    literal predicate beside it, through the dispatch mirror against numpy."""
+    rng = np.random.default_rng(21)
+    n = 200_000
+    commit = rng.integers(0, 2500, n).astype(np.int32)
+    receipt = (commit + rng.integers(-30, 31, n)).astype(np.int32)
+    qty = rng.integers(1, 51, n).astype(np.int64)
+    t = make(l_commitdate=commit, l_receiptdate=receipt, qty=qty)
+    late = receipt > commit
+    q = SelectQuery([Aggregate("count", "*")], where=[("l_receiptdate", ">", Column("l_commitdate"))])
+    assert execute_dispatched(q, t).scalar() == int(late.sum())
+    q = SelectQuery([Aggregate("sum", "qty")],
+                    where=[("qty", "<", 24), ("l_receiptdate", ">", Column("l_commitdate"))])
+    assert execute_dispatched(q, t).scalar() == int(qty[late & (qty < 24)].sum())
+    q = SelectQuery([Aggregate("count", "*")], where=[("l_receiptdate", "<=", Column("l_commitdate")),
+                                                     ("l_commitdate", ">=", 100)])
+    assert execute_dispatched(q, t).scalar() == int(((receipt <= commit) & (commit >= 100)).sum())

@@ -116,3 +116,55 @@ def test_scan_rejects_bad_arguments(ctx):
     with pytest.raises(A.TgpError):
         ctx.scan_filter([col], [(0, EQ, 1, True)], 0, 11)   # row_end beyond the column
     col.free()
+
+
+def _cells(a):
+    a = np.asarray(a)
+    return a.astype(np.int64).view(np.uint64) if a.dtype.kind == "i" else a.astype(np.uint64)
+
+
+@pytest.mark.parametrize("da,db", [(np.int64, np.int64), (np.int32, np.int64), (np.uint64, np.uint64),
+                                   (np.int8, np.uint8), (np.uint32, np.int16)])
+def test_compare_columns_vs_closed_form(ctx, da, db):
+    """tgp_scan_compare_columns (expr.rs:1382-1451 Col-op-Col): bit i stays set only
+    where a op b, signed or u64 compare of the widened cells; narrows a selection;
+    ragged and unaligned ranges, extremes of both widths."""
+    from turbogp_amd import _capi as A
+    rng = np.random.default_rng(np.dtype(da).itemsize * 10 + np.dtype(db).itemsize)
+    n = 100_003
+    ia, ib = np.iinfo(da), np.iinfo(db)
+    a = rng.integers(max(ia.min, -40), min(ia.max, 40), n, endpoint=True).astype(da)
+    b = rng.integers(max(ib.min, -40), min(ib.max, 40), n, endpoint=True).astype(db)
+    a[::17], b[::19] = ia.max, ib.min
+    a[5::23], b[7::29] = ia.min, ib.max
+    b[::31] = a[::31].astype(db) if np.can_cast(da, db) else b[::31]     # some ties
+    ca, cb = _cells(a), _cells(b)
+    dev_a, dev_b = DeviceColumn.from_numpy(ctx, a), DeviceColumn.from_numpy(ctx, b)
+    ops = {A.EQ: np.equal, A.NE: np.not_equal, A.LT: np.less, A.LE: np.less_equal,
+           A.GT: np.greater, A.GE: np.greater_equal}
+    try:
+        for signed in (True, False):
+            x = ca.view(np.int64) if signed else ca
+            y = cb.view(np.int64) if signed else cb
+            for op, f in ops.items():
+                for b0, e0 in ((0, n), (37, n - 11), (64, 64 + 640), (5, 5)):
+                    bm = ctx.compare_columns(dev_a, op, dev_b, signed, b0, e0)
+                    try:
+                        want = f(x[b0:e0], y[b0:e0])
+                        got = np.unpackbits(bm.words().view(np.uint8), bitorder="little")
+                        assert np.array_equal(got[:e0 - b0].astype(bool), want), (op, signed, b0)
+                        assert not got[e0 - b0:].any()          # tail bits cleared
+                    finally:
+                        bm.free()
+            # narrowing an existing selection
+            sel, _ = ctx.scan_filter([dev_a], [(0, A.GE, 0, True)], 0, n)
+            try:
+                ctx.compare_columns(dev_a, A.LT, dev_b, signed, 0, n, sel)
+                got = np.unpackbits(sel.words().view(np.uint8), bitorder="little")[:n].astype(bool)
+                want = (ca.view(np.int64) >= 0) & ((x < y))
+                assert np.array_equal(got, want), signed
+            finally:
+                sel.free()
+    finally:
+        dev_a.free()
+        dev_b.free()

@@ -240,6 +240,8 @@ SIGNATURES = {
     "tgp_hash_probe_pairs_sync": (I32_, [P, P, C.POINTER(Column), U64_, U64_, P, P, U64_,
                                          C.POINTER(U64_), P]),
     "tgp_hash_probe_agg": (I32_, [P, C.POINTER(JoinAggQuery), C.POINTER(GroupResult), P]),
+    "tgp_scan_compare_columns": (I32_, [P, C.POINTER(Column), I32_, C.POINTER(Column), I32_,
+                                        U64_, U64_, P, P]),
     "tgp_hash_build_multi": (I32_, [P, C.POINTER(Column), I32_, C.POINTER(Column),
                                     C.POINTER(C.c_void_p), P]),
     "tgp_hash_join_pairs_sync": (I32_, [P, P, C.POINTER(Column), I32_, I32_, U64_, U64_, P, P,

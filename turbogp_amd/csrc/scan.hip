@@ -75,6 +75,55 @@ __global__ void __launch_bounds__(256) k_bitmap_combine(int op, const uint64_t* 
   }
 }
 
+// A cell widened to 64 bits by its column's signedness (the reference's u64 cells).
+__device__ __forceinline__ uint64_t cell64(const void* base, int32_t dtype, uint64_t i) {
+  switch (dtype) {
+    case TGP_I8: return (uint64_t)(int64_t)static_cast<const int8_t*>(base)[i];
+    case TGP_U8: return static_cast<const uint8_t*>(base)[i];
+    case TGP_I16: return (uint64_t)(int64_t)static_cast<const int16_t*>(base)[i];
+    case TGP_U16: return static_cast<const uint16_t*>(base)[i];
+    case TGP_I32: return (uint64_t)(int64_t)static_cast<const int32_t*>(base)[i];
+    case TGP_U32: return static_cast<const uint32_t*>(base)[i];
+    default: return static_cast<const uint64_t*>(base)[i];
+  }
+}
+
+template <bool SIGNED>
+__device__ __forceinline__ bool compare_cells(int32_t op, uint64_t x, uint64_t y) {
+  const bool lt = SIGNED ? (int64_t)x < (int64_t)y : x < y;
+  const bool gt = SIGNED ? (int64_t)x > (int64_t)y : x > y;
+  switch (op) {
+    case TGP_EQ: return x == y;
+    case TGP_NE: return x != y;
+    case TGP_LT: return lt;
+    case TGP_LE: return !gt;
+    case TGP_GT: return gt;
+    default: return !lt;  // TGP_GE
+  }
+}
+
+// Column-vs-column narrowing of a selection: wave w handles bitmap word w (64
+// rows, one per lane) and keeps bit j only where a[r] op b[r],
+// r = row_begin + 64 w + j; bits past n are cleared.
+template <bool SIGNED>
+__global__ void __launch_bounds__(256) k_compare_columns(const void* a, int32_t adt, const void* b,
+                                                         int32_t bdt, int32_t op,
+                                                         uint64_t row_begin, uint64_t n,
+                                                         uint64_t* __restrict__ bitmap) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t nwords = (n + 63) / 64;
+  const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x / 64);
+  for (uint64_t w = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / 64; w < nwords;
+       w += nwaves) {
+    const uint64_t i = w * 64 + lane;
+    bool keep = false;
+    if (i < n)
+      keep = compare_cells<SIGNED>(op, cell64(a, adt, row_begin + i), cell64(b, bdt, row_begin + i));
+    const uint64_t bits = __ballot(keep);
+    if (lane == 0) bitmap[w] &= bits;
+  }
+}
+
 __global__ void __launch_bounds__(256) k_bitmap_not(const uint64_t* __restrict__ a, uint64_t* out,
                                                     uint64_t nbits) {
   const uint64_t nw = (nbits + 63) / 64;
@@ -320,6 +369,32 @@ int32_t tgp_bitmap_not(tgp_ctx* ctx, const uint64_t* d_a, uint64_t* d_out, uint6
   if (nbits == 0) return TGP_OK;
   k_bitmap_not<<<grid_for((nbits + 63) / 64), 256, 0, pick_stream(ctx, stream)>>>(d_a, d_out,
                                                                                  nbits);
+  TGP_HIP(hipGetLastError());
+  return TGP_OK;
+}
+
+int32_t tgp_scan_compare_columns(tgp_ctx* ctx, const tgp_column* a, int32_t op,
+                                 const tgp_column* b, int32_t is_signed, uint64_t row_begin,
+                                 uint64_t row_end, uint64_t* d_bitmap, void* stream) {
+  TGP_REQUIRE(ctx, "tgp_scan_compare_columns: null ctx");
+  TGP_TRY(check_injected(ctx));
+  TGP_TRY(check_column(a, "tgp_scan_compare_columns left column"));
+  TGP_TRY(check_column(b, "tgp_scan_compare_columns right column"));
+  TGP_REQUIRE(op >= TGP_EQ && op <= TGP_GE, "tgp_scan_compare_columns: op %d", op);
+  TGP_REQUIRE(row_begin <= row_end && row_end <= a->len && row_end <= b->len,
+              "tgp_scan_compare_columns: rows [%llu, %llu) outside the columns",
+              (unsigned long long)row_begin, (unsigned long long)row_end);
+  const uint64_t n = row_end - row_begin;
+  if (n == 0) return TGP_OK;
+  TGP_REQUIRE(d_bitmap, "tgp_scan_compare_columns: null bitmap");
+  const unsigned g = grid_for((n + 63) / 64 * 64);
+  hipStream_t st = pick_stream(ctx, stream);
+  if (is_signed)
+    k_compare_columns<true><<<g, 256, 0, st>>>(a->data, a->dtype, b->data, b->dtype, op,
+                                               row_begin, n, d_bitmap);
+  else
+    k_compare_columns<false><<<g, 256, 0, st>>>(a->data, a->dtype, b->data, b->dtype, op,
+                                                row_begin, n, d_bitmap);
   TGP_HIP(hipGetLastError());
   return TGP_OK;
 }

@@ -253,6 +253,28 @@ class Context:
                 cnt.free()
         return bm, c
 
+    def compare_columns(self, a, op, b, is_signed, row_begin=0, row_end=None, bitmap=None,
+                        stream=None):
+        """Rows of [row_begin, row_end) where a op b (tgp_scan_compare_columns): narrows
+        `bitmap` in place (bit i <-> row_begin + i) or, without one, returns a new
+        DeviceBitmap of the comparison alone."""
+        row_end = min(a.n, b.n) if row_end is None else row_end
+        own = bitmap is None
+        bm = DeviceBitmap(self, row_end - row_begin) if own else bitmap
+        try:
+            if own:
+                bm.buf.fill_u64(~0, nwords=bm.nwords, stream=stream)
+            da, db = a.desc(), b.desc()
+            A.check(A.lib().tgp_scan_compare_columns(self.handle, C.byref(da), op, C.byref(db),
+                                                     int(bool(is_signed)), row_begin, row_end,
+                                                     bm.buf.ptr, stream),
+                    "tgp_scan_compare_columns")
+        except Exception:
+            if own:
+                bm.free()
+            raise
+        return bm
+
     # ---- COUNT(DISTINCT col) ------------------------------------------------------------
     def count_distinct(self, col, row_begin=0, row_end=None, sel_bitmap=None,
                        mode=A.DISTINCT_AUTO, stream=None):

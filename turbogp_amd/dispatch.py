@@ -80,7 +80,8 @@ class Star:
 @dataclass
 class SelectQuery:
     select: List[object]
-    where: List[Tuple[str, str, int]] = field(default_factory=list)  # (column, op, literal)
+    # (column, op, literal) or (column, op, Column(other)): an AND of comparisons
+    where: List[Tuple[str, str, object]] = field(default_factory=list)
     group_by: List[str] = field(default_factory=list)
     order_by: List[Tuple[str, bool]] = field(default_factory=list)  # (column, ascending)
     limit: Optional[int] = None
@@ -325,18 +326,28 @@ class _Executor:
     def mask(self):
         if not self.q.where:
             return None, self.t.row_count
-        names = sorted({c for c, _, _ in self.q.where})
-        cols = [self.t.col(c) for c in names]
-        preds = []
-        for c, op, lit in self.q.where:
+        signed = (A.I8, A.I16, A.I32, A.I64)
+        for _, op, _ in self.q.where:
             if op not in OPS:
                 raise ValueError(f"unsupported comparison {op!r}")
-            dt = self.t.col(c).dtype
-            preds.append((names.index(c), OPS[op], int(lit), dt in (A.I8, A.I16, A.I32, A.I64)))
+        lits = [w for w in self.q.where if not isinstance(w[2], Column)]
+        pairs = [w for w in self.q.where if isinstance(w[2], Column)]
+        names = sorted({c for c, _, _ in lits})
+        cols = [self.t.col(c) for c in names]
+        preds = [(names.index(c), OPS[op], int(lit), self.t.col(c).dtype in signed)
+                 for c, op, lit in lits]
         bm, n = self.ctx.scan_filter(cols, preds, 0, self.t.row_count)
         self.owned.append(bm)
+        # column-vs-column comparisons narrow the selection (expr.rs:1382-1451:
+        # signed for signed columns, as INT / DATE compare)
+        for c, op, other in pairs:
+            a, b = self.t.col(c), self.t.col(other.name)
+            self.ctx.compare_columns(a, OPS[op], b, a.dtype in signed, 0, self.t.row_count, bm)
+        if pairs:
+            n = bm.count_ones()
         # a comparison with a NULL cell is not true
-        bm, n = self.without_nulls(bm, n, names)
+        used = names + [c for c, _, _ in pairs] + [o.name for _, _, o in pairs]
+        bm, n = self.without_nulls(bm, n, used)
         return bm, n
 
     def without_nulls(self, bm, n, columns):
