@@ -32,3 +32,11 @@ def test_other_shapes():
     assert classify_query(SelectQuery([Column("a")])) == QueryShape.SelectColumn
     assert classify_query(SelectQuery([Column("a"), Column("b")])) == QueryShape.SelectMulti
     assert classify_query(SelectQuery([Column("a"), Aggregate("sum", "b")])) == QueryShape.Complex
+
+
+def test_having_is_not_dispatched():
+    """dispatch.rs:329-331: a query with HAVING goes to the interpreter (None)."""
+    from turbogp_amd.dispatch import Aggregate, Column, SelectQuery, execute_dispatched
+    q = SelectQuery([Column("dept"), Aggregate("count", "*")], group_by=["dept"],
+                    having=(Aggregate("count", "*"), ">", 2))
+    assert execute_dispatched(q, None) is None

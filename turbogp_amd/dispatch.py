@@ -27,7 +27,7 @@ SUM, AVG, MIN, MAX and COUNT(DISTINCT) skip NULL cells of their columns
 per group; a comparison with a NULL cell is false in WHERE (SQL; the dispatch
 path compares the stored cell); SELECT results carry a null_mask.
 """
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 from decimal import Decimal
 from enum import Enum
 from typing import List, Optional, Sequence, Tuple
@@ -85,6 +85,10 @@ class SelectQuery:
     group_by: List[str] = field(default_factory=list)
     order_by: List[Tuple[str, bool]] = field(default_factory=list)  # (column, ascending)
     limit: Optional[int] = None
+    # HAVING <aggregate or result column> <op> <literal> (accum_grouped.rs HavingLit):
+    # an Aggregate (matched to the SELECT list, or computed for the filter alone) or
+    # the name of a result column
+    having: Optional[Tuple[object, str, object]] = None
 
 
 @dataclass
@@ -202,11 +206,57 @@ def classify_query(q: SelectQuery) -> QueryShape:
 
 
 def execute_dispatched(q: SelectQuery, t: Table) -> Optional[QueryResult]:
-    """dispatch.rs:304-333: None for shapes the dispatcher does not run."""
+    """dispatch.rs:304-333: None for shapes the dispatcher does not run, and for any
+    query with HAVING (dispatch.rs:329-331 sends those to the interpreter)."""
     shape = classify_query(q)
-    if shape == QueryShape.Complex:
+    if shape == QueryShape.Complex or q.having is not None:
         return None
     return _Executor(q, t).run(shape)
+
+
+def execute_grouped(q: SelectQuery, t: Table) -> QueryResult:
+    """The interpreter's accumulator path for a GROUP BY of one column
+    (accum_grouped.rs:55-409 try_accumulator_grouped): the groups and their exact
+    aggregates on the device, then HAVING <aggregate> <op> <literal> (HavingLit,
+    :148-160, :339-349), then ORDER BY and LIMIT, in that order."""
+    if not q.group_by:
+        raise ValueError("execute_grouped needs a GROUP BY")
+    hidden = False
+    if q.having is not None and isinstance(q.having[0], Aggregate):
+        # HAVING on an aggregate: its result column, computed for the filter only
+        # when the SELECT list does not hold it
+        h = q.having[0]
+        same = [s for s in q.select if isinstance(s, Aggregate) and
+                s.func.upper() == h.func.upper() and s.arg == h.arg]
+        if same:
+            name = same[0].alias or same[0].func.lower()
+        else:
+            name, hidden = "__having", True
+            q = replace(q, select=list(q.select) + [Aggregate(h.func, h.arg, name)])
+        q = replace(q, having=(name, q.having[1], q.having[2]))
+    ex = _Executor(q, t)
+    try:
+        bm, _ = ex.mask()
+        r = ex.group_by(bm)
+    finally:
+        ex.free()
+    if hidden:
+        r = QueryResult([c for c in r.columns if c.name != "__having"], r.row_count)
+    return r
+
+
+def _signed_value(c, i):
+    """A result cell as a number: NUMERIC text as Decimal, an INT8 cell as a signed
+    64-bit integer."""
+    if c.string_values:
+        return Decimal(c.string_values[i])
+    v = int(c.values[i])
+    return v - (1 << 64) if v >= (1 << 63) else v
+
+
+_HAVING_OPS = {"=": lambda a, b: a == b, "!=": lambda a, b: a != b, "<>": lambda a, b: a != b,
+               "<": lambda a, b: a < b, "<=": lambda a, b: a <= b, ">": lambda a, b: a > b,
+               ">=": lambda a, b: a >= b}
 
 
 @dataclass
@@ -505,14 +555,22 @@ class _Executor:
                     cols.append(ResultColumn(name, [sums[g][k] % 2**64 for g in gids], None,
                                              INT8_OID))
         rows = list(range(len(gids)))
+        names = [c.name for c in cols]
+        if q.having is not None:
+            hname, hop, hlit = q.having
+            if hname not in names:
+                raise ValueError(f"HAVING {hname!r} is not a result column")
+            if hop not in _HAVING_OPS:
+                raise ValueError(f"unsupported HAVING comparison {hop!r}")
+            hc = cols[names.index(hname)]
+            lit = Decimal(str(hlit))
+            rows = [i for i in rows if _HAVING_OPS[hop](_signed_value(hc, i), lit)]
         if q.order_by:
             oname, asc = q.order_by[0]
-            names = [c.name for c in cols]
             if oname not in names:
                 raise ValueError(f"ORDER BY {oname!r} is not a result column")
             c = cols[names.index(oname)]
-            keyf = (lambda i: Decimal(c.string_values[i]) if c.string_values else c.values[i])
-            rows = sorted(rows, key=keyf, reverse=not asc)
+            rows = sorted(rows, key=lambda i: _signed_value(c, i), reverse=not asc)
         if q.limit is not None:
             rows = rows[: q.limit]
         for c in cols:
