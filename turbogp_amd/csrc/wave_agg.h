@@ -30,10 +30,16 @@ inline int wave_rshift(uint64_t words) {
   return r;
 }
 
-__device__ __forceinline__ bool big43(uint64_t v) {
-  const int64_t s = (int64_t)v;
-  return (((uint64_t)(s ^ (s >> 63))) >> kWaveBigBits) != 0;
+// Folds value v into m: (m >> 12) != 0 afterwards when some folded value lies
+// outside [-2^43, 2^43).  v is inside exactly when its high word plus 2^11 is
+// below 2^12 as an unsigned 32-bit value (2^43 has a zero low word, so the low
+// word never carries into it): two plain VALU ops per value, where a 64-bit
+// compare per value costs a compare, a hazard wait and a select.
+static_assert(kWaveBigBits == 43, "big_fold tests the high word against 2^(43-32)");
+__device__ __forceinline__ void big_fold(uint32_t& m, uint64_t v) {
+  m |= (uint32_t)(v >> 32) + (1u << (kWaveBigBits - 32));
 }
+__device__ __forceinline__ bool big_any(uint32_t m) { return (m >> (kWaveBigBits - 31)) != 0; }
 
 // One slice of a query of shape S into the wave-private region `reg`: the
 // values `v` of the slice starting at query-relative slice `slice` are loaded.
@@ -64,10 +70,12 @@ __device__ __forceinline__ void wave_slice(const SpecArgs& A, uint64_t slice,
     constexpr int a = decltype(AI)::value;
     uint64_t val[kSR];
     agg_values<S, a>(A, v, val);
-    bool b = false;
+    // every row of the slice is tested, selected or not: a large unselected
+    // value only sends the slice to the exact path
+    uint32_t m = 0;
 #pragma unroll
-    for (int r = 0; r < kSR; ++r) b |= ((sel >> r) & 1) && big43(val[r]);
-    if (__ballot(b)) {
+    for (int r = 0; r < kSR; ++r) big_fold(m, val[r]);
+    if (__ballot(big_any(m))) {
 #pragma unroll
       for (int r = 0; r < kSR; ++r)
         if ((sel >> r) & 1)
