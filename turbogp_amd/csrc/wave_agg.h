@@ -63,11 +63,18 @@ __device__ __forceinline__ void wave_slice(const SpecArgs& A, uint64_t slice,
   const uint32_t sel = eval_slice<S>(A, row, v, gid, oor);
   if (!__ballot(sel != 0)) return;
   constexpr int W1 = 1 + S.naggs;
+  // word w of the lane's copy is mine[w << rshift]: a row's words start at
+  // rb = (gid * W1) << rshift and aggregate a's word is rb + ((1 + a) << rshift),
+  // the second term the same for the whole wave
+  uint32_t rb[kSR];
 #pragma unroll
-  for (int r = 0; r < kSR; ++r)
-    if ((sel >> r) & 1) atomicAdd(&mine[(uint32_t)(gid[r] * W1) << rshift], 1ull);
+  for (int r = 0; r < kSR; ++r) {
+    rb[r] = (uint32_t)(gid[r] * W1) << rshift;
+    if ((sel >> r) & 1) atomicAdd(&mine[rb[r]], 1ull);
+  }
   static_for<S.naggs>([&](auto AI) {
     constexpr int a = decltype(AI)::value;
+    const uint32_t off = (uint32_t)(1 + a) << rshift;
     uint64_t val[kSR];
     agg_values<S, a>(A, v, val);
     // every row of the slice is tested, selected or not: a large unselected
@@ -83,8 +90,7 @@ __device__ __forceinline__ void wave_slice(const SpecArgs& A, uint64_t slice,
     } else {
 #pragma unroll
       for (int r = 0; r < kSR; ++r)
-        if ((sel >> r) & 1)
-          atomicAdd(&mine[(uint32_t)(gid[r] * W1 + 1 + a) << rshift], (unsigned long long)val[r]);
+        if ((sel >> r) & 1) atomicAdd(&mine[rb[r] + off], (unsigned long long)val[r]);
     }
   });
 }
