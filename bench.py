@@ -79,43 +79,91 @@ def host_cpu():
             f"{logical} logical CPUs, {usable} usable by this process")
 
 
-def cpu_baseline(params, rows_cap, reps=3):
-    """The oracle's restatement of the reference CPU path (filter passes ->
-    bitmap -> compaction -> chunked hash grouping -> f64 sum_vec), all host
-    cores, on the first batch queries up to `rows_cap` rows; reported as the
-    queries/s the same per-row rate gives for the whole batch."""
+def _cells(arr):
+    """A column as the reference stores it: one u64 cell per value (csv.rs:144-161;
+    signed values sign-extended)."""
+    return arr.astype(np.int64 if arr.dtype.kind == "i" else np.uint64)
+
+
+def _threads():
+    return (len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+            else os.cpu_count()) or 1
+
+
+def cpu_baseline(params, rows_cap, join_tables=None, reps=3):
+    """The reference's CPU path restated in the oracle (u64 cells; per-leaf filter
+    passes, AVX-512 4x8 where the host has it; tzcnt compaction; 65 536-row chunk
+    hash grouping; f64 sum_vec; the join as chained table + materialisation +
+    accumulator grouping), on every host core, timed around filter -> aggregate
+    (column generation excluded):
+      C4  the first batch queries up to `rows_cap` rows, one run, as queries/s
+          scaled by rows to the whole batch (the headline `value`);
+      C2a / C2b  the two scan-aggregates over 60M rows, median of `reps`;
+      C3  the two join aggregates (2M build, 60M probe), one run each."""
     from oracle import binding as B
     from turbogp_amd import workloads as W
+    threads = _threads()
+    gen = lambda c, b, e: _cells(B.synth_uniform(W.FACT_SCHEMA[c][1], W.SEED, W.FACT_SCHEMA[c][0],
+                                                 W.FACT_SCHEMA[c][2], W.FACT_SCHEMA[c][3], b, e))
+    # ---- C4 sample
     sample, rows = [], 0
     for p in params:
         if rows + (p[1] - p[0]) > rows_cap and sample:
             break
         sample.append(p)
         rows += p[1] - p[0]
-    threads = (len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
-               else os.cpu_count()) or 1
-    # host columns of each sampled range, generated by the same counter-based generator
-    queries = []
+    t4 = 0.0
     for (b, e, cutoff, g) in sample:
         names, preds, parts, aggs = W.q1_spec(date_cutoff=cutoff, group=g)
-        cols = [B.synth_uniform(W.FACT_SCHEMA[c][1], W.SEED, W.FACT_SCHEMA[c][0],
-                                W.FACT_SCHEMA[c][2], W.FACT_SCHEMA[c][3], b, e) for c in names]
-        queries.append(B.make_query(cols, preds, parts, aggs))
-    times = []
-    for _ in range(reps):
+        q = B.make_query([gen(c, b, e) for c in names], preds, parts, aggs)
         t0 = time.perf_counter()
-        for q in queries:
-            B.group_agg(q, mode=1, nthreads=threads)
-        times.append(time.perf_counter() - t0)
-    t = statistics.median(times)
+        B.group_agg(q, mode=1, nthreads=threads)
+        t4 += time.perf_counter() - t0
     total_rows = sum(p[1] - p[0] for p in params)
-    batch_s = t * total_rows / rows
+    batch_s = t4 * total_rows / rows
+    log(f"cpu C4: {len(sample)} queries, {rows} rows, {t4:.2f} s")
+    legs = {}
+    # ---- C2a / C2b over 60M rows
+    host = {c: gen(c, 0, SCAN_ROWS) for c in W.FACT_SCHEMA}
+    for name, spec, bpr in (("c2a_q1_groupby_60M", W.q1_spec(), W.Q1_BYTES_PER_ROW),
+                            ("c2b_q6_scan_sum_60M", W.q6_spec(), W.Q6_BYTES_PER_ROW)):
+        names, preds, parts, aggs = spec
+        q = B.make_query([host[c] for c in names], preds, parts, aggs)
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            B.group_agg(q, mode=1, nthreads=threads)
+            ts.append(time.perf_counter() - t0)
+        t = statistics.median(ts)
+        log(f"cpu {name}: {t * 1e3:.1f} ms")
+        legs[name] = {"ms": round(t * 1e3, 2), "rows_per_s": round(SCAN_ROWS / t, 1),
+                      "GB_s": round(SCAN_ROWS * bpr / 1e9 / t, 2), "runs": reps}
+    del host
+    # ---- C3 join aggregates
+    if join_tables is not None:
+        from turbogp_amd.joinload import join_bytes
+        build, probe = join_tables
+        bk, bp = build["bkey"].to_numpy(), build["pay"].to_numpy()
+        fk, val, gk = probe["fk"].to_numpy(), probe["val"].to_numpy(), probe["gkey"].to_numpy()
+        jb = join_bytes(len(bk), len(fk))
+        for name, gb, ng, key in (("c3_group_by_payload", 0, W.JOIN_PAY_CARD, "by_payload"),
+                                  ("c3_group_by_probe_key", 1, W.JOIN_GKEY_CARD, "by_gkey")):
+            t0 = time.perf_counter()
+            cnt, _, _ = B.join_agg(bk, bp, fk, val, gk, gb, 0, ng, nthreads=threads)
+            t = time.perf_counter() - t0
+            assert int(cnt.sum()) == len(fk)
+            log(f"cpu {name}: {t * 1e3:.1f} ms")
+            legs[name] = {"ms": round(t * 1e3, 1), "probes_per_s": round(len(fk) / t, 1),
+                          "GB_s": round(jb[key] / 1e9 / t, 2), "runs": 1}
+    simd = B.lib().orc_simd_level()
     return {"value": round(len(params) / batch_s, 2), "unit": "queries/s", "cores": threads,
-            "kind": "port", "host": host_cpu(),
-            "sample": f"first {len(sample)} of the {len(params)} batch queries ({rows} rows), "
-                      f"median of {reps}: {t:.3f} s, scaled by rows to the whole batch "
-                      f"({total_rows} rows, {batch_s:.1f} s); reference algorithm (u64 "
-                      f"cells, f64 sums) in the oracle, built -O3 -march=x86-64-v3 -fopenmp"}
+            "kind": "port", "host": host_cpu(), "simd": "avx512" if simd >= 512 else "scalar",
+            "sample": f"C4: first {len(sample)} of the {len(params)} batch queries ({rows} rows, "
+                      f"{t4:.2f} s), scaled by rows to the whole batch ({total_rows} rows, "
+                      f"{batch_s:.1f} s); legs: C2a/C2b over 60M rows, C3 2M x 60M; the "
+                      f"reference algorithm (u64 cells, f64 sums) in the oracle, built -O3 "
+                      f"-march=x86-64-v3 -fopenmp with the AVX-512 filter chosen at run time",
+            "legs": legs}
 
 
 def measured_traffic(name, rows):
@@ -393,7 +441,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--per-query-reps", type=int, default=3)
-    ap.add_argument("--cpu-rows", type=int, default=40_000_000)
+    ap.add_argument("--cpu-rows", type=int, default=300_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=2000)
@@ -514,6 +562,13 @@ def main():
         join = None
 
     achieved = batch_bytes / 1e9 / (ms_batch / 1e3)
+    # roofline of the batch kernel: the bytes the batch must move at least (the union of
+    # the cells its queries read, each once) over the launch time; per-query bytes
+    # count a row once per query that covers it and so exceed what the kernel reads
+    achieved_unique = unique_bytes / 1e9 / (ms_batch / 1e3)
+    frac = achieved_unique / HBM_PEAK_GBS
+    assert frac <= 1.0, f"roofline fraction {frac} above 1: the byte count is wrong"
+    traffic = measured_traffic("k_batch_4096q_256M", n)
     nq = len(queries)
     out = {
         "metric": METRIC,
@@ -561,20 +616,24 @@ def main():
         "count_distinct_60M": distinct,
         "select_60M": select,
         "serving": serve,
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic("k_batch_4096q_256M", n),
+        "roofline": {"bound": "hbm", "achieved": round(achieved_unique, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(frac, 4), "traffic": traffic,
+                     "bytes_per_launch": int(unique_bytes),
+                     "traffic_over_bytes": None if traffic is None else
+                     round(traffic / unique_bytes, 3),
+                     "per_query_bytes_per_launch": int(batch_bytes),
                      "kernel": "persistent batch kernel (k_batch_items in table order, or "
-                               "k_batch in query order, as autotuned): algorithmic bytes of the "
-                               "4096 queries per launch (a row read by several queries counts "
-                               "each time) / mean launch time (hipEvents on the batch stream, "
-                               "graph replay incl. the result-zeroing fill); unique_bytes in "
-                               "'batch' is the union the queries read, traffic the HBM bytes "
-                               "FETCH_SIZE / WRITE_SIZE measured"},
+                               "k_batch in query order, as autotuned): the cells the batch reads, "
+                               "each once (per column, the union of the row ranges of the "
+                               "queries that read it, x its width) / mean launch time (hipEvents "
+                               "on the batch stream, graph replay incl. the result-zeroing "
+                               "fill); traffic = HBM bytes per launch from FETCH_SIZE / "
+                               "WRITE_SIZE; per_query_bytes counts a row once per query that "
+                               "covers it"},
     }
     if not args.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(params, args.cpu_rows)
+        out["cpu_baseline"] = cpu_baseline(params, args.cpu_rows, join_tables)
     ctx.close()
     print(json.dumps(out), flush=True)
 

@@ -218,10 +218,15 @@ enum tgp_tune_key {
                                    low n bits (1..63) of the packed key, so that distinct
                                    keys collide and the full-key verification is exercised
                                    (tests); 0 (default): all 64 bits */
-  TGP_TUNE_DISTINCT_SHRINK = 8  /* exact COUNT(DISTINCT) over >= 2^20 rows sizes its set from
+  TGP_TUNE_DISTINCT_SHRINK = 8, /* exact COUNT(DISTINCT) over >= 2^20 rows sizes its set from
                                    the HyperLogLog estimate; n > 0 divides that size by 2^n
                                    (0..30), so that the set overflows and the count is redone
                                    in a set sized by the row count (tests); 0 (default) */
+  TGP_TUNE_GROUP_TAG_BITS = 9   /* hash GROUP BY over several key columns: n in 1..32 makes
+                                   every such query take the hashed form with full-key
+                                   verification and keeps only n bits of each slot's hash tag,
+                                   so that distinct tuples collide (tests); 0 (default): exact
+                                   packing when the key ranges fit 63 bits */
 };
 int32_t tgp_tune_set(tgp_ctx* ctx, int32_t key, int64_t value);
 int32_t tgp_tune_get(tgp_ctx* ctx, int32_t key, int64_t* value);
@@ -313,6 +318,50 @@ int32_t tgp_group_agg(tgp_ctx* ctx, const tgp_agg_query* q, const tgp_group_resu
 int32_t tgp_group_agg_explain(const tgp_agg_query* q, int32_t* strategy, int32_t* spec_shape);
 /* Number of groups the query's key space spans (prod of card). */
 int32_t tgp_group_agg_ngroups(const tgp_agg_query* q, uint64_t* out_ngroups);
+
+/* ---- GROUP BY arbitrary integer key tuples (device hash table) ------------------ */
+/* The groups are the distinct key tuples of the selected rows, for any key values
+ * (sparse int64 ids, several columns, NULL keys): what the reference's accumulator
+ * path keys by a combined hash in per-chunk FxHashMaps merged per key
+ * (query_interpreter/accum_grouped.rs:198-335, aggregate.rs:582-627) and the
+ * dispatch path groups in an FxHashMap (engine/dispatch.rs:1125-1140 one key,
+ * :1377-1385 several).  Here groups are the true key tuples (no two tuples merge)
+ * and a NULL key is a value of its own: all rows whose key k is NULL share one
+ * group for that part.  Keys are compared as the reference's u64 cells (narrow
+ * columns widened by their signedness).
+ *
+ * `agg` gives the columns, the filter (preds AND the optional selection bitmap),
+ * the row range and the aggregates exactly as for tgp_group_agg; agg.nparts must
+ * be 0.  key_col[k] indexes agg.cols; d_key_nulls[k] (optional) is a NULL bitmap
+ * of key k: bit i set = the key of row row_begin + i is NULL. */
+#define TGP_MAX_GROUP_KEYS 4
+typedef struct tgp_hash_group_query {
+  tgp_agg_query agg;
+  int32_t nkeys;                          /* 1..TGP_MAX_GROUP_KEYS */
+  int32_t key_col[TGP_MAX_GROUP_KEYS];
+  int32_t _pad[3];
+  const uint64_t* d_key_nulls[TGP_MAX_GROUP_KEYS];
+} tgp_hash_group_query;
+/* Output, one row per group for up to `capacity` groups (device buffers), in no
+ * particular order:
+ *   d_keys[g * nkeys + k]   key k of group g as its u64 cell (0 when NULL)
+ *   d_key_nulls[g]          bit k set = key k of group g is NULL (may be NULL)
+ *   d_count[g], d_sums[(g * naggs + a) * 2 + {0,1}]  as tgp_group_result */
+typedef struct tgp_hash_group_result {
+  uint64_t* d_keys;
+  uint32_t* d_key_nulls;
+  uint64_t* d_count;
+  uint64_t* d_sums;
+  uint64_t capacity;
+} tgp_hash_group_result;
+/* Synchronous: *h_ngroups = number of groups written.  More than `capacity`
+ * groups: TGP_ERR_CAPACITY with *h_ngroups = the number of groups when the table
+ * held them all, else a lower bound above capacity (call again with at least
+ * that capacity).  Device memory for the table: about 16 * capacity * (1 + 2 *
+ * naggs) bytes of the stream's scratch. */
+int32_t tgp_hash_group_agg_sync(tgp_ctx* ctx, const tgp_hash_group_query* q,
+                                const tgp_hash_group_result* out, uint64_t* h_ngroups,
+                                void* stream);
 
 /* ---- batched executor (many queries, one persistent launch) --------------- */
 /* A batch holds up to max_queries tgp_agg_query descriptors.  PERSISTENT mode

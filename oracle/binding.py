@@ -75,6 +75,9 @@ def lib():
             "orc_hll_registers": (None, [P, I32, U64, P, P]),
             "orc_hll_estimate": (U64, [P]),
             "orc_count_distinct": (U64, [P, I32, U64, P, I32]),
+            "orc_group_by_keys": (I64, [C.POINTER(OQuery), I32, P, P, U64, P, P, P, P]),
+            "orc_simd_level": (I32, []),
+            "orc_set_simd": (None, [I32]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -205,6 +208,42 @@ def group_agg(q, mode=0, nthreads=0):
     else:
         s = [[float(f64[g * q.naggs + a]) for a in range(q.naggs)] for g in range(ng)]
     return cnt, s, int(oor[0])
+
+
+def group_by_keys(q, key_cols, key_nulls=None, signed=None):
+    """GROUP BY the key tuple of q.cols[key_cols] (orc_group_by_keys): a dict
+    {key tuple (None for a NULL part; signed[k] reads key k as int64): (count, [sums])}
+    with exact Python-int sums (MIN/MAX: the value).  key_nulls[k]: bitmap words
+    (bit i <-> row row_begin + i) or None."""
+    nk = len(key_cols)
+    kc = np.asarray(key_cols, np.int32)
+    nb = [None if b is None else np.ascontiguousarray(b, np.uint64)
+          for b in (key_nulls or [None] * nk)]
+    nbp = (C.c_void_p * nk)(*[ptr(b) if b is not None else None for b in nb])
+    G = lib().orc_group_by_keys(C.byref(q), nk, ptr(kc), C.cast(nbp, C.c_void_p), 0, None, None,
+                                None, None)
+    assert G >= 0, G
+    keys = np.zeros(max(1, G * nk), np.uint64)
+    nulls = np.zeros(max(1, G), np.uint32)
+    cnt = np.zeros(max(1, G), np.uint64)
+    sums = np.zeros(max(1, G * q.naggs * 2), np.uint64)
+    G2 = lib().orc_group_by_keys(C.byref(q), nk, ptr(kc), C.cast(nbp, C.c_void_p), G, ptr(keys),
+                                 ptr(nulls), ptr(cnt), ptr(sums))
+    assert G2 == G
+    signed = signed or [True] * nk
+    out = {}
+    for g in range(G):
+        t = []
+        for k in range(nk):
+            if (int(nulls[g]) >> k) & 1:
+                t.append(None)
+            else:
+                v = int(keys[g * nk + k])
+                t.append(v - (1 << 64) if signed[k] and v >> 63 else v)
+        out[tuple(t)] = (int(cnt[g]), [_i128(sums[(g * q.naggs + a) * 2],
+                                             sums[(g * q.naggs + a) * 2 + 1])
+                                       for a in range(q.naggs)])
+    return out
 
 
 def query_filter(q):

@@ -11,6 +11,7 @@
 #include "tgp_oracle.h"
 
 #include <math.h>
+#include <immintrin.h>
 #include <omp.h>
 #include <stdlib.h>
 #include <string.h>
@@ -97,12 +98,81 @@ void orc_widen_cells(const void* col, int32_t dtype, uint64_t n, uint64_t* cells
   for (uint64_t i = 0; i < n; ++i) cells[i] = (uint64_t)get_native(col, dtype, i);
 }
 
-/* filter_{eq,ne,lt,le,gt,ge}_{u64,i64} scalar paths, bitmap.rs:321-477:
- * bit i = cmp(cells[i], lit), LSB-first bytes (bitmap.rs:36-45). */
+/* filter_u64_avx512_inner (bitmap.rs:642-694): four 8-lane compares per step into
+ * four mask bytes, then one 8-lane step at a time, then the scalar tail.  Chosen at
+ * run time when the host has AVX-512F/BW, as the reference's filter_* pick their
+ * AVX-512 path with is_x86_feature_detected (the library is built in one place
+ * and run on another host, so the dispatch is at run time, not -march=native). */
+#define ORC_FILTER_LOOP(CMP, PRED)                                                   \
+  do {                                                                               \
+    while (i + 32 <= n) {                                                            \
+      const __m512i v0 = _mm512_loadu_si512((const void*)(cells + i));               \
+      const __m512i v1 = _mm512_loadu_si512((const void*)(cells + i + 8));           \
+      const __m512i v2 = _mm512_loadu_si512((const void*)(cells + i + 16));          \
+      const __m512i v3 = _mm512_loadu_si512((const void*)(cells + i + 24));          \
+      bits[b] = (uint8_t)CMP(v0, t, PRED);                                           \
+      bits[b + 1] = (uint8_t)CMP(v1, t, PRED);                                       \
+      bits[b + 2] = (uint8_t)CMP(v2, t, PRED);                                       \
+      bits[b + 3] = (uint8_t)CMP(v3, t, PRED);                                       \
+      i += 32;                                                                       \
+      b += 4;                                                                        \
+    }                                                                                \
+    while (i + 8 <= n) {                                                             \
+      bits[b++] = (uint8_t)CMP(_mm512_loadu_si512((const void*)(cells + i)), t, PRED); \
+      i += 8;                                                                        \
+    }                                                                                \
+  } while (0)
+
+__attribute__((target("avx512f,avx512bw"))) static uint64_t filter_cells_avx512(
+    const uint64_t* cells, uint64_t n, int32_t op, int32_t is_signed, int64_t lit, uint8_t* bits) {
+  const __m512i t = _mm512_set1_epi64(lit);
+  uint64_t i = 0, b = 0;
+  if (is_signed) {
+    switch (op) {
+      case 0: ORC_FILTER_LOOP(_mm512_cmp_epi64_mask, _MM_CMPINT_EQ); break;
+      case 1: ORC_FILTER_LOOP(_mm512_cmp_epi64_mask, _MM_CMPINT_NE); break;
+      case 2: ORC_FILTER_LOOP(_mm512_cmp_epi64_mask, _MM_CMPINT_LT); break;
+      case 3: ORC_FILTER_LOOP(_mm512_cmp_epi64_mask, _MM_CMPINT_LE); break;
+      case 4: ORC_FILTER_LOOP(_mm512_cmp_epi64_mask, _MM_CMPINT_NLE); break;
+      default: ORC_FILTER_LOOP(_mm512_cmp_epi64_mask, _MM_CMPINT_NLT); break;
+    }
+  } else {
+    switch (op) {
+      case 0: ORC_FILTER_LOOP(_mm512_cmp_epu64_mask, _MM_CMPINT_EQ); break;
+      case 1: ORC_FILTER_LOOP(_mm512_cmp_epu64_mask, _MM_CMPINT_NE); break;
+      case 2: ORC_FILTER_LOOP(_mm512_cmp_epu64_mask, _MM_CMPINT_LT); break;
+      case 3: ORC_FILTER_LOOP(_mm512_cmp_epu64_mask, _MM_CMPINT_LE); break;
+      case 4: ORC_FILTER_LOOP(_mm512_cmp_epu64_mask, _MM_CMPINT_NLE); break;
+      default: ORC_FILTER_LOOP(_mm512_cmp_epu64_mask, _MM_CMPINT_NLT); break;
+    }
+  }
+  return i; /* rows done; the caller finishes the tail */
+}
+
+static int g_use_avx512 = -1; /* -1: not probed yet; orc_set_simd overrides */
+
+int32_t orc_simd_level(void) {
+  if (g_use_avx512 < 0) {
+    __builtin_cpu_init();
+    g_use_avx512 = __builtin_cpu_supports("avx512f") && __builtin_cpu_supports("avx512bw");
+  }
+  return g_use_avx512 ? 512 : 0;
+}
+
+void orc_set_simd(int32_t level) {
+  g_use_avx512 = -1;
+  if (level < 512) g_use_avx512 = 0;
+  else (void)orc_simd_level(); /* only where the host has it */
+}
+
+/* filter_{eq,ne,lt,le,gt,ge}_{u64,i64}, bitmap.rs:321-477 (scalar) and :642-694
+ * (AVX-512): bit i = cmp(cells[i], lit), LSB-first bytes (bitmap.rs:36-45). */
 void orc_filter_cells(const uint64_t* cells, uint64_t n, int32_t op, int32_t is_signed, int64_t lit,
                       uint8_t* bits) {
   memset(bits, 0, (n + 7) / 8);
-  for (uint64_t i = 0; i < n; ++i) {
+  uint64_t start = 0;
+  if (orc_simd_level() >= 512) start = filter_cells_avx512(cells, n, op, is_signed, lit, bits);
+  for (uint64_t i = start; i < n; ++i) {
     int hit;
     if (is_signed) {
       const int64_t c = (int64_t)cells[i];
@@ -212,8 +282,14 @@ int32_t orc_query_filter(const orc_query* q, uint8_t* bits) {
   for (int p = 0; p < q->npreds; ++p) {
     const orc_column* c = &q->cols[q->preds[p].col];
     const int w = dt_width(c->dtype);
-    orc_widen_cells((const uint8_t*)c->data + q->row_begin * w, c->dtype, n, cells);
-    orc_filter_cells(cells, n, q->preds[p].op, q->preds[p].is_signed, q->preds[p].lit,
+    /* u64 cells (the reference's column layout) are filtered where they lie;
+     * narrower columns are widened to cells first */
+    const uint64_t* src = (const uint64_t*)((const uint8_t*)c->data + q->row_begin * w);
+    if (w != 8) {
+      orc_widen_cells((const uint8_t*)c->data + q->row_begin * w, c->dtype, n, cells);
+      src = cells;
+    }
+    orc_filter_cells(src, n, q->preds[p].op, q->preds[p].is_signed, q->preds[p].lit,
                      p == 0 ? bits : leaf);
     if (p > 0) orc_bitmap_and_inplace(bits, leaf, nb);
   }
@@ -434,6 +510,155 @@ int32_t orc_group_agg(const orc_query* q, int32_t mode, int32_t nthreads, uint64
   gmap_free(&global, 1);
   if (out_oor) *out_oor = oor;
   return 0;
+}
+
+/* ---- GROUP BY the key tuple itself -------------------------------------------------
+ * accum_grouped.rs:198-335 groups the selected rows per 65 536-row chunk in an
+ * FxHashMap keyed by the combined hash h = h * 0x517cc1b727220a95 + cell of the key
+ * cells (:213-218) and merges the chunk maps per key (:291-335); dispatch.rs:1377-1385
+ * does the same for a multi-key GROUP BY.  Here a group is the key TUPLE (so two
+ * tuples with one combined hash stay apart) and a NULL key part is a value of its
+ * own.  One map over all selected rows in row order; COUNT(*), exact int128 sums and
+ * MIN/MAX as orc_group_agg mode 0 computes them.  Groups are written in ascending
+ * tuple order: key k compared as int64 when key_signed[k], else as u64, NULL after
+ * every value. */
+typedef struct {
+  uint64_t key[4];
+  uint32_t nulls;
+  int used;
+  uint64_t count;
+  i128 sum[6];
+  uint64_t best[6];
+} tgrp_t;
+typedef struct { tgrp_t* s; uint64_t cap, n; } tmap_t;
+
+static uint64_t tuple_hash(const uint64_t* key, int nk, uint32_t nulls) {
+  uint64_t h = 0x6a09e667f3bcc908ull ^ nulls;
+  for (int k = 0; k < nk; ++k) h = orc_mix64(h ^ key[k]);
+  return h;
+}
+
+static tgrp_t* tmap_get(tmap_t* m, const uint64_t* key, int nk, uint32_t nulls);
+static void tmap_grow(tmap_t* m, int nk) {
+  tmap_t nm = {(tgrp_t*)calloc(m->cap * 2, sizeof(tgrp_t)), m->cap * 2, 0};
+  for (uint64_t i = 0; i < m->cap; ++i)
+    if (m->s[i].used) {
+      tgrp_t* g = tmap_get(&nm, m->s[i].key, nk, m->s[i].nulls);
+      *g = m->s[i];
+    }
+  free(m->s);
+  *m = nm;
+}
+static tgrp_t* tmap_get(tmap_t* m, const uint64_t* key, int nk, uint32_t nulls) {
+  if ((m->n + 1) * 2 > m->cap) tmap_grow(m, nk);
+  uint64_t i = tuple_hash(key, nk, nulls) & (m->cap - 1);
+  for (;;) {
+    tgrp_t* g = &m->s[i];
+    if (!g->used) {
+      memset(g, 0, sizeof(*g));
+      g->used = 1;
+      memcpy(g->key, key, sizeof(uint64_t) * (size_t)nk);
+      g->nulls = nulls;
+      m->n++;
+      return g;
+    }
+    if (g->nulls == nulls && !memcmp(g->key, key, sizeof(uint64_t) * (size_t)nk)) return g;
+    i = (i + 1) & (m->cap - 1);
+  }
+}
+
+static int g_sort_nk;
+static const int32_t* g_sort_signed;
+static int cmp_tuple(const void* pa, const void* pb) {
+  const tgrp_t* a = (const tgrp_t*)pa;
+  const tgrp_t* b = (const tgrp_t*)pb;
+  for (int k = 0; k < g_sort_nk; ++k) {
+    const int na = (a->nulls >> k) & 1, nb = (b->nulls >> k) & 1;
+    if (na != nb) return na - nb;
+    if (na) continue;
+    if (a->key[k] == b->key[k]) continue;
+    if (g_sort_signed[k]) return (int64_t)a->key[k] < (int64_t)b->key[k] ? -1 : 1;
+    return a->key[k] < b->key[k] ? -1 : 1;
+  }
+  return 0;
+}
+
+int64_t orc_group_by_keys(const orc_query* q, int32_t nkeys, const int32_t* key_cols,
+                          const uint64_t* const* key_nulls, uint64_t capacity, uint64_t* out_keys,
+                          uint32_t* out_nulls, uint64_t* out_count, uint64_t* out_sums) {
+  if (nkeys < 1 || nkeys > 4) return -1;
+  const uint64_t n = q->row_end - q->row_begin;
+  uint8_t* bits = (uint8_t*)malloc((n + 7) / 8 + 1);
+  uint64_t* idx = (uint64_t*)malloc((n ? n : 1) * 8);
+  if (!bits || !idx) { free(bits); free(idx); return -3; }
+  orc_query_filter(q, bits);
+  const uint64_t nsel = orc_set_bits(bits, n, idx);
+  free(bits);
+  tmap_t m = {(tgrp_t*)calloc(64, sizeof(tgrp_t)), 64, 0};
+  int32_t sgn[4] = {0, 0, 0, 0};
+  for (int k = 0; k < nkeys; ++k) {
+    const int32_t dt = q->cols[key_cols[k]].dtype;
+    sgn[k] = dt == 0 || dt == 2 || dt == 4 || dt == 6;
+  }
+  for (uint64_t i = 0; i < nsel; ++i) {
+    const uint64_t rel = idx[i], row = q->row_begin + rel;
+    uint64_t key[4] = {0, 0, 0, 0};
+    uint32_t nulls = 0;
+    for (int k = 0; k < nkeys; ++k) {
+      const uint64_t* nb = key_nulls ? key_nulls[k] : NULL;
+      if (nb && ((nb[rel >> 6] >> (rel & 63)) & 1)) { nulls |= 1u << k; continue; }
+      const orc_column* c = &q->cols[key_cols[k]];
+      key[k] = (uint64_t)get_native(c->data, c->dtype, row);
+    }
+    tgrp_t* g = tmap_get(&m, key, nkeys, nulls);
+    g->count++;
+    for (int a = 0; a < q->naggs; ++a) {
+      const orc_agg* A = &q->aggs[a];
+      const uint64_t v = (uint64_t)agg_row_value(q, A, row);
+      if (A->kind >= 2 && A->kind <= 5) {
+        const int sg = A->kind == 2 || A->kind == 3, is_min = A->kind == 2 || A->kind == 4;
+        int better;
+        if (g->count == 1) better = 1;
+        else if (sg) better = is_min ? (int64_t)v < (int64_t)g->best[a]
+                                     : (int64_t)v > (int64_t)g->best[a];
+        else better = is_min ? v < g->best[a] : v > g->best[a];
+        if (better) g->best[a] = v;
+      } else {
+        g->sum[a] += (i128)(int64_t)v;
+      }
+    }
+  }
+  free(idx);
+  tgrp_t* list = (tgrp_t*)malloc((m.n ? m.n : 1) * sizeof(tgrp_t));
+  uint64_t G = 0;
+  for (uint64_t i = 0; i < m.cap; ++i)
+    if (m.s[i].used) list[G++] = m.s[i];
+  free(m.s);
+  g_sort_nk = nkeys;
+  g_sort_signed = sgn;
+  qsort(list, G, sizeof(tgrp_t), cmp_tuple);
+  if (G <= capacity) {
+    for (uint64_t g = 0; g < G; ++g) {
+      for (int k = 0; k < nkeys; ++k) out_keys[g * nkeys + k] = list[g].key[k];
+      if (out_nulls) out_nulls[g] = list[g].nulls;
+      out_count[g] = list[g].count;
+      for (int a = 0; a < q->naggs; ++a) {
+        const orc_agg* A = &q->aggs[a];
+        uint64_t lo, hi;
+        if (A->kind >= 2 && A->kind <= 5) {
+          lo = list[g].best[a];
+          hi = (A->kind == 2 || A->kind == 3) && (int64_t)lo < 0 ? ~0ull : 0ull;
+        } else {
+          lo = (uint64_t)(u128)list[g].sum[a];
+          hi = (uint64_t)((u128)list[g].sum[a] >> 64);
+        }
+        out_sums[(g * q->naggs + a) * 2] = lo;
+        out_sums[(g * q->naggs + a) * 2 + 1] = hi;
+      }
+    }
+  }
+  free(list);
+  return (int64_t)G;
 }
 
 /* ---- JoinHashTable (exec/join_hash_table.rs) ------------------------------------ */

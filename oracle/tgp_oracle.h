@@ -51,6 +51,11 @@ void orc_synth_uniform(void* out, int32_t dtype, uint64_t seed, uint64_t column_
 void orc_synth_cdf(void* out, int32_t dtype, uint64_t seed, uint64_t column_id, const double* cdf,
                    uint64_t n_cdf, const int64_t* table, uint64_t row_begin, uint64_t row_end);
 
+/* 512 when the AVX-512 filter path runs on this host, else 0; orc_set_simd(0)
+ * forces the scalar path (tests compare both), orc_set_simd(512) restores it. */
+int32_t orc_simd_level(void);
+void orc_set_simd(int32_t level);
+
 /* bitmap.rs */
 void orc_widen_cells(const void* col, int32_t dtype, uint64_t n, uint64_t* cells);
 void orc_filter_cells(const uint64_t* cells, uint64_t n, int32_t op, int32_t is_signed, int64_t lit,
@@ -76,6 +81,17 @@ int32_t orc_query_filter(const orc_query* q, uint8_t* bits /* (n+7)/8 bytes */);
  * ngroups = prod(card).  nthreads <= 0 => all cores. */
 int32_t orc_group_agg(const orc_query* q, int32_t mode, int32_t nthreads, uint64_t* out_count,
                       uint64_t* out_sums, double* out_f64, uint64_t* out_oor);
+
+/* GROUP BY the key tuple (accum_grouped.rs:198-335, dispatch.rs:1377-1385) with a
+ * tuple-keyed map instead of the combined hash; NULL parts (key_nulls[k]: bitmap
+ * words, bit i <-> row row_begin + i, or NULL) are values of their own.  Returns
+ * the group count G; when G <= capacity writes the groups in ascending tuple order
+ * (signed keys as int64, others as u64, NULL last): out_keys[g*nkeys+k] (u64 cells,
+ * 0 for NULL), out_nulls[g] (bit k = key k NULL), out_count[g], out_sums as
+ * orc_group_agg mode 0. */
+int64_t orc_group_by_keys(const orc_query* q, int32_t nkeys, const int32_t* key_cols,
+                          const uint64_t* const* key_nulls, uint64_t capacity, uint64_t* out_keys,
+                          uint32_t* out_nulls, uint64_t* out_count, uint64_t* out_sums);
 
 /* join_hash_table.rs + join.rs + accumulator grouping.
  * group_by: 0 build payload, 1 probe column, 2 none.

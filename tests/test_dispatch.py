@@ -40,3 +40,27 @@ def test_having_is_not_dispatched():
     q = SelectQuery([Column("dept"), Aggregate("count", "*")], group_by=["dept"],
                     having=(Aggregate("count", "*"), ">", 2))
     assert execute_dispatched(q, None) is None
+
+
+def test_unsupported_details_return_none():
+    """Shapes the device path does not take fall back (None), as dispatch.rs:304-333
+    does, instead of raising: a non-grouped SELECT column, COUNT(DISTINCT) per
+    group, five GROUP BY columns, a LIKE filter.  These are decided before any
+    device work, so a table of column descriptors without a device suffices."""
+    from turbogp_amd import _capi as A
+    from turbogp_amd.dispatch import Table, execute_dispatched, execute_grouped
+
+    class Col:
+        def __init__(self, dtype):
+            self.n, self.dtype = 8, dtype
+
+    t = Table(None, {c: Col(A.I64) for c in "abcdef"})
+    cnt = Aggregate("count", "*")
+    for q in (SelectQuery([Column("a"), Column("b"), cnt], group_by=["a"]),
+              SelectQuery([Column("a"), Aggregate("count_distinct", "b")], group_by=["a"]),
+              SelectQuery([Column("a"), cnt], group_by=list("abcde")),
+              SelectQuery([Column("a"), cnt], group_by=["a", "a"]),
+              SelectQuery([Aggregate("sum", "a")], where=[("b", "LIKE", 3)])):
+        assert execute_dispatched(q, t) is None, q
+    assert execute_grouped(SelectQuery([Column("a"), Column("b"), cnt], group_by=["a"]),
+                           t) is None

@@ -243,3 +243,51 @@ def test_having_order_limit_vs_numpy(make):
     # ascending sums, ties in key order (the groups come in key order, the sort is stable)
     want = sorted((s, k) for k, s in sums.items() if s < -1000)[:10]
     assert list(zip(got, keys)) == want
+
+
+def test_group_by_several_nullable_and_sparse_keys(ctx):
+    """GROUP BY two columns, a nullable column and a sparse int64 column (the hash
+    table path) against Python dicts over the same rows; NULL keys form their own
+    group and carry a null_mask; groups come out in key order, NULL last."""
+    from turbogp_amd import DeviceBitmap
+    rng = np.random.default_rng(21)
+    n = 200_000
+    host = {"fa": rng.integers(0, 3, n).astype(np.uint8),
+            "fb": rng.integers(0, 2, n).astype(np.uint8),
+            "sid": rng.integers(-(2 ** 62), 2 ** 62, 3000, dtype=np.int64)[
+                rng.integers(0, 3000, n)],
+            "v": rng.integers(-(10 ** 12), 10 ** 12, n).astype(np.int64)}
+    null_sid = rng.random(n) < 0.1
+    null_v = rng.random(n) < 0.05
+    dev = {k: DeviceColumn.from_numpy(ctx, v) for k, v in host.items()}
+    nulls = {"sid": DeviceBitmap.from_null_flags(ctx, null_sid),
+             "v": DeviceBitmap.from_null_flags(ctx, null_v)}
+    try:
+        t = Table(ctx, dev, nulls)
+        sel = host["v"] > -(5 * 10 ** 11)
+
+        def ref(keys):
+            out = {}
+            for i in np.flatnonzero(sel & ~null_v):
+                k = tuple(None if (c == "sid" and null_sid[i]) else int(host[c][i]) for c in keys)
+                c, s = out.get(k, (0, 0))
+                out[k] = (c + 1, s + int(host["v"][i]))
+            return dict(sorted(out.items(), key=lambda kv: [(1, 0) if x is None else (0, x)
+                                                             for x in kv[0]]))
+
+        for keys in (["fa", "fb"], ["sid"], ["fb", "sid"]):
+            q = SelectQuery([Column(k) for k in keys] +
+                            [Aggregate("count", "*"), Aggregate("sum", "v")],
+                            where=[("v", ">", -(5 * 10 ** 11))], group_by=keys)
+            r = execute_dispatched(q, t)
+            want = ref(keys)
+            got_keys = list(zip(*[[None if (c.null_mask and c.null_mask[i]) else
+                                   (v - 2 ** 64 if v >= 2 ** 63 else v)
+                                   for i, v in enumerate(c.values)]
+                                  for c in r.columns[:len(keys)]]))
+            assert got_keys == list(want), keys
+            assert r.columns[len(keys)].values == [c for c, _ in want.values()]
+            assert r.columns[len(keys) + 1].values == [s % 2 ** 64 for _, s in want.values()]
+    finally:
+        for c in list(dev.values()) + list(nulls.values()):
+            c.free()

@@ -337,3 +337,56 @@ def test_multi_key_table_rejects_single_key_calls(ctx):
         t.free()
         a.free()
         b2.free()
+
+
+@pytest.mark.parametrize("ngroups", [25, 3000, 9000])
+def test_probe_agg_duplicate_heavy_build_vs_oracle(ctx, ngroups):
+    """A build side with many rows per key gives probe rows many matches each, so the
+    additions per accumulator word are not bounded by the probe rows: such tables take
+    the carry-exact int128 form, in LDS (25 groups) and in the global table (3000 and
+    9000 payload groups), and equal the oracle."""
+    rng = np.random.default_rng(ngroups)
+    nb, npb = 40_000, 200_000
+    bk = rng.integers(0, 300, nb).astype(np.int64)       # ~133 build rows per key
+    bp = rng.integers(0, ngroups, nb).astype(np.int32)
+    pk = bk[rng.integers(0, nb, npb)]
+    pk[::7] += 1000                                       # misses
+    pv = rng.integers(-2**62, 2**62, npb).astype(np.int64)
+    t, keep = build(ctx, bk, bp)
+    cols = [dev(ctx, pk), dev(ctx, pv)]
+    try:
+        assert t.info()[2] == 1  # has duplicates
+        got = t.probe_agg(cols[0], cols[1], A.JOIN_GROUP_BUILD_PAYLOAD, ngroups, 0)
+    finally:
+        t.free()
+        for c in keep + cols:
+            c.free()
+    cnt, sums, oor = B.join_agg(bk, bp, pk, pv, None, A.JOIN_GROUP_BUILD_PAYLOAD, 0, ngroups)
+    assert got.n_out_of_range == oor == 0
+    assert np.array_equal(got.count, cnt)
+    assert [s[0] for s in got.sums()] == sums
+
+
+def test_probe_agg_past_2_32_additions_per_word(ctx):
+    """One payload group gets more than 2^32 additions of 2^32 - 1 (4352 build rows of
+    one key x 2^20 probe rows): a split low-word sum would wrap past 2^64; the exact
+    form gives the closed form count x value.  3000 other payload groups keep the
+    table out of LDS."""
+    dup, npb = 4352, 1 << 20
+    bk = np.concatenate([np.full(dup, 77, np.int64), np.arange(1000, 4000, dtype=np.int64)])
+    bp = np.concatenate([np.zeros(dup, np.int32), np.arange(1, 3001, dtype=np.int32)])
+    pk = np.full(npb, 77, np.int64)
+    v = (1 << 32) - 1
+    pv = np.full(npb, v, np.int64)
+    t, keep = build(ctx, bk, bp)
+    cols = [dev(ctx, pk), dev(ctx, pv)]
+    try:
+        got = t.probe_agg(cols[0], cols[1], A.JOIN_GROUP_BUILD_PAYLOAD, 3001, 0)
+    finally:
+        t.free()
+        for c in keep + cols:
+            c.free()
+    m = dup * npb
+    assert m > 2 ** 32
+    assert int(got.count[0]) == m and int(got.count[1:].sum()) == 0
+    assert got.sums()[0][0] == m * v

@@ -310,3 +310,27 @@ def test_generator_closed_form():
     key = 12345 ^ (3 << 40)
     assert [int(x) for x in out] == [-7 + ((mix(key ^ i) * 1008) >> 64) for i in range(10, 20)]
     assert B.lib().orc_mix64(0) == 0 and B.lib().orc_mix64(1) == mix(1)
+
+
+def test_avx512_filter_equals_scalar():
+    """The AVX-512 4x8 filter path (bitmap.rs:642-694) gives the scalar path's
+    bitmap for every op x signedness, at lengths that end in each tail form."""
+    lib = B.lib()
+    if lib.orc_simd_level() < 512:
+        pytest.skip("host without AVX-512F/BW: only the scalar path runs")
+    rng = np.random.default_rng(5)
+    edge = [0, 1, -1, 2 ** 63 - 1, -(2 ** 63), 12345]
+    try:
+        for n in (0, 1, 7, 8, 31, 32, 33, 100, 1031):
+            cells = rng.choice(np.array(edge + list(rng.integers(-5, 5, 10)), np.int64), n)
+            cells = cells.view(np.uint64)
+            for op in range(6):
+                for sg in (0, 1):
+                    for lit in edge:
+                        lib.orc_set_simd(512)
+                        a = B.filter_cells(cells, op, sg, lit)
+                        lib.orc_set_simd(0)
+                        b = B.filter_cells(cells, op, sg, lit)
+                        assert np.array_equal(a, b), (n, op, sg, lit)
+    finally:
+        lib.orc_set_simd(512)

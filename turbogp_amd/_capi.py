@@ -34,6 +34,7 @@ AGG_MINMAX = {"min": AGG_MIN, "max": AGG_MAX, "min_u64": AGG_MIN_U64, "max_u64":
 JOIN_GROUP_BUILD_PAYLOAD, JOIN_GROUP_PROBE_COL, JOIN_GROUP_NONE = 0, 1, 2
 TUNE_WG_PER_CU, TUNE_GRID_MAX, TUNE_SPECIALIZE, TUNE_SPEC_KERNEL, TUNE_CHUNK = 0, 1, 2, 3, 4
 TUNE_SLICES_PER_WAVE, TUNE_JOIN_ATOMICS, TUNE_JOIN_PACK_BITS, TUNE_DISTINCT_SHRINK = 5, 6, 7, 8
+TUNE_GROUP_TAG_BITS = 9
 JOIN_INNER, JOIN_LEFT = 0, 1
 JOIN_NO_MATCH = 0xFFFFFFFF
 MAX_JOIN_KEYS = 4
@@ -71,6 +72,19 @@ class AggQuery(C.Structure):
                 ("naggs", C.c_int32),
                 ("preds", Pred * MAX_PREDS), ("parts", GroupPart * MAX_GROUP_PARTS),
                 ("aggs", Agg * MAX_AGGS), ("d_sel_bitmap", C.c_void_p)]
+
+
+MAX_GROUP_KEYS = 4
+
+
+class HashGroupQuery(C.Structure):
+    _fields_ = [("agg", AggQuery), ("nkeys", C.c_int32), ("key_col", C.c_int32 * MAX_GROUP_KEYS),
+                ("_pad", C.c_int32 * 3), ("d_key_nulls", C.c_void_p * MAX_GROUP_KEYS)]
+
+
+class HashGroupResult(C.Structure):
+    _fields_ = [("d_keys", C.c_void_p), ("d_key_nulls", C.c_void_p), ("d_count", C.c_void_p),
+                ("d_sums", C.c_void_p), ("capacity", C.c_uint64)]
 
 
 class GroupResult(C.Structure):
@@ -190,6 +204,8 @@ SIGNATURES = {
     "tgp_group_agg": (I32_, [P, C.POINTER(AggQuery), C.POINTER(GroupResult), P]),
     "tgp_group_agg_ngroups": (I32_, [C.POINTER(AggQuery), C.POINTER(U64_)]),
     "tgp_group_agg_explain": (I32_, [C.POINTER(AggQuery), C.POINTER(I32_), C.POINTER(I32_)]),
+    "tgp_hash_group_agg_sync": (I32_, [P, C.POINTER(HashGroupQuery), C.POINTER(HashGroupResult),
+                                       C.POINTER(U64_), P]),
     "tgp_synth_uniform": (I32_, [P, C.POINTER(Column), U64_, U64_, I64_, I64_, U64_, U64_, P]),
     "tgp_synth_lookup_cdf": (I32_, [P, C.POINTER(Column), U64_, U64_, P, U64_, P, U64_, U64_,
                                     P]),

@@ -358,7 +358,18 @@ __device__ __forceinline__ void acc_add_lds(const LdsTab& t, uint64_t ng, uint64
   if (h) atomicAdd(t.at(ng + 2 * g + 1), (unsigned long long)h);
 }
 
-template <bool USE_LDS>
+// Exact for any number of additions: the (lo, hi) words hold an int128 and the
+// carry out of the low word is taken from the value the atomic returns.  A table
+// with duplicate build keys gives a probe row several matches, so the number of
+// additions per word is not bounded by the probe rows; these tables use this form.
+__device__ __forceinline__ void i128_atomic_add(unsigned long long* lo, unsigned long long* hi,
+                                                int64_t v) {
+  const unsigned long long old = atomicAdd(lo, (unsigned long long)v);
+  const int64_t h = (v >> 63) + (int64_t)((old + (unsigned long long)v) < old);
+  if (h) atomicAdd(hi, (unsigned long long)h);
+}
+
+template <bool USE_LDS, bool EXACT>
 __global__ void __launch_bounds__(256) k_probe_agg(JoinTableDev T, ProbeAggDev Q,
                                                    uint32_t rshift, uint32_t shared_table) {
   extern __shared__ __align__(16) unsigned long long ltab[];
@@ -374,8 +385,19 @@ __global__ void __launch_bounds__(256) k_probe_agg(JoinTableDev T, ProbeAggDev Q
     __syncthreads();
   }
   auto add = [&](uint64_t g, int64_t v) {
-    if constexpr (USE_LDS) acc_add_lds(lt, Q.ngroups, g, v);
-    else acc_add(cnt, sum, g, v);
+    if constexpr (EXACT) {
+      if constexpr (USE_LDS) {
+        atomicAdd(lt.at(g), 1ull);
+        i128_atomic_add(lt.at(Q.ngroups + 2 * g), lt.at(Q.ngroups + 2 * g + 1), v);
+      } else {
+        atomicAdd(&cnt[g], 1ull);
+        i128_atomic_add(&sum[2 * g], &sum[2 * g + 1], v);
+      }
+    } else if constexpr (USE_LDS) {
+      acc_add_lds(lt, Q.ngroups, g, v);
+    } else {
+      acc_add(cnt, sum, g, v);
+    }
   };
   uint32_t oor = 0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -430,7 +452,23 @@ __global__ void __launch_bounds__(256) k_probe_agg(JoinTableDev T, ProbeAggDev Q
       atomicAdd(&Q.out_count[gi], c);
       uint64_t lo;
       int64_t hi;
-      split_to_i128(total(Q.ngroups + 2 * gi), (int64_t)total(Q.ngroups + 2 * gi + 1), &lo, &hi);
+      if constexpr (EXACT) {
+        // int128 copies: add the low words with their carries, then the high words
+        lo = 0;
+        hi = 0;
+        for (uint32_t wv = 0; wv < nwt; ++wv)
+          for (uint32_t c2 = 0; c2 < (1u << rshift); ++c2) {
+            const uint64_t base = wv * region + c2;
+            const uint64_t l = ltab[base + ((Q.ngroups + 2 * gi) << rshift)];
+            const uint64_t h = ltab[base + ((Q.ngroups + 2 * gi + 1) << rshift)];
+            const uint64_t nl = lo + l;
+            hi += (int64_t)h + (int64_t)(nl < lo);
+            lo = nl;
+          }
+      } else {
+        split_to_i128(total(Q.ngroups + 2 * gi), (int64_t)total(Q.ngroups + 2 * gi + 1), &lo,
+                      &hi);
+      }
       if (lo || hi) {
         const unsigned long long old = atomicAdd(&Q.out_sums[2 * gi], (unsigned long long)lo);
         const int64_t h = hi + (int64_t)((old + lo) < old);
@@ -1325,9 +1363,17 @@ int32_t tgp_hash_probe_agg(tgp_ctx* ctx, const tgp_join_agg_query* q, const tgp_
     if ((tab << r) * 4 <= kLdsMax) rshift = r;
   if (rshift < 0 && tab <= kLdsMax) { rshift = 0; shared_table = 1; }
   const uint64_t nparts = (q->ngroups + (1u << kPartShift) - 1) >> kPartShift;
+  // duplicate build keys: a probe row adds once per match, so the split lo32/hi32
+  // words (exact below 2^32 additions each) give way to carry-exact int128 words
+  const bool exact = q->table->has_dup != 0;
   if (rshift >= 0) {
     const uint64_t lds = (tab << rshift) * (shared_table ? 1 : 4);
-    k_probe_agg<true><<<grid, 256, lds, s>>>(q->table->dev(), Q, (uint32_t)rshift, shared_table);
+    if (exact)
+      k_probe_agg<true, true><<<grid, 256, lds, s>>>(q->table->dev(), Q, (uint32_t)rshift,
+                                                     shared_table);
+    else
+      k_probe_agg<true, false><<<grid, 256, lds, s>>>(q->table->dev(), Q, (uint32_t)rshift,
+                                                      shared_table);
   } else if (q->group_by == TGP_JOIN_GROUP_PROBE_COL && nparts < kMaxParts &&
              ctx->tune_join_partition.load() == 0) {
     TGP_REQUIRE(n < (1ull << 32),
@@ -1362,9 +1408,13 @@ int32_t tgp_hash_probe_agg(tgp_ctx* ctx, const tgp_join_agg_query* q, const tgp_
   } else {
     TGP_REQUIRE(q->row_end - q->row_begin < (1ull << 32),
                 "tgp_hash_probe_agg: a global-table call takes fewer than 2^32 probe rows");
-    k_probe_agg<false><<<grid, 256, 0, s>>>(q->table->dev(), Q, 0, 0);
-    TGP_HIP(hipGetLastError());
-    k_split_finalize<<<grid_for(q->ngroups), 256, 0, s>>>(Q.out_sums, q->ngroups);
+    if (exact) {
+      k_probe_agg<false, true><<<grid, 256, 0, s>>>(q->table->dev(), Q, 0, 0);
+    } else {
+      k_probe_agg<false, false><<<grid, 256, 0, s>>>(q->table->dev(), Q, 0, 0);
+      TGP_HIP(hipGetLastError());
+      k_split_finalize<<<grid_for(q->ngroups), 256, 0, s>>>(Q.out_sums, q->ngroups);
+    }
   }
   TGP_HIP(hipGetLastError());
   return TGP_OK;

@@ -359,6 +359,45 @@ class Context:
             res.free()
         return out
 
+    def hash_group_agg(self, q, key_cols, key_nulls=None, capacity=None, stream=None):
+        """GROUP BY the key tuple of columns q.cols[key_cols] (any values; NULL keys
+        from the optional per-key NULL bitmaps) through tgp_hash_group_agg_sync.
+        Returns HashGroups (groups in no particular order).  `capacity` is the first
+        guess of the group count; a larger table is taken as the call reports."""
+        n = q.row_end - q.row_begin
+        nk = len(key_cols)
+        hq = A.HashGroupQuery()
+        hq.agg = q
+        hq.nkeys = nk
+        for k, c in enumerate(key_cols):
+            hq.key_col[k] = c
+            nb = (key_nulls or [None] * nk)[k]
+            hq.d_key_nulls[k] = nb.buf.ptr if nb is not None else None
+        cap = max(1, min(n, 1 << 16) if capacity is None else capacity)
+        while True:
+            keys = DeviceBuffer(self, max(8, cap * nk * 8))
+            nulls = DeviceBuffer(self, max(8, cap * 4))
+            res = GroupAggBuffers(self, cap, q.naggs)
+            try:
+                out = A.HashGroupResult(keys.ptr, nulls.ptr, res.count.ptr, res.sums.ptr, cap)
+                ng = C.c_uint64()
+                rc = A.lib().tgp_hash_group_agg_sync(self.handle, C.byref(hq), C.byref(out),
+                                                     C.byref(ng), stream)
+                if rc == A.ERR_CAPACITY and ng.value > cap:
+                    cap = max(int(ng.value), min(max(n, 1), 4 * cap))
+                    continue
+                A.check(rc, "tgp_hash_group_agg_sync")
+                g = int(ng.value)
+                kv = keys.download(np.uint64, g * nk).reshape(g, nk)
+                nf = nulls.download(np.uint32, g)
+                cnt = res.count.download(np.uint64, g)
+                sums = res.sums.download(np.uint64, g * q.naggs * 2).reshape(g, q.naggs, 2)
+                return HashGroups(kv, nf, GroupAggResult(cnt, sums, 0))
+            finally:
+                keys.free()
+                nulls.free()
+                res.free()
+
     def group_agg_async(self, q, res, stream=None):
         """Queues tgp_group_agg into preallocated GroupAggBuffers (no sync, no copy)."""
         A.check(A.lib().tgp_group_agg(self.handle, C.byref(q), C.byref(res.desc), stream),
@@ -480,6 +519,43 @@ class GroupAggResult:
         """Present groups as (gid, count, [sums]) sorted by gid (compare after sorting)."""
         s = self.sums()
         return [(g, int(self.count[g]), s[g]) for g in range(len(self.count)) if self.count[g]]
+
+
+class HashGroups:
+    """Groups of tgp_hash_group_agg_sync: keys[g] (u64 cells, one per key column),
+    null_flags[g] (bit k = key k is NULL) and the aggregates as a GroupAggResult whose
+    group g is the g-th key tuple."""
+
+    def __init__(self, keys, null_flags, result):
+        self.keys = keys
+        self.null_flags = null_flags
+        self.result = result
+
+    def __len__(self):
+        return len(self.null_flags)
+
+    def key_tuples(self, signed=None):
+        """Key tuples as Python ints (None for NULL parts); signed[k] reads key k's
+        cell as int64."""
+        nk = self.keys.shape[1] if self.keys.ndim == 2 else 0
+        signed = signed or [True] * nk
+        out = []
+        for g in range(len(self)):
+            t = []
+            for k in range(nk):
+                if (int(self.null_flags[g]) >> k) & 1:
+                    t.append(None)
+                    continue
+                v = int(self.keys[g, k])
+                t.append(v - (1 << 64) if signed[k] and v >= (1 << 63) else v)
+            out.append(tuple(t))
+        return out
+
+    def as_dict(self, signed=None):
+        """{key tuple: (count, [exact sums])}."""
+        s = self.result.sums()
+        return {t: (int(self.result.count[g]), s[g])
+                for g, t in enumerate(self.key_tuples(signed))}
 
 
 _MINMAX = ("min", "max", "min_u64", "max_u64")

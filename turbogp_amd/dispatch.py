@@ -39,6 +39,15 @@ from .device import DeviceBitmap, DeviceBuffer, HashTable, make_agg_query, numer
 
 OPS = {"=": A.EQ, "!=": A.NE, "<>": A.NE, "<": A.LT, "<=": A.LE, ">": A.GT, ">=": A.GE}
 INT8_OID, NUMERIC_OID = 20, 1700
+# a single GROUP BY key whose value range is at most this wide groups by dense ids
+# (tgp_group_agg); wider or sparse keys, several keys and nullable keys go through
+# the device hash table (tgp_hash_group_agg_sync)
+DENSE_KEY_RANGE = 1 << 24
+
+
+class Unsupported(Exception):
+    """A query shape the device dispatcher does not run: execute_dispatched returns
+    None for it, as dispatch.rs:304-333 hands such queries back to the interpreter."""
 
 
 class QueryShape(Enum):
@@ -206,19 +215,27 @@ def classify_query(q: SelectQuery) -> QueryShape:
 
 
 def execute_dispatched(q: SelectQuery, t: Table) -> Optional[QueryResult]:
-    """dispatch.rs:304-333: None for shapes the dispatcher does not run, and for any
-    query with HAVING (dispatch.rs:329-331 sends those to the interpreter)."""
+    """dispatch.rs:304-333: None for shapes the dispatcher does not run, for any
+    query with HAVING (dispatch.rs:329-331 sends those to the interpreter), and for
+    a recognised shape whose details the device path does not take (Unsupported:
+    e.g. COUNT(DISTINCT) per group, a SELECT column that is neither grouped nor
+    aggregated, more than four GROUP BY columns).  Malformed input (an unknown
+    column, a bad operator) still raises."""
     shape = classify_query(q)
     if shape == QueryShape.Complex or q.having is not None:
         return None
-    return _Executor(q, t).run(shape)
+    try:
+        return _Executor(q, t).run(shape)
+    except Unsupported:
+        return None
 
 
-def execute_grouped(q: SelectQuery, t: Table) -> QueryResult:
+def execute_grouped(q: SelectQuery, t: Table) -> Optional[QueryResult]:
     """The interpreter's accumulator path for a GROUP BY of one column
     (accum_grouped.rs:55-409 try_accumulator_grouped): the groups and their exact
     aggregates on the device, then HAVING <aggregate> <op> <literal> (HavingLit,
-    :148-160, :339-349), then ORDER BY and LIMIT, in that order."""
+    :148-160, :339-349), then ORDER BY and LIMIT, in that order.  None when the
+    device path does not take the query (as try_accumulator_grouped returns None)."""
     if not q.group_by:
         raise ValueError("execute_grouped needs a GROUP BY")
     hidden = False
@@ -238,6 +255,8 @@ def execute_grouped(q: SelectQuery, t: Table) -> QueryResult:
     try:
         bm, _ = ex.mask()
         r = ex.group_by(bm)
+    except Unsupported:
+        return None
     finally:
         ex.free()
     if hidden:
@@ -356,6 +375,13 @@ def _exact_column(name, vals):
     return ResultColumn(name, [0] * len(vals), [str(v) for v in vals], NUMERIC_OID)
 
 
+def _avg_text(count, total):
+    """AVG as NUMERIC text of an exact sum over a count (tgp_int128_avg_to_numeric)."""
+    from .device import GroupAggResult
+    w = np.array([[[total % 2**64, (total >> 64) % 2**64]]], np.uint64)
+    return GroupAggResult(np.array([count], np.uint64), w, 0).avg_numeric(0)[0]
+
+
 def _arg_columns(arg):
     if isinstance(arg, str):
         return [] if arg == "*" else [arg]
@@ -378,8 +404,8 @@ class _Executor:
             return None, self.t.row_count
         signed = (A.I8, A.I16, A.I32, A.I64)
         for _, op, _ in self.q.where:
-            if op not in OPS:
-                raise ValueError(f"unsupported comparison {op!r}")
+            if op not in OPS:  # e.g. LIKE (try_string_like_filter): not a device filter
+                raise Unsupported(f"comparison {op!r}")
         lits = [w for w in self.q.where if not isinstance(w[2], Column)]
         pairs = [w for w in self.q.where if isinstance(w[2], Column)]
         names = sorted({c for c, _, _ in lits})
@@ -418,6 +444,11 @@ class _Executor:
 
     # ---- aggregates without GROUP BY -------------------------------------------------
     def agg_query(self, items, parts=(), bitmap=None, names_extra=()):
+        return self.ctx.group_agg(self.build_query(items, parts, bitmap, names_extra))
+
+    def build_query(self, items, parts=(), bitmap=None, names_extra=()):
+        """AggQuery of the aggregates `items` over all rows; the columns named in
+        names_extra take the first column slots, in that order."""
         names = []
 
         def slot(c):
@@ -438,11 +469,12 @@ class _Executor:
             elif func in ("MIN", "MAX"):
                 aggs.append((func.lower() + "_u64", slot(arg)))
             else:
-                raise ValueError(func)
+                raise Unsupported(f"aggregate {func}")
         parts = [(slot(c), div, base, card) for c, div, base, card in parts]
+        if len(names) > A.MAX_COLS or len(aggs) > A.MAX_AGGS:
+            raise Unsupported(f"{len(names)} columns / {len(aggs)} aggregates in one pass")
         cols = [self.t.col(c) for c in names] if names else [next(iter(self.t.columns.values()))]
-        return self.ctx.group_agg(make_agg_query(cols, (), parts, aggs, 0, self.t.row_count,
-                                                 bitmap))
+        return make_agg_query(cols, (), parts, aggs, 0, self.t.row_count, bitmap)
 
     def run(self, shape):
         q, t = self.q, self.t
@@ -486,28 +518,50 @@ class _Executor:
             return self.group_by(bm)
         return self.select(shape, bm)
 
-    # ---- GROUP BY one column: dense key range, exact aggregates -------------------
+    # ---- GROUP BY: dense ids for one narrow key, else the device hash table ------------
     def group_by(self, bm):
         q, t = self.q, self.t
-        if len(q.group_by) != 1:
-            raise ValueError("GROUP BY of one column only")
-        key = q.group_by[0]
-        if key in t.nulls:
-            raise ValueError(f"GROUP BY the nullable column {key!r} is not dispatched")
-        lo, hi = t.col(key).minmax()
-        card = hi - lo + 1 if t.row_count else 1
-        if card > (1 << 24):
-            raise ValueError(f"GROUP BY {key}: key range {card} too wide for a dense table")
-        parts = [(key, 1, lo, card)]
-        base = self.agg_query([], parts=parts, bitmap=bm, names_extra=(key,))
-        gids = [g for g in range(len(base.count)) if base.count[g]]
+        keys = [t.resolve(k) for k in q.group_by]
+        if not 1 <= len(keys) <= A.MAX_GROUP_KEYS or len(set(keys)) != len(keys):
+            raise Unsupported(f"GROUP BY of {len(keys)} columns")
+        for s in q.select:
+            if isinstance(s, Column) and t.resolve(s.name) not in keys:
+                # the reference emits the first row's cell of such a column
+                raise Unsupported(f"column {s.name!r} is neither grouped nor aggregated")
+            if isinstance(s, Aggregate):
+                if s.func.upper() == "COUNT_DISTINCT":
+                    raise Unsupported("COUNT(DISTINCT) per group")
+                for c in _arg_columns(s.arg):
+                    t.col(c)
+        signed = [t.col(k).dtype in (A.I8, A.I16, A.I32, A.I64) for k in keys]
+        dense = None
+        if len(keys) == 1 and keys[0] not in t.nulls and t.row_count:
+            lo, hi = t.col(keys[0]).minmax()  # int64 readings of the cells
+            narrow = (hi - lo + 1 <= DENSE_KEY_RANGE) if signed[0] else \
+                (0 <= lo <= hi < DENSE_KEY_RANGE)
+            if narrow:
+                dense = (lo, hi - lo + 1)
+
+        def groups(items, bitmap):
+            """{key tuple: (count, [exact values per item])} of the rows in `bitmap`."""
+            if dense is not None:
+                lo, card = dense
+                r = self.agg_query(items, parts=[(keys[0], 1, lo, card)], bitmap=bitmap,
+                                   names_extra=keys)
+                sums = r.sums()
+                return {(lo + g,): (int(r.count[g]), sums[g]) for g in range(card) if r.count[g]}
+            qq = self.build_query(items, bitmap=bitmap, names_extra=keys)
+            hg = self.ctx.hash_group_agg(qq, list(range(len(keys))),
+                                         [t.nulls.get(k) for k in keys])
+            return hg.as_dict(signed)
+
+        base = groups([], bm)
+        tuples = sorted(base, key=lambda tp: [(1, 0) if v is None else (0, v) for v in tp])
         # one pass per set of nullable columns the aggregates read: each aggregate
         # sees the selected rows where its own columns are not NULL
         passes = {}
         for s in q.select:
             if isinstance(s, Aggregate) and not (s.func.upper() == "COUNT" and s.arg == "*"):
-                for c in _arg_columns(s.arg):
-                    t.col(c)
                 ns = tuple(sorted(c for c in _arg_columns(s.arg) if c in t.nulls))
                 passes.setdefault(ns, []).append(s)
         results = {}
@@ -515,46 +569,40 @@ class _Executor:
             items = [(s.func.upper(), s.arg) for s in aggs
                      if s.func.upper() in ("SUM", "AVG", "MIN", "MAX")]
             pbm, _ = self.without_nulls(bm, 0, list(ns)) if ns else (bm, 0)
-            results[ns] = (self.agg_query(items, parts=parts, bitmap=pbm, names_extra=(key,))
-                           if ns else None, items)
+            results[ns] = (groups(items, pbm), items)
         cols = []
         for s in q.select:
             if isinstance(s, Column):
-                if s.name != key:
-                    raise ValueError(f"column {s.name!r} is neither grouped nor aggregated")
-                cols.append(_exact_column(s.name, [lo + g for g in gids]))
-            elif isinstance(s, Aggregate):
-                f = s.func.upper()
-                name = s.alias or f.lower()
-                if f == "COUNT_DISTINCT":
-                    raise ValueError("COUNT(DISTINCT) per group is not dispatched here")
-                if f == "COUNT" and s.arg == "*":
-                    cols.append(ResultColumn(name, [int(base.count[g]) for g in gids], None,
-                                             INT8_OID))
-                    continue
-                ns = tuple(sorted(c for c in _arg_columns(s.arg) if c in t.nulls))
-                r, items = results[ns]
-                if r is None:  # no nullable column: the base pass has the same rows
-                    r = results[ns] = (self.agg_query(items, parts=parts, bitmap=bm,
-                                                      names_extra=(key,)), items)
-                    r = r[0]
-                if f == "COUNT":
-                    cols.append(ResultColumn(name, [int(r.count[g]) for g in gids], None,
-                                             INT8_OID))
-                    continue
-                k = items.index((f, s.arg))
-                sums = r.sums()
-                if f == "SUM":
-                    cols.append(_exact_column(name, [sums[g][k] for g in gids]))
-                elif f == "AVG":
-                    v = r.avg_numeric(k)
-                    cols.append(ResultColumn(name, [0] * len(gids),
-                                             [v[g] if v[g] is not None else "0" for g in gids],
-                                             NUMERIC_OID))
-                else:
-                    cols.append(ResultColumn(name, [sums[g][k] % 2**64 for g in gids], None,
-                                             INT8_OID))
-        rows = list(range(len(gids)))
+                k = keys.index(t.resolve(s.name))
+                vals = [tp[k] for tp in tuples]
+                col = _exact_column(s.name, [0 if v is None else v for v in vals])
+                if any(v is None for v in vals):
+                    col.null_mask = [v is None for v in vals]
+                cols.append(col)
+                continue
+            f = s.func.upper()
+            name = s.alias or f.lower()
+            if f == "COUNT" and s.arg == "*":
+                cols.append(ResultColumn(name, [base[tp][0] for tp in tuples], None, INT8_OID))
+                continue
+            ns = tuple(sorted(c for c in _arg_columns(s.arg) if c in t.nulls))
+            got, items = results[ns]
+            if f == "COUNT":
+                cols.append(ResultColumn(name, [got.get(tp, (0, []))[0] for tp in tuples], None,
+                                         INT8_OID))
+                continue
+            k = items.index((f, s.arg))
+            if f == "SUM":
+                cols.append(_exact_column(name, [got[tp][1][k] if tp in got else 0
+                                                 for tp in tuples]))
+            elif f == "AVG":
+                cols.append(ResultColumn(name, [0] * len(tuples),
+                                         [_avg_text(*got[tp][:1], got[tp][1][k]) if tp in got
+                                          else "0" for tp in tuples], NUMERIC_OID))
+            else:
+                cols.append(ResultColumn(name, [(got[tp][1][k] if tp in got else 0) % 2**64
+                                                for tp in tuples], None, INT8_OID))
+        rows = list(range(len(tuples)))
         names = [c.name for c in cols]
         if q.having is not None:
             hname, hop, hlit = q.having
@@ -577,6 +625,8 @@ class _Executor:
             c.values = [c.values[i] for i in rows]
             if c.string_values:
                 c.string_values = [c.string_values[i] for i in rows]
+            if c.null_mask:
+                c.null_mask = [c.null_mask[i] for i in rows]
         return QueryResult(cols, len(rows))
 
     # ---- SELECT rows ---------------------------------------------------------------
