@@ -36,6 +36,10 @@
  *   src/exec/bitmap.rs:321-477  filter_{eq..ge}_{u64,i64}-> tgp_pred
  *   query_interpreter/expr.rs:419-1577  build_mask (AND/OR/BETWEEN) -> tgp_scan_filter + tgp_bitmap_combine
  *   query_interpreter/aggregate.rs:507-697, accum_grouped.rs:55-409 -> tgp_group_agg
+ *   accum_grouped.rs:198-335, engine/dispatch.rs:1125-1140, :1377-1385 (GROUP BY any
+ *       key tuple)                                                  -> tgp_hash_group_agg_sync
+ *   kernel/aggregate.rs:19-147 SumF64*, exec/vectorized.rs:128-136, :525-533 -> tgp_sum_f64_sync
+ *   kernel/aggregate.rs:149-185 CountDistinctScalar   -> tgp_kernel_execute (AggregateCountDistinct)
  *   src/exec/join_hash_table.rs:38-241, kernel/hash.rs:84-225      -> tgp_hash_build / tgp_hash_probe_*
  *   query_interpreter/join.rs:544-939 hash_join_with_keys (+ A9 agg)-> tgp_hash_probe_agg
  *   src/exec/join.rs:56-199 hash_join / semi_join / anti_join       -> tgp_hash_probe_pairs / _mark
@@ -416,6 +420,22 @@ int32_t tgp_batch_result_layout(tgp_batch* b, uint32_t i, uint64_t* count_off, u
 /* Copies the first nwords words of the result arena to the host (synchronous). */
 int32_t tgp_batch_results_sync(tgp_batch* b, uint64_t* h_words, uint64_t nwords, void* stream);
 
+/* ---- SUM over f64 cells (FLOAT / DECIMAL columns) ------------------------------ */
+/* SUM of the doubles whose bits are d_cells[i], i < n, selected by the optional
+ * bitmap (bit i <-> cell i; NULL = all), into *h_sum: the exact sum rounded once
+ * to the nearest double (ties to even), whatever the order of the cells.
+ * Replaces the AggregateSumF64 kernels (kernel/aggregate.rs:19-147) and the
+ * dispatch path's masked f64 sums (exec/vectorized.rs:128-136, :525-533;
+ * engine/dispatch.rs:400-404), which add in sequence (or lane by lane) and round
+ * at every step; where their partial sums are exact the two agree.  NaN, +Inf and
+ * -Inf propagate as in IEEE addition.  A zero total is -0.0 only when
+ * neg_zero_start and every selected cell is -0.0 (or none is selected), as
+ * Iterator::sum over f64 starts from -0.0; the dispatch sums start from +0.0
+ * (pass 0).  Synchronous. */
+int32_t tgp_sum_f64_sync(tgp_ctx* ctx, const uint64_t* d_cells, uint64_t n,
+                         const uint64_t* d_sel_bitmap, int32_t neg_zero_start, double* h_sum,
+                         void* stream);
+
 /* ---- COUNT(DISTINCT col) ------------------------------------------------------ */
 /* Distinct u64 cells (values widened as the reference stores them, signed types
  * sign-extended) among rows [row_begin, row_end) of `col` selected by the
@@ -716,8 +736,11 @@ int32_t tgp_kernel_info(const tgp_kernel* k, int32_t* op, int32_t* target, int32
  *   scans:      input = cell_count u64 cells; result.count (+ mask of first 64)
  *   HashBuild:  input = cell_count u64 keys; writes a tgp_hash_table* to output
  *   HashProbe:  input = [tgp_hash_table* (8 B)][cell_count u64 keys]; result.count
- *   AggregateSumF64: input = cell_count f64 bits; result.sum (f64, fixed
- *               pairwise order) and result.count
+ *   AggregateSumF64: input = cell_count f64 bits; result.sum (the exact sum
+ *               rounded once, tgp_sum_f64_sync with the -0.0 start of
+ *               Iterator::sum) and result.count = cell_count (aggregate.rs:19-147)
+ *   AggregateCountDistinct: input = cell_count u64 cells; result.count = the
+ *               number of distinct cells (aggregate.rs:149-185)
  * Host inputs are staged through HBM (PCIe); tier HBM kernels expect device
  * pointers.  Errors return a status; `result` is then zeroed. */
 int32_t tgp_kernel_execute(const tgp_kernel* k, const void* input, void* output,

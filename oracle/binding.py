@@ -364,3 +364,36 @@ def select_rows(row_begin, row_end, sel_words=None, order_col=None, descending=F
         perm = np.argsort(~keys if descending else keys, kind="stable")
         rows = rows[perm]
     return rows if limit is None else rows[:limit]
+
+
+def sum_f64(bits, sel_words=None, neg_zero_start=True):
+    """SUM over f64 cells (their u64 bit patterns) under an optional bitmap, exact
+    and rounded once: the sum of the finite values as a Fraction (each double is a
+    dyadic rational, converted exactly), rounded to the nearest double by Python's
+    int / int division (correctly rounded, ties to even; OverflowError -> +-inf).
+    NaN, or +inf with -inf, give NaN; else an infinity gives itself.  A zero total
+    is -0.0 only when neg_zero_start and every summed cell is -0.0 (or there are
+    none): Iterator::sum over f64 starts from -0.0 (kernel/aggregate.rs:41); the
+    dispatch sums start from +0.0 (exec/vectorized.rs:129, :526)."""
+    import math
+    from fractions import Fraction
+    vals = np.ascontiguousarray(bits, np.uint64).view(np.float64)
+    if sel_words is not None:
+        w = np.ascontiguousarray(sel_words, np.uint64)
+        idx = np.arange(len(vals))
+        keep = (w[idx >> 6] >> (idx & 63).astype(np.uint64)) & np.uint64(1)
+        vals = vals[keep.astype(bool)]
+    nan = bool(np.isnan(vals).any())
+    pinf, ninf = bool((vals == np.inf).any()), bool((vals == -np.inf).any())
+    if nan or (pinf and ninf):
+        return math.nan
+    if pinf or ninf:
+        return math.inf if pinf else -math.inf
+    all_neg_zero = bool(np.all((vals == 0) & np.signbit(vals)))
+    total = sum((Fraction(float(x)) for x in vals if x != 0), Fraction(0))
+    if total == 0:
+        return -0.0 if (neg_zero_start and all_neg_zero) else 0.0
+    try:
+        return total.numerator / total.denominator
+    except OverflowError:
+        return math.inf if total > 0 else -math.inf

@@ -235,6 +235,7 @@ SIGNATURES = {
     "tgp_hash_build": (I32_, [P, C.POINTER(Column), C.POINTER(Column), C.POINTER(P), P]),
     "tgp_hash_table_destroy": (I32_, [P, P]),
     "tgp_hash_table_footprint": (I32_, [P, C.POINTER(U64_)]),
+    "tgp_sum_f64_sync": (I32_, [P, P, U64_, P, I32_, C.POINTER(C.c_double), P]),
     "tgp_count_distinct": (I32_, [P, C.POINTER(Column), U64_, U64_, P, I32_,
                                   C.POINTER(U64_), P]),
     "tgp_hll_sketch": (I32_, [P, C.POINTER(Column), U64_, U64_, P, P, P]),

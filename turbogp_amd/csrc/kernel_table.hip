@@ -57,11 +57,13 @@ struct Staged {
   uint64_t* words = nullptr;  // 2 device words for counts / the first bitmap word
 };
 
+// `slot` 1 for operators whose own calls use the stream's slot-0 scratch (the
+// f64 sum's limbs, the distinct set), so the staged cells are not overwritten.
 static int32_t stage_cells(tgp_ctx* ctx, const void* input, uint64_t n, bool force_copy,
-                           Staged* st) {
+                           Staged* st, int slot = 0) {
   const uint64_t bytes = ((n * 8 + 255) / 256) * 256;
   void* scratch = nullptr;
-  TGP_TRY(tgp::get_scratch(ctx, ctx->stream, bytes + 256, &scratch));
+  TGP_TRY(tgp::get_scratch(ctx, ctx->stream, bytes + 256, &scratch, slot));
   st->words = reinterpret_cast<uint64_t*>(static_cast<uint8_t*>(scratch) + bytes);
   const bool dev = input && is_device_ptr(input);
   if (dev && !force_copy && (reinterpret_cast<uintptr_t>(input) & 15) == 0) {
@@ -233,6 +235,63 @@ class HashProbeGfx950 : public Kernel {
   }
 };
 
+class SumF64Gfx950 : public Kernel {
+ public:
+  explicit SumF64Gfx950(int32_t tier) : tier_(tier) {}
+  Operator op() const override { return Operator::AggregateSumF64; }
+  int32_t tier() const override { return tier_; }
+  const char* name() const override {
+    return tier_ == TGP_TIER_HBM ? "sum_f64_gfx950_hbm" : "sum_f64_gfx950_staged";
+  }
+  // SumF64Scalar / Avx2 / Avx512 (kernel/aggregate.rs:19-147): input = cell_count
+  // f64 bits; result.sum = their sum, result.count = cell_count.  The sum here is
+  // exact and rounded once (see tgp_sum_f64_sync); Iterator::sum starts from -0.0.
+  int32_t execute(tgp_ctx* ctx, const void* in, void*, const tgp_kernel_params& p,
+                  tgp_kernel_result* r) const override {
+    Staged st;
+    TGP_TRY(stage_cells(ctx, in, p.cell_count, false, &st, 1));
+    double sum = 0.0;
+    TGP_TRY(tgp::sum_f64_exact(ctx, st.cells, p.cell_count, nullptr, true, ctx->stream, &sum));
+    r->count = p.cell_count;
+    r->sum = sum;
+    r->mask = 0;
+    return TGP_OK;
+  }
+
+ private:
+  int32_t tier_;
+};
+
+class CountDistinctGfx950 : public Kernel {
+ public:
+  explicit CountDistinctGfx950(int32_t tier) : tier_(tier) {}
+  Operator op() const override { return Operator::AggregateCountDistinct; }
+  int32_t tier() const override { return tier_; }
+  const char* name() const override {
+    return tier_ == TGP_TIER_HBM ? "count_distinct_gfx950_hbm" : "count_distinct_gfx950_staged";
+  }
+  // CountDistinctScalar (kernel/aggregate.rs:149-185): result.count = the number
+  // of distinct u64 cells among the cell_count inputs (an exact device set).
+  int32_t execute(tgp_ctx* ctx, const void* in, void*, const tgp_kernel_params& p,
+                  tgp_kernel_result* r) const override {
+    uint64_t c = 0;
+    if (p.cell_count) {
+      Staged st;
+      TGP_TRY(stage_cells(ctx, in, p.cell_count, false, &st, 1));
+      tgp_column col{st.cells, p.cell_count, TGP_U64, 0};
+      TGP_TRY(tgp_count_distinct(ctx, &col, 0, p.cell_count, nullptr, TGP_DISTINCT_EXACT, &c,
+                                 ctx->stream));
+    }
+    r->count = c;
+    r->sum = 0.0;
+    r->mask = 0;
+    return TGP_OK;
+  }
+
+ private:
+  int32_t tier_;
+};
+
 // (op, target, tier) -> kernel; select() follows KernelTable::select
 // (kernel/mod.rs:209-221) with the device target in the place of the
 // detected CPU and the HBM tier as the fallback tier.
@@ -300,7 +359,8 @@ int32_t tgp_kernel_table_new(tgp_ctx* ctx, tgp_kernel_table** out) {
   TGP_REQUIRE(ctx && out, "tgp_kernel_table_new: null argument");
   auto* t = new tgp_kernel_table();
   t->ctx = ctx;
-  // register_scan_kernels / register_hash_kernels (kernel/mod.rs:250-310)
+  // register_scan_kernels / register_hash_kernels / register_aggregate_kernels
+  // (kernel/mod.rs:250-346)
   for (int32_t tier : {TGP_TIER_HBM, TGP_TIER_L3, TGP_TIER_DDR5}) {
     t->table.add(std::make_shared<ScanEqGfx950>(tier));
     t->table.add(std::make_shared<ScanRangeGfx950>(tier));
@@ -308,6 +368,10 @@ int32_t tgp_kernel_table_new(tgp_ctx* ctx, tgp_kernel_table** out) {
   }
   t->table.add(std::make_shared<HashBuildGfx950>());
   t->table.add(std::make_shared<HashProbeGfx950>());
+  // the reference registers SumF64 under L3 and CountDistinct under Ddr5
+  for (int32_t tier : {TGP_TIER_HBM, TGP_TIER_L3}) t->table.add(std::make_shared<SumF64Gfx950>(tier));
+  for (int32_t tier : {TGP_TIER_HBM, TGP_TIER_DDR5})
+    t->table.add(std::make_shared<CountDistinctGfx950>(tier));
   {
     std::lock_guard<std::mutex> lk(g_owner_mu);
     for (size_t i = 0; i < t->table.size(); ++i) g_owner[t->table.at(i)] = ctx;

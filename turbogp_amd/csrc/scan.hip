@@ -271,15 +271,18 @@ using namespace tgp;
 
 namespace tgp {
 
-int32_t compact_begin(tgp_ctx* ctx, hipStream_t s, const uint64_t* d_bitmap, uint64_t nbits,
-                      uint64_t* total) {
+// Blocks of the compaction passes over nbits bits; the passes keep nblocks + 1
+// offset words (every word is written before it is read: nothing to clear).
+static uint64_t compact_blocks(uint64_t nbits) {
+  return ((nbits + 63) / 64 + kCompactWords - 1) / kCompactWords;
+}
+
+// compact_begin / compact_finish with the block offsets in `bc`.
+static int32_t compact_count_into(hipStream_t s, const uint64_t* d_bitmap, uint64_t nbits,
+                                  uint64_t* bc, uint64_t* total) {
   *total = 0;
   if (nbits == 0) return TGP_OK;
-  const uint64_t nw = (nbits + 63) / 64;
-  const uint64_t nblocks = (nw + kCompactWords - 1) / kCompactWords;
-  void* scratch = nullptr;
-  TGP_TRY(get_scratch(ctx, s, (nblocks + 1) * 8, &scratch));
-  uint64_t* bc = static_cast<uint64_t*>(scratch);
+  const uint64_t nblocks = compact_blocks(nbits);
   k_compact_count<<<(unsigned)nblocks, 256, 0, s>>>(d_bitmap, nbits, bc);
   TGP_HIP(hipGetLastError());
   k_compact_scan<<<1, 1024, 0, s>>>(bc, nblocks);
@@ -289,14 +292,11 @@ int32_t compact_begin(tgp_ctx* ctx, hipStream_t s, const uint64_t* d_bitmap, uin
   return TGP_OK;
 }
 
-int32_t compact_finish(tgp_ctx* ctx, hipStream_t s, const uint64_t* d_bitmap, uint64_t nbits,
-                       uint64_t base, int32_t index_bytes, void* d_out) {
+static int32_t compact_scatter_from(hipStream_t s, const uint64_t* d_bitmap, uint64_t nbits,
+                                    uint64_t base, int32_t index_bytes, void* d_out,
+                                    const uint64_t* bc) {
   if (nbits == 0) return TGP_OK;
-  const uint64_t nw = (nbits + 63) / 64;
-  const uint64_t nblocks = (nw + kCompactWords - 1) / kCompactWords;
-  void* scratch = nullptr;  // the block offsets compact_begin left (same size: same buffer)
-  TGP_TRY(get_scratch(ctx, s, (nblocks + 1) * 8, &scratch));
-  uint64_t* bc = static_cast<uint64_t*>(scratch);
+  const uint64_t nblocks = compact_blocks(nbits);
   if (index_bytes == 8)
     k_compact_scatter<unsigned long long><<<(unsigned)nblocks, 256, 0, s>>>(
         d_bitmap, nbits, base, bc, static_cast<unsigned long long*>(d_out));
@@ -305,6 +305,27 @@ int32_t compact_finish(tgp_ctx* ctx, hipStream_t s, const uint64_t* d_bitmap, ui
                                                                   static_cast<uint32_t*>(d_out));
   TGP_HIP(hipGetLastError());
   return TGP_OK;
+}
+
+int32_t compact_begin(tgp_ctx* ctx, hipStream_t s, const uint64_t* d_bitmap, uint64_t nbits,
+                      uint64_t* total) {
+  *total = 0;
+  if (nbits == 0) return TGP_OK;
+  const uint64_t nw = (nbits + 63) / 64;
+  const uint64_t nblocks = (nw + kCompactWords - 1) / kCompactWords;
+  void* scratch = nullptr;
+  TGP_TRY(get_scratch(ctx, s, (nblocks + 1) * 8, &scratch));
+  uint64_t* bc = static_cast<uint64_t*>(scratch);
+  return compact_count_into(s, d_bitmap, nbits, bc, total);
+}
+
+int32_t compact_finish(tgp_ctx* ctx, hipStream_t s, const uint64_t* d_bitmap, uint64_t nbits,
+                       uint64_t base, int32_t index_bytes, void* d_out) {
+  if (nbits == 0) return TGP_OK;
+  void* scratch = nullptr;  // the block offsets compact_begin left (same size: same buffer)
+  TGP_TRY(get_scratch(ctx, s, (compact_blocks(nbits) + 1) * 8, &scratch));
+  return compact_scatter_from(s, d_bitmap, nbits, base, index_bytes, d_out,
+                              static_cast<const uint64_t*>(scratch));
 }
 
 }  // namespace tgp

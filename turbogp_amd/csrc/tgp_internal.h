@@ -137,6 +137,11 @@ constexpr int kMaxZeroRanges = 4;
 int32_t zero_ranges(hipStream_t s, int count, uint64_t* const* ptrs, const uint64_t* n);
 // Number of workgroups that fill the device for a persistent streaming kernel.
 int persistent_grid(tgp_ctx* ctx, int wg_per_cu);
+// Correctly rounded f64 sum of n cells (f64 bits) under an optional bitmap
+// (synchronous; *out on return).
+int32_t sum_f64_exact(tgp_ctx* ctx, const uint64_t* d_cells, uint64_t n,
+                      const uint64_t* d_sel_bitmap, bool neg_zero_start, hipStream_t s,
+                      double* out);
 // Validates a column descriptor (non-null data, known dtype, 16-byte alignment).
 int32_t check_column(const tgp_column* c, const char* what);
 }  // namespace tgp

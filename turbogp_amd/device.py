@@ -359,6 +359,18 @@ class Context:
             res.free()
         return out
 
+    def sum_f64(self, cells, n=None, sel_bitmap=None, neg_zero_start=True, stream=None):
+        """Exact, once-rounded SUM of f64 cells (tgp_sum_f64_sync); `cells` is a
+        DeviceBuffer / DeviceColumn holding the doubles' bits."""
+        buf = cells.buf if hasattr(cells, "buf") else cells
+        n = (cells.n if hasattr(cells, "n") else buf.nbytes // 8) if n is None else n
+        out = C.c_double()
+        A.check(A.lib().tgp_sum_f64_sync(self.handle, buf.ptr, n,
+                                         sel_bitmap.buf.ptr if sel_bitmap is not None else None,
+                                         int(bool(neg_zero_start)), C.byref(out), stream),
+                "tgp_sum_f64_sync")
+        return out.value
+
     def hash_group_agg(self, q, key_cols, key_nulls=None, capacity=None, stream=None):
         """GROUP BY the key tuple of columns q.cols[key_cols] (any values; NULL keys
         from the optional per-key NULL bitmaps) through tgp_hash_group_agg_sync.
