@@ -306,6 +306,19 @@ int32_t tgp_bitmap_to_indices_sync(tgp_ctx* ctx, const uint64_t* d_bitmap, uint6
                                    uint64_t base, int32_t index_bytes, void* d_out,
                                    uint64_t out_capacity, uint64_t* h_count, void* stream);
 
+/* tgp_bitmap_to_indices_sync without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under graph capture of its
+ * neighbours, tight serving loops, harnesses that forbid hipMalloc).  The block
+ * offsets live in a fixed 512 KiB device array per GPU (set up at context
+ * creation, held until the process ends), so nbits is at most 2^30
+ * (TGP_ERR_INVALID beyond).  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (other streams, threads or contexts) race and must be
+ * serialised by the caller.  Same results as tgp_bitmap_to_indices_sync. */
+int32_t tgp_bitmap_to_indices_noalloc_sync(tgp_ctx* ctx, const uint64_t* d_bitmap,
+                                           uint64_t nbits, uint64_t base, int32_t index_bytes,
+                                           void* d_out, uint64_t out_capacity, uint64_t* h_count,
+                                           void* stream);
+
 /* ---- fused filter + GROUP BY + exact aggregation ---------------------------- */
 /* One pass over the referenced columns: preds AND-ed (plus the optional
  * bitmap), dense composite group key, COUNT(*) and up to TGP_MAX_AGGS sums,

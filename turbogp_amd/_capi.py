@@ -201,6 +201,8 @@ SIGNATURES = {
     "tgp_bitmap_not": (I32_, [P, P, P, U64_, P]),
     "tgp_bitmap_count_range": (I32_, [P, P, U64_, U64_, U64_, P, P]),
     "tgp_bitmap_to_indices_sync": (I32_, [P, P, U64_, U64_, I32_, P, U64_, C.POINTER(U64_), P]),
+    "tgp_bitmap_to_indices_noalloc_sync": (I32_, [P, P, U64_, U64_, I32_, P, U64_,
+                                                  C.POINTER(U64_), P]),
     "tgp_group_agg": (I32_, [P, C.POINTER(AggQuery), C.POINTER(GroupResult), P]),
     "tgp_group_agg_ngroups": (I32_, [C.POINTER(AggQuery), C.POINTER(U64_)]),
     "tgp_group_agg_explain": (I32_, [C.POINTER(AggQuery), C.POINTER(I32_), C.POINTER(I32_)]),

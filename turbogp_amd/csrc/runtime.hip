@@ -201,6 +201,13 @@ int32_t tgp_ctx_create(int32_t device, uint64_t hbm_budget_bytes, tgp_ctx** out)
     delete ctx;
     return TGP_ERR_HIP;
   }
+  // the device address of the no-allocation compaction's fixed scratch (once per device)
+  const int32_t rc = compact_noalloc_init(device);
+  if (rc != TGP_OK) {
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return rc;
+  }
   *out = ctx;
   return TGP_OK;
 }

@@ -307,6 +307,34 @@ static int32_t compact_scatter_from(hipStream_t s, const uint64_t* d_bitmap, uin
   return TGP_OK;
 }
 
+// ---- compaction without allocation -----------------------------------------------------
+// The block offsets of tgp_bitmap_to_indices_noalloc_sync live in this file-scope
+// device array (512 KiB per GPU, held until the process ends) instead of the
+// stream's scratch, which is allocated on first use.  Host code never passes the
+// array's name to a kernel: its device address, per device, comes from
+// hipGetSymbolAddress at context creation (compact_noalloc_init).
+constexpr uint64_t kNoAllocMaxBlocks = 65536;  // 2^30 bits of 16384-bit blocks
+constexpr uint64_t kNoAllocMaxBits = kNoAllocMaxBlocks * kCompactWords * 64;
+static __device__ __align__(256) uint64_t scratch[kNoAllocMaxBlocks + 1];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_noalloc_mu;
+static uint64_t* g_noalloc_ptr[kMaxDevices] = {};
+
+int32_t compact_noalloc_init(int32_t device) {
+  TGP_REQUIRE(device >= 0 && device < kMaxDevices, "device %d out of range", device);
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (g_noalloc_ptr[device]) return TGP_OK;
+  int cur = -1;
+  TGP_HIP(hipGetDevice(&cur));
+  TGP_REQUIRE(cur == device, "compact_noalloc_init: device %d is not current (%d)", device, cur);
+  void* p = nullptr;
+  TGP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  TGP_REQUIRE(p != nullptr, "hipGetSymbolAddress gave no address for the compaction scratch");
+  g_noalloc_ptr[device] = static_cast<uint64_t*>(p);
+  return TGP_OK;
+}
+
 int32_t compact_begin(tgp_ctx* ctx, hipStream_t s, const uint64_t* d_bitmap, uint64_t nbits,
                       uint64_t* total) {
   *total = 0;
@@ -458,6 +486,46 @@ int32_t tgp_bitmap_to_indices_sync(tgp_ctx* ctx, const uint64_t* d_bitmap, uint6
   if (total == 0) return TGP_OK;
   TGP_REQUIRE(d_out, "tgp_bitmap_to_indices_sync: null output");
   TGP_TRY(compact_finish(ctx, s, d_bitmap, nbits, base, index_bytes, d_out));
+  TGP_HIP(hipStreamSynchronize(s));
+  return TGP_OK;
+}
+
+int32_t tgp_bitmap_to_indices_noalloc_sync(tgp_ctx* ctx, const uint64_t* d_bitmap,
+                                           uint64_t nbits, uint64_t base, int32_t index_bytes,
+                                           void* d_out, uint64_t out_capacity, uint64_t* h_count,
+                                           void* stream) {
+  TGP_REQUIRE(ctx && h_count, "tgp_bitmap_to_indices_noalloc_sync: null argument");
+  TGP_REQUIRE(index_bytes == 4 || index_bytes == 8, "index_bytes must be 4 or 8");
+  TGP_TRY(check_injected(ctx));
+  *h_count = 0;
+  TGP_REQUIRE(nbits <= kNoAllocMaxBits,
+              "tgp_bitmap_to_indices_noalloc_sync: %llu bits > the maximum %llu (2^30) its "
+              "fixed scratch covers; use tgp_bitmap_to_indices_sync",
+              (unsigned long long)nbits, (unsigned long long)kNoAllocMaxBits);
+  if (nbits == 0) return TGP_OK;
+  TGP_REQUIRE(d_bitmap, "tgp_bitmap_to_indices_noalloc_sync: null bitmap");
+  if (index_bytes == 4)
+    TGP_REQUIRE(base + nbits <= (1ull << 32), "u32 indices cannot hold rows up to %llu",
+                (unsigned long long)(base + nbits));
+  uint64_t* bc = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    bc = ctx->device >= 0 && ctx->device < kMaxDevices ? g_noalloc_ptr[ctx->device] : nullptr;
+  }
+  TGP_REQUIRE(bc != nullptr, "tgp_bitmap_to_indices_noalloc_sync: scratch of device %d not set up",
+              ctx->device);
+  hipStream_t s = pick_stream(ctx, stream);
+  uint64_t total = 0;
+  TGP_TRY(compact_count_into(s, d_bitmap, nbits, bc, &total));
+  *h_count = total;
+  if (total > out_capacity) {
+    set_error("compaction needs %llu entries, capacity %llu", (unsigned long long)total,
+              (unsigned long long)out_capacity);
+    return TGP_ERR_CAPACITY;
+  }
+  if (total == 0) return TGP_OK;
+  TGP_REQUIRE(d_out, "tgp_bitmap_to_indices_noalloc_sync: null output");
+  TGP_TRY(compact_scatter_from(s, d_bitmap, nbits, base, index_bytes, d_out, bc));
   TGP_HIP(hipStreamSynchronize(s));
   return TGP_OK;
 }

@@ -126,6 +126,9 @@ int32_t compact_begin(tgp_ctx* ctx, hipStream_t s, const uint64_t* d_bitmap, uin
                       uint64_t* total);
 int32_t compact_finish(tgp_ctx* ctx, hipStream_t s, const uint64_t* d_bitmap, uint64_t nbits,
                        uint64_t base, int32_t index_bytes, void* d_out);
+// Sets up, once per device (the device current), the device address of the fixed
+// scratch that tgp_bitmap_to_indices_noalloc_sync uses.
+int32_t compact_noalloc_init(int32_t device);
 // Tier-accounted allocation used by library objects (hash tables).
 int32_t hbm_alloc(tgp_ctx* ctx, uint64_t bytes, void** out);
 int32_t hbm_free(tgp_ctx* ctx, void* p);
