@@ -75,6 +75,13 @@ def test_batch_matches_single_queries_and_oracle(ctx, table):
                      items_per_grab=5)
         b.run(A.BATCH_PERSISTENT)
         table2 = b.results()
+        b.set_tuning(2, use_graph=True, chunk_slices=1, order=3, cached_loads=True)
+        b.run(A.BATCH_PERSISTENT)
+        shared1 = b.results()
+        b.set_tuning(0, use_graph=True, chunk_slices=37, order=3, cached_loads=False)
+        b.run(A.BATCH_PERSISTENT)
+        b.run(A.BATCH_PERSISTENT)     # replay: the shared scan's counters start at zero
+        shared2 = b.results()
     finally:
         b.close()
     for i in range(len(qs)):
@@ -82,6 +89,7 @@ def test_batch_matches_single_queries_and_oracle(ctx, table):
         assert same(persistent[i], replay[i]) and same(persistent[i], replay2[i]), i
         assert same(persistent[i], plain[i]) and same(persistent[i], pipe1[i]), i
         assert same(persistent[i], table1[i]) and same(persistent[i], table2[i]), i
+        assert same(persistent[i], shared1[i]) and same(persistent[i], shared2[i]), i
     for i in range(40):
         assert same(persistent[i], single[i]), i
     # oracle on a subset (all edge queries included)
@@ -104,7 +112,7 @@ def test_batch_flush_every_2_20_rows_with_a_tiny_grid(ctx, table):
         b.set_queries(qs)
         ctx.tune(A.TUNE_GRID_MAX, 2)   # 8 waves: each wave adds millions of rows
         got = []
-        for order, chunk in ((0, 65536), (1, 7000), (2, 4500)):
+        for order, chunk in ((0, 65536), (1, 7000), (2, 4500), (3, 65536), (3, 4096)):
             b.set_tuning(0, use_graph=False, chunk_slices=chunk, order=order)
             b.run(A.BATCH_PERSISTENT)
             got.append(b.results())
@@ -126,8 +134,8 @@ def test_batch_autotune_keeps_results(ctx, table):
         b.run()
         before = b.results()
         best, ms = b.autotune()
-        assert 0 <= best["wg_per_cu"] <= 4 and best["order"] in (0, 1, 2) and ms > 0
-        assert best["chunk_slices"] in (8, 16, 32, 64, 128) and b.tuning() == best
+        assert 0 <= best["wg_per_cu"] <= 4 and best["order"] in (0, 1, 2, 3) and ms > 0
+        assert best["chunk_slices"] in (4, 8, 16, 32, 64, 128, 256) and b.tuning() == best
         b.run()
         after = b.results()
         tot, per = b.run_timed(A.BATCH_PERSISTENT)
@@ -137,3 +145,52 @@ def test_batch_autotune_keeps_results(ctx, table):
     finally:
         b.close()
     assert all(same(x, y) for x, y in zip(before, after))
+
+
+def test_shared_scan_with_non_members_and_big_values(ctx, table):
+    """Order 3 runs the family (Q1-layout queries over one set of columns with one set
+    of aggregate arguments) in the shared scan and every other batched query in the
+    table-ordered kernel of the same launch: queries over a second copy of the table,
+    a month key with another divisor, and other aggregate coefficients are not
+    members.  A copy of the table with prices near 2^50 sends its slices' sums down
+    the exact int128 path.  All equal single-query launches."""
+    host, dev = table
+    n2 = 400_000
+    big = {k: v[:n2].copy() for k, v in host.items()}
+    big["price"] = big["price"] + (1 << 50)
+    dev2 = {k: DeviceColumn.from_numpy(ctx, v) for k, v in big.items()}
+    try:
+        qs = W.batch_queries(dev, params(120))
+        first_big = len(qs)
+        qs += W.batch_queries(dev2, [(0, n2, 2000, "ab"), (1000, 300_000, 900, "month"),
+                                     (5, 6, 2525, "a")])
+        names, preds, parts, aggs = W.q1_spec(date_cutoff=1800, group="month")
+        parts7 = [(parts[0][0], 31, 0, 82)]           # dt / 31: another divisor
+        qs.append(make_agg_query([dev[c] for c in names], preds, parts7, aggs, 50, 2_000_000))
+        aggs3 = list(aggs)
+        aggs3[3] = ("sum", 1, [(2, 0, 1), (3, 99, -1), (4, 100, 1)])  # 99 - disc
+        qs.append(make_agg_query([dev[c] for c in names], preds, parts, aggs3, 0, 1_000_000))
+        b = Batch(ctx, len(qs))
+        try:
+            b.set_queries(qs)
+            assert b.info()["fallback"] == 0
+            got = []
+            for chunk in (2, 32, 4096):
+                b.set_tuning(0, use_graph=True, chunk_slices=chunk, order=3, cached_loads=True)
+                b.run(A.BATCH_PERSISTENT)
+                got.append(b.results())
+        finally:
+            b.close()
+        for i, q in enumerate(qs):
+            want = ctx.group_agg(q)
+            assert all(same(g[i], want) for g in got), i
+        # the oracle on the big-valued queries
+        for j, (bb, e, c, g) in enumerate([(0, n2, 2000, "ab"), (1000, 300_000, 900, "month"),
+                                           (5, 6, 2525, "a")]):
+            nm, pr, pa, ag = W.q1_spec(date_cutoff=c, group=g)
+            cnt, sums, oor = B.group_agg(B.make_query([big[x] for x in nm], pr, pa, ag, bb, e))
+            r = got[1][first_big + j]
+            assert [int(x) for x in r.count] == [int(x) for x in cnt] and r.sums() == sums
+    finally:
+        for c in dev2.values():
+            c.free()

@@ -178,6 +178,249 @@ __global__ void __launch_bounds__(kBlock) k_batch_items(const BatchQuery* __rest
   }
 }
 
+// ---- shared scan (order 3) ----------------------------------------------------------
+// The queries of a batch that read the same columns with the same aggregate forms
+// (the Q1 family: date range, one of the five key forms, the four sums) differ only
+// in their row range, date bounds and key parts.  A workgroup takes one table chunk
+// and the family queries overlapping it; each wave loads a 256-row slice of the
+// chunk ONCE, computes the row values (the four summed values, the key sources
+// flagA, flagB and dt / D, the big-value flags) once, and then applies every query
+// of the chunk to them from registers: its range and date test, its group id, and
+// 64-bit LDS atomics into the query's part of the workgroup's LDS region (the
+// planner gives each query an offset and 2^rshift copies so the region fits).  The
+// region is added to the exact int128 results once per chunk.  Against one pass per
+// query over the chunk, every row is read and multiplied once instead of once per
+// covering query.  A chunk has at most 2^20 rows, so with |value| < 2^43 the 64-bit
+// partials stay exact; a slice holding a larger value adds that aggregate straight
+// into the int128 result.
+constexpr int kSharedWords = 6144;           // LDS region per workgroup (48 KiB)
+constexpr uint32_t kSharedMaxSlices = 4096;  // rows per chunk <= 2^20
+
+struct SharedQuery {
+  uint64_t row_begin, row_end;
+  int64_t plo, phi;         // dt in [plo, phi]
+  int64_t gbase[2], gcard[2];
+  int32_t nparts;           // 0..2 key parts
+  int32_t src[2];           // key source: 0 dt / divisor, 1 flagA, 2 flagB
+  int32_t stride1;          // group id = part0 + part1 * stride1
+  int32_t words;            // groups * 5
+  int32_t kind;             // key form: the Q1 shape index (0 flagA x flagB, 1 flagA,
+                            // 2 flagB, 3 dt / divisor, 4 no key)
+  unsigned long long* out_count;
+  unsigned long long* out_sums;
+  unsigned long long* out_oor;
+};
+typedef const __attribute__((address_space(4))) SharedQuery ConstShared;
+typedef const __attribute__((address_space(4))) uint32_t ConstU32;
+
+struct SharedFamily {
+  SpecArgs a;         // the family's columns and aggregate arguments (Q1 layout)
+  uint32_t magic;     // dt / divisor (Granlund-Montgomery), when a member groups by it
+  int32_t shift;
+  int32_t divisor;    // 0: no member divides
+  int32_t _pad;
+};
+
+// Row values of one slice shared by the queries of a chunk (this lane's 4 rows).
+struct SharedRows {
+  int32_t dt[kSR];
+  uint32_t fa[kSR], fb[kSR];
+  int64_t dq[kSR];               // dt / divisor
+  uint64_t val[4][kSR];          // the four summed values
+  bool big[4];                   // wave-uniform: the slice has |value| >= 2^43
+  bool any_big;
+};
+
+// One query of the chunk on one slice: range and date test, group id of key form
+// KIND, and the row's five LDS atomics under one branch per row, at compile-time
+// offsets (2^RS copies per word, lane l on copy l mod 2^RS).
+template <int KIND, int RS>
+__device__ __forceinline__ void shared_query(ConstShared& D, const SharedRows& v, uint64_t row0,
+                                             uint64_t row, unsigned long long* reg) {
+  constexpr int W1 = 5;
+  uint32_t sel = (1u << kSR) - 1;
+  if (row0 < D.row_begin || row0 + kSliceRows > D.row_end) {
+    sel = 0;
+#pragma unroll
+    for (int r = 0; r < kSR; ++r)
+      if (row + r >= D.row_begin && row + r < D.row_end) sel |= 1u << r;
+  }
+  const uint32_t lo = (uint32_t)(int32_t)D.plo;
+  const uint32_t span = (uint32_t)(int32_t)D.phi - lo;  // modulo 2^32: hi - lo when lo <= hi
+  const bool never = (int32_t)D.phi < (int32_t)D.plo;
+#pragma unroll
+  for (int r = 0; r < kSR; ++r)  // lo <= dt <= hi as one unsigned compare
+    if (never || (uint32_t)v.dt[r] - lo > span) sel &= ~(1u << r);
+  if (!__ballot(sel != 0)) return;
+  uint32_t gid[kSR], ok = (1u << kSR) - 1;
+#pragma unroll
+  for (int r = 0; r < kSR; ++r) {
+    uint64_t p0 = 0, p1 = 0;
+    if constexpr (KIND == 0 || KIND == 1) p0 = (uint64_t)((int64_t)v.fa[r] - D.gbase[0]);
+    if constexpr (KIND == 2) p0 = (uint64_t)((int64_t)v.fb[r] - D.gbase[0]);
+    if constexpr (KIND == 3) p0 = (uint64_t)(v.dq[r] - D.gbase[0]);
+    if constexpr (KIND == 0) p1 = (uint64_t)((int64_t)v.fb[r] - D.gbase[1]);
+    bool in = true;
+    if constexpr (KIND <= 3) in = p0 < (uint64_t)D.gcard[0];
+    if constexpr (KIND == 0) in = in && p1 < (uint64_t)D.gcard[1];
+    if (!in) ok &= ~(1u << r);
+    uint32_t g = (uint32_t)p0;
+    if constexpr (KIND == 0) g = (uint32_t)p0 + __umul24((uint32_t)p1, (uint32_t)D.stride1);
+    gid[r] = __umul24(g, (uint32_t)(W1 << RS));
+  }
+  if constexpr (KIND <= 3) {
+    const uint32_t bad = sel & ~ok;
+    if (bad) atomicAdd(D.out_oor, (unsigned long long)__popc(bad));
+    sel &= ok;
+  }
+  unsigned long long* mine = reg + ((threadIdx.x & 63) & ((1u << RS) - 1));
+  if (!v.any_big) {
+#pragma unroll
+    for (int r = 0; r < kSR; ++r)
+      if ((sel >> r) & 1) {
+        unsigned long long* p = mine + gid[r];
+        atomicAdd(p, 1ull);
+        atomicAdd(p + (1 << RS), (unsigned long long)v.val[0][r]);
+        atomicAdd(p + (2 << RS), (unsigned long long)v.val[1][r]);
+        atomicAdd(p + (3 << RS), (unsigned long long)v.val[2][r]);
+        atomicAdd(p + (4 << RS), (unsigned long long)v.val[3][r]);
+      }
+    return;
+  }
+#pragma unroll
+  for (int r = 0; r < kSR; ++r)
+    if ((sel >> r) & 1) atomicAdd(mine + gid[r], 1ull);
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    if (v.big[a]) {
+#pragma unroll
+      for (int r = 0; r < kSR; ++r)
+        if ((sel >> r) & 1) {
+          const uint64_t g = gid[r] / (uint32_t)(W1 << RS);
+          atomic_add128_i64(&D.out_sums[(g * 4 + a) * 2], (int64_t)v.val[a][r]);
+        }
+    } else {
+#pragma unroll
+      for (int r = 0; r < kSR; ++r)
+        if ((sel >> r) & 1)
+          atomicAdd(mine + gid[r] + ((1 + a) << RS), (unsigned long long)v.val[a][r]);
+    }
+  }
+}
+
+template <int I>
+__device__ __forceinline__ void shared_dispatch(int sel_i, ConstShared& D, const SharedRows& v,
+                                                uint64_t row0, uint64_t row,
+                                                unsigned long long* reg) {
+  if constexpr (I < 5 * 7) {
+    if (sel_i == I) {
+      shared_query<I / 7, I % 7>(D, v, row0, row, reg);
+      return;
+    }
+    shared_dispatch<I + 1>(sel_i, D, v, row0, row, reg);
+  }
+}
+
+// items: 4 words each (first absolute slice, slices, first entry, entries);
+// entries: 2 words each (query index into the family, region offset | rshift << 16)
+template <bool NT>
+__global__ void __launch_bounds__(kBlock) k_batch_shared(SharedFamily F,
+                                                         const SharedQuery* __restrict__ fq,
+                                                         const uint32_t* __restrict__ items,
+                                                         const uint32_t* __restrict__ entries,
+                                                         ItemQueues Q,
+                                                         unsigned long long* counters) {
+  constexpr Shape S = kShapes[0];  // column dtypes and aggregate forms of the family
+  constexpr int W1 = 1 + S.naggs;
+  __shared__ unsigned long long region[kSharedWords];
+  __shared__ uint32_t s_item;
+  ConstShared* qs = (ConstShared*)fq;
+  ConstU32* it = (ConstU32*)items;
+  ConstU32* en = (ConstU32*)entries;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (uint32_t w = threadIdx.x; w < (uint32_t)kSharedWords; w += kBlock) region[w] = 0;
+  uint64_t min_len = F.a.col_len[0];
+#pragma unroll
+  for (int c = 1; c < S.nslots; ++c) min_len = F.a.col_len[c] < min_len ? F.a.col_len[c] : min_len;
+  const uint32_t home = blockIdx.x % Q.nq;
+  for (uint32_t t = 0; t < Q.nq; ++t) {
+    const uint32_t qi = home + t < Q.nq ? home + t : home + t - Q.nq;
+    const uint64_t base = Q.off[qi], n = Q.off[qi + 1] - base;
+    for (;;) {
+      __syncthreads();  // the region is clear and s_item free for the next grab
+      if (threadIdx.x == 0) s_item = (uint32_t)atomicAdd(&counters[qi], 1ull);
+      __syncthreads();
+      const uint64_t k = s_item;
+      if (k >= n) break;
+      const uint64_t i = base + k;
+      const uint64_t sl0 = it[4 * i];
+      const uint32_t nsl = it[4 * i + 1], e0 = it[4 * i + 2], ne = it[4 * i + 3];
+      for (uint32_t sj = wave; sj < nsl; sj += kBlock / kWave) {
+        const uint64_t row0 = (sl0 + sj) * kSliceRows;
+        const uint64_t row = row0 + lane * kSR;
+        SliceVals<S> v;
+        load_all<S, 0, NT>(F.a, row, row0 + kSliceRows <= min_len, v);
+        // the row values every query of the chunk shares: the key sources flagA,
+        // flagB and dt / divisor, the summed values and their big-value flags
+        SharedRows sr;
+#pragma unroll
+        for (int r = 0; r < kSR; ++r) {
+          sr.dt[r] = (int32_t)v.x[0][r];
+          sr.fa[r] = (uint32_t)v.x[5][r];
+          sr.fb[r] = (uint32_t)v.x[6][r];
+          int64_t x = v.x[0][r];
+          if (F.divisor > 1) {
+            if (x >= 0 && x <= 0xffffffffll) {
+              const uint32_t nn = (uint32_t)x;
+              const uint32_t q = __umulhi(F.magic, nn);
+              x = (int64_t)((q + ((nn - q) >> 1)) >> F.shift);
+            } else {
+              x = x / F.divisor;
+            }
+          }
+          sr.dq[r] = x;
+        }
+        sr.any_big = false;
+        static_for<S.naggs>([&](auto AI) {
+          constexpr int a = decltype(AI)::value;
+          agg_values<S, a>(F.a, v, sr.val[a]);
+          uint32_t m = 0;
+#pragma unroll
+          for (int r = 0; r < kSR; ++r) big_fold(m, sr.val[a][r]);
+          sr.big[a] = __ballot(big_any(m)) != 0;
+          sr.any_big = sr.any_big || sr.big[a];
+        });
+        for (uint32_t e = e0; e < e0 + ne; ++e) {
+          const uint32_t q = en[2 * e], pack = en[2 * e + 1];
+          ConstShared& D = qs[q];
+          if (row0 >= D.row_end || row0 + kSliceRows <= D.row_begin) continue;
+          shared_dispatch<0>(D.kind * 7 + (int)(pack >> 16), D, sr, row0, row,
+                             region + (pack & 0xffffu));
+        }
+      }
+      __syncthreads();
+      // every query's copies summed into its exact result; the region cleared
+      for (uint32_t e = e0; e < e0 + ne; ++e) {
+        const uint32_t q = en[2 * e], pack = en[2 * e + 1];
+        ConstShared& D = qs[q];
+        const uint32_t rs = pack >> 16, R = 1u << rs;
+        unsigned long long* reg = region + (pack & 0xffffu);
+        for (uint32_t w = threadIdx.x; w < (uint32_t)D.words; w += kBlock) {
+          unsigned long long x = 0;
+          for (uint32_t c = 0; c < R; ++c) {
+            x += reg[(w << rs) + c];
+            reg[(w << rs) + c] = 0;
+          }
+          if (!x) continue;
+          const bool is_count = w % W1 == 0;
+          add_result_word(D.out_count, D.out_sums, w, S.naggs, x,
+                          (!is_count && (int64_t)x < 0) ? -1 : 0);
+        }
+      }
+    }
+  }
+}
+
 }  // namespace tgp
 
 using namespace tgp;
@@ -215,6 +458,19 @@ struct tgp_batch {
   hipGraph_t graph_def = nullptr;
   hipStream_t graph_stream = nullptr;
   std::vector<void*> retired;  // outgrown device buffers, freed with the batch
+  // shared scan (order 3): the family members and their chunk items
+  std::vector<uint8_t> in_family;  // per plan entry
+  std::vector<SharedQuery> fam;    // members, in plan order
+  std::vector<uint32_t> fam_plan;  // plan index of each member
+  SharedFamily family{};
+  SharedQuery* d_fam = nullptr;
+  uint64_t fam_cap = 0;
+  uint32_t* d_sitems = nullptr;
+  uint32_t* d_sentries = nullptr;
+  uint64_t sitems_cap = 0, sentries_cap = 0, nsitems = 0;
+  uint64_t squeue_off[kQueues + 1] = {};
+  uint32_t sitems_chunk = 0;
+  bool sitems_valid = false;
 };
 
 // Grows a device buffer to at least `need` elements.  The old buffer is not
@@ -258,7 +514,8 @@ static int32_t persistent_launch_grid(tgp_batch* b, K kern, uint64_t units, uint
 template <bool NT>
 static int32_t launch_variant(tgp_batch* b, hipStream_t s) {
   auto* counter = reinterpret_cast<unsigned long long*>(b->d_results + b->result_words);
-  // counter[0]: chunk counter of the query-ordered kernel; counter[0..nq): queue counters
+  // counter[0]: chunk counter of the query-ordered kernel; counter[0..nq): queue counters;
+  // counter[kQueues..2 kQueues): the shared scan's queue counters
   uint64_t grid = 0;
   if (b->order == 0) {
     auto kern = k_batch<NT>;
@@ -268,15 +525,124 @@ static int32_t launch_variant(tgp_batch* b, hipStream_t s) {
                                                        b->total_slices, b->d_granule, b->chunk,
                                                        counter);
   } else {
-    auto kern = k_batch_items<NT>;
-    TGP_TRY(persistent_launch_grid(b, kern, (b->nitems + b->per_grab - 1) / b->per_grab, &grid));
-    ItemQueues Q{};
-    Q.nq = b->nqueues;
-    for (uint32_t i = 0; i <= kQueues; ++i) Q.off[i] = b->queue_off[i];
-    kern<<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>(b->d_plan, b->d_items, Q, counter,
-                                                       b->per_grab);
+    if (b->order == 3 && b->nsitems) {
+      auto skern = k_batch_shared<NT>;
+      int per_cu = 0;
+      TGP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, skern, kBlock, 0));
+      if (per_cu < 1) per_cu = 1;
+      if (b->wg_per_cu && (int)b->wg_per_cu < per_cu) per_cu = (int)b->wg_per_cu;
+      uint64_t sgrid = (uint64_t)persistent_grid(b->ctx, per_cu);
+      if (sgrid > b->nsitems) sgrid = b->nsitems;
+      ItemQueues SQ{};
+      SQ.nq = kQueues;
+      for (uint32_t i = 0; i <= kQueues; ++i) SQ.off[i] = b->squeue_off[i];
+      skern<<<dim3((unsigned)sgrid), dim3(kBlock), 0, s>>>(b->family, b->d_fam, b->d_sitems,
+                                                          b->d_sentries, SQ, counter + kQueues);
+      TGP_HIP(hipGetLastError());
+    }
+    if (b->nitems) {
+      auto kern = k_batch_items<NT>;
+      TGP_TRY(persistent_launch_grid(b, kern, (b->nitems + b->per_grab - 1) / b->per_grab,
+                                     &grid));
+      ItemQueues Q{};
+      Q.nq = b->nqueues;
+      for (uint32_t i = 0; i <= kQueues; ++i) Q.off[i] = b->queue_off[i];
+      kern<<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>(b->d_plan, b->d_items, Q, counter,
+                                                         b->per_grab);
+    }
   }
   TGP_HIP(hipGetLastError());
+  return TGP_OK;
+}
+
+// The shared scan's items for chunks of `chunk` slices: chunk k lists the family
+// members overlapping it, with each member's part of the workgroup LDS region
+// (offset, 2^rshift copies: up to 64 for one group, fewer as groups grow, fewer
+// again while the chunk's members do not fit); a chunk whose members still do not
+// fit with one copy each becomes several items over the same slices.  Chunk k's
+// items go to queue k % kQueues.
+static int32_t build_shared_items(tgp_batch* b, hipStream_t s) {
+  const uint64_t C = b->chunk < kSharedMaxSlices ? b->chunk : kSharedMaxSlices;
+  const size_t nf = b->fam.size();
+  std::vector<uint64_t> sb(nf), se(nf);
+  uint64_t kmax = 0;
+  for (size_t m = 0; m < nf; ++m) {
+    sb[m] = b->fam[m].row_begin / kSliceRows;
+    se[m] = b->fam[m].row_end > b->fam[m].row_begin
+                ? (b->fam[m].row_end + kSliceRows - 1) / kSliceRows : sb[m];
+    if (se[m] > sb[m]) kmax = (se[m] + C - 1) / C > kmax ? (se[m] + C - 1) / C : kmax;
+  }
+  std::vector<std::vector<uint32_t>> lists(kmax);
+  for (size_t m = 0; m < nf; ++m)
+    if (se[m] > sb[m])
+      for (uint64_t k = sb[m] / C; k * C < se[m]; ++k) lists[k].push_back((uint32_t)m);
+  std::vector<uint32_t> items, entries;
+  uint64_t total = 0;
+  auto rcap = [](int32_t words) {  // copies by group count (a row's words spread the lanes)
+    const int32_t groups = words / 5;
+    return groups <= 1 ? 6 : groups <= 3 ? 4 : groups <= 8 ? 3 : groups <= 32 ? 1 : 0;
+  };
+  for (uint32_t qi = 0; qi < kQueues; ++qi) {
+    b->squeue_off[qi] = total;
+    for (uint64_t k = qi; k < kmax; k += kQueues) {
+      const std::vector<uint32_t>& L = lists[k];
+      if (L.empty()) continue;
+      uint64_t s0 = (k + 1) * C, s1 = k * C;
+      for (uint32_t m : L) {
+        s0 = std::min(s0, std::max(k * C, sb[m]));
+        s1 = std::max(s1, std::min((k + 1) * C, se[m]));
+      }
+      std::vector<int32_t> r(L.size());
+      uint64_t need = 0;
+      for (size_t j = 0; j < L.size(); ++j) {
+        const int32_t w = b->fam[L[j]].words;
+        r[j] = rcap(w);
+        while (r[j] > 0 && ((uint64_t)w << r[j]) > (uint64_t)kSharedWords) --r[j];
+        need += (uint64_t)w << r[j];
+      }
+      while (need > (uint64_t)kSharedWords) {  // fewer copies for the largest parts
+        size_t jm = L.size();
+        for (size_t j = 0; j < L.size(); ++j)
+          if (r[j] > 0 && (jm == L.size() || ((uint64_t)b->fam[L[j]].words << r[j]) >
+                                                  ((uint64_t)b->fam[L[jm]].words << r[jm])))
+            jm = j;
+        if (jm == L.size()) break;
+        need -= (uint64_t)b->fam[L[jm]].words << (r[jm] - 1);
+        --r[jm];
+      }
+      // greedy items over the same slices while the parts do not fit one region
+      size_t j = 0;
+      while (j < L.size()) {
+        const uint32_t e0 = (uint32_t)(entries.size() / 2);
+        uint64_t off = 0;
+        while (j < L.size()) {
+          const uint64_t sz = (uint64_t)b->fam[L[j]].words << r[j];
+          if (off + sz > (uint64_t)kSharedWords) break;
+          entries.push_back(L[j]);
+          entries.push_back((uint32_t)off | ((uint32_t)r[j] << 16));
+          off += sz;
+          ++j;
+        }
+        items.push_back((uint32_t)s0);
+        items.push_back((uint32_t)(s1 - s0));
+        items.push_back(e0);
+        items.push_back((uint32_t)(entries.size() / 2) - e0);
+        ++total;
+      }
+    }
+  }
+  b->squeue_off[kQueues] = total;
+  if (items.empty()) items.assign(4, 0);
+  if (entries.empty()) entries.assign(2, 0);
+  TGP_TRY(ensure_cap(b, &b->d_sitems, &b->sitems_cap, (uint64_t)items.size()));
+  TGP_TRY(ensure_cap(b, &b->d_sentries, &b->sentries_cap, (uint64_t)entries.size()));
+  TGP_HIP(hipMemcpyAsync(b->d_sitems, items.data(), items.size() * 4, hipMemcpyHostToDevice, s));
+  TGP_HIP(hipMemcpyAsync(b->d_sentries, entries.data(), entries.size() * 4,
+                         hipMemcpyHostToDevice, s));
+  TGP_HIP(hipStreamSynchronize(s));
+  b->nsitems = total;
+  b->sitems_chunk = b->chunk;
+  b->sitems_valid = true;
   return TGP_OK;
 }
 
@@ -288,10 +654,14 @@ static int32_t launch_variant(tgp_batch* b, hipStream_t s) {
 //            has one item per chunk it overlaps, so the items of all queries on
 //            chunk k cover the same rows; chunk k's items go to queue k % kQueues.
 static int32_t build_items(tgp_batch* b, hipStream_t s) {
+  if (b->order == 3 && !(b->sitems_valid && b->sitems_chunk == b->chunk))
+    TGP_TRY(build_shared_items(b, s));
   if (b->order == 0 || (b->items_chunk == b->chunk && b->items_order == b->order &&
                         b->items_valid))
     return TGP_OK;
-  const uint32_t nq = b->order == 2 ? kQueues : 1;
+  const uint32_t nq = b->order >= 2 ? kQueues : 1;
+  // order 3: the family members run in the shared scan, the rest as in order 2
+  auto skip = [&](uint32_t q) { return b->order == 3 && b->in_family[q]; };
   std::vector<uint32_t> items;
   uint64_t total = 0;
   if (b->order == 1) {
@@ -324,18 +694,22 @@ static int32_t build_items(tgp_batch* b, hipStream_t s) {
     // overlap it, visited in query order.
     const uint64_t C = (uint64_t)b->chunk * kSliceRows;  // rows per absolute chunk
     uint64_t kmax = 0;
-    for (const BatchQuery& bq : b->plan)
-      if (bq.a.nslices) {
+    for (uint32_t q = 0; q < b->plan.size(); ++q) {
+      const BatchQuery& bq = b->plan[q];
+      if (bq.a.nslices && !skip(q)) {
         const uint64_t end = bq.a.base0 + bq.a.nslices * kSliceRows;
         kmax = (end + C - 1) / C > kmax ? (end + C - 1) / C : kmax;
       }
+    }
     // position of chunk k's first item: queues in order, chunks k = i, i + 8, ... in queue i
     std::vector<uint64_t> at(kmax + 1, 0);
-    for (const BatchQuery& bq : b->plan)
-      if (bq.a.nslices) {
+    for (uint32_t q = 0; q < b->plan.size(); ++q) {
+      const BatchQuery& bq = b->plan[q];
+      if (bq.a.nslices && !skip(q)) {
         const uint64_t end = bq.a.base0 + bq.a.nslices * kSliceRows;
         for (uint64_t k = bq.a.base0 / C; k * C < end; ++k) at[k] += 1;
       }
+    }
     for (uint32_t i = 0; i < nq; ++i) {
       b->queue_off[i] = total;
       for (uint64_t k = i; k < kmax; k += nq) {
@@ -347,7 +721,7 @@ static int32_t build_items(tgp_batch* b, hipStream_t s) {
     items.resize(3 * (total ? total : 1));
     for (uint32_t q = 0; q < b->plan.size(); ++q) {
       const SpecArgs& A = b->plan[q].a;
-      if (A.nslices == 0) continue;
+      if (A.nslices == 0 || skip(q)) continue;
       const uint64_t end = A.base0 + A.nslices * kSliceRows;
       for (uint64_t k = A.base0 / C; k * C < end; ++k) {
         const uint64_t r0 = k * C > A.base0 ? k * C : A.base0;
@@ -374,7 +748,7 @@ static int32_t build_items(tgp_batch* b, hipStream_t s) {
 
 static int32_t batch_launch(tgp_batch* b, hipStream_t s) {
   // results and the counters behind them start at zero
-  TGP_TRY(fill_u64(s, b->d_results, b->result_words + kQueues, 0));
+  TGP_TRY(fill_u64(s, b->d_results, b->result_words + 2 * kQueues, 0));
   if (!b->total_slices) return TGP_OK;
   return b->cached ? launch_variant<false>(b, s) : launch_variant<true>(b, s);
 }
@@ -398,6 +772,9 @@ int32_t tgp_batch_destroy(tgp_batch* b) {
   if (b->d_results) hbm_free(b->ctx, b->d_results);
   if (b->d_granule) hbm_free(b->ctx, b->d_granule);
   if (b->d_items) hbm_free(b->ctx, b->d_items);
+  if (b->d_fam) hbm_free(b->ctx, b->d_fam);
+  if (b->d_sitems) hbm_free(b->ctx, b->d_sitems);
+  if (b->d_sentries) hbm_free(b->ctx, b->d_sentries);
   for (void* p : b->retired) hbm_free(b->ctx, p);
   delete b;
   return TGP_OK;
@@ -434,7 +811,7 @@ int32_t tgp_batch_set_queries(tgp_batch* b, const tgp_agg_query* qs, uint32_t n,
   }
   b->result_words = words;
   // the arena ends with the persistent kernels' work counters
-  TGP_TRY(ensure_cap(b, &b->d_results, &b->results_cap, words + kQueues));
+  TGP_TRY(ensure_cap(b, &b->d_results, &b->results_cap, words + 2 * kQueues));
   uint64_t slices = 0;
   for (uint32_t i = 0; i < n; ++i) {
     Shape sh;
@@ -476,15 +853,78 @@ int32_t tgp_batch_set_queries(tgp_batch* b, const tgp_agg_query* qs, uint32_t n,
     TGP_HIP(hipMemcpyAsync(b->d_granule, gran.data(), gran.size() * 4, hipMemcpyHostToDevice, s));
     TGP_HIP(hipStreamSynchronize(s));
   }
+  // the shared-scan family: Q1-layout queries on the first such query's columns
+  // with its aggregate arguments (and, grouping by dt / divisor, its divisor)
+  b->in_family.assign(b->plan.size(), 0);
+  b->fam.clear();
+  b->fam_plan.clear();
+  b->family = SharedFamily{};
+  int32_t ref = -1;
+  for (uint32_t q = 0; q < b->plan.size(); ++q) {
+    const BatchQuery& bq = b->plan[q];
+    if (bq.shape > 4 || bq.naggs != kShapes[0].naggs) continue;
+    const SpecArgs& A = bq.a;
+    if (ref < 0) {
+      ref = (int32_t)q;
+      b->family.a = A;
+    }
+    const SpecArgs& R = b->family.a;
+    bool same = true;
+    for (int c = 0; c < kShapes[0].nslots; ++c)
+      same = same && A.col[c] == R.col[c] && A.col_len[c] == R.col_len[c];
+    for (int a = 0; a < kShapes[0].naggs; ++a) {
+      same = same && A.coef[a] == R.coef[a];
+      for (int k = 0; k < TGP_MAX_FACTORS; ++k)
+        same = same && A.fa[a][k] == R.fa[a][k] && A.fb[a][k] == R.fb[a][k];
+    }
+    const Shape& S = kShapes[bq.shape];
+    for (int k = 0; k < S.nparts; ++k)
+      if (S.gdiv[k]) {
+        if (b->family.divisor == 0) {
+          b->family.divisor = A.gdivisor[k];
+          b->family.magic = A.gmagic[k];
+          b->family.shift = A.gshift[k];
+        }
+        same = same && A.gdivisor[k] == b->family.divisor;
+      }
+    if (!same) continue;
+    SharedQuery m{};
+    m.row_begin = A.row_begin;
+    m.row_end = A.row_end;
+    m.plo = A.plo[0];
+    m.phi = A.phi[0];
+    m.nparts = S.nparts;
+    for (int k = 0; k < S.nparts; ++k) {
+      m.src[k] = S.gslot[k] == 5 ? 1 : S.gslot[k] == 6 ? 2 : 0;
+      m.gbase[k] = A.gbase[k];
+      m.gcard[k] = A.gcard[k];
+    }
+    m.stride1 = S.nparts == 2 ? (int32_t)A.gstride[1] : 0;
+    m.words = (int32_t)A.words;
+    m.kind = bq.shape;
+    m.out_count = A.out_count;
+    m.out_sums = A.out_sums;
+    m.out_oor = A.out_oor;
+    b->fam.push_back(m);
+    b->fam_plan.push_back(q);
+    b->in_family[q] = 1;
+  }
+  if (!b->fam.empty()) {
+    TGP_TRY(ensure_cap(b, &b->d_fam, &b->fam_cap, (uint64_t)b->fam.size()));
+    TGP_HIP(hipMemcpyAsync(b->d_fam, b->fam.data(), b->fam.size() * sizeof(SharedQuery),
+                           hipMemcpyHostToDevice, s));
+    TGP_HIP(hipStreamSynchronize(s));
+  }
   b->items_valid = false;
+  b->sitems_valid = false;
   return build_items(b, s);
 }
 
 int32_t tgp_batch_set_tuning(tgp_batch* b, const tgp_batch_tuning* t) {
   TGP_REQUIRE(b && t, "tgp_batch_set_tuning: null argument");
   TGP_REQUIRE(t->chunk_slices <= (1u << 16), "chunk of %u slices above 65536", t->chunk_slices);
-  TGP_REQUIRE(t->order >= 0 && t->order <= 2, "order %d not 0 (queries), 1 (table) or 2 "
-              "(table, XCD-affine)", t->order);
+  TGP_REQUIRE(t->order >= 0 && t->order <= 3, "order %d not 0 (queries), 1 (table), 2 "
+              "(table, XCD-affine) or 3 (shared scan)", t->order);
   TGP_REQUIRE(t->items_per_grab <= 64, "items_per_grab %u above 64", t->items_per_grab);
   drop_graph(b);
   b->wg_per_cu = t->wg_per_cu;
@@ -669,7 +1109,10 @@ int32_t tgp_batch_autotune(tgp_batch* b, void* stream, tgp_batch_tuning* best_ou
   for (int cached = 0; cached < 2; ++cached)
     for (uint32_t chunk : {8u, 16u, 32u, 64u}) trial(0, chunk, 1, cached != 0, 1);
   for (uint32_t chunk : {8u, 16u, 32u, 64u}) trial(0, chunk, 2, true, 1);
-  if (best_t.order != 0) {
+  if (!b->fam.empty())
+    for (int cached = 0; cached < 2; ++cached)
+      for (uint32_t chunk : {16u, 32u, 64u, 128u, 256u}) trial(0, chunk, 3, cached != 0, 1);
+  if (best_t.order == 1 || best_t.order == 2) {
     const tgp_batch_tuning t1 = best_t;
     for (uint32_t chunk : {4u, 8u, 16u, 32u})
       for (uint32_t grab : {2u, 4u, 8u}) trial(0, chunk, t1.order, t1.cached_loads != 0, grab);
