@@ -622,8 +622,8 @@ def main():
                      "traffic_over_bytes": None if traffic is None else
                      round(traffic / unique_bytes, 3),
                      "per_query_bytes_per_launch": int(batch_bytes),
-                     "kernel": "persistent batch kernel (k_batch_items in table order, or "
-                               "k_batch in query order, as autotuned): the cells the batch reads, "
+                     "kernel": "persistent batch kernel (k_batch_shared, the shared scan, or "
+                               "k_batch_items / k_batch, as autotuned): the cells the batch reads, "
                                "each once (per column, the union of the row ranges of the "
                                "queries that read it, x its width) / mean launch time (hipEvents "
                                "on the batch stream, graph replay incl. the result-zeroing "

@@ -193,7 +193,9 @@ __global__ void __launch_bounds__(kBlock) k_batch_items(const BatchQuery* __rest
 // covering query.  A chunk has at most 2^20 rows, so with |value| < 2^43 the 64-bit
 // partials stay exact; a slice holding a larger value adds that aggregate straight
 // into the int128 result.
-constexpr int kSharedWords = 6144;           // LDS region per workgroup (48 KiB)
+constexpr int kSharedWords = 6144;           // LDS region per workgroup (48 KiB): 3 per CU
+static_assert(kShapes[0].dt[0] == TGP_I32 && kShapes[0].dt[5] == TGP_U8 &&
+              kShapes[0].dt[6] == TGP_U8, "the shared scan packs the Q1 key columns");
 constexpr uint32_t kSharedMaxSlices = 4096;  // rows per chunk <= 2^20
 
 struct SharedQuery {
@@ -224,8 +226,8 @@ struct SharedFamily {
 // Row values of one slice shared by the queries of a chunk (this lane's 4 rows).
 struct SharedRows {
   int32_t dt[kSR];
-  uint32_t fa[kSR], fb[kSR];
-  int64_t dq[kSR];               // dt / divisor
+  uint32_t fab[kSR];             // flagA | flagB << 8 (u8 columns)
+  int32_t dq[kSR];               // dt / divisor (|dt / divisor| < 2^31 for int32 dt)
   uint64_t val[4][kSR];          // the four summed values
   bool big[4];                   // wave-uniform: the slice has |value| >= 2^43
   bool any_big;
@@ -256,10 +258,10 @@ __device__ __forceinline__ void shared_query(ConstShared& D, const SharedRows& v
 #pragma unroll
   for (int r = 0; r < kSR; ++r) {
     uint64_t p0 = 0, p1 = 0;
-    if constexpr (KIND == 0 || KIND == 1) p0 = (uint64_t)((int64_t)v.fa[r] - D.gbase[0]);
-    if constexpr (KIND == 2) p0 = (uint64_t)((int64_t)v.fb[r] - D.gbase[0]);
-    if constexpr (KIND == 3) p0 = (uint64_t)(v.dq[r] - D.gbase[0]);
-    if constexpr (KIND == 0) p1 = (uint64_t)((int64_t)v.fb[r] - D.gbase[1]);
+    if constexpr (KIND == 0 || KIND == 1) p0 = (uint64_t)((int64_t)(v.fab[r] & 0xffu) - D.gbase[0]);
+    if constexpr (KIND == 2) p0 = (uint64_t)((int64_t)(v.fab[r] >> 8) - D.gbase[0]);
+    if constexpr (KIND == 3) p0 = (uint64_t)((int64_t)v.dq[r] - D.gbase[0]);
+    if constexpr (KIND == 0) p1 = (uint64_t)((int64_t)(v.fab[r] >> 8) - D.gbase[1]);
     bool in = true;
     if constexpr (KIND <= 3) in = p0 < (uint64_t)D.gcard[0];
     if constexpr (KIND == 0) in = in && p1 < (uint64_t)D.gcard[1];
@@ -308,17 +310,22 @@ __device__ __forceinline__ void shared_query(ConstShared& D, const SharedRows& v
   }
 }
 
-template <int I>
+// shared_query for key form sel_i / 7 and 2^(sel_i % 7) copies, through one switch
+// (a jump table on the wave-uniform index, not a chain of compares).
 __device__ __forceinline__ void shared_dispatch(int sel_i, ConstShared& D, const SharedRows& v,
                                                 uint64_t row0, uint64_t row,
                                                 unsigned long long* reg) {
-  if constexpr (I < 5 * 7) {
-    if (sel_i == I) {
-      shared_query<I / 7, I % 7>(D, v, row0, row, reg);
-      return;
-    }
-    shared_dispatch<I + 1>(sel_i, D, v, row0, row, reg);
+#define TGP_SQ_CASE(I) \
+  case I: shared_query<(I) / 7, (I) % 7>(D, v, row0, row, reg); break;
+#define TGP_SQ_KIND(K) \
+  TGP_SQ_CASE(7 * (K)) TGP_SQ_CASE(7 * (K) + 1) TGP_SQ_CASE(7 * (K) + 2) TGP_SQ_CASE(7 * (K) + 3) \
+  TGP_SQ_CASE(7 * (K) + 4) TGP_SQ_CASE(7 * (K) + 5) TGP_SQ_CASE(7 * (K) + 6)
+  switch (sel_i) {
+    TGP_SQ_KIND(0) TGP_SQ_KIND(1) TGP_SQ_KIND(2) TGP_SQ_KIND(3) TGP_SQ_KIND(4)
+    default: break;
   }
+#undef TGP_SQ_KIND
+#undef TGP_SQ_CASE
 }
 
 // items: 4 words each (first absolute slice, slices, first entry, entries);
@@ -366,8 +373,7 @@ __global__ void __launch_bounds__(kBlock) k_batch_shared(SharedFamily F,
 #pragma unroll
         for (int r = 0; r < kSR; ++r) {
           sr.dt[r] = (int32_t)v.x[0][r];
-          sr.fa[r] = (uint32_t)v.x[5][r];
-          sr.fb[r] = (uint32_t)v.x[6][r];
+          sr.fab[r] = (uint32_t)v.x[5][r] | ((uint32_t)v.x[6][r] << 8);
           int64_t x = v.x[0][r];
           if (F.divisor > 1) {
             if (x >= 0 && x <= 0xffffffffll) {
@@ -378,7 +384,7 @@ __global__ void __launch_bounds__(kBlock) k_batch_shared(SharedFamily F,
               x = x / F.divisor;
             }
           }
-          sr.dq[r] = x;
+          sr.dq[r] = (int32_t)x;
         }
         sr.any_big = false;
         static_for<S.naggs>([&](auto AI) {
@@ -394,7 +400,7 @@ __global__ void __launch_bounds__(kBlock) k_batch_shared(SharedFamily F,
           const uint32_t q = en[2 * e], pack = en[2 * e + 1];
           ConstShared& D = qs[q];
           if (row0 >= D.row_end || row0 + kSliceRows <= D.row_begin) continue;
-          shared_dispatch<0>(D.kind * 7 + (int)(pack >> 16), D, sr, row0, row,
+          shared_dispatch(D.kind * 7 + (int)(pack >> 16), D, sr, row0, row,
                              region + (pack & 0xffffu));
         }
       }
@@ -578,9 +584,9 @@ static int32_t build_shared_items(tgp_batch* b, hipStream_t s) {
       for (uint64_t k = sb[m] / C; k * C < se[m]; ++k) lists[k].push_back((uint32_t)m);
   std::vector<uint32_t> items, entries;
   uint64_t total = 0;
-  auto rcap = [](int32_t words) {  // copies by group count (a row's words spread the lanes)
-    const int32_t groups = words / 5;
-    return groups <= 1 ? 6 : groups <= 3 ? 4 : groups <= 8 ? 3 : groups <= 32 ? 1 : 0;
+  auto rcap = [](int32_t words) {  // copies by group count: lanes of one atomic spread
+    const int32_t groups = words / 5;  // over groups x copies words
+    return groups <= 3 ? 6 : groups <= 8 ? 5 : groups <= 32 ? 2 : 1;
   };
   for (uint32_t qi = 0; qi < kQueues; ++qi) {
     b->squeue_off[qi] = total;
