@@ -401,8 +401,12 @@ int32_t tgp_batch_set_queries(tgp_batch* b, const tgp_agg_query* qs, uint32_t n,
  * then of query 1, ...), 1 = units sorted by table position, so the queries
  * that cover a table region read it at about the same time, 2 = as 1 with
  * units cut at absolute table chunks and chunk k served by XCD k % 8, so those
- * queries also share one L2; cached_loads: 1 = keep column lines in the caches
- * (pays off with orders 1 and 2), 0 = streaming loads;
+ * queries also share one L2, 3 = shared scan: the queries that read the same
+ * columns with the same aggregate forms (the Q1 layout) share one read of each
+ * table chunk (chunk_slices <= 4096 per chunk): a workgroup loads each slice of
+ * the chunk once and applies every overlapping query to it; the other queries
+ * run as in 2; cached_loads: 1 = keep column lines in the caches (pays off with
+ * orders 1 and 2), 0 = streaming loads;
  * use_graph: replay a captured hipGraph (0 = plain launches); items_per_grab:
  * work units a wave takes per counter atomic in orders 1 and 2 (0 = 1; at
  * most 64). */
@@ -604,7 +608,9 @@ typedef struct tgp_join_agg_query {
   tgp_column value;       /* summed column (int64 / int32 ...) */
   tgp_column group_col;   /* used when group_by == TGP_JOIN_GROUP_PROBE_COL */
   int32_t group_by;
-  int32_t _pad;
+  uint32_t build_rows;    /* 0: every build row matches; n > 0: only build rows below n
+                             (a join with the first n rows of a table built over more of
+                             the same build side, e.g. one kept resident for many joins) */
   int64_t key_base;
   uint64_t ngroups;
   uint64_t row_begin, row_end;
@@ -647,7 +653,12 @@ typedef struct tgp_serve_config {
   uint32_t wg_per_cu, chunk_slices; /* batch tuning (tgp_batch_tuning) */
   int32_t order, cached_loads;
   int32_t per_query_launch;    /* 1: no batching, one tgp_group_agg launch per request */
-  int32_t _pad2;
+  int32_t share_build_tables;  /* 1: one resident hash table per build side (key and payload
+                                  columns) serves every join on a prefix of it, probed with
+                                  the request's prefix bound (build_rows); it grows (a new
+                                  build) when a request needs a longer prefix and is held in
+                                  the HBM tier until the server ends.  0: every join builds
+                                  its own table, admitted by its footprint */
 } tgp_serve_config;
 typedef struct tgp_serve_timing {
   int64_t arrival_ns, start_ns, done_ns; /* tgp_serve_now_ns() clock */
@@ -661,6 +672,8 @@ typedef struct tgp_serve_stats {
    * work (batch planning and upload, or the single launches), and completing it
    * (result copies) */
   uint64_t submit_ns, launch_ns, harvest_ns;
+  uint64_t table_builds;       /* hash tables built (with share_build_tables: per build side
+                                  and growth, not per join) */
 } tgp_serve_stats;
 int64_t tgp_serve_now_ns(void);
 int32_t tgp_serve_create(tgp_ctx* ctx, const tgp_serve_config* cfg, tgp_server** out);

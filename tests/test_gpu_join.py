@@ -390,3 +390,44 @@ def test_probe_agg_past_2_32_additions_per_word(ctx):
     assert m > 2 ** 32
     assert int(got.count[0]) == m and int(got.count[1:].sum()) == 0
     assert got.sums()[0][0] == m * v
+
+
+@pytest.mark.parametrize("group_by,ngroups", [(A.JOIN_GROUP_BUILD_PAYLOAD, 25),
+                                              (A.JOIN_GROUP_BUILD_PAYLOAD, 4000),
+                                              (A.JOIN_GROUP_PROBE_COL, 70_000),
+                                              (A.JOIN_GROUP_NONE, 1)])
+@pytest.mark.parametrize("dups", [False, True])
+def test_probe_agg_over_a_build_prefix(ctx, group_by, ngroups, dups):
+    """build_rows = nb joins with the first nb build rows of a table built over all of
+    them: equal to a table built on the prefix alone (LDS, global and partitioned
+    forms), with unique and with duplicate build keys, and to the oracle."""
+    rng = np.random.default_rng(40 + ngroups + dups)
+    nb_all, npb = 30_000, 400_000
+    bk = (rng.integers(0, nb_all // 3, nb_all) if dups else rng.permutation(nb_all) * 13 + 5)
+    bk = bk.astype(np.int64)
+    bp = rng.integers(0, min(ngroups, 4000), nb_all).astype(np.int32)
+    pk = bk[rng.integers(0, nb_all, npb)]
+    pk[::9] -= 1                                   # misses
+    pv = rng.integers(-2**40, 2**40, npb).astype(np.int64)
+    pg = rng.integers(0, ngroups, npb).astype(np.int32)
+    full, keep = build(ctx, bk, bp)
+    cols = [dev(ctx, pk), dev(ctx, pv), dev(ctx, pg)]
+    try:
+        for nb in (1, 777, 12_345, nb_all):
+            part, keep2 = build(ctx, bk[:nb], bp[:nb])
+            try:
+                want = part.probe_agg(cols[0], cols[1], group_by, ngroups, 0, cols[2])
+            finally:
+                part.free()
+                for c in keep2:
+                    c.free()
+            got = full.probe_agg(cols[0], cols[1], group_by, ngroups, 0, cols[2],
+                                 build_rows=nb)
+            assert np.array_equal(got.count, want.count), nb
+            assert got.sums() == want.sums() and got.n_out_of_range == want.n_out_of_range, nb
+            cnt, sums, oor = B.join_agg(bk[:nb], bp[:nb], pk, pv, pg, group_by, 0, ngroups)
+            assert np.array_equal(got.count, cnt) and [x[0] for x in got.sums()] == sums, nb
+    finally:
+        full.free()
+        for c in keep + cols:
+            c.free()

@@ -118,7 +118,8 @@ def test_joins_wait_for_hbm_budget_and_oversized_ones_are_rejected(ctx, tables):
            ("join", 7, 300_000, N_BUILD // 2), ("q1", 0, 1_000_000, 0)]
     reqs = W.serve_requests(fact, build, probe, mix)
     # room for one full-size table at a time: the second join waits for the first
-    srv = Server(ctx, join_slots=4, hash_budget_bytes=one + one // 4)
+    # (every join builds its own table here)
+    srv = Server(ctx, join_slots=4, hash_budget_bytes=one + one // 4, share_build_tables=False)
     try:
         ids, _ = srv.replay(reqs, [0] * len(reqs))
         got = [srv.result(i) for i in ids]
@@ -129,16 +130,62 @@ def test_joins_wait_for_hbm_budget_and_oversized_ones_are_rejected(ctx, tables):
     for (res, t), desc in zip(got, mix):
         assert t["status"] == A.OK
         assert same(res, direct(ctx, tables, desc)), desc
-    # a join whose table can never fit is rejected, not queued forever
-    srv = Server(ctx, hash_budget_bytes=one // 2)
+    # a join whose table can never fit is rejected, not queued forever (either mode)
+    for share in (False, True):
+        srv = Server(ctx, hash_budget_bytes=one // 2, share_build_tables=share)
+        try:
+            rid = srv.submit(reqs[0])
+            srv.wait(rid)
+            res, t = srv.result(rid)
+            st = srv.stats()
+        finally:
+            srv.close()
+        assert res is None and t["status"] == A.ERR_BUDGET and st["rejected"] == 1, share
+
+
+def test_joins_share_one_table_per_build_side(ctx, tables):
+    """Joins over prefixes of one build side probe one resident table with their prefix
+    bound (built again only when a longer prefix is asked for); joins over another
+    build side (a copy of the key and payload columns) get a table of their own.  Every
+    result equals the direct build-and-probe of its own prefix, in both serving forms
+    and in a mix with scans."""
+    from turbogp_amd import DeviceColumn
+    fact, build, probe = tables
+    rng = np.random.default_rng(77)
+    nbs = [int(x) for x in rng.integers(1, N_BUILD + 1, 60)] + [N_BUILD, 5, N_BUILD // 3]
+    mix = [("join", int(b), int(b) + 200_000, nb)
+           for b, nb in zip(rng.integers(0, N_PROBE - 200_000, len(nbs)), nbs)]
+    mix += [("q6", 100, 900_000, 0), ("q1", 0, 2_000_000, 0)]
+    reqs = W.serve_requests(fact, build, probe, mix)
+    copy = {k: DeviceColumn.from_numpy(ctx, build[k].to_numpy()) for k in ("bkey", "pay")}
     try:
-        rid = srv.submit(reqs[0])
-        srv.wait(rid)
-        res, t = srv.result(rid)
-        st = srv.stats()
+        mix2 = [("join", 0, 1_000_000, 150_000), ("join", 3, 2_000_000, N_BUILD)]
+        reqs += W.serve_requests(fact, copy, probe, mix2)
+        got = {}
+        for share in (True, False):
+            for per_query in (False, True):
+                srv = Server(ctx, join_slots=4, batch_slots=2, share_build_tables=share,
+                             per_query_launch=per_query)
+                try:
+                    ids, _ = srv.replay(reqs, W.poisson_offsets(len(reqs), 20_000))
+                    got[(share, per_query)] = [srv.result(i)[0] for i in ids]
+                    st = srv.stats()
+                finally:
+                    srv.close()
+                njoins = len(nbs) + len(mix2)
+                if share:
+                    assert 2 <= st["table_builds"] < njoins // 3, st
+                else:
+                    assert st["table_builds"] == njoins, st
+        for i, desc in enumerate(mix):
+            want = direct(ctx, tables, desc)
+            assert all(same(g[i], want) for g in got.values()), (i, desc)
+        for j, desc in enumerate(mix2):
+            want = direct(ctx, (fact, copy, probe), desc)
+            assert all(same(g[len(mix) + j], want) for g in got.values()), desc
     finally:
-        srv.close()
-    assert res is None and t["status"] == A.ERR_BUDGET and st["rejected"] == 1
+        for c in copy.values():
+            c.free()
 
 
 def test_heavy_requests_batch_on_their_own(ctx, tables):

@@ -101,7 +101,7 @@ class DeviceInfo(C.Structure):
 
 class JoinAggQuery(C.Structure):
     _fields_ = [("table", C.c_void_p), ("probe_keys", Column), ("value", Column),
-                ("group_col", Column), ("group_by", C.c_int32), ("_pad", C.c_int32),
+                ("group_col", Column), ("group_by", C.c_int32), ("build_rows", C.c_uint32),
                 ("key_base", C.c_int64), ("ngroups", C.c_uint64),
                 ("row_begin", C.c_uint64), ("row_end", C.c_uint64)]
 
@@ -149,7 +149,7 @@ class ServeConfig(C.Structure):
                 ("heavy_rows", C.c_uint64),
                 ("hash_budget_bytes", C.c_uint64), ("wg_per_cu", C.c_uint32),
                 ("chunk_slices", C.c_uint32), ("order", C.c_int32), ("cached_loads", C.c_int32),
-                ("per_query_launch", C.c_int32), ("_pad2", C.c_int32)]
+                ("per_query_launch", C.c_int32), ("share_build_tables", C.c_int32)]
 
 
 class ServeTiming(C.Structure):
@@ -161,7 +161,7 @@ class ServeStats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "submitted", "completed", "failed", "rejected", "deferred", "admitted_joins",
         "batches", "batched_requests", "max_batch", "heavy_in_flight",
-        "submit_ns", "launch_ns", "harvest_ns")]
+        "submit_ns", "launch_ns", "harvest_ns", "table_builds")]
 
 
 # name -> (restype, argtypes)

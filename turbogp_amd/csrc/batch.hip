@@ -19,6 +19,7 @@
 // baseline mode.  The reference has no batched executor; its per-query entry is
 // QueryEngine::execute (engine/mod.rs:1487).
 #include <algorithm>
+#include <map>
 #include <utility>
 #include <vector>
 
@@ -208,6 +209,8 @@ struct SharedQuery {
   int32_t words;            // groups * 5
   int32_t kind;             // key form: the Q1 shape index (0 flagA x flagB, 1 flagA,
                             // 2 flagB, 3 dt / divisor, 4 no key)
+  uint32_t sig;             // equal for members with equal date bounds and key parts
+  uint32_t _pad;
   unsigned long long* out_count;
   unsigned long long* out_sums;
   unsigned long long* out_oor;
@@ -233,13 +236,16 @@ struct SharedRows {
   bool any_big;
 };
 
-// One query of the chunk on one slice: range and date test, group id of key form
-// KIND, and the row's five LDS atomics under one branch per row, at compile-time
-// offsets (2^RS copies per word, lane l on copy l mod 2^RS).
+// One query of the chunk on one slice, first half: the rows it selects (range and
+// date test), their group ids of key form KIND (premultiplied by the word stride of
+// 2^RS copies) and the selected rows whose key part is out of range (`bad`, left
+// out of sel).  Returns sel; no lane selecting anything leaves gid unset.
 template <int KIND, int RS>
-__device__ __forceinline__ void shared_query(ConstShared& D, const SharedRows& v, uint64_t row0,
-                                             uint64_t row, unsigned long long* reg) {
+__device__ __forceinline__ uint32_t shared_select(ConstShared& D, const SharedRows& v,
+                                                  uint64_t row0, uint64_t row,
+                                                  uint32_t (&gid)[kSR], uint32_t& bad) {
   constexpr int W1 = 5;
+  bad = 0;
   uint32_t sel = (1u << kSR) - 1;
   if (row0 < D.row_begin || row0 + kSliceRows > D.row_end) {
     sel = 0;
@@ -253,8 +259,8 @@ __device__ __forceinline__ void shared_query(ConstShared& D, const SharedRows& v
 #pragma unroll
   for (int r = 0; r < kSR; ++r)  // lo <= dt <= hi as one unsigned compare
     if (never || (uint32_t)v.dt[r] - lo > span) sel &= ~(1u << r);
-  if (!__ballot(sel != 0)) return;
-  uint32_t gid[kSR], ok = (1u << kSR) - 1;
+  if (!__ballot(sel != 0)) return 0;
+  uint32_t ok = (1u << kSR) - 1;
 #pragma unroll
   for (int r = 0; r < kSR; ++r) {
     uint64_t p0 = 0, p1 = 0;
@@ -271,10 +277,20 @@ __device__ __forceinline__ void shared_query(ConstShared& D, const SharedRows& v
     gid[r] = __umul24(g, (uint32_t)(W1 << RS));
   }
   if constexpr (KIND <= 3) {
-    const uint32_t bad = sel & ~ok;
-    if (bad) atomicAdd(D.out_oor, (unsigned long long)__popc(bad));
+    bad = sel & ~ok;
     sel &= ok;
   }
+  return sel;
+}
+
+// Second half: the selected rows' five LDS atomics under one branch per row, at
+// compile-time offsets (lane l on copy l mod 2^RS); a slice holding a big value
+// adds that aggregate straight into the int128 result.
+template <int RS>
+__device__ __forceinline__ void shared_accumulate(ConstShared& D, const SharedRows& v,
+                                                  uint32_t sel, const uint32_t (&gid)[kSR],
+                                                  unsigned long long* reg) {
+  constexpr int W1 = 5;
   unsigned long long* mine = reg + ((threadIdx.x & 63) & ((1u << RS) - 1));
   if (!v.any_big) {
 #pragma unroll
@@ -310,22 +326,41 @@ __device__ __forceinline__ void shared_query(ConstShared& D, const SharedRows& v
   }
 }
 
-// shared_query for key form sel_i / 7 and 2^(sel_i % 7) copies, through one switch
-// (a jump table on the wave-uniform index, not a chain of compares).
-__device__ __forceinline__ void shared_dispatch(int sel_i, ConstShared& D, const SharedRows& v,
-                                                uint64_t row0, uint64_t row,
-                                                unsigned long long* reg) {
+// shared_select for key form i / 7 and 2^(i % 7) copies, and shared_accumulate for
+// 2^rs copies, through switches (jump tables on the wave-uniform index).
+__device__ __forceinline__ uint32_t shared_select_dispatch(int i, ConstShared& D,
+                                                           const SharedRows& v, uint64_t row0,
+                                                           uint64_t row, uint32_t (&gid)[kSR],
+                                                           uint32_t& bad) {
 #define TGP_SQ_CASE(I) \
-  case I: shared_query<(I) / 7, (I) % 7>(D, v, row0, row, reg); break;
+  case I: return shared_select<(I) / 7, (I) % 7>(D, v, row0, row, gid, bad);
 #define TGP_SQ_KIND(K) \
   TGP_SQ_CASE(7 * (K)) TGP_SQ_CASE(7 * (K) + 1) TGP_SQ_CASE(7 * (K) + 2) TGP_SQ_CASE(7 * (K) + 3) \
   TGP_SQ_CASE(7 * (K) + 4) TGP_SQ_CASE(7 * (K) + 5) TGP_SQ_CASE(7 * (K) + 6)
-  switch (sel_i) {
+  switch (i) {
     TGP_SQ_KIND(0) TGP_SQ_KIND(1) TGP_SQ_KIND(2) TGP_SQ_KIND(3) TGP_SQ_KIND(4)
     default: break;
   }
 #undef TGP_SQ_KIND
 #undef TGP_SQ_CASE
+  bad = 0;
+  return 0;
+}
+
+__device__ __forceinline__ void shared_accumulate_dispatch(int rs, ConstShared& D,
+                                                           const SharedRows& v, uint32_t sel,
+                                                           const uint32_t (&gid)[kSR],
+                                                           unsigned long long* reg) {
+  switch (rs) {
+    case 0: shared_accumulate<0>(D, v, sel, gid, reg); break;
+    case 1: shared_accumulate<1>(D, v, sel, gid, reg); break;
+    case 2: shared_accumulate<2>(D, v, sel, gid, reg); break;
+    case 3: shared_accumulate<3>(D, v, sel, gid, reg); break;
+    case 4: shared_accumulate<4>(D, v, sel, gid, reg); break;
+    case 5: shared_accumulate<5>(D, v, sel, gid, reg); break;
+    case 6: shared_accumulate<6>(D, v, sel, gid, reg); break;
+    default: break;
+  }
 }
 
 // items: 4 words each (first absolute slice, slices, first entry, entries);
@@ -396,12 +431,24 @@ __global__ void __launch_bounds__(kBlock) k_batch_shared(SharedFamily F,
           sr.big[a] = __ballot(big_any(m)) != 0;
           sr.any_big = sr.any_big || sr.big[a];
         });
+        // a query whose date bounds and key parts equal the previous one's (same
+        // signature and copies) and that covers the slice, as the previous did,
+        // takes its selection and group ids as they are
+        uint32_t c_sel = 0, c_bad = 0, c_gid[kSR] = {0, 0, 0, 0};
+        uint32_t c_key = ~0u;
         for (uint32_t e = e0; e < e0 + ne; ++e) {
           const uint32_t q = en[2 * e], pack = en[2 * e + 1];
           ConstShared& D = qs[q];
           if (row0 >= D.row_end || row0 + kSliceRows <= D.row_begin) continue;
-          shared_dispatch(D.kind * 7 + (int)(pack >> 16), D, sr, row0, row,
-                             region + (pack & 0xffffu));
+          const int rs = (int)(pack >> 16);
+          const bool full = row0 >= D.row_begin && row0 + kSliceRows <= D.row_end;
+          const uint32_t key = full ? ((D.sig << 3) | (uint32_t)rs) : ~0u;
+          if (key == ~0u || key != c_key)
+            c_sel = shared_select_dispatch(D.kind * 7 + rs, D, sr, row0, row, c_gid, c_bad);
+          c_key = key;
+          if (c_bad) atomicAdd(D.out_oor, (unsigned long long)__popc(c_bad));
+          if (__ballot(c_sel != 0))
+            shared_accumulate_dispatch(rs, D, sr, c_sel, c_gid, region + (pack & 0xffffu));
         }
       }
       __syncthreads();
@@ -571,34 +618,62 @@ static int32_t build_shared_items(tgp_batch* b, hipStream_t s) {
   const uint64_t C = b->chunk < kSharedMaxSlices ? b->chunk : kSharedMaxSlices;
   const size_t nf = b->fam.size();
   std::vector<uint64_t> sb(nf), se(nf);
-  uint64_t kmax = 0;
+  // (chunk, member) pairs of the chunks each member overlaps, placed by counting in
+  // (queue, chunk, member) order: chunk k is in queue k % kQueues
+  uint64_t kend = 0;
   for (size_t m = 0; m < nf; ++m) {
     sb[m] = b->fam[m].row_begin / kSliceRows;
     se[m] = b->fam[m].row_end > b->fam[m].row_begin
                 ? (b->fam[m].row_end + kSliceRows - 1) / kSliceRows : sb[m];
-    if (se[m] > sb[m]) kmax = (se[m] + C - 1) / C > kmax ? (se[m] + C - 1) / C : kmax;
+    if (se[m] > sb[m]) kend = std::max(kend, (se[m] + C - 1) / C);
   }
-  std::vector<std::vector<uint32_t>> lists(kmax);
+  const uint64_t per_q = (kend + kQueues - 1) / kQueues;
+  auto pos_of = [&](uint64_t k) { return (k % kQueues) * per_q + k / kQueues; };
+  std::vector<uint32_t> at(kQueues * per_q + 1, 0);
   for (size_t m = 0; m < nf; ++m)
     if (se[m] > sb[m])
-      for (uint64_t k = sb[m] / C; k * C < se[m]; ++k) lists[k].push_back((uint32_t)m);
+      for (uint64_t k = sb[m] / C; k * C < se[m]; ++k) at[pos_of(k) + 1] += 1;
+  for (size_t i = 1; i < at.size(); ++i) at[i] += at[i - 1];
+  std::vector<std::pair<uint64_t, uint32_t>> pairs(at.back());
+  for (size_t m = 0; m < nf; ++m)  // members in order: each chunk's list stays ascending
+    if (se[m] > sb[m])
+      for (uint64_t k = sb[m] / C; k * C < se[m]; ++k) pairs[at[pos_of(k)]++] = {k, (uint32_t)m};
   std::vector<uint32_t> items, entries;
   uint64_t total = 0;
-  auto rcap = [](int32_t words) {  // copies by group count: lanes of one atomic spread
-    const int32_t groups = words / 5;  // over groups x copies words
+  // copies per word: lanes of one atomic spread over groups x copies words.  Queries
+  // of at most 8 groups keep at least 8 copies (fewer made their atomics conflict);
+  // a chunk whose queries then do not fit one region becomes several items over the
+  // same slices (its later items read the slices from the L2)
+  auto rcap = [](int32_t words) {
+    const int32_t groups = words / 5;
     return groups <= 3 ? 6 : groups <= 8 ? 5 : groups <= 32 ? 2 : 1;
   };
-  for (uint32_t qi = 0; qi < kQueues; ++qi) {
-    b->squeue_off[qi] = total;
-    for (uint64_t k = qi; k < kmax; k += kQueues) {
-      const std::vector<uint32_t>& L = lists[k];
-      if (L.empty()) continue;
-      uint64_t s0 = (k + 1) * C, s1 = k * C;
-      for (uint32_t m : L) {
-        s0 = std::min(s0, std::max(k * C, sb[m]));
-        s1 = std::max(s1, std::min((k + 1) * C, se[m]));
-      }
-      std::vector<int32_t> r(L.size());
+  auto rmin = [](int32_t words) { return words / 5 <= 8 ? 3 : 0; };
+  size_t p = 0;
+  uint32_t cur_queue = 0;
+  b->squeue_off[0] = 0;
+  // the previous chunk's member list and its items' entry ranges: a chunk with the
+  // same members (the chunks inside the same long queries) reuses them
+  std::vector<uint32_t> prevL;
+  std::vector<std::pair<uint32_t, uint32_t>> prev_ranges;
+  std::vector<uint32_t> L;
+  std::vector<int32_t> r;
+  while (p < pairs.size()) {
+    const uint64_t k = pairs[p].first;
+    size_t pe = p;
+    while (pe < pairs.size() && pairs[pe].first == k) ++pe;
+    const uint32_t q = (uint32_t)(k % kQueues);
+    while (cur_queue < q) b->squeue_off[++cur_queue] = total;
+    L.clear();
+    for (size_t j = p; j < pe; ++j) L.push_back(pairs[j].second);
+    p = pe;
+    uint64_t s0 = (k + 1) * C, s1 = k * C;
+    for (uint32_t m : L) {
+      s0 = std::min(s0, std::max(k * C, sb[m]));
+      s1 = std::max(s1, std::min((k + 1) * C, se[m]));
+    }
+    if (L != prevL) {
+      r.assign(L.size(), 0);
       uint64_t need = 0;
       for (size_t j = 0; j < L.size(); ++j) {
         const int32_t w = b->fam[L[j]].words;
@@ -606,17 +681,21 @@ static int32_t build_shared_items(tgp_batch* b, hipStream_t s) {
         while (r[j] > 0 && ((uint64_t)w << r[j]) > (uint64_t)kSharedWords) --r[j];
         need += (uint64_t)w << r[j];
       }
-      while (need > (uint64_t)kSharedWords) {  // fewer copies for the largest parts
-        size_t jm = L.size();
+      // fewer copies for the largest parts: every part of the largest size drops one
+      // level per pass, down to its minimum
+      while (need > (uint64_t)kSharedWords) {
+        uint64_t big = 0;
         for (size_t j = 0; j < L.size(); ++j)
-          if (r[j] > 0 && (jm == L.size() || ((uint64_t)b->fam[L[j]].words << r[j]) >
-                                                  ((uint64_t)b->fam[L[jm]].words << r[jm])))
-            jm = j;
-        if (jm == L.size()) break;
-        need -= (uint64_t)b->fam[L[jm]].words << (r[jm] - 1);
-        --r[jm];
+          if (r[j] > rmin(b->fam[L[j]].words))
+            big = std::max(big, (uint64_t)b->fam[L[j]].words << r[j]);
+        if (big == 0) break;
+        for (size_t j = 0; j < L.size() && need > (uint64_t)kSharedWords; ++j)
+          if (r[j] > rmin(b->fam[L[j]].words) && ((uint64_t)b->fam[L[j]].words << r[j]) == big) {
+            need -= (uint64_t)b->fam[L[j]].words << (r[j] - 1);
+            --r[j];
+          }
       }
-      // greedy items over the same slices while the parts do not fit one region
+      prev_ranges.clear();
       size_t j = 0;
       while (j < L.size()) {
         const uint32_t e0 = (uint32_t)(entries.size() / 2);
@@ -629,14 +708,19 @@ static int32_t build_shared_items(tgp_batch* b, hipStream_t s) {
           off += sz;
           ++j;
         }
-        items.push_back((uint32_t)s0);
-        items.push_back((uint32_t)(s1 - s0));
-        items.push_back(e0);
-        items.push_back((uint32_t)(entries.size() / 2) - e0);
-        ++total;
+        prev_ranges.push_back({e0, (uint32_t)(entries.size() / 2) - e0});
       }
+      prevL = L;
+    }
+    for (const auto& er : prev_ranges) {
+      items.push_back((uint32_t)s0);
+      items.push_back((uint32_t)(s1 - s0));
+      items.push_back(er.first);
+      items.push_back(er.second);
+      ++total;
     }
   }
+  while (cur_queue < kQueues - 1) b->squeue_off[++cur_queue] = total;
   b->squeue_off[kQueues] = total;
   if (items.empty()) items.assign(4, 0);
   if (entries.empty()) entries.assign(2, 0);
@@ -866,6 +950,17 @@ int32_t tgp_batch_set_queries(tgp_batch* b, const tgp_agg_query* qs, uint32_t n,
   b->fam_plan.clear();
   b->family = SharedFamily{};
   int32_t ref = -1;
+  // signature ids: members with equal date bounds, key form and key parts
+  std::map<std::vector<int64_t>, uint32_t> sigs;
+  auto sig_of = [&](const SharedQuery& m) {
+    const std::vector<int64_t> k{m.plo, m.phi, m.kind, m.nparts, m.gbase[0], m.gbase[1],
+                                 m.gcard[0], m.gcard[1], m.stride1, m.src[0], m.src[1]};
+    auto it = sigs.find(k);
+    if (it != sigs.end()) return it->second;
+    const uint32_t id = (uint32_t)sigs.size();
+    sigs.emplace(k, id);
+    return id;
+  };
   for (uint32_t q = 0; q < b->plan.size(); ++q) {
     const BatchQuery& bq = b->plan[q];
     if (bq.shape > 4 || bq.naggs != kShapes[0].naggs) continue;
@@ -908,6 +1003,7 @@ int32_t tgp_batch_set_queries(tgp_batch* b, const tgp_agg_query* qs, uint32_t n,
     m.stride1 = S.nparts == 2 ? (int32_t)A.gstride[1] : 0;
     m.words = (int32_t)A.words;
     m.kind = bq.shape;
+    m.sig = sig_of(m);
     m.out_count = A.out_count;
     m.out_sums = A.out_sums;
     m.out_oor = A.out_oor;

@@ -312,6 +312,8 @@ struct ProbeAggDev {
   int64_t key_base;
   uint64_t ngroups;
   uint64_t b, e;
+  uint32_t row_limit;  // build rows at or above it do not match (~0u: none excluded)
+  uint32_t _pad;
   unsigned long long* out_count;
   unsigned long long* out_sums;
   unsigned long long* out_oor;
@@ -421,13 +423,15 @@ __global__ void __launch_bounds__(256) k_probe_agg(JoinTableDev T, ProbeAggDev Q
       if (Q.group_by != TGP_JOIN_GROUP_PROBE_COL) g[u] = 0;
     }
     if (Q.group_by == TGP_JOIN_GROUP_BUILD_PAYLOAD) {
-      probe_keys(T, k, ok, [&](auto I, uint32_t, uint32_t pay) {
+      probe_keys(T, k, ok, [&](auto I, uint32_t row, uint32_t pay) {
+        if (row >= Q.row_limit) return;
         const int64_t gg = (int64_t)(int32_t)pay - Q.key_base;
         if (gg < 0 || (uint64_t)gg >= Q.ngroups) { ++oor; return; }
         add((uint64_t)gg, v[decltype(I)::value]);
       });
     } else {
-      probe_keys(T, k, ok, [&](auto I, uint32_t, uint32_t) {
+      probe_keys(T, k, ok, [&](auto I, uint32_t row, uint32_t) {
+        if (row >= Q.row_limit) return;
         constexpr int u = decltype(I)::value;
         const int64_t gg = Q.group_by == TGP_JOIN_GROUP_PROBE_COL ? g[u] - Q.key_base : 0;
         if (gg < 0 || (uint64_t)gg >= Q.ngroups) { ++oor; return; }
@@ -700,7 +704,9 @@ __global__ void __launch_bounds__(256) k_part_scatter(JoinTableDev T, ProbeAggDe
         v[h * U + u] = vv[u];
         g[u] -= Q.key_base;
       });
-      probe_keys(T, k, ok, [&](auto I, uint32_t, uint32_t) { ++m[decltype(I)::value]; });
+      probe_keys(T, k, ok, [&](auto I, uint32_t row, uint32_t) {
+        if (row < Q.row_limit) ++m[decltype(I)::value];
+      });
       static_for<U>([&](auto I) {
         constexpr int u = decltype(I)::value;
         constexpr int r = h * U + u;
@@ -1341,6 +1347,8 @@ int32_t tgp_hash_probe_agg(tgp_ctx* ctx, const tgp_join_agg_query* q, const tgp_
   Q.ngroups = q->ngroups;
   Q.b = q->row_begin;
   Q.e = q->row_end;
+  Q.row_limit = q->build_rows ? q->build_rows : ~0u;
+  Q._pad = 0;
   Q.out_count = reinterpret_cast<unsigned long long*>(out->d_count);
   Q.out_sums = reinterpret_cast<unsigned long long*>(out->d_sums);
   Q.out_oor = reinterpret_cast<unsigned long long*>(out->d_n_out_of_range);

@@ -9,9 +9,8 @@
 //     (tgp_batch_*), so requests that arrive while others run join the next
 //     batch and batches overlap on the device.  Requests whose row range is at
 //     least `heavy_rows` (full-table scans) are batched on a slot of their own
-//     (slot 0, with 128-slice chunks in the XCD-affine table order: requests
-//     over the same rows then read them through one L2), so they do not hold
-//     back the small requests.
+//     (slot 0, in the shared scan with 128-slice chunks: each slice is read
+//     once for all of them), so they do not hold back the small requests.
 //   - `join_slots` join workers (a thread and a stream each) admit a join when
 //     the hash table it will build fits the HBM budget (the context's tier
 //     budget, optionally capped by `hash_budget_bytes`), build it and run the
@@ -80,10 +79,28 @@ struct AggSlot {
   uint64_t d_cap = 0;
 };
 
+// A resident hash table over a prefix of one build side (key and payload
+// columns), shared by every join on that build side: a join over the first nb
+// build rows probes it with the rows from nb on excluded (build_rows).  A join
+// that needs a longer prefix builds a replacement; the outgrown table is kept
+// until no join in flight probes it.
+struct SharedTable {
+  const void* keys = nullptr;
+  const void* pay = nullptr;
+  int32_t kdt = 0, pdt = 0;
+  tgp_hash_table* table = nullptr;  // ready table (nullptr until the first build ends)
+  uint64_t rows = 0;                // build rows it holds
+  uint64_t building = 0;            // rows of the build in progress (0: none)
+  uint32_t refs = 0;                // joins in flight probing `table`
+};
+
 struct JoinSlot {
   hipStream_t stream = nullptr;
   hipEvent_t done_ev = nullptr;
   Request* req = nullptr;
+  SharedTable* shared = nullptr;    // the shared table this join probes
+  tgp_hash_table* shared_table = nullptr;  // ... the instance it holds a reference to
+  SharedTable* builds = nullptr;    // the shared table this slot is (re)building
   tgp_hash_table* table = nullptr;
   uint64_t table_bytes = 0;
   void* pool = nullptr;       // the slot's table memory, reused join after join
@@ -116,6 +133,9 @@ struct tgp_server {
   uint64_t pool_held = 0;    // bytes of all join slots' table pools
   bool heavy_slot = false;  // slot 0 serves the heavy queue
   tgp_serve_stats stats{};
+  std::vector<std::unique_ptr<SharedTable>> shared;  // share_build_tables: per build side
+  // outgrown shared tables still probed by joins in flight: (table, refs)
+  std::vector<std::pair<tgp_hash_table*, uint32_t>> retired;
 };
 
 namespace {
@@ -389,17 +409,45 @@ int32_t harvest_join(tgp_server* s, JoinSlot& sl) {
     rc = TGP_ERR_HIP;
   }
   // the table lives in the slot's pool: destroying it frees no device memory
-  const int32_t rd = tgp_hash_table_destroy(s->ctx, sl.table);
-  if (rc == TGP_OK) rc = rd;
-  std::lock_guard<std::mutex> lk(s->mu);
-  s->hash_in_use -= sl.table_bytes;
-  sl.table = nullptr;
-  sl.table_bytes = 0;
-  sl.req = nullptr;
-  sl.busy = false;
-  finish(s, r, rc);
-  s->done_cv.notify_all();
+  if (sl.table) {
+    const int32_t rd = tgp_hash_table_destroy(s->ctx, sl.table);
+    if (rc == TGP_OK) rc = rd;
+  }
+  tgp_hash_table* free_now = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->hash_in_use -= sl.table_bytes;
+    if (sl.shared) {
+      if (sl.shared_table == sl.shared->table) {
+        sl.shared->refs -= 1;
+      } else {  // the table was outgrown while this join probed it
+        for (size_t i = 0; i < s->retired.size(); ++i)
+          if (s->retired[i].first == sl.shared_table && --s->retired[i].second == 0) {
+            free_now = sl.shared_table;
+            s->retired.erase(s->retired.begin() + (long)i);
+            break;
+          }
+      }
+    }
+    sl.shared = nullptr;
+    sl.shared_table = nullptr;
+    sl.table = nullptr;
+    sl.table_bytes = 0;
+    sl.req = nullptr;
+    sl.busy = false;
+    finish(s, r, rc);
+    s->done_cv.notify_all();
+  }
+  if (free_now) (void)tgp_hash_table_destroy(s->ctx, free_now);
   return rc;
+}
+
+SharedTable* find_shared(tgp_server* s, const tgp_serve_join& J) {  // caller holds s->mu
+  for (auto& t : s->shared)
+    if (t->keys == J.build_keys.data && t->pay == J.build_payload.data &&
+        t->kdt == J.build_keys.dtype && (!J.build_payload.data || t->pdt == J.build_payload.dtype))
+      return t.get();
+  return nullptr;
 }
 
 void drop_pool(tgp_server* s, JoinSlot& sl) {  // caller holds s->mu
@@ -444,14 +492,48 @@ int32_t start_join(tgp_server* s, JoinSlot& sl, Request* r, uint64_t need) {
     sl.d_res = static_cast<uint64_t*>(p);
     sl.d_cap = r->words;
   }
-  if (sl.pool_bytes < need) TGP_TRY(grow_pool(s, sl, need));
-  tgp_hash_table* t = nullptr;
   const tgp_column* pay = J.build_payload.data ? &J.build_payload : nullptr;
-  TGP_TRY(tgp_hash_build_in(s->ctx, &J.build_keys, pay, sl.pool, sl.pool_bytes, &t, sl.stream));
-  sl.table = t;
-  sl.table_bytes = need;
   tgp_join_agg_query q = J.probe;
-  q.table = t;
+  if (sl.builds) {
+    // (re)build the build side's shared table over this join's prefix; the tier
+    // accounts it from now on, so the reservation taken at admission is released
+    tgp_hash_table* t = nullptr;
+    const int32_t rc = tgp_hash_build(s->ctx, &J.build_keys, pay, &t, sl.stream);
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->hash_in_use -= sl.table_bytes;
+    sl.table_bytes = 0;
+    SharedTable* st = sl.builds;
+    sl.builds = nullptr;
+    st->building = 0;
+    if (rc != TGP_OK) return rc;
+    s->stats.table_builds += 1;
+    if (st->table) {
+      if (st->refs) s->retired.push_back({st->table, st->refs});
+      else (void)tgp_hash_table_destroy(s->ctx, st->table);
+    }
+    st->table = t;
+    st->rows = J.build_keys.len;
+    st->refs = 0;
+    sl.shared = st;
+  }
+  if (sl.shared) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    sl.shared->refs += 1;
+    sl.shared_table = sl.shared->table;
+    q.table = sl.shared_table;
+    q.build_rows = (uint32_t)J.build_keys.len;
+  } else {
+    if (sl.pool_bytes < need) TGP_TRY(grow_pool(s, sl, need));
+    tgp_hash_table* t = nullptr;
+    TGP_TRY(tgp_hash_build_in(s->ctx, &J.build_keys, pay, sl.pool, sl.pool_bytes, &t, sl.stream));
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->stats.table_builds += 1;
+    }
+    sl.table = t;
+    sl.table_bytes = need;
+    q.table = t;
+  }
   q.ngroups = r->ngroups;
   tgp_group_result out{sl.d_res, sl.d_res + r->ngroups, sl.d_res + r->ngroups * 3};
   TGP_TRY(tgp_hash_probe_agg(s->ctx, &q, &out, sl.stream));
@@ -474,7 +556,67 @@ void join_worker(tgp_server* s, size_t slot) {
     }
     Request* r = nullptr;
     uint64_t need = 0;
-    if (!sl.busy) {
+    if (!sl.busy && s->cfg.share_build_tables) {
+      // a join on a build side whose resident table holds its prefix probes it now;
+      // one that needs a (longer) table builds it when the budget allows, in arrival
+      // order; joins waiting for a build in progress let later ones pass
+      std::lock_guard<std::mutex> lk(s->mu);
+      for (auto it = s->join_q.begin(); it != s->join_q.end();) {
+        Request* q = *it;
+        const tgp_serve_join& J = q->req.join;
+        SharedTable* st = find_shared(s, J);
+        if (st && st->table && st->rows >= J.build_keys.len) {
+          s->join_q.erase(it);
+          s->stats.admitted_joins += 1;
+          sl.busy = true;
+          sl.shared = st;
+          r = q;
+          break;
+        }
+        if (st && st->building) {
+          ++it;
+          continue;
+        }
+        need = tgp_hash_table_bytes_for(J.build_keys.len);
+        uint64_t used = 0, budget = 0, peak = 0;
+        (void)tgp_hbm_usage(s->ctx, &used, &budget, &peak);
+        const uint64_t cap = s->cfg.hash_budget_bytes ? s->cfg.hash_budget_bytes : budget;
+        if (need > cap) {
+          it = s->join_q.erase(it);
+          s->stats.rejected += 1;
+          set_error("serve: join needs %llu bytes of hash table, budget %llu",
+                    (unsigned long long)need, (unsigned long long)cap);
+          finish(s, q, TGP_ERR_BUDGET);
+          s->done_cv.notify_all();
+          progressed = true;
+          continue;
+        }
+        if (s->hash_in_use + need > hash_budget(s)) {
+          if (!q->deferred) {
+            q->deferred = true;
+            s->stats.deferred += 1;
+          }
+          break;
+        }
+        if (!st) {
+          s->shared.push_back(std::make_unique<SharedTable>());
+          st = s->shared.back().get();
+          st->keys = J.build_keys.data;
+          st->pay = J.build_payload.data;
+          st->kdt = J.build_keys.dtype;
+          st->pdt = J.build_payload.dtype;
+        }
+        st->building = J.build_keys.len;
+        s->join_q.erase(it);
+        s->hash_in_use += need;
+        s->stats.admitted_joins += 1;
+        sl.busy = true;
+        sl.builds = st;
+        sl.table_bytes = need;  // reserved until the build ends
+        r = q;
+        break;
+      }
+    } else if (!sl.busy) {
       std::lock_guard<std::mutex> lk(s->mu);
       while (!s->join_q.empty()) {
         Request* front = s->join_q.front();
@@ -506,6 +648,7 @@ void join_worker(tgp_server* s, size_t slot) {
         s->hash_in_use += need;
         s->stats.admitted_joins += 1;
         sl.busy = true;  // claimed; start_join fills the slot
+        sl.table_bytes = need;
         r = front;
         break;
       }
@@ -517,7 +660,17 @@ void join_worker(tgp_server* s, size_t slot) {
         std::lock_guard<std::mutex> lk(s->mu);
         bool others = false;
         for (JoinSlot& o : s->join_slots) others |= o.busy && &o != &sl;
-        s->hash_in_use -= need;
+        if (sl.builds) sl.builds->building = 0;  // failed before its build started
+        if (sl.shared && sl.shared_table) {      // failed after taking a reference
+          if (sl.shared_table == sl.shared->table) sl.shared->refs -= 1;
+          else
+            for (auto& rt : s->retired)
+              if (rt.first == sl.shared_table) rt.second -= 1;  // freed with the server
+        }
+        sl.builds = nullptr;
+        sl.shared = nullptr;
+        sl.shared_table = nullptr;
+        s->hash_in_use -= sl.table_bytes;
         sl.table = nullptr;
         sl.table_bytes = 0;
         sl.req = nullptr;
@@ -554,6 +707,11 @@ void release(tgp_server* s) {
     if (sl.h_arena) (void)hipHostFree(sl.h_arena);
     if (sl.d_res) (void)hbm_free(s->ctx, sl.d_res);
   }
+  for (auto& st : s->shared)
+    if (st->table) (void)tgp_hash_table_destroy(s->ctx, st->table);
+  s->shared.clear();
+  for (auto& rt : s->retired) (void)tgp_hash_table_destroy(s->ctx, rt.first);
+  s->retired.clear();
   for (JoinSlot& sl : s->join_slots) {
     if (sl.table) (void)tgp_hash_table_destroy(s->ctx, sl.table);
     drop_pool(s, sl);
@@ -590,10 +748,11 @@ int32_t tgp_serve_create(tgp_ctx* ctx, const tgp_serve_config* cfg, tgp_server**
     AggSlot& sl = s->agg_slots[si];
     if (rc == TGP_OK) rc = tgp_batch_create(ctx, cfg->max_batch_queries, &sl.batch);
     if (rc == TGP_OK) {
-      // the heavy slot's batches are many scans over the same large ranges
+      // the heavy slot's batches are many scans over the same large ranges: the
+      // shared scan (order 3) reads each slice once for all of them, in 128-slice chunks
       const bool heavy = s->heavy_slot && si == 0;
       const tgp_batch_tuning t{cfg->wg_per_cu, heavy ? 128u : cfg->chunk_slices,
-                               heavy ? 2 : cfg->order, heavy ? 1 : cfg->cached_loads, 0, 1};
+                               heavy ? 3 : cfg->order, heavy ? 1 : cfg->cached_loads, 0, 1};
       rc = tgp_batch_set_tuning(sl.batch, &t);
     }
     if (rc == TGP_OK && (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||

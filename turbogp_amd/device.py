@@ -768,9 +768,13 @@ class HashTable:
         return bm
 
     def probe_agg(self, keys, value, group_by, ngroups=1, key_base=0, group_col=None,
-                  row_begin=0, row_end=None, result=None, stream=None, fetch=True):
+                  row_begin=0, row_end=None, result=None, stream=None, fetch=True,
+                  build_rows=0):
+        """Fused probe -> GROUP BY -> SUM(value), COUNT(*); build_rows n > 0 joins
+        with the first n build rows only (a table built over more of the build side)."""
         q = A.JoinAggQuery()
         q.table = self.handle
+        q.build_rows = build_rows
         q.probe_keys = keys.desc()
         q.value = value.desc()
         if group_col is not None:

@@ -64,12 +64,12 @@ def parse_words(words, ngroups, naggs):
 
 class Server:
     def __init__(self, ctx, max_batch_queries=1024, batch_slots=2, join_slots=2, heavy_rows=0,
-                 hash_budget_bytes=0, wg_per_cu=0, chunk_slices=32, order=2, cached_loads=True,
-                 per_query_launch=False, batch_wait_us=250):
+                 hash_budget_bytes=0, wg_per_cu=0, chunk_slices=64, order=3, cached_loads=True,
+                 per_query_launch=False, batch_wait_us=250, share_build_tables=True):
         self.ctx = ctx
         cfg = A.ServeConfig(max_batch_queries, batch_slots, join_slots, batch_wait_us, heavy_rows,
                             hash_budget_bytes, wg_per_cu, chunk_slices, order, int(cached_loads),
-                            int(per_query_launch), 0)
+                            int(per_query_launch), int(share_build_tables))
         h = C.c_void_p()
         A.check(A.lib().tgp_serve_create(ctx.handle, C.byref(cfg), C.byref(h)), "tgp_serve_create")
         self.handle = h.value
