@@ -194,3 +194,59 @@ def test_shared_scan_with_non_members_and_big_values(ctx, table):
     finally:
         for c in dev2.values():
             c.free()
+
+
+def test_shared_scan_signature_groups(ctx, table):
+    """Queries with equal date bounds and key parts (one signature) share partial
+    aggregates of the chunks they cover whole, each taking the running-sum
+    difference over its own chunks and scanning its edge chunks row by row.  Many
+    queries per signature over the whole table, random ranges, chunk-aligned and
+    one-row ranges, the table's last rows, empty ranges, out-of-range keys and a
+    table copy with prices near 2^50 (int128 partials): every result equals its
+    single-query launch, for several chunk sizes and graph replays."""
+    host, dev = table
+    n2 = 400_000
+    big = {k: v[:n2].copy() for k, v in host.items()}
+    big["price"] = big["price"] + (1 << 50)
+    dev2 = {k: DeviceColumn.from_numpy(ctx, v) for k, v in big.items()}
+    rng = np.random.default_rng(31)
+
+    def ranges(n, k):
+        out = [(0, n), (0, n), (0, n), (n - 300, n), (5, 5), (1000, 1001), (0, 256 * 64),
+               (256 * 32, 256 * 96), (256 * 7 + 3, n - 1)]
+        for _ in range(k):
+            L = int(np.exp(rng.uniform(0, np.log(n))))
+            bb = int(rng.integers(0, n - L + 1))
+            out.append((bb, bb + L))
+        return out
+
+    try:
+        qs = []
+        for c, g in ((2000, "ab"), (1500, "month"), (2525, "a")):
+            qs += W.batch_queries(dev, [(bb, e, c, g) for bb, e in ranges(N, 25)])
+        # out-of-range keys: flaga has three values, the key part takes two
+        names, preds, parts, aggs = W.q1_spec(date_cutoff=2200, group="ab")
+        narrow = [parts[0][:3] + (2,), parts[1]]
+        for bb, e in ranges(N, 8):
+            qs.append(make_agg_query([dev[x] for x in names], preds, narrow, aggs, bb, e))
+        qs += W.batch_queries(dev2, [(bb, e, 2000, "ab") for bb, e in ranges(n2, 8)])
+        b = Batch(ctx, len(qs))
+        try:
+            b.set_queries(qs)
+            assert b.info()["fallback"] == 0
+            got = []
+            for chunk in (1, 3, 32, 64, 4096):
+                b.set_tuning(0, use_graph=True, chunk_slices=chunk, order=3, cached_loads=True)
+                b.run(A.BATCH_PERSISTENT)
+                got.append(b.results())
+                b.run(A.BATCH_PERSISTENT)  # graph replay: the partials start at zero again
+                got.append(b.results())
+        finally:
+            b.close()
+        for i, q in enumerate(qs):
+            want = ctx.group_agg(q)
+            assert all(same(g[i], want) for g in got), i
+        assert any(r.n_out_of_range for r in got[0])
+    finally:
+        for c in dev2.values():
+            c.free()

@@ -210,6 +210,9 @@ struct SharedQuery {
   int32_t kind;             // key form: the Q1 shape index (0 flagA x flagB, 1 flagA,
                             // 2 flagB, 3 dt / divisor, 4 no key)
   uint32_t sig;             // equal for members with equal date bounds and key parts
+  uint32_t pstride;         // 0: a member; else a signature group's partials query:
+                            // out_count is its partial rows, pstride words each
+  uint32_t pk0;             // the group's first chunk (partials query)
   uint32_t _pad;
   unsigned long long* out_count;
   unsigned long long* out_sums;
@@ -223,7 +226,7 @@ struct SharedFamily {
   uint32_t magic;     // dt / divisor (Granlund-Montgomery), when a member groups by it
   int32_t shift;
   int32_t divisor;    // 0: no member divides
-  int32_t _pad;
+  uint32_t chunk;     // slices per chunk (items start inside chunk first slice / chunk)
 };
 
 // Row values of one slice shared by the queries of a chunk (this lane's 4 rows).
@@ -235,6 +238,35 @@ struct SharedRows {
   bool big[4];                   // wave-uniform: the slice has |value| >= 2^43
   bool any_big;
 };
+
+// Adds (lo, hi) to an int128 whose words are `st` words apart (64-bit atomics;
+// the carries out of the low word are counted exactly, as in atomic_add128).
+__device__ __forceinline__ void atomic_add128_split(unsigned long long* p, uint32_t st,
+                                                    uint64_t lo, int64_t hi) {
+  if (lo == 0 && hi == 0) return;
+  const uint64_t old = atomicAdd(p, (unsigned long long)lo);
+  const int64_t h = hi + (int64_t)((old + lo) < old);
+  if (h != 0) atomicAdd(p + st, (unsigned long long)h);
+}
+
+// Where one entry of a chunk adds its results.  A member adds into its own result
+// (count[g], sums[(g * 4 + a) * 2], oor, consecutive words); a signature group's
+// partials query adds into the chunk's column of the group's partial rows: row g
+// the counts, rows G + 2 (g * 4 + a) and the next the low and high sum words, row
+// 9 G the out-of-range count, position 1 + (chunk - first chunk) of each row.
+struct EntryOut {
+  unsigned long long* count;
+  unsigned long long* sums;
+  unsigned long long* oor;
+  uint32_t st;  // word distance of consecutive result words
+};
+
+__device__ __forceinline__ EntryOut entry_out(ConstShared& D, uint64_t kchunk) {
+  if (!D.pstride) return EntryOut{D.out_count, D.out_sums, D.out_oor, 1u};
+  const uint32_t st = D.pstride, G = (uint32_t)D.words / 5;
+  unsigned long long* base = D.out_count + 1 + (kchunk - D.pk0);
+  return EntryOut{base, base + (uint64_t)G * st, base + (uint64_t)9 * G * st, st};
+}
 
 // One query of the chunk on one slice, first half: the rows it selects (range and
 // date test), their group ids of key form KIND (premultiplied by the word stride of
@@ -287,7 +319,7 @@ __device__ __forceinline__ uint32_t shared_select(ConstShared& D, const SharedRo
 // compile-time offsets (lane l on copy l mod 2^RS); a slice holding a big value
 // adds that aggregate straight into the int128 result.
 template <int RS>
-__device__ __forceinline__ void shared_accumulate(ConstShared& D, const SharedRows& v,
+__device__ __forceinline__ void shared_accumulate(const EntryOut& O, const SharedRows& v,
                                                   uint32_t sel, const uint32_t (&gid)[kSR],
                                                   unsigned long long* reg) {
   constexpr int W1 = 5;
@@ -315,7 +347,8 @@ __device__ __forceinline__ void shared_accumulate(ConstShared& D, const SharedRo
       for (int r = 0; r < kSR; ++r)
         if ((sel >> r) & 1) {
           const uint64_t g = gid[r] / (uint32_t)(W1 << RS);
-          atomic_add128_i64(&D.out_sums[(g * 4 + a) * 2], (int64_t)v.val[a][r]);
+          const int64_t x = (int64_t)v.val[a][r];
+          atomic_add128_split(O.sums + (g * 4 + a) * 2 * O.st, O.st, (uint64_t)x, x < 0 ? -1 : 0);
         }
     } else {
 #pragma unroll
@@ -347,18 +380,18 @@ __device__ __forceinline__ uint32_t shared_select_dispatch(int i, ConstShared& D
   return 0;
 }
 
-__device__ __forceinline__ void shared_accumulate_dispatch(int rs, ConstShared& D,
+__device__ __forceinline__ void shared_accumulate_dispatch(int rs, const EntryOut& O,
                                                            const SharedRows& v, uint32_t sel,
                                                            const uint32_t (&gid)[kSR],
                                                            unsigned long long* reg) {
   switch (rs) {
-    case 0: shared_accumulate<0>(D, v, sel, gid, reg); break;
-    case 1: shared_accumulate<1>(D, v, sel, gid, reg); break;
-    case 2: shared_accumulate<2>(D, v, sel, gid, reg); break;
-    case 3: shared_accumulate<3>(D, v, sel, gid, reg); break;
-    case 4: shared_accumulate<4>(D, v, sel, gid, reg); break;
-    case 5: shared_accumulate<5>(D, v, sel, gid, reg); break;
-    case 6: shared_accumulate<6>(D, v, sel, gid, reg); break;
+    case 0: shared_accumulate<0>(O, v, sel, gid, reg); break;
+    case 1: shared_accumulate<1>(O, v, sel, gid, reg); break;
+    case 2: shared_accumulate<2>(O, v, sel, gid, reg); break;
+    case 3: shared_accumulate<3>(O, v, sel, gid, reg); break;
+    case 4: shared_accumulate<4>(O, v, sel, gid, reg); break;
+    case 5: shared_accumulate<5>(O, v, sel, gid, reg); break;
+    case 6: shared_accumulate<6>(O, v, sel, gid, reg); break;
     default: break;
   }
 }
@@ -397,6 +430,7 @@ __global__ void __launch_bounds__(kBlock) k_batch_shared(SharedFamily F,
       const uint64_t i = base + k;
       const uint64_t sl0 = it[4 * i];
       const uint32_t nsl = it[4 * i + 1], e0 = it[4 * i + 2], ne = it[4 * i + 3];
+      const uint64_t kchunk = sl0 / F.chunk;
       for (uint32_t sj = wave; sj < nsl; sj += kBlock / kWave) {
         const uint64_t row0 = (sl0 + sj) * kSliceRows;
         const uint64_t row = row0 + lane * kSR;
@@ -446,9 +480,10 @@ __global__ void __launch_bounds__(kBlock) k_batch_shared(SharedFamily F,
           if (key == ~0u || key != c_key)
             c_sel = shared_select_dispatch(D.kind * 7 + rs, D, sr, row0, row, c_gid, c_bad);
           c_key = key;
-          if (c_bad) atomicAdd(D.out_oor, (unsigned long long)__popc(c_bad));
+          const EntryOut O = entry_out(D, kchunk);
+          if (c_bad) atomicAdd(O.oor, (unsigned long long)__popc(c_bad));
           if (__ballot(c_sel != 0))
-            shared_accumulate_dispatch(rs, D, sr, c_sel, c_gid, region + (pack & 0xffffu));
+            shared_accumulate_dispatch(rs, O, sr, c_sel, c_gid, region + (pack & 0xffffu));
         }
       }
       __syncthreads();
@@ -458,6 +493,7 @@ __global__ void __launch_bounds__(kBlock) k_batch_shared(SharedFamily F,
         ConstShared& D = qs[q];
         const uint32_t rs = pack >> 16, R = 1u << rs;
         unsigned long long* reg = region + (pack & 0xffffu);
+        const EntryOut O = entry_out(D, kchunk);
         for (uint32_t w = threadIdx.x; w < (uint32_t)D.words; w += kBlock) {
           unsigned long long x = 0;
           for (uint32_t c = 0; c < R; ++c) {
@@ -465,11 +501,108 @@ __global__ void __launch_bounds__(kBlock) k_batch_shared(SharedFamily F,
             reg[(w << rs) + c] = 0;
           }
           if (!x) continue;
-          const bool is_count = w % W1 == 0;
-          add_result_word(D.out_count, D.out_sums, w, S.naggs, x,
-                          (!is_count && (int64_t)x < 0) ? -1 : 0);
+          const uint32_t g = w / W1, k = w % W1;
+          if (k == 0)
+            atomicAdd(O.count + (uint64_t)g * O.st, x);
+          else
+            atomic_add128_split(O.sums + ((uint64_t)g * S.naggs + (k - 1)) * 2 * O.st, O.st, x,
+                                (int64_t)x < 0 ? -1 : 0);
         }
       }
+    }
+  }
+}
+
+// Signature groups: members of the shared scan with equal date bounds and key parts
+// share one partials query over the chunks any of them covers whole.  Its partial
+// rows hold, per result word, position 0 = 0 and position 1 + c = chunk c's part.
+struct PartialGroup {
+  unsigned long long* rows;
+  uint32_t nk1;     // positions per row: chunks + 1
+  uint32_t groups;  // G: rows G + 2 G naggs + 1
+};
+
+// Turns every partial row into its running sums: position p then holds the sum of
+// chunks [0, p).  One workgroup per (group, result word); sum words are int128
+// (row pairs), count and out-of-range words 64-bit.
+__global__ void __launch_bounds__(256) k_partials_scan(const PartialGroup* __restrict__ pg,
+                                                       uint32_t naggs) {
+  const PartialGroup P = pg[blockIdx.x];
+  const uint32_t G = P.groups, units = G + G * naggs + 1, u = blockIdx.y;
+  if (u >= units) return;
+  const bool wide = u >= G && u < G + G * naggs;
+  const uint32_t row = u < G ? u : wide ? G + 2 * (u - G) : G + 2 * G * naggs;
+  unsigned long long* lo_row = P.rows + (uint64_t)row * P.nk1;
+  unsigned long long* hi_row = lo_row + P.nk1;
+  const uint32_t n = P.nk1, per = (n + 255) / 256;
+  const uint32_t b = threadIdx.x * per, e = b + per < n ? b + per : n;
+  uint64_t lo = 0;
+  int64_t hi = 0;
+  for (uint32_t i = b; i < e; ++i) {
+    const uint64_t x = lo_row[i];
+    const uint64_t s = lo + x;
+    if (wide) hi += (int64_t)hi_row[i] + (int64_t)(s < lo);
+    lo = s;
+  }
+  __shared__ uint64_t s_lo[256];
+  __shared__ int64_t s_hi[256];
+  s_lo[threadIdx.x] = lo;
+  s_hi[threadIdx.x] = hi;
+  __syncthreads();
+  if (threadIdx.x == 0) {  // exclusive prefix of the 256 block totals
+    uint64_t rl = 0;
+    int64_t rh = 0;
+    for (int t = 0; t < 256; ++t) {
+      const uint64_t xl = s_lo[t];
+      const int64_t xh = s_hi[t];
+      s_lo[t] = rl;
+      s_hi[t] = rh;
+      const uint64_t sl = rl + xl;
+      rh += xh + (int64_t)(sl < rl);
+      rl = sl;
+    }
+  }
+  __syncthreads();
+  lo = s_lo[threadIdx.x];
+  hi = s_hi[threadIdx.x];
+  for (uint32_t i = b; i < e; ++i) {
+    const uint64_t x = lo_row[i];
+    const uint64_t s = lo + x;
+    if (wide) hi += (int64_t)hi_row[i] + (int64_t)(s < lo);
+    lo = s;
+    lo_row[i] = lo;
+    if (wide) hi_row[i] = (uint64_t)hi;
+  }
+}
+
+// One member of a signature group: positions p0 < p1 of the chunks it covers whole.
+struct PartialLink {
+  const unsigned long long* rows;
+  unsigned long long* out_count;
+  unsigned long long* out_sums;
+  unsigned long long* out_oor;
+  uint32_t nk1, groups, p0, p1;
+};
+
+// Adds the group's sums over chunks [p0, p1) (running sum at p1 minus at p0) into
+// every member's result.  One workgroup per member.
+__global__ void __launch_bounds__(256) k_partials_links(const PartialLink* __restrict__ pl,
+                                                        uint32_t naggs) {
+  const PartialLink L = pl[blockIdx.x];
+  const uint32_t G = L.groups, units = G + G * naggs + 1;
+  for (uint32_t u = threadIdx.x; u < units; u += blockDim.x) {
+    const bool wide = u >= G && u < G + G * naggs;
+    const uint32_t row = u < G ? u : wide ? G + 2 * (u - G) : G + 2 * G * naggs;
+    const unsigned long long* r = L.rows + (uint64_t)row * L.nk1;
+    const uint64_t a = r[L.p1], c = r[L.p0];
+    const uint64_t lo = a - c;
+    if (u < G) {
+      if (lo) atomicAdd(&L.out_count[u], lo);
+    } else if (wide) {
+      const int64_t hi = (int64_t)(r[L.nk1 + L.p1] - r[L.nk1 + L.p0]) - (int64_t)(a < c);
+      atomic_add128(&L.out_sums[(uint64_t)(u - G) * 2], lo, hi);
+    } else if (lo) {
+      atomicAdd(L.out_oor, lo);
     }
   }
 }
@@ -524,6 +657,15 @@ struct tgp_batch {
   uint64_t squeue_off[kQueues + 1] = {};
   uint32_t sitems_chunk = 0;
   bool sitems_valid = false;
+  // signature groups: their partials queries follow the members in d_fam
+  std::vector<SharedQuery> sfam;
+  std::vector<PartialGroup> pgroups;
+  std::vector<PartialLink> plinks;
+  PartialGroup* d_pgroups = nullptr;
+  PartialLink* d_plinks = nullptr;
+  uint64_t* d_partials = nullptr;
+  uint64_t pgroups_cap = 0, plinks_cap = 0, partials_cap = 0, partial_words = 0;
+  uint32_t pmax_units = 0;
 };
 
 // Grows a device buffer to at least `need` elements.  The old buffer is not
@@ -592,6 +734,14 @@ static int32_t launch_variant(tgp_batch* b, hipStream_t s) {
       skern<<<dim3((unsigned)sgrid), dim3(kBlock), 0, s>>>(b->family, b->d_fam, b->d_sitems,
                                                           b->d_sentries, SQ, counter + kQueues);
       TGP_HIP(hipGetLastError());
+      if (!b->pgroups.empty()) {
+        k_partials_scan<<<dim3((unsigned)b->pgroups.size(), b->pmax_units), dim3(256), 0, s>>>(
+            b->d_pgroups, (uint32_t)kShapes[0].naggs);
+        TGP_HIP(hipGetLastError());
+        k_partials_links<<<dim3((unsigned)b->plinks.size()), dim3(256), 0, s>>>(
+            b->d_plinks, (uint32_t)kShapes[0].naggs);
+        TGP_HIP(hipGetLastError());
+      }
     }
     if (b->nitems) {
       auto kern = k_batch_items<NT>;
@@ -617,27 +767,113 @@ static int32_t launch_variant(tgp_batch* b, hipStream_t s) {
 static int32_t build_shared_items(tgp_batch* b, hipStream_t s) {
   const uint64_t C = b->chunk < kSharedMaxSlices ? b->chunk : kSharedMaxSlices;
   const size_t nf = b->fam.size();
-  std::vector<uint64_t> sb(nf), se(nf);
-  // (chunk, member) pairs of the chunks each member overlaps, placed by counting in
-  // (queue, chunk, member) order: chunk k is in queue k % kQueues
-  uint64_t kend = 0;
+  constexpr int NA = kShapes[0].naggs;
+  b->family.chunk = (uint32_t)C;
+  // chunks each member covers whole, [f0, f1): chunk k holds rows
+  // [k C 256, min((k + 1) C 256, T)) of the T-row table
+  uint64_t T = b->family.a.col_len[0];
+  for (int c = 1; c < kShapes[0].nslots; ++c) T = std::min(T, b->family.a.col_len[c]);
+  const uint64_t CR = C * kSliceRows, K = (T + CR - 1) / CR;
+  std::vector<uint64_t> f0(nf, 0), f1(nf, 0);
+  std::map<uint32_t, std::vector<uint32_t>> by_sig;
   for (size_t m = 0; m < nf; ++m) {
-    sb[m] = b->fam[m].row_begin / kSliceRows;
-    se[m] = b->fam[m].row_end > b->fam[m].row_begin
-                ? (b->fam[m].row_end + kSliceRows - 1) / kSliceRows : sb[m];
+    const SharedQuery& M = b->fam[m];
+    if (M.row_end <= M.row_begin) continue;
+    f0[m] = (M.row_begin + CR - 1) / CR;
+    f1[m] = M.row_end >= T ? K : M.row_end / CR;
+    if (f1[m] > f0[m]) by_sig[M.sig].push_back((uint32_t)m);
+  }
+  // a signature with two or more such members gets a partials query over the
+  // chunks they cover whole; each of them then runs only on its other chunks
+  b->sfam = b->fam;
+  b->pgroups.clear();
+  b->plinks.clear();
+  b->pmax_units = 0;
+  std::vector<int32_t> grp(nf, -1);
+  std::vector<std::vector<uint8_t>> gcover;
+  std::vector<uint64_t> goff;
+  std::vector<uint32_t> link_group;
+  uint64_t pw = 0;
+  for (auto& kv : by_sig) {
+    const std::vector<uint32_t>& ms = kv.second;
+    if (ms.size() < 2) continue;
+    uint64_t F0 = ~0ull, F1 = 0;
+    for (uint32_t m : ms) {
+      F0 = std::min(F0, f0[m]);
+      F1 = std::max(F1, f1[m]);
+    }
+    const uint32_t G = (uint32_t)b->fam[ms[0]].words / (1 + NA);
+    const uint64_t nk1 = F1 - F0 + 1;
+    SharedQuery v = b->fam[ms[0]];
+    v.row_begin = F0 * CR;
+    v.row_end = std::min(F1 * CR, T);
+    v.pstride = (uint32_t)nk1;
+    v.pk0 = (uint32_t)F0;
+    v.out_sums = v.out_oor = nullptr;
+    const int32_t vi = (int32_t)b->sfam.size();
+    b->sfam.push_back(v);
+    goff.push_back(pw);
+    b->pgroups.push_back(PartialGroup{nullptr, (uint32_t)nk1, G});
+    b->pmax_units = std::max(b->pmax_units, G + G * NA + 1);
+    std::vector<uint8_t> cover(F1 - F0, 0);
+    for (uint32_t m : ms) {
+      grp[m] = vi;
+      for (uint64_t k = f0[m]; k < f1[m]; ++k) cover[k - F0] = 1;
+      const SharedQuery& M = b->fam[m];
+      b->plinks.push_back(PartialLink{nullptr, M.out_count, M.out_sums, M.out_oor,
+                                      (uint32_t)nk1, G, (uint32_t)(f0[m] - F0),
+                                      (uint32_t)(f1[m] - F0)});
+      link_group.push_back((uint32_t)b->pgroups.size() - 1);
+    }
+    gcover.push_back(std::move(cover));
+    pw += (uint64_t)(G + 2 * G * NA + 1) * nk1;
+  }
+  b->partial_words = pw;
+  if (pw) {
+    TGP_TRY(ensure_cap(b, &b->d_partials, &b->partials_cap, pw));
+    for (size_t g = 0; g < b->pgroups.size(); ++g) {
+      auto* rows = reinterpret_cast<unsigned long long*>(b->d_partials + goff[g]);
+      b->pgroups[g].rows = rows;
+      b->sfam[nf + g].out_count = rows;
+    }
+    for (size_t li = 0; li < b->plinks.size(); ++li)
+      b->plinks[li].rows = b->pgroups[link_group[li]].rows;
+  }
+  const size_t ns = b->sfam.size();
+  // member m's chunk k is one of its entries unless the group's partials query
+  // takes it; the partials query g's chunks are those its members cover whole
+  auto member_chunk = [&](size_t m, uint64_t k) {
+    return !(grp[m] >= 0 && k >= f0[m] && k < f1[m]);
+  };
+  std::vector<uint64_t> sb(ns), se(ns);
+  uint64_t kend = 0;
+  for (size_t m = 0; m < ns; ++m) {
+    sb[m] = b->sfam[m].row_begin / kSliceRows;
+    se[m] = b->sfam[m].row_end > b->sfam[m].row_begin
+                ? (b->sfam[m].row_end + kSliceRows - 1) / kSliceRows : sb[m];
     if (se[m] > sb[m]) kend = std::max(kend, (se[m] + C - 1) / C);
   }
+  // (chunk, query) pairs placed by counting in (queue, chunk, query) order: chunk k
+  // is in queue k % kQueues
   const uint64_t per_q = (kend + kQueues - 1) / kQueues;
   auto pos_of = [&](uint64_t k) { return (k % kQueues) * per_q + k / kQueues; };
   std::vector<uint32_t> at(kQueues * per_q + 1, 0);
-  for (size_t m = 0; m < nf; ++m)
-    if (se[m] > sb[m])
-      for (uint64_t k = sb[m] / C; k * C < se[m]; ++k) at[pos_of(k) + 1] += 1;
+  auto each_pair = [&](auto&& fn) {
+    for (size_t m = 0; m < nf; ++m)
+      if (se[m] > sb[m])
+        for (uint64_t k = sb[m] / C; k * C < se[m]; ++k)
+          if (member_chunk(m, k)) fn(k, (uint32_t)m);
+    for (size_t g = 0; g < b->pgroups.size(); ++g) {
+      const uint64_t F0 = b->sfam[nf + g].pk0;
+      for (uint64_t k = 0; k < gcover[g].size(); ++k)
+        if (gcover[g][k]) fn(F0 + k, (uint32_t)(nf + g));
+    }
+  };
+  each_pair([&](uint64_t k, uint32_t) { at[pos_of(k) + 1] += 1; });
   for (size_t i = 1; i < at.size(); ++i) at[i] += at[i - 1];
   std::vector<std::pair<uint64_t, uint32_t>> pairs(at.back());
-  for (size_t m = 0; m < nf; ++m)  // members in order: each chunk's list stays ascending
-    if (se[m] > sb[m])
-      for (uint64_t k = sb[m] / C; k * C < se[m]; ++k) pairs[at[pos_of(k)]++] = {k, (uint32_t)m};
+  // queries in order: each chunk's list stays ascending
+  each_pair([&](uint64_t k, uint32_t m) { pairs[at[pos_of(k)]++] = {k, m}; });
   std::vector<uint32_t> items, entries;
   uint64_t total = 0;
   // copies per word: lanes of one atomic spread over groups x copies words.  Queries
@@ -676,7 +912,7 @@ static int32_t build_shared_items(tgp_batch* b, hipStream_t s) {
       r.assign(L.size(), 0);
       uint64_t need = 0;
       for (size_t j = 0; j < L.size(); ++j) {
-        const int32_t w = b->fam[L[j]].words;
+        const int32_t w = b->sfam[L[j]].words;
         r[j] = rcap(w);
         while (r[j] > 0 && ((uint64_t)w << r[j]) > (uint64_t)kSharedWords) --r[j];
         need += (uint64_t)w << r[j];
@@ -686,12 +922,12 @@ static int32_t build_shared_items(tgp_batch* b, hipStream_t s) {
       while (need > (uint64_t)kSharedWords) {
         uint64_t big = 0;
         for (size_t j = 0; j < L.size(); ++j)
-          if (r[j] > rmin(b->fam[L[j]].words))
-            big = std::max(big, (uint64_t)b->fam[L[j]].words << r[j]);
+          if (r[j] > rmin(b->sfam[L[j]].words))
+            big = std::max(big, (uint64_t)b->sfam[L[j]].words << r[j]);
         if (big == 0) break;
         for (size_t j = 0; j < L.size() && need > (uint64_t)kSharedWords; ++j)
-          if (r[j] > rmin(b->fam[L[j]].words) && ((uint64_t)b->fam[L[j]].words << r[j]) == big) {
-            need -= (uint64_t)b->fam[L[j]].words << (r[j] - 1);
+          if (r[j] > rmin(b->sfam[L[j]].words) && ((uint64_t)b->sfam[L[j]].words << r[j]) == big) {
+            need -= (uint64_t)b->sfam[L[j]].words << (r[j] - 1);
             --r[j];
           }
       }
@@ -701,7 +937,7 @@ static int32_t build_shared_items(tgp_batch* b, hipStream_t s) {
         const uint32_t e0 = (uint32_t)(entries.size() / 2);
         uint64_t off = 0;
         while (j < L.size()) {
-          const uint64_t sz = (uint64_t)b->fam[L[j]].words << r[j];
+          const uint64_t sz = (uint64_t)b->sfam[L[j]].words << r[j];
           if (off + sz > (uint64_t)kSharedWords) break;
           entries.push_back(L[j]);
           entries.push_back((uint32_t)off | ((uint32_t)r[j] << 16));
@@ -724,6 +960,19 @@ static int32_t build_shared_items(tgp_batch* b, hipStream_t s) {
   b->squeue_off[kQueues] = total;
   if (items.empty()) items.assign(4, 0);
   if (entries.empty()) entries.assign(2, 0);
+  if (!b->sfam.empty()) {
+    TGP_TRY(ensure_cap(b, &b->d_fam, &b->fam_cap, (uint64_t)b->sfam.size()));
+    TGP_HIP(hipMemcpyAsync(b->d_fam, b->sfam.data(), b->sfam.size() * sizeof(SharedQuery),
+                           hipMemcpyHostToDevice, s));
+  }
+  if (!b->pgroups.empty()) {
+    TGP_TRY(ensure_cap(b, &b->d_pgroups, &b->pgroups_cap, (uint64_t)b->pgroups.size()));
+    TGP_TRY(ensure_cap(b, &b->d_plinks, &b->plinks_cap, (uint64_t)b->plinks.size()));
+    TGP_HIP(hipMemcpyAsync(b->d_pgroups, b->pgroups.data(),
+                           b->pgroups.size() * sizeof(PartialGroup), hipMemcpyHostToDevice, s));
+    TGP_HIP(hipMemcpyAsync(b->d_plinks, b->plinks.data(), b->plinks.size() * sizeof(PartialLink),
+                           hipMemcpyHostToDevice, s));
+  }
   TGP_TRY(ensure_cap(b, &b->d_sitems, &b->sitems_cap, (uint64_t)items.size()));
   TGP_TRY(ensure_cap(b, &b->d_sentries, &b->sentries_cap, (uint64_t)entries.size()));
   TGP_HIP(hipMemcpyAsync(b->d_sitems, items.data(), items.size() * 4, hipMemcpyHostToDevice, s));
@@ -840,6 +1089,8 @@ static int32_t batch_launch(tgp_batch* b, hipStream_t s) {
   // results and the counters behind them start at zero
   TGP_TRY(fill_u64(s, b->d_results, b->result_words + 2 * kQueues, 0));
   if (!b->total_slices) return TGP_OK;
+  if (b->order == 3 && b->partial_words)
+    TGP_TRY(fill_u64(s, b->d_partials, b->partial_words, 0));
   return b->cached ? launch_variant<false>(b, s) : launch_variant<true>(b, s);
 }
 
@@ -865,6 +1116,9 @@ int32_t tgp_batch_destroy(tgp_batch* b) {
   if (b->d_fam) hbm_free(b->ctx, b->d_fam);
   if (b->d_sitems) hbm_free(b->ctx, b->d_sitems);
   if (b->d_sentries) hbm_free(b->ctx, b->d_sentries);
+  if (b->d_pgroups) hbm_free(b->ctx, b->d_pgroups);
+  if (b->d_plinks) hbm_free(b->ctx, b->d_plinks);
+  if (b->d_partials) hbm_free(b->ctx, b->d_partials);
   for (void* p : b->retired) hbm_free(b->ctx, p);
   delete b;
   return TGP_OK;
@@ -1010,12 +1264,6 @@ int32_t tgp_batch_set_queries(tgp_batch* b, const tgp_agg_query* qs, uint32_t n,
     b->fam.push_back(m);
     b->fam_plan.push_back(q);
     b->in_family[q] = 1;
-  }
-  if (!b->fam.empty()) {
-    TGP_TRY(ensure_cap(b, &b->d_fam, &b->fam_cap, (uint64_t)b->fam.size()));
-    TGP_HIP(hipMemcpyAsync(b->d_fam, b->fam.data(), b->fam.size() * sizeof(SharedQuery),
-                           hipMemcpyHostToDevice, s));
-    TGP_HIP(hipStreamSynchronize(s));
   }
   b->items_valid = false;
   b->sitems_valid = false;
