@@ -617,6 +617,16 @@ typedef struct tgp_join_agg_query {
 } tgp_join_agg_query;
 int32_t tgp_hash_probe_agg(tgp_ctx* ctx, const tgp_join_agg_query* q, const tgp_group_result* out,
                            void* stream);
+/* n probe aggregations in one launch, with the results of n tgp_hash_probe_agg
+ * calls, bit for bit (the queries may probe different tables, row ranges and build
+ * prefixes).  For the serving loop's join requests, which the reference runs one
+ * at a time (route_and_execute, engine/mod.rs:522-545, each a hash_join_with_keys,
+ * join.rs:544-939): work items of 16K probe rows of any query go to persistent
+ * workgroups.  A query of more than 213 groups runs alone within the call.
+ * Returns once the work is queued (it waits for the stream's earlier work and its
+ * argument copies, not for the probes). */
+int32_t tgp_hash_probe_agg_batch(tgp_ctx* ctx, const tgp_join_agg_query* qs, uint32_t n,
+                                 const tgp_group_result* outs, void* stream);
 
 /* ---- serving: continuous batching over resident columns -------------------- */
 /* The reference serves one query at a time per pooled connection
@@ -652,13 +662,17 @@ typedef struct tgp_serve_config {
   uint64_t hash_budget_bytes;  /* cap on hash-table HBM (0 = the tier's budget) */
   uint32_t wg_per_cu, chunk_slices; /* batch tuning (tgp_batch_tuning) */
   int32_t order, cached_loads;
-  int32_t per_query_launch;    /* 1: no batching, one tgp_group_agg launch per request */
+  int32_t per_query_launch;    /* 1: no batching, one tgp_group_agg launch per aggregate
+                                  request and one tgp_hash_probe_agg launch per join */
   int32_t share_build_tables;  /* 1: one resident hash table per build side (key and payload
                                   columns) serves every join on a prefix of it, probed with
                                   the request's prefix bound (build_rows); it grows (a new
-                                  build) when a request needs a longer prefix and is held in
-                                  the HBM tier until the server ends.  0: every join builds
-                                  its own table, admitted by its footprint */
+                                  build, over the longest prefix a queued join asks for) when
+                                  a request needs a longer prefix and is held in the HBM tier
+                                  until the server ends.  The joins whose table is ready are
+                                  probed together, one tgp_hash_probe_agg_batch launch per
+                                  take.  0: every join builds its own table, admitted by its
+                                  footprint */
 } tgp_serve_config;
 typedef struct tgp_serve_timing {
   int64_t arrival_ns, start_ns, done_ns; /* tgp_serve_now_ns() clock */

@@ -431,3 +431,55 @@ def test_probe_agg_over_a_build_prefix(ctx, group_by, ngroups, dups):
         full.free()
         for c in keep + cols:
             c.free()
+
+
+def test_probe_agg_batch_equals_single_calls(ctx):
+    """tgp_hash_probe_agg_batch: queries over two tables (unique and duplicate build
+    keys), every grouping form, row ranges from empty through one row to ranges
+    over many 16K-row work items, build prefixes, groups out of range, and queries
+    of 4000 groups that run alone within the call, all equal one
+    tgp_hash_probe_agg call each, bit for bit, and a sample equals the oracle."""
+    rng = np.random.default_rng(77)
+    nb, npb = 50_000, 300_007
+    bk_u = (rng.permutation(nb) * 7 + 3).astype(np.int64)
+    bk_d = rng.integers(0, nb // 4, nb).astype(np.int64)
+    bp = rng.integers(-3, 40, nb).astype(np.int32)
+    pk = np.concatenate([bk_u, bk_d])[rng.integers(0, 2 * nb, npb)]
+    pk[::11] += 1
+    pv = rng.integers(-2**45, 2**45, npb).astype(np.int64)
+    pg = rng.integers(-5, 120, npb).astype(np.int32)
+    tu, keep_u = build(ctx, bk_u, bp)
+    td, keep_d = build(ctx, bk_d, bp)
+    cols = [dev(ctx, pk), dev(ctx, pv), dev(ctx, pg)]
+    specs = []
+    forms = [(A.JOIN_GROUP_BUILD_PAYLOAD, 25, 0), (A.JOIN_GROUP_BUILD_PAYLOAD, 40, -3),
+             (A.JOIN_GROUP_PROBE_COL, 100, 0), (A.JOIN_GROUP_NONE, 1, 0),
+             (A.JOIN_GROUP_PROBE_COL, 4000, -5)]
+    ranges = [(0, npb), (0, 0), (17, 18), (16_384 - 5, 3 * 16_384 + 9), (1000, 250_000),
+              (npb - 1, npb)]
+    for t, tk in ((tu, bk_u), (td, bk_d)):
+        for gb, ng, kb in forms:
+            for b, e in ranges:
+                lim = int(rng.choice([0, 1, 999, nb]))
+                specs.append((t, tk, gb, ng, kb, b, e, lim))
+    try:
+        qs = [t.agg_query(cols[0], cols[1], gb, ng, kb, cols[2], b, e, lim)
+              for t, _, gb, ng, kb, b, e, lim in specs]
+        got = ctx.probe_agg_batch(qs)
+        again = ctx.probe_agg_batch(qs[::-1])[::-1]
+        for i, (t, tk, gb, ng, kb, b, e, lim) in enumerate(specs):
+            want = t.probe_agg(cols[0], cols[1], gb, ng, kb, cols[2], b, e, build_rows=lim)
+            for g in (got[i], again[i]):
+                assert np.array_equal(g.count, want.count), i
+                assert g.sums() == want.sums() and g.n_out_of_range == want.n_out_of_range, i
+            if i % 7 == 0:
+                n = lim or nb
+                cnt, sums, oor = B.join_agg(tk[:n], bp[:n], pk, pv, pg, gb, kb, ng, b, e)
+                assert np.array_equal(got[i].count, cnt) and \
+                    [x[0] for x in got[i].sums()] == sums and got[i].n_out_of_range == oor, i
+        assert ctx.probe_agg_batch([]) == []
+    finally:
+        for t in (tu, td):
+            t.free()
+        for c in keep_u + keep_d + cols:
+            c.free()

@@ -639,6 +639,7 @@ struct tgp_batch {
   bool items_valid = false;
   uint32_t* d_granule = nullptr;
   uint64_t granule_cap = 0;
+  bool granule_valid = false;
   bool use_graph = true;
   hipGraphExec_t graph = nullptr;
   hipGraph_t graph_def = nullptr;
@@ -842,9 +843,6 @@ static int32_t build_shared_items(tgp_batch* b, hipStream_t s) {
   const size_t ns = b->sfam.size();
   // member m's chunk k is one of its entries unless the group's partials query
   // takes it; the partials query g's chunks are those its members cover whole
-  auto member_chunk = [&](size_t m, uint64_t k) {
-    return !(grp[m] >= 0 && k >= f0[m] && k < f1[m]);
-  };
   std::vector<uint64_t> sb(ns), se(ns);
   uint64_t kend = 0;
   for (size_t m = 0; m < ns; ++m) {
@@ -859,10 +857,16 @@ static int32_t build_shared_items(tgp_batch* b, hipStream_t s) {
   auto pos_of = [&](uint64_t k) { return (k % kQueues) * per_q + k / kQueues; };
   std::vector<uint32_t> at(kQueues * per_q + 1, 0);
   auto each_pair = [&](auto&& fn) {
-    for (size_t m = 0; m < nf; ++m)
-      if (se[m] > sb[m])
-        for (uint64_t k = sb[m] / C; k * C < se[m]; ++k)
-          if (member_chunk(m, k)) fn(k, (uint32_t)m);
+    for (size_t m = 0; m < nf; ++m) {
+      if (se[m] <= sb[m]) continue;
+      const uint64_t k0 = sb[m] / C, k1 = (se[m] + C - 1) / C;
+      if (grp[m] < 0) {
+        for (uint64_t k = k0; k < k1; ++k) fn(k, (uint32_t)m);
+      } else {  // the chunks before and after those the partials query takes
+        for (uint64_t k = k0; k < std::min(k1, f0[m]); ++k) fn(k, (uint32_t)m);
+        for (uint64_t k = std::max(k0, f1[m]); k < k1; ++k) fn(k, (uint32_t)m);
+      }
+    }
     for (size_t g = 0; g < b->pgroups.size(); ++g) {
       const uint64_t F0 = b->sfam[nf + g].pk0;
       for (uint64_t k = 0; k < gcover[g].size(); ++k)
@@ -992,7 +996,24 @@ static int32_t build_shared_items(tgp_batch* b, hipStream_t s) {
 //   order 2: the table cut into absolute chunks of chunk * 256 rows; every query
 //            has one item per chunk it overlaps, so the items of all queries on
 //            chunk k cover the same rows; chunk k's items go to queue k % kQueues.
+// The query-ordered kernel's slice -> query table: granule g holds the plan index
+// of the query that contains slice g * kGranSlices (synchronous upload).
+static int32_t build_granules(tgp_batch* b, hipStream_t s) {
+  const uint64_t ngran = (b->total_slices + kGranSlices - 1) / kGranSlices;
+  std::vector<uint32_t> gran(ngran ? ngran : 1, 0);
+  for (uint64_t g = 0, q = 0; g < ngran; ++g) {
+    while (q + 1 < b->plan.size() && b->plan[q + 1].slice0 <= g * kGranSlices) ++q;
+    gran[g] = (uint32_t)q;
+  }
+  TGP_TRY(ensure_cap(b, &b->d_granule, &b->granule_cap, gran.size()));
+  TGP_HIP(hipMemcpyAsync(b->d_granule, gran.data(), gran.size() * 4, hipMemcpyHostToDevice, s));
+  TGP_HIP(hipStreamSynchronize(s));
+  b->granule_valid = true;
+  return TGP_OK;
+}
+
 static int32_t build_items(tgp_batch* b, hipStream_t s) {
+  if (b->order == 0 && !b->granule_valid) TGP_TRY(build_granules(b, s));
   if (b->order == 3 && !(b->sitems_valid && b->sitems_chunk == b->chunk))
     TGP_TRY(build_shared_items(b, s));
   if (b->order == 0 || (b->items_chunk == b->chunk && b->items_order == b->order &&
@@ -1183,18 +1204,9 @@ int32_t tgp_batch_set_queries(tgp_batch* b, const tgp_agg_query* qs, uint32_t n,
   b->total_slices = slices;
   const uint64_t bytes = b->plan.size() * sizeof(BatchQuery);
   TGP_TRY(ensure_cap(b, &b->d_plan, &b->plan_cap, b->plan.size() ? b->plan.size() : 1));
-  // slice -> query table: granule g holds the plan index of the query that
-  // contains slice g * kGranSlices
-  const uint64_t ngran = (slices + kGranSlices - 1) / kGranSlices;
-  std::vector<uint32_t> gran(ngran ? ngran : 1, 0);
-  for (uint64_t g = 0, q = 0; g < ngran; ++g) {
-    while (q + 1 < b->plan.size() && b->plan[q + 1].slice0 <= g * kGranSlices) ++q;
-    gran[g] = (uint32_t)q;
-  }
-  TGP_TRY(ensure_cap(b, &b->d_granule, &b->granule_cap, gran.size()));
+  b->granule_valid = false;  // built when the query-ordered kernel runs
   if (bytes) {
     TGP_HIP(hipMemcpyAsync(b->d_plan, b->plan.data(), bytes, hipMemcpyHostToDevice, s));
-    TGP_HIP(hipMemcpyAsync(b->d_granule, gran.data(), gran.size() * 4, hipMemcpyHostToDevice, s));
     TGP_HIP(hipStreamSynchronize(s));
   }
   // the shared-scan family: Q1-layout queries on the first such query's columns

@@ -371,47 +371,21 @@ __device__ __forceinline__ void i128_atomic_add(unsigned long long* lo, unsigned
   if (h) atomicAdd(hi, (unsigned long long)h);
 }
 
-template <bool USE_LDS, bool EXACT>
-__global__ void __launch_bounds__(256) k_probe_agg(JoinTableDev T, ProbeAggDev Q,
-                                                   uint32_t rshift, uint32_t shared_table) {
-  extern __shared__ __align__(16) unsigned long long ltab[];
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint64_t words = 3 * Q.ngroups;                  // per copy
-  const uint64_t region = words << rshift;               // per wave (or per workgroup)
-  const uint32_t nwt = shared_table ? 1 : 4;             // tables in the workgroup
-  LdsTab lt{ltab + (shared_table ? 0 : wave) * region, rshift, lane & ((1u << rshift) - 1)};
-  unsigned long long* cnt = Q.out_count;  // global form: the result buffers hold the
-  unsigned long long* sum = Q.out_sums;   // split sums until k_split_finalize
-  if (USE_LDS) {
-    for (uint64_t i = threadIdx.x; i < nwt * region; i += blockDim.x) ltab[i] = 0;
-    __syncthreads();
-  }
-  auto add = [&](uint64_t g, int64_t v) {
-    if constexpr (EXACT) {
-      if constexpr (USE_LDS) {
-        atomicAdd(lt.at(g), 1ull);
-        i128_atomic_add(lt.at(Q.ngroups + 2 * g), lt.at(Q.ngroups + 2 * g + 1), v);
-      } else {
-        atomicAdd(&cnt[g], 1ull);
-        i128_atomic_add(&sum[2 * g], &sum[2 * g + 1], v);
-      }
-    } else if constexpr (USE_LDS) {
-      acc_add_lds(lt, Q.ngroups, g, v);
-    } else {
-      acc_add(cnt, sum, g, v);
-    }
-  };
-  uint32_t oor = 0;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i0 = Q.b + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i0 < Q.e;
-       i0 += stride * kProbeUnroll) {
+// Probes rows i_start, i_start + stride, ... below e (kProbeUnroll rows per lane at
+// a time) and calls add(group, value) per match inside the query's build prefix;
+// matches whose group falls outside [0, ngroups) count in oor.
+template <typename Add>
+__device__ __forceinline__ void probe_rows(const JoinTableDev& T, const ProbeAggDev& Q,
+                                           uint64_t i_start, uint64_t e, uint64_t stride,
+                                           Add&& add, uint32_t& oor) {
+  for (uint64_t i0 = i_start; i0 < e; i0 += stride * kProbeUnroll) {
     uint64_t idx[kProbeUnroll];
     bool ok[kProbeUnroll];
     unsigned long long k[kProbeUnroll];
     int64_t kv[kProbeUnroll], v[kProbeUnroll], g[kProbeUnroll];
 #pragma unroll
     for (int u = 0; u < kProbeUnroll; ++u) {
-      ok[u] = i0 + u * stride < Q.e;
+      ok[u] = i0 + u * stride < e;
       idx[u] = ok[u] ? i0 + u * stride : i0;
     }
     load_rows(Q.keys, Q.kdt, idx, ok, kv);
@@ -439,48 +413,162 @@ __global__ void __launch_bounds__(256) k_probe_agg(JoinTableDev T, ProbeAggDev Q
       });
     }
   }
-  if (USE_LDS) {
-    __syncthreads();
-    // word w summed over the 4 waves and R copies: lo32 sums and counts stay
-    // below 2^64 (fewer than 2^32 additions per word and call), hi32 sums are
-    // signed 64-bit
-    auto total = [&](uint64_t w) {
-      unsigned long long t = 0;
+}
+
+// Adds the workgroup's LDS tables (nwt tables of `region` words, each group's
+// words in 2^rshift copies) into the query's result and clears them.  Called by
+// every thread after a barrier; each thread reads and clears only its groups.
+template <bool EXACT>
+__device__ __forceinline__ void flush_lds_tables(unsigned long long* ltab, uint64_t region,
+                                                 uint32_t nwt, uint32_t rshift,
+                                                 const ProbeAggDev& Q) {
+  // word w summed over the tables and copies: lo32 sums and counts stay below
+  // 2^64 (fewer than 2^32 additions per word between flushes), hi32 sums are
+  // signed 64-bit
+  auto total = [&](uint64_t w) {
+    unsigned long long t = 0;
+    for (uint32_t wv = 0; wv < nwt; ++wv)
+      for (uint32_t c = 0; c < (1u << rshift); ++c) {
+        unsigned long long& x = ltab[wv * region + (w << rshift) + c];
+        t += x;
+        x = 0;
+      }
+    return t;
+  };
+  for (uint64_t gi = threadIdx.x; gi < Q.ngroups; gi += blockDim.x) {
+    const unsigned long long c = total(gi);
+    uint64_t lo;
+    int64_t hi;
+    if constexpr (EXACT) {
+      // int128 copies: add the low words with their carries, then the high words
+      lo = 0;
+      hi = 0;
       for (uint32_t wv = 0; wv < nwt; ++wv)
-        for (uint32_t c = 0; c < (1u << rshift); ++c) t += ltab[wv * region + (w << rshift) + c];
-      return t;
-    };
-    for (uint64_t gi = threadIdx.x; gi < Q.ngroups; gi += blockDim.x) {
-      const unsigned long long c = total(gi);
-      if (!c) continue;
-      atomicAdd(&Q.out_count[gi], c);
-      uint64_t lo;
-      int64_t hi;
-      if constexpr (EXACT) {
-        // int128 copies: add the low words with their carries, then the high words
-        lo = 0;
-        hi = 0;
-        for (uint32_t wv = 0; wv < nwt; ++wv)
-          for (uint32_t c2 = 0; c2 < (1u << rshift); ++c2) {
-            const uint64_t base = wv * region + c2;
-            const uint64_t l = ltab[base + ((Q.ngroups + 2 * gi) << rshift)];
-            const uint64_t h = ltab[base + ((Q.ngroups + 2 * gi + 1) << rshift)];
-            const uint64_t nl = lo + l;
-            hi += (int64_t)h + (int64_t)(nl < lo);
-            lo = nl;
-          }
-      } else {
-        split_to_i128(total(Q.ngroups + 2 * gi), (int64_t)total(Q.ngroups + 2 * gi + 1), &lo,
-                      &hi);
-      }
-      if (lo || hi) {
-        const unsigned long long old = atomicAdd(&Q.out_sums[2 * gi], (unsigned long long)lo);
-        const int64_t h = hi + (int64_t)((old + lo) < old);
-        if (h) atomicAdd(&Q.out_sums[2 * gi + 1], (unsigned long long)h);
-      }
+        for (uint32_t c2 = 0; c2 < (1u << rshift); ++c2) {
+          const uint64_t base = wv * region + c2;
+          unsigned long long& l = ltab[base + ((Q.ngroups + 2 * gi) << rshift)];
+          unsigned long long& h = ltab[base + ((Q.ngroups + 2 * gi + 1) << rshift)];
+          const uint64_t nl = lo + l;
+          hi += (int64_t)h + (int64_t)(nl < lo);
+          lo = nl;
+          l = 0;
+          h = 0;
+        }
+    } else {
+      const unsigned long long slo = total(Q.ngroups + 2 * gi);
+      split_to_i128(slo, (int64_t)total(Q.ngroups + 2 * gi + 1), &lo, &hi);
+    }
+    if (!c) continue;
+    atomicAdd(&Q.out_count[gi], c);
+    if (lo || hi) {
+      const unsigned long long old = atomicAdd(&Q.out_sums[2 * gi], (unsigned long long)lo);
+      const int64_t h = hi + (int64_t)((old + lo) < old);
+      if (h) atomicAdd(&Q.out_sums[2 * gi + 1], (unsigned long long)h);
     }
   }
+}
+
+template <bool USE_LDS, bool EXACT>
+__device__ __forceinline__ void lds_or_global_add(const LdsTab& lt, const ProbeAggDev& Q,
+                                                  uint64_t g, int64_t v) {
+  if constexpr (EXACT) {
+    if constexpr (USE_LDS) {
+      atomicAdd(lt.at(g), 1ull);
+      i128_atomic_add(lt.at(Q.ngroups + 2 * g), lt.at(Q.ngroups + 2 * g + 1), v);
+    } else {  // the result buffers hold int128 words
+      atomicAdd(&Q.out_count[g], 1ull);
+      i128_atomic_add(&Q.out_sums[2 * g], &Q.out_sums[2 * g + 1], v);
+    }
+  } else if constexpr (USE_LDS) {
+    acc_add_lds(lt, Q.ngroups, g, v);
+  } else {  // the result buffers hold the split sums until k_split_finalize
+    acc_add(Q.out_count, Q.out_sums, g, v);
+  }
+}
+
+template <bool USE_LDS, bool EXACT>
+__global__ void __launch_bounds__(256) k_probe_agg(JoinTableDev T, ProbeAggDev Q,
+                                                   uint32_t rshift, uint32_t shared_table) {
+  extern __shared__ __align__(16) unsigned long long ltab[];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t words = 3 * Q.ngroups;                  // per copy
+  const uint64_t region = words << rshift;               // per wave (or per workgroup)
+  const uint32_t nwt = shared_table ? 1 : 4;             // tables in the workgroup
+  LdsTab lt{ltab + (shared_table ? 0 : wave) * region, rshift, lane & ((1u << rshift) - 1)};
+  if (USE_LDS) {
+    for (uint64_t i = threadIdx.x; i < nwt * region; i += blockDim.x) ltab[i] = 0;
+    __syncthreads();
+  }
+  uint32_t oor = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  probe_rows(T, Q, Q.b + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x, Q.e, stride,
+             [&](uint64_t g, int64_t v) { lds_or_global_add<USE_LDS, EXACT>(lt, Q, g, v); },
+             oor);
+  if (USE_LDS) {
+    __syncthreads();
+    flush_lds_tables<EXACT>(ltab, region, nwt, rshift, Q);
+  }
   if (oor && Q.out_oor) atomicAdd(Q.out_oor, (unsigned long long)oor);
+}
+
+// ---- many probe aggregations in one launch -----------------------------------------------
+// Each query of a batch probes its own rows of its own table into its own result;
+// work items of up to kBatchItemRows probe rows are taken from a counter by
+// persistent workgroups, whose four waves keep per-wave LDS tables of the item's
+// query (2^rshift copies, as in k_probe_agg) and add them to the query's result
+// after every item.
+constexpr uint64_t kBatchItemRows = 16384;
+constexpr uint32_t kBatchWaveWords = 640;  // 5 KiB per wave, 20 KiB per workgroup: 8 per CU
+
+struct ProbeBatchQuery {
+  JoinTableDev T;
+  ProbeAggDev Q;
+  uint32_t rshift;
+  uint32_t _pad;
+};
+
+struct ProbeItem {
+  uint64_t b;      // first probe row
+  uint32_t q, n;   // query, rows
+};
+
+template <bool EXACT>
+__global__ void __launch_bounds__(256) k_probe_agg_batch(const ProbeBatchQuery* __restrict__ qs,
+                                                         const ProbeItem* __restrict__ items,
+                                                         uint64_t nitems,
+                                                         unsigned long long* counter) {
+  __shared__ __align__(16) unsigned long long ltab[4 * kBatchWaveWords];
+  __shared__ uint32_t s_item;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (uint32_t i = threadIdx.x; i < 4 * kBatchWaveWords; i += blockDim.x) ltab[i] = 0;
+  for (;;) {
+    __syncthreads();  // tables clear, s_item free
+    if (threadIdx.x == 0) s_item = (uint32_t)atomicAdd(counter, 1ull);
+    __syncthreads();
+    const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
+    if (it >= nitems) break;
+    const uint64_t b = items[it].b;
+    const uint32_t q = items[it].q, n = items[it].n;
+    const JoinTableDev T = qs[q].T;
+    const ProbeAggDev Q = qs[q].Q;
+    const uint32_t rshift = qs[q].rshift;
+    const uint64_t region = (3 * Q.ngroups) << rshift;
+    LdsTab lt{ltab + wave * region, rshift, lane & ((1u << rshift) - 1)};
+    uint32_t oor = 0;
+    probe_rows(T, Q, b + threadIdx.x, b + n, blockDim.x,
+               [&](uint64_t g, int64_t v) { lds_or_global_add<true, EXACT>(lt, Q, g, v); }, oor);
+    if (oor && Q.out_oor) atomicAdd(Q.out_oor, (unsigned long long)oor);
+    __syncthreads();
+    flush_lds_tables<EXACT>(ltab, region, 4, rshift, Q);
+  }
+}
+
+// Zeroes every query's result (count, sums, out-of-range word): one workgroup each.
+__global__ void __launch_bounds__(256) k_probe_batch_zero(const ProbeBatchQuery* __restrict__ qs) {
+  const ProbeAggDev& Q = qs[blockIdx.x].Q;
+  for (uint64_t i = threadIdx.x; i < Q.ngroups; i += blockDim.x) Q.out_count[i] = 0;
+  for (uint64_t i = threadIdx.x; i < 2 * Q.ngroups; i += blockDim.x) Q.out_sums[i] = 0;
+  if (threadIdx.x == 0 && Q.out_oor) *Q.out_oor = 0;
 }
 
 // In place: (sum of lo32, sum of hi32) pairs -> int128 (lo, hi) words.
@@ -973,6 +1061,58 @@ static int32_t check_key_column(const tgp_column* c, const char* what) {
   return TGP_OK;
 }
 
+static int32_t require_single_key(const tgp_hash_table* t, const char* what) {
+  TGP_REQUIRE(t->bkeys.n <= 1,
+              "%s: the table has %d key columns (use tgp_hash_join_pairs_sync / tgp_hash_join_mark)",
+              what, t->bkeys.n);
+  return TGP_OK;
+}
+
+// The argument rules of a probe aggregation (tgp_hash_probe_agg and each query
+// of tgp_hash_probe_agg_batch).
+static int32_t check_probe_agg(const tgp_join_agg_query* q, const tgp_group_result* out) {
+  TGP_REQUIRE(q->table, "tgp_hash_probe_agg: null table");
+  TGP_TRY(require_single_key(q->table, "tgp_hash_probe_agg"));
+  TGP_REQUIRE(out->d_count && out->d_sums, "tgp_hash_probe_agg: null result buffer");
+  TGP_TRY(check_key_column(&q->probe_keys, "probe keys"));
+  TGP_TRY(check_key_column(&q->value, "value column"));
+  TGP_REQUIRE(q->group_by >= TGP_JOIN_GROUP_BUILD_PAYLOAD && q->group_by <= TGP_JOIN_GROUP_NONE,
+              "bad group_by %d", q->group_by);
+  if (q->group_by == TGP_JOIN_GROUP_PROBE_COL) {
+    TGP_TRY(check_key_column(&q->group_col, "group column"));
+    TGP_REQUIRE(q->group_col.len >= q->row_end, "group column shorter than row_end");
+  }
+  TGP_REQUIRE(q->ngroups >= 1 && q->ngroups <= (1ull << 32), "ngroups out of range");
+  TGP_REQUIRE(q->group_by != TGP_JOIN_GROUP_NONE || q->ngroups == 1,
+              "GROUP_NONE needs ngroups == 1");
+  TGP_REQUIRE(q->row_begin <= q->row_end && q->row_end <= q->probe_keys.len &&
+                  q->row_end <= q->value.len,
+              "bad probe row range");
+  return TGP_OK;
+}
+
+// The kernel arguments of one probe aggregation.
+static ProbeAggDev probe_agg_dev(const tgp_join_agg_query* q, const tgp_group_result* out) {
+  ProbeAggDev Q;
+  Q.keys = q->probe_keys.data;
+  Q.kdt = q->probe_keys.dtype;
+  Q.val = q->value.data;
+  Q.vdt = q->value.dtype;
+  Q.grp = q->group_col.data;
+  Q.gdt = q->group_col.dtype;
+  Q.group_by = q->group_by;
+  Q.key_base = q->group_by == TGP_JOIN_GROUP_NONE ? 0 : q->key_base;
+  Q.ngroups = q->ngroups;
+  Q.b = q->row_begin;
+  Q.e = q->row_end;
+  Q.row_limit = q->build_rows ? q->build_rows : ~0u;
+  Q._pad = 0;
+  Q.out_count = reinterpret_cast<unsigned long long*>(out->d_count);
+  Q.out_sums = reinterpret_cast<unsigned long long*>(out->d_sums);
+  Q.out_oor = reinterpret_cast<unsigned long long*>(out->d_n_out_of_range);
+  return Q;
+}
+
 extern "C" {
 
 uint64_t tgp_hash_table_bytes_for(uint64_t n_build_rows) {
@@ -1137,13 +1277,6 @@ int32_t tgp_hash_table_info(const tgp_hash_table* t, uint64_t* n_rows, uint64_t*
   if (n_rows) *n_rows = t->n_rows;
   if (capacity) *capacity = t->cap;
   if (has_duplicates) *has_duplicates = t->has_dup;
-  return TGP_OK;
-}
-
-static int32_t require_single_key(const tgp_hash_table* t, const char* what) {
-  TGP_REQUIRE(t->bkeys.n <= 1,
-              "%s: the table has %d key columns (use tgp_hash_join_pairs_sync / tgp_hash_join_mark)",
-              what, t->bkeys.n);
   return TGP_OK;
 }
 
@@ -1313,45 +1446,14 @@ int32_t tgp_hash_probe_agg(tgp_ctx* ctx, const tgp_join_agg_query* q, const tgp_
                            void* stream) {
   TGP_REQUIRE(ctx && q && out && q->table, "tgp_hash_probe_agg: null argument");
   TGP_TRY(check_injected(ctx));
-  TGP_TRY(require_single_key(q->table, "tgp_hash_probe_agg"));
-  TGP_REQUIRE(out->d_count && out->d_sums, "tgp_hash_probe_agg: null result buffer");
-  TGP_TRY(check_key_column(&q->probe_keys, "probe keys"));
-  TGP_TRY(check_key_column(&q->value, "value column"));
-  TGP_REQUIRE(q->group_by >= TGP_JOIN_GROUP_BUILD_PAYLOAD && q->group_by <= TGP_JOIN_GROUP_NONE,
-              "bad group_by %d", q->group_by);
-  if (q->group_by == TGP_JOIN_GROUP_PROBE_COL) {
-    TGP_TRY(check_key_column(&q->group_col, "group column"));
-    TGP_REQUIRE(q->group_col.len >= q->row_end, "group column shorter than row_end");
-  }
-  TGP_REQUIRE(q->ngroups >= 1 && q->ngroups <= (1ull << 32), "ngroups out of range");
-  TGP_REQUIRE(q->group_by != TGP_JOIN_GROUP_NONE || q->ngroups == 1,
-              "GROUP_NONE needs ngroups == 1");
-  TGP_REQUIRE(q->row_begin <= q->row_end && q->row_end <= q->probe_keys.len &&
-                  q->row_end <= q->value.len,
-              "bad probe row range");
+  TGP_TRY(check_probe_agg(q, out));
   hipStream_t s = pick_stream(ctx, stream);
   TGP_TRY(fill_u64(s, out->d_count, q->ngroups, 0));
   TGP_TRY(fill_u64(s, out->d_sums, 2 * q->ngroups, 0));
   if (out->d_n_out_of_range) TGP_TRY(fill_u64(s, out->d_n_out_of_range, 1, 0));
   const uint64_t n = q->row_end - q->row_begin;
   if (n == 0) return TGP_OK;
-  ProbeAggDev Q;
-  Q.keys = q->probe_keys.data;
-  Q.kdt = q->probe_keys.dtype;
-  Q.val = q->value.data;
-  Q.vdt = q->value.dtype;
-  Q.grp = q->group_col.data;
-  Q.gdt = q->group_col.dtype;
-  Q.group_by = q->group_by;
-  Q.key_base = q->group_by == TGP_JOIN_GROUP_NONE ? 0 : q->key_base;
-  Q.ngroups = q->ngroups;
-  Q.b = q->row_begin;
-  Q.e = q->row_end;
-  Q.row_limit = q->build_rows ? q->build_rows : ~0u;
-  Q._pad = 0;
-  Q.out_count = reinterpret_cast<unsigned long long*>(out->d_count);
-  Q.out_sums = reinterpret_cast<unsigned long long*>(out->d_sums);
-  Q.out_oor = reinterpret_cast<unsigned long long*>(out->d_n_out_of_range);
+  const ProbeAggDev Q = probe_agg_dev(q, out);
   // 32 workgroups per CU, several rounds of them: the probe rows of a workgroup
   // meet chains of different lengths, and smaller shares even out the CUs' end
   // times (8 per CU: 0.97 ms for 60M probes into 25 groups; 32: 0.76 ms)
@@ -1425,6 +1527,77 @@ int32_t tgp_hash_probe_agg(tgp_ctx* ctx, const tgp_join_agg_query* q, const tgp_
     }
   }
   TGP_HIP(hipGetLastError());
+  return TGP_OK;
+}
+
+int32_t tgp_hash_probe_agg_batch(tgp_ctx* ctx, const tgp_join_agg_query* qs, uint32_t n,
+                                 const tgp_group_result* outs, void* stream) {
+  TGP_REQUIRE(ctx && (n == 0 || (qs && outs)), "tgp_hash_probe_agg_batch: null argument");
+  TGP_TRY(check_injected(ctx));
+  for (uint32_t i = 0; i < n; ++i) {
+    TGP_REQUIRE(qs[i].table, "tgp_hash_probe_agg_batch: query %u has no table", i);
+    TGP_TRY(check_probe_agg(&qs[i], &outs[i]));
+  }
+  hipStream_t s = pick_stream(ctx, stream);
+  // queries whose per-wave table fits kBatchWaveWords words run in the batch
+  // kernel; the others (more than 213 groups) run alone, after it
+  std::vector<ProbeBatchQuery> bq;
+  std::vector<ProbeItem> items;
+  std::vector<uint32_t> alone;
+  bool exact = false;
+  for (uint32_t i = 0; i < n; ++i) {
+    const tgp_join_agg_query& q = qs[i];
+    const uint64_t words = 3 * q.ngroups;
+    if (words > kBatchWaveWords) {
+      alone.push_back(i);
+      continue;
+    }
+    uint32_t r = 6;
+    while ((words << r) > kBatchWaveWords) --r;
+    ProbeBatchQuery b{};
+    b.T = q.table->dev();
+    b.Q = probe_agg_dev(&q, &outs[i]);
+    b.rshift = r;
+    exact = exact || q.table->has_dup != 0;
+    const uint32_t qi = (uint32_t)bq.size();
+    bq.push_back(b);
+    for (uint64_t r0 = q.row_begin; r0 < q.row_end; r0 += kBatchItemRows)
+      items.push_back(ProbeItem{r0, qi, (uint32_t)std::min<uint64_t>(kBatchItemRows,
+                                                                     q.row_end - r0)});
+  }
+  if (!bq.empty()) {
+    // scratch (slot 1): counter | queries | items
+    const uint64_t qoff = 256, ioff = (qoff + bq.size() * sizeof(ProbeBatchQuery) + 255) & ~255ull;
+    const uint64_t bytes = ioff + items.size() * sizeof(ProbeItem) + 16;
+    void* scr = nullptr;
+    TGP_TRY(get_scratch(ctx, s, bytes, &scr, 1));
+    auto* base = static_cast<uint8_t*>(scr);
+    auto* counter = reinterpret_cast<unsigned long long*>(base);
+    auto* d_q = reinterpret_cast<ProbeBatchQuery*>(base + qoff);
+    auto* d_items = reinterpret_cast<ProbeItem*>(base + ioff);
+    TGP_HIP(hipMemcpyAsync(d_q, bq.data(), bq.size() * sizeof(ProbeBatchQuery),
+                           hipMemcpyHostToDevice, s));
+    if (!items.empty())
+      TGP_HIP(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(ProbeItem),
+                             hipMemcpyHostToDevice, s));
+    // the copies read the host vectors before they go (the kernels are not waited for)
+    TGP_HIP(hipStreamSynchronize(s));
+    TGP_TRY(fill_u64(s, reinterpret_cast<uint64_t*>(counter), 1, 0));
+    k_probe_batch_zero<<<(unsigned)bq.size(), 256, 0, s>>>(d_q);
+    TGP_HIP(hipGetLastError());
+    if (!items.empty()) {
+      // 20 KiB of LDS per workgroup: eight per CU
+      const uint64_t grid = std::min<uint64_t>(items.size(), (uint64_t)persistent_grid(ctx, 8));
+      if (exact)
+        k_probe_agg_batch<true><<<(unsigned)grid, 256, 0, s>>>(d_q, d_items, items.size(),
+                                                               counter);
+      else
+        k_probe_agg_batch<false><<<(unsigned)grid, 256, 0, s>>>(d_q, d_items, items.size(),
+                                                                counter);
+      TGP_HIP(hipGetLastError());
+    }
+  }
+  for (uint32_t i : alone) TGP_TRY(tgp_hash_probe_agg(ctx, &qs[i], &outs[i], s));
   return TGP_OK;
 }
 

@@ -84,6 +84,8 @@ struct AggSlot {
 // build rows probes it with the rows from nb on excluded (build_rows).  A join
 // that needs a longer prefix builds a replacement; the outgrown table is kept
 // until no join in flight probes it.
+constexpr size_t kMaxProbeBatch = 1024;  // joins per probe batch
+
 struct SharedTable {
   const void* keys = nullptr;
   const void* pay = nullptr;
@@ -101,13 +103,19 @@ struct JoinSlot {
   SharedTable* shared = nullptr;    // the shared table this join probes
   tgp_hash_table* shared_table = nullptr;  // ... the instance it holds a reference to
   SharedTable* builds = nullptr;    // the shared table this slot is (re)building
+  uint64_t build_len = 0;           // ... over this many build rows
   tgp_hash_table* table = nullptr;
   uint64_t table_bytes = 0;
   void* pool = nullptr;       // the slot's table memory, reused join after join
   uint64_t pool_bytes = 0;    // grows to the largest table the slot has built
-  uint64_t* d_res = nullptr;  // count[ng] | sums[ng*2] | oor
+  uint64_t* d_res = nullptr;  // count[ng] | sums[ng*2] | oor (per request of a probe batch)
   uint64_t d_cap = 0;
   bool busy = false;
+  // a probe batch: joins whose shared tables are ready, probed in one launch, each
+  // holding a reference to the table instance it probes
+  std::vector<Request*> batch;
+  std::vector<std::pair<SharedTable*, tgp_hash_table*>> holds;
+  std::vector<uint64_t> h_res;
 };
 
 }  // namespace
@@ -398,7 +406,94 @@ uint64_t hash_budget(tgp_server* s) {
   return room;
 }
 
+// Drops one join's reference to the shared table instance it probed; returns an
+// outgrown instance that no join probes any more (to be destroyed outside the lock).
+tgp_hash_table* drop_ref(tgp_server* s, SharedTable* st, tgp_hash_table* t) {  // holds s->mu
+  if (t == st->table) {
+    st->refs -= 1;
+    return nullptr;
+  }
+  for (size_t i = 0; i < s->retired.size(); ++i)  // outgrown while the join probed it
+    if (s->retired[i].first == t && --s->retired[i].second == 0) {
+      s->retired.erase(s->retired.begin() + (long)i);
+      return t;
+    }
+  return nullptr;
+}
+
+// A probe batch has ended: one copy of all its results, split per request.
+int32_t harvest_probe_batch(tgp_server* s, JoinSlot& sl) {
+  uint64_t total = 0;
+  for (Request* r : sl.batch) total += r->words;
+  sl.h_res.resize(total);
+  int32_t rc = TGP_OK;
+  if (hipMemcpyAsync(sl.h_res.data(), sl.d_res, total * 8, hipMemcpyDeviceToHost,
+                     sl.stream) != hipSuccess ||
+      hipStreamSynchronize(sl.stream) != hipSuccess) {
+    set_error("serve: join result copy failed");
+    rc = TGP_ERR_HIP;
+  }
+  std::vector<tgp_hash_table*> free_now;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    uint64_t off = 0;
+    for (size_t i = 0; i < sl.batch.size(); ++i) {
+      Request* r = sl.batch[i];
+      r->result.assign(sl.h_res.begin() + (long)off, sl.h_res.begin() + (long)(off + r->words));
+      off += r->words;
+      if (tgp_hash_table* t = drop_ref(s, sl.holds[i].first, sl.holds[i].second))
+        free_now.push_back(t);
+      finish(s, r, rc);
+    }
+    sl.batch.clear();
+    sl.holds.clear();
+    sl.busy = false;
+    s->done_cv.notify_all();
+  }
+  for (tgp_hash_table* t : free_now) (void)tgp_hash_table_destroy(s->ctx, t);
+  return rc;
+}
+
+// Probes every join of the slot's batch in one tgp_hash_probe_agg_batch launch.
+int32_t start_probe_batch(tgp_server* s, JoinSlot& sl) {
+  uint64_t total = 0;
+  const int64_t t0 = now_ns();
+  for (Request* r : sl.batch) {
+    r->t_start = t0;
+    total += r->words;
+  }
+  if (sl.d_cap < total) {
+    if (sl.d_res) TGP_TRY(hbm_free(s->ctx, sl.d_res));
+    sl.d_res = nullptr;
+    sl.d_cap = 0;
+    void* p = nullptr;
+    const uint64_t want = std::max<uint64_t>(total, 2 * sl.d_cap);
+    TGP_TRY(hbm_alloc(s->ctx, want * 8, &p));
+    sl.d_res = static_cast<uint64_t*>(p);
+    sl.d_cap = want;
+  }
+  std::vector<tgp_join_agg_query> qs(sl.batch.size());
+  std::vector<tgp_group_result> outs(sl.batch.size());
+  uint64_t off = 0;
+  for (size_t i = 0; i < sl.batch.size(); ++i) {
+    const Request* r = sl.batch[i];
+    const tgp_serve_join& J = r->req.join;
+    qs[i] = J.probe;
+    qs[i].table = sl.holds[i].second;
+    qs[i].build_rows = (uint32_t)J.build_keys.len;
+    qs[i].ngroups = r->ngroups;
+    uint64_t* base = sl.d_res + off;
+    outs[i] = tgp_group_result{base, base + r->ngroups, base + r->ngroups * 3};
+    off += r->words;
+  }
+  TGP_TRY(tgp_hash_probe_agg_batch(s->ctx, qs.data(), (uint32_t)qs.size(), outs.data(),
+                                   sl.stream));
+  TGP_HIP(hipEventRecord(sl.done_ev, sl.stream));
+  return TGP_OK;
+}
+
 int32_t harvest_join(tgp_server* s, JoinSlot& sl) {
+  if (!sl.batch.empty()) return harvest_probe_batch(s, sl);
   Request* r = sl.req;
   r->result.resize(r->words);
   int32_t rc = TGP_OK;
@@ -417,18 +512,7 @@ int32_t harvest_join(tgp_server* s, JoinSlot& sl) {
   {
     std::lock_guard<std::mutex> lk(s->mu);
     s->hash_in_use -= sl.table_bytes;
-    if (sl.shared) {
-      if (sl.shared_table == sl.shared->table) {
-        sl.shared->refs -= 1;
-      } else {  // the table was outgrown while this join probed it
-        for (size_t i = 0; i < s->retired.size(); ++i)
-          if (s->retired[i].first == sl.shared_table && --s->retired[i].second == 0) {
-            free_now = sl.shared_table;
-            s->retired.erase(s->retired.begin() + (long)i);
-            break;
-          }
-      }
-    }
+    if (sl.shared) free_now = drop_ref(s, sl.shared, sl.shared_table);
     sl.shared = nullptr;
     sl.shared_table = nullptr;
     sl.table = nullptr;
@@ -498,7 +582,13 @@ int32_t start_join(tgp_server* s, JoinSlot& sl, Request* r, uint64_t need) {
     // (re)build the build side's shared table over this join's prefix; the tier
     // accounts it from now on, so the reservation taken at admission is released
     tgp_hash_table* t = nullptr;
-    const int32_t rc = tgp_hash_build(s->ctx, &J.build_keys, pay, &t, sl.stream);
+    tgp_column bk = J.build_keys, bp{};
+    bk.len = sl.build_len;
+    if (pay) {
+      bp = *pay;
+      bp.len = sl.build_len;
+    }
+    const int32_t rc = tgp_hash_build(s->ctx, &bk, pay ? &bp : nullptr, &t, sl.stream);
     std::lock_guard<std::mutex> lk(s->mu);
     s->hash_in_use -= sl.table_bytes;
     sl.table_bytes = 0;
@@ -512,7 +602,7 @@ int32_t start_join(tgp_server* s, JoinSlot& sl, Request* r, uint64_t need) {
       else (void)tgp_hash_table_destroy(s->ctx, st->table);
     }
     st->table = t;
-    st->rows = J.build_keys.len;
+    st->rows = sl.build_len;
     st->refs = 0;
     sl.shared = st;
   }
@@ -555,32 +645,66 @@ void join_worker(tgp_server* s, size_t slot) {
       progressed = true;
     }
     Request* r = nullptr;
+    bool new_batch = false;
     uint64_t need = 0;
     if (!sl.busy && s->cfg.share_build_tables) {
       // a join on a build side whose resident table holds its prefix probes it now;
       // one that needs a (longer) table builds it when the budget allows, in arrival
       // order; joins waiting for a build in progress let later ones pass
       std::lock_guard<std::mutex> lk(s->mu);
-      for (auto it = s->join_q.begin(); it != s->join_q.end();) {
+      // every queued join whose resident table holds its prefix: one probe batch (the
+      // one-launch-per-query form: the first such join, alone)
+      for (auto it = s->join_q.begin();
+           it != s->join_q.end() && sl.batch.size() < kMaxProbeBatch;) {
+        SharedTable* st = find_shared(s, (*it)->req.join);
+        if (st && st->table && st->rows >= (*it)->req.join.build_keys.len) {
+          s->stats.admitted_joins += 1;
+          if (s->cfg.per_query_launch) {
+            sl.shared = st;
+            r = *it;
+            s->join_q.erase(it);
+            sl.busy = true;
+            break;
+          }
+          sl.batch.push_back(*it);
+          sl.holds.push_back({st, st->table});
+          st->refs += 1;
+          it = s->join_q.erase(it);
+        } else {
+          ++it;
+        }
+      }
+      if (!sl.batch.empty()) {
+        sl.busy = true;
+        new_batch = true;
+      }
+      for (auto it = s->join_q.begin(); !sl.busy && it != s->join_q.end();) {
         Request* q = *it;
         const tgp_serve_join& J = q->req.join;
         SharedTable* st = find_shared(s, J);
-        if (st && st->table && st->rows >= J.build_keys.len) {
-          s->join_q.erase(it);
-          s->stats.admitted_joins += 1;
-          sl.busy = true;
-          sl.shared = st;
-          r = q;
-          break;
-        }
         if (st && st->building) {
           ++it;
           continue;
+        }
+        // the (re)build covers the longest prefix of this build side that a queued
+        // join asks for, so the joins behind this one need no further build
+        uint64_t len = J.build_keys.len;
+        for (Request* o : s->join_q) {
+          const tgp_serve_join& K = o->req.join;
+          if (K.build_keys.data == J.build_keys.data && K.build_keys.dtype == J.build_keys.dtype &&
+              K.build_payload.data == J.build_payload.data &&
+              (!J.build_payload.data || K.build_payload.dtype == J.build_payload.dtype))
+            len = std::max<uint64_t>(len, K.build_keys.len);
         }
         need = tgp_hash_table_bytes_for(J.build_keys.len);
         uint64_t used = 0, budget = 0, peak = 0;
         (void)tgp_hbm_usage(s->ctx, &used, &budget, &peak);
         const uint64_t cap = s->cfg.hash_budget_bytes ? s->cfg.hash_budget_bytes : budget;
+        if (tgp_hash_table_bytes_for(len) + s->hash_in_use <= hash_budget(s) &&
+            tgp_hash_table_bytes_for(len) <= cap)
+          need = tgp_hash_table_bytes_for(len);
+        else
+          len = J.build_keys.len;
         if (need > cap) {
           it = s->join_q.erase(it);
           s->stats.rejected += 1;
@@ -606,7 +730,8 @@ void join_worker(tgp_server* s, size_t slot) {
           st->kdt = J.build_keys.dtype;
           st->pdt = J.build_payload.dtype;
         }
-        st->building = J.build_keys.len;
+        st->building = len;
+        sl.build_len = len;
         s->join_q.erase(it);
         s->hash_in_use += need;
         s->stats.admitted_joins += 1;
@@ -652,6 +777,26 @@ void join_worker(tgp_server* s, size_t slot) {
         r = front;
         break;
       }
+    }
+    if (new_batch) {
+      const int32_t rc = start_probe_batch(s, sl);
+      if (rc != TGP_OK) {
+        std::vector<tgp_hash_table*> free_now;
+        {
+          std::lock_guard<std::mutex> lk(s->mu);
+          for (size_t i = 0; i < sl.batch.size(); ++i) {
+            if (tgp_hash_table* t = drop_ref(s, sl.holds[i].first, sl.holds[i].second))
+              free_now.push_back(t);
+            finish(s, sl.batch[i], rc);
+          }
+          sl.batch.clear();
+          sl.holds.clear();
+          sl.busy = false;
+          s->done_cv.notify_all();
+        }
+        for (tgp_hash_table* t : free_now) (void)tgp_hash_table_destroy(s->ctx, t);
+      }
+      progressed = true;
     }
     if (r) {
       const int32_t rc = start_join(s, sl, r, need);

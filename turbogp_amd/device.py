@@ -410,6 +410,21 @@ class Context:
                 nulls.free()
                 res.free()
 
+    def probe_agg_batch(self, queries, stream=None):
+        """n probe aggregations (tgp_join_agg_query, e.g. HashTable.agg_query) in one
+        tgp_hash_probe_agg_batch call; returns their GroupAggResults in order."""
+        n = len(queries)
+        bufs = [GroupAggBuffers(self, q.ngroups, 1) for q in queries]
+        try:
+            qa = (A.JoinAggQuery * max(1, n))(*queries)
+            ra = (A.GroupResult * max(1, n))(*[b.desc for b in bufs])
+            A.check(A.lib().tgp_hash_probe_agg_batch(self.handle, qa, n, ra, stream),
+                    "tgp_hash_probe_agg_batch")
+            return [b.fetch(stream) for b in bufs]
+        finally:
+            for b in bufs:
+                b.free()
+
     def group_agg_async(self, q, res, stream=None):
         """Queues tgp_group_agg into preallocated GroupAggBuffers (no sync, no copy)."""
         A.check(A.lib().tgp_group_agg(self.handle, C.byref(q), C.byref(res.desc), stream),
@@ -767,11 +782,9 @@ class HashTable:
             raise
         return bm
 
-    def probe_agg(self, keys, value, group_by, ngroups=1, key_base=0, group_col=None,
-                  row_begin=0, row_end=None, result=None, stream=None, fetch=True,
-                  build_rows=0):
-        """Fused probe -> GROUP BY -> SUM(value), COUNT(*); build_rows n > 0 joins
-        with the first n build rows only (a table built over more of the build side)."""
+    def agg_query(self, keys, value, group_by, ngroups=1, key_base=0, group_col=None,
+                  row_begin=0, row_end=None, build_rows=0):
+        """The tgp_join_agg_query of a probe aggregation over this table."""
         q = A.JoinAggQuery()
         q.table = self.handle
         q.build_rows = build_rows
@@ -784,6 +797,15 @@ class HashTable:
         q.ngroups = ngroups
         q.row_begin = row_begin
         q.row_end = keys.n if row_end is None else row_end
+        return q
+
+    def probe_agg(self, keys, value, group_by, ngroups=1, key_base=0, group_col=None,
+                  row_begin=0, row_end=None, result=None, stream=None, fetch=True,
+                  build_rows=0):
+        """Fused probe -> GROUP BY -> SUM(value), COUNT(*); build_rows n > 0 joins
+        with the first n build rows only (a table built over more of the build side)."""
+        q = self.agg_query(keys, value, group_by, ngroups, key_base, group_col, row_begin,
+                           row_end, build_rows)
         res = result or GroupAggBuffers(self.ctx, ngroups, 1)
         A.check(A.lib().tgp_hash_probe_agg(self.ctx.handle, C.byref(q), C.byref(res.desc),
                                            stream), "tgp_hash_probe_agg")
