@@ -319,7 +319,7 @@ __device__ __forceinline__ uint32_t shared_select(ConstShared& D, const SharedRo
 // compile-time offsets (lane l on copy l mod 2^RS); a slice holding a big value
 // adds that aggregate straight into the int128 result.
 template <int RS>
-__device__ __forceinline__ void shared_accumulate(const EntryOut& O, const SharedRows& v,
+__device__ __forceinline__ void shared_accumulate(ConstShared& D, uint64_t kchunk, const SharedRows& v,
                                                   uint32_t sel, const uint32_t (&gid)[kSR],
                                                   unsigned long long* reg) {
   constexpr int W1 = 5;
@@ -348,6 +348,7 @@ __device__ __forceinline__ void shared_accumulate(const EntryOut& O, const Share
         if ((sel >> r) & 1) {
           const uint64_t g = gid[r] / (uint32_t)(W1 << RS);
           const int64_t x = (int64_t)v.val[a][r];
+          const EntryOut O = entry_out(D, kchunk);
           atomic_add128_split(O.sums + (g * 4 + a) * 2 * O.st, O.st, (uint64_t)x, x < 0 ? -1 : 0);
         }
     } else {
@@ -359,14 +360,24 @@ __device__ __forceinline__ void shared_accumulate(const EntryOut& O, const Share
   }
 }
 
-// shared_select for key form i / 7 and 2^(i % 7) copies, and shared_accumulate for
-// 2^rs copies, through switches (jump tables on the wave-uniform index).
-__device__ __forceinline__ uint32_t shared_select_dispatch(int i, ConstShared& D,
-                                                           const SharedRows& v, uint64_t row0,
-                                                           uint64_t row, uint32_t (&gid)[kSR],
-                                                           uint32_t& bad) {
+// One query of the chunk on one slice: both halves, and the out-of-range count.
+template <int KIND, int RS>
+__device__ __forceinline__ void shared_query(ConstShared& D, uint64_t kchunk, const SharedRows& v,
+                                             uint64_t row0, uint64_t row,
+                                             unsigned long long* reg) {
+  uint32_t gid[kSR], bad = 0;
+  const uint32_t sel = shared_select<KIND, RS>(D, v, row0, row, gid, bad);
+  if (bad) atomicAdd(entry_out(D, kchunk).oor, (unsigned long long)__popc(bad));
+  if (__ballot(sel != 0)) shared_accumulate<RS>(D, kchunk, v, sel, gid, reg);
+}
+
+// shared_query for key form i / 7 and 2^(i % 7) copies through one switch (a jump
+// table on the wave-uniform index, not a chain of compares).
+__device__ __forceinline__ void shared_query_dispatch(int i, ConstShared& D, uint64_t kchunk,
+                                                      const SharedRows& v, uint64_t row0,
+                                                      uint64_t row, unsigned long long* reg) {
 #define TGP_SQ_CASE(I) \
-  case I: return shared_select<(I) / 7, (I) % 7>(D, v, row0, row, gid, bad);
+  case I: shared_query<(I) / 7, (I) % 7>(D, kchunk, v, row0, row, reg); break;
 #define TGP_SQ_KIND(K) \
   TGP_SQ_CASE(7 * (K)) TGP_SQ_CASE(7 * (K) + 1) TGP_SQ_CASE(7 * (K) + 2) TGP_SQ_CASE(7 * (K) + 3) \
   TGP_SQ_CASE(7 * (K) + 4) TGP_SQ_CASE(7 * (K) + 5) TGP_SQ_CASE(7 * (K) + 6)
@@ -376,24 +387,6 @@ __device__ __forceinline__ uint32_t shared_select_dispatch(int i, ConstShared& D
   }
 #undef TGP_SQ_KIND
 #undef TGP_SQ_CASE
-  bad = 0;
-  return 0;
-}
-
-__device__ __forceinline__ void shared_accumulate_dispatch(int rs, const EntryOut& O,
-                                                           const SharedRows& v, uint32_t sel,
-                                                           const uint32_t (&gid)[kSR],
-                                                           unsigned long long* reg) {
-  switch (rs) {
-    case 0: shared_accumulate<0>(O, v, sel, gid, reg); break;
-    case 1: shared_accumulate<1>(O, v, sel, gid, reg); break;
-    case 2: shared_accumulate<2>(O, v, sel, gid, reg); break;
-    case 3: shared_accumulate<3>(O, v, sel, gid, reg); break;
-    case 4: shared_accumulate<4>(O, v, sel, gid, reg); break;
-    case 5: shared_accumulate<5>(O, v, sel, gid, reg); break;
-    case 6: shared_accumulate<6>(O, v, sel, gid, reg); break;
-    default: break;
-  }
 }
 
 // items: 4 words each (first absolute slice, slices, first entry, entries);
@@ -465,25 +458,12 @@ __global__ void __launch_bounds__(kBlock) k_batch_shared(SharedFamily F,
           sr.big[a] = __ballot(big_any(m)) != 0;
           sr.any_big = sr.any_big || sr.big[a];
         });
-        // a query whose date bounds and key parts equal the previous one's (same
-        // signature and copies) and that covers the slice, as the previous did,
-        // takes its selection and group ids as they are
-        uint32_t c_sel = 0, c_bad = 0, c_gid[kSR] = {0, 0, 0, 0};
-        uint32_t c_key = ~0u;
         for (uint32_t e = e0; e < e0 + ne; ++e) {
           const uint32_t q = en[2 * e], pack = en[2 * e + 1];
           ConstShared& D = qs[q];
           if (row0 >= D.row_end || row0 + kSliceRows <= D.row_begin) continue;
-          const int rs = (int)(pack >> 16);
-          const bool full = row0 >= D.row_begin && row0 + kSliceRows <= D.row_end;
-          const uint32_t key = full ? ((D.sig << 3) | (uint32_t)rs) : ~0u;
-          if (key == ~0u || key != c_key)
-            c_sel = shared_select_dispatch(D.kind * 7 + rs, D, sr, row0, row, c_gid, c_bad);
-          c_key = key;
-          const EntryOut O = entry_out(D, kchunk);
-          if (c_bad) atomicAdd(O.oor, (unsigned long long)__popc(c_bad));
-          if (__ballot(c_sel != 0))
-            shared_accumulate_dispatch(rs, O, sr, c_sel, c_gid, region + (pack & 0xffffu));
+          shared_query_dispatch(D.kind * 7 + (int)(pack >> 16), D, kchunk, sr, row0, row,
+                                region + (pack & 0xffffu));
         }
       }
       __syncthreads();
@@ -523,13 +503,13 @@ struct PartialGroup {
 };
 
 // Turns every partial row into its running sums: position p then holds the sum of
-// chunks [0, p).  One workgroup per (group, result word); sum words are int128
-// (row pairs), count and out-of-range words 64-bit.
+// chunks [0, p).  One workgroup per (group, result word) (units[blockIdx.x] = group |
+// word << 16); sum words are int128 (row pairs), count and out-of-range words 64-bit.
 __global__ void __launch_bounds__(256) k_partials_scan(const PartialGroup* __restrict__ pg,
+                                                       const uint32_t* __restrict__ units,
                                                        uint32_t naggs) {
-  const PartialGroup P = pg[blockIdx.x];
-  const uint32_t G = P.groups, units = G + G * naggs + 1, u = blockIdx.y;
-  if (u >= units) return;
+  const PartialGroup P = pg[units[blockIdx.x] & 0xffffu];
+  const uint32_t G = P.groups, u = units[blockIdx.x] >> 16;
   const bool wide = u >= G && u < G + G * naggs;
   const uint32_t row = u < G ? u : wide ? G + 2 * (u - G) : G + 2 * G * naggs;
   unsigned long long* lo_row = P.rows + (uint64_t)row * P.nk1;
@@ -665,8 +645,9 @@ struct tgp_batch {
   PartialGroup* d_pgroups = nullptr;
   PartialLink* d_plinks = nullptr;
   uint64_t* d_partials = nullptr;
-  uint64_t pgroups_cap = 0, plinks_cap = 0, partials_cap = 0, partial_words = 0;
-  uint32_t pmax_units = 0;
+  std::vector<uint32_t> punits;  // per scan workgroup: group | result word << 16
+  uint32_t* d_punits = nullptr;
+  uint64_t pgroups_cap = 0, plinks_cap = 0, partials_cap = 0, partial_words = 0, punits_cap = 0;
 };
 
 // Grows a device buffer to at least `need` elements.  The old buffer is not
@@ -736,8 +717,8 @@ static int32_t launch_variant(tgp_batch* b, hipStream_t s) {
                                                           b->d_sentries, SQ, counter + kQueues);
       TGP_HIP(hipGetLastError());
       if (!b->pgroups.empty()) {
-        k_partials_scan<<<dim3((unsigned)b->pgroups.size(), b->pmax_units), dim3(256), 0, s>>>(
-            b->d_pgroups, (uint32_t)kShapes[0].naggs);
+        k_partials_scan<<<dim3((unsigned)b->punits.size()), dim3(256), 0, s>>>(
+            b->d_pgroups, b->d_punits, (uint32_t)kShapes[0].naggs);
         TGP_HIP(hipGetLastError());
         k_partials_links<<<dim3((unsigned)b->plinks.size()), dim3(256), 0, s>>>(
             b->d_plinks, (uint32_t)kShapes[0].naggs);
@@ -789,7 +770,7 @@ static int32_t build_shared_items(tgp_batch* b, hipStream_t s) {
   b->sfam = b->fam;
   b->pgroups.clear();
   b->plinks.clear();
-  b->pmax_units = 0;
+  b->punits.clear();
   std::vector<int32_t> grp(nf, -1);
   std::vector<std::vector<uint8_t>> gcover;
   std::vector<uint64_t> goff;
@@ -798,11 +779,21 @@ static int32_t build_shared_items(tgp_batch* b, hipStream_t s) {
   for (auto& kv : by_sig) {
     const std::vector<uint32_t>& ms = kv.second;
     if (ms.size() < 2) continue;
-    uint64_t F0 = ~0ull, F1 = 0;
+    if (b->pgroups.size() >= 0xffffu) break;  // group indices take 16 bits
+    uint64_t F0 = ~0ull, F1 = 0, covered = 0;
     for (uint32_t m : ms) {
       F0 = std::min(F0, f0[m]);
       F1 = std::max(F1, f1[m]);
+      covered += f1[m] - f0[m];
     }
+    // worth it when the members cover their chunks twice over on average: the
+    // partials query then scans each chunk once instead of twice or more
+    std::vector<uint8_t> cover(F1 - F0, 0);
+    for (uint32_t m : ms)
+      for (uint64_t k = f0[m]; k < f1[m]; ++k) cover[k - F0] = 1;
+    uint64_t uni = 0;
+    for (uint8_t c : cover) uni += c;
+    if (covered < 2 * uni) continue;
     const uint32_t G = (uint32_t)b->fam[ms[0]].words / (1 + NA);
     const uint64_t nk1 = F1 - F0 + 1;
     SharedQuery v = b->fam[ms[0]];
@@ -815,11 +806,10 @@ static int32_t build_shared_items(tgp_batch* b, hipStream_t s) {
     b->sfam.push_back(v);
     goff.push_back(pw);
     b->pgroups.push_back(PartialGroup{nullptr, (uint32_t)nk1, G});
-    b->pmax_units = std::max(b->pmax_units, G + G * NA + 1);
-    std::vector<uint8_t> cover(F1 - F0, 0);
+    for (uint32_t u = 0; u < G + G * NA + 1; ++u)
+      b->punits.push_back(((uint32_t)b->pgroups.size() - 1) | (u << 16));
     for (uint32_t m : ms) {
       grp[m] = vi;
-      for (uint64_t k = f0[m]; k < f1[m]; ++k) cover[k - F0] = 1;
       const SharedQuery& M = b->fam[m];
       b->plinks.push_back(PartialLink{nullptr, M.out_count, M.out_sums, M.out_oor,
                                       (uint32_t)nk1, G, (uint32_t)(f0[m] - F0),
@@ -975,6 +965,9 @@ static int32_t build_shared_items(tgp_batch* b, hipStream_t s) {
     TGP_HIP(hipMemcpyAsync(b->d_pgroups, b->pgroups.data(),
                            b->pgroups.size() * sizeof(PartialGroup), hipMemcpyHostToDevice, s));
     TGP_HIP(hipMemcpyAsync(b->d_plinks, b->plinks.data(), b->plinks.size() * sizeof(PartialLink),
+                           hipMemcpyHostToDevice, s));
+    TGP_TRY(ensure_cap(b, &b->d_punits, &b->punits_cap, (uint64_t)b->punits.size()));
+    TGP_HIP(hipMemcpyAsync(b->d_punits, b->punits.data(), b->punits.size() * 4,
                            hipMemcpyHostToDevice, s));
   }
   TGP_TRY(ensure_cap(b, &b->d_sitems, &b->sitems_cap, (uint64_t)items.size()));
@@ -1139,6 +1132,7 @@ int32_t tgp_batch_destroy(tgp_batch* b) {
   if (b->d_sentries) hbm_free(b->ctx, b->d_sentries);
   if (b->d_pgroups) hbm_free(b->ctx, b->d_pgroups);
   if (b->d_plinks) hbm_free(b->ctx, b->d_plinks);
+  if (b->d_punits) hbm_free(b->ctx, b->d_punits);
   if (b->d_partials) hbm_free(b->ctx, b->d_partials);
   for (void* p : b->retired) hbm_free(b->ctx, p);
   delete b;
