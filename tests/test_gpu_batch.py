@@ -250,3 +250,45 @@ def test_shared_scan_signature_groups(ctx, table):
     finally:
         for c in dev2.values():
             c.free()
+
+
+def test_shared_scan_first_value_packing(ctx, table):
+    """The shared scan adds the row count and small first values (qty in [0, 2^16))
+    with one atomic; slices whose first value is wider (at least 2^16, negative, or
+    at least 2^43, the int128 path) add it on its own.  A table copy with each kind
+    of qty block, in ranges that cross the blocks, equals single-query launches and
+    the oracle."""
+    host, dev = table
+    n2 = 400_000
+    wide = {k: v[:n2].copy() for k, v in host.items()}
+    q = wide["qty"]
+    q[:100_000] += 70_000
+    q[100_000:200_000] *= -1
+    q[300_000:] += 1 << 44
+    q[250_000:250_300] += 1 << 16          # one slice-sized run just past the packed range
+    dev2 = {k: DeviceColumn.from_numpy(ctx, v) for k, v in wide.items()}
+    try:
+        spec = [(0, n2, 2000, "ab"), (50_000, 150_000, 2525, "a"), (190_000, 310_000, 1500, "b"),
+                (240_000, 260_000, 2400, "month"), (0, n2, 2525, ""), (99_000, 101_000, 2525, "ab")]
+        qs = W.batch_queries(dev2, spec) + W.batch_queries(dev2, spec)  # twice: a signature group
+        b = Batch(ctx, len(qs))
+        try:
+            b.set_queries(qs)
+            got = []
+            for chunk in (1, 16, 4096):
+                b.set_tuning(0, use_graph=True, chunk_slices=chunk, order=3, cached_loads=True)
+                b.run(A.BATCH_PERSISTENT)
+                got.append(b.results())
+        finally:
+            b.close()
+        for i, qq in enumerate(qs):
+            want = ctx.group_agg(qq)
+            assert all(same(g[i], want) for g in got), i
+        for j, (bb, e, c, g) in enumerate(spec):
+            nm, pr, pa, ag = W.q1_spec(date_cutoff=c, group=g)
+            cnt, sums, oor = B.group_agg(B.make_query([wide[x] for x in nm], pr, pa, ag, bb, e))
+            r = got[0][j]
+            assert [int(x) for x in r.count] == [int(x) for x in cnt] and r.sums() == sums, j
+    finally:
+        for c in dev2.values():
+            c.free()

@@ -237,7 +237,16 @@ struct SharedRows {
   uint64_t val[4][kSR];          // the four summed values
   bool big[4];                   // wave-uniform: the slice has |value| >= 2^43
   bool any_big;
+  bool small0;                   // wave-uniform: every first value is in [0, 2^16)
 };
+
+// Word 0 of a group holds its row count in bits 40..63 and, for slices whose first
+// value lies in [0, 2^16), the sum of those values in bits 0..39 (one atomic for
+// both): a region sees at most 2^20 rows, so the count stays below 2^24 and the
+// packed sum below 2^36.  Other slices add their first value to word 1.
+constexpr unsigned long long kCountOne = 1ull << 40;
+static_assert((uint64_t)kSharedMaxSlices * kSliceRows <= (1ull << 20),
+              "a region's count and packed sum need chunks of at most 2^20 rows");
 
 // Adds (lo, hi) to an int128 whose words are `st` words apart (64-bit atomics;
 // the carries out of the low word are counted exactly, as in atomic_add128).
@@ -324,12 +333,24 @@ __device__ __forceinline__ void shared_accumulate(ConstShared& D, uint64_t kchun
                                                   unsigned long long* reg) {
   constexpr int W1 = 5;
   unsigned long long* mine = reg + ((threadIdx.x & 63) & ((1u << RS) - 1));
+  if (!v.any_big && v.small0) {  // four atomics: the count shares word 0 with value 0
+#pragma unroll
+    for (int r = 0; r < kSR; ++r)
+      if ((sel >> r) & 1) {
+        unsigned long long* p = mine + gid[r];
+        atomicAdd(p, kCountOne + (unsigned long long)v.val[0][r]);
+        atomicAdd(p + (2 << RS), (unsigned long long)v.val[1][r]);
+        atomicAdd(p + (3 << RS), (unsigned long long)v.val[2][r]);
+        atomicAdd(p + (4 << RS), (unsigned long long)v.val[3][r]);
+      }
+    return;
+  }
   if (!v.any_big) {
 #pragma unroll
     for (int r = 0; r < kSR; ++r)
       if ((sel >> r) & 1) {
         unsigned long long* p = mine + gid[r];
-        atomicAdd(p, 1ull);
+        atomicAdd(p, kCountOne);
         atomicAdd(p + (1 << RS), (unsigned long long)v.val[0][r]);
         atomicAdd(p + (2 << RS), (unsigned long long)v.val[1][r]);
         atomicAdd(p + (3 << RS), (unsigned long long)v.val[2][r]);
@@ -339,7 +360,7 @@ __device__ __forceinline__ void shared_accumulate(ConstShared& D, uint64_t kchun
   }
 #pragma unroll
   for (int r = 0; r < kSR; ++r)
-    if ((sel >> r) & 1) atomicAdd(mine + gid[r], 1ull);
+    if ((sel >> r) & 1) atomicAdd(mine + gid[r], kCountOne);
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
     if (v.big[a]) {
@@ -458,6 +479,12 @@ __global__ void __launch_bounds__(kBlock) k_batch_shared(SharedFamily F,
           sr.big[a] = __ballot(big_any(m)) != 0;
           sr.any_big = sr.any_big || sr.big[a];
         });
+        {
+          bool small = true;
+#pragma unroll
+          for (int r = 0; r < kSR; ++r) small = small && sr.val[0][r] < (1ull << 16);
+          sr.small0 = __ballot(!small) == 0;
+        }
         for (uint32_t e = e0; e < e0 + ne; ++e) {
           const uint32_t q = en[2 * e], pack = en[2 * e + 1];
           ConstShared& D = qs[q];
@@ -482,9 +509,12 @@ __global__ void __launch_bounds__(kBlock) k_batch_shared(SharedFamily F,
           }
           if (!x) continue;
           const uint32_t g = w / W1, k = w % W1;
-          if (k == 0)
-            atomicAdd(O.count + (uint64_t)g * O.st, x);
-          else
+          if (k == 0) {  // the count, and the packed part of value 0's sum
+            if (x >> 40) atomicAdd(O.count + (uint64_t)g * O.st, x >> 40);
+            const uint64_t packed = x & (kCountOne - 1);
+            if (packed)
+              atomic_add128_split(O.sums + (uint64_t)g * S.naggs * 2 * O.st, O.st, packed, 0);
+          } else
             atomic_add128_split(O.sums + ((uint64_t)g * S.naggs + (k - 1)) * 2 * O.st, O.st, x,
                                 (int64_t)x < 0 ? -1 : 0);
         }
