@@ -39,11 +39,11 @@ def test_batch_matches_single_queries_and_oracle(ctx, table):
     host, dev = table
     p = params(300)
     qs = W.batch_queries(dev, p)
-    # one query with a shape that has no batch kernel: it runs alone (fallback)
+    # one query with a shape that has no batch kernel: a generic item of the launch
     names6, preds6, parts6, aggs6 = W.q6_spec()
     qs.append(make_agg_query([dev[c] for c in names6], preds6 + [(2, A.NE, 7, True)], parts6,
                              aggs6 + [("count_nonzero", 1)], 10, 2_000_000))
-    # MIN/MAX aggregates run in the generic engine, also alone
+    # MIN/MAX aggregates: generic items too
     names1, preds1, parts1, aggs1 = W.q1_spec()
     qs.append(make_agg_query([dev[c] for c in names1], preds1, parts1,
                              [("min", 0), ("max_u64", 1), ("max", 1, [(1, 0, 1), (2, 100, -1)])],
@@ -52,7 +52,7 @@ def test_batch_matches_single_queries_and_oracle(ctx, table):
     try:
         b.set_queries(qs)
         info = b.info()
-        assert info["batched"] == len(qs) - 2 and info["fallback"] == 2
+        assert info["batched"] == len(qs) and info["fallback"] == 0
         b.run(A.BATCH_PERSISTENT)
         persistent = b.results()
         b.run(A.BATCH_PERSISTENT)          # graph replay
@@ -292,3 +292,89 @@ def test_shared_scan_first_value_packing(ctx, table):
     finally:
         for c in dev2.values():
             c.free()
+
+
+def test_mixed_shape_batch_has_no_fallback(ctx, table):
+    """A 4096-query batch of every query form: the six compiled shapes, Q6 with a fifth
+    predicate, MIN / MAX, an input selection bitmap, a price column of another width,
+    group counts beyond the LDS table (straight into the result) and empty ranges.
+    All run in the batch launch (no query runs alone) and every result equals its
+    one-launch-per-query result, bit for bit; a sample equals the oracle."""
+    from turbogp_amd import DeviceBitmap
+    host, dev = table
+    rng = np.random.default_rng(5)
+    price32 = DeviceColumn.from_numpy(ctx, host["price"].astype(np.int32))
+    words = np.zeros((N + 63) // 64, np.uint64)
+    bits = rng.random(N) < 0.3
+    idx = np.flatnonzero(bits)
+    np.bitwise_or.at(words, idx >> 6, np.left_shift(np.uint64(1), (idx & 63).astype(np.uint64)))
+    names1, preds1, parts1, aggs1 = W.q1_spec()
+    names6, preds6, parts6, aggs6 = W.q6_spec()
+    specs, hostq = [], []
+    for k in range(4096):
+        L = int(np.exp(rng.uniform(0, np.log(N))))
+        b0 = int(rng.integers(0, N - L + 1))
+        e0 = b0 + L if k % 97 else b0          # some empty ranges
+        form = k % 8
+        if form in (0, 1):
+            c, g = int(rng.integers(0, 2526)), ["ab", "a", "b", "month", ""][k % 5]
+            nm, pr, pa, ag = W.q1_spec(date_cutoff=c, group=g)
+            cols, sel = [dev[x] for x in nm], None
+        elif form == 2:
+            nm, pr, pa, ag = names6, preds6, parts6, aggs6
+            cols, sel = [dev[x] for x in nm], None
+        elif form == 3:                        # five predicates
+            nm, pr, pa, ag = names6, preds6 + [(2, A.NE, int(rng.integers(1, 50)), True)], \
+                parts6, aggs6
+            cols, sel = [dev[x] for x in nm], None
+        elif form == 4:                        # MIN / MAX next to sums
+            nm, pr, pa = names1, preds1, parts1
+            ag = [("min", 2), ("max_u64", 1), ("sum", 1, [(2, 0, 1)]),
+                  ("max", 1, [(1, 0, 1), (3, 100, -1)])]
+            cols, sel = [dev[x] for x in nm], None
+        elif form == 5:                        # an input selection bitmap over [0, N)
+            nm, pr, pa, ag = names1, preds1, parts1, aggs1
+            cols, sel, b0, e0 = [dev[x] for x in nm], "bitmap", 0, N
+        elif form == 6:                        # price as int32: another column layout
+            nm, pr, pa, ag = names1, preds1, parts1, aggs1
+            cols = [dev[x] if x != "price" else price32 for x in nm]
+            sel = None
+        else:                                  # 2526 x 2 groups: beyond the LDS table
+            nm, pr, ag = names1, preds1, aggs1
+            pa = [(0, 1, 0, 2526), (6, 1, 0, 2)]
+            cols, sel = [dev[x] for x in nm], None
+        specs.append((cols, pr, pa, ag, b0, e0, sel))
+        hostq.append((nm, pr, pa, ag, b0, e0, sel, form))
+    bm = DeviceBitmap(ctx, N)
+    bm.buf.upload(words)
+    try:
+        qs = [make_agg_query(c, pr, pa, ag, b0, e0, bm if sel else None)
+              for c, pr, pa, ag, b0, e0, sel in specs]
+        b = Batch(ctx, len(qs))
+        try:
+            b.set_queries(qs)
+            info = b.info()
+            assert info["batched"] == len(qs) and info["fallback"] == 0, info
+            b.set_tuning(0, use_graph=True, chunk_slices=64, order=3, cached_loads=True)
+            b.run(A.BATCH_PERSISTENT)
+            batched = b.results()
+            b.run(A.BATCH_PERSISTENT)
+            replay = b.results()
+            b.run(A.BATCH_PER_QUERY)
+            single = b.results()
+        finally:
+            b.close()
+        for i in range(len(qs)):
+            assert same(batched[i], single[i]) and same(replay[i], single[i]), (i, hostq[i][7])
+        for i in list(range(0, 64, 3)) + [8 * j + 7 for j in range(3)]:
+            nm, pr, pa, ag, b0, e0, sel, form = hostq[i]
+            arrays = [host[x].astype(np.int32) if (form == 6 and x == "price") else host[x]
+                      for x in nm]
+            cnt, sums, oor = B.group_agg(B.make_query(arrays, pr, pa, ag, b0, e0,
+                                                      words if sel else None))
+            r = batched[i]
+            assert [int(x) for x in r.count] == [int(x) for x in cnt], (i, form)
+            assert r.sums() == sums and r.n_out_of_range == oor, (i, form)
+    finally:
+        bm.free()
+        price32.free()

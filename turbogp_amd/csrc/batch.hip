@@ -628,7 +628,7 @@ struct tgp_batch {
   std::vector<tgp_agg_query> queries;
   std::vector<BatchQuery> plan;
   std::vector<uint64_t> count_off, sums_off, oor_off, ngroups;
-  std::vector<int32_t> fallback;  // queries without a batch shape (run alone)
+  std::vector<int32_t> fallback;  // set_queries: the queries of no batch shape (generic items)
   uint64_t result_words = 0, total_slices = 0;
   // device
   BatchQuery* d_plan = nullptr;
@@ -676,6 +676,15 @@ struct tgp_batch {
   PartialLink* d_plinks = nullptr;
   uint64_t* d_partials = nullptr;
   std::vector<uint32_t> punits;  // per scan workgroup: group | result word << 16
+  // generic items: the queries of no batch shape, through the tile engine
+  std::vector<DevPlan> gplans;
+  std::vector<GenericItem> gitems;
+  std::vector<uint32_t> gminmax;  // plans with MIN / MAX aggregates
+  DevPlan* d_gplans = nullptr;
+  GenericItem* d_gitems = nullptr;
+  uint32_t* d_gminmax = nullptr;
+  uint64_t gplans_cap = 0, gitems_cap = 0, gminmax_cap = 0, gminmax_words = 0;
+  uint32_t gstage_max = 0, gtable_max = 0;
   uint32_t* d_punits = nullptr;
   uint64_t pgroups_cap = 0, plinks_cap = 0, partials_cap = 0, partial_words = 0, punits_cap = 0;
 };
@@ -1035,6 +1044,62 @@ static int32_t build_granules(tgp_batch* b, hipStream_t s) {
   return TGP_OK;
 }
 
+// The queries of no batch shape run as generic items of kGenericTiles tiles in one
+// more persistent launch: their plans (results in the arena), items and the LDS
+// they need (two stage buffers; a table for every plan whose groups fit beside
+// them, the others add straight into the result).
+constexpr uint32_t kGenericTiles = 16;
+constexpr uint32_t kGenericLds = 160 * 1024 - (uint32_t)sizeof(DevPlan) - 1024;
+
+static int32_t plan_generic(tgp_batch* b, const tgp_agg_query* qs, hipStream_t s) {
+  b->gplans.clear();
+  b->gitems.clear();
+  b->gminmax.clear();
+  b->gstage_max = b->gtable_max = 0;
+  b->gminmax_words = 0;
+  for (int32_t i : b->fallback) {
+    DevPlan P;
+    PlanInfo info;
+    TGP_TRY(plan_agg(&qs[i], &P, &info));
+    P.out_count = reinterpret_cast<unsigned long long*>(b->d_results + b->count_off[i]);
+    P.out_sums = reinterpret_cast<unsigned long long*>(b->d_results + b->sums_off[i]);
+    P.out_oor = reinterpret_cast<unsigned long long*>(b->d_results + b->oor_off[i]);
+    b->gstage_max = std::max(b->gstage_max, P.stage_bytes);
+    b->gplans.push_back(P);
+  }
+  for (uint32_t q = 0; q < b->gplans.size(); ++q) {
+    const DevPlan& P = b->gplans[q];
+    const uint64_t tb = P.ngroups * (1 + 2 * (uint64_t)P.naggs) * 8;
+    const bool lds = 2ull * b->gstage_max + tb <= kGenericLds;
+    if (lds) b->gtable_max = std::max<uint32_t>(b->gtable_max, (uint32_t)tb);
+    if (P.minmax_kinds && P.ngroups && P.naggs) {
+      b->gminmax.push_back(q);
+      b->gminmax_words = std::max(b->gminmax_words, P.ngroups * (uint64_t)P.naggs);
+    }
+    for (uint64_t t = 0; t < P.ntiles; t += kGenericTiles)
+      b->gitems.push_back(GenericItem{q, (uint32_t)t,
+                                      (uint32_t)std::min<uint64_t>(t + kGenericTiles, P.ntiles),
+                                      lds ? 1u : 0u});
+  }
+  b->fallback.clear();  // every query runs in the batch launch
+  if (b->gplans.empty()) return TGP_OK;
+  TGP_TRY(ensure_cap(b, &b->d_gplans, &b->gplans_cap, (uint64_t)b->gplans.size()));
+  TGP_HIP(hipMemcpyAsync(b->d_gplans, b->gplans.data(), b->gplans.size() * sizeof(DevPlan),
+                         hipMemcpyHostToDevice, s));
+  if (!b->gitems.empty()) {
+    TGP_TRY(ensure_cap(b, &b->d_gitems, &b->gitems_cap, (uint64_t)b->gitems.size()));
+    TGP_HIP(hipMemcpyAsync(b->d_gitems, b->gitems.data(), b->gitems.size() * sizeof(GenericItem),
+                           hipMemcpyHostToDevice, s));
+  }
+  if (!b->gminmax.empty()) {
+    TGP_TRY(ensure_cap(b, &b->d_gminmax, &b->gminmax_cap, (uint64_t)b->gminmax.size()));
+    TGP_HIP(hipMemcpyAsync(b->d_gminmax, b->gminmax.data(), b->gminmax.size() * 4,
+                           hipMemcpyHostToDevice, s));
+  }
+  TGP_HIP(hipStreamSynchronize(s));
+  return TGP_OK;
+}
+
 static int32_t build_items(tgp_batch* b, hipStream_t s) {
   if (b->order == 0 && !b->granule_valid) TGP_TRY(build_granules(b, s));
   if (b->order == 3 && !(b->sitems_valid && b->sitems_chunk == b->chunk))
@@ -1131,7 +1196,14 @@ static int32_t build_items(tgp_batch* b, hipStream_t s) {
 
 static int32_t batch_launch(tgp_batch* b, hipStream_t s) {
   // results and the counters behind them start at zero
-  TGP_TRY(fill_u64(s, b->d_results, b->result_words + 2 * kQueues, 0));
+  TGP_TRY(fill_u64(s, b->d_results, b->result_words + 2 * kQueues + 1, 0));
+  if (!b->gplans.empty()) {
+    auto* counter = reinterpret_cast<unsigned long long*>(b->d_results + b->result_words +
+                                                          2 * kQueues);
+    TGP_TRY(launch_batch_generic(b->ctx, b->d_gplans, b->d_gitems, b->gitems.size(), counter,
+                                 b->gstage_max, b->gtable_max, b->d_gminmax,
+                                 (uint32_t)b->gminmax.size(), b->gminmax_words, s));
+  }
   if (!b->total_slices) return TGP_OK;
   if (b->order == 3 && b->partial_words)
     TGP_TRY(fill_u64(s, b->d_partials, b->partial_words, 0));
@@ -1163,6 +1235,9 @@ int32_t tgp_batch_destroy(tgp_batch* b) {
   if (b->d_pgroups) hbm_free(b->ctx, b->d_pgroups);
   if (b->d_plinks) hbm_free(b->ctx, b->d_plinks);
   if (b->d_punits) hbm_free(b->ctx, b->d_punits);
+  if (b->d_gplans) hbm_free(b->ctx, b->d_gplans);
+  if (b->d_gitems) hbm_free(b->ctx, b->d_gitems);
+  if (b->d_gminmax) hbm_free(b->ctx, b->d_gminmax);
   if (b->d_partials) hbm_free(b->ctx, b->d_partials);
   for (void* p : b->retired) hbm_free(b->ctx, p);
   delete b;
@@ -1200,7 +1275,7 @@ int32_t tgp_batch_set_queries(tgp_batch* b, const tgp_agg_query* qs, uint32_t n,
   }
   b->result_words = words;
   // the arena ends with the persistent kernels' work counters
-  TGP_TRY(ensure_cap(b, &b->d_results, &b->results_cap, words + 2 * kQueues));
+  TGP_TRY(ensure_cap(b, &b->d_results, &b->results_cap, words + 2 * kQueues + 1));
   uint64_t slices = 0;
   for (uint32_t i = 0; i < n; ++i) {
     Shape sh;
@@ -1229,6 +1304,7 @@ int32_t tgp_batch_set_queries(tgp_batch* b, const tgp_agg_query* qs, uint32_t n,
   const uint64_t bytes = b->plan.size() * sizeof(BatchQuery);
   TGP_TRY(ensure_cap(b, &b->d_plan, &b->plan_cap, b->plan.size() ? b->plan.size() : 1));
   b->granule_valid = false;  // built when the query-ordered kernel runs
+  TGP_TRY(plan_generic(b, qs, s));
   if (bytes) {
     TGP_HIP(hipMemcpyAsync(b->d_plan, b->plan.data(), bytes, hipMemcpyHostToDevice, s));
     TGP_HIP(hipStreamSynchronize(s));
@@ -1367,12 +1443,6 @@ int32_t tgp_batch_run(tgp_batch* b, int32_t mode, void* stream) {
   } else {
     TGP_TRY(batch_launch(b, s));
   }
-  // queries without a batch shape run alone, into the same arena
-  for (int32_t i : b->fallback) {
-    tgp_group_result r{b->d_results + b->count_off[i], b->d_results + b->sums_off[i],
-                       b->d_results + b->oor_off[i]};
-    TGP_TRY(tgp_group_agg(b->ctx, &b->queries[i], &r, s));
-  }
   return TGP_OK;
 }
 
@@ -1419,7 +1489,7 @@ int32_t tgp_batch_info(tgp_batch* b, uint64_t* result_words, uint64_t* total_sli
   TGP_REQUIRE(b, "tgp_batch_info: null batch");
   if (result_words) *result_words = b->result_words;
   if (total_slices) *total_slices = b->total_slices;
-  if (n_batched) *n_batched = (uint32_t)b->plan.size();
+  if (n_batched) *n_batched = (uint32_t)(b->plan.size() + b->gplans.size());
   if (n_fallback) *n_fallback = (uint32_t)b->fallback.size();
   return TGP_OK;
 }

@@ -149,6 +149,64 @@ __global__ void __launch_bounds__(kBlock, 2) k_agg_reg(DevPlan P) {
 }
 
 // ---- table strategies (LDS table or global table) ----------------------------------------
+// One tile's rows into a table of counts (tcnt[g]) and int128 sums / MIN-MAX codes
+// (tsum[(g * naggs + a) * 2]), in LDS or in the result itself.
+__device__ __forceinline__ void table_tile(const DevPlan& P, const uint8_t* buf, uint64_t row0,
+                                           unsigned long long* tcnt, unsigned long long* tsum,
+                                           uint32_t& oor) {
+  uint32_t sel = eval_select(P, buf, row0);
+  if (__ballot(sel != 0) == 0) return;
+  int64_t gid[kRows];
+  eval_groups(P, buf, gid);
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    if (!((sel >> r) & 1)) continue;
+    if (gid[r] < 0) {
+      ++oor;
+      sel &= ~(1u << r);
+      continue;
+    }
+    atomicAdd(&tcnt[gid[r]], 1ull);
+  }
+  for (int a = 0; a < P.naggs; ++a) {
+    uint64_t v[kRows];
+    eval_agg(P.aggs[a], buf, v);
+    const int32_t mk = (P.minmax_kinds >> (3 * a)) & 7;
+    if (mk) {
+#pragma unroll
+      for (int r = 0; r < kRows; ++r)
+        if ((sel >> r) & 1)
+          atomicMax(&tsum[((uint64_t)gid[r] * P.naggs + a) * 2],
+                    (unsigned long long)mm_encode(mk, v[r]));
+      continue;
+    }
+#pragma unroll
+    for (int r = 0; r < kRows; ++r)
+      if ((sel >> r) & 1)
+        atomic_add128_i64(&tsum[((uint64_t)gid[r] * P.naggs + a) * 2], (int64_t)v[r]);
+  }
+}
+
+// Adds an LDS table into the result (and clears it when `clear`).
+__device__ __forceinline__ void table_flush(const DevPlan& P, unsigned long long* tcnt,
+                                            unsigned long long* tsum, bool clear) {
+  for (uint64_t g = threadIdx.x; g < P.ngroups; g += kBlock) {
+    const unsigned long long c = tcnt[g];
+    if (!c) continue;
+    atomicAdd(&P.out_count[g], c);
+    if (clear) tcnt[g] = 0;
+    for (int a = 0; a < P.naggs; ++a) {
+      const uint64_t i = (g * P.naggs + a) * 2;
+      if ((P.minmax_kinds >> (3 * a)) & 7) atomicMax(&P.out_sums[i], tsum[i]);
+      else atomic_add128(&P.out_sums[i], tsum[i], (int64_t)tsum[i + 1]);
+      if (clear) {
+        tsum[i] = 0;
+        tsum[i + 1] = 0;
+      }
+    }
+  }
+}
+
 template <bool USE_LDS>
 __global__ void __launch_bounds__(kBlock, 2) k_agg_table(DevPlan P) {
   extern __shared__ __align__(16) uint8_t lds[];
@@ -166,51 +224,64 @@ __global__ void __launch_bounds__(kBlock, 2) k_agg_table(DevPlan P) {
   }
   uint32_t oor = 0;
   for_each_tile(P, lds, [&](const uint8_t* buf, uint64_t row0) {
-    uint32_t sel = eval_select(P, buf, row0);
-    if (__ballot(sel != 0) == 0) return;
-    int64_t gid[kRows];
-    eval_groups(P, buf, gid);
-#pragma unroll
-    for (int r = 0; r < kRows; ++r) {
-      if (!((sel >> r) & 1)) continue;
-      if (gid[r] < 0) {
-        ++oor;
-        sel &= ~(1u << r);
-        continue;
-      }
-      atomicAdd(&tcnt[gid[r]], 1ull);
-    }
-    for (int a = 0; a < P.naggs; ++a) {
-      uint64_t v[kRows];
-      eval_agg(P.aggs[a], buf, v);
-      const int32_t mk = (P.minmax_kinds >> (3 * a)) & 7;
-      if (mk) {
-#pragma unroll
-        for (int r = 0; r < kRows; ++r)
-          if ((sel >> r) & 1)
-            atomicMax(&tsum[((uint64_t)gid[r] * P.naggs + a) * 2],
-                      (unsigned long long)mm_encode(mk, v[r]));
-        continue;
-      }
-#pragma unroll
-      for (int r = 0; r < kRows; ++r)
-        if ((sel >> r) & 1)
-          atomic_add128_i64(&tsum[((uint64_t)gid[r] * P.naggs + a) * 2], (int64_t)v[r]);
-    }
+    table_tile(P, buf, row0, tcnt, tsum, oor);
   });
-  if (USE_LDS) {
-    for (uint64_t g = threadIdx.x; g < P.ngroups; g += kBlock) {
-      const unsigned long long c = tcnt[g];
-      if (!c) continue;
-      atomicAdd(&P.out_count[g], c);
-      for (int a = 0; a < P.naggs; ++a) {
-        const uint64_t i = (g * P.naggs + a) * 2;
-        if ((P.minmax_kinds >> (3 * a)) & 7) atomicMax(&P.out_sums[i], tsum[i]);
-        else atomic_add128(&P.out_sums[i], tsum[i], (int64_t)tsum[i + 1]);
-      }
-    }
-  }
+  if (USE_LDS) table_flush(P, tcnt, tsum, false);
   if (oor && P.out_oor) atomicAdd(P.out_oor, (unsigned long long)oor);
+}
+
+// ---- many queries of any form in one launch (the batch's generic items) -----------------
+// Persistent workgroups take items (query, tiles [t0, t1)) from a counter, copy
+// the item's plan into LDS, stage its tiles double-buffered and aggregate them
+// into an LDS table (flushed after the item) or straight into the result.
+__global__ void __launch_bounds__(kBlock) k_batch_generic(const DevPlan* __restrict__ plans,
+                                                          const GenericItem* __restrict__ items,
+                                                          uint64_t nitems,
+                                                          unsigned long long* counter,
+                                                          uint32_t stage_max) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  __shared__ DevPlan sp;
+  __shared__ uint32_t s_item;
+  static_assert(sizeof(DevPlan) % 8 == 0, "plan copied as 64-bit words");
+  unsigned long long* table = reinterpret_cast<unsigned long long*>(lds + 2 * stage_max);
+  for (;;) {
+    __syncthreads();  // the previous item is flushed; sp and s_item are free
+    if (threadIdx.x == 0) s_item = (uint32_t)atomicAdd(counter, 1ull);
+    __syncthreads();
+    const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
+    if (it >= nitems) break;
+    const GenericItem I = items[it];
+    {
+      const uint64_t* src = reinterpret_cast<const uint64_t*>(&plans[I.q]);
+      uint64_t* dst = reinterpret_cast<uint64_t*>(&sp);
+      for (uint32_t i = threadIdx.x; i < sizeof(DevPlan) / 8; i += kBlock) dst[i] = src[i];
+    }
+    unsigned long long* tcnt = I.lds_table ? table : nullptr;
+    __syncthreads();
+    const DevPlan& P = sp;
+    unsigned long long* tsum = nullptr;
+    if (I.lds_table) {
+      tsum = tcnt + P.ngroups;
+      const uint64_t words = P.ngroups * (1 + 2 * (uint64_t)P.naggs);
+      for (uint64_t i = threadIdx.x; i < words; i += kBlock) tcnt[i] = 0;
+    } else {
+      tcnt = P.out_count;
+      tsum = P.out_sums;
+    }
+    uint32_t oor = 0;
+    stage_issue(P, I.t0, lds);
+    for (uint32_t t = I.t0, i = 0; t < I.t1; ++t, ++i) {
+      stage_wait();
+      __syncthreads();  // tile t is in LDS (and the table clear is visible)
+      uint8_t* cur = lds + (i & 1) * stage_max;
+      uint8_t* nxt = lds + ((i + 1) & 1) * stage_max;
+      if (t + 1 < I.t1) stage_issue(P, t + 1, nxt);
+      table_tile(P, cur, P.base0 + (uint64_t)t * kTileRows, tcnt, tsum, oor);
+    }
+    __syncthreads();
+    if (I.lds_table) table_flush(P, tcnt, tsum, false);
+    if (oor && P.out_oor) atomicAdd(P.out_oor, (unsigned long long)oor);
+  }
 }
 
 // Turns the MIN/MAX codes into values: low word = the value, high word = its
@@ -229,6 +300,51 @@ __global__ void __launch_bounds__(256) k_minmax_finalize(const unsigned long lon
   const bool sgn = mk == TGP_AGG_MIN || mk == TGP_AGG_MAX;
   sums[2 * i] = v;
   sums[2 * i + 1] = sgn && (int64_t)v < 0 ? ~0ull : 0ull;
+}
+
+// MIN/MAX codes of the batch's generic queries turned into values: blockIdx.y
+// picks the query (list[y] indexes plans).
+__global__ void __launch_bounds__(256) k_batch_minmax(const DevPlan* __restrict__ plans,
+                                                      const uint32_t* __restrict__ list) {
+  const DevPlan& P = plans[list[blockIdx.y]];
+  const uint64_t n = P.ngroups * (uint64_t)P.naggs;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t g = i / (uint32_t)P.naggs;
+    const int32_t a = (int32_t)(i - g * (uint32_t)P.naggs);
+    const int32_t mk = (P.minmax_kinds >> (3 * a)) & 7;
+    if (!mk) continue;
+    const uint64_t v = P.out_count[g] ? mm_decode(mk, P.out_sums[2 * i]) : 0;
+    const bool sgn = mk == TGP_AGG_MIN || mk == TGP_AGG_MAX;
+    P.out_sums[2 * i] = v;
+    P.out_sums[2 * i + 1] = sgn && (int64_t)v < 0 ? ~0ull : 0ull;
+  }
+}
+
+int32_t launch_batch_generic(tgp_ctx* ctx, const DevPlan* d_plans, const GenericItem* d_items,
+                             uint64_t nitems, unsigned long long* counter, uint32_t stage_max,
+                             uint32_t table_max, const uint32_t* d_minmax, uint32_t nminmax,
+                             uint64_t minmax_words, hipStream_t s) {
+  if (nitems) {
+    const uint32_t lds = 2 * stage_max + table_max;
+    if (lds > 65536)
+      TGP_HIP(hipFuncSetAttribute((const void*)k_batch_generic,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = 0;
+    TGP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_generic, kBlock, lds));
+    if (per_cu < 1) per_cu = 1;
+    uint64_t grid = (uint64_t)persistent_grid(ctx, per_cu);
+    if (grid > nitems) grid = nitems;
+    k_batch_generic<<<dim3((unsigned)grid), dim3(kBlock), lds, s>>>(d_plans, d_items, nitems,
+                                                                    counter, stage_max);
+    TGP_HIP(hipGetLastError());
+  }
+  if (nminmax) {
+    const uint64_t gx = std::min<uint64_t>((minmax_words + 255) / 256, 1024);
+    k_batch_minmax<<<dim3((unsigned)gx, nminmax), dim3(256), 0, s>>>(d_plans, d_minmax);
+    TGP_HIP(hipGetLastError());
+  }
+  return TGP_OK;
 }
 
 int32_t try_spec(tgp_ctx* ctx, const tgp_agg_query* q, const tgp_group_result* out,

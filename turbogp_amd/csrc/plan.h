@@ -18,6 +18,20 @@ struct PlanInfo {
   int strategy = kAggReg1;
 };
 
+// A work item of the batch's generic path: tiles [t0, t1) of plan q, aggregated in
+// an LDS table (lds_table 1) or straight into the result (0).
+struct GenericItem {
+  uint32_t q, t0, t1, lds_table;
+};
+
+// Queues the generic items (one persistent launch; 2 stage buffers of stage_max
+// bytes + an LDS table of table_max bytes per workgroup) and the MIN/MAX
+// finalisation of the plans in d_minmax (minmax_words: the largest groups x aggs).
+int32_t launch_batch_generic(tgp_ctx* ctx, const DevPlan* d_plans, const GenericItem* d_items,
+                             uint64_t nitems, unsigned long long* counter, uint32_t stage_max,
+                             uint32_t table_max, const uint32_t* d_minmax, uint32_t nminmax,
+                             uint64_t minmax_words, hipStream_t s);
+
 // LDS table limit of the kAggLds strategy.
 constexpr uint32_t kLdsTableBytes = 48 * 1024;
 // LDS of the register strategies' final workgroup reduction.
