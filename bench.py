@@ -248,6 +248,60 @@ class DeviceCounter:
         self.buf.free()
 
 
+def bench_mixed(ctx, table):
+    """A 4096-query batch of every query form (compiled Q1 / Q6 shapes, a fifth
+    predicate, MIN / MAX, an input selection bitmap, an int32 price column, 5052
+    groups) over the 256M-row table: the batch launch against one launch per query,
+    results compared."""
+    from turbogp_amd import DeviceBitmap, Event, _capi as A
+    from turbogp_amd import workloads as W
+    from turbogp_amd.batch import Batch
+    n = W.BATCH_TABLE_ROWS
+    price32 = W.device_uniform(ctx, n, A.I32, W.FACT_SCHEMA["price"][0], 90_000, 10_500_000)
+    rng = np.random.default_rng(W.SEED ^ 0xB17)
+    words = rng.integers(0, 2 ** 63, (n + 63) // 64, dtype=np.int64).astype(np.uint64)
+    bm = DeviceBitmap(ctx, n)
+    bm.buf.upload(words)
+    params = W.mixed_batch_params(W.BATCH_QUERIES, n)
+    qs = W.mixed_batch_queries(table, price32, bm, params)
+    b = Batch(ctx, len(qs))
+    try:
+        b.set_queries(qs)
+        info = b.info()
+        log(f"mixed batch: {info}")
+        tuned, _ = b.autotune()
+        log(f"mixed batch autotuned {tuned}")
+        ev = [Event(ctx) for _ in range(2)]
+        b.run()
+        ctx.sync()
+        ts = []
+        for _ in range(10):
+            ev[0].record()
+            b.run()
+            ev[1].record()
+            ts.append(ev[0].elapsed_ms(ev[1]))
+        batched = b.results()
+        tot, _ = b.run_timed(A.BATCH_PER_QUERY)
+        single = b.results()
+        same = all(np.array_equal(x.count, y.count) and np.array_equal(x.sums_words, y.sums_words)
+                   and x.n_out_of_range == y.n_out_of_range for x, y in zip(batched, single))
+        assert same, "mixed batch differs from per-query launches"
+        for e in ev:
+            e.free()
+    finally:
+        b.close()
+        bm.free()
+        price32.free()
+    ms = statistics.mean(ts)
+    log(f"mixed batch: {ms:.3f} ms, per-query {tot:.3f} ms")
+    return {"queries": len(qs), "forms": {f: sum(1 for p in params if p[0] == f)
+                                          for f in dict.fromkeys(W.MIXED_FORMS)},
+            "batched_queries": info["batched"], "fallback": info["fallback"],
+            "tuning": tuned, "ms": round(ms, 4), "queries_per_s": round(len(qs) / (ms / 1e3), 1),
+            "per_query_ms": round(tot, 4), "speedup_vs_per_query": round(tot / ms, 2),
+            "results_identical_batched_vs_per_query": bool(same)}
+
+
 def bench_distinct(ctx, table):
     """COUNT(DISTINCT price) and COUNT(DISTINCT qty) over the table's first 60M
     rows: HyperLogLog (the reference's path for >= 100K selected rows) and the
@@ -445,7 +499,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=2000)
-    ap.add_argument("--sections", default="batch,scan,join,serve,distinct,select",
+    ap.add_argument("--sections", default="batch,scan,join,serve,mixed,distinct,select",
                     help="which parts to run (profiling runs restrict them so a kernel's "
                          "launches all come from one workload)")
     ap.add_argument("--tuning", default="",
@@ -556,6 +610,7 @@ def main():
     join = bench_join(ctx, args) if "join" in sections or "serve" in sections else None
     join_tables = join.pop("_tables") if join else None
     serve = bench_serve(ctx, table, join_tables, args) if "serve" in sections else None
+    mixed = bench_mixed(ctx, table) if "mixed" in sections else None
     distinct = bench_distinct(ctx, table) if "distinct" in sections else None
     select = bench_select(ctx, table) if "select" in sections else None
     if "join" not in sections:
@@ -616,6 +671,7 @@ def main():
         "join_2M_x_60M": join,
         "count_distinct_60M": distinct,
         "select_60M": select,
+        "batch_mixed_4096q_256M": mixed,
         "serving": serve,
         "roofline": {"bound": "hbm", "achieved": round(achieved_unique, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(frac, 4), "traffic": traffic,

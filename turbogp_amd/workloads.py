@@ -303,3 +303,71 @@ def poisson_offsets(n, qps, seed=SEED):
     rng = np.random.Generator(np.random.PCG64((seed ^ 0xA5517) & 0xFFFFFFFF))
     gaps = rng.exponential(1e9 / qps, n)
     return np.cumsum(gaps).astype(np.int64)
+
+
+# ---- mixed-shape batch: every query form in one batch --------------------------------------
+MIXED_FORMS = ("q1", "q1", "q6", "q6_5preds", "minmax", "bitmap", "price_i32", "wide_groups")
+
+
+def mixed_batch_params(n_queries, table_rows, seed=SEED):
+    """(form, row_begin, row_end, date cutoff, group, literal) per query: the compiled
+    Q1 / Q6 shapes, Q6 with a fifth predicate (qty <> literal), MIN / MAX beside sums,
+    an input selection bitmap (whole table), price as int32 (another column layout)
+    and GROUP BY (dt, flagb): 2526 x 2 groups.  Every range is log-uniform in 1K..4M
+    rows; the bitmap queries start on a multiple of 64 rows."""
+    rng = np.random.default_rng(seed ^ 0x313D)
+    out = []
+    for k in range(n_queries):
+        form = MIXED_FORMS[k % len(MIXED_FORMS)]
+        L = int(_log_uniform(rng, 1_000, min(4_000_000, table_rows)))
+        b = int(rng.integers(0, table_rows - L + 1))
+        e = b + L
+        if form == "bitmap":  # the range starts on a bitmap word: a view of one bitmap
+            b &= ~63
+        out.append((form, b, e, int(rng.integers(0, 2526)), ["ab", "a", "b", "month", ""][k % 5],
+                    int(rng.integers(1, 50))))
+    return out
+
+
+class _BitmapView:
+    """The words of a table-wide selection bitmap from row `row` (a multiple of 64) on."""
+
+    class _Buf:
+        def __init__(self, ptr):
+            self.ptr = ptr
+
+    def __init__(self, bitmap, row):
+        assert row % 64 == 0
+        self.buf = self._Buf(bitmap.buf.ptr + (row // 64) * 8)
+
+
+def mixed_batch_queries(table, price_i32, bitmap, params):
+    """tgp_agg_query objects of mixed_batch_params() descriptions."""
+    q1n, q1p, q1parts, q1aggs = q1_spec()
+    q6n, q6p, q6parts, q6aggs = q6_spec()
+    out = []
+    for form, b, e, cut, grp, lit in params:
+        sel = None
+        if form == "q1":
+            nm, pr, pa, ag = q1_spec(date_cutoff=cut, group=grp)
+            cols = [table[x] for x in nm]
+        elif form == "q6":
+            cols, pr, pa, ag = [table[x] for x in q6n], q6p, q6parts, q6aggs
+        elif form == "q6_5preds":
+            cols, pr, pa, ag = [table[x] for x in q6n], q6p + [(2, A.NE, lit, True)], q6parts, q6aggs
+        elif form == "minmax":
+            cols, pr, pa = [table[x] for x in q1n], q1p, q1parts
+            ag = [("min", 2), ("max_u64", 1), ("sum", 1, [(2, 0, 1)]),
+                  ("max", 1, [(1, 0, 1), (3, 100, -1)])]
+        elif form == "bitmap":  # rows [b, e) select by bits b.. of the table-wide bitmap
+            cols, pr, pa, ag = [table[x] for x in q1n], q1p, q1parts, q1aggs
+            sel = _BitmapView(bitmap, b)
+        elif form == "price_i32":
+            cols = [table[x] if x != "price" else price_i32 for x in q1n]
+            pr, pa, ag = q1p, q1parts, q1aggs
+        else:
+            cols, pr, ag = [table[x] for x in q1n], q1p, q1aggs
+            pa = [(0, 1, 0, 2526), (6, 1, 0, 2)]
+        out.append(make_agg_query(cols, pr, pa, ag, b, e, sel))
+    return out
+
