@@ -248,6 +248,48 @@ class DeviceCounter:
         self.buf.free()
 
 
+def bench_generic(ctx, table, args):
+    """The generic tile engine (queries of no compiled shape) on 60M rows: Q1 with
+    price as int32 (another column layout; 6 groups, register strategy) and Q1
+    grouped by (dt, flagb) (5052 groups, result table)."""
+    from turbogp_amd import Event, GroupAggBuffers, _capi as A
+    from turbogp_amd import workloads as W
+    import ctypes as C
+    price32 = W.device_uniform(ctx, SCAN_ROWS, A.I32, W.FACT_SCHEMA["price"][0], 90_000,
+                               10_500_000)
+    out = {}
+    ev = [Event(ctx) for _ in range(2)]
+    try:
+        for name, form, bpr in (("q1_price_i32_generic", "price_i32", 34),
+                                ("q1_5052_groups_generic", "wide_groups", 38)):
+            q = W.mixed_batch_queries(table, price32, None, [(form, 0, SCAN_ROWS, 0, "ab", 0)])[0]
+            g = C.c_uint64()
+            A.check(A.lib().tgp_group_agg_ngroups(C.byref(q), C.byref(g)), "ngroups")
+            st, sh = C.c_int32(), C.c_int32()
+            A.check(A.lib().tgp_group_agg_explain(C.byref(q), C.byref(st), C.byref(sh)), "explain")
+            res = GroupAggBuffers(ctx, g.value, q.naggs)
+            for _ in range(args.warmup):
+                ctx.group_agg_async(q, res)
+            ts = []
+            for _ in range(args.steps):
+                ev[0].record()
+                ctx.group_agg_async(q, res)
+                ev[1].record()
+                ts.append(ev[0].elapsed_ms(ev[1]))
+            res.free()
+            ms = statistics.mean(ts)
+            gbs = SCAN_ROWS * bpr / 1e9 / (ms / 1e3)
+            out[name] = {"ms": round(ms, 4), "GB_s": round(gbs, 1),
+                         "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_row": bpr,
+                         "groups": int(g.value), "strategy": int(st.value)}
+            log(f"{name}: {ms:.3f} ms, {gbs:.0f} GB/s")
+    finally:
+        for e in ev:
+            e.free()
+        price32.free()
+    return out
+
+
 def bench_mixed(ctx, table):
     """A 4096-query batch of every query form (compiled Q1 / Q6 shapes, a fifth
     predicate, MIN / MAX, an input selection bitmap, an int32 price column, 5052
@@ -606,6 +648,7 @@ def main():
     for e in ev + ev3:
         e.free()
 
+    generic = bench_generic(ctx, table, args)
     sections = set(args.sections.split(","))
     join = bench_join(ctx, args) if "join" in sections or "serve" in sections else None
     join_tables = join.pop("_tables") if join else None
@@ -667,7 +710,8 @@ def main():
             "q6_scan_sum": {"ms": round(ms_b, 4), "GB_s": round(gbs_b, 1),
                             "hbm_frac": round(gbs_b / HBM_PEAK_GBS, 4),
                             "bytes_per_row": W.Q6_BYTES_PER_ROW,
-                            "selected": int(res_b.count[0])}},
+                            "selected": int(res_b.count[0])},
+            **generic},
         "join_2M_x_60M": join,
         "count_distinct_60M": distinct,
         "select_60M": select,

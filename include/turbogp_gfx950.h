@@ -330,7 +330,8 @@ int32_t tgp_group_agg(tgp_ctx* ctx, const tgp_agg_query* q, const tgp_group_resu
                       void* stream);
 /* Plan of a query without running it (host only): the accumulation strategy
  * of the generic engine (0 registers/1 group, 1 registers/<=8 groups, 2 LDS
- * table, 3 global table) and the index of the shape-specialised kernel that
+ * table, 3 global table, 4 global table of int128 words for ranges of 2^32 rows or
+ * more) and the index of the shape-specialised kernel that
  * would run it (-1: none; the generic engine runs it). */
 int32_t tgp_group_agg_explain(const tgp_agg_query* q, int32_t* strategy, int32_t* spec_shape);
 /* Number of groups the query's key space spans (prod of card). */

@@ -679,7 +679,7 @@ struct tgp_batch {
   // generic items: the queries of no batch shape, through the tile engine
   std::vector<DevPlan> gplans;
   std::vector<GenericItem> gitems;
-  std::vector<uint32_t> gminmax;  // plans with MIN / MAX aggregates
+  std::vector<uint32_t> gminmax;  // plans to finalise: index | split sums << 31
   DevPlan* d_gplans = nullptr;
   GenericItem* d_gitems = nullptr;
   uint32_t* d_gminmax = nullptr;
@@ -1068,18 +1068,21 @@ static int32_t plan_generic(tgp_batch* b, const tgp_agg_query* qs, hipStream_t s
     b->gplans.push_back(P);
   }
   for (uint32_t q = 0; q < b->gplans.size(); ++q) {
-    const DevPlan& P = b->gplans[q];
+    DevPlan& P = b->gplans[q];
     const uint64_t tb = P.ngroups * (1 + 2 * (uint64_t)P.naggs) * 8;
     const bool lds = 2ull * b->gstage_max + tb <= kGenericLds;
-    if (lds) b->gtable_max = std::max<uint32_t>(b->gtable_max, (uint32_t)tb);
-    if (P.minmax_kinds && P.ngroups && P.naggs) {
-      b->gminmax.push_back(q);
+    while (P.lds_rshift > 0 && 2ull * b->gstage_max + (tb << P.lds_rshift) > kGenericLds)
+      --P.lds_rshift;
+    if (lds) b->gtable_max = std::max<uint32_t>(b->gtable_max, (uint32_t)(tb << P.lds_rshift));
+    const bool split = !lds && P.row_end - P.row_begin < (1ull << 32);
+    if ((P.minmax_kinds || split) && P.ngroups && P.naggs && P.ntiles) {
+      b->gminmax.push_back(q | (split ? 1u << 31 : 0u));
       b->gminmax_words = std::max(b->gminmax_words, P.ngroups * (uint64_t)P.naggs);
     }
     for (uint64_t t = 0; t < P.ntiles; t += kGenericTiles)
       b->gitems.push_back(GenericItem{q, (uint32_t)t,
                                       (uint32_t)std::min<uint64_t>(t + kGenericTiles, P.ntiles),
-                                      lds ? 1u : 0u});
+                                      lds ? 1u : split ? 0u : 2u});
   }
   b->fallback.clear();  // every query runs in the batch launch
   if (b->gplans.empty()) return TGP_OK;

@@ -149,11 +149,46 @@ __global__ void __launch_bounds__(kBlock, 2) k_agg_reg(DevPlan P) {
 }
 
 // ---- table strategies (LDS table or global table) ----------------------------------------
-// One tile's rows into a table of counts (tcnt[g]) and int128 sums / MIN-MAX codes
-// (tsum[(g * naggs + a) * 2]), in LDS or in the result itself.
+// Where a table's words live.  An LDS table holds group g's words (the count, then
+// per aggregate the low and high split sums or a MIN/MAX code) at g * W + k, every
+// word in 2^rs copies (lane l adds to copy l mod 2^rs, so that the lanes of one
+// atomic rarely meet on a word when groups are few); the result table keeps the
+// result's layout (count[g], sums[(g * naggs + a) * 2]).
+struct TableRef {
+  unsigned long long* cnt;  // LDS: the table; result: the counts
+  unsigned long long* sum;  // result: the sums
+  uint32_t W, rs, copy;
+  bool lds;
+  __device__ __forceinline__ unsigned long long* count_at(uint64_t g) const {
+    return lds ? cnt + ((g * W) << rs) + copy : cnt + g;
+  }
+  __device__ __forceinline__ unsigned long long* sum_at(uint64_t g, int a) const {
+    return lds ? cnt + ((g * W + 1 + 2 * a) << rs) + copy : sum + (g * ((W - 1) / 2) + a) * 2;
+  }
+  __device__ __forceinline__ uint32_t hi_step() const { return lds ? 1u << rs : 1u; }
+};
+
+__device__ __forceinline__ TableRef lds_table(const DevPlan& P, unsigned long long* base) {
+  return TableRef{base, nullptr, 1 + 2 * (uint32_t)P.naggs, P.lds_rshift,
+                  (threadIdx.x & 63) & ((1u << P.lds_rshift) - 1), true};
+}
+
+__device__ __forceinline__ TableRef result_table(const DevPlan& P) {
+  return TableRef{P.out_count, P.out_sums, 1 + 2 * (uint32_t)P.naggs, 0, 0, false};
+}
+
+__device__ __forceinline__ uint64_t lds_table_words(const DevPlan& P) {
+  return (P.ngroups * (1 + 2 * (uint64_t)P.naggs)) << P.lds_rshift;
+}
+
+// One tile's rows into a table of counts and split sums / MIN-MAX codes (sum of
+// the low 32 bits, sum of the high 32 bits, with non-returning atomics).  A table
+// holds fewer than 2^32 rows' additions per word: an LDS table is flushed per
+// workgroup (or per batch item), the result holds the split sums of fewer than
+// 2^32 rows until the finalisation turns them into int128 words; without `split`
+// (the result of a range of 2^32 rows or more) the sums are int128 words.
 __device__ __forceinline__ void table_tile(const DevPlan& P, const uint8_t* buf, uint64_t row0,
-                                           unsigned long long* tcnt, unsigned long long* tsum,
-                                           uint32_t& oor) {
+                                           const TableRef& T, uint32_t& oor, bool split) {
   uint32_t sel = eval_select(P, buf, row0);
   if (__ballot(sel != 0) == 0) return;
   int64_t gid[kRows];
@@ -166,8 +201,9 @@ __device__ __forceinline__ void table_tile(const DevPlan& P, const uint8_t* buf,
       sel &= ~(1u << r);
       continue;
     }
-    atomicAdd(&tcnt[gid[r]], 1ull);
+    atomicAdd(T.count_at((uint64_t)gid[r]), 1ull);
   }
+  const uint32_t step = T.hi_step();
   for (int a = 0; a < P.naggs; ++a) {
     uint64_t v[kRows];
     eval_agg(P.aggs[a], buf, v);
@@ -176,57 +212,74 @@ __device__ __forceinline__ void table_tile(const DevPlan& P, const uint8_t* buf,
 #pragma unroll
       for (int r = 0; r < kRows; ++r)
         if ((sel >> r) & 1)
-          atomicMax(&tsum[((uint64_t)gid[r] * P.naggs + a) * 2],
-                    (unsigned long long)mm_encode(mk, v[r]));
+          atomicMax(T.sum_at((uint64_t)gid[r], a), (unsigned long long)mm_encode(mk, v[r]));
       continue;
     }
+    if (split) {
 #pragma unroll
-    for (int r = 0; r < kRows; ++r)
-      if ((sel >> r) & 1)
-        atomic_add128_i64(&tsum[((uint64_t)gid[r] * P.naggs + a) * 2], (int64_t)v[r]);
+      for (int r = 0; r < kRows; ++r)
+        if ((sel >> r) & 1) {
+          unsigned long long* p = T.sum_at((uint64_t)gid[r], a);
+          atomicAdd(p, (unsigned long long)(uint32_t)v[r]);
+          const int64_t h = (int64_t)v[r] >> 32;
+          if (h) atomicAdd(p + step, (unsigned long long)h);
+        }
+    } else {
+#pragma unroll
+      for (int r = 0; r < kRows; ++r)
+        if ((sel >> r) & 1) atomic_add128_i64(T.sum_at((uint64_t)gid[r], a), (int64_t)v[r]);
+    }
   }
 }
 
-// Adds an LDS table into the result (and clears it when `clear`).
-__device__ __forceinline__ void table_flush(const DevPlan& P, unsigned long long* tcnt,
-                                            unsigned long long* tsum, bool clear) {
+// Adds an LDS table (copies summed, split sums rebuilt) into the result's words.
+__device__ __forceinline__ void table_flush(const DevPlan& P, const unsigned long long* tab) {
+  const uint32_t W = 1 + 2 * (uint32_t)P.naggs, rs = P.lds_rshift, R = 1u << rs;
   for (uint64_t g = threadIdx.x; g < P.ngroups; g += kBlock) {
-    const unsigned long long c = tcnt[g];
+    const unsigned long long* w0 = tab + ((g * W) << rs);
+    unsigned long long c = 0;
+    for (uint32_t k = 0; k < R; ++k) c += w0[k];
     if (!c) continue;
     atomicAdd(&P.out_count[g], c);
-    if (clear) tcnt[g] = 0;
     for (int a = 0; a < P.naggs; ++a) {
+      const unsigned long long* wl = w0 + ((uint64_t)(1 + 2 * a) << rs);
       const uint64_t i = (g * P.naggs + a) * 2;
-      if ((P.minmax_kinds >> (3 * a)) & 7) atomicMax(&P.out_sums[i], tsum[i]);
-      else atomic_add128(&P.out_sums[i], tsum[i], (int64_t)tsum[i + 1]);
-      if (clear) {
-        tsum[i] = 0;
-        tsum[i + 1] = 0;
+      if ((P.minmax_kinds >> (3 * a)) & 7) {
+        unsigned long long m = 0;
+        for (uint32_t k = 0; k < R; ++k) m = wl[k] > m ? wl[k] : m;
+        atomicMax(&P.out_sums[i], m);
+      } else {
+        unsigned long long slo = 0, shi = 0;
+        for (uint32_t k = 0; k < R; ++k) {
+          slo += wl[k];
+          shi += wl[k + R];
+        }
+        uint64_t lo;
+        int64_t hi;
+        split_to_i128(slo, (int64_t)shi, &lo, &hi);
+        atomic_add128(&P.out_sums[i], lo, hi);
       }
     }
   }
 }
 
-template <bool USE_LDS>
+// USE_LDS: an LDS table of split sums; else the result table, of split sums when
+// SPLIT (fewer than 2^32 rows), of int128 words otherwise.
+template <bool USE_LDS, bool SPLIT>
 __global__ void __launch_bounds__(kBlock, 2) k_agg_table(DevPlan P) {
   extern __shared__ __align__(16) uint8_t lds[];
-  unsigned long long* tcnt;
-  unsigned long long* tsum;
+  unsigned long long* tab = reinterpret_cast<unsigned long long*>(lds + 2 * P.stage_bytes);
   if (USE_LDS) {
-    tcnt = reinterpret_cast<unsigned long long*>(lds + 2 * P.stage_bytes);
-    tsum = tcnt + P.ngroups;
-    const uint64_t words = P.ngroups * (1 + 2 * (uint64_t)P.naggs);
-    for (uint64_t i = threadIdx.x; i < words; i += kBlock) tcnt[i] = 0;
+    const uint64_t words = lds_table_words(P);
+    for (uint64_t i = threadIdx.x; i < words; i += kBlock) tab[i] = 0;
     // made visible by the barrier at the top of the first tile
-  } else {
-    tcnt = P.out_count;
-    tsum = P.out_sums;
   }
+  const TableRef T = USE_LDS ? lds_table(P, tab) : result_table(P);
   uint32_t oor = 0;
   for_each_tile(P, lds, [&](const uint8_t* buf, uint64_t row0) {
-    table_tile(P, buf, row0, tcnt, tsum, oor);
+    table_tile(P, buf, row0, T, oor, USE_LDS || SPLIT);
   });
-  if (USE_LDS) table_flush(P, tcnt, tsum, false);
+  if (USE_LDS) table_flush(P, tab);
   if (oor && P.out_oor) atomicAdd(P.out_oor, (unsigned long long)oor);
 }
 
@@ -256,18 +309,13 @@ __global__ void __launch_bounds__(kBlock) k_batch_generic(const DevPlan* __restr
       uint64_t* dst = reinterpret_cast<uint64_t*>(&sp);
       for (uint32_t i = threadIdx.x; i < sizeof(DevPlan) / 8; i += kBlock) dst[i] = src[i];
     }
-    unsigned long long* tcnt = I.lds_table ? table : nullptr;
     __syncthreads();
     const DevPlan& P = sp;
-    unsigned long long* tsum = nullptr;
-    if (I.lds_table) {
-      tsum = tcnt + P.ngroups;
-      const uint64_t words = P.ngroups * (1 + 2 * (uint64_t)P.naggs);
-      for (uint64_t i = threadIdx.x; i < words; i += kBlock) tcnt[i] = 0;
-    } else {
-      tcnt = P.out_count;
-      tsum = P.out_sums;
+    if (I.mode == 1) {
+      const uint64_t words = lds_table_words(P);
+      for (uint64_t i = threadIdx.x; i < words; i += kBlock) table[i] = 0;
     }
+    const TableRef T = I.mode == 1 ? lds_table(P, table) : result_table(P);
     uint32_t oor = 0;
     stage_issue(P, I.t0, lds);
     for (uint32_t t = I.t0, i = 0; t < I.t1; ++t, ++i) {
@@ -276,48 +324,59 @@ __global__ void __launch_bounds__(kBlock) k_batch_generic(const DevPlan* __restr
       uint8_t* cur = lds + (i & 1) * stage_max;
       uint8_t* nxt = lds + ((i + 1) & 1) * stage_max;
       if (t + 1 < I.t1) stage_issue(P, t + 1, nxt);
-      table_tile(P, cur, P.base0 + (uint64_t)t * kTileRows, tcnt, tsum, oor);
+      table_tile(P, cur, P.base0 + (uint64_t)t * kTileRows, T, oor, I.mode != 2);
     }
     __syncthreads();
-    if (I.lds_table) table_flush(P, tcnt, tsum, false);
+    if (I.mode == 1) table_flush(P, table);
     if (oor && P.out_oor) atomicAdd(P.out_oor, (unsigned long long)oor);
   }
 }
 
-// Turns the MIN/MAX codes into values: low word = the value, high word = its
-// sign extension for the signed kinds; 0 for groups without rows.
-__global__ void __launch_bounds__(256) k_minmax_finalize(const unsigned long long* __restrict__ cnt,
-                                                         unsigned long long* __restrict__ sums,
-                                                         uint64_t ngroups, int32_t naggs,
-                                                         uint32_t kinds) {
-  const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  if (i >= ngroups * (uint64_t)naggs) return;
-  const uint64_t g = i / (uint32_t)naggs;
-  const int32_t a = (int32_t)(i - g * (uint32_t)naggs);
+// Result word pair i = (group, aggregate): a MIN/MAX code becomes the value (low
+// word; high word its sign extension for the signed kinds; 0 for groups without
+// rows), and, when `split`, a split sum becomes int128 words.
+__device__ __forceinline__ void finalize_pair(const unsigned long long* cnt,
+                                              unsigned long long* sums, uint64_t i, uint64_t g,
+                                              int32_t a, uint32_t kinds, bool split) {
   const int32_t mk = (kinds >> (3 * a)) & 7;
-  if (!mk) return;
-  const uint64_t v = cnt[g] ? mm_decode(mk, sums[2 * i]) : 0;
-  const bool sgn = mk == TGP_AGG_MIN || mk == TGP_AGG_MAX;
-  sums[2 * i] = v;
-  sums[2 * i + 1] = sgn && (int64_t)v < 0 ? ~0ull : 0ull;
+  if (mk) {
+    const uint64_t v = cnt[g] ? mm_decode(mk, sums[2 * i]) : 0;
+    const bool sgn = mk == TGP_AGG_MIN || mk == TGP_AGG_MAX;
+    sums[2 * i] = v;
+    sums[2 * i + 1] = sgn && (int64_t)v < 0 ? ~0ull : 0ull;
+  } else if (split) {
+    uint64_t lo;
+    int64_t hi;
+    split_to_i128(sums[2 * i], (int64_t)sums[2 * i + 1], &lo, &hi);
+    sums[2 * i] = lo;
+    sums[2 * i + 1] = (uint64_t)hi;
+  }
 }
 
-// MIN/MAX codes of the batch's generic queries turned into values: blockIdx.y
-// picks the query (list[y] indexes plans).
-__global__ void __launch_bounds__(256) k_batch_minmax(const DevPlan* __restrict__ plans,
-                                                      const uint32_t* __restrict__ list) {
-  const DevPlan& P = plans[list[blockIdx.y]];
+__global__ void __launch_bounds__(256) k_agg_finalize(const unsigned long long* __restrict__ cnt,
+                                                      unsigned long long* __restrict__ sums,
+                                                      uint64_t ngroups, int32_t naggs,
+                                                      uint32_t kinds, int32_t split) {
+  const uint64_t n = ngroups * (uint64_t)naggs;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t g = i / (uint32_t)naggs;
+    finalize_pair(cnt, sums, i, g, (int32_t)(i - g * (uint32_t)naggs), kinds, split != 0);
+  }
+}
+
+// The batch's generic queries finalised: blockIdx.y picks list entry y = plan
+// index | split << 31 (the plan's sums went straight into the result, split).
+__global__ void __launch_bounds__(256) k_batch_finalize(const DevPlan* __restrict__ plans,
+                                                        const uint32_t* __restrict__ list) {
+  const uint32_t e = list[blockIdx.y];
+  const DevPlan& P = plans[e & 0x7fffffffu];
   const uint64_t n = P.ngroups * (uint64_t)P.naggs;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t g = i / (uint32_t)P.naggs;
-    const int32_t a = (int32_t)(i - g * (uint32_t)P.naggs);
-    const int32_t mk = (P.minmax_kinds >> (3 * a)) & 7;
-    if (!mk) continue;
-    const uint64_t v = P.out_count[g] ? mm_decode(mk, P.out_sums[2 * i]) : 0;
-    const bool sgn = mk == TGP_AGG_MIN || mk == TGP_AGG_MAX;
-    P.out_sums[2 * i] = v;
-    P.out_sums[2 * i + 1] = sgn && (int64_t)v < 0 ? ~0ull : 0ull;
+    finalize_pair(P.out_count, P.out_sums, i, g, (int32_t)(i - g * (uint32_t)P.naggs),
+                  P.minmax_kinds, (e >> 31) != 0);
   }
 }
 
@@ -341,7 +400,7 @@ int32_t launch_batch_generic(tgp_ctx* ctx, const DevPlan* d_plans, const Generic
   }
   if (nminmax) {
     const uint64_t gx = std::min<uint64_t>((minmax_words + 255) / 256, 1024);
-    k_batch_minmax<<<dim3((unsigned)gx, nminmax), dim3(256), 0, s>>>(d_plans, d_minmax);
+    k_batch_finalize<<<dim3((unsigned)gx, nminmax), dim3(256), 0, s>>>(d_plans, d_minmax);
     TGP_HIP(hipGetLastError());
   }
   return TGP_OK;
@@ -369,13 +428,19 @@ int32_t run_agg(tgp_ctx* ctx, const DevPlan& P, const PlanInfo& info, hipStream_
   switch (info.strategy) {
     case kAggReg1: TGP_TRY(launch_tiles(ctx, k_agg_reg<1, TGP_MAX_AGGS>, P, info.lds_bytes, s)); break;
     case kAggReg8: TGP_TRY(launch_tiles(ctx, k_agg_reg<8, 4>, P, info.lds_bytes, s)); break;
-    case kAggLds: TGP_TRY(launch_tiles(ctx, k_agg_table<true>, P, info.lds_bytes, s)); break;
-    default: TGP_TRY(launch_tiles(ctx, k_agg_table<false>, P, info.lds_bytes, s)); break;
+    case kAggLds: TGP_TRY(launch_tiles(ctx, k_agg_table<true, true>, P, info.lds_bytes, s)); break;
+    case kAggGlobal:
+      TGP_TRY(launch_tiles(ctx, k_agg_table<false, true>, P, info.lds_bytes, s));
+      break;
+    default: TGP_TRY(launch_tiles(ctx, k_agg_table<false, false>, P, info.lds_bytes, s)); break;
   }
-  if (P.minmax_kinds && P.ngroups && P.naggs) {
+  // the global table holds split sums; MIN/MAX hold codes
+  const bool split = info.strategy == kAggGlobal;
+  if ((P.minmax_kinds || split) && P.ngroups && P.naggs && P.ntiles) {
     const uint64_t items = P.ngroups * (uint64_t)P.naggs;
-    hipLaunchKernelGGL(k_minmax_finalize, dim3((uint32_t)((items + 255) / 256)), dim3(256), 0, s,
-                       P.out_count, P.out_sums, P.ngroups, P.naggs, P.minmax_kinds);
+    const uint64_t grid = std::min<uint64_t>((items + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_agg_finalize, dim3((uint32_t)grid), dim3(256), 0, s, P.out_count,
+                       P.out_sums, P.ngroups, P.naggs, P.minmax_kinds, (int32_t)split);
     TGP_HIP(hipGetLastError());
   }
   return TGP_OK;

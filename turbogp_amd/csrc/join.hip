@@ -332,14 +332,6 @@ __device__ __forceinline__ void acc_add(unsigned long long* cnt, unsigned long l
   if (h) atomicAdd(&sum[2 * g + 1], (unsigned long long)h);
 }
 
-// (sum of lo32, sum of hi32) -> int128 words (lo, hi).
-__device__ __forceinline__ void split_to_i128(uint64_t slo, int64_t shi, uint64_t* lo,
-                                              int64_t* hi) {
-  const uint64_t add = (uint64_t)shi << 32;
-  const uint64_t l = slo + add;
-  *hi = (shi >> 32) + (int64_t)(l < slo);
-  *lo = l;
-}
 
 // LDS form: every wave owns a table of ngroups x (count, sum lo32, sum hi32)
 // words in R = 2^rshift copies, word w of copy c at (w << rshift) + c, and lane

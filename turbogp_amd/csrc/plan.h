@@ -10,7 +10,8 @@ enum AggStrategy {
   kAggReg1 = 0,   // one group: per-thread registers
   kAggReg8 = 1,   // <= 8 groups, <= 4 aggregates: per-thread registers
   kAggLds = 2,    // per-workgroup LDS table + 64-bit LDS atomics
-  kAggGlobal = 3  // global table, 64-bit device atomics per row
+  kAggGlobal = 3,       // global table of split sums, 64-bit device atomics per row
+  kAggGlobalExact = 4   // global table of int128 sums (2^32 rows or more)
 };
 
 struct PlanInfo {
@@ -19,9 +20,10 @@ struct PlanInfo {
 };
 
 // A work item of the batch's generic path: tiles [t0, t1) of plan q, aggregated in
-// an LDS table (lds_table 1) or straight into the result (0).
+// an LDS table of split sums (mode 1), straight into the result as split sums
+// (mode 0, fewer than 2^32 rows) or as int128 words (mode 2).
 struct GenericItem {
-  uint32_t q, t0, t1, lds_table;
+  uint32_t q, t0, t1, mode;
 };
 
 // Queues the generic items (one persistent launch; 2 stage buffers of stage_max
@@ -32,8 +34,9 @@ int32_t launch_batch_generic(tgp_ctx* ctx, const DevPlan* d_plans, const Generic
                              uint32_t table_max, const uint32_t* d_minmax, uint32_t nminmax,
                              uint64_t minmax_words, hipStream_t s);
 
-// LDS table limit of the kAggLds strategy.
+// LDS table limit of the kAggLds strategy, and of its copies.
 constexpr uint32_t kLdsTableBytes = 48 * 1024;
+constexpr uint32_t kLdsCopyBytes = 16 * 1024;
 // LDS of the register strategies' final workgroup reduction.
 constexpr uint32_t kRegReduceBytes = 4 * 8 * (1 + 2 * TGP_MAX_AGGS) * 8;
 

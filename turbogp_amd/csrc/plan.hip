@@ -139,15 +139,22 @@ int32_t plan_agg(const tgp_agg_query* q, DevPlan* P, PlanInfo* info) {
   }
   P->sel_bitmap = q->d_sel_bitmap;
   const uint64_t table_bytes = ngroups * (1 + 2 * (uint64_t)q->naggs) * 8;
+  // copies of the LDS table's words while they fit kLdsCopyBytes (up to 32: the
+  // lanes of one atomic then rarely meet on a word)
+  uint32_t rs = 0;
+  while (rs < 5 && (table_bytes << (rs + 1)) <= kLdsCopyBytes) ++rs;
+  P->lds_rshift = rs;
   if (ngroups == 1)
     info->strategy = kAggReg1;
-  else if (ngroups <= 8 && q->naggs <= 4)
+  else if (ngroups <= 8 && q->naggs <= 4)  // faster than an LDS table with copies
     info->strategy = kAggReg8;
+  else if (q->row_end - q->row_begin >= (1ull << 32))  // split sums need fewer rows
+    info->strategy = kAggGlobalExact;
   else if (table_bytes <= kLdsTableBytes)
     info->strategy = kAggLds;
   else
     info->strategy = kAggGlobal;
-  if (info->strategy == kAggLds) info->lds_bytes += (uint32_t)table_bytes;
+  if (info->strategy == kAggLds) info->lds_bytes += (uint32_t)(table_bytes << rs);
   if ((info->strategy == kAggReg1 || info->strategy == kAggReg8) &&
       info->lds_bytes < kRegReduceBytes)
     info->lds_bytes = kRegReduceBytes;

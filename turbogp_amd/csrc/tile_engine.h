@@ -83,6 +83,8 @@ struct DevPlan {
   int32_t nslots, npreds, nparts, naggs;
   uint32_t stage_bytes;  // LDS bytes of one stage buffer (sum of widths x 1 KiB)
   uint32_t minmax_kinds; // 3 bits per aggregate: its kind if MIN/MAX, else 0
+  uint32_t lds_rshift;   // LDS table: 2^lds_rshift copies of every word
+  uint32_t _pad1;
   uint64_t ngroups;
   const uint64_t* sel_bitmap;
   // outputs
@@ -287,6 +289,25 @@ __host__ __device__ __forceinline__ uint64_t mm_encode(int32_t kind, uint64_t v)
 __host__ __device__ __forceinline__ uint64_t mm_decode(int32_t kind, uint64_t c) {
   const uint64_t k = (kind == TGP_AGG_MIN || kind == TGP_AGG_MIN_U64) ? ~c : c;
   return (kind == TGP_AGG_MIN || kind == TGP_AGG_MAX) ? k ^ (1ull << 63) : k;
+}
+
+// Split sums: a value v is added as its unsigned low 32 bits into one 64-bit word
+// and its signed high 32 bits into another, with non-returning atomics.  Each word
+// stays exact for fewer than 2^32 additions, and
+//   sum(v) = sum(lo32) + 2^32 * sum(hi32)
+// is rebuilt as int128 words (lo, hi) by split_to_i128.
+__device__ __forceinline__ void split_add(unsigned long long* p, int64_t v) {
+  atomicAdd(p, (unsigned long long)(uint32_t)v);
+  const int64_t h = v >> 32;
+  if (h) atomicAdd(p + 1, (unsigned long long)h);
+}
+
+__host__ __device__ __forceinline__ void split_to_i128(uint64_t slo, int64_t shi, uint64_t* lo,
+                                                       int64_t* hi) {
+  const uint64_t add = (uint64_t)shi << 32;
+  const uint64_t l = slo + add;
+  *hi = (shi >> 32) + (int64_t)(l < slo);
+  *lo = l;
 }
 
 // Exact 128-bit accumulation: value = hi * 2^64 + lo.
