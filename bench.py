@@ -219,6 +219,37 @@ def bench_join(ctx, args):
     res_pay = rp.fetch()
     assert matches == npr and int(res_pay.count.sum()) == npr, (matches, res_pay.count.sum())
     hash_group = bench_hash_group(ctx, probe)
+    # probes that mostly miss: 10 % of the foreign keys kept, the rest random int64 keys
+    rng = np.random.default_rng(W.SEED ^ 0xB100)
+    fk = probe["fk"].to_numpy()
+    miss = rng.random(npr) >= 0.1
+    fk[miss] = rng.integers(-2 ** 63, 2 ** 63 - 1, int(miss.sum()), dtype=np.int64)
+    from turbogp_amd import DeviceColumn
+    fk10 = DeviceColumn.from_numpy(ctx, fk)
+    del fk
+    bloom = {"probe_rows": npr, "hit_fraction": 0.1}
+    tf = HashTable(ctx, build["bkey"], build["pay"])
+    t0 = time.perf_counter()
+    tf.add_bloom(8)
+    ctx.sync()
+    bloom["bloom_build_ms_wall"] = round((time.perf_counter() - t0) * 1e3, 4)
+    for name, tab in (("no_filter", t), ("bloom_8_bits_per_key", tf)):
+        c10 = timed(lambda: tab.probe_count(fk10, result=cnt.buf, fetch=False), 10)
+        m10 = cnt.value()
+        p10 = timed(lambda: tab.probe_agg(fk10, probe["val"], A.JOIN_GROUP_BUILD_PAYLOAD,
+                                          ngroups=W.JOIN_PAY_CARD, result=rp, fetch=False), 10)
+        c100 = timed(lambda: tab.probe_count(probe["fk"], result=cnt.buf, fetch=False), 10)
+        p100 = timed(lambda: tab.probe_agg(probe["fk"], probe["val"], A.JOIN_GROUP_BUILD_PAYLOAD,
+                                           ngroups=W.JOIN_PAY_CARD, result=rp, fetch=False), 10)
+        bloom[name] = {"hit10_probe_count_ms": round(c10, 4), "hit10_matches": int(m10),
+                       "hit10_group_by_payload_ms": round(p10, 4),
+                       "hit100_probe_count_ms": round(c100, 4),
+                       "hit100_group_by_payload_ms": round(p100, 4)}
+        log(f"join {name}: 10 % hits count {c10:.3f} ms, payload {p10:.3f} ms; "
+            f"100 % hits count {c100:.3f} ms, payload {p100:.3f} ms")
+    assert bloom["no_filter"]["hit10_matches"] == bloom["bloom_8_bits_per_key"]["hit10_matches"]
+    tf.free()
+    fk10.free()
     jb = join_bytes(nb, npr)
     for e in ev:
         e.free()
@@ -232,7 +263,7 @@ def bench_join(ctx, args):
                                  "GB_s": round(jb["by_payload"] / 1e9 / (pay_ms / 1e3), 1)},
             "group_by_probe_key": {"ms": round(gkey_ms, 4), "groups": W.JOIN_GKEY_CARD,
                                    "GB_s": round(jb["by_gkey"] / 1e9 / (gkey_ms / 1e3), 1)},
-            "matches": int(matches), "hash_group_by_60M": hash_group,
+            "matches": int(matches), "hash_group_by_60M": hash_group, "bloom_prefilter": bloom,
             "_tables": (build, probe)}
 
 

@@ -618,6 +618,15 @@ typedef struct tgp_join_agg_query {
 } tgp_join_agg_query;
 int32_t tgp_hash_probe_agg(tgp_ctx* ctx, const tgp_join_agg_query* q, const tgp_group_result* out,
                            void* stream);
+/* Adds a blocked bloom filter to a single-key table (the reference's join pre-filter,
+ * exec/bloom_filter.rs:62-330: "definitely absent" keys skip the table): every stored
+ * key sets 4 bits of one 64-bit word, the filter has bits_per_key bits per build row
+ * (rounded up to a power of two of words).  Every later probe of the table tests a
+ * key's word first (one load the L2 holds) and probes the table only when all 4 bits
+ * are set.  No false negatives: every result equals the unfiltered one.  Pays off
+ * when most probe keys miss the build side. */
+int32_t tgp_hash_table_add_bloom(tgp_ctx* ctx, tgp_hash_table* t, uint32_t bits_per_key,
+                                 void* stream);
 /* n probe aggregations in one launch, with the results of n tgp_hash_probe_agg
  * calls, bit for bit (the queries may probe different tables, row ranges and build
  * prefixes).  For the serving loop's join requests, which the reference runs one

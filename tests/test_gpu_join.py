@@ -483,3 +483,61 @@ def test_probe_agg_batch_equals_single_calls(ctx):
             t.free()
         for c in keep_u + keep_d + cols:
             c.free()
+
+
+@pytest.mark.parametrize("dups", [False, True])
+def test_bloom_filter_changes_no_result(ctx, dups):
+    """A blocked bloom filter in front of the table (tgp_hash_table_add_bloom): every
+    build key passes it (no false negatives: probing the build keys finds them all),
+    and counts, marks, pairs and every probe aggregation (alone and batched, with
+    build prefixes) over probes that mostly miss equal those of the unfiltered table;
+    the INT64_MIN key lives beside the table and is still found."""
+    rng = np.random.default_rng(91 + dups)
+    nb, npb = 40_000, 500_003
+    bk = (rng.integers(-2**62, 2**62, nb // 4)[rng.integers(0, nb // 4, nb)] if dups
+          else rng.permutation(nb).astype(np.int64) * 7919 - 2**40)
+    bk = bk.astype(np.int64)
+    bk[5] = I64MIN
+    bp = rng.integers(0, 25, nb).astype(np.int32)
+    pk = rng.integers(-2**63, 2**63 - 1, npb, dtype=np.int64)
+    hit = rng.random(npb) < 0.1
+    pk[hit] = bk[rng.integers(0, nb, int(hit.sum()))]
+    pk[::1001] = I64MIN
+    pv = rng.integers(-2**40, 2**40, npb).astype(np.int64)
+    pg = rng.integers(0, 300, npb).astype(np.int32)
+    plain, keep1 = build(ctx, bk, bp)
+    filt, keep2 = build(ctx, bk, bp)
+    cols = [dev(ctx, pk), dev(ctx, pv), dev(ctx, pg)]
+    bkc = dev(ctx, bk)
+    try:
+        filt.add_bloom(8)
+        assert filt.probe_count(bkc) == plain.probe_count(bkc) >= nb
+        assert plain.probe_count(cols[0]) == filt.probe_count(cols[0])
+        a, b = plain.join_mark(cols[0]), filt.join_mark(cols[0])
+        try:
+            assert np.array_equal(a.words(), b.words())
+        finally:
+            a.free()
+            b.free()
+        pa, pb = plain.probe_pairs(cols[0]), filt.probe_pairs(cols[0])
+        assert all(np.array_equal(x, y) for x, y in zip(pa, pb))
+        forms = [(A.JOIN_GROUP_BUILD_PAYLOAD, 25), (A.JOIN_GROUP_PROBE_COL, 300),
+                 (A.JOIN_GROUP_NONE, 1)]
+        qs = []
+        for gb, ng in forms:
+            for lim in (0, 777):
+                want = plain.probe_agg(cols[0], cols[1], gb, ng, 0, cols[2], build_rows=lim)
+                got = filt.probe_agg(cols[0], cols[1], gb, ng, 0, cols[2], build_rows=lim)
+                assert np.array_equal(got.count, want.count) and got.sums() == want.sums()
+                assert got.n_out_of_range == want.n_out_of_range
+                qs.append((filt.agg_query(cols[0], cols[1], gb, ng, 0, cols[2], 0, None, lim), want))
+        for (q, want), got in zip(qs, ctx.probe_agg_batch([q for q, _ in qs])):
+            assert np.array_equal(got.count, want.count) and got.sums() == want.sums()
+        cnt, sums, oor = B.join_agg(bk, bp, pk, pv, pg, A.JOIN_GROUP_BUILD_PAYLOAD, 0, 25)
+        got = filt.probe_agg(cols[0], cols[1], A.JOIN_GROUP_BUILD_PAYLOAD, 25)
+        assert np.array_equal(got.count, cnt) and [x[0] for x in got.sums()] == sums
+    finally:
+        for t in (plain, filt):
+            t.free()
+        for c in keep1 + keep2 + cols + [bkc]:
+            c.free()

@@ -259,6 +259,7 @@ SIGNATURES = {
     "tgp_hash_probe_pairs_sync": (I32_, [P, P, C.POINTER(Column), U64_, U64_, P, P, U64_,
                                          C.POINTER(U64_), P]),
     "tgp_hash_probe_agg": (I32_, [P, C.POINTER(JoinAggQuery), C.POINTER(GroupResult), P]),
+    "tgp_hash_table_add_bloom": (I32_, [P, P, C.c_uint32, P]),
     "tgp_hash_probe_agg_batch": (I32_, [P, C.POINTER(JoinAggQuery), C.c_uint32, C.POINTER(GroupResult),
                                         P]),
     "tgp_scan_compare_columns": (I32_, [P, C.POINTER(Column), I32_, C.POINTER(Column), I32_,

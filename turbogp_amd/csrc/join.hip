@@ -40,6 +40,8 @@ struct JoinTableDev {
   uint64_t n_side;
   int32_t has_dup;
   int32_t _pad;
+  const unsigned long long* bloom;  // optional blocked bloom filter (nullptr: none)
+  uint64_t bloom_mask;              // its words - 1
 };
 
 __device__ __forceinline__ uint64_t hmix(uint64_t z) {
@@ -48,6 +50,18 @@ __device__ __forceinline__ uint64_t hmix(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
   return z ^ (z >> 31);
+}
+
+// Blocked bloom filter over the table's keys: a key sets 4 bits of one 64-bit word
+// (word and bits from a second mix of the key), so a probe tests it with one load
+// that the L2 holds (2M keys: 2 MiB at 8 bits per key).  A key whose bits are not
+// all set is not in the table; the others are probed as usual.
+__device__ __forceinline__ void bloom_pos(unsigned long long k, uint64_t mask, uint64_t* word,
+                                          unsigned long long* bits) {
+  const uint64_t h = hmix(k ^ 0x6A09E667F3BCC909ull);
+  *word = (h >> 32) & mask;
+  *bits = (1ull << (h & 63)) | (1ull << ((h >> 6) & 63)) | (1ull << ((h >> 12) & 63)) |
+          (1ull << ((h >> 18) & 63));
 }
 
 __device__ __forceinline__ int64_t load_i64(const void* base, int32_t dtype, uint64_t i) {
@@ -156,6 +170,12 @@ __device__ __forceinline__ uint32_t probe_key(const JoinTableDev& T, unsigned lo
     }
     return m;
   }
+  if (T.bloom) {
+    uint64_t wi;
+    unsigned long long b;
+    bloom_pos(k, T.bloom_mask, &wi, &b);
+    if ((T.bloom[wi] & b) != b) return 0;
+  }
   uint64_t s = hmix(k) & T.mask;
   for (;;) {
     const Slot e = T.slots[s];
@@ -198,15 +218,30 @@ __device__ __forceinline__ void probe_keys(const JoinTableDev& T, const unsigned
                                            const bool (&ok)[U], F&& f) {
   uint64_t s[U];
   Slot e[U];
+  bool go[U];
+  static_for<U>([&](auto I) { go[decltype(I)::value] = ok[decltype(I)::value]; });
+  if (T.bloom) {  // keys whose bloom bits are not all set skip the table
+    unsigned long long w[U], b[U];
+    static_for<U>([&](auto I) {
+      constexpr int u = decltype(I)::value;
+      uint64_t wi;
+      bloom_pos(k[u], T.bloom_mask, &wi, &b[u]);
+      w[u] = (ok[u] && k[u] != kEmpty) ? T.bloom[wi] : ~0ull;
+    });
+    static_for<U>([&](auto I) {
+      constexpr int u = decltype(I)::value;
+      if (k[u] != kEmpty && (w[u] & b[u]) != b[u]) go[u] = false;
+    });
+  }
   static_for<U>([&](auto I) {
     constexpr int u = decltype(I)::value;
     s[u] = hmix(k[u]) & T.mask;
     e[u] = Slot{kEmpty, 0, 0};
-    if (ok[u] && k[u] != kEmpty) e[u] = T.slots[s[u]];
+    if (go[u] && k[u] != kEmpty) e[u] = T.slots[s[u]];
   });
   static_for<U>([&](auto I) {
     constexpr int u = decltype(I)::value;
-    if (ok[u]) probe_from(T, k[u], s[u], e[u], [&](uint32_t row, uint32_t pay) { f(I, row, pay); });
+    if (go[u]) probe_from(T, k[u], s[u], e[u], [&](uint32_t row, uint32_t pay) { f(I, row, pay); });
   });
 }
 
@@ -1010,6 +1045,21 @@ __global__ void __launch_bounds__(256) k_pairs_write(JoinTableDev T, KeyCols P, 
   }
 }
 
+// Sets every stored key's bloom bits (the keys of the filled slots; the sentinel key
+// lives on the side list and never consults the filter).
+__global__ void __launch_bounds__(256) k_bloom_build(const Slot* __restrict__ slots, uint64_t cap,
+                                                     unsigned long long* bloom, uint64_t mask) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < cap;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const unsigned long long k = slots[i].key;
+    if (k == kEmpty) continue;
+    uint64_t w;
+    unsigned long long b;
+    bloom_pos(k, mask, &w, &b);
+    if ((bloom[w] & b) != b) atomicOr(&bloom[w], b);
+  }
+}
+
 static unsigned grid_for(uint64_t items, uint64_t cap = 8192) {
   uint64_t g = (items + 255) / 256;
   if (g > cap) g = cap;
@@ -1028,6 +1078,8 @@ struct tgp_hash_table {
   uint64_t cap = 0, n_rows = 0, n_side = 0, bytes = 0;
   int32_t has_dup = 0;
   KeyCols bkeys{};  // build key columns (n > 1: a packed multi-column table)
+  unsigned long long* bloom = nullptr;  // tgp_hash_table_add_bloom
+  uint64_t bloom_words = 0;
   JoinTableDev dev() const {
     JoinTableDev d;
     d.slots = static_cast<Slot*>(mem);
@@ -1036,6 +1088,8 @@ struct tgp_hash_table {
     d.n_side = n_side;
     d.has_dup = has_dup;
     d._pad = 0;
+    d.bloom = bloom;
+    d.bloom_mask = bloom_words ? bloom_words - 1 : 0;
     return d;
   }
 };
@@ -1252,6 +1306,10 @@ int32_t tgp_hash_table_destroy(tgp_ctx* ctx, tgp_hash_table* t) {
     TGP_HIP(hipStreamSynchronize(ctx->stream));
     if (t->owns_mem) rc = hbm_free(ctx, t->mem);
     if (t->side_mem) hbm_free(ctx, t->side_mem);
+  }
+  if (t->bloom) {
+    TGP_HIP(hipStreamSynchronize(ctx->stream));
+    hbm_free(ctx, t->bloom);
   }
   delete t;
   return rc;
@@ -1590,6 +1648,28 @@ int32_t tgp_hash_probe_agg_batch(tgp_ctx* ctx, const tgp_join_agg_query* qs, uin
     }
   }
   for (uint32_t i : alone) TGP_TRY(tgp_hash_probe_agg(ctx, &qs[i], &outs[i], s));
+  return TGP_OK;
+}
+
+int32_t tgp_hash_table_add_bloom(tgp_ctx* ctx, tgp_hash_table* t, uint32_t bits_per_key,
+                                 void* stream) {
+  TGP_REQUIRE(ctx && t && t->ctx == ctx, "tgp_hash_table_add_bloom: bad argument");
+  TGP_TRY(check_injected(ctx));
+  TGP_REQUIRE(t->bkeys.n <= 1, "tgp_hash_table_add_bloom: single-key tables only");
+  TGP_REQUIRE(bits_per_key >= 1 && bits_per_key <= 64, "bits_per_key %u not in 1..64",
+              bits_per_key);
+  TGP_REQUIRE(!t->bloom, "tgp_hash_table_add_bloom: the table has a filter already");
+  hipStream_t s = pick_stream(ctx, stream);
+  uint64_t words = 1;
+  while (words * 64 < t->n_rows * (uint64_t)bits_per_key) words <<= 1;
+  void* p = nullptr;
+  TGP_TRY(hbm_alloc(ctx, words * 8, &p));
+  t->bloom = static_cast<unsigned long long*>(p);
+  t->bloom_words = words;
+  TGP_TRY(fill_u64(s, reinterpret_cast<uint64_t*>(t->bloom), words, 0));
+  k_bloom_build<<<grid_for(t->cap), 256, 0, s>>>(static_cast<const Slot*>(t->mem), t->cap, t->bloom,
+                                                 words - 1);
+  TGP_HIP(hipGetLastError());
   return TGP_OK;
 }
 

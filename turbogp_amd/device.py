@@ -782,6 +782,11 @@ class HashTable:
             raise
         return bm
 
+    def add_bloom(self, bits_per_key=8, stream=None):
+        """Blocked bloom filter in front of the table (tgp_hash_table_add_bloom)."""
+        A.check(A.lib().tgp_hash_table_add_bloom(self.ctx.handle, self.handle, bits_per_key,
+                                                 stream), "tgp_hash_table_add_bloom")
+
     def agg_query(self, keys, value, group_by, ngroups=1, key_base=0, group_col=None,
                   row_begin=0, row_end=None, build_rows=0):
         """The tgp_join_agg_query of a probe aggregation over this table."""
