@@ -218,6 +218,22 @@ def bench_join(ctx, args):
                                         result=rg, fetch=False), 10)
     res_pay = rp.fetch()
     assert matches == npr and int(res_pay.count.sum()) == npr, (matches, res_pay.count.sum())
+    sums_hot = (res_pay.sums(), rg.fetch().sums())
+    # the same three calls without the hot-key image, for comparison
+    ctx.tune(A.TUNE_JOIN_HOT, 1)
+    no_hot = {
+        "probe_count_ms": timed(lambda: t.probe_count(probe["fk"], result=cnt.buf, fetch=False), 10),
+        "group_by_payload_ms": timed(lambda: t.probe_agg(
+            probe["fk"], probe["val"], A.JOIN_GROUP_BUILD_PAYLOAD, ngroups=W.JOIN_PAY_CARD,
+            result=rp, fetch=False), 10),
+        "group_by_probe_key_ms": timed(lambda: t.probe_agg(
+            probe["fk"], probe["val"], A.JOIN_GROUP_PROBE_COL, ngroups=W.JOIN_GKEY_CARD,
+            group_col=probe["gkey"], result=rg, fetch=False), 10)}
+    ctx.tune(A.TUNE_JOIN_HOT, 0)
+    no_hot = {k: round(v, 4) for k, v in no_hot.items()}
+    assert (rp.fetch().sums(), rg.fetch().sums()) == sums_hot, "hot-key image changed a result"
+    log(f"join: count {count_ms:.3f} ms, payload {pay_ms:.3f} ms, probe key {gkey_ms:.3f} ms; "
+        f"without the hot-key image {no_hot}")
     hash_group = bench_hash_group(ctx, probe)
     # probes that mostly miss: 10 % of the foreign keys kept, the rest random int64 keys
     rng = np.random.default_rng(W.SEED ^ 0xB100)
@@ -263,7 +279,8 @@ def bench_join(ctx, args):
                                  "GB_s": round(jb["by_payload"] / 1e9 / (pay_ms / 1e3), 1)},
             "group_by_probe_key": {"ms": round(gkey_ms, 4), "groups": W.JOIN_GKEY_CARD,
                                    "GB_s": round(jb["by_gkey"] / 1e9 / (gkey_ms / 1e3), 1)},
-            "matches": int(matches), "hash_group_by_60M": hash_group, "bloom_prefilter": bloom,
+            "matches": int(matches), "without_hot_key_image": no_hot,
+            "hash_group_by_60M": hash_group, "bloom_prefilter": bloom,
             "_tables": (build, probe)}
 
 

@@ -226,11 +226,16 @@ enum tgp_tune_key {
                                    the HyperLogLog estimate; n > 0 divides that size by 2^n
                                    (0..30), so that the set overflows and the count is redone
                                    in a set sized by the row count (tests); 0 (default) */
-  TGP_TUNE_GROUP_TAG_BITS = 9   /* hash GROUP BY over several key columns: n in 1..32 makes
+  TGP_TUNE_GROUP_TAG_BITS = 9,  /* hash GROUP BY over several key columns: n in 1..32 makes
                                    every such query take the hashed form with full-key
                                    verification and keeps only n bits of each slot's hash tag,
                                    so that distinct tuples collide (tests); 0 (default): exact
                                    packing when the key ranges fit 63 bits */
+  TGP_TUNE_JOIN_HOT = 10        /* hot-key cache of join probes (a sampled image of the most
+                                   frequent probe keys' entries, looked up in LDS before the
+                                   table): 0 (default) for unique-key single-column tables of
+                                   >= 8 MiB probed by >= 4M rows; 1 off; 2 for every probe of a
+                                   unique-key single-column table (tests) */
 };
 int32_t tgp_tune_set(tgp_ctx* ctx, int32_t key, int64_t value);
 int32_t tgp_tune_get(tgp_ctx* ctx, int32_t key, int64_t* value);

@@ -541,3 +541,65 @@ def test_bloom_filter_changes_no_result(ctx, dups):
             t.free()
         for c in keep1 + keep2 + cols + [bkc]:
             c.free()
+
+
+@pytest.mark.parametrize("bloom", [False, True])
+def test_hot_key_cache_changes_no_result(ctx, bloom):
+    """The hot-key image (TGP_TUNE_JOIN_HOT = 2 forces it on small probes; the match
+    count and the match bits of the radix-partitioned GROUP BY use it): skewed
+    foreign keys, hot keys absent from the table, a hot INT64_MIN key (never
+    cached: it lives beside the table), build prefixes, probe ranges shorter than
+    the sample, and every probe-aggregate form (per-wave LDS tables, one workgroup
+    table, the radix-partitioned form over match bits, global atomics) give the
+    same results with the image as without it, and equal the oracle."""
+    rng = np.random.default_rng(5 + bloom)
+    nb, npb = 60_000, 700_001
+    bk = (rng.permutation(nb).astype(np.int64) * 104_729 - 2**41)
+    bk[11] = I64MIN
+    bp = rng.integers(-2, 4_003, nb).astype(np.int32)
+    rank = np.arange(1, nb + 1, dtype=np.float64)
+    w = rank ** -1.1
+    pk = bk[rng.choice(nb, npb, p=w / w.sum())]
+    absent = np.array([7, -9, 2**60 + 1], np.int64)          # hot keys with no build row
+    sel = rng.random(npb) < 0.06
+    pk[sel] = absent[rng.integers(0, 3, int(sel.sum()))]
+    pk[rng.random(npb) < 0.02] = I64MIN
+    pv = rng.integers(-2**50, 2**50, npb).astype(np.int64)
+    pg = rng.integers(-3, 70_000, npb).astype(np.int32)
+    t, keep = build(ctx, bk, bp)
+    cols = [dev(ctx, pk), dev(ctx, pv), dev(ctx, pg)]
+    forms = [(A.JOIN_GROUP_BUILD_PAYLOAD, 25, 0, 0), (A.JOIN_GROUP_BUILD_PAYLOAD, 4_000, 0, 0),
+             (A.JOIN_GROUP_PROBE_COL, 70_000, 0, 0), (A.JOIN_GROUP_PROBE_COL, 70_000, 0, 1),
+             (A.JOIN_GROUP_NONE, 1, 0, 0)]
+    try:
+        if bloom:
+            t.add_bloom(8)
+        for b, e in ((0, npb), (333, 100_333), (npb - 5, npb)):
+            counts = []
+            for mode in (1, 2):
+                ctx.tune(A.TUNE_JOIN_HOT, mode)
+                counts.append(t.probe_count(cols[0], b, e))
+            assert counts[0] == counts[1], (b, e)
+            for gb, ng, kb, atomics in forms:
+                for lim in (0, 20_000):
+                    ctx.tune(A.TUNE_JOIN_ATOMICS, atomics)
+                    res = []
+                    for mode in (1, 2):
+                        ctx.tune(A.TUNE_JOIN_HOT, mode)
+                        res.append(t.probe_agg(cols[0], cols[1], gb, ng, kb, cols[2], b, e,
+                                               build_rows=lim))
+                    assert np.array_equal(res[0].count, res[1].count), (gb, ng, lim, b)
+                    assert res[0].sums() == res[1].sums(), (gb, ng, lim, b)
+                    assert res[0].n_out_of_range == res[1].n_out_of_range, (gb, ng, lim, b)
+                    if b == 0 and lim in (0, 20_000) and atomics == 0:
+                        n = lim or nb
+                        cnt, sums, oor = B.join_agg(bk[:n], bp[:n], pk, pv, pg, gb, kb, ng, b, e)
+                        assert np.array_equal(res[1].count, cnt), (gb, ng, lim)
+                        assert [x[0] for x in res[1].sums()] == sums and \
+                            res[1].n_out_of_range == oor, (gb, ng, lim)
+    finally:
+        ctx.tune(A.TUNE_JOIN_HOT, 0)
+        ctx.tune(A.TUNE_JOIN_ATOMICS, 0)
+        t.free()
+        for c in keep + cols:
+            c.free()

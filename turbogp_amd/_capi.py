@@ -34,7 +34,7 @@ AGG_MINMAX = {"min": AGG_MIN, "max": AGG_MAX, "min_u64": AGG_MIN_U64, "max_u64":
 JOIN_GROUP_BUILD_PAYLOAD, JOIN_GROUP_PROBE_COL, JOIN_GROUP_NONE = 0, 1, 2
 TUNE_WG_PER_CU, TUNE_GRID_MAX, TUNE_SPECIALIZE, TUNE_SPEC_KERNEL, TUNE_CHUNK = 0, 1, 2, 3, 4
 TUNE_SLICES_PER_WAVE, TUNE_JOIN_ATOMICS, TUNE_JOIN_PACK_BITS, TUNE_DISTINCT_SHRINK = 5, 6, 7, 8
-TUNE_GROUP_TAG_BITS = 9
+TUNE_GROUP_TAG_BITS, TUNE_JOIN_HOT = 9, 10
 JOIN_INNER, JOIN_LEFT = 0, 1
 JOIN_NO_MATCH = 0xFFFFFFFF
 MAX_JOIN_KEYS = 4

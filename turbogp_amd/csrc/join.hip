@@ -81,12 +81,16 @@ __device__ __forceinline__ int64_t load_i64(const void* base, int32_t dtype, uin
 // together: with a switch per load the compiler waits for each load inside its
 // branch.  idx[u] must be a valid row even where !ok[u] (callers pass a row of
 // the range), so that every load is unconditional.
+// The loads are non-temporal: probe columns stream through once, and cached
+// loads would evict the hash table's lines from the L2.
 template <typename T, int U>
 __device__ __forceinline__ void load_rows_as(const void* base, const uint64_t (&idx)[U],
                                              const bool (&ok)[U], int64_t (&out)[U]) {
   const T* p = static_cast<const T*>(base);
   T raw[U];
-  static_for<U>([&](auto I) { raw[decltype(I)::value] = p[idx[decltype(I)::value]]; });
+  static_for<U>([&](auto I) {
+    raw[decltype(I)::value] = __builtin_nontemporal_load(p + idx[decltype(I)::value]);
+  });
   static_for<U>([&](auto I) {
     constexpr int u = decltype(I)::value;
     out[u] = ok[u] ? (int64_t)raw[u] : 0;
@@ -213,13 +217,92 @@ __device__ __forceinline__ void probe_from(const JoinTableDev& T, unsigned long 
   }
 }
 
-template <int U, typename F>
+// ---- hot-key cache -------------------------------------------------------------------------
+// Foreign keys are often skewed (a few build keys take most of the probes), and
+// every probe of the 16-byte-slot table is a random read of a table far larger
+// than an XCD's L2.  Before a large probe of a unique-key table, a sample of the
+// probe keys is counted, the keys seen at least kHotThreshold times are looked
+// up once in the table, and their exact entries (build row and payload, or
+// "absent") go into a small bucketed image that every probe workgroup copies
+// into LDS.  A probe whose key is in the image takes its match from LDS and
+// skips the table; any other key probes the table as before, so results do not
+// depend on what the image holds.  A key goes into way 0..3 of bucket
+// (hmix(key) >> 40) mod kHotBuckets, or is left out when the bucket is full.
+constexpr uint32_t kHotBuckets = 256, kHotWays = 4, kHotEntries = kHotBuckets * kHotWays;
+constexpr uint32_t kHotAbsent = 0xffffffffu;  // image entry of a key with no build row
+constexpr uint32_t kHotSampleChunks = 256, kHotSampleRows = 1024;  // 256K sampled probe keys
+constexpr uint32_t kHotThreshold = 16;     // sampled 16 times: about 1 probe in 16K or more
+constexpr uint32_t kHotMapSlots = 1u << 19;  // sample count map: at most 256K keys, load <= 0.5
+constexpr uint32_t kHotMapProbes = 64;      // longest map probe (a key past it is not counted)
+constexpr uint64_t kHotLdsBytes = kHotEntries * 16ull;  // keys + entries in LDS
+// the image is used when its keys took at least 1/kHotMinShare of the sample
+constexpr uint64_t kHotMinShare = 8;
+
+// Header words of the image: [0] keys that reached the threshold, [1] sampled
+// occurrences of the keys placed in the image, [2] sampled rows (sentinel keys
+// excluded).
+struct HotDev {
+  const unsigned long long* key;  // kHotEntries keys (kEmpty: free way)
+  const uint2* val;               // kHotEntries (build row | kHotAbsent, payload)
+  const unsigned long long* hdr;
+};
+
+// The image in LDS (filled by hot_to_lds).
+struct HotLds {
+  const unsigned long long* key;
+  const uint2* val;
+};
+
+// True when the image covers enough of the sample to be worth its LDS lookups
+// (the same answer in every thread: two scalar loads).
+__device__ __forceinline__ bool hot_in_use(const HotDev& H) {
+  if (!H.hdr) return false;
+  const unsigned long long cov = __builtin_amdgcn_readfirstlane(H.hdr[1]);
+  const unsigned long long tot = __builtin_amdgcn_readfirstlane(H.hdr[2]);
+  return cov > 0 && cov * kHotMinShare >= tot;
+}
+
+// Copies the image into `lds` (kHotLdsBytes) and returns its LDS view; every
+// thread of the workgroup calls it, and a barrier follows before the lookups.
+__device__ __forceinline__ HotLds hot_to_lds(const HotDev& H, unsigned long long* lds) {
+  uint2* lv = reinterpret_cast<uint2*>(lds + kHotEntries);
+  for (uint32_t i = threadIdx.x; i < kHotEntries; i += blockDim.x) {
+    lds[i] = H.key[i];
+    lv[i] = H.val[i];
+  }
+  return HotLds{lds, lv};
+}
+
+__device__ __forceinline__ uint32_t hot_bucket(uint64_t h) {
+  return (uint32_t)(h >> 40) & (kHotBuckets - 1);
+}
+
+template <bool HOT, int U, typename F>
 __device__ __forceinline__ void probe_keys(const JoinTableDev& T, const unsigned long long (&k)[U],
-                                           const bool (&ok)[U], F&& f) {
+                                           const bool (&ok)[U], F&& f, const HotLds& H) {
   uint64_t s[U];
   Slot e[U];
   bool go[U];
-  static_for<U>([&](auto I) { go[decltype(I)::value] = ok[decltype(I)::value]; });
+  uint64_t h[U];
+  static_for<U>([&](auto I) {
+    constexpr int u = decltype(I)::value;
+    go[u] = ok[u];
+    h[u] = hmix(k[u]);
+  });
+  if constexpr (HOT) {  // keys in the LDS image take their entry from it
+    static_for<U>([&](auto I) {
+      constexpr int u = decltype(I)::value;
+      if (!go[u] || k[u] == kEmpty) return;
+      const uint32_t b = hot_bucket(h[u]) * kHotWays;
+      const ulonglong2 a = *reinterpret_cast<const ulonglong2*>(H.key + b);
+      const ulonglong2 c = *reinterpret_cast<const ulonglong2*>(H.key + b + 2);
+      const int w = a.x == k[u] ? 0 : a.y == k[u] ? 1 : c.x == k[u] ? 2 : c.y == k[u] ? 3 : -1;
+      if (w < 0) return;
+      go[u] = false;
+      const uint2 x = H.val[b + w];
+      if (x.x != kHotAbsent) f(I, x.x, x.y);
+    });
+  }
   if (T.bloom) {  // keys whose bloom bits are not all set skip the table
     unsigned long long w[U], b[U];
     static_for<U>([&](auto I) {
@@ -235,7 +318,7 @@ __device__ __forceinline__ void probe_keys(const JoinTableDev& T, const unsigned
   }
   static_for<U>([&](auto I) {
     constexpr int u = decltype(I)::value;
-    s[u] = hmix(k[u]) & T.mask;
+    s[u] = h[u] & T.mask;
     e[u] = Slot{kEmpty, 0, 0};
     if (go[u] && k[u] != kEmpty) e[u] = T.slots[s[u]];
   });
@@ -245,7 +328,125 @@ __device__ __forceinline__ void probe_keys(const JoinTableDev& T, const unsigned
   });
 }
 
+template <int U, typename F>
+__device__ __forceinline__ void probe_keys(const JoinTableDev& T, const unsigned long long (&k)[U],
+                                           const bool (&ok)[U], F&& f) {
+  probe_keys<false>(T, k, ok, f, HotLds{nullptr, nullptr});
+}
+
 constexpr int kProbeUnroll = 4;
+
+// Scratch of one image: count map keys, count map counts, hot list, image keys,
+// image entries, header.
+struct HotScratch {
+  unsigned long long* map_key;  // kHotMapSlots (kEmpty: free)
+  uint32_t* map_cnt;            // kHotMapSlots
+  unsigned long long* list;     // kHotEntries keys that reached the threshold
+  unsigned long long* key;      // kHotEntries image keys
+  uint2* val;                   // kHotEntries image entries
+  unsigned long long* hdr;      // 4 words
+};
+constexpr uint64_t kHotScratchBytes =
+    kHotMapSlots * 8ull + kHotMapSlots * 4ull + kHotEntries * 8ull * 2 + kHotEntries * 8ull + 64;
+
+__global__ void __launch_bounds__(256) k_hot_clear(HotScratch S) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  for (uint64_t i = t; i < kHotMapSlots; i += stride) {
+    S.map_key[i] = kEmpty;
+    S.map_cnt[i] = 0;
+  }
+  for (uint64_t i = t; i < kHotEntries; i += stride) S.key[i] = kEmpty;
+  if (t < 4) S.hdr[t] = 0;
+}
+
+// Counts kHotSampleRows consecutive probe keys from each of kHotSampleChunks
+// evenly spaced places of [b, e) (one workgroup per place).  The workgroup
+// counts its rows in an LDS map first and adds each distinct key's count to
+// the device map once (the hottest keys fill a tenth of a sample, and device
+// atomics on one word serialise); a key whose device count crosses
+// kHotThreshold joins the hot list (once: only the addition that crosses it
+// appends).
+constexpr uint32_t kHotLdsMap = 2 * kHotSampleRows;
+__global__ void __launch_bounds__(256) k_hot_sample(HotScratch S, const void* keys, int32_t kdt,
+                                                    uint64_t b, uint64_t e) {
+  __shared__ unsigned long long lkey[kHotLdsMap];
+  __shared__ uint32_t lcnt[kHotLdsMap];
+  for (uint32_t i = threadIdx.x; i < kHotLdsMap; i += blockDim.x) {
+    lkey[i] = kEmpty;
+    lcnt[i] = 0;
+  }
+  __syncthreads();
+  const uint64_t c0 = b + (e - b) * blockIdx.x / gridDim.x;
+  uint32_t sampled = 0;
+  for (uint32_t r = threadIdx.x; r < kHotSampleRows; r += blockDim.x) {
+    const uint64_t i = c0 + r;
+    if (i >= e) break;
+    const unsigned long long k = (unsigned long long)load_i64(keys, kdt, i);
+    if (k == kEmpty) continue;  // the side list's key: never cached
+    ++sampled;
+    // at most kHotSampleRows keys in 2 x as many slots: a free slot or the key
+    // is always found
+    uint32_t s = (uint32_t)(hmix(k) >> 24) & (kHotLdsMap - 1);
+    for (;;) {
+      const unsigned long long cur = atomicCAS(&lkey[s], kEmpty, k);
+      if (cur == kEmpty || cur == k) {
+        atomicAdd(&lcnt[s], 1u);
+        break;
+      }
+      s = (s + 1) & (kHotLdsMap - 1);
+    }
+  }
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < kHotLdsMap; j += blockDim.x) {
+    const uint32_t c = lcnt[j];
+    if (!c) continue;
+    const unsigned long long k = lkey[j];
+    uint32_t s = (uint32_t)(hmix(k) >> 24) & (kHotMapSlots - 1);
+    for (uint32_t p = 0; p < kHotMapProbes; ++p, s = (s + 1) & (kHotMapSlots - 1)) {
+      unsigned long long cur = __atomic_load_n(&S.map_key[s], __ATOMIC_RELAXED);
+      if (cur == kEmpty) {
+        cur = atomicCAS(&S.map_key[s], kEmpty, k);
+        if (cur == kEmpty) cur = k;  // claimed
+      }
+      if (cur != k) continue;
+      const uint32_t old = atomicAdd(&S.map_cnt[s], c);
+      if (old < kHotThreshold && old + c >= kHotThreshold) {
+        const unsigned long long at = atomicAdd(&S.hdr[0], 1ull);
+        if (at < kHotEntries) S.list[at] = k;
+      }
+      break;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) sampled += __shfl_xor(sampled, off);
+  if ((threadIdx.x & 63) == 0 && sampled) atomicAdd(&S.hdr[2], (unsigned long long)sampled);
+}
+
+// Looks every hot key up in the (unique-key) table and places (key, build row
+// and payload or kHotAbsent) in its bucket's first free way; adds the key's
+// sampled count to hdr[1] when it is placed.
+__global__ void __launch_bounds__(256) k_hot_image(HotScratch S, JoinTableDev T) {
+  const unsigned long long nh = S.hdr[0] < kHotEntries ? S.hdr[0] : kHotEntries;
+  const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (i >= nh) return;
+  const unsigned long long k = S.list[i];
+  uint2 x = make_uint2(kHotAbsent, 0u);
+  T.bloom = nullptr;
+  probe_key(T, k, [&](uint32_t row, uint32_t pay) { x = make_uint2(row, pay); });
+  const uint32_t b = hot_bucket(hmix(k)) * kHotWays;
+  for (uint32_t w = 0; w < kHotWays; ++w) {
+    if (atomicCAS(&S.key[b + w], kEmpty, k) != kEmpty) continue;
+    S.val[b + w] = x;
+    // the key's count in the sample (it is in the map: it reached the threshold)
+    uint32_t s = (uint32_t)(hmix(k) >> 24) & (kHotMapSlots - 1);
+    for (uint32_t p = 0; p < kHotMapProbes; ++p, s = (s + 1) & (kHotMapSlots - 1))
+      if (S.map_key[s] == k) {
+        atomicAdd(&S.hdr[1], (unsigned long long)S.map_cnt[s]);
+        break;
+      }
+    return;
+  }
+}
 
 // ---- multi-column keys -------------------------------------------------------------------
 // A table over several key columns is keyed by a 64-bit packing of the row's
@@ -285,9 +486,10 @@ __global__ void __launch_bounds__(256) k_pack_keys(KeyCols K, uint64_t n,
 }
 
 // ---- match count (HashProbe) ----------------------------------------------------------
-__global__ void __launch_bounds__(256) k_probe_count(JoinTableDev T, const void* keys, int32_t kdt,
-                                                     uint64_t b, uint64_t e,
-                                                     unsigned long long* out) {
+template <bool HOT>
+__device__ __forceinline__ unsigned long long count_rows(const JoinTableDev& T, const void* keys,
+                                                         int32_t kdt, uint64_t b, uint64_t e,
+                                                         const HotLds& H) {
   unsigned long long c = 0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i0 = b + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i0 < e;
@@ -304,7 +506,23 @@ __global__ void __launch_bounds__(256) k_probe_count(JoinTableDev T, const void*
     load_rows(keys, kdt, idx, ok, kv);
 #pragma unroll
     for (int u = 0; u < kProbeUnroll; ++u) k[u] = (unsigned long long)kv[u];
-    probe_keys(T, k, ok, [&](auto, uint32_t, uint32_t) { ++c; });
+    probe_keys<HOT>(T, k, ok, [&](auto, uint32_t, uint32_t) { ++c; }, H);
+  }
+  return c;
+}
+
+// HD.hdr non-null: the launch gives kHotLdsBytes of dynamic LDS for the image.
+__global__ void __launch_bounds__(256) k_probe_count(JoinTableDev T, HotDev HD, const void* keys,
+                                                     int32_t kdt, uint64_t b, uint64_t e,
+                                                     unsigned long long* out) {
+  extern __shared__ __align__(16) unsigned long long dyn[];
+  unsigned long long c;
+  if (hot_in_use(HD)) {
+    const HotLds H = hot_to_lds(HD, dyn);
+    __syncthreads();
+    c = count_rows<true>(T, keys, kdt, b, e, H);
+  } else {
+    c = count_rows<false>(T, keys, kdt, b, e, HotLds{nullptr, nullptr});
   }
   for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
   if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
@@ -335,6 +553,66 @@ __global__ void __launch_bounds__(256) k_probe_mark(JoinTableDev T, KeyCols P, K
     const unsigned long long w = __ballot(hit);
     const uint64_t wbase = base + (threadIdx.x & ~63u);
     if ((threadIdx.x & 63) == 0 && wbase < n) bitmap[wbase >> 6] = w;
+  }
+}
+
+// ---- match bits of a unique-key table --------------------------------------------------
+// Bit (i - b) of `bits` is set when probe row i matches a build row below
+// row_limit (a unique-key table: at most one).  Rows are taken as in
+// k_probe_count (lane-consecutive, kProbeUnroll rows a grid stride apart); the
+// grid stride is a multiple of 64, so each of a wave's kProbeUnroll groups of
+// 64 rows is one bitmap word, written with one ballot.  LIMIT false: no build
+// prefix (the slot's build row is not read).
+template <bool HOT, bool LIMIT>
+__device__ __forceinline__ void probe_bits_rows(const JoinTableDev& T, const void* keys,
+                                                int32_t kdt, uint64_t b, uint64_t e,
+                                                uint32_t row_limit, uint64_t* bits,
+                                                const HotLds& H) {
+  const uint64_t n = e - b;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t r0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r0 - lane < n;
+       r0 += stride * kProbeUnroll) {
+    uint64_t idx[kProbeUnroll];
+    bool ok[kProbeUnroll], hit[kProbeUnroll];
+    int64_t kv[kProbeUnroll];
+    unsigned long long k[kProbeUnroll];
+    static_for<kProbeUnroll>([&](auto I) {
+      constexpr int u = decltype(I)::value;
+      const uint64_t r = r0 + u * stride;
+      ok[u] = r < n;
+      idx[u] = b + (ok[u] ? r : r0 - lane);  // r0 - lane < n: a row of the range
+      hit[u] = false;
+    });
+    load_rows(keys, kdt, idx, ok, kv);
+    static_for<kProbeUnroll>([&](auto I) {
+      k[decltype(I)::value] = (unsigned long long)kv[decltype(I)::value];
+    });
+    probe_keys<HOT>(T, k, ok, [&](auto I, uint32_t row, uint32_t) {
+      if (!LIMIT || row < row_limit) hit[decltype(I)::value] = true;
+    }, H);
+    static_for<kProbeUnroll>([&](auto I) {
+      constexpr int u = decltype(I)::value;
+      const uint64_t w0 = r0 - lane + u * stride;  // the wave's first row of group u
+      const unsigned long long word = __ballot(hit[u]);
+      if (lane == 0 && w0 < n) bits[w0 >> 6] = word;
+    });
+  }
+}
+
+// HD.hdr non-null: the launch gives kHotLdsBytes of dynamic LDS for the image.
+template <bool LIMIT>
+__global__ void __launch_bounds__(256) k_probe_bits(JoinTableDev T, HotDev HD, const void* keys,
+                                                    int32_t kdt, uint64_t b, uint64_t e,
+                                                    uint32_t row_limit, uint64_t* bits) {
+  extern __shared__ __align__(16) unsigned long long dyn[];
+  T.has_dup = 0;  // unique keys: stop at the first match
+  if (hot_in_use(HD)) {
+    const HotLds H = hot_to_lds(HD, dyn);
+    __syncthreads();
+    probe_bits_rows<true, LIMIT>(T, keys, kdt, b, e, row_limit, bits, H);
+  } else {
+    probe_bits_rows<false, LIMIT>(T, keys, kdt, b, e, row_limit, bits, HotLds{nullptr, nullptr});
   }
 }
 
@@ -401,10 +679,10 @@ __device__ __forceinline__ void i128_atomic_add(unsigned long long* lo, unsigned
 // Probes rows i_start, i_start + stride, ... below e (kProbeUnroll rows per lane at
 // a time) and calls add(group, value) per match inside the query's build prefix;
 // matches whose group falls outside [0, ngroups) count in oor.
-template <typename Add>
+template <bool HOT, typename Add>
 __device__ __forceinline__ void probe_rows(const JoinTableDev& T, const ProbeAggDev& Q,
                                            uint64_t i_start, uint64_t e, uint64_t stride,
-                                           Add&& add, uint32_t& oor) {
+                                           Add&& add, uint32_t& oor, const HotLds& H) {
   for (uint64_t i0 = i_start; i0 < e; i0 += stride * kProbeUnroll) {
     uint64_t idx[kProbeUnroll];
     bool ok[kProbeUnroll];
@@ -424,20 +702,20 @@ __device__ __forceinline__ void probe_rows(const JoinTableDev& T, const ProbeAgg
       if (Q.group_by != TGP_JOIN_GROUP_PROBE_COL) g[u] = 0;
     }
     if (Q.group_by == TGP_JOIN_GROUP_BUILD_PAYLOAD) {
-      probe_keys(T, k, ok, [&](auto I, uint32_t row, uint32_t pay) {
+      probe_keys<HOT>(T, k, ok, [&](auto I, uint32_t row, uint32_t pay) {
         if (row >= Q.row_limit) return;
         const int64_t gg = (int64_t)(int32_t)pay - Q.key_base;
         if (gg < 0 || (uint64_t)gg >= Q.ngroups) { ++oor; return; }
         add((uint64_t)gg, v[decltype(I)::value]);
-      });
+      }, H);
     } else {
-      probe_keys(T, k, ok, [&](auto I, uint32_t row, uint32_t) {
+      probe_keys<HOT>(T, k, ok, [&](auto I, uint32_t row, uint32_t) {
         if (row >= Q.row_limit) return;
         constexpr int u = decltype(I)::value;
         const int64_t gg = Q.group_by == TGP_JOIN_GROUP_PROBE_COL ? g[u] - Q.key_base : 0;
         if (gg < 0 || (uint64_t)gg >= Q.ngroups) { ++oor; return; }
         add((uint64_t)gg, v[u]);
-      });
+      }, H);
     }
   }
 }
@@ -528,9 +806,9 @@ __global__ void __launch_bounds__(256) k_probe_agg(JoinTableDev T, ProbeAggDev Q
   }
   uint32_t oor = 0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  probe_rows(T, Q, Q.b + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x, Q.e, stride,
-             [&](uint64_t g, int64_t v) { lds_or_global_add<USE_LDS, EXACT>(lt, Q, g, v); },
-             oor);
+  probe_rows<false>(T, Q, Q.b + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x, Q.e, stride,
+                    [&](uint64_t g, int64_t v) { lds_or_global_add<USE_LDS, EXACT>(lt, Q, g, v); },
+                    oor, HotLds{nullptr, nullptr});
   if (USE_LDS) {
     __syncthreads();
     flush_lds_tables<EXACT>(ltab, region, nwt, rshift, Q);
@@ -582,8 +860,9 @@ __global__ void __launch_bounds__(256) k_probe_agg_batch(const ProbeBatchQuery* 
     const uint64_t region = (3 * Q.ngroups) << rshift;
     LdsTab lt{ltab + wave * region, rshift, lane & ((1u << rshift) - 1)};
     uint32_t oor = 0;
-    probe_rows(T, Q, b + threadIdx.x, b + n, blockDim.x,
-               [&](uint64_t g, int64_t v) { lds_or_global_add<true, EXACT>(lt, Q, g, v); }, oor);
+    probe_rows<false>(T, Q, b + threadIdx.x, b + n, blockDim.x,
+                      [&](uint64_t g, int64_t v) { lds_or_global_add<true, EXACT>(lt, Q, g, v); },
+                      oor, HotLds{nullptr, nullptr});
     if (oor && Q.out_oor) atomicAdd(Q.out_oor, (unsigned long long)oor);
     __syncthreads();
     flush_lds_tables<EXACT>(ltab, region, 4, rshift, Q);
@@ -742,7 +1021,9 @@ __device__ __forceinline__ uint64_t row_lo(const ProbeAggDev& Q, uint64_t w, uin
   return Q.b + (Q.e - Q.b) * w / G;
 }
 
+// bits non-null: only rows whose bit (i - Q.b) is set are counted (k_part_scatter_bits).
 __global__ void __launch_bounds__(256) k_part_hist(ProbeAggDev Q, uint32_t nparts,
+                                                   const uint64_t* bits,
                                                    unsigned long long* mat) {
   extern __shared__ __align__(16) uint32_t h[];  // nparts + 1 counters
   const uint64_t G = gridDim.x;
@@ -760,6 +1041,12 @@ __global__ void __launch_bounds__(256) k_part_hist(ProbeAggDev Q, uint32_t npart
       idx[u] = ok[u] ? i0 + (uint64_t)u * blockDim.x : i0;
     });
     load_rows(Q.grp, Q.gdt, idx, ok, g);
+    if (bits)
+      static_for<U>([&](auto I) {
+        constexpr int u = decltype(I)::value;
+        const uint64_t r = idx[u] - Q.b;
+        ok[u] = ok[u] && ((bits[r >> 6] >> (r & 63)) & 1);
+      });
     static_for<U>([&](auto I) {
       constexpr int u = decltype(I)::value;
       if (ok[u]) atomicAdd(&h[part_of(Q, g[u], nparts)], 1u);
@@ -777,17 +1064,16 @@ __global__ void __launch_bounds__(256) k_part_hist(ProbeAggDev Q, uint32_t npart
 // rank.  With hundreds of partitions a tile holds only a few rows of each, so
 // sorting the tile first would not make the writes contiguous; the small LDS
 // footprint keeps 8 workgroups per CU to hide the probes' latency instead.
-__global__ void __launch_bounds__(256) k_part_scatter(JoinTableDev T, ProbeAggDev Q,
-                                                      uint32_t nparts,
-                                                      const unsigned long long* mat,
-                                                      Rec12* recs) {
-  extern __shared__ __align__(16) unsigned long long cursor[];        // nparts + 1
-  uint32_t* cnt = reinterpret_cast<uint32_t*>(cursor + nparts + 1);  // nparts + 1
+// BITS: the matches come from `bits` (bit i - Q.b set: probe row i has exactly one
+// match, a unique-key table probed beforehand by k_probe_bits) and only matched
+// rows get records; else every row is probed here and gets a record.
+template <bool BITS>
+__device__ __forceinline__ void scatter_rows(const JoinTableDev& T, const ProbeAggDev& Q,
+                                             uint32_t nparts, unsigned long long* cursor,
+                                             uint32_t* cnt, Rec12* recs, const uint64_t* bits) {
   constexpr int U = 4, R = kScatterRows;
   static_assert(R % U == 0, "rows per thread: whole probe groups");
   const uint64_t G = gridDim.x;
-  for (uint32_t p = threadIdx.x; p <= nparts; p += blockDim.x)
-    cursor[p] = mat[(uint64_t)p * G + blockIdx.x];
   const uint64_t lo = row_lo(Q, blockIdx.x, G), hi = row_lo(Q, blockIdx.x + 1, G);
   unsigned long long oor_multi = 0;  // matches of out-of-range rows with several matches
   for (uint64_t t0 = lo; t0 < hi; t0 += (uint64_t)R * blockDim.x) {
@@ -810,23 +1096,33 @@ __global__ void __launch_bounds__(256) k_part_scatter(JoinTableDev T, ProbeAggDe
         idx[u] = ok[u] ? i : t0;  // t0 < hi: a row of the share
         m[u] = 0;
       });
-      load_rows(Q.keys, Q.kdt, idx, ok, kv);
+      if constexpr (BITS) {
+        static_for<U>([&](auto I) {
+          constexpr int u = decltype(I)::value;
+          const uint64_t r = idx[u] - Q.b;
+          m[u] = ok[u] ? (uint32_t)((bits[r >> 6] >> (r & 63)) & 1) : 0u;
+        });
+      } else {
+        load_rows(Q.keys, Q.kdt, idx, ok, kv);
+      }
       load_rows(Q.val, Q.vdt, idx, ok, vv);
       load_rows(Q.grp, Q.gdt, idx, ok, g);
       static_for<U>([&](auto I) {
         constexpr int u = decltype(I)::value;
-        k[u] = (unsigned long long)kv[u];
+        if constexpr (!BITS) k[u] = (unsigned long long)kv[u];
         v[h * U + u] = vv[u];
         g[u] -= Q.key_base;
       });
-      probe_keys(T, k, ok, [&](auto I, uint32_t row, uint32_t) {
-        if (row < Q.row_limit) ++m[decltype(I)::value];
-      });
+      if constexpr (!BITS) {
+        probe_keys(T, k, ok, [&](auto I, uint32_t row, uint32_t) {
+          if (row < Q.row_limit) ++m[decltype(I)::value];
+        });
+      }
       static_for<U>([&](auto I) {
         constexpr int u = decltype(I)::value;
         constexpr int r = h * U + u;
         pr[r] = 0xffffffffu;
-        if (!ok[u]) return;
+        if (!ok[u] || (BITS && m[u] == 0)) return;
         const bool in = g[u] >= 0 && (uint64_t)g[u] < Q.ngroups;
         const uint32_t p = in ? (uint32_t)((uint64_t)g[u] >> kPartShift) : nparts;
         if (m[u] > 1) {  // duplicate build keys: matches x value straight into the result
@@ -855,6 +1151,30 @@ __global__ void __launch_bounds__(256) k_part_scatter(JoinTableDev T, ProbeAggDe
     for (uint32_t p = threadIdx.x; p <= nparts; p += blockDim.x) cursor[p] += cnt[p];
   }
   if (oor_multi && Q.out_oor) atomicAdd(Q.out_oor, oor_multi);
+}
+
+__global__ void __launch_bounds__(256) k_part_scatter(JoinTableDev T, ProbeAggDev Q,
+                                                      uint32_t nparts,
+                                                      const unsigned long long* mat,
+                                                      Rec12* recs) {
+  extern __shared__ __align__(16) unsigned long long cursor[];        // nparts + 1
+  uint32_t* cnt = reinterpret_cast<uint32_t*>(cursor + nparts + 1);  // nparts + 1
+  for (uint32_t p = threadIdx.x; p <= nparts; p += blockDim.x)
+    cursor[p] = mat[(uint64_t)p * gridDim.x + blockIdx.x];
+  // the tile loop's first barrier orders these LDS writes before any read
+  scatter_rows<false>(T, Q, nparts, cursor, cnt, recs, nullptr);
+}
+
+__global__ void __launch_bounds__(256) k_part_scatter_bits(ProbeAggDev Q, uint32_t nparts,
+                                                           const uint64_t* bits,
+                                                           const unsigned long long* mat,
+                                                           Rec12* recs) {
+  extern __shared__ __align__(16) unsigned long long cursor[];        // nparts + 1
+  uint32_t* cnt = reinterpret_cast<uint32_t*>(cursor + nparts + 1);  // nparts + 1
+  for (uint32_t p = threadIdx.x; p <= nparts; p += blockDim.x)
+    cursor[p] = mat[(uint64_t)p * gridDim.x + blockIdx.x];
+  JoinTableDev none{};
+  scatter_rows<true>(none, Q, nparts, cursor, cnt, recs, bits);
 }
 
 // One workgroup per partition (partition nparts: the out-of-range rows, whose
@@ -1066,6 +1386,36 @@ static unsigned grid_for(uint64_t items, uint64_t cap = 8192) {
   return (unsigned)(g ? g : 1);
 }
 
+// Hot-key image sizes: tables below kHotMinTableBytes stay in the L2s and the
+// MALL anyway, and below kHotMinProbes the sample costs more than it saves.
+constexpr uint64_t kHotMinTableBytes = 8ull << 20;
+constexpr uint64_t kHotMinProbes = 4ull << 20;
+
+// Queues the hot-key image of probe keys [b, e) of t into `region`
+// (kHotScratchBytes, 256-byte aligned) and returns the kernels' view of it.
+static int32_t build_hot(hipStream_t s, const JoinTableDev& T, const void* keys, int32_t kdt,
+                         uint64_t b, uint64_t e, void* region, HotDev* out) {
+  uint8_t* p = static_cast<uint8_t*>(region);
+  HotScratch S;
+  S.map_key = reinterpret_cast<unsigned long long*>(p);
+  p += kHotMapSlots * 8ull;
+  S.map_cnt = reinterpret_cast<uint32_t*>(p);
+  p += kHotMapSlots * 4ull;
+  S.list = reinterpret_cast<unsigned long long*>(p);
+  p += kHotEntries * 8ull;
+  S.key = reinterpret_cast<unsigned long long*>(p);
+  p += kHotEntries * 8ull;
+  S.val = reinterpret_cast<uint2*>(p);
+  p += kHotEntries * 8ull;
+  S.hdr = reinterpret_cast<unsigned long long*>(p);
+  k_hot_clear<<<256, 256, 0, s>>>(S);
+  k_hot_sample<<<kHotSampleChunks, 256, 0, s>>>(S, keys, kdt, b, e);
+  k_hot_image<<<kHotEntries / 256, 256, 0, s>>>(S, T);
+  TGP_HIP(hipGetLastError());
+  *out = HotDev{S.key, S.val, S.hdr};
+  return TGP_OK;
+}
+
 }  // namespace tgp
 
 using namespace tgp;
@@ -1105,6 +1455,15 @@ static int32_t check_key_column(const tgp_column* c, const char* what) {
   TGP_REQUIRE(dtype_width(c->dtype) > 0, "%s: unknown dtype", what);
   TGP_REQUIRE(c->len == 0 || c->data != nullptr, "%s: null data", what);
   return TGP_OK;
+}
+
+// Whether a probe of n rows of t takes the hot-key image (unique-key single-column
+// tables only: an image entry is then a key's only match).
+static bool want_hot(tgp_ctx* ctx, const tgp_hash_table* t, uint64_t n) {
+  const int64_t mode = ctx->tune_join_hot.load();
+  if (mode == 1 || t->has_dup || t->bkeys.n > 1 || n == 0) return false;
+  if (mode == 2) return true;
+  return n >= kHotMinProbes && t->cap * sizeof(Slot) >= kHotMinTableBytes;
 }
 
 static int32_t require_single_key(const tgp_hash_table* t, const char* what) {
@@ -1380,9 +1739,17 @@ int32_t tgp_hash_probe_count(tgp_ctx* ctx, const tgp_hash_table* t, const tgp_co
   // 0.59 at 16 per CU and 0.72 at 4
   const unsigned grid = grid_for((row_end - row_begin + kProbeUnroll - 1) / kProbeUnroll,
                                  (uint64_t)persistent_grid(ctx, 8));
-  k_probe_count<<<grid, 256, 0, pick_stream(ctx, stream)>>>(t->dev(), probe_keys->data, probe_keys->dtype,
-                                              row_begin, row_end,
-                                              reinterpret_cast<unsigned long long*>(d_count));
+  hipStream_t s = pick_stream(ctx, stream);
+  HotDev hd{nullptr, nullptr, nullptr};
+  if (want_hot(ctx, t, row_end - row_begin)) {
+    void* region = nullptr;
+    TGP_TRY(get_scratch(ctx, s, kHotScratchBytes, &region));
+    TGP_TRY(build_hot(s, t->dev(), probe_keys->data, probe_keys->dtype, row_begin, row_end,
+                      region, &hd));
+  }
+  k_probe_count<<<grid, 256, hd.hdr ? kHotLdsBytes : 0, s>>>(
+      t->dev(), hd, probe_keys->data, probe_keys->dtype, row_begin, row_end,
+      reinterpret_cast<unsigned long long*>(d_count));
   TGP_HIP(hipGetLastError());
   return TGP_OK;
 }
@@ -1526,30 +1893,38 @@ int32_t tgp_hash_probe_agg(tgp_ctx* ctx, const tgp_join_agg_query* q, const tgp_
   // duplicate build keys: a probe row adds once per match, so the split lo32/hi32
   // words (exact below 2^32 additions each) give way to carry-exact int128 words
   const bool exact = q->table->has_dup != 0;
+  const JoinTableDev T = q->table->dev();
   if (rshift >= 0) {
     const uint64_t lds = (tab << rshift) * (shared_table ? 1 : 4);
     if (exact)
-      k_probe_agg<true, true><<<grid, 256, lds, s>>>(q->table->dev(), Q, (uint32_t)rshift,
-                                                     shared_table);
+      k_probe_agg<true, true><<<grid, 256, lds, s>>>(T, Q, (uint32_t)rshift, shared_table);
     else
-      k_probe_agg<true, false><<<grid, 256, lds, s>>>(q->table->dev(), Q, (uint32_t)rshift,
-                                                      shared_table);
+      k_probe_agg<true, false><<<grid, 256, lds, s>>>(T, Q, (uint32_t)rshift, shared_table);
   } else if (q->group_by == TGP_JOIN_GROUP_PROBE_COL && nparts < kMaxParts &&
              ctx->tune_join_partition.load() == 0) {
     TGP_REQUIRE(n < (1ull << 32),
                 "tgp_hash_probe_agg: a partitioned call takes fewer than 2^32 probe rows");
-    // scratch: mat[(nparts + 1) * G + 1] | records[n]
+    // A unique-key table is probed first into one match bit per row, and only
+    // matched rows are partitioned (the histogram and the scatter then stream
+    // the columns without probing); a table with duplicate keys is probed in
+    // the scatter, which adds the rows of several matches directly.
+    const bool use_bits = !q->table->has_dup;
+    const bool hot = use_bits && want_hot(ctx, q->table, n);
+    // scratch: mat[(nparts + 1) * G + 1] | records[n] | match bits | hot-key image
     const uint64_t G = (uint64_t)persistent_grid(ctx, 8) < (n + kPartTile - 1) / kPartTile
                            ? (uint64_t)persistent_grid(ctx, 8)
                            : (n + kPartTile - 1) / kPartTile;
     const uint64_t mat_n = (nparts + 1) * G + 1;
     const uint64_t nsb = (mat_n - 1 + kScanChunk - 1) / kScanChunk + 1;  // scan block sums
     const uint64_t head = (8 * (mat_n + nsb) + 255) & ~255ull;
-    const uint64_t bytes = head + sizeof(Rec12) * n + 256;
+    const uint64_t boff = (head + sizeof(Rec12) * n + 256 + 255) & ~255ull;
+    const uint64_t hoff = (boff + (use_bits ? 8 * ((n + 63) / 64 + 4) : 0) + 255) & ~255ull;
+    const uint64_t bytes = hoff + (hot ? kHotScratchBytes : 0);
     void* scr = nullptr;
     TGP_TRY(get_scratch(ctx, s, bytes, &scr));
     auto* mat = static_cast<unsigned long long*>(scr);
     auto* recs = reinterpret_cast<Rec12*>(static_cast<uint8_t*>(scr) + head);
+    auto* bits = use_bits ? reinterpret_cast<uint64_t*>(static_cast<uint8_t*>(scr) + boff) : nullptr;
     const uint32_t np = (uint32_t)nparts;
     // the aggregation adds to the result (several-match rows arrive first)
     {
@@ -1559,19 +1934,36 @@ int32_t tgp_hash_probe_agg(tgp_ctx* ctx, const tgp_join_agg_query* q, const tgp_
       const uint64_t zn[3] = {q->ngroups, 2 * q->ngroups, Q.out_oor ? 1ull : 0ull};
       TGP_TRY(zero_ranges(s, 3, zp, zn));
     }
-    k_part_hist<<<(unsigned)G, 256, (np + 1) * 4, s>>>(Q, np, mat);
+    if (use_bits) {
+      HotDev hd{nullptr, nullptr, nullptr};
+      if (hot)
+        TGP_TRY(build_hot(s, T, Q.keys, Q.kdt, Q.b, Q.e, static_cast<uint8_t*>(scr) + hoff, &hd));
+      const unsigned bgrid = grid_for((n + kProbeUnroll - 1) / kProbeUnroll,
+                                      (uint64_t)persistent_grid(ctx, 8));
+      const uint64_t blds = hd.hdr ? kHotLdsBytes : 0;
+      if (Q.row_limit == ~0u)
+        k_probe_bits<false><<<bgrid, 256, blds, s>>>(T, hd, Q.keys, Q.kdt, Q.b, Q.e, Q.row_limit,
+                                                     bits);
+      else
+        k_probe_bits<true><<<bgrid, 256, blds, s>>>(T, hd, Q.keys, Q.kdt, Q.b, Q.e, Q.row_limit,
+                                                    bits);
+    }
+    k_part_hist<<<(unsigned)G, 256, (np + 1) * 4, s>>>(Q, np, bits, mat);
     TGP_TRY(scan_u64(s, reinterpret_cast<uint64_t*>(mat), mat_n - 1,
                      reinterpret_cast<uint64_t*>(mat + mat_n)));
     const uint32_t lds = (np + 1) * 12 + 16;
-    k_part_scatter<<<(unsigned)G, 256, lds, s>>>(q->table->dev(), Q, np, mat, recs);
+    if (use_bits)
+      k_part_scatter_bits<<<(unsigned)G, 256, lds, s>>>(Q, np, bits, mat, recs);
+    else
+      k_part_scatter<<<(unsigned)G, 256, lds, s>>>(T, Q, np, mat, recs);
     k_part_agg<<<np + 1, 1024, 0, s>>>(Q, np, G, mat, recs);
   } else {
     TGP_REQUIRE(q->row_end - q->row_begin < (1ull << 32),
                 "tgp_hash_probe_agg: a global-table call takes fewer than 2^32 probe rows");
     if (exact) {
-      k_probe_agg<false, true><<<grid, 256, 0, s>>>(q->table->dev(), Q, 0, 0);
+      k_probe_agg<false, true><<<grid, 256, 0, s>>>(T, Q, 0, 0);
     } else {
-      k_probe_agg<false, false><<<grid, 256, 0, s>>>(q->table->dev(), Q, 0, 0);
+      k_probe_agg<false, false><<<grid, 256, 0, s>>>(T, Q, 0, 0);
       TGP_HIP(hipGetLastError());
       k_split_finalize<<<grid_for(q->ngroups), 256, 0, s>>>(Q.out_sums, q->ngroups);
     }

@@ -361,6 +361,10 @@ int32_t tgp_tune_set(tgp_ctx* ctx, int32_t key, int64_t value) {
       TGP_REQUIRE(value <= 32, "TGP_TUNE_GROUP_TAG_BITS is 0..32");
       ctx->tune_group_tag_bits.store(value);
       return TGP_OK;
+    case TGP_TUNE_JOIN_HOT:
+      TGP_REQUIRE(value <= 2, "TGP_TUNE_JOIN_HOT is 0, 1 or 2");
+      ctx->tune_join_hot.store(value);
+      return TGP_OK;
   }
   set_error("tgp_tune_set: unknown key %d", key);
   return TGP_ERR_INVALID;
@@ -379,6 +383,7 @@ int32_t tgp_tune_get(tgp_ctx* ctx, int32_t key, int64_t* value) {
     case TGP_TUNE_JOIN_PACK_BITS: *value = ctx->tune_join_pack_bits.load(); return TGP_OK;
     case TGP_TUNE_DISTINCT_SHRINK: *value = ctx->tune_distinct_shrink.load(); return TGP_OK;
     case TGP_TUNE_GROUP_TAG_BITS: *value = ctx->tune_group_tag_bits.load(); return TGP_OK;
+    case TGP_TUNE_JOIN_HOT: *value = ctx->tune_join_hot.load(); return TGP_OK;
   }
   set_error("tgp_tune_get: unknown key %d", key);
   return TGP_ERR_INVALID;

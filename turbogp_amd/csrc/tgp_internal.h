@@ -105,6 +105,7 @@ struct tgp_ctx {
   std::atomic<int64_t> tune_join_pack_bits{0};   // multi-column join keys: 0 = 64-bit packing
   std::atomic<int64_t> tune_distinct_shrink{0};  // exact distinct: sketch-sized set / 2^n
   std::atomic<int64_t> tune_group_tag_bits{0};   // hash GROUP BY: 0 = auto; n = verified form, n-bit tags
+  std::atomic<int64_t> tune_join_hot{0};         // join hot-key cache: 0 auto, 1 off, 2 on
 };
 
 namespace tgp {
