@@ -335,9 +335,11 @@ class DeviceCounter:
 
 
 def bench_generic(ctx, table, args):
-    """The generic tile engine (queries of no compiled shape) on 60M rows: Q1 with
-    price as int32 (another column layout; 6 groups, register strategy) and Q1
-    grouped by (dt, flagb) (5052 groups, result table)."""
+    """Queries of no shape compiled ahead of time, on 60M rows: Q1 with price as
+    int32 (another column layout; 6 groups) through the generic tile engine and
+    through its shape-specialised kernel compiled at run time (the compile is
+    timed apart, before the steps), and Q1 grouped by (dt, flagb) (5052 groups,
+    generic engine's result table; too many groups for a specialised form)."""
     from turbogp_amd import Event, GroupAggBuffers, _capi as A
     from turbogp_amd import workloads as W
     import ctypes as C
@@ -345,31 +347,67 @@ def bench_generic(ctx, table, args):
                                10_500_000)
     out = {}
     ev = [Event(ctx) for _ in range(2)]
+
+    def jit_info():
+        c, f, l, ms = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_double()
+        A.check(A.lib().tgp_jit_info(C.byref(c), C.byref(f), C.byref(l), C.byref(ms)), "jit")
+        return c.value, f.value, ms.value
+
+    def timed(q, res):
+        for _ in range(args.warmup):
+            ctx.group_agg_async(q, res)
+        ts = []
+        for _ in range(args.steps):
+            ev[0].record()
+            ctx.group_agg_async(q, res)
+            ev[1].record()
+            ts.append(ev[0].elapsed_ms(ev[1]))
+        return statistics.mean(ts)
+
     try:
-        for name, form, bpr in (("q1_price_i32_generic", "price_i32", 34),
-                                ("q1_5052_groups_generic", "wide_groups", 38)):
+        for name, form, bpr in (("q1_price_i32", "price_i32", 34),
+                                ("q1_5052_groups", "wide_groups", 38)):
             q = W.mixed_batch_queries(table, price32, None, [(form, 0, SCAN_ROWS, 0, "ab", 0)])[0]
             g = C.c_uint64()
             A.check(A.lib().tgp_group_agg_ngroups(C.byref(q), C.byref(g)), "ngroups")
             st, sh = C.c_int32(), C.c_int32()
             A.check(A.lib().tgp_group_agg_explain(C.byref(q), C.byref(st), C.byref(sh)), "explain")
             res = GroupAggBuffers(ctx, g.value, q.naggs)
-            for _ in range(args.warmup):
+            runs = [("generic", 1)]
+            if sh.value == A.SPEC_SHAPE_RUNTIME:  # compiled before the first call returns
+                runs.append(("compiled_at_run_time", 2))
+            for path, mode in runs:
+                ctx.tune(A.TUNE_JIT, mode)
+                extra = {}
+                if mode == 2:
+                    c0, f0, ms0 = jit_info()
+                    t0 = time.perf_counter()
+                    ctx.group_agg_async(q, res)
+                    ctx.sync()
+                    c1, f1, ms1 = jit_info()
+                    assert f1 == f0, "run-time compile failed"
+                    extra = {"first_call_ms_wall": round((time.perf_counter() - t0) * 1e3, 1),
+                             "compile_ms": round(ms1 - ms0, 1), "shapes_compiled": c1 - c0}
+                ms = timed(q, res)
+                gbs = SCAN_ROWS * bpr / 1e9 / (ms / 1e3)
+                out[f"{name}_{path}"] = {"ms": round(ms, 4), "GB_s": round(gbs, 1),
+                                         "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                                         "bytes_per_row": bpr, "groups": int(g.value),
+                                         "strategy": int(st.value), **extra}
+                log(f"{name} {path}: {ms:.3f} ms, {gbs:.0f} GB/s {extra}")
+            if len(runs) == 2:
+                ctx.tune(A.TUNE_JIT, 2)
                 ctx.group_agg_async(q, res)
-            ts = []
-            for _ in range(args.steps):
-                ev[0].record()
+                want = res.fetch()
+                ctx.tune(A.TUNE_JIT, 1)
                 ctx.group_agg_async(q, res)
-                ev[1].record()
-                ts.append(ev[0].elapsed_ms(ev[1]))
+                got = res.fetch()
+            ctx.tune(A.TUNE_JIT, 0)
+            if len(runs) == 2:
+                assert got.sums() == want.sums() and list(got.count) == list(want.count), name
             res.free()
-            ms = statistics.mean(ts)
-            gbs = SCAN_ROWS * bpr / 1e9 / (ms / 1e3)
-            out[name] = {"ms": round(ms, 4), "GB_s": round(gbs, 1),
-                         "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_row": bpr,
-                         "groups": int(g.value), "strategy": int(st.value)}
-            log(f"{name}: {ms:.3f} ms, {gbs:.0f} GB/s")
     finally:
+        ctx.tune(A.TUNE_JIT, 0)
         for e in ev:
             e.free()
         price32.free()

@@ -47,8 +47,10 @@
 #ifndef TURBOGP_GFX950_H
 #define TURBOGP_GFX950_H
 
+#ifndef __HIPCC_RTC__ /* hipRTC (run-time compiled kernels) predefines these types */
 #include <stddef.h>
 #include <stdint.h>
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -231,11 +233,17 @@ enum tgp_tune_key {
                                    verification and keeps only n bits of each slot's hash tag,
                                    so that distinct tuples collide (tests); 0 (default): exact
                                    packing when the key ranges fit 63 bits */
-  TGP_TUNE_JOIN_HOT = 10        /* hot-key cache of join probes (a sampled image of the most
+  TGP_TUNE_JOIN_HOT = 10,       /* hot-key cache of join probes (a sampled image of the most
                                    frequent probe keys' entries, looked up in LDS before the
                                    table): 0 (default) for unique-key single-column tables of
                                    >= 8 MiB probed by >= 4M rows; 1 off; 2 for every probe of a
                                    unique-key single-column table (tests) */
+  TGP_TUNE_JIT = 11             /* queries with a shape-specialised form (tgp_group_agg_explain
+                                   spec_shape >= 0) but no kernel compiled ahead of time: 0
+                                   (default) compile that form at run time (hipRTC, about a
+                                   second per shape, cached for the process) for queries of
+                                   >= 2^20 rows, run smaller ones on the generic engine; 1
+                                   never compile; 2 always (tests) */
 };
 int32_t tgp_tune_set(tgp_ctx* ctx, int32_t key, int64_t value);
 int32_t tgp_tune_get(tgp_ctx* ctx, int32_t key, int64_t* value);
@@ -336,9 +344,17 @@ int32_t tgp_group_agg(tgp_ctx* ctx, const tgp_agg_query* q, const tgp_group_resu
 /* Plan of a query without running it (host only): the accumulation strategy
  * of the generic engine (0 registers/1 group, 1 registers/<=8 groups, 2 LDS
  * table, 3 global table, 4 global table of int128 words for ranges of 2^32 rows or
- * more) and the index of the shape-specialised kernel that
- * would run it (-1: none; the generic engine runs it). */
+ * more) and the index of the shape-specialised kernel compiled ahead of time that
+ * would run it, or TGP_SPEC_SHAPE_RUNTIME when the query has a specialised form
+ * that is compiled at run time (see TGP_TUNE_JIT), or -1 (none: the generic
+ * engine runs it). */
+#define TGP_SPEC_SHAPE_RUNTIME 1000
 int32_t tgp_group_agg_explain(const tgp_agg_query* q, int32_t* strategy, int32_t* spec_shape);
+/* Counters of the run-time compiled specialised kernels of this process: shapes
+ * compiled, compiles that failed (those queries run on the generic engine),
+ * launches, and the milliseconds spent compiling.  No counterpart in the
+ * reference (its kernels are monomorphised by rustc ahead of time). */
+int32_t tgp_jit_info(uint64_t* compiled, uint64_t* failed, uint64_t* launches, double* compile_ms);
 /* Number of groups the query's key space spans (prod of card). */
 int32_t tgp_group_agg_ngroups(const tgp_agg_query* q, uint64_t* out_ngroups);
 

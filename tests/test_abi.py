@@ -95,6 +95,9 @@ def test_benchmark_queries_take_the_specialised_kernels():
     assert _explain((names, [(0, _capi.GE, 731, False)], parts, aggs))[1] == -1
     # strategies of the generic engine
     assert _explain(W.q1_spec(group="ab"))[0] == 1 and _explain(W.q1_spec(group="month"))[0] == 2
+    # a specialised form without an ahead-of-time kernel is compiled at run time
+    names, preds, parts, aggs = W.q1_spec(group="a")
+    assert _explain((names, preds, parts, aggs[:3]))[1] == _capi.SPEC_SHAPE_RUNTIME
 
 
 def test_batch_and_serve_entry_points_validate_without_a_device():

@@ -34,7 +34,8 @@ AGG_MINMAX = {"min": AGG_MIN, "max": AGG_MAX, "min_u64": AGG_MIN_U64, "max_u64":
 JOIN_GROUP_BUILD_PAYLOAD, JOIN_GROUP_PROBE_COL, JOIN_GROUP_NONE = 0, 1, 2
 TUNE_WG_PER_CU, TUNE_GRID_MAX, TUNE_SPECIALIZE, TUNE_SPEC_KERNEL, TUNE_CHUNK = 0, 1, 2, 3, 4
 TUNE_SLICES_PER_WAVE, TUNE_JOIN_ATOMICS, TUNE_JOIN_PACK_BITS, TUNE_DISTINCT_SHRINK = 5, 6, 7, 8
-TUNE_GROUP_TAG_BITS, TUNE_JOIN_HOT = 9, 10
+TUNE_GROUP_TAG_BITS, TUNE_JOIN_HOT, TUNE_JIT = 9, 10, 11
+SPEC_SHAPE_RUNTIME = 1000
 JOIN_INNER, JOIN_LEFT = 0, 1
 JOIN_NO_MATCH = 0xFFFFFFFF
 MAX_JOIN_KEYS = 4
@@ -206,6 +207,8 @@ SIGNATURES = {
     "tgp_group_agg": (I32_, [P, C.POINTER(AggQuery), C.POINTER(GroupResult), P]),
     "tgp_group_agg_ngroups": (I32_, [C.POINTER(AggQuery), C.POINTER(U64_)]),
     "tgp_group_agg_explain": (I32_, [C.POINTER(AggQuery), C.POINTER(I32_), C.POINTER(I32_)]),
+    "tgp_jit_info": (I32_, [C.POINTER(U64_), C.POINTER(U64_), C.POINTER(U64_),
+                            C.POINTER(C.c_double)]),
     "tgp_hash_group_agg_sync": (I32_, [P, C.POINTER(HashGroupQuery), C.POINTER(HashGroupResult),
                                        C.POINTER(U64_), P]),
     "tgp_synth_uniform": (I32_, [P, C.POINTER(Column), U64_, U64_, I64_, I64_, U64_, U64_, P]),

@@ -29,10 +29,12 @@ struct GenericItem {
 // Queues the generic items (one persistent launch; 2 stage buffers of stage_max
 // bytes + an LDS table of table_max bytes per workgroup) and the MIN/MAX
 // finalisation of the plans in d_minmax (minmax_words: the largest groups x aggs).
+#ifndef __HIPCC_RTC__
 int32_t launch_batch_generic(tgp_ctx* ctx, const DevPlan* d_plans, const GenericItem* d_items,
                              uint64_t nitems, unsigned long long* counter, uint32_t stage_max,
                              uint32_t table_max, const uint32_t* d_minmax, uint32_t nminmax,
                              uint64_t minmax_words, hipStream_t s);
+#endif
 
 // LDS table limit of the kAggLds strategy, and of its copies.
 constexpr uint32_t kLdsTableBytes = 48 * 1024;
@@ -51,6 +53,7 @@ int32_t plan_agg(const tgp_agg_query* q, DevPlan* P, PlanInfo* info);
 // Round-up multiplier for unsigned 32-bit division by d >= 2.
 void divisor_magic(uint32_t d, uint32_t* magic, int32_t* shift);
 
+#ifndef __HIPCC_RTC__
 // Workgroups per CU for the persistent kernels: as many as LDS and registers
 // allow, at most 4.
 template <typename K>
@@ -74,5 +77,6 @@ inline int32_t launch_tiles(tgp_ctx* ctx, K kern, const DevPlan& P, uint32_t lds
   TGP_HIP(hipGetLastError());
   return TGP_OK;
 }
+#endif  // __HIPCC_RTC__
 
 }  // namespace tgp

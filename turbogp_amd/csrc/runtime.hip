@@ -365,6 +365,10 @@ int32_t tgp_tune_set(tgp_ctx* ctx, int32_t key, int64_t value) {
       TGP_REQUIRE(value <= 2, "TGP_TUNE_JOIN_HOT is 0, 1 or 2");
       ctx->tune_join_hot.store(value);
       return TGP_OK;
+    case TGP_TUNE_JIT:
+      TGP_REQUIRE(value <= 2, "TGP_TUNE_JIT is 0, 1 or 2");
+      ctx->tune_jit.store(value);
+      return TGP_OK;
   }
   set_error("tgp_tune_set: unknown key %d", key);
   return TGP_ERR_INVALID;
@@ -384,6 +388,7 @@ int32_t tgp_tune_get(tgp_ctx* ctx, int32_t key, int64_t* value) {
     case TGP_TUNE_DISTINCT_SHRINK: *value = ctx->tune_distinct_shrink.load(); return TGP_OK;
     case TGP_TUNE_GROUP_TAG_BITS: *value = ctx->tune_group_tag_bits.load(); return TGP_OK;
     case TGP_TUNE_JOIN_HOT: *value = ctx->tune_join_hot.load(); return TGP_OK;
+    case TGP_TUNE_JIT: *value = ctx->tune_jit.load(); return TGP_OK;
   }
   set_error("tgp_tune_get: unknown key %d", key);
   return TGP_ERR_INVALID;

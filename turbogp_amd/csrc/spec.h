@@ -24,9 +24,10 @@
 //   - larger group counts use a workgroup LDS table with carry-exact atomics.
 #pragma once
 
-#include <type_traits>
-#include <utility>
+#ifndef __HIPCC_RTC__
+#include <cstring>
 #include <vector>
+#endif
 
 #include "plan.h"
 
@@ -362,11 +363,13 @@ inline constexpr Shape kShapes[] = {
 };
 inline constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
+#ifndef __HIPCC_RTC__
 inline bool shape_eq(const Shape& a, const Shape& b) {
   return std::memcmp(&a, &b, sizeof(Shape)) == 0;
 }
 
 // Builds the shape and arguments of q; false when q has no specialised form.
 bool extract_shape(const tgp_agg_query* q, Shape* s, SpecArgs* A);
+#endif
 
 }  // namespace tgp

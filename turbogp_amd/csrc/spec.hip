@@ -1,170 +1,18 @@
 // Shape-specialised scan-aggregate kernel (one query per launch) and the
 // planner's shape extraction.  Design notes are in spec.h.
-#include "wave_agg.h"
+#include "jit.h"
 
 namespace tgp {
 
 template <Shape S>
 __global__ void __launch_bounds__(kBlock) k_spec(SpecArgs A) {
-  extern __shared__ __align__(16) unsigned long long lmem[];
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  // table 0: per-wave histograms of kHistWords words; table 1: one workgroup table
-  // (counts[ngroups], then (lo, hi) sums) followed by nothing else.
-  unsigned long long* hist = S.table == 0 ? lmem + wave * kHistWords : lmem;
-  if constexpr (S.table == 0) {
-    hist[lane] = 0;
-  } else {
-    const uint64_t words = A.ngroups * (1 + 2 * (uint64_t)S.naggs);
-    for (uint64_t i = threadIdx.x; i < words; i += kBlock) lmem[i] = 0;
-    __syncthreads();
-  }
-  // lane-owned exact accumulator of histogram word `lane` (table 0)
-  uint64_t acc_lo = 0;
-  int64_t acc_hi = 0;
-  uint32_t oor = 0;
-  const uint64_t nwaves = (uint64_t)gridDim.x * (kBlock / kWave);
-  const uint64_t gw = (uint64_t)blockIdx.x * (kBlock / kWave) + wave;
-  uint64_t min_len = A.col_len[0];
-#pragma unroll
-  for (int s = 1; s < S.nslots; ++s) min_len = A.col_len[s] < min_len ? A.col_len[s] : min_len;
-  for (uint64_t k = gw; k < A.nslices; k += nwaves) {
-    const uint64_t slice0 = A.base0 + k * kSliceRows;
-    const uint64_t row = slice0 + lane * kSR;
-    const bool fast = slice0 + kSliceRows <= min_len;
-    SliceVals<S> v;
-    load_all<S, 0>(A, row, fast, v);
-    int32_t gid[kSR];
-    const uint32_t sel = eval_slice<S>(A, row, v, gid, oor);
-    if (!__ballot(sel != 0)) continue;
-#pragma unroll
-    for (int r = 0; r < kSR; ++r)
-      if ((sel >> r) & 1) {
-        if constexpr (S.table == 0) atomicAdd(&hist[gid[r] * (1 + S.naggs)], 1ull);
-        else atomicAdd(&hist[gid[r]], 1ull);
-      }
-    bool big = false;
-    accumulate_aggs<S, 0>(A, v, sel, gid, hist, big);
-    if constexpr (S.table == 0) {
-      // fold the slice's histogram into the lane-owned int128 accumulators
-      __builtin_amdgcn_s_waitcnt(0);
-      __builtin_amdgcn_wave_barrier();
-      const unsigned long long w = hist[lane];
-      hist[lane] = 0;
-      const uint64_t s = acc_lo + w;
-      // count words are non-negative; sum words are signed 64-bit partials
-      const bool is_count = (lane % (1 + S.naggs)) == 0;
-      acc_hi += (int64_t)(s < acc_lo) - ((!is_count && (int64_t)w < 0) ? 1 : 0);
-      acc_lo = s;
-      __builtin_amdgcn_wave_barrier();
-    }
-  }
-  // ---- flush to the result ----
-  if constexpr (S.table == 0) {
-    // workgroup reduction of the 4 waves' lane accumulators, then device atomics
-    __syncthreads();
-    unsigned long long* red = lmem + 4 * kHistWords;  // [wave][lane] (lo, hi)
-    red[(wave * 64 + lane) * 2] = acc_lo;
-    red[(wave * 64 + lane) * 2 + 1] = (unsigned long long)acc_hi;
-    __syncthreads();
-    if (threadIdx.x < A.words) {
-      uint64_t lo = 0;
-      int64_t hi = 0;
-      for (int w = 0; w < kBlock / kWave; ++w) {
-        const uint64_t l2 = red[(w * 64 + threadIdx.x) * 2];
-        const uint64_t s = lo + l2;
-        hi += (int64_t)red[(w * 64 + threadIdx.x) * 2 + 1] + (int64_t)(s < lo);
-        lo = s;
-      }
-      const uint32_t g = threadIdx.x / (1 + S.naggs), a = threadIdx.x % (1 + S.naggs);
-      if (a == 0) {
-        if (lo) atomicAdd(&A.out_count[g], (unsigned long long)lo);
-      } else {
-        atomic_add128(&A.out_sums[((uint64_t)g * S.naggs + (a - 1)) * 2], lo, hi);
-      }
-    }
-  } else {
-    __syncthreads();
-    for (uint64_t g = threadIdx.x; g < A.ngroups; g += kBlock) {
-      const unsigned long long c = lmem[g];
-      if (!c) continue;
-      atomicAdd(&A.out_count[g], c);
-      for (int a = 0; a < S.naggs; ++a) {
-        const uint64_t i = A.ngroups + (g * S.naggs + a) * 2;
-        atomic_add128(&A.out_sums[(g * S.naggs + a) * 2], lmem[i], (int64_t)lmem[i + 1]);
-      }
-    }
-  }
-  if (oor && A.out_oor) atomicAdd(A.out_oor, (unsigned long long)oor);
-}
-
-
-// Single-query kernel on the wave-level blocks of wave_agg.h.  With chunk == 0
-// wave w of the grid takes slices w, w + nwaves, ... (static interleave: all
-// waves sweep the table together, no atomics); with chunk > 0 the waves take
-// chunks of `chunk` slices from a device counter (a slow wave takes fewer).  A
-// wave accumulates in its replicated LDS histogram and flushes into the result
-// every 2^20 rows and at the end.  The LDS region of a wave is words << rshift
-// words (dynamic shared memory of 4 such regions per workgroup).
-template <Shape S>
-__device__ __forceinline__ void wave_rows(const SpecArgs& A, uint64_t s, uint64_t min_len,
-                                          unsigned long long* reg, uint32_t rshift,
-                                          uint32_t& oor, uint64_t& since) {
-  if (since + kSliceRows > kFlushRows) {
-    wave_flush(A.out_count, A.out_sums, A.words, S.naggs, reg, rshift);
-    since = 0;
-  }
-  SliceVals<S> v;
-  wave_load<S>(A, s, min_len, v);
-  wave_slice<S>(A, s, v, reg, rshift, oor);
-  since += kSliceRows;
+  spec_body<S>(A);
 }
 
 template <Shape S>
 __global__ void __launch_bounds__(kBlock) k_wave(SpecArgs A, uint32_t rshift, uint32_t chunk,
                                                  unsigned long long* counter) {
-  extern __shared__ __align__(16) unsigned long long lreg[];
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t region = A.words << rshift;
-  unsigned long long* reg = lreg + wave * region;
-  for (uint32_t w = lane; w < region; w += 64) reg[w] = 0;
-  __builtin_amdgcn_wave_barrier();
-  uint64_t min_len = A.col_len[0];
-#pragma unroll
-  for (int s = 1; s < S.nslots; ++s) min_len = A.col_len[s] < min_len ? A.col_len[s] : min_len;
-  uint32_t oor = 0;
-  uint64_t since = 0;  // rows added to the region since its last flush
-  if (chunk == 0) {
-    const uint64_t nwaves = (uint64_t)gridDim.x * (kBlock / kWave);
-    for (uint64_t s = (uint64_t)blockIdx.x * (kBlock / kWave) + wave; s < A.nslices; s += nwaves)
-      wave_rows<S>(A, s, min_len, reg, rshift, oor, since);
-  } else {
-    uint64_t k = grab_chunk(counter);
-    while (k * chunk < A.nslices) {
-      const uint64_t c0 = k * chunk;
-      const uint64_t c1 = c0 + chunk < A.nslices ? c0 + chunk : A.nslices;
-      const uint64_t next = grab_chunk(counter);  // in flight while this chunk runs
-      for (uint64_t s = c0; s < c1; ++s) wave_rows<S>(A, s, min_len, reg, rshift, oor, since);
-      k = next;
-    }
-  }
-  // workgroup flush: every wave folds its copies, then one thread per word adds
-  // the 4 waves' totals (as int128) into the result
-  wave_fold_copies(reg, A.words, rshift);
-  __syncthreads();
-  constexpr uint32_t W1 = 1 + S.naggs;
-  for (uint32_t w = threadIdx.x; w < A.words; w += kBlock) {
-    const bool is_count = w % W1 == 0;
-    uint64_t lo = 0;
-    int64_t hi = 0;
-    for (int v = 0; v < kBlock / kWave; ++v) {
-      const uint64_t x = lreg[v * region + w];
-      const uint64_t t = lo + x;
-      hi += (int64_t)(t < lo) - ((!is_count && (int64_t)x < 0) ? 1 : 0);
-      lo = t;
-    }
-    if (lo || hi) add_result_word(A.out_count, A.out_sums, w, S.naggs, lo, hi);
-  }
-  if (oor && A.out_oor) atomicAdd(A.out_oor, (unsigned long long)oor);
+  wave_body<S>(A, rshift, chunk, counter);
 }
 
 // Queries of at least this many slices (4M rows) take the replicated form, and
@@ -190,82 +38,124 @@ static int32_t zero_result(hipStream_t st, const SpecArgs& A, int naggs,
   return zero_ranges(st, 4, p, n);
 }
 
+// The kernels that run one shape: ahead-of-time template instances, or the
+// run-time compiled module functions of the same bodies (jit.hip).
+struct ShapeKernels {
+  const void* spec = nullptr;  // k_spec<S>
+  const void* wave = nullptr;  // k_wave<S>
+  JitShapeKernels jit;         // used when spec is null
+  int32_t occupancy(bool use_wave, uint32_t lds, int* per_cu) const {
+    if (spec) {
+      TGP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, use_wave ? wave : spec, kBlock,
+                                                           lds));
+    } else {
+      TGP_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(
+          per_cu, use_wave ? jit.wave : jit.spec, kBlock, lds));
+    }
+    return TGP_OK;
+  }
+  int32_t launch(bool use_wave, uint64_t grid, uint32_t lds, hipStream_t st, void** args) const {
+    if (spec) {
+      TGP_HIP(hipLaunchKernel(use_wave ? wave : spec, dim3((unsigned)grid), dim3(kBlock), args,
+                              lds, st));
+    } else {
+      TGP_HIP(hipModuleLaunchKernel(use_wave ? jit.wave : jit.spec, (unsigned)grid, 1, 1, kBlock,
+                                    1, 1, lds, st, args, nullptr));
+    }
+    return TGP_OK;
+  }
+};
+
+// Chooses the kernel form and grid of shape S for the query's arguments and
+// launches it (after zeroing the result in the same stream).
+static int32_t run_shape(tgp_ctx* ctx, const Shape& S, const ShapeKernels& K, SpecArgs A,
+                         hipStream_t st) {
+  // Kernel form (measured over 20K..60M rows): the replicated-histogram waves
+  // for large queries of several groups, and from 256K rows up for queries
+  // whose histogram outgrows a wave's registers (the register fold then
+  // goes through a workgroup table with carry-exact LDS atomics); the
+  // register fold otherwise (less LDS to set up and flush, which small
+  // queries feel).
+  const int rshift = wave_rshift(A.words);
+  const int64_t form = ctx->tune_spec_kernel.load();
+  const bool large = A.nslices >= kWaveMinSlices;
+  const bool many = A.words > (uint32_t)kHistWords && A.nslices >= kWaveManySlices;
+  const bool wave = rshift >= 0 &&
+                    (form == 2 || (form == 0 && A.ngroups >= 2 && (large || many)));
+  int per_cu = 0;
+  uint32_t lds = 0;
+  if (wave) {
+    lds = (uint32_t)((A.words << rshift) * (kBlock / kWave) * 8);
+  } else {
+    lds = S.table == 0 ? (4 * kHistWords + 4 * 64 * 2) * 8
+                       : (uint32_t)(A.ngroups * (1 + 2 * S.naggs) * 8);
+  }
+  TGP_TRY(K.occupancy(wave, lds, &per_cu));
+  if (per_cu < 1) per_cu = 1;
+  const int64_t want = ctx->tune_wg_per_cu.load();
+  if (want > 0 && want < per_cu) per_cu = (int)want;
+  // Large scans stream best from few concurrent waves as long as those hide
+  // the per-row work: measured on 60M rows, one group and 2..4 groups run
+  // fastest at 2 workgroups per CU, more groups at 3.
+  if (want == 0 && large && (wave || A.ngroups == 1)) {
+    const int cap = A.ngroups <= 4 ? 2 : 3;
+    if (per_cu > cap) per_cu = cap;
+  }
+  uint64_t grid = (uint64_t)persistent_grid(ctx, per_cu);
+  const int64_t gmax = ctx->tune_grid_max.load();
+  if (wave) {
+    // 0: static interleave (default); > 0: slices per dynamically taken chunk
+    uint32_t chunk = (uint32_t)ctx->tune_chunk.load();
+    // every workgroup ends with one device atomic per histogram word, all on
+    // the same addresses: queries of many words run on fewer, longer waves
+    uint64_t spw = min_slices_per_wave(ctx);
+    if (ctx->tune_slices_per_wave.load() == 0 && A.words / 64 > spw) spw = A.words / 64;
+    const uint64_t per_wg = (kBlock / kWave) * (chunk ? chunk : spw);
+    const uint64_t need = (A.nslices + per_wg - 1) / per_wg;
+    if (grid > need) grid = need;
+    if (gmax > 0 && grid > (uint64_t)gmax) grid = (uint64_t)gmax;
+    if (grid == 0) grid = 1;
+    void* counter = nullptr;
+    TGP_TRY(get_scratch(ctx, st, 256, &counter));
+    auto* cnt = static_cast<unsigned long long*>(counter);
+    TGP_TRY(zero_result(st, A, S.naggs, cnt));
+    uint32_t rs = (uint32_t)rshift;
+    void* args[] = {&A, &rs, &chunk, &cnt};
+    TGP_TRY(K.launch(true, grid, lds, st, args));
+  } else {
+    const uint64_t per_wg = (kBlock / kWave) * min_slices_per_wave(ctx);
+    const uint64_t need = (A.nslices + per_wg - 1) / per_wg;
+    if (grid > need) grid = need;
+    if (gmax > 0 && grid > (uint64_t)gmax) grid = (uint64_t)gmax;
+    if (grid == 0) grid = 1;
+    TGP_TRY(zero_result(st, A, S.naggs, nullptr));
+    void* args[] = {&A};
+    TGP_TRY(K.launch(false, grid, lds, st, args));
+  }
+  return TGP_OK;
+}
+
+// The ahead-of-time kernels of s, if s is one of kShapes.
 template <int I>
-int32_t launch_shape(tgp_ctx* ctx, const Shape& s, const SpecArgs& A, hipStream_t st,
-                     bool* launched) {
+bool aot_kernels(const Shape& s, ShapeKernels* K) {
   if constexpr (I < kNumShapes) {
     if (shape_eq(s, kShapes[I])) {
       constexpr Shape S = kShapes[I];
-      // Kernel form (measured over 20K..60M rows): the replicated-histogram waves
-      // for large queries of several groups, and from 256K rows up for queries
-      // whose histogram outgrows a wave's registers (the register fold then
-      // goes through a workgroup table with carry-exact LDS atomics); the
-      // register fold otherwise (less LDS to set up and flush, which small
-      // queries feel).
-      const int rshift = wave_rshift(A.words);
-      const int64_t form = ctx->tune_spec_kernel.load();
-      const bool large = A.nslices >= kWaveMinSlices;
-      const bool many = A.words > (uint32_t)kHistWords && A.nslices >= kWaveManySlices;
-      const bool wave = rshift >= 0 &&
-                        (form == 2 || (form == 0 && A.ngroups >= 2 && (large || many)));
-      int per_cu = 0;
-      uint32_t lds = 0;
-      if (wave) {
-        lds = (uint32_t)((A.words << rshift) * (kBlock / kWave) * 8);
-        TGP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_wave<S>, kBlock, lds));
-      } else {
-        lds = S.table == 0 ? (4 * kHistWords + 4 * 64 * 2) * 8
-                           : (uint32_t)(A.ngroups * (1 + 2 * S.naggs) * 8);
-        TGP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_spec<S>, kBlock, lds));
-      }
-      if (per_cu < 1) per_cu = 1;
-      const int64_t want = ctx->tune_wg_per_cu.load();
-      if (want > 0 && want < per_cu) per_cu = (int)want;
-      // Large scans stream best from few concurrent waves as long as those hide
-      // the per-row work: measured on 60M rows, one group and 2..4 groups run
-      // fastest at 2 workgroups per CU, more groups at 3.
-      if (want == 0 && large && (wave || A.ngroups == 1)) {
-        const int cap = A.ngroups <= 4 ? 2 : 3;
-        if (per_cu > cap) per_cu = cap;
-      }
-      uint64_t grid = (uint64_t)persistent_grid(ctx, per_cu);
-      const int64_t gmax = ctx->tune_grid_max.load();
-      if (wave) {
-        // 0: static interleave (default); > 0: slices per dynamically taken chunk
-        const uint64_t chunk = (uint64_t)ctx->tune_chunk.load();
-        // every workgroup ends with one device atomic per histogram word, all on
-        // the same addresses: queries of many words run on fewer, longer waves
-        uint64_t spw = min_slices_per_wave(ctx);
-        if (ctx->tune_slices_per_wave.load() == 0 && A.words / 64 > spw) spw = A.words / 64;
-        const uint64_t per_wg = (kBlock / kWave) * (chunk ? chunk : spw);
-        const uint64_t need = (A.nslices + per_wg - 1) / per_wg;
-        if (grid > need) grid = need;
-        if (gmax > 0 && grid > (uint64_t)gmax) grid = (uint64_t)gmax;
-        if (grid == 0) grid = 1;
-        void* counter = nullptr;
-        TGP_TRY(get_scratch(ctx, st, 256, &counter));
-        auto* cnt = static_cast<unsigned long long*>(counter);
-        TGP_TRY(zero_result(st, A, S.naggs, cnt));
-        k_wave<S><<<dim3((unsigned)grid), dim3(kBlock), lds, st>>>(A, (uint32_t)rshift,
-                                                                  (uint32_t)chunk, cnt);
-      } else {
-        const uint64_t per_wg = (kBlock / kWave) * min_slices_per_wave(ctx);
-        const uint64_t need = (A.nslices + per_wg - 1) / per_wg;
-        if (grid > need) grid = need;
-        if (gmax > 0 && grid > (uint64_t)gmax) grid = (uint64_t)gmax;
-        if (grid == 0) grid = 1;
-        TGP_TRY(zero_result(st, A, S.naggs, nullptr));
-        k_spec<S><<<dim3((unsigned)grid), dim3(kBlock), lds, st>>>(A);
-      }
-      TGP_HIP(hipGetLastError());
-      *launched = true;
-      return TGP_OK;
+      K->spec = reinterpret_cast<const void*>(&k_spec<S>);
+      K->wave = reinterpret_cast<const void*>(&k_wave<S>);
+      return true;
     }
-    return launch_shape<I + 1>(ctx, s, A, st, launched);
+    return aot_kernels<I + 1>(s, K);
   } else {
-    return TGP_OK;
+    return false;
   }
 }
+
+// Queries of a shape without ahead-of-time kernels, TGP_TUNE_JIT 0 (default):
+// from kJitMinRows rows the shape is compiled on a background thread and the
+// query runs on the generic engine until the compile is done; 1: never
+// compiled; 2: compiled before the query runs, whatever its size.
+constexpr uint64_t kJitMinRows = 1ull << 20;
 
 // ---- host: shape extraction ----------------------------------------------------------
 
@@ -479,7 +369,7 @@ int32_t spec_shape_index(const tgp_agg_query* q) {
   if (!extract_shape(q, &s, &A)) return -1;
   for (int i = 0; i < kNumShapes; ++i)
     if (shape_eq(s, kShapes[i])) return i;
-  return -1;
+  return TGP_SPEC_SHAPE_RUNTIME;
 }
 
 // Launches the specialised kernel for q when one is compiled (after zeroing the
@@ -499,7 +389,29 @@ int32_t try_spec(tgp_ctx* ctx, const tgp_agg_query* q, const tgp_group_result* o
     *launched = true;
     return TGP_OK;
   }
-  return launch_shape<0>(ctx, s, A, st, launched);
+  ShapeKernels K;
+  if (!aot_kernels<0>(s, &K)) {
+    const int64_t mode = ctx->tune_jit.load();
+    if (mode == 1 || (mode == 0 && A.row_end - A.row_begin < kJitMinRows)) return TGP_OK;
+    // a shape still compiling, or that failed to compile, runs on the generic
+    // engine (a failure stays in the compile cache and tgp_jit_info's count)
+    bool ready = false;
+    if (jit_shape_kernels(s, mode == 2, &K.jit, &ready) != TGP_OK || !ready) return TGP_OK;
+  }
+  TGP_TRY(run_shape(ctx, s, K, A, st));
+  *launched = true;
+  return TGP_OK;
 }
 
 }  // namespace tgp
+
+extern "C" int32_t tgp_jit_info(uint64_t* compiled, uint64_t* failed, uint64_t* launches,
+                                double* compile_ms) {
+  TGP_REQUIRE(compiled && failed && launches && compile_ms, "tgp_jit_info: null argument");
+  const tgp::JitStats st = tgp::jit_stats();
+  *compiled = st.compiled;
+  *failed = st.failed;
+  *launches = st.launches;
+  *compile_ms = st.compile_ms;
+  return TGP_OK;
+}

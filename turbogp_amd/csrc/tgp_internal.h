@@ -1,6 +1,10 @@
 // Internal definitions shared by the operator library's translation units.
 #pragma once
 
+// The device parts of the library's headers also compile under hipRTC
+// (__HIPCC_RTC__: the run-time compiled kernels of jit.hip); the host runtime
+// declarations are left out there.
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -10,8 +14,17 @@
 #include <map>
 #include <mutex>
 #include <string>
-#include <type_traits>
-#include <utility>
+#else
+typedef __hip_internal::int8_t int8_t;
+typedef __hip_internal::int16_t int16_t;
+typedef __hip_internal::int32_t int32_t;
+typedef __hip_internal::int64_t int64_t;
+typedef __hip_internal::uint8_t uint8_t;
+typedef __hip_internal::uint16_t uint16_t;
+typedef __hip_internal::uint32_t uint32_t;
+typedef __hip_internal::uint64_t uint64_t;
+typedef __hip_internal::size_t size_t;
+#endif
 
 #include "../../include/turbogp_gfx950.h"
 
@@ -44,21 +57,33 @@ void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
     if (rc_ != TGP_OK) return rc_;    \
   } while (0)
 
-// Calls f(std::integral_constant<int, I>{}) for I = 0 .. N-1 (compile-time loop:
-// arrays indexed by I stay in registers).
-template <typename F, int... I>
-__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
-  (f(std::integral_constant<int, I>{}), ...);
+// Calls f(IntC<I>{}) for I = 0 .. N-1 (compile-time loop: arrays indexed by
+// decltype(I)::value stay in registers).
+template <int V>
+struct IntC {
+  static constexpr int value = V;
+};
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for_impl(F& f) {
+  if constexpr (I < N) {
+    f(IntC<I>{});
+    static_for_impl<I + 1, N>(f);
+  }
 }
 template <int N, typename F>
 __device__ __forceinline__ void static_for(F&& f) {
-  static_for_impl(f, std::make_integer_sequence<int, N>{});
+  static_for_impl<0, N>(f);
 }
 
 // ---- launch geometry -----------------------------------------------------------
 constexpr int kWave = 64;
 constexpr int kBlock = 256;            // threads per workgroup for streaming kernels
 constexpr int kTileRows = 1024;        // rows staged per tile in the fused kernels
+
+}  // namespace tgp
+
+#ifndef __HIPCC_RTC__
+namespace tgp {
 
 inline int dtype_width(int32_t dt) {
   switch (dt) {
@@ -106,6 +131,7 @@ struct tgp_ctx {
   std::atomic<int64_t> tune_distinct_shrink{0};  // exact distinct: sketch-sized set / 2^n
   std::atomic<int64_t> tune_group_tag_bits{0};   // hash GROUP BY: 0 = auto; n = verified form, n-bit tags
   std::atomic<int64_t> tune_join_hot{0};         // join hot-key cache: 0 auto, 1 off, 2 on
+  std::atomic<int64_t> tune_jit{0};              // run-time compiled shapes: 0 auto, 1 off, 2 on
 };
 
 namespace tgp {
@@ -149,3 +175,4 @@ int32_t sum_f64_exact(tgp_ctx* ctx, const uint64_t* d_cells, uint64_t n,
 // Validates a column descriptor (non-null data, known dtype, 16-byte alignment).
 int32_t check_column(const tgp_column* c, const char* what);
 }  // namespace tgp
+#endif  // __HIPCC_RTC__
