@@ -301,6 +301,65 @@ int32_t tgp_scan_compare_columns(tgp_ctx* ctx, const tgp_column* a, int32_t op,
                                  const tgp_column* b, int32_t is_signed, uint64_t row_begin,
                                  uint64_t row_end, uint64_t* d_bitmap, void* stream);
 
+/* f64 column (cells = f64 bits, as the reference stores FLOAT columns) vs an f64
+ * literal -> a fresh bitmap (bit i <-> row row_begin+i; d_count, if non-null,
+ * receives the number of set bits, added to its current value).  op TGP_EQ with
+ * eq_epsilon = 0 is IEEE equality (filter_eq_f64's AVX-512 path, _CMP_EQ_OQ:
+ * -0.0 == 0.0, NaN never equal); eq_epsilon = 1 is |cell - lit| <= 1e-6 *
+ * max(|lit|, 1) (filter_eq_f64_epsilon, what `WHERE f = lit` runs); TGP_NE is
+ * NOT of the epsilon equality (filter_ne_f64); TGP_LT/LE/GT/GE are ordered IEEE
+ * compares (false for NaN).  Replaces exec/bitmap.rs:479-580 and :696-753. */
+int32_t tgp_scan_filter_f64(tgp_ctx* ctx, const tgp_column* col, int32_t op, int32_t eq_epsilon,
+                            double lit, uint64_t row_begin, uint64_t row_end, uint64_t* d_bitmap,
+                            uint64_t* d_count, void* stream);
+
+/* ---- expression predicates (the interpreter's per-row eval fallback) ------------------
+ * A WHERE term that is not a column-vs-literal comparison (arithmetic on columns,
+ * AND / OR / NOT trees, BETWEEN, IN, CASE) is evaluated per row by the reference
+ * (query_interpreter/expr.rs:1453-1463: eval of both sides, binop, truthy) with
+ * Value2 semantics (expr.rs:1626-1800, :2037-2141; types.rs:351-368):
+ *   - a column reads as Int (INT: the cell as int64), Float (FLOAT: the f64 bits)
+ *     or Date (DATE: the low 32 bits as int32);
+ *   - + - * of two Ints wrap in int64; otherwise (a Float or a Date operand) the
+ *     f64 result; / is always f64 (NULL when the divisor is 0); % of two Ints is the
+ *     truncated int64 remainder, else f64 fmod (NULL for 0);
+ *   - = <> < <= > >= compare as f64 (so Ints above 2^53 compare rounded); NULL
+ *     makes = < <= > >= false and <> true (<> is NOT of =);
+ *   - AND / OR / NOT on truthiness (Int != 0, Float != 0; NULL and Date false);
+ *     unary minus wraps Ints, negates Floats, gives NULL otherwise;
+ *   - BETWEEN lo AND hi is lo <= v AND v <= hi; IN is any = ; CASE takes the first
+ *     truthy WHEN (TGP_EX_IF nests them), NULL without ELSE.
+ * The expression arrives as a postfix program of at most TGP_MAX_EXPR_OPS ops over
+ * a stack at most TGP_MAX_EXPR_DEPTH deep (checked); bit i of d_bitmap (row
+ * row_begin+i) stays set only where the program's final value is truthy, and bits
+ * past the range are cleared. */
+#define TGP_MAX_EXPR_OPS 64
+#define TGP_MAX_EXPR_DEPTH 8
+enum tgp_expr_code {
+  TGP_EX_COL = 0,    /* push column `arg` read as imm's kind (TGP_EX_INT/FLOAT/DATE) */
+  TGP_EX_INT = 1,    /* push Int(imm) */
+  TGP_EX_FLOAT = 2,  /* push Float(f64 bits in imm) */
+  TGP_EX_DATE = 3,   /* push Date(int32 imm) */
+  TGP_EX_NULL = 4,   /* push NULL */
+  TGP_EX_ADD = 5, TGP_EX_SUB = 6, TGP_EX_MUL = 7, TGP_EX_DIV = 8, TGP_EX_MOD = 9,
+  TGP_EX_EQ = 10, TGP_EX_NE = 11, TGP_EX_LT = 12, TGP_EX_LE = 13, TGP_EX_GT = 14,
+  TGP_EX_GE = 15,
+  TGP_EX_AND = 16, TGP_EX_OR = 17, TGP_EX_NOT = 18, TGP_EX_NEG = 19,
+  TGP_EX_BETWEEN = 20,      /* v lo hi -> lo <= v AND v <= hi */
+  TGP_EX_NOT_BETWEEN = 21,
+  TGP_EX_IN = 22,           /* v x1 .. xn (n = arg >= 1) -> any v = xk */
+  TGP_EX_NOT_IN = 23,
+  TGP_EX_IF = 24            /* cond then else -> then if cond is truthy, else else */
+};
+typedef struct tgp_expr_op {
+  int32_t code; /* enum tgp_expr_code */
+  int32_t arg;  /* column index (TGP_EX_COL), list length (TGP_EX_IN / NOT_IN) */
+  int64_t imm;  /* literal bits, or the column kind of TGP_EX_COL */
+} tgp_expr_op;
+int32_t tgp_scan_expr(tgp_ctx* ctx, const tgp_column* cols, int32_t ncols,
+                      const tgp_expr_op* prog, int32_t nops, uint64_t row_begin,
+                      uint64_t row_end, uint64_t* d_bitmap, void* stream);
+
 /* Bitmap algebra over nbits bits (bitmap.rs and/or/not/and_inplace):
  * out = a op b (out may alias a); NOT clears tail bits (bitmap.rs:121-136). */
 int32_t tgp_bitmap_combine(tgp_ctx* ctx, int32_t op, const uint64_t* d_a, const uint64_t* d_b,

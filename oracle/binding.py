@@ -21,6 +21,10 @@ class OColumn(C.Structure):
                 ("_pad", C.c_int32)]
 
 
+class OExprOp(C.Structure):
+    _fields_ = [("code", C.c_int32), ("arg", C.c_int32), ("imm", C.c_int64)]
+
+
 class OPred(C.Structure):
     _fields_ = [("col", C.c_int32), ("op", C.c_int32), ("is_signed", C.c_int32),
                 ("_pad", C.c_int32), ("lit", C.c_int64)]
@@ -77,6 +81,8 @@ def lib():
             "orc_count_distinct": (U64, [P, I32, U64, P, I32]),
             "orc_group_by_keys": (I64, [C.POINTER(OQuery), I32, P, P, U64, P, P, P, P]),
             "orc_simd_level": (I32, []),
+            "orc_filter_f64": (None, [P, U64, I32, I32, C.c_double, P]),
+            "orc_scan_expr": (I32, [P, I32, P, I32, U64, U64, P]),
             "orc_set_simd": (None, [I32]),
         }
         for name, (res, args) in sig.items():
@@ -397,3 +403,35 @@ def sum_f64(bits, sel_words=None, neg_zero_start=True):
         return total.numerator / total.denominator
     except OverflowError:
         return math.inf if total > 0 else -math.inf
+
+
+def filter_f64(values, op, eps, lit):
+    """exec/bitmap.rs filter_*_f64 over f64 values -> LSB-first bitmap bytes."""
+    cells = np.ascontiguousarray(values, np.float64).view(np.uint64)
+    bits = np.zeros((len(cells) + 7) // 8, np.uint8)
+    lib().orc_filter_f64(ptr(cells), len(cells), op, int(eps), float(lit), ptr(bits))
+    return bits
+
+
+def scan_expr(arrays, prog, row_begin=0, row_end=None, bits=None):
+    """The interpreter's per-row eval of a postfix program (code, arg, imm) over host
+    column arrays: the selection bits (all rows if bits is None) of [row_begin, row_end)
+    narrowed to the rows whose value is truthy."""
+    keep = [np.ascontiguousarray(a) for a in arrays]
+    n = len(keep[0]) if keep else 0
+    row_end = n if row_end is None else row_end
+    cols = (OColumn * max(1, len(keep)))()
+    for i, a in enumerate(keep):
+        cols[i] = OColumn(ptr(a), len(a), _FROM_NP[a.dtype], 0)
+    ops = (OExprOp * len(prog))(*[OExprOp(c, a, im) for c, a, im in prog])
+    nb = row_end - row_begin
+    if bits is None:
+        bits = np.zeros((nb + 7) // 8, np.uint8)
+        bits[: nb // 8] = 0xFF
+        if nb % 8:
+            bits[nb // 8] = (1 << (nb % 8)) - 1
+    bits = np.ascontiguousarray(bits, np.uint8).copy()
+    rc = lib().orc_scan_expr(cols, len(keep), ops, len(prog), row_begin, row_end, ptr(bits))
+    if rc != 0:
+        raise ValueError("malformed expression program")
+    return bits

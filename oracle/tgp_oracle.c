@@ -964,3 +964,181 @@ uint64_t orc_count_distinct(const void* col, int32_t dtype, uint64_t n, const ui
   free(v);
   return d;
 }
+
+/* ---- f64 filters (exec/bitmap.rs:479-580) -------------------------------------------
+ * op 0 EQ (eps = 0: `==` as the AVX-512 path's _CMP_EQ_OQ, :482-486; eps = 1:
+ * filter_eq_f64_epsilon, :505-515), 1 NE (NOT of the epsilon equality, :498-500),
+ * 2 LT, 3 LE, 4 GT, 5 GE (:523-580).  Cells are f64 bits; bits[i] for row i. */
+void orc_filter_f64(const uint64_t* cells, uint64_t n, int32_t op, int32_t eps, double val,
+                    uint8_t* bits) {
+  const double tol = 1e-6 * (fabs(val) > 1.0 ? fabs(val) : 1.0);
+  memset(bits, 0, (size_t)((n + 7) / 8));
+  for (uint64_t i = 0; i < n; ++i) {
+    double cv;
+    memcpy(&cv, &cells[i], 8);
+    int set;
+    switch (op) {
+      case 0: set = eps ? fabs(cv - val) <= tol : cv == val; break;
+      case 1: set = !(fabs(cv - val) <= tol); break;
+      case 2: set = cv < val; break;
+      case 3: set = cv <= val; break;
+      case 4: set = cv > val; break;
+      default: set = cv >= val; break;
+    }
+    if (set) bits[i >> 3] |= (uint8_t)(1u << (i & 7));
+  }
+}
+
+/* ---- expression predicates: the interpreter's per-row eval -------------------------
+ * Value2 (types.rs:8-15 / :351-368) and Interpreter::eval / binop / cmp_* / truthy
+ * (query_interpreter/expr.rs:1615-1622, :1626-1800, :2037-2141) over the postfix
+ * program of tgp_scan_expr (codes as in tgp_expr_code). */
+enum { V_NULL = 0, V_INT = 1, V_FLOAT = 2, V_DATE = 3 };
+typedef struct { int tag; int64_t i; double f; int32_t d; } orc_value;
+
+static int v_as_f64(const orc_value* v, double* out) { /* Value2::as_f64 */
+  switch (v->tag) {
+    case V_INT: *out = (double)v->i; return 1;
+    case V_FLOAT: *out = v->f; return 1;
+    case V_DATE: *out = (double)v->d; return 1;
+    default: return 0;
+  }
+}
+static int v_truthy(const orc_value* v) { /* expr.rs:1615-1622 */
+  if (v->tag == V_INT) return v->i != 0;
+  if (v->tag == V_FLOAT) return v->f != 0.0;
+  return 0;
+}
+static int v_cmp_eq(const orc_value* a, const orc_value* b) { /* :2106-2124 (no strings) */
+  double x, y;
+  if (a->tag == V_NULL || b->tag == V_NULL) return 0;
+  if (!v_as_f64(a, &x) || !v_as_f64(b, &y)) return 0;
+  return x == y;
+}
+static int v_cmp_lt(const orc_value* a, const orc_value* b) { /* :2126-2138 */
+  double x, y;
+  if (a->tag == V_NULL || b->tag == V_NULL) return 0;
+  if (!v_as_f64(a, &x) || !v_as_f64(b, &y)) return 0;
+  return x < y;
+}
+static int v_cmp_le(const orc_value* a, const orc_value* b) { /* :2139-2141 */
+  return v_cmp_lt(a, b) || v_cmp_eq(a, b);
+}
+static orc_value v_int(int64_t i) { orc_value v = {V_INT, i, 0.0, 0}; return v; }
+static orc_value v_float(double f) { orc_value v = {V_FLOAT, 0, f, 0}; return v; }
+static orc_value v_null(void) { orc_value v = {V_NULL, 0, 0.0, 0}; return v; }
+
+/* binop (expr.rs:2037-2104); code as tgp_expr_code ADD..OR. */
+static orc_value v_binop(int32_t code, const orc_value* lv, const orc_value* rv) {
+  if (code >= 5 && code <= 9) { /* Add Sub Mul Div Mod */
+    double l, r;
+    if (!v_as_f64(lv, &l) || !v_as_f64(rv, &r)) return v_null();
+    double res;
+    switch (code) {
+      case 5: res = l + r; break;
+      case 6: res = l - r; break;
+      case 7: res = l * r; break;
+      case 8:
+        if (r == 0.0) return v_null();
+        res = l / r;
+        break;
+      default:
+        if (r == 0.0) return v_null();
+        res = fmod(l, r);
+        break;
+    }
+    const int ints = lv->tag == V_INT && rv->tag == V_INT;
+    if (ints && code != 8 && code != 9) { /* wrapping_add / _sub / _mul */
+      const uint64_t li = (uint64_t)lv->i, ri = (uint64_t)rv->i;
+      return v_int((int64_t)(code == 5 ? li + ri : code == 6 ? li - ri : li * ri));
+    }
+    if (ints && code == 9) { /* li % ri; i64::MIN % -1 (a panic in Rust) is taken as 0 */
+      if (rv->i == 0) return v_null();
+      return v_int(rv->i == -1 ? 0 : lv->i % rv->i);
+    }
+    return v_float(res);
+  }
+  switch (code) {
+    case 10: return v_int(v_cmp_eq(lv, rv));
+    case 11: return v_int(!v_cmp_eq(lv, rv));
+    case 12: return v_int(v_cmp_lt(lv, rv));
+    case 13: return v_int(v_cmp_le(lv, rv));
+    case 14: return v_int(v_cmp_lt(rv, lv));
+    case 15: return v_int(v_cmp_le(rv, lv));
+    case 16: return v_int(v_truthy(lv) && v_truthy(rv));
+    default: return v_int(v_truthy(lv) || v_truthy(rv));
+  }
+}
+
+/* eval of column `c` at row (expr.rs:1628-1640): Int = the cell as i64, Float = the
+ * f64 bits, Date = the cell's low 32 bits as i32. */
+static orc_value v_column(const orc_column* c, int64_t kind, uint64_t row) {
+  const int64_t cell = get_native(c->data, c->dtype, row);
+  orc_value v = v_null();
+  if (kind == V_INT) v = v_int(cell);
+  else if (kind == V_FLOAT) { v.tag = V_FLOAT; memcpy(&v.f, &cell, 8); }
+  else { v.tag = V_DATE; v.d = (int32_t)(uint32_t)(uint64_t)cell; }
+  return v;
+}
+
+/* Clears bit (row - row_begin) of bits (LSB-first bytes) for every row of
+ * [row_begin, row_end) whose program value is not truthy (build_mask's
+ * fallback, expr.rs:1453-1463).  Returns 0, or -1 for a malformed program. */
+int32_t orc_scan_expr(const orc_column* cols, int32_t ncols, const orc_expr_op* prog,
+                      int32_t nops, uint64_t row_begin, uint64_t row_end, uint8_t* bits) {
+  orc_value st[64];
+  for (uint64_t row = row_begin; row < row_end; ++row) {
+    const uint64_t i = row - row_begin;
+    if (!((bits[i >> 3] >> (i & 7)) & 1)) continue;
+    int sp = 0;
+    for (int32_t k = 0; k < nops; ++k) {
+      const orc_expr_op* o = &prog[k];
+      switch (o->code) {
+        case 0:
+          if (o->arg < 0 || o->arg >= ncols) return -1;
+          st[sp++] = v_column(&cols[o->arg], o->imm, row);
+          break;
+        case 1: st[sp++] = v_int(o->imm); break;
+        case 2: { orc_value v = v_null(); v.tag = V_FLOAT; memcpy(&v.f, &o->imm, 8); st[sp++] = v; break; }
+        case 3: { orc_value v = v_null(); v.tag = V_DATE; v.d = (int32_t)o->imm; st[sp++] = v; break; }
+        case 4: st[sp++] = v_null(); break;
+        case 18: st[sp - 1] = v_int(!v_truthy(&st[sp - 1])); break;           /* Not :1741-1744 */
+        case 19: {                                                             /* Neg :1733-1740 */
+          orc_value v = st[sp - 1];
+          if (v.tag == V_INT) st[sp - 1] = v_int((int64_t)(0 - (uint64_t)v.i));
+          else if (v.tag == V_FLOAT) st[sp - 1] = v_float(-v.f);
+          else st[sp - 1] = v_null();
+          break;
+        }
+        case 20: case 21: {                                                    /* :1773-1779 */
+          const int in = v_cmp_le(&st[sp - 2], &st[sp - 3]) && v_cmp_le(&st[sp - 3], &st[sp - 1]);
+          sp -= 2;
+          st[sp - 1] = v_int(o->code == 20 ? in : !in);
+          break;
+        }
+        case 22: case 23: {                                                    /* :1780-1790 */
+          int found = 0;
+          for (int32_t j = 0; j < o->arg && !found; ++j)
+            found = v_cmp_eq(&st[sp - 1 - o->arg], &st[sp - o->arg + j]);
+          sp -= o->arg;
+          st[sp - 1] = v_int(o->code == 22 ? found : !found);
+          break;
+        }
+        case 24: {                                                             /* Case :1791-1800 */
+          const orc_value r = v_truthy(&st[sp - 3]) ? st[sp - 2] : st[sp - 1];
+          sp -= 2;
+          st[sp - 1] = r;
+          break;
+        }
+        default:
+          if (sp < 2 || o->code < 5 || o->code > 17) return -1;
+          st[sp - 2] = v_binop(o->code, &st[sp - 2], &st[sp - 1]);
+          --sp;
+      }
+      if (sp < 1 || sp > 60) return -1;
+    }
+    if (sp != 1) return -1;
+    if (!v_truthy(&st[0])) bits[i >> 3] &= (uint8_t)~(1u << (i & 7));
+  }
+  return 0;
+}

@@ -9,7 +9,7 @@
  * Reference files restated (paths relative to the turboGP checkout):
  *   src/exec/bitmap.rs            Bitmap, filter_*, and/or/not, SetBitIter
  *   src/kernel/scan.rs            ScanEq / ScanRange / ScanMultiPredicate (scalar)
- *   src/engine/query_interpreter/expr.rs      build_mask AND chain, binop
+ *   src/engine/query_interpreter/expr.rs      build_mask AND chain, per-row eval, binop
  *   src/engine/query_interpreter/aggregate.rs execute_grouped, sum_vec
  *   src/engine/query_interpreter/accum_grouped.rs  combined key hash
  *   src/exec/join_hash_table.rs   JoinHashTable (CRC32 hash, tagged chains)
@@ -122,6 +122,17 @@ void orc_hll_registers(const void* col, int32_t dtype, uint64_t n, const uint64_
 uint64_t orc_hll_estimate(const uint8_t* regs);
 uint64_t orc_count_distinct(const void* col, int32_t dtype, uint64_t n, const uint64_t* sel,
                             int32_t mode);
+
+/* f64 filters (exec/bitmap.rs:479-580): op 0 EQ (eps: filter_eq_f64_epsilon),
+ * 1 NE, 2 LT, 3 LE, 4 GT, 5 GE; cells are f64 bits; bits: (n+7)/8 bytes. */
+void orc_filter_f64(const uint64_t* cells, uint64_t n, int32_t op, int32_t eps, double val,
+                    uint8_t* bits);
+
+/* Expression predicates (query_interpreter/expr.rs eval / binop): the postfix
+ * program of tgp_scan_expr; clears the bits of rows whose value is not truthy. */
+typedef struct { int32_t code, arg; int64_t imm; } orc_expr_op;
+int32_t orc_scan_expr(const orc_column* cols, int32_t ncols, const orc_expr_op* prog,
+                      int32_t nops, uint64_t row_begin, uint64_t row_end, uint8_t* bits);
 
 #ifdef __cplusplus
 }

@@ -35,6 +35,10 @@ JOIN_GROUP_BUILD_PAYLOAD, JOIN_GROUP_PROBE_COL, JOIN_GROUP_NONE = 0, 1, 2
 TUNE_WG_PER_CU, TUNE_GRID_MAX, TUNE_SPECIALIZE, TUNE_SPEC_KERNEL, TUNE_CHUNK = 0, 1, 2, 3, 4
 TUNE_SLICES_PER_WAVE, TUNE_JOIN_ATOMICS, TUNE_JOIN_PACK_BITS, TUNE_DISTINCT_SHRINK = 5, 6, 7, 8
 TUNE_GROUP_TAG_BITS, TUNE_JOIN_HOT, TUNE_JIT = 9, 10, 11
+(EX_COL, EX_INT, EX_FLOAT, EX_DATE, EX_NULL, EX_ADD, EX_SUB, EX_MUL, EX_DIV, EX_MOD, EX_EQ, EX_NE,
+ EX_LT, EX_LE, EX_GT, EX_GE, EX_AND, EX_OR, EX_NOT, EX_NEG, EX_BETWEEN, EX_NOT_BETWEEN, EX_IN,
+ EX_NOT_IN, EX_IF) = range(25)
+MAX_EXPR_OPS, MAX_EXPR_DEPTH = 64, 8
 SPEC_SHAPE_RUNTIME = 1000
 JOIN_INNER, JOIN_LEFT = 0, 1
 JOIN_NO_MATCH = 0xFFFFFFFF
@@ -48,6 +52,10 @@ MAX_COLS, MAX_PREDS, MAX_AGGS, MAX_FACTORS, MAX_GROUP_PARTS = 8, 8, 6, 3, 2
 class Column(C.Structure):
     _fields_ = [("data", C.c_void_p), ("len", C.c_uint64), ("dtype", C.c_int32),
                 ("_pad", C.c_int32)]
+
+
+class ExprOp(C.Structure):
+    _fields_ = [("code", C.c_int32), ("arg", C.c_int32), ("imm", C.c_int64)]
 
 
 class Pred(C.Structure):
@@ -209,6 +217,10 @@ SIGNATURES = {
     "tgp_group_agg_explain": (I32_, [C.POINTER(AggQuery), C.POINTER(I32_), C.POINTER(I32_)]),
     "tgp_jit_info": (I32_, [C.POINTER(U64_), C.POINTER(U64_), C.POINTER(U64_),
                             C.POINTER(C.c_double)]),
+    "tgp_scan_filter_f64": (I32_, [P, C.POINTER(Column), I32_, I32_, C.c_double, U64_, U64_, P,
+                                   P, P]),
+    "tgp_scan_expr": (I32_, [P, C.POINTER(Column), I32_, C.POINTER(ExprOp), I32_, U64_, U64_, P,
+                             P]),
     "tgp_hash_group_agg_sync": (I32_, [P, C.POINTER(HashGroupQuery), C.POINTER(HashGroupResult),
                                        C.POINTER(U64_), P]),
     "tgp_synth_uniform": (I32_, [P, C.POINTER(Column), U64_, U64_, I64_, I64_, U64_, U64_, P]),

@@ -235,6 +235,7 @@ constexpr uint32_t kHotThreshold = 16;     // sampled 16 times: about 1 probe in
 constexpr uint32_t kHotMapSlots = 1u << 19;  // sample count map: at most 256K keys, load <= 0.5
 constexpr uint32_t kHotMapProbes = 64;      // longest map probe (a key past it is not counted)
 constexpr uint64_t kHotLdsBytes = kHotEntries * 16ull;  // keys + entries in LDS
+static_assert((uint64_t)kHotSampleChunks * kHotSampleRows < (1ull << 32), "sample counts fit 32 bits");
 // the image is used when its keys took at least 1/kHotMinShare of the sample
 constexpr uint64_t kHotMinShare = 8;
 
@@ -257,8 +258,9 @@ struct HotLds {
 // (the same answer in every thread: two scalar loads).
 __device__ __forceinline__ bool hot_in_use(const HotDev& H) {
   if (!H.hdr) return false;
-  const unsigned long long cov = __builtin_amdgcn_readfirstlane(H.hdr[1]);
-  const unsigned long long tot = __builtin_amdgcn_readfirstlane(H.hdr[2]);
+  // both at most kHotSampleChunks * kHotSampleRows (2^18): 32 bits hold them
+  const uint64_t cov = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)H.hdr[1]);
+  const uint64_t tot = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)H.hdr[2]);
   return cov > 0 && cov * kHotMinShare >= tot;
 }
 

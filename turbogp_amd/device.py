@@ -253,6 +253,46 @@ class Context:
                 cnt.free()
         return bm, c
 
+    def scan_filter_f64(self, col, op, lit, eq_epsilon=False, row_begin=0, row_end=None,
+                        stream=None):
+        """FLOAT column (f64 bits in 8-byte cells) op literal (tgp_scan_filter_f64) ->
+        (DeviceBitmap over [row_begin, row_end), count of set bits)."""
+        row_end = col.n if row_end is None else row_end
+        bm = DeviceBitmap(self, row_end - row_begin)
+        cnt = DeviceBuffer(self, 8)
+        try:
+            cnt.fill_u64(0, stream=stream)
+            d = col.desc()
+            A.check(A.lib().tgp_scan_filter_f64(self.handle, C.byref(d), op, int(bool(eq_epsilon)),
+                                                float(lit), row_begin, row_end, bm.buf.ptr,
+                                                cnt.ptr, stream), "tgp_scan_filter_f64")
+            return bm, int(cnt.download(np.uint64, 1, stream)[0])
+        except Exception:
+            bm.free()
+            raise
+        finally:
+            cnt.free()
+
+    def scan_expr(self, cols, prog, row_begin=0, row_end=None, bitmap=None, stream=None):
+        """Rows of [row_begin, row_end) whose expression (a postfix program of (code, arg,
+        imm), see turbogp_amd.expr) is truthy (tgp_scan_expr): narrows `bitmap` in place
+        (bit i <-> row_begin + i) or, without one, returns a new DeviceBitmap."""
+        row_end = (cols[0].n if cols else 0) if row_end is None else row_end
+        own = bitmap is None
+        bm = DeviceBitmap(self, row_end - row_begin) if own else bitmap
+        try:
+            if own:
+                bm.buf.fill_u64(~0, nwords=bm.nwords, stream=stream)
+            arr = (A.Column * max(1, len(cols)))(*[c.desc() for c in cols])
+            ops = (A.ExprOp * len(prog))(*[A.ExprOp(c, a, im) for c, a, im in prog])
+            A.check(A.lib().tgp_scan_expr(self.handle, arr, len(cols), ops, len(prog), row_begin,
+                                          row_end, bm.buf.ptr, stream), "tgp_scan_expr")
+            return bm
+        except Exception:
+            if own:
+                bm.free()
+            raise
+
     def compare_columns(self, a, op, b, is_signed, row_begin=0, row_end=None, bitmap=None,
                         stream=None):
         """Rows of [row_begin, row_end) where a op b (tgp_scan_compare_columns): narrows
