@@ -291,3 +291,48 @@ def test_group_by_several_nullable_and_sparse_keys(ctx):
     finally:
         for c in list(dev.values()) + list(nulls.values()):
             c.free()
+
+
+def test_where_expressions_and_float_columns(ctx):
+    """WHERE terms the interpreter evaluates per row (arithmetic, OR, IN, CASE) run
+    as device expressions; FLOAT columns filter through the f64 filters, `=` with
+    apply_comparison's epsilon; the answers equal numpy over the same rows
+    (tests/interpreter_fallback.rs shapes on a 1M-row table)."""
+    from turbogp_amd import expr as X
+    rng = np.random.default_rng(17)
+    n = 1_000_003
+    price = rng.integers(90_000, 10_500_000, n).astype(np.int64)
+    disc = rng.integers(0, 11, n).astype(np.int8)
+    region = rng.integers(1, 6, n).astype(np.int32)
+    fprice = price.astype(np.float64) / 100.0
+    fprice[::1001] = 2500.0 + 1e-7                          # within the epsilon of 2500.0
+    dt = rng.integers(0, 2526, n).astype(np.int32)
+    dev = {k: DeviceColumn.from_numpy(ctx, np.ascontiguousarray(v)) for k, v in
+           {"price": price, "disc": disc, "region": region, "fprice": fprice.view(np.uint64),
+            "dt": dt}.items()}
+    t = Table(ctx, dev, types={"fprice": "float", "dt": "date"})
+    try:
+        P, D, R = X.Col("price"), X.Col("disc"), X.Col("region")
+        prod = price * disc.astype(np.int64)
+        q = SelectQuery([Aggregate("count", "*")],
+                        where=[X.BinOp(">", X.BinOp("*", P, D), X.Lit(40_000_000)), ("dt", "<", 1500)])
+        assert execute_dispatched(q, t).scalar() == int(((prod > 40_000_000) & (dt < 1500)).sum())
+        e = X.BinOp("OR", X.BinOp("AND", X.BinOp("=", R, X.Lit(1)), X.BinOp(">", P, X.Lit(5_000_000))),
+                    X.InList(R, (X.Lit(4), X.Lit(5))))
+        want = ((region == 1) & (price > 5_000_000)) | np.isin(region, [4, 5])
+        q = SelectQuery([Aggregate("sum", "price")], where=[e])
+        assert execute_dispatched(q, t).scalar() == int(price[want].sum())
+        case = X.Case(((X.BinOp(">", D, X.Lit(7)), X.Lit(1)),), X.Lit(0))
+        q = SelectQuery([Aggregate("count", "*")], where=[X.BinOp("=", case, X.Lit(1))])
+        assert execute_dispatched(q, t).scalar() == int((disc > 7).sum())
+        q = SelectQuery([Aggregate("count", "*")],
+                        where=[X.BinOp(">", X.BinOp("/", X.Col("fprice"), D), X.Lit(20_000.0))])
+        with np.errstate(divide="ignore", invalid="ignore"):
+            assert execute_dispatched(q, t).scalar() == int(((disc != 0) & (fprice / disc > 20_000.0)).sum())
+        for op, lit, w in (("<", 1000.5, fprice < 1000.5), (">=", 50_000.25, fprice >= 50_000.25),
+                           ("=", 2500.0, np.abs(fprice - 2500.0) <= 1e-6 * 2500.0),
+                           ("<>", 2500.0, ~(np.abs(fprice - 2500.0) <= 1e-6 * 2500.0))):
+            q = SelectQuery([Aggregate("count", "*")], where=[("fprice", op, lit)])
+            assert execute_dispatched(q, t).scalar() == int(w.sum()), op
+    finally:
+        t.free()

@@ -35,6 +35,7 @@ from typing import List, Optional, Sequence, Tuple
 import numpy as np
 
 from . import _capi as A
+from . import expr as X
 from .device import DeviceBitmap, DeviceBuffer, HashTable, make_agg_query, numeric_text
 
 OPS = {"=": A.EQ, "!=": A.NE, "<>": A.NE, "<": A.LT, "<=": A.LE, ">": A.GT, ">=": A.GE}
@@ -89,8 +90,10 @@ class Star:
 @dataclass
 class SelectQuery:
     select: List[object]
-    # (column, op, literal) or (column, op, Column(other)): an AND of comparisons
-    where: List[Tuple[str, str, object]] = field(default_factory=list)
+    # an AND of terms: (column, op, literal), (column, op, Column(other)), or an
+    # expression tree of turbogp_amd.expr (arithmetic, AND / OR / NOT, BETWEEN, IN,
+    # CASE), which the interpreter evaluates per row (expr.rs:1453-1463)
+    where: List[object] = field(default_factory=list)
     group_by: List[str] = field(default_factory=list)
     order_by: List[Tuple[str, bool]] = field(default_factory=list)  # (column, ascending)
     limit: Optional[int] = None
@@ -144,9 +147,13 @@ class Table:
     """Resident device columns by name (all of one length), and the NULL bitmaps
     (DeviceBitmap, bit set = NULL) of the nullable ones."""
 
-    def __init__(self, ctx, columns: dict, nulls: Optional[dict] = None):
+    def __init__(self, ctx, columns: dict, nulls: Optional[dict] = None,
+                 types: Optional[dict] = None):
+        """types: column name -> 'int' (default), 'float' (f64 bits in 8-byte cells,
+        ColType::Float) or 'date' (ColType::Date)."""
         self.ctx = ctx
         self.columns = dict(columns)
+        self.types = dict(types or {})
         lens = {c.n for c in self.columns.values()}
         assert len(lens) <= 1, "columns of different lengths"
         self.row_count = lens.pop() if lens else 0
@@ -169,6 +176,9 @@ class Table:
 
     def col(self, name):
         return self.columns[self.resolve(name)]
+
+    def col_type(self, name):
+        return self.types.get(self.resolve(name), "int")
 
     def free(self):
         for c in self.columns.values():
@@ -398,31 +408,59 @@ class _Executor:
             b.free()
         self.owned = []
 
-    # ---- WHERE: AND of comparisons -> selection bitmap over all rows --------------
+    # ---- WHERE: AND of terms -> selection bitmap over all rows ----------------------
     def mask(self):
         if not self.q.where:
             return None, self.t.row_count
         signed = (A.I8, A.I16, A.I32, A.I64)
-        for _, op, _ in self.q.where:
+        exprs = [w for w in self.q.where if not isinstance(w, tuple)]
+        terms = [w for w in self.q.where if isinstance(w, tuple)]
+        for _, op, _ in terms:
             if op not in OPS:  # e.g. LIKE (try_string_like_filter): not a device filter
                 raise Unsupported(f"comparison {op!r}")
-        lits = [w for w in self.q.where if not isinstance(w[2], Column)]
-        pairs = [w for w in self.q.where if isinstance(w[2], Column)]
+        # FLOAT columns against a literal: the f64 filters, `=` with the epsilon of
+        # apply_comparison (expr.rs:1509-1517: filter_eq_f64_epsilon, filter_ne_f64)
+        floats = [w for w in terms if not isinstance(w[2], Column) and self.t.col_type(w[0]) == "float"]
+        lits = [w for w in terms if not isinstance(w[2], Column) and w not in floats]
+        pairs = [w for w in terms if isinstance(w[2], Column)]
         names = sorted({c for c, _, _ in lits})
         cols = [self.t.col(c) for c in names]
         preds = [(names.index(c), OPS[op], int(lit), self.t.col(c).dtype in signed)
                  for c, op, lit in lits]
         bm, n = self.ctx.scan_filter(cols, preds, 0, self.t.row_count)
         self.owned.append(bm)
+        narrowed = False
+        for c, op, lit in floats:
+            fb, _ = self.ctx.scan_filter_f64(self.t.col(c), OPS[op], float(lit), OPS[op] == A.EQ,
+                                             0, self.t.row_count)
+            both = bm.combine(fb, A.BIT_AND)
+            fb.free()
+            self.owned.append(both)
+            bm, narrowed = both, True
         # column-vs-column comparisons narrow the selection (expr.rs:1382-1451:
         # signed for signed columns, as INT / DATE compare)
         for c, op, other in pairs:
             a, b = self.t.col(c), self.t.col(other.name)
             self.ctx.compare_columns(a, OPS[op], b, a.dtype in signed, 0, self.t.row_count, bm)
-        if pairs:
+            narrowed = True
+        # expression terms: the per-row eval, on the device (tgp_scan_expr)
+        kinds = {"int": X.INT, "float": X.FLOAT, "date": X.DATE}
+        used_expr = []
+        for e in exprs:
+            names_e = [self.t.resolve(c) for c in X.columns(e)]
+            used_expr += names_e
+            prog = X.compile_expr(e, lambda nm: (names_e.index(self.t.resolve(nm)),
+                                                 kinds[self.t.col_type(nm)]))
+            if len(names_e) > A.MAX_COLS or len(prog) > A.MAX_EXPR_OPS:
+                raise Unsupported("expression too large for the device evaluator")
+            self.ctx.scan_expr([self.t.columns[c] for c in names_e], prog, 0, self.t.row_count, bm)
+            narrowed = True
+        if narrowed:
             n = bm.count_ones()
-        # a comparison with a NULL cell is not true
-        used = names + [c for c, _, _ in pairs] + [o.name for _, _, o in pairs]
+        # a comparison with a NULL cell is not true; an expression term drops the rows
+        # where any column it reads is NULL (as the AND of comparisons does)
+        used = (names + [c for c, _, _ in floats] + [c for c, _, _ in pairs] +
+                [o.name for _, _, o in pairs] + used_expr)
         bm, n = self.without_nulls(bm, n, used)
         return bm, n
 
