@@ -414,6 +414,71 @@ def bench_generic(ctx, table, args):
     return out
 
 
+def bench_predicates(ctx, table, args):
+    """WHERE terms outside the AND-of-comparisons filter, over the table's first 60M
+    rows: an expression evaluated per row (price * disc > 4e7, the interpreter's
+    fallback, tgp_scan_expr) narrowing an all-rows selection, and a FLOAT column
+    against a literal (tgp_scan_filter_f64) writing a fresh bitmap."""
+    from turbogp_amd import DeviceColumn, Event, _capi as A
+    from turbogp_amd import expr as X
+    n = SCAN_ROWS
+    ev = [Event(ctx) for _ in range(2)]
+    out = {}
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        ts = []
+        for _ in range(args.steps):
+            ev[0].record()
+            fn()
+            ev[1].record()
+            ts.append(ev[0].elapsed_ms(ev[1]))
+        return statistics.mean(ts)
+    try:
+        cols = [table["price"], table["disc"]]
+        prog = X.compile_expr(X.BinOp(">", X.BinOp("*", X.Col("price"), X.Col("disc")),
+                                      X.Lit(40_000_000)),
+                              lambda nm: ({"price": 0, "disc": 1}[nm], X.INT))
+        bm = ctx.scan_expr(cols, prog, 0, n)
+        ms = timed(lambda: ctx.scan_expr(cols, prog, 0, n, bm))  # narrowing an all-ones bitmap
+        bm.buf.fill_u64(~0, nwords=bm.nwords)
+        ctx.scan_expr(cols, prog, 0, n, bm)
+        cnt = bm.count_ones()
+        bm.free()
+        byts = n * (8 + 8) + 2 * n // 8
+        out["expr_price_x_disc_gt_4e7"] = {"ms": round(ms, 4), "rows_selected": cnt,
+                                           "GB_s": round(byts / 1e9 / (ms / 1e3), 1),
+                                           "hbm_frac": round(byts / 1e9 / (ms / 1e3) / HBM_PEAK_GBS, 4),
+                                           "bytes": byts}
+        log(f"expression price*disc > 4e7 over {n} rows: {ms:.3f} ms, {cnt} rows")
+        import ctypes as C
+        from turbogp_amd import DeviceBitmap, DeviceBuffer
+        fp = (table["price"].buf.download(np.int64, n).astype(np.float64) / 100.0).view(np.uint64)
+        fc = DeviceColumn.from_numpy(ctx, fp)
+        del fp
+        fbm, cntb = DeviceBitmap(ctx, n), DeviceBuffer(ctx, 8)
+        d = fc.desc()
+        ms = timed(lambda: A.check(A.lib().tgp_scan_filter_f64(
+            ctx.handle, C.byref(d), A.LT, 0, 50_000.25, 0, n, fbm.buf.ptr, None, None), "f64"))
+        cntb.fill_u64(0)
+        A.check(A.lib().tgp_scan_filter_f64(ctx.handle, C.byref(d), A.LT, 0, 50_000.25, 0, n,
+                                            fbm.buf.ptr, cntb.ptr, None), "f64")
+        sel = int(cntb.download(np.uint64, 1)[0])
+        for x in (fbm, cntb, fc):
+            x.free()
+        byts = n * 8 + n // 8
+        out["f64_lt_filter"] = {"ms": round(ms, 4), "rows_selected": sel,
+                                "GB_s": round(byts / 1e9 / (ms / 1e3), 1),
+                                "hbm_frac": round(byts / 1e9 / (ms / 1e3) / HBM_PEAK_GBS, 4),
+                                "bytes": byts}
+        log(f"f64 filter over {n} rows: {ms:.3f} ms")
+    finally:
+        for e in ev:
+            e.free()
+    return out
+
+
 def bench_mixed(ctx, table):
     """A 4096-query batch of every query form (compiled Q1 / Q6 shapes, a fifth
     predicate, MIN / MAX, an input selection bitmap, an int32 price column, 5052
@@ -776,6 +841,7 @@ def main():
 
     generic = bench_generic(ctx, table, args)
     sections = set(args.sections.split(","))
+    predicates = bench_predicates(ctx, table, args) if "scan" in sections else None
     join = bench_join(ctx, args) if "join" in sections or "serve" in sections else None
     join_tables = join.pop("_tables") if join else None
     serve = bench_serve(ctx, table, join_tables, args) if "serve" in sections else None
@@ -837,7 +903,7 @@ def main():
                             "hbm_frac": round(gbs_b / HBM_PEAK_GBS, 4),
                             "bytes_per_row": W.Q6_BYTES_PER_ROW,
                             "selected": int(res_b.count[0])},
-            **generic},
+            **generic, **(predicates or {})},
         "join_2M_x_60M": join,
         "count_distinct_60M": distinct,
         "select_60M": select,
