@@ -728,7 +728,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--per-query-reps", type=int, default=3)
-    ap.add_argument("--cpu-rows", type=int, default=300_000_000)
+    ap.add_argument("--cpu-rows", type=int, default=750_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=2000)
