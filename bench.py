@@ -266,6 +266,29 @@ def bench_join(ctx, args):
     assert bloom["no_filter"]["hit10_matches"] == bloom["bloom_8_bits_per_key"]["hit10_matches"]
     tf.free()
     fk10.free()
+    # the random-read roofline of a probe: 60M uniformly random keys (every one a hit)
+    # into unique-key tables of 4 MiB (L2 resident) to 256 MiB of slots
+    rate = {}
+    for nbk in (1 << 17, 1 << 19, 1 << 21, 1 << 23):
+        # keys 0 .. nbk-1 (the table's SplitMix64 slot hash scatters them), probes
+        # uniform over the same range
+        keys = DeviceColumn.from_numpy(ctx, np.arange(nbk, dtype=np.int64))
+        probes = W.device_uniform(ctx, npr, A.I64, 0x9E37, 0, nbk - 1)
+        tu = HashTable(ctx, keys)
+        ms_u = timed(lambda: tu.probe_count(probes, result=cnt.buf, fetch=False), 5)
+        assert cnt.value() == npr
+        ctx.tune(A.TUNE_JOIN_HOT, 1)
+        ms_nohot = timed(lambda: tu.probe_count(probes, result=cnt.buf, fetch=False), 5)
+        ctx.tune(A.TUNE_JOIN_HOT, 0)
+        _, cap, _ = tu.info()
+        rate[f"table_{cap * 16 >> 20}_MiB"] = {
+            "build_keys": nbk, "ms": round(ms_nohot, 4),
+            "G_probes_s": round(npr / (ms_nohot / 1e3) / 1e9, 1)}
+        log(f"uniform probes into a {cap * 16 >> 20} MiB table: {ms_nohot:.3f} ms "
+            f"({npr / (ms_nohot / 1e3) / 1e9:.1f} G/s; {ms_u:.3f} ms with the image policy)")
+        tu.free()
+        keys.free()
+        probes.free()
     jb = join_bytes(nb, npr)
     for e in ev:
         e.free()
@@ -280,6 +303,7 @@ def bench_join(ctx, args):
             "group_by_probe_key": {"ms": round(gkey_ms, 4), "groups": W.JOIN_GKEY_CARD,
                                    "GB_s": round(jb["by_gkey"] / 1e9 / (gkey_ms / 1e3), 1)},
             "matches": int(matches), "without_hot_key_image": no_hot,
+            "uniform_probe_rate_by_table_size": rate,
             "hash_group_by_60M": hash_group, "bloom_prefilter": bloom,
             "_tables": (build, probe)}
 
