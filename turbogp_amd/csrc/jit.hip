@@ -70,6 +70,16 @@ std::mutex g_mu;
 std::condition_variable g_done;
 std::map<std::string, std::unique_ptr<Entry>> g_cache;  // key: the Shape's bytes
 JitStats g_stats;
+int g_pending = 0;  // background compiles not finished
+
+// Destroyed before the objects above (reverse order of definition): waits for the
+// background compiles, which use them, before the library's statics go away.
+struct DrainAtExit {
+  ~DrainAtExit() {
+    std::unique_lock<std::mutex> lk(g_mu);
+    g_done.wait(lk, [] { return g_pending == 0; });
+  }
+} g_drain;
 
 // Compiles s for `arch` into a code object (no device calls: safe off the
 // calling thread).  Returns false with the message in *err on failure.
@@ -104,7 +114,7 @@ bool compile(const Shape& s, const std::string& arch, std::vector<char>* code, s
 }
 
 // Runs the compile of entry e (registered as pending) and publishes its result.
-void compile_entry(Entry* e, Shape s, std::string arch) {
+void compile_entry(Entry* e, Shape s, std::string arch, bool background) {
   const auto t0 = std::chrono::steady_clock::now();
   std::vector<char> code;
   std::string err;
@@ -122,6 +132,7 @@ void compile_entry(Entry* e, Shape s, std::string arch) {
     ++g_stats.failed;
   }
   g_stats.compile_ms += ms;
+  if (background) --g_pending;
   g_done.notify_all();
 }
 
@@ -141,10 +152,11 @@ int32_t jit_shape_kernels(const Shape& s, bool wait, JitShapeKernels* out, bool*
     Entry* e = it->second.get();
     if (wait) {
       lk.unlock();
-      compile_entry(e, s, prop.gcnArchName);
+      compile_entry(e, s, prop.gcnArchName, false);
       lk.lock();
     } else {
-      std::thread(compile_entry, e, s, std::string(prop.gcnArchName)).detach();
+      ++g_pending;
+      std::thread(compile_entry, e, s, std::string(prop.gcnArchName), true).detach();
     }
   }
   Entry* e = it->second.get();
