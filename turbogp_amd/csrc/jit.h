@@ -24,5 +24,7 @@ struct JitStats {
 // safe; a failed compile is remembered and its error returned again.
 int32_t jit_shape_kernels(const Shape& s, bool wait, JitShapeKernels* out, bool* ready);
 JitStats jit_stats();
+// Waits (at most two minutes) for the background compiles started so far.
+void jit_drain();
 
 }  // namespace tgp

@@ -66,20 +66,20 @@ struct Entry {
   std::map<int, Loaded> dev;
 };
 
-std::mutex g_mu;
-std::condition_variable g_done;
-std::map<std::string, std::unique_ptr<Entry>> g_cache;  // key: the Shape's bytes
-JitStats g_stats;
-int g_pending = 0;  // background compiles not finished
-
-// Destroyed before the objects above (reverse order of definition): waits for the
-// background compiles, which use them, before the library's statics go away.
-struct DrainAtExit {
-  ~DrainAtExit() {
-    std::unique_lock<std::mutex> lk(g_mu);
-    g_done.wait(lk, [] { return g_pending == 0; });
-  }
-} g_drain;
+// The cache and its lock live for the whole process and are never destroyed: a
+// background compile still running while the process exits must not find them
+// gone.  Waiting for it at exit instead is not safe: the exit handlers of the
+// libraries the compile loads (run before this library's) can leave it stuck.
+// jit_drain, called when a context is destroyed, waits for the compiles while
+// everything is still alive.
+struct JitState {
+  std::mutex mu;
+  std::condition_variable done;
+  std::map<std::string, std::unique_ptr<Entry>> cache;  // key: the Shape's bytes
+  JitStats stats;
+  int pending = 0;  // background compiles not finished
+};
+JitState& g_jit = *new JitState;
 
 // Compiles s for `arch` into a code object (no device calls: safe off the
 // calling thread).  Returns false with the message in *err on failure.
@@ -121,19 +121,19 @@ void compile_entry(Entry* e, Shape s, std::string arch, bool background) {
   const bool ok = compile(s, arch, &code, &err);
   const double ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  std::lock_guard<std::mutex> lk(g_mu);
+  std::lock_guard<std::mutex> lk(g_jit.mu);
   if (ok) {
     e->code = std::move(code);
     e->state = Entry::kCompiled;
-    ++g_stats.compiled;
+    ++g_jit.stats.compiled;
   } else {
     e->err = std::move(err);
     e->state = Entry::kFailed;
-    ++g_stats.failed;
+    ++g_jit.stats.failed;
   }
-  g_stats.compile_ms += ms;
-  if (background) --g_pending;
-  g_done.notify_all();
+  g_jit.stats.compile_ms += ms;
+  if (background) --g_jit.pending;
+  g_jit.done.notify_all();
 }
 
 }  // namespace
@@ -143,26 +143,26 @@ int32_t jit_shape_kernels(const Shape& s, bool wait, JitShapeKernels* out, bool*
   int device = 0;
   TGP_HIP(hipGetDevice(&device));
   const std::string key(reinterpret_cast<const char*>(&s), sizeof(Shape));
-  std::unique_lock<std::mutex> lk(g_mu);
-  auto it = g_cache.find(key);
-  if (it == g_cache.end()) {
+  std::unique_lock<std::mutex> lk(g_jit.mu);
+  auto it = g_jit.cache.find(key);
+  if (it == g_jit.cache.end()) {
     hipDeviceProp_t prop;
     TGP_HIP(hipGetDeviceProperties(&prop, device));
-    it = g_cache.emplace(key, std::make_unique<Entry>()).first;
+    it = g_jit.cache.emplace(key, std::make_unique<Entry>()).first;
     Entry* e = it->second.get();
     if (wait) {
       lk.unlock();
       compile_entry(e, s, prop.gcnArchName, false);
       lk.lock();
     } else {
-      ++g_pending;
+      ++g_jit.pending;
       std::thread(compile_entry, e, s, std::string(prop.gcnArchName), true).detach();
     }
   }
   Entry* e = it->second.get();
   if (e->state == Entry::kPending) {
     if (!wait) return TGP_OK;
-    g_done.wait(lk, [&] { return e->state != Entry::kPending; });
+    g_jit.done.wait(lk, [&] { return e->state != Entry::kPending; });
   }
   if (e->state == Entry::kFailed) {
     set_error("%s", e->err.c_str());
@@ -176,15 +176,20 @@ int32_t jit_shape_kernels(const Shape& s, bool wait, JitShapeKernels* out, bool*
     TGP_HIP(hipModuleGetFunction(&L.k.wave, L.mod, "tgp_jit_wave"));
     d = e->dev.emplace(device, L).first;
   }
-  ++g_stats.launches;
+  ++g_jit.stats.launches;
   *out = d->second.k;
   *ready = true;
   return TGP_OK;
 }
 
+void jit_drain() {
+  std::unique_lock<std::mutex> lk(g_jit.mu);
+  g_jit.done.wait_for(lk, std::chrono::minutes(2), [] { return g_jit.pending == 0; });
+}
+
 JitStats jit_stats() {
-  std::lock_guard<std::mutex> lk(g_mu);
-  return g_stats;
+  std::lock_guard<std::mutex> lk(g_jit.mu);
+  return g_jit.stats;
 }
 
 }  // namespace tgp

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "tgp_internal.h"
+#include "jit.h"
 
 namespace tgp {
 
@@ -214,6 +215,8 @@ int32_t tgp_ctx_create(int32_t device, uint64_t hbm_budget_bytes, tgp_ctx** out)
 
 int32_t tgp_ctx_destroy(tgp_ctx* ctx) {
   if (!ctx) return TGP_OK;
+  // run-time compiles still in flight finish while every library is alive
+  jit_drain();
   (void)hipSetDevice(ctx->device);
   // Streams the caller created may already be destroyed: wait for the device,
   // not for the streams the scratch map remembers.
@@ -369,6 +372,10 @@ int32_t tgp_tune_set(tgp_ctx* ctx, int32_t key, int64_t value) {
       TGP_REQUIRE(value <= 2, "TGP_TUNE_JIT is 0, 1 or 2");
       ctx->tune_jit.store(value);
       return TGP_OK;
+    case TGP_TUNE_AGG_PART:
+      TGP_REQUIRE(value <= 2, "TGP_TUNE_AGG_PART is 0, 1 or 2");
+      ctx->tune_agg_part.store(value);
+      return TGP_OK;
   }
   set_error("tgp_tune_set: unknown key %d", key);
   return TGP_ERR_INVALID;
@@ -389,6 +396,7 @@ int32_t tgp_tune_get(tgp_ctx* ctx, int32_t key, int64_t* value) {
     case TGP_TUNE_GROUP_TAG_BITS: *value = ctx->tune_group_tag_bits.load(); return TGP_OK;
     case TGP_TUNE_JOIN_HOT: *value = ctx->tune_join_hot.load(); return TGP_OK;
     case TGP_TUNE_JIT: *value = ctx->tune_jit.load(); return TGP_OK;
+    case TGP_TUNE_AGG_PART: *value = ctx->tune_agg_part.load(); return TGP_OK;
   }
   set_error("tgp_tune_get: unknown key %d", key);
   return TGP_ERR_INVALID;
