@@ -341,6 +341,32 @@ def bench_hash_group(ctx, probe):
         finally:
             for b in bufs:
                 b.free()
+    # the int32 key as dense group ids (its range known): the generic engine's
+    # partitioned form (rows sorted into 733 partitions of 2048 groups), events
+    # around the asynchronous call
+    from turbogp_amd import Event, GroupAggBuffers
+    q = make_agg_query([probe["gkey"], probe["val"]], (), [(0, 1, 0, W.JOIN_GKEY_CARD)],
+                       [("sum", 1, [(1, 0, 1)])])
+    res = GroupAggBuffers(ctx, W.JOIN_GKEY_CARD, 1)
+    ev = [Event(ctx) for _ in range(2)]
+    try:
+        ts = []
+        for i in range(8):
+            ev[0].record()
+            ctx.group_agg_async(q, res)
+            ev[1].record()
+            if i >= 2:
+                ts.append(ev[0].elapsed_ms(ev[1]))
+        ms = statistics.mean(ts)
+        got = res.fetch()
+        out["by_int32_key_dense_ids_partitioned"] = {
+            "ms": round(ms, 4), "groups": int((got.count != 0).sum()),
+            "rows_per_s": round(W.JOIN_PROBE_ROWS / (ms / 1e3), 1)}
+        log(f"dense-id group by int32 key: {ms:.3f} ms")
+    finally:
+        res.free()
+        for e in ev:
+            e.free()
     return out
 
 
@@ -400,8 +426,11 @@ def bench_generic(ctx, table, args):
             runs = [("generic", 1)]
             if sh.value == A.SPEC_SHAPE_RUNTIME:  # compiled before the first call returns
                 runs.append(("compiled_at_run_time", 2))
+            if st.value == 5:  # partitioned by group range; also time the one global table
+                runs.append(("global_table", 1))
             for path, mode in runs:
                 ctx.tune(A.TUNE_JIT, mode)
+                ctx.tune(A.TUNE_AGG_PART, 1 if path == "global_table" else 0)
                 extra = {}
                 if mode == 2:
                     c0, f0, ms0 = jit_info()
@@ -417,9 +446,17 @@ def bench_generic(ctx, table, args):
                 out[f"{name}_{path}"] = {"ms": round(ms, 4), "GB_s": round(gbs, 1),
                                          "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
                                          "bytes_per_row": bpr, "groups": int(g.value),
-                                         "strategy": int(st.value), **extra}
+                                         "strategy": 3 if path == "global_table"
+                                         else int(st.value), **extra}
                 log(f"{name} {path}: {ms:.3f} ms, {gbs:.0f} GB/s {extra}")
-            if len(runs) == 2:
+                if path == "global_table":
+                    got = res.fetch()
+                    ctx.tune(A.TUNE_AGG_PART, 0)
+                    ctx.group_agg_async(q, res)
+                    want = res.fetch()
+                    assert got.sums() == want.sums() and list(got.count) == list(want.count)
+            ctx.tune(A.TUNE_AGG_PART, 0)
+            if sh.value == A.SPEC_SHAPE_RUNTIME:
                 ctx.tune(A.TUNE_JIT, 2)
                 ctx.group_agg_async(q, res)
                 want = res.fetch()
@@ -427,11 +464,12 @@ def bench_generic(ctx, table, args):
                 ctx.group_agg_async(q, res)
                 got = res.fetch()
             ctx.tune(A.TUNE_JIT, 0)
-            if len(runs) == 2:
+            if sh.value == A.SPEC_SHAPE_RUNTIME:
                 assert got.sums() == want.sums() and list(got.count) == list(want.count), name
             res.free()
     finally:
         ctx.tune(A.TUNE_JIT, 0)
+        ctx.tune(A.TUNE_AGG_PART, 0)
         for e in ev:
             e.free()
         price32.free()

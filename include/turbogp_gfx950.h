@@ -238,12 +238,17 @@ enum tgp_tune_key {
                                    table): 0 (default) for unique-key single-column tables of
                                    >= 8 MiB probed by >= 4M rows; 1 off; 2 for every probe of a
                                    unique-key single-column table (tests) */
-  TGP_TUNE_JIT = 11             /* queries with a shape-specialised form (tgp_group_agg_explain
+  TGP_TUNE_JIT = 11,            /* queries with a shape-specialised form (tgp_group_agg_explain
                                    spec_shape >= 0) but no kernel compiled ahead of time: 0
                                    (default) compile that form at run time (hipRTC, about a
                                    second per shape, cached for the process) for queries of
                                    >= 2^20 rows, run smaller ones on the generic engine; 1
                                    never compile; 2 always (tests) */
+  TGP_TUNE_AGG_PART = 12        /* generic engine, groups too many for an LDS table (strategy
+                                   3 or 5 of tgp_group_agg_explain): 0 (default) partition the
+                                   rows by group range and aggregate each partition in LDS for
+                                   >= 2^18 rows; 1 never (one global table); 2 at any row
+                                   count (tests).  Never while the stream is being captured */
 };
 int32_t tgp_tune_set(tgp_ctx* ctx, int32_t key, int64_t value);
 int32_t tgp_tune_get(tgp_ctx* ctx, int32_t key, int64_t* value);
@@ -403,7 +408,7 @@ int32_t tgp_group_agg(tgp_ctx* ctx, const tgp_agg_query* q, const tgp_group_resu
 /* Plan of a query without running it (host only): the accumulation strategy
  * of the generic engine (0 registers/1 group, 1 registers/<=8 groups, 2 LDS
  * table, 3 global table, 4 global table of int128 words for ranges of 2^32 rows or
- * more) and the index of the shape-specialised kernel compiled ahead of time that
+ * more, 5 rows partitioned by group range into LDS tables) and the index of the shape-specialised kernel compiled ahead of time that
  * would run it, or TGP_SPEC_SHAPE_RUNTIME when the query has a specialised form
  * that is compiled at run time (see TGP_TUNE_JIT), or -1 (none: the generic
  * engine runs it). */

@@ -34,7 +34,7 @@ AGG_MINMAX = {"min": AGG_MIN, "max": AGG_MAX, "min_u64": AGG_MIN_U64, "max_u64":
 JOIN_GROUP_BUILD_PAYLOAD, JOIN_GROUP_PROBE_COL, JOIN_GROUP_NONE = 0, 1, 2
 TUNE_WG_PER_CU, TUNE_GRID_MAX, TUNE_SPECIALIZE, TUNE_SPEC_KERNEL, TUNE_CHUNK = 0, 1, 2, 3, 4
 TUNE_SLICES_PER_WAVE, TUNE_JOIN_ATOMICS, TUNE_JOIN_PACK_BITS, TUNE_DISTINCT_SHRINK = 5, 6, 7, 8
-TUNE_GROUP_TAG_BITS, TUNE_JOIN_HOT, TUNE_JIT = 9, 10, 11
+TUNE_GROUP_TAG_BITS, TUNE_JOIN_HOT, TUNE_JIT, TUNE_AGG_PART = 9, 10, 11, 12
 (EX_COL, EX_INT, EX_FLOAT, EX_DATE, EX_NULL, EX_ADD, EX_SUB, EX_MUL, EX_DIV, EX_MOD, EX_EQ, EX_NE,
  EX_LT, EX_LE, EX_GT, EX_GE, EX_AND, EX_OR, EX_NOT, EX_NEG, EX_BETWEEN, EX_NOT_BETWEEN, EX_IN,
  EX_NOT_IN, EX_IF) = range(25)

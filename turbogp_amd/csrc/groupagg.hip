@@ -283,6 +283,177 @@ __global__ void __launch_bounds__(kBlock, 2) k_agg_table(DevPlan P) {
   if (oor && P.out_oor) atomicAdd(P.out_oor, (unsigned long long)oor);
 }
 
+// ---- radix-partitioned aggregation (kAggPart) ---------------------------------------------
+// A table too large for LDS costs one device atomic per row for the count and one or
+// two per aggregate, all at random words.  kAggPart instead sorts the rows into
+// partitions of 2^part_shift consecutive groups and aggregates each partition in an
+// LDS table:
+//   (1) k_part_hist_tiles: through a plan that stages only the predicate and group
+//       columns, workgroup w of G counts its selected rows per partition
+//       p = gid >> part_shift into mat[p * G + w];
+//   (2) an exclusive scan of mat (partition-major) gives every (partition,
+//       workgroup) pair its range of records;
+//   (3) k_part_scatter_tiles: the same tiles on the same G workgroups (so the
+//       counts match) through the full plan; each selected row takes a rank among
+//       its tile's rows of the same partition (LDS counter) and is written at the
+//       pair's cursor plus that rank, as the local group (16 bits) and one 64-bit
+//       value per aggregate (structure of arrays);
+//   (4) k_part_agg_recs: workgroup c takes records [c * kPartChunk, ...), a run of
+//       whole or partial partitions since the records are sorted by partition,
+//       aggregates each partition's run in an LDS table of 2^part_shift groups
+//       (count, split sums or MIN/MAX codes) and adds its nonzero groups to the
+//       result with carry-exact device atomics.
+// The order of records inside a range varies from run to run; the totals do not.
+constexpr uint64_t kPartChunk = 1ull << 15;  // records per aggregation workgroup
+constexpr int kPartAggBlock = 1024;
+
+__global__ void __launch_bounds__(kBlock) k_part_hist_tiles(DevPlan P, uint32_t shift,
+                                                            uint32_t nparts,
+                                                            unsigned long long* __restrict__ mat) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  uint32_t* h = reinterpret_cast<uint32_t*>(lds + 2 * P.stage_bytes);
+  for (uint32_t p = threadIdx.x; p < nparts; p += kBlock) h[p] = 0;
+  // made visible by the barrier at the top of the first tile
+  for_each_tile(P, lds, [&](const uint8_t* buf, uint64_t row0) {
+    const uint32_t sel = eval_select(P, buf, row0);
+    if (__ballot(sel != 0) == 0) return;
+    int64_t gid[kRows];
+    eval_groups(P, buf, gid);
+#pragma unroll
+    for (int r = 0; r < kRows; ++r)
+      if (((sel >> r) & 1) && gid[r] >= 0) atomicAdd(&h[(uint64_t)gid[r] >> shift], 1u);
+  });
+  __syncthreads();
+  for (uint32_t p = threadIdx.x; p < nparts; p += kBlock)
+    mat[(uint64_t)p * gridDim.x + blockIdx.x] = h[p];
+}
+
+// mat: exclusive offsets after the scan.  cap: the length of each record array.
+__global__ void __launch_bounds__(kBlock) k_part_scatter_tiles(
+    DevPlan P, uint32_t shift, uint32_t nparts, const unsigned long long* __restrict__ mat,
+    uint64_t cap, uint16_t* __restrict__ rgrp, uint64_t* __restrict__ rval) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  unsigned long long* cursor = reinterpret_cast<unsigned long long*>(lds + 2 * P.stage_bytes);
+  uint32_t* cnt = reinterpret_cast<uint32_t*>(cursor + nparts);
+  for (uint32_t p = threadIdx.x; p < nparts; p += kBlock) {
+    cursor[p] = mat[(uint64_t)p * gridDim.x + blockIdx.x];
+    cnt[p] = 0;
+  }
+  // made visible by the barrier at the top of the first tile
+  const uint32_t lmask = (1u << shift) - 1;
+  uint32_t oor = 0;
+  for_each_tile(P, lds, [&](const uint8_t* buf, uint64_t row0) {
+    const uint32_t sel = eval_select(P, buf, row0);
+    if (!__syncthreads_or(sel != 0)) return;  // the same for the whole workgroup
+    int64_t gid[kRows];
+    eval_groups(P, buf, gid);
+    uint32_t pr[kRows], rk[kRows];
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      pr[r] = 0xffffffffu;
+      rk[r] = 0;
+      if (!((sel >> r) & 1)) continue;
+      if (gid[r] < 0) {
+        ++oor;
+        continue;
+      }
+      pr[r] = (uint32_t)((uint64_t)gid[r] >> shift);
+      rk[r] = atomicAdd(&cnt[pr[r]], 1u);
+    }
+    __syncthreads();  // every rank taken
+    uint64_t dst[kRows];
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      dst[r] = 0;
+      if (pr[r] == 0xffffffffu) continue;
+      dst[r] = cursor[pr[r]] + rk[r];
+      rgrp[dst[r]] = (uint16_t)((uint64_t)gid[r] & lmask);
+    }
+    for (int a = 0; a < P.naggs; ++a) {
+      uint64_t v[kRows];
+      eval_agg(P.aggs[a], buf, v);
+      uint64_t* out = rval + (uint64_t)a * cap;
+#pragma unroll
+      for (int r = 0; r < kRows; ++r)
+        if (pr[r] != 0xffffffffu) out[dst[r]] = v[r];
+    }
+    __syncthreads();  // every cursor read
+    // the row ranked first in its partition advances that partition's cursor; the
+    // barrier at the top of the next tile orders this before the next ranks
+#pragma unroll
+    for (int r = 0; r < kRows; ++r)
+      if (pr[r] != 0xffffffffu && rk[r] == 0) {
+        cursor[pr[r]] += cnt[pr[r]];
+        cnt[pr[r]] = 0;
+      }
+  });
+  if (oor && P.out_oor) atomicAdd(P.out_oor, (unsigned long long)oor);
+}
+
+// G: the workgroups of the histogram and the scatter.  LDS: 2^shift x (1 + 2 naggs)
+// words.  Results (zeroed) receive int128 sums and MIN/MAX codes.
+__global__ void __launch_bounds__(kPartAggBlock) k_part_agg_recs(
+    DevPlan P, uint32_t shift, uint32_t nparts, uint64_t G,
+    const unsigned long long* __restrict__ mat, uint64_t cap,
+    const uint16_t* __restrict__ rgrp, const uint64_t* __restrict__ rval) {
+  extern __shared__ __align__(16) unsigned long long tab[];
+  const uint64_t total = mat[(uint64_t)nparts * G];
+  uint64_t lo = (uint64_t)blockIdx.x * kPartChunk;
+  if (lo >= total) return;
+  const uint64_t hi = lo + kPartChunk < total ? lo + kPartChunk : total;
+  const uint32_t W = 1 + 2 * (uint32_t)P.naggs, gp = 1u << shift;
+  // the partition holding record lo: the last p with mat[p * G] <= lo
+  uint32_t pa = 0, pb = nparts - 1;
+  while (pa < pb) {
+    const uint32_t m = (pa + pb + 1) >> 1;
+    if (mat[(uint64_t)m * G] <= lo) pa = m;
+    else pb = m - 1;
+  }
+  for (uint32_t p = pa; lo < hi && p < nparts; ++p) {
+    const uint64_t pe = mat[(uint64_t)(p + 1) * G];
+    const uint64_t end = pe < hi ? pe : hi;
+    if (end <= lo) continue;
+    for (uint32_t i = threadIdx.x; i < gp * W; i += kPartAggBlock) tab[i] = 0;
+    __syncthreads();
+    for (uint64_t i = lo + threadIdx.x; i < end; i += kPartAggBlock) {
+      unsigned long long* w = tab + rgrp[i] * W;
+      atomicAdd(w, 1ull);
+      for (int k = 0; k < P.naggs; ++k) {
+        const uint64_t v = rval[(uint64_t)k * cap + i];
+        const int32_t mk = (P.minmax_kinds >> (3 * k)) & 7;
+        if (mk) {
+          atomicMax(w + 1 + 2 * k, (unsigned long long)mm_encode(mk, v));
+        } else {
+          atomicAdd(w + 1 + 2 * k, (unsigned long long)(uint32_t)v);
+          const int64_t h = (int64_t)v >> 32;
+          if (h) atomicAdd(w + 2 + 2 * k, (unsigned long long)h);
+        }
+      }
+    }
+    __syncthreads();
+    const uint64_t g0 = (uint64_t)p << shift;
+    for (uint32_t gl = threadIdx.x; gl < gp; gl += kPartAggBlock) {
+      const unsigned long long* w = tab + gl * W;
+      if (!w[0]) continue;
+      const uint64_t g = g0 + gl;
+      atomicAdd(&P.out_count[g], w[0]);
+      for (int k = 0; k < P.naggs; ++k) {
+        unsigned long long* o = &P.out_sums[(g * P.naggs + k) * 2];
+        if ((P.minmax_kinds >> (3 * k)) & 7) {
+          atomicMax(o, w[1 + 2 * k]);
+        } else {
+          uint64_t l;
+          int64_t h;
+          split_to_i128(w[1 + 2 * k], (int64_t)w[2 + 2 * k], &l, &h);
+          atomic_add128(o, l, h);
+        }
+      }
+    }
+    __syncthreads();  // every flush read before the next partition clears the table
+    lo = end;
+  }
+}
+
 // ---- many queries of any form in one launch (the batch's generic items) -----------------
 // Persistent workgroups take items (query, tiles [t0, t1)) from a counter, copy
 // the item's plan into LDS, stage its tiles double-buffered and aggregate them
@@ -410,6 +581,80 @@ int32_t try_spec(tgp_ctx* ctx, const tgp_agg_query* q, const tgp_group_result* o
                  hipStream_t st, bool* launched);
 int32_t spec_shape_index(const tgp_agg_query* q);
 
+// The histogram's plan: q with only its predicate and group-key columns (the same
+// row range, selection bitmap, tiles and group ids) and no aggregates.
+static int32_t plan_part_hist(const tgp_agg_query* q, DevPlan* H) {
+  tgp_agg_query rq = *q;
+  int map[TGP_MAX_COLS];
+  for (int c = 0; c < TGP_MAX_COLS; ++c) map[c] = -1;
+  rq.ncols = 0;
+  auto use = [&](int c) {
+    if (map[c] < 0) {
+      map[c] = rq.ncols;
+      rq.cols[rq.ncols++] = q->cols[c];
+    }
+    return map[c];
+  };
+  for (int p = 0; p < q->npreds; ++p) rq.preds[p].col = use(q->preds[p].col);
+  for (int k = 0; k < q->nparts; ++k) rq.parts[k].col = use(q->parts[k].col);
+  rq.naggs = 0;
+  PlanInfo info;
+  return plan_agg(&rq, H, &info);
+}
+
+// LDS of the partitioned passes' tile kernels.
+static uint32_t part_hist_lds(const DevPlan& H, uint32_t nparts) {
+  return 2 * H.stage_bytes + nparts * 4;
+}
+static uint32_t part_scatter_lds(const DevPlan& P, uint32_t nparts) {
+  return 2 * P.stage_bytes + nparts * 12;
+}
+constexpr uint32_t kLdsMax = 160 * 1024;
+
+template <typename K>
+static int32_t allow_lds(K kern, uint32_t bytes) {
+  if (bytes > 65536)
+    TGP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)bytes));
+  return TGP_OK;
+}
+
+// kAggPart: histogram, scan, scatter and per-partition aggregation on stream s
+// (the result already zeroed).
+static int32_t run_part(tgp_ctx* ctx, const DevPlan& P, const DevPlan& H, const PlanInfo& info,
+                        hipStream_t s) {
+  if (P.ntiles == 0) return TGP_OK;
+  const uint32_t np = info.nparts, sh = info.part_shift;
+  const uint64_t rows = P.row_end - P.row_begin;
+  const uint64_t G = std::min<uint64_t>(P.ntiles, (uint64_t)persistent_grid(ctx, 2));
+  const uint64_t mat_n = (uint64_t)np * G;
+  const uint64_t nb = (mat_n + 4095) / 4096;
+  // scratch: mat[mat_n + 1] | scan block sums[nb + 1] | values[naggs][rows] | groups[rows]
+  const uint64_t words = mat_n + 1 + nb + 1 + (uint64_t)P.naggs * rows;
+  void* scr = nullptr;
+  TGP_TRY(get_scratch(ctx, s, words * 8 + rows * 2, &scr));
+  auto* mat = static_cast<unsigned long long*>(scr);
+  auto* bsum = reinterpret_cast<uint64_t*>(mat + mat_n + 1);
+  auto* rval = bsum + nb + 1;
+  auto* rgrp = reinterpret_cast<uint16_t*>(rval + (uint64_t)P.naggs * rows);
+  const uint32_t lh = part_hist_lds(H, np), ls = part_scatter_lds(P, np);
+  TGP_TRY(allow_lds(k_part_hist_tiles, lh));
+  TGP_TRY(allow_lds(k_part_scatter_tiles, ls));
+  k_part_hist_tiles<<<dim3((unsigned)G), dim3(kBlock), lh, s>>>(H, sh, np, mat);
+  TGP_HIP(hipGetLastError());
+  TGP_TRY(scan_u64(s, reinterpret_cast<uint64_t*>(mat), mat_n, bsum));
+  k_part_scatter_tiles<<<dim3((unsigned)G), dim3(kBlock), ls, s>>>(P, sh, np, mat, rows, rgrp,
+                                                                   rval);
+  TGP_HIP(hipGetLastError());
+  const uint32_t la = (uint32_t)((1ull << sh) * (1 + 2 * (uint64_t)P.naggs) * 8);
+  TGP_TRY(allow_lds(k_part_agg_recs, la));
+  const uint64_t ga = (rows + kPartChunk - 1) / kPartChunk;
+  k_part_agg_recs<<<dim3((unsigned)ga), dim3(kPartAggBlock), la, s>>>(P, sh, np, G, mat, rows,
+                                                                      rgrp, rval);
+  TGP_HIP(hipGetLastError());
+  return TGP_OK;
+}
+
 int32_t run_agg(tgp_ctx* ctx, const DevPlan& P, const PlanInfo& info, hipStream_t s,
                 const tgp_agg_query* q = nullptr, const tgp_group_result* out = nullptr) {
   if (q) {
@@ -425,17 +670,33 @@ int32_t run_agg(tgp_ctx* ctx, const DevPlan& P, const PlanInfo& info, hipStream_
                     reinterpret_cast<uint64_t*>(P.out_oor)};
   const uint64_t n[3] = {P.ngroups, P.ngroups * P.naggs * 2, 1};
   TGP_TRY(zero_ranges(s, 3, p, n));
-  switch (info.strategy) {
+  int strategy = info.strategy;
+  DevPlan H;
+  if (info.nparts && (strategy == kAggGlobal || strategy == kAggPart)) {
+    const int64_t mode = ctx->tune_agg_part.load();
+    bool part = mode == 2 || (mode == 0 && strategy == kAggPart);
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    TGP_HIP(hipStreamIsCapturing(s, &cs));
+    if (!q || cs != hipStreamCaptureStatusNone) part = false;  // scratch may be allocated
+    if (part) {
+      TGP_TRY(plan_part_hist(q, &H));
+      part = part_hist_lds(H, info.nparts) <= kLdsMax &&
+             part_scatter_lds(P, info.nparts) <= kLdsMax;
+    }
+    strategy = part ? kAggPart : kAggGlobal;
+  }
+  switch (strategy) {
     case kAggReg1: TGP_TRY(launch_tiles(ctx, k_agg_reg<1, TGP_MAX_AGGS>, P, info.lds_bytes, s)); break;
     case kAggReg8: TGP_TRY(launch_tiles(ctx, k_agg_reg<8, 4>, P, info.lds_bytes, s)); break;
     case kAggLds: TGP_TRY(launch_tiles(ctx, k_agg_table<true, true>, P, info.lds_bytes, s)); break;
     case kAggGlobal:
       TGP_TRY(launch_tiles(ctx, k_agg_table<false, true>, P, info.lds_bytes, s));
       break;
+    case kAggPart: TGP_TRY(run_part(ctx, P, H, info, s)); break;
     default: TGP_TRY(launch_tiles(ctx, k_agg_table<false, false>, P, info.lds_bytes, s)); break;
   }
   // the global table holds split sums; MIN/MAX hold codes
-  const bool split = info.strategy == kAggGlobal;
+  const bool split = strategy == kAggGlobal;
   if ((P.minmax_kinds || split) && P.ngroups && P.naggs && P.ntiles) {
     const uint64_t items = P.ngroups * (uint64_t)P.naggs;
     const uint64_t grid = std::min<uint64_t>((items + 255) / 256, 65536);

@@ -967,7 +967,7 @@ __global__ void __launch_bounds__(256) k_scan_chunks(uint64_t* v, uint64_t n, co
 }
 
 // bsum holds nblocks + 1 words.
-static int32_t scan_u64(hipStream_t s, uint64_t* v, uint64_t n, uint64_t* bsum) {
+int32_t scan_u64(hipStream_t s, uint64_t* v, uint64_t n, uint64_t* bsum) {
   const uint64_t nb = (n + kScanChunk - 1) / kScanChunk;
   if (nb == 0) return fill_u64(s, v, 1, 0);
   k_scan_sums<<<(unsigned)nb, 256, 0, s>>>(v, n, bsum);

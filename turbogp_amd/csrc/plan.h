@@ -11,13 +11,25 @@ enum AggStrategy {
   kAggReg8 = 1,   // <= 8 groups, <= 4 aggregates: per-thread registers
   kAggLds = 2,    // per-workgroup LDS table + 64-bit LDS atomics
   kAggGlobal = 3,       // global table of split sums, 64-bit device atomics per row
-  kAggGlobalExact = 4   // global table of int128 sums (2^32 rows or more)
+  kAggGlobalExact = 4,  // global table of int128 sums (2^32 rows or more)
+  kAggPart = 5          // rows partitioned by group range, one LDS table per partition
 };
 
 struct PlanInfo {
   uint32_t lds_bytes = 0;  // dynamic LDS of the kernel (2 stage buffers + table)
   int strategy = kAggReg1;
+  // kAggPart geometry, set whenever the groups can be partitioned (nparts > 0):
+  // 2^part_shift groups per partition
+  uint32_t part_shift = 0, nparts = 0;
 };
+
+// Radix-partitioned aggregation (kAggPart): the LDS table of one partition, the
+// most partitions (the scatter keeps 12 bytes of LDS per partition), and the
+// smallest row range that takes it by default (below it the global table's
+// device atomics cost less than the extra passes).
+constexpr uint32_t kPartTableBytes = 64 * 1024;
+constexpr uint32_t kPartMax = 4096;
+constexpr uint64_t kPartMinRows = 1ull << 18;
 
 // A work item of the batch's generic path: tiles [t0, t1) of plan q, aggregated in
 // an LDS table of split sums (mode 1), straight into the result as split sums

@@ -154,6 +154,19 @@ int32_t plan_agg(const tgp_agg_query* q, DevPlan* P, PlanInfo* info) {
     info->strategy = kAggLds;
   else
     info->strategy = kAggGlobal;
+  if (info->strategy == kAggGlobal) {
+    // partitions of 2^s groups whose table (count, two words per aggregate) fits
+    // kPartTableBytes; a record keeps the local group in 16 bits
+    const uint64_t wbytes = (1 + 2 * (uint64_t)q->naggs) * 8;
+    uint32_t sh = 0;
+    while (sh < 16 && (wbytes << (sh + 1)) <= kPartTableBytes) ++sh;
+    const uint64_t np = (ngroups + (1ull << sh) - 1) >> sh;
+    if (np <= kPartMax) {
+      info->part_shift = sh;
+      info->nparts = (uint32_t)np;
+      if (q->row_end - q->row_begin >= kPartMinRows) info->strategy = kAggPart;
+    }
+  }
   if (info->strategy == kAggLds) info->lds_bytes += (uint32_t)(table_bytes << rs);
   if ((info->strategy == kAggReg1 || info->strategy == kAggReg8) &&
       info->lds_bytes < kRegReduceBytes)

@@ -132,6 +132,7 @@ struct tgp_ctx {
   std::atomic<int64_t> tune_group_tag_bits{0};   // hash GROUP BY: 0 = auto; n = verified form, n-bit tags
   std::atomic<int64_t> tune_join_hot{0};         // join hot-key cache: 0 auto, 1 off, 2 on
   std::atomic<int64_t> tune_jit{0};              // run-time compiled shapes: 0 auto, 1 off, 2 on
+  std::atomic<int64_t> tune_agg_part{0};         // partitioned wide GROUP BY: 0 auto, 1 off, 2 on
 };
 
 namespace tgp {
@@ -174,5 +175,8 @@ int32_t sum_f64_exact(tgp_ctx* ctx, const uint64_t* d_cells, uint64_t n,
                       double* out);
 // Validates a column descriptor (non-null data, known dtype, 16-byte alignment).
 int32_t check_column(const tgp_column* c, const char* what);
+// Exclusive prefix sum of n words in place, the total stored at v[n]; bsum is
+// scratch of ceil(n / 4096) + 1 words.
+int32_t scan_u64(hipStream_t s, uint64_t* v, uint64_t n, uint64_t* bsum);
 }  // namespace tgp
 #endif  // __HIPCC_RTC__
