@@ -671,6 +671,26 @@ int32_t tgp_hash_join_pairs_sync(tgp_ctx* ctx, const tgp_hash_table* t,
                                  uint64_t row_begin, uint64_t row_end, uint64_t* d_probe_idx,
                                  uint32_t* d_build_payload, uint64_t capacity, uint64_t* h_npairs,
                                  void* stream);
+/* The non-equi conjuncts of a join's ON clause, applied to each match as the
+ * interpreter's hash_join does (query_interpreter/join.rs:1146-1195 with
+ * eval_non_equi_match :1233-1300): the npairs pairs of
+ * tgp_hash_join_pairs_sync over probe rows [row_begin, row_end) (its build
+ * payloads being build row indices) are kept when the program (tgp_scan_expr's
+ * codes and semantics), whose column c < nleft is left_cols[c] at the probe
+ * row and whose column c >= nleft is right_cols[c - nleft] at the build row, is
+ * truthy.  Under TGP_JOIN_LEFT a probe row whose matches all fail keeps one pair
+ * (row, TGP_JOIN_NO_MATCH) in its place, and (row, TGP_JOIN_NO_MATCH) pairs stay.
+ * Order is kept.  *h_nout <= npairs pairs are written to d_out_probe_idx /
+ * d_out_build_row (not the input arrays).  A pair naming a row outside
+ * [row_begin, row_end) or beyond the right columns fails the call.
+ * nleft + nright <= TGP_MAX_COLS.  Synchronous. */
+int32_t tgp_join_residual_sync(tgp_ctx* ctx, const tgp_column* left_cols, int32_t nleft,
+                               const tgp_column* right_cols, int32_t nright,
+                               const tgp_expr_op* prog, int32_t nops, int32_t join_type,
+                               uint64_t row_begin, uint64_t row_end,
+                               const uint64_t* d_probe_idx, const uint32_t* d_build_row,
+                               uint64_t npairs, uint64_t* d_out_probe_idx,
+                               uint32_t* d_out_build_row, uint64_t* h_nout, void* stream);
 /* Semi-join marks over nkeys probe key columns (exec/join.rs:162-199
  * semi_join / anti_join: anti = tgp_bitmap_not of the marks). */
 int32_t tgp_hash_join_mark(tgp_ctx* ctx, const tgp_hash_table* t, const tgp_column* probe_keys,

@@ -83,6 +83,7 @@ def lib():
             "orc_simd_level": (I32, []),
             "orc_filter_f64": (None, [P, U64, I32, I32, C.c_double, P]),
             "orc_scan_expr": (I32, [P, I32, P, I32, U64, U64, P]),
+            "orc_join_residual": (I64, [P, I32, I32, P, I32, I32, P, P, U64, C.c_uint32, P, P]),
             "orc_set_simd": (None, [I32]),
         }
         for name, (res, args) in sig.items():
@@ -435,3 +436,27 @@ def scan_expr(arrays, prog, row_begin=0, row_end=None, bits=None):
     if rc != 0:
         raise ValueError("malformed expression program")
     return bits
+
+
+NO_BUILD_ROW = 0xFFFFFFFF
+
+
+def join_residual(left_arrays, right_arrays, prog, probe, build, left=False):
+    """Filters equi-join pairs (probe row, build row; NO_BUILD_ROW: a LEFT row with no
+    match) by a residual program whose columns 0..len(left_arrays)-1 read the probe
+    table at the probe row and the rest the build table at the build row; LEFT keeps
+    a probe row whose matches all fail as (row, NO_BUILD_ROW)."""
+    keep = [np.ascontiguousarray(a) for a in list(left_arrays) + list(right_arrays)]
+    cols = (OColumn * max(1, len(keep)))()
+    for i, a in enumerate(keep):
+        cols[i] = OColumn(ptr(a), len(a), _FROM_NP[a.dtype], 0)
+    ops = (OExprOp * len(prog))(*[OExprOp(c, a, im) for c, a, im in prog])
+    probe = np.ascontiguousarray(probe, np.uint64)
+    build = np.ascontiguousarray(build, np.uint32)
+    n = len(probe)
+    op, ob = np.zeros(max(1, n), np.uint64), np.zeros(max(1, n), np.uint32)
+    m = lib().orc_join_residual(cols, len(keep), len(left_arrays), ops, len(prog), int(left),
+                                ptr(probe), ptr(build), n, NO_BUILD_ROW, ptr(op), ptr(ob))
+    if m < 0:
+        raise ValueError("malformed expression program")
+    return op[:m].copy(), ob[:m].copy()

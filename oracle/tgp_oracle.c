@@ -1081,64 +1081,115 @@ static orc_value v_column(const orc_column* c, int64_t kind, uint64_t row) {
   return v;
 }
 
+/* The program's value at one row, as truthy (1) or not (0); -1 for a malformed
+ * program.  Column c < nleft is read at row_l of cols[c], column c >= nleft at
+ * row_r of cols[c] (one combined row of two tables, join.rs:1276-1296). */
+static int eval_prog(const orc_column* cols, int32_t ncols, int32_t nleft, uint64_t row_l,
+                     uint64_t row_r, const orc_expr_op* prog, int32_t nops) {
+  orc_value st[64];
+  int sp = 0;
+  for (int32_t k = 0; k < nops; ++k) {
+    const orc_expr_op* o = &prog[k];
+    switch (o->code) {
+      case 0:
+        if (o->arg < 0 || o->arg >= ncols) return -1;
+        st[sp++] = v_column(&cols[o->arg], o->imm, o->arg < nleft ? row_l : row_r);
+        break;
+      case 1: st[sp++] = v_int(o->imm); break;
+      case 2: { orc_value v = v_null(); v.tag = V_FLOAT; memcpy(&v.f, &o->imm, 8); st[sp++] = v; break; }
+      case 3: { orc_value v = v_null(); v.tag = V_DATE; v.d = (int32_t)o->imm; st[sp++] = v; break; }
+      case 4: st[sp++] = v_null(); break;
+      case 18: st[sp - 1] = v_int(!v_truthy(&st[sp - 1])); break;           /* Not :1741-1744 */
+      case 19: {                                                             /* Neg :1733-1740 */
+        orc_value v = st[sp - 1];
+        if (v.tag == V_INT) st[sp - 1] = v_int((int64_t)(0 - (uint64_t)v.i));
+        else if (v.tag == V_FLOAT) st[sp - 1] = v_float(-v.f);
+        else st[sp - 1] = v_null();
+        break;
+      }
+      case 20: case 21: {                                                    /* :1773-1779 */
+        const int in = v_cmp_le(&st[sp - 2], &st[sp - 3]) && v_cmp_le(&st[sp - 3], &st[sp - 1]);
+        sp -= 2;
+        st[sp - 1] = v_int(o->code == 20 ? in : !in);
+        break;
+      }
+      case 22: case 23: {                                                    /* :1780-1790 */
+        int found = 0;
+        for (int32_t j = 0; j < o->arg && !found; ++j)
+          found = v_cmp_eq(&st[sp - 1 - o->arg], &st[sp - o->arg + j]);
+        sp -= o->arg;
+        st[sp - 1] = v_int(o->code == 22 ? found : !found);
+        break;
+      }
+      case 24: {                                                             /* Case :1791-1800 */
+        const orc_value r = v_truthy(&st[sp - 3]) ? st[sp - 2] : st[sp - 1];
+        sp -= 2;
+        st[sp - 1] = r;
+        break;
+      }
+      default:
+        if (sp < 2 || o->code < 5 || o->code > 17) return -1;
+        st[sp - 2] = v_binop(o->code, &st[sp - 2], &st[sp - 1]);
+        --sp;
+    }
+    if (sp < 1 || sp > 60) return -1;
+  }
+  if (sp != 1) return -1;
+  return v_truthy(&st[0]);
+}
+
 /* Clears bit (row - row_begin) of bits (LSB-first bytes) for every row of
  * [row_begin, row_end) whose program value is not truthy (build_mask's
  * fallback, expr.rs:1453-1463).  Returns 0, or -1 for a malformed program. */
 int32_t orc_scan_expr(const orc_column* cols, int32_t ncols, const orc_expr_op* prog,
                       int32_t nops, uint64_t row_begin, uint64_t row_end, uint8_t* bits) {
-  orc_value st[64];
   for (uint64_t row = row_begin; row < row_end; ++row) {
     const uint64_t i = row - row_begin;
     if (!((bits[i >> 3] >> (i & 7)) & 1)) continue;
-    int sp = 0;
-    for (int32_t k = 0; k < nops; ++k) {
-      const orc_expr_op* o = &prog[k];
-      switch (o->code) {
-        case 0:
-          if (o->arg < 0 || o->arg >= ncols) return -1;
-          st[sp++] = v_column(&cols[o->arg], o->imm, row);
-          break;
-        case 1: st[sp++] = v_int(o->imm); break;
-        case 2: { orc_value v = v_null(); v.tag = V_FLOAT; memcpy(&v.f, &o->imm, 8); st[sp++] = v; break; }
-        case 3: { orc_value v = v_null(); v.tag = V_DATE; v.d = (int32_t)o->imm; st[sp++] = v; break; }
-        case 4: st[sp++] = v_null(); break;
-        case 18: st[sp - 1] = v_int(!v_truthy(&st[sp - 1])); break;           /* Not :1741-1744 */
-        case 19: {                                                             /* Neg :1733-1740 */
-          orc_value v = st[sp - 1];
-          if (v.tag == V_INT) st[sp - 1] = v_int((int64_t)(0 - (uint64_t)v.i));
-          else if (v.tag == V_FLOAT) st[sp - 1] = v_float(-v.f);
-          else st[sp - 1] = v_null();
-          break;
-        }
-        case 20: case 21: {                                                    /* :1773-1779 */
-          const int in = v_cmp_le(&st[sp - 2], &st[sp - 3]) && v_cmp_le(&st[sp - 3], &st[sp - 1]);
-          sp -= 2;
-          st[sp - 1] = v_int(o->code == 20 ? in : !in);
-          break;
-        }
-        case 22: case 23: {                                                    /* :1780-1790 */
-          int found = 0;
-          for (int32_t j = 0; j < o->arg && !found; ++j)
-            found = v_cmp_eq(&st[sp - 1 - o->arg], &st[sp - o->arg + j]);
-          sp -= o->arg;
-          st[sp - 1] = v_int(o->code == 22 ? found : !found);
-          break;
-        }
-        case 24: {                                                             /* Case :1791-1800 */
-          const orc_value r = v_truthy(&st[sp - 3]) ? st[sp - 2] : st[sp - 1];
-          sp -= 2;
-          st[sp - 1] = r;
-          break;
-        }
-        default:
-          if (sp < 2 || o->code < 5 || o->code > 17) return -1;
-          st[sp - 2] = v_binop(o->code, &st[sp - 2], &st[sp - 1]);
-          --sp;
-      }
-      if (sp < 1 || sp > 60) return -1;
-    }
-    if (sp != 1) return -1;
-    if (!v_truthy(&st[0])) bits[i >> 3] &= (uint8_t)~(1u << (i & 7));
+    const int t = eval_prog(cols, ncols, ncols, row, row, prog, nops);
+    if (t < 0) return -1;
+    if (!t) bits[i >> 3] &= (uint8_t)~(1u << (i & 7));
   }
   return 0;
+}
+
+/* The non-equi ON conjuncts of a hash join applied per match
+ * (query_interpreter/join.rs:1146-1195 with eval_non_equi_match :1233-1300): the
+ * pairs (probe row, build row) of an equi join in probe order, a probe row's
+ * matches consecutive (build row no_row: a LEFT row without a match), are kept
+ * when the program, over columns cols[0..nleft) of the probe table at the probe
+ * row and cols[nleft..ncols) of the build table at the build row, is truthy.  A
+ * LEFT join (left != 0) emits (probe row, no_row) for a probe row whose matches
+ * all fail.  Returns the number of output pairs, or -1 for a malformed program. */
+int64_t orc_join_residual(const orc_column* cols, int32_t ncols, int32_t nleft,
+                          const orc_expr_op* prog, int32_t nops, int32_t left,
+                          const uint64_t* probe, const uint32_t* build, uint64_t npairs,
+                          uint32_t no_row, uint64_t* out_probe, uint32_t* out_build) {
+  uint64_t n = 0;
+  for (uint64_t i = 0; i < npairs;) {
+    uint64_t j = i;
+    while (j < npairs && probe[j] == probe[i]) ++j;  /* the probe row's matches [i, j) */
+    int passed = 0;
+    for (uint64_t k = i; k < j; ++k) {
+      if (build[k] == no_row) {  /* no match: LEFT's row as it is */
+        out_probe[n] = probe[k];
+        out_build[n++] = no_row;
+        passed = 1;
+        continue;
+      }
+      const int t = eval_prog(cols, ncols, nleft, probe[k], build[k], prog, nops);
+      if (t < 0) return -1;
+      if (t) {
+        out_probe[n] = probe[k];
+        out_build[n++] = build[k];
+        passed = 1;
+      }
+    }
+    if (left && !passed) {
+      out_probe[n] = probe[i];
+      out_build[n++] = no_row;
+    }
+    i = j;
+  }
+  return (int64_t)n;
 }

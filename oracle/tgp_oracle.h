@@ -133,6 +133,10 @@ void orc_filter_f64(const uint64_t* cells, uint64_t n, int32_t op, int32_t eps, 
 typedef struct { int32_t code, arg; int64_t imm; } orc_expr_op;
 int32_t orc_scan_expr(const orc_column* cols, int32_t ncols, const orc_expr_op* prog,
                       int32_t nops, uint64_t row_begin, uint64_t row_end, uint8_t* bits);
+int64_t orc_join_residual(const orc_column* cols, int32_t ncols, int32_t nleft,
+                          const orc_expr_op* prog, int32_t nops, int32_t left,
+                          const uint64_t* probe, const uint32_t* build, uint64_t npairs,
+                          uint32_t no_row, uint64_t* out_probe, uint32_t* out_build);
 
 #ifdef __cplusplus
 }

@@ -221,6 +221,9 @@ SIGNATURES = {
                                    P, P]),
     "tgp_scan_expr": (I32_, [P, C.POINTER(Column), I32_, C.POINTER(ExprOp), I32_, U64_, U64_, P,
                              P]),
+    "tgp_join_residual_sync": (I32_, [P, C.POINTER(Column), I32_, C.POINTER(Column), I32_,
+                                      C.POINTER(ExprOp), I32_, I32_, U64_, U64_, P, P, U64_, P,
+                                      P, C.POINTER(U64_), P]),
     "tgp_hash_group_agg_sync": (I32_, [P, C.POINTER(HashGroupQuery), C.POINTER(HashGroupResult),
                                        C.POINTER(U64_), P]),
     "tgp_synth_uniform": (I32_, [P, C.POINTER(Column), U64_, U64_, I64_, I64_, U64_, U64_, P]),

@@ -293,6 +293,40 @@ class Context:
                 bm.free()
             raise
 
+    def join_residual(self, left_cols, right_cols, prog, probe_idx, build_row, left=False,
+                      row_begin=0, row_end=None, stream=None):
+        """Equi-join pairs (host arrays: probe rows, build rows / JOIN_NO_MATCH) filtered
+        by a residual program over left_cols (read at the probe row) then right_cols (at
+        the build row), LEFT keeping a probe row whose matches all fail
+        (tgp_join_residual_sync).  Returns the kept (probe rows u64, build rows u32)."""
+        probe_idx = np.ascontiguousarray(probe_idx, np.uint64)
+        build_row = np.ascontiguousarray(build_row, np.uint32)
+        n = len(probe_idx)
+        if row_end is None:
+            row_end = left_cols[0].n if left_cols else (int(probe_idx.max()) + 1 if n else 0)
+        bufs = [DeviceBuffer(self, max(8, 8 * n)), DeviceBuffer(self, max(8, 4 * n)),
+                DeviceBuffer(self, max(8, 8 * n)), DeviceBuffer(self, max(8, 4 * n))]
+        try:
+            if n:
+                bufs[0].upload(probe_idx)
+                bufs[1].upload(build_row)
+            larr = (A.Column * max(1, len(left_cols)))(*[c.desc() for c in left_cols])
+            rarr = (A.Column * max(1, len(right_cols)))(*[c.desc() for c in right_cols])
+            ops = (A.ExprOp * len(prog))(*[A.ExprOp(c, a, im) for c, a, im in prog])
+            m = C.c_uint64()
+            A.check(A.lib().tgp_join_residual_sync(
+                self.handle, larr, len(left_cols), rarr, len(right_cols), ops, len(prog),
+                A.JOIN_LEFT if left else A.JOIN_INNER,
+                row_begin, row_end, bufs[0].ptr, bufs[1].ptr, n, bufs[2].ptr, bufs[3].ptr,
+                C.byref(m), stream), "tgp_join_residual_sync")
+            k = m.value
+            if k == 0:
+                return np.zeros(0, np.uint64), np.zeros(0, np.uint32)
+            return bufs[2].download(np.uint64, k), bufs[3].download(np.uint32, k)
+        finally:
+            for b in bufs:
+                b.free()
+
     def compare_columns(self, a, op, b, is_signed, row_begin=0, row_end=None, bitmap=None,
                         stream=None):
         """Rows of [row_begin, row_end) where a op b (tgp_scan_compare_columns): narrows

@@ -292,10 +292,15 @@ _HAVING_OPS = {"=": lambda a, b: a == b, "!=": lambda a, b: a != b, "<>": lambda
 class Join:
     """One JOIN clause: the joined table, the ON equalities as (left column, right
     column) pairs (what extract_join_keys yields, exec/join.rs), and the type:
-    INNER, LEFT, RIGHT, FULL or CROSS (executor.rs:1766-1788)."""
+    INNER, LEFT, RIGHT, FULL or CROSS (executor.rs:1766-1788).  `residual`: the ON
+    clause's other conjuncts as one expression (turbogp_amd.expr, AND of them), applied
+    to every match as the interpreter's hash_join does (query_interpreter/join.rs:
+    1146-1195, eval_non_equi_match :1233-1300; INNER and LEFT only, its JoinType2).
+    split_on() divides an ON expression into the two as the interpreter does."""
     table: str
     on: List[Tuple[str, str]] = field(default_factory=list)
     join_type: str = "INNER"
+    residual: object = None
 
 
 def execute_with_join(q: SelectQuery, from_table: str, joins: Sequence[Join],
@@ -318,9 +323,91 @@ def execute_with_join(q: SelectQuery, from_table: str, joins: Sequence[Join],
                 running.free()
             running = joined
         return execute_dispatched(q, running)
+    except Unsupported:
+        return None
     finally:
         if running is not base:
             running.free()
+
+
+_KIND = {"int": X.INT, "float": X.FLOAT, "date": X.DATE}
+
+
+def split_on(on, left: Table, right: Table, right_name: str):
+    """An ON expression (turbogp_amd.expr) as (key pairs, residual): the interpreter's
+    collect_keys (join.rs:1313-1334: every `a = b` reachable through ANDs with one
+    column in each table, as (left column, right column)) and its non-equi conjuncts
+    (:1105-1109: the AND-conjuncts that are not `column = column`, ANDed back, or
+    None).  As there, `x = y` of two columns of one table is neither."""
+    def conjuncts(e):
+        if isinstance(e, X.BinOp) and e.op.upper() == "AND":
+            return conjuncts(e.left) + conjuncts(e.right)
+        return [e]
+
+    def col_in(e, t, tname):
+        if not isinstance(e, X.Col):
+            return None
+        parts = e.name.split(".")
+        if t is right and len(parts) == 2:
+            return parts[1] if parts[0] == tname and parts[1] in t.columns else None
+        try:
+            return t.resolve(e.name)
+        except KeyError:
+            return None
+
+    keys, rest = [], []
+    for c in conjuncts(on):
+        if isinstance(c, X.BinOp) and c.op == "=" and isinstance(c.left, X.Col) \
+                and isinstance(c.right, X.Col):
+            lk, rk = col_in(c.left, left, None), col_in(c.right, right, right_name)
+            if lk is not None and rk is not None:
+                keys.append((lk, rk))
+            else:
+                rk, lk = col_in(c.left, right, right_name), col_in(c.right, left, None)
+                if lk is not None and rk is not None:
+                    keys.append((lk, rk))
+            continue
+        rest.append(c)
+    residual = None
+    for c in rest:
+        residual = c if residual is None else X.BinOp("AND", residual, c)
+    return keys, residual
+
+
+def _residual_pairs(left: Table, right: Table, left_name: str, j: Join, p, b):
+    """The pairs (p, b) of left JOIN right kept by j.residual.  Column names resolve as
+    the interpreter's combined column map (join.rs:1127-1140): `table.column` of the
+    joined table is a right column, other names the left table's, an unqualified
+    name present in both is the right one."""
+    names = X.columns(j.residual)
+    lrefs, rrefs = [], []
+    for nm in names:
+        parts = nm.split(".")
+        side = None
+        if len(parts) == 2 and parts[0] == j.table and parts[1] in right.columns:
+            side = ("R", parts[1])
+        elif len(parts) == 1 and nm in right.columns:
+            side = ("R", nm)
+        else:
+            try:
+                side = ("L", left.resolve(nm if len(parts) == 1 or parts[0] != left_name
+                                          else parts[1]))
+            except KeyError:
+                raise Unsupported(f"residual column {nm!r}")
+        (rrefs if side[0] == "R" else lrefs).append((nm, side[1]))
+    lidx = {nm: i for i, (nm, _) in enumerate(lrefs)}
+    ridx = {nm: len(lrefs) + i for i, (nm, _) in enumerate(rrefs)}
+
+    def ref(nm):
+        if nm in lidx:
+            return lidx[nm], _KIND[left.col_type(dict(lrefs)[nm])]
+        return ridx[nm], _KIND[right.col_type(dict(rrefs)[nm])]
+    prog = X.compile_expr(j.residual, ref)
+    if len(lrefs) + len(rrefs) > A.MAX_COLS or len(prog) > A.MAX_EXPR_OPS:
+        raise Unsupported("residual too large for the device program")
+    return left.ctx.join_residual([left.columns[c] for _, c in lrefs],
+                                  [right.columns[c] for _, c in rrefs], prog, p, b,
+                                  left=j.join_type.upper() == "LEFT")
 
 
 def _join_tables(left: Table, right: Table, left_name: str, j: Join) -> Table:
@@ -334,6 +421,8 @@ def _join_tables(left: Table, right: Table, left_name: str, j: Join) -> Table:
     else:
         if not j.on:
             raise ValueError("JOIN needs at least one ON equality (hash_join requires a key)")
+        if j.residual is not None and jt not in ("INNER", "LEFT"):
+            raise Unsupported(f"{jt} JOIN with non-equi conditions")
         lk = [left.col(a) for a, _ in j.on]
         rk = [right.col(b) for _, b in j.on]
         t = HashTable(ctx, rk if len(rk) > 1 else rk[0])
@@ -343,6 +432,8 @@ def _join_tables(left: Table, right: Table, left_name: str, j: Join) -> Table:
                                 A.JOIN_LEFT if outer_left else A.JOIN_INNER)
         finally:
             t.free()
+        if j.residual is not None:
+            p, b = _residual_pairs(left, right, left_name, j, p, b)
         hit = b != A.JOIN_NO_MATCH
         li = p.astype(u64)
         ri = np.where(hit, b.astype(u64), u64(A.GATHER_NO_ROW))
@@ -353,7 +444,7 @@ def _join_tables(left: Table, right: Table, left_name: str, j: Join) -> Table:
             li = np.concatenate([li, np.full(len(extra), A.GATHER_NO_ROW, u64)])
             ri = np.concatenate([ri, extra])
     n = len(li)
-    cols = {}
+    cols, types = {}, {}
     bufs = [DeviceBuffer(ctx, max(8, 8 * n)) for _ in range(2)]
     try:
         if n:
@@ -363,11 +454,13 @@ def _join_tables(left: Table, right: Table, left_name: str, j: Join) -> Table:
             for name in left.names:
                 qn = name if "." in name else f"{left_name}.{name}"
                 cols[qn] = ctx.gather_column(left.columns[name], bufs[0], n)
+                types[qn] = left.col_type(name)
             for name in right.names:
                 qn = f"{j.table}.{name}"
                 if qn in cols:
                     cols[qn].free()
                 cols[qn] = ctx.gather_column(right.columns[name], bufs[1], n)
+                types[qn] = right.col_type(name)
         except Exception:
             for c in cols.values():
                 c.free()
@@ -375,7 +468,7 @@ def _join_tables(left: Table, right: Table, left_name: str, j: Join) -> Table:
     finally:
         for bf in bufs:
             bf.free()
-    return Table(ctx, cols)
+    return Table(ctx, cols, types=types)
 
 
 def _exact_column(name, vals):
