@@ -289,9 +289,11 @@ __global__ void __launch_bounds__(kBlock, 2) k_agg_table(DevPlan P) {
 // partitions of 2^part_shift consecutive groups and aggregates each partition in an
 // LDS table:
 //   (1) k_part_hist_tiles: through a plan that stages only the predicate and group
-//       columns, workgroup w of G counts its selected rows per partition
-//       p = gid >> part_shift into mat[p * G + w];
-//   (2) an exclusive scan of mat (partition-major) gives every (partition,
+//       columns, workgroup h of k G (k per scatter workgroup, for occupancy: its LDS
+//       is small) counts its selected rows per partition p = gid >> part_shift into
+//       mat[p * k G + (h mod G) k + h / G]: the k counts of the tiles scatter
+//       workgroup w will take are adjacent;
+//   (2) an exclusive scan of mat (partition-major) gives every (partition, scatter
 //       workgroup) pair its range of records;
 //   (3) k_part_scatter_tiles: the same tiles on the same G workgroups (so the
 //       counts match) through the full plan; each selected row takes a rank among
@@ -308,7 +310,7 @@ constexpr uint64_t kPartChunk = 1ull << 15;  // records per aggregation workgrou
 constexpr int kPartAggBlock = 1024;
 
 __global__ void __launch_bounds__(kBlock) k_part_hist_tiles(DevPlan P, uint32_t shift,
-                                                            uint32_t nparts,
+                                                            uint32_t nparts, uint32_t k,
                                                             unsigned long long* __restrict__ mat) {
   extern __shared__ __align__(16) uint8_t lds[];
   uint32_t* h = reinterpret_cast<uint32_t*>(lds + 2 * P.stage_bytes);
@@ -324,19 +326,23 @@ __global__ void __launch_bounds__(kBlock) k_part_hist_tiles(DevPlan P, uint32_t 
       if (((sel >> r) & 1) && gid[r] >= 0) atomicAdd(&h[(uint64_t)gid[r] >> shift], 1u);
   });
   __syncthreads();
+  const uint32_t G = gridDim.x / k;
+  const uint64_t col = (uint64_t)(blockIdx.x % G) * k + blockIdx.x / G;
   for (uint32_t p = threadIdx.x; p < nparts; p += kBlock)
-    mat[(uint64_t)p * gridDim.x + blockIdx.x] = h[p];
+    mat[(uint64_t)p * gridDim.x + col] = h[p];
 }
 
-// mat: exclusive offsets after the scan.  cap: the length of each record array.
+// mat: exclusive offsets after the scan (k columns per workgroup).  cap: the length
+// of each record array.
 __global__ void __launch_bounds__(kBlock) k_part_scatter_tiles(
-    DevPlan P, uint32_t shift, uint32_t nparts, const unsigned long long* __restrict__ mat,
-    uint64_t cap, uint16_t* __restrict__ rgrp, uint64_t* __restrict__ rval) {
+    DevPlan P, uint32_t shift, uint32_t nparts, uint32_t k,
+    const unsigned long long* __restrict__ mat, uint64_t cap, uint16_t* __restrict__ rgrp,
+    uint64_t* __restrict__ rval) {
   extern __shared__ __align__(16) uint8_t lds[];
   unsigned long long* cursor = reinterpret_cast<unsigned long long*>(lds + 2 * P.stage_bytes);
   uint32_t* cnt = reinterpret_cast<uint32_t*>(cursor + nparts);
   for (uint32_t p = threadIdx.x; p < nparts; p += kBlock) {
-    cursor[p] = mat[(uint64_t)p * gridDim.x + blockIdx.x];
+    cursor[p] = mat[((uint64_t)p * gridDim.x + blockIdx.x) * k];
     cnt[p] = 0;
   }
   // made visible by the barrier at the top of the first tile
@@ -390,8 +396,8 @@ __global__ void __launch_bounds__(kBlock) k_part_scatter_tiles(
   if (oor && P.out_oor) atomicAdd(P.out_oor, (unsigned long long)oor);
 }
 
-// G: the workgroups of the histogram and the scatter.  LDS: 2^shift x (1 + 2 naggs)
-// words.  Results (zeroed) receive int128 sums and MIN/MAX codes.
+// G: mat's columns (the histogram's workgroups).  LDS: 2^shift x (1 + 2 naggs) words.
+// Results (zeroed) receive int128 sums and MIN/MAX codes.
 __global__ void __launch_bounds__(kPartAggBlock) k_part_agg_recs(
     DevPlan P, uint32_t shift, uint32_t nparts, uint64_t G,
     const unsigned long long* __restrict__ mat, uint64_t cap,
@@ -415,18 +421,37 @@ __global__ void __launch_bounds__(kPartAggBlock) k_part_agg_recs(
     if (end <= lo) continue;
     for (uint32_t i = threadIdx.x; i < gp * W; i += kPartAggBlock) tab[i] = 0;
     __syncthreads();
-    for (uint64_t i = lo + threadIdx.x; i < end; i += kPartAggBlock) {
-      unsigned long long* w = tab + rgrp[i] * W;
-      atomicAdd(w, 1ull);
+    // U records per thread per step: their loads are issued before their atomics
+    constexpr int U = 4;
+    for (uint64_t i0 = lo + threadIdx.x; i0 < end; i0 += (uint64_t)U * kPartAggBlock) {
+      uint32_t g[U];
+      bool ok[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint64_t i = i0 + (uint64_t)u * kPartAggBlock;
+        ok[u] = i < end;
+        g[u] = ok[u] ? rgrp[i] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (ok[u]) atomicAdd(tab + g[u] * W, 1ull);
       for (int k = 0; k < P.naggs; ++k) {
-        const uint64_t v = rval[(uint64_t)k * cap + i];
+        uint64_t v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          v[u] = ok[u] ? rval[(uint64_t)k * cap + i0 + (uint64_t)u * kPartAggBlock] : 0;
         const int32_t mk = (P.minmax_kinds >> (3 * k)) & 7;
-        if (mk) {
-          atomicMax(w + 1 + 2 * k, (unsigned long long)mm_encode(mk, v));
-        } else {
-          atomicAdd(w + 1 + 2 * k, (unsigned long long)(uint32_t)v);
-          const int64_t h = (int64_t)v >> 32;
-          if (h) atomicAdd(w + 2 + 2 * k, (unsigned long long)h);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (!ok[u]) continue;
+          unsigned long long* w = tab + g[u] * W + 1 + 2 * k;
+          if (mk) {
+            atomicMax(w, (unsigned long long)mm_encode(mk, v[u]));
+          } else {
+            atomicAdd(w, (unsigned long long)(uint32_t)v[u]);
+            const int64_t h = (int64_t)v[u] >> 32;
+            if (h) atomicAdd(w + 1, (unsigned long long)h);
+          }
         }
       }
     }
@@ -627,7 +652,10 @@ static int32_t run_part(tgp_ctx* ctx, const DevPlan& P, const DevPlan& H, const 
   const uint32_t np = info.nparts, sh = info.part_shift;
   const uint64_t rows = P.row_end - P.row_begin;
   const uint64_t G = std::min<uint64_t>(P.ntiles, (uint64_t)persistent_grid(ctx, 2));
-  const uint64_t mat_n = (uint64_t)np * G;
+  // histogram workgroups per scatter workgroup: up to 8 per CU while mat stays small
+  uint32_t k = 1;
+  while (k < 4 && G * 2 * k <= P.ntiles && (uint64_t)np * G * 2 * k <= (1u << 22)) k *= 2;
+  const uint64_t mat_n = (uint64_t)np * G * k;
   const uint64_t nb = (mat_n + 4095) / 4096;
   // scratch: mat[mat_n + 1] | scan block sums[nb + 1] | values[naggs][rows] | groups[rows]
   const uint64_t words = mat_n + 1 + nb + 1 + (uint64_t)P.naggs * rows;
@@ -640,17 +668,17 @@ static int32_t run_part(tgp_ctx* ctx, const DevPlan& P, const DevPlan& H, const 
   const uint32_t lh = part_hist_lds(H, np), ls = part_scatter_lds(P, np);
   TGP_TRY(allow_lds(k_part_hist_tiles, lh));
   TGP_TRY(allow_lds(k_part_scatter_tiles, ls));
-  k_part_hist_tiles<<<dim3((unsigned)G), dim3(kBlock), lh, s>>>(H, sh, np, mat);
+  k_part_hist_tiles<<<dim3((unsigned)(G * k)), dim3(kBlock), lh, s>>>(H, sh, np, k, mat);
   TGP_HIP(hipGetLastError());
   TGP_TRY(scan_u64(s, reinterpret_cast<uint64_t*>(mat), mat_n, bsum));
-  k_part_scatter_tiles<<<dim3((unsigned)G), dim3(kBlock), ls, s>>>(P, sh, np, mat, rows, rgrp,
-                                                                   rval);
+  k_part_scatter_tiles<<<dim3((unsigned)G), dim3(kBlock), ls, s>>>(P, sh, np, k, mat, rows,
+                                                                   rgrp, rval);
   TGP_HIP(hipGetLastError());
   const uint32_t la = (uint32_t)((1ull << sh) * (1 + 2 * (uint64_t)P.naggs) * 8);
   TGP_TRY(allow_lds(k_part_agg_recs, la));
   const uint64_t ga = (rows + kPartChunk - 1) / kPartChunk;
-  k_part_agg_recs<<<dim3((unsigned)ga), dim3(kPartAggBlock), la, s>>>(P, sh, np, G, mat, rows,
-                                                                      rgrp, rval);
+  k_part_agg_recs<<<dim3((unsigned)ga), dim3(kPartAggBlock), la, s>>>(P, sh, np, G * k, mat,
+                                                                      rows, rgrp, rval);
   TGP_HIP(hipGetLastError());
   return TGP_OK;
 }
