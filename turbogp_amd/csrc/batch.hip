@@ -685,6 +685,23 @@ struct tgp_batch {
   uint32_t* d_gminmax = nullptr;
   uint64_t gplans_cap = 0, gitems_cap = 0, gminmax_cap = 0, gminmax_words = 0;
   uint32_t gstage_max = 0, gtable_max = 0;
+  // generic queries whose groups exceed an LDS table: kAggPart's passes over all of
+  // them (histogram plans, queries, items, chunks; counters | counts | scan sums;
+  // records)
+  std::vector<DevPlan> hplans;
+  std::vector<PartQuery> pqs;
+  std::vector<PartItem> pitems;
+  std::vector<PartChunk> pchunks;
+  DevPlan* d_hplans = nullptr;
+  PartQuery* d_pqs = nullptr;
+  PartItem* d_pitems = nullptr;
+  PartChunk* d_pchunks = nullptr;
+  unsigned long long* d_pmat = nullptr;
+  uint16_t* d_pgrp = nullptr;
+  uint64_t* d_pval = nullptr;
+  uint64_t hplans_cap = 0, pqs_cap = 0, pitems_cap = 0, pchunks_cap = 0, pmat_cap = 0,
+           pgrp_cap = 0, pval_cap = 0, pmat_n = 0;
+  uint32_t phstage_max = 0, pstage_max = 0, pnparts_max = 0, ptable_max = 0;
   uint32_t* d_punits = nullptr;
   uint64_t pgroups_cap = 0, plinks_cap = 0, partials_cap = 0, partial_words = 0, punits_cap = 0;
 };
@@ -1050,6 +1067,9 @@ static int32_t build_granules(tgp_batch* b, hipStream_t s) {
 // them, the others add straight into the result).
 constexpr uint32_t kGenericTiles = 16;
 constexpr uint32_t kGenericLds = 160 * 1024 - (uint32_t)sizeof(DevPlan) - 1024;
+// Generic queries of at least this many rows whose groups exceed an LDS table take
+// the partitioned passes (smaller ones add into the result as items).
+constexpr uint64_t kBatchPartRows = 1ull << 16;
 
 static int32_t plan_generic(tgp_batch* b, const tgp_agg_query* qs, hipStream_t s) {
   b->gplans.clear();
@@ -1057,6 +1077,13 @@ static int32_t plan_generic(tgp_batch* b, const tgp_agg_query* qs, hipStream_t s
   b->gminmax.clear();
   b->gstage_max = b->gtable_max = 0;
   b->gminmax_words = 0;
+  b->hplans.clear();
+  b->pqs.clear();
+  b->pitems.clear();
+  b->pchunks.clear();
+  b->phstage_max = b->pstage_max = b->pnparts_max = b->ptable_max = 0;
+  std::vector<PlanInfo> infos;
+  std::vector<int32_t> orig;
   for (int32_t i : b->fallback) {
     DevPlan P;
     PlanInfo info;
@@ -1066,7 +1093,10 @@ static int32_t plan_generic(tgp_batch* b, const tgp_agg_query* qs, hipStream_t s
     P.out_oor = reinterpret_cast<unsigned long long*>(b->d_results + b->oor_off[i]);
     b->gstage_max = std::max(b->gstage_max, P.stage_bytes);
     b->gplans.push_back(P);
+    infos.push_back(info);
+    orig.push_back(i);
   }
+  uint64_t mat_n = 0, grp_n = 0, val_n = 0;
   for (uint32_t q = 0; q < b->gplans.size(); ++q) {
     DevPlan& P = b->gplans[q];
     const uint64_t tb = P.ngroups * (1 + 2 * (uint64_t)P.naggs) * 8;
@@ -1075,6 +1105,41 @@ static int32_t plan_generic(tgp_batch* b, const tgp_agg_query* qs, hipStream_t s
       --P.lds_rshift;
     if (lds) b->gtable_max = std::max<uint32_t>(b->gtable_max, (uint32_t)(tb << P.lds_rshift));
     const bool split = !lds && P.row_end - P.row_begin < (1ull << 32);
+    const uint64_t rows = P.row_end - P.row_begin;
+    if (split && infos[q].nparts && rows >= kBatchPartRows && P.ntiles) {
+      DevPlan H;
+      TGP_TRY(plan_part_hist(&qs[orig[q]], &H));
+      const uint32_t np = infos[q].nparts;
+      const uint32_t hst = std::max(b->phstage_max, H.stage_bytes);
+      const uint32_t st = std::max(b->pstage_max, P.stage_bytes);
+      const uint32_t npm = std::max(b->pnparts_max, np);
+      if (2ull * hst + 4ull * npm <= kGenericLds && 2ull * st + 12ull * npm <= kGenericLds) {
+        b->phstage_max = hst;
+        b->pstage_max = st;
+        b->pnparts_max = npm;
+        const uint32_t nit = (uint32_t)((P.ntiles + kGenericTiles - 1) / kGenericTiles);
+        const uint32_t pq = (uint32_t)b->pqs.size();
+        b->pqs.push_back(PartQuery{q, (uint32_t)b->hplans.size(), infos[q].part_shift, np, nit,
+                                   (uint32_t)P.naggs, mat_n, grp_n, val_n});
+        b->hplans.push_back(H);
+        mat_n += (uint64_t)np * nit;
+        grp_n += rows;
+        val_n += rows * (uint64_t)P.naggs;
+        for (uint32_t j = 0; j < nit; ++j)
+          b->pitems.push_back(PartItem{pq, j, j * kGenericTiles,
+                                       (uint32_t)std::min<uint64_t>((uint64_t)(j + 1) * kGenericTiles,
+                                                                    P.ntiles)});
+        for (uint64_t c = 0; c * kPartChunk < rows; ++c)
+          b->pchunks.push_back(PartChunk{pq, (uint32_t)c});
+        b->ptable_max = std::max<uint32_t>(
+            b->ptable_max, (uint32_t)((1ull << infos[q].part_shift) * (1 + 2 * P.naggs) * 8));
+        if (P.minmax_kinds && P.naggs) {  // the partitioned sums are int128 words already
+          b->gminmax.push_back(q);
+          b->gminmax_words = std::max(b->gminmax_words, P.ngroups * (uint64_t)P.naggs);
+        }
+        continue;
+      }
+    }
     if ((P.minmax_kinds || split) && P.ngroups && P.naggs && P.ntiles) {
       b->gminmax.push_back(q | (split ? 1u << 31 : 0u));
       b->gminmax_words = std::max(b->gminmax_words, P.ngroups * (uint64_t)P.naggs);
@@ -1085,6 +1150,26 @@ static int32_t plan_generic(tgp_batch* b, const tgp_agg_query* qs, hipStream_t s
                                       lds ? 1u : split ? 0u : 2u});
   }
   b->fallback.clear();  // every query runs in the batch launch
+  b->pmat_n = mat_n;
+  if (!b->pqs.empty()) {
+    // d_pmat: 3 counters, a pad word, mat_n counts + the total, the scan's block sums
+    const uint64_t words = 4 + mat_n + 1 + (mat_n + 4095) / 4096 + 1;
+    TGP_TRY(ensure_cap(b, &b->d_pmat, &b->pmat_cap, words));
+    TGP_TRY(ensure_cap(b, &b->d_pgrp, &b->pgrp_cap, std::max<uint64_t>(grp_n, 1)));
+    TGP_TRY(ensure_cap(b, &b->d_pval, &b->pval_cap, std::max<uint64_t>(val_n, 1)));
+    TGP_TRY(ensure_cap(b, &b->d_hplans, &b->hplans_cap, (uint64_t)b->hplans.size()));
+    TGP_TRY(ensure_cap(b, &b->d_pqs, &b->pqs_cap, (uint64_t)b->pqs.size()));
+    TGP_TRY(ensure_cap(b, &b->d_pitems, &b->pitems_cap, (uint64_t)b->pitems.size()));
+    TGP_TRY(ensure_cap(b, &b->d_pchunks, &b->pchunks_cap, (uint64_t)b->pchunks.size()));
+    TGP_HIP(hipMemcpyAsync(b->d_hplans, b->hplans.data(), b->hplans.size() * sizeof(DevPlan),
+                           hipMemcpyHostToDevice, s));
+    TGP_HIP(hipMemcpyAsync(b->d_pqs, b->pqs.data(), b->pqs.size() * sizeof(PartQuery),
+                           hipMemcpyHostToDevice, s));
+    TGP_HIP(hipMemcpyAsync(b->d_pitems, b->pitems.data(), b->pitems.size() * sizeof(PartItem),
+                           hipMemcpyHostToDevice, s));
+    TGP_HIP(hipMemcpyAsync(b->d_pchunks, b->pchunks.data(),
+                           b->pchunks.size() * sizeof(PartChunk), hipMemcpyHostToDevice, s));
+  }
   if (b->gplans.empty()) return TGP_OK;
   TGP_TRY(ensure_cap(b, &b->d_gplans, &b->gplans_cap, (uint64_t)b->gplans.size()));
   TGP_HIP(hipMemcpyAsync(b->d_gplans, b->gplans.data(), b->gplans.size() * sizeof(DevPlan),
@@ -1200,6 +1285,15 @@ static int32_t build_items(tgp_batch* b, hipStream_t s) {
 static int32_t batch_launch(tgp_batch* b, hipStream_t s) {
   // results and the counters behind them start at zero
   TGP_TRY(fill_u64(s, b->d_results, b->result_words + 2 * kQueues + 1, 0));
+  if (!b->pqs.empty()) {  // before the generic launch, whose finalisation covers them
+    TGP_TRY(fill_u64(s, reinterpret_cast<uint64_t*>(b->d_pmat), 3, 0));
+    unsigned long long* mat = b->d_pmat + 4;
+    TGP_TRY(launch_batch_part(b->ctx, b->d_gplans, b->d_hplans, b->d_pqs, b->d_pitems,
+                              b->pitems.size(), b->d_pchunks, b->pchunks.size(), b->phstage_max,
+                              b->pstage_max, b->pnparts_max, b->ptable_max, b->d_pmat, mat,
+                              b->pmat_n, reinterpret_cast<uint64_t*>(mat + b->pmat_n + 1),
+                              b->d_pgrp, b->d_pval, s));
+  }
   if (!b->gplans.empty()) {
     auto* counter = reinterpret_cast<unsigned long long*>(b->d_results + b->result_words +
                                                           2 * kQueues);
@@ -1242,6 +1336,9 @@ int32_t tgp_batch_destroy(tgp_batch* b) {
   if (b->d_gitems) hbm_free(b->ctx, b->d_gitems);
   if (b->d_gminmax) hbm_free(b->ctx, b->d_gminmax);
   if (b->d_partials) hbm_free(b->ctx, b->d_partials);
+  for (void* p : {(void*)b->d_hplans, (void*)b->d_pqs, (void*)b->d_pitems, (void*)b->d_pchunks,
+                  (void*)b->d_pmat, (void*)b->d_pgrp, (void*)b->d_pval})
+    if (p) hbm_free(b->ctx, p);
   for (void* p : b->retired) hbm_free(b->ctx, p);
   delete b;
   return TGP_OK;

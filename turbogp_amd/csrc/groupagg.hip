@@ -306,7 +306,6 @@ __global__ void __launch_bounds__(kBlock, 2) k_agg_table(DevPlan P) {
 //       (count, split sums or MIN/MAX codes) and adds its nonzero groups to the
 //       result with carry-exact device atomics.
 // The order of records inside a range varies from run to run; the totals do not.
-constexpr uint64_t kPartChunk = 1ull << 15;  // records per aggregation workgroup
 constexpr int kPartAggBlock = 1024;
 
 __global__ void __launch_bounds__(kBlock) k_part_hist_tiles(DevPlan P, uint32_t shift,
@@ -396,6 +395,73 @@ __global__ void __launch_bounds__(kBlock) k_part_scatter_tiles(
   if (oor && P.out_oor) atomicAdd(P.out_oor, (unsigned long long)oor);
 }
 
+// Aggregates one partition's run of n records (local groups grp[0..n), value k of
+// record i = val(i, k)) in the LDS table tab of gp groups and adds its nonzero groups
+// to the result at groups g0.. with carry-exact device atomics.  Called by the whole
+// workgroup (kPartAggBlock threads); ends with the table free again.
+template <typename Val>
+__device__ __forceinline__ void agg_run(int32_t naggs, uint32_t minmax_kinds,
+                                        unsigned long long* out_count,
+                                        unsigned long long* out_sums, uint64_t g0, uint32_t gp,
+                                        unsigned long long* tab, const uint16_t* grp, uint64_t n,
+                                        Val&& val) {
+  const uint32_t W = 1 + 2 * (uint32_t)naggs;
+  for (uint32_t i = threadIdx.x; i < gp * W; i += kPartAggBlock) tab[i] = 0;
+  __syncthreads();
+  // U records per thread per step: their loads are issued before their atomics
+  constexpr int U = 4;
+  for (uint64_t i0 = threadIdx.x; i0 < n; i0 += (uint64_t)U * kPartAggBlock) {
+    uint32_t g[U];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t i = i0 + (uint64_t)u * kPartAggBlock;
+      ok[u] = i < n;
+      g[u] = ok[u] ? grp[i] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (ok[u]) atomicAdd(tab + g[u] * W, 1ull);
+    for (int k = 0; k < naggs; ++k) {
+      uint64_t v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = ok[u] ? val(i0 + (uint64_t)u * kPartAggBlock, k) : 0;
+      const int32_t mk = (minmax_kinds >> (3 * k)) & 7;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (!ok[u]) continue;
+        unsigned long long* w = tab + g[u] * W + 1 + 2 * k;
+        if (mk) {
+          atomicMax(w, (unsigned long long)mm_encode(mk, v[u]));
+        } else {
+          atomicAdd(w, (unsigned long long)(uint32_t)v[u]);
+          const int64_t h = (int64_t)v[u] >> 32;
+          if (h) atomicAdd(w + 1, (unsigned long long)h);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (uint32_t gl = threadIdx.x; gl < gp; gl += kPartAggBlock) {
+    const unsigned long long* w = tab + gl * W;
+    if (!w[0]) continue;
+    const uint64_t g = g0 + gl;
+    atomicAdd(&out_count[g], w[0]);
+    for (int k = 0; k < naggs; ++k) {
+      unsigned long long* o = &out_sums[(g * naggs + k) * 2];
+      if ((minmax_kinds >> (3 * k)) & 7) {
+        atomicMax(o, w[1 + 2 * k]);
+      } else {
+        uint64_t l;
+        int64_t h;
+        split_to_i128(w[1 + 2 * k], (int64_t)w[2 + 2 * k], &l, &h);
+        atomic_add128(o, l, h);
+      }
+    }
+  }
+  __syncthreads();  // every flush read before the table is cleared again
+}
+
 // G: mat's columns (the histogram's workgroups).  LDS: 2^shift x (1 + 2 naggs) words.
 // Results (zeroed) receive int128 sums and MIN/MAX codes.
 __global__ void __launch_bounds__(kPartAggBlock) k_part_agg_recs(
@@ -407,7 +473,7 @@ __global__ void __launch_bounds__(kPartAggBlock) k_part_agg_recs(
   uint64_t lo = (uint64_t)blockIdx.x * kPartChunk;
   if (lo >= total) return;
   const uint64_t hi = lo + kPartChunk < total ? lo + kPartChunk : total;
-  const uint32_t W = 1 + 2 * (uint32_t)P.naggs, gp = 1u << shift;
+  const uint32_t gp = 1u << shift;
   // the partition holding record lo: the last p with mat[p * G] <= lo
   uint32_t pa = 0, pb = nparts - 1;
   while (pa < pb) {
@@ -419,64 +485,244 @@ __global__ void __launch_bounds__(kPartAggBlock) k_part_agg_recs(
     const uint64_t pe = mat[(uint64_t)(p + 1) * G];
     const uint64_t end = pe < hi ? pe : hi;
     if (end <= lo) continue;
-    for (uint32_t i = threadIdx.x; i < gp * W; i += kPartAggBlock) tab[i] = 0;
-    __syncthreads();
-    // U records per thread per step: their loads are issued before their atomics
-    constexpr int U = 4;
-    for (uint64_t i0 = lo + threadIdx.x; i0 < end; i0 += (uint64_t)U * kPartAggBlock) {
-      uint32_t g[U];
-      bool ok[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const uint64_t i = i0 + (uint64_t)u * kPartAggBlock;
-        ok[u] = i < end;
-        g[u] = ok[u] ? rgrp[i] : 0;
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-        if (ok[u]) atomicAdd(tab + g[u] * W, 1ull);
-      for (int k = 0; k < P.naggs; ++k) {
-        uint64_t v[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-          v[u] = ok[u] ? rval[(uint64_t)k * cap + i0 + (uint64_t)u * kPartAggBlock] : 0;
-        const int32_t mk = (P.minmax_kinds >> (3 * k)) & 7;
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          if (!ok[u]) continue;
-          unsigned long long* w = tab + g[u] * W + 1 + 2 * k;
-          if (mk) {
-            atomicMax(w, (unsigned long long)mm_encode(mk, v[u]));
-          } else {
-            atomicAdd(w, (unsigned long long)(uint32_t)v[u]);
-            const int64_t h = (int64_t)v[u] >> 32;
-            if (h) atomicAdd(w + 1, (unsigned long long)h);
-          }
-        }
-      }
-    }
-    __syncthreads();
-    const uint64_t g0 = (uint64_t)p << shift;
-    for (uint32_t gl = threadIdx.x; gl < gp; gl += kPartAggBlock) {
-      const unsigned long long* w = tab + gl * W;
-      if (!w[0]) continue;
-      const uint64_t g = g0 + gl;
-      atomicAdd(&P.out_count[g], w[0]);
-      for (int k = 0; k < P.naggs; ++k) {
-        unsigned long long* o = &P.out_sums[(g * P.naggs + k) * 2];
-        if ((P.minmax_kinds >> (3 * k)) & 7) {
-          atomicMax(o, w[1 + 2 * k]);
-        } else {
-          uint64_t l;
-          int64_t h;
-          split_to_i128(w[1 + 2 * k], (int64_t)w[2 + 2 * k], &l, &h);
-          atomic_add128(o, l, h);
-        }
-      }
-    }
-    __syncthreads();  // every flush read before the next partition clears the table
+    agg_run(P.naggs, P.minmax_kinds, P.out_count, P.out_sums, (uint64_t)p << shift, gp, tab,
+            rgrp + lo, end - lo, [&](uint64_t i, int k) { return rval[(uint64_t)k * cap + lo + i]; });
     lo = end;
   }
+}
+
+constexpr uint32_t kLdsMax = 160 * 1024;
+
+template <typename K>
+static int32_t allow_lds(K kern, uint32_t bytes) {
+  if (bytes > 65536)
+    TGP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)bytes));
+  return TGP_OK;
+}
+
+// ---- kAggPart over many queries at once (the batch's partitioned queries) -----------------
+// The passes of kAggPart, each one persistent launch over the work of every such
+// query: histogram items (query, tiles [t0, t1), its item j) through the query's
+// histogram plan, one exclusive scan of all queries' counts, scatter items (the same
+// items through the full plans), then aggregation chunks (query, 32K records).  A
+// query's counts sit partition-major (mat[mat_base + p * nitems + j]), so after the
+// scan its records are contiguous, sorted by partition, starting at mat[mat_base].
+
+// Copies plan q into the workgroup's LDS plan (all threads; a barrier must follow).
+__device__ __forceinline__ void load_plan(const DevPlan* __restrict__ plans, uint32_t q,
+                                          DevPlan* sp) {
+  const uint64_t* src = reinterpret_cast<const uint64_t*>(&plans[q]);
+  uint64_t* dst = reinterpret_cast<uint64_t*>(sp);
+  for (uint32_t i = threadIdx.x; i < sizeof(DevPlan) / 8; i += blockDim.x) dst[i] = src[i];
+}
+
+// Takes the next work item (whole workgroup); UINT32_MAX-free: returns the index.
+__device__ __forceinline__ uint32_t next_item(unsigned long long* counter, uint32_t* s_item) {
+  __syncthreads();  // the previous item is done; s_item and the LDS plan are free
+  if (threadIdx.x == 0) *s_item = (uint32_t)atomicAdd(counter, 1ull);
+  __syncthreads();
+  return __builtin_amdgcn_readfirstlane(*s_item);
+}
+
+__global__ void __launch_bounds__(kBlock) k_bpart_hist(const DevPlan* __restrict__ hplans,
+                                                       const PartQuery* __restrict__ pqs,
+                                                       const PartItem* __restrict__ items,
+                                                       uint64_t nitems,
+                                                       unsigned long long* counter,
+                                                       uint32_t stage_max,
+                                                       unsigned long long* __restrict__ mat) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  __shared__ DevPlan sp;
+  __shared__ uint32_t s_item;
+  uint32_t* h = reinterpret_cast<uint32_t*>(lds + 2 * stage_max);
+  for (;;) {
+    const uint32_t it = next_item(counter, &s_item);
+    if (it >= nitems) break;
+    const PartItem I = items[it];
+    const PartQuery Q = pqs[I.pq];
+    load_plan(hplans, Q.hplan, &sp);
+    for (uint32_t p = threadIdx.x; p < Q.nparts; p += kBlock) h[p] = 0;
+    __syncthreads();
+    const DevPlan& P = sp;
+    stage_issue(P, I.t0, lds);
+    for (uint32_t t = I.t0, i = 0; t < I.t1; ++t, ++i) {
+      stage_wait();
+      __syncthreads();  // tile t is in LDS; every wave is done with the other buffer
+      uint8_t* cur = lds + (i & 1) * stage_max;
+      uint8_t* nxt = lds + ((i + 1) & 1) * stage_max;
+      if (t + 1 < I.t1) stage_issue(P, t + 1, nxt);
+      const uint32_t sel = eval_select(P, cur, P.base0 + (uint64_t)t * kTileRows);
+      if (__ballot(sel != 0) == 0) continue;
+      int64_t gid[kRows];
+      eval_groups(P, cur, gid);
+#pragma unroll
+      for (int r = 0; r < kRows; ++r)
+        if (((sel >> r) & 1) && gid[r] >= 0) atomicAdd(&h[(uint64_t)gid[r] >> Q.shift], 1u);
+    }
+    __syncthreads();
+    for (uint32_t p = threadIdx.x; p < Q.nparts; p += kBlock)
+      mat[Q.mat_base + (uint64_t)p * Q.nitems + I.j] = h[p];
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_bpart_scatter(const DevPlan* __restrict__ plans,
+                                                          const PartQuery* __restrict__ pqs,
+                                                          const PartItem* __restrict__ items,
+                                                          uint64_t nitems,
+                                                          unsigned long long* counter,
+                                                          uint32_t stage_max, uint32_t nparts_max,
+                                                          const unsigned long long* __restrict__ mat,
+                                                          uint16_t* __restrict__ grp,
+                                                          uint64_t* __restrict__ val) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  __shared__ DevPlan sp;
+  __shared__ uint32_t s_item;
+  unsigned long long* cursor = reinterpret_cast<unsigned long long*>(lds + 2 * stage_max);
+  uint32_t* cnt = reinterpret_cast<uint32_t*>(cursor + nparts_max);
+  for (;;) {
+    const uint32_t it = next_item(counter, &s_item);
+    if (it >= nitems) break;
+    const PartItem I = items[it];
+    const PartQuery Q = pqs[I.pq];
+    load_plan(plans, Q.plan, &sp);
+    const uint64_t rec0 = mat[Q.mat_base];  // the query's first record
+    for (uint32_t p = threadIdx.x; p < Q.nparts; p += kBlock) {
+      cursor[p] = mat[Q.mat_base + (uint64_t)p * Q.nitems + I.j] - rec0;
+      cnt[p] = 0;
+    }
+    __syncthreads();
+    const DevPlan& P = sp;
+    const uint32_t lmask = (1u << Q.shift) - 1;
+    uint16_t* qgrp = grp + Q.grp_base;
+    uint64_t* qval = val + Q.val_base;
+    uint32_t oor = 0;
+    stage_issue(P, I.t0, lds);
+    for (uint32_t t = I.t0, i = 0; t < I.t1; ++t, ++i) {
+      stage_wait();
+      __syncthreads();  // tile t is in LDS; every wave is done with the other buffer
+      uint8_t* cur = lds + (i & 1) * stage_max;
+      uint8_t* nxt = lds + ((i + 1) & 1) * stage_max;
+      if (t + 1 < I.t1) stage_issue(P, t + 1, nxt);
+      const uint32_t sel = eval_select(P, cur, P.base0 + (uint64_t)t * kTileRows);
+      if (!__syncthreads_or(sel != 0)) continue;  // the same for the whole workgroup
+      int64_t gid[kRows];
+      eval_groups(P, cur, gid);
+      uint32_t pr[kRows], rk[kRows];
+#pragma unroll
+      for (int r = 0; r < kRows; ++r) {
+        pr[r] = 0xffffffffu;
+        rk[r] = 0;
+        if (!((sel >> r) & 1)) continue;
+        if (gid[r] < 0) {
+          ++oor;
+          continue;
+        }
+        pr[r] = (uint32_t)((uint64_t)gid[r] >> Q.shift);
+        rk[r] = atomicAdd(&cnt[pr[r]], 1u);
+      }
+      __syncthreads();  // every rank taken
+      uint64_t dst[kRows];
+#pragma unroll
+      for (int r = 0; r < kRows; ++r) {
+        dst[r] = 0;
+        if (pr[r] == 0xffffffffu) continue;
+        dst[r] = cursor[pr[r]] + rk[r];
+        qgrp[dst[r]] = (uint16_t)((uint64_t)gid[r] & lmask);
+      }
+      for (int a = 0; a < P.naggs; ++a) {
+        uint64_t v[kRows];
+        eval_agg(P.aggs[a], cur, v);
+#pragma unroll
+        for (int r = 0; r < kRows; ++r)
+          if (pr[r] != 0xffffffffu) qval[dst[r] * Q.naggs + a] = v[r];
+      }
+      __syncthreads();  // every cursor read
+#pragma unroll
+      for (int r = 0; r < kRows; ++r)
+        if (pr[r] != 0xffffffffu && rk[r] == 0) {
+          cursor[pr[r]] += cnt[pr[r]];
+          cnt[pr[r]] = 0;
+        }
+      // the next tile's first barrier orders these before its ranks
+    }
+    if (oor && P.out_oor) atomicAdd(P.out_oor, (unsigned long long)oor);
+  }
+}
+
+__global__ void __launch_bounds__(kPartAggBlock) k_bpart_agg(
+    const DevPlan* __restrict__ plans, const PartQuery* __restrict__ pqs,
+    const PartChunk* __restrict__ chunks, uint64_t nchunks, unsigned long long* counter,
+    const unsigned long long* __restrict__ mat, const uint16_t* __restrict__ grp,
+    const uint64_t* __restrict__ val) {
+  extern __shared__ __align__(16) unsigned long long tab[];
+  __shared__ uint32_t s_item;
+  for (;;) {
+    const uint32_t it = next_item(counter, &s_item);
+    if (it >= nchunks) break;
+    const PartChunk C = chunks[it];
+    const PartQuery Q = pqs[C.pq];
+    const DevPlan& P = plans[Q.plan];
+    const uint64_t rec0 = mat[Q.mat_base];
+    const uint64_t rend = mat[Q.mat_base + (uint64_t)Q.nparts * Q.nitems];
+    uint64_t lo = rec0 + (uint64_t)C.c * kPartChunk;
+    if (lo >= rend) continue;
+    const uint64_t hi = lo + kPartChunk < rend ? lo + kPartChunk : rend;
+    // the partition holding record lo: the last p with its first record <= lo
+    uint32_t pa = 0, pb = Q.nparts - 1;
+    while (pa < pb) {
+      const uint32_t m = (pa + pb + 1) >> 1;
+      if (mat[Q.mat_base + (uint64_t)m * Q.nitems] <= lo) pa = m;
+      else pb = m - 1;
+    }
+    const uint16_t* qgrp = grp + Q.grp_base;
+    const uint64_t* qval = val + Q.val_base;
+    for (uint32_t p = pa; lo < hi && p < Q.nparts; ++p) {
+      const uint64_t pe = p + 1 < Q.nparts ? mat[Q.mat_base + (uint64_t)(p + 1) * Q.nitems] : rend;
+      const uint64_t end = pe < hi ? pe : hi;
+      if (end <= lo) continue;
+      const uint64_t l0 = lo - rec0;  // local index of the run's first record
+      agg_run(P.naggs, P.minmax_kinds, P.out_count, P.out_sums, (uint64_t)p << Q.shift,
+              1u << Q.shift, tab, qgrp + l0, end - lo,
+              [&](uint64_t i, int k) { return qval[(l0 + i) * Q.naggs + k]; });
+      lo = end;
+    }
+  }
+}
+
+int32_t launch_batch_part(tgp_ctx* ctx, const DevPlan* d_plans, const DevPlan* d_hplans,
+                          const PartQuery* d_pq, const PartItem* d_items, uint64_t nitems,
+                          const PartChunk* d_chunks, uint64_t nchunks, uint32_t hstage_max,
+                          uint32_t stage_max, uint32_t nparts_max, uint32_t table_max,
+                          unsigned long long* counters, unsigned long long* mat, uint64_t mat_n,
+                          uint64_t* bsum, uint16_t* grp, uint64_t* val, hipStream_t s) {
+  if (!nitems) return TGP_OK;
+  const uint32_t lh = 2 * hstage_max + 4 * nparts_max;
+  const uint32_t ls = 2 * stage_max + 12 * nparts_max;
+  TGP_TRY(allow_lds(k_bpart_hist, lh));
+  TGP_TRY(allow_lds(k_bpart_scatter, ls));
+  TGP_TRY(allow_lds(k_bpart_agg, table_max));
+  auto grid_of = [&](auto kern, uint32_t block, uint32_t lds, uint64_t units) -> unsigned {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, lds) != hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    uint64_t g = (uint64_t)persistent_grid(ctx, per_cu);
+    return (unsigned)(g < units ? g : units);
+  };
+  k_bpart_hist<<<grid_of(k_bpart_hist, kBlock, lh, nitems), kBlock, lh, s>>>(
+      d_hplans, d_pq, d_items, nitems, counters, hstage_max, mat);
+  TGP_HIP(hipGetLastError());
+  TGP_TRY(scan_u64(s, reinterpret_cast<uint64_t*>(mat), mat_n, bsum));
+  k_bpart_scatter<<<grid_of(k_bpart_scatter, kBlock, ls, nitems), kBlock, ls, s>>>(
+      d_plans, d_pq, d_items, nitems, counters + 1, stage_max, nparts_max, mat, grp, val);
+  TGP_HIP(hipGetLastError());
+  if (nchunks) {
+    k_bpart_agg<<<grid_of(k_bpart_agg, kPartAggBlock, table_max, nchunks), kPartAggBlock,
+                  table_max, s>>>(d_plans, d_pq, d_chunks, nchunks, counters + 2, mat, grp, val);
+    TGP_HIP(hipGetLastError());
+  }
+  return TGP_OK;
 }
 
 // ---- many queries of any form in one launch (the batch's generic items) -----------------
@@ -608,7 +854,7 @@ int32_t spec_shape_index(const tgp_agg_query* q);
 
 // The histogram's plan: q with only its predicate and group-key columns (the same
 // row range, selection bitmap, tiles and group ids) and no aggregates.
-static int32_t plan_part_hist(const tgp_agg_query* q, DevPlan* H) {
+int32_t plan_part_hist(const tgp_agg_query* q, DevPlan* H) {
   tgp_agg_query rq = *q;
   int map[TGP_MAX_COLS];
   for (int c = 0; c < TGP_MAX_COLS; ++c) map[c] = -1;
@@ -634,15 +880,6 @@ static uint32_t part_hist_lds(const DevPlan& H, uint32_t nparts) {
 static uint32_t part_scatter_lds(const DevPlan& P, uint32_t nparts) {
   return 2 * P.stage_bytes + nparts * 12;
 }
-constexpr uint32_t kLdsMax = 160 * 1024;
-
-template <typename K>
-static int32_t allow_lds(K kern, uint32_t bytes) {
-  if (bytes > 65536)
-    TGP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)bytes));
-  return TGP_OK;
-}
 
 // kAggPart: histogram, scan, scatter and per-partition aggregation on stream s
 // (the result already zeroed).
@@ -651,7 +888,9 @@ static int32_t run_part(tgp_ctx* ctx, const DevPlan& P, const DevPlan& H, const 
   if (P.ntiles == 0) return TGP_OK;
   const uint32_t np = info.nparts, sh = info.part_shift;
   const uint64_t rows = P.row_end - P.row_begin;
-  const uint64_t G = std::min<uint64_t>(P.ntiles, (uint64_t)persistent_grid(ctx, 2));
+  uint64_t G = std::min<uint64_t>(P.ntiles, (uint64_t)persistent_grid(ctx, 2));
+  const int64_t gmax = ctx->tune_grid_max.load();
+  if (gmax > 0 && G > (uint64_t)gmax) G = (uint64_t)gmax;
   // histogram workgroups per scatter workgroup: up to 8 per CU while mat stays small
   uint32_t k = 1;
   while (k < 4 && G * 2 * k <= P.ntiles && (uint64_t)np * G * 2 * k <= (1u << 22)) k *= 2;

@@ -1913,9 +1913,11 @@ int32_t tgp_hash_probe_agg(tgp_ctx* ctx, const tgp_join_agg_query* q, const tgp_
     const bool use_bits = !q->table->has_dup;
     const bool hot = use_bits && want_hot(ctx, q->table, n);
     // scratch: mat[(nparts + 1) * G + 1] | records[n] | match bits | hot-key image
-    const uint64_t G = (uint64_t)persistent_grid(ctx, 8) < (n + kPartTile - 1) / kPartTile
-                           ? (uint64_t)persistent_grid(ctx, 8)
-                           : (n + kPartTile - 1) / kPartTile;
+    uint64_t G = (uint64_t)persistent_grid(ctx, 8) < (n + kPartTile - 1) / kPartTile
+                     ? (uint64_t)persistent_grid(ctx, 8)
+                     : (n + kPartTile - 1) / kPartTile;
+    const int64_t gmax = ctx->tune_grid_max.load();
+    if (gmax > 0 && G > (uint64_t)gmax) G = (uint64_t)gmax;
     const uint64_t mat_n = (nparts + 1) * G + 1;
     const uint64_t nsb = (mat_n - 1 + kScanChunk - 1) / kScanChunk + 1;  // scan block sums
     const uint64_t head = (8 * (mat_n + nsb) + 255) & ~255ull;

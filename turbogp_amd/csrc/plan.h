@@ -38,6 +38,40 @@ struct GenericItem {
   uint32_t q, t0, t1, mode;
 };
 
+// The batch's queries whose groups exceed an LDS table (kAggPart's passes, for all
+// of them at once): per query, its plans, partition geometry, its items of
+// kGenericTiles tiles (histogram and scatter) and where its counts and records live.
+struct PartQuery {
+  uint32_t plan;      // full plan: index in the generic plans
+  uint32_t hplan;     // histogram plan (predicate and key columns only)
+  uint32_t shift, nparts;
+  uint32_t nitems, naggs;
+  uint64_t mat_base;  // counts: mat[mat_base + p * nitems + j] (item j, partition p)
+  uint64_t grp_base;  // records (local index l): grp[grp_base + l],
+  uint64_t val_base;  //   val[val_base + l * naggs + a]
+};
+struct PartItem {  // tiles [t0, t1) of query pq, its item j
+  uint32_t pq, j, t0, t1;
+};
+struct PartChunk {  // records [c * kPartChunk, (c + 1) * kPartChunk) of query pq
+  uint32_t pq, c;
+};
+constexpr uint64_t kPartChunk = 1ull << 15;  // records per aggregation work item
+
+#ifndef __HIPCC_RTC__
+// The histogram plan of q for kAggPart: only its predicate and key columns.
+int32_t plan_part_hist(const tgp_agg_query* q, DevPlan* H);
+// Queues the passes of the batch's partitioned queries: histogram items, the scan of
+// mat (mat_n words, the total at mat[mat_n]; bsum ceil(mat_n / 4096) + 1 words),
+// scatter items, aggregation chunks.  counters: 3 words, zero.
+int32_t launch_batch_part(tgp_ctx* ctx, const DevPlan* d_plans, const DevPlan* d_hplans,
+                          const PartQuery* d_pq, const PartItem* d_items, uint64_t nitems,
+                          const PartChunk* d_chunks, uint64_t nchunks, uint32_t hstage_max,
+                          uint32_t stage_max, uint32_t nparts_max, uint32_t table_max,
+                          unsigned long long* counters, unsigned long long* mat, uint64_t mat_n,
+                          uint64_t* bsum, uint16_t* grp, uint64_t* val, hipStream_t s);
+#endif
+
 // Queues the generic items (one persistent launch; 2 stage buffers of stage_max
 // bytes + an LDS table of table_max bytes per workgroup) and the MIN/MAX
 // finalisation of the plans in d_minmax (minmax_words: the largest groups x aggs).
