@@ -366,6 +366,25 @@ def test_mixed_shape_batch_has_no_fallback(ctx, table):
             b.close()
         for i in range(len(qs)):
             assert same(batched[i], single[i]) and same(replay[i], single[i]), (i, hostq[i][7])
+        # the same batch with the specialised shapes compiled at planning: those queries
+        # run their compiled kernels inside the batch's graph
+        ctx.tune(A.TUNE_JIT, 2)
+        b = Batch(ctx, len(qs))
+        try:
+            b.set_queries(qs)
+            ctx.tune(A.TUNE_JIT, 0)
+            assert b.info()["fallback"] == 0
+            b.set_tuning(0, use_graph=True, chunk_slices=64, order=3, cached_loads=True)
+            b.run(A.BATCH_PERSISTENT)
+            compiled = b.results()
+            b.run(A.BATCH_PERSISTENT)
+            compiled_replay = b.results()
+        finally:
+            ctx.tune(A.TUNE_JIT, 0)
+            b.close()
+        for i in range(len(qs)):
+            assert same(compiled[i], single[i]) and same(compiled_replay[i], single[i]), \
+                (i, hostq[i][7])
         for i in list(range(0, 64, 3)) + [8 * j + 7 for j in range(3)]:
             nm, pr, pa, ag, b0, e0, sel, form = hostq[i]
             arrays = [host[x].astype(np.int32) if (form == 6 and x == "price") else host[x]

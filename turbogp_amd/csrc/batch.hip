@@ -23,6 +23,7 @@
 #include <utility>
 #include <vector>
 
+#include "jit.h"
 #include "wave_agg.h"
 
 namespace tgp {
@@ -702,6 +703,16 @@ struct tgp_batch {
   uint64_t hplans_cap = 0, pqs_cap = 0, pitems_cap = 0, pchunks_cap = 0, pmat_cap = 0,
            pgrp_cap = 0, pval_cap = 0, pmat_n = 0;
   uint32_t phstage_max = 0, pstage_max = 0, pnparts_max = 0, ptable_max = 0;
+  // generic queries with a specialised shape whose kernels are ready (compiled ahead
+  // of time or at run time): one launch of that kernel each, in the batch's graph
+  struct SpecRun {
+    Shape shape;
+    ShapeKernels k;
+    SpecArgs a;
+  };
+  std::vector<SpecRun> spec_runs;
+  unsigned long long* d_spec_counter = nullptr;
+  uint64_t spec_counter_cap = 0;
   uint32_t* d_punits = nullptr;
   uint64_t pgroups_cap = 0, plinks_cap = 0, partials_cap = 0, partial_words = 0, punits_cap = 0;
 };
@@ -1071,7 +1082,40 @@ constexpr uint32_t kGenericLds = 160 * 1024 - (uint32_t)sizeof(DevPlan) - 1024;
 // the partitioned passes (smaller ones add into the result as items).
 constexpr uint64_t kBatchPartRows = 1ull << 16;
 
+// Generic queries of at least kSpecRunRows rows whose specialised shape has kernels
+// ready run those kernels, one launch each (smaller ones cost more in launches than
+// as items); under TGP_TUNE_JIT 0 a shape not compiled yet starts compiling in the
+// background (later batches use it) and its queries stay generic items; 2 compiles
+// it now; 1 never.
+constexpr uint64_t kSpecRunRows = 1ull << 20;
+static int32_t plan_spec_runs(tgp_batch* b, const tgp_agg_query* qs) {
+  b->spec_runs.clear();
+  const int64_t mode = b->ctx->tune_jit.load();
+  if (mode == 1 || b->ctx->tune_no_spec.load()) return TGP_OK;
+  std::vector<int32_t> rest;
+  for (int32_t i : b->fallback) {
+    tgp_batch::SpecRun r;
+    bool ok = false;
+    if (qs[i].row_end - qs[i].row_begin >= kSpecRunRows &&
+        extract_shape(&qs[i], &r.shape, &r.a))
+      TGP_TRY(find_shape_kernels(r.shape, mode == 2, &r.k, &ok));
+    if (!ok) {
+      rest.push_back(i);
+      continue;
+    }
+    r.a.out_count = reinterpret_cast<unsigned long long*>(b->d_results + b->count_off[i]);
+    r.a.out_sums = reinterpret_cast<unsigned long long*>(b->d_results + b->sums_off[i]);
+    r.a.out_oor = reinterpret_cast<unsigned long long*>(b->d_results + b->oor_off[i]);
+    b->spec_runs.push_back(r);
+  }
+  b->fallback.swap(rest);
+  if (!b->spec_runs.empty())
+    TGP_TRY(ensure_cap(b, &b->d_spec_counter, &b->spec_counter_cap, 1));
+  return TGP_OK;
+}
+
 static int32_t plan_generic(tgp_batch* b, const tgp_agg_query* qs, hipStream_t s) {
+  TGP_TRY(plan_spec_runs(b, qs));
   b->gplans.clear();
   b->gitems.clear();
   b->gminmax.clear();
@@ -1285,6 +1329,8 @@ static int32_t build_items(tgp_batch* b, hipStream_t s) {
 static int32_t batch_launch(tgp_batch* b, hipStream_t s) {
   // results and the counters behind them start at zero
   TGP_TRY(fill_u64(s, b->d_results, b->result_words + 2 * kQueues + 1, 0));
+  for (const auto& r : b->spec_runs)
+    TGP_TRY(launch_shape(b->ctx, r.shape, r.k, r.a, s, b->d_spec_counter, true));
   if (!b->pqs.empty()) {  // before the generic launch, whose finalisation covers them
     TGP_TRY(fill_u64(s, reinterpret_cast<uint64_t*>(b->d_pmat), 3, 0));
     unsigned long long* mat = b->d_pmat + 4;
@@ -1337,7 +1383,8 @@ int32_t tgp_batch_destroy(tgp_batch* b) {
   if (b->d_gminmax) hbm_free(b->ctx, b->d_gminmax);
   if (b->d_partials) hbm_free(b->ctx, b->d_partials);
   for (void* p : {(void*)b->d_hplans, (void*)b->d_pqs, (void*)b->d_pitems, (void*)b->d_pchunks,
-                  (void*)b->d_pmat, (void*)b->d_pgrp, (void*)b->d_pval})
+                  (void*)b->d_pmat, (void*)b->d_pgrp, (void*)b->d_pval,
+                  (void*)b->d_spec_counter})
     if (p) hbm_free(b->ctx, p);
   for (void* p : b->retired) hbm_free(b->ctx, p);
   delete b;
@@ -1589,7 +1636,7 @@ int32_t tgp_batch_info(tgp_batch* b, uint64_t* result_words, uint64_t* total_sli
   TGP_REQUIRE(b, "tgp_batch_info: null batch");
   if (result_words) *result_words = b->result_words;
   if (total_slices) *total_slices = b->total_slices;
-  if (n_batched) *n_batched = (uint32_t)(b->plan.size() + b->gplans.size());
+  if (n_batched) *n_batched = (uint32_t)(b->plan.size() + b->gplans.size() + b->spec_runs.size());
   if (n_fallback) *n_fallback = (uint32_t)b->fallback.size();
   return TGP_OK;
 }

@@ -40,36 +40,11 @@ static int32_t zero_result(hipStream_t st, const SpecArgs& A, int naggs,
 
 // The kernels that run one shape: ahead-of-time template instances, or the
 // run-time compiled module functions of the same bodies (jit.hip).
-struct ShapeKernels {
-  const void* spec = nullptr;  // k_spec<S>
-  const void* wave = nullptr;  // k_wave<S>
-  JitShapeKernels jit;         // used when spec is null
-  int32_t occupancy(bool use_wave, uint32_t lds, int* per_cu) const {
-    if (spec) {
-      TGP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, use_wave ? wave : spec, kBlock,
-                                                           lds));
-    } else {
-      TGP_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(
-          per_cu, use_wave ? jit.wave : jit.spec, kBlock, lds));
-    }
-    return TGP_OK;
-  }
-  int32_t launch(bool use_wave, uint64_t grid, uint32_t lds, hipStream_t st, void** args) const {
-    if (spec) {
-      TGP_HIP(hipLaunchKernel(use_wave ? wave : spec, dim3((unsigned)grid), dim3(kBlock), args,
-                              lds, st));
-    } else {
-      TGP_HIP(hipModuleLaunchKernel(use_wave ? jit.wave : jit.spec, (unsigned)grid, 1, 1, kBlock,
-                                    1, 1, lds, st, args, nullptr));
-    }
-    return TGP_OK;
-  }
-};
 
 // Chooses the kernel form and grid of shape S for the query's arguments and
 // launches it (after zeroing the result in the same stream).
-static int32_t run_shape(tgp_ctx* ctx, const Shape& S, const ShapeKernels& K, SpecArgs A,
-                         hipStream_t st) {
+int32_t launch_shape(tgp_ctx* ctx, const Shape& S, const ShapeKernels& K, SpecArgs A,
+                     hipStream_t st, unsigned long long* counter, bool zeroed) {
   // Kernel form (measured over 20K..60M rows): the replicated-histogram waves
   // for large queries of several groups, and from 256K rows up for queries
   // whose histogram outgrows a wave's registers (the register fold then
@@ -115,10 +90,14 @@ static int32_t run_shape(tgp_ctx* ctx, const Shape& S, const ShapeKernels& K, Sp
     if (grid > need) grid = need;
     if (gmax > 0 && grid > (uint64_t)gmax) grid = (uint64_t)gmax;
     if (grid == 0) grid = 1;
-    void* counter = nullptr;
-    TGP_TRY(get_scratch(ctx, st, 256, &counter));
-    auto* cnt = static_cast<unsigned long long*>(counter);
-    TGP_TRY(zero_result(st, A, S.naggs, cnt));
+    unsigned long long* cnt = counter;
+    if (!cnt) {
+      void* scr = nullptr;
+      TGP_TRY(get_scratch(ctx, st, 256, &scr));
+      cnt = static_cast<unsigned long long*>(scr);
+    }
+    if (!zeroed) TGP_TRY(zero_result(st, A, S.naggs, cnt));
+    else if (chunk) TGP_TRY(fill_u64(st, reinterpret_cast<uint64_t*>(cnt), 1, 0));
     uint32_t rs = (uint32_t)rshift;
     void* args[] = {&A, &rs, &chunk, &cnt};
     TGP_TRY(K.launch(true, grid, lds, st, args));
@@ -128,7 +107,7 @@ static int32_t run_shape(tgp_ctx* ctx, const Shape& S, const ShapeKernels& K, Sp
     if (grid > need) grid = need;
     if (gmax > 0 && grid > (uint64_t)gmax) grid = (uint64_t)gmax;
     if (grid == 0) grid = 1;
-    TGP_TRY(zero_result(st, A, S.naggs, nullptr));
+    if (!zeroed) TGP_TRY(zero_result(st, A, S.naggs, nullptr));
     void* args[] = {&A};
     TGP_TRY(K.launch(false, grid, lds, st, args));
   }
@@ -156,6 +135,19 @@ bool aot_kernels(const Shape& s, ShapeKernels* K) {
 // query runs on the generic engine until the compile is done; 1: never
 // compiled; 2: compiled before the query runs, whatever its size.
 constexpr uint64_t kJitMinRows = 1ull << 20;
+
+int32_t find_shape_kernels(const Shape& s, bool wait, ShapeKernels* K, bool* ok) {
+  *ok = false;
+  *K = ShapeKernels{};
+  if (aot_kernels<0>(s, K)) {
+    *ok = true;
+    return TGP_OK;
+  }
+  bool ready = false;
+  if (jit_shape_kernels(s, wait, &K->jit, &ready) != TGP_OK) return TGP_OK;  // failed: none
+  *ok = ready;
+  return TGP_OK;
+}
 
 // ---- host: shape extraction ----------------------------------------------------------
 
@@ -398,7 +390,7 @@ int32_t try_spec(tgp_ctx* ctx, const tgp_agg_query* q, const tgp_group_result* o
     bool ready = false;
     if (jit_shape_kernels(s, mode == 2, &K.jit, &ready) != TGP_OK || !ready) return TGP_OK;
   }
-  TGP_TRY(run_shape(ctx, s, K, A, st));
+  TGP_TRY(launch_shape(ctx, s, K, A, st, nullptr, false));
   *launched = true;
   return TGP_OK;
 }
