@@ -18,6 +18,15 @@ EQ, NE, LT, LE, GT, GE = range(6)
 SIGNED = {A.I8, A.I16, A.I32, A.I64}
 
 
+@pytest.fixture(params=[1, 2], ids=["one_pass", "partitioned"])
+def part_mode(request, ctx):
+    """The one-pass form (every row into the global table) or the partitioned form
+    (rows by key hash, per-partition LDS tables merged into it), forced."""
+    ctx.tune(A.TUNE_AGG_PART, request.param)
+    yield request.param
+    ctx.tune(A.TUNE_AGG_PART, 0)
+
+
 def _words(flags):
     n = len(flags)
     w = np.zeros(max(1, (n + 63) // 64), np.uint64)
@@ -66,7 +75,7 @@ AGGS = [("sum", 1, [(2, 0, 1)]), ("sum", 1, [(2, 0, 1), (3, 100, -1)]), ("count_
         ("min", 2), ("max_u64", 2)]
 
 
-def test_sparse_int64_keys(ctx):
+def test_sparse_int64_keys(ctx, part_mode):
     rng = np.random.default_rng(11)
     n = 1_000_000
     pool = rng.integers(-(2 ** 62), 2 ** 62, 5000, dtype=np.int64)
@@ -77,7 +86,7 @@ def test_sparse_int64_keys(ctx):
     assert 4000 < len(got) <= 5000
 
 
-def test_two_column_keys_packed_and_nulls(ctx):
+def test_two_column_keys_packed_and_nulls(ctx, part_mode):
     rng = np.random.default_rng(12)
     n = 300_000
     a = rng.integers(0, 3, n).astype(np.uint8)
@@ -90,7 +99,7 @@ def test_two_column_keys_packed_and_nulls(ctx):
 
 
 @pytest.mark.parametrize("tag_bits", [0, 1, 3, 32])
-def test_two_wide_columns_hashed_form(ctx, tag_bits):
+def test_two_wide_columns_hashed_form(ctx, part_mode, tag_bits):
     rng = np.random.default_rng(13)
     n = 200_000
     a = rng.integers(-(2 ** 62), 2 ** 62, 700, dtype=np.int64)[rng.integers(0, 700, n)]
@@ -106,7 +115,7 @@ def test_two_wide_columns_hashed_form(ctx, tag_bits):
     assert len(got) > 10_000
 
 
-def test_three_and_four_keys_with_tags_cut(ctx):
+def test_three_and_four_keys_with_tags_cut(ctx, part_mode):
     rng = np.random.default_rng(14)
     n = 100_000
     cols = [rng.integers(0, 4, n).astype(np.int16), rng.integers(0, 5, n).astype(np.uint16),
@@ -123,7 +132,7 @@ def test_three_and_four_keys_with_tags_cut(ctx):
                 ctx.tune(A.TUNE_GROUP_TAG_BITS, 0)
 
 
-def test_special_keys_int64_min_and_null(ctx):
+def test_special_keys_int64_min_and_null(ctx, part_mode):
     n = 50_000
     rng = np.random.default_rng(15)
     k = rng.choice(np.array([np.iinfo(np.int64).min, -1, 0, 1, np.iinfo(np.int64).max],
@@ -134,7 +143,7 @@ def test_special_keys_int64_min_and_null(ctx):
     assert (np.iinfo(np.int64).min,) in got and (None,) in got
 
 
-def test_hot_keys_lds_cache_and_big_values(ctx):
+def test_hot_keys_lds_cache_and_big_values(ctx, part_mode):
     # six sparse keys over 4M rows: the LDS cache takes nearly every row; values
     # near 2^62 make the int128 carries matter
     rng = np.random.default_rng(16)
@@ -147,7 +156,7 @@ def test_hot_keys_lds_cache_and_big_values(ctx):
     assert len(got) == 6
 
 
-def test_selection_bitmap_row_range_and_empty(ctx):
+def test_selection_bitmap_row_range_and_empty(ctx, part_mode):
     rng = np.random.default_rng(17)
     n = 100_003
     k = rng.integers(0, 2 ** 40, 2000, dtype=np.int64)[rng.integers(0, 2000, n)]
@@ -161,7 +170,7 @@ def test_selection_bitmap_row_range_and_empty(ctx):
     assert run_both(ctx, [k, np.zeros(n, np.uint8), v, v], [], [0], AGGS[:1], b=10, e=10) == {}
 
 
-def test_capacity_protocol(ctx):
+def test_capacity_protocol(ctx, part_mode):
     rng = np.random.default_rng(18)
     n = 200_000
     k = rng.integers(-(2 ** 62), 2 ** 62, 30_000, dtype=np.int64)[rng.integers(0, 30_000, n)]
@@ -190,6 +199,27 @@ def test_capacity_protocol(ctx):
     finally:
         for c in cols + bufs:
             c.free()
+
+
+def test_partitioned_form_lds_overflow_and_hot_keys(ctx):
+    """Keys whose hash puts them all into one partition, more than its LDS table holds,
+    so most go to the global table directly; plus one key holding half the rows."""
+    rng = np.random.default_rng(12)
+    cand = rng.integers(-2**62, 2**62, 400_000, dtype=np.int64)
+    # keys whose top 3 hash bits are 0: with 5000 groups (16 partitions by the top 4
+    # bits) they crowd 2 partitions, 2500 keys each against 1024 LDS slots
+    keys = cand[(W.mix64_np(cand.view(np.uint64)) >> np.uint64(61)) == 0][:5000]
+    assert len(keys) == 5000
+    n = 400_000
+    k = keys[rng.integers(0, len(keys), n)]
+    k[::2] = keys[7]                                   # a hot key
+    v = rng.integers(-2**40, 2**40, n).astype(np.int64)
+    ctx.tune(A.TUNE_AGG_PART, 2)
+    try:
+        run_both(ctx, [k, v], [], [0], [("sum", 1, [(1, 0, 1)]), ("max", 1), ("min_u64", 1)],
+                 capacity=len(keys))
+    finally:
+        ctx.tune(A.TUNE_AGG_PART, 0)
 
 
 def test_1_5m_groups_over_60m_rows(ctx):

@@ -327,6 +327,359 @@ __global__ void __launch_bounds__(256) k_hg_minmax(const uint8_t* __restrict__ d
   }
 }
 
+// ---- partitioned form (raw and packed key words, many groups) ------------------------------
+// Each row of the global table's form costs a random probe and random atomics in a
+// table of millions of slots.  Instead the rows are partitioned by the top bits of
+// their key word's hash (the raw form's NULL and INT64_MIN keys into one more
+// partition), as kAggPart partitions dense ids: a histogram through a plan of the
+// predicate and key columns, a scan, a scatter of records (key word, one int64 per
+// aggregate) through the full plan, then workgroups of kPartChunk records that
+// aggregate each partition's run in an LDS hash table and merge its groups into the
+// global table (one probe and the atomics per group, not per row).  A key that
+// finds no LDS slot goes to the global table directly, so the result never
+// depends on the LDS table fitting; the global table's slots, accumulators and
+// capacity rules are those of the one-pass form, and so is everything after it.
+constexpr int kHgpProbe = 32;  // LDS probe steps before a key goes to the global table
+constexpr uint64_t kHgPartMinGroups = 1ull << 15;  // capacities below: the one-pass form
+
+// Key words and partitions of the thread's rows of a staged tile (part = nparts for
+// the raw form's special keys, ~0u for rows not selected); each key column is folded
+// into the word as it is loaded.
+__device__ __forceinline__ void hg_tile_words(const DevPlan& P, const HgDev& H, const uint8_t* buf,
+                                              uint64_t row0, uint32_t sel, uint32_t pbits,
+                                              uint32_t nparts, uint64_t (&K)[kRows],
+                                              uint32_t (&part)[kRows]) {
+  uint32_t special = 0;  // bit r: row r has a raw special key (K[r] 0: NULL, 1: INT64_MIN)
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) K[r] = 0;
+  for (int k = 0; k < H.nkeys; ++k) {
+    int64_t x[kRows];
+    load_col(H.c[k], buf, x);
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      const uint64_t row = row0 + threadIdx.x + r * kBlock;
+      const bool isnull = ((sel >> r) & 1) && hg_null(H.nulls[k], row - P.row_begin);
+      if (H.mode == kHgRaw) {
+        if (isnull || (uint64_t)x[r] == kEmptyRaw) {
+          special |= 1u << r;
+          K[r] = isnull ? 0 : 1;
+        } else {
+          K[r] = (uint64_t)x[r];
+        }
+      } else {
+        K[r] += (isnull ? H.kcard[k] - 1 : (uint64_t)x[r] - (uint64_t)H.kmin[k]) * H.kmul[k];
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kRows; ++r)
+    part[r] = !((sel >> r) & 1) ? ~0u
+              : ((special >> r) & 1) ? nparts : (uint32_t)(hg_mix(K[r]) >> (64 - pbits));
+}
+
+// mat[p * gridDim.x + (h mod G) k + h / G] = workgroup h's rows of partition p
+// (p <= nparts), as k_part_hist_tiles lays them out.
+__global__ void __launch_bounds__(kBlock) k_hgp_hist(DevPlan P, HgDev H, uint32_t pbits,
+                                                     uint32_t nparts, uint32_t k,
+                                                     unsigned long long* __restrict__ mat) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  uint32_t* h = reinterpret_cast<uint32_t*>(lds + 2 * P.stage_bytes);
+  for (uint32_t p = threadIdx.x; p <= nparts; p += kBlock) h[p] = 0;
+  // made visible by the barrier at the top of the first tile
+  for_each_tile(P, lds, [&](const uint8_t* buf, uint64_t row0) {
+    const uint32_t sel = eval_select(P, buf, row0);
+    if (__ballot(sel != 0) == 0) return;
+    uint64_t K[kRows];
+    uint32_t part[kRows];
+    hg_tile_words(P, H, buf, row0, sel, pbits, nparts, K, part);
+#pragma unroll
+    for (int r = 0; r < kRows; ++r)
+      if (part[r] != ~0u) atomicAdd(&h[part[r]], 1u);
+  });
+  __syncthreads();
+  const uint32_t G = gridDim.x / k;
+  const uint64_t col = (uint64_t)(blockIdx.x % G) * k + blockIdx.x / G;
+  for (uint32_t p = threadIdx.x; p <= nparts; p += kBlock)
+    mat[(uint64_t)p * gridDim.x + col] = h[p];
+}
+
+// Records: rkey[r] (the key word, or 0 / 1 for the special keys) and
+// rval[a * cap + r]; mat holds the scan's exclusive offsets.
+__global__ void __launch_bounds__(kBlock) k_hgp_scatter(DevPlan P, HgDev H, uint32_t pbits,
+                                                        uint32_t nparts, uint32_t k,
+                                                        const unsigned long long* __restrict__ mat,
+                                                        uint64_t cap, uint64_t* __restrict__ rkey,
+                                                        uint64_t* __restrict__ rval) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  unsigned long long* cursor = reinterpret_cast<unsigned long long*>(lds + 2 * P.stage_bytes);
+  uint32_t* cnt = reinterpret_cast<uint32_t*>(cursor + nparts + 1);
+  for (uint32_t p = threadIdx.x; p <= nparts; p += kBlock) {
+    cursor[p] = mat[((uint64_t)p * gridDim.x + blockIdx.x) * k];
+    cnt[p] = 0;
+  }
+  // made visible by the barrier at the top of the first tile
+  for_each_tile(P, lds, [&](const uint8_t* buf, uint64_t row0) {
+    const uint32_t sel = eval_select(P, buf, row0);
+    if (!__syncthreads_or(sel != 0)) return;  // the same for the whole workgroup
+    uint64_t K[kRows];
+    uint32_t part[kRows], rk[kRows];
+    hg_tile_words(P, H, buf, row0, sel, pbits, nparts, K, part);
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) rk[r] = part[r] != ~0u ? atomicAdd(&cnt[part[r]], 1u) : 0u;
+    __syncthreads();  // every rank taken
+    uint64_t dst[kRows];
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      dst[r] = 0;
+      if (part[r] == ~0u) continue;
+      dst[r] = cursor[part[r]] + rk[r];
+      rkey[dst[r]] = K[r];
+    }
+    for (int a = 0; a < P.naggs; ++a) {
+      uint64_t v[kRows];
+      eval_agg(P.aggs[a], buf, v);
+      uint64_t* out = rval + (uint64_t)a * cap;
+#pragma unroll
+      for (int r = 0; r < kRows; ++r)
+        if (part[r] != ~0u) out[dst[r]] = v[r];
+    }
+    __syncthreads();  // every cursor read
+#pragma unroll
+    for (int r = 0; r < kRows; ++r)
+      if (part[r] != ~0u && rk[r] == 0) {
+        cursor[part[r]] += cnt[part[r]];
+        cnt[part[r]] = 0;
+      }
+    // the next tile's first barrier orders these before its ranks
+  });
+}
+
+// Adds the accumulators `a` (count, then per aggregate split sums or a MIN/MAX code)
+// into global slot s (int128 sums, codes).
+__device__ __forceinline__ void hg_merge(const HgDev& H, int32_t naggs, uint32_t kinds,
+                                         uint64_t s, const unsigned long long* a) {
+  unsigned long long* g = &H.acc[s * H.W];
+  atomicAdd(g, a[0]);
+  for (int j = 0; j < naggs; ++j) {
+    if ((kinds >> (3 * j)) & 7) {
+      atomicMax(&g[1 + 2 * j], a[1 + 2 * j]);
+    } else {
+      uint64_t lo;
+      int64_t hi;
+      split_to_i128(a[1 + 2 * j], (int64_t)a[2 + 2 * j], &lo, &hi);
+      atomic_add128(&g[1 + 2 * j], lo, hi);
+    }
+  }
+}
+
+// Global slot of key word K (special keys: their own slots), or -1 (table full).
+__device__ __forceinline__ int64_t hg_global_slot(const HgDev& H, uint64_t K, bool special,
+                                                  uint64_t empty) {
+  const uint64_t cap = H.mask + 1;
+  if (special) return (int64_t)(K == 0 ? cap + 1 : cap);
+  return hg_probe_word(H, K, empty);
+}
+
+// kPartAggBlock threads; LDS: S key words, then S x W accumulators.
+constexpr int kHgpBlock = 1024;
+__global__ void __launch_bounds__(kHgpBlock) k_hgp_agg(HgDev H, int32_t naggs, uint32_t kinds,
+                                                       uint32_t nparts, uint64_t G,
+                                                       const unsigned long long* __restrict__ mat,
+                                                       uint64_t cap, const uint64_t* __restrict__ rkey,
+                                                       const uint64_t* __restrict__ rval,
+                                                       uint32_t S, uint64_t empty) {
+  extern __shared__ __align__(16) unsigned long long lmem[];
+  unsigned long long* key = lmem;
+  unsigned long long* acc = lmem + S;
+  const uint32_t W = (uint32_t)H.W;
+  const uint64_t total = mat[(uint64_t)(nparts + 1) * G];
+  uint64_t lo = (uint64_t)blockIdx.x * kPartChunk;
+  if (lo >= total) return;
+  const uint64_t hi = lo + kPartChunk < total ? lo + kPartChunk : total;
+  uint32_t pa = 0, pb = nparts;  // partitions 0..nparts (nparts: the special keys)
+  while (pa < pb) {
+    const uint32_t m = (pa + pb + 1) >> 1;
+    if (mat[(uint64_t)m * G] <= lo) pa = m;
+    else pb = m - 1;
+  }
+  for (uint32_t p = pa; lo < hi && p <= nparts; ++p) {
+    const uint64_t pe = mat[(uint64_t)(p + 1) * G];
+    const uint64_t end = pe < hi ? pe : hi;
+    if (end <= lo) continue;
+    const bool special = p == nparts;
+    for (uint32_t i = threadIdx.x; i < S; i += kHgpBlock) key[i] = empty;
+    for (uint32_t i = threadIdx.x; i < S * W; i += kHgpBlock) acc[i] = 0;
+    __syncthreads();
+    for (uint64_t i = lo + threadIdx.x; i < end; i += kHgpBlock) {
+      const uint64_t K = rkey[i];
+      // the LDS slot of K (claimed when free), or none after kHgpProbe steps
+      uint32_t pos = (uint32_t)hg_mix(K) & (S - 1);
+      int32_t e = -1;
+      for (int step = 0; step < kHgpProbe; ++step) {
+        unsigned long long cur = key[pos];
+        if (cur == empty)
+          cur = atomicCAS(&key[pos], (unsigned long long)empty, (unsigned long long)K);
+        if (cur == empty || cur == K) {
+          e = (int32_t)pos;
+          break;
+        }
+        pos = (pos + 1) & (S - 1);
+      }
+      if (e >= 0) {
+        unsigned long long* a = &acc[(uint32_t)e * W];
+        atomicAdd(a, 1ull);
+        for (int j = 0; j < naggs; ++j) {
+          const uint64_t v = rval[(uint64_t)j * cap + i];
+          const int32_t mk = (kinds >> (3 * j)) & 7;
+          if (mk) {
+            atomicMax(&a[1 + 2 * j], (unsigned long long)mm_encode(mk, v));
+          } else {
+            atomicAdd(&a[1 + 2 * j], (unsigned long long)(uint32_t)v);
+            const int64_t h = (int64_t)v >> 32;
+            if (h) atomicAdd(&a[2 + 2 * j], (unsigned long long)h);
+          }
+        }
+      } else {  // straight into the global table
+        const int64_t s = hg_global_slot(H, K, special, empty);
+        if (s < 0) {
+          atomicOr(H.overflow, 1u);
+          continue;
+        }
+        unsigned long long* g = &H.acc[(uint64_t)s * W];
+        atomicAdd(g, 1ull);
+        for (int j = 0; j < naggs; ++j) {
+          const uint64_t v = rval[(uint64_t)j * cap + i];
+          const int32_t mk = (kinds >> (3 * j)) & 7;
+          if (mk) atomicMax(&g[1 + 2 * j], (unsigned long long)mm_encode(mk, v));
+          else atomic_add128_i64(&g[1 + 2 * j], (int64_t)v);
+        }
+      }
+    }
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < S; e += kHgpBlock) {
+      const uint64_t K = key[e];
+      if (K == empty) continue;
+      const int64_t s = hg_global_slot(H, K, special, empty);
+      if (s < 0) {
+        atomicOr(H.overflow, 1u);
+        continue;
+      }
+      hg_merge(H, naggs, kinds, (uint64_t)s, &acc[e * W]);
+    }
+    __syncthreads();  // every merge read before the next partition clears the table
+    lo = end;
+  }
+}
+
+// The histogram's plan of the partitioned form: q's predicate and key columns only,
+// and H's key columns placed in that plan's stage.
+int32_t plan_hg_hist(const tgp_agg_query* q, const tgp_hash_group_query* hq, const HgDev& H,
+                     DevPlan* Ph, HgDev* Hh) {
+  tgp_agg_query rq = *q;
+  int map[TGP_MAX_COLS];
+  for (int c = 0; c < TGP_MAX_COLS; ++c) map[c] = -1;
+  rq.ncols = 0;
+  auto use = [&](int c) {
+    if (map[c] < 0) {
+      map[c] = rq.ncols;
+      rq.cols[rq.ncols++] = q->cols[c];
+    }
+    return map[c];
+  };
+  for (int p = 0; p < q->npreds; ++p) rq.preds[p].col = use(q->preds[p].col);
+  int kc[TGP_MAX_GROUP_KEYS] = {};
+  for (int k = 0; k < hq->nkeys; ++k) kc[k] = use(hq->key_col[k]);
+  rq.nparts = 0;
+  rq.naggs = 0;
+  PlanInfo info;
+  TGP_TRY(plan_agg(&rq, Ph, &info));
+  *Hh = H;
+  for (int k = 0; k < hq->nkeys; ++k) Hh->c[k].lds_off = Ph->slots[kc[k]].lds_off;
+  return TGP_OK;
+}
+
+// Runs the partitioned form's passes into the (initialised) global table H.
+// *done false: the query's stage and partition counters do not fit LDS; nothing ran.
+int32_t run_hg_part(tgp_ctx* ctx, const tgp_agg_query* q, const tgp_hash_group_query* hq,
+                    const DevPlan& P, const HgDev& H, uint64_t capacity, uint64_t empty,
+                    hipStream_t s, bool* done) {
+  *done = false;
+  if (P.ntiles == 0) {
+    *done = true;
+    return TGP_OK;
+  }
+  DevPlan Ph;
+  HgDev Hh;
+  TGP_TRY(plan_hg_hist(q, hq, H, &Ph, &Hh));
+  // an LDS hash table of S slots (key word + W accumulator words) in
+  // kPartTableBytes; partitions of about S / 2 groups each, at most 4096
+  const uint32_t W = (uint32_t)H.W;
+  uint32_t S = 64;
+  while ((uint64_t)S * 2 * (8 + 8 * W) <= kPartTableBytes) S *= 2;
+  const uint64_t want = (capacity + S / 2 - 1) / (S / 2);
+  uint32_t pbits = 1;
+  while ((1ull << pbits) < want && pbits < 12) ++pbits;
+  const uint32_t np = 1u << pbits;
+  const uint64_t rows = P.row_end - P.row_begin;
+  const uint32_t lh = 2 * Ph.stage_bytes + 4 * (np + 1);
+  const uint32_t ls = 2 * P.stage_bytes + 12 * (np + 1);
+  const uint32_t la = S * 8 * (1 + W);
+  if (lh > 160 * 1024 || ls > 160 * 1024) return TGP_OK;
+  uint64_t G = std::min<uint64_t>(P.ntiles, (uint64_t)persistent_grid(ctx, 2));
+  const int64_t gmax = ctx->tune_grid_max.load();
+  if (gmax > 0 && G > (uint64_t)gmax) G = (uint64_t)gmax;
+  uint32_t k = 1;  // histogram workgroups per scatter workgroup (its LDS is small)
+  while (k < 4 && G * 2 * k <= P.ntiles && (uint64_t)(np + 1) * G * 2 * k <= (1u << 22)) k *= 2;
+  const uint64_t mat_n = (uint64_t)(np + 1) * G * k;
+  const uint64_t nb = (mat_n + 4095) / 4096;
+  // scratch: mat[mat_n + 1] | scan block sums[nb + 1] | keys[rows] | values[naggs][rows]
+  const uint64_t words = mat_n + 1 + nb + 1 + rows * (1 + (uint64_t)q->naggs);
+  void* scr = nullptr;
+  TGP_TRY(get_scratch(ctx, s, words * 8, &scr));
+  auto* mat = static_cast<unsigned long long*>(scr);
+  auto* bsum = reinterpret_cast<uint64_t*>(mat + mat_n + 1);
+  uint64_t* rkey = bsum + nb + 1;
+  uint64_t* rval = rkey + rows;
+  for (auto kern : {(const void*)k_hgp_hist, (const void*)k_hgp_scatter, (const void*)k_hgp_agg}) {
+    const uint32_t b = kern == (const void*)k_hgp_hist ? lh
+                     : kern == (const void*)k_hgp_scatter ? ls : la;
+    if (b > 65536)
+      TGP_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b));
+  }
+  k_hgp_hist<<<dim3((unsigned)(G * k)), dim3(kBlock), lh, s>>>(Ph, Hh, pbits, np, k, mat);
+  TGP_HIP(hipGetLastError());
+  TGP_TRY(scan_u64(s, reinterpret_cast<uint64_t*>(mat), mat_n, bsum));
+  k_hgp_scatter<<<dim3((unsigned)G), dim3(kBlock), ls, s>>>(P, H, pbits, np, k, mat, rows, rkey,
+                                                            rval);
+  TGP_HIP(hipGetLastError());
+  const uint64_t ga = (rows + kPartChunk - 1) / kPartChunk;
+  k_hgp_agg<<<dim3((unsigned)ga), dim3(kHgpBlock), la, s>>>(H, q->naggs, P.minmax_kinds, np,
+                                                            G * k, mat, rows, rkey, rval, S,
+                                                            empty);
+  TGP_HIP(hipGetLastError());
+  *done = true;
+  return TGP_OK;
+}
+
+// The one-pass form: every row into the global table through the LDS cache of hot keys.
+int32_t run_hg_one_pass(tgp_ctx* ctx, const DevPlan& P, const HgDev& H, hipStream_t s) {
+  // one workgroup may meet any number of rows: the LDS partials are int128
+  const uint32_t lds = 2 * P.stage_bytes + kHgCache * 4 + kHgCache * H.W * 8;
+  if (lds > 65536)
+    TGP_HIP(hipFuncSetAttribute((const void*)k_hash_group,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  int per_cu = 0;
+  TGP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_hash_group, kBlock, lds));
+  if (per_cu < 1) per_cu = 1;
+  if (per_cu > 4) per_cu = 4;
+  uint64_t grid = (uint64_t)persistent_grid(ctx, per_cu);
+  if (grid > P.ntiles) grid = P.ntiles;
+  if (grid == 0) return TGP_OK;
+  k_hash_group<<<(unsigned)grid, kBlock, lds, s>>>(P, H);
+  TGP_HIP(hipGetLastError());
+  return TGP_OK;
+}
+
 unsigned grid_for(uint64_t items) {
   uint64_t g = (items + 255) / 256;
   if (g > 8192) g = 8192;
@@ -449,20 +802,16 @@ int32_t tgp_hash_group_agg_sync(tgp_ctx* ctx, const tgp_hash_group_query* hq,
     const uint64_t zn[2] = {nslots * H.W, 1};
     TGP_TRY(zero_ranges(s, 2, zp, zn));
   }
-  // one workgroup may meet any number of rows: the LDS partials are int128
-  const uint32_t lds = 2 * P.stage_bytes + kHgCache * 4 + kHgCache * H.W * 8;
-  if (lds > 65536)
-    TGP_HIP(hipFuncSetAttribute((const void*)k_hash_group,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  int per_cu = 0;
-  TGP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_hash_group, kBlock, lds));
-  if (per_cu < 1) per_cu = 1;
-  if (per_cu > 4) per_cu = 4;
-  uint64_t grid = (uint64_t)persistent_grid(ctx, per_cu);
-  if (grid > P.ntiles) grid = P.ntiles;
-  k_hash_group<<<(unsigned)grid, kBlock, lds, s>>>(P, H);
-  TGP_HIP(hipGetLastError());
+  // many groups (raw and packed forms): the partitioned form, unless its LDS does not fit
+  const int64_t part_mode = ctx->tune_agg_part.load();
+  bool parted = false;
+  if ((H.mode == kHgRaw || H.mode == kHgPacked) &&
+      (part_mode == 2 ||
+       (part_mode == 0 && n >= kPartMinRows && out->capacity >= kHgPartMinGroups)))
+    TGP_TRY(run_hg_part(ctx, q, hq, P, H, out->capacity ? out->capacity : 1, empty, s, &parted));
+  if (!parted) TGP_TRY(run_hg_one_pass(ctx, P, H, s));
   k_hg_occupied<<<(unsigned)((words * 64 + 255) / 256), 256, 0, s>>>(H.acc, H.W, nslots, bits);
+
   TGP_HIP(hipGetLastError());
   uint64_t G = 0;
   TGP_TRY(compact_begin(ctx, s, bits, nslots, &G));  // synchronous
