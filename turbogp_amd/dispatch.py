@@ -32,6 +32,8 @@ from decimal import Decimal
 from enum import Enum
 from typing import List, Optional, Sequence, Tuple
 
+import struct
+
 import numpy as np
 
 from . import _capi as A
@@ -39,7 +41,7 @@ from . import expr as X
 from .device import DeviceBitmap, DeviceBuffer, HashTable, make_agg_query, numeric_text
 
 OPS = {"=": A.EQ, "!=": A.NE, "<>": A.NE, "<": A.LT, "<=": A.LE, ">": A.GT, ">=": A.GE}
-INT8_OID, NUMERIC_OID = 20, 1700
+INT8_OID, NUMERIC_OID, FLOAT8_OID = 20, 1700, 701
 # a single GROUP BY key whose value range is at most this wide groups by dense ids
 # (tgp_group_agg); wider or sparse keys, several keys and nullable keys go through
 # the device hash table (tgp_hash_group_agg_sync)
@@ -591,6 +593,11 @@ class _Executor:
             slot(c)
         aggs = []
         for func, arg in items:
+            if func in ("SUM", "AVG") and any(self.t.col_type(c) == "float"
+                                             for c in _arg_columns(arg)):
+                # the reference sums FLOAT cells as doubles; the exact integer
+                # engine would add their bit patterns
+                raise Unsupported(f"{func} over a FLOAT column in this shape")
             if func in ("SUM", "AVG"):
                 if isinstance(arg, str):
                     aggs.append(("sum", 1, [(slot(arg), 0, 1)]))
@@ -631,6 +638,14 @@ class _Executor:
             for c in _arg_columns(s0.arg):
                 t.col(c)
             bm, nsel = self.without_nulls(bm, nsel, _arg_columns(s0.arg))
+            if func in ("SUM", "AVG") and isinstance(s0.arg, str) and t.col_type(s0.arg) == "float":
+                # sum_masked_f64 / avg_masked_f64 (dispatch.rs:399-403, :428-432): a FLOAT
+                # column sums as doubles from +0.0; AVG of no rows reads 0
+                total = ctx.sum_f64(t.col(s0.arg), t.row_count, bm, neg_zero_start=False)
+                v = total if func == "SUM" else (total / nsel if nsel else 0.0)
+                bits = struct.unpack("<Q", struct.pack("<d", v))[0]
+                return QueryResult([ResultColumn(s0.alias or func.lower(), [bits], [repr(v)],
+                                                 FLOAT8_OID)], 1)
             r = self.agg_query([(func, s0.arg)], bitmap=bm)
             name = s0.alias or func.lower()
             if func == "AVG":
