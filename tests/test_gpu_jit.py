@@ -204,3 +204,25 @@ def test_default_compiles_in_the_background(ctx):
     finally:
         for c in d:
             c.free()
+
+
+MINMAX_AGGS = [("min", 2), ("max", 1), ("min_u64", 4), ("max_u64", 0),
+               ("max", 1, [(2, 0, 1), (3, 100, -1)]), ("sum", 1, [(2, 0, 1)])]
+
+
+@pytest.mark.parametrize("group", [[], [(5, 1, 0, 3)], [(5, 1, 0, 3), (6, 1, 0, 2)],
+                                   [(0, 30, 0, 85)]])
+def test_min_max_shapes(ctx, table, group):
+    """MIN / MAX (signed and raw u64 order, of a column and of an expression) beside a
+    sum: order codes with a maximum in every kernel form (per-slice register fold,
+    replicated histograms, workgroup table), decoded after the kernel."""
+    check(ctx, table, q1_cols(), [(0, LE, 2435, False)], group, MINMAX_AGGS)
+
+
+def test_min_max_negative_and_unsigned_values(ctx, table):
+    names = ["neg", "price_u64", "flaga", "dt"]
+    aggs = [("min", 0), ("max", 0), ("min_u64", 0), ("max_u64", 1),
+            ("min", 1, [(0, 0, 1), (2, 1, 1)])]
+    check(ctx, table, names, [(3, GT, 100, True)], [(2, 1, 0, 3)], aggs, 5, N - 3)
+    # nothing selected: every group empty, values 0
+    check(ctx, table, names, [(3, LT, 0, True)], [(2, 1, 0, 3)], aggs)

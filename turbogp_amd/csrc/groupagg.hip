@@ -822,6 +822,17 @@ __global__ void __launch_bounds__(256) k_batch_finalize(const DevPlan* __restric
   }
 }
 
+int32_t finalize_minmax(hipStream_t s, unsigned long long* cnt, unsigned long long* sums,
+                        uint64_t ngroups, int32_t naggs, uint32_t kinds) {
+  if (!kinds || !ngroups || naggs <= 0) return TGP_OK;
+  const uint64_t items = ngroups * (uint64_t)naggs;
+  const uint64_t grid = std::min<uint64_t>((items + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_agg_finalize, dim3((uint32_t)grid), dim3(256), 0, s, cnt, sums, ngroups,
+                     naggs, kinds, 0);
+  TGP_HIP(hipGetLastError());
+  return TGP_OK;
+}
+
 int32_t launch_batch_generic(tgp_ctx* ctx, const DevPlan* d_plans, const GenericItem* d_items,
                              uint64_t nitems, unsigned long long* counter, uint32_t stage_max,
                              uint32_t table_max, const uint32_t* d_minmax, uint32_t nminmax,

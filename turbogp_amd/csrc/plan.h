@@ -59,6 +59,10 @@ struct PartChunk {  // records [c * kPartChunk, (c + 1) * kPartChunk) of query p
 constexpr uint64_t kPartChunk = 1ull << 15;  // records per aggregation work item
 
 #ifndef __HIPCC_RTC__
+// Turns MIN / MAX codes (kinds: 3 bits per aggregate) of a result of ngroups x naggs
+// into values (0 for an empty group); the sums are left as they are.
+int32_t finalize_minmax(hipStream_t s, unsigned long long* cnt, unsigned long long* sums,
+                        uint64_t ngroups, int32_t naggs, uint32_t kinds);
 // The histogram plan of q for kAggPart: only its predicate and key columns.
 int32_t plan_part_hist(const tgp_agg_query* q, DevPlan* H);
 // Queues the passes of the batch's partitioned queries: histogram items, the scan of

@@ -287,11 +287,32 @@ __device__ __forceinline__ bool big_value(uint64_t v) {
   return (m >> 55) != 0;
 }
 
+// MIN / MAX aggregates of shape s as a plan's minmax_kinds (3 bits per aggregate:
+// its kind, 0 for the sums).  MIN / MAX accumulate an order code (mm_encode) with
+// an unsigned maximum whose identity is 0, and the result is decoded afterwards.
+constexpr uint32_t shape_minmax_kinds(const Shape& s) {
+  uint32_t k = 0;
+  for (int a = 0; a < s.naggs; ++a)
+    if (is_minmax(s.kind[a])) k |= (uint32_t)s.kind[a] << (3 * a);
+  return k;
+}
+
 template <Shape S, int a>
 __device__ __forceinline__ void accumulate_aggs(const SpecArgs& A, const SliceVals<S>& v,
                                                 uint32_t sel, const int32_t (&gid)[kSR],
                                                 unsigned long long* hist, bool& big) {
-  if constexpr (a < S.naggs) {
+  if constexpr (a < S.naggs && is_minmax(S.kind[a])) {
+    uint64_t val[kSR];
+    agg_values<S, a>(A, v, val);
+#pragma unroll
+    for (int r = 0; r < kSR; ++r)
+      if ((sel >> r) & 1) {
+        const unsigned long long c = mm_encode(S.kind[a], val[r]);
+        if constexpr (S.table == 0) atomicMax(&hist[gid[r] * (1 + S.naggs) + 1 + a], c);
+        else atomicMax(&hist[A.ngroups + ((uint64_t)gid[r] * S.naggs + a) * 2], c);
+      }
+    accumulate_aggs<S, a + 1>(A, v, sel, gid, hist, big);
+  } else if constexpr (a < S.naggs) {
     uint64_t val[kSR];
     agg_values<S, a>(A, v, val);
     bool b = false;

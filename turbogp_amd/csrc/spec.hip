@@ -111,7 +111,8 @@ int32_t launch_shape(tgp_ctx* ctx, const Shape& S, const ShapeKernels& K, SpecAr
     void* args[] = {&A};
     TGP_TRY(K.launch(false, grid, lds, st, args));
   }
-  return TGP_OK;
+  // MIN / MAX accumulate order codes: decoded into values
+  return finalize_minmax(st, A.out_count, A.out_sums, A.ngroups, S.naggs, shape_minmax_kinds(S));
 }
 
 // The ahead-of-time kernels of s, if s is one of kShapes.
@@ -324,7 +325,8 @@ bool extract_shape(const tgp_agg_query* q, Shape* s, SpecArgs* A) {
   s->naggs = q->naggs;
   for (int a = 0; a < q->naggs; ++a) {
     const tgp_agg& G = q->aggs[a];
-    if (G.kind != TGP_AGG_SUM && G.kind != TGP_AGG_COUNT_NONZERO) return false;  // MIN/MAX
+    if (G.kind != TGP_AGG_SUM && G.kind != TGP_AGG_COUNT_NONZERO && !is_minmax(G.kind))
+      return false;
     s->kind[a] = G.kind;
     if (G.kind == TGP_AGG_COUNT_NONZERO) {
       s->nf[a] = 1;

@@ -23,9 +23,13 @@ __device__ __forceinline__ void spec_body(const SpecArgs& A) {
     for (uint64_t i = threadIdx.x; i < words; i += kBlock) lmem[i] = 0;
     __syncthreads();
   }
-  // lane-owned exact accumulator of histogram word `lane` (table 0)
+  // lane-owned exact accumulator of histogram word `lane` (table 0); a MIN / MAX
+  // word keeps the largest order code instead
   uint64_t acc_lo = 0;
   int64_t acc_hi = 0;
+  constexpr uint32_t kMM = shape_minmax_kinds(S);
+  const uint32_t lk = lane % (1 + S.naggs);
+  const bool lane_mm = kMM != 0 && lk > 0 && ((kMM >> (3 * (lk - 1))) & 7) != 0;
   uint32_t oor = 0;
   const uint64_t nwaves = (uint64_t)gridDim.x * (kBlock / kWave);
   const uint64_t gw = (uint64_t)blockIdx.x * (kBlock / kWave) + wave;
@@ -55,11 +59,15 @@ __device__ __forceinline__ void spec_body(const SpecArgs& A) {
       __builtin_amdgcn_wave_barrier();
       const unsigned long long w = hist[lane];
       hist[lane] = 0;
-      const uint64_t s = acc_lo + w;
-      // count words are non-negative; sum words are signed 64-bit partials
-      const bool is_count = (lane % (1 + S.naggs)) == 0;
-      acc_hi += (int64_t)(s < acc_lo) - ((!is_count && (int64_t)w < 0) ? 1 : 0);
-      acc_lo = s;
+      if (lane_mm) {
+        acc_lo = w > acc_lo ? w : acc_lo;
+      } else {
+        const uint64_t s = acc_lo + w;
+        // count words are non-negative; sum words are signed 64-bit partials
+        const bool is_count = lk == 0;
+        acc_hi += (int64_t)(s < acc_lo) - ((!is_count && (int64_t)w < 0) ? 1 : 0);
+        acc_lo = s;
+      }
       __builtin_amdgcn_wave_barrier();
     }
   }
@@ -72,17 +80,24 @@ __device__ __forceinline__ void spec_body(const SpecArgs& A) {
     red[(wave * 64 + lane) * 2 + 1] = (unsigned long long)acc_hi;
     __syncthreads();
     if (threadIdx.x < A.words) {
+      const uint32_t g = threadIdx.x / (1 + S.naggs), a = threadIdx.x % (1 + S.naggs);
+      const bool mm = kMM != 0 && a > 0 && ((kMM >> (3 * (a - 1))) & 7) != 0;
       uint64_t lo = 0;
       int64_t hi = 0;
       for (int w = 0; w < kBlock / kWave; ++w) {
         const uint64_t l2 = red[(w * 64 + threadIdx.x) * 2];
+        if (mm) {
+          lo = l2 > lo ? l2 : lo;
+          continue;
+        }
         const uint64_t s = lo + l2;
         hi += (int64_t)red[(w * 64 + threadIdx.x) * 2 + 1] + (int64_t)(s < lo);
         lo = s;
       }
-      const uint32_t g = threadIdx.x / (1 + S.naggs), a = threadIdx.x % (1 + S.naggs);
       if (a == 0) {
         if (lo) atomicAdd(&A.out_count[g], (unsigned long long)lo);
+      } else if (mm) {
+        if (lo) atomicMax(&A.out_sums[((uint64_t)g * S.naggs + (a - 1)) * 2], lo);
       } else {
         atomic_add128(&A.out_sums[((uint64_t)g * S.naggs + (a - 1)) * 2], lo, hi);
       }
@@ -95,7 +110,11 @@ __device__ __forceinline__ void spec_body(const SpecArgs& A) {
       atomicAdd(&A.out_count[g], c);
       for (int a = 0; a < S.naggs; ++a) {
         const uint64_t i = A.ngroups + (g * S.naggs + a) * 2;
-        atomic_add128(&A.out_sums[(g * S.naggs + a) * 2], lmem[i], (int64_t)lmem[i + 1]);
+        if (kMM != 0 && ((kMM >> (3 * a)) & 7) != 0) {
+          if (lmem[i]) atomicMax(&A.out_sums[(g * S.naggs + a) * 2], lmem[i]);
+        } else {
+          atomic_add128(&A.out_sums[(g * S.naggs + a) * 2], lmem[i], (int64_t)lmem[i + 1]);
+        }
       }
     }
   }
@@ -115,7 +134,7 @@ __device__ __forceinline__ void wave_rows(const SpecArgs& A, uint64_t s, uint64_
                                           unsigned long long* reg, uint32_t rshift,
                                           uint32_t& oor, uint64_t& since) {
   if (since + kSliceRows > kFlushRows) {
-    wave_flush(A.out_count, A.out_sums, A.words, S.naggs, reg, rshift);
+    wave_flush(A.out_count, A.out_sums, A.words, S.naggs, reg, rshift, shape_minmax_kinds(S));
     since = 0;
   }
   SliceVals<S> v;
@@ -154,20 +173,27 @@ __device__ __forceinline__ void wave_body(const SpecArgs& A, uint32_t rshift, ui
   }
   // workgroup flush: every wave folds its copies, then one thread per word adds
   // the 4 waves' totals (as int128) into the result
-  wave_fold_copies(reg, A.words, rshift);
-  __syncthreads();
+  constexpr uint32_t kMM = shape_minmax_kinds(S);
   constexpr uint32_t W1 = 1 + S.naggs;
+  wave_fold_copies(reg, A.words, rshift, W1, kMM);
+  __syncthreads();
   for (uint32_t w = threadIdx.x; w < A.words; w += kBlock) {
-    const bool is_count = w % W1 == 0;
+    const uint32_t k = w % W1;
+    const bool is_count = k == 0;
+    const bool mm = kMM != 0 && k > 0 && ((kMM >> (3 * (k - 1))) & 7) != 0;
     uint64_t lo = 0;
     int64_t hi = 0;
     for (int v = 0; v < kBlock / kWave; ++v) {
       const uint64_t x = lreg[v * region + w];
+      if (mm) {
+        lo = x > lo ? x : lo;
+        continue;
+      }
       const uint64_t t = lo + x;
       hi += (int64_t)(t < lo) - ((!is_count && (int64_t)x < 0) ? 1 : 0);
       lo = t;
     }
-    if (lo || hi) add_result_word(A.out_count, A.out_sums, w, S.naggs, lo, hi);
+    if (lo || hi) add_result_word(A.out_count, A.out_sums, w, S.naggs, lo, hi, kMM);
   }
   if (oor && A.out_oor) atomicAdd(A.out_oor, (unsigned long long)oor);
 }

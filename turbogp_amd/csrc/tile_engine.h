@@ -279,7 +279,7 @@ __device__ __forceinline__ void eval_agg(const DevAgg& A, const uint8_t* buf,
 // MIN / MAX run as an unsigned maximum of an order-preserving code with
 // identity 0 (the value a cleared result holds): signed values flip the sign
 // bit, and MIN complements the code.
-__host__ __device__ __forceinline__ bool is_minmax(int32_t kind) {
+__host__ __device__ constexpr bool is_minmax(int32_t kind) {
   return kind >= TGP_AGG_MIN && kind <= TGP_AGG_MAX_U64;
 }
 __host__ __device__ __forceinline__ uint64_t mm_encode(int32_t kind, uint64_t v) {

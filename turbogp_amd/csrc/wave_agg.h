@@ -77,6 +77,13 @@ __device__ __forceinline__ void wave_slice(const SpecArgs& A, uint64_t slice,
     const uint32_t off = (uint32_t)(1 + a) << rshift;
     uint64_t val[kSR];
     agg_values<S, a>(A, v, val);
+    if constexpr (is_minmax(S.kind[a])) {  // the largest order code; no exact path needed
+#pragma unroll
+      for (int r = 0; r < kSR; ++r)
+        if ((sel >> r) & 1)
+          atomicMax(&mine[rb[r] + off], (unsigned long long)mm_encode(S.kind[a], val[r]));
+      return;
+    }
     // every row of the slice is tested, selected or not: a large unselected
     // value only sends the slice to the exact path
     uint32_t m = 0;
@@ -95,14 +102,23 @@ __device__ __forceinline__ void wave_slice(const SpecArgs& A, uint64_t slice,
   });
 }
 
-// Sums the R = 2^rshift copies of every word of the wave region in place:
-// afterwards reg[w] (w < words) holds word w's 64-bit total and the rest of the
-// region is zero.  Lanes read consecutive words (no bank conflicts); the copies
-// of a word then sit in R consecutive lanes and are added with a butterfly.
-// Every copy holds partials of at most 2^20 values of magnitude < 2^43, so the
-// 64-bit total is exact.
+// True when word w of a region of groups x (count, aggregates) is a MIN / MAX word
+// (kinds: 3 bits per aggregate, a plan's minmax_kinds; W1 = 1 + aggregates).
+__device__ __forceinline__ bool minmax_word(uint32_t w, uint32_t W1, uint32_t kinds) {
+  if (!kinds) return false;
+  const uint32_t k = w % W1;
+  return k > 0 && ((kinds >> (3 * (k - 1))) & 7) != 0;
+}
+
+// Sums the R = 2^rshift copies of every word of the wave region in place (the
+// largest for a MIN / MAX word): afterwards reg[w] (w < words) holds word w's
+// 64-bit total and the rest of the region is zero.  Lanes read consecutive words
+// (no bank conflicts); the copies of a word then sit in R consecutive lanes and
+// are combined with a butterfly.  Every copy holds partials of at most 2^20
+// values of magnitude < 2^43, so the 64-bit total is exact.
 __device__ __forceinline__ void wave_fold_copies(unsigned long long* reg, uint32_t words,
-                                                 uint32_t rshift) {
+                                                 uint32_t rshift, uint32_t W1 = 1,
+                                                 uint32_t kinds = 0) {
   const uint32_t lane = threadIdx.x & 63;
   __builtin_amdgcn_s_waitcnt(0);
   __builtin_amdgcn_wave_barrier();
@@ -112,7 +128,14 @@ __device__ __forceinline__ void wave_fold_copies(unsigned long long* reg, uint32
   for (uint32_t base = 0; base < T; base += 64) {
     const uint32_t idx = base + lane;
     unsigned long long x = idx < T ? reg[idx] : 0ull;
-    for (uint32_t m = 1; m < R; m <<= 1) x += __shfl_xor(x, (int)m);
+    if (kinds && minmax_word(idx >> rshift, W1, kinds)) {
+      for (uint32_t m = 1; m < R; m <<= 1) {
+        const unsigned long long y = __shfl_xor(x, (int)m);
+        x = y > x ? y : x;
+      }
+    } else {
+      for (uint32_t m = 1; m < R; m <<= 1) x += __shfl_xor(x, (int)m);
+    }
     if (idx < T) reg[idx] = 0;
     if ((lane & (R - 1)) == 0 && idx < T) reg[idx >> rshift] = x;
   }
@@ -120,14 +143,16 @@ __device__ __forceinline__ void wave_fold_copies(unsigned long long* reg, uint32
 }
 
 // Adds word w's total (a count when w % (1 + naggs) == 0, else a signed 64-bit
-// partial sum of aggregate w % (1 + naggs) - 1) given as int128 (lo, hi) into
-// the query's result.
+// partial sum of aggregate w % (1 + naggs) - 1, or a MIN / MAX code in lo) given as
+// int128 (lo, hi) into the query's result.
 __device__ __forceinline__ void add_result_word(unsigned long long* out_count,
                                                 unsigned long long* out_sums, uint32_t w,
-                                                int naggs, uint64_t lo, int64_t hi) {
+                                                int naggs, uint64_t lo, int64_t hi,
+                                                uint32_t kinds = 0) {
   const uint32_t W1 = 1 + naggs;
   const uint32_t g = w / W1, k = w % W1;
   if (k == 0) atomicAdd(&out_count[g], (unsigned long long)lo);
+  else if (minmax_word(w, W1, kinds)) atomicMax(&out_sums[((uint64_t)g * naggs + (k - 1)) * 2], lo);
   else atomic_add128(&out_sums[((uint64_t)g * naggs + (k - 1)) * 2], lo, hi);
 }
 
@@ -135,17 +160,19 @@ __device__ __forceinline__ void add_result_word(unsigned long long* out_count,
 // copies) into the query's exact result and clears it.
 __device__ __forceinline__ void wave_flush(unsigned long long* out_count,
                                            unsigned long long* out_sums, uint32_t words,
-                                           int naggs, unsigned long long* reg, uint32_t rshift) {
+                                           int naggs, unsigned long long* reg, uint32_t rshift,
+                                           uint32_t kinds = 0) {
   const uint32_t lane = threadIdx.x & 63;
-  wave_fold_copies(reg, words, rshift);
   const uint32_t W1 = 1 + naggs;
+  wave_fold_copies(reg, words, rshift, W1, kinds);
   for (uint32_t w = lane; w < words; w += 64) {
     const unsigned long long x = reg[w];
     reg[w] = 0;
     if (!x) continue;
     const bool is_count = w % W1 == 0;
+    const bool signed_word = !is_count && !minmax_word(w, W1, kinds);
     add_result_word(out_count, out_sums, w, naggs, x,
-                    (!is_count && (int64_t)x < 0) ? -1 : 0);
+                    (signed_word && (int64_t)x < 0) ? -1 : 0, kinds);
   }
   __builtin_amdgcn_wave_barrier();
 }
