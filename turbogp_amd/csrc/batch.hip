@@ -195,7 +195,7 @@ __global__ void __launch_bounds__(kBlock) k_batch_items(const BatchQuery* __rest
 // covering query.  A chunk has at most 2^20 rows, so with |value| < 2^43 the 64-bit
 // partials stay exact; a slice holding a larger value adds that aggregate straight
 // into the int128 result.
-constexpr int kSharedWords = 6144;           // LDS region per workgroup (48 KiB): 3 per CU
+constexpr int kSharedWords = 4992;           // LDS region per workgroup (39 KiB): 4 per CU
 static_assert(kShapes[0].dt[0] == TGP_I32 && kShapes[0].dt[5] == TGP_U8 &&
               kShapes[0].dt[6] == TGP_U8, "the shared scan packs the Q1 key columns");
 constexpr uint32_t kSharedMaxSlices = 4096;  // rows per chunk <= 2^20
@@ -214,7 +214,12 @@ struct SharedQuery {
   uint32_t pstride;         // 0: a member; else a signature group's partials query:
                             // out_count is its partial rows, pstride words each
   uint32_t pk0;             // the group's first chunk (partials query)
-  uint32_t _pad;
+  uint32_t live;            // 0: no dt passes the date test
+  int32_t dlo;              // the date test on int32 dt: (uint32)(dt - dlo) <= dspan
+  uint32_t dspan;
+  uint32_t ksig;            // equal for members whose key parts below are equal
+  uint32_t klo[2], kspan[2];  // key part k in range: (uint32)(x_k - klo[k]) < kspan[k]
+  uint32_t kc, st5;         // group id x 5 = 5 (x_0 - klo[0]) + st5 (x_1 - klo[1]) + kc
   unsigned long long* out_count;
   unsigned long long* out_sums;
   unsigned long long* out_oor;
@@ -238,7 +243,9 @@ struct SharedRows {
   uint64_t val[4][kSR];          // the four summed values
   bool big[4];                   // wave-uniform: the slice has |value| >= 2^43
   bool any_big;
-  bool small0;                   // wave-uniform: every first value is in [0, 2^16)
+  int32_t mode;                  // wave-uniform: 0 no big value and every first value
+                                 // in [0, 2^16) (val[0] then holds kCountOne + value),
+                                 // 1 no big value, 2 some big value
 };
 
 // Word 0 of a group holds its row count in bits 40..63 and, for slices whose first
@@ -271,86 +278,120 @@ struct EntryOut {
   uint32_t st;  // word distance of consecutive result words
 };
 
-__device__ __forceinline__ EntryOut entry_out(ConstShared& D, uint64_t kchunk) {
+template <typename SQ>
+__device__ __forceinline__ EntryOut entry_out(SQ& D, uint64_t kchunk) {
   if (!D.pstride) return EntryOut{D.out_count, D.out_sums, D.out_oor, 1u};
   const uint32_t st = D.pstride, G = (uint32_t)D.words / 5;
   unsigned long long* base = D.out_count + 1 + (kchunk - D.pk0);
   return EntryOut{base, base + (uint64_t)G * st, base + (uint64_t)9 * G * st, st};
 }
 
-// One query of the chunk on one slice, first half: the rows it selects (range and
-// date test), their group ids of key form KIND (premultiplied by the word stride of
-// 2^RS copies) and the selected rows whose key part is out of range (`bad`, left
-// out of sel).  Returns sel; no lane selecting anything leaves gid unset.
-template <int KIND, int RS>
-__device__ __forceinline__ uint32_t shared_select(ConstShared& D, const SharedRows& v,
-                                                  uint64_t row0, uint64_t row,
-                                                  uint32_t (&gid)[kSR], uint32_t& bad) {
-  constexpr int W1 = 5;
-  bad = 0;
-  uint32_t sel = (1u << kSR) - 1;
-  if (row0 < D.row_begin || row0 + kSliceRows > D.row_end) {
-    sel = 0;
-#pragma unroll
-    for (int r = 0; r < kSR; ++r)
-      if (row + r >= D.row_begin && row + r < D.row_end) sel |= 1u << r;
-  }
-  const uint32_t lo = (uint32_t)(int32_t)D.plo;
-  const uint32_t span = (uint32_t)(int32_t)D.phi - lo;  // modulo 2^32: hi - lo when lo <= hi
-  const bool never = (int32_t)D.phi < (int32_t)D.plo;
-#pragma unroll
-  for (int r = 0; r < kSR; ++r)  // lo <= dt <= hi as one unsigned compare
-    if (never || (uint32_t)v.dt[r] - lo > span) sel &= ~(1u << r);
-  if (!__ballot(sel != 0)) return 0;
-  uint32_t ok = (1u << kSR) - 1;
-#pragma unroll
-  for (int r = 0; r < kSR; ++r) {
-    uint64_t p0 = 0, p1 = 0;
-    if constexpr (KIND == 0 || KIND == 1) p0 = (uint64_t)((int64_t)(v.fab[r] & 0xffu) - D.gbase[0]);
-    if constexpr (KIND == 2) p0 = (uint64_t)((int64_t)(v.fab[r] >> 8) - D.gbase[0]);
-    if constexpr (KIND == 3) p0 = (uint64_t)((int64_t)v.dq[r] - D.gbase[0]);
-    if constexpr (KIND == 0) p1 = (uint64_t)((int64_t)(v.fab[r] >> 8) - D.gbase[1]);
-    bool in = true;
-    if constexpr (KIND <= 3) in = p0 < (uint64_t)D.gcard[0];
-    if constexpr (KIND == 0) in = in && p1 < (uint64_t)D.gcard[1];
-    if (!in) ok &= ~(1u << r);
-    uint32_t g = (uint32_t)p0;
-    if constexpr (KIND == 0) g = (uint32_t)p0 + __umul24((uint32_t)p1, (uint32_t)D.stride1);
-    gid[r] = __umul24(g, (uint32_t)(W1 << RS));
-  }
-  if constexpr (KIND <= 3) {
-    bad = sel & ~ok;
-    sel &= ok;
-  }
-  return sel;
+// An item's entries are held one per lane (lane j: entry j of the item, at most
+// kWave per item) and read back with v_readlane, so the loop over a slice's entries
+// issues no scalar or LDS loads: a scalar load there would wait for the previous
+// entry's LDS atomics, which share its completion counter.
+struct EntryLane {
+  uint32_t pack;           // region offset | rshift << 16 | key form << 20
+  uint32_t rb, re;         // the item's rows [rb, re) the query covers (item-relative)
+  uint32_t dlo, dspan;     // the date test: (uint32)(dt - dlo) <= dspan
+  uint32_t ksig;           // key signature: equal ids give equal group ids and tests
+  uint32_t klo0, kspan0, klo1, kspan1, kc, st5;  // the key parts (SharedQuery)
+  uint32_t sums_lo, sums_hi, oor_lo, oor_hi, st;  // EntryOut of the item's chunk
+};
+
+__device__ __forceinline__ EntryLane load_entry_lane(const SharedQuery* __restrict__ fq,
+                                                     const uint32_t* __restrict__ entries,
+                                                     uint32_t e, uint64_t irow0, uint32_t nrows,
+                                                     uint64_t kchunk) {
+  EntryLane E;
+  const SharedQuery& D = fq[entries[2 * e]];
+  E.pack = entries[2 * e + 1] | ((uint32_t)D.kind << 20);
+  const uint64_t b = D.row_begin > irow0 ? D.row_begin - irow0 : 0;
+  const uint64_t x = D.row_end > irow0 ? D.row_end - irow0 : 0;
+  E.rb = (uint32_t)(b < nrows ? b : nrows);
+  E.re = (uint32_t)(x < nrows ? x : nrows);
+  if (!D.live) E.rb = E.re = 0;  // no row passes the date test
+  E.dlo = (uint32_t)D.dlo;
+  E.dspan = D.dspan;
+  E.ksig = D.ksig;
+  E.klo0 = D.klo[0];
+  E.kspan0 = D.kspan[0];
+  E.klo1 = D.klo[1];
+  E.kspan1 = D.kspan[1];
+  E.kc = D.kc;
+  E.st5 = D.st5;
+  const EntryOut O = entry_out(D, kchunk);
+  E.sums_lo = (uint32_t)(uintptr_t)O.sums;
+  E.sums_hi = (uint32_t)((uintptr_t)O.sums >> 32);
+  E.oor_lo = (uint32_t)(uintptr_t)O.oor;
+  E.oor_hi = (uint32_t)((uintptr_t)O.oor >> 32);
+  E.st = O.st;
+  return E;
 }
 
-// Second half: the selected rows' five LDS atomics under one branch per row, at
-// compile-time offsets (lane l on copy l mod 2^RS); a slice holding a big value
-// adds that aggregate straight into the int128 result.
+__device__ __forceinline__ uint32_t lane_of(uint32_t v, uint32_t j) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)j);
+}
+
+__device__ __forceinline__ unsigned long long* lane_ptr(uint32_t lo, uint32_t hi, uint32_t j) {
+  return (unsigned long long*)(uintptr_t)(((uint64_t)lane_of(hi, j) << 32) | lane_of(lo, j));
+}
+
+// Group ids (x 5, the words of a group) and key range tests of this lane's rows for
+// the key form of lane j's entry: part k's source x_k is in range when
+// (uint32)(x_k - klo_k) < kspan_k, and then 5 group = 5 t0 + st5 t1 + kc with
+// t_k = x_k - klo_k (exact modulo 2^32: the planner keeps 5 groups below 2^31).
+__device__ __forceinline__ void key_ids(const EntryLane& E, uint32_t j, const SharedRows& v,
+                                        uint32_t (&gid5)[kSR], bool (&ok)[kSR]) {
+  const uint32_t kind = lane_of(E.pack, j) >> 20;
+  const uint32_t lo0 = lane_of(E.klo0, j), sp0 = lane_of(E.kspan0, j);
+  const uint32_t lo1 = lane_of(E.klo1, j), sp1 = lane_of(E.kspan1, j);
+  const uint32_t kc = lane_of(E.kc, j), st5 = lane_of(E.st5, j);
+#pragma unroll
+  for (int r = 0; r < kSR; ++r) {
+    const uint32_t x0 = kind == 3 ? (uint32_t)v.dq[r] : kind == 2 ? v.fab[r] >> 8 : v.fab[r] & 0xffu;
+    const uint32_t t0 = x0 - lo0;
+    bool in = t0 < sp0;
+    uint32_t g = (t0 << 2) + t0 + kc;
+    if (kind == 0) {
+      const uint32_t t1 = (v.fab[r] >> 8) - lo1;
+      in = in && t1 < sp1;
+      g += __umul24(t1, st5);
+    }
+    if (kind == 4) {
+      in = true;
+      g = 0;
+    }
+    ok[r] = in;
+    gid5[r] = g;
+  }
+}
+
+// The selected rows' LDS atomics (lane l on copy l mod 2^RS, compile-time word
+// offsets); a slice holding a big value adds that aggregate straight into the
+// int128 result.
 template <int RS>
-__device__ __forceinline__ void shared_accumulate(ConstShared& D, uint64_t kchunk, const SharedRows& v,
-                                                  uint32_t sel, const uint32_t (&gid)[kSR],
-                                                  unsigned long long* reg) {
-  constexpr int W1 = 5;
+__device__ __forceinline__ void shared_add(const EntryLane& E, uint32_t j, const SharedRows& v,
+                                           const bool (&sel)[kSR], const uint32_t (&gid5)[kSR],
+                                           unsigned long long* reg) {
   unsigned long long* mine = reg + ((threadIdx.x & 63) & ((1u << RS) - 1));
-  if (!v.any_big && v.small0) {  // four atomics: the count shares word 0 with value 0
+  if (v.mode == 0) {  // four atomics: word 0 holds the count and value 0 (kCountOne added)
 #pragma unroll
     for (int r = 0; r < kSR; ++r)
-      if ((sel >> r) & 1) {
-        unsigned long long* p = mine + gid[r];
-        atomicAdd(p, kCountOne + (unsigned long long)v.val[0][r]);
+      if (sel[r]) {
+        unsigned long long* p = mine + (gid5[r] << RS);
+        atomicAdd(p, (unsigned long long)v.val[0][r]);
         atomicAdd(p + (2 << RS), (unsigned long long)v.val[1][r]);
         atomicAdd(p + (3 << RS), (unsigned long long)v.val[2][r]);
         atomicAdd(p + (4 << RS), (unsigned long long)v.val[3][r]);
       }
     return;
   }
-  if (!v.any_big) {
+  if (v.mode == 1) {
 #pragma unroll
     for (int r = 0; r < kSR; ++r)
-      if ((sel >> r) & 1) {
-        unsigned long long* p = mine + gid[r];
+      if (sel[r]) {
+        unsigned long long* p = mine + (gid5[r] << RS);
         atomicAdd(p, kCountOne);
         atomicAdd(p + (1 << RS), (unsigned long long)v.val[0][r]);
         atomicAdd(p + (2 << RS), (unsigned long long)v.val[1][r]);
@@ -361,60 +402,204 @@ __device__ __forceinline__ void shared_accumulate(ConstShared& D, uint64_t kchun
   }
 #pragma unroll
   for (int r = 0; r < kSR; ++r)
-    if ((sel >> r) & 1) atomicAdd(mine + gid[r], kCountOne);
+    if (sel[r]) atomicAdd(mine + (gid5[r] << RS), kCountOne);
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
     if (v.big[a]) {
+      unsigned long long* sums = lane_ptr(E.sums_lo, E.sums_hi, j);
+      const uint32_t st = lane_of(E.st, j);
 #pragma unroll
       for (int r = 0; r < kSR; ++r)
-        if ((sel >> r) & 1) {
-          const uint64_t g = gid[r] / (uint32_t)(W1 << RS);
+        if (sel[r]) {
+          const uint64_t g = gid5[r] / 5u;
           const int64_t x = (int64_t)v.val[a][r];
-          const EntryOut O = entry_out(D, kchunk);
-          atomic_add128_split(O.sums + (g * 4 + a) * 2 * O.st, O.st, (uint64_t)x, x < 0 ? -1 : 0);
+          atomic_add128_split(sums + (g * 4 + a) * 2 * st, st, (uint64_t)x, x < 0 ? -1 : 0);
         }
     } else {
 #pragma unroll
       for (int r = 0; r < kSR; ++r)
-        if ((sel >> r) & 1)
-          atomicAdd(mine + gid[r] + ((1 + a) << RS), (unsigned long long)v.val[a][r]);
+        if (sel[r]) atomicAdd(mine + (gid5[r] << RS) + ((1 + a) << RS), (unsigned long long)v.val[a][r]);
     }
   }
 }
 
-// One query of the chunk on one slice: both halves, and the out-of-range count.
-template <int KIND, int RS>
-__device__ __forceinline__ void shared_query(ConstShared& D, uint64_t kchunk, const SharedRows& v,
-                                             uint64_t row0, uint64_t row,
-                                             unsigned long long* reg) {
-  uint32_t gid[kSR], bad = 0;
-  const uint32_t sel = shared_select<KIND, RS>(D, v, row0, row, gid, bad);
-  if (bad) atomicAdd(entry_out(D, kchunk).oor, (unsigned long long)__popc(bad));
-  if (__ballot(sel != 0)) shared_accumulate<RS>(D, kchunk, v, sel, gid, reg);
+// The common case of one entry on one slice: every key of the slice in range and
+// no big value (mode 0 or 1).  Each row's date test (and, on a slice the query
+// covers in part, its row-range test) guards that row's atomics directly.
+template <int MODE, int RS>
+__device__ __forceinline__ void shared_rows(const SharedRows& v, const uint32_t (&gid5)[kSR],
+                                            uint32_t dlo, uint32_t dspan, bool part,
+                                            uint32_t rb, uint32_t re, uint32_t rel0,
+                                            unsigned long long* reg) {
+  unsigned long long* mine = reg + ((threadIdx.x & 63) & ((1u << RS) - 1));
+#pragma unroll
+  for (int r = 0; r < kSR; ++r) {
+    bool c = (uint32_t)v.dt[r] - dlo <= dspan;
+    if (part) c = c && rel0 + r - rb < re - rb;
+    if (c) {
+      unsigned long long* p = mine + (gid5[r] << RS);
+      if constexpr (MODE == 0) {  // word 0: kCountOne + value 0
+        atomicAdd(p, (unsigned long long)v.val[0][r]);
+      } else {
+        atomicAdd(p, kCountOne);
+        atomicAdd(p + (1 << RS), (unsigned long long)v.val[0][r]);
+      }
+      atomicAdd(p + (2 << RS), (unsigned long long)v.val[1][r]);
+      atomicAdd(p + (3 << RS), (unsigned long long)v.val[2][r]);
+      atomicAdd(p + (4 << RS), (unsigned long long)v.val[3][r]);
+    }
+  }
 }
 
-// shared_query for key form i / 7 and 2^(i % 7) copies through one switch (a jump
-// table on the wave-uniform index, not a chain of compares).
-__device__ __forceinline__ void shared_query_dispatch(int i, ConstShared& D, uint64_t kchunk,
-                                                      const SharedRows& v, uint64_t row0,
-                                                      uint64_t row, unsigned long long* reg) {
-#define TGP_SQ_CASE(I) \
-  case I: shared_query<(I) / 7, (I) % 7>(D, kchunk, v, row0, row, reg); break;
-#define TGP_SQ_KIND(K) \
-  TGP_SQ_CASE(7 * (K)) TGP_SQ_CASE(7 * (K) + 1) TGP_SQ_CASE(7 * (K) + 2) TGP_SQ_CASE(7 * (K) + 3) \
-  TGP_SQ_CASE(7 * (K) + 4) TGP_SQ_CASE(7 * (K) + 5) TGP_SQ_CASE(7 * (K) + 6)
-  switch (i) {
-    TGP_SQ_KIND(0) TGP_SQ_KIND(1) TGP_SQ_KIND(2) TGP_SQ_KIND(3) TGP_SQ_KIND(4)
+template <int MODE>
+__device__ __forceinline__ void shared_rows_rs(uint32_t rs, const SharedRows& v,
+                                               const uint32_t (&gid5)[kSR], uint32_t dlo,
+                                               uint32_t dspan, bool part, uint32_t rb, uint32_t re,
+                                               uint32_t rel0, unsigned long long* reg) {
+#define TGP_SR_CASE(K) \
+  case K: shared_rows<MODE, K>(v, gid5, dlo, dspan, part, rb, re, rel0, reg); break;
+  switch (rs) {
+    TGP_SR_CASE(0) TGP_SR_CASE(1) TGP_SR_CASE(2) TGP_SR_CASE(3) TGP_SR_CASE(4) TGP_SR_CASE(5)
+    TGP_SR_CASE(6)
     default: break;
   }
-#undef TGP_SQ_KIND
-#undef TGP_SQ_CASE
+#undef TGP_SR_CASE
+}
+
+// One entry on one slice where some key is out of range or a value is big: the
+// selected rows (date and row-range tests, in-range keys), the out-of-range ones
+// counted, and the atomics of shared_add.
+__device__ __forceinline__ void shared_entry_slow(const EntryLane& E, uint32_t j,
+                                                  const SharedRows& v, const uint32_t (&gid5)[kSR],
+                                                  const bool (&ok)[kSR], bool all_ok, uint32_t s_rel,
+                                                  uint32_t rb, uint32_t re, uint32_t dlo,
+                                                  uint32_t dspan, uint32_t pack,
+                                                  unsigned long long* region) {
+  const uint32_t lane = threadIdx.x & 63;
+  bool sel[kSR];
+#pragma unroll
+  for (int r = 0; r < kSR; ++r) sel[r] = (uint32_t)v.dt[r] - dlo <= dspan;
+  if (rb > s_rel || re < s_rel + kSliceRows) {
+#pragma unroll
+    for (int r = 0; r < kSR; ++r) {
+      const uint32_t rel = s_rel + lane * kSR + r;
+      sel[r] = sel[r] && rel - rb < re - rb;
+    }
+  }
+  if (!all_ok) {
+    uint32_t nbad = 0;
+#pragma unroll
+    for (int r = 0; r < kSR; ++r) {
+      nbad += (uint32_t)__popcll(__ballot(sel[r] && !ok[r]));
+      sel[r] = sel[r] && ok[r];
+    }
+    if (nbad && lane == 0) atomicAdd(lane_ptr(E.oor_lo, E.oor_hi, j), (unsigned long long)nbad);
+  }
+  if (!__ballot(sel[0] || sel[1] || sel[2] || sel[3])) return;
+  unsigned long long* reg = region + (pack & 0xffffu);
+  switch ((pack >> 16) & 15u) {
+    case 0: shared_add<0>(E, j, v, sel, gid5, reg); break;
+    case 1: shared_add<1>(E, j, v, sel, gid5, reg); break;
+    case 2: shared_add<2>(E, j, v, sel, gid5, reg); break;
+    case 3: shared_add<3>(E, j, v, sel, gid5, reg); break;
+    case 4: shared_add<4>(E, j, v, sel, gid5, reg); break;
+    case 5: shared_add<5>(E, j, v, sel, gid5, reg); break;
+    case 6: shared_add<6>(E, j, v, sel, gid5, reg); break;
+    default: break;
+  }
+}
+
+// Every entry of the item on one slice (item-relative first row s_rel): the date
+// and row-range tests, out-of-range keys counted, and the selected rows added into
+// the entry's part of the region.  Entries arrive sorted by key signature, so the
+// group ids are computed once per signature.
+__device__ __forceinline__ void shared_entries(const EntryLane& E, uint32_t ne, const SharedRows& v,
+                                               uint32_t s_rel, unsigned long long* region) {
+  const uint32_t lane = threadIdx.x & 63;
+  uint32_t cur = ~0u;
+  uint32_t gid5[kSR];
+  bool ok[kSR];
+  bool all_ok = true;
+#pragma unroll
+  for (int r = 0; r < kSR; ++r) {
+    gid5[r] = 0;
+    ok[r] = true;
+  }
+  for (uint32_t j = 0; j < ne; ++j) {
+    const uint32_t rb = lane_of(E.rb, j), re = lane_of(E.re, j);
+    if (re <= s_rel || rb >= s_rel + kSliceRows) continue;
+    const uint32_t ks = lane_of(E.ksig, j);
+    if (ks != cur) {
+      cur = ks;
+      key_ids(E, j, v, gid5, ok);
+      all_ok = __ballot(!(ok[0] && ok[1] && ok[2] && ok[3])) == 0;
+    }
+    const uint32_t dlo = lane_of(E.dlo, j), dspan = lane_of(E.dspan, j);
+    const uint32_t pack = lane_of(E.pack, j);
+    if (all_ok && v.mode != 2) {
+      const bool part = rb > s_rel || re < s_rel + kSliceRows;
+      const uint32_t rs = (pack >> 16) & 15u;
+      unsigned long long* reg = region + (pack & 0xffffu);
+      if (v.mode == 0)
+        shared_rows_rs<0>(rs, v, gid5, dlo, dspan, part, rb, re, s_rel + lane * kSR, reg);
+      else
+        shared_rows_rs<1>(rs, v, gid5, dlo, dspan, part, rb, re, s_rel + lane * kSR, reg);
+      continue;
+    }
+    shared_entry_slow(E, j, v, gid5, ok, all_ok, s_rel, rb, re, dlo, dspan, pack, region);
+  }
+}
+
+// The row values every query of a chunk shares for this lane's rows of one slice:
+// the key sources flagA, flagB and dt / divisor, the summed values, their big-value
+// flags and the slice's accumulation mode.
+template <bool NT>
+__device__ __forceinline__ void shared_rows_of(const SharedFamily& F, uint64_t row0,
+                                               uint64_t min_len, SharedRows& sr) {
+  constexpr Shape S = kShapes[0];
+  const uint64_t row = row0 + (threadIdx.x & 63) * kSR;
+  SliceVals<S> v;
+  load_all<S, 0, NT>(F.a, row, row0 + kSliceRows <= min_len, v);
+#pragma unroll
+  for (int r = 0; r < kSR; ++r) {
+    sr.dt[r] = (int32_t)v.x[0][r];
+    sr.fab[r] = (uint32_t)v.x[5][r] | ((uint32_t)v.x[6][r] << 8);
+    int64_t x = v.x[0][r];
+    if (F.divisor > 1) {
+      if (x >= 0 && x <= 0xffffffffll) {
+        const uint32_t nn = (uint32_t)x;
+        const uint32_t q = __umulhi(F.magic, nn);
+        x = (int64_t)((q + ((nn - q) >> 1)) >> F.shift);
+      } else {
+        x = x / F.divisor;
+      }
+    }
+    sr.dq[r] = (int32_t)x;
+  }
+  sr.any_big = false;
+  static_for<S.naggs>([&](auto AI) {
+    constexpr int a = decltype(AI)::value;
+    agg_values<S, a>(F.a, v, sr.val[a]);
+    uint32_t m = 0;
+#pragma unroll
+    for (int r = 0; r < kSR; ++r) big_fold(m, sr.val[a][r]);
+    sr.big[a] = __ballot(big_any(m)) != 0;
+    sr.any_big = sr.any_big || sr.big[a];
+  });
+  bool small = true;
+#pragma unroll
+  for (int r = 0; r < kSR; ++r) small = small && sr.val[0][r] < (1ull << 16);
+  sr.mode = sr.any_big ? 2 : __ballot(!small) == 0 ? 0 : 1;
+  if (sr.mode == 0) {
+#pragma unroll
+    for (int r = 0; r < kSR; ++r) sr.val[0][r] += kCountOne;
+  }
 }
 
 // items: 4 words each (first absolute slice, slices, first entry, entries);
 // entries: 2 words each (query index into the family, region offset | rshift << 16)
 template <bool NT>
-__global__ void __launch_bounds__(kBlock) k_batch_shared(SharedFamily F,
+__global__ void __launch_bounds__(kBlock, 4) k_batch_shared(SharedFamily F,
                                                          const SharedQuery* __restrict__ fq,
                                                          const uint32_t* __restrict__ items,
                                                          const uint32_t* __restrict__ entries,
@@ -427,7 +612,7 @@ __global__ void __launch_bounds__(kBlock) k_batch_shared(SharedFamily F,
   ConstShared* qs = (ConstShared*)fq;
   ConstU32* it = (ConstU32*)items;
   ConstU32* en = (ConstU32*)entries;
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & 63, wave = wave_index();
   for (uint32_t w = threadIdx.x; w < (uint32_t)kSharedWords; w += kBlock) region[w] = 0;
   uint64_t min_len = F.a.col_len[0];
 #pragma unroll
@@ -446,53 +631,14 @@ __global__ void __launch_bounds__(kBlock) k_batch_shared(SharedFamily F,
       const uint64_t sl0 = it[4 * i];
       const uint32_t nsl = it[4 * i + 1], e0 = it[4 * i + 2], ne = it[4 * i + 3];
       const uint64_t kchunk = sl0 / F.chunk;
+      // the item's entry descriptors, one per lane (the planner keeps ne <= kWave)
+      EntryLane E{};
+      if (lane < ne)
+        E = load_entry_lane(fq, entries, e0 + lane, sl0 * kSliceRows, nsl * kSliceRows, kchunk);
       for (uint32_t sj = wave; sj < nsl; sj += kBlock / kWave) {
-        const uint64_t row0 = (sl0 + sj) * kSliceRows;
-        const uint64_t row = row0 + lane * kSR;
-        SliceVals<S> v;
-        load_all<S, 0, NT>(F.a, row, row0 + kSliceRows <= min_len, v);
-        // the row values every query of the chunk shares: the key sources flagA,
-        // flagB and dt / divisor, the summed values and their big-value flags
         SharedRows sr;
-#pragma unroll
-        for (int r = 0; r < kSR; ++r) {
-          sr.dt[r] = (int32_t)v.x[0][r];
-          sr.fab[r] = (uint32_t)v.x[5][r] | ((uint32_t)v.x[6][r] << 8);
-          int64_t x = v.x[0][r];
-          if (F.divisor > 1) {
-            if (x >= 0 && x <= 0xffffffffll) {
-              const uint32_t nn = (uint32_t)x;
-              const uint32_t q = __umulhi(F.magic, nn);
-              x = (int64_t)((q + ((nn - q) >> 1)) >> F.shift);
-            } else {
-              x = x / F.divisor;
-            }
-          }
-          sr.dq[r] = (int32_t)x;
-        }
-        sr.any_big = false;
-        static_for<S.naggs>([&](auto AI) {
-          constexpr int a = decltype(AI)::value;
-          agg_values<S, a>(F.a, v, sr.val[a]);
-          uint32_t m = 0;
-#pragma unroll
-          for (int r = 0; r < kSR; ++r) big_fold(m, sr.val[a][r]);
-          sr.big[a] = __ballot(big_any(m)) != 0;
-          sr.any_big = sr.any_big || sr.big[a];
-        });
-        {
-          bool small = true;
-#pragma unroll
-          for (int r = 0; r < kSR; ++r) small = small && sr.val[0][r] < (1ull << 16);
-          sr.small0 = __ballot(!small) == 0;
-        }
-        for (uint32_t e = e0; e < e0 + ne; ++e) {
-          const uint32_t q = en[2 * e], pack = en[2 * e + 1];
-          ConstShared& D = qs[q];
-          if (row0 >= D.row_end || row0 + kSliceRows <= D.row_begin) continue;
-          shared_query_dispatch(D.kind * 7 + (int)(pack >> 16), D, kchunk, sr, row0, row,
-                                region + (pack & 0xffffu));
-        }
+        shared_rows_of<NT>(F, (sl0 + sj) * kSliceRows, min_len, sr);
+        shared_entries(E, ne, sr, sj * kSliceRows, region);
       }
       __syncthreads();
       // every query's copies summed into its exact result; the region cleared
@@ -807,12 +953,48 @@ static int32_t launch_variant(tgp_batch* b, hipStream_t s) {
   return TGP_OK;
 }
 
+// A member's date and key tests in the 32-bit forms the shared scan evaluates (the
+// int64 bounds clamped to what int32 dt, the u8 flags and int32 dt / divisor can
+// hold), and its group id as 5 t0 + st5 t1 + kc.  False: the query stays out of
+// the family (a key part of more than 2^30 values, a stride past 24-bit products).
+static bool shared_tests(SharedQuery& m) {
+  const int64_t lo = std::max<int64_t>(m.plo, INT32_MIN), hi = std::min<int64_t>(m.phi, INT32_MAX);
+  m.live = lo <= hi ? 1u : 0u;
+  m.dlo = m.live ? (int32_t)lo : 0;
+  m.dspan = m.live ? (uint32_t)(hi - lo) : 0u;
+  static constexpr int kParts[5] = {2, 1, 1, 1, 0};  // key parts of key forms 0..4
+  if (m.kind < 0 || m.kind > 4 || m.nparts != kParts[m.kind]) return false;
+  uint32_t c[2] = {0, 0};
+  m.klo[0] = m.klo[1] = m.kspan[0] = m.kspan[1] = 0;
+  for (int k = 0; k < m.nparts; ++k) {
+    if (m.gcard[k] < 1 || m.gcard[k] > (1 << 30)) return false;
+    const bool wide = m.kind == 3;  // dt / divisor (int32); else a u8 flag
+    const __int128 smin = wide ? (__int128)INT32_MIN : 0;
+    const __int128 send = wide ? (__int128)INT32_MAX + 1 : 256;
+    const __int128 l = std::max<__int128>(m.gbase[k], smin);
+    const __int128 h = std::min<__int128>((__int128)m.gbase[k] + m.gcard[k], send);
+    if (h > l) {  // else no value is in range: kspan 0
+      m.klo[k] = (uint32_t)(int64_t)l;
+      m.kspan[k] = (uint32_t)(h - l);
+      c[k] = (uint32_t)(l - m.gbase[k]);  // below gcard
+    }
+  }
+  m.st5 = 0;
+  if (m.kind == 0) {
+    if (m.stride1 < 0 || m.stride1 >= (1 << 22)) return false;
+    m.st5 = (uint32_t)m.stride1 * 5u;
+  }
+  m.kc = 5u * c[0] + m.st5 * c[1];
+  return true;
+}
+
 // The shared scan's items for chunks of `chunk` slices: chunk k lists the family
 // members overlapping it, with each member's part of the workgroup LDS region
 // (offset, 2^rshift copies: up to 64 for one group, fewer as groups grow, fewer
-// again while the chunk's members do not fit); a chunk whose members still do not
-// fit with one copy each becomes several items over the same slices.  Chunk k's
-// items go to queue k % kQueues.
+// again while the chunk's members do not fit), sorted by key signature; a chunk
+// whose members still do not fit with one copy each, or number more than kWave,
+// becomes several items over the same slices.  Chunk k's items go to queue
+// k % kQueues.
 static int32_t build_shared_items(tgp_batch* b, hipStream_t s) {
   const uint64_t C = b->chunk < kSharedMaxSlices ? b->chunk : kSharedMaxSlices;
   const size_t nf = b->fam.size();
@@ -963,6 +1145,10 @@ static int32_t build_shared_items(tgp_batch* b, hipStream_t s) {
     while (cur_queue < q) b->squeue_off[++cur_queue] = total;
     L.clear();
     for (size_t j = p; j < pe; ++j) L.push_back(pairs[j].second);
+    // by key signature: the kernel computes group ids once per signature and slice
+    std::sort(L.begin(), L.end(), [&](uint32_t x, uint32_t y) {
+      return b->sfam[x].ksig != b->sfam[y].ksig ? b->sfam[x].ksig < b->sfam[y].ksig : x < y;
+    });
     p = pe;
     uint64_t s0 = (k + 1) * C, s1 = k * C;
     for (uint32_t m : L) {
@@ -1000,6 +1186,7 @@ static int32_t build_shared_items(tgp_batch* b, hipStream_t s) {
         while (j < L.size()) {
           const uint64_t sz = (uint64_t)b->sfam[L[j]].words << r[j];
           if (off + sz > (uint64_t)kSharedWords) break;
+          if (entries.size() / 2 - e0 >= (uint64_t)kWave) break;  // one entry per lane
           entries.push_back(L[j]);
           entries.push_back((uint32_t)off | ((uint32_t)r[j] << 16));
           off += sz;
@@ -1474,6 +1661,17 @@ int32_t tgp_batch_set_queries(tgp_batch* b, const tgp_agg_query* qs, uint32_t n,
     sigs.emplace(k, id);
     return id;
   };
+  // key signature ids: members whose group ids and key tests are equal
+  std::map<std::vector<uint32_t>, uint32_t> ksigs;
+  auto ksig_of = [&](const SharedQuery& m) {
+    const std::vector<uint32_t> k{(uint32_t)m.kind, m.klo[0], m.kspan[0], m.klo[1], m.kspan[1],
+                                  m.kc, m.st5};
+    auto it = ksigs.find(k);
+    if (it != ksigs.end()) return it->second;
+    const uint32_t id = (uint32_t)ksigs.size();
+    ksigs.emplace(k, id);
+    return id;
+  };
   for (uint32_t q = 0; q < b->plan.size(); ++q) {
     const BatchQuery& bq = b->plan[q];
     if (bq.shape > 4 || bq.naggs != kShapes[0].naggs) continue;
@@ -1516,7 +1714,9 @@ int32_t tgp_batch_set_queries(tgp_batch* b, const tgp_agg_query* qs, uint32_t n,
     m.stride1 = S.nparts == 2 ? (int32_t)A.gstride[1] : 0;
     m.words = (int32_t)A.words;
     m.kind = bq.shape;
+    if (!shared_tests(m)) continue;
     m.sig = sig_of(m);
+    m.ksig = ksig_of(m);
     m.out_count = A.out_count;
     m.out_sums = A.out_sums;
     m.out_oor = A.out_oor;

@@ -80,6 +80,13 @@ constexpr int kWave = 64;
 constexpr int kBlock = 256;            // threads per workgroup for streaming kernels
 constexpr int kTileRows = 1024;        // rows staged per tile in the fused kernels
 
+// This thread's wave within its workgroup, as a value the compiler holds in a
+// scalar register: loops and branches on it (a wave's slice, its rows' range
+// tests) then stay scalar instead of running under exec masks.
+__device__ __forceinline__ uint32_t wave_index() {
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+
 }  // namespace tgp
 
 #ifndef __HIPCC_RTC__
