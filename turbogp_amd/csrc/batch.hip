@@ -106,7 +106,7 @@ __global__ void __launch_bounds__(kBlock) k_batch(const BatchQuery* __restrict__
   ConstQuery* qs = (ConstQuery*)plan;
   const __attribute__((address_space(4))) uint32_t* gq =
       (const __attribute__((address_space(4))) uint32_t*)granule_q;
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & 63, wave = wave_index();
   unsigned long long* reg = lreg + wave * kWaveWords;
   for (int w = lane; w < kWaveWords; w += 64) reg[w] = 0;
   __builtin_amdgcn_wave_barrier();
@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(kBlock) k_batch_items(const BatchQuery* __rest
   ConstQuery* qs = (ConstQuery*)plan;
   const __attribute__((address_space(4))) uint32_t* it =
       (const __attribute__((address_space(4))) uint32_t*)items;
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & 63, wave = wave_index();
   unsigned long long* reg = lreg + wave * kWaveWords;
   for (int w = lane; w < kWaveWords; w += 64) reg[w] = 0;
   __builtin_amdgcn_wave_barrier();

@@ -27,7 +27,7 @@ template <int GMAX, int AMAX>
 __global__ void __launch_bounds__(kBlock, 2) k_agg_reg(DevPlan P) {
   extern __shared__ __align__(16) uint8_t lds[];
   const uint32_t lane = threadIdx.x & (kWave - 1);
-  const uint32_t wave = threadIdx.x / kWave;
+  const uint32_t wave = wave_index();
   uint32_t cnt[GMAX];
   uint64_t lo[GMAX][AMAX];
   int32_t hi[GMAX][AMAX];

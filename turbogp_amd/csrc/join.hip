@@ -797,7 +797,7 @@ template <bool USE_LDS, bool EXACT>
 __global__ void __launch_bounds__(256) k_probe_agg(JoinTableDev T, ProbeAggDev Q,
                                                    uint32_t rshift, uint32_t shared_table) {
   extern __shared__ __align__(16) unsigned long long ltab[];
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & 63, wave = wave_index();
   const uint64_t words = 3 * Q.ngroups;                  // per copy
   const uint64_t region = words << rshift;               // per wave (or per workgroup)
   const uint32_t nwt = shared_table ? 1 : 4;             // tables in the workgroup
@@ -846,7 +846,7 @@ __global__ void __launch_bounds__(256) k_probe_agg_batch(const ProbeBatchQuery* 
                                                          unsigned long long* counter) {
   __shared__ __align__(16) unsigned long long ltab[4 * kBatchWaveWords];
   __shared__ uint32_t s_item;
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & 63, wave = wave_index();
   for (uint32_t i = threadIdx.x; i < 4 * kBatchWaveWords; i += blockDim.x) ltab[i] = 0;
   for (;;) {
     __syncthreads();  // tables clear, s_item free
@@ -918,7 +918,7 @@ __global__ void __launch_bounds__(1024) k_scan_blocks(uint64_t* v, uint64_t n) {
 // shuffles, then the 4 wave totals); *total receives the workgroup's sum.
 __device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t x, uint64_t* total) {
   __shared__ unsigned long long wave_tot[4];
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & 63, wave = wave_index();
   uint64_t incl = x;
   for (int off = 1; off < 64; off <<= 1) {
     const uint64_t y = __shfl_up(incl, off);

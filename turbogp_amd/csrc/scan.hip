@@ -18,7 +18,7 @@ namespace tgp {
 __global__ void __launch_bounds__(kBlock) k_scan(DevPlan P) {
   extern __shared__ __align__(16) uint8_t lds[];
   const uint32_t lane = threadIdx.x & (kWave - 1);
-  const uint32_t wave = threadIdx.x / kWave;
+  const uint32_t wave = wave_index();
   const uint32_t d = (uint32_t)(P.row_begin & 63);
   const uint64_t nwords = (P.row_end - P.row_begin + 63) / 64;
   unsigned long long cnt = 0;
@@ -220,7 +220,7 @@ __global__ void __launch_bounds__(256) k_compact_scatter(const uint64_t* __restr
     pre[threadIdx.x] += v;
     __syncthreads();
   }
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x / 64;
+  const uint32_t lane = threadIdx.x & 63, wave = wave_index();
   const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
   const uint64_t boff = block_off[blockIdx.x];
   for (int i = wave; i < kCompactWords; i += 4) {

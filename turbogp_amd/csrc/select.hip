@@ -267,7 +267,7 @@ __global__ void __launch_bounds__(256) k_rs_pick(RsState* st, int pass) {
   // read before the barrier: the winning thread rewrites it after the barrier
   const uint64_t rem = st->remaining;
   // exclusive scan of the per-thread sums
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & 63, wave = wave_index();
   uint64_t incl = mine;
   for (int off = 1; off < 64; off <<= 1) {
     const uint64_t y = __shfl_up(incl, off);

@@ -12,7 +12,7 @@ namespace tgp {
 template <Shape S>
 __device__ __forceinline__ void spec_body(const SpecArgs& A) {
   extern __shared__ __align__(16) unsigned long long lmem[];
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & 63, wave = wave_index();
   // table 0: per-wave histograms of kHistWords words; table 1: one workgroup table
   // (counts[ngroups], then (lo, hi) sums) followed by nothing else.
   unsigned long long* hist = S.table == 0 ? lmem + wave * kHistWords : lmem;
@@ -147,7 +147,7 @@ template <Shape S>
 __device__ __forceinline__ void wave_body(const SpecArgs& A, uint32_t rshift, uint32_t chunk,
                                           unsigned long long* counter) {
   extern __shared__ __align__(16) unsigned long long lreg[];
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & 63, wave = wave_index();
   const uint32_t region = A.words << rshift;
   unsigned long long* reg = lreg + wave * region;
   for (uint32_t w = lane; w < region; w += 64) reg[w] = 0;

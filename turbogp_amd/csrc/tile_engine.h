@@ -100,7 +100,7 @@ struct DevPlan {
 // A group that straddles the end of a column is copied byte by byte (only the
 // final tile of a column can have one).
 __device__ __forceinline__ void stage_issue(const DevPlan& P, uint64_t t, uint8_t* buf) {
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & 63, wave = wave_index();
   const uint64_t row0 = P.base0 + t * kTileRows;
   uint32_t g = 0;  // group index across slots, uniform
   for (int s = 0; s < P.nslots; ++s) {
