@@ -857,8 +857,15 @@ struct tgp_batch {
     SpecArgs a;
   };
   std::vector<SpecRun> spec_runs;
-  unsigned long long* d_spec_counter = nullptr;
+  unsigned long long* d_spec_counter = nullptr;  // one chunk counter per run
   uint64_t spec_counter_cap = 0;
+  // the runs go round-robin to side streams (default priority) forked from the
+  // batch stream after the result fill and joined into it at the launch's end, so
+  // small runs overlap each other and the batch's other kernels
+  static constexpr int kSpecStreams = 3;
+  hipStream_t spec_streams[kSpecStreams] = {};
+  hipEvent_t spec_fork = nullptr;
+  hipEvent_t spec_join[kSpecStreams] = {};
   uint32_t* d_punits = nullptr;
   uint64_t pgroups_cap = 0, plinks_cap = 0, partials_cap = 0, partial_words = 0, punits_cap = 0;
 };
@@ -1296,8 +1303,17 @@ static int32_t plan_spec_runs(tgp_batch* b, const tgp_agg_query* qs) {
     b->spec_runs.push_back(r);
   }
   b->fallback.swap(rest);
-  if (!b->spec_runs.empty())
-    TGP_TRY(ensure_cap(b, &b->d_spec_counter, &b->spec_counter_cap, 1));
+  if (!b->spec_runs.empty()) {
+    TGP_TRY(ensure_cap(b, &b->d_spec_counter, &b->spec_counter_cap, b->spec_runs.size()));
+    // created here, outside any graph capture of the launch
+    if (!b->spec_fork) {
+      TGP_HIP(hipEventCreateWithFlags(&b->spec_fork, hipEventDisableTiming));
+      for (int i = 0; i < tgp_batch::kSpecStreams; ++i) {
+        TGP_HIP(hipStreamCreateWithFlags(&b->spec_streams[i], hipStreamNonBlocking));
+        TGP_HIP(hipEventCreateWithFlags(&b->spec_join[i], hipEventDisableTiming));
+      }
+    }
+  }
   return TGP_OK;
 }
 
@@ -1513,11 +1529,36 @@ static int32_t build_items(tgp_batch* b, hipStream_t s) {
   return TGP_OK;
 }
 
+static int32_t batch_launch_rest(tgp_batch* b, hipStream_t s);
+
+// Side streams used by the specialised runs: min(runs, kSpecStreams).
+static int spec_stream_count(const tgp_batch* b) {
+  return (int)std::min<size_t>(tgp_batch::kSpecStreams, b->spec_runs.size());
+}
+
 static int32_t batch_launch(tgp_batch* b, hipStream_t s) {
   // results and the counters behind them start at zero
   TGP_TRY(fill_u64(s, b->d_results, b->result_words + 2 * kQueues + 1, 0));
-  for (const auto& r : b->spec_runs)
-    TGP_TRY(launch_shape(b->ctx, r.shape, r.k, r.a, s, b->d_spec_counter, true));
+  const int ns = spec_stream_count(b);
+  if (ns) {
+    TGP_HIP(hipEventRecord(b->spec_fork, s));
+    for (int i = 0; i < ns; ++i) TGP_HIP(hipStreamWaitEvent(b->spec_streams[i], b->spec_fork, 0));
+    for (size_t k = 0; k < b->spec_runs.size(); ++k) {
+      const auto& r = b->spec_runs[k];
+      TGP_TRY(launch_shape(b->ctx, r.shape, r.k, r.a, b->spec_streams[k % ns],
+                           b->d_spec_counter + k, true));
+    }
+  }
+  const int32_t rc = batch_launch_rest(b, s);
+  for (int i = 0; i < ns; ++i) {  // joined on every path: a capture must end joined
+    TGP_HIP(hipEventRecord(b->spec_join[i], b->spec_streams[i]));
+    TGP_HIP(hipStreamWaitEvent(s, b->spec_join[i], 0));
+  }
+  return rc;
+}
+
+// The batch's kernels on its own stream after the result fill.
+static int32_t batch_launch_rest(tgp_batch* b, hipStream_t s) {
   if (!b->pqs.empty()) {  // before the generic launch, whose finalisation covers them
     TGP_TRY(fill_u64(s, reinterpret_cast<uint64_t*>(b->d_pmat), 3, 0));
     unsigned long long* mat = b->d_pmat + 4;
@@ -1555,6 +1596,14 @@ int32_t tgp_batch_destroy(tgp_batch* b) {
   if (!b) return TGP_OK;
   (void)hipStreamSynchronize(b->ctx->stream);
   drop_graph(b);
+  for (int i = 0; i < tgp_batch::kSpecStreams; ++i) {
+    if (b->spec_streams[i]) {
+      (void)hipStreamSynchronize(b->spec_streams[i]);
+      (void)hipStreamDestroy(b->spec_streams[i]);
+    }
+    if (b->spec_join[i]) (void)hipEventDestroy(b->spec_join[i]);
+  }
+  if (b->spec_fork) (void)hipEventDestroy(b->spec_fork);
   if (b->d_plan) hbm_free(b->ctx, b->d_plan);
   if (b->d_results) hbm_free(b->ctx, b->d_results);
   if (b->d_granule) hbm_free(b->ctx, b->d_granule);
