@@ -397,3 +397,48 @@ def test_mixed_shape_batch_has_no_fallback(ctx, table):
     finally:
         bm.free()
         price32.free()
+
+
+def test_shared_scan_many_members_per_chunk_and_wide_bounds(ctx, table):
+    """150 family members over the same few chunks (more than one per lane of a wave:
+    the planner splits each chunk's members into items of at most 64), key parts
+    shifted so some keys fall out of range, and date bounds beyond int32, against one
+    launch per query, a graph replay and the oracle."""
+    host, dev = table
+    rng = np.random.default_rng(11)
+    specs = []
+    for i in range(150):
+        names, preds, parts, aggs = W.q1_spec(int(rng.integers(0, 2526)),
+                                              ("a", "b", "ab", "month")[i % 4])
+        if i % 5 == 1:
+            preds = [(preds[0][0], A.LE, 2 ** 40, False)]
+        if i % 5 == 2:
+            preds = preds + [(preds[0][0], A.GE, -2 ** 40, False)]
+        if i % 7 == 3:
+            parts = [(c, d, base + 1, card) for (c, d, base, card) in parts]
+        b0 = int(rng.integers(0, 40_000))
+        specs.append((names, preds, parts, aggs, b0, b0 + int(rng.integers(100_000, 300_000))))
+    qs = [make_agg_query([dev[c] for c in nm], pr, pa, ag, b0, e0)
+          for (nm, pr, pa, ag, b0, e0) in specs]
+    b = Batch(ctx, len(qs))
+    try:
+        b.set_queries(qs)
+        assert b.info()["fallback"] == 0
+        b.set_tuning(0, use_graph=True, chunk_slices=128, order=3, cached_loads=True)
+        b.run(A.BATCH_PERSISTENT)
+        got = b.results()
+        b.run(A.BATCH_PERSISTENT)  # graph replay
+        replay = b.results()
+        b.run(A.BATCH_PER_QUERY)
+        ref = b.results()
+    finally:
+        b.close()
+    for i in range(len(qs)):
+        assert same(got[i], ref[i]) and same(got[i], replay[i]), i
+    assert any(r.n_out_of_range for r in got)
+    for i in range(0, len(qs), 7):
+        nm, pr, pa, ag, b0, e0 = specs[i]
+        cnt, sums, oor = B.group_agg(B.make_query([host[x] for x in nm], pr, pa, ag, b0, e0))
+        assert [int(x) for x in got[i].count] == [int(x) for x in cnt], i
+        assert got[i].sums() == sums, i
+        assert got[i].n_out_of_range == oor, i
